@@ -1,0 +1,170 @@
+"""Throughput of the field engine on one MI355X.
+
+Workload: the 400^3 single-domain dielectric-sphere scatter run (dx = 2.5 mm,
+12-cell CPML with order 3 / sigma 0.8 sigma_opt / kappa_max 1 / alpha 0.05,
+eps_r = 4 sphere of radius 100 cells at the grid centre, x-polarised Gaussian
+plane wave along +z through a TFSF box 16 cells from the edge, NF-FF surface
+14 cells from the edge at 3 frequencies), FP32 fields.  A step is one full
+time step of the reference's run loop: E curl + CPML, TFSF E faces, H curl +
+CPML, 1-D incident grid, TFSF H faces, NF-FF running DFT.
+
+Prints one JSON line.  value = N^3 * steps / wall time of the timed steps
+(Gcell-updates/s), inputs resident in HBM.  roofline: algorithmic bytes of
+the dominant curl kernel (18*s/2 + 1 bytes per cell per half step: read 3 E +
+3 H, write 3 of one kind, one material byte) over its average HIP-event
+duration inside the timed region.  cpu_baseline: the FP64 CPU oracle, one
+thread, on a few steps of the same 400^3 run (set-up excluded).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Yee Gcells/s & % HBM peak; batched vs plain; adjoint grad time at stated tol"
+
+
+def scene_400(n=400, dl=2.5e-3, pml=12, radius_cells=100, steps_total=2000):
+    import numpy as np
+
+    from xfdtd_amd import scene as S
+    objs = [S.Object("sphere", S.Sphere((0.0, 0.0, 0.0), radius_cells * dl),
+                     S.Material("sphere", 4.0, 1.0, 0.0, 0.0))]
+    sc = S.cube_domain_scene(
+        n, dl, pml, cfl=0.9, objects=objs, tfsf_gap=pml + 4, nffft_gap=pml + 2,
+        freqs=[S.C_0 / (40 * dl), S.C_0 / (30 * dl), S.C_0 / (20 * dl)],
+        theta=np.linspace(0.0, math.pi, 181), phi=[0.0, math.pi / 2])
+    return sc, steps_total
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def run_cpu_baseline(sc, total, steps):
+    from oracle.oracle import OracleSim, set_threads
+    threads = set_threads(1)
+    o = OracleSim(sc, total)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    nx, ny, nz = o.dims
+    o.close()
+    cells = nx * ny * nz
+    return {
+        "value": cells * steps / dt / 1e9,
+        "unit": "Gcells/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP64 oracle, {nx}x{ny}x{nz} grid of the same run, first {steps} steps "
+                  f"timed after set-up ({dt:.1f} s), {threads} thread on '{cpu_model()}' "
+                  f"(nproc {os.cpu_count()})",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
+    ap.add_argument("--size", type=int, default=400)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tiling", type=str, default="")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md, not built.", file=sys.stderr)
+        sys.exit(2)
+
+    from xfdtd_amd.sim import Simulation
+    sc, total = scene_400(n=args.size)
+    if args.warmup + args.steps > total:
+        total = args.warmup + args.steps
+    sim = Simulation(sc, total, precision=args.precision)
+    if args.tiling:
+        tz, ty, tx = (int(v) for v in args.tiling.split(","))
+        sim.set_tiling(tz, ty, tx)
+    nx, ny, nz = sim.dims
+    cells = nx * ny * nz
+    sim.step(args.warmup)
+    sim.sync()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    ms_e, ms_h, nsamp = sim.kernel_times()
+    sim.set_timing(False)
+    value = cells * args.steps / wall / 1e9
+    s = 4 if args.precision == 32 else 8
+    bytes_half = 9 * s + 1  # per cell per half step
+    ms_dom = max(ms_e, ms_h)
+    dom = "k_update_h" if ms_h >= ms_e else "k_update_e"
+    achieved = bytes_half * cells / (ms_dom * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == 32 else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain single-domain dielectric sphere {nx}x{ny}x{nz}, dx=2.5mm, "
+                        "12-cell CPML, eps_r=4 sphere r=100 cells, TFSF Gaussian plane wave, "
+                        "NF-FF 3 freqs",
+            "grid": [nx, ny, nz],
+            "precision": args.precision,
+            "total_steps_of_run": total,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": bytes_half * cells,
+            "avg_ms": {"k_update_e": round(ms_e, 4), "k_update_h": round(ms_h, 4)},
+            "event_samples": nsamp,
+        },
+        "cpu_baseline": None,
+    }
+    sim.close()
+    if not args.no_cpu and args.cpu_steps > 0:
+        line["cpu_baseline"] = run_cpu_baseline(sc, total, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,399 @@
+// xfdtd.hpp -- C++ host API of the MI355X FDTD engine.
+//
+// The classes keep the names, constructor arguments and call order of the
+// reference (Mrwatermolen/XFDTD, src/include/...), so a scatter program written
+// against the reference builds against this header with one change: xtensor
+// arrays become std::vector<double> (xtensor is not a dependency here), and
+// field/coefficient getters return host copies instead of references into
+// host memory, because the fields live in HBM.
+//
+//   Simulation      include/simulation/simulation.h:23-170
+//   Shape/Cube/Sphere include/shape/*.h
+//   Material        include/material/material.h:14-78
+//   Object          include/object/object.h
+//   Boundary/PML    include/boundary/boundary.h, perfect_match_layer.h
+//   Waveform/GaussianWaveform  include/waveform/*.h
+//   TFSF/TFSF3D     include/tfsf/tfsf.h, tfsf_3d.h
+//   NFFFT/NffftFd   include/nffft/nffft.h, nffft_fd.h
+//   LumpedElement/CurrentSource include/lumped_element/*.h
+//
+// Set-up runs on the CPU in FP64 exactly as the reference computes it; each
+// time step runs on the GPU through the C-ABI in include/xfdtd_engine.h.
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <complex>
+#include <cstddef>
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <vector>
+
+struct xf_engine;
+
+namespace xfdtd {
+
+namespace constant {
+inline constexpr double PI{M_PI};
+inline constexpr double EPSILON_0{8.854187817e-12};
+inline constexpr double MU_0{4 * PI * 1e-7};
+inline constexpr double SQUARED_C_0{1 / (EPSILON_0 * MU_0)};
+inline const double ETA_0{std::sqrt(MU_0 / EPSILON_0)};
+inline const double C_0{std::sqrt(SQUARED_C_0)};
+inline constexpr double TOLERABLE_EPSILON{10 * std::numeric_limits<double>::epsilon()};
+}  // namespace constant
+
+using PointVector = std::array<double, 3>;
+using SpatialIndex = size_t;
+enum class Orientation { XN, XP, YN, YP, ZN, ZP };
+enum class Axis { X, Y, Z };
+enum class EMComponent { EX, EY, EZ, HX, HY, HZ };
+
+// ------------------------------------------------------------------ shapes
+class Shape {
+ public:
+  virtual ~Shape() = default;
+  virtual bool isPointInside(const PointVector &point) const = 0;
+  virtual std::unique_ptr<Shape> getWrappedBox() const = 0;
+  virtual std::unique_ptr<Shape> clone() const = 0;
+};
+
+class Cube : public Shape {
+ public:
+  Cube(PointVector point, PointVector size) : _point{point}, _size{size} {}
+  bool isPointInside(const PointVector &point) const override;
+  std::unique_ptr<Shape> getWrappedBox() const override { return std::make_unique<Cube>(*this); }
+  std::unique_ptr<Shape> clone() const override { return std::make_unique<Cube>(*this); }
+  double getXmin() const { return _point[0]; }
+  double getYmin() const { return _point[1]; }
+  double getZmin() const { return _point[2]; }
+  double getXmax() const { return _point[0] + _size[0]; }
+  double getYmax() const { return _point[1] + _size[1]; }
+  double getZmax() const { return _point[2] + _size[2]; }
+  PointVector getSize() const { return _size; }
+  PointVector getPoint() const { return _point; }
+
+ private:
+  PointVector _point, _size;
+};
+
+class Sphere : public Shape {
+ public:
+  Sphere(PointVector center, double radius) : _center{center}, _radius{radius} {}
+  bool isPointInside(const PointVector &point) const override;
+  std::unique_ptr<Shape> getWrappedBox() const override;
+  std::unique_ptr<Shape> clone() const override { return std::make_unique<Sphere>(*this); }
+  double getRadius() const { return _radius; }
+  PointVector getCenter() const { return _center; }
+
+ private:
+  PointVector _center;
+  double _radius;
+};
+
+// --------------------------------------------------------------- material
+class Material {
+ public:
+  Material(std::string_view name, float eps_r, float mu_r, double sigma_e, double sigma_m,
+           bool is_dispersion = false)
+      : _name{name}, _eps_r{eps_r}, _mu_r{mu_r}, _eps{_eps_r * constant::EPSILON_0},
+        _mu{_mu_r * constant::MU_0}, _sigma_e{sigma_e}, _sigma_m{sigma_m},
+        _is_dispersion{is_dispersion} {}
+  static std::unique_ptr<Material> createAir(const std::string &name = "XFDTD_AIR") {
+    return std::make_unique<Material>(name, 1, 1, 0, 0);
+  }
+  static std::unique_ptr<Material> createPEC() {
+    return std::make_unique<Material>("XFDTD_PEC", 1, 1, 1e10, 0);
+  }
+  double getPermittivityE() const { return _eps; }
+  double getPermeabilityM() const { return _mu; }
+  double getElectricalConductivity() const { return _sigma_e; }
+  double getMagneticConductivity() const { return _sigma_m; }
+  bool isDispersion() const { return _is_dispersion; }
+  const std::string &name() const { return _name; }
+
+ private:
+  std::string _name;
+  float _eps_r, _mu_r;
+  double _eps, _mu, _sigma_e, _sigma_m;
+  bool _is_dispersion;
+};
+
+class Object {
+ public:
+  Object(std::string_view name, std::unique_ptr<Shape> shape, Material material)
+      : _name{name}, _shape{std::move(shape)}, _material{std::make_unique<Material>(material)} {}
+  Object(std::string_view name, std::unique_ptr<Shape> shape, std::unique_ptr<Material> material)
+      : _name{name}, _shape{std::move(shape)}, _material{std::move(material)} {}
+  std::unique_ptr<Shape> getWrappedBox() const { return _shape->getWrappedBox(); }
+  const Shape *getShape() const { return _shape.get(); }
+  const Material &getMaterial() const { return *_material; }
+  const std::string &getName() const { return _name; }
+
+ private:
+  std::string _name;
+  std::unique_ptr<Shape> _shape;
+  std::unique_ptr<Material> _material;
+};
+
+// --------------------------------------------------------------- waveform
+class Waveform {
+ public:
+  explicit Waveform(double amplitude = 1.0) : _amplitude{amplitude} {}
+  virtual ~Waveform() = default;
+  virtual std::unique_ptr<Waveform> clone() const = 0;
+  virtual double getValueByTime(double time) const = 0;
+  virtual void init(const std::vector<double> &time_array);
+  double operator()(size_t step) const { return _value.at(step); }
+  const std::vector<double> &getValue() const { return _value; }
+  double getAmplitude() const { return _amplitude; }
+  void setAmplitude(double amplitude) { _amplitude = amplitude; }
+
+ private:
+  double _amplitude;
+  std::vector<double> _value;
+};
+
+class GaussianWaveform : public Waveform {
+ public:
+  GaussianWaveform(double amplitude, double tau, double t_0)
+      : Waveform{amplitude}, _tau{tau}, _t_0{t_0} {}
+  GaussianWaveform(double tau, double t_0) : _tau{tau}, _t_0{t_0} {}
+  std::unique_ptr<Waveform> clone() const override { return std::make_unique<GaussianWaveform>(*this); }
+  double getValueByTime(double time) const override;
+
+ private:
+  double _tau, _t_0;
+};
+
+// --------------------------------------------------------------- boundary
+class Boundary {
+ public:
+  virtual ~Boundary() = default;
+  virtual SpatialIndex getSize() const = 0;
+  virtual Orientation getOrientation() const = 0;
+};
+
+class PML : public Boundary {
+ public:
+  PML(Orientation orientation, int thickness, int order = 4, double sigma_ratio = 1,
+      double alpha_min = 0, double alpha_max = 0.05, double kappa_max = 10)
+      : _orientation{orientation}, _thickness{thickness}, _order{order}, _sigma_ratio{sigma_ratio},
+        _alpha_min{alpha_min}, _alpha_max{alpha_max}, _kappa_max{kappa_max} {}
+  SpatialIndex getSize() const override { return (SpatialIndex)_thickness; }
+  Orientation getOrientation() const override { return _orientation; }
+  bool isOrientatingPositive() const {
+    return _orientation == Orientation::XP || _orientation == Orientation::YP ||
+           _orientation == Orientation::ZP;
+  }
+  // profile tables of perfect_match_layer.cpp:133-155 / 302-325 (per layer)
+  struct Profile {
+    std::vector<double> b_e, a_e, b_m, a_m, kappa_e, kappa_m;
+  };
+  Profile computeProfile(double dl, double dt) const;
+
+ private:
+  Orientation _orientation;
+  int _thickness, _order;
+  double _sigma_ratio, _alpha_min, _alpha_max, _kappa_max;
+};
+
+// ------------------------------------------------------------------- TFSF
+class TFSF {
+ public:
+  TFSF(SpatialIndex distance_x, SpatialIndex distance_y, SpatialIndex distance_z, double e_0,
+       double theta_inc, double phi_inc, double psi, std::shared_ptr<Waveform> waveform);
+  virtual ~TFSF() = default;
+  double getIncidentFieldAmplitude() const { return _e_0; }
+  double getIncidentTheta() const { return _theta_inc; }
+  double getIncidentPhi() const { return _phi_inc; }
+  double getIncidentPsi() const { return _psi; }
+  PointVector getKVector() const { return _k; }
+  std::array<SpatialIndex, 3> getDistance() const { return {_distance_x, _distance_y, _distance_z}; }
+  // tfsf/tfsf.cpp:63-66 with util/dft.h
+  std::vector<std::complex<double>> getIncidentWaveFourierTransform(
+      const std::vector<double> &frequencies) const;
+  const std::vector<double> &getIncidentWaveValue() const { return _waveform->getValue(); }
+  Waveform &waveform() { return *_waveform; }
+  double dt() const { return _dt; }
+
+ protected:
+  SpatialIndex _distance_x, _distance_y, _distance_z;
+  double _e_0, _theta_inc, _phi_inc, _psi;
+  double _sin_theta_inc, _cos_theta_inc, _sin_phi_inc, _cos_phi_inc, _sin_psi, _cos_psi;
+  PointVector _k;
+  std::shared_ptr<Waveform> _waveform;
+  double _dt = 0;
+  friend class Simulation;
+};
+
+class TFSF3D : public TFSF {
+ public:
+  TFSF3D(SpatialIndex distance_x, SpatialIndex distance_y, SpatialIndex distance_z, double e_0,
+         double theta_inc, double phi_inc, double psi, std::shared_ptr<Waveform> waveform);
+  PointVector getTransformE() const { return _transform_e; }
+  PointVector getTransformH() const { return _transform_h; }
+
+ private:
+  PointVector _transform_e, _transform_h;
+  friend class Simulation;
+};
+
+// ------------------------------------------------------------------ NF-FF
+class NFFFT {
+ public:
+  NFFFT(SpatialIndex distance_x, SpatialIndex distance_y, SpatialIndex distance_z,
+        std::string output_dir_path)
+      : _distance_x{distance_x}, _distance_y{distance_y}, _distance_z{distance_z},
+        _output_dir_path{std::move(output_dir_path)} {}
+  virtual ~NFFFT() = default;
+  std::array<SpatialIndex, 3> getDistance() const { return {_distance_x, _distance_y, _distance_z}; }
+  const std::string &getOutputDirPath() const { return _output_dir_path; }
+
+ protected:
+  SpatialIndex _distance_x, _distance_y, _distance_z;
+  std::string _output_dir_path;
+};
+
+// far-field results of NffftFd::calculateFarfield (nffft_fd.cpp:379-421),
+// each [nf][ntheta][nphi] C order
+struct FarField {
+  std::vector<std::complex<double>> n_theta, n_phi, l_theta, l_phi, e_theta, e_phi, h_theta, h_phi;
+  std::vector<double> power_theta, power_phi;
+};
+
+class NffftFd : public NFFFT {
+ public:
+  NffftFd(SpatialIndex distance_x, SpatialIndex distance_y, SpatialIndex distance_z,
+          std::vector<double> frequencies, std::vector<double> theta, std::vector<double> phi,
+          std::string output_dir_path);
+  const std::vector<double> &frequencies() const { return _frequencies; }
+  const std::vector<double> &theta() const { return _theta; }
+  const std::vector<double> &phi() const { return _phi; }
+  const std::vector<double> &waveNumber() const { return _wave_number; }
+
+ private:
+  std::vector<double> _frequencies, _wave_number, _theta, _phi;
+};
+
+// ---------------------------------------------------------- lumped element
+class LumpedElement {
+ public:
+  explicit LumpedElement(std::unique_ptr<Cube> cube) : _cube{std::move(cube)} {}
+  virtual ~LumpedElement() = default;
+  const Cube &getCube() const { return *_cube; }
+
+ protected:
+  std::unique_ptr<Cube> _cube;
+};
+
+class CurrentSource : public LumpedElement {
+ public:
+  CurrentSource(std::unique_ptr<Cube> cube, Orientation orientation, double resistance,
+                std::unique_ptr<Waveform> waveform)
+      : LumpedElement{std::move(cube)}, _orientation{orientation},
+        _resistance{resistance == 0 ? 1e-20 : resistance}, _waveform{std::move(waveform)} {}
+  Orientation getOrientation() const { return _orientation; }
+  double getResistance() const { return _resistance; }
+  Waveform &waveform() { return *_waveform; }
+
+ private:
+  Orientation _orientation;
+  double _resistance;
+  std::unique_ptr<Waveform> _waveform;
+  friend class Simulation;
+};
+
+// ---------------------------------------------------------------- monitor
+// Point sample of one component after every step (the monitor slot of the
+// run loop, simulation_update.cpp:215-219).
+struct Probe {
+  EMComponent component;
+  SpatialIndex i, j, k;
+};
+
+// ------------------------------------------------------------- simulation
+class Simulation {
+ public:
+  inline static constexpr float DEFAULT_CFL{0.99};
+
+  explicit Simulation(double cell_size, float cfl = DEFAULT_CFL);
+  Simulation(double cell_size, std::vector<std::shared_ptr<Object>> objects,
+             std::vector<std::shared_ptr<Boundary>> boundaries, std::unique_ptr<TFSF> tfsf,
+             float cfl = DEFAULT_CFL);
+  Simulation(double cell_size, std::vector<std::shared_ptr<Object>> objects,
+             std::vector<std::shared_ptr<Boundary>> boundaries, std::unique_ptr<TFSF> tfsf,
+             std::unique_ptr<NFFFT> nffft, float cfl = DEFAULT_CFL);
+  ~Simulation();
+  Simulation(const Simulation &) = delete;
+  Simulation &operator=(const Simulation &) = delete;
+
+  void addObject(std::shared_ptr<Object> object);
+  void addBoundary(std::shared_ptr<Boundary> boundary);
+  void addLumpedElement(std::shared_ptr<LumpedElement> lumped_element);
+  void addTFSFSource(std::shared_ptr<TFSF> tfsf);
+  void addNFFFT(std::shared_ptr<NFFFT> nffft);
+  void addProbe(Probe probe);
+
+  // Simulation::run (simulation_update.cpp:5-40): init, the time loop on the
+  // GPU, then NF-FF output when an output directory is set.
+  void run(size_t total_time_steps);
+
+  // The same loop in pieces, for callers that interleave host work.
+  void init(size_t total_time_steps);
+  void step(size_t n);
+  void sync();
+  size_t currentTimeStep() const;
+
+  // FP64 (default; parity with the reference) or FP32 fields.
+  void setPrecision(int bits) { _precision = bits; }
+  int precision() const { return _precision; }
+
+  double getDt() const { return _dt; }
+  SpatialIndex getNx() const { return _n[0]; }
+  SpatialIndex getNy() const { return _n[1]; }
+  SpatialIndex getNz() const { return _n[2]; }
+  std::vector<double> getField(EMComponent c) const;
+  void setField(EMComponent c, const std::vector<double> &v);
+  std::array<size_t, 3> fieldShape(EMComponent c) const;
+
+  // NF-FF accumulators and far field (NffftFd::calculateFarfield)
+  std::vector<std::complex<double>> getNffftAccumulators() const;
+  FarField calculateFarField() const;
+  void outputData() const;  // NffftFd::outputData .npy files
+  std::vector<double> getProbeData() const;  // [n_probes][total_steps]
+
+  xf_engine *engine() const { return _engine; }
+  // node coordinates of the uniform grid (grid_space.cpp:625-638)
+  const std::vector<double> &eNodeCoord(int axis) const { return _ec[axis]; }
+
+ private:
+  double _cell_size;
+  float _cfl;
+  int _precision = 64;
+  std::vector<std::shared_ptr<Object>> _objects;
+  std::vector<std::shared_ptr<Boundary>> _boundaries;
+  std::vector<std::shared_ptr<LumpedElement>> _lumped_elements;
+  std::shared_ptr<TFSF> _tfsf;
+  std::shared_ptr<NFFFT> _nffft;
+  std::vector<Probe> _probes;
+
+  // GridSpace
+  double _min[3], _max[3];
+  size_t _n[3] = {0, 0, 0};
+  std::vector<double> _ec[3], _hc[3], _es[3], _hs[3];
+  double _dt = 0;
+  size_t _total_time_steps = 0;
+  size_t _nffft_box[6] = {};
+  xf_engine *_engine = nullptr;
+
+  void initGridSpace();
+  void resizeSpaceSize(const double nmin[3], const double nmax[3]);
+  size_t handleTransform(int axis, double x) const;
+};
+
+}  // namespace xfdtd

@@ -1,0 +1,180 @@
+/*
+ * xfdtd_engine.h -- C-ABI of the MI355X (gfx950) FDTD field engine.
+ *
+ * This is the device boundary of the drop-in: the C++ host classes
+ * (include/xfdtd/xfdtd.hpp) do the reference's set-up on the CPU and hand the
+ * per-step work to these entry points.  Every call replaces a piece of the
+ * reference's run loop, cited by file:line under the reference's src/:
+ *
+ *   xf_engine_step          Simulation::run loop body        simulation/simulation_update.cpp:9-36
+ *                           (updateE 130-197, correctE 282-290, updateBoundaryE 124-128,
+ *                            updateH 50-108, updateTFSFIncidentField 42-48, correctH 292-300,
+ *                            updateBoundaryH 118-122, updateNFFFT 207-213, updateMonitor 215-219)
+ *   xf_engine_set_materials FDTDBasicCoff arrays             fdtd_basic_coff/fdtd_basic_coff.cpp:31-122
+ *                           + Object::init masks             object/object.cpp:53-73
+ *   xf_engine_set_cpml      PML::init                        boundary/perfect_match_layer.cpp:22-96, 133-432
+ *   xf_engine_set_tfsf      TFSF3D::init                     tfsf/tfsf_3d.cpp:33-151
+ *   xf_engine_set_nffft     NffftFd::init / initDFT          nffft/nffft_fd.cpp:17-106
+ *   xf_engine_add_current_source  CurrentSource::correctE    lumped_element/current_source.cpp:140-163
+ *   xf_engine_add_probe     TimeDomainFieldMonitor sample    monitor/field_monitor.cpp:29-122
+ *   xf_engine_get_field     EMF::getEx..getHz                include/simulation/simulation.h:92-97
+ *   xf_engine_get_nffft     NffftFd accumulators             nffft/nffft_fd.cpp:126-202
+ *   xf_engine_farfield_sums NffftFd::calculateFarfieldX/Y/Z  nffft/nffft_fd.cpp:423-556
+ *
+ * Conventions: status codes, never exceptions; the engine owns device memory,
+ * the caller owns every host buffer; one HIP stream per engine; an engine is
+ * not thread-safe.  Field arrays crossing the boundary are FP64, C order, in
+ * the reference's shapes (simulation/simulation_init.cpp:28-35):
+ *   Ex (nx,ny+1,nz+1)  Ey (nx+1,ny,nz+1)  Ez (nx+1,ny+1,nz)
+ *   Hx (nx+1,ny,nz)    Hy (nx,ny+1,nz)    Hz (nx,ny,nz+1)
+ */
+#ifndef XFDTD_ENGINE_H
+#define XFDTD_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xf_engine xf_engine;
+
+typedef enum {
+  XF_OK = 0,
+  XF_EINVAL = 1,  /* bad argument or shape */
+  XF_ENOMEM = 2,  /* device allocation failed */
+  XF_EDEVICE = 3, /* HIP runtime error, or no usable gfx950 device */
+  XF_ESTATE = 4   /* call out of order (e.g. step before set-up) */
+} xf_status;
+
+enum { XF_EX = 0, XF_EY, XF_EZ, XF_HX, XF_HY, XF_HZ };
+
+/* curl-coefficient tables, each indexed [material][index along the
+   derivative axis].  The reference's 18 per-cell arrays
+   (fdtd_basic_coff.cpp:93-121) are functions of the cell's material and of
+   one node size along the derivative axis, so 6 self terms per material plus
+   these 12 one-dimensional tables reproduce them exactly. */
+enum {
+  XF_CB_EXHY = 0, /* cexhy, z: -(2dt/hz[k])/(2eps+sigma dt)   */
+  XF_CB_EXHZ,     /* cexhz, y:  (2dt/hy[j])/(2eps+sigma dt)   */
+  XF_CB_EYHZ,     /* ceyhz, x: -(2dt/hx[i])/(...)             */
+  XF_CB_EYHX,     /* ceyhx, z:  (2dt/hz[k])/(...)             */
+  XF_CB_EZHX,     /* cezhx, y: -(2dt/hy[j])/(...)             */
+  XF_CB_EZHY,     /* cezhy, x:  (2dt/hx[i])/(...)             */
+  XF_CB_HXEY,     /* chxey, z:  (2dt/ez[k])/(2mu+sigma_m dt)  */
+  XF_CB_HXEZ,     /* chxez, y: -(2dt/ey[j])/(...)             */
+  XF_CB_HYEZ,     /* chyez, x:  (2dt/ex[i])/(...)             */
+  XF_CB_HYEX,     /* chyex, z: -(2dt/ez[k])/(...)             */
+  XF_CB_HZEX,     /* chzex, y:  (2dt/ey[j])/(...)             */
+  XF_CB_HZEY,     /* chzey, x: -(2dt/ex[i])/(...)             */
+  XF_CB_COUNT
+};
+
+typedef struct {
+  int64_t nx, ny, nz; /* cell counts */
+  int precision;      /* 64: FP64 fields (parity mode); 32: FP32 fields */
+  int n_materials;    /* 1..255 */
+} xf_grid_desc;
+
+xf_status xf_engine_create(const xf_grid_desc *g, xf_engine **out);
+void xf_engine_destroy(xf_engine *e);
+/* thread-local message of the last failing call */
+const char *xf_last_error(void);
+
+/* index: material per cell over (nx+1)*(ny+1)*(nz+1), C order; the entries
+   at i==nx, j==ny or k==nz are the padded default cells of the reference's
+   staggered arrays (fdtd_basic_coff.cpp:313-339).
+   ca: [6][n_materials] self coefficients cexe,ceye,ceze,chxh,chyh,chzh.
+   cb: XF_CB_COUNT tables, table t of length n_materials*(n_axis(t)+1), with
+       the CPML kappa division already applied (perfect_match_layer.cpp:187-208).
+   cpsi: same shapes, the coefficient times dl captured before that division
+       (the reference's _c_psi_* arrays); read only inside CPML slabs. */
+xf_status xf_engine_set_materials(xf_engine *e, const uint8_t *index,
+                                  const double *ca, const double *const *cb,
+                                  const double *const *cpsi);
+
+/* CPML along `axis` (0 x, 1 y, 2 z): d_n layers on the N face, d_p on the P
+   face (0 = no face).  b_e/a_e are indexed by the E-node index along the
+   axis (n in [1,d_n] and [N-d_p, N-1]); b_m/a_m by the H index
+   (n in [0,d_n) and [N-d_p, N)).  Length N+1 each.  Faces are applied in
+   x, y, z order, the XN..ZP insertion order of every reference example. */
+xf_status xf_engine_set_cpml(xf_engine *e, int axis, int d_n, int d_p,
+                             const double *b_e, const double *a_e,
+                             const double *b_m, const double *a_m);
+
+typedef struct {
+  int64_t li, ri, lj, rj, lk, rk; /* total-field box: tfsf/tfsf.cpp:45-49 */
+  int64_t aux_size;               /* L: tfsf/tfsf_3d.cpp:53-57 */
+  const double *proj_full[3];     /* lengths (r-l)+1 */
+  const double *proj_half[3];     /* lengths (r-l)+2 */
+  double transform_e[3], transform_h[3];
+  double ceie, ceihi, chih, chiei; /* 1-D grid coefficients */
+  double abc_c1, abc_c2;           /* tfsf/tfsf_3d.cpp:161-165 */
+  double ca[3], cb[3];             /* dt/(eps0 d), dt/(mu0 d) per axis */
+  const double *source;            /* e_0 * g(n dt), n < total_steps */
+  int64_t total_steps;
+} xf_tfsf_desc;
+xf_status xf_engine_set_tfsf(xf_engine *e, const xf_tfsf_desc *d);
+
+typedef struct {
+  int64_t li, ri, lj, rj, lk, rk; /* output box: nffft/nffft.cpp:29-33 */
+  int nf;
+  int64_t total_steps;
+  const double *tj; /* interleaved complex [nf][total_steps], (t+1/2) dt */
+  const double *tm; /* interleaved complex [nf][total_steps], t dt */
+} xf_nffft_desc;
+xf_status xf_engine_set_nffft(xf_engine *e, const xf_nffft_desc *d);
+/* size (complex entries) of the accumulator block returned by get_nffft */
+int64_t xf_engine_nffft_size(const xf_engine *e);
+/* 24 accumulators, interleaved complex, in the order
+   x pair: my_xn,mz_xn,jy_xn,jz_xn,my_xp,mz_xp,jy_xp,jz_xp  [nf][nyb][nzb]
+   y pair: mz_yn,mx_yn,jz_yn,jx_yn,mz_yp,mx_yp,jz_yp,jx_yp  [nf][nzb][nxb]
+   z pair: mx_zn,my_zn,jx_zn,jy_zn,mx_zp,my_zp,jx_zp,jy_zp  [nf][nxb][nyb] */
+xf_status xf_engine_get_nffft(xf_engine *e, double *out);
+/* Far-field face sums for every direction (f, theta, phi), FP64 on device:
+   out [nf][ntheta][nphi][3 pairs][4] interleaved complex, where pair x holds
+   (sum_jy, sum_jz, sum_my, sum_mz), pair y (sum_jz, sum_jx, sum_mz, sum_mx),
+   pair z (sum_jx, sum_jy, sum_mx, sum_my) of nffft_fd.cpp:423-556 before the
+   area factor.  wave_number[nf] = 2 pi f / c0; dl = cell size. */
+xf_status xf_engine_farfield_sums(xf_engine *e, double dl,
+                                  const double *wave_number,
+                                  const double *theta, int ntheta,
+                                  const double *phi, int nphi, double *out);
+
+/* E += coff[cell] * wave[t] over the box [i0,i1)x[j0,j1)x[k0,k1) of component
+   comp (XF_EX/EY/EZ); coff has (i1-i0)(j1-j0)(k1-k0) entries, wave total_steps. */
+xf_status xf_engine_add_current_source(xf_engine *e, int comp, int64_t i0,
+                                       int64_t i1, int64_t j0, int64_t j1,
+                                       int64_t k0, int64_t k1,
+                                       const double *coff, const double *wave,
+                                       int64_t total_steps);
+
+/* point sample of comp at (i,j,k) after every step */
+xf_status xf_engine_add_probe(xf_engine *e, int comp, int64_t i, int64_t j,
+                              int64_t k);
+/* out [n_probes][total_steps] */
+xf_status xf_engine_get_probes(xf_engine *e, double *out, int64_t total_steps);
+
+/* Run n steps of the reference's time loop from the current step; the
+   launches are enqueued on the engine's stream and the call returns at once. */
+xf_status xf_engine_step(xf_engine *e, int64_t n);
+xf_status xf_engine_sync(xf_engine *e);
+int64_t xf_engine_current_step(const xf_engine *e);
+
+xf_status xf_engine_get_field(xf_engine *e, int comp, double *out);
+xf_status xf_engine_set_field(xf_engine *e, int comp, const double *in);
+
+/* average device time (ms) of the E and H update kernels over the steps run
+   since the last reset, measured with HIP events on the engine's stream */
+xf_status xf_engine_kernel_times(xf_engine *e, double *ms_e, double *ms_h,
+                                 int64_t *samples);
+xf_status xf_engine_set_timing(xf_engine *e, int enabled);
+/* launch geometry of the curl kernels: z threads per block (multiple of 64),
+   y rows per block, x cells per block (0 = whole axis) */
+xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

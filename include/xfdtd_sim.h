@@ -1,0 +1,81 @@
+/*
+ * xfdtd_sim.h -- C-ABI over the C++ Simulation of include/xfdtd/xfdtd.hpp.
+ *
+ * The reference is a C++ library with no foreign-language binding; these
+ * entry points are the plain-C form of its construction and run calls
+ * (include/simulation/simulation.h:27-66), for callers that cannot use C++
+ * (ctypes, cgo, JNI, the test harness):
+ *
+ *   xf_sim_create                Simulation(double cell_size, float cfl)   simulation.cpp:20-23
+ *   xf_sim_add_cube / _sphere    addObject(Object(name, Cube|Sphere, Material))  simulation.cpp:115-117
+ *   xf_sim_add_pml               addBoundary(PML(...))                      simulation.cpp:119-121
+ *   xf_sim_set_tfsf3d            addTFSFSource(TFSF3D(..., GaussianWaveform))  simulation.cpp:128-130
+ *   xf_sim_set_nffft_fd          addNFFFT(NffftFd(...))                     simulation.cpp:132-134
+ *   xf_sim_add_current_source    addLumpedElement(CurrentSource(...))       simulation.cpp:123-126
+ *   xf_sim_add_probe             addMonitor (point sample)                  simulation.cpp:136-138
+ *   xf_sim_init / xf_sim_step    run(T) split into init + time loop         simulation_update.cpp:5-40
+ *   xf_sim_get_field             getEx()..getHz()                           simulation.h:92-97
+ *   xf_sim_farfield              NffftFd::calculateFarfield                 nffft_fd.cpp:379-421
+ *   xf_sim_output_nffft          NffftFd::outputData                        nffft_fd.cpp:579-634
+ *
+ * Every call returns 0 on success and -1 on failure; xf_sim_last_error()
+ * then holds the message of the C++ exception the reference would throw.
+ */
+#ifndef XFDTD_SIM_H
+#define XFDTD_SIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xf_sim xf_sim;
+
+const char *xf_sim_last_error(void);
+xf_sim *xf_sim_create(double cell_size, float cfl);
+void xf_sim_destroy(xf_sim *s);
+
+int xf_sim_add_cube(xf_sim *s, const double origin[3], const double size[3], float eps_r,
+                    float mu_r, double sigma_e, double sigma_m);
+int xf_sim_add_sphere(xf_sim *s, const double center[3], double radius, float eps_r, float mu_r,
+                      double sigma_e, double sigma_m);
+int xf_sim_add_pml(xf_sim *s, int orientation, int thickness, int order, double sigma_ratio,
+                   double alpha_min, double alpha_max, double kappa_max);
+int xf_sim_set_tfsf3d(xf_sim *s, const size_t distance[3], double e0, double theta, double phi,
+                      double psi, double amplitude, double tau, double t0);
+int xf_sim_set_nffft_fd(xf_sim *s, const size_t distance[3], const double *freqs, int nf,
+                        const double *theta, int ntheta, const double *phi, int nphi,
+                        const char *output_dir);
+int xf_sim_add_current_source(xf_sim *s, const double origin[3], const double size[3],
+                              int orientation, double resistance, double amplitude, double tau,
+                              double t0);
+int xf_sim_add_probe(xf_sim *s, int comp, size_t i, size_t j, size_t k);
+
+/* precision: 64 (FP64 fields, parity with the reference) or 32 */
+int xf_sim_init(xf_sim *s, size_t total_steps, int precision);
+int xf_sim_step(xf_sim *s, size_t n);
+int xf_sim_sync(xf_sim *s);
+size_t xf_sim_current_step(const xf_sim *s);
+
+int xf_sim_dims(const xf_sim *s, size_t out[3]);
+double xf_sim_dt(const xf_sim *s);
+int xf_sim_get_field(xf_sim *s, int comp, double *out);
+int xf_sim_set_field(xf_sim *s, int comp, const double *in);
+/* accumulators as xf_engine_get_nffft; returns their complex count */
+int64_t xf_sim_nffft_size(const xf_sim *s);
+int xf_sim_get_nffft(xf_sim *s, double *out);
+/* out: 8 complex arrays (n_theta, n_phi, l_theta, l_phi, e_theta, e_phi,
+   h_theta, h_phi) then 2 real arrays (power_theta, power_phi), each
+   [nf][ntheta][nphi] */
+int xf_sim_farfield(xf_sim *s, double *out);
+int xf_sim_output_nffft(xf_sim *s);
+int xf_sim_get_probes(xf_sim *s, double *out);
+/* the engine behind the simulation (valid after xf_sim_init) */
+struct xf_engine *xf_sim_engine(xf_sim *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1377 @@
+/*
+ * xfdtd_oracle.c -- FP64 CPU restatement of the XFDTD reference's 3-D run loop.
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the checker for the HIP engine.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ *
+ * The reference (Mrwatermolen/XFDTD) cannot be built here: its xtensor / xtl /
+ * xtensor-blas / xtensor-fftw / FFTW3 / BLAS / LAPACK dependencies are absent.
+ * This file restates, loop for loop, the reference code on the run-loop path.
+ * Paths below are relative to the reference's src/ directory.
+ *
+ *   grid sizing / coordinates   grid/grid_space.cpp:364-464, 501-549, 576-638, 737-757
+ *   shape masks                 grid/grid_space.cpp:274-304, shape/sphere.cpp:27-30,
+ *                               shape/cube.cpp:27-37, util/float_compare.h:9-24
+ *   materials                   material/material.cpp:7-16, object/object.cpp:53-73
+ *   dt and coefficients         fdtd_basic_coff/fdtd_basic_coff.cpp:11-122, 313-354
+ *   E / H curl updates          simulation/simulation_update.cpp:50-108, 130-197
+ *   run-loop order              simulation/simulation_update.cpp:5-40, simulation_init.cpp:10-22
+ *   CPML                        boundary/perfect_match_layer.cpp:12-718
+ *   TFSF 3-D                    tfsf/tfsf.cpp:15-82, tfsf/tfsf_3d.cpp:13-483
+ *   Gaussian waveform           waveform/gaussian_waveform.cpp:18-24
+ *   NF-FF frequency domain      nffft/nffft.cpp:22-34, nffft/nffft_fd.cpp:17-203, 379-556
+ *   face-centre interpolation   electromagnetic_field/linear_interpolation_emf.cpp:286-421
+ *   current source              lumped_element/current_source.cpp:5-187
+ *
+ * Resolved ambiguities (the reference leaves these to uninitialised memory or
+ * to an xtensor behaviour that no reference test pins):
+ *   - TFSF e_inc/h_inc and the XP/YP CPML psi arrays are never initialised by
+ *     the reference (xarray::resize); they are zero here.
+ *   - NffftFd::calculateFarfield broadcasts the (nf,) wave-number array against
+ *     (nf,ntheta,nphi) arrays, which in xtensor aligns trailing axes; the
+ *     evident intent (one wave number per frequency, axis 0) is used here.
+ *   - CurrentSource's Y branch never assigns _db (current_source.cpp:85-88);
+ *     the evident intent (da = hz[k], db = hx[i], dc = ey[j]) is used.
+ *   - xt::linspace samples are start + step*i with the last sample = stop.
+ *   - the 3-vector dot of TFSF3D's transform (BLAS dgemv) sums left to right.
+ */
+#include "xfdtd_oracle.h"
+
+#include <complex.h>
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* util/constant.h */
+#define XO_PI 3.14159265358979323846
+static const double EPS0 = 8.854187817e-12;
+static double MU0(void) { return 4 * XO_PI * 1e-7; }
+static double C0(void) { return sqrt(1 / (EPS0 * MU0())); }
+static double ETA0(void) { return sqrt(MU0() / EPS0); }
+#define TOL (10 * DBL_EPSILON)
+
+static __thread char g_err[512];
+const char *xo_last_error(void) { return g_err; }
+#define FAIL(...)                                  \
+  do {                                             \
+    snprintf(g_err, sizeof g_err, __VA_ARGS__);    \
+    return -1;                                     \
+  } while (0)
+
+/* util/float_compare.h:14-24 */
+static int is_less_or_equal(double a, double b) {
+  return (fabs(a - b) < TOL) ? 1 : (a < b);
+}
+static int is_greater_or_equal(double a, double b) {
+  return (fabs(a - b) < TOL) ? 1 : (a > b);
+}
+
+typedef struct {
+  double *d;
+  size_t n0, n1, n2;
+} a3;
+#define AT(a, i, j, k) ((a).d[((size_t)(i) * (a).n1 + (size_t)(j)) * (a).n2 + (size_t)(k)])
+
+static int a3_alloc(a3 *a, size_t n0, size_t n1, size_t n2, double fill) {
+  a->n0 = n0;
+  a->n1 = n1;
+  a->n2 = n2;
+  size_t n = n0 * n1 * n2;
+  a->d = (double *)malloc((n ? n : 1) * sizeof(double));
+  if (!a->d) return -1;
+  for (size_t q = 0; q < n; ++q) a->d[q] = fill;
+  return 0;
+}
+static void a3_free(a3 *a) {
+  free(a->d);
+  a->d = NULL;
+}
+
+typedef struct {
+  int kind; /* 0 cube, 1 sphere */
+  double p[3], s[3], r;
+  float eps_r, mu_r;
+  double sigma_e, sigma_m;
+} obj_t;
+
+typedef struct {
+  int ori, d, order;
+  double sigma_ratio, alpha_min, alpha_max, kappa_max;
+  double dl;
+  size_t start;
+  double *be, *ae, *bm, *am, *ke, *km;
+  /* psi and c_psi for the four components (ea, eb, ha, hb), each laid out
+     [layer][t1][t2] over the full transverse extent of the component */
+  double *psi[4], *cpsi[4];
+  size_t tn1[4], tn2[4];
+} pml_t;
+
+typedef struct {
+  size_t dist[3];
+  double e0, theta, phi, psi, amp, tau, t0;
+  size_t li, ri, lj, rj, lk, rk;
+  double te[3], th[3], ratio, scaled_dl, ceihi, chiei, ceie, chih, a, b;
+  size_t L;
+  double *e_inc, *h_inc, *wave;
+  double *px_full, *py_full, *pz_full, *px_half, *py_half, *pz_half;
+} tfsf_t;
+
+typedef struct {
+  size_t dist[3];
+  int nf, nt, np;
+  double *freqs, *theta, *phi, *k;
+  size_t li, ri, lj, rj, lk, rk, nxb, nyb, nzb;
+  double complex *acc; /* 24 current arrays, see nf_off() */
+  double complex *tj, *tm;
+} nffft_t;
+
+typedef struct {
+  double origin[3], size[3];
+  int ori;
+  double R, amp, tau, t0, rf;
+  size_t is, ie, js, je, ks, ke;
+  double *wave, *coff_i;
+} csrc_t;
+
+typedef struct {
+  int comp;
+  size_t i, j, k;
+} probe_t;
+
+struct xo_sim {
+  double dl;
+  float cfl;
+  /* GridSpace */
+  double minc[3], maxc[3];
+  size_t n[3];
+  double *ec[3], *hc[3], *es[3], *hs[3];
+  double dt;
+  size_t T, cur;
+  int inited;
+  obj_t *objs;
+  int nobj;
+  pml_t *pml;
+  int npml;
+  int has_tfsf;
+  tfsf_t tf;
+  int has_nf;
+  nffft_t nf;
+  csrc_t *src;
+  int nsrc;
+  probe_t *probes;
+  int nprobe;
+  double *probe_buf;
+  a3 f[6];    /* Ex Ey Ez Hx Hy Hz */
+  a3 c[18];   /* coefficient arrays */
+};
+
+/* coefficient indices */
+enum {
+  CEXE, CEXHY, CEXHZ, CEYE, CEYHZ, CEYHX, CEZE, CEZHX, CEZHY,
+  CHXH, CHXEY, CHXEZ, CHYH, CHYEZ, CHYEX, CHZH, CHZEX, CHZEY
+};
+
+static void field_shape(const xo_sim *s, int comp, size_t sh[3]) {
+  /* simulation/simulation_init.cpp:28-35 */
+  size_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
+  switch (comp) {
+    case XO_EX: sh[0] = nx; sh[1] = ny + 1; sh[2] = nz + 1; break;
+    case XO_EY: sh[0] = nx + 1; sh[1] = ny; sh[2] = nz + 1; break;
+    case XO_EZ: sh[0] = nx + 1; sh[1] = ny + 1; sh[2] = nz; break;
+    case XO_HX: sh[0] = nx + 1; sh[1] = ny; sh[2] = nz; break;
+    case XO_HY: sh[0] = nx; sh[1] = ny + 1; sh[2] = nz; break;
+    default: sh[0] = nx; sh[1] = ny; sh[2] = nz + 1; break;
+  }
+}
+
+xo_sim *xo_sim_create(double cell_size, float cfl) {
+  xo_sim *s = (xo_sim *)calloc(1, sizeof(xo_sim));
+  if (!s) return NULL;
+  s->dl = cell_size;
+  s->cfl = cfl;
+  /* grid/grid_space.cpp:14-27 */
+  for (int a = 0; a < 3; ++a) {
+    s->minc[a] = DBL_MAX;
+    s->maxc[a] = DBL_MIN;
+    s->n[a] = 0;
+  }
+  return s;
+}
+
+void xo_sim_destroy(xo_sim *s) {
+  if (!s) return;
+  for (int a = 0; a < 3; ++a) {
+    free(s->ec[a]); free(s->hc[a]); free(s->es[a]); free(s->hs[a]);
+  }
+  for (int q = 0; q < 6; ++q) a3_free(&s->f[q]);
+  for (int q = 0; q < 18; ++q) a3_free(&s->c[q]);
+  for (int b = 0; b < s->npml; ++b) {
+    pml_t *p = &s->pml[b];
+    free(p->be); free(p->ae); free(p->bm); free(p->am); free(p->ke); free(p->km);
+    for (int q = 0; q < 4; ++q) { free(p->psi[q]); free(p->cpsi[q]); }
+  }
+  free(s->pml);
+  free(s->objs);
+  tfsf_t *t = &s->tf;
+  free(t->e_inc); free(t->h_inc); free(t->wave);
+  free(t->px_full); free(t->py_full); free(t->pz_full);
+  free(t->px_half); free(t->py_half); free(t->pz_half);
+  nffft_t *nf = &s->nf;
+  free(nf->freqs); free(nf->theta); free(nf->phi); free(nf->k);
+  free(nf->acc); free(nf->tj); free(nf->tm);
+  for (int q = 0; q < s->nsrc; ++q) { free(s->src[q].wave); free(s->src[q].coff_i); }
+  free(s->src);
+  free(s->probes);
+  free(s->probe_buf);
+  free(s);
+}
+
+/* ---------------------------------------------------------------- builder */
+
+static int add_obj(xo_sim *s, obj_t o) {
+  if (s->inited) FAIL("simulation already initialised");
+  obj_t *n = (obj_t *)realloc(s->objs, (size_t)(s->nobj + 1) * sizeof(obj_t));
+  if (!n) FAIL("out of memory");
+  s->objs = n;
+  s->objs[s->nobj++] = o;
+  return 0;
+}
+
+int xo_sim_add_cube(xo_sim *s, const double origin[3], const double size[3],
+                    float eps_r, float mu_r, double sigma_e, double sigma_m) {
+  obj_t o;
+  memset(&o, 0, sizeof o);
+  o.kind = 0;
+  for (int a = 0; a < 3; ++a) { o.p[a] = origin[a]; o.s[a] = size[a]; }
+  o.eps_r = eps_r; o.mu_r = mu_r; o.sigma_e = sigma_e; o.sigma_m = sigma_m;
+  return add_obj(s, o);
+}
+
+int xo_sim_add_sphere(xo_sim *s, const double center[3], double radius,
+                      float eps_r, float mu_r, double sigma_e, double sigma_m) {
+  obj_t o;
+  memset(&o, 0, sizeof o);
+  o.kind = 1;
+  for (int a = 0; a < 3; ++a) o.p[a] = center[a];
+  o.r = radius;
+  o.eps_r = eps_r; o.mu_r = mu_r; o.sigma_e = sigma_e; o.sigma_m = sigma_m;
+  return add_obj(s, o);
+}
+
+int xo_sim_add_pml(xo_sim *s, int orientation, int thickness, int order,
+                   double sigma_ratio, double alpha_min, double alpha_max,
+                   double kappa_max) {
+  if (s->inited) FAIL("simulation already initialised");
+  if (orientation < 0 || orientation > 5 || thickness <= 0) FAIL("bad PML");
+  pml_t *n = (pml_t *)realloc(s->pml, (size_t)(s->npml + 1) * sizeof(pml_t));
+  if (!n) FAIL("out of memory");
+  s->pml = n;
+  pml_t *p = &s->pml[s->npml++];
+  memset(p, 0, sizeof *p);
+  p->ori = orientation; p->d = thickness; p->order = order;
+  p->sigma_ratio = sigma_ratio; p->alpha_min = alpha_min;
+  p->alpha_max = alpha_max; p->kappa_max = kappa_max;
+  return 0;
+}
+
+int xo_sim_set_tfsf(xo_sim *s, const size_t distance[3], double e0,
+                    double theta, double phi, double psi, double amplitude,
+                    double tau, double t0) {
+  if (s->inited) FAIL("simulation already initialised");
+  tfsf_t *t = &s->tf;
+  memset(t, 0, sizeof *t);
+  for (int a = 0; a < 3; ++a) t->dist[a] = distance[a];
+  t->e0 = e0; t->theta = theta; t->phi = phi; t->psi = psi;
+  t->amp = amplitude; t->tau = tau; t->t0 = t0;
+  s->has_tfsf = 1;
+  return 0;
+}
+
+int xo_sim_set_nffft(xo_sim *s, const size_t distance[3], const double *freqs,
+                     int nf, const double *theta, int ntheta,
+                     const double *phi, int nphi) {
+  if (s->inited) FAIL("simulation already initialised");
+  nffft_t *f = &s->nf;
+  memset(f, 0, sizeof *f);
+  for (int a = 0; a < 3; ++a) f->dist[a] = distance[a];
+  f->nf = nf; f->nt = ntheta; f->np = nphi;
+  f->freqs = (double *)malloc(sizeof(double) * (size_t)(nf > 0 ? nf : 1));
+  f->k = (double *)malloc(sizeof(double) * (size_t)(nf > 0 ? nf : 1));
+  f->theta = (double *)malloc(sizeof(double) * (size_t)(ntheta > 0 ? ntheta : 1));
+  f->phi = (double *)malloc(sizeof(double) * (size_t)(nphi > 0 ? nphi : 1));
+  for (int q = 0; q < nf; ++q) {
+    f->freqs[q] = freqs[q];
+    /* nffft/nffft_fd.h: _wave_number{2 * PI * _frequencies / C_0} */
+    f->k[q] = 2 * XO_PI * freqs[q] / C0();
+  }
+  for (int q = 0; q < ntheta; ++q) f->theta[q] = theta[q];
+  for (int q = 0; q < nphi; ++q) f->phi[q] = phi[q];
+  s->has_nf = 1;
+  return 0;
+}
+
+int xo_sim_add_current_source(xo_sim *s, const double origin[3],
+                              const double size[3], int orientation,
+                              double resistance, double amplitude, double tau,
+                              double t0) {
+  if (s->inited) FAIL("simulation already initialised");
+  csrc_t *n = (csrc_t *)realloc(s->src, (size_t)(s->nsrc + 1) * sizeof(csrc_t));
+  if (!n) FAIL("out of memory");
+  s->src = n;
+  csrc_t *c = &s->src[s->nsrc++];
+  memset(c, 0, sizeof *c);
+  for (int a = 0; a < 3; ++a) { c->origin[a] = origin[a]; c->size[a] = size[a]; }
+  c->ori = orientation;
+  /* lumped_element/current_source.cpp:12-15 */
+  c->R = (resistance == 0) ? 1e-20 : resistance;
+  c->amp = amplitude; c->tau = tau; c->t0 = t0;
+  return 0;
+}
+
+int xo_sim_add_probe(xo_sim *s, int comp, size_t i, size_t j, size_t k) {
+  if (s->inited) FAIL("simulation already initialised");
+  probe_t *n = (probe_t *)realloc(s->probes, (size_t)(s->nprobe + 1) * sizeof(probe_t));
+  if (!n) FAIL("out of memory");
+  s->probes = n;
+  s->probes[s->nprobe].comp = comp;
+  s->probes[s->nprobe].i = i;
+  s->probes[s->nprobe].j = j;
+  s->probes[s->nprobe].k = k;
+  s->nprobe++;
+  return 0;
+}
+
+/* ------------------------------------------------------------- grid space */
+
+/* grid/grid_space.cpp:576-618 */
+static void resize_space(xo_sim *s, const double nmin[3], const double nmax[3]) {
+  for (int a = 0; a < 3; ++a) {
+    if (s->minc[a] > nmin[a]) s->minc[a] = nmin[a];
+    if (s->maxc[a] < nmax[a]) s->maxc[a] = nmax[a];
+  }
+  for (int a = 0; a < 3; ++a) {
+    size_t nn = (size_t)round((s->maxc[a] - s->minc[a]) / s->dl);
+    if (s->maxc[a] < s->minc[a]) nn = 1;
+    if (nn > s->n[a]) s->n[a] = nn;
+  }
+}
+
+static void wrapped_box(const obj_t *o, double mn[3], double mx[3]) {
+  if (o->kind == 0) {
+    /* shape/cube.cpp:80-82; cube getXmax = point + size */
+    for (int a = 0; a < 3; ++a) { mn[a] = o->p[a]; mx[a] = o->p[a] + o->s[a]; }
+  } else {
+    /* shape/sphere.cpp:32-36 */
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = o->p[a] - o->r;
+      mx[a] = mn[a] + 2 * o->r;
+    }
+  }
+}
+
+static int is_inside(const obj_t *o, const double pt[3]) {
+  if (o->kind == 0) {
+    /* shape/cube.cpp:27-37 */
+    return is_less_or_equal(o->p[0], pt[0]) && is_less_or_equal(o->p[1], pt[1]) &&
+           is_less_or_equal(o->p[2], pt[2]) &&
+           is_greater_or_equal(o->p[0] + o->s[0], pt[0]) &&
+           is_greater_or_equal(o->p[1] + o->s[1], pt[1]) &&
+           is_greater_or_equal(o->p[2] + o->s[2], pt[2]);
+  }
+  /* shape/sphere.cpp:27-30 */
+  double dx = pt[0] - o->p[0], dy = pt[1] - o->p[1], dz = pt[2] - o->p[2];
+  double dist = sqrt(dx * dx + dy * dy + dz * dz);
+  return is_less_or_equal(dist, o->r);
+}
+
+static int init_grid(xo_sim *s) {
+  /* simulation/simulation_init.cpp:105-146 */
+  for (int q = 0; q < s->nobj; ++q) {
+    double mn[3], mx[3];
+    wrapped_box(&s->objs[q], mn, mx);
+    resize_space(s, mn, mx);
+  }
+  for (int b = 0; b < s->npml; ++b) {
+    /* grid/grid_space.cpp:364-464 */
+    int ori = s->pml[b].ori;
+    int a = ori / 2;
+    double size = s->pml[b].d * s->dl;
+    if (s->n[a] <= 1) FAIL("GridSpace is not extended along axis %d", a);
+    double mn[3] = {s->minc[0], s->minc[1], s->minc[2]};
+    double mx[3] = {s->maxc[0], s->maxc[1], s->maxc[2]};
+    if (ori % 2 == 0) mn[a] = s->minc[a] - size; else mx[a] = s->maxc[a] + size;
+    resize_space(s, mn, mx);
+  }
+  /* grid/grid_space.cpp:501-536: only the 3-D branch is on this path */
+  for (int a = 0; a < 3; ++a) {
+    if (s->n[a] < 1) s->n[a] = 1;
+    if (s->n[a] <= 1) FAIL("only 3-D grids are supported (axis %d has %zu cells)", a, s->n[a]);
+  }
+  /* grid/grid_space.cpp:625-638, 737-757 */
+  for (int a = 0; a < 3; ++a) {
+    size_t n = s->n[a];
+    s->ec[a] = (double *)malloc(sizeof(double) * (n + 1));
+    s->hc[a] = (double *)malloc(sizeof(double) * n);
+    s->es[a] = (double *)malloc(sizeof(double) * n);
+    s->hs[a] = (double *)malloc(sizeof(double) * (n + 1));
+    for (size_t i = 0; i <= n; ++i) s->ec[a][i] = s->minc[a] + (double)i * s->dl;
+    for (size_t i = 0; i < n; ++i) s->hc[a][i] = (s->ec[a][i] + s->ec[a][i + 1]) / 2;
+    for (size_t i = 0; i < n; ++i) s->es[a][i] = s->ec[a][i + 1] - s->ec[a][i];
+    /* virtual h nodes: [h0 - dl, h..., h_{n-1} + dl] */
+    for (size_t i = 0; i <= n; ++i) {
+      double lo = (i == 0) ? s->hc[a][0] - s->dl : s->hc[a][i - 1];
+      double hi = (i == n) ? s->hc[a][n - 1] + s->dl : s->hc[a][i];
+      s->hs[a][i] = hi - lo;
+    }
+  }
+  return 0;
+}
+
+/* grid/grid_space.cpp:551-574: first minimum of |e_node - x| */
+static int transform(const xo_sim *s, int a, double x, size_t *out) {
+  if (x < s->minc[a]) FAIL("coordinate out of range");
+  size_t best = 0;
+  double bv = fabs(s->ec[a][0] - x);
+  for (size_t i = 1; i <= s->n[a]; ++i) {
+    double v = fabs(s->ec[a][i] - x);
+    if (v < bv) { bv = v; best = i; }
+  }
+  *out = best;
+  return 0;
+}
+
+/* ------------------------------------------------------------ coefficients */
+
+/* fdtd_basic_coff/fdtd_basic_coff.cpp:77-91 */
+static void calc_coff_e(a3 cece, a3 cecha, a3 cechb, a3 eps, a3 sig, double dt,
+                        const double *da, int da_ax, const double *db, int db_ax) {
+  for (size_t i = 0; i < eps.n0; ++i)
+    for (size_t j = 0; j < eps.n1; ++j)
+      for (size_t k = 0; k < eps.n2; ++k) {
+        size_t idx[3] = {i, j, k};
+        double e = AT(eps, i, j, k), sg = AT(sig, i, j, k);
+        double a = da[idx[da_ax]], b = db[idx[db_ax]];
+        AT(cece, i, j, k) = (2 * e - dt * sg) / (2 * e + dt * sg);
+        AT(cecha, i, j, k) = -(2 * dt / b) / (2 * e + sg * dt);
+        AT(cechb, i, j, k) = (2 * dt / a) / (2 * e + sg * dt);
+      }
+}
+
+static void calc_coff_h(a3 chch, a3 chcea, a3 chceb, a3 mu, a3 sm, double dt,
+                        const double *da, int da_ax, const double *db, int db_ax) {
+  for (size_t i = 0; i < mu.n0; ++i)
+    for (size_t j = 0; j < mu.n1; ++j)
+      for (size_t k = 0; k < mu.n2; ++k) {
+        size_t idx[3] = {i, j, k};
+        double m = AT(mu, i, j, k), sg = AT(sm, i, j, k);
+        double a = da[idx[da_ax]], b = db[idx[db_ax]];
+        AT(chch, i, j, k) = (2 * m - dt * sg) / (2 * m + dt * sg);
+        AT(chcea, i, j, k) = (2 * dt / b) / (2 * m + sg * dt);
+        AT(chceb, i, j, k) = -(2 * dt / a) / (2 * m + sg * dt);
+      }
+}
+
+/* ----------------------------------------------------------------- CPML */
+
+/* which field arrays a face acts on: perfect_match_layer.cpp:98-126
+   X: (ea,eb,ha,hb) = (Ey,Ez,Hy,Hz); Y: (Ez,Ex,Hz,Hx); Z: (Ex,Ey,Hx,Hy) */
+static const int PML_COMP[3][4] = {{XO_EY, XO_EZ, XO_HY, XO_HZ},
+                                   {XO_EZ, XO_EX, XO_HZ, XO_HX},
+                                   {XO_EX, XO_EY, XO_HX, XO_HY}};
+/* coefficient arrays (ceahb, cebha, chaeb, chbea): perfect_match_layer.cpp:62-95 */
+static const int PML_COEF[3][4] = {{CEYHZ, CEZHY, CHYEZ, CHZEY},
+                                   {CEZHX, CEXHZ, CHZEX, CHXEZ},
+                                   {CEXHY, CEYHX, CHXEY, CHYEX}};
+
+/* element (layer n along axis a, transverse t1,t2) of a field array */
+static double *fa_at(a3 *f, int a, size_t n, size_t t1, size_t t2) {
+  size_t i, j, k;
+  if (a == 0) { i = n; j = t1; k = t2; }
+  else if (a == 1) { i = t1; j = n; k = t2; }
+  else { i = t1; j = t2; k = n; }
+  return &AT(*f, i, j, k);
+}
+static void transverse_extent(const a3 *f, int a, size_t *t1, size_t *t2) {
+  if (a == 0) { *t1 = f->n1; *t2 = f->n2; }
+  else if (a == 1) { *t1 = f->n0; *t2 = f->n2; }
+  else { *t1 = f->n0; *t2 = f->n1; }
+}
+
+static int init_pml(xo_sim *s, pml_t *p) {
+  /* perfect_match_layer.cpp:49-96 */
+  int a = p->ori / 2;
+  int positive = p->ori % 2;
+  p->dl = s->dl;
+  p->start = positive ? s->n[a] - (size_t)p->d : 0;
+  int d = p->d;
+  p->be = (double *)malloc(sizeof(double) * (size_t)d);
+  p->ae = (double *)malloc(sizeof(double) * (size_t)d);
+  p->bm = (double *)malloc(sizeof(double) * (size_t)d);
+  p->am = (double *)malloc(sizeof(double) * (size_t)d);
+  p->ke = (double *)malloc(sizeof(double) * (size_t)d);
+  p->km = (double *)malloc(sizeof(double) * (size_t)d);
+  /* perfect_match_layer.cpp:133-155 (P) and 302-325 (N) */
+  double e2m = MU0() / EPS0;
+  double sigma_max = p->sigma_ratio * (p->order + 1) / (150.0 * XO_PI * p->dl);
+  double dt = s->dt;
+  for (int i = 0; i < d; ++i) {
+    double rho_e, rho_m;
+    if (positive) {
+      rho_e = (i + 1 - 0.75) / (double)d;
+      rho_m = (i + 1 - 0.25) / (double)d;
+    } else {
+      rho_e = (d - i - 0.75) / (double)d;
+      rho_m = (d - i - 0.25) / (double)d;
+    }
+    double sigma_e = sigma_max * pow(rho_e, p->order);
+    double sigma_m = sigma_max * pow(rho_m, p->order);
+    sigma_m = sigma_m * e2m;
+    double kappa_e = 1 + (p->kappa_max - 1) * pow(rho_e, p->order);
+    double kappa_m = 1 + (p->kappa_max - 1) * pow(rho_m, p->order);
+    double alpha_e = p->alpha_min + (p->alpha_max - p->alpha_min) * (1 - rho_e);
+    double alpha_m = p->alpha_min + (p->alpha_max - p->alpha_min) * (1 - rho_m);
+    alpha_m = alpha_m * e2m;
+    p->be[i] = exp((-dt / EPS0) * (sigma_e / kappa_e + alpha_e));
+    p->ae[i] = (1 / p->dl) * (p->be[i] - 1) * sigma_e /
+               (kappa_e * (sigma_e + kappa_e * alpha_e));
+    p->bm[i] = exp((-dt / MU0()) * (sigma_m / kappa_m + alpha_m));
+    p->am[i] = (1 / p->dl) * (p->bm[i] - 1) * sigma_m /
+               (kappa_m * (sigma_m + kappa_m * alpha_m));
+    p->ke[i] = kappa_e;
+    p->km[i] = kappa_m;
+  }
+  /* psi arrays (zero) and c_psi capture + kappa division:
+     perfect_match_layer.cpp:157-299 (P) and 327-431 (N).  The E arrays sit at
+     start+l+1 on N faces and start+l on P faces; H arrays at start+l. */
+  for (int q = 0; q < 4; ++q) {
+    a3 *f = &s->f[PML_COMP[a][q]];
+    a3 *cf = &s->c[PML_COEF[a][q]];
+    size_t t1, t2;
+    transverse_extent(f, a, &t1, &t2);
+    p->tn1[q] = t1;
+    p->tn2[q] = t2;
+    size_t sz = (size_t)d * t1 * t2;
+    p->psi[q] = (double *)calloc(sz ? sz : 1, sizeof(double));
+    p->cpsi[q] = (double *)calloc(sz ? sz : 1, sizeof(double));
+    if (!p->psi[q] || !p->cpsi[q]) FAIL("out of memory");
+    int is_e = q < 2;
+    for (int l = 0; l < d; ++l) {
+      size_t n = p->start + (size_t)l + ((is_e && !positive) ? 1 : 0);
+      double kap = is_e ? p->ke[l] : p->km[l];
+      for (size_t u = 0; u < t1; ++u)
+        for (size_t v = 0; v < t2; ++v) {
+          double *cc = fa_at(cf, a, n, u, v);
+          p->cpsi[q][((size_t)l * t1 + u) * t2 + v] = *cc * p->dl;
+          *cc = *cc / kap;
+        }
+    }
+  }
+  return 0;
+}
+
+/* perfect_match_layer.cpp:574-718 */
+static void pml_update_e(xo_sim *s, pml_t *p) {
+  int a = p->ori / 2;
+  int positive = p->ori % 2;
+  /* ea uses hb, eb uses ha */
+  for (int q = 0; q < 2; ++q) {
+    a3 *e = &s->f[PML_COMP[a][q]];
+    a3 *h = &s->f[PML_COMP[a][q == 0 ? 3 : 2]];
+    size_t t1 = p->tn1[q], t2 = p->tn2[q];
+    for (int l = 0; l < p->d; ++l) {
+      size_t n = p->start + (size_t)l + (positive ? 0 : 1);
+      for (size_t u = 0; u < t1; ++u)
+        for (size_t v = 0; v < t2; ++v) {
+          size_t pi = ((size_t)l * t1 + u) * t2 + v;
+          double *ps = &p->psi[q][pi];
+          *ps = p->be[l] * *ps + p->ae[l] * (*fa_at(h, a, n, u, v) - *fa_at(h, a, n - 1, u, v));
+          double *ev = fa_at(e, a, n, u, v);
+          *ev = *ev + p->cpsi[q][pi] * *ps;
+        }
+    }
+  }
+}
+
+/* perfect_match_layer.cpp:434-572 */
+static void pml_update_h(xo_sim *s, pml_t *p) {
+  int a = p->ori / 2;
+  /* ha uses eb, hb uses ea */
+  for (int q = 3; q >= 2; --q) {
+    a3 *h = &s->f[PML_COMP[a][q]];
+    a3 *e = &s->f[PML_COMP[a][q == 3 ? 0 : 1]];
+    size_t t1 = p->tn1[q], t2 = p->tn2[q];
+    for (int l = 0; l < p->d; ++l) {
+      size_t n = p->start + (size_t)l;
+      for (size_t u = 0; u < t1; ++u)
+        for (size_t v = 0; v < t2; ++v) {
+          size_t pi = ((size_t)l * t1 + u) * t2 + v;
+          double *ps = &p->psi[q][pi];
+          *ps = p->bm[l] * *ps + p->am[l] * (*fa_at(e, a, n + 1, u, v) - *fa_at(e, a, n, u, v));
+          double *hv = fa_at(h, a, n, u, v);
+          *hv = *hv + p->cpsi[q][pi] * *ps;
+        }
+    }
+  }
+}
+
+/* ----------------------------------------------------------------- TFSF */
+
+static double gaussian(double amp, double tau, double t0, double t) {
+  /* waveform/gaussian_waveform.cpp:22-24 */
+  return amp * exp(-pow((t - t0) / tau, 2));
+}
+
+static int init_tfsf(xo_sim *s) {
+  tfsf_t *t = &s->tf;
+  /* tfsf/tfsf.cpp:15-33, 35-53 */
+  double st = sin(t->theta), ct = cos(t->theta), sp = sin(t->phi), cp = cos(t->phi);
+  double sps = sin(t->psi), cps = cos(t->psi);
+  double kv[3] = {st * cp, st * sp, ct};
+  /* tfsf/tfsf_3d.cpp:18-30 */
+  double u[3][3] = {{-sp, ct * cp, st * cp}, {cp, ct * sp, st * sp}, {0, -st, ct}};
+  double ke[3] = {sps, cps, 0};
+  for (int r = 0; r < 3; ++r) t->te[r] = u[r][0] * ke[0] + u[r][1] * ke[1] + u[r][2] * ke[2];
+  t->th[0] = kv[1] * t->te[2] - kv[2] * t->te[1];
+  t->th[1] = kv[2] * t->te[0] - kv[0] * t->te[2];
+  t->th[2] = kv[0] * t->te[1] - kv[1] * t->te[0];
+  size_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
+  if (2 * t->dist[0] >= nx || 2 * t->dist[1] >= ny || 2 * t->dist[2] >= nz ||
+      t->dist[0] < 1 || t->dist[1] < 1 || t->dist[2] < 1)
+    FAIL("TFSF box does not fit the grid");
+  t->li = t->dist[0]; t->ri = nx - t->dist[0];
+  t->lj = t->dist[1]; t->rj = ny - t->dist[1];
+  t->lk = t->dist[2]; t->rk = nz - t->dist[2];
+  size_t bx = t->ri - t->li, by = t->rj - t->lj, bz = t->rk - t->lk;
+  t->wave = (double *)malloc(sizeof(double) * (s->T ? s->T : 1));
+  for (size_t n = 0; n < s->T; ++n)
+    t->wave[n] = gaussian(t->amp, t->tau, t->t0, (double)n * s->dt);
+  /* tfsf/tfsf_3d.cpp:33-151 */
+  t->ratio = 1 / sqrt(pow(sin(t->theta), 4) * (pow(cos(t->phi), 4) + pow(sin(t->phi), 4)) +
+                      pow(cos(t->theta), 4));
+  double diag = sqrt(pow((double)bx, 2) + pow((double)by, 2) + pow((double)bz, 2)) * t->ratio;
+  t->L = (size_t)ceil(diag) + 4 + 1;
+  t->e_inc = (double *)calloc(t->L, sizeof(double));
+  t->h_inc = (double *)calloc(t->L - 1, sizeof(double));
+  double extra[3] = {2 * kv[0] / t->ratio, 2 * kv[1] / t->ratio, 2 * kv[2] / t->ratio};
+  double org[3];
+  org[0] = (kv[0] >= 0) ? (double)t->li : (double)t->ri;
+  org[1] = (kv[1] >= 0) ? (double)t->lj : (double)t->rj;
+  org[2] = (kv[2] >= 0) ? (double)t->lk : (double)t->rk;
+  for (int a = 0; a < 3; ++a) org[a] -= extra[a];
+  size_t bn[3] = {bx, by, bz}, st3[3] = {t->li, t->lj, t->lk};
+  double *full[3], *half[3];
+  for (int a = 0; a < 3; ++a) {
+    full[a] = (double *)malloc(sizeof(double) * (bn[a] + 1));
+    half[a] = (double *)malloc(sizeof(double) * (bn[a] + 2));
+    for (size_t i = 0; i < bn[a] + 1; ++i)
+      full[a][i] = ((double)(i + st3[a]) - org[a]) * kv[a] * t->ratio;
+    for (size_t i = 0; i < bn[a] + 2; ++i)
+      half[a][i] = ((double)(i + st3[a]) - org[a] - 0.5) * kv[a] * t->ratio;
+  }
+  t->px_full = full[0]; t->py_full = full[1]; t->pz_full = full[2];
+  t->px_half = half[0]; t->py_half = half[1]; t->pz_half = half[2];
+  t->scaled_dl = s->dl / t->ratio;
+  t->ceihi = -(s->dt / (EPS0 * t->scaled_dl));
+  t->chiei = -(s->dt / (MU0() * t->scaled_dl));
+  t->ceie = 1.0;
+  t->chih = 1.0;
+  t->a = 0;
+  t->b = 0;
+  return 0;
+}
+
+/* tfsf/tfsf_3d.cpp:153-183 */
+static void tfsf_update_incident(xo_sim *s, size_t step) {
+  tfsf_t *t = &s->tf;
+  double dt = s->dt;
+  size_t L = t->L;
+  double *e = t->e_inc, *h = t->h_inc;
+  e[0] = t->e0 * t->wave[step];
+  double x = e[L - 2], y = e[L - 1];
+  for (size_t i = 1; i < L - 1; ++i) e[i] = t->ceie * e[i] + t->ceihi * (h[i] - h[i - 1]);
+  double c1 = (C0() * dt - t->scaled_dl) / (C0() * dt + t->scaled_dl);
+  double c2 = 2 * t->scaled_dl / (C0() * dt + t->scaled_dl);
+  e[L - 1] = -t->a + c1 * (e[L - 2] + t->b) + c2 * (x + y);
+  t->a = x;
+  t->b = y;
+  for (size_t i = 0; i < L - 1; ++i) h[i] = t->chih * h[i] + t->chiei * (e[i + 1] - e[i]);
+}
+
+/* tfsf/tfsf_3d.cpp:413-483: comp 0..5 = Ex..Hz incident value at (i,j,k) */
+static double tfsf_inc(const tfsf_t *t, int comp, size_t i, size_t j, size_t k) {
+  double proj;
+  const double *arr;
+  double scale;
+  switch (comp) {
+    case XO_EX:
+      proj = t->px_half[i - t->li + 1] + t->py_full[j - t->lj] + t->pz_full[k - t->lk];
+      arr = t->e_inc; scale = t->te[0]; break;
+    case XO_EY:
+      proj = t->px_full[i - t->li] + t->py_half[j - t->lj + 1] + t->pz_full[k - t->lk];
+      arr = t->e_inc; scale = t->te[1]; break;
+    case XO_EZ:
+      proj = t->px_full[i - t->li] + t->py_full[j - t->lj] + t->pz_half[k - t->lk + 1];
+      arr = t->e_inc; scale = t->te[2]; break;
+    case XO_HX:
+      proj = t->px_full[i - t->li] + t->py_half[j - t->lj + 1] + t->pz_half[k - t->lk + 1] - 0.5;
+      arr = t->h_inc; scale = t->th[0]; break;
+    case XO_HY:
+      proj = t->px_half[i - t->li + 1] + t->py_full[j - t->lj] + t->pz_half[k - t->lk + 1] - 0.5;
+      arr = t->h_inc; scale = t->th[1]; break;
+    default:
+      proj = t->px_half[i - t->li + 1] + t->py_half[j - t->lj + 1] + t->pz_full[k - t->lk] - 0.5;
+      arr = t->h_inc; scale = t->th[2]; break;
+  }
+  size_t idx = (size_t)proj;
+  double w = proj - (double)idx;
+  return (1 - w) * (scale * arr[idx]) + w * (scale * arr[idx + 1]);
+}
+
+/* tfsf/tfsf_3d.cpp:185-297 */
+static void tfsf_update_h(xo_sim *s) {
+  tfsf_t *t = &s->tf;
+  a3 *hx = &s->f[XO_HX], *hy = &s->f[XO_HY], *hz = &s->f[XO_HZ];
+  size_t li = t->li, ri = t->ri, lj = t->lj, rj = t->rj, lk = t->lk, rk = t->rk;
+  double cbx = s->dt / (MU0() * s->dl), cby = cbx, cbz = cbx;
+  for (size_t j = lj; j < rj; ++j)
+    for (size_t k = lk; k < rk + 1; ++k) AT(*hz, li - 1, j, k) += cbx * tfsf_inc(t, XO_EY, li, j, k);
+  for (size_t j = lj; j < rj + 1; ++j)
+    for (size_t k = lk; k < rk; ++k) AT(*hy, li - 1, j, k) -= cbx * tfsf_inc(t, XO_EZ, li, j, k);
+  for (size_t j = lj; j < rj; ++j)
+    for (size_t k = lk; k < rk + 1; ++k) AT(*hz, ri, j, k) -= cbx * tfsf_inc(t, XO_EY, ri, j, k);
+  for (size_t j = lj; j < rj + 1; ++j)
+    for (size_t k = lk; k < rk; ++k) AT(*hy, ri, j, k) += cbx * tfsf_inc(t, XO_EZ, ri, j, k);
+  for (size_t k = lk; k < rk; ++k)
+    for (size_t i = li; i < ri + 1; ++i) AT(*hx, i, lj - 1, k) += cby * tfsf_inc(t, XO_EZ, i, lj, k);
+  for (size_t k = lk; k < rk + 1; ++k)
+    for (size_t i = li; i < ri; ++i) AT(*hz, i, lj - 1, k) -= cby * tfsf_inc(t, XO_EX, i, lj, k);
+  for (size_t k = lk; k < rk; ++k)
+    for (size_t i = li; i < ri + 1; ++i) AT(*hx, i, rj, k) -= cby * tfsf_inc(t, XO_EZ, i, rj, k);
+  for (size_t k = lk; k < rk + 1; ++k)
+    for (size_t i = li; i < ri; ++i) AT(*hz, i, rj, k) += cby * tfsf_inc(t, XO_EX, i, rj, k);
+  for (size_t i = li; i < ri; ++i)
+    for (size_t j = lj; j < rj + 1; ++j) AT(*hy, i, j, lk - 1) += cbz * tfsf_inc(t, XO_EX, i, j, lk);
+  for (size_t i = li; i < ri + 1; ++i)
+    for (size_t j = lj; j < rj; ++j) AT(*hx, i, j, lk - 1) -= cbz * tfsf_inc(t, XO_EY, i, j, lk);
+  for (size_t i = li; i < ri; ++i)
+    for (size_t j = lj; j < rj + 1; ++j) AT(*hy, i, j, rk) -= cbz * tfsf_inc(t, XO_EX, i, j, rk);
+  for (size_t i = li; i < ri + 1; ++i)
+    for (size_t j = lj; j < rj; ++j) AT(*hx, i, j, rk) += cbz * tfsf_inc(t, XO_EY, i, j, rk);
+}
+
+/* tfsf/tfsf_3d.cpp:299-411 */
+static void tfsf_update_e(xo_sim *s) {
+  tfsf_t *t = &s->tf;
+  a3 *ex = &s->f[XO_EX], *ey = &s->f[XO_EY], *ez = &s->f[XO_EZ];
+  size_t li = t->li, ri = t->ri, lj = t->lj, rj = t->rj, lk = t->lk, rk = t->rk;
+  double cax = s->dt / (EPS0 * s->dl), cay = cax, caz = cax;
+  for (size_t j = lj; j < rj; ++j)
+    for (size_t k = lk; k < rk + 1; ++k) AT(*ey, li, j, k) += cax * tfsf_inc(t, XO_HZ, li - 1, j, k);
+  for (size_t j = lj; j < rj + 1; ++j)
+    for (size_t k = lk; k < rk; ++k) AT(*ez, li, j, k) -= cax * tfsf_inc(t, XO_HY, li - 1, j, k);
+  for (size_t j = lj; j < rj; ++j)
+    for (size_t k = lk; k < rk + 1; ++k) AT(*ey, ri, j, k) -= cax * tfsf_inc(t, XO_HZ, ri, j, k);
+  for (size_t j = lj; j < rj + 1; ++j)
+    for (size_t k = lk; k < rk; ++k) AT(*ez, ri, j, k) += cax * tfsf_inc(t, XO_HY, ri, j, k);
+  for (size_t k = lk; k < rk; ++k)
+    for (size_t i = li; i < ri + 1; ++i) AT(*ez, i, lj, k) += cay * tfsf_inc(t, XO_HX, i, lj - 1, k);
+  for (size_t k = lk; k < rk + 1; ++k)
+    for (size_t i = li; i < ri; ++i) AT(*ex, i, lj, k) -= cay * tfsf_inc(t, XO_HZ, i, lj - 1, k);
+  for (size_t k = lk; k < rk; ++k)
+    for (size_t i = li; i < ri + 1; ++i) AT(*ez, i, rj, k) -= cay * tfsf_inc(t, XO_HX, i, rj, k);
+  for (size_t k = lk; k < rk + 1; ++k)
+    for (size_t i = li; i < ri; ++i) AT(*ex, i, rj, k) += cay * tfsf_inc(t, XO_HZ, i, rj, k);
+  for (size_t i = li; i < ri; ++i)
+    for (size_t j = lj; j < rj + 1; ++j) AT(*ex, i, j, lk) += caz * tfsf_inc(t, XO_HY, i, j, lk - 1);
+  for (size_t i = li; i < ri + 1; ++i)
+    for (size_t j = lj; j < rj; ++j) AT(*ey, i, j, lk) -= caz * tfsf_inc(t, XO_HX, i, j, lk - 1);
+  for (size_t i = li; i < ri; ++i)
+    for (size_t j = lj; j < rj + 1; ++j) AT(*ex, i, j, rk) -= caz * tfsf_inc(t, XO_HY, i, j, rk);
+  for (size_t i = li; i < ri + 1; ++i)
+    for (size_t j = lj; j < rj; ++j) AT(*ey, i, j, rk) += caz * tfsf_inc(t, XO_HX, i, j, rk);
+}
+
+/* ---------------------------------------------------------------- NF-FF */
+
+/* accumulator offset of current c (0..7) of face pair a (0 x, 1 y, 2 z).
+   x pair: my_xn, mz_xn, jy_xn, jz_xn, my_xp, mz_xp, jy_xp, jz_xp  (nf, nyb, nzb)
+   y pair: mz_yn, mx_yn, jz_yn, jx_yn, mz_yp, mx_yp, jz_yp, jx_yp  (nf, nzb, nxb)
+   z pair: mx_zn, my_zn, jx_zn, jy_zn, mx_zp, my_zp, jx_zp, jy_zp  (nf, nxb, nyb) */
+static size_t nf_face_cells(const nffft_t *f, int a) {
+  if (a == 0) return f->nyb * f->nzb;
+  if (a == 1) return f->nzb * f->nxb;
+  return f->nxb * f->nyb;
+}
+static size_t nf_off(const nffft_t *f, int a, int c) {
+  size_t off = 0;
+  for (int b = 0; b < a; ++b) off += 8 * (size_t)f->nf * nf_face_cells(f, b);
+  return off + (size_t)c * (size_t)f->nf * nf_face_cells(f, a);
+}
+
+size_t xo_sim_nffft_size(const xo_sim *s) {
+  if (!s->has_nf || !s->inited) return 0;
+  return nf_off(&s->nf, 3, 0);
+}
+
+static int init_nffft(xo_sim *s) {
+  nffft_t *f = &s->nf;
+  /* nffft/nffft.cpp:22-34 */
+  size_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
+  if (2 * f->dist[0] >= nx || 2 * f->dist[1] >= ny || 2 * f->dist[2] >= nz ||
+      f->dist[0] < 1 || f->dist[1] < 1 || f->dist[2] < 1)
+    FAIL("NF-FF box does not fit the grid");
+  f->li = f->dist[0]; f->ri = nx - f->dist[0];
+  f->lj = f->dist[1]; f->rj = ny - f->dist[1];
+  f->lk = f->dist[2]; f->rk = nz - f->dist[2];
+  f->nxb = f->ri - f->li; f->nyb = f->rj - f->lj; f->nzb = f->rk - f->lk;
+  size_t tot = nf_off(f, 3, 0);
+  f->acc = (double complex *)calloc(tot ? tot : 1, sizeof(double complex));
+  /* nffft/nffft_fd.cpp:91-106 */
+  size_t T = s->T;
+  f->tj = (double complex *)malloc(sizeof(double complex) * (size_t)f->nf * (T ? T : 1));
+  f->tm = (double complex *)malloc(sizeof(double complex) * (size_t)f->nf * (T ? T : 1));
+  if (!f->acc || !f->tj || !f->tm) FAIL("out of memory");
+  double dt = s->dt;
+  const double complex mi2pi = ((-0.0 - 1.0 * I) * 2.0) * XO_PI;
+  for (int n = 0; n < f->nf; ++n)
+    for (size_t t = 0; t < T; ++t) {
+      double argj = f->freqs[n] * ((double)t + 0.5) * dt;
+      double argm = f->freqs[n] * (double)t * dt;
+      double complex zj = CMPLX(creal(mi2pi) * argj, cimag(mi2pi) * argj);
+      double complex zm = CMPLX(creal(mi2pi) * argm, cimag(mi2pi) * argm);
+      double complex ej = cexp(zj), em = cexp(zm);
+      f->tj[(size_t)n * T + t] = CMPLX(dt * creal(ej), dt * cimag(ej));
+      f->tm[(size_t)n * T + t] = CMPLX(dt * creal(em), dt * cimag(em));
+    }
+  return 0;
+}
+
+#define EX(i, j, k) AT(s->f[XO_EX], i, j, k)
+#define EY(i, j, k) AT(s->f[XO_EY], i, j, k)
+#define EZ(i, j, k) AT(s->f[XO_EZ], i, j, k)
+#define HX(i, j, k) AT(s->f[XO_HX], i, j, k)
+#define HY(i, j, k) AT(s->f[XO_HY], i, j, k)
+#define HZ(i, j, k) AT(s->f[XO_HZ], i, j, k)
+
+static void acc_add(double complex *a, double v, double complex tw) {
+  *a = CMPLX(creal(*a) + v * creal(tw), cimag(*a) + v * cimag(tw));
+}
+
+/* nffft/nffft_fd.cpp:108-203 with the stencils of
+   electromagnetic_field/linear_interpolation_emf.cpp:286-388 */
+static void nffft_update(xo_sim *s) {
+  nffft_t *f = &s->nf;
+  size_t li = f->li, ri = f->ri, lj = f->lj, rj = f->rj, lk = f->lk, rk = f->rk;
+  size_t nt = s->cur, T = s->T;
+  int NF = f->nf;
+  size_t cx = nf_face_cells(f, 0), cy = nf_face_cells(f, 1), cz = nf_face_cells(f, 2);
+  for (size_t j = lj; j < rj; ++j)
+    for (size_t k = lk; k < rk; ++k) {
+      double v[8];
+      v[0] = -1.0 * (0.5 * (EZ(li, j, k) + EZ(li, j + 1, k)));
+      v[1] = 0.5 * (EY(li, j, k) + EY(li, j, k + 1));
+      v[2] = 0.25 * (HZ(li, j, k) + HZ(li, j, k + 1) + HZ(li - 1, j, k) + HZ(li - 1, j, k + 1));
+      v[3] = -1.0 * (0.25 * (HY(li, j, k) + HY(li, j + 1, k) + HY(li - 1, j, k) + HY(li - 1, j + 1, k)));
+      v[4] = 0.5 * (EZ(ri, j, k) + EZ(ri, j + 1, k));
+      v[5] = -1.0 * (0.5 * (EY(ri, j, k) + EY(ri, j, k + 1)));
+      v[6] = -1.0 * (0.25 * (HZ(ri, j, k) + HZ(ri, j, k + 1) + HZ(ri - 1, j, k) + HZ(ri - 1, j, k + 1)));
+      v[7] = 0.25 * (HY(ri, j, k) + HY(ri, j + 1, k) + HY(ri - 1, j, k) + HY(ri - 1, j + 1, k));
+      size_t cell = (j - lj) * f->nzb + (k - lk);
+      for (int n = 0; n < NF; ++n)
+        for (int c = 0; c < 8; ++c) {
+          double complex tw = ((c % 4) < 2) ? f->tm[(size_t)n * T + nt] : f->tj[(size_t)n * T + nt];
+          acc_add(&f->acc[nf_off(f, 0, c) + (size_t)n * cx + cell], v[c], tw);
+        }
+    }
+  for (size_t k = lk; k < rk; ++k)
+    for (size_t i = li; i < ri; ++i) {
+      double v[8];
+      v[0] = -1.0 * (0.5 * (EX(i, lj, k) + EX(i, lj + 1, k)));
+      v[1] = 0.5 * (EZ(i, lj, k) + EZ(i + 1, lj, k));
+      v[2] = 0.25 * (HX(i + 1, lj, k) + HX(i, lj, k) + HX(i + 1, lj - 1, k) + HX(i, lj - 1, k));
+      v[3] = -1.0 * (0.25 * (HZ(i, lj, k) + HZ(i, lj, k + 1) + HZ(i, lj - 1, k) + HZ(i, lj - 1, k + 1)));
+      v[4] = 0.5 * (EX(i, rj, k) + EX(i, rj + 1, k));
+      v[5] = -1.0 * (0.5 * (EZ(i, rj, k) + EZ(i + 1, rj, k)));
+      v[6] = -1.0 * (0.25 * (HX(i + 1, rj, k) + HX(i, rj, k) + HX(i + 1, rj - 1, k) + HX(i, rj - 1, k)));
+      v[7] = 0.25 * (HZ(i, rj, k) + HZ(i, rj, k + 1) + HZ(i, rj - 1, k) + HZ(i, rj - 1, k + 1));
+      size_t cell = (k - lk) * f->nxb + (i - li);
+      for (int n = 0; n < NF; ++n)
+        for (int c = 0; c < 8; ++c) {
+          double complex tw = ((c % 4) < 2) ? f->tm[(size_t)n * T + nt] : f->tj[(size_t)n * T + nt];
+          acc_add(&f->acc[nf_off(f, 1, c) + (size_t)n * cy + cell], v[c], tw);
+        }
+    }
+  for (size_t i = li; i < ri; ++i)
+    for (size_t j = lj; j < rj; ++j) {
+      double v[8];
+      v[0] = -1.0 * (0.5 * (EY(i, j, lk) + EY(i + 1, j, lk)));
+      v[1] = 0.5 * (EX(i, j, lk) + EX(i, j, lk + 1));
+      v[2] = 0.25 * (HY(i, j, lk) + HY(i, j + 1, lk) + HY(i, j, lk - 1) + HY(i, j + 1, lk - 1));
+      v[3] = -1.0 * (0.25 * (HX(i + 1, j, lk) + HX(i, j, lk) + HX(i + 1, j, lk - 1) + HX(i, j, lk - 1)));
+      v[4] = 0.5 * (EY(i, j, rk) + EY(i + 1, j, rk));
+      v[5] = -1.0 * (0.5 * (EX(i, j, rk) + EX(i, j, rk + 1)));
+      v[6] = -1.0 * (0.25 * (HY(i, j, rk) + HY(i, j + 1, rk) + HY(i, j, rk - 1) + HY(i, j + 1, rk - 1)));
+      v[7] = 0.25 * (HX(i + 1, j, rk) + HX(i, j, rk) + HX(i + 1, j, rk - 1) + HX(i, j, rk - 1));
+      size_t cell = (i - li) * f->nyb + (j - lj);
+      for (int n = 0; n < NF; ++n)
+        for (int c = 0; c < 8; ++c) {
+          double complex tw = ((c % 4) < 2) ? f->tm[(size_t)n * T + nt] : f->tj[(size_t)n * T + nt];
+          acc_add(&f->acc[nf_off(f, 2, c) + (size_t)n * cz + cell], v[c], tw);
+        }
+    }
+}
+
+/* complex product as evaluated for finite operands (a*c-b*d, a*d+b*c) */
+static double complex cmul(double complex x, double complex y) {
+  double a = creal(x), b = cimag(x), c = creal(y), d = cimag(y);
+  return CMPLX(a * c - b * d, a * d + b * c);
+}
+static double complex cadd(double complex x, double complex y) {
+  return CMPLX(creal(x) + creal(y), cimag(x) + cimag(y));
+}
+static double complex cscale(double s, double complex x) {
+  return CMPLX(creal(x) * s, cimag(x) * s);
+}
+static double complex cexpi(double d) {
+  /* std::exp(1i * d) with a zero real part */
+  return cexp(CMPLX(0.0 * d, 1.0 * d));
+}
+
+/* nffft/nffft_fd.cpp:379-556.  out layout: 8 complex arrays
+   (a_theta, a_phi, f_theta, f_phi, e_theta, e_phi, h_theta, h_phi) then
+   2 real arrays (power_theta, power_phi), each nf*ntheta*nphi, C order. */
+int xo_sim_farfield(xo_sim *s, double *out) {
+  if (!s->inited || !s->has_nf) FAIL("no NF-FF surface");
+  nffft_t *f = &s->nf;
+  size_t NF = (size_t)f->nf, NT = (size_t)f->nt, NP = (size_t)f->np, M = NF * NT * NP;
+  double complex *res = (double complex *)calloc(8 * M, sizeof(double complex));
+  double *pw = out + 16 * M;
+  double dl = s->dl;
+  double eta = ETA0();
+  for (size_t n = 0; n < NF; ++n)
+    for (size_t t = 0; t < NT; ++t)
+      for (size_t p = 0; p < NP; ++p) {
+        double cos_t = cos(f->theta[t]), sin_t = sin(f->theta[t]);
+        double cos_p = cos(f->phi[p]), sin_p = sin(f->phi[p]);
+        double stcp = sin_t * cos_p, stsp = sin_t * sin_p;
+        double complex ax = 0, ay = 0, az = 0, fx = 0, fy = 0, fz = 0;
+        double dk = dl * f->k[n];
+        double area = dl * dl;
+        {
+          double complex sjy = 0, sjz = 0, smy = 0, smz = 0;
+          size_t cx = nf_face_cells(f, 0);
+          for (size_t j = f->lj; j < f->rj; ++j)
+            for (size_t k = f->lk; k < f->rk; ++k) {
+              double dn = dk * ((double)f->li * stcp + ((double)j + 0.5) * stsp + ((double)k + 0.5) * cos_t);
+              double dp = dk * ((double)f->ri * stcp + ((double)j + 0.5) * stsp + ((double)k + 0.5) * cos_t);
+              double complex pn = cexpi(dn), pp = cexpi(dp);
+              size_t cell = n * cx + (j - f->lj) * f->nzb + (k - f->lk);
+              double complex *A = f->acc;
+              sjy = cadd(sjy, cadd(cmul(A[nf_off(f, 0, 2) + cell], pn), cmul(A[nf_off(f, 0, 6) + cell], pp)));
+              sjz = cadd(sjz, cadd(cmul(A[nf_off(f, 0, 3) + cell], pn), cmul(A[nf_off(f, 0, 7) + cell], pp)));
+              smy = cadd(smy, cadd(cmul(A[nf_off(f, 0, 0) + cell], pn), cmul(A[nf_off(f, 0, 4) + cell], pp)));
+              smz = cadd(smz, cadd(cmul(A[nf_off(f, 0, 1) + cell], pn), cmul(A[nf_off(f, 0, 5) + cell], pp)));
+            }
+          ay = cadd(ay, cscale(area, sjy)); az = cadd(az, cscale(area, sjz));
+          fy = cadd(fy, cscale(area, smy)); fz = cadd(fz, cscale(area, smz));
+        }
+        {
+          double complex sjz = 0, sjx = 0, smz = 0, smx = 0;
+          size_t cy = nf_face_cells(f, 1);
+          for (size_t k = f->lk; k < f->rk; ++k)
+            for (size_t i = f->li; i < f->ri; ++i) {
+              double dn = dk * (((double)i + 0.5) * stcp + (double)f->lj * stsp + ((double)k + 0.5) * cos_t);
+              double dp = dk * (((double)i + 0.5) * stcp + (double)f->rj * stsp + ((double)k + 0.5) * cos_t);
+              double complex pn = cexpi(dn), pp = cexpi(dp);
+              size_t cell = n * cy + (k - f->lk) * f->nxb + (i - f->li);
+              double complex *A = f->acc;
+              sjz = cadd(sjz, cadd(cmul(A[nf_off(f, 1, 2) + cell], pn), cmul(A[nf_off(f, 1, 6) + cell], pp)));
+              sjx = cadd(sjx, cadd(cmul(A[nf_off(f, 1, 3) + cell], pn), cmul(A[nf_off(f, 1, 7) + cell], pp)));
+              smz = cadd(smz, cadd(cmul(A[nf_off(f, 1, 0) + cell], pn), cmul(A[nf_off(f, 1, 4) + cell], pp)));
+              smx = cadd(smx, cadd(cmul(A[nf_off(f, 1, 1) + cell], pn), cmul(A[nf_off(f, 1, 5) + cell], pp)));
+            }
+          az = cadd(az, cscale(area, sjz)); ax = cadd(ax, cscale(area, sjx));
+          fz = cadd(fz, cscale(area, smz)); fx = cadd(fx, cscale(area, smx));
+        }
+        {
+          double complex sjx = 0, sjy = 0, smx = 0, smy = 0;
+          size_t cz = nf_face_cells(f, 2);
+          for (size_t j = f->lj; j < f->rj; ++j)
+            for (size_t i = f->li; i < f->ri; ++i) {
+              double dn = dk * (((double)i + 0.5) * stcp + ((double)j + 0.5) * stsp + (double)f->lk * cos_t);
+              double dp = dk * (((double)i + 0.5) * stcp + ((double)j + 0.5) * stsp + (double)f->rk * cos_t);
+              double complex pn = cexpi(dn), pp = cexpi(dp);
+              size_t cell = n * cz + (i - f->li) * f->nyb + (j - f->lj);
+              double complex *A = f->acc;
+              sjx = cadd(sjx, cadd(cmul(A[nf_off(f, 2, 2) + cell], pn), cmul(A[nf_off(f, 2, 6) + cell], pp)));
+              sjy = cadd(sjy, cadd(cmul(A[nf_off(f, 2, 3) + cell], pn), cmul(A[nf_off(f, 2, 7) + cell], pp)));
+              smx = cadd(smx, cadd(cmul(A[nf_off(f, 2, 0) + cell], pn), cmul(A[nf_off(f, 2, 4) + cell], pp)));
+              smy = cadd(smy, cadd(cmul(A[nf_off(f, 2, 1) + cell], pn), cmul(A[nf_off(f, 2, 5) + cell], pp)));
+            }
+          ax = cadd(ax, cscale(area, sjx)); ay = cadd(ay, cscale(area, sjy));
+          fx = cadd(fx, cscale(area, smx)); fy = cadd(fy, cscale(area, smy));
+        }
+        double ctcp = cos_t * cos_p, ctsp = cos_t * sin_p;
+        /* cos_t_cos_p * a_x + cos_t_sin_p * a_y - sin_t * a_z */
+        double complex a_phi = cadd(cscale(-sin_p, ax), cscale(cos_p, ay));
+        double complex f_phi = cadd(cscale(-sin_p, fx), cscale(cos_p, fy));
+        double complex a_theta = CMPLX(creal(cscale(ctcp, ax)) + creal(cscale(ctsp, ay)) - sin_t * creal(az),
+                        cimag(cscale(ctcp, ax)) + cimag(cscale(ctsp, ay)) - sin_t * cimag(az));
+        double complex f_theta = CMPLX(creal(cscale(ctcp, fx)) + creal(cscale(ctsp, fy)) - sin_t * creal(fz),
+                        cimag(cscale(ctcp, fx)) + cimag(cscale(ctsp, fy)) - sin_t * cimag(fz));
+        size_t m = (n * NT + t) * NP + p;
+        res[0 * M + m] = a_theta;
+        res[1 * M + m] = a_phi;
+        res[2 * M + m] = f_theta;
+        res[3 * M + m] = f_phi;
+        double kk = f->k[n];
+        double four_pi = 4 * XO_PI;
+        double complex mik = CMPLX(-0.0 * kk, -1.0 * kk), pik = CMPLX(0.0 * kk, 1.0 * kk);
+        double complex z1 = cadd(cscale(eta, a_theta), f_phi);
+        double complex z2 = cadd(cscale(-eta, a_phi), f_theta);
+        double complex z3 = CMPLX(creal(a_phi) - creal(f_theta) / eta, cimag(a_phi) - cimag(f_theta) / eta);
+        double complex z4 = CMPLX(creal(a_theta) + creal(f_phi) / eta, cimag(a_theta) + cimag(f_phi) / eta);
+        double complex w;
+        w = cmul(mik, z1); res[4 * M + m] = CMPLX(creal(w) / four_pi, cimag(w) / four_pi);
+        w = cmul(pik, z2); res[5 * M + m] = CMPLX(creal(w) / four_pi, cimag(w) / four_pi);
+        w = cmul(pik, z3); res[6 * M + m] = CMPLX(creal(w) / four_pi, cimag(w) / four_pi);
+        w = cmul(mik, z4); res[7 * M + m] = CMPLX(creal(w) / four_pi, cimag(w) / four_pi);
+        double coff = pow(kk, 2) / (32 * XO_PI * XO_PI * eta);
+        pw[0 * M + m] = coff * pow(cabs(z1), 2);
+        pw[1 * M + m] = coff * pow(cabs(z2), 2);
+      }
+  for (size_t q = 0; q < 8 * M; ++q) {
+    out[2 * q] = creal(res[q]);
+    out[2 * q + 1] = cimag(res[q]);
+  }
+  free(res);
+  return 0;
+}
+
+/* --------------------------------------------------------- current source */
+
+static int init_csrc(xo_sim *s, csrc_t *c) {
+  /* lumped_element/current_source.cpp:34-116 */
+  size_t st[3], en[3];
+  for (int a = 0; a < 3; ++a) {
+    double lo = c->origin[a], hi = c->origin[a] + c->size[a];
+    if (transform(s, a, lo, &st[a]) || transform(s, a, hi, &en[a])) return -1;
+    if (en[a] < st[a]) FAIL("end < start");
+  }
+  c->is = st[0]; c->ie = en[0]; c->js = st[1]; c->je = en[1]; c->ks = st[2]; c->ke = en[2];
+  int a = c->ori / 2;
+  size_t ni = c->ie - c->is, nj = c->je - c->js, nk = c->ke - c->ks;
+  double rf, af;
+  if (a == 0) {
+    rf = c->R * (double)nk * (double)nj / (double)ni;
+    af = c->amp / (double)(nk * nj);
+  } else if (a == 1) {
+    rf = c->R * (double)nk * (double)ni / (double)nj;
+    af = c->amp / (double)(nk * ni);
+  } else {
+    rf = c->R * (double)ni * (double)nj / (double)nk;
+    af = c->amp / (double)(ni * nj);
+  }
+  if (c->ori % 2 == 0) af *= -1;
+  c->amp = af; /* setAmplitude(_current_amplitude_factor) */
+  c->rf = rf;
+  size_t T = s->T;
+  c->wave = (double *)malloc(sizeof(double) * (T ? T : 1));
+  double start = 0.5 * s->dt, stop = ((double)T - 0.5) * s->dt;
+  double step = (stop - start) / fmax(1.0, (double)T - 1.0);
+  for (size_t n = 0; n < T; ++n) {
+    double tt = (n + 1 == T && T > 1) ? stop : start + step * (double)n;
+    c->wave[n] = gaussian(c->amp, c->tau, c->t0, tt);
+  }
+  c->coff_i = (double *)calloc((ni * nj * nk) > 0 ? (ni * nj * nk) : 1, sizeof(double));
+  return 0;
+}
+
+static void csrc_correct_coff(xo_sim *s, csrc_t *c, a3 *eps_arr, a3 *sig_arr) {
+  /* lumped_element/current_source.cpp:118-138, 167-187 */
+  int a = c->ori / 2;
+  double rf = c->rf;
+  double dt = s->dt;
+  int ce, ca, cb;
+  if (a == 0) { ce = CEXE; ca = CEXHY; cb = CEXHZ; }
+  else if (a == 1) { ce = CEYE; ca = CEYHZ; cb = CEYHX; }
+  else { ce = CEZE; ca = CEZHX; cb = CEZHY; }
+  size_t nj = c->je - c->js, nk = c->ke - c->ks;
+  for (size_t i = c->is; i < c->ie; ++i)
+    for (size_t j = c->js; j < c->je; ++j)
+      for (size_t k = c->ks; k < c->ke; ++k) {
+        double da, db, dc;
+        if (a == 0) { da = s->hs[1][j]; db = s->hs[2][k]; dc = s->es[0][i]; }
+        else if (a == 1) { da = s->hs[2][k]; db = s->hs[0][i]; dc = s->es[1][j]; }
+        else { da = s->hs[0][i]; db = s->hs[1][j]; dc = s->es[2][k]; }
+        double beta = (dt * dc) / (rf * da * db);
+        double e = AT(*eps_arr, i, j, k), sg = AT(*sig_arr, i, j, k);
+        AT(s->c[ce], i, j, k) = (2 * e - dt * sg - beta) / (2 * e + dt * sg + beta);
+        AT(s->c[ca], i, j, k) = -2 * dt / ((2 * e + dt * sg + beta) * db);
+        AT(s->c[cb], i, j, k) = 2 * dt / ((2 * e + dt * sg + beta) * da);
+        c->coff_i[((i - c->is) * nj + (j - c->js)) * nk + (k - c->ks)] =
+            -2 * dt / ((2 * e + dt * sg + beta) * (db * da));
+      }
+}
+
+static void csrc_correct_e(xo_sim *s, csrc_t *c) {
+  /* lumped_element/current_source.cpp:140-163 */
+  int a = c->ori / 2;
+  a3 *f = &s->f[a == 0 ? XO_EX : (a == 1 ? XO_EY : XO_EZ)];
+  double w = c->wave[s->cur];
+  size_t nj = c->je - c->js, nk = c->ke - c->ks;
+  for (size_t i = c->is; i < c->ie; ++i)
+    for (size_t j = c->js; j < c->je; ++j)
+      for (size_t k = c->ks; k < c->ke; ++k)
+        AT(*f, i, j, k) += c->coff_i[((i - c->is) * nj + (j - c->js)) * nk + (k - c->ks)] * w;
+}
+
+/* ------------------------------------------------------------------ init */
+
+int xo_sim_init(xo_sim *s, size_t total_steps) {
+  if (s->inited) FAIL("already initialised");
+  s->T = total_steps;
+  s->cur = 0;
+  if (s->nobj == 0) FAIL("no objects");
+  /* simulation_init.cpp:10-22 */
+  if (init_grid(s)) return -1;
+  size_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
+  /* fdtd_basic_coff.cpp:11-29, 342-346: cfl arrives as float */
+  double cfl = (double)s->cfl;
+  double dx = s->dl, dy = s->dl, dz = s->dl;
+  s->dt = cfl / (C0() * sqrt(1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz)));
+  for (int q = 0; q < 6; ++q) {
+    size_t sh[3];
+    field_shape(s, q, sh);
+    if (a3_alloc(&s->f[q], sh[0], sh[1], sh[2], 0.0)) FAIL("out of memory");
+  }
+  /* fdtd_basic_coff.cpp:313-339: eps/mu/sigma arrays per component */
+  a3 eps[3], mu[3], sg[3], sm[3];
+  for (int q = 0; q < 3; ++q) {
+    size_t she[3], shh[3];
+    field_shape(s, q, she);
+    field_shape(s, q + 3, shh);
+    if (a3_alloc(&eps[q], she[0], she[1], she[2], EPS0) ||
+        a3_alloc(&sg[q], she[0], she[1], she[2], 0.0) ||
+        a3_alloc(&mu[q], shh[0], shh[1], shh[2], MU0()) ||
+        a3_alloc(&sm[q], shh[0], shh[1], shh[2], 0.0))
+      FAIL("out of memory");
+  }
+  /* object/object.cpp:53-73: mask at cell centres, applied at the same
+     (i,j,k) index of every staggered array */
+  for (int q = 0; q < s->nobj; ++q) {
+    const obj_t *o = &s->objs[q];
+    double e = (double)o->eps_r * EPS0, m = (double)o->mu_r * MU0();
+    for (size_t i = 0; i < nx; ++i)
+      for (size_t j = 0; j < ny; ++j)
+        for (size_t k = 0; k < nz; ++k) {
+          double pt[3] = {s->hc[0][i], s->hc[1][j], s->hc[2][k]};
+          if (!is_inside(o, pt)) continue;
+          for (int c = 0; c < 3; ++c) {
+            AT(eps[c], i, j, k) = e;
+            AT(mu[c], i, j, k) = m;
+            AT(sg[c], i, j, k) = o->sigma_e;
+            AT(sm[c], i, j, k) = o->sigma_m;
+          }
+        }
+  }
+  if (s->has_tfsf && init_tfsf(s)) return -1;
+  if (s->has_nf && init_nffft(s)) return -1;
+  for (int q = 0; q < s->nsrc; ++q)
+    if (init_csrc(s, &s->src[q])) return -1;
+  /* fdtd_basic_coff.cpp:31-122 */
+  for (int q = 0; q < 18; ++q) {
+    size_t sh[3];
+    int comp = q / 3;
+    field_shape(s, comp, sh);
+    if (a3_alloc(&s->c[q], sh[0], sh[1], sh[2], 0.0)) FAIL("out of memory");
+  }
+  double dt = s->dt;
+  double **es = s->es, **hs = s->hs;
+  calc_coff_e(s->c[CEXE], s->c[CEXHY], s->c[CEXHZ], eps[0], sg[0], dt, hs[1], 1, hs[2], 2);
+  calc_coff_e(s->c[CEYE], s->c[CEYHZ], s->c[CEYHX], eps[1], sg[1], dt, hs[2], 2, hs[0], 0);
+  calc_coff_e(s->c[CEZE], s->c[CEZHX], s->c[CEZHY], eps[2], sg[2], dt, hs[0], 0, hs[1], 1);
+  calc_coff_h(s->c[CHXH], s->c[CHXEY], s->c[CHXEZ], mu[0], sm[0], dt, es[1], 1, es[2], 2);
+  calc_coff_h(s->c[CHYH], s->c[CHYEZ], s->c[CHYEX], mu[1], sm[1], dt, es[2], 2, es[0], 0);
+  calc_coff_h(s->c[CHZH], s->c[CHZEX], s->c[CHZEY], mu[2], sm[2], dt, es[0], 0, es[1], 1);
+  for (int q = 0; q < s->nsrc; ++q) {
+    int a = s->src[q].ori / 2;
+    csrc_correct_coff(s, &s->src[q], &eps[a], &sg[a]);
+  }
+  for (int q = 0; q < 3; ++q) {
+    a3_free(&eps[q]); a3_free(&mu[q]); a3_free(&sg[q]); a3_free(&sm[q]);
+  }
+  /* simulation_init.cpp:93-97 */
+  for (int b = 0; b < s->npml; ++b)
+    if (init_pml(s, &s->pml[b])) return -1;
+  if (s->nprobe) {
+    s->probe_buf = (double *)calloc((size_t)s->nprobe * (s->T ? s->T : 1), sizeof(double));
+    for (int p = 0; p < s->nprobe; ++p) {
+      size_t sh[3];
+      const probe_t *pr = &s->probes[p];
+      if (pr->comp < 0 || pr->comp > 5) FAIL("bad probe component");
+      field_shape(s, pr->comp, sh);
+      if (pr->i >= sh[0] || pr->j >= sh[1] || pr->k >= sh[2]) FAIL("probe outside the field array");
+    }
+  }
+  s->inited = 1;
+  return 0;
+}
+
+/* ----------------------------------------------------------------- step */
+
+/* simulation/simulation_update.cpp:170-196 */
+static void update_e(xo_sim *s) {
+  size_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
+  a3 *c = s->c;
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < nx; ++i)
+    for (size_t j = 1; j < ny; ++j)
+      for (size_t k = 1; k < nz; ++k)
+        EX(i, j, k) = AT(c[CEXE], i, j, k) * EX(i, j, k) +
+                      AT(c[CEXHY], i, j, k) * (HY(i, j, k) - HY(i, j, k - 1)) +
+                      AT(c[CEXHZ], i, j, k) * (HZ(i, j, k) - HZ(i, j - 1, k));
+#pragma omp parallel for schedule(static)
+  for (size_t j = 0; j < ny; ++j)
+    for (size_t k = 1; k < nz; ++k)
+      for (size_t i = 1; i < nx; ++i)
+        EY(i, j, k) = AT(c[CEYE], i, j, k) * EY(i, j, k) +
+                      AT(c[CEYHX], i, j, k) * (HX(i, j, k) - HX(i, j, k - 1)) +
+                      AT(c[CEYHZ], i, j, k) * (HZ(i, j, k) - HZ(i - 1, j, k));
+#pragma omp parallel for schedule(static)
+  for (size_t k = 0; k < nz; ++k)
+    for (size_t i = 1; i < nx; ++i)
+      for (size_t j = 1; j < ny; ++j)
+        EZ(i, j, k) = AT(c[CEZE], i, j, k) * EZ(i, j, k) +
+                      AT(c[CEZHX], i, j, k) * (HX(i, j, k) - HX(i, j - 1, k)) +
+                      AT(c[CEZHY], i, j, k) * (HY(i, j, k) - HY(i - 1, j, k));
+}
+
+/* simulation/simulation_update.cpp:84-100 */
+static void update_h(xo_sim *s) {
+  size_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
+  a3 *c = s->c;
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < nx; ++i)
+    for (size_t j = 0; j < ny; ++j)
+      for (size_t k = 0; k < nz; ++k) {
+        HX(i, j, k) = AT(c[CHXH], i, j, k) * HX(i, j, k) +
+                      AT(c[CHXEY], i, j, k) * (EY(i, j, k + 1) - EY(i, j, k)) +
+                      AT(c[CHXEZ], i, j, k) * (EZ(i, j + 1, k) - EZ(i, j, k));
+        HY(i, j, k) = AT(c[CHYH], i, j, k) * HY(i, j, k) +
+                      AT(c[CHYEZ], i, j, k) * (EZ(i + 1, j, k) - EZ(i, j, k)) +
+                      AT(c[CHYEX], i, j, k) * (EX(i, j, k + 1) - EX(i, j, k));
+        HZ(i, j, k) = AT(c[CHZH], i, j, k) * HZ(i, j, k) +
+                      AT(c[CHZEX], i, j, k) * (EX(i, j + 1, k) - EX(i, j, k)) +
+                      AT(c[CHZEY], i, j, k) * (EY(i + 1, j, k) - EY(i, j, k));
+      }
+}
+
+int xo_sim_step(xo_sim *s, size_t nsteps) {
+  if (!s->inited) FAIL("not initialised");
+  if (s->cur + nsteps > s->T) FAIL("step beyond total_steps");
+  for (size_t q = 0; q < nsteps; ++q) {
+    /* simulation/simulation_update.cpp:9-36 */
+    update_e(s);
+    for (int c = 0; c < s->nsrc; ++c) csrc_correct_e(s, &s->src[c]);
+    if (s->has_tfsf) tfsf_update_e(s);
+    for (int b = 0; b < s->npml; ++b) pml_update_e(s, &s->pml[b]);
+    update_h(s);
+    if (s->has_tfsf) tfsf_update_incident(s, s->cur);
+    if (s->has_tfsf) tfsf_update_h(s);
+    for (int b = 0; b < s->npml; ++b) pml_update_h(s, &s->pml[b]);
+    if (s->has_nf) nffft_update(s);
+    for (int p = 0; p < s->nprobe; ++p) {
+      const probe_t *pr = &s->probes[p];
+      s->probe_buf[(size_t)p * s->T + s->cur] = AT(s->f[pr->comp], pr->i, pr->j, pr->k);
+    }
+    s->cur++;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------- accessors */
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+/* threads for the loops that are Jacobi-style (bitwise independent of the
+   thread count); 1 reproduces the reference's single-threaded run */
+int xo_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+int xo_sim_dims(const xo_sim *s, size_t out[3]) {
+  for (int a = 0; a < 3; ++a) out[a] = s->n[a];
+  return s->inited ? 0 : -1;
+}
+double xo_sim_dt(const xo_sim *s) { return s->dt; }
+size_t xo_sim_current_step(const xo_sim *s) { return s->cur; }
+
+size_t xo_sim_field_size(const xo_sim *s, int comp) {
+  if (!s->inited || comp < 0 || comp > 5) return 0;
+  return s->f[comp].n0 * s->f[comp].n1 * s->f[comp].n2;
+}
+int xo_sim_get_field(const xo_sim *s, int comp, double *out) {
+  if (!s->inited || comp < 0 || comp > 5) FAIL("bad field request");
+  memcpy(out, s->f[comp].d, xo_sim_field_size(s, comp) * sizeof(double));
+  return 0;
+}
+int xo_sim_set_field(xo_sim *s, int comp, const double *in) {
+  if (!s->inited || comp < 0 || comp > 5) FAIL("bad field request");
+  memcpy(s->f[comp].d, in, xo_sim_field_size(s, comp) * sizeof(double));
+  return 0;
+}
+int xo_sim_get_coeff(const xo_sim *s, int which, double *out) {
+  if (!s->inited || which < 0 || which >= 18) FAIL("bad coefficient request");
+  const a3 *a = &s->c[which];
+  memcpy(out, a->d, a->n0 * a->n1 * a->n2 * sizeof(double));
+  return 0;
+}
+int xo_sim_get_pml_profile(const xo_sim *s, int b, double *out) {
+  if (!s->inited || b < 0 || b >= s->npml) FAIL("bad boundary index");
+  const pml_t *p = &s->pml[b];
+  int d = p->d;
+  for (int i = 0; i < d; ++i) {
+    out[0 * d + i] = p->be[i]; out[1 * d + i] = p->ae[i];
+    out[2 * d + i] = p->bm[i]; out[3 * d + i] = p->am[i];
+    out[4 * d + i] = p->ke[i]; out[5 * d + i] = p->km[i];
+  }
+  return 0;
+}
+size_t xo_sim_tfsf_aux_size(const xo_sim *s) {
+  return (s->inited && s->has_tfsf) ? s->tf.L : 0;
+}
+int xo_sim_get_tfsf_aux(const xo_sim *s, double *e_inc, double *h_inc) {
+  if (!s->inited || !s->has_tfsf) FAIL("no TFSF source");
+  memcpy(e_inc, s->tf.e_inc, s->tf.L * sizeof(double));
+  memcpy(h_inc, s->tf.h_inc, (s->tf.L - 1) * sizeof(double));
+  return 0;
+}
+int xo_sim_get_waveform(const xo_sim *s, double *out) {
+  if (!s->inited || !s->has_tfsf) FAIL("no TFSF source");
+  memcpy(out, s->tf.wave, s->T * sizeof(double));
+  return 0;
+}
+int xo_sim_get_nffft(const xo_sim *s, double *out) {
+  if (!s->inited || !s->has_nf) FAIL("no NF-FF surface");
+  size_t n = xo_sim_nffft_size(s);
+  for (size_t q = 0; q < n; ++q) {
+    out[2 * q] = creal(s->nf.acc[q]);
+    out[2 * q + 1] = cimag(s->nf.acc[q]);
+  }
+  return 0;
+}
+int xo_sim_get_probes(const xo_sim *s, double *out) {
+  if (!s->inited) FAIL("not initialised");
+  memcpy(out, s->probe_buf, (size_t)s->nprobe * s->T * sizeof(double));
+  return 0;
+}

@@ -1,0 +1,92 @@
+/*
+ * xfdtd_oracle.h -- FP64 CPU restatement of the XFDTD reference's 3-D run loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ * No product code links or calls it.
+ *
+ * The builder calls mirror the reference's Simulation API
+ * (include/simulation/simulation.h:27-66): objects, boundaries (PML), one TFSF3D
+ * source, one NffftFd, lumped current sources, then init + step.  Every
+ * numerical routine in xfdtd_oracle.c cites the reference file:line it restates.
+ */
+#ifndef XFDTD_ORACLE_H
+#define XFDTD_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xo_sim xo_sim;
+
+/* Orientation order of include/util/type_define.h:14 */
+enum { XO_XN = 0, XO_XP, XO_YN, XO_YP, XO_ZN, XO_ZP };
+/* field components */
+enum { XO_EX = 0, XO_EY, XO_EZ, XO_HX, XO_HY, XO_HZ };
+
+xo_sim *xo_sim_create(double cell_size, float cfl);
+void xo_sim_destroy(xo_sim *s);
+const char *xo_last_error(void);
+
+/* Object(name, Cube(origin,size) | Sphere(center,radius), Material(eps_r,mu_r,sigma_e,sigma_m)) */
+int xo_sim_add_cube(xo_sim *s, const double origin[3], const double size[3],
+                    float eps_r, float mu_r, double sigma_e, double sigma_m);
+int xo_sim_add_sphere(xo_sim *s, const double center[3], double radius,
+                      float eps_r, float mu_r, double sigma_e, double sigma_m);
+/* PML(orientation, thickness, order, sigma_ratio, alpha_min, alpha_max, kappa_max) */
+int xo_sim_add_pml(xo_sim *s, int orientation, int thickness, int order,
+                   double sigma_ratio, double alpha_min, double alpha_max,
+                   double kappa_max);
+/* TFSF3D(dx,dy,dz,e0,theta,phi,psi, GaussianWaveform(amplitude,tau,t0)) */
+int xo_sim_set_tfsf(xo_sim *s, const size_t distance[3], double e0,
+                    double theta, double phi, double psi, double amplitude,
+                    double tau, double t0);
+/* NffftFd(dx,dy,dz, frequencies, theta, phi) */
+int xo_sim_set_nffft(xo_sim *s, const size_t distance[3], const double *freqs,
+                     int nf, const double *theta, int ntheta,
+                     const double *phi, int nphi);
+/* CurrentSource(Cube, orientation, resistance, GaussianWaveform(amp,tau,t0)) */
+int xo_sim_add_current_source(xo_sim *s, const double origin[3],
+                              const double size[3], int orientation,
+                              double resistance, double amplitude, double tau,
+                              double t0);
+/* point probe sampled after every step (monitor update slot) */
+int xo_sim_add_probe(xo_sim *s, int comp, size_t i, size_t j, size_t k);
+
+int xo_sim_init(xo_sim *s, size_t total_steps);
+/* run `n` steps of the time loop starting at the current step */
+int xo_sim_step(xo_sim *s, size_t n);
+/* run only the E half-step / H half-step of the current step (tests) */
+
+int xo_sim_dims(const xo_sim *s, size_t out[3]);
+double xo_sim_dt(const xo_sim *s);
+size_t xo_sim_field_size(const xo_sim *s, int comp);
+int xo_sim_get_field(const xo_sim *s, int comp, double *out);
+int xo_sim_set_field(xo_sim *s, int comp, const double *in);
+/* 18 coefficient arrays in the order cexe,cexhy,cexhz,ceye,ceyhz,ceyhx,
+   ceze,cezhx,cezhy,chxh,chxey,chxez,chyh,chyez,chyex,chzh,chzex,chzey */
+int xo_sim_get_coeff(const xo_sim *s, int which, double *out);
+/* CPML profile of boundary `b` (insertion order): b_e,a_e,b_m,a_m,kappa_e,kappa_m */
+int xo_sim_get_pml_profile(const xo_sim *s, int b, double *out);
+/* TFSF: aux size and current e_inc/h_inc (sizes L and L-1) */
+size_t xo_sim_tfsf_aux_size(const xo_sim *s);
+int xo_sim_get_tfsf_aux(const xo_sim *s, double *e_inc, double *h_inc);
+int xo_sim_get_waveform(const xo_sim *s, double *out /* total_steps */);
+/* NF-FF accumulators, 24 currents in the order documented in xfdtd_oracle.c */
+size_t xo_sim_nffft_size(const xo_sim *s);
+int xo_sim_get_nffft(const xo_sim *s, double *out /* interleaved re,im */);
+/* far field: out holds 10 complex arrays (n_theta, n_phi, l_theta, l_phi,
+   e_theta, e_phi, h_theta, h_phi) + 2 real (power_theta, power_phi), each of
+   nf*ntheta*nphi, packed as documented in xfdtd_oracle.c */
+int xo_sim_farfield(xo_sim *s, double *out);
+int xo_sim_get_probes(const xo_sim *s, double *out /* nprobe * total_steps */);
+size_t xo_sim_current_step(const xo_sim *s);
+int xo_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

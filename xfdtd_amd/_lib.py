@@ -1,0 +1,83 @@
+"""ctypes loader of xfdtd_amd/libxfdtd_amd.so (the HIP engine + C++ host).
+
+The library is built in-tree by `make -C xfdtd_amd/csrc` (or
+__graft_entry__.build()).  There is no fallback: if it is missing, every entry
+point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxfdtd_amd.so")
+_LIB = None
+
+# every extern "C" symbol declared in include/xfdtd_engine.h and include/xfdtd_sim.h
+ENGINE_SYMBOLS = [
+    "xf_engine_create", "xf_engine_destroy", "xf_last_error", "xf_engine_set_materials",
+    "xf_engine_set_cpml", "xf_engine_set_tfsf", "xf_engine_set_nffft", "xf_engine_nffft_size",
+    "xf_engine_get_nffft", "xf_engine_farfield_sums", "xf_engine_add_current_source",
+    "xf_engine_add_probe", "xf_engine_get_probes", "xf_engine_step", "xf_engine_sync",
+    "xf_engine_current_step", "xf_engine_get_field", "xf_engine_set_field",
+    "xf_engine_kernel_times", "xf_engine_set_timing", "xf_engine_set_tiling",
+]
+SIM_SYMBOLS = [
+    "xf_sim_last_error", "xf_sim_create", "xf_sim_destroy", "xf_sim_add_cube", "xf_sim_add_sphere",
+    "xf_sim_add_pml", "xf_sim_set_tfsf3d", "xf_sim_set_nffft_fd", "xf_sim_add_current_source",
+    "xf_sim_add_probe", "xf_sim_init", "xf_sim_step", "xf_sim_sync", "xf_sim_current_step",
+    "xf_sim_dims", "xf_sim_dt", "xf_sim_get_field", "xf_sim_set_field", "xf_sim_nffft_size",
+    "xf_sim_get_nffft", "xf_sim_farfield", "xf_sim_output_nffft", "xf_sim_get_probes",
+    "xf_sim_engine",
+]
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is not built; run `make -C xfdtd_amd/csrc` "
+            "(or __graft_entry__.build()). There is no CPU fallback.")
+    L = C.CDLL(LIB_PATH)
+    vp, d, sz, i, i64 = C.c_void_p, C.c_double, C.c_size_t, C.c_int, C.c_int64
+    dp = C.POINTER(C.c_double)
+    szp = C.POINTER(C.c_size_t)
+    L.xf_last_error.restype = C.c_char_p
+    L.xf_sim_last_error.restype = C.c_char_p
+    L.xf_sim_create.restype = vp
+    L.xf_sim_create.argtypes = [d, C.c_float]
+    L.xf_sim_destroy.argtypes = [vp]
+    L.xf_sim_add_cube.argtypes = [vp, dp, dp, C.c_float, C.c_float, d, d]
+    L.xf_sim_add_sphere.argtypes = [vp, dp, d, C.c_float, C.c_float, d, d]
+    L.xf_sim_add_pml.argtypes = [vp, i, i, i, d, d, d, d]
+    L.xf_sim_set_tfsf3d.argtypes = [vp, szp, d, d, d, d, d, d, d]
+    L.xf_sim_set_nffft_fd.argtypes = [vp, szp, dp, i, dp, i, dp, i, C.c_char_p]
+    L.xf_sim_add_current_source.argtypes = [vp, dp, dp, i, d, d, d, d]
+    L.xf_sim_add_probe.argtypes = [vp, i, sz, sz, sz]
+    L.xf_sim_init.argtypes = [vp, sz, i]
+    L.xf_sim_step.argtypes = [vp, sz]
+    L.xf_sim_sync.argtypes = [vp]
+    L.xf_sim_current_step.restype = sz
+    L.xf_sim_current_step.argtypes = [vp]
+    L.xf_sim_dims.argtypes = [vp, szp]
+    L.xf_sim_dt.restype = d
+    L.xf_sim_dt.argtypes = [vp]
+    L.xf_sim_get_field.argtypes = [vp, i, dp]
+    L.xf_sim_set_field.argtypes = [vp, i, dp]
+    L.xf_sim_nffft_size.restype = i64
+    L.xf_sim_nffft_size.argtypes = [vp]
+    L.xf_sim_get_nffft.argtypes = [vp, dp]
+    L.xf_sim_farfield.argtypes = [vp, dp]
+    L.xf_sim_output_nffft.argtypes = [vp]
+    L.xf_sim_get_probes.argtypes = [vp, dp]
+    L.xf_sim_engine.restype = vp
+    L.xf_sim_engine.argtypes = [vp]
+    L.xf_engine_kernel_times.argtypes = [vp, dp, dp, C.POINTER(C.c_int64)]
+    L.xf_engine_set_timing.argtypes = [vp, i]
+    L.xf_engine_set_tiling.argtypes = [vp, i, i, i]
+    L.xf_engine_sync.argtypes = [vp]
+    L.xf_engine_step.argtypes = [vp, i64]
+    _LIB = L
+    return L

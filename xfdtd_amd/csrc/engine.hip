@@ -1,0 +1,651 @@
+// engine.hip -- C-ABI of the gfx950 FDTD field engine (include/xfdtd_engine.h).
+//
+// Device state of one simulation and the per-step launch sequence of the
+// reference's run loop (simulation/simulation_update.cpp:9-36):
+//   E curl + CPML-E (fused) -> current sources -> TFSF E faces
+//   -> H curl + CPML-H (fused) -> 1-D incident grid step -> TFSF H faces
+//   -> NF-FF running DFT -> probes
+// The reference applies CPML after the lumped and TFSF corrections; the host
+// validates that the lumped cells and TFSF faces lie outside every CPML slab,
+// where the two orders touch disjoint cells and give identical bits.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine_kernels.hpp"
+#include "xfdtd_engine.h"
+
+using namespace xfk;
+
+static thread_local std::string g_err;
+extern "C" const char *xf_last_error(void) { return g_err.c_str(); }
+
+#define FAILS(code, ...)                                   \
+  do {                                                     \
+    char b_[512];                                          \
+    snprintf(b_, sizeof b_, __VA_ARGS__);                  \
+    g_err = b_;                                            \
+    return code;                                           \
+  } while (0)
+#define HIPCHK(x)                                                              \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) FAILS(XF_EDEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+struct SourceDev {
+  int comp, i0, i1, j0, j1, k0, k1;
+  void *coff, *wave;
+};
+
+}  // namespace
+
+struct xf_engine {
+  xf_grid_desc g{};
+  int nx = 0, ny = 0, nz = 0, pz = 0, nmat = 0, prec = 64;
+  size_t esz = 8;
+  long long si = 0, sj = 0, ncell = 0;
+  hipStream_t stream = nullptr;
+  void *f[6] = {};
+  uint8_t *mat = nullptr;
+  void *ca = nullptr;
+  void *cb[12] = {}, *cpsi[12] = {};
+  bool materials_set = false;
+  int dn[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
+  void *be[3] = {}, *ae[3] = {}, *bm[3] = {}, *am[3] = {};
+  void *psi_e[6] = {}, *psi_h[6] = {};
+  // TFSF
+  bool has_tfsf = false;
+  int tli = 0, tri = 0, tlj = 0, trj = 0, tlk = 0, trk = 0, L = 0;
+  void *pf[3] = {}, *ph[3] = {}, *e_inc = nullptr, *h_inc = nullptr, *ab = nullptr, *src = nullptr;
+  double te[3] = {}, th[3] = {}, ceie = 0, ceihi = 0, chih = 0, chiei = 0, c1 = 0, c2 = 0;
+  double tca[3] = {}, tcb[3] = {};
+  long long tfsf_T = 0;
+  int nEy = 0, nEz = 0, nEx = 0;
+  TfsfHSeg hseg{};
+  // NF-FF
+  bool has_nf = false;
+  Nffft nf{};
+  double2 *acc = nullptr, *tj = nullptr, *tm = nullptr;
+  long long nf_size = 0;
+  // sources, probes
+  std::vector<SourceDev> sources;
+  long long src_T = 0;
+  std::vector<int> probe_comp;
+  std::vector<long long> probe_cell;
+  int *d_probe_comp = nullptr;
+  long long *d_probe_cell = nullptr;
+  double *probe_buf = nullptr;
+  long long probe_T = 0;
+  bool probes_dirty = true;
+  long long cur = 0;
+  long long total = -1;  // smallest total_steps among the attached tables
+  // tiling
+  int tz = 64, ty = 4, tx = 0;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;  // 4 events per timed step: E start/end, H start/end
+  size_t ev_used = 0;
+  std::vector<void *> owned;
+};
+
+static xf_status dev_alloc(xf_engine *e, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t r = hipMalloc(p, bytes);
+  if (r != hipSuccess) FAILS(XF_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(r));
+  e->owned.push_back(*p);
+  // zero on the engine stream and wait, so that later synchronous uploads on
+  // the null stream cannot be overtaken by the fill
+  r = hipMemsetAsync(*p, 0, bytes, e->stream);
+  if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+  if (r != hipSuccess) FAILS(XF_EDEVICE, "hipMemsetAsync: %s", hipGetErrorString(r));
+  return XF_OK;
+}
+
+// upload FP64 host data as the engine's working precision
+static xf_status upload_real(xf_engine *e, void **dst, const double *src, size_t n) {
+  xf_status s = dev_alloc(e, dst, n * e->esz);
+  if (s != XF_OK) return s;
+  if (n == 0) return XF_OK;
+  if (e->prec == 64) {
+    HIPCHK(hipMemcpy(*dst, src, n * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> tmp(n);
+    for (size_t q = 0; q < n; ++q) tmp[q] = (float)src[q];
+    HIPCHK(hipMemcpy(*dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return XF_OK;
+}
+
+static void comp_shape(const xf_engine *e, int comp, int sh[3]) {
+  const int nx = e->nx, ny = e->ny, nz = e->nz;
+  switch (comp) {
+    case XF_EX: sh[0] = nx; sh[1] = ny + 1; sh[2] = nz + 1; break;
+    case XF_EY: sh[0] = nx + 1; sh[1] = ny; sh[2] = nz + 1; break;
+    case XF_EZ: sh[0] = nx + 1; sh[1] = ny + 1; sh[2] = nz; break;
+    case XF_HX: sh[0] = nx + 1; sh[1] = ny; sh[2] = nz; break;
+    case XF_HY: sh[0] = nx; sh[1] = ny + 1; sh[2] = nz; break;
+    default: sh[0] = nx; sh[1] = ny; sh[2] = nz + 1; break;
+  }
+}
+
+extern "C" xf_status xf_engine_create(const xf_grid_desc *g, xf_engine **out) {
+  if (!g || !out) FAILS(XF_EINVAL, "null argument");
+  *out = nullptr;
+  if (g->nx < 2 || g->ny < 2 || g->nz < 2) FAILS(XF_EINVAL, "grid must be 3-D (got %lld,%lld,%lld)",
+                                                 (long long)g->nx, (long long)g->ny, (long long)g->nz);
+  if (g->nx > 100000 || g->ny > 100000 || g->nz > 100000) FAILS(XF_EINVAL, "grid too large");
+  if (g->precision != 32 && g->precision != 64) FAILS(XF_EINVAL, "precision must be 32 or 64");
+  if (g->n_materials < 1 || g->n_materials > 255) FAILS(XF_EINVAL, "n_materials must be 1..255");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) FAILS(XF_EDEVICE, "no HIP device");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, 0));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    FAILS(XF_EDEVICE, "device is %s; this engine is built for gfx950", prop.gcnArchName);
+  xf_engine *e = new xf_engine();
+  e->g = *g;
+  e->nx = (int)g->nx; e->ny = (int)g->ny; e->nz = (int)g->nz;
+  e->prec = g->precision;
+  e->esz = e->prec == 64 ? 8 : 4;
+  e->nmat = g->n_materials;
+  e->pz = ((e->nz + 1) + 31) / 32 * 32;
+  e->sj = e->pz;
+  e->si = (long long)(e->ny + 1) * e->pz;
+  e->ncell = (long long)(e->nx + 1) * e->si;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    FAILS(XF_EDEVICE, "hipStreamCreate failed");
+  }
+  for (int q = 0; q < 6; ++q) {
+    xf_status s = dev_alloc(e, &e->f[q], (size_t)e->ncell * e->esz);
+    if (s != XF_OK) { xf_engine_destroy(e); return s; }
+  }
+  *out = e;
+  return XF_OK;
+}
+
+extern "C" void xf_engine_destroy(xf_engine *e) {
+  if (!e) return;
+  if (e->stream) hipStreamSynchronize(e->stream);
+  for (void *p : e->owned) hipFree(p);
+  for (hipEvent_t ev : e->ev) hipEventDestroy(ev);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+}
+
+extern "C" xf_status xf_engine_set_materials(xf_engine *e, const uint8_t *index, const double *ca,
+                                             const double *const *cb, const double *const *cpsi) {
+  if (!e || !index || !ca || !cb || !cpsi) FAILS(XF_EINVAL, "null argument");
+  const int nx = e->nx, ny = e->ny, nz = e->nz, nm = e->nmat;
+  // material index into the padded layout
+  std::vector<uint8_t> m((size_t)e->ncell, 0);
+  for (int i = 0; i <= nx; ++i)
+    for (int j = 0; j <= ny; ++j) {
+      const uint8_t *src = index + ((size_t)i * (ny + 1) + j) * (nz + 1);
+      uint8_t *dst = m.data() + (size_t)i * e->si + (size_t)j * e->sj;
+      for (int k = 0; k <= nz; ++k) {
+        if (src[k] >= nm) FAILS(XF_EINVAL, "material index %d >= n_materials %d", src[k], nm);
+        dst[k] = src[k];
+      }
+    }
+  void *pm = nullptr;
+  xf_status s = dev_alloc(e, &pm, (size_t)e->ncell);
+  if (s != XF_OK) return s;
+  HIPCHK(hipMemcpy(pm, m.data(), (size_t)e->ncell, hipMemcpyHostToDevice));
+  e->mat = (uint8_t *)pm;
+  if ((s = upload_real(e, &e->ca, ca, 6 * (size_t)nm)) != XF_OK) return s;
+  // table t: derivative axis = z for 0,3,6,9; y for 1,4,7,10; x for 2,5,8,11
+  const int axis_of[12] = {2, 1, 0, 2, 1, 0, 2, 1, 0, 2, 1, 0};
+  const int nax[3] = {nx, ny, nz};
+  for (int t = 0; t < 12; ++t) {
+    size_t n = (size_t)nm * (nax[axis_of[t]] + 1);
+    if ((s = upload_real(e, &e->cb[t], cb[t], n)) != XF_OK) return s;
+    if ((s = upload_real(e, &e->cpsi[t], cpsi[t], n)) != XF_OK) return s;
+  }
+  // default (absent) CPML tables
+  for (int a = 0; a < 3; ++a) {
+    if (!e->be[a]) {
+      std::vector<double> z(nax[a] + 1, 0.0);
+      if ((s = upload_real(e, &e->be[a], z.data(), z.size())) != XF_OK) return s;
+      if ((s = upload_real(e, &e->ae[a], z.data(), z.size())) != XF_OK) return s;
+      if ((s = upload_real(e, &e->bm[a], z.data(), z.size())) != XF_OK) return s;
+      if ((s = upload_real(e, &e->am[a], z.data(), z.size())) != XF_OK) return s;
+    }
+  }
+  for (int q = 0; q < 6; ++q)
+    if (!e->psi_e[q]) {
+      if ((s = dev_alloc(e, &e->psi_e[q], 16)) != XF_OK) return s;
+      if ((s = dev_alloc(e, &e->psi_h[q], 16)) != XF_OK) return s;
+    }
+  e->materials_set = true;
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_set_cpml(xf_engine *e, int axis, int d_n, int d_p, const double *b_e,
+                                        const double *a_e, const double *b_m, const double *a_m) {
+  if (!e || axis < 0 || axis > 2 || !b_e || !a_e || !b_m || !a_m) FAILS(XF_EINVAL, "bad argument");
+  const int nax[3] = {e->nx, e->ny, e->nz};
+  const int N = nax[axis];
+  if (d_n < 0 || d_p < 0 || d_n + d_p >= N) FAILS(XF_EINVAL, "CPML layers %d+%d do not fit %d cells", d_n, d_p, N);
+  e->dn[axis] = d_n;
+  e->dp[axis] = d_p;
+  xf_status s;
+  if ((s = upload_real(e, &e->be[axis], b_e, (size_t)N + 1)) != XF_OK) return s;
+  if ((s = upload_real(e, &e->ae[axis], a_e, (size_t)N + 1)) != XF_OK) return s;
+  if ((s = upload_real(e, &e->bm[axis], b_m, (size_t)N + 1)) != XF_OK) return s;
+  if ((s = upload_real(e, &e->am[axis], a_m, (size_t)N + 1)) != XF_OK) return s;
+  const int Lay = d_n + d_p;
+  const size_t nx1 = e->nx + 1, ny1 = e->ny + 1;
+  size_t n;
+  // psi arrays of the two E and two H components this axis acts on
+  if (axis == 0) n = (size_t)Lay * ny1 * e->pz;       // [L][ny+1][pz]
+  else if (axis == 1) n = nx1 * Lay * e->pz;          // [nx+1][L][pz]
+  else n = nx1 * ny1 * Lay;                           // [nx+1][ny+1][L]
+  static const int e_slots[3][2] = {{P_BX, P_CX}, {P_AY, P_CY}, {P_AZ, P_BZ}};
+  for (int q = 0; q < 2; ++q) {
+    const int sl = e_slots[axis][q];
+    if ((s = dev_alloc(e, &e->psi_e[sl], n * e->esz)) != XF_OK) return s;
+    if ((s = dev_alloc(e, &e->psi_h[sl], n * e->esz)) != XF_OK) return s;
+  }
+  return XF_OK;
+}
+
+static void set_total(xf_engine *e, long long T) {
+  if (e->total < 0 || T < e->total) e->total = T;
+}
+
+extern "C" xf_status xf_engine_set_tfsf(xf_engine *e, const xf_tfsf_desc *d) {
+  if (!e || !d) FAILS(XF_EINVAL, "null argument");
+  if (d->li < 1 || d->lj < 1 || d->lk < 1 || d->ri >= e->nx || d->rj >= e->ny || d->rk >= e->nz ||
+      d->ri <= d->li || d->rj <= d->lj || d->rk <= d->lk)
+    FAILS(XF_EINVAL, "TFSF box outside the grid");
+  if (d->aux_size < 3) FAILS(XF_EINVAL, "TFSF aux size too small");
+  e->tli = (int)d->li; e->tri = (int)d->ri; e->tlj = (int)d->lj; e->trj = (int)d->rj;
+  e->tlk = (int)d->lk; e->trk = (int)d->rk; e->L = (int)d->aux_size;
+  const int bn[3] = {e->tri - e->tli, e->trj - e->tlj, e->trk - e->tlk};
+  xf_status s;
+  for (int a = 0; a < 3; ++a) {
+    if ((s = upload_real(e, &e->pf[a], d->proj_full[a], (size_t)bn[a] + 1)) != XF_OK) return s;
+    if ((s = upload_real(e, &e->ph[a], d->proj_half[a], (size_t)bn[a] + 2)) != XF_OK) return s;
+    e->te[a] = d->transform_e[a];
+    e->th[a] = d->transform_h[a];
+    e->tca[a] = d->ca[a];
+    e->tcb[a] = d->cb[a];
+  }
+  // projections must stay inside the aux arrays for every face cell
+  e->ceie = d->ceie; e->ceihi = d->ceihi; e->chih = d->chih; e->chiei = d->chiei;
+  e->c1 = d->abc_c1; e->c2 = d->abc_c2;
+  if ((s = dev_alloc(e, &e->e_inc, (size_t)e->L * e->esz)) != XF_OK) return s;
+  if ((s = dev_alloc(e, &e->h_inc, (size_t)(e->L - 1) * e->esz)) != XF_OK) return s;
+  if ((s = dev_alloc(e, &e->ab, 2 * e->esz)) != XF_OK) return s;
+  if ((s = upload_real(e, &e->src, d->source, (size_t)d->total_steps)) != XF_OK) return s;
+  e->tfsf_T = d->total_steps;
+  set_total(e, d->total_steps);
+  const int A0 = bn[0] + 1, A1 = bn[1] + 1, A2 = bn[2] + 1;
+  e->nEy = bn[1] * (2 * A0 + 2 * (A2 - 2));
+  e->nEz = bn[2] * (2 * A0 + 2 * (A1 - 2));
+  e->nEx = bn[0] * (2 * A1 + 2 * (A2 - 2));
+  const int ni = bn[0], nj = bn[1], nk = bn[2];
+  const int cnt[12] = {nj * (nk + 1), (nj + 1) * nk, nj * (nk + 1), (nj + 1) * nk,
+                       nk * (ni + 1), (nk + 1) * ni, nk * (ni + 1), (nk + 1) * ni,
+                       ni * (nj + 1), (ni + 1) * nj, ni * (nj + 1), (ni + 1) * nj};
+  e->hseg.off[0] = 0;
+  for (int q = 0; q < 12; ++q) e->hseg.off[q + 1] = e->hseg.off[q] + cnt[q];
+  e->has_tfsf = true;
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_set_nffft(xf_engine *e, const xf_nffft_desc *d) {
+  if (!e || !d || !d->tj || !d->tm) FAILS(XF_EINVAL, "null argument");
+  if (d->li < 1 || d->lj < 1 || d->lk < 1 || d->ri >= e->nx || d->rj >= e->ny || d->rk >= e->nz ||
+      d->ri <= d->li || d->rj <= d->lj || d->rk <= d->lk || d->nf < 1)
+    FAILS(XF_EINVAL, "NF-FF box outside the grid");
+  Nffft &n = e->nf;
+  n.li = (int)d->li; n.ri = (int)d->ri; n.lj = (int)d->lj; n.rj = (int)d->rj;
+  n.lk = (int)d->lk; n.rk = (int)d->rk; n.nf = d->nf;
+  n.nxb = n.ri - n.li; n.nyb = n.rj - n.lj; n.nzb = n.rk - n.lk;
+  n.T_total = d->total_steps;
+  const long long cells[3] = {(long long)n.nyb * n.nzb, (long long)n.nzb * n.nxb, (long long)n.nxb * n.nyb};
+  n.base[0] = 0;
+  n.base[1] = 8LL * n.nf * cells[0];
+  n.base[2] = n.base[1] + 8LL * n.nf * cells[1];
+  e->nf_size = n.base[2] + 8LL * n.nf * cells[2];
+  xf_status s;
+  void *p;
+  if ((s = dev_alloc(e, &p, (size_t)e->nf_size * sizeof(double2))) != XF_OK) return s;
+  e->acc = (double2 *)p;
+  const size_t tw = (size_t)d->nf * d->total_steps;
+  if ((s = dev_alloc(e, &p, tw * sizeof(double2))) != XF_OK) return s;
+  e->tj = (double2 *)p;
+  HIPCHK(hipMemcpy(e->tj, d->tj, tw * sizeof(double2), hipMemcpyHostToDevice));
+  if ((s = dev_alloc(e, &p, tw * sizeof(double2))) != XF_OK) return s;
+  e->tm = (double2 *)p;
+  HIPCHK(hipMemcpy(e->tm, d->tm, tw * sizeof(double2), hipMemcpyHostToDevice));
+  n.tj = e->tj; n.tm = e->tm; n.acc = e->acc;
+  set_total(e, d->total_steps);
+  e->has_nf = true;
+  return XF_OK;
+}
+
+extern "C" int64_t xf_engine_nffft_size(const xf_engine *e) { return e && e->has_nf ? e->nf_size : 0; }
+
+extern "C" xf_status xf_engine_get_nffft(xf_engine *e, double *out) {
+  if (!e || !out || !e->has_nf) FAILS(XF_ESTATE, "no NF-FF surface");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipMemcpy(out, e->acc, (size_t)e->nf_size * sizeof(double2), hipMemcpyDeviceToHost));
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_farfield_sums(xf_engine *e, double dl, const double *wave_number,
+                                             const double *theta, int ntheta, const double *phi,
+                                             int nphi, double *out) {
+  if (!e || !out || !e->has_nf || !wave_number || !theta || !phi || ntheta < 1 || nphi < 1)
+    FAILS(XF_EINVAL, "bad far-field request");
+  const int nd = ntheta * nphi;
+  // direction cosines on the host with the same libm as the reference
+  // (nffft_fd.cpp:383-388)
+  std::vector<double> hs(3 * (size_t)nd), hk(e->nf.nf);
+  for (int t = 0; t < ntheta; ++t)
+    for (int p = 0; p < nphi; ++p) {
+      const double cos_t = cos(theta[t]), sin_t = sin(theta[t]);
+      const double cos_p = cos(phi[p]), sin_p = sin(phi[p]);
+      hs[t * nphi + p] = sin_t * cos_p;
+      hs[nd + t * nphi + p] = sin_t * sin_p;
+      hs[2 * nd + t * nphi + p] = cos_t;
+    }
+  for (int f = 0; f < e->nf.nf; ++f) hk[f] = dl * wave_number[f];
+  double *dd = nullptr, *dk = nullptr;
+  double2 *dout = nullptr;
+  const size_t outn = (size_t)e->nf.nf * nd * 12;
+  HIPCHK(hipMalloc((void **)&dd, hs.size() * sizeof(double)));
+  HIPCHK(hipMalloc((void **)&dk, hk.size() * sizeof(double)));
+  HIPCHK(hipMalloc((void **)&dout, outn * sizeof(double2)));
+  HIPCHK(hipMemcpy(dd, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dk, hk.data(), hk.size() * sizeof(double), hipMemcpyHostToDevice));
+  FarDir fd;
+  fd.stcp = dd; fd.stsp = dd + nd; fd.ct = dd + 2 * nd; fd.dk = dk; fd.nt = ntheta; fd.np = nphi;
+  hipLaunchKernelGGL(k_farfield, dim3(nd, e->nf.nf), dim3(256), 0, e->stream, e->nf, fd, dout);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipMemcpy(out, dout, outn * sizeof(double2), hipMemcpyDeviceToHost));
+  hipFree(dd);
+  hipFree(dk);
+  hipFree(dout);
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_add_current_source(xf_engine *e, int comp, int64_t i0, int64_t i1,
+                                                  int64_t j0, int64_t j1, int64_t k0, int64_t k1,
+                                                  const double *coff, const double *wave,
+                                                  int64_t total_steps) {
+  if (!e || !coff || !wave || comp < XF_EX || comp > XF_EZ) FAILS(XF_EINVAL, "bad source");
+  int sh[3];
+  comp_shape(e, comp, sh);
+  if (i0 < 0 || j0 < 0 || k0 < 0 || i1 > sh[0] || j1 > sh[1] || k1 > sh[2] || i1 <= i0 ||
+      j1 <= j0 || k1 <= k0)
+    FAILS(XF_EINVAL, "source box outside the component array");
+  SourceDev sd;
+  sd.comp = comp; sd.i0 = (int)i0; sd.i1 = (int)i1; sd.j0 = (int)j0; sd.j1 = (int)j1;
+  sd.k0 = (int)k0; sd.k1 = (int)k1;
+  const size_t nc = (size_t)(i1 - i0) * (j1 - j0) * (k1 - k0);
+  xf_status s;
+  if ((s = upload_real(e, &sd.coff, coff, nc)) != XF_OK) return s;
+  if ((s = upload_real(e, &sd.wave, wave, (size_t)total_steps)) != XF_OK) return s;
+  e->sources.push_back(sd);
+  set_total(e, total_steps);
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_add_probe(xf_engine *e, int comp, int64_t i, int64_t j, int64_t k) {
+  if (!e || comp < 0 || comp > 5) FAILS(XF_EINVAL, "bad probe");
+  int sh[3];
+  comp_shape(e, comp, sh);
+  if (i < 0 || j < 0 || k < 0 || i >= sh[0] || j >= sh[1] || k >= sh[2])
+    FAILS(XF_EINVAL, "probe outside the component array");
+  e->probe_comp.push_back(comp);
+  e->probe_cell.push_back((long long)i * e->si + (long long)j * e->sj + k);
+  e->probes_dirty = true;
+  return XF_OK;
+}
+
+static xf_status ensure_probes(xf_engine *e, long long T) {
+  if (e->probe_comp.empty()) return XF_OK;
+  if (!e->probes_dirty && e->probe_T >= T) return XF_OK;
+  const size_t n = e->probe_comp.size();
+  void *p;
+  xf_status s;
+  if ((s = dev_alloc(e, &p, n * sizeof(int))) != XF_OK) return s;
+  e->d_probe_comp = (int *)p;
+  if ((s = dev_alloc(e, &p, n * sizeof(long long))) != XF_OK) return s;
+  e->d_probe_cell = (long long *)p;
+  if ((s = dev_alloc(e, &p, n * (size_t)T * sizeof(double))) != XF_OK) return s;
+  e->probe_buf = (double *)p;
+  HIPCHK(hipMemcpy(e->d_probe_comp, e->probe_comp.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->d_probe_cell, e->probe_cell.data(), n * sizeof(long long), hipMemcpyHostToDevice));
+  e->probe_T = T;
+  e->probes_dirty = false;
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_get_probes(xf_engine *e, double *out, int64_t total_steps) {
+  if (!e || !out) FAILS(XF_EINVAL, "null argument");
+  if (e->probe_comp.empty()) return XF_OK;
+  if (e->probe_buf == nullptr || total_steps != e->probe_T) FAILS(XF_ESTATE, "probe buffer holds %lld steps", e->probe_T);
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipMemcpy(out, e->probe_buf, e->probe_comp.size() * (size_t)total_steps * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  return XF_OK;
+}
+
+template <typename T>
+static Grid<T> make_grid(xf_engine *e) {
+  Grid<T> g;
+  for (int q = 0; q < 6; ++q) g.f[q] = (T *)e->f[q];
+  g.mat = e->mat;
+  g.ca = (const T *)e->ca;
+  for (int q = 0; q < 12; ++q) {
+    g.cb[q] = (const T *)e->cb[q];
+    g.cpsi[q] = (const T *)e->cpsi[q];
+  }
+  for (int a = 0; a < 3; ++a) {
+    g.dn[a] = e->dn[a]; g.dp[a] = e->dp[a];
+    g.be[a] = (const T *)e->be[a]; g.ae[a] = (const T *)e->ae[a];
+    g.bm[a] = (const T *)e->bm[a]; g.am[a] = (const T *)e->am[a];
+  }
+  for (int q = 0; q < 6; ++q) {
+    g.psi_e[q] = (T *)e->psi_e[q];
+    g.psi_h[q] = (T *)e->psi_h[q];
+  }
+  g.nx = e->nx; g.ny = e->ny; g.nz = e->nz; g.pz = e->pz; g.nmat = e->nmat;
+  g.sj = e->sj; g.si = e->si;
+  return g;
+}
+
+template <typename T>
+static Tfsf<T> make_tfsf(xf_engine *e) {
+  Tfsf<T> t;
+  t.li = e->tli; t.ri = e->tri; t.lj = e->tlj; t.rj = e->trj; t.lk = e->tlk; t.rk = e->trk; t.L = e->L;
+  for (int a = 0; a < 3; ++a) {
+    t.pf[a] = (const T *)e->pf[a];
+    t.ph[a] = (const T *)e->ph[a];
+    t.te[a] = (T)e->te[a];
+    t.th[a] = (T)e->th[a];
+    t.ca[a] = (T)e->tca[a];
+    t.cb[a] = (T)e->tcb[a];
+  }
+  t.ceie = (T)e->ceie; t.ceihi = (T)e->ceihi; t.chih = (T)e->chih; t.chiei = (T)e->chiei;
+  t.c1 = (T)e->c1; t.c2 = (T)e->c2;
+  t.e_inc = (T *)e->e_inc; t.h_inc = (T *)e->h_inc; t.ab = (T *)e->ab; t.src = (const T *)e->src;
+  return t;
+}
+
+template <typename T>
+static xf_status run_steps(xf_engine *e, long long n) {
+  Grid<T> g = make_grid<T>(e);
+  Tfsf<T> t{};
+  if (e->has_tfsf) t = make_tfsf<T>(e);
+  const int tz = e->tz, ty = e->ty;
+  const int tx = e->tx > 0 ? e->tx : e->nx + 1;
+  dim3 blk(tz, ty, 1);
+  dim3 grd((e->nz + 1 + tz - 1) / tz, (e->ny + 1 + ty - 1) / ty, (e->nx + 1 + tx - 1) / tx);
+  for (long long q = 0; q < n; ++q) {
+    const long long step = e->cur;
+    hipEvent_t *ev = nullptr;
+    if (e->timing) {
+      if (e->ev_used + 4 > e->ev.size()) {
+        for (int r = 0; r < 64; ++r) {
+          hipEvent_t x;
+          HIPCHK(hipEventCreate(&x));
+          e->ev.push_back(x);
+        }
+      }
+      ev = &e->ev[e->ev_used];
+      e->ev_used += 4;
+    }
+    if (ev) hipEventRecord(ev[0], e->stream);
+    hipLaunchKernelGGL(k_update_e<T>, grd, blk, 0, e->stream, g, tx);
+    if (ev) hipEventRecord(ev[1], e->stream);
+    for (const SourceDev &sd : e->sources) {
+      const int nj = sd.j1 - sd.j0, nk = sd.k1 - sd.k0;
+      const int nc = (sd.i1 - sd.i0) * nj * nk;
+      hipLaunchKernelGGL(k_source<T>, dim3((nc + 255) / 256), dim3(256), 0, e->stream, g.f[sd.comp],
+                         e->si, e->sj, sd.i0, nj, nk, sd.j0, sd.k0, nc, (const T *)sd.coff,
+                         (const T *)sd.wave, step);
+    }
+    if (e->has_tfsf) {
+      const int tot = e->nEy + e->nEz + e->nEx;
+      hipLaunchKernelGGL(k_tfsf_e<T>, dim3((tot + 255) / 256), dim3(256), 0, e->stream, g, t, e->nEy,
+                         e->nEz, e->nEx);
+    }
+    if (ev) hipEventRecord(ev[2], e->stream);
+    hipLaunchKernelGGL(k_update_h<T>, grd, blk, 0, e->stream, g, tx);
+    if (ev) hipEventRecord(ev[3], e->stream);
+    if (e->has_tfsf) {
+      hipLaunchKernelGGL(k_tfsf_ifa<T>, dim3(1), dim3(1024), 0, e->stream, t, step);
+      const int tot = e->hseg.off[12];
+      hipLaunchKernelGGL(k_tfsf_h<T>, dim3((tot + 255) / 256), dim3(256), 0, e->stream, g, t, e->hseg);
+    }
+    if (e->has_nf) {
+      const long long tot = (long long)e->nf.nyb * e->nf.nzb + (long long)e->nf.nzb * e->nf.nxb +
+                            (long long)e->nf.nxb * e->nf.nyb;
+      hipLaunchKernelGGL(k_nffft<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, e->stream, g,
+                         e->nf, step);
+    }
+    if (!e->probe_comp.empty()) {
+      const int np = (int)e->probe_comp.size();
+      hipLaunchKernelGGL(k_probe<T>, dim3((np + 63) / 64), dim3(64), 0, e->stream, g, e->d_probe_comp,
+                         e->d_probe_cell, np, e->probe_buf, e->probe_T, step);
+    }
+    HIPCHK(hipGetLastError());
+    e->cur++;
+  }
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_step(xf_engine *e, int64_t n) {
+  if (!e) FAILS(XF_EINVAL, "null engine");
+  if (!e->materials_set) FAILS(XF_ESTATE, "materials not set");
+  if (n < 0) FAILS(XF_EINVAL, "negative step count");
+  if (e->total >= 0 && e->cur + n > e->total)
+    FAILS(XF_ESTATE, "step %lld + %lld beyond the %lld steps the source tables hold", e->cur, (long long)n,
+          e->total);
+  if (!e->probe_comp.empty()) {
+    if (e->total < 0) FAILS(XF_ESTATE, "probes need a step horizon (a source or NF-FF table)");
+    xf_status s = ensure_probes(e, e->total);
+    if (s != XF_OK) return s;
+  }
+  return e->prec == 64 ? run_steps<double>(e, n) : run_steps<float>(e, n);
+}
+
+extern "C" xf_status xf_engine_sync(xf_engine *e) {
+  if (!e) FAILS(XF_EINVAL, "null engine");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return XF_OK;
+}
+
+extern "C" int64_t xf_engine_current_step(const xf_engine *e) { return e ? e->cur : -1; }
+
+extern "C" xf_status xf_engine_get_field(xf_engine *e, int comp, double *out) {
+  if (!e || !out || comp < 0 || comp > 5) FAILS(XF_EINVAL, "bad field request");
+  int sh[3];
+  comp_shape(e, comp, sh);
+  HIPCHK(hipStreamSynchronize(e->stream));
+  std::vector<unsigned char> buf((size_t)e->ncell * e->esz);
+  HIPCHK(hipMemcpy(buf.data(), e->f[comp], buf.size(), hipMemcpyDeviceToHost));
+  for (int i = 0; i < sh[0]; ++i)
+    for (int j = 0; j < sh[1]; ++j) {
+      const size_t base = (size_t)i * e->si + (size_t)j * e->sj;
+      double *o = out + ((size_t)i * sh[1] + j) * sh[2];
+      if (e->prec == 64) {
+        const double *src = (const double *)buf.data() + base;
+        for (int k = 0; k < sh[2]; ++k) o[k] = src[k];
+      } else {
+        const float *src = (const float *)buf.data() + base;
+        for (int k = 0; k < sh[2]; ++k) o[k] = (double)src[k];
+      }
+    }
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_set_field(xf_engine *e, int comp, const double *in) {
+  if (!e || !in || comp < 0 || comp > 5) FAILS(XF_EINVAL, "bad field request");
+  int sh[3];
+  comp_shape(e, comp, sh);
+  HIPCHK(hipStreamSynchronize(e->stream));
+  std::vector<unsigned char> buf((size_t)e->ncell * e->esz, 0);
+  for (int i = 0; i < sh[0]; ++i)
+    for (int j = 0; j < sh[1]; ++j) {
+      const size_t base = (size_t)i * e->si + (size_t)j * e->sj;
+      const double *src = in + ((size_t)i * sh[1] + j) * sh[2];
+      if (e->prec == 64) {
+        double *o = (double *)buf.data() + base;
+        for (int k = 0; k < sh[2]; ++k) o[k] = src[k];
+      } else {
+        float *o = (float *)buf.data() + base;
+        for (int k = 0; k < sh[2]; ++k) o[k] = (float)src[k];
+      }
+    }
+  HIPCHK(hipMemcpy(e->f[comp], buf.data(), buf.size(), hipMemcpyHostToDevice));
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_kernel_times(xf_engine *e, double *ms_e, double *ms_h, int64_t *samples) {
+  if (!e) FAILS(XF_EINVAL, "null engine");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  double se = 0, sh = 0;
+  const size_t n = e->ev_used / 4;
+  for (size_t q = 0; q < n; ++q) {
+    float a = 0, b = 0;
+    HIPCHK(hipEventElapsedTime(&a, e->ev[4 * q], e->ev[4 * q + 1]));
+    HIPCHK(hipEventElapsedTime(&b, e->ev[4 * q + 2], e->ev[4 * q + 3]));
+    se += a;
+    sh += b;
+  }
+  const double d = n > 0 ? (double)n : 1.0;
+  if (ms_e) *ms_e = se / d;
+  if (ms_h) *ms_h = sh / d;
+  if (samples) *samples = (int64_t)n;
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_set_timing(xf_engine *e, int enabled) {
+  if (!e) FAILS(XF_EINVAL, "null engine");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  e->timing = enabled != 0;
+  e->ev_used = 0;
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx) {
+  if (!e) FAILS(XF_EINVAL, "null engine");
+  if (tz < 64 || tz % 64 != 0 || ty < 1 || tz * ty > 256 || tx < 0)
+    FAILS(XF_EINVAL, "tiling %d x %d x %d: need tz a multiple of 64, tz*ty <= 256", tz, ty, tx);
+  e->tz = tz; e->ty = ty; e->tx = tx;
+  return XF_OK;
+}

@@ -1,0 +1,830 @@
+// host.cpp -- CPU set-up of the reference's Simulation, handed to the engine.
+//
+// Everything here runs once per simulation in FP64 with the reference's
+// formulas and operand order (compiled with -ffp-contract=off), then the
+// per-step work goes through include/xfdtd_engine.h.  Citations are to the
+// reference's src/ tree.
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+
+#include "xfdtd/xfdtd.hpp"
+#include "xfdtd_engine.h"
+
+namespace xfdtd {
+
+namespace {
+
+bool isLessOrEqual(double a, double b) {
+  // util/float_compare.h:14-17
+  return (std::abs(a - b) < constant::TOLERABLE_EPSILON) ? true : (a < b);
+}
+bool isGreaterOrEqual(double a, double b) {
+  return (std::abs(a - b) < constant::TOLERABLE_EPSILON) ? true : (a > b);
+}
+
+void check(xf_status s) {
+  if (s != XF_OK) throw std::runtime_error(std::string("xfdtd engine: ") + xf_last_error());
+}
+
+int compIndex(EMComponent c) { return static_cast<int>(c); }
+
+}  // namespace
+
+// shape/cube.cpp:27-37
+bool Cube::isPointInside(const PointVector &p) const {
+  return isLessOrEqual(_point[0], p[0]) && isLessOrEqual(_point[1], p[1]) &&
+         isLessOrEqual(_point[2], p[2]) && isGreaterOrEqual(_point[0] + _size[0], p[0]) &&
+         isGreaterOrEqual(_point[1] + _size[1], p[1]) && isGreaterOrEqual(_point[2] + _size[2], p[2]);
+}
+
+// shape/sphere.cpp:27-36
+bool Sphere::isPointInside(const PointVector &p) const {
+  const double dx = p[0] - _center[0], dy = p[1] - _center[1], dz = p[2] - _center[2];
+  const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+  return isLessOrEqual(dist, _radius);
+}
+std::unique_ptr<Shape> Sphere::getWrappedBox() const {
+  return std::make_unique<Cube>(
+      PointVector{_center[0] - _radius, _center[1] - _radius, _center[2] - _radius},
+      PointVector{2 * _radius, 2 * _radius, 2 * _radius});
+}
+
+// waveform/gaussian_waveform.cpp:18-24
+double GaussianWaveform::getValueByTime(double time) const {
+  return getAmplitude() * std::exp(-std::pow((time - _t_0) / _tau, 2));
+}
+void Waveform::init(const std::vector<double> &time_array) {
+  _value.resize(time_array.size());
+  for (size_t q = 0; q < time_array.size(); ++q) _value[q] = getValueByTime(time_array[q]);
+}
+
+// perfect_match_layer.cpp:133-155 (P faces) and 302-325 (N faces)
+PML::Profile PML::computeProfile(double dl, double dt) const {
+  Profile p;
+  const int d = _thickness;
+  p.b_e.resize(d); p.a_e.resize(d); p.b_m.resize(d); p.a_m.resize(d);
+  p.kappa_e.resize(d); p.kappa_m.resize(d);
+  const double e2m{constant::MU_0 / constant::EPSILON_0};
+  const double sigma_max = _sigma_ratio * (_order + 1) / (150.0 * constant::PI * dl);
+  const bool pos = isOrientatingPositive();
+  for (int i = 0; i < d; ++i) {
+    double rho_e, rho_m;
+    if (pos) {
+      rho_e = (i + 1 - 0.75) / static_cast<double>(d);
+      rho_m = (i + 1 - 0.25) / static_cast<double>(d);
+    } else {
+      rho_e = (d - i - 0.75) / static_cast<double>(d);
+      rho_m = (d - i - 0.25) / static_cast<double>(d);
+    }
+    const double sigma_e = sigma_max * std::pow(rho_e, _order);
+    double sigma_m = sigma_max * std::pow(rho_m, _order);
+    sigma_m = sigma_m * e2m;
+    const double kappa_e = 1 + (_kappa_max - 1) * std::pow(rho_e, _order);
+    const double kappa_m = 1 + (_kappa_max - 1) * std::pow(rho_m, _order);
+    const double alpha_e = _alpha_min + (_alpha_max - _alpha_min) * (1 - rho_e);
+    double alpha_m = _alpha_min + (_alpha_max - _alpha_min) * (1 - rho_m);
+    alpha_m = alpha_m * e2m;
+    p.b_e[i] = std::exp((-dt / constant::EPSILON_0) * (sigma_e / kappa_e + alpha_e));
+    p.a_e[i] = (1 / dl) * (p.b_e[i] - 1) * sigma_e / (kappa_e * (sigma_e + kappa_e * alpha_e));
+    p.b_m[i] = std::exp((-dt / constant::MU_0) * (sigma_m / kappa_m + alpha_m));
+    p.a_m[i] = (1 / dl) * (p.b_m[i] - 1) * sigma_m / (kappa_m * (sigma_m + kappa_m * alpha_m));
+    p.kappa_e[i] = kappa_e;
+    p.kappa_m[i] = kappa_m;
+  }
+  return p;
+}
+
+// tfsf/tfsf.cpp:15-33
+TFSF::TFSF(SpatialIndex distance_x, SpatialIndex distance_y, SpatialIndex distance_z, double e_0,
+           double theta_inc, double phi_inc, double psi, std::shared_ptr<Waveform> waveform)
+    : _distance_x{distance_x}, _distance_y{distance_y}, _distance_z{distance_z}, _e_0{e_0},
+      _theta_inc{theta_inc}, _phi_inc{phi_inc}, _psi{psi}, _sin_theta_inc{std::sin(theta_inc)},
+      _cos_theta_inc{std::cos(theta_inc)}, _sin_phi_inc{std::sin(phi_inc)},
+      _cos_phi_inc{std::cos(phi_inc)}, _sin_psi{std::sin(psi)}, _cos_psi{std::cos(psi)},
+      _k{PointVector{_sin_theta_inc * _cos_phi_inc, _sin_theta_inc * _sin_phi_inc, _cos_theta_inc}},
+      _waveform{std::move(waveform)} {}
+
+// util/dft.h:11-27
+std::vector<std::complex<double>> TFSF::getIncidentWaveFourierTransform(
+    const std::vector<double> &frequencies) const {
+  using namespace std::complex_literals;
+  const auto &v = _waveform->getValue();
+  std::vector<std::complex<double>> res(frequencies.size());
+  for (size_t i = 0; i < frequencies.size(); ++i) {
+    std::complex<double> sum{0.0, 0.0};
+    for (size_t j = 0; j < v.size(); ++j) {
+      auto t{j * _dt};
+      sum += v[j] * std::exp(-2.0 * 1i * constant::PI * frequencies[i] * t);
+    }
+    res[i] = sum * _dt;
+  }
+  return res;
+}
+
+// tfsf/tfsf_3d.cpp:13-31
+TFSF3D::TFSF3D(SpatialIndex distance_x, SpatialIndex distance_y, SpatialIndex distance_z,
+               double e_0, double theta_inc, double phi_inc, double psi,
+               std::shared_ptr<Waveform> waveform)
+    : TFSF(distance_x, distance_y, distance_z, e_0, theta_inc, phi_inc, psi, std::move(waveform)) {
+  const double sp = _sin_phi_inc, cp = _cos_phi_inc, st = _sin_theta_inc, ct = _cos_theta_inc;
+  const double u[3][3] = {{-sp, ct * cp, st * cp}, {cp, ct * sp, st * sp}, {0, -st, ct}};
+  const double ke[3] = {_sin_psi, _cos_psi, 0};
+  for (int r = 0; r < 3; ++r) _transform_e[r] = u[r][0] * ke[0] + u[r][1] * ke[1] + u[r][2] * ke[2];
+  const PointVector &k = _k;
+  const PointVector &e = _transform_e;
+  _transform_h = {k[1] * e[2] - k[2] * e[1], k[2] * e[0] - k[0] * e[2], k[0] * e[1] - k[1] * e[0]};
+}
+
+// include/nffft/nffft_fd.h constructor
+NffftFd::NffftFd(SpatialIndex distance_x, SpatialIndex distance_y, SpatialIndex distance_z,
+                 std::vector<double> frequencies, std::vector<double> theta,
+                 std::vector<double> phi, std::string output_dir_path)
+    : NFFFT(distance_x, distance_y, distance_z, std::move(output_dir_path)),
+      _frequencies{std::move(frequencies)}, _theta{std::move(theta)}, _phi{std::move(phi)} {
+  _wave_number.resize(_frequencies.size());
+  for (size_t q = 0; q < _frequencies.size(); ++q)
+    _wave_number[q] = 2 * constant::PI * _frequencies[q] / constant::C_0;
+}
+
+// ------------------------------------------------------------- Simulation
+
+Simulation::Simulation(double cell_size, float cfl) : _cell_size{cell_size}, _cfl{cfl} {
+  // grid/grid_space.cpp:14-27
+  for (int a = 0; a < 3; ++a) {
+    _min[a] = std::numeric_limits<double>::max();
+    _max[a] = std::numeric_limits<double>::min();
+  }
+}
+Simulation::Simulation(double cell_size, std::vector<std::shared_ptr<Object>> objects,
+                       std::vector<std::shared_ptr<Boundary>> boundaries,
+                       std::unique_ptr<TFSF> tfsf, float cfl)
+    : Simulation(cell_size, cfl) {
+  _objects = std::move(objects);
+  _boundaries = std::move(boundaries);
+  _tfsf = std::move(tfsf);
+}
+Simulation::Simulation(double cell_size, std::vector<std::shared_ptr<Object>> objects,
+                       std::vector<std::shared_ptr<Boundary>> boundaries,
+                       std::unique_ptr<TFSF> tfsf, std::unique_ptr<NFFFT> nffft, float cfl)
+    : Simulation(cell_size, std::move(objects), std::move(boundaries), std::move(tfsf), cfl) {
+  _nffft = std::move(nffft);
+}
+Simulation::~Simulation() {
+  if (_engine) xf_engine_destroy(_engine);
+}
+
+void Simulation::addObject(std::shared_ptr<Object> o) { _objects.push_back(std::move(o)); }
+void Simulation::addBoundary(std::shared_ptr<Boundary> b) { _boundaries.push_back(std::move(b)); }
+void Simulation::addLumpedElement(std::shared_ptr<LumpedElement> l) {
+  _lumped_elements.push_back(std::move(l));
+}
+void Simulation::addTFSFSource(std::shared_ptr<TFSF> t) { _tfsf = std::move(t); }
+void Simulation::addNFFFT(std::shared_ptr<NFFFT> n) { _nffft = std::move(n); }
+void Simulation::addProbe(Probe p) { _probes.push_back(p); }
+
+// grid/grid_space.cpp:576-618
+void Simulation::resizeSpaceSize(const double nmin[3], const double nmax[3]) {
+  for (int a = 0; a < 3; ++a) {
+    if (_min[a] > nmin[a]) _min[a] = nmin[a];
+    if (_max[a] < nmax[a]) _max[a] = nmax[a];
+  }
+  for (int a = 0; a < 3; ++a) {
+    size_t nn = static_cast<size_t>(std::round((_max[a] - _min[a]) / _cell_size));
+    if (_max[a] < _min[a]) nn = 1;
+    _n[a] = std::max(_n[a], nn);
+  }
+}
+
+// simulation_init.cpp:105-146 with grid_space.cpp:364-549, 625-638, 737-757
+void Simulation::initGridSpace() {
+  if (_objects.empty()) throw std::runtime_error("Simulation has no objects");
+  for (const auto &o : _objects) {
+    auto box = o->getWrappedBox();
+    auto *c = dynamic_cast<Cube *>(box.get());
+    if (!c) throw std::runtime_error("Shape Type Error");
+    const double mn[3] = {c->getXmin(), c->getYmin(), c->getZmin()};
+    const double mx[3] = {c->getXmax(), c->getYmax(), c->getZmax()};
+    resizeSpaceSize(mn, mx);
+  }
+  for (const auto &b : _boundaries) {
+    const int ori = static_cast<int>(b->getOrientation());
+    const int a = ori / 2;
+    const double size = b->getSize() * _cell_size;
+    if (_n[a] <= 1) throw std::runtime_error("GridSpace is not extended");
+    double mn[3] = {_min[0], _min[1], _min[2]}, mx[3] = {_max[0], _max[1], _max[2]};
+    if (ori % 2 == 0) mn[a] = _min[a] - size; else mx[a] = _max[a] + size;
+    resizeSpaceSize(mn, mx);
+  }
+  for (int a = 0; a < 3; ++a) {
+    _n[a] = std::max<size_t>(_n[a], 1);
+    if (_n[a] <= 1) throw std::runtime_error("only 3-D grids are supported");
+  }
+  for (int a = 0; a < 3; ++a) {
+    const size_t n = _n[a];
+    _ec[a].resize(n + 1); _hc[a].resize(n); _es[a].resize(n); _hs[a].resize(n + 1);
+    for (size_t i = 0; i <= n; ++i) _ec[a][i] = _min[a] + static_cast<double>(i) * _cell_size;
+    for (size_t i = 0; i < n; ++i) _hc[a][i] = (_ec[a][i] + _ec[a][i + 1]) / 2;
+    for (size_t i = 0; i < n; ++i) _es[a][i] = _ec[a][i + 1] - _ec[a][i];
+    for (size_t i = 0; i <= n; ++i) {
+      const double lo = (i == 0) ? _hc[a][0] - _cell_size : _hc[a][i - 1];
+      const double hi = (i == n) ? _hc[a][n - 1] + _cell_size : _hc[a][i];
+      _hs[a][i] = hi - lo;
+    }
+  }
+}
+
+// grid/grid_space.cpp:551-574
+size_t Simulation::handleTransform(int a, double x) const {
+  if (x < _min[a]) throw std::out_of_range("coordinate is out of range");
+  size_t best = 0;
+  double bv = std::abs(_ec[a][0] - x);
+  for (size_t i = 1; i < _ec[a].size(); ++i) {
+    const double v = std::abs(_ec[a][i] - x);
+    if (v < bv) { bv = v; best = i; }
+  }
+  return best;
+}
+
+namespace {
+
+struct Mat {
+  double eps, mu, sigma, sigma_m;
+};
+
+// fdtd_basic_coff.cpp:77-91 for one (material, node size) pair
+double coffSelfE(const Mat &m, double dt) {
+  return (2 * m.eps - dt * m.sigma) / (2 * m.eps + dt * m.sigma);
+}
+double coffSelfH(const Mat &m, double dt) {
+  return (2 * m.mu - dt * m.sigma_m) / (2 * m.mu + dt * m.sigma_m);
+}
+double coffEa(const Mat &m, double dt, double db) {  // cecha
+  return -(2 * dt / db) / (2 * m.eps + m.sigma * dt);
+}
+double coffEb(const Mat &m, double dt, double da) {  // cechb
+  return (2 * dt / da) / (2 * m.eps + m.sigma * dt);
+}
+double coffHa(const Mat &m, double dt, double db) {  // chcea
+  return (2 * dt / db) / (2 * m.mu + m.sigma_m * dt);
+}
+double coffHb(const Mat &m, double dt, double da) {  // chceb
+  return -(2 * dt / da) / (2 * m.mu + m.sigma_m * dt);
+}
+
+}  // namespace
+
+void Simulation::init(size_t total_time_steps) {
+  if (_engine) throw std::runtime_error("Simulation already initialised");
+  _total_time_steps = total_time_steps;
+  // simulation_init.cpp:10-22
+  initGridSpace();
+  const int nx = (int)_n[0], ny = (int)_n[1], nz = (int)_n[2];
+  const int N[3] = {nx, ny, nz};
+  // fdtd_basic_coff.cpp:11-29, 342-346: cfl arrives as float
+  const double cfl = static_cast<double>(_cfl);
+  const double dx = _cell_size, dy = _cell_size, dz = _cell_size;
+  _dt = cfl / (constant::C_0 * std::sqrt(1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz)));
+  const double dt = _dt;
+
+  // ---- materials: 0 = default vacuum of the staggered arrays
+  // (fdtd_basic_coff.cpp:313-339), 1..n = objects in insertion order
+  std::vector<Mat> mats;
+  mats.push_back({constant::EPSILON_0, constant::MU_0, 0.0, 0.0});
+  for (const auto &o : _objects) {
+    const Material &m = o->getMaterial();
+    if (m.isDispersion()) throw std::runtime_error("Dispersion is not supported yet.");
+    mats.push_back({m.getPermittivityE(), m.getPermeabilityM(), m.getElectricalConductivity(),
+                    m.getMagneticConductivity()});
+  }
+  // material index over (nx+1)(ny+1)(nz+1): object/object.cpp:53-73 evaluates
+  // each shape at the cell centres (h-node coordinates) of the (nx,ny,nz)
+  // cells; later objects overwrite earlier ones.  Only cells whose centre lies
+  // in the shape's wrapped box (plus one cell) are tested: outside it every
+  // shape here reports "outside".
+  const size_t sj = (size_t)nz + 1, si = (size_t)(ny + 1) * sj;
+  std::vector<uint8_t> midx((size_t)(nx + 1) * si, 0);
+  for (size_t q = 0; q < _objects.size(); ++q) {
+    const Shape *sh = _objects[q]->getShape();
+    auto box = sh->getWrappedBox();
+    auto *c = dynamic_cast<Cube *>(box.get());
+    const double lo[3] = {c->getXmin(), c->getYmin(), c->getZmin()};
+    const double hi[3] = {c->getXmax(), c->getYmax(), c->getZmax()};
+    int r0[3], r1[3];
+    for (int a = 0; a < 3; ++a) {
+      r0[a] = 0;
+      r1[a] = N[a];
+      while (r0[a] < N[a] && _hc[a][r0[a]] < lo[a] - _cell_size) ++r0[a];
+      while (r1[a] > 0 && _hc[a][r1[a] - 1] > hi[a] + _cell_size) --r1[a];
+    }
+    for (int i = r0[0]; i < r1[0]; ++i)
+      for (int j = r0[1]; j < r1[1]; ++j)
+        for (int k = r0[2]; k < r1[2]; ++k)
+          if (sh->isPointInside({_hc[0][i], _hc[1][j], _hc[2][k]}))
+            midx[(size_t)i * si + (size_t)j * sj + k] = (uint8_t)(q + 1);
+  }
+
+  // ---- lumped current sources (lumped_element/current_source.cpp:34-187).
+  // Each source cell gets its own material slot holding the corrected
+  // coefficients of its component; the other components keep the base values.
+  struct SrcCell {
+    int comp, i, j, k;
+    double coff;
+  };
+  struct SrcInfo {
+    int comp, is, ie, js, je, ks, ke;
+    std::vector<double> coff, wave;
+  };
+  std::vector<SrcInfo> srcs;
+  struct Override {
+    int m_base, comp, i, j, k;
+    double ce, ca, cb;
+  };
+  std::vector<Override> overrides;
+  for (const auto &le : _lumped_elements) {
+    auto *cs = dynamic_cast<CurrentSource *>(le.get());
+    if (!cs) throw std::runtime_error("only CurrentSource lumped elements are supported");
+    const Cube &cube = cs->getCube();
+    const double lo[3] = {cube.getXmin(), cube.getYmin(), cube.getZmin()};
+    const double hi[3] = {cube.getXmax(), cube.getYmax(), cube.getZmax()};
+    size_t st[3], en[3];
+    for (int a = 0; a < 3; ++a) {
+      st[a] = handleTransform(a, lo[a]);
+      en[a] = handleTransform(a, hi[a]);
+      if (en[a] < st[a]) throw std::range_error("end < start");
+      if (en[a] == st[a]) throw std::runtime_error("current source box has no cells");
+    }
+    SrcInfo si_;
+    const int a = static_cast<int>(cs->getOrientation()) / 2;
+    si_.comp = a;  // XF_EX / XF_EY / XF_EZ
+    si_.is = (int)st[0]; si_.ie = (int)en[0]; si_.js = (int)st[1]; si_.je = (int)en[1];
+    si_.ks = (int)st[2]; si_.ke = (int)en[2];
+    const size_t ni = en[0] - st[0], nj = en[1] - st[1], nk = en[2] - st[2];
+    double rf, af;
+    const double R = cs->getResistance();
+    if (a == 0) { rf = R * nk * nj / ni; af = cs->waveform().getAmplitude() / (nk * nj); }
+    else if (a == 1) { rf = R * nk * ni / nj; af = cs->waveform().getAmplitude() / (nk * ni); }
+    else { rf = R * ni * nj / nk; af = cs->waveform().getAmplitude() / (ni * nj); }
+    const int ori = static_cast<int>(cs->getOrientation());
+    if (ori % 2 == 0) af *= -1;
+    cs->waveform().setAmplitude(af);
+    // xt::linspace(0.5 dt, (T - 0.5) dt, T)
+    const size_t T = total_time_steps;
+    std::vector<double> times(T);
+    const double start = 0.5 * dt, stop = (T - 0.5) * dt;
+    const double step = (stop - start) / std::fmax(1.0, (double)T - 1.0);
+    for (size_t n = 0; n < T; ++n) times[n] = (n + 1 == T && T > 1) ? stop : start + step * (double)n;
+    cs->waveform().init(times);
+    si_.wave = cs->waveform().getValue();
+    for (size_t i = st[0]; i < en[0]; ++i)
+      for (size_t j = st[1]; j < en[1]; ++j)
+        for (size_t k = st[2]; k < en[2]; ++k) {
+          const int mb = midx[i * si + j * sj + k];
+          const Mat &m = mats[mb];
+          double da, db, dc;
+          if (a == 0) { da = _hs[1][j]; db = _hs[2][k]; dc = _es[0][i]; }
+          else if (a == 1) { da = _hs[2][k]; db = _hs[0][i]; dc = _es[1][j]; }
+          else { da = _hs[0][i]; db = _hs[1][j]; dc = _es[2][k]; }
+          const double beta = (dt * dc) / (rf * da * db);
+          const double eps = m.eps, sg = m.sigma;
+          Override ov;
+          ov.m_base = mb; ov.comp = a; ov.i = (int)i; ov.j = (int)j; ov.k = (int)k;
+          ov.ce = (2 * eps - dt * sg - beta) / (2 * eps + dt * sg + beta);
+          ov.ca = -2 * dt / ((2 * eps + dt * sg + beta) * db);
+          ov.cb = 2 * dt / ((2 * eps + dt * sg + beta) * da);
+          overrides.push_back(ov);
+          si_.coff.push_back(-2 * dt / ((2 * eps + dt * sg + beta) * (db * da)));
+        }
+    srcs.push_back(std::move(si_));
+  }
+  const int nbase = (int)mats.size();
+  const int nmat = nbase + (int)overrides.size();
+  if (nmat > 255) throw std::runtime_error("more than 255 material slots");
+
+  // ---- coefficient tables (fdtd_basic_coff.cpp:93-121)
+  auto axlen = [&](int t) { static const int ax[12] = {2, 1, 0, 2, 1, 0, 2, 1, 0, 2, 1, 0}; return N[ax[t]] + 1; };
+  std::vector<double> ca(6 * (size_t)nmat);
+  std::vector<std::vector<double>> cb(12), cpsi(12);
+  for (int t = 0; t < 12; ++t) {
+    cb[t].assign((size_t)nmat * axlen(t), 0.0);
+  }
+  auto fill_material = [&](int slot, const Mat &m) {
+    for (int c = 0; c < 3; ++c) ca[c * nmat + slot] = coffSelfE(m, dt);
+    for (int c = 3; c < 6; ++c) ca[c * nmat + slot] = coffSelfH(m, dt);
+    for (int k = 0; k <= nz; ++k) {
+      cb[XF_CB_EXHY][(size_t)slot * (nz + 1) + k] = coffEa(m, dt, _hs[2][k]);
+      cb[XF_CB_EYHX][(size_t)slot * (nz + 1) + k] = coffEb(m, dt, _hs[2][k]);
+      if (k < nz) {
+        cb[XF_CB_HXEY][(size_t)slot * (nz + 1) + k] = coffHa(m, dt, _es[2][k]);
+        cb[XF_CB_HYEX][(size_t)slot * (nz + 1) + k] = coffHb(m, dt, _es[2][k]);
+      }
+    }
+    for (int j = 0; j <= ny; ++j) {
+      cb[XF_CB_EXHZ][(size_t)slot * (ny + 1) + j] = coffEb(m, dt, _hs[1][j]);
+      cb[XF_CB_EZHX][(size_t)slot * (ny + 1) + j] = coffEa(m, dt, _hs[1][j]);
+      if (j < ny) {
+        cb[XF_CB_HXEZ][(size_t)slot * (ny + 1) + j] = coffHb(m, dt, _es[1][j]);
+        cb[XF_CB_HZEX][(size_t)slot * (ny + 1) + j] = coffHa(m, dt, _es[1][j]);
+      }
+    }
+    for (int i = 0; i <= nx; ++i) {
+      cb[XF_CB_EYHZ][(size_t)slot * (nx + 1) + i] = coffEa(m, dt, _hs[0][i]);
+      cb[XF_CB_EZHY][(size_t)slot * (nx + 1) + i] = coffEb(m, dt, _hs[0][i]);
+      if (i < nx) {
+        cb[XF_CB_HYEZ][(size_t)slot * (nx + 1) + i] = coffHa(m, dt, _es[0][i]);
+        cb[XF_CB_HZEY][(size_t)slot * (nx + 1) + i] = coffHb(m, dt, _es[0][i]);
+      }
+    }
+  };
+  for (int q = 0; q < nbase; ++q) fill_material(q, mats[q]);
+  for (size_t q = 0; q < overrides.size(); ++q) {
+    const Override &ov = overrides[q];
+    const int slot = nbase + (int)q;
+    fill_material(slot, mats[ov.m_base]);
+    // current_source.cpp:182-185: cece, cecha (db), cechb (da) of its component
+    ca[ov.comp * nmat + slot] = ov.ce;
+    if (ov.comp == 0) {
+      cb[XF_CB_EXHY][(size_t)slot * (nz + 1) + ov.k] = ov.ca;
+      cb[XF_CB_EXHZ][(size_t)slot * (ny + 1) + ov.j] = ov.cb;
+    } else if (ov.comp == 1) {
+      cb[XF_CB_EYHZ][(size_t)slot * (nx + 1) + ov.i] = ov.ca;
+      cb[XF_CB_EYHX][(size_t)slot * (nz + 1) + ov.k] = ov.cb;
+    } else {
+      cb[XF_CB_EZHX][(size_t)slot * (ny + 1) + ov.j] = ov.ca;
+      cb[XF_CB_EZHY][(size_t)slot * (nx + 1) + ov.i] = ov.cb;
+    }
+    midx[(size_t)ov.i * si + (size_t)ov.j * sj + ov.k] = (uint8_t)slot;
+  }
+
+  // ---- CPML (perfect_match_layer.cpp:49-96, 157-431): per axis, capture
+  // c_psi = coefficient * dl, then divide the coefficient by kappa
+  std::vector<double> be[3], ae[3], bm[3], am[3];
+  int dn[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
+  bool seen[6] = {false, false, false, false, false, false};
+  for (int t = 0; t < 12; ++t) cpsi[t] = cb[t];
+  for (auto &v : cpsi)
+    for (double &x : v) x = x * _cell_size;
+  // tables whose derivative axis is x / y / z: E pair then H pair
+  static const int axis_tables[3][4] = {{XF_CB_EYHZ, XF_CB_EZHY, XF_CB_HYEZ, XF_CB_HZEY},
+                                        {XF_CB_EXHZ, XF_CB_EZHX, XF_CB_HXEZ, XF_CB_HZEX},
+                                        {XF_CB_EXHY, XF_CB_EYHX, XF_CB_HXEY, XF_CB_HYEX}};
+  for (int a = 0; a < 3; ++a) {
+    be[a].assign(N[a] + 1, 0.0); ae[a].assign(N[a] + 1, 0.0);
+    bm[a].assign(N[a] + 1, 0.0); am[a].assign(N[a] + 1, 0.0);
+  }
+  std::vector<char> touched_e[3], touched_h[3];
+  for (int a = 0; a < 3; ++a) { touched_e[a].assign(N[a] + 1, 0); touched_h[a].assign(N[a] + 1, 0); }
+  for (const auto &b : _boundaries) {
+    auto *pml = dynamic_cast<PML *>(b.get());
+    if (!pml) throw std::runtime_error("only PML boundaries are supported");
+    const int ori = static_cast<int>(pml->getOrientation());
+    if (seen[ori]) throw std::runtime_error("two boundaries on the same face");
+    seen[ori] = true;
+    const int a = ori / 2;
+    const bool pos = pml->isOrientatingPositive();
+    const int d = (int)pml->getSize();
+    if (d < 1 || d >= N[a]) throw std::runtime_error("PML thickness does not fit the grid");
+    const PML::Profile pr = pml->computeProfile(_cell_size, dt);
+    const int start = pos ? N[a] - d : 0;
+    if (pos) dp[a] = d; else dn[a] = d;
+    for (int l = 0; l < d; ++l) {
+      const int ne = start + l + (pos ? 0 : 1);
+      const int nh = start + l;
+      if (touched_e[a][ne] || touched_h[a][nh]) throw std::runtime_error("CPML faces overlap");
+      touched_e[a][ne] = touched_h[a][nh] = 1;
+      be[a][ne] = pr.b_e[l]; ae[a][ne] = pr.a_e[l];
+      bm[a][nh] = pr.b_m[l]; am[a][nh] = pr.a_m[l];
+      for (int m = 0; m < nmat; ++m) {
+        for (int q = 0; q < 2; ++q) {
+          double &c = cb[axis_tables[a][q]][(size_t)m * (N[a] + 1) + ne];
+          c = c / pr.kappa_e[l];
+        }
+        for (int q = 2; q < 4; ++q) {
+          double &c = cb[axis_tables[a][q]][(size_t)m * (N[a] + 1) + nh];
+          c = c / pr.kappa_m[l];
+        }
+      }
+    }
+  }
+  // The engine applies the lumped and TFSF corrections before CPML, the
+  // reference after; that only matters for cells in both, which are refused.
+  auto in_slab_e = [&](int a, int n) { return (n >= 1 && n <= dn[a]) || (dp[a] > 0 && n >= N[a] - dp[a] && n <= N[a] - 1); };
+  auto in_slab_h = [&](int a, int n) { return (n < dn[a]) || (dp[a] > 0 && n >= N[a] - dp[a] && n < N[a]); };
+  for (const auto &s : srcs) {
+    const int lo[3] = {s.is, s.js, s.ks}, hi[3] = {s.ie, s.je, s.ke};
+    for (int a = 0; a < 3; ++a)
+      for (int n = lo[a]; n < hi[a]; ++n)
+        if (in_slab_e(a, n)) throw std::runtime_error("current source inside a CPML slab is not supported");
+  }
+
+  // ---- engine
+  xf_grid_desc gd;
+  gd.nx = nx; gd.ny = ny; gd.nz = nz; gd.precision = _precision; gd.n_materials = nmat;
+  check(xf_engine_create(&gd, &_engine));
+  const double *cbp[12], *cpp[12];
+  for (int t = 0; t < 12; ++t) { cbp[t] = cb[t].data(); cpp[t] = cpsi[t].data(); }
+  for (int a = 0; a < 3; ++a)
+    if (dn[a] + dp[a] > 0)
+      check(xf_engine_set_cpml(_engine, a, dn[a], dp[a], be[a].data(), ae[a].data(), bm[a].data(), am[a].data()));
+  check(xf_engine_set_materials(_engine, midx.data(), ca.data(), cbp, cpp));
+  for (const auto &s : srcs)
+    check(xf_engine_add_current_source(_engine, s.comp, s.is, s.ie, s.js, s.je, s.ks, s.ke, s.coff.data(),
+                                       s.wave.data(), (int64_t)total_time_steps));
+
+  // ---- TFSF (tfsf/tfsf.cpp:35-53, tfsf_3d.cpp:33-151)
+  if (_tfsf) {
+    auto *t3 = dynamic_cast<TFSF3D *>(_tfsf.get());
+    if (!t3) throw std::runtime_error("only TFSF3D sources are supported");
+    const auto dist = t3->getDistance();
+    for (int a = 0; a < 3; ++a)
+      if (dist[a] < 1 || 2 * dist[a] >= (size_t)N[a]) throw std::runtime_error("TFSF box does not fit the grid");
+    const int l[3] = {(int)dist[0], (int)dist[1], (int)dist[2]};
+    const int r[3] = {nx - l[0], ny - l[1], nz - l[2]};
+    for (int a = 0; a < 3; ++a) {
+      // corrected E cells sit at l..r, H cells at l-1 and r
+      if (in_slab_e(a, l[a]) || in_slab_e(a, r[a]) || in_slab_h(a, l[a] - 1) || in_slab_h(a, r[a]) ||
+          dn[a] >= l[a] || (dp[a] > 0 && r[a] >= N[a] - dp[a]))
+        throw std::runtime_error("TFSF box must lie outside the CPML slabs");
+    }
+    t3->_dt = dt;
+    std::vector<double> time_arr(total_time_steps);
+    for (size_t n = 0; n < total_time_steps; ++n) time_arr[n] = static_cast<double>(n) * dt;
+    t3->_waveform->init(time_arr);
+    std::vector<double> source(total_time_steps);
+    for (size_t n = 0; n < total_time_steps; ++n) source[n] = t3->_e_0 * (*t3->_waveform)(n);
+    const double th = t3->_theta_inc, ph = t3->_phi_inc;
+    const double ratio = 1 / std::sqrt(std::pow(std::sin(th), 4) * (std::pow(std::cos(ph), 4) + std::pow(std::sin(ph), 4)) +
+                                       std::pow(std::cos(th), 4));
+    const PointVector k = t3->_k;
+    const size_t bn[3] = {(size_t)(r[0] - l[0]), (size_t)(r[1] - l[1]), (size_t)(r[2] - l[2])};
+    const double diag = std::sqrt(std::pow(bn[0], 2) + std::pow(bn[1], 2) + std::pow(bn[2], 2)) * ratio;
+    const size_t L = static_cast<size_t>(std::ceil(diag)) + 4 + 1;
+    double org[3];
+    for (int a = 0; a < 3; ++a) org[a] = (k[a] >= 0) ? (double)l[a] : (double)r[a];
+    for (int a = 0; a < 3; ++a) org[a] -= 2 * k[a] / ratio;
+    std::vector<double> full[3], half[3];
+    for (int a = 0; a < 3; ++a) {
+      full[a].resize(bn[a] + 1);
+      half[a].resize(bn[a] + 2);
+      for (size_t i = 0; i < bn[a] + 1; ++i) full[a][i] = ((double)(i + (size_t)l[a]) - org[a]) * k[a] * ratio;
+      for (size_t i = 0; i < bn[a] + 2; ++i) half[a][i] = ((double)(i + (size_t)l[a]) - org[a] - 0.5) * k[a] * ratio;
+    }
+    // every interpolation index (tfsf_3d.cpp:418-422) must fall inside the
+    // 1-D grid: bound the sum of the three projections from both sides
+    double psum = 0, pmin = -0.5;
+    for (int a = 0; a < 3; ++a) {
+      double mx = full[a][0], mn = full[a][0];
+      for (double v : half[a]) { mx = std::max(mx, v); mn = std::min(mn, v); }
+      for (double v : full[a]) { mx = std::max(mx, v); mn = std::min(mn, v); }
+      psum += mx;
+      pmin += mn;
+    }
+    if (psum + 1 >= (double)L - 1 || pmin < 0)
+      throw std::runtime_error("TFSF projection leaves the 1-D grid");
+    const double scaled_dl = dx / ratio;
+    xf_tfsf_desc td;
+    td.li = l[0]; td.ri = r[0]; td.lj = l[1]; td.rj = r[1]; td.lk = l[2]; td.rk = r[2];
+    td.aux_size = (int64_t)L;
+    for (int a = 0; a < 3; ++a) {
+      td.proj_full[a] = full[a].data();
+      td.proj_half[a] = half[a].data();
+      td.transform_e[a] = t3->_transform_e[a];
+      td.transform_h[a] = t3->_transform_h[a];
+    }
+    td.ceie = 1.0;
+    td.chih = 1.0;
+    td.ceihi = -(dt / (constant::EPSILON_0 * scaled_dl));
+    td.chiei = -(dt / (constant::MU_0 * scaled_dl));
+    td.abc_c1 = (constant::C_0 * dt - scaled_dl) / (constant::C_0 * dt + scaled_dl);
+    td.abc_c2 = 2 * scaled_dl / (constant::C_0 * dt + scaled_dl);
+    const double d3[3] = {dx, dy, dz};
+    for (int a = 0; a < 3; ++a) {
+      td.ca[a] = dt / (constant::EPSILON_0 * d3[a]);
+      td.cb[a] = dt / (constant::MU_0 * d3[a]);
+    }
+    td.source = source.data();
+    td.total_steps = (int64_t)total_time_steps;
+    check(xf_engine_set_tfsf(_engine, &td));
+  }
+
+  // ---- NF-FF (nffft/nffft.cpp:22-34, nffft_fd.cpp:17-106)
+  if (_nffft) {
+    auto *nf = dynamic_cast<NffftFd *>(_nffft.get());
+    if (!nf) throw std::runtime_error("only NffftFd is supported");
+    const auto dist = nf->getDistance();
+    for (int a = 0; a < 3; ++a)
+      if (dist[a] < 1 || 2 * dist[a] >= (size_t)N[a]) throw std::runtime_error("NF-FF box does not fit the grid");
+    xf_nffft_desc nd;
+    nd.li = (int64_t)dist[0]; nd.ri = nx - (int64_t)dist[0];
+    nd.lj = (int64_t)dist[1]; nd.rj = ny - (int64_t)dist[1];
+    nd.lk = (int64_t)dist[2]; nd.rk = nz - (int64_t)dist[2];
+    for (int q = 0; q < 6; ++q) _nffft_box[q] = (size_t)(&nd.li)[q];
+    nd.nf = (int)nf->frequencies().size();
+    nd.total_steps = (int64_t)total_time_steps;
+    const size_t T = total_time_steps;
+    std::vector<std::complex<double>> tj((size_t)nd.nf * T), tm((size_t)nd.nf * T);
+    using namespace std::complex_literals;
+    for (int n = 0; n < nd.nf; ++n)
+      for (size_t t = 0; t < T; ++t) {
+        const double f = nf->frequencies()[n];
+        tj[n * T + t] = dt * std::exp(-1i * 2.0 * constant::PI * (f * (t + 0.5) * dt));
+        tm[n * T + t] = dt * std::exp(-1i * 2.0 * constant::PI * (f * t * dt));
+      }
+    nd.tj = reinterpret_cast<const double *>(tj.data());
+    nd.tm = reinterpret_cast<const double *>(tm.data());
+    check(xf_engine_set_nffft(_engine, &nd));
+  }
+
+  for (const Probe &p : _probes)
+    check(xf_engine_add_probe(_engine, compIndex(p.component), (int64_t)p.i, (int64_t)p.j, (int64_t)p.k));
+}
+
+void Simulation::step(size_t n) {
+  if (!_engine) throw std::runtime_error("Simulation is not initialised");
+  check(xf_engine_step(_engine, (int64_t)n));
+}
+
+void Simulation::sync() {
+  if (_engine) check(xf_engine_sync(_engine));
+}
+
+size_t Simulation::currentTimeStep() const {
+  return _engine ? (size_t)xf_engine_current_step(_engine) : 0;
+}
+
+void Simulation::run(size_t total_time_steps) {
+  init(total_time_steps);
+  step(total_time_steps);
+  sync();
+  if (_nffft && !_nffft->getOutputDirPath().empty()) outputData();
+}
+
+std::array<size_t, 3> Simulation::fieldShape(EMComponent c) const {
+  const size_t nx = _n[0], ny = _n[1], nz = _n[2];
+  switch (c) {
+    case EMComponent::EX: return {nx, ny + 1, nz + 1};
+    case EMComponent::EY: return {nx + 1, ny, nz + 1};
+    case EMComponent::EZ: return {nx + 1, ny + 1, nz};
+    case EMComponent::HX: return {nx + 1, ny, nz};
+    case EMComponent::HY: return {nx, ny + 1, nz};
+    default: return {nx, ny, nz + 1};
+  }
+}
+
+std::vector<double> Simulation::getField(EMComponent c) const {
+  if (!_engine) throw std::runtime_error("Simulation is not initialised");
+  const auto sh = fieldShape(c);
+  std::vector<double> out(sh[0] * sh[1] * sh[2]);
+  check(xf_engine_get_field(_engine, compIndex(c), out.data()));
+  return out;
+}
+
+void Simulation::setField(EMComponent c, const std::vector<double> &v) {
+  if (!_engine) throw std::runtime_error("Simulation is not initialised");
+  const auto sh = fieldShape(c);
+  if (v.size() != sh[0] * sh[1] * sh[2]) throw std::invalid_argument("field size mismatch");
+  check(xf_engine_set_field(_engine, compIndex(c), v.data()));
+}
+
+std::vector<std::complex<double>> Simulation::getNffftAccumulators() const {
+  if (!_engine || !_nffft) throw std::runtime_error("no NF-FF surface");
+  std::vector<std::complex<double>> out((size_t)xf_engine_nffft_size(_engine));
+  check(xf_engine_get_nffft(_engine, reinterpret_cast<double *>(out.data())));
+  return out;
+}
+
+// nffft_fd.cpp:379-421: the per-face sums come from the engine; the area
+// factors, angular projections and far-zone fields follow the reference.
+FarField Simulation::calculateFarField() const {
+  auto *nf = dynamic_cast<NffftFd *>(_nffft.get());
+  if (!_engine || !nf) throw std::runtime_error("no NF-FF surface");
+  using C = std::complex<double>;
+  using namespace std::complex_literals;
+  const size_t NF = nf->frequencies().size(), NT = nf->theta().size(), NP = nf->phi().size();
+  const size_t M = NF * NT * NP;
+  std::vector<C> sums(M * 12);
+  check(xf_engine_farfield_sums(_engine, _cell_size, nf->waveNumber().data(), nf->theta().data(), (int)NT,
+                                nf->phi().data(), (int)NP, reinterpret_cast<double *>(sums.data())));
+  FarField r;
+  for (auto *v : {&r.n_theta, &r.n_phi, &r.l_theta, &r.l_phi, &r.e_theta, &r.e_phi, &r.h_theta, &r.h_phi})
+    v->resize(M);
+  r.power_theta.resize(M);
+  r.power_phi.resize(M);
+  const double dl = _cell_size;
+  const double area = dl * dl;
+  const double eta = constant::ETA_0;
+  for (size_t n = 0; n < NF; ++n)
+    for (size_t t = 0; t < NT; ++t)
+      for (size_t p = 0; p < NP; ++p) {
+        const size_t m = (n * NT + t) * NP + p;
+        const C *s = &sums[m * 12];
+        C a_x{0, 0}, a_y{0, 0}, a_z{0, 0}, f_x{0, 0}, f_y{0, 0}, f_z{0, 0};
+        // calculateFarfieldX: (sum_jy, sum_jz, sum_my, sum_mz)
+        a_y += s[0] * area; a_z += s[1] * area; f_y += s[2] * area; f_z += s[3] * area;
+        // calculateFarfieldY: (sum_jz, sum_jx, sum_mz, sum_mx)
+        a_z += s[4] * area; a_x += s[5] * area; f_z += s[6] * area; f_x += s[7] * area;
+        // calculateFarfieldZ: (sum_jx, sum_jy, sum_mx, sum_my)
+        a_x += s[8] * area; a_y += s[9] * area; f_x += s[10] * area; f_y += s[11] * area;
+        const double cos_t = std::cos(nf->theta()[t]), sin_t = std::sin(nf->theta()[t]);
+        const double cos_p = std::cos(nf->phi()[p]), sin_p = std::sin(nf->phi()[p]);
+        const double cos_t_cos_p = cos_t * cos_p, cos_t_sin_p = cos_t * sin_p;
+        const C a_theta = cos_t_cos_p * a_x + cos_t_sin_p * a_y - sin_t * a_z;
+        const C a_phi = -sin_p * a_x + cos_p * a_y;
+        const C f_theta = cos_t_cos_p * f_x + cos_t_sin_p * f_y - sin_t * f_z;
+        const C f_phi = -sin_p * f_x + cos_p * f_y;
+        const double k = nf->waveNumber()[n];
+        const double coff = std::pow(k, 2) / (32 * constant::PI * constant::PI * eta);
+        r.n_theta[m] = a_theta;
+        r.n_phi[m] = a_phi;
+        r.l_theta[m] = f_theta;
+        r.l_phi[m] = f_phi;
+        r.e_theta[m] = (-1i * k) * (eta * a_theta + f_phi) / (4 * constant::PI);
+        r.e_phi[m] = (1i * k) * (-eta * a_phi + f_theta) / (4 * constant::PI);
+        r.h_theta[m] = (1i * k) * (a_phi - f_theta / eta) / (4 * constant::PI);
+        r.h_phi[m] = (-1i * k) * (a_theta + f_phi / eta) / (4 * constant::PI);
+        r.power_theta[m] = coff * std::pow(std::abs(eta * a_theta + f_phi), 2);
+        r.power_phi[m] = coff * std::pow(std::abs(-eta * a_phi + f_theta), 2);
+      }
+  return r;
+}
+
+namespace {
+
+// minimal .npy (format 1.0) writer, little-endian C order
+void writeNpy(const std::string &path, const void *data, const char *descr, size_t elem,
+              const std::vector<size_t> &shape) {
+  std::string sh = "(";
+  for (size_t q = 0; q < shape.size(); ++q) sh += std::to_string(shape[q]) + (shape.size() == 1 || q + 1 < shape.size() ? "," : "");
+  sh += ")";
+  std::string hdr = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': " + sh + ", }";
+  size_t total = 10 + hdr.size() + 1;
+  hdr.append((64 - total % 64) % 64, ' ');
+  hdr += '\n';
+  std::ofstream f(path, std::ios::binary);
+  if (!f) throw std::runtime_error("cannot write " + path);
+  const char magic[] = "\x93NUMPY";
+  f.write(magic, 6);
+  const char ver[2] = {1, 0};
+  f.write(ver, 2);
+  const uint16_t hl = (uint16_t)hdr.size();
+  f.write(reinterpret_cast<const char *>(&hl), 2);
+  f.write(hdr.data(), (std::streamsize)hdr.size());
+  size_t n = elem;
+  for (size_t s : shape) n *= s;
+  f.write(static_cast<const char *>(data), (std::streamsize)n);
+}
+
+void makeDirs(const std::string &path) {
+  std::string cur;
+  for (size_t q = 0; q < path.size(); ++q) {
+    cur += path[q];
+    if (path[q] == '/' || q + 1 == path.size()) {
+      if (mkdir(cur.c_str(), 0755) != 0 && errno != EEXIST)
+        throw std::runtime_error("cannot create directory " + cur);
+    }
+  }
+}
+
+}  // namespace
+
+// nffft_fd.cpp:579-634
+void Simulation::outputData() const {
+  auto *nf = dynamic_cast<NffftFd *>(_nffft.get());
+  if (!nf) return;
+  const FarField r = calculateFarField();
+  const std::string dir = nf->getOutputDirPath();
+  try {
+    makeDirs(dir);
+  } catch (const std::exception &) {
+    std::fprintf(stderr, "Error: cannot create directory %s\n", dir.c_str());
+    return;
+  }
+  const std::vector<size_t> shp = {nf->frequencies().size(), nf->theta().size(), nf->phi().size()};
+  auto p = [&](const char *name) { return dir + "/" + name; };
+  writeNpy(p("frequencies.npy"), nf->frequencies().data(), "<f8", 8, {nf->frequencies().size()});
+  writeNpy(p("theta.npy"), nf->theta().data(), "<f8", 8, {nf->theta().size()});
+  writeNpy(p("phi.npy"), nf->phi().data(), "<f8", 8, {nf->phi().size()});
+  writeNpy(p("e_theta.npy"), r.e_theta.data(), "<c16", 16, shp);
+  writeNpy(p("e_phi.npy"), r.e_phi.data(), "<c16", 16, shp);
+  writeNpy(p("h_theta.npy"), r.h_theta.data(), "<c16", 16, shp);
+  writeNpy(p("h_phi.npy"), r.h_phi.data(), "<c16", 16, shp);
+  writeNpy(p("n_theta.npy"), r.n_theta.data(), "<c16", 16, shp);
+  writeNpy(p("n_phi.npy"), r.n_phi.data(), "<c16", 16, shp);
+  writeNpy(p("l_theta.npy"), r.l_theta.data(), "<c16", 16, shp);
+  writeNpy(p("l_phi.npy"), r.l_phi.data(), "<c16", 16, shp);
+  writeNpy(p("power_theta.npy"), r.power_theta.data(), "<f8", 8, shp);
+  writeNpy(p("power_phi.npy"), r.power_phi.data(), "<f8", 8, shp);
+}
+
+std::vector<double> Simulation::getProbeData() const {
+  if (!_engine) throw std::runtime_error("Simulation is not initialised");
+  std::vector<double> out(_probes.size() * _total_time_steps);
+  if (!_probes.empty()) check(xf_engine_get_probes(_engine, out.data(), (int64_t)_total_time_steps));
+  return out;
+}
+
+}  // namespace xfdtd

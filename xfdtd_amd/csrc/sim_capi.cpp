@@ -1,0 +1,194 @@
+// sim_capi.cpp -- include/xfdtd_sim.h over the C++ Simulation.
+#include <cstring>
+#include <string>
+
+#include "xfdtd/xfdtd.hpp"
+#include "xfdtd_engine.h"
+#include "xfdtd_sim.h"
+
+using namespace xfdtd;
+
+struct xf_sim {
+  Simulation sim;
+  std::shared_ptr<NffftFd> nffft;
+  xf_sim(double dl, float cfl) : sim(dl, cfl) {}
+};
+
+static thread_local std::string g_sim_err;
+extern "C" const char *xf_sim_last_error(void) { return g_sim_err.c_str(); }
+
+template <typename F>
+static int guarded(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception &e) {
+    g_sim_err = e.what();
+  } catch (...) {
+    g_sim_err = "unknown error";
+  }
+  return -1;
+}
+
+extern "C" xf_sim *xf_sim_create(double cell_size, float cfl) {
+  try {
+    return new xf_sim(cell_size, cfl);
+  } catch (const std::exception &e) {
+    g_sim_err = e.what();
+    return nullptr;
+  }
+}
+
+extern "C" void xf_sim_destroy(xf_sim *s) { delete s; }
+
+static Material mat(float eps_r, float mu_r, double sigma_e, double sigma_m) {
+  return Material("material", eps_r, mu_r, sigma_e, sigma_m);
+}
+
+extern "C" int xf_sim_add_cube(xf_sim *s, const double origin[3], const double size[3], float eps_r,
+                               float mu_r, double sigma_e, double sigma_m) {
+  return guarded([&] {
+    s->sim.addObject(std::make_shared<Object>(
+        "cube", std::make_unique<Cube>(PointVector{origin[0], origin[1], origin[2]},
+                                       PointVector{size[0], size[1], size[2]}),
+        mat(eps_r, mu_r, sigma_e, sigma_m)));
+  });
+}
+
+extern "C" int xf_sim_add_sphere(xf_sim *s, const double center[3], double radius, float eps_r,
+                                 float mu_r, double sigma_e, double sigma_m) {
+  return guarded([&] {
+    s->sim.addObject(std::make_shared<Object>(
+        "sphere", std::make_unique<Sphere>(PointVector{center[0], center[1], center[2]}, radius),
+        mat(eps_r, mu_r, sigma_e, sigma_m)));
+  });
+}
+
+extern "C" int xf_sim_add_pml(xf_sim *s, int orientation, int thickness, int order, double sigma_ratio,
+                              double alpha_min, double alpha_max, double kappa_max) {
+  return guarded([&] {
+    if (orientation < 0 || orientation > 5) throw std::invalid_argument("bad orientation");
+    s->sim.addBoundary(std::make_shared<PML>(static_cast<Orientation>(orientation), thickness, order,
+                                             sigma_ratio, alpha_min, alpha_max, kappa_max));
+  });
+}
+
+extern "C" int xf_sim_set_tfsf3d(xf_sim *s, const size_t distance[3], double e0, double theta,
+                                 double phi, double psi, double amplitude, double tau, double t0) {
+  return guarded([&] {
+    s->sim.addTFSFSource(std::make_shared<TFSF3D>(distance[0], distance[1], distance[2], e0, theta, phi,
+                                                  psi, std::make_shared<GaussianWaveform>(amplitude, tau, t0)));
+  });
+}
+
+extern "C" int xf_sim_set_nffft_fd(xf_sim *s, const size_t distance[3], const double *freqs, int nf,
+                                   const double *theta, int ntheta, const double *phi, int nphi,
+                                   const char *output_dir) {
+  return guarded([&] {
+    s->nffft = std::make_shared<NffftFd>(
+        distance[0], distance[1], distance[2], std::vector<double>(freqs, freqs + nf),
+        std::vector<double>(theta, theta + ntheta), std::vector<double>(phi, phi + nphi),
+        output_dir ? std::string(output_dir) : std::string());
+    s->sim.addNFFFT(s->nffft);
+  });
+}
+
+extern "C" int xf_sim_add_current_source(xf_sim *s, const double origin[3], const double size[3],
+                                         int orientation, double resistance, double amplitude,
+                                         double tau, double t0) {
+  return guarded([&] {
+    if (orientation < 0 || orientation > 5) throw std::invalid_argument("bad orientation");
+    s->sim.addLumpedElement(std::make_shared<CurrentSource>(
+        std::make_unique<Cube>(PointVector{origin[0], origin[1], origin[2]},
+                               PointVector{size[0], size[1], size[2]}),
+        static_cast<Orientation>(orientation), resistance,
+        std::make_unique<GaussianWaveform>(amplitude, tau, t0)));
+  });
+}
+
+extern "C" int xf_sim_add_probe(xf_sim *s, int comp, size_t i, size_t j, size_t k) {
+  return guarded([&] {
+    if (comp < 0 || comp > 5) throw std::invalid_argument("bad component");
+    s->sim.addProbe(Probe{static_cast<EMComponent>(comp), i, j, k});
+  });
+}
+
+extern "C" int xf_sim_init(xf_sim *s, size_t total_steps, int precision) {
+  return guarded([&] {
+    if (precision != 32 && precision != 64) throw std::invalid_argument("precision must be 32 or 64");
+    s->sim.setPrecision(precision);
+    s->sim.init(total_steps);
+  });
+}
+
+extern "C" int xf_sim_step(xf_sim *s, size_t n) {
+  return guarded([&] { s->sim.step(n); });
+}
+
+extern "C" int xf_sim_sync(xf_sim *s) {
+  return guarded([&] { s->sim.sync(); });
+}
+
+extern "C" size_t xf_sim_current_step(const xf_sim *s) { return s->sim.currentTimeStep(); }
+
+extern "C" int xf_sim_dims(const xf_sim *s, size_t out[3]) {
+  out[0] = s->sim.getNx();
+  out[1] = s->sim.getNy();
+  out[2] = s->sim.getNz();
+  return 0;
+}
+
+extern "C" double xf_sim_dt(const xf_sim *s) { return s->sim.getDt(); }
+
+extern "C" int xf_sim_get_field(xf_sim *s, int comp, double *out) {
+  return guarded([&] {
+    if (comp < 0 || comp > 5) throw std::invalid_argument("bad component");
+    auto v = s->sim.getField(static_cast<EMComponent>(comp));
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+  });
+}
+
+extern "C" int xf_sim_set_field(xf_sim *s, int comp, const double *in) {
+  return guarded([&] {
+    if (comp < 0 || comp > 5) throw std::invalid_argument("bad component");
+    const auto sh = s->sim.fieldShape(static_cast<EMComponent>(comp));
+    std::vector<double> v(in, in + sh[0] * sh[1] * sh[2]);
+    s->sim.setField(static_cast<EMComponent>(comp), v);
+  });
+}
+
+extern "C" int64_t xf_sim_nffft_size(const xf_sim *s) {
+  return s->sim.engine() ? xf_engine_nffft_size(s->sim.engine()) : 0;
+}
+
+extern "C" int xf_sim_get_nffft(xf_sim *s, double *out) {
+  return guarded([&] {
+    auto v = s->sim.getNffftAccumulators();
+    std::memcpy(out, v.data(), v.size() * sizeof(std::complex<double>));
+  });
+}
+
+extern "C" int xf_sim_farfield(xf_sim *s, double *out) {
+  return guarded([&] {
+    FarField r = s->sim.calculateFarField();
+    const size_t M = r.e_theta.size();
+    const std::vector<std::complex<double>> *c[8] = {&r.n_theta, &r.n_phi, &r.l_theta, &r.l_phi,
+                                                     &r.e_theta, &r.e_phi, &r.h_theta, &r.h_phi};
+    for (int q = 0; q < 8; ++q) std::memcpy(out + 2 * M * q, c[q]->data(), M * sizeof(std::complex<double>));
+    std::memcpy(out + 16 * M, r.power_theta.data(), M * sizeof(double));
+    std::memcpy(out + 17 * M, r.power_phi.data(), M * sizeof(double));
+  });
+}
+
+extern "C" int xf_sim_output_nffft(xf_sim *s) {
+  return guarded([&] { s->sim.outputData(); });
+}
+
+extern "C" int xf_sim_get_probes(xf_sim *s, double *out) {
+  return guarded([&] {
+    auto v = s->sim.getProbeData();
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+  });
+}
+
+extern "C" struct xf_engine *xf_sim_engine(xf_sim *s) { return s->sim.engine(); }

@@ -1,0 +1,200 @@
+"""Scene description mirroring the reference's C++ construction API.
+
+Each class carries the constructor arguments of its reference counterpart:
+
+  Cube(origin, size)                      shape/cube.cpp:54-55
+  Sphere(center, radius)                  shape/sphere.cpp:14-15
+  Material(name, eps_r, mu_r, sigma_e, sigma_m)
+                                          material/material.cpp:7-16 (eps_r, mu_r are float)
+  Object(name, shape, material)           object/object.cpp:18-26
+  PML(orientation, thickness, order=4, sigma_ratio=1, alpha_min=0,
+      alpha_max=0.05, kappa_max=10)       include/boundary/perfect_match_layer.h:33-35
+  GaussianWaveform(amplitude, tau, t_0)   waveform/gaussian_waveform.cpp:17-21
+  TFSF3D(distance_x, distance_y, distance_z, e_0, theta, phi, psi, waveform)
+                                          tfsf/tfsf_3d.cpp:13-31
+  NffftFd(distance_x, distance_y, distance_z, frequencies, theta, phi)
+                                          include/nffft/nffft_fd.h (constructor)
+  CurrentSource(cube, orientation, resistance, waveform)
+                                          lumped_element/current_source.cpp:5-16
+  Simulation(cell_size, cfl=0.99f)        include/simulation/simulation.h:25-27
+
+A `Scene` bundles them the way `Simulation::add*` does.  Both the HIP engine
+(through the C-ABI) and the CPU oracle read this one description.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+# include/util/type_define.h:14
+XN, XP, YN, YP, ZN, ZP = range(6)
+ORIENTATIONS = {"XN": XN, "XP": XP, "YN": YN, "YP": YP, "ZN": ZN, "ZP": ZP}
+EX, EY, EZ, HX, HY, HZ = range(6)
+COMPONENTS = {"EX": EX, "EY": EY, "EZ": EZ, "HX": HX, "HY": HY, "HZ": HZ}
+
+# util/constant.h
+PI = math.pi
+EPSILON_0 = 8.854187817e-12
+MU_0 = 4 * PI * 1e-7
+C_0 = math.sqrt(1 / (EPSILON_0 * MU_0))
+ETA_0 = math.sqrt(MU_0 / EPSILON_0)
+
+
+@dataclass
+class Cube:
+    origin: Tuple[float, float, float]
+    size: Tuple[float, float, float]
+
+
+@dataclass
+class Sphere:
+    center: Tuple[float, float, float]
+    radius: float
+
+
+@dataclass
+class Material:
+    name: str = "air"
+    eps_r: float = 1.0
+    mu_r: float = 1.0
+    sigma_e: float = 0.0
+    sigma_m: float = 0.0
+
+    @staticmethod
+    def air():
+        return Material("XFDTD_AIR", 1, 1, 0, 0)
+
+    @staticmethod
+    def pec():
+        # material/material.cpp:32-34
+        return Material("XFDTD_PEC", 1, 1, 1e10, 0)
+
+
+@dataclass
+class Object:
+    name: str
+    shape: object
+    material: Material
+
+
+@dataclass
+class PML:
+    orientation: int
+    thickness: int
+    order: int = 4
+    sigma_ratio: float = 1.0
+    alpha_min: float = 0.0
+    alpha_max: float = 0.05
+    kappa_max: float = 10.0
+
+
+@dataclass
+class GaussianWaveform:
+    tau: float
+    t_0: float
+    amplitude: float = 1.0
+
+
+@dataclass
+class TFSF3D:
+    distance_x: int
+    distance_y: int
+    distance_z: int
+    e_0: float
+    theta: float
+    phi: float
+    psi: float
+    waveform: GaussianWaveform
+
+
+@dataclass
+class NffftFd:
+    distance_x: int
+    distance_y: int
+    distance_z: int
+    frequencies: Sequence[float]
+    theta: Sequence[float]
+    phi: Sequence[float]
+
+
+@dataclass
+class CurrentSource:
+    cube: Cube
+    orientation: int
+    resistance: float
+    waveform: GaussianWaveform
+
+
+@dataclass
+class Probe:
+    comp: int
+    i: int
+    j: int
+    k: int
+
+
+@dataclass
+class Scene:
+    cell_size: float
+    cfl: float = 0.99
+    objects: List[Object] = field(default_factory=list)
+    boundaries: List[PML] = field(default_factory=list)
+    tfsf: Optional[TFSF3D] = None
+    nffft: Optional[NffftFd] = None
+    sources: List[CurrentSource] = field(default_factory=list)
+    probes: List[Probe] = field(default_factory=list)
+
+    def add_object(self, o: Object):
+        self.objects.append(o)
+        return self
+
+    def add_boundary(self, b: PML):
+        self.boundaries.append(b)
+        return self
+
+
+def cpml_all_faces(thickness, order=3, sigma_ratio=0.8, alpha_min=0.0,
+                   alpha_max=0.05, kappa_max=1.0):
+    """Six CPML faces in the XN, XP, YN, YP, ZN, ZP insertion order."""
+    return [PML(o, thickness, order, sigma_ratio, alpha_min, alpha_max, kappa_max)
+            for o in range(6)]
+
+
+def cube_domain_scene(n_total, dl, pml, cfl=0.9, objects=(), tfsf_gap=None,
+                      nffft_gap=None, freqs=(), theta=(), phi=(),
+                      incidence=(0.0, 0.0, 0.0), tau_steps=20.0):
+    """A cubic vacuum domain of n_total cells including `pml` CPML cells per
+    face, centred at the origin, like the reference's scatter examples
+    (examples/3d_pec_sphere_bistatic_rcs.cpp:34-101).
+
+    objects: extra Objects added after the vacuum domain.
+    tfsf_gap: TFSF distance from the grid edge (None = no TFSF source).
+    nffft_gap: NF-FF distance from the grid edge (None = no NF-FF surface).
+    The Gaussian has tau = tau_steps*dt and t_0 = 4.5*tau.
+    """
+    n_in = n_total - 2 * pml
+    half = n_in * dl / 2
+    sc = Scene(cell_size=dl, cfl=cfl)
+    sc.add_object(Object("domain", Cube((-half, -half, -half), (2 * half,) * 3),
+                         Material.air()))
+    for o in objects:
+        sc.add_object(o)
+    sc.boundaries = cpml_all_faces(pml)
+    dt = courant_dt(dl, cfl)
+    tau = tau_steps * dt
+    if tfsf_gap is not None:
+        th, ph, ps = incidence
+        sc.tfsf = TFSF3D(tfsf_gap, tfsf_gap, tfsf_gap, 1.0, th, ph, ps,
+                         GaussianWaveform(tau, 4.5 * tau, 1.0))
+    if nffft_gap is not None:
+        sc.nffft = NffftFd(nffft_gap, nffft_gap, nffft_gap, list(freqs),
+                           list(theta), list(phi))
+    return sc
+
+
+def courant_dt(dl, cfl):
+    """fdtd_basic_coff/fdtd_basic_coff.cpp:342-346 with cfl rounded to float."""
+    import numpy as np
+    c = float(np.float32(cfl))
+    return c / (C_0 * math.sqrt(3.0 / (dl * dl)))

@@ -1,0 +1,169 @@
+"""Python handle on the C++ Simulation through include/xfdtd_sim.h.
+
+`Simulation(scene, total_steps, precision)` builds the reference's objects
+from a scene description (xfdtd_amd/scene.py), runs the set-up on the CPU and
+steps the fields on the GPU.  All numerical work happens in the HIP engine;
+this module only moves arguments and result buffers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import lib
+
+FAR_FIELD_NAMES = ["n_theta", "n_phi", "l_theta", "l_phi", "e_theta", "e_phi", "h_theta", "h_phi"]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(v):
+    return np.ascontiguousarray(np.array(v, dtype=np.float64))
+
+
+class Simulation:
+    def __init__(self, scene, total_steps, precision=64, output_dir=None):
+        L = lib()
+        self.scene = scene
+        self.T = int(total_steps)
+        self.precision = int(precision)
+        self.h = L.xf_sim_create(float(scene.cell_size), float(scene.cfl))
+        if not self.h:
+            raise RuntimeError("xf_sim_create: " + L.xf_sim_last_error().decode())
+        try:
+            for o in scene.objects:
+                m = o.material
+                sh = o.shape
+                if hasattr(sh, "radius"):
+                    c = _f64(sh.center)
+                    self._chk(L.xf_sim_add_sphere(self.h, _dp(c), float(sh.radius), m.eps_r, m.mu_r,
+                                                  m.sigma_e, m.sigma_m))
+                else:
+                    p, s = _f64(sh.origin), _f64(sh.size)
+                    self._chk(L.xf_sim_add_cube(self.h, _dp(p), _dp(s), m.eps_r, m.mu_r, m.sigma_e,
+                                                m.sigma_m))
+            for b in scene.boundaries:
+                self._chk(L.xf_sim_add_pml(self.h, int(b.orientation), int(b.thickness), int(b.order),
+                                           b.sigma_ratio, b.alpha_min, b.alpha_max, b.kappa_max))
+            t = scene.tfsf
+            if t is not None:
+                dist = (C.c_size_t * 3)(t.distance_x, t.distance_y, t.distance_z)
+                w = t.waveform
+                self._chk(L.xf_sim_set_tfsf3d(self.h, dist, t.e_0, t.theta, t.phi, t.psi,
+                                              w.amplitude, w.tau, w.t_0))
+            n = scene.nffft
+            if n is not None:
+                dist = (C.c_size_t * 3)(n.distance_x, n.distance_y, n.distance_z)
+                fr, th, ph = _f64(n.frequencies), _f64(n.theta), _f64(n.phi)
+                od = output_dir.encode() if output_dir else None
+                self._chk(L.xf_sim_set_nffft_fd(self.h, dist, _dp(fr), len(fr), _dp(th), len(th),
+                                                _dp(ph), len(ph), od))
+            for s in scene.sources:
+                p, sz = _f64(s.cube.origin), _f64(s.cube.size)
+                w = s.waveform
+                self._chk(L.xf_sim_add_current_source(self.h, _dp(p), _dp(sz), int(s.orientation),
+                                                      float(s.resistance), w.amplitude, w.tau, w.t_0))
+            for p in scene.probes:
+                self._chk(L.xf_sim_add_probe(self.h, int(p.comp), p.i, p.j, p.k))
+            self._chk(L.xf_sim_init(self.h, self.T, self.precision))
+        except Exception:
+            L.xf_sim_destroy(self.h)
+            self.h = None
+            raise
+        d = (C.c_size_t * 3)()
+        L.xf_sim_dims(self.h, d)
+        self.dims = (d[0], d[1], d[2])
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError("xfdtd_amd: " + lib().xf_sim_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().xf_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def dt(self):
+        return lib().xf_sim_dt(self.h)
+
+    @property
+    def current_step(self):
+        return lib().xf_sim_current_step(self.h)
+
+    @property
+    def engine(self):
+        return lib().xf_sim_engine(self.h)
+
+    def field_shape(self, comp):
+        nx, ny, nz = self.dims
+        return [(nx, ny + 1, nz + 1), (nx + 1, ny, nz + 1), (nx + 1, ny + 1, nz),
+                (nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)][comp]
+
+    def step(self, n):
+        self._chk(lib().xf_sim_step(self.h, int(n)))
+
+    def sync(self):
+        self._chk(lib().xf_sim_sync(self.h))
+
+    def get_field(self, comp):
+        out = np.empty(self.field_shape(comp), dtype=np.float64)
+        self._chk(lib().xf_sim_get_field(self.h, comp, _dp(out)))
+        return out
+
+    def set_field(self, comp, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self.field_shape(comp)
+        self._chk(lib().xf_sim_set_field(self.h, comp, _dp(a)))
+
+    def get_nffft(self):
+        n = lib().xf_sim_nffft_size(self.h)
+        out = np.empty(2 * n, dtype=np.float64)
+        self._chk(lib().xf_sim_get_nffft(self.h, _dp(out)))
+        return out.view(np.complex128)
+
+    def farfield(self):
+        n = self.scene.nffft
+        shp = (len(n.frequencies), len(n.theta), len(n.phi))
+        M = shp[0] * shp[1] * shp[2]
+        out = np.empty(18 * M, dtype=np.float64)
+        self._chk(lib().xf_sim_farfield(self.h, _dp(out)))
+        cplx = out[:16 * M].view(np.complex128).reshape(8, *shp)
+        pw = out[16 * M:].reshape(2, *shp)
+        res = {k: cplx[q].copy() for q, k in enumerate(FAR_FIELD_NAMES)}
+        res["power_theta"] = pw[0].copy()
+        res["power_phi"] = pw[1].copy()
+        return res
+
+    def output_nffft(self):
+        self._chk(lib().xf_sim_output_nffft(self.h))
+
+    def get_probes(self):
+        out = np.empty((len(self.scene.probes), self.T), dtype=np.float64)
+        self._chk(lib().xf_sim_get_probes(self.h, _dp(out)))
+        return out
+
+    # ---- engine controls
+    def set_timing(self, on=True):
+        L = lib()
+        if L.xf_engine_set_timing(self.engine, 1 if on else 0) != 0:
+            raise RuntimeError(L.xf_last_error().decode())
+
+    def kernel_times(self):
+        """Average device time (ms) of the E and H update kernels per step."""
+        L = lib()
+        a, b, n = C.c_double(), C.c_double(), C.c_int64()
+        if L.xf_engine_kernel_times(self.engine, C.byref(a), C.byref(b), C.byref(n)) != 0:
+            raise RuntimeError(L.xf_last_error().decode())
+        return a.value, b.value, n.value
+
+    def set_tiling(self, tz, ty, tx):
+        L = lib()
+        if L.xf_engine_set_tiling(self.engine, int(tz), int(ty), int(tx)) != 0:
+            raise RuntimeError(L.xf_last_error().decode())
