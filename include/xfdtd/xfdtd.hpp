@@ -37,6 +37,8 @@ struct xf_engine;
 
 namespace xfdtd {
 
+struct PlanData;
+
 namespace constant {
 inline constexpr double PI{M_PI};
 inline constexpr double EPSILON_0{8.854187817e-12};
@@ -344,6 +346,9 @@ class Simulation {
   void run(size_t total_time_steps);
 
   // The same loop in pieces, for callers that interleave host work.
+  // plan() does the CPU set-up only (no device needed); init() plans if
+  // needed and uploads the tables to a new engine.
+  void plan(size_t total_time_steps);
   void init(size_t total_time_steps);
   void step(size_t n);
   void sync();
@@ -368,6 +373,10 @@ class Simulation {
   std::vector<double> getProbeData() const;  // [n_probes][total_steps]
 
   xf_engine *engine() const { return _engine; }
+  // per-cell coefficient array `which` (0..17: cexe, cexhy, cexhz, ceye, ceyhz,
+  // ceyhx, ceze, cezhx, cezhy, chxh, chxey, chxez, chyh, chyez, chyex, chzh,
+  // chzex, chzey) implied by the plan's material map and tables
+  std::vector<double> planCoefficient(int which) const;
   // node coordinates of the uniform grid (grid_space.cpp:625-638)
   const std::vector<double> &eNodeCoord(int axis) const { return _ec[axis]; }
 
@@ -390,8 +399,10 @@ class Simulation {
   size_t _total_time_steps = 0;
   size_t _nffft_box[6] = {};
   xf_engine *_engine = nullptr;
+  std::shared_ptr<PlanData> _plan;
 
   void initGridSpace();
+  void upload();
   void resizeSpaceSize(const double nmin[3], const double nmax[3]);
   size_t handleTransform(int axis, double x) const;
 };

@@ -53,6 +53,15 @@ int xf_sim_add_current_source(xf_sim *s, const double origin[3], const double si
                               double t0);
 int xf_sim_add_probe(xf_sim *s, int comp, size_t i, size_t j, size_t k);
 
+/* CPU set-up only (Simulation::init without the device): grid, materials,
+   coefficient tables, CPML, TFSF and NF-FF tables.  Needs no GPU. */
+int xf_sim_plan(xf_sim *s, size_t total_steps);
+/* per-cell coefficient array `which` (0..17: cexe, cexhy, cexhz, ceye, ceyhz,
+   ceyhx, ceze, cezhx, cezhy, chxh, chxey, chxez, chyh, chyez, chyex, chzh,
+   chzex, chzey; fdtd_basic_coff.h) implied by the plan, in the reference's
+   array shape: the check that material index + tables reproduce the
+   reference's 18 arrays */
+int xf_sim_plan_coeff(xf_sim *s, int which, double *out);
 /* precision: 64 (FP64 fields, parity with the reference) or 32 */
 int xf_sim_init(xf_sim *s, size_t total_steps, int precision);
 int xf_sim_step(xf_sim *s, size_t n);

@@ -86,7 +86,7 @@ struct xf_engine {
   long long cur = 0;
   long long total = -1;  // smallest total_steps among the attached tables
   // tiling
-  int tz = 64, ty = 4, tx = 0;
+  int tz = 64, ty = 4, tx = 8;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 4 events per timed step: E start/end, H start/end
@@ -490,10 +490,19 @@ static xf_status run_steps(xf_engine *e, long long n) {
   Grid<T> g = make_grid<T>(e);
   Tfsf<T> t{};
   if (e->has_tfsf) t = make_tfsf<T>(e);
-  const int tz = e->tz, ty = e->ty;
-  const int tx = e->tx > 0 ? e->tx : e->nx + 1;
-  dim3 blk(tz, ty, 1);
-  dim3 grd((e->nz + 1 + tz - 1) / tz, (e->ny + 1 + ty - 1) / ty, (e->nx + 1 + tx - 1) / tx);
+  Tiling tl;
+  tl.tz = e->tz;
+  tl.ty = e->ty;
+  tl.tx = e->tx > 0 ? e->tx : e->nx + 1;
+  tl.ntz = (e->nz + 1 + tl.tz - 1) / tl.tz;
+  tl.nty = (e->ny + 1 + tl.ty - 1) / tl.ty;
+  tl.ntx = (e->nx + 1 + tl.tx - 1) / tl.tx;
+  tl.nblocks = tl.ntz * tl.nty * tl.ntx;
+  dim3 blk(tl.tz * tl.ty);
+  dim3 grd(tl.nblocks);
+  // coefficient tables of a tile go to LDS when they fit in 32 KB
+  const size_t lds = lds_table_bytes<T>(e->nmat, tl);
+  const bool use_lds = lds <= 32768;
   for (long long q = 0; q < n; ++q) {
     const long long step = e->cur;
     hipEvent_t *ev = nullptr;
@@ -509,7 +518,10 @@ static xf_status run_steps(xf_engine *e, long long n) {
       e->ev_used += 4;
     }
     if (ev) hipEventRecord(ev[0], e->stream);
-    hipLaunchKernelGGL(k_update_e<T>, grd, blk, 0, e->stream, g, tx);
+    if (use_lds)
+      hipLaunchKernelGGL((k_update_e<T, true>), grd, blk, lds, e->stream, g, tl);
+    else
+      hipLaunchKernelGGL((k_update_e<T, false>), grd, blk, 0, e->stream, g, tl);
     if (ev) hipEventRecord(ev[1], e->stream);
     for (const SourceDev &sd : e->sources) {
       const int nj = sd.j1 - sd.j0, nk = sd.k1 - sd.k0;
@@ -524,7 +536,10 @@ static xf_status run_steps(xf_engine *e, long long n) {
                          e->nEz, e->nEx);
     }
     if (ev) hipEventRecord(ev[2], e->stream);
-    hipLaunchKernelGGL(k_update_h<T>, grd, blk, 0, e->stream, g, tx);
+    if (use_lds)
+      hipLaunchKernelGGL((k_update_h<T, true>), grd, blk, lds, e->stream, g, tl);
+    else
+      hipLaunchKernelGGL((k_update_h<T, false>), grd, blk, 0, e->stream, g, tl);
     if (ev) hipEventRecord(ev[3], e->stream);
     if (e->has_tfsf) {
       hipLaunchKernelGGL(k_tfsf_ifa<T>, dim3(1), dim3(1024), 0, e->stream, t, step);

@@ -52,47 +52,192 @@ __device__ __forceinline__ int h_layer(int n, int N, int dn, int dp) {
   return -1;
 }
 
+// Launch tiling of the curl kernels: a 1-D grid of tiles, each TZ x TY
+// threads over (k, j) marching over TX x-planes.  The hardware hands
+// consecutive workgroup ids to the 8 XCDs round-robin; tile ids are remapped
+// so that each XCD works on one contiguous run of tiles, which keeps a tile's
+// y-1 neighbour row in the same XCD's L2.
+struct Tiling {
+  int tz, ty, tx;           // threads along k, j; planes along i
+  int ntz, nty, ntx;        // tile counts
+  int nblocks;
+};
+
+__device__ __forceinline__ void tile_of_block(const Tiling &t, int &bz, int &by, int &bx) {
+  const int b = blockIdx.x;
+  const int q = t.nblocks >> 3, r = t.nblocks & 7;
+  const int xcd = b & 7, local = b >> 3;
+  const int id = xcd * q + min(xcd, r) + local;
+  bz = id % t.ntz;
+  const int rest = id / t.ntz;
+  by = rest % t.nty;
+  bx = rest / t.nty;
+}
+
+// ---------------------------------------------------------------------------
+// Coefficient tables of one tile, staged in LDS when they fit.  The curl
+// coefficients of a cell are functions of its material and of one node index
+// (fdtd_basic_coff.cpp:93-121), so a tile of TZ x TY threads marching over TX
+// planes needs, per material: 3 self terms, 2 tables over its k range, 2 over
+// its j range and 2 over its i range.
+template <typename T>
+struct TileTables {
+  const T *ca;    // [3][nmat]
+  const T *tk0, *tk1;  // [nmat][TZ]
+  const T *tj0, *tj1;  // [nmat][TY]
+  const T *ti0, *ti1;  // [nmat][TX]
+  int sk, sj, si;      // row stride (entries per material) of the k/j/i tables
+  int k0, j0, i0;      // tile origin subtracted from global indices
+};
+
+template <typename T>
+__host__ __device__ inline size_t lds_table_bytes(int nmat, const Tiling &tl) {
+  return sizeof(T) * (size_t)nmat * (3 + 2 * tl.tz + 2 * tl.ty + 2 * tl.tx);
+}
+
+// stage the tables of one tile (cooperative; ends with a barrier).
+// self: ca entries ca_base..ca_base+2; k/j/i: global tables with lengths
+// (n+1) per material.
+template <typename T>
+__device__ __forceinline__ TileTables<T> stage_tables(T *lds, const T *ca, int ca_base, int nmat,
+                                                     const T *gk0, const T *gk1, int nk1,
+                                                     const T *gj0, const T *gj1, int nj1,
+                                                     const T *gi0, const T *gi1, int ni1,
+                                                     const Tiling &tl, int k0, int j0, int i0) {
+  TileTables<T> tt;
+  T *p = lds;
+  T *sca = p; p += 3 * nmat;
+  T *sk0 = p; p += nmat * tl.tz;
+  T *sk1 = p; p += nmat * tl.tz;
+  T *sj0 = p; p += nmat * tl.ty;
+  T *sj1 = p; p += nmat * tl.ty;
+  T *si0 = p; p += nmat * tl.tx;
+  T *si1 = p;
+  const int nth = blockDim.x;
+  for (int q = threadIdx.x; q < 3 * nmat; q += nth) sca[q] = ca[(ca_base + q / nmat) * nmat + q % nmat];
+  for (int q = threadIdx.x; q < nmat * tl.tz; q += nth) {
+    const int m = q / tl.tz, n = min(k0 + q % tl.tz, nk1 - 1);
+    sk0[q] = gk0[m * nk1 + n];
+    sk1[q] = gk1[m * nk1 + n];
+  }
+  for (int q = threadIdx.x; q < nmat * tl.ty; q += nth) {
+    const int m = q / tl.ty, n = min(j0 + q % tl.ty, nj1 - 1);
+    sj0[q] = gj0[m * nj1 + n];
+    sj1[q] = gj1[m * nj1 + n];
+  }
+  for (int q = threadIdx.x; q < nmat * tl.tx; q += nth) {
+    const int m = q / tl.tx, n = min(i0 + q % tl.tx, ni1 - 1);
+    si0[q] = gi0[m * ni1 + n];
+    si1[q] = gi1[m * ni1 + n];
+  }
+  __syncthreads();
+  tt.ca = sca;
+  tt.tk0 = sk0; tt.tk1 = sk1; tt.tj0 = sj0; tt.tj1 = sj1; tt.ti0 = si0; tt.ti1 = si1;
+  tt.sk = tl.tz; tt.sj = tl.ty; tt.si = tl.tx;
+  tt.k0 = k0; tt.j0 = j0; tt.i0 = i0;
+  return tt;
+}
+
+// the same view over the global tables (no staging)
+template <typename T>
+__device__ __forceinline__ TileTables<T> global_tables(const T *ca, int ca_base, int nmat,
+                                                      const T *gk0, const T *gk1, int nk1,
+                                                      const T *gj0, const T *gj1, int nj1,
+                                                      const T *gi0, const T *gi1, int ni1) {
+  TileTables<T> tt;
+  tt.ca = ca + ca_base * nmat;
+  tt.tk0 = gk0; tt.tk1 = gk1; tt.tj0 = gj0; tt.tj1 = gj1; tt.ti0 = gi0; tt.ti1 = gi1;
+  tt.sk = nk1; tt.sj = nj1; tt.si = ni1;
+  tt.k0 = 0; tt.j0 = 0; tt.i0 = 0;
+  return tt;
+}
+
 // ---------------------------------------------------------------------------
 // E half-step: Simulation::updateE (simulation_update.cpp:170-196) fused with
 // PML::updateE of every face (perfect_match_layer.cpp:574-718), faces in x,
-// y, z order.  One thread owns a (j,k) column and marches over an x range.
-template <typename T>
-__global__ void __launch_bounds__(256) k_update_e(Grid<T> g, int tx) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
+// y, z order.  One thread owns a (j,k) column and marches over TX x-planes;
+// the loads of plane i+1 are issued before plane i is computed.
+template <typename T, bool LDS>
+__global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl) {
+  extern __shared__ unsigned char lds_raw[];
+  int bz, by, bx;
+  tile_of_block(tl, bz, by, bx);
+  const int kl = (int)(threadIdx.x % tl.tz), jl = (int)(threadIdx.x / tl.tz);
+  const int k = bz * tl.tz + kl;
+  const int j = by * tl.ty + jl;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
+  const int i0 = bx * tl.tx;
+  const int i1 = min(i0 + tl.tx, nx + 1);
+  TileTables<T> tt;
+  if (LDS)
+    tt = stage_tables<T>(reinterpret_cast<T *>(lds_raw), g.ca, EX, nmat, g.cb[CB_EXHY], g.cb[CB_EYHX],
+                         nz + 1, g.cb[CB_EXHZ], g.cb[CB_EZHX], ny + 1, g.cb[CB_EYHZ], g.cb[CB_EZHY],
+                         nx + 1, tl, bz * tl.tz, by * tl.ty, i0);
+  else
+    tt = global_tables<T>(g.ca, EX, nmat, g.cb[CB_EXHY], g.cb[CB_EYHX], nz + 1, g.cb[CB_EXHZ],
+                          g.cb[CB_EZHX], ny + 1, g.cb[CB_EYHZ], g.cb[CB_EZHY], nx + 1);
   if (k > nz || j > ny) return;
-  const int i0 = blockIdx.z * tx;
-  const int i1 = min(i0 + tx, nx + 1);
   const int nm1y = ny + 1, nm1z = nz + 1, nm1x = nx + 1;
-  T *Ex = g.f[EX], *Ey = g.f[EY], *Ez = g.f[EZ];
-  const T *Hx = g.f[HX], *Hy = g.f[HY], *Hz = g.f[HZ];
-  const int Lx = g.dn[0] + g.dp[0], Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2];
+  T *__restrict__ Ex = g.f[EX];
+  T *__restrict__ Ey = g.f[EY];
+  T *__restrict__ Ez = g.f[EZ];
+  const T *__restrict__ Hx = g.f[HX];
+  const T *__restrict__ Hy = g.f[HY];
+  const T *__restrict__ Hz = g.f[HZ];
+  const uint8_t *__restrict__ mat = g.mat;
+  const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2];
   const int ly = e_layer(j, ny, g.dn[1], g.dp[1]);
   const int lz = e_layer(k, nz, g.dn[2], g.dp[2]);
-  for (int i = i0; i < i1; ++i) {
-    const long long c = (long long)i * g.si + (long long)j * g.sj + k;
-    const int m = g.mat[c];
+  const bool kin = (k >= 1 && k < nz), jin = (j >= 1 && j < ny);
+  const long long cj = (long long)j * g.sj + k;
+  // plane registers: current (x_c) and next (x_n)
+  long long c = (long long)i0 * g.si + cj;
+  T hz_prev = (i0 >= 1) ? Hz[c - g.si] : T(0);
+  T hy_prev = (i0 >= 1) ? Hy[c - g.si] : T(0);
+  int m_n = mat[c];
+  T ex_n = Ex[c], ey_n = Ey[c], ez_n = Ez[c];
+  T hx_n = Hx[c], hy_n = Hy[c], hz_n = Hz[c];
+  T hyk_n = (k >= 1) ? Hy[c - 1] : T(0);
+  T hxk_n = (k >= 1) ? Hx[c - 1] : T(0);
+  T hzj_n = (j >= 1) ? Hz[c - g.sj] : T(0);
+  T hxj_n = (j >= 1) ? Hx[c - g.sj] : T(0);
+  for (int i = i0; i < i1; ++i, c += g.si) {
+    const int m = m_n;
+    const T ex = ex_n, ey = ey_n, ez = ez_n, hx = hx_n, hy = hy_n, hz = hz_n;
+    const T hyk = hyk_n, hxk = hxk_n, hzj = hzj_n, hxj = hxj_n;
+    if (i + 1 < i1) {
+      const long long cn = c + g.si;
+      m_n = mat[cn];
+      ex_n = Ex[cn]; ey_n = Ey[cn]; ez_n = Ez[cn];
+      hx_n = Hx[cn]; hy_n = Hy[cn]; hz_n = Hz[cn];
+      hyk_n = (k >= 1) ? Hy[cn - 1] : T(0);
+      hxk_n = (k >= 1) ? Hx[cn - 1] : T(0);
+      hzj_n = (j >= 1) ? Hz[cn - g.sj] : T(0);
+      hxj_n = (j >= 1) ? Hx[cn - g.sj] : T(0);
+    }
     const int lx = e_layer(i, nx, g.dn[0], g.dp[0]);
+    const T dhy_k = hy - hyk;      // Hy(k) - Hy(k-1)
+    const T dhz_j = hz - hzj;      // Hz(j) - Hz(j-1)
+    const T dhx_k = hx - hxk;      // Hx(k) - Hx(k-1)
+    const T dhz_i = hz - hz_prev;  // Hz(i) - Hz(i-1)
+    const T dhx_j = hx - hxj;      // Hx(j) - Hx(j-1)
+    const T dhy_i = hy - hy_prev;  // Hy(i) - Hy(i-1)
+    const int mk = m * tt.sk + (k - tt.k0), mj = m * tt.sj + (j - tt.j0), mi = m * tt.si + (i - tt.i0);
     // Ex: shape (nx, ny+1, nz+1); Yee j in [1,ny), k in [1,nz)
     if (i < nx) {
-      const bool yee = (j >= 1 && j < ny && k >= 1 && k < nz);
+      const bool yee = jin && kin;
       if (yee || ly >= 0 || lz >= 0) {
-        T v = Ex[c];
-        const T dhy = (k >= 1) ? (Hy[c] - Hy[c - 1]) : T(0);
-        const T dhz = (j >= 1) ? (Hz[c] - Hz[c - g.sj]) : T(0);
-        if (yee)
-          v = g.ca[EX * g.nmat + m] * v + g.cb[CB_EXHY][m * nm1z + k] * dhy +
-              g.cb[CB_EXHZ][m * nm1y + j] * dhz;
+        T v = ex;
+        if (yee) v = tt.ca[m] * v + tt.tk0[mk] * dhy_k + tt.tj0[mj] * dhz_j;
         if (ly >= 0) {
           T *p = &g.psi_e[P_AY][((long long)i * Ly + ly) * g.pz + k];
-          const T ps = g.be[1][j] * (*p) + g.ae[1][j] * dhz;
+          const T ps = g.be[1][j] * (*p) + g.ae[1][j] * dhz_j;
           *p = ps;
           v = v + g.cpsi[CB_EXHZ][m * nm1y + j] * ps;
         }
         if (lz >= 0) {
           T *p = &g.psi_e[P_AZ][((long long)i * nm1y + j) * Lz + lz];
-          const T ps = g.be[2][k] * (*p) + g.ae[2][k] * dhy;
+          const T ps = g.be[2][k] * (*p) + g.ae[2][k] * dhy_k;
           *p = ps;
           v = v + g.cpsi[CB_EXHY][m * nm1z + k] * ps;
         }
@@ -101,23 +246,19 @@ __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, int tx) {
     }
     // Ey: shape (nx+1, ny, nz+1); Yee k in [1,nz), i in [1,nx)
     if (j < ny) {
-      const bool yee = (i >= 1 && i < nx && k >= 1 && k < nz);
+      const bool yee = (i >= 1 && i < nx) && kin;
       if (yee || lx >= 0 || lz >= 0) {
-        T v = Ey[c];
-        const T dhx = (k >= 1) ? (Hx[c] - Hx[c - 1]) : T(0);
-        const T dhz = (i >= 1) ? (Hz[c] - Hz[c - g.si]) : T(0);
-        if (yee)
-          v = g.ca[EY * g.nmat + m] * v + g.cb[CB_EYHX][m * nm1z + k] * dhx +
-              g.cb[CB_EYHZ][m * nm1x + i] * dhz;
+        T v = ey;
+        if (yee) v = tt.ca[nmat + m] * v + tt.tk1[mk] * dhx_k + tt.ti0[mi] * dhz_i;
         if (lx >= 0) {
           T *p = &g.psi_e[P_BX][((long long)lx * nm1y + j) * g.pz + k];
-          const T ps = g.be[0][i] * (*p) + g.ae[0][i] * dhz;
+          const T ps = g.be[0][i] * (*p) + g.ae[0][i] * dhz_i;
           *p = ps;
           v = v + g.cpsi[CB_EYHZ][m * nm1x + i] * ps;
         }
         if (lz >= 0) {
           T *p = &g.psi_e[P_BZ][((long long)i * nm1y + j) * Lz + lz];
-          const T ps = g.be[2][k] * (*p) + g.ae[2][k] * dhx;
+          const T ps = g.be[2][k] * (*p) + g.ae[2][k] * dhx_k;
           *p = ps;
           v = v + g.cpsi[CB_EYHX][m * nm1z + k] * ps;
         }
@@ -126,73 +267,116 @@ __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, int tx) {
     }
     // Ez: shape (nx+1, ny+1, nz); Yee i in [1,nx), j in [1,ny)
     if (k < nz) {
-      const bool yee = (i >= 1 && i < nx && j >= 1 && j < ny);
+      const bool yee = (i >= 1 && i < nx) && jin;
       if (yee || lx >= 0 || ly >= 0) {
-        T v = Ez[c];
-        const T dhx = (j >= 1) ? (Hx[c] - Hx[c - g.sj]) : T(0);
-        const T dhy = (i >= 1) ? (Hy[c] - Hy[c - g.si]) : T(0);
-        if (yee)
-          v = g.ca[EZ * g.nmat + m] * v + g.cb[CB_EZHX][m * nm1y + j] * dhx +
-              g.cb[CB_EZHY][m * nm1x + i] * dhy;
+        T v = ez;
+        if (yee) v = tt.ca[2 * nmat + m] * v + tt.tj1[mj] * dhx_j + tt.ti1[mi] * dhy_i;
         if (lx >= 0) {
           T *p = &g.psi_e[P_CX][((long long)lx * nm1y + j) * g.pz + k];
-          const T ps = g.be[0][i] * (*p) + g.ae[0][i] * dhy;
+          const T ps = g.be[0][i] * (*p) + g.ae[0][i] * dhy_i;
           *p = ps;
           v = v + g.cpsi[CB_EZHY][m * nm1x + i] * ps;
         }
         if (ly >= 0) {
           T *p = &g.psi_e[P_CY][((long long)i * Ly + ly) * g.pz + k];
-          const T ps = g.be[1][j] * (*p) + g.ae[1][j] * dhx;
+          const T ps = g.be[1][j] * (*p) + g.ae[1][j] * dhx_j;
           *p = ps;
           v = v + g.cpsi[CB_EZHX][m * nm1y + j] * ps;
         }
         Ez[c] = v;
       }
     }
+    hz_prev = hz;
+    hy_prev = hy;
   }
-  (void)Lx;
 }
 
 // ---------------------------------------------------------------------------
 // H half-step: Simulation::updateH (simulation_update.cpp:84-100) fused with
 // PML::updateH (perfect_match_layer.cpp:434-572).
-template <typename T>
-__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, int tx) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
+template <typename T, bool LDS>
+__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl) {
+  extern __shared__ unsigned char lds_raw[];
+  int bz, by, bx;
+  tile_of_block(tl, bz, by, bx);
+  const int kl = (int)(threadIdx.x % tl.tz), jl = (int)(threadIdx.x / tl.tz);
+  const int k = bz * tl.tz + kl;
+  const int j = by * tl.ty + jl;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
+  const int i0 = bx * tl.tx;
+  const int i1 = min(i0 + tl.tx, nx + 1);
+  TileTables<T> tt;
+  if (LDS)
+    tt = stage_tables<T>(reinterpret_cast<T *>(lds_raw), g.ca, HX, nmat, g.cb[CB_HXEY], g.cb[CB_HYEX],
+                         nz + 1, g.cb[CB_HXEZ], g.cb[CB_HZEX], ny + 1, g.cb[CB_HYEZ], g.cb[CB_HZEY],
+                         nx + 1, tl, bz * tl.tz, by * tl.ty, i0);
+  else
+    tt = global_tables<T>(g.ca, HX, nmat, g.cb[CB_HXEY], g.cb[CB_HYEX], nz + 1, g.cb[CB_HXEZ],
+                          g.cb[CB_HZEX], ny + 1, g.cb[CB_HYEZ], g.cb[CB_HZEY], nx + 1);
   if (k > nz || j > ny) return;
-  const int i0 = blockIdx.z * tx;
-  const int i1 = min(i0 + tx, nx + 1);
   const int nm1y = ny + 1, nm1z = nz + 1, nm1x = nx + 1;
-  const T *Ex = g.f[EX], *Ey = g.f[EY], *Ez = g.f[EZ];
-  T *Hx = g.f[HX], *Hy = g.f[HY], *Hz = g.f[HZ];
+  const T *__restrict__ Ex = g.f[EX];
+  const T *__restrict__ Ey = g.f[EY];
+  const T *__restrict__ Ez = g.f[EZ];
+  T *__restrict__ Hx = g.f[HX];
+  T *__restrict__ Hy = g.f[HY];
+  T *__restrict__ Hz = g.f[HZ];
+  const uint8_t *__restrict__ mat = g.mat;
   const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2];
   const int ly = h_layer(j, ny, g.dn[1], g.dp[1]);
   const int lz = h_layer(k, nz, g.dn[2], g.dp[2]);
-  for (int i = i0; i < i1; ++i) {
-    const long long c = (long long)i * g.si + (long long)j * g.sj + k;
-    const int m = g.mat[c];
+  const bool kin = k < nz, jin = j < ny;
+  long long c = (long long)i0 * g.si + (long long)j * g.sj + k;
+  // Ez(i), Ey(i) come from the previous plane's (i+1) loads
+  T ez_c = Ez[c], ey_c = Ey[c];
+  int m_n = mat[c];
+  T hx_n = Hx[c], hy_n = Hy[c], hz_n = Hz[c];
+  T ex_n = Ex[c];
+  T eyk_n = kin ? Ey[c + 1] : T(0);    // Ey(k+1)
+  T exk_n = kin ? Ex[c + 1] : T(0);    // Ex(k+1)
+  T ezj_n = jin ? Ez[c + g.sj] : T(0); // Ez(j+1)
+  T exj_n = jin ? Ex[c + g.sj] : T(0); // Ex(j+1)
+  T ezi_n = (i0 < nx) ? Ez[c + g.si] : T(0);  // Ez(i+1)
+  T eyi_n = (i0 < nx) ? Ey[c + g.si] : T(0);  // Ey(i+1)
+  for (int i = i0; i < i1; ++i, c += g.si) {
+    const int m = m_n;
+    const T hx = hx_n, hy = hy_n, hz = hz_n, ex = ex_n;
+    const T eyk = eyk_n, exk = exk_n, ezj = ezj_n, exj = exj_n, ezi = ezi_n, eyi = eyi_n;
+    if (i + 1 < i1) {
+      const long long cn = c + g.si;
+      m_n = mat[cn];
+      hx_n = Hx[cn]; hy_n = Hy[cn]; hz_n = Hz[cn];
+      ex_n = Ex[cn];
+      eyk_n = kin ? Ey[cn + 1] : T(0);
+      exk_n = kin ? Ex[cn + 1] : T(0);
+      ezj_n = jin ? Ez[cn + g.sj] : T(0);
+      exj_n = jin ? Ex[cn + g.sj] : T(0);
+      ezi_n = (i + 1 < nx) ? Ez[cn + g.si] : T(0);
+      eyi_n = (i + 1 < nx) ? Ey[cn + g.si] : T(0);
+    }
     const int lx = h_layer(i, nx, g.dn[0], g.dp[0]);
+    const T dey_k = eyk - ey_c;  // Ey(k+1) - Ey(k)
+    const T dez_j = ezj - ez_c;  // Ez(j+1) - Ez(j)
+    const T dez_i = ezi - ez_c;  // Ez(i+1) - Ez(i)
+    const T dex_k = exk - ex;    // Ex(k+1) - Ex(k)
+    const T dex_j = exj - ex;    // Ex(j+1) - Ex(j)
+    const T dey_i = eyi - ey_c;  // Ey(i+1) - Ey(i)
+    const int mk = m * tt.sk + (k - tt.k0), mj = m * tt.sj + (j - tt.j0), mi = m * tt.si + (i - tt.i0);
     // Hx: shape (nx+1, ny, nz); Yee i < nx
-    if (j < ny && k < nz) {
+    if (jin && kin) {
       const bool yee = (i < nx);
       if (yee || ly >= 0 || lz >= 0) {
-        T v = Hx[c];
-        const T dey = Ey[c + 1] - Ey[c];
-        const T dez = Ez[c + g.sj] - Ez[c];
-        if (yee)
-          v = g.ca[HX * g.nmat + m] * v + g.cb[CB_HXEY][m * nm1z + k] * dey +
-              g.cb[CB_HXEZ][m * nm1y + j] * dez;
+        T v = hx;
+        if (yee) v = tt.ca[m] * v + tt.tk0[mk] * dey_k + tt.tj0[mj] * dez_j;
         if (ly >= 0) {
           T *p = &g.psi_h[P_AY][((long long)i * Ly + ly) * g.pz + k];
-          const T ps = g.bm[1][j] * (*p) + g.am[1][j] * dez;
+          const T ps = g.bm[1][j] * (*p) + g.am[1][j] * dez_j;
           *p = ps;
           v = v + g.cpsi[CB_HXEZ][m * nm1y + j] * ps;
         }
         if (lz >= 0) {
           T *p = &g.psi_h[P_AZ][((long long)i * nm1y + j) * Lz + lz];
-          const T ps = g.bm[2][k] * (*p) + g.am[2][k] * dey;
+          const T ps = g.bm[2][k] * (*p) + g.am[2][k] * dey_k;
           *p = ps;
           v = v + g.cpsi[CB_HXEY][m * nm1z + k] * ps;
         }
@@ -200,24 +384,20 @@ __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, int tx) {
       }
     }
     // Hy: shape (nx, ny+1, nz); Yee j < ny
-    if (i < nx && k < nz) {
-      const bool yee = (j < ny);
+    if (i < nx && kin) {
+      const bool yee = jin;
       if (yee || lx >= 0 || lz >= 0) {
-        T v = Hy[c];
-        const T dez = Ez[c + g.si] - Ez[c];
-        const T dex = Ex[c + 1] - Ex[c];
-        if (yee)
-          v = g.ca[HY * g.nmat + m] * v + g.cb[CB_HYEZ][m * nm1x + i] * dez +
-              g.cb[CB_HYEX][m * nm1z + k] * dex;
+        T v = hy;
+        if (yee) v = tt.ca[nmat + m] * v + tt.ti0[mi] * dez_i + tt.tk1[mk] * dex_k;
         if (lx >= 0) {
           T *p = &g.psi_h[P_BX][((long long)lx * nm1y + j) * g.pz + k];
-          const T ps = g.bm[0][i] * (*p) + g.am[0][i] * dez;
+          const T ps = g.bm[0][i] * (*p) + g.am[0][i] * dez_i;
           *p = ps;
           v = v + g.cpsi[CB_HYEZ][m * nm1x + i] * ps;
         }
         if (lz >= 0) {
           T *p = &g.psi_h[P_BZ][((long long)i * nm1y + j) * Lz + lz];
-          const T ps = g.bm[2][k] * (*p) + g.am[2][k] * dex;
+          const T ps = g.bm[2][k] * (*p) + g.am[2][k] * dex_k;
           *p = ps;
           v = v + g.cpsi[CB_HYEX][m * nm1z + k] * ps;
         }
@@ -225,33 +405,30 @@ __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, int tx) {
       }
     }
     // Hz: shape (nx, ny, nz+1); Yee k < nz
-    if (i < nx && j < ny) {
-      const bool yee = (k < nz);
+    if (i < nx && jin) {
+      const bool yee = kin;
       if (yee || lx >= 0 || ly >= 0) {
-        T v = Hz[c];
-        const T dex = Ex[c + g.sj] - Ex[c];
-        const T dey = Ey[c + g.si] - Ey[c];
-        if (yee)
-          v = g.ca[HZ * g.nmat + m] * v + g.cb[CB_HZEX][m * nm1y + j] * dex +
-              g.cb[CB_HZEY][m * nm1x + i] * dey;
+        T v = hz;
+        if (yee) v = tt.ca[2 * nmat + m] * v + tt.tj1[mj] * dex_j + tt.ti1[mi] * dey_i;
         if (lx >= 0) {
           T *p = &g.psi_h[P_CX][((long long)lx * nm1y + j) * g.pz + k];
-          const T ps = g.bm[0][i] * (*p) + g.am[0][i] * dey;
+          const T ps = g.bm[0][i] * (*p) + g.am[0][i] * dey_i;
           *p = ps;
           v = v + g.cpsi[CB_HZEY][m * nm1x + i] * ps;
         }
         if (ly >= 0) {
           T *p = &g.psi_h[P_CY][((long long)i * Ly + ly) * g.pz + k];
-          const T ps = g.bm[1][j] * (*p) + g.am[1][j] * dex;
+          const T ps = g.bm[1][j] * (*p) + g.am[1][j] * dex_j;
           *p = ps;
           v = v + g.cpsi[CB_HZEX][m * nm1y + j] * ps;
         }
         Hz[c] = v;
       }
     }
+    ez_c = ezi;
+    ey_c = eyi;
   }
 }
-
 // ---------------------------------------------------------------------------
 // TFSF (tfsf/tfsf_3d.cpp)
 template <typename T>

@@ -35,6 +35,31 @@ int compIndex(EMComponent c) { return static_cast<int>(c); }
 
 }  // namespace
 
+// Host-side tables of one simulation, computed by Simulation::plan and
+// handed to the engine by Simulation::upload.
+struct PlanData {
+  int nmat = 0;
+  std::vector<uint8_t> midx;
+  std::vector<double> ca, cb[12], cpsi[12];
+  int dn[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
+  std::vector<double> be[3], ae[3], bm[3], am[3];
+  struct Src {
+    int comp, is, ie, js, je, ks, ke;
+    std::vector<double> coff, wave;
+  };
+  std::vector<Src> srcs;
+  struct Tfsf {
+    bool present = false;
+    xf_tfsf_desc desc{};
+    std::vector<double> full[3], half[3], source;
+  } tfsf;
+  struct Nf {
+    bool present = false;
+    xf_nffft_desc desc{};
+    std::vector<std::complex<double>> tj, tm;
+  } nf;
+};
+
 // shape/cube.cpp:27-37
 bool Cube::isPointInside(const PointVector &p) const {
   return isLessOrEqual(_point[0], p[0]) && isLessOrEqual(_point[1], p[1]) &&
@@ -278,8 +303,9 @@ double coffHb(const Mat &m, double dt, double da) {  // chceb
 
 }  // namespace
 
-void Simulation::init(size_t total_time_steps) {
-  if (_engine) throw std::runtime_error("Simulation already initialised");
+void Simulation::plan(size_t total_time_steps) {
+  if (_plan) throw std::runtime_error("Simulation already planned");
+  _plan = std::make_shared<PlanData>();
   _total_time_steps = total_time_steps;
   // simulation_init.cpp:10-22
   initGridSpace();
@@ -521,19 +547,24 @@ void Simulation::init(size_t total_time_steps) {
         if (in_slab_e(a, n)) throw std::runtime_error("current source inside a CPML slab is not supported");
   }
 
-  // ---- engine
-  xf_grid_desc gd;
-  gd.nx = nx; gd.ny = ny; gd.nz = nz; gd.precision = _precision; gd.n_materials = nmat;
-  check(xf_engine_create(&gd, &_engine));
-  const double *cbp[12], *cpp[12];
-  for (int t = 0; t < 12; ++t) { cbp[t] = cb[t].data(); cpp[t] = cpsi[t].data(); }
-  for (int a = 0; a < 3; ++a)
-    if (dn[a] + dp[a] > 0)
-      check(xf_engine_set_cpml(_engine, a, dn[a], dp[a], be[a].data(), ae[a].data(), bm[a].data(), am[a].data()));
-  check(xf_engine_set_materials(_engine, midx.data(), ca.data(), cbp, cpp));
-  for (const auto &s : srcs)
-    check(xf_engine_add_current_source(_engine, s.comp, s.is, s.ie, s.js, s.je, s.ks, s.ke, s.coff.data(),
-                                       s.wave.data(), (int64_t)total_time_steps));
+  // ---- the plan: every table the engine needs
+  PlanData &P = *_plan;
+  P.nmat = nmat;
+  P.midx = std::move(midx);
+  P.ca = std::move(ca);
+  for (int t = 0; t < 12; ++t) { P.cb[t] = std::move(cb[t]); P.cpsi[t] = std::move(cpsi[t]); }
+  for (int a = 0; a < 3; ++a) {
+    P.dn[a] = dn[a]; P.dp[a] = dp[a];
+    P.be[a] = std::move(be[a]); P.ae[a] = std::move(ae[a]);
+    P.bm[a] = std::move(bm[a]); P.am[a] = std::move(am[a]);
+  }
+  for (auto &s_ : srcs) {
+    PlanData::Src ps;
+    ps.comp = s_.comp; ps.is = s_.is; ps.ie = s_.ie; ps.js = s_.js; ps.je = s_.je; ps.ks = s_.ks; ps.ke = s_.ke;
+    ps.coff = std::move(s_.coff);
+    ps.wave = std::move(s_.wave);
+    P.srcs.push_back(std::move(ps));
+  }
 
   // ---- TFSF (tfsf/tfsf.cpp:35-53, tfsf_3d.cpp:33-151)
   if (_tfsf) {
@@ -554,8 +585,9 @@ void Simulation::init(size_t total_time_steps) {
     std::vector<double> time_arr(total_time_steps);
     for (size_t n = 0; n < total_time_steps; ++n) time_arr[n] = static_cast<double>(n) * dt;
     t3->_waveform->init(time_arr);
-    std::vector<double> source(total_time_steps);
-    for (size_t n = 0; n < total_time_steps; ++n) source[n] = t3->_e_0 * (*t3->_waveform)(n);
+    PlanData::Tfsf &tp = P.tfsf;
+    tp.source.resize(total_time_steps);
+    for (size_t n = 0; n < total_time_steps; ++n) tp.source[n] = t3->_e_0 * (*t3->_waveform)(n);
     const double th = t3->_theta_inc, ph = t3->_phi_inc;
     const double ratio = 1 / std::sqrt(std::pow(std::sin(th), 4) * (std::pow(std::cos(ph), 4) + std::pow(std::sin(ph), 4)) +
                                        std::pow(std::cos(th), 4));
@@ -566,7 +598,7 @@ void Simulation::init(size_t total_time_steps) {
     double org[3];
     for (int a = 0; a < 3; ++a) org[a] = (k[a] >= 0) ? (double)l[a] : (double)r[a];
     for (int a = 0; a < 3; ++a) org[a] -= 2 * k[a] / ratio;
-    std::vector<double> full[3], half[3];
+    std::vector<double> *full = tp.full, *half = tp.half;
     for (int a = 0; a < 3; ++a) {
       full[a].resize(bn[a] + 1);
       half[a].resize(bn[a] + 2);
@@ -586,7 +618,7 @@ void Simulation::init(size_t total_time_steps) {
     if (psum + 1 >= (double)L - 1 || pmin < 0)
       throw std::runtime_error("TFSF projection leaves the 1-D grid");
     const double scaled_dl = dx / ratio;
-    xf_tfsf_desc td;
+    xf_tfsf_desc &td = tp.desc;
     td.li = l[0]; td.ri = r[0]; td.lj = l[1]; td.rj = r[1]; td.lk = l[2]; td.rk = r[2];
     td.aux_size = (int64_t)L;
     for (int a = 0; a < 3; ++a) {
@@ -606,9 +638,8 @@ void Simulation::init(size_t total_time_steps) {
       td.ca[a] = dt / (constant::EPSILON_0 * d3[a]);
       td.cb[a] = dt / (constant::MU_0 * d3[a]);
     }
-    td.source = source.data();
     td.total_steps = (int64_t)total_time_steps;
-    check(xf_engine_set_tfsf(_engine, &td));
+    tp.present = true;
   }
 
   // ---- NF-FF (nffft/nffft.cpp:22-34, nffft_fd.cpp:17-106)
@@ -618,7 +649,8 @@ void Simulation::init(size_t total_time_steps) {
     const auto dist = nf->getDistance();
     for (int a = 0; a < 3; ++a)
       if (dist[a] < 1 || 2 * dist[a] >= (size_t)N[a]) throw std::runtime_error("NF-FF box does not fit the grid");
-    xf_nffft_desc nd;
+    PlanData::Nf &np_ = P.nf;
+    xf_nffft_desc &nd = np_.desc;
     nd.li = (int64_t)dist[0]; nd.ri = nx - (int64_t)dist[0];
     nd.lj = (int64_t)dist[1]; nd.rj = ny - (int64_t)dist[1];
     nd.lk = (int64_t)dist[2]; nd.rk = nz - (int64_t)dist[2];
@@ -626,7 +658,9 @@ void Simulation::init(size_t total_time_steps) {
     nd.nf = (int)nf->frequencies().size();
     nd.total_steps = (int64_t)total_time_steps;
     const size_t T = total_time_steps;
-    std::vector<std::complex<double>> tj((size_t)nd.nf * T), tm((size_t)nd.nf * T);
+    std::vector<std::complex<double>> &tj = np_.tj, &tm = np_.tm;
+    tj.resize((size_t)nd.nf * T);
+    tm.resize((size_t)nd.nf * T);
     using namespace std::complex_literals;
     for (int n = 0; n < nd.nf; ++n)
       for (size_t t = 0; t < T; ++t) {
@@ -634,13 +668,84 @@ void Simulation::init(size_t total_time_steps) {
         tj[n * T + t] = dt * std::exp(-1i * 2.0 * constant::PI * (f * (t + 0.5) * dt));
         tm[n * T + t] = dt * std::exp(-1i * 2.0 * constant::PI * (f * t * dt));
       }
-    nd.tj = reinterpret_cast<const double *>(tj.data());
-    nd.tm = reinterpret_cast<const double *>(tm.data());
-    check(xf_engine_set_nffft(_engine, &nd));
+    np_.present = true;
   }
 
+}
+
+void Simulation::upload() {
+  PlanData &P = *_plan;
+  xf_grid_desc gd;
+  gd.nx = (int64_t)_n[0]; gd.ny = (int64_t)_n[1]; gd.nz = (int64_t)_n[2];
+  gd.precision = _precision;
+  gd.n_materials = P.nmat;
+  check(xf_engine_create(&gd, &_engine));
+  const double *cbp[12], *cpp[12];
+  for (int t = 0; t < 12; ++t) { cbp[t] = P.cb[t].data(); cpp[t] = P.cpsi[t].data(); }
+  for (int a = 0; a < 3; ++a)
+    if (P.dn[a] + P.dp[a] > 0)
+      check(xf_engine_set_cpml(_engine, a, P.dn[a], P.dp[a], P.be[a].data(), P.ae[a].data(), P.bm[a].data(),
+                               P.am[a].data()));
+  check(xf_engine_set_materials(_engine, P.midx.data(), P.ca.data(), cbp, cpp));
+  for (const auto &s : P.srcs)
+    check(xf_engine_add_current_source(_engine, s.comp, s.is, s.ie, s.js, s.je, s.ks, s.ke, s.coff.data(),
+                                       s.wave.data(), (int64_t)_total_time_steps));
+  if (P.tfsf.present) {
+    xf_tfsf_desc td = P.tfsf.desc;
+    for (int a = 0; a < 3; ++a) {
+      td.proj_full[a] = P.tfsf.full[a].data();
+      td.proj_half[a] = P.tfsf.half[a].data();
+    }
+    td.source = P.tfsf.source.data();
+    check(xf_engine_set_tfsf(_engine, &td));
+  }
+  if (P.nf.present) {
+    xf_nffft_desc nd = P.nf.desc;
+    nd.tj = reinterpret_cast<const double *>(P.nf.tj.data());
+    nd.tm = reinterpret_cast<const double *>(P.nf.tm.data());
+    check(xf_engine_set_nffft(_engine, &nd));
+  }
   for (const Probe &p : _probes)
     check(xf_engine_add_probe(_engine, compIndex(p.component), (int64_t)p.i, (int64_t)p.j, (int64_t)p.k));
+}
+
+void Simulation::init(size_t total_time_steps) {
+  if (_engine) throw std::runtime_error("Simulation already initialised");
+  if (!_plan) plan(total_time_steps);
+  else if (_total_time_steps != total_time_steps) throw std::runtime_error("plan made for another step count");
+  upload();
+}
+
+// per-cell value of coefficient array `which` (cexe, cexhy, ..., chzey order of
+// fdtd_basic_coff.h) implied by the plan, in the reference's array shape
+std::vector<double> Simulation::planCoefficient(int which) const {
+  if (!_plan) throw std::runtime_error("no plan");
+  const PlanData &P = *_plan;
+  static const int table_of[18] = {-1, XF_CB_EXHY, XF_CB_EXHZ, -1, XF_CB_EYHZ, XF_CB_EYHX,
+                                   -1, XF_CB_EZHX, XF_CB_EZHY, -1, XF_CB_HXEY, XF_CB_HXEZ,
+                                   -1, XF_CB_HYEZ, XF_CB_HYEX, -1, XF_CB_HZEX, XF_CB_HZEY};
+  static const int axis_of_table[12] = {2, 1, 0, 2, 1, 0, 2, 1, 0, 2, 1, 0};
+  if (which < 0 || which >= 18) throw std::invalid_argument("coefficient index 0..17");
+  const int comp = which / 3;
+  const auto sh = fieldShape(static_cast<EMComponent>(comp));
+  const size_t nz1 = _n[2] + 1, ny1 = _n[1] + 1;
+  std::vector<double> out(sh[0] * sh[1] * sh[2]);
+  const int t = table_of[which];
+  for (size_t i = 0; i < sh[0]; ++i)
+    for (size_t j = 0; j < sh[1]; ++j)
+      for (size_t k = 0; k < sh[2]; ++k) {
+        const int m = P.midx[(i * ny1 + j) * nz1 + k];
+        double v;
+        if (t < 0) {
+          v = P.ca[comp * P.nmat + m];
+        } else {
+          const int a = axis_of_table[t];
+          const size_t n = a == 0 ? i : (a == 1 ? j : k);
+          v = P.cb[t][(size_t)m * (_n[a] + 1) + n];
+        }
+        out[(i * sh[1] + j) * sh[2] + k] = v;
+      }
+  return out;
 }
 
 void Simulation::step(size_t n) {

@@ -113,6 +113,17 @@ extern "C" int xf_sim_add_probe(xf_sim *s, int comp, size_t i, size_t j, size_t 
   });
 }
 
+extern "C" int xf_sim_plan(xf_sim *s, size_t total_steps) {
+  return guarded([&] { s->sim.plan(total_steps); });
+}
+
+extern "C" int xf_sim_plan_coeff(xf_sim *s, int which, double *out) {
+  return guarded([&] {
+    auto v = s->sim.planCoefficient(which);
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+  });
+}
+
 extern "C" int xf_sim_init(xf_sim *s, size_t total_steps, int precision) {
   return guarded([&] {
     if (precision != 32 && precision != 64) throw std::invalid_argument("precision must be 32 or 64");

@@ -25,7 +25,7 @@ def _f64(v):
 
 
 class Simulation:
-    def __init__(self, scene, total_steps, precision=64, output_dir=None):
+    def __init__(self, scene, total_steps, precision=64, output_dir=None, plan_only=False):
         L = lib()
         self.scene = scene
         self.T = int(total_steps)
@@ -68,7 +68,10 @@ class Simulation:
                                                       float(s.resistance), w.amplitude, w.tau, w.t_0))
             for p in scene.probes:
                 self._chk(L.xf_sim_add_probe(self.h, int(p.comp), p.i, p.j, p.k))
-            self._chk(L.xf_sim_init(self.h, self.T, self.precision))
+            if plan_only:
+                self._chk(L.xf_sim_plan(self.h, self.T))
+            else:
+                self._chk(L.xf_sim_init(self.h, self.T, self.precision))
         except Exception:
             L.xf_sim_destroy(self.h)
             self.h = None
@@ -105,6 +108,12 @@ class Simulation:
         nx, ny, nz = self.dims
         return [(nx, ny + 1, nz + 1), (nx + 1, ny, nz + 1), (nx + 1, ny + 1, nz),
                 (nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)][comp]
+
+    def plan_coeff(self, which):
+        """Per-cell coefficient array `which` implied by the CPU plan."""
+        out = np.empty(self.field_shape(which // 3), dtype=np.float64)
+        self._chk(lib().xf_sim_plan_coeff(self.h, int(which), _dp(out)))
+        return out
 
     def step(self, n):
         self._chk(lib().xf_sim_step(self.h, int(n)))
