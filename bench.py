@@ -77,6 +77,22 @@ def run_cpu_baseline(sc, total, steps):
     }
 
 
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of known size: its FETCH_SIZE/WRITE_SIZE
+    counters calibrate the counter readings of the field kernels."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes": nbytes, "copy_ms": dt * 1e3,
+                      "copy_GBps_read_plus_write": 2 * nbytes / dt / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,7 +103,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tiling", type=str, default="")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="time a plain 1 GiB device copy (reference for HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is described in DESIGN.md, not built.", file=sys.stderr)
@@ -99,8 +120,7 @@ def main():
         total = args.warmup + args.steps
     sim = Simulation(sc, total, precision=args.precision)
     if args.tiling:
-        tz, ty, tx = (int(v) for v in args.tiling.split(","))
-        sim.set_tiling(tz, ty, tx)
+        sim.set_tiling(*(int(v) for v in args.tiling.split(",")))
     nx, ny, nz = sim.dims
     cells = nx * ny * nz
     sim.step(args.warmup)

@@ -170,9 +170,14 @@ xf_status xf_engine_set_field(xf_engine *e, int comp, const double *in);
 xf_status xf_engine_kernel_times(xf_engine *e, double *ms_e, double *ms_h,
                                  int64_t *samples);
 xf_status xf_engine_set_timing(xf_engine *e, int enabled);
-/* launch geometry of the curl kernels: z threads per block (multiple of 64),
-   y rows per block, x cells per block (0 = whole axis) */
-xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx);
+/* launch geometry of the curl kernels: tz threads along z per block
+   (multiple of 64), ty rows along y (0 = flat runs of tz cells over the
+   padded (y,z) plane), tx x-planes marched per thread (0 = whole axis), and
+   flags: XF_TILE_PREFETCH double-buffers planes, XF_TILE_GLOBAL_TABLES reads
+   coefficient tables from global memory instead of LDS, XF_TILE_X_FASTEST
+   orders tiles along x first.  Every geometry gives identical results. */
+enum { XF_TILE_PREFETCH = 1, XF_TILE_GLOBAL_TABLES = 2, XF_TILE_X_FASTEST = 4 };
+xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags);
 
 #ifdef __cplusplus
 }

@@ -78,7 +78,7 @@ def lib():
     L.xf_sim_engine.argtypes = [vp]
     L.xf_engine_kernel_times.argtypes = [vp, dp, dp, C.POINTER(C.c_int64)]
     L.xf_engine_set_timing.argtypes = [vp, i]
-    L.xf_engine_set_tiling.argtypes = [vp, i, i, i]
+    L.xf_engine_set_tiling.argtypes = [vp, i, i, i, i]
     L.xf_engine_sync.argtypes = [vp]
     L.xf_engine_step.argtypes = [vp, i64]
     _LIB = L

@@ -86,7 +86,7 @@ struct xf_engine {
   long long cur = 0;
   long long total = -1;  // smallest total_steps among the attached tables
   // tiling
-  int tz = 64, ty = 4, tx = 8;
+  int tz = 64, ty = 4, tx = 16, tile_flags = 0;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 4 events per timed step: E start/end, H start/end
@@ -158,6 +158,9 @@ extern "C" xf_status xf_engine_create(const xf_grid_desc *g, xf_engine **out) {
   e->sj = e->pz;
   e->si = (long long)(e->ny + 1) * e->pz;
   e->ncell = (long long)(e->nx + 1) * e->si;
+  // the curl kernels address a field with a 32-bit byte offset
+  if ((unsigned long long)e->ncell * e->esz >= (1ULL << 32))
+    FAILS(XF_EINVAL, "grid of %lld padded cells exceeds 4 GiB per field component", e->ncell);
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
     delete e;
     FAILS(XF_EDEVICE, "hipStreamCreate failed");
@@ -494,15 +497,28 @@ static xf_status run_steps(xf_engine *e, long long n) {
   tl.tz = e->tz;
   tl.ty = e->ty;
   tl.tx = e->tx > 0 ? e->tx : e->nx + 1;
-  tl.ntz = (e->nz + 1 + tl.tz - 1) / tl.tz;
-  tl.nty = (e->ny + 1 + tl.ty - 1) / tl.ty;
+  if (tl.ty == 0) {
+    // flat runs of tz cells over the padded (j,k) plane
+    const long long plane = (long long)(e->ny + 1) * e->pz;
+    tl.ntz = (int)((plane + tl.tz - 1) / tl.tz);
+    tl.nty = 1;
+    tl.krows = tl.tz;
+    tl.jrows = (tl.tz + e->pz - 1) / e->pz + 1;
+  } else {
+    tl.ntz = (e->nz + 1 + tl.tz - 1) / tl.tz;
+    tl.nty = (e->ny + 1 + tl.ty - 1) / tl.ty;
+    tl.krows = tl.tz;
+    tl.jrows = tl.ty;
+  }
   tl.ntx = (e->nx + 1 + tl.tx - 1) / tl.tx;
   tl.nblocks = tl.ntz * tl.nty * tl.ntx;
-  dim3 blk(tl.tz * tl.ty);
+  dim3 blk(tl.ty == 0 ? tl.tz : tl.tz * tl.ty);
   dim3 grd(tl.nblocks);
   // coefficient tables of a tile go to LDS when they fit in 32 KB
   const size_t lds = lds_table_bytes<T>(e->nmat, tl);
-  const bool use_lds = lds <= 32768;
+  const bool use_lds = lds <= 32768 && !(e->tile_flags & XF_TILE_GLOBAL_TABLES);
+  const bool prefetch = (e->tile_flags & XF_TILE_PREFETCH) != 0;
+  tl.x_fastest = (e->tile_flags & XF_TILE_X_FASTEST) != 0;
   for (long long q = 0; q < n; ++q) {
     const long long step = e->cur;
     hipEvent_t *ev = nullptr;
@@ -518,10 +534,12 @@ static xf_status run_steps(xf_engine *e, long long n) {
       e->ev_used += 4;
     }
     if (ev) hipEventRecord(ev[0], e->stream);
-    if (use_lds)
-      hipLaunchKernelGGL((k_update_e<T, true>), grd, blk, lds, e->stream, g, tl);
+    if (use_lds && prefetch)
+      hipLaunchKernelGGL((k_update_e<T, true, true>), grd, blk, lds, e->stream, g, tl);
+    else if (use_lds)
+      hipLaunchKernelGGL((k_update_e<T, true, false>), grd, blk, lds, e->stream, g, tl);
     else
-      hipLaunchKernelGGL((k_update_e<T, false>), grd, blk, 0, e->stream, g, tl);
+      hipLaunchKernelGGL((k_update_e<T, false, false>), grd, blk, 0, e->stream, g, tl);
     if (ev) hipEventRecord(ev[1], e->stream);
     for (const SourceDev &sd : e->sources) {
       const int nj = sd.j1 - sd.j0, nk = sd.k1 - sd.k0;
@@ -536,10 +554,12 @@ static xf_status run_steps(xf_engine *e, long long n) {
                          e->nEz, e->nEx);
     }
     if (ev) hipEventRecord(ev[2], e->stream);
-    if (use_lds)
-      hipLaunchKernelGGL((k_update_h<T, true>), grd, blk, lds, e->stream, g, tl);
+    if (use_lds && prefetch)
+      hipLaunchKernelGGL((k_update_h<T, true, true>), grd, blk, lds, e->stream, g, tl);
+    else if (use_lds)
+      hipLaunchKernelGGL((k_update_h<T, true, false>), grd, blk, lds, e->stream, g, tl);
     else
-      hipLaunchKernelGGL((k_update_h<T, false>), grd, blk, 0, e->stream, g, tl);
+      hipLaunchKernelGGL((k_update_h<T, false, false>), grd, blk, 0, e->stream, g, tl);
     if (ev) hipEventRecord(ev[3], e->stream);
     if (e->has_tfsf) {
       hipLaunchKernelGGL(k_tfsf_ifa<T>, dim3(1), dim3(1024), 0, e->stream, t, step);
@@ -657,10 +677,12 @@ extern "C" xf_status xf_engine_set_timing(xf_engine *e, int enabled) {
   return XF_OK;
 }
 
-extern "C" xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx) {
+extern "C" xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags) {
   if (!e) FAILS(XF_EINVAL, "null engine");
-  if (tz < 64 || tz % 64 != 0 || ty < 1 || tz * ty > 256 || tx < 0)
-    FAILS(XF_EINVAL, "tiling %d x %d x %d: need tz a multiple of 64, tz*ty <= 256", tz, ty, tx);
+  if (tz < 64 || tz % 64 != 0 || ty < 0 || tx < 0 || (ty > 0 ? tz * ty : tz) > 256 || flags < 0 || flags > 7)
+    FAILS(XF_EINVAL, "tiling %d x %d x %d (flags %d): need tz a multiple of 64 and at most 256 threads", tz,
+          ty, tx, flags);
   e->tz = tz; e->ty = ty; e->tx = tx;
+  e->tile_flags = flags;
   return XF_OK;
 }

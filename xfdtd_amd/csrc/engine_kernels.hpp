@@ -58,9 +58,11 @@ __device__ __forceinline__ int h_layer(int n, int N, int dn, int dp) {
 // so that each XCD works on one contiguous run of tiles, which keeps a tile's
 // y-1 neighbour row in the same XCD's L2.
 struct Tiling {
-  int tz, ty, tx;           // threads along k, j; planes along i
+  int tz, ty, tx;           // threads along k, j (ty = 0: flat); planes along i
   int ntz, nty, ntx;        // tile counts
   int nblocks;
+  int krows, jrows;         // coefficient-table slots of a tile along k and j
+  int x_fastest;            // tile order: x chunks first (else z, y, x)
 };
 
 __device__ __forceinline__ void tile_of_block(const Tiling &t, int &bz, int &by, int &bx) {
@@ -68,31 +70,66 @@ __device__ __forceinline__ void tile_of_block(const Tiling &t, int &bz, int &by,
   const int q = t.nblocks >> 3, r = t.nblocks & 7;
   const int xcd = b & 7, local = b >> 3;
   const int id = xcd * q + min(xcd, r) + local;
+  if (t.x_fastest) {
+    bx = id % t.ntx;
+    const int rest = id / t.ntx;
+    bz = rest % t.ntz;
+    by = rest / t.ntz;
+    return;
+  }
   bz = id % t.ntz;
   const int rest = id / t.ntz;
   by = rest % t.nty;
   bx = rest / t.nty;
 }
 
+// Cell of a thread.  Two tile shapes: a TZ x TY block of (k, j), or (TY = 0)
+// a flat run of TZ consecutive cells of the padded (j, k) plane, which keeps
+// each wave on one contiguous 256-byte (FP32) piece of every field and lets
+// a workgroup cover 1 KB of a plane row.  ks / js are the thread's slots in
+// the tile's staged k and j coefficient tables.
+struct Cell {
+  int j, k, ks, js, j_first;
+};
+
+__device__ __forceinline__ Cell thread_cell(const Tiling &tl, int bz, int by, int pz) {
+  Cell c;
+  if (tl.ty == 0) {
+    const int p = bz * tl.tz + (int)threadIdx.x;
+    c.j = p / pz;
+    c.k = p - c.j * pz;
+    c.ks = (int)threadIdx.x;
+    c.j_first = (bz * tl.tz) / pz;
+    c.js = c.j - c.j_first;
+  } else {
+    c.ks = (int)(threadIdx.x % tl.tz);
+    c.js = (int)(threadIdx.x / tl.tz);
+    c.k = bz * tl.tz + c.ks;
+    c.j_first = by * tl.ty;
+    c.j = c.j_first + c.js;
+  }
+  return c;
+}
+
 // ---------------------------------------------------------------------------
 // Coefficient tables of one tile, staged in LDS when they fit.  The curl
 // coefficients of a cell are functions of its material and of one node index
-// (fdtd_basic_coff.cpp:93-121), so a tile of TZ x TY threads marching over TX
-// planes needs, per material: 3 self terms, 2 tables over its k range, 2 over
-// its j range and 2 over its i range.
+// (fdtd_basic_coff.cpp:93-121), so a tile marching over TX planes needs, per
+// material: 3 self terms, 2 tables over its k slots, 2 over its j rows and 2
+// over its i planes.
 template <typename T>
 struct TileTables {
   const T *ca;    // [3][nmat]
-  const T *tk0, *tk1;  // [nmat][TZ]
-  const T *tj0, *tj1;  // [nmat][TY]
+  const T *tk0, *tk1;  // [nmat][krows]
+  const T *tj0, *tj1;  // [nmat][jrows]
   const T *ti0, *ti1;  // [nmat][TX]
-  int sk, sj, si;      // row stride (entries per material) of the k/j/i tables
-  int k0, j0, i0;      // tile origin subtracted from global indices
+  int sk, sj, si;      // entries per material of the k/j/i tables
+  int ks, js, i0;      // this thread's k and j slots, tile's first plane
 };
 
 template <typename T>
 __host__ __device__ inline size_t lds_table_bytes(int nmat, const Tiling &tl) {
-  return sizeof(T) * (size_t)nmat * (3 + 2 * tl.tz + 2 * tl.ty + 2 * tl.tx);
+  return sizeof(T) * (size_t)nmat * (3 + 2 * tl.krows + 2 * tl.jrows + 2 * tl.tx);
 }
 
 // stage the tables of one tile (cooperative; ends with a barrier).
@@ -103,25 +140,28 @@ __device__ __forceinline__ TileTables<T> stage_tables(T *lds, const T *ca, int c
                                                      const T *gk0, const T *gk1, int nk1,
                                                      const T *gj0, const T *gj1, int nj1,
                                                      const T *gi0, const T *gi1, int ni1,
-                                                     const Tiling &tl, int k0, int j0, int i0) {
+                                                     const Tiling &tl, const Cell &cell, int i0) {
   TileTables<T> tt;
   T *p = lds;
   T *sca = p; p += 3 * nmat;
-  T *sk0 = p; p += nmat * tl.tz;
-  T *sk1 = p; p += nmat * tl.tz;
-  T *sj0 = p; p += nmat * tl.ty;
-  T *sj1 = p; p += nmat * tl.ty;
+  T *sk0 = p; p += nmat * tl.krows;
+  T *sk1 = p; p += nmat * tl.krows;
+  T *sj0 = p; p += nmat * tl.jrows;
+  T *sj1 = p; p += nmat * tl.jrows;
   T *si0 = p; p += nmat * tl.tx;
   T *si1 = p;
   const int nth = blockDim.x;
   for (int q = threadIdx.x; q < 3 * nmat; q += nth) sca[q] = ca[(ca_base + q / nmat) * nmat + q % nmat];
-  for (int q = threadIdx.x; q < nmat * tl.tz; q += nth) {
-    const int m = q / tl.tz, n = min(k0 + q % tl.tz, nk1 - 1);
-    sk0[q] = gk0[m * nk1 + n];
-    sk1[q] = gk1[m * nk1 + n];
+  if (cell.js == 0 || tl.ty == 0) {
+    // each k slot is written by the thread that owns it (row 0 of a 2-D tile)
+    const int n = min(cell.k, nk1 - 1);
+    for (int m = 0; m < nmat; ++m) {
+      sk0[m * tl.krows + cell.ks] = gk0[m * nk1 + n];
+      sk1[m * tl.krows + cell.ks] = gk1[m * nk1 + n];
+    }
   }
-  for (int q = threadIdx.x; q < nmat * tl.ty; q += nth) {
-    const int m = q / tl.ty, n = min(j0 + q % tl.ty, nj1 - 1);
+  for (int q = threadIdx.x; q < nmat * tl.jrows; q += nth) {
+    const int m = q / tl.jrows, n = min(cell.j_first + q % tl.jrows, nj1 - 1);
     sj0[q] = gj0[m * nj1 + n];
     sj1[q] = gj1[m * nj1 + n];
   }
@@ -133,8 +173,8 @@ __device__ __forceinline__ TileTables<T> stage_tables(T *lds, const T *ca, int c
   __syncthreads();
   tt.ca = sca;
   tt.tk0 = sk0; tt.tk1 = sk1; tt.tj0 = sj0; tt.tj1 = sj1; tt.ti0 = si0; tt.ti1 = si1;
-  tt.sk = tl.tz; tt.sj = tl.ty; tt.si = tl.tx;
-  tt.k0 = k0; tt.j0 = j0; tt.i0 = i0;
+  tt.sk = tl.krows; tt.sj = tl.jrows; tt.si = tl.tx;
+  tt.ks = cell.ks; tt.js = cell.js; tt.i0 = i0;
   return tt;
 }
 
@@ -143,28 +183,166 @@ template <typename T>
 __device__ __forceinline__ TileTables<T> global_tables(const T *ca, int ca_base, int nmat,
                                                       const T *gk0, const T *gk1, int nk1,
                                                       const T *gj0, const T *gj1, int nj1,
-                                                      const T *gi0, const T *gi1, int ni1) {
+                                                      const T *gi0, const T *gi1, int ni1,
+                                                      const Cell &cell) {
   TileTables<T> tt;
   tt.ca = ca + ca_base * nmat;
   tt.tk0 = gk0; tt.tk1 = gk1; tt.tj0 = gj0; tt.tj1 = gj1; tt.ti0 = gi0; tt.ti1 = gi1;
   tt.sk = nk1; tt.sj = nj1; tt.si = ni1;
-  tt.k0 = 0; tt.j0 = 0; tt.i0 = 0;
+  tt.ks = min(cell.k, nk1 - 1); tt.js = min(cell.j, nj1 - 1); tt.i0 = 0;
   return tt;
 }
+
+// Field loads and stores by 32-bit byte offset from a wave-uniform base: all
+// six components and the material map share one offset per cell, which
+// lets the compiler use the SGPR-base + VGPR-offset addressing form.
+template <typename T>
+__device__ __forceinline__ T ldo(const T *base, unsigned off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ void sto(T *base, unsigned off, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
+__device__ __forceinline__ unsigned ldm(const uint8_t *base, unsigned cell) { return base[cell]; }
 
 // ---------------------------------------------------------------------------
 // E half-step: Simulation::updateE (simulation_update.cpp:170-196) fused with
 // PML::updateE of every face (perfect_match_layer.cpp:574-718), faces in x,
-// y, z order.  One thread owns a (j,k) column and marches over TX x-planes;
-// the loads of plane i+1 are issued before plane i is computed.
-template <typename T, bool LDS>
+// y, z order.  One thread owns a (j,k) column and marches over TX x-planes.
+// Two plane buffers (A, B) alternate so that the loads of plane i+1 are in
+// flight while plane i is computed, without register copies between them.
+template <typename T>
+struct PlaneE {
+  unsigned m;
+  T ex, ey, ez, hx, hy, hz, hyk, hxk, hzj, hxj;
+};
+
+template <typename T>
+struct ColE {
+  T *__restrict__ Ex;
+  T *__restrict__ Ey;
+  T *__restrict__ Ez;
+  const T *__restrict__ Hx;
+  const T *__restrict__ Hy;
+  const T *__restrict__ Hz;
+  const uint8_t *__restrict__ mat;
+  long long sj;
+  unsigned bsj;  // byte stride of j
+  bool k1, j1;   // k >= 1, j >= 1
+};
+
+template <typename T>
+__device__ __forceinline__ void load_plane_e(const ColE<T> &q, long long c, PlaneE<T> &p) {
+  const unsigned cell = (unsigned)c, o = cell * (unsigned)sizeof(T);
+  p.m = ldm(q.mat, cell);
+  p.ex = ldo(q.Ex, o);
+  p.ey = ldo(q.Ey, o);
+  p.ez = ldo(q.Ez, o);
+  p.hx = ldo(q.Hx, o);
+  p.hy = ldo(q.Hy, o);
+  p.hz = ldo(q.Hz, o);
+  p.hyk = q.k1 ? ldo(q.Hy, o - (unsigned)sizeof(T)) : T(0);
+  p.hxk = q.k1 ? ldo(q.Hx, o - (unsigned)sizeof(T)) : T(0);
+  p.hzj = q.j1 ? ldo(q.Hz, o - q.bsj) : T(0);
+  p.hxj = q.j1 ? ldo(q.Hx, o - q.bsj) : T(0);
+}
+
+template <typename T>
+__device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTables<T> &tt, const ColE<T> &q,
+                                                const PlaneE<T> &p, int i, int j, int k, long long c,
+                                                int ly, int lz, bool jin, bool kin, T hz_prev,
+                                                T hy_prev) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
+  const int nm1y = ny + 1, nm1z = nz + 1, nm1x = nx + 1;
+  const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2];
+  const int m = (int)p.m;
+  const int lx = e_layer(i, nx, g.dn[0], g.dp[0]);
+  const T dhy_k = p.hy - p.hyk;     // Hy(k) - Hy(k-1)
+  const T dhz_j = p.hz - p.hzj;     // Hz(j) - Hz(j-1)
+  const T dhx_k = p.hx - p.hxk;     // Hx(k) - Hx(k-1)
+  const T dhz_i = p.hz - hz_prev;   // Hz(i) - Hz(i-1)
+  const T dhx_j = p.hx - p.hxj;     // Hx(j) - Hx(j-1)
+  const T dhy_i = p.hy - hy_prev;   // Hy(i) - Hy(i-1)
+  const int mk = m * tt.sk + tt.ks, mj = m * tt.sj + tt.js, mi = m * tt.si + (i - tt.i0);
+  const unsigned o = (unsigned)c * (unsigned)sizeof(T);
+  // Yee terms for all three components, evaluated unconditionally so the
+  // plane's loads are consumed on every path (one wait per plane)
+  const T yx = tt.ca[m] * p.ex + tt.tk0[mk] * dhy_k + tt.tj0[mj] * dhz_j;
+  const T yy = tt.ca[nmat + m] * p.ey + tt.tk1[mk] * dhx_k + tt.ti0[mi] * dhz_i;
+  const T yz = tt.ca[2 * nmat + m] * p.ez + tt.tj1[mj] * dhx_j + tt.ti1[mi] * dhy_i;
+  const bool ix = i >= 1 && i < nx;
+  // Ex: shape (nx, ny+1, nz+1); Yee j in [1,ny), k in [1,nz)
+  {
+    const bool yee = (i < nx) && jin && kin;
+    T v = yee ? yx : p.ex;
+    if (i < nx && (ly >= 0 || lz >= 0)) {
+      if (ly >= 0) {
+        T *ps_ = &g.psi_e[P_AY][((long long)i * Ly + ly) * g.pz + k];
+        const T ps = g.be[1][j] * (*ps_) + g.ae[1][j] * dhz_j;
+        *ps_ = ps;
+        v = v + g.cpsi[CB_EXHZ][m * nm1y + j] * ps;
+      }
+      if (lz >= 0) {
+        T *ps_ = &g.psi_e[P_AZ][((long long)i * nm1y + j) * Lz + lz];
+        const T ps = g.be[2][k] * (*ps_) + g.ae[2][k] * dhy_k;
+        *ps_ = ps;
+        v = v + g.cpsi[CB_EXHY][m * nm1z + k] * ps;
+      }
+    }
+    if (yee || (i < nx && (ly >= 0 || lz >= 0))) sto(q.Ex, o, v);
+  }
+  // Ey: shape (nx+1, ny, nz+1); Yee k in [1,nz), i in [1,nx)
+  {
+    const bool yee = ix && kin && (j < ny);
+    T v = yee ? yy : p.ey;
+    const bool slab = (j < ny) && (lx >= 0 || lz >= 0);
+    if (slab) {
+      if (lx >= 0) {
+        T *ps_ = &g.psi_e[P_BX][((long long)lx * nm1y + j) * g.pz + k];
+        const T ps = g.be[0][i] * (*ps_) + g.ae[0][i] * dhz_i;
+        *ps_ = ps;
+        v = v + g.cpsi[CB_EYHZ][m * nm1x + i] * ps;
+      }
+      if (lz >= 0) {
+        T *ps_ = &g.psi_e[P_BZ][((long long)i * nm1y + j) * Lz + lz];
+        const T ps = g.be[2][k] * (*ps_) + g.ae[2][k] * dhx_k;
+        *ps_ = ps;
+        v = v + g.cpsi[CB_EYHX][m * nm1z + k] * ps;
+      }
+    }
+    if (yee || slab) sto(q.Ey, o, v);
+  }
+  // Ez: shape (nx+1, ny+1, nz); Yee i in [1,nx), j in [1,ny)
+  {
+    const bool yee = ix && jin && (k < nz);
+    T v = yee ? yz : p.ez;
+    const bool slab = (k < nz) && (lx >= 0 || ly >= 0);
+    if (slab) {
+      if (lx >= 0) {
+        T *ps_ = &g.psi_e[P_CX][((long long)lx * nm1y + j) * g.pz + k];
+        const T ps = g.be[0][i] * (*ps_) + g.ae[0][i] * dhy_i;
+        *ps_ = ps;
+        v = v + g.cpsi[CB_EZHY][m * nm1x + i] * ps;
+      }
+      if (ly >= 0) {
+        T *ps_ = &g.psi_e[P_CY][((long long)i * Ly + ly) * g.pz + k];
+        const T ps = g.be[1][j] * (*ps_) + g.ae[1][j] * dhx_j;
+        *ps_ = ps;
+        v = v + g.cpsi[CB_EZHX][m * nm1y + j] * ps;
+      }
+    }
+    if (yee || slab) sto(q.Ez, o, v);
+  }
+}
+
+template <typename T, bool LDS, bool PF>
 __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl) {
   extern __shared__ unsigned char lds_raw[];
   int bz, by, bx;
   tile_of_block(tl, bz, by, bx);
-  const int kl = (int)(threadIdx.x % tl.tz), jl = (int)(threadIdx.x / tl.tz);
-  const int k = bz * tl.tz + kl;
-  const int j = by * tl.ty + jl;
+  const Cell cell = thread_cell(tl, bz, by, g.pz);
+  const int k = cell.k, j = cell.j;
   const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
   const int i0 = bx * tl.tx;
   const int i1 = min(i0 + tl.tx, nx + 1);
@@ -172,136 +350,190 @@ __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl) {
   if (LDS)
     tt = stage_tables<T>(reinterpret_cast<T *>(lds_raw), g.ca, EX, nmat, g.cb[CB_EXHY], g.cb[CB_EYHX],
                          nz + 1, g.cb[CB_EXHZ], g.cb[CB_EZHX], ny + 1, g.cb[CB_EYHZ], g.cb[CB_EZHY],
-                         nx + 1, tl, bz * tl.tz, by * tl.ty, i0);
+                         nx + 1, tl, cell, i0);
   else
     tt = global_tables<T>(g.ca, EX, nmat, g.cb[CB_EXHY], g.cb[CB_EYHX], nz + 1, g.cb[CB_EXHZ],
-                          g.cb[CB_EZHX], ny + 1, g.cb[CB_EYHZ], g.cb[CB_EZHY], nx + 1);
+                          g.cb[CB_EZHX], ny + 1, g.cb[CB_EYHZ], g.cb[CB_EZHY], nx + 1, cell);
   if (k > nz || j > ny) return;
-  const int nm1y = ny + 1, nm1z = nz + 1, nm1x = nx + 1;
-  T *__restrict__ Ex = g.f[EX];
-  T *__restrict__ Ey = g.f[EY];
-  T *__restrict__ Ez = g.f[EZ];
-  const T *__restrict__ Hx = g.f[HX];
-  const T *__restrict__ Hy = g.f[HY];
-  const T *__restrict__ Hz = g.f[HZ];
-  const uint8_t *__restrict__ mat = g.mat;
-  const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2];
+  ColE<T> q;
+  q.Ex = g.f[EX]; q.Ey = g.f[EY]; q.Ez = g.f[EZ];
+  q.Hx = g.f[HX]; q.Hy = g.f[HY]; q.Hz = g.f[HZ];
+  q.mat = g.mat;
+  q.sj = g.sj;
+  q.bsj = (unsigned)(g.sj * (long long)sizeof(T));
+  q.k1 = k >= 1;
+  q.j1 = j >= 1;
   const int ly = e_layer(j, ny, g.dn[1], g.dp[1]);
   const int lz = e_layer(k, nz, g.dn[2], g.dp[2]);
   const bool kin = (k >= 1 && k < nz), jin = (j >= 1 && j < ny);
-  const long long cj = (long long)j * g.sj + k;
-  // plane registers: current (x_c) and next (x_n)
-  long long c = (long long)i0 * g.si + cj;
-  T hz_prev = (i0 >= 1) ? Hz[c - g.si] : T(0);
-  T hy_prev = (i0 >= 1) ? Hy[c - g.si] : T(0);
-  int m_n = mat[c];
-  T ex_n = Ex[c], ey_n = Ey[c], ez_n = Ez[c];
-  T hx_n = Hx[c], hy_n = Hy[c], hz_n = Hz[c];
-  T hyk_n = (k >= 1) ? Hy[c - 1] : T(0);
-  T hxk_n = (k >= 1) ? Hx[c - 1] : T(0);
-  T hzj_n = (j >= 1) ? Hz[c - g.sj] : T(0);
-  T hxj_n = (j >= 1) ? Hx[c - g.sj] : T(0);
-  for (int i = i0; i < i1; ++i, c += g.si) {
-    const int m = m_n;
-    const T ex = ex_n, ey = ey_n, ez = ez_n, hx = hx_n, hy = hy_n, hz = hz_n;
-    const T hyk = hyk_n, hxk = hxk_n, hzj = hzj_n, hxj = hxj_n;
-    if (i + 1 < i1) {
-      const long long cn = c + g.si;
-      m_n = mat[cn];
-      ex_n = Ex[cn]; ey_n = Ey[cn]; ez_n = Ez[cn];
-      hx_n = Hx[cn]; hy_n = Hy[cn]; hz_n = Hz[cn];
-      hyk_n = (k >= 1) ? Hy[cn - 1] : T(0);
-      hxk_n = (k >= 1) ? Hx[cn - 1] : T(0);
-      hzj_n = (j >= 1) ? Hz[cn - g.sj] : T(0);
-      hxj_n = (j >= 1) ? Hx[cn - g.sj] : T(0);
+  long long c = (long long)i0 * g.si + (long long)j * g.sj + k;
+  // x-neighbours are carried in registers: Hz(i-1), Hy(i-1)
+  T hz_prev = (i0 >= 1) ? ldo(q.Hz, (unsigned)(c - g.si) * (unsigned)sizeof(T)) : T(0);
+  T hy_prev = (i0 >= 1) ? ldo(q.Hy, (unsigned)(c - g.si) * (unsigned)sizeof(T)) : T(0);
+  if (!PF) {
+    // one plane at a time: its loads are issued together, latency is hidden
+    // by the other waves on the CU
+    for (int i = i0; i < i1; ++i, c += g.si) {
+      PlaneE<T> A;
+      load_plane_e(q, c, A);
+      compute_plane_e(g, tt, q, A, i, j, k, c, ly, lz, jin, kin, hz_prev, hy_prev);
+      hz_prev = A.hz;
+      hy_prev = A.hy;
     }
-    const int lx = e_layer(i, nx, g.dn[0], g.dp[0]);
-    const T dhy_k = hy - hyk;      // Hy(k) - Hy(k-1)
-    const T dhz_j = hz - hzj;      // Hz(j) - Hz(j-1)
-    const T dhx_k = hx - hxk;      // Hx(k) - Hx(k-1)
-    const T dhz_i = hz - hz_prev;  // Hz(i) - Hz(i-1)
-    const T dhx_j = hx - hxj;      // Hx(j) - Hx(j-1)
-    const T dhy_i = hy - hy_prev;  // Hy(i) - Hy(i-1)
-    const int mk = m * tt.sk + (k - tt.k0), mj = m * tt.sj + (j - tt.j0), mi = m * tt.si + (i - tt.i0);
-    // Ex: shape (nx, ny+1, nz+1); Yee j in [1,ny), k in [1,nz)
-    if (i < nx) {
-      const bool yee = jin && kin;
-      if (yee || ly >= 0 || lz >= 0) {
-        T v = ex;
-        if (yee) v = tt.ca[m] * v + tt.tk0[mk] * dhy_k + tt.tj0[mj] * dhz_j;
-        if (ly >= 0) {
-          T *p = &g.psi_e[P_AY][((long long)i * Ly + ly) * g.pz + k];
-          const T ps = g.be[1][j] * (*p) + g.ae[1][j] * dhz_j;
-          *p = ps;
-          v = v + g.cpsi[CB_EXHZ][m * nm1y + j] * ps;
-        }
-        if (lz >= 0) {
-          T *p = &g.psi_e[P_AZ][((long long)i * nm1y + j) * Lz + lz];
-          const T ps = g.be[2][k] * (*p) + g.ae[2][k] * dhy_k;
-          *p = ps;
-          v = v + g.cpsi[CB_EXHY][m * nm1z + k] * ps;
-        }
-        Ex[c] = v;
-      }
-    }
-    // Ey: shape (nx+1, ny, nz+1); Yee k in [1,nz), i in [1,nx)
-    if (j < ny) {
-      const bool yee = (i >= 1 && i < nx) && kin;
-      if (yee || lx >= 0 || lz >= 0) {
-        T v = ey;
-        if (yee) v = tt.ca[nmat + m] * v + tt.tk1[mk] * dhx_k + tt.ti0[mi] * dhz_i;
-        if (lx >= 0) {
-          T *p = &g.psi_e[P_BX][((long long)lx * nm1y + j) * g.pz + k];
-          const T ps = g.be[0][i] * (*p) + g.ae[0][i] * dhz_i;
-          *p = ps;
-          v = v + g.cpsi[CB_EYHZ][m * nm1x + i] * ps;
-        }
-        if (lz >= 0) {
-          T *p = &g.psi_e[P_BZ][((long long)i * nm1y + j) * Lz + lz];
-          const T ps = g.be[2][k] * (*p) + g.ae[2][k] * dhx_k;
-          *p = ps;
-          v = v + g.cpsi[CB_EYHX][m * nm1z + k] * ps;
-        }
-        Ey[c] = v;
-      }
-    }
-    // Ez: shape (nx+1, ny+1, nz); Yee i in [1,nx), j in [1,ny)
-    if (k < nz) {
-      const bool yee = (i >= 1 && i < nx) && jin;
-      if (yee || lx >= 0 || ly >= 0) {
-        T v = ez;
-        if (yee) v = tt.ca[2 * nmat + m] * v + tt.tj1[mj] * dhx_j + tt.ti1[mi] * dhy_i;
-        if (lx >= 0) {
-          T *p = &g.psi_e[P_CX][((long long)lx * nm1y + j) * g.pz + k];
-          const T ps = g.be[0][i] * (*p) + g.ae[0][i] * dhy_i;
-          *p = ps;
-          v = v + g.cpsi[CB_EZHY][m * nm1x + i] * ps;
-        }
-        if (ly >= 0) {
-          T *p = &g.psi_e[P_CY][((long long)i * Ly + ly) * g.pz + k];
-          const T ps = g.be[1][j] * (*p) + g.ae[1][j] * dhx_j;
-          *p = ps;
-          v = v + g.cpsi[CB_EZHX][m * nm1y + j] * ps;
-        }
-        Ez[c] = v;
-      }
-    }
-    hz_prev = hz;
-    hy_prev = hy;
+    return;
+  }
+  PlaneE<T> A, B;
+  load_plane_e(q, c, A);
+  for (int i = i0; i < i1; i += 2, c += 2 * g.si) {
+    if (i + 1 < i1) load_plane_e(q, c + g.si, B);
+    compute_plane_e(g, tt, q, A, i, j, k, c, ly, lz, jin, kin, hz_prev, hy_prev);
+    hz_prev = A.hz;
+    hy_prev = A.hy;
+    if (i + 1 >= i1) break;
+    if (i + 2 < i1) load_plane_e(q, c + 2 * g.si, A);
+    compute_plane_e(g, tt, q, B, i + 1, j, k, c + g.si, ly, lz, jin, kin, hz_prev, hy_prev);
+    hz_prev = B.hz;
+    hy_prev = B.hy;
   }
 }
 
 // ---------------------------------------------------------------------------
 // H half-step: Simulation::updateH (simulation_update.cpp:84-100) fused with
-// PML::updateH (perfect_match_layer.cpp:434-572).
-template <typename T, bool LDS>
+// PML::updateH (perfect_match_layer.cpp:434-572).  Same A/B plane buffers;
+// Ez(i), Ey(i) are the (i+1) values loaded for the previous plane.
+template <typename T>
+struct PlaneH {
+  unsigned m;
+  T hx, hy, hz, ex, eyk, exk, ezj, exj, ezi, eyi;
+};
+
+template <typename T>
+struct ColH {
+  const T *__restrict__ Ex;
+  const T *__restrict__ Ey;
+  const T *__restrict__ Ez;
+  T *__restrict__ Hx;
+  T *__restrict__ Hy;
+  T *__restrict__ Hz;
+  const uint8_t *__restrict__ mat;
+  long long sj, si;
+  unsigned bsj, bsi;  // byte strides of j and i
+  bool kin, jin;
+  int nx;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_plane_h(const ColH<T> &q, long long c, int i, PlaneH<T> &p) {
+  const unsigned cell = (unsigned)c, o = cell * (unsigned)sizeof(T);
+  p.m = ldm(q.mat, cell);
+  p.hx = ldo(q.Hx, o);
+  p.hy = ldo(q.Hy, o);
+  p.hz = ldo(q.Hz, o);
+  p.ex = ldo(q.Ex, o);
+  p.eyk = q.kin ? ldo(q.Ey, o + (unsigned)sizeof(T)) : T(0);  // Ey(k+1)
+  p.exk = q.kin ? ldo(q.Ex, o + (unsigned)sizeof(T)) : T(0);  // Ex(k+1)
+  p.ezj = q.jin ? ldo(q.Ez, o + q.bsj) : T(0);                // Ez(j+1)
+  p.exj = q.jin ? ldo(q.Ex, o + q.bsj) : T(0);                // Ex(j+1)
+  p.ezi = (i < q.nx) ? ldo(q.Ez, o + q.bsi) : T(0);           // Ez(i+1)
+  p.eyi = (i < q.nx) ? ldo(q.Ey, o + q.bsi) : T(0);           // Ey(i+1)
+}
+
+template <typename T>
+__device__ __forceinline__ void compute_plane_h(const Grid<T> &g, const TileTables<T> &tt, const ColH<T> &q,
+                                                const PlaneH<T> &p, int i, int j, int k, long long c,
+                                                int ly, int lz, T ez_c, T ey_c) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
+  const int nm1y = ny + 1, nm1z = nz + 1, nm1x = nx + 1;
+  const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2];
+  const bool kin = q.kin, jin = q.jin;
+  const int m = (int)p.m;
+  const int lx = h_layer(i, nx, g.dn[0], g.dp[0]);
+  const T dey_k = p.eyk - ey_c;  // Ey(k+1) - Ey(k)
+  const T dez_j = p.ezj - ez_c;  // Ez(j+1) - Ez(j)
+  const T dez_i = p.ezi - ez_c;  // Ez(i+1) - Ez(i)
+  const T dex_k = p.exk - p.ex;  // Ex(k+1) - Ex(k)
+  const T dex_j = p.exj - p.ex;  // Ex(j+1) - Ex(j)
+  const T dey_i = p.eyi - ey_c;  // Ey(i+1) - Ey(i)
+  const int mk = m * tt.sk + tt.ks, mj = m * tt.sj + tt.js, mi = m * tt.si + (i - tt.i0);
+  const unsigned o = (unsigned)c * (unsigned)sizeof(T);
+  const T yx = tt.ca[m] * p.hx + tt.tk0[mk] * dey_k + tt.tj0[mj] * dez_j;
+  const T yy = tt.ca[nmat + m] * p.hy + tt.ti0[mi] * dez_i + tt.tk1[mk] * dex_k;
+  const T yz = tt.ca[2 * nmat + m] * p.hz + tt.tj1[mj] * dex_j + tt.ti1[mi] * dey_i;
+  // Hx: shape (nx+1, ny, nz); Yee i < nx
+  {
+    const bool in = jin && kin;
+    const bool yee = in && (i < nx);
+    T v = yee ? yx : p.hx;
+    const bool slab = in && (ly >= 0 || lz >= 0);
+    if (slab) {
+      if (ly >= 0) {
+        T *ps_ = &g.psi_h[P_AY][((long long)i * Ly + ly) * g.pz + k];
+        const T ps = g.bm[1][j] * (*ps_) + g.am[1][j] * dez_j;
+        *ps_ = ps;
+        v = v + g.cpsi[CB_HXEZ][m * nm1y + j] * ps;
+      }
+      if (lz >= 0) {
+        T *ps_ = &g.psi_h[P_AZ][((long long)i * nm1y + j) * Lz + lz];
+        const T ps = g.bm[2][k] * (*ps_) + g.am[2][k] * dey_k;
+        *ps_ = ps;
+        v = v + g.cpsi[CB_HXEY][m * nm1z + k] * ps;
+      }
+    }
+    if (yee || slab) sto(q.Hx, o, v);
+  }
+  // Hy: shape (nx, ny+1, nz); Yee j < ny
+  {
+    const bool in = (i < nx) && kin;
+    const bool yee = in && jin;
+    T v = yee ? yy : p.hy;
+    const bool slab = in && (lx >= 0 || lz >= 0);
+    if (slab) {
+      if (lx >= 0) {
+        T *ps_ = &g.psi_h[P_BX][((long long)lx * nm1y + j) * g.pz + k];
+        const T ps = g.bm[0][i] * (*ps_) + g.am[0][i] * dez_i;
+        *ps_ = ps;
+        v = v + g.cpsi[CB_HYEZ][m * nm1x + i] * ps;
+      }
+      if (lz >= 0) {
+        T *ps_ = &g.psi_h[P_BZ][((long long)i * nm1y + j) * Lz + lz];
+        const T ps = g.bm[2][k] * (*ps_) + g.am[2][k] * dex_k;
+        *ps_ = ps;
+        v = v + g.cpsi[CB_HYEX][m * nm1z + k] * ps;
+      }
+    }
+    if (yee || slab) sto(q.Hy, o, v);
+  }
+  // Hz: shape (nx, ny, nz+1); Yee k < nz
+  {
+    const bool in = (i < nx) && jin;
+    const bool yee = in && kin;
+    T v = yee ? yz : p.hz;
+    const bool slab = in && (lx >= 0 || ly >= 0);
+    if (slab) {
+      if (lx >= 0) {
+        T *ps_ = &g.psi_h[P_CX][((long long)lx * nm1y + j) * g.pz + k];
+        const T ps = g.bm[0][i] * (*ps_) + g.am[0][i] * dey_i;
+        *ps_ = ps;
+        v = v + g.cpsi[CB_HZEY][m * nm1x + i] * ps;
+      }
+      if (ly >= 0) {
+        T *ps_ = &g.psi_h[P_CY][((long long)i * Ly + ly) * g.pz + k];
+        const T ps = g.bm[1][j] * (*ps_) + g.am[1][j] * dex_j;
+        *ps_ = ps;
+        v = v + g.cpsi[CB_HZEX][m * nm1y + j] * ps;
+      }
+    }
+    if (yee || slab) sto(q.Hz, o, v);
+  }
+}
+
+template <typename T, bool LDS, bool PF>
 __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl) {
   extern __shared__ unsigned char lds_raw[];
   int bz, by, bx;
   tile_of_block(tl, bz, by, bx);
-  const int kl = (int)(threadIdx.x % tl.tz), jl = (int)(threadIdx.x / tl.tz);
-  const int k = bz * tl.tz + kl;
-  const int j = by * tl.ty + jl;
+  const Cell cell = thread_cell(tl, bz, by, g.pz);
+  const int k = cell.k, j = cell.j;
   const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
   const int i0 = bx * tl.tx;
   const int i1 = min(i0 + tl.tx, nx + 1);
@@ -309,124 +541,48 @@ __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl) {
   if (LDS)
     tt = stage_tables<T>(reinterpret_cast<T *>(lds_raw), g.ca, HX, nmat, g.cb[CB_HXEY], g.cb[CB_HYEX],
                          nz + 1, g.cb[CB_HXEZ], g.cb[CB_HZEX], ny + 1, g.cb[CB_HYEZ], g.cb[CB_HZEY],
-                         nx + 1, tl, bz * tl.tz, by * tl.ty, i0);
+                         nx + 1, tl, cell, i0);
   else
     tt = global_tables<T>(g.ca, HX, nmat, g.cb[CB_HXEY], g.cb[CB_HYEX], nz + 1, g.cb[CB_HXEZ],
-                          g.cb[CB_HZEX], ny + 1, g.cb[CB_HYEZ], g.cb[CB_HZEY], nx + 1);
+                          g.cb[CB_HZEX], ny + 1, g.cb[CB_HYEZ], g.cb[CB_HZEY], nx + 1, cell);
   if (k > nz || j > ny) return;
-  const int nm1y = ny + 1, nm1z = nz + 1, nm1x = nx + 1;
-  const T *__restrict__ Ex = g.f[EX];
-  const T *__restrict__ Ey = g.f[EY];
-  const T *__restrict__ Ez = g.f[EZ];
-  T *__restrict__ Hx = g.f[HX];
-  T *__restrict__ Hy = g.f[HY];
-  T *__restrict__ Hz = g.f[HZ];
-  const uint8_t *__restrict__ mat = g.mat;
-  const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2];
+  ColH<T> q;
+  q.Ex = g.f[EX]; q.Ey = g.f[EY]; q.Ez = g.f[EZ];
+  q.Hx = g.f[HX]; q.Hy = g.f[HY]; q.Hz = g.f[HZ];
+  q.mat = g.mat;
+  q.sj = g.sj;
+  q.si = g.si;
+  q.bsj = (unsigned)(g.sj * (long long)sizeof(T));
+  q.bsi = (unsigned)(g.si * (long long)sizeof(T));
+  q.kin = k < nz;
+  q.jin = j < ny;
+  q.nx = nx;
   const int ly = h_layer(j, ny, g.dn[1], g.dp[1]);
   const int lz = h_layer(k, nz, g.dn[2], g.dp[2]);
-  const bool kin = k < nz, jin = j < ny;
   long long c = (long long)i0 * g.si + (long long)j * g.sj + k;
-  // Ez(i), Ey(i) come from the previous plane's (i+1) loads
-  T ez_c = Ez[c], ey_c = Ey[c];
-  int m_n = mat[c];
-  T hx_n = Hx[c], hy_n = Hy[c], hz_n = Hz[c];
-  T ex_n = Ex[c];
-  T eyk_n = kin ? Ey[c + 1] : T(0);    // Ey(k+1)
-  T exk_n = kin ? Ex[c + 1] : T(0);    // Ex(k+1)
-  T ezj_n = jin ? Ez[c + g.sj] : T(0); // Ez(j+1)
-  T exj_n = jin ? Ex[c + g.sj] : T(0); // Ex(j+1)
-  T ezi_n = (i0 < nx) ? Ez[c + g.si] : T(0);  // Ez(i+1)
-  T eyi_n = (i0 < nx) ? Ey[c + g.si] : T(0);  // Ey(i+1)
-  for (int i = i0; i < i1; ++i, c += g.si) {
-    const int m = m_n;
-    const T hx = hx_n, hy = hy_n, hz = hz_n, ex = ex_n;
-    const T eyk = eyk_n, exk = exk_n, ezj = ezj_n, exj = exj_n, ezi = ezi_n, eyi = eyi_n;
-    if (i + 1 < i1) {
-      const long long cn = c + g.si;
-      m_n = mat[cn];
-      hx_n = Hx[cn]; hy_n = Hy[cn]; hz_n = Hz[cn];
-      ex_n = Ex[cn];
-      eyk_n = kin ? Ey[cn + 1] : T(0);
-      exk_n = kin ? Ex[cn + 1] : T(0);
-      ezj_n = jin ? Ez[cn + g.sj] : T(0);
-      exj_n = jin ? Ex[cn + g.sj] : T(0);
-      ezi_n = (i + 1 < nx) ? Ez[cn + g.si] : T(0);
-      eyi_n = (i + 1 < nx) ? Ey[cn + g.si] : T(0);
+  T ez_c = ldo(q.Ez, (unsigned)c * (unsigned)sizeof(T)), ey_c = ldo(q.Ey, (unsigned)c * (unsigned)sizeof(T));
+  if (!PF) {
+    for (int i = i0; i < i1; ++i, c += g.si) {
+      PlaneH<T> A;
+      load_plane_h(q, c, i, A);
+      compute_plane_h(g, tt, q, A, i, j, k, c, ly, lz, ez_c, ey_c);
+      ez_c = A.ezi;
+      ey_c = A.eyi;
     }
-    const int lx = h_layer(i, nx, g.dn[0], g.dp[0]);
-    const T dey_k = eyk - ey_c;  // Ey(k+1) - Ey(k)
-    const T dez_j = ezj - ez_c;  // Ez(j+1) - Ez(j)
-    const T dez_i = ezi - ez_c;  // Ez(i+1) - Ez(i)
-    const T dex_k = exk - ex;    // Ex(k+1) - Ex(k)
-    const T dex_j = exj - ex;    // Ex(j+1) - Ex(j)
-    const T dey_i = eyi - ey_c;  // Ey(i+1) - Ey(i)
-    const int mk = m * tt.sk + (k - tt.k0), mj = m * tt.sj + (j - tt.j0), mi = m * tt.si + (i - tt.i0);
-    // Hx: shape (nx+1, ny, nz); Yee i < nx
-    if (jin && kin) {
-      const bool yee = (i < nx);
-      if (yee || ly >= 0 || lz >= 0) {
-        T v = hx;
-        if (yee) v = tt.ca[m] * v + tt.tk0[mk] * dey_k + tt.tj0[mj] * dez_j;
-        if (ly >= 0) {
-          T *p = &g.psi_h[P_AY][((long long)i * Ly + ly) * g.pz + k];
-          const T ps = g.bm[1][j] * (*p) + g.am[1][j] * dez_j;
-          *p = ps;
-          v = v + g.cpsi[CB_HXEZ][m * nm1y + j] * ps;
-        }
-        if (lz >= 0) {
-          T *p = &g.psi_h[P_AZ][((long long)i * nm1y + j) * Lz + lz];
-          const T ps = g.bm[2][k] * (*p) + g.am[2][k] * dey_k;
-          *p = ps;
-          v = v + g.cpsi[CB_HXEY][m * nm1z + k] * ps;
-        }
-        Hx[c] = v;
-      }
-    }
-    // Hy: shape (nx, ny+1, nz); Yee j < ny
-    if (i < nx && kin) {
-      const bool yee = jin;
-      if (yee || lx >= 0 || lz >= 0) {
-        T v = hy;
-        if (yee) v = tt.ca[nmat + m] * v + tt.ti0[mi] * dez_i + tt.tk1[mk] * dex_k;
-        if (lx >= 0) {
-          T *p = &g.psi_h[P_BX][((long long)lx * nm1y + j) * g.pz + k];
-          const T ps = g.bm[0][i] * (*p) + g.am[0][i] * dez_i;
-          *p = ps;
-          v = v + g.cpsi[CB_HYEZ][m * nm1x + i] * ps;
-        }
-        if (lz >= 0) {
-          T *p = &g.psi_h[P_BZ][((long long)i * nm1y + j) * Lz + lz];
-          const T ps = g.bm[2][k] * (*p) + g.am[2][k] * dex_k;
-          *p = ps;
-          v = v + g.cpsi[CB_HYEX][m * nm1z + k] * ps;
-        }
-        Hy[c] = v;
-      }
-    }
-    // Hz: shape (nx, ny, nz+1); Yee k < nz
-    if (i < nx && jin) {
-      const bool yee = kin;
-      if (yee || lx >= 0 || ly >= 0) {
-        T v = hz;
-        if (yee) v = tt.ca[2 * nmat + m] * v + tt.tj1[mj] * dex_j + tt.ti1[mi] * dey_i;
-        if (lx >= 0) {
-          T *p = &g.psi_h[P_CX][((long long)lx * nm1y + j) * g.pz + k];
-          const T ps = g.bm[0][i] * (*p) + g.am[0][i] * dey_i;
-          *p = ps;
-          v = v + g.cpsi[CB_HZEY][m * nm1x + i] * ps;
-        }
-        if (ly >= 0) {
-          T *p = &g.psi_h[P_CY][((long long)i * Ly + ly) * g.pz + k];
-          const T ps = g.bm[1][j] * (*p) + g.am[1][j] * dex_j;
-          *p = ps;
-          v = v + g.cpsi[CB_HZEX][m * nm1y + j] * ps;
-        }
-        Hz[c] = v;
-      }
-    }
-    ez_c = ezi;
-    ey_c = eyi;
+    return;
+  }
+  PlaneH<T> A, B;
+  load_plane_h(q, c, i0, A);
+  for (int i = i0; i < i1; i += 2, c += 2 * g.si) {
+    if (i + 1 < i1) load_plane_h(q, c + g.si, i + 1, B);
+    compute_plane_h(g, tt, q, A, i, j, k, c, ly, lz, ez_c, ey_c);
+    ez_c = A.ezi;
+    ey_c = A.eyi;
+    if (i + 1 >= i1) break;
+    if (i + 2 < i1) load_plane_h(q, c + 2 * g.si, i + 2, A);
+    compute_plane_h(g, tt, q, B, i + 1, j, k, c + g.si, ly, lz, ez_c, ey_c);
+    ez_c = B.ezi;
+    ey_c = B.eyi;
   }
 }
 // ---------------------------------------------------------------------------

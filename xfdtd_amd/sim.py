@@ -172,7 +172,7 @@ class Simulation:
             raise RuntimeError(L.xf_last_error().decode())
         return a.value, b.value, n.value
 
-    def set_tiling(self, tz, ty, tx):
+    def set_tiling(self, tz, ty, tx, flags=0):
         L = lib()
-        if L.xf_engine_set_tiling(self.engine, int(tz), int(ty), int(tx)) != 0:
+        if L.xf_engine_set_tiling(self.engine, int(tz), int(ty), int(tx), int(flags)) != 0:
             raise RuntimeError(L.xf_last_error().decode())
