@@ -12,8 +12,11 @@ Prints one JSON line.  value = N^3 * steps / wall time of the timed steps
 (Gcell-updates/s), inputs resident in HBM.  roofline: algorithmic bytes of
 the dominant curl kernel (18*s/2 + 1 bytes per cell per half step: read 3 E +
 3 H, write 3 of one kind, one material byte) over its average HIP-event
-duration inside the timed region.  cpu_baseline: the FP64 CPU oracle, one
-thread, on a few steps of the same 400^3 run (set-up excluded).
+duration inside the timed region; traffic: the HBM bytes per launch of that
+kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by the same
+counters' reading for a plain 1 GiB copy (profiles/hbm_traffic.json).
+cpu_baseline: the FP64 CPU oracle, one thread, on two steps of the same 400^3
+run (set-up excluded); cpu_baseline_threads: the same with every host core.
 """
 from __future__ import annotations
 
@@ -56,9 +59,9 @@ def cpu_model():
     return platform.processor()
 
 
-def run_cpu_baseline(sc, total, steps):
+def run_cpu_baseline(sc, total, steps, threads=1):
     from oracle.oracle import OracleSim, set_threads
-    threads = set_threads(1)
+    threads = set_threads(threads)
     o = OracleSim(sc, total)
     t0 = time.perf_counter()
     o.step(steps)
@@ -72,9 +75,20 @@ def run_cpu_baseline(sc, total, steps):
         "cores": threads,
         "kind": "port",
         "sample": f"FP64 oracle, {nx}x{ny}x{nz} grid of the same run, first {steps} steps "
-                  f"timed after set-up ({dt:.1f} s), {threads} thread on '{cpu_model()}' "
+                  f"timed after set-up ({dt:.1f} s), {threads} thread(s) on '{cpu_model()}' "
                   f"(nproc {os.cpu_count()})",
     }
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a curl kernel from the committed counter runs."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["kernels"][kernel]["bytes"], d
+    except (OSError, KeyError, ValueError):
+        return None, None
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -100,7 +114,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--size", type=int, default=400)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="threads of the second, multi-core CPU column (0: skip it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tiling", type=str, default="")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -144,6 +160,9 @@ def main():
     ms_dom = max(ms_e, ms_h)
     dom = "k_update_h" if ms_h >= ms_e else "k_update_e"
     achieved = bytes_half * cells / (ms_dom * 1e-3) / 1e9
+    traffic, tinfo = (None, None)
+    if args.precision == 32 and args.size == 400 and not args.tiling:
+        traffic, tinfo = measured_traffic(dom)
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -173,7 +192,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": None if traffic is None else round(traffic / 1e9, 4),
+            "traffic_unit": "GB per launch",
+            "traffic_source": None if tinfo is None else tinfo.get("source"),
             "bytes_per_launch": bytes_half * cells,
             "avg_ms": {"k_update_e": round(ms_e, 4), "k_update_h": round(ms_h, 4)},
             "event_samples": nsamp,
@@ -183,6 +204,8 @@ def main():
     sim.close()
     if not args.no_cpu and args.cpu_steps > 0:
         line["cpu_baseline"] = run_cpu_baseline(sc, total, args.cpu_steps)
+        if args.cpu_threads > 1:
+            line["cpu_baseline_threads"] = run_cpu_baseline(sc, total, args.cpu_steps, args.cpu_threads)
     print(json.dumps(line), flush=True)
 
 

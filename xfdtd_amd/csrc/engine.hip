@@ -86,7 +86,7 @@ struct xf_engine {
   long long cur = 0;
   long long total = -1;  // smallest total_steps among the attached tables
   // tiling
-  int tz = 64, ty = 4, tx = 16, tile_flags = 0;
+  int tz = 64, ty = 4, tx = 8, tile_flags = 0;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 4 events per timed step: E start/end, H start/end
@@ -512,13 +512,14 @@ static xf_status run_steps(xf_engine *e, long long n) {
   }
   tl.ntx = (e->nx + 1 + tl.tx - 1) / tl.tx;
   tl.nblocks = tl.ntz * tl.nty * tl.ntx;
+  tl.x_fastest = (e->tile_flags & XF_TILE_X_FASTEST) != 0;
+  tl.xcd_runs = (e->tile_flags & XF_TILE_XCD_RUNS) != 0;
   dim3 blk(tl.ty == 0 ? tl.tz : tl.tz * tl.ty);
   dim3 grd(tl.nblocks);
   // coefficient tables of a tile go to LDS when they fit in 32 KB
   const size_t lds = lds_table_bytes<T>(e->nmat, tl);
   const bool use_lds = lds <= 32768 && !(e->tile_flags & XF_TILE_GLOBAL_TABLES);
   const bool prefetch = (e->tile_flags & XF_TILE_PREFETCH) != 0;
-  tl.x_fastest = (e->tile_flags & XF_TILE_X_FASTEST) != 0;
   for (long long q = 0; q < n; ++q) {
     const long long step = e->cur;
     hipEvent_t *ev = nullptr;
@@ -679,7 +680,7 @@ extern "C" xf_status xf_engine_set_timing(xf_engine *e, int enabled) {
 
 extern "C" xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags) {
   if (!e) FAILS(XF_EINVAL, "null engine");
-  if (tz < 64 || tz % 64 != 0 || ty < 0 || tx < 0 || (ty > 0 ? tz * ty : tz) > 256 || flags < 0 || flags > 7)
+  if (tz < 64 || tz % 64 != 0 || ty < 0 || tx < 0 || (ty > 0 ? tz * ty : tz) > 256 || flags < 0 || flags > 15)
     FAILS(XF_EINVAL, "tiling %d x %d x %d (flags %d): need tz a multiple of 64 and at most 256 threads", tz,
           ty, tx, flags);
   e->tz = tz; e->ty = ty; e->tx = tx;

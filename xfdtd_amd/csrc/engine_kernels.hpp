@@ -54,22 +54,27 @@ __device__ __forceinline__ int h_layer(int n, int N, int dn, int dp) {
 
 // Launch tiling of the curl kernels: a 1-D grid of tiles, each TZ x TY
 // threads over (k, j) marching over TX x-planes.  The hardware hands
-// consecutive workgroup ids to the 8 XCDs round-robin; tile ids are remapped
-// so that each XCD works on one contiguous run of tiles, which keeps a tile's
-// y-1 neighbour row in the same XCD's L2.
+// consecutive workgroup ids to the 8 XCDs round-robin, and each XCD has its
+// own L2.  By default tile ids follow workgroup ids (z, y, x order): the eight
+// XCDs stream through neighbouring tiles together, and a tile's y-1 row and
+// x-1 plane, read by another XCD, come back through the fabric.  xcd_runs
+// instead gives each XCD one contiguous run of tiles, which keeps those
+// neighbours in its L2 (about 20% less read traffic) but measures about 15%
+// slower on the 400^3 grid; so does giving each XCD a band of y rows.
 struct Tiling {
   int tz, ty, tx;           // threads along k, j (ty = 0: flat); planes along i
   int ntz, nty, ntx;        // tile counts
   int nblocks;
   int krows, jrows;         // coefficient-table slots of a tile along k and j
   int x_fastest;            // tile order: x chunks first (else z, y, x)
+  int xcd_runs;             // one contiguous run of tile ids per XCD
 };
 
 __device__ __forceinline__ void tile_of_block(const Tiling &t, int &bz, int &by, int &bx) {
   const int b = blockIdx.x;
   const int q = t.nblocks >> 3, r = t.nblocks & 7;
   const int xcd = b & 7, local = b >> 3;
-  const int id = xcd * q + min(xcd, r) + local;
+  const int id = t.xcd_runs ? xcd * q + min(xcd, r) + local : b;
   if (t.x_fastest) {
     bx = id % t.ntx;
     const int rest = id / t.ntx;
@@ -119,60 +124,96 @@ __device__ __forceinline__ Cell thread_cell(const Tiling &tl, int bz, int by, in
 // over its i planes.
 template <typename T>
 struct TileTables {
-  const T *ca;    // [3][nmat]
-  const T *tk0, *tk1;  // [nmat][krows]
-  const T *tj0, *tj1;  // [nmat][jrows]
-  const T *ti0, *ti1;  // [nmat][TX]
-  int sk, sj, si;      // entries per material of the k/j/i tables
-  int ks, js, i0;      // this thread's k and j slots, tile's first plane
+  const T *ca;                                 // [3][nmat] self terms
+  const T *tk0, *tk1, *tj0, *tj1, *ti0, *ti1;  // curl coefficients [nmat][slots]
+  const T *pk0, *pk1, *pj0, *pj1, *pi0, *pi1;  // CPML psi coefficients, same slots
+  int sk, sj, si;                              // slots per material along k, j, i
+  int ks, js, i0;                              // this thread's k and j slots, tile's first plane
+};
+
+// The global tables one curl kernel reads: three self terms from ca_base, and
+// per axis two curl tables and the two psi tables of the same components,
+// each [nmat][n + 1].
+template <typename T>
+struct TableSrc {
+  int ca_base;
+  const T *k0, *k1, *j0, *j1, *i0, *i1;
+  const T *pk0, *pk1, *pj0, *pj1, *pi0, *pi1;
+  int nk1, nj1, ni1;
 };
 
 template <typename T>
 __host__ __device__ inline size_t lds_table_bytes(int nmat, const Tiling &tl) {
-  return sizeof(T) * (size_t)nmat * (3 + 2 * tl.krows + 2 * tl.jrows + 2 * tl.tx);
+  return sizeof(T) * (size_t)nmat * (3 + 4 * tl.krows + 4 * tl.jrows + 4 * tl.tx);
 }
 
-// stage the tables of one tile (cooperative; ends with a barrier).
-// self: ca entries ca_base..ca_base+2; k/j/i: global tables with lengths
-// (n+1) per material.
+// does the index range [lo, hi] meet a CPML slab of an axis of N cells?  The
+// E layers are n in [1, dn] and [N-dp, N-1], the H layers [0, dn) and
+// [N-dp, N); both lie inside [0, dn] and [N-dp-1, N].
+__device__ __forceinline__ bool meets_slab(int lo, int hi, int N, int dn, int dp) {
+  return (dn > 0 && lo <= dn) || (dp > 0 && hi >= N - dp - 1);
+}
+
+// Stage the tables of one tile in LDS (cooperative; ends with a barrier).
+// The psi tables of an axis are staged only when the tile meets a slab of
+// that axis: a thread reads them only for a cell inside one.
 template <typename T>
-__device__ __forceinline__ TileTables<T> stage_tables(T *lds, const T *ca, int ca_base, int nmat,
-                                                     const T *gk0, const T *gk1, int nk1,
-                                                     const T *gj0, const T *gj1, int nj1,
-                                                     const T *gi0, const T *gi1, int ni1,
-                                                     const Tiling &tl, const Cell &cell, int i0) {
-  TileTables<T> tt;
+__device__ __forceinline__ TileTables<T> stage_tables(T *lds, const T *ca, int nmat, const TableSrc<T> &s,
+                                                     const Tiling &tl, const Cell &cell, int i0,
+                                                     bool psi_k, bool psi_j, bool psi_i) {
+  const int nk = nmat * tl.krows, nj = nmat * tl.jrows, ni = nmat * tl.tx;
   T *p = lds;
   T *sca = p; p += 3 * nmat;
-  T *sk0 = p; p += nmat * tl.krows;
-  T *sk1 = p; p += nmat * tl.krows;
-  T *sj0 = p; p += nmat * tl.jrows;
-  T *sj1 = p; p += nmat * tl.jrows;
-  T *si0 = p; p += nmat * tl.tx;
-  T *si1 = p;
+  T *sk0 = p; p += nk;
+  T *sk1 = p; p += nk;
+  T *qk0 = p; p += nk;
+  T *qk1 = p; p += nk;
+  T *sj0 = p; p += nj;
+  T *sj1 = p; p += nj;
+  T *qj0 = p; p += nj;
+  T *qj1 = p; p += nj;
+  T *si0 = p; p += ni;
+  T *si1 = p; p += ni;
+  T *qi0 = p; p += ni;
+  T *qi1 = p;
   const int nth = blockDim.x;
-  for (int q = threadIdx.x; q < 3 * nmat; q += nth) sca[q] = ca[(ca_base + q / nmat) * nmat + q % nmat];
+  for (int q = threadIdx.x; q < 3 * nmat; q += nth) sca[q] = ca[(s.ca_base + q / nmat) * nmat + q % nmat];
   if (cell.js == 0 || tl.ty == 0) {
     // each k slot is written by the thread that owns it (row 0 of a 2-D tile)
-    const int n = min(cell.k, nk1 - 1);
+    const int n = min(cell.k, s.nk1 - 1);
     for (int m = 0; m < nmat; ++m) {
-      sk0[m * tl.krows + cell.ks] = gk0[m * nk1 + n];
-      sk1[m * tl.krows + cell.ks] = gk1[m * nk1 + n];
+      const int d = m * tl.krows + cell.ks, o = m * s.nk1 + n;
+      sk0[d] = s.k0[o];
+      sk1[d] = s.k1[o];
+      if (psi_k) {
+        qk0[d] = s.pk0[o];
+        qk1[d] = s.pk1[o];
+      }
     }
   }
-  for (int q = threadIdx.x; q < nmat * tl.jrows; q += nth) {
-    const int m = q / tl.jrows, n = min(cell.j_first + q % tl.jrows, nj1 - 1);
-    sj0[q] = gj0[m * nj1 + n];
-    sj1[q] = gj1[m * nj1 + n];
+  for (int q = threadIdx.x; q < nj; q += nth) {
+    const int o = (q / tl.jrows) * s.nj1 + min(cell.j_first + q % tl.jrows, s.nj1 - 1);
+    sj0[q] = s.j0[o];
+    sj1[q] = s.j1[o];
+    if (psi_j) {
+      qj0[q] = s.pj0[o];
+      qj1[q] = s.pj1[o];
+    }
   }
-  for (int q = threadIdx.x; q < nmat * tl.tx; q += nth) {
-    const int m = q / tl.tx, n = min(i0 + q % tl.tx, ni1 - 1);
-    si0[q] = gi0[m * ni1 + n];
-    si1[q] = gi1[m * ni1 + n];
+  for (int q = threadIdx.x; q < ni; q += nth) {
+    const int o = (q / tl.tx) * s.ni1 + min(i0 + q % tl.tx, s.ni1 - 1);
+    si0[q] = s.i0[o];
+    si1[q] = s.i1[o];
+    if (psi_i) {
+      qi0[q] = s.pi0[o];
+      qi1[q] = s.pi1[o];
+    }
   }
   __syncthreads();
+  TileTables<T> tt;
   tt.ca = sca;
   tt.tk0 = sk0; tt.tk1 = sk1; tt.tj0 = sj0; tt.tj1 = sj1; tt.ti0 = si0; tt.ti1 = si1;
+  tt.pk0 = qk0; tt.pk1 = qk1; tt.pj0 = qj0; tt.pj1 = qj1; tt.pi0 = qi0; tt.pi1 = qi1;
   tt.sk = tl.krows; tt.sj = tl.jrows; tt.si = tl.tx;
   tt.ks = cell.ks; tt.js = cell.js; tt.i0 = i0;
   return tt;
@@ -180,16 +221,14 @@ __device__ __forceinline__ TileTables<T> stage_tables(T *lds, const T *ca, int c
 
 // the same view over the global tables (no staging)
 template <typename T>
-__device__ __forceinline__ TileTables<T> global_tables(const T *ca, int ca_base, int nmat,
-                                                      const T *gk0, const T *gk1, int nk1,
-                                                      const T *gj0, const T *gj1, int nj1,
-                                                      const T *gi0, const T *gi1, int ni1,
+__device__ __forceinline__ TileTables<T> global_tables(const T *ca, int nmat, const TableSrc<T> &s,
                                                       const Cell &cell) {
   TileTables<T> tt;
-  tt.ca = ca + ca_base * nmat;
-  tt.tk0 = gk0; tt.tk1 = gk1; tt.tj0 = gj0; tt.tj1 = gj1; tt.ti0 = gi0; tt.ti1 = gi1;
-  tt.sk = nk1; tt.sj = nj1; tt.si = ni1;
-  tt.ks = min(cell.k, nk1 - 1); tt.js = min(cell.j, nj1 - 1); tt.i0 = 0;
+  tt.ca = ca + s.ca_base * nmat;
+  tt.tk0 = s.k0; tt.tk1 = s.k1; tt.tj0 = s.j0; tt.tj1 = s.j1; tt.ti0 = s.i0; tt.ti1 = s.i1;
+  tt.pk0 = s.pk0; tt.pk1 = s.pk1; tt.pj0 = s.pj0; tt.pj1 = s.pj1; tt.pi0 = s.pi0; tt.pi1 = s.pi1;
+  tt.sk = s.nk1; tt.sj = s.nj1; tt.si = s.ni1;
+  tt.ks = min(cell.k, s.nk1 - 1); tt.js = min(cell.j, s.nj1 - 1); tt.i0 = 0;
   return tt;
 }
 
@@ -206,16 +245,61 @@ __device__ __forceinline__ void sto(T *base, unsigned off, T v) {
 }
 __device__ __forceinline__ unsigned ldm(const uint8_t *base, unsigned cell) { return base[cell]; }
 
+// psi element offsets.  Slab arrays of the y axis are [nx+1][Ly][pz], of the
+// z axis [nx+1][ny+1][Lz], of the x axis [Lx][ny+1][pz]; every offset fits 32
+// bits because every slab array is smaller than a field.
+__device__ __forceinline__ unsigned psi_y(int i, int ly, int k, int Ly, int pz) {
+  return ((unsigned)i * (unsigned)Ly + (unsigned)ly) * (unsigned)pz + (unsigned)k;
+}
+__device__ __forceinline__ unsigned psi_z(int i, int j, int lz, int nm1y, int Lz) {
+  return ((unsigned)i * (unsigned)nm1y + (unsigned)j) * (unsigned)Lz + (unsigned)lz;
+}
+__device__ __forceinline__ unsigned psi_x(int lx, int j, int k, int nm1y, int pz) {
+  return ((unsigned)lx * (unsigned)nm1y + (unsigned)j) * (unsigned)pz + (unsigned)k;
+}
+
+// CPML state of one (j, k) column: its layer along y and z (-1 outside a
+// slab) and the b/a coefficients of those layers, read once per column.
+template <typename T>
+struct ColCpml {
+  int ly, lz;
+  T by, ay, bz, az;
+};
+
+template <typename T>
+__device__ __forceinline__ ColCpml<T> col_cpml(const Grid<T> &g, const T *const *b, const T *const *a,
+                                               int ly, int lz, int j, int k) {
+  ColCpml<T> cc;
+  cc.ly = ly;
+  cc.lz = lz;
+  cc.by = cc.ay = cc.bz = cc.az = T(0);
+  if (ly >= 0) {
+    cc.by = b[1][j];
+    cc.ay = a[1][j];
+  }
+  if (lz >= 0) {
+    cc.bz = b[2][k];
+    cc.az = a[2][k];
+  }
+  return cc;
+}
+
 // ---------------------------------------------------------------------------
 // E half-step: Simulation::updateE (simulation_update.cpp:170-196) fused with
 // PML::updateE of every face (perfect_match_layer.cpp:574-718), faces in x,
 // y, z order.  One thread owns a (j,k) column and marches over TX x-planes.
-// Two plane buffers (A, B) alternate so that the loads of plane i+1 are in
-// flight while plane i is computed, without register copies between them.
+// A plane's loads -- the six field values, the material byte, the four
+// in-plane neighbours and, inside a CPML slab, the psi values of the cell --
+// are issued together, so a plane costs one memory round trip.  With PF two
+// plane buffers (A, B) alternate so that the loads of plane i+1 are in flight
+// while plane i is computed.
 template <typename T>
 struct PlaneE {
   unsigned m;
   T ex, ey, ez, hx, hy, hz, hyk, hxk, hzj, hxj;
+  int lx;     // layer along x (-1 outside the x slabs)
+  T bx, ax;   // its CPML b/a (wave-uniform)
+  T ps[6];    // psi values of the cell, by P_* slot
 };
 
 template <typename T>
@@ -227,13 +311,14 @@ struct ColE {
   const T *__restrict__ Hy;
   const T *__restrict__ Hz;
   const uint8_t *__restrict__ mat;
-  long long sj;
   unsigned bsj;  // byte stride of j
   bool k1, j1;   // k >= 1, j >= 1
+  ColCpml<T> cp;
 };
 
 template <typename T>
-__device__ __forceinline__ void load_plane_e(const ColE<T> &q, long long c, PlaneE<T> &p) {
+__device__ __forceinline__ void load_plane_e(const Grid<T> &g, const ColE<T> &q, long long c, int i, int j,
+                                             int k, PlaneE<T> &p) {
   const unsigned cell = (unsigned)c, o = cell * (unsigned)sizeof(T);
   p.m = ldm(q.mat, cell);
   p.ex = ldo(q.Ex, o);
@@ -246,18 +331,39 @@ __device__ __forceinline__ void load_plane_e(const ColE<T> &q, long long c, Plan
   p.hxk = q.k1 ? ldo(q.Hx, o - (unsigned)sizeof(T)) : T(0);
   p.hzj = q.j1 ? ldo(q.Hz, o - q.bsj) : T(0);
   p.hxj = q.j1 ? ldo(q.Hx, o - q.bsj) : T(0);
+  p.lx = e_layer(i, g.nx, g.dn[0], g.dp[0]);
+  p.bx = p.ax = T(0);
+  for (int s = 0; s < 6; ++s) p.ps[s] = T(0);
+  const int ly = q.cp.ly, lz = q.cp.lz, lx = p.lx;
+  if (lx < 0 && ly < 0 && lz < 0) return;
+  const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = g.ny + 1;
+  if (lx >= 0) {
+    p.bx = g.be[0][i];
+    p.ax = g.ae[0][i];
+  }
+  if (i < g.nx) {
+    if (ly >= 0) p.ps[P_AY] = g.psi_e[P_AY][psi_y(i, ly, k, Ly, g.pz)];
+    if (lz >= 0) p.ps[P_AZ] = g.psi_e[P_AZ][psi_z(i, j, lz, nm1y, Lz)];
+  }
+  if (j < g.ny) {
+    if (lx >= 0) p.ps[P_BX] = g.psi_e[P_BX][psi_x(lx, j, k, nm1y, g.pz)];
+    if (lz >= 0) p.ps[P_BZ] = g.psi_e[P_BZ][psi_z(i, j, lz, nm1y, Lz)];
+  }
+  if (k < g.nz) {
+    if (lx >= 0) p.ps[P_CX] = g.psi_e[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
+    if (ly >= 0) p.ps[P_CY] = g.psi_e[P_CY][psi_y(i, ly, k, Ly, g.pz)];
+  }
 }
 
 template <typename T>
 __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTables<T> &tt, const ColE<T> &q,
                                                 const PlaneE<T> &p, int i, int j, int k, long long c,
-                                                int ly, int lz, bool jin, bool kin, T hz_prev,
-                                                T hy_prev) {
+                                                bool jin, bool kin, T hz_prev, T hy_prev) {
   const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
-  const int nm1y = ny + 1, nm1z = nz + 1, nm1x = nx + 1;
+  const int nm1y = ny + 1;
   const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2];
   const int m = (int)p.m;
-  const int lx = e_layer(i, nx, g.dn[0], g.dp[0]);
+  const int lx = p.lx, ly = q.cp.ly, lz = q.cp.lz;
   const T dhy_k = p.hy - p.hyk;     // Hy(k) - Hy(k-1)
   const T dhz_j = p.hz - p.hzj;     // Hz(j) - Hz(j-1)
   const T dhx_k = p.hx - p.hxk;     // Hx(k) - Hx(k-1)
@@ -276,21 +382,20 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
   {
     const bool yee = (i < nx) && jin && kin;
     T v = yee ? yx : p.ex;
-    if (i < nx && (ly >= 0 || lz >= 0)) {
+    const bool slab = (i < nx) && (ly >= 0 || lz >= 0);
+    if (slab) {
       if (ly >= 0) {
-        T *ps_ = &g.psi_e[P_AY][((long long)i * Ly + ly) * g.pz + k];
-        const T ps = g.be[1][j] * (*ps_) + g.ae[1][j] * dhz_j;
-        *ps_ = ps;
-        v = v + g.cpsi[CB_EXHZ][m * nm1y + j] * ps;
+        const T ps = q.cp.by * p.ps[P_AY] + q.cp.ay * dhz_j;
+        g.psi_e[P_AY][psi_y(i, ly, k, Ly, g.pz)] = ps;
+        v = v + tt.pj0[mj] * ps;
       }
       if (lz >= 0) {
-        T *ps_ = &g.psi_e[P_AZ][((long long)i * nm1y + j) * Lz + lz];
-        const T ps = g.be[2][k] * (*ps_) + g.ae[2][k] * dhy_k;
-        *ps_ = ps;
-        v = v + g.cpsi[CB_EXHY][m * nm1z + k] * ps;
+        const T ps = q.cp.bz * p.ps[P_AZ] + q.cp.az * dhy_k;
+        g.psi_e[P_AZ][psi_z(i, j, lz, nm1y, Lz)] = ps;
+        v = v + tt.pk0[mk] * ps;
       }
     }
-    if (yee || (i < nx && (ly >= 0 || lz >= 0))) sto(q.Ex, o, v);
+    if (yee || slab) sto(q.Ex, o, v);
   }
   // Ey: shape (nx+1, ny, nz+1); Yee k in [1,nz), i in [1,nx)
   {
@@ -299,16 +404,14 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
     const bool slab = (j < ny) && (lx >= 0 || lz >= 0);
     if (slab) {
       if (lx >= 0) {
-        T *ps_ = &g.psi_e[P_BX][((long long)lx * nm1y + j) * g.pz + k];
-        const T ps = g.be[0][i] * (*ps_) + g.ae[0][i] * dhz_i;
-        *ps_ = ps;
-        v = v + g.cpsi[CB_EYHZ][m * nm1x + i] * ps;
+        const T ps = p.bx * p.ps[P_BX] + p.ax * dhz_i;
+        g.psi_e[P_BX][psi_x(lx, j, k, nm1y, g.pz)] = ps;
+        v = v + tt.pi0[mi] * ps;
       }
       if (lz >= 0) {
-        T *ps_ = &g.psi_e[P_BZ][((long long)i * nm1y + j) * Lz + lz];
-        const T ps = g.be[2][k] * (*ps_) + g.ae[2][k] * dhx_k;
-        *ps_ = ps;
-        v = v + g.cpsi[CB_EYHX][m * nm1z + k] * ps;
+        const T ps = q.cp.bz * p.ps[P_BZ] + q.cp.az * dhx_k;
+        g.psi_e[P_BZ][psi_z(i, j, lz, nm1y, Lz)] = ps;
+        v = v + tt.pk1[mk] * ps;
       }
     }
     if (yee || slab) sto(q.Ey, o, v);
@@ -320,20 +423,30 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
     const bool slab = (k < nz) && (lx >= 0 || ly >= 0);
     if (slab) {
       if (lx >= 0) {
-        T *ps_ = &g.psi_e[P_CX][((long long)lx * nm1y + j) * g.pz + k];
-        const T ps = g.be[0][i] * (*ps_) + g.ae[0][i] * dhy_i;
-        *ps_ = ps;
-        v = v + g.cpsi[CB_EZHY][m * nm1x + i] * ps;
+        const T ps = p.bx * p.ps[P_CX] + p.ax * dhy_i;
+        g.psi_e[P_CX][psi_x(lx, j, k, nm1y, g.pz)] = ps;
+        v = v + tt.pi1[mi] * ps;
       }
       if (ly >= 0) {
-        T *ps_ = &g.psi_e[P_CY][((long long)i * Ly + ly) * g.pz + k];
-        const T ps = g.be[1][j] * (*ps_) + g.ae[1][j] * dhx_j;
-        *ps_ = ps;
-        v = v + g.cpsi[CB_EZHX][m * nm1y + j] * ps;
+        const T ps = q.cp.by * p.ps[P_CY] + q.cp.ay * dhx_j;
+        g.psi_e[P_CY][psi_y(i, ly, k, Ly, g.pz)] = ps;
+        v = v + tt.pj1[mj] * ps;
       }
     }
     if (yee || slab) sto(q.Ez, o, v);
   }
+}
+
+// block-uniform test: does this tile meet a CPML slab along each axis?
+template <typename T>
+__device__ __forceinline__ void tile_slabs(const Grid<T> &g, const Tiling &tl, int bz, const Cell &cell, int i0,
+                                           int i1, bool &sk, bool &sj, bool &si) {
+  if (tl.ty == 0)
+    sk = g.dn[2] + g.dp[2] > 0;
+  else
+    sk = meets_slab(bz * tl.tz, bz * tl.tz + tl.tz - 1, g.nz, g.dn[2], g.dp[2]);
+  sj = meets_slab(cell.j_first, cell.j_first + tl.jrows - 1, g.ny, g.dn[1], g.dp[1]);
+  si = meets_slab(i0, i1 - 1, g.nx, g.dn[0], g.dp[0]);
 }
 
 template <typename T, bool LDS, bool PF>
@@ -346,52 +459,65 @@ __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl) {
   const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
   const int i0 = bx * tl.tx;
   const int i1 = min(i0 + tl.tx, nx + 1);
-  TileTables<T> tt;
-  if (LDS)
-    tt = stage_tables<T>(reinterpret_cast<T *>(lds_raw), g.ca, EX, nmat, g.cb[CB_EXHY], g.cb[CB_EYHX],
-                         nz + 1, g.cb[CB_EXHZ], g.cb[CB_EZHX], ny + 1, g.cb[CB_EYHZ], g.cb[CB_EZHY],
-                         nx + 1, tl, cell, i0);
-  else
-    tt = global_tables<T>(g.ca, EX, nmat, g.cb[CB_EXHY], g.cb[CB_EYHX], nz + 1, g.cb[CB_EXHZ],
-                          g.cb[CB_EZHX], ny + 1, g.cb[CB_EYHZ], g.cb[CB_EZHY], nx + 1, cell);
-  if (k > nz || j > ny) return;
+  const bool live = k <= nz && j <= ny;
   ColE<T> q;
   q.Ex = g.f[EX]; q.Ey = g.f[EY]; q.Ez = g.f[EZ];
   q.Hx = g.f[HX]; q.Hy = g.f[HY]; q.Hz = g.f[HZ];
   q.mat = g.mat;
-  q.sj = g.sj;
   q.bsj = (unsigned)(g.sj * (long long)sizeof(T));
   q.k1 = k >= 1;
   q.j1 = j >= 1;
-  const int ly = e_layer(j, ny, g.dn[1], g.dp[1]);
-  const int lz = e_layer(k, nz, g.dn[2], g.dp[2]);
   const bool kin = (k >= 1 && k < nz), jin = (j >= 1 && j < ny);
   long long c = (long long)i0 * g.si + (long long)j * g.sj + k;
   // x-neighbours are carried in registers: Hz(i-1), Hy(i-1)
-  T hz_prev = (i0 >= 1) ? ldo(q.Hz, (unsigned)(c - g.si) * (unsigned)sizeof(T)) : T(0);
-  T hy_prev = (i0 >= 1) ? ldo(q.Hy, (unsigned)(c - g.si) * (unsigned)sizeof(T)) : T(0);
+  T hz_prev = T(0), hy_prev = T(0);
+  PlaneE<T> A, B;
+  // the first plane's loads go out before the tables are staged
+  if (live) {
+    q.cp = col_cpml(g, g.be, g.ae, e_layer(j, ny, g.dn[1], g.dp[1]), e_layer(k, nz, g.dn[2], g.dp[2]), j, k);
+    if (i0 >= 1) {
+      hz_prev = ldo(q.Hz, (unsigned)(c - g.si) * (unsigned)sizeof(T));
+      hy_prev = ldo(q.Hy, (unsigned)(c - g.si) * (unsigned)sizeof(T));
+    }
+    load_plane_e(g, q, c, i0, j, k, A);
+  }
+  TableSrc<T> src;
+  src.ca_base = EX;
+  src.k0 = g.cb[CB_EXHY]; src.k1 = g.cb[CB_EYHX];
+  src.j0 = g.cb[CB_EXHZ]; src.j1 = g.cb[CB_EZHX];
+  src.i0 = g.cb[CB_EYHZ]; src.i1 = g.cb[CB_EZHY];
+  src.pk0 = g.cpsi[CB_EXHY]; src.pk1 = g.cpsi[CB_EYHX];
+  src.pj0 = g.cpsi[CB_EXHZ]; src.pj1 = g.cpsi[CB_EZHX];
+  src.pi0 = g.cpsi[CB_EYHZ]; src.pi1 = g.cpsi[CB_EZHY];
+  src.nk1 = nz + 1; src.nj1 = ny + 1; src.ni1 = nx + 1;
+  TileTables<T> tt;
+  if (LDS) {
+    bool sk, sj, si;
+    tile_slabs(g, tl, bz, cell, i0, i1, sk, sj, si);
+    tt = stage_tables<T>(reinterpret_cast<T *>(lds_raw), g.ca, nmat, src, tl, cell, i0, sk, sj, si);
+  } else {
+    tt = global_tables<T>(g.ca, nmat, src, cell);
+  }
+  if (!live) return;
   if (!PF) {
-    // one plane at a time: its loads are issued together, latency is hidden
-    // by the other waves on the CU
-    for (int i = i0; i < i1; ++i, c += g.si) {
-      PlaneE<T> A;
-      load_plane_e(q, c, A);
-      compute_plane_e(g, tt, q, A, i, j, k, c, ly, lz, jin, kin, hz_prev, hy_prev);
+    for (int i = i0;;) {
+      compute_plane_e(g, tt, q, A, i, j, k, c, jin, kin, hz_prev, hy_prev);
       hz_prev = A.hz;
       hy_prev = A.hy;
+      if (++i >= i1) break;
+      c += g.si;
+      load_plane_e(g, q, c, i, j, k, A);
     }
     return;
   }
-  PlaneE<T> A, B;
-  load_plane_e(q, c, A);
   for (int i = i0; i < i1; i += 2, c += 2 * g.si) {
-    if (i + 1 < i1) load_plane_e(q, c + g.si, B);
-    compute_plane_e(g, tt, q, A, i, j, k, c, ly, lz, jin, kin, hz_prev, hy_prev);
+    if (i + 1 < i1) load_plane_e(g, q, c + g.si, i + 1, j, k, B);
+    compute_plane_e(g, tt, q, A, i, j, k, c, jin, kin, hz_prev, hy_prev);
     hz_prev = A.hz;
     hy_prev = A.hy;
     if (i + 1 >= i1) break;
-    if (i + 2 < i1) load_plane_e(q, c + 2 * g.si, A);
-    compute_plane_e(g, tt, q, B, i + 1, j, k, c + g.si, ly, lz, jin, kin, hz_prev, hy_prev);
+    if (i + 2 < i1) load_plane_e(g, q, c + 2 * g.si, i + 2, j, k, A);
+    compute_plane_e(g, tt, q, B, i + 1, j, k, c + g.si, jin, kin, hz_prev, hy_prev);
     hz_prev = B.hz;
     hy_prev = B.hy;
   }
@@ -399,12 +525,15 @@ __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl) {
 
 // ---------------------------------------------------------------------------
 // H half-step: Simulation::updateH (simulation_update.cpp:84-100) fused with
-// PML::updateH (perfect_match_layer.cpp:434-572).  Same A/B plane buffers;
-// Ez(i), Ey(i) are the (i+1) values loaded for the previous plane.
+// PML::updateH (perfect_match_layer.cpp:434-572).  Same plane scheme as the
+// E kernel; Ez(i), Ey(i) are the (i+1) values loaded for the previous plane.
 template <typename T>
 struct PlaneH {
   unsigned m;
   T hx, hy, hz, ex, eyk, exk, ezj, exj, ezi, eyi;
+  int lx;
+  T bx, ax;
+  T ps[6];
 };
 
 template <typename T>
@@ -416,14 +545,14 @@ struct ColH {
   T *__restrict__ Hy;
   T *__restrict__ Hz;
   const uint8_t *__restrict__ mat;
-  long long sj, si;
   unsigned bsj, bsi;  // byte strides of j and i
-  bool kin, jin;
-  int nx;
+  bool kin, jin;      // k < nz, j < ny
+  ColCpml<T> cp;
 };
 
 template <typename T>
-__device__ __forceinline__ void load_plane_h(const ColH<T> &q, long long c, int i, PlaneH<T> &p) {
+__device__ __forceinline__ void load_plane_h(const Grid<T> &g, const ColH<T> &q, long long c, int i, int j,
+                                             int k, PlaneH<T> &p) {
   const unsigned cell = (unsigned)c, o = cell * (unsigned)sizeof(T);
   p.m = ldm(q.mat, cell);
   p.hx = ldo(q.Hx, o);
@@ -434,20 +563,42 @@ __device__ __forceinline__ void load_plane_h(const ColH<T> &q, long long c, int 
   p.exk = q.kin ? ldo(q.Ex, o + (unsigned)sizeof(T)) : T(0);  // Ex(k+1)
   p.ezj = q.jin ? ldo(q.Ez, o + q.bsj) : T(0);                // Ez(j+1)
   p.exj = q.jin ? ldo(q.Ex, o + q.bsj) : T(0);                // Ex(j+1)
-  p.ezi = (i < q.nx) ? ldo(q.Ez, o + q.bsi) : T(0);           // Ez(i+1)
-  p.eyi = (i < q.nx) ? ldo(q.Ey, o + q.bsi) : T(0);           // Ey(i+1)
+  p.ezi = (i < g.nx) ? ldo(q.Ez, o + q.bsi) : T(0);           // Ez(i+1)
+  p.eyi = (i < g.nx) ? ldo(q.Ey, o + q.bsi) : T(0);           // Ey(i+1)
+  p.lx = h_layer(i, g.nx, g.dn[0], g.dp[0]);
+  p.bx = p.ax = T(0);
+  for (int s = 0; s < 6; ++s) p.ps[s] = T(0);
+  const int ly = q.cp.ly, lz = q.cp.lz, lx = p.lx;
+  if (lx < 0 && ly < 0 && lz < 0) return;
+  const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = g.ny + 1;
+  if (lx >= 0) {
+    p.bx = g.bm[0][i];
+    p.ax = g.am[0][i];
+  }
+  if (q.jin && q.kin) {
+    if (ly >= 0) p.ps[P_AY] = g.psi_h[P_AY][psi_y(i, ly, k, Ly, g.pz)];
+    if (lz >= 0) p.ps[P_AZ] = g.psi_h[P_AZ][psi_z(i, j, lz, nm1y, Lz)];
+  }
+  if (i < g.nx && q.kin) {
+    if (lx >= 0) p.ps[P_BX] = g.psi_h[P_BX][psi_x(lx, j, k, nm1y, g.pz)];
+    if (lz >= 0) p.ps[P_BZ] = g.psi_h[P_BZ][psi_z(i, j, lz, nm1y, Lz)];
+  }
+  if (i < g.nx && q.jin) {
+    if (lx >= 0) p.ps[P_CX] = g.psi_h[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
+    if (ly >= 0) p.ps[P_CY] = g.psi_h[P_CY][psi_y(i, ly, k, Ly, g.pz)];
+  }
 }
 
 template <typename T>
 __device__ __forceinline__ void compute_plane_h(const Grid<T> &g, const TileTables<T> &tt, const ColH<T> &q,
                                                 const PlaneH<T> &p, int i, int j, int k, long long c,
-                                                int ly, int lz, T ez_c, T ey_c) {
-  const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
-  const int nm1y = ny + 1, nm1z = nz + 1, nm1x = nx + 1;
+                                                T ez_c, T ey_c) {
+  const int nx = g.nx, ny = g.ny, nmat = g.nmat;
+  const int nm1y = ny + 1;
   const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2];
   const bool kin = q.kin, jin = q.jin;
   const int m = (int)p.m;
-  const int lx = h_layer(i, nx, g.dn[0], g.dp[0]);
+  const int lx = p.lx, ly = q.cp.ly, lz = q.cp.lz;
   const T dey_k = p.eyk - ey_c;  // Ey(k+1) - Ey(k)
   const T dez_j = p.ezj - ez_c;  // Ez(j+1) - Ez(j)
   const T dez_i = p.ezi - ez_c;  // Ez(i+1) - Ez(i)
@@ -467,16 +618,14 @@ __device__ __forceinline__ void compute_plane_h(const Grid<T> &g, const TileTabl
     const bool slab = in && (ly >= 0 || lz >= 0);
     if (slab) {
       if (ly >= 0) {
-        T *ps_ = &g.psi_h[P_AY][((long long)i * Ly + ly) * g.pz + k];
-        const T ps = g.bm[1][j] * (*ps_) + g.am[1][j] * dez_j;
-        *ps_ = ps;
-        v = v + g.cpsi[CB_HXEZ][m * nm1y + j] * ps;
+        const T ps = q.cp.by * p.ps[P_AY] + q.cp.ay * dez_j;
+        g.psi_h[P_AY][psi_y(i, ly, k, Ly, g.pz)] = ps;
+        v = v + tt.pj0[mj] * ps;
       }
       if (lz >= 0) {
-        T *ps_ = &g.psi_h[P_AZ][((long long)i * nm1y + j) * Lz + lz];
-        const T ps = g.bm[2][k] * (*ps_) + g.am[2][k] * dey_k;
-        *ps_ = ps;
-        v = v + g.cpsi[CB_HXEY][m * nm1z + k] * ps;
+        const T ps = q.cp.bz * p.ps[P_AZ] + q.cp.az * dey_k;
+        g.psi_h[P_AZ][psi_z(i, j, lz, nm1y, Lz)] = ps;
+        v = v + tt.pk0[mk] * ps;
       }
     }
     if (yee || slab) sto(q.Hx, o, v);
@@ -489,16 +638,14 @@ __device__ __forceinline__ void compute_plane_h(const Grid<T> &g, const TileTabl
     const bool slab = in && (lx >= 0 || lz >= 0);
     if (slab) {
       if (lx >= 0) {
-        T *ps_ = &g.psi_h[P_BX][((long long)lx * nm1y + j) * g.pz + k];
-        const T ps = g.bm[0][i] * (*ps_) + g.am[0][i] * dez_i;
-        *ps_ = ps;
-        v = v + g.cpsi[CB_HYEZ][m * nm1x + i] * ps;
+        const T ps = p.bx * p.ps[P_BX] + p.ax * dez_i;
+        g.psi_h[P_BX][psi_x(lx, j, k, nm1y, g.pz)] = ps;
+        v = v + tt.pi0[mi] * ps;
       }
       if (lz >= 0) {
-        T *ps_ = &g.psi_h[P_BZ][((long long)i * nm1y + j) * Lz + lz];
-        const T ps = g.bm[2][k] * (*ps_) + g.am[2][k] * dex_k;
-        *ps_ = ps;
-        v = v + g.cpsi[CB_HYEX][m * nm1z + k] * ps;
+        const T ps = q.cp.bz * p.ps[P_BZ] + q.cp.az * dex_k;
+        g.psi_h[P_BZ][psi_z(i, j, lz, nm1y, Lz)] = ps;
+        v = v + tt.pk1[mk] * ps;
       }
     }
     if (yee || slab) sto(q.Hy, o, v);
@@ -511,16 +658,14 @@ __device__ __forceinline__ void compute_plane_h(const Grid<T> &g, const TileTabl
     const bool slab = in && (lx >= 0 || ly >= 0);
     if (slab) {
       if (lx >= 0) {
-        T *ps_ = &g.psi_h[P_CX][((long long)lx * nm1y + j) * g.pz + k];
-        const T ps = g.bm[0][i] * (*ps_) + g.am[0][i] * dey_i;
-        *ps_ = ps;
-        v = v + g.cpsi[CB_HZEY][m * nm1x + i] * ps;
+        const T ps = p.bx * p.ps[P_CX] + p.ax * dey_i;
+        g.psi_h[P_CX][psi_x(lx, j, k, nm1y, g.pz)] = ps;
+        v = v + tt.pi1[mi] * ps;
       }
       if (ly >= 0) {
-        T *ps_ = &g.psi_h[P_CY][((long long)i * Ly + ly) * g.pz + k];
-        const T ps = g.bm[1][j] * (*ps_) + g.am[1][j] * dex_j;
-        *ps_ = ps;
-        v = v + g.cpsi[CB_HZEX][m * nm1y + j] * ps;
+        const T ps = q.cp.by * p.ps[P_CY] + q.cp.ay * dex_j;
+        g.psi_h[P_CY][psi_y(i, ly, k, Ly, g.pz)] = ps;
+        v = v + tt.pj1[mj] * ps;
       }
     }
     if (yee || slab) sto(q.Hz, o, v);
@@ -537,50 +682,62 @@ __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl) {
   const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
   const int i0 = bx * tl.tx;
   const int i1 = min(i0 + tl.tx, nx + 1);
-  TileTables<T> tt;
-  if (LDS)
-    tt = stage_tables<T>(reinterpret_cast<T *>(lds_raw), g.ca, HX, nmat, g.cb[CB_HXEY], g.cb[CB_HYEX],
-                         nz + 1, g.cb[CB_HXEZ], g.cb[CB_HZEX], ny + 1, g.cb[CB_HYEZ], g.cb[CB_HZEY],
-                         nx + 1, tl, cell, i0);
-  else
-    tt = global_tables<T>(g.ca, HX, nmat, g.cb[CB_HXEY], g.cb[CB_HYEX], nz + 1, g.cb[CB_HXEZ],
-                          g.cb[CB_HZEX], ny + 1, g.cb[CB_HYEZ], g.cb[CB_HZEY], nx + 1, cell);
-  if (k > nz || j > ny) return;
+  const bool live = k <= nz && j <= ny;
   ColH<T> q;
   q.Ex = g.f[EX]; q.Ey = g.f[EY]; q.Ez = g.f[EZ];
   q.Hx = g.f[HX]; q.Hy = g.f[HY]; q.Hz = g.f[HZ];
   q.mat = g.mat;
-  q.sj = g.sj;
-  q.si = g.si;
   q.bsj = (unsigned)(g.sj * (long long)sizeof(T));
   q.bsi = (unsigned)(g.si * (long long)sizeof(T));
   q.kin = k < nz;
   q.jin = j < ny;
-  q.nx = nx;
-  const int ly = h_layer(j, ny, g.dn[1], g.dp[1]);
-  const int lz = h_layer(k, nz, g.dn[2], g.dp[2]);
   long long c = (long long)i0 * g.si + (long long)j * g.sj + k;
-  T ez_c = ldo(q.Ez, (unsigned)c * (unsigned)sizeof(T)), ey_c = ldo(q.Ey, (unsigned)c * (unsigned)sizeof(T));
+  T ez_c = T(0), ey_c = T(0);
+  PlaneH<T> A, B;
+  // the first plane's loads go out before the tables are staged
+  if (live) {
+    q.cp = col_cpml(g, g.bm, g.am, h_layer(j, ny, g.dn[1], g.dp[1]), h_layer(k, nz, g.dn[2], g.dp[2]), j, k);
+    ez_c = ldo(q.Ez, (unsigned)c * (unsigned)sizeof(T));
+    ey_c = ldo(q.Ey, (unsigned)c * (unsigned)sizeof(T));
+    load_plane_h(g, q, c, i0, j, k, A);
+  }
+  TableSrc<T> src;
+  src.ca_base = HX;
+  src.k0 = g.cb[CB_HXEY]; src.k1 = g.cb[CB_HYEX];
+  src.j0 = g.cb[CB_HXEZ]; src.j1 = g.cb[CB_HZEX];
+  src.i0 = g.cb[CB_HYEZ]; src.i1 = g.cb[CB_HZEY];
+  src.pk0 = g.cpsi[CB_HXEY]; src.pk1 = g.cpsi[CB_HYEX];
+  src.pj0 = g.cpsi[CB_HXEZ]; src.pj1 = g.cpsi[CB_HZEX];
+  src.pi0 = g.cpsi[CB_HYEZ]; src.pi1 = g.cpsi[CB_HZEY];
+  src.nk1 = nz + 1; src.nj1 = ny + 1; src.ni1 = nx + 1;
+  TileTables<T> tt;
+  if (LDS) {
+    bool sk, sj, si;
+    tile_slabs(g, tl, bz, cell, i0, i1, sk, sj, si);
+    tt = stage_tables<T>(reinterpret_cast<T *>(lds_raw), g.ca, nmat, src, tl, cell, i0, sk, sj, si);
+  } else {
+    tt = global_tables<T>(g.ca, nmat, src, cell);
+  }
+  if (!live) return;
   if (!PF) {
-    for (int i = i0; i < i1; ++i, c += g.si) {
-      PlaneH<T> A;
-      load_plane_h(q, c, i, A);
-      compute_plane_h(g, tt, q, A, i, j, k, c, ly, lz, ez_c, ey_c);
+    for (int i = i0;;) {
+      compute_plane_h(g, tt, q, A, i, j, k, c, ez_c, ey_c);
       ez_c = A.ezi;
       ey_c = A.eyi;
+      if (++i >= i1) break;
+      c += g.si;
+      load_plane_h(g, q, c, i, j, k, A);
     }
     return;
   }
-  PlaneH<T> A, B;
-  load_plane_h(q, c, i0, A);
   for (int i = i0; i < i1; i += 2, c += 2 * g.si) {
-    if (i + 1 < i1) load_plane_h(q, c + g.si, i + 1, B);
-    compute_plane_h(g, tt, q, A, i, j, k, c, ly, lz, ez_c, ey_c);
+    if (i + 1 < i1) load_plane_h(g, q, c + g.si, i + 1, j, k, B);
+    compute_plane_h(g, tt, q, A, i, j, k, c, ez_c, ey_c);
     ez_c = A.ezi;
     ey_c = A.eyi;
     if (i + 1 >= i1) break;
-    if (i + 2 < i1) load_plane_h(q, c + 2 * g.si, i + 2, A);
-    compute_plane_h(g, tt, q, B, i + 1, j, k, c + g.si, ly, lz, ez_c, ey_c);
+    if (i + 2 < i1) load_plane_h(g, q, c + 2 * g.si, i + 2, j, k, A);
+    compute_plane_h(g, tt, q, B, i + 1, j, k, c + g.si, ez_c, ey_c);
     ez_c = B.ezi;
     ey_c = B.eyi;
   }
