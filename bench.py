@@ -15,7 +15,7 @@ the dominant curl kernel (18*s/2 + 1 bytes per cell per half step: read 3 E +
 duration inside the timed region; traffic: the HBM bytes per launch of that
 kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by the same
 counters' reading for a plain 1 GiB copy (profiles/hbm_traffic.json).
-cpu_baseline: the FP64 CPU oracle, one thread, on two steps of the same 400^3
+cpu_baseline: the FP64 CPU oracle, one thread, on six steps of the same 400^3
 run (set-up excluded); cpu_baseline_threads: the same with every host core.
 """
 from __future__ import annotations
@@ -114,7 +114,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--size", type=int, default=400)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--cpu-threads", type=int, default=16,
                     help="threads of the second, multi-core CPU column (0: skip it)")
     ap.add_argument("--no-cpu", action="store_true")

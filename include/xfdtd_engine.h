@@ -178,7 +178,13 @@ xf_status xf_engine_set_timing(xf_engine *e, int enabled);
    orders tiles along x first, XF_TILE_XCD_RUNS gives each XCD one contiguous
    run of tiles instead of handing them out in workgroup-id order.  Every
    geometry gives identical results.  Default: 64, 4, 8, 0. */
-enum { XF_TILE_PREFETCH = 1, XF_TILE_GLOBAL_TABLES = 2, XF_TILE_X_FASTEST = 4, XF_TILE_XCD_RUNS = 8 };
+enum {
+  XF_TILE_PREFETCH = 1,
+  XF_TILE_GLOBAL_TABLES = 2,
+  XF_TILE_X_FASTEST = 4,
+  XF_TILE_XCD_RUNS = 8,
+  XF_TILE_XCD_COLS = 16
+};
 xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags);
 
 #ifdef __cplusplus

@@ -171,7 +171,7 @@ def test_step_beyond_horizon_is_refused():
 
 @pytest.mark.parametrize("tiling", [(64, 4, 16, 0), (256, 0, 16, 0), (64, 0, 3, 2), (128, 2, 5, 1),
                                     (64, 1, 0, 0), (256, 0, 1, 6), (64, 4, 2, 4), (64, 4, 8, 8),
-                                    (128, 2, 3, 9)])
+                                    (128, 2, 3, 9), (64, 4, 8, 16), (64, 2, 5, 17)])
 def test_launch_tilings_bitwise(tiling, oracle_threads):
     """Every launch geometry of the curl kernels (2-D tiles, flat plane runs,
     double-buffered planes, whole-axis marches) gives the same bits."""

@@ -512,6 +512,8 @@ static xf_status run_steps(xf_engine *e, long long n) {
   }
   tl.ntx = (e->nx + 1 + tl.tx - 1) / tl.tx;
   tl.nblocks = tl.ntz * tl.nty * tl.ntx;
+  tl.xcd_cols = (e->tile_flags & XF_TILE_XCD_COLS) != 0;
+  if (tl.xcd_cols) tl.nblocks = 8 * tl.ntz * ((tl.nty * tl.ntx + 7) / 8);
   tl.x_fastest = (e->tile_flags & XF_TILE_X_FASTEST) != 0;
   tl.xcd_runs = (e->tile_flags & XF_TILE_XCD_RUNS) != 0;
   dim3 blk(tl.ty == 0 ? tl.tz : tl.tz * tl.ty);
@@ -680,7 +682,7 @@ extern "C" xf_status xf_engine_set_timing(xf_engine *e, int enabled) {
 
 extern "C" xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags) {
   if (!e) FAILS(XF_EINVAL, "null engine");
-  if (tz < 64 || tz % 64 != 0 || ty < 0 || tx < 0 || (ty > 0 ? tz * ty : tz) > 256 || flags < 0 || flags > 15)
+  if (tz < 64 || tz % 64 != 0 || ty < 0 || tx < 0 || (ty > 0 ? tz * ty : tz) > 256 || flags < 0 || flags > 31)
     FAILS(XF_EINVAL, "tiling %d x %d x %d (flags %d): need tz a multiple of 64 and at most 256 threads", tz,
           ty, tx, flags);
   e->tz = tz; e->ty = ty; e->tx = tx;

@@ -68,24 +68,34 @@ struct Tiling {
   int krows, jrows;         // coefficient-table slots of a tile along k and j
   int x_fastest;            // tile order: x chunks first (else z, y, x)
   int xcd_runs;             // one contiguous run of tile ids per XCD
+  int xcd_cols;             // all k tiles of a (j, i) column on one XCD
 };
 
-__device__ __forceinline__ void tile_of_block(const Tiling &t, int &bz, int &by, int &bx) {
+// tile of this workgroup; false for a padding workgroup of the column order
+__device__ __forceinline__ bool tile_of_block(const Tiling &t, int &bz, int &by, int &bx) {
   const int b = blockIdx.x;
   const int q = t.nblocks >> 3, r = t.nblocks & 7;
   const int xcd = b & 7, local = b >> 3;
+  if (t.xcd_cols) {
+    bz = local % t.ntz;
+    const int col = (local / t.ntz) * 8 + xcd;
+    by = col % t.nty;
+    bx = col / t.nty;
+    return bx < t.ntx;
+  }
   const int id = t.xcd_runs ? xcd * q + min(xcd, r) + local : b;
   if (t.x_fastest) {
     bx = id % t.ntx;
     const int rest = id / t.ntx;
     bz = rest % t.ntz;
     by = rest / t.ntz;
-    return;
+    return true;
   }
   bz = id % t.ntz;
   const int rest = id / t.ntz;
   by = rest % t.nty;
   bx = rest / t.nty;
+  return true;
 }
 
 // Cell of a thread.  Two tile shapes: a TZ x TY block of (k, j), or (TY = 0)
@@ -453,7 +463,7 @@ template <typename T, bool LDS, bool PF>
 __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl) {
   extern __shared__ unsigned char lds_raw[];
   int bz, by, bx;
-  tile_of_block(tl, bz, by, bx);
+  if (!tile_of_block(tl, bz, by, bx)) return;
   const Cell cell = thread_cell(tl, bz, by, g.pz);
   const int k = cell.k, j = cell.j;
   const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
@@ -676,7 +686,7 @@ template <typename T, bool LDS, bool PF>
 __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl) {
   extern __shared__ unsigned char lds_raw[];
   int bz, by, bx;
-  tile_of_block(tl, bz, by, bx);
+  if (!tile_of_block(tl, bz, by, bx)) return;
   const Cell cell = thread_cell(tl, bz, by, g.pz);
   const int k = cell.k, j = cell.j;
   const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
