@@ -173,18 +173,13 @@ xf_status xf_engine_set_timing(xf_engine *e, int enabled);
 /* launch geometry of the curl kernels: tz threads along z per block
    (multiple of 64), ty rows along y (0 = flat runs of tz cells over the
    padded (y,z) plane), tx x-planes marched per thread (0 = whole axis), and
-   flags: XF_TILE_PREFETCH double-buffers planes, XF_TILE_GLOBAL_TABLES reads
-   coefficient tables from global memory instead of LDS, XF_TILE_X_FASTEST
-   orders tiles along x first, XF_TILE_XCD_RUNS gives each XCD one contiguous
-   run of tiles instead of handing them out in workgroup-id order.  Every
-   geometry gives identical results.  Default: 64, 4, 8, 0. */
-enum {
-  XF_TILE_PREFETCH = 1,
-  XF_TILE_GLOBAL_TABLES = 2,
-  XF_TILE_X_FASTEST = 4,
-  XF_TILE_XCD_RUNS = 8,
-  XF_TILE_XCD_COLS = 16
-};
+   flags: XF_TILE_GLOBAL_TABLES reads coefficient tables from global memory
+   instead of LDS, XF_TILE_X_FASTEST orders tiles along x first,
+   XF_TILE_XCD_RUNS gives each XCD one contiguous run of tiles instead of
+   handing them out in workgroup-id order, XF_TILE_XCD_COLS keeps all k tiles
+   of a (j, i) column on one XCD.  Every geometry gives identical results.
+   Default: 64, 4, 8, 0. */
+enum { XF_TILE_GLOBAL_TABLES = 2, XF_TILE_X_FASTEST = 4, XF_TILE_XCD_RUNS = 8, XF_TILE_XCD_COLS = 16 };
 xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags);
 
 #ifdef __cplusplus

@@ -53,8 +53,8 @@ struct xf_engine {
   hipStream_t stream = nullptr;
   void *f[6] = {};
   uint8_t *mat = nullptr;
-  void *ca = nullptr;
-  void *cb[12] = {}, *cpsi[12] = {};
+  // coefficient records per side (0: E, 1: H), see Grid in engine_kernels.hpp
+  void *self[2] = {}, *kt[2] = {}, *jt[2] = {}, *it[2] = {}, *kp[2] = {}, *jp[2] = {}, *ip[2] = {};
   bool materials_set = false;
   int dn[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
   void *be[3] = {}, *ae[3] = {}, *bm[3] = {}, *am[3] = {};
@@ -202,15 +202,40 @@ extern "C" xf_status xf_engine_set_materials(xf_engine *e, const uint8_t *index,
   if (s != XF_OK) return s;
   HIPCHK(hipMemcpy(pm, m.data(), (size_t)e->ncell, hipMemcpyHostToDevice));
   e->mat = (uint8_t *)pm;
-  if ((s = upload_real(e, &e->ca, ca, 6 * (size_t)nm)) != XF_OK) return s;
-  // table t: derivative axis = z for 0,3,6,9; y for 1,4,7,10; x for 2,5,8,11
-  const int axis_of[12] = {2, 1, 0, 2, 1, 0, 2, 1, 0, 2, 1, 0};
   const int nax[3] = {nx, ny, nz};
-  for (int t = 0; t < 12; ++t) {
-    size_t n = (size_t)nm * (nax[axis_of[t]] + 1);
-    if ((s = upload_real(e, &e->cb[t], cb[t], n)) != XF_OK) return s;
-    if ((s = upload_real(e, &e->cpsi[t], cpsi[t], n)) != XF_OK) return s;
+  // self terms of each side as 4-wide records [nmat][4]
+  for (int side = 0; side < 2; ++side) {
+    std::vector<double> rec(4 * (size_t)nm, 0.0);
+    for (int m = 0; m < nm; ++m)
+      for (int c = 0; c < 3; ++c) rec[4 * (size_t)m + c] = ca[(3 * side + c) * (size_t)nm + m];
+    if ((s = upload_real(e, &e->self[side], rec.data(), rec.size())) != XF_OK) return s;
   }
+  // per side and axis, the two tables whose derivative runs along that axis,
+  // interleaved as [n][material][2] (the kernels read both with one load):
+  //   E: z (cexhy, ceyhx), y (cexhz, cezhx), x (ceyhz, cezhy)
+  //   H: z (chxey, chyex), y (chxez, chzex), x (chyez, chzey)
+  static const int pairs[2][3][2] = {
+      {{XF_CB_EXHY, XF_CB_EYHX}, {XF_CB_EXHZ, XF_CB_EZHX}, {XF_CB_EYHZ, XF_CB_EZHY}},
+      {{XF_CB_HXEY, XF_CB_HYEX}, {XF_CB_HXEZ, XF_CB_HZEX}, {XF_CB_HYEZ, XF_CB_HZEY}}};
+  static const int pair_axis[3] = {2, 1, 0};
+  for (int side = 0; side < 2; ++side)
+    for (int q = 0; q < 3; ++q) {
+      const int ax = pair_axis[q], n1 = nax[ax] + 1;
+      const int t0 = pairs[side][q][0], t1 = pairs[side][q][1];
+      std::vector<double> cp(2 * (size_t)n1 * nm), pp(cp.size());
+      for (int n = 0; n < n1; ++n)
+        for (int m = 0; m < nm; ++m) {
+          const size_t d = 2 * ((size_t)n * nm + m), src = (size_t)m * n1 + n;
+          cp[d] = cb[t0][src];
+          cp[d + 1] = cb[t1][src];
+          pp[d] = cpsi[t0][src];
+          pp[d + 1] = cpsi[t1][src];
+        }
+      void **dc = q == 0 ? &e->kt[side] : q == 1 ? &e->jt[side] : &e->it[side];
+      void **dp = q == 0 ? &e->kp[side] : q == 1 ? &e->jp[side] : &e->ip[side];
+      if ((s = upload_real(e, dc, cp.data(), cp.size())) != XF_OK) return s;
+      if ((s = upload_real(e, dp, pp.data(), pp.size())) != XF_OK) return s;
+    }
   // default (absent) CPML tables
   for (int a = 0; a < 3; ++a) {
     if (!e->be[a]) {
@@ -451,10 +476,14 @@ static Grid<T> make_grid(xf_engine *e) {
   Grid<T> g;
   for (int q = 0; q < 6; ++q) g.f[q] = (T *)e->f[q];
   g.mat = e->mat;
-  g.ca = (const T *)e->ca;
-  for (int q = 0; q < 12; ++q) {
-    g.cb[q] = (const T *)e->cb[q];
-    g.cpsi[q] = (const T *)e->cpsi[q];
+  for (int side = 0; side < 2; ++side) {
+    g.self[side] = (const T *)e->self[side];
+    g.kt[side] = (const T *)e->kt[side];
+    g.jt[side] = (const T *)e->jt[side];
+    g.it[side] = (const T *)e->it[side];
+    g.kp[side] = (const T *)e->kp[side];
+    g.jp[side] = (const T *)e->jp[side];
+    g.ip[side] = (const T *)e->ip[side];
   }
   for (int a = 0; a < 3; ++a) {
     g.dn[a] = e->dn[a]; g.dp[a] = e->dp[a];
@@ -521,7 +550,6 @@ static xf_status run_steps(xf_engine *e, long long n) {
   // coefficient tables of a tile go to LDS when they fit in 32 KB
   const size_t lds = lds_table_bytes<T>(e->nmat, tl);
   const bool use_lds = lds <= 32768 && !(e->tile_flags & XF_TILE_GLOBAL_TABLES);
-  const bool prefetch = (e->tile_flags & XF_TILE_PREFETCH) != 0;
   for (long long q = 0; q < n; ++q) {
     const long long step = e->cur;
     hipEvent_t *ev = nullptr;
@@ -537,12 +565,10 @@ static xf_status run_steps(xf_engine *e, long long n) {
       e->ev_used += 4;
     }
     if (ev) hipEventRecord(ev[0], e->stream);
-    if (use_lds && prefetch)
-      hipLaunchKernelGGL((k_update_e<T, true, true>), grd, blk, lds, e->stream, g, tl);
-    else if (use_lds)
-      hipLaunchKernelGGL((k_update_e<T, true, false>), grd, blk, lds, e->stream, g, tl);
+    if (use_lds)
+      hipLaunchKernelGGL((k_update_e<T, true>), grd, blk, lds, e->stream, g, tl);
     else
-      hipLaunchKernelGGL((k_update_e<T, false, false>), grd, blk, 0, e->stream, g, tl);
+      hipLaunchKernelGGL((k_update_e<T, false>), grd, blk, 0, e->stream, g, tl);
     if (ev) hipEventRecord(ev[1], e->stream);
     for (const SourceDev &sd : e->sources) {
       const int nj = sd.j1 - sd.j0, nk = sd.k1 - sd.k0;
@@ -557,12 +583,10 @@ static xf_status run_steps(xf_engine *e, long long n) {
                          e->nEz, e->nEx);
     }
     if (ev) hipEventRecord(ev[2], e->stream);
-    if (use_lds && prefetch)
-      hipLaunchKernelGGL((k_update_h<T, true, true>), grd, blk, lds, e->stream, g, tl);
-    else if (use_lds)
-      hipLaunchKernelGGL((k_update_h<T, true, false>), grd, blk, lds, e->stream, g, tl);
+    if (use_lds)
+      hipLaunchKernelGGL((k_update_h<T, true>), grd, blk, lds, e->stream, g, tl);
     else
-      hipLaunchKernelGGL((k_update_h<T, false, false>), grd, blk, 0, e->stream, g, tl);
+      hipLaunchKernelGGL((k_update_h<T, false>), grd, blk, 0, e->stream, g, tl);
     if (ev) hipEventRecord(ev[3], e->stream);
     if (e->has_tfsf) {
       hipLaunchKernelGGL(k_tfsf_ifa<T>, dim3(1), dim3(1024), 0, e->stream, t, step);
@@ -682,7 +706,7 @@ extern "C" xf_status xf_engine_set_timing(xf_engine *e, int enabled) {
 
 extern "C" xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags) {
   if (!e) FAILS(XF_EINVAL, "null engine");
-  if (tz < 64 || tz % 64 != 0 || ty < 0 || tx < 0 || (ty > 0 ? tz * ty : tz) > 256 || flags < 0 || flags > 31)
+  if (tz < 64 || tz % 64 != 0 || ty < 0 || tx < 0 || (ty > 0 ? tz * ty : tz) > 256 || (flags & ~30) != 0)
     FAILS(XF_EINVAL, "tiling %d x %d x %d (flags %d): need tz a multiple of 64 and at most 256 threads", tz,
           ty, tx, flags);
   e->tz = tz; e->ty = ty; e->tx = tx;

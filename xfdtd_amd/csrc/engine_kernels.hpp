@@ -30,9 +30,11 @@ template <typename T>
 struct Grid {
   T *f[6];
   const uint8_t *mat;
-  const T *ca;      // [6][nmat]
-  const T *cb[12];  // [nmat][n_axis+1]
-  const T *cpsi[12];
+  // coefficient records per side (0: E, 1: H): self terms [nmat][4]; curl
+  // and psi coefficient pairs per axis [n_axis+1][nmat][2]
+  const T *self[2];
+  const T *kt[2], *jt[2], *it[2];
+  const T *kp[2], *jp[2], *ip[2];
   // CPML per axis
   int dn[3], dp[3];
   const T *be[3], *ae[3], *bm[3], *am[3];
@@ -132,29 +134,32 @@ __device__ __forceinline__ Cell thread_cell(const Tiling &tl, int bz, int by, in
 // (fdtd_basic_coff.cpp:93-121), so a tile marching over TX planes needs, per
 // material: 3 self terms, 2 tables over its k slots, 2 over its j rows and 2
 // over its i planes.
+// Coefficient records.  For each half-step (side 0: E, 1: H) the engine
+// keeps, per material, the three self terms in one 4-wide record, and per
+// axis a table of pairs [n][material] holding the two curl coefficients of
+// the two components whose update differentiates along that axis (and the
+// same for the CPML psi coefficients), so one LDS read returns both.
 template <typename T>
-struct TileTables {
-  const T *ca;                                 // [3][nmat] self terms
-  const T *tk0, *tk1, *tj0, *tj1, *ti0, *ti1;  // curl coefficients [nmat][slots]
-  const T *pk0, *pk1, *pj0, *pj1, *pi0, *pi1;  // CPML psi coefficients, same slots
-  int sk, sj, si;                              // slots per material along k, j, i
-  int ks, js, i0;                              // this thread's k and j slots, tile's first plane
+struct alignas(2 * sizeof(T)) P2 {
+  T x, y;
+};
+template <typename T>
+struct alignas(4 * sizeof(T)) P4 {
+  T x, y, z, w;
 };
 
-// The global tables one curl kernel reads: three self terms from ca_base, and
-// per axis two curl tables and the two psi tables of the same components,
-// each [nmat][n + 1].
+// The tables of one tile: staged in LDS, or views of the global tables.
 template <typename T>
-struct TableSrc {
-  int ca_base;
-  const T *k0, *k1, *j0, *j1, *i0, *i1;
-  const T *pk0, *pk1, *pj0, *pj1, *pi0, *pi1;
-  int nk1, nj1, ni1;
+struct TileTables {
+  const P4<T> *self;          // [nmat]
+  const P2<T> *k, *j, *i;     // curl pairs [slot][nmat]
+  const P2<T> *pk, *pj, *pi;  // psi pairs [slot][nmat]
+  int kcol, jcol, i0;         // this thread's k slot * nmat, j slot * nmat; tile's first plane
 };
 
 template <typename T>
 __host__ __device__ inline size_t lds_table_bytes(int nmat, const Tiling &tl) {
-  return sizeof(T) * (size_t)nmat * (3 + 4 * tl.krows + 4 * tl.jrows + 4 * tl.tx);
+  return sizeof(T) * (size_t)nmat * (4 + 4 * (size_t)tl.krows + 4 * (size_t)tl.jrows + 4 * (size_t)tl.tx);
 }
 
 // does the index range [lo, hi] meet a CPML slab of an axis of N cells?  The
@@ -168,78 +173,84 @@ __device__ __forceinline__ bool meets_slab(int lo, int hi, int N, int dn, int dp
 // The psi tables of an axis are staged only when the tile meets a slab of
 // that axis: a thread reads them only for a cell inside one.
 template <typename T>
-__device__ __forceinline__ TileTables<T> stage_tables(T *lds, const T *ca, int nmat, const TableSrc<T> &s,
+__device__ __forceinline__ TileTables<T> stage_tables(unsigned char *lds, const Grid<T> &g, int side,
                                                      const Tiling &tl, const Cell &cell, int i0,
                                                      bool psi_k, bool psi_j, bool psi_i) {
+  const int nmat = g.nmat;
   const int nk = nmat * tl.krows, nj = nmat * tl.jrows, ni = nmat * tl.tx;
-  T *p = lds;
-  T *sca = p; p += 3 * nmat;
-  T *sk0 = p; p += nk;
-  T *sk1 = p; p += nk;
-  T *qk0 = p; p += nk;
-  T *qk1 = p; p += nk;
-  T *sj0 = p; p += nj;
-  T *sj1 = p; p += nj;
-  T *qj0 = p; p += nj;
-  T *qj1 = p; p += nj;
-  T *si0 = p; p += ni;
-  T *si1 = p; p += ni;
-  T *qi0 = p; p += ni;
-  T *qi1 = p;
+  P4<T> *sself = reinterpret_cast<P4<T> *>(lds);
+  P2<T> *p = reinterpret_cast<P2<T> *>(sself + nmat);
+  P2<T> *sk = p; p += nk;
+  P2<T> *qk = p; p += nk;
+  P2<T> *sj = p; p += nj;
+  P2<T> *qj = p; p += nj;
+  P2<T> *si = p; p += ni;
+  P2<T> *qi = p;
+  const P4<T> *gself = reinterpret_cast<const P4<T> *>(g.self[side]);
+  const P2<T> *gk = reinterpret_cast<const P2<T> *>(g.kt[side]);
+  const P2<T> *gj = reinterpret_cast<const P2<T> *>(g.jt[side]);
+  const P2<T> *gi = reinterpret_cast<const P2<T> *>(g.it[side]);
+  const P2<T> *hk = reinterpret_cast<const P2<T> *>(g.kp[side]);
+  const P2<T> *hj = reinterpret_cast<const P2<T> *>(g.jp[side]);
+  const P2<T> *hi = reinterpret_cast<const P2<T> *>(g.ip[side]);
   const int nth = blockDim.x;
-  for (int q = threadIdx.x; q < 3 * nmat; q += nth) sca[q] = ca[(s.ca_base + q / nmat) * nmat + q % nmat];
+  for (int q = threadIdx.x; q < nmat; q += nth) sself[q] = gself[q];
   if (cell.js == 0 || tl.ty == 0) {
     // each k slot is written by the thread that owns it (row 0 of a 2-D tile)
-    const int n = min(cell.k, s.nk1 - 1);
+    const int src = min(cell.k, g.nz) * nmat, dst = cell.ks * nmat;
     for (int m = 0; m < nmat; ++m) {
-      const int d = m * tl.krows + cell.ks, o = m * s.nk1 + n;
-      sk0[d] = s.k0[o];
-      sk1[d] = s.k1[o];
-      if (psi_k) {
-        qk0[d] = s.pk0[o];
-        qk1[d] = s.pk1[o];
-      }
+      sk[dst + m] = gk[src + m];
+      if (psi_k) qk[dst + m] = hk[src + m];
     }
   }
   for (int q = threadIdx.x; q < nj; q += nth) {
-    const int o = (q / tl.jrows) * s.nj1 + min(cell.j_first + q % tl.jrows, s.nj1 - 1);
-    sj0[q] = s.j0[o];
-    sj1[q] = s.j1[o];
-    if (psi_j) {
-      qj0[q] = s.pj0[o];
-      qj1[q] = s.pj1[o];
-    }
+    const int n = min(cell.j_first + q / nmat, g.ny), o = n * nmat + q % nmat;
+    sj[q] = gj[o];
+    if (psi_j) qj[q] = hj[o];
   }
   for (int q = threadIdx.x; q < ni; q += nth) {
-    const int o = (q / tl.tx) * s.ni1 + min(i0 + q % tl.tx, s.ni1 - 1);
-    si0[q] = s.i0[o];
-    si1[q] = s.i1[o];
-    if (psi_i) {
-      qi0[q] = s.pi0[o];
-      qi1[q] = s.pi1[o];
-    }
+    const int n = min(i0 + q / nmat, g.nx), o = n * nmat + q % nmat;
+    si[q] = gi[o];
+    if (psi_i) qi[q] = hi[o];
   }
   __syncthreads();
   TileTables<T> tt;
-  tt.ca = sca;
-  tt.tk0 = sk0; tt.tk1 = sk1; tt.tj0 = sj0; tt.tj1 = sj1; tt.ti0 = si0; tt.ti1 = si1;
-  tt.pk0 = qk0; tt.pk1 = qk1; tt.pj0 = qj0; tt.pj1 = qj1; tt.pi0 = qi0; tt.pi1 = qi1;
-  tt.sk = tl.krows; tt.sj = tl.jrows; tt.si = tl.tx;
-  tt.ks = cell.ks; tt.js = cell.js; tt.i0 = i0;
+  tt.self = sself;
+  tt.k = sk; tt.j = sj; tt.i = si;
+  tt.pk = qk; tt.pj = qj; tt.pi = qi;
+  tt.kcol = cell.ks * nmat;
+  tt.jcol = cell.js * nmat;
+  tt.i0 = i0;
   return tt;
 }
 
 // the same view over the global tables (no staging)
 template <typename T>
-__device__ __forceinline__ TileTables<T> global_tables(const T *ca, int nmat, const TableSrc<T> &s,
-                                                      const Cell &cell) {
+__device__ __forceinline__ TileTables<T> global_tables(const Grid<T> &g, int side, const Cell &cell) {
   TileTables<T> tt;
-  tt.ca = ca + s.ca_base * nmat;
-  tt.tk0 = s.k0; tt.tk1 = s.k1; tt.tj0 = s.j0; tt.tj1 = s.j1; tt.ti0 = s.i0; tt.ti1 = s.i1;
-  tt.pk0 = s.pk0; tt.pk1 = s.pk1; tt.pj0 = s.pj0; tt.pj1 = s.pj1; tt.pi0 = s.pi0; tt.pi1 = s.pi1;
-  tt.sk = s.nk1; tt.sj = s.nj1; tt.si = s.ni1;
-  tt.ks = min(cell.k, s.nk1 - 1); tt.js = min(cell.j, s.nj1 - 1); tt.i0 = 0;
+  tt.self = reinterpret_cast<const P4<T> *>(g.self[side]);
+  tt.k = reinterpret_cast<const P2<T> *>(g.kt[side]);
+  tt.j = reinterpret_cast<const P2<T> *>(g.jt[side]);
+  tt.i = reinterpret_cast<const P2<T> *>(g.it[side]);
+  tt.pk = reinterpret_cast<const P2<T> *>(g.kp[side]);
+  tt.pj = reinterpret_cast<const P2<T> *>(g.jp[side]);
+  tt.pi = reinterpret_cast<const P2<T> *>(g.ip[side]);
+  tt.kcol = min(cell.k, g.nz) * g.nmat;
+  tt.jcol = min(cell.j, g.ny) * g.nmat;
+  tt.i0 = 0;
   return tt;
+}
+
+// block-uniform test: does this tile meet a CPML slab along each axis?
+template <typename T>
+__device__ __forceinline__ void tile_slabs(const Grid<T> &g, const Tiling &tl, int bz, const Cell &cell, int i0,
+                                           int i1, bool &sk, bool &sj, bool &si) {
+  if (tl.ty == 0)
+    sk = g.dn[2] + g.dp[2] > 0;
+  else
+    sk = meets_slab(bz * tl.tz, bz * tl.tz + tl.tz - 1, g.nz, g.dn[2], g.dp[2]);
+  sj = meets_slab(cell.j_first, cell.j_first + tl.jrows - 1, g.ny, g.dn[1], g.dp[1]);
+  si = meets_slab(i0, i1 - 1, g.nx, g.dn[0], g.dp[0]);
 }
 
 // Field loads and stores by 32-bit byte offset from a wave-uniform base: all
@@ -277,8 +288,8 @@ struct ColCpml {
 };
 
 template <typename T>
-__device__ __forceinline__ ColCpml<T> col_cpml(const Grid<T> &g, const T *const *b, const T *const *a,
-                                               int ly, int lz, int j, int k) {
+__device__ __forceinline__ ColCpml<T> col_cpml(const T *const *b, const T *const *a, int ly, int lz, int j,
+                                               int k) {
   ColCpml<T> cc;
   cc.ly = ly;
   cc.lz = lz;
@@ -294,464 +305,418 @@ __device__ __forceinline__ ColCpml<T> col_cpml(const Grid<T> &g, const T *const 
   return cc;
 }
 
+// One x-plane of a column: the loads of the cell (fields, material, the
+// in-plane neighbours and, on the CPML path, the psi values), issued
+// together so that a plane costs one memory round trip.
+template <typename T>
+struct Plane {
+  unsigned m;
+  T f0, f1, f2;  // the three components this half-step updates
+  T g0, g1, g2;  // the three components it reads
+  T n0, n1, n2, n3, n4, n5;  // neighbours (see the kernels)
+  int lx;        // layer along x (-1 outside the x slabs)
+  T bx, ax;      // its CPML b/a (wave-uniform)
+  T ps[6];       // psi values of the cell, by P_* slot
+};
+
+// Per-column constants of a curl kernel.
+template <typename T>
+struct Col {
+  T *__restrict__ u0;  // updated components (Ex, Ey, Ez or Hx, Hy, Hz)
+  T *__restrict__ u1;
+  T *__restrict__ u2;
+  const T *__restrict__ r0;  // read components (Hx, Hy, Hz or Ex, Ey, Ez)
+  const T *__restrict__ r1;
+  const T *__restrict__ r2;
+  const uint8_t *__restrict__ mat;
+  // byte offsets of the k and j neighbours; 0 where the neighbour does not
+  // exist, in which case every term that would use it is discarded
+  unsigned dk, dj;
+  bool kin, jin;  // column inside the Yee ranges along k and j
+  ColCpml<T> cp;
+  int j, k;
+};
+
 // ---------------------------------------------------------------------------
 // E half-step: Simulation::updateE (simulation_update.cpp:170-196) fused with
 // PML::updateE of every face (perfect_match_layer.cpp:574-718), faces in x,
 // y, z order.  One thread owns a (j,k) column and marches over TX x-planes.
-// A plane's loads -- the six field values, the material byte, the four
-// in-plane neighbours and, inside a CPML slab, the psi values of the cell --
-// are issued together, so a plane costs one memory round trip.  With PF two
-// plane buffers (A, B) alternate so that the loads of plane i+1 are in flight
-// while plane i is computed.
-template <typename T>
-struct PlaneE {
-  unsigned m;
-  T ex, ey, ez, hx, hy, hz, hyk, hxk, hzj, hxj;
-  int lx;     // layer along x (-1 outside the x slabs)
-  T bx, ax;   // its CPML b/a (wave-uniform)
-  T ps[6];    // psi values of the cell, by P_* slot
-};
-
-template <typename T>
-struct ColE {
-  T *__restrict__ Ex;
-  T *__restrict__ Ey;
-  T *__restrict__ Ez;
-  const T *__restrict__ Hx;
-  const T *__restrict__ Hy;
-  const T *__restrict__ Hz;
-  const uint8_t *__restrict__ mat;
-  unsigned bsj;  // byte stride of j
-  bool k1, j1;   // k >= 1, j >= 1
-  ColCpml<T> cp;
-};
-
-template <typename T>
-__device__ __forceinline__ void load_plane_e(const Grid<T> &g, const ColE<T> &q, long long c, int i, int j,
-                                             int k, PlaneE<T> &p) {
-  const unsigned cell = (unsigned)c, o = cell * (unsigned)sizeof(T);
+// Planes that meet no CPML slab take the SLAB = false path, which has no psi
+// code at all.  Every plane stores all three components: cells outside the
+// Yee and CPML ranges get their own value back (each cell is owned by one
+// thread, so this is exact), which keeps the stores free of branches.
+//
+// Plane fields: f = Ex, Ey, Ez; g = Hx, Hy, Hz; n0 = Hy(k-1), n1 = Hx(k-1),
+// n2 = Hz(j-1), n3 = Hx(j-1).
+template <typename T, bool SLAB>
+__device__ __forceinline__ void load_plane_e(const Grid<T> &g, const Col<T> &q, unsigned cell, int i,
+                                             Plane<T> &p) {
+  const unsigned o = cell * (unsigned)sizeof(T);
   p.m = ldm(q.mat, cell);
-  p.ex = ldo(q.Ex, o);
-  p.ey = ldo(q.Ey, o);
-  p.ez = ldo(q.Ez, o);
-  p.hx = ldo(q.Hx, o);
-  p.hy = ldo(q.Hy, o);
-  p.hz = ldo(q.Hz, o);
-  p.hyk = q.k1 ? ldo(q.Hy, o - (unsigned)sizeof(T)) : T(0);
-  p.hxk = q.k1 ? ldo(q.Hx, o - (unsigned)sizeof(T)) : T(0);
-  p.hzj = q.j1 ? ldo(q.Hz, o - q.bsj) : T(0);
-  p.hxj = q.j1 ? ldo(q.Hx, o - q.bsj) : T(0);
-  p.lx = e_layer(i, g.nx, g.dn[0], g.dp[0]);
-  p.bx = p.ax = T(0);
-  for (int s = 0; s < 6; ++s) p.ps[s] = T(0);
-  const int ly = q.cp.ly, lz = q.cp.lz, lx = p.lx;
-  if (lx < 0 && ly < 0 && lz < 0) return;
-  const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = g.ny + 1;
-  if (lx >= 0) {
-    p.bx = g.be[0][i];
-    p.ax = g.ae[0][i];
-  }
-  if (i < g.nx) {
-    if (ly >= 0) p.ps[P_AY] = g.psi_e[P_AY][psi_y(i, ly, k, Ly, g.pz)];
-    if (lz >= 0) p.ps[P_AZ] = g.psi_e[P_AZ][psi_z(i, j, lz, nm1y, Lz)];
-  }
-  if (j < g.ny) {
-    if (lx >= 0) p.ps[P_BX] = g.psi_e[P_BX][psi_x(lx, j, k, nm1y, g.pz)];
-    if (lz >= 0) p.ps[P_BZ] = g.psi_e[P_BZ][psi_z(i, j, lz, nm1y, Lz)];
-  }
-  if (k < g.nz) {
-    if (lx >= 0) p.ps[P_CX] = g.psi_e[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
-    if (ly >= 0) p.ps[P_CY] = g.psi_e[P_CY][psi_y(i, ly, k, Ly, g.pz)];
+  p.f0 = ldo(q.u0, o);
+  p.f1 = ldo(q.u1, o);
+  p.f2 = ldo(q.u2, o);
+  p.g0 = ldo(q.r0, o);
+  p.g1 = ldo(q.r1, o);
+  p.g2 = ldo(q.r2, o);
+  p.n0 = ldo(q.r1, o - q.dk);
+  p.n1 = ldo(q.r0, o - q.dk);
+  p.n2 = ldo(q.r2, o - q.dj);
+  p.n3 = ldo(q.r0, o - q.dj);
+  if constexpr (SLAB) {
+    p.lx = e_layer(i, g.nx, g.dn[0], g.dp[0]);
+    p.bx = p.ax = T(0);
+    for (int s = 0; s < 6; ++s) p.ps[s] = T(0);
+    const int ly = q.cp.ly, lz = q.cp.lz, lx = p.lx, j = q.j, k = q.k;
+    const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = g.ny + 1;
+    if (lx >= 0) {
+      p.bx = g.be[0][i];
+      p.ax = g.ae[0][i];
+    }
+    if (i < g.nx) {
+      if (ly >= 0) p.ps[P_AY] = g.psi_e[P_AY][psi_y(i, ly, k, Ly, g.pz)];
+      if (lz >= 0) p.ps[P_AZ] = g.psi_e[P_AZ][psi_z(i, j, lz, nm1y, Lz)];
+    }
+    if (j < g.ny) {
+      if (lx >= 0) p.ps[P_BX] = g.psi_e[P_BX][psi_x(lx, j, k, nm1y, g.pz)];
+      if (lz >= 0) p.ps[P_BZ] = g.psi_e[P_BZ][psi_z(i, j, lz, nm1y, Lz)];
+    }
+    if (k < g.nz) {
+      if (lx >= 0) p.ps[P_CX] = g.psi_e[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
+      if (ly >= 0) p.ps[P_CY] = g.psi_e[P_CY][psi_y(i, ly, k, Ly, g.pz)];
+    }
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTables<T> &tt, const ColE<T> &q,
-                                                const PlaneE<T> &p, int i, int j, int k, long long c,
-                                                bool jin, bool kin, T hz_prev, T hy_prev) {
-  const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
-  const int nm1y = ny + 1;
-  const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2];
+template <typename T, bool SLAB>
+__device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTables<T> &tt, const Col<T> &q,
+                                                const Plane<T> &p, int i, unsigned cell, T hz_prev,
+                                                T hy_prev) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz, j = q.j, k = q.k;
   const int m = (int)p.m;
-  const int lx = p.lx, ly = q.cp.ly, lz = q.cp.lz;
-  const T dhy_k = p.hy - p.hyk;     // Hy(k) - Hy(k-1)
-  const T dhz_j = p.hz - p.hzj;     // Hz(j) - Hz(j-1)
-  const T dhx_k = p.hx - p.hxk;     // Hx(k) - Hx(k-1)
-  const T dhz_i = p.hz - hz_prev;   // Hz(i) - Hz(i-1)
-  const T dhx_j = p.hx - p.hxj;     // Hx(j) - Hx(j-1)
-  const T dhy_i = p.hy - hy_prev;   // Hy(i) - Hy(i-1)
-  const int mk = m * tt.sk + tt.ks, mj = m * tt.sj + tt.js, mi = m * tt.si + (i - tt.i0);
-  const unsigned o = (unsigned)c * (unsigned)sizeof(T);
-  // Yee terms for all three components, evaluated unconditionally so the
-  // plane's loads are consumed on every path (one wait per plane)
-  const T yx = tt.ca[m] * p.ex + tt.tk0[mk] * dhy_k + tt.tj0[mj] * dhz_j;
-  const T yy = tt.ca[nmat + m] * p.ey + tt.tk1[mk] * dhx_k + tt.ti0[mi] * dhz_i;
-  const T yz = tt.ca[2 * nmat + m] * p.ez + tt.tj1[mj] * dhx_j + tt.ti1[mi] * dhy_i;
+  const T ex = p.f0, ey = p.f1, ez = p.f2, hx = p.g0, hy = p.g1, hz = p.g2;
+  const T dhy_k = hy - p.n0;     // Hy(k) - Hy(k-1)
+  const T dhz_j = hz - p.n2;     // Hz(j) - Hz(j-1)
+  const T dhx_k = hx - p.n1;     // Hx(k) - Hx(k-1)
+  const T dhz_i = hz - hz_prev;  // Hz(i) - Hz(i-1)
+  const T dhx_j = hx - p.n3;     // Hx(j) - Hx(j-1)
+  const T dhy_i = hy - hy_prev;  // Hy(i) - Hy(i-1)
+  const int mi = (i - tt.i0) * g.nmat + m;
+  const P4<T> cs = tt.self[m];
+  const P2<T> ck = tt.k[tt.kcol + m], cj = tt.j[tt.jcol + m], ci = tt.i[mi];
+  // cexe*Ex + cexhy*dHy/dz + cexhz*dHz/dy, and likewise for Ey, Ez
+  const T yx = cs.x * ex + ck.x * dhy_k + cj.x * dhz_j;
+  const T yy = cs.y * ey + ck.y * dhx_k + ci.x * dhz_i;
+  const T yz = cs.z * ez + cj.y * dhx_j + ci.y * dhy_i;
   const bool ix = i >= 1 && i < nx;
-  // Ex: shape (nx, ny+1, nz+1); Yee j in [1,ny), k in [1,nz)
-  {
-    const bool yee = (i < nx) && jin && kin;
-    T v = yee ? yx : p.ex;
-    const bool slab = (i < nx) && (ly >= 0 || lz >= 0);
-    if (slab) {
+  // Ex (nx, ny+1, nz+1): Yee i < nx, j in [1,ny), k in [1,nz)
+  // Ey (nx+1, ny, nz+1): Yee i in [1,nx), j < ny, k in [1,nz)
+  // Ez (nx+1, ny+1, nz): Yee i in [1,nx), j in [1,ny), k < nz
+  T vx = (i < nx && q.jin && q.kin) ? yx : ex;
+  T vy = (ix && j < ny && q.kin) ? yy : ey;
+  T vz = (ix && q.jin && k < nz) ? yz : ez;
+  if constexpr (SLAB) {
+    const int ly = q.cp.ly, lz = q.cp.lz, lx = p.lx;
+    const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = ny + 1;
+    if (i < nx) {
       if (ly >= 0) {
         const T ps = q.cp.by * p.ps[P_AY] + q.cp.ay * dhz_j;
         g.psi_e[P_AY][psi_y(i, ly, k, Ly, g.pz)] = ps;
-        v = v + tt.pj0[mj] * ps;
+        vx = vx + tt.pj[tt.jcol + m].x * ps;
       }
       if (lz >= 0) {
         const T ps = q.cp.bz * p.ps[P_AZ] + q.cp.az * dhy_k;
         g.psi_e[P_AZ][psi_z(i, j, lz, nm1y, Lz)] = ps;
-        v = v + tt.pk0[mk] * ps;
+        vx = vx + tt.pk[tt.kcol + m].x * ps;
       }
     }
-    if (yee || slab) sto(q.Ex, o, v);
-  }
-  // Ey: shape (nx+1, ny, nz+1); Yee k in [1,nz), i in [1,nx)
-  {
-    const bool yee = ix && kin && (j < ny);
-    T v = yee ? yy : p.ey;
-    const bool slab = (j < ny) && (lx >= 0 || lz >= 0);
-    if (slab) {
+    if (j < ny) {
       if (lx >= 0) {
         const T ps = p.bx * p.ps[P_BX] + p.ax * dhz_i;
         g.psi_e[P_BX][psi_x(lx, j, k, nm1y, g.pz)] = ps;
-        v = v + tt.pi0[mi] * ps;
+        vy = vy + tt.pi[mi].x * ps;
       }
       if (lz >= 0) {
         const T ps = q.cp.bz * p.ps[P_BZ] + q.cp.az * dhx_k;
         g.psi_e[P_BZ][psi_z(i, j, lz, nm1y, Lz)] = ps;
-        v = v + tt.pk1[mk] * ps;
+        vy = vy + tt.pk[tt.kcol + m].y * ps;
       }
     }
-    if (yee || slab) sto(q.Ey, o, v);
-  }
-  // Ez: shape (nx+1, ny+1, nz); Yee i in [1,nx), j in [1,ny)
-  {
-    const bool yee = ix && jin && (k < nz);
-    T v = yee ? yz : p.ez;
-    const bool slab = (k < nz) && (lx >= 0 || ly >= 0);
-    if (slab) {
+    if (k < nz) {
       if (lx >= 0) {
         const T ps = p.bx * p.ps[P_CX] + p.ax * dhy_i;
         g.psi_e[P_CX][psi_x(lx, j, k, nm1y, g.pz)] = ps;
-        v = v + tt.pi1[mi] * ps;
+        vz = vz + tt.pi[mi].y * ps;
       }
       if (ly >= 0) {
         const T ps = q.cp.by * p.ps[P_CY] + q.cp.ay * dhx_j;
         g.psi_e[P_CY][psi_y(i, ly, k, Ly, g.pz)] = ps;
-        v = v + tt.pj1[mj] * ps;
+        vz = vz + tt.pj[tt.jcol + m].y * ps;
       }
     }
-    if (yee || slab) sto(q.Ez, o, v);
   }
+  const unsigned o = cell * (unsigned)sizeof(T);
+  sto(q.u0, o, vx);
+  sto(q.u1, o, vy);
+  sto(q.u2, o, vz);
 }
 
-// block-uniform test: does this tile meet a CPML slab along each axis?
-template <typename T>
-__device__ __forceinline__ void tile_slabs(const Grid<T> &g, const Tiling &tl, int bz, const Cell &cell, int i0,
-                                           int i1, bool &sk, bool &sj, bool &si) {
-  if (tl.ty == 0)
-    sk = g.dn[2] + g.dp[2] > 0;
-  else
-    sk = meets_slab(bz * tl.tz, bz * tl.tz + tl.tz - 1, g.nz, g.dn[2], g.dp[2]);
-  sj = meets_slab(cell.j_first, cell.j_first + tl.jrows - 1, g.ny, g.dn[1], g.dp[1]);
-  si = meets_slab(i0, i1 - 1, g.nx, g.dn[0], g.dp[0]);
-}
-
-template <typename T, bool LDS, bool PF>
+template <typename T, bool LDS>
 __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl) {
-  extern __shared__ unsigned char lds_raw[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
   int bz, by, bx;
   if (!tile_of_block(tl, bz, by, bx)) return;
   const Cell cell = thread_cell(tl, bz, by, g.pz);
   const int k = cell.k, j = cell.j;
-  const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int i0 = bx * tl.tx;
   const int i1 = min(i0 + tl.tx, nx + 1);
   const bool live = k <= nz && j <= ny;
-  ColE<T> q;
-  q.Ex = g.f[EX]; q.Ey = g.f[EY]; q.Ez = g.f[EZ];
-  q.Hx = g.f[HX]; q.Hy = g.f[HY]; q.Hz = g.f[HZ];
+  Col<T> q;
+  q.u0 = g.f[EX]; q.u1 = g.f[EY]; q.u2 = g.f[EZ];
+  q.r0 = g.f[HX]; q.r1 = g.f[HY]; q.r2 = g.f[HZ];
   q.mat = g.mat;
-  q.bsj = (unsigned)(g.sj * (long long)sizeof(T));
-  q.k1 = k >= 1;
-  q.j1 = j >= 1;
-  const bool kin = (k >= 1 && k < nz), jin = (j >= 1 && j < ny);
-  long long c = (long long)i0 * g.si + (long long)j * g.sj + k;
+  q.dk = k >= 1 ? (unsigned)sizeof(T) : 0u;
+  q.dj = j >= 1 ? (unsigned)(g.sj * (long long)sizeof(T)) : 0u;
+  q.kin = k >= 1 && k < nz;
+  q.jin = j >= 1 && j < ny;
+  q.j = j;
+  q.k = k;
+  q.cp = col_cpml(g.be, g.ae, live ? e_layer(j, ny, g.dn[1], g.dp[1]) : -1,
+                  live ? e_layer(k, nz, g.dn[2], g.dp[2]) : -1, j, k);
+  // a wave whose column meets no y or z slab runs the plain path on every
+  // plane outside the x slabs
+  const bool col_slab = __ballot(q.cp.ly >= 0 || q.cp.lz >= 0) != 0ull;
+  unsigned c = (unsigned)((long long)i0 * g.si + (long long)j * g.sj + k);
   // x-neighbours are carried in registers: Hz(i-1), Hy(i-1)
   T hz_prev = T(0), hy_prev = T(0);
-  PlaneE<T> A, B;
+  Plane<T> A;
+  bool slab = col_slab || e_layer(i0, nx, g.dn[0], g.dp[0]) >= 0;
   // the first plane's loads go out before the tables are staged
   if (live) {
-    q.cp = col_cpml(g, g.be, g.ae, e_layer(j, ny, g.dn[1], g.dp[1]), e_layer(k, nz, g.dn[2], g.dp[2]), j, k);
     if (i0 >= 1) {
-      hz_prev = ldo(q.Hz, (unsigned)(c - g.si) * (unsigned)sizeof(T));
-      hy_prev = ldo(q.Hy, (unsigned)(c - g.si) * (unsigned)sizeof(T));
+      hz_prev = ldo(q.r2, (c - (unsigned)g.si) * (unsigned)sizeof(T));
+      hy_prev = ldo(q.r1, (c - (unsigned)g.si) * (unsigned)sizeof(T));
     }
-    load_plane_e(g, q, c, i0, j, k, A);
+    if (slab)
+      load_plane_e<T, true>(g, q, c, i0, A);
+    else
+      load_plane_e<T, false>(g, q, c, i0, A);
   }
-  TableSrc<T> src;
-  src.ca_base = EX;
-  src.k0 = g.cb[CB_EXHY]; src.k1 = g.cb[CB_EYHX];
-  src.j0 = g.cb[CB_EXHZ]; src.j1 = g.cb[CB_EZHX];
-  src.i0 = g.cb[CB_EYHZ]; src.i1 = g.cb[CB_EZHY];
-  src.pk0 = g.cpsi[CB_EXHY]; src.pk1 = g.cpsi[CB_EYHX];
-  src.pj0 = g.cpsi[CB_EXHZ]; src.pj1 = g.cpsi[CB_EZHX];
-  src.pi0 = g.cpsi[CB_EYHZ]; src.pi1 = g.cpsi[CB_EZHY];
-  src.nk1 = nz + 1; src.nj1 = ny + 1; src.ni1 = nx + 1;
   TileTables<T> tt;
   if (LDS) {
     bool sk, sj, si;
     tile_slabs(g, tl, bz, cell, i0, i1, sk, sj, si);
-    tt = stage_tables<T>(reinterpret_cast<T *>(lds_raw), g.ca, nmat, src, tl, cell, i0, sk, sj, si);
+    tt = stage_tables<T>(lds_raw, g, 0, tl, cell, i0, sk, sj, si);
   } else {
-    tt = global_tables<T>(g.ca, nmat, src, cell);
+    tt = global_tables<T>(g, 0, cell);
   }
   if (!live) return;
-  if (!PF) {
-    for (int i = i0;;) {
-      compute_plane_e(g, tt, q, A, i, j, k, c, jin, kin, hz_prev, hy_prev);
-      hz_prev = A.hz;
-      hy_prev = A.hy;
-      if (++i >= i1) break;
-      c += g.si;
-      load_plane_e(g, q, c, i, j, k, A);
-    }
-    return;
-  }
-  for (int i = i0; i < i1; i += 2, c += 2 * g.si) {
-    if (i + 1 < i1) load_plane_e(g, q, c + g.si, i + 1, j, k, B);
-    compute_plane_e(g, tt, q, A, i, j, k, c, jin, kin, hz_prev, hy_prev);
-    hz_prev = A.hz;
-    hy_prev = A.hy;
-    if (i + 1 >= i1) break;
-    if (i + 2 < i1) load_plane_e(g, q, c + 2 * g.si, i + 2, j, k, A);
-    compute_plane_e(g, tt, q, B, i + 1, j, k, c + g.si, jin, kin, hz_prev, hy_prev);
-    hz_prev = B.hz;
-    hy_prev = B.hy;
+  for (int i = i0;;) {
+    if (slab)
+      compute_plane_e<T, true>(g, tt, q, A, i, c, hz_prev, hy_prev);
+    else
+      compute_plane_e<T, false>(g, tt, q, A, i, c, hz_prev, hy_prev);
+    hz_prev = A.g2;
+    hy_prev = A.g1;
+    if (++i >= i1) break;
+    c += (unsigned)g.si;
+    slab = col_slab || e_layer(i, nx, g.dn[0], g.dp[0]) >= 0;
+    if (slab)
+      load_plane_e<T, true>(g, q, c, i, A);
+    else
+      load_plane_e<T, false>(g, q, c, i, A);
   }
 }
 
 // ---------------------------------------------------------------------------
 // H half-step: Simulation::updateH (simulation_update.cpp:84-100) fused with
 // PML::updateH (perfect_match_layer.cpp:434-572).  Same plane scheme as the
-// E kernel; Ez(i), Ey(i) are the (i+1) values loaded for the previous plane.
-template <typename T>
-struct PlaneH {
-  unsigned m;
-  T hx, hy, hz, ex, eyk, exk, ezj, exj, ezi, eyi;
-  int lx;
-  T bx, ax;
-  T ps[6];
-};
-
-template <typename T>
-struct ColH {
-  const T *__restrict__ Ex;
-  const T *__restrict__ Ey;
-  const T *__restrict__ Ez;
-  T *__restrict__ Hx;
-  T *__restrict__ Hy;
-  T *__restrict__ Hz;
-  const uint8_t *__restrict__ mat;
-  unsigned bsj, bsi;  // byte strides of j and i
-  bool kin, jin;      // k < nz, j < ny
-  ColCpml<T> cp;
-};
-
-template <typename T>
-__device__ __forceinline__ void load_plane_h(const Grid<T> &g, const ColH<T> &q, long long c, int i, int j,
-                                             int k, PlaneH<T> &p) {
-  const unsigned cell = (unsigned)c, o = cell * (unsigned)sizeof(T);
+// E kernel; the i+1 values Ez(i+1), Ey(i+1) loaded for one plane are the
+// Ez(i), Ey(i) of the next.
+//
+// Plane fields: f = Hx, Hy, Hz; g0 = Ex; n0 = Ey(k+1), n1 = Ex(k+1),
+// n2 = Ez(j+1), n3 = Ex(j+1), n4 = Ez(i+1), n5 = Ey(i+1).
+template <typename T, bool SLAB>
+__device__ __forceinline__ void load_plane_h(const Grid<T> &g, const Col<T> &q, unsigned cell, int i,
+                                             unsigned di, Plane<T> &p) {
+  const unsigned o = cell * (unsigned)sizeof(T);
   p.m = ldm(q.mat, cell);
-  p.hx = ldo(q.Hx, o);
-  p.hy = ldo(q.Hy, o);
-  p.hz = ldo(q.Hz, o);
-  p.ex = ldo(q.Ex, o);
-  p.eyk = q.kin ? ldo(q.Ey, o + (unsigned)sizeof(T)) : T(0);  // Ey(k+1)
-  p.exk = q.kin ? ldo(q.Ex, o + (unsigned)sizeof(T)) : T(0);  // Ex(k+1)
-  p.ezj = q.jin ? ldo(q.Ez, o + q.bsj) : T(0);                // Ez(j+1)
-  p.exj = q.jin ? ldo(q.Ex, o + q.bsj) : T(0);                // Ex(j+1)
-  p.ezi = (i < g.nx) ? ldo(q.Ez, o + q.bsi) : T(0);           // Ez(i+1)
-  p.eyi = (i < g.nx) ? ldo(q.Ey, o + q.bsi) : T(0);           // Ey(i+1)
-  p.lx = h_layer(i, g.nx, g.dn[0], g.dp[0]);
-  p.bx = p.ax = T(0);
-  for (int s = 0; s < 6; ++s) p.ps[s] = T(0);
-  const int ly = q.cp.ly, lz = q.cp.lz, lx = p.lx;
-  if (lx < 0 && ly < 0 && lz < 0) return;
-  const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = g.ny + 1;
-  if (lx >= 0) {
-    p.bx = g.bm[0][i];
-    p.ax = g.am[0][i];
-  }
-  if (q.jin && q.kin) {
-    if (ly >= 0) p.ps[P_AY] = g.psi_h[P_AY][psi_y(i, ly, k, Ly, g.pz)];
-    if (lz >= 0) p.ps[P_AZ] = g.psi_h[P_AZ][psi_z(i, j, lz, nm1y, Lz)];
-  }
-  if (i < g.nx && q.kin) {
-    if (lx >= 0) p.ps[P_BX] = g.psi_h[P_BX][psi_x(lx, j, k, nm1y, g.pz)];
-    if (lz >= 0) p.ps[P_BZ] = g.psi_h[P_BZ][psi_z(i, j, lz, nm1y, Lz)];
-  }
-  if (i < g.nx && q.jin) {
-    if (lx >= 0) p.ps[P_CX] = g.psi_h[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
-    if (ly >= 0) p.ps[P_CY] = g.psi_h[P_CY][psi_y(i, ly, k, Ly, g.pz)];
+  p.f0 = ldo(q.u0, o);
+  p.f1 = ldo(q.u1, o);
+  p.f2 = ldo(q.u2, o);
+  p.g0 = ldo(q.r0, o);
+  p.n0 = ldo(q.r1, o + q.dk);
+  p.n1 = ldo(q.r0, o + q.dk);
+  p.n2 = ldo(q.r2, o + q.dj);
+  p.n3 = ldo(q.r0, o + q.dj);
+  p.n4 = ldo(q.r2, o + di);
+  p.n5 = ldo(q.r1, o + di);
+  if constexpr (SLAB) {
+    p.lx = h_layer(i, g.nx, g.dn[0], g.dp[0]);
+    p.bx = p.ax = T(0);
+    for (int s = 0; s < 6; ++s) p.ps[s] = T(0);
+    const int ly = q.cp.ly, lz = q.cp.lz, lx = p.lx, j = q.j, k = q.k;
+    const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = g.ny + 1;
+    if (lx >= 0) {
+      p.bx = g.bm[0][i];
+      p.ax = g.am[0][i];
+    }
+    if (q.jin && q.kin) {
+      if (ly >= 0) p.ps[P_AY] = g.psi_h[P_AY][psi_y(i, ly, k, Ly, g.pz)];
+      if (lz >= 0) p.ps[P_AZ] = g.psi_h[P_AZ][psi_z(i, j, lz, nm1y, Lz)];
+    }
+    if (i < g.nx && q.kin) {
+      if (lx >= 0) p.ps[P_BX] = g.psi_h[P_BX][psi_x(lx, j, k, nm1y, g.pz)];
+      if (lz >= 0) p.ps[P_BZ] = g.psi_h[P_BZ][psi_z(i, j, lz, nm1y, Lz)];
+    }
+    if (i < g.nx && q.jin) {
+      if (lx >= 0) p.ps[P_CX] = g.psi_h[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
+      if (ly >= 0) p.ps[P_CY] = g.psi_h[P_CY][psi_y(i, ly, k, Ly, g.pz)];
+    }
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void compute_plane_h(const Grid<T> &g, const TileTables<T> &tt, const ColH<T> &q,
-                                                const PlaneH<T> &p, int i, int j, int k, long long c,
-                                                T ez_c, T ey_c) {
-  const int nx = g.nx, ny = g.ny, nmat = g.nmat;
-  const int nm1y = ny + 1;
-  const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2];
+template <typename T, bool SLAB>
+__device__ __forceinline__ void compute_plane_h(const Grid<T> &g, const TileTables<T> &tt, const Col<T> &q,
+                                                const Plane<T> &p, int i, unsigned cell, T ez_c, T ey_c) {
+  const int nx = g.nx, j = q.j, k = q.k;
   const bool kin = q.kin, jin = q.jin;
   const int m = (int)p.m;
-  const int lx = p.lx, ly = q.cp.ly, lz = q.cp.lz;
-  const T dey_k = p.eyk - ey_c;  // Ey(k+1) - Ey(k)
-  const T dez_j = p.ezj - ez_c;  // Ez(j+1) - Ez(j)
-  const T dez_i = p.ezi - ez_c;  // Ez(i+1) - Ez(i)
-  const T dex_k = p.exk - p.ex;  // Ex(k+1) - Ex(k)
-  const T dex_j = p.exj - p.ex;  // Ex(j+1) - Ex(j)
-  const T dey_i = p.eyi - ey_c;  // Ey(i+1) - Ey(i)
-  const int mk = m * tt.sk + tt.ks, mj = m * tt.sj + tt.js, mi = m * tt.si + (i - tt.i0);
-  const unsigned o = (unsigned)c * (unsigned)sizeof(T);
-  const T yx = tt.ca[m] * p.hx + tt.tk0[mk] * dey_k + tt.tj0[mj] * dez_j;
-  const T yy = tt.ca[nmat + m] * p.hy + tt.ti0[mi] * dez_i + tt.tk1[mk] * dex_k;
-  const T yz = tt.ca[2 * nmat + m] * p.hz + tt.tj1[mj] * dex_j + tt.ti1[mi] * dey_i;
-  // Hx: shape (nx+1, ny, nz); Yee i < nx
-  {
-    const bool in = jin && kin;
-    const bool yee = in && (i < nx);
-    T v = yee ? yx : p.hx;
-    const bool slab = in && (ly >= 0 || lz >= 0);
-    if (slab) {
+  const T hx = p.f0, hy = p.f1, hz = p.f2, ex = p.g0;
+  const T dey_k = p.n0 - ey_c;  // Ey(k+1) - Ey(k)
+  const T dez_j = p.n2 - ez_c;  // Ez(j+1) - Ez(j)
+  const T dez_i = p.n4 - ez_c;  // Ez(i+1) - Ez(i)
+  const T dex_k = p.n1 - ex;    // Ex(k+1) - Ex(k)
+  const T dex_j = p.n3 - ex;    // Ex(j+1) - Ex(j)
+  const T dey_i = p.n5 - ey_c;  // Ey(i+1) - Ey(i)
+  const int mi = (i - tt.i0) * g.nmat + m;
+  const P4<T> cs = tt.self[m];
+  const P2<T> ck = tt.k[tt.kcol + m], cj = tt.j[tt.jcol + m], ci = tt.i[mi];
+  // chxh*Hx + chxey*dEy/dz + chxez*dEz/dy, and likewise for Hy, Hz
+  const T yx = cs.x * hx + ck.x * dey_k + cj.x * dez_j;
+  const T yy = cs.y * hy + ci.x * dez_i + ck.y * dex_k;
+  const T yz = cs.z * hz + cj.y * dex_j + ci.y * dey_i;
+  // all three H components update over i < nx, j < ny, k < nz
+  const bool yee = (i < nx) && jin && kin;
+  T vx = yee ? yx : hx;
+  T vy = yee ? yy : hy;
+  T vz = yee ? yz : hz;
+  if constexpr (SLAB) {
+    const int ly = q.cp.ly, lz = q.cp.lz, lx = p.lx;
+    const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = g.ny + 1;
+    // Hx (nx+1, ny, nz): CPML over j < ny, k < nz
+    if (jin && kin) {
       if (ly >= 0) {
         const T ps = q.cp.by * p.ps[P_AY] + q.cp.ay * dez_j;
         g.psi_h[P_AY][psi_y(i, ly, k, Ly, g.pz)] = ps;
-        v = v + tt.pj0[mj] * ps;
+        vx = vx + tt.pj[tt.jcol + m].x * ps;
       }
       if (lz >= 0) {
         const T ps = q.cp.bz * p.ps[P_AZ] + q.cp.az * dey_k;
         g.psi_h[P_AZ][psi_z(i, j, lz, nm1y, Lz)] = ps;
-        v = v + tt.pk0[mk] * ps;
+        vx = vx + tt.pk[tt.kcol + m].x * ps;
       }
     }
-    if (yee || slab) sto(q.Hx, o, v);
-  }
-  // Hy: shape (nx, ny+1, nz); Yee j < ny
-  {
-    const bool in = (i < nx) && kin;
-    const bool yee = in && jin;
-    T v = yee ? yy : p.hy;
-    const bool slab = in && (lx >= 0 || lz >= 0);
-    if (slab) {
+    // Hy (nx, ny+1, nz): CPML over i < nx, k < nz
+    if (i < nx && kin) {
       if (lx >= 0) {
         const T ps = p.bx * p.ps[P_BX] + p.ax * dez_i;
         g.psi_h[P_BX][psi_x(lx, j, k, nm1y, g.pz)] = ps;
-        v = v + tt.pi0[mi] * ps;
+        vy = vy + tt.pi[mi].x * ps;
       }
       if (lz >= 0) {
         const T ps = q.cp.bz * p.ps[P_BZ] + q.cp.az * dex_k;
         g.psi_h[P_BZ][psi_z(i, j, lz, nm1y, Lz)] = ps;
-        v = v + tt.pk1[mk] * ps;
+        vy = vy + tt.pk[tt.kcol + m].y * ps;
       }
     }
-    if (yee || slab) sto(q.Hy, o, v);
-  }
-  // Hz: shape (nx, ny, nz+1); Yee k < nz
-  {
-    const bool in = (i < nx) && jin;
-    const bool yee = in && kin;
-    T v = yee ? yz : p.hz;
-    const bool slab = in && (lx >= 0 || ly >= 0);
-    if (slab) {
+    // Hz (nx, ny, nz+1): CPML over i < nx, j < ny
+    if (i < nx && jin) {
       if (lx >= 0) {
         const T ps = p.bx * p.ps[P_CX] + p.ax * dey_i;
         g.psi_h[P_CX][psi_x(lx, j, k, nm1y, g.pz)] = ps;
-        v = v + tt.pi1[mi] * ps;
+        vz = vz + tt.pi[mi].y * ps;
       }
       if (ly >= 0) {
         const T ps = q.cp.by * p.ps[P_CY] + q.cp.ay * dex_j;
         g.psi_h[P_CY][psi_y(i, ly, k, Ly, g.pz)] = ps;
-        v = v + tt.pj1[mj] * ps;
+        vz = vz + tt.pj[tt.jcol + m].y * ps;
       }
     }
-    if (yee || slab) sto(q.Hz, o, v);
   }
+  const unsigned o = cell * (unsigned)sizeof(T);
+  sto(q.u0, o, vx);
+  sto(q.u1, o, vy);
+  sto(q.u2, o, vz);
 }
 
-template <typename T, bool LDS, bool PF>
+template <typename T, bool LDS>
 __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl) {
-  extern __shared__ unsigned char lds_raw[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
   int bz, by, bx;
   if (!tile_of_block(tl, bz, by, bx)) return;
   const Cell cell = thread_cell(tl, bz, by, g.pz);
   const int k = cell.k, j = cell.j;
-  const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int i0 = bx * tl.tx;
   const int i1 = min(i0 + tl.tx, nx + 1);
   const bool live = k <= nz && j <= ny;
-  ColH<T> q;
-  q.Ex = g.f[EX]; q.Ey = g.f[EY]; q.Ez = g.f[EZ];
-  q.Hx = g.f[HX]; q.Hy = g.f[HY]; q.Hz = g.f[HZ];
+  const unsigned bsi = (unsigned)(g.si * (long long)sizeof(T));
+  Col<T> q;
+  q.u0 = g.f[HX]; q.u1 = g.f[HY]; q.u2 = g.f[HZ];
+  q.r0 = g.f[EX]; q.r1 = g.f[EY]; q.r2 = g.f[EZ];
   q.mat = g.mat;
-  q.bsj = (unsigned)(g.sj * (long long)sizeof(T));
-  q.bsi = (unsigned)(g.si * (long long)sizeof(T));
+  q.dk = k < nz ? (unsigned)sizeof(T) : 0u;
+  q.dj = j < ny ? (unsigned)(g.sj * (long long)sizeof(T)) : 0u;
   q.kin = k < nz;
   q.jin = j < ny;
-  long long c = (long long)i0 * g.si + (long long)j * g.sj + k;
+  q.j = j;
+  q.k = k;
+  q.cp = col_cpml(g.bm, g.am, live ? h_layer(j, ny, g.dn[1], g.dp[1]) : -1,
+                  live ? h_layer(k, nz, g.dn[2], g.dp[2]) : -1, j, k);
+  const bool col_slab = __ballot(q.cp.ly >= 0 || q.cp.lz >= 0) != 0ull;
+  unsigned c = (unsigned)((long long)i0 * g.si + (long long)j * g.sj + k);
   T ez_c = T(0), ey_c = T(0);
-  PlaneH<T> A, B;
+  Plane<T> A;
+  bool slab = col_slab || h_layer(i0, nx, g.dn[0], g.dp[0]) >= 0;
   // the first plane's loads go out before the tables are staged
   if (live) {
-    q.cp = col_cpml(g, g.bm, g.am, h_layer(j, ny, g.dn[1], g.dp[1]), h_layer(k, nz, g.dn[2], g.dp[2]), j, k);
-    ez_c = ldo(q.Ez, (unsigned)c * (unsigned)sizeof(T));
-    ey_c = ldo(q.Ey, (unsigned)c * (unsigned)sizeof(T));
-    load_plane_h(g, q, c, i0, j, k, A);
+    ez_c = ldo(q.r2, c * (unsigned)sizeof(T));
+    ey_c = ldo(q.r1, c * (unsigned)sizeof(T));
+    const unsigned di = i0 < nx ? bsi : 0u;
+    if (slab)
+      load_plane_h<T, true>(g, q, c, i0, di, A);
+    else
+      load_plane_h<T, false>(g, q, c, i0, di, A);
   }
-  TableSrc<T> src;
-  src.ca_base = HX;
-  src.k0 = g.cb[CB_HXEY]; src.k1 = g.cb[CB_HYEX];
-  src.j0 = g.cb[CB_HXEZ]; src.j1 = g.cb[CB_HZEX];
-  src.i0 = g.cb[CB_HYEZ]; src.i1 = g.cb[CB_HZEY];
-  src.pk0 = g.cpsi[CB_HXEY]; src.pk1 = g.cpsi[CB_HYEX];
-  src.pj0 = g.cpsi[CB_HXEZ]; src.pj1 = g.cpsi[CB_HZEX];
-  src.pi0 = g.cpsi[CB_HYEZ]; src.pi1 = g.cpsi[CB_HZEY];
-  src.nk1 = nz + 1; src.nj1 = ny + 1; src.ni1 = nx + 1;
   TileTables<T> tt;
   if (LDS) {
     bool sk, sj, si;
     tile_slabs(g, tl, bz, cell, i0, i1, sk, sj, si);
-    tt = stage_tables<T>(reinterpret_cast<T *>(lds_raw), g.ca, nmat, src, tl, cell, i0, sk, sj, si);
+    tt = stage_tables<T>(lds_raw, g, 1, tl, cell, i0, sk, sj, si);
   } else {
-    tt = global_tables<T>(g.ca, nmat, src, cell);
+    tt = global_tables<T>(g, 1, cell);
   }
   if (!live) return;
-  if (!PF) {
-    for (int i = i0;;) {
-      compute_plane_h(g, tt, q, A, i, j, k, c, ez_c, ey_c);
-      ez_c = A.ezi;
-      ey_c = A.eyi;
-      if (++i >= i1) break;
-      c += g.si;
-      load_plane_h(g, q, c, i, j, k, A);
-    }
-    return;
-  }
-  for (int i = i0; i < i1; i += 2, c += 2 * g.si) {
-    if (i + 1 < i1) load_plane_h(g, q, c + g.si, i + 1, j, k, B);
-    compute_plane_h(g, tt, q, A, i, j, k, c, ez_c, ey_c);
-    ez_c = A.ezi;
-    ey_c = A.eyi;
-    if (i + 1 >= i1) break;
-    if (i + 2 < i1) load_plane_h(g, q, c + 2 * g.si, i + 2, j, k, A);
-    compute_plane_h(g, tt, q, B, i + 1, j, k, c + g.si, ez_c, ey_c);
-    ez_c = B.ezi;
-    ey_c = B.eyi;
+  for (int i = i0;;) {
+    if (slab)
+      compute_plane_h<T, true>(g, tt, q, A, i, c, ez_c, ey_c);
+    else
+      compute_plane_h<T, false>(g, tt, q, A, i, c, ez_c, ey_c);
+    ez_c = A.n4;
+    ey_c = A.n5;
+    if (++i >= i1) break;
+    c += (unsigned)g.si;
+    slab = col_slab || h_layer(i, nx, g.dn[0], g.dp[0]) >= 0;
+    const unsigned di = i < nx ? bsi : 0u;
+    if (slab)
+      load_plane_h<T, true>(g, q, c, i, di, A);
+    else
+      load_plane_h<T, false>(g, q, c, i, di, A);
   }
 }
+
 // ---------------------------------------------------------------------------
 // TFSF (tfsf/tfsf_3d.cpp)
 template <typename T>
