@@ -17,6 +17,14 @@ kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by the same
 counters' reading for a plain 1 GiB copy (profiles/hbm_traffic.json).
 cpu_baseline: the FP64 CPU oracle, one thread, on six steps of the same 400^3
 run (set-up excluded); cpu_baseline_threads: the same with every host core.
+
+batched: 64 seeded scatter instances of uneven size (edges uniform in
+{48..160}, 8-cell CPML, 1-3 random dielectric objects, TFSF at a random
+angle, NF-FF at one frequency; xfdtd_amd/workloads.py), FP32, tile shapes from
+the autotuner per size class.  The plain loop steps each instance alone and
+waits for it before the next, as one Simulation at a time runs; the batched
+run steps all of them together (xf_batch_step).  Both report
+sum(N^3) * steps / wall time.
 """
 from __future__ import annotations
 
@@ -107,6 +115,65 @@ def copy_calibration(nbytes=1 << 30, reps=5):
                       "copy_GBps_read_plus_write": 2 * nbytes / dt / 1e9}), flush=True)
 
 
+def run_batched(count, steps, warmup=10, log=None):
+    import numpy as np
+
+    from xfdtd_amd import autotune
+    from xfdtd_amd.sim import Batch, Simulation
+    from xfdtd_amd.workloads import batched_scatter_set
+    table = autotune.tune(precision=32, log=log)
+    scenes = batched_scatter_set(count)
+    total = warmup + steps
+
+    def build():
+        sims = []
+        for sc in scenes:
+            sim = Simulation(sc, total, precision=32)
+            sim.set_tiling(*autotune.tiling_for(table, sim.dims[0]))
+            sims.append(sim)
+        return sims
+
+    cells = 0
+    plain = build()
+    for sim in plain:
+        nx, ny, nz = sim.dims
+        cells += nx * ny * nz
+        sim.step(warmup)
+        sim.sync()
+    t0 = time.perf_counter()
+    for sim in plain:
+        sim.step(steps)
+        sim.sync()
+    t_plain = time.perf_counter() - t0
+    batch_sims = build()
+    b = Batch(batch_sims)
+    b.step(warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    # the batched run must reproduce the plain one exactly (same kernels)
+    same = all(np.array_equal(plain[q].get_field(c), batch_sims[q].get_field(c))
+               for q in (0, count - 1) for c in range(6))
+    b.close()
+    for sim in plain + batch_sims:
+        sim.close()
+    return {
+        "workload": f"{count} seeded scatter instances (splitmix64 seed 0xC3), grid edges uniform in "
+                    "{48..160} incl. 8-cell CPML, 1-3 dielectric spheres/boxes, TFSF at random incidence, "
+                    "NF-FF 1 freq, FP32",
+        "instances": count,
+        "cells_per_step": cells,
+        "steps": steps,
+        "plain_loop_gcells": round(cells * steps / t_plain / 1e9, 3),
+        "batched_gcells": round(cells * steps / t_batch / 1e9, 3),
+        "speedup": round(t_plain / t_batch, 3),
+        "bit_identical": bool(same),
+        "tilings": {f"{lo}-{hi}": list(t) for (lo, hi), t in table.items()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +186,8 @@ def main():
                     help="threads of the second, multi-core CPU column (0: skip it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tiling", type=str, default="")
+    ap.add_argument("--batch", type=int, default=64, help="instances of the batched leg (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 1 GiB device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -202,6 +271,9 @@ def main():
         "cpu_baseline": None,
     }
     sim.close()
+    if args.batch > 0:
+        line["batched"] = run_batched(args.batch, args.batch_steps,
+                                      log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu and args.cpu_steps > 0:
         line["cpu_baseline"] = run_cpu_baseline(sc, total, args.cpu_steps)
         if args.cpu_threads > 1:

@@ -182,6 +182,22 @@ xf_status xf_engine_set_timing(xf_engine *e, int enabled);
 enum { XF_TILE_GLOBAL_TABLES = 2, XF_TILE_X_FASTEST = 4, XF_TILE_XCD_RUNS = 8, XF_TILE_XCD_COLS = 16 };
 xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags);
 
+/* Batches: several set-up engines of one precision, all at the same step,
+   stepped together.  Each per-step kernel (curl + CPML, TFSF faces, 1-D
+   incident grids, NF-FF) is one launch over every instance, so a batch of
+   small, uneven problems fills the GPU where one engine at a time cannot.
+   Results are bit-identical to stepping each engine alone.  The batch has its
+   own stream; the member engines' other calls (get_field, get_nffft, step,
+   ...) wait for it.  Destroy the batch before its engines.  No counterpart in
+   the reference, which runs one Simulation at a time. */
+typedef struct xf_batch xf_batch;
+xf_status xf_batch_create(xf_engine *const *engines, int n, xf_batch **out);
+/* n steps of every member (enqueued; returns at once) */
+xf_status xf_batch_step(xf_batch *b, int64_t n);
+xf_status xf_batch_sync(xf_batch *b);
+int xf_batch_size(const xf_batch *b);
+void xf_batch_destroy(xf_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,6 +21,7 @@ ENGINE_SYMBOLS = [
     "xf_engine_add_probe", "xf_engine_get_probes", "xf_engine_step", "xf_engine_sync",
     "xf_engine_current_step", "xf_engine_get_field", "xf_engine_set_field",
     "xf_engine_kernel_times", "xf_engine_set_timing", "xf_engine_set_tiling",
+    "xf_batch_create", "xf_batch_step", "xf_batch_sync", "xf_batch_size", "xf_batch_destroy",
 ]
 SIM_SYMBOLS = [
     "xf_sim_last_error", "xf_sim_create", "xf_sim_destroy", "xf_sim_add_cube", "xf_sim_add_sphere",
@@ -81,5 +82,12 @@ def lib():
     L.xf_engine_set_tiling.argtypes = [vp, i, i, i, i]
     L.xf_engine_sync.argtypes = [vp]
     L.xf_engine_step.argtypes = [vp, i64]
+    L.xf_engine_current_step.restype = i64
+    L.xf_engine_current_step.argtypes = [vp]
+    L.xf_batch_create.argtypes = [C.POINTER(vp), i, C.POINTER(vp)]
+    L.xf_batch_step.argtypes = [vp, i64]
+    L.xf_batch_sync.argtypes = [vp]
+    L.xf_batch_size.argtypes = [vp]
+    L.xf_batch_destroy.argtypes = [vp]
     _LIB = L
     return L

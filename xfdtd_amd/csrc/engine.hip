@@ -92,7 +92,19 @@ struct xf_engine {
   std::vector<hipEvent_t> ev;  // 4 events per timed step: E start/end, H start/end
   size_t ev_used = 0;
   std::vector<void *> owned;
+  // stream of the batch this engine belongs to (work on it precedes any use
+  // of the engine's own stream)
+  hipStream_t batch_stream = nullptr;
 };
+
+// wait for everything enqueued for this engine, on its own stream and on
+// the stream of a batch it belongs to
+static hipError_t engine_wait(xf_engine *e) {
+  hipError_t r = hipSuccess;
+  if (e->batch_stream) r = hipStreamSynchronize(e->batch_stream);
+  if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+  return r;
+}
 
 static xf_status dev_alloc(xf_engine *e, void **p, size_t bytes) {
   if (bytes == 0) bytes = 16;
@@ -175,7 +187,7 @@ extern "C" xf_status xf_engine_create(const xf_grid_desc *g, xf_engine **out) {
 
 extern "C" void xf_engine_destroy(xf_engine *e) {
   if (!e) return;
-  if (e->stream) hipStreamSynchronize(e->stream);
+  if (e->stream) engine_wait(e);
   for (void *p : e->owned) hipFree(p);
   for (hipEvent_t ev : e->ev) hipEventDestroy(ev);
   if (e->stream) hipStreamDestroy(e->stream);
@@ -365,7 +377,7 @@ extern "C" int64_t xf_engine_nffft_size(const xf_engine *e) { return e && e->has
 
 extern "C" xf_status xf_engine_get_nffft(xf_engine *e, double *out) {
   if (!e || !out || !e->has_nf) FAILS(XF_ESTATE, "no NF-FF surface");
-  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(engine_wait(e));
   HIPCHK(hipMemcpy(out, e->acc, (size_t)e->nf_size * sizeof(double2), hipMemcpyDeviceToHost));
   return XF_OK;
 }
@@ -400,7 +412,7 @@ extern "C" xf_status xf_engine_farfield_sums(xf_engine *e, double dl, const doub
   fd.stcp = dd; fd.stsp = dd + nd; fd.ct = dd + 2 * nd; fd.dk = dk; fd.nt = ntheta; fd.np = nphi;
   hipLaunchKernelGGL(k_farfield, dim3(nd, e->nf.nf), dim3(256), 0, e->stream, e->nf, fd, dout);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(engine_wait(e));
   HIPCHK(hipMemcpy(out, dout, outn * sizeof(double2), hipMemcpyDeviceToHost));
   hipFree(dd);
   hipFree(dk);
@@ -465,7 +477,7 @@ extern "C" xf_status xf_engine_get_probes(xf_engine *e, double *out, int64_t tot
   if (!e || !out) FAILS(XF_EINVAL, "null argument");
   if (e->probe_comp.empty()) return XF_OK;
   if (e->probe_buf == nullptr || total_steps != e->probe_T) FAILS(XF_ESTATE, "probe buffer holds %lld steps", e->probe_T);
-  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(engine_wait(e));
   HIPCHK(hipMemcpy(out, e->probe_buf, e->probe_comp.size() * (size_t)total_steps * sizeof(double),
                    hipMemcpyDeviceToHost));
   return XF_OK;
@@ -517,11 +529,8 @@ static Tfsf<T> make_tfsf(xf_engine *e) {
   return t;
 }
 
-template <typename T>
-static xf_status run_steps(xf_engine *e, long long n) {
-  Grid<T> g = make_grid<T>(e);
-  Tfsf<T> t{};
-  if (e->has_tfsf) t = make_tfsf<T>(e);
+// launch geometry of the curl kernels from the engine's tiling settings
+static Tiling make_tiling(const xf_engine *e) {
   Tiling tl;
   tl.tz = e->tz;
   tl.ty = e->ty;
@@ -545,7 +554,18 @@ static xf_status run_steps(xf_engine *e, long long n) {
   if (tl.xcd_cols) tl.nblocks = 8 * tl.ntz * ((tl.nty * tl.ntx + 7) / 8);
   tl.x_fastest = (e->tile_flags & XF_TILE_X_FASTEST) != 0;
   tl.xcd_runs = (e->tile_flags & XF_TILE_XCD_RUNS) != 0;
-  dim3 blk(tl.ty == 0 ? tl.tz : tl.tz * tl.ty);
+  return tl;
+}
+
+static int tiling_threads(const Tiling &tl) { return tl.ty == 0 ? tl.tz : tl.tz * tl.ty; }
+
+template <typename T>
+static xf_status run_steps(xf_engine *e, long long n) {
+  Grid<T> g = make_grid<T>(e);
+  Tfsf<T> t{};
+  if (e->has_tfsf) t = make_tfsf<T>(e);
+  const Tiling tl = make_tiling(e);
+  dim3 blk(tiling_threads(tl));
   dim3 grd(tl.nblocks);
   // coefficient tables of a tile go to LDS when they fit in 32 KB
   const size_t lds = lds_table_bytes<T>(e->nmat, tl);
@@ -622,12 +642,13 @@ extern "C" xf_status xf_engine_step(xf_engine *e, int64_t n) {
     xf_status s = ensure_probes(e, e->total);
     if (s != XF_OK) return s;
   }
+  if (e->batch_stream) HIPCHK(hipStreamSynchronize(e->batch_stream));
   return e->prec == 64 ? run_steps<double>(e, n) : run_steps<float>(e, n);
 }
 
 extern "C" xf_status xf_engine_sync(xf_engine *e) {
   if (!e) FAILS(XF_EINVAL, "null engine");
-  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(engine_wait(e));
   return XF_OK;
 }
 
@@ -637,7 +658,7 @@ extern "C" xf_status xf_engine_get_field(xf_engine *e, int comp, double *out) {
   if (!e || !out || comp < 0 || comp > 5) FAILS(XF_EINVAL, "bad field request");
   int sh[3];
   comp_shape(e, comp, sh);
-  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(engine_wait(e));
   std::vector<unsigned char> buf((size_t)e->ncell * e->esz);
   HIPCHK(hipMemcpy(buf.data(), e->f[comp], buf.size(), hipMemcpyDeviceToHost));
   for (int i = 0; i < sh[0]; ++i)
@@ -659,7 +680,7 @@ extern "C" xf_status xf_engine_set_field(xf_engine *e, int comp, const double *i
   if (!e || !in || comp < 0 || comp > 5) FAILS(XF_EINVAL, "bad field request");
   int sh[3];
   comp_shape(e, comp, sh);
-  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(engine_wait(e));
   std::vector<unsigned char> buf((size_t)e->ncell * e->esz, 0);
   for (int i = 0; i < sh[0]; ++i)
     for (int j = 0; j < sh[1]; ++j) {
@@ -679,7 +700,7 @@ extern "C" xf_status xf_engine_set_field(xf_engine *e, int comp, const double *i
 
 extern "C" xf_status xf_engine_kernel_times(xf_engine *e, double *ms_e, double *ms_h, int64_t *samples) {
   if (!e) FAILS(XF_EINVAL, "null engine");
-  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(engine_wait(e));
   double se = 0, sh = 0;
   const size_t n = e->ev_used / 4;
   for (size_t q = 0; q < n; ++q) {
@@ -698,7 +719,7 @@ extern "C" xf_status xf_engine_kernel_times(xf_engine *e, double *ms_e, double *
 
 extern "C" xf_status xf_engine_set_timing(xf_engine *e, int enabled) {
   if (!e) FAILS(XF_EINVAL, "null engine");
-  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(engine_wait(e));
   e->timing = enabled != 0;
   e->ev_used = 0;
   return XF_OK;
@@ -713,3 +734,197 @@ extern "C" xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, 
   e->tile_flags = flags;
   return XF_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Batches: many engines of one precision stepped together.  Each per-step
+// kernel is one launch over all instances; a workgroup finds its instance and
+// local block in a per-launch map.  Point probes and lumped sources, which
+// are a handful of cells, keep per-instance launches on the batch stream.
+struct xf_batch {
+  std::vector<xf_engine *> e;
+  int prec = 64;
+  hipStream_t stream = nullptr;
+  void *inst = nullptr;  // Inst<T>[n] on the device
+  int2 *map_curl = nullptr, *map_te = nullptr, *map_th = nullptr, *map_ifa = nullptr, *map_nf = nullptr;
+  int n_curl = 0, n_te = 0, n_th = 0, n_ifa = 0, n_nf = 0;
+  int threads = 256;
+  size_t lds = 0;
+  bool use_lds = true;
+  std::vector<void *> owned;
+};
+
+static xf_status batch_upload(xf_batch *b, void **dst, const void *src, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  HIPCHK(hipMalloc(dst, bytes));
+  b->owned.push_back(*dst);
+  if (src) HIPCHK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  return XF_OK;
+}
+
+template <typename T>
+static xf_status batch_build(xf_batch *b) {
+  const int n = (int)b->e.size();
+  std::vector<Inst<T>> in(n);
+  std::vector<int2> mc, mte, mth, mifa, mnf;
+  b->lds = 0;
+  b->use_lds = true;
+  for (int q = 0; q < n; ++q) {
+    xf_engine *e = b->e[q];
+    Inst<T> &a = in[q];
+    std::memset(&a, 0, sizeof a);
+    a.g = make_grid<T>(e);
+    a.tl = make_tiling(e);
+    const size_t lds = lds_table_bytes<T>(e->nmat, a.tl);
+    b->lds = std::max(b->lds, lds);
+    if (lds > 32768 || (e->tile_flags & XF_TILE_GLOBAL_TABLES)) b->use_lds = false;
+    for (int w = 0; w < a.tl.nblocks; ++w) mc.push_back(make_int2(q, w));
+    if (e->has_tfsf) {
+      a.t = make_tfsf<T>(e);
+      a.nEy = e->nEy; a.nEz = e->nEz; a.nEx = e->nEx;
+      a.hs = e->hseg;
+      const int te = (e->nEy + e->nEz + e->nEx + 255) / 256, th = (e->hseg.off[12] + 255) / 256;
+      for (int w = 0; w < te; ++w) mte.push_back(make_int2(q, w));
+      for (int w = 0; w < th; ++w) mth.push_back(make_int2(q, w));
+      mifa.push_back(make_int2(q, 0));
+    }
+    if (e->has_nf) {
+      a.nf = e->nf;
+      const long long tot = (long long)e->nf.nyb * e->nf.nzb + (long long)e->nf.nzb * e->nf.nxb +
+                            (long long)e->nf.nxb * e->nf.nyb;
+      for (long long w = 0; w < (tot + 255) / 256; ++w) mnf.push_back(make_int2(q, (int)w));
+    }
+  }
+  if (!b->use_lds) b->lds = 0;
+  if (mc.size() > 0x7fffffffu) FAILS(XF_EINVAL, "batch too large");
+  xf_status s;
+  if ((s = batch_upload(b, &b->inst, in.data(), in.size() * sizeof(Inst<T>))) != XF_OK) return s;
+  void *p;
+  if ((s = batch_upload(b, &p, mc.data(), mc.size() * sizeof(int2))) != XF_OK) return s;
+  b->map_curl = (int2 *)p; b->n_curl = (int)mc.size();
+  if ((s = batch_upload(b, &p, mte.data(), mte.size() * sizeof(int2))) != XF_OK) return s;
+  b->map_te = (int2 *)p; b->n_te = (int)mte.size();
+  if ((s = batch_upload(b, &p, mth.data(), mth.size() * sizeof(int2))) != XF_OK) return s;
+  b->map_th = (int2 *)p; b->n_th = (int)mth.size();
+  if ((s = batch_upload(b, &p, mifa.data(), mifa.size() * sizeof(int2))) != XF_OK) return s;
+  b->map_ifa = (int2 *)p; b->n_ifa = (int)mifa.size();
+  if ((s = batch_upload(b, &p, mnf.data(), mnf.size() * sizeof(int2))) != XF_OK) return s;
+  b->map_nf = (int2 *)p; b->n_nf = (int)mnf.size();
+  return XF_OK;
+}
+
+extern "C" void xf_batch_destroy(xf_batch *b) {
+  if (!b) return;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  for (xf_engine *e : b->e)
+    if (e->batch_stream == b->stream) e->batch_stream = nullptr;
+  for (void *p : b->owned) hipFree(p);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+extern "C" xf_status xf_batch_create(xf_engine *const *engines, int n, xf_batch **out) {
+  if (!engines || !out || n < 1) FAILS(XF_EINVAL, "need at least one engine");
+  *out = nullptr;
+  const xf_engine *e0 = engines[0];
+  if (!e0) FAILS(XF_EINVAL, "null engine");
+  int threads = -1;
+  for (int q = 0; q < n; ++q) {
+    xf_engine *e = engines[q];
+    if (!e) FAILS(XF_EINVAL, "null engine %d", q);
+    if (!e->materials_set) FAILS(XF_ESTATE, "engine %d is not set up", q);
+    if (e->batch_stream) FAILS(XF_ESTATE, "engine %d already belongs to a batch", q);
+    if (e->prec != e0->prec) FAILS(XF_EINVAL, "engine %d has precision %d, engine 0 has %d", q, e->prec, e0->prec);
+    if (e->cur != e0->cur) FAILS(XF_ESTATE, "engine %d is at step %lld, engine 0 at %lld", q, e->cur, e0->cur);
+    for (int r = 0; r < q; ++r)
+      if (engines[r] == e) FAILS(XF_EINVAL, "engine %d appears twice", q);
+    const int th = tiling_threads(make_tiling(e));
+    if (threads >= 0 && th != threads)
+      FAILS(XF_EINVAL, "engine %d tiles with %d threads per block, engine 0 with %d", q, th, threads);
+    threads = th;
+  }
+  xf_batch *b = new xf_batch();
+  b->e.assign(engines, engines + n);
+  b->prec = e0->prec;
+  b->threads = threads;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    FAILS(XF_EDEVICE, "hipStreamCreateWithFlags failed");
+  }
+  for (xf_engine *e : b->e) {
+    if (!e->probe_comp.empty() && e->total >= 0) {
+      xf_status s = ensure_probes(e, e->total);
+      if (s != XF_OK) { xf_batch_destroy(b); return s; }
+    }
+    if (engine_wait(e) != hipSuccess) { xf_batch_destroy(b); FAILS(XF_EDEVICE, "engine stream failed"); }
+  }
+  xf_status s = b->prec == 64 ? batch_build<double>(b) : batch_build<float>(b);
+  if (s != XF_OK) { xf_batch_destroy(b); return s; }
+  for (xf_engine *e : b->e) e->batch_stream = b->stream;
+  *out = b;
+  return XF_OK;
+}
+
+template <typename T>
+static xf_status batch_run(xf_batch *b, long long n) {
+  const Inst<T> *in = (const Inst<T> *)b->inst;
+  const dim3 blk(b->threads);
+  for (long long q = 0; q < n; ++q) {
+    const long long step = b->e[0]->cur;
+    if (b->use_lds)
+      hipLaunchKernelGGL((kb_update_e<T, true>), dim3(b->n_curl), blk, b->lds, b->stream, in, b->map_curl);
+    else
+      hipLaunchKernelGGL((kb_update_e<T, false>), dim3(b->n_curl), blk, 0, b->stream, in, b->map_curl);
+    for (xf_engine *e : b->e)
+      for (const SourceDev &sd : e->sources) {
+        const int nj = sd.j1 - sd.j0, nk = sd.k1 - sd.k0;
+        const int nc = (sd.i1 - sd.i0) * nj * nk;
+        hipLaunchKernelGGL(k_source<T>, dim3((nc + 255) / 256), dim3(256), 0, b->stream, (T *)e->f[sd.comp],
+                           e->si, e->sj, sd.i0, nj, nk, sd.j0, sd.k0, nc, (const T *)sd.coff,
+                           (const T *)sd.wave, step);
+      }
+    if (b->n_te > 0) hipLaunchKernelGGL(kb_tfsf_e<T>, dim3(b->n_te), dim3(256), 0, b->stream, in, b->map_te);
+    if (b->use_lds)
+      hipLaunchKernelGGL((kb_update_h<T, true>), dim3(b->n_curl), blk, b->lds, b->stream, in, b->map_curl);
+    else
+      hipLaunchKernelGGL((kb_update_h<T, false>), dim3(b->n_curl), blk, 0, b->stream, in, b->map_curl);
+    if (b->n_ifa > 0) {
+      hipLaunchKernelGGL(kb_tfsf_ifa<T>, dim3(b->n_ifa), dim3(1024), 0, b->stream, in, b->map_ifa, step);
+      hipLaunchKernelGGL(kb_tfsf_h<T>, dim3(b->n_th), dim3(256), 0, b->stream, in, b->map_th);
+    }
+    if (b->n_nf > 0) hipLaunchKernelGGL(kb_nffft<T>, dim3(b->n_nf), dim3(256), 0, b->stream, in, b->map_nf, step);
+    for (xf_engine *e : b->e)
+      if (!e->probe_comp.empty()) {
+        const int np = (int)e->probe_comp.size();
+        hipLaunchKernelGGL(k_probe<T>, dim3((np + 63) / 64), dim3(64), 0, b->stream, make_grid<T>(e),
+                           e->d_probe_comp, e->d_probe_cell, np, e->probe_buf, e->probe_T, step);
+      }
+    HIPCHK(hipGetLastError());
+    for (xf_engine *e : b->e) e->cur++;
+  }
+  return XF_OK;
+}
+
+extern "C" xf_status xf_batch_step(xf_batch *b, int64_t n) {
+  if (!b) FAILS(XF_EINVAL, "null batch");
+  if (n < 0) FAILS(XF_EINVAL, "negative step count");
+  for (size_t q = 0; q < b->e.size(); ++q) {
+    xf_engine *e = b->e[q];
+    if (e->cur != b->e[0]->cur) FAILS(XF_ESTATE, "engine %zu was stepped outside the batch", q);
+    if (e->total >= 0 && e->cur + n > e->total)
+      FAILS(XF_ESTATE, "engine %zu: step %lld + %lld beyond the %lld steps its tables hold", q, e->cur,
+            (long long)n, e->total);
+    if (!e->probe_comp.empty() && (e->probe_buf == nullptr || e->probes_dirty))
+      FAILS(XF_ESTATE, "engine %zu: probes added after the batch was made", q);
+    // work enqueued on the engine's own stream goes first
+    HIPCHK(hipStreamSynchronize(e->stream));
+  }
+  return b->prec == 64 ? batch_run<double>(b, n) : batch_run<float>(b, n);
+}
+
+extern "C" xf_status xf_batch_sync(xf_batch *b) {
+  if (!b) FAILS(XF_EINVAL, "null batch");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return XF_OK;
+}
+
+extern "C" int xf_batch_size(const xf_batch *b) { return b ? (int)b->e.size() : 0; }

@@ -73,9 +73,8 @@ struct Tiling {
   int xcd_cols;             // all k tiles of a (j, i) column on one XCD
 };
 
-// tile of this workgroup; false for a padding workgroup of the column order
-__device__ __forceinline__ bool tile_of_block(const Tiling &t, int &bz, int &by, int &bx) {
-  const int b = blockIdx.x;
+// tile of workgroup b; false for a padding workgroup of the column order
+__device__ __forceinline__ bool tile_of_block(const Tiling &t, int b, int &bz, int &by, int &bx) {
   const int q = t.nblocks >> 3, r = t.nblocks & 7;
   const int xcd = b & 7, local = b >> 3;
   if (t.xcd_cols) {
@@ -462,10 +461,10 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
 }
 
 template <typename T, bool LDS>
-__global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl) {
+__device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl, int block) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   int bz, by, bx;
-  if (!tile_of_block(tl, bz, by, bx)) return;
+  if (!tile_of_block(tl, block, bz, by, bx)) return;
   const Cell cell = thread_cell(tl, bz, by, g.pz);
   const int k = cell.k, j = cell.j;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -652,10 +651,10 @@ __device__ __forceinline__ void compute_plane_h(const Grid<T> &g, const TileTabl
 }
 
 template <typename T, bool LDS>
-__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl) {
+__device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl, int block) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   int bz, by, bx;
-  if (!tile_of_block(tl, bz, by, bx)) return;
+  if (!tile_of_block(tl, block, bz, by, bx)) return;
   const Cell cell = thread_cell(tl, bz, by, g.pz);
   const int k = cell.k, j = cell.j;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -717,6 +716,15 @@ __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl) {
   }
 }
 
+template <typename T, bool LDS>
+__global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl) {
+  update_e_body<T, LDS>(g, tl, blockIdx.x);
+}
+template <typename T, bool LDS>
+__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl) {
+  update_h_body<T, LDS>(g, tl, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // TFSF (tfsf/tfsf_3d.cpp)
 template <typename T>
@@ -756,7 +764,7 @@ __device__ __forceinline__ T tfsf_inc(const Tfsf<T> &t, int comp, int i, int j, 
 
 // TFSF3D::updateIncidentField (tfsf_3d.cpp:153-183): one workgroup
 template <typename T>
-__global__ void __launch_bounds__(1024) k_tfsf_ifa(Tfsf<T> t, long long step) {
+__device__ __forceinline__ void tfsf_ifa_body(const Tfsf<T> &t, long long step) {
   const int L = t.L;
   T *e = t.e_inc, *h = t.h_inc;
   const T x = e[L - 2], y = e[L - 1];
@@ -775,6 +783,10 @@ __global__ void __launch_bounds__(1024) k_tfsf_ifa(Tfsf<T> t, long long step) {
   for (int i = threadIdx.x; i < L - 1; i += blockDim.x)
     h[i] = t.chih * h[i] + t.chiei * (e[i + 1] - e[i]);
 }
+template <typename T>
+__global__ void __launch_bounds__(1024) k_tfsf_ifa(Tfsf<T> t, long long step) {
+  tfsf_ifa_body<T>(t, step);
+}
 
 __device__ __forceinline__ void perimeter(int p, int A, int B, int &u, int &v) {
   if (p < A) { u = p; v = 0; return; }
@@ -788,8 +800,9 @@ __device__ __forceinline__ void perimeter(int p, int A, int B, int &u, int &v) {
 // one thread owns each target cell and applies them in the reference's
 // xn, xp, yn, yp, zn, zp order.
 template <typename T>
-__global__ void k_tfsf_e(Grid<T> g, Tfsf<T> t, int nEy, int nEz, int nEx) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void tfsf_e_body(const Grid<T> &g, const Tfsf<T> &t, int nEy, int nEz, int nEx,
+                                            int block) {
+  int q = block * (int)blockDim.x + (int)threadIdx.x;
   const int li = t.li, ri = t.ri, lj = t.lj, rj = t.rj, lk = t.lk, rk = t.rk;
   if (q < nEy) {
     const int A = ri - li + 1, B = rk - lk + 1, P = 2 * A + 2 * (B - 2);
@@ -838,14 +851,19 @@ __global__ void k_tfsf_e(Grid<T> g, Tfsf<T> t, int nEy, int nEz, int nEx) {
     *p = val;
   }
 }
+template <typename T>
+__global__ void k_tfsf_e(Grid<T> g, Tfsf<T> t, int nEy, int nEz, int nEx) {
+  tfsf_e_body<T>(g, t, nEy, nEz, nEx, blockIdx.x);
+}
 
 // TFSF3D::updateH (tfsf_3d.cpp:185-297).  No H cell takes two corrections.
 struct TfsfHSeg {
   int off[13];  // prefix offsets of the 12 face loops
 };
 template <typename T>
-__global__ void k_tfsf_h(Grid<T> g, Tfsf<T> t, TfsfHSeg s) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void tfsf_h_body(const Grid<T> &g, const Tfsf<T> &t, const TfsfHSeg &s,
+                                            int block) {
+  const int q = block * (int)blockDim.x + (int)threadIdx.x;
   if (q >= s.off[12]) return;
   int seg = 0;
   while (q >= s.off[seg + 1]) ++seg;
@@ -888,6 +906,10 @@ __global__ void k_tfsf_h(Grid<T> g, Tfsf<T> t, TfsfHSeg s) {
   const T inc = tfsf_inc(t, src, si_, sj_, sk_);
   T *p = &g.f[comp][(long long)i * g.si + (long long)j * g.sj + k];
   *p = add ? (*p + coef * inc) : (*p - coef * inc);
+}
+template <typename T>
+__global__ void k_tfsf_h(Grid<T> g, Tfsf<T> t, TfsfHSeg s) {
+  tfsf_h_body<T>(g, t, s, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -933,8 +955,8 @@ __device__ __forceinline__ void acc_add(double2 *a, double v, double2 tw) {
 }
 
 template <typename T>
-__global__ void k_nffft(Grid<T> g, Nffft n, long long step) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void nffft_body(const Grid<T> &g, const Nffft &n, long long step, int block) {
+  int q = block * (int)blockDim.x + (int)threadIdx.x;
   const int cx = n.nyb * n.nzb, cy = n.nzb * n.nxb, cz = n.nxb * n.nyb;
   const T *Ex = g.f[EX], *Ey = g.f[EY], *Ez = g.f[EZ];
   const T *Hx = g.f[HX], *Hy = g.f[HY], *Hz = g.f[HZ];
@@ -990,6 +1012,10 @@ __global__ void k_nffft(Grid<T> g, Nffft n, long long step) {
       acc_add(a, (double)v[c], (c % 4) < 2 ? twm : twj);
     }
   }
+}
+template <typename T>
+__global__ void k_nffft(Grid<T> g, Nffft n, long long step) {
+  nffft_body<T>(g, n, step, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1073,6 +1099,55 @@ __global__ void __launch_bounds__(256) k_farfield(Nffft n, FarDir d, double2 *ou
     double *o = reinterpret_cast<double *>(out + base);
     o[threadIdx.x] = acc;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches: many independent simulations of one precision stepped
+// together, each per-step kernel one launch over all of them.  A launch's
+// workgroup b handles local block map[b].y of instance map[b].x, whose
+// arguments are read from device memory.
+template <typename T>
+struct Inst {
+  Grid<T> g;
+  Tiling tl;
+  Tfsf<T> t;
+  int nEy, nEz, nEx;
+  TfsfHSeg hs;
+  Nffft nf;
+};
+
+template <typename T, bool LDS>
+__global__ void __launch_bounds__(256) kb_update_e(const Inst<T> *__restrict__ in, const int2 *__restrict__ map) {
+  const int2 w = map[blockIdx.x];
+  update_e_body<T, LDS>(in[w.x].g, in[w.x].tl, w.y);
+}
+template <typename T, bool LDS>
+__global__ void __launch_bounds__(256) kb_update_h(const Inst<T> *__restrict__ in, const int2 *__restrict__ map) {
+  const int2 w = map[blockIdx.x];
+  update_h_body<T, LDS>(in[w.x].g, in[w.x].tl, w.y);
+}
+template <typename T>
+__global__ void kb_tfsf_e(const Inst<T> *__restrict__ in, const int2 *__restrict__ map) {
+  const int2 w = map[blockIdx.x];
+  const Inst<T> &a = in[w.x];
+  tfsf_e_body<T>(a.g, a.t, a.nEy, a.nEz, a.nEx, w.y);
+}
+template <typename T>
+__global__ void kb_tfsf_h(const Inst<T> *__restrict__ in, const int2 *__restrict__ map) {
+  const int2 w = map[blockIdx.x];
+  const Inst<T> &a = in[w.x];
+  tfsf_h_body<T>(a.g, a.t, a.hs, w.y);
+}
+// one workgroup per instance with a TFSF source: map[b].x
+template <typename T>
+__global__ void __launch_bounds__(1024) kb_tfsf_ifa(const Inst<T> *__restrict__ in, const int2 *__restrict__ map,
+                                                    long long step) {
+  tfsf_ifa_body<T>(in[map[blockIdx.x].x].t, step);
+}
+template <typename T>
+__global__ void kb_nffft(const Inst<T> *__restrict__ in, const int2 *__restrict__ map, long long step) {
+  const int2 w = map[blockIdx.x];
+  nffft_body<T>(in[w.x].g, in[w.x].nf, step, w.y);
 }
 
 }  // namespace xfk

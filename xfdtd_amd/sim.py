@@ -176,3 +176,38 @@ class Simulation:
         L = lib()
         if L.xf_engine_set_tiling(self.engine, int(tz), int(ty), int(tx), int(flags)) != 0:
             raise RuntimeError(L.xf_last_error().decode())
+
+
+class Batch:
+    """Several set-up simulations of one precision stepped together on the
+    GPU (include/xfdtd_engine.h, xf_batch_*): every per-step kernel is one
+    launch over all of them.  Results equal stepping each one alone; read
+    them through the member Simulations after `sync()`.  Close the batch
+    before its members."""
+
+    def __init__(self, sims):
+        L = lib()
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.engine for s in self.sims])
+        h = C.c_void_p()
+        if L.xf_batch_create(arr, len(self.sims), C.byref(h)) != 0:
+            raise RuntimeError("xfdtd_amd: " + L.xf_last_error().decode())
+        self.h = h
+
+    def step(self, n):
+        L = lib()
+        if L.xf_batch_step(self.h, int(n)) != 0:
+            raise RuntimeError("xfdtd_amd: " + L.xf_last_error().decode())
+
+    def sync(self):
+        L = lib()
+        if L.xf_batch_sync(self.h) != 0:
+            raise RuntimeError("xfdtd_amd: " + L.xf_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().xf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,90 @@
+"""Seeded synthetic scatter workloads.
+
+Random draws use a repository-owned splitmix64 generator with explicit
+transforms, so a seed gives the same scenes on every platform and numpy
+version.
+"""
+from __future__ import annotations
+
+import math
+
+from . import scene as S
+
+_MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    """splitmix64 (Steele, Lea, Flood 2014): state += golden gamma, then two
+    xor-shift-multiply rounds."""
+
+    def __init__(self, seed):
+        self.state = seed & _MASK
+
+    def next_u64(self):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0):
+        """53-bit uniform double in [lo, hi)."""
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
+
+    def integer(self, lo, hi):
+        """Uniform integer in [lo, hi] (inclusive), by rejection."""
+        span = hi - lo + 1
+        limit = (1 << 64) - ((1 << 64) % span)
+        while True:
+            x = self.next_u64()
+            if x < limit:
+                return lo + x % span
+
+
+def random_scatter_scene(rng, n, dl=5e-3, pml=8, tfsf_gap=12, cfl=0.9, nffft=True):
+    """One batched-scatter instance: an n^3 total grid (CPML included) with
+    1-3 dielectric spheres or boxes inside the total-field box and a plane
+    wave at a random angle on the upper hemisphere.
+
+    - objects: sphere or cube with probability 1/2 each; eps_r uniform in
+      [1.5, 6] (rounded to float by the material model); radius / half-edge
+      uniform in [0.05 n, 0.25 n] cells, capped so the object fits inside the
+      TF box; centre uniform over the positions where it fits;
+    - incidence: theta = arccos(u), u uniform in [0, 1]; phi uniform in
+      [0, 2 pi); psi = 0;
+    - NF-FF surface 2 cells inside the TF box at f = c0 / (20 dl).
+    """
+    n_obj = rng.integer(1, 3)
+    half = (n - 2 * pml) * dl / 2
+    tf_lo = -half + (tfsf_gap - pml) * dl      # TF box in coordinates
+    tf_hi = half - (tfsf_gap - pml) * dl
+    objs = []
+    for q in range(n_obj):
+        kind = "sphere" if rng.uniform() < 0.5 else "cube"
+        eps = rng.uniform(1.5, 6.0)
+        r_cells = rng.uniform(0.05 * n, 0.25 * n)
+        r = min(r_cells * dl, 0.45 * (tf_hi - tf_lo))
+        lo, hi = tf_lo + r, tf_hi - r
+        c = tuple(rng.uniform(lo, hi) for _ in range(3))
+        mat = S.Material(f"m{q}", eps, 1.0, 0.0, 0.0)
+        if kind == "sphere":
+            objs.append(S.Object(f"o{q}", S.Sphere(c, r), mat))
+        else:
+            objs.append(S.Object(f"o{q}", S.Cube(tuple(x - r for x in c), (2 * r,) * 3), mat))
+    theta = math.acos(rng.uniform())
+    phi = rng.uniform(0.0, 2 * math.pi)
+    return S.cube_domain_scene(
+        n, dl, pml, cfl=cfl, objects=objs, tfsf_gap=tfsf_gap,
+        nffft_gap=(tfsf_gap - 2) if nffft else None,
+        freqs=[S.C_0 / (20 * dl)], theta=[i * math.pi / 18 for i in range(19)], phi=[0.0],
+        incidence=(theta, phi, 0.0))
+
+
+def batched_scatter_set(count=64, seed=0xC3, n_min=48, n_max=160, **kw):
+    """`count` instances with grid edges uniform in {n_min..n_max}."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        n = rng.integer(n_min, n_max)
+        out.append(random_scatter_scene(rng, n, **kw))
+    return out
