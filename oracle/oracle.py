@@ -59,6 +59,11 @@ def lib():
         L.xo_sim_current_step.argtypes = [vp]
         L.xo_set_threads.argtypes = [i]
         L.xo_set_threads.restype = i
+        L.xo_sim_set_design.argtypes = [vp, szp, C.POINTER(C.c_float)]
+        L.xo_sim_set_dft_plane.argtypes = [vp, i, i, sz, sz, sz, sz, sz, d]
+        L.xo_sim_get_dft.argtypes = [vp, dp]
+        L.xo_sim_grad.argtypes = [vp, dp, dp]
+        L.xo_sim_adjoint_run.argtypes = [vp, sz, dp]
         _LIB = L
     return _LIB
 
@@ -124,6 +129,16 @@ class OracleSim:
                                                       w.t_0))
             for p in scene.probes:
                 self._chk(L.xo_sim_add_probe(self.h, int(p.comp), p.i, p.j, p.k))
+            dz = getattr(scene, "design", None)
+            if dz is not None:
+                box = (C.c_size_t * 6)(*[int(v) for v in dz.box])
+                er = np.ascontiguousarray(np.asarray(dz.eps_r, dtype=np.float32).ravel())
+                self._chk(L.xo_sim_set_design(self.h, box, er.ctypes.data_as(C.POINTER(C.c_float))))
+            dp_ = getattr(scene, "dft_plane", None)
+            if dp_ is not None:
+                self._chk(L.xo_sim_set_dft_plane(self.h, int(dp_.comp), int(dp_.axis), int(dp_.index),
+                                                 int(dp_.lo1), int(dp_.hi1), int(dp_.lo2), int(dp_.hi2),
+                                                 float(dp_.freq)))
             self._chk(L.xo_sim_init(self.h, self.T))
         except Exception:
             L.xo_sim_destroy(self.h)
@@ -214,6 +229,31 @@ class OracleSim:
         res["power_theta"] = pw[0].copy()
         res["power_phi"] = pw[1].copy()
         return res
+
+    def get_dft(self):
+        dp_ = self.scene.dft_plane
+        out = np.empty(2 * (dp_.hi1 - dp_.lo1) * (dp_.hi2 - dp_.lo2), dtype=np.float64)
+        self._chk(lib().xo_sim_get_dft(self.h, _dp(out)))
+        return out.view(np.complex128).reshape(dp_.hi1 - dp_.lo1, dp_.hi2 - dp_.lo2)
+
+    def grad(self):
+        """(J, dJ/d eps_r over the design box) from a fresh simulation."""
+        b = self.scene.design.box
+        g = np.empty((b[1] - b[0], b[3] - b[2], b[5] - b[4]), dtype=np.float64)
+        J = C.c_double()
+        self._chk(lib().xo_sim_grad(self.h, _dp(g), C.byref(J)))
+        return J.value, g
+
+    def adjoint_run(self, nsteps, fields):
+        """Reverse-run nsteps of the field map on six adjoint fields."""
+        packed = np.concatenate([np.ascontiguousarray(f, dtype=np.float64).ravel() for f in fields])
+        self._chk(lib().xo_sim_adjoint_run(self.h, int(nsteps), _dp(packed)))
+        out, off = [], 0
+        for c in range(6):
+            n = int(np.prod(self.field_shape(c)))
+            out.append(packed[off:off + n].reshape(self.field_shape(c)).copy())
+            off += n
+        return out
 
     def get_probes(self):
         out = np.empty((len(self.scene.probes), self.T), dtype=np.float64)

@@ -165,6 +165,17 @@ struct xo_sim {
   double *probe_buf;
   a3 f[6];    /* Ex Ey Ez Hx Hy Hz */
   a3 c[18];   /* coefficient arrays */
+  /* per-cell permittivity of a design box (lossless, mu_r = 1), applied
+     after the objects as one-cell objects would be */
+  int has_design;
+  size_t dbox[6];        /* i0,i1,j0,j1,k0,k1 */
+  float *design_eps;     /* eps_r per design cell, C order */
+  /* DFT monitor of one component on a plane (the adjoint objective) */
+  int has_dft;
+  int dft_comp, dft_axis;
+  size_t dft_index, dft_lo1, dft_hi1, dft_lo2, dft_hi2;
+  double dft_freq;
+  double complex *dft_acc;  /* [hi1-lo1][hi2-lo2] */
 };
 
 /* coefficient indices */
@@ -225,6 +236,8 @@ void xo_sim_destroy(xo_sim *s) {
   free(s->src);
   free(s->probes);
   free(s->probe_buf);
+  free(s->design_eps);
+  free(s->dft_acc);
   free(s);
 }
 
@@ -1177,6 +1190,31 @@ int xo_sim_init(xo_sim *s, size_t total_steps) {
           }
         }
   }
+  if (s->has_design) {
+    const size_t *b = s->dbox;
+    if (b[1] > nx || b[3] > ny || b[5] > nz) FAIL("design box outside the grid");
+    size_t q = 0;
+    for (size_t i = b[0]; i < b[1]; ++i)
+      for (size_t j = b[2]; j < b[3]; ++j)
+        for (size_t k = b[4]; k < b[5]; ++k, ++q)
+          for (int c = 0; c < 3; ++c) {
+            AT(eps[c], i, j, k) = (double)s->design_eps[q] * EPS0;
+            AT(mu[c], i, j, k) = MU0();
+            AT(sg[c], i, j, k) = 0.0;
+            AT(sm[c], i, j, k) = 0.0;
+          }
+  }
+  if (s->has_dft) {
+    size_t sh[3];
+    field_shape(s, s->dft_comp, sh);
+    size_t lim1 = sh[s->dft_axis == 0 ? 1 : 0], lim2 = sh[s->dft_axis == 2 ? 1 : 2];
+    if (s->dft_index >= sh[s->dft_axis] || s->dft_hi1 > lim1 || s->dft_hi2 > lim2 ||
+        s->dft_lo1 >= s->dft_hi1 || s->dft_lo2 >= s->dft_hi2)
+      FAIL("DFT plane outside the component array");
+    s->dft_acc = (double complex *)calloc((s->dft_hi1 - s->dft_lo1) * (s->dft_hi2 - s->dft_lo2),
+                                          sizeof(double complex));
+    if (!s->dft_acc) FAIL("out of memory");
+  }
   if (s->has_tfsf && init_tfsf(s)) return -1;
   if (s->has_nf && init_nffft(s)) return -1;
   for (int q = 0; q < s->nsrc; ++q)
@@ -1218,6 +1256,32 @@ int xo_sim_init(xo_sim *s, size_t total_steps) {
   }
   s->inited = 1;
   return 0;
+}
+
+/* ------------------------------------------------------ DFT plane monitor */
+
+/* element (u, v) of the monitor plane: axis a fixed at dft_index, the other
+   two axes in increasing order */
+static double *dft_cell(a3 *f, int a, size_t idx, size_t u, size_t v) {
+  if (a == 0) return &AT(*f, idx, u, v);
+  if (a == 1) return &AT(*f, u, idx, v);
+  return &AT(*f, u, v, idx);
+}
+
+/* weight of the sample taken after step t: the magnetic-current weight of
+   nffft_fd.cpp:102-103, dt exp(-i 2 pi f t dt) */
+static double complex dft_weight(const xo_sim *s, size_t t) {
+  double ph = -2 * XO_PI * s->dft_freq * ((double)t * s->dt);
+  return s->dt * (cos(ph) + I * sin(ph));
+}
+
+static void dft_update(xo_sim *s, size_t t) {
+  double complex w = dft_weight(s, t);
+  size_t n2 = s->dft_hi2 - s->dft_lo2;
+  for (size_t u = s->dft_lo1; u < s->dft_hi1; ++u)
+    for (size_t v = s->dft_lo2; v < s->dft_hi2; ++v)
+      s->dft_acc[(u - s->dft_lo1) * n2 + (v - s->dft_lo2)] +=
+          *dft_cell(&s->f[s->dft_comp], s->dft_axis, s->dft_index, u, v) * w;
 }
 
 /* ----------------------------------------------------------------- step */
@@ -1283,6 +1347,7 @@ int xo_sim_step(xo_sim *s, size_t nsteps) {
     if (s->has_tfsf) tfsf_update_h(s);
     for (int b = 0; b < s->npml; ++b) pml_update_h(s, &s->pml[b]);
     if (s->has_nf) nffft_update(s);
+    if (s->has_dft) dft_update(s, s->cur);
     for (int p = 0; p < s->nprobe; ++p) {
       const probe_t *pr = &s->probes[p];
       s->probe_buf[(size_t)p * s->T + s->cur] = AT(s->f[pr->comp], pr->i, pr->j, pr->k);
@@ -1373,5 +1438,285 @@ int xo_sim_get_nffft(const xo_sim *s, double *out) {
 int xo_sim_get_probes(const xo_sim *s, double *out) {
   if (!s->inited) FAIL("not initialised");
   memcpy(out, s->probe_buf, (size_t)s->nprobe * s->T * sizeof(double));
+  return 0;
+}
+
+/* ---------------------------------------------------------------- adjoint
+ * Discrete adjoint of the run loop with respect to the fields, and the
+ * gradient of a DFT-plane objective with respect to the permittivity of a
+ * design box.  The reference has no adjoint; this is the exact transpose of
+ * the forward operations above, undone one by one in reverse order:
+ *   forward step t: update_e, sources, TFSF-E, CPML-E (faces in order),
+ *                   update_h, incident grid, TFSF-H, CPML-H (faces in order),
+ *                   NF-FF, DFT tap
+ *   reverse step t: DFT tap^T (inject), CPML-H^T (faces reversed), update_h^T,
+ *                   CPML-E^T (faces reversed), gradient, update_e^T
+ * Sources, TFSF corrections and the incident grid add terms that do not
+ * depend on the fields, so their transposes are empty.
+ */
+
+typedef struct {
+  a3 f[6];       /* adjoint of Ex..Hz */
+  double **psi;  /* [npml][4] adjoint of each face's psi arrays */
+} adj_t;
+
+static int adj_alloc(xo_sim *s, adj_t *a) {
+  memset(a, 0, sizeof *a);
+  for (int q = 0; q < 6; ++q) {
+    size_t sh[3];
+    field_shape(s, q, sh);
+    if (a3_alloc(&a->f[q], sh[0], sh[1], sh[2], 0.0)) FAIL("out of memory");
+  }
+  a->psi = (double **)calloc((size_t)(s->npml ? s->npml : 1) * 4, sizeof(double *));
+  if (!a->psi) FAIL("out of memory");
+  for (int b = 0; b < s->npml; ++b)
+    for (int q = 0; q < 4; ++q) {
+      size_t sz = (size_t)s->pml[b].d * s->pml[b].tn1[q] * s->pml[b].tn2[q];
+      a->psi[4 * b + q] = (double *)calloc(sz ? sz : 1, sizeof(double));
+      if (!a->psi[4 * b + q]) FAIL("out of memory");
+    }
+  return 0;
+}
+
+static void adj_free(xo_sim *s, adj_t *a) {
+  for (int q = 0; q < 6; ++q) a3_free(&a->f[q]);
+  if (a->psi)
+    for (int b = 0; b < 4 * s->npml; ++b) free(a->psi[b]);
+  free(a->psi);
+}
+
+/* transpose of pml_update_h: psi' = bm psi + am (e(n+1) - e(n)); h(n) += cpsi psi' */
+static void adj_pml_h(xo_sim *s, pml_t *p, double **pbar, adj_t *ad) {
+  int a = p->ori / 2;
+  for (int q = 2; q <= 3; ++q) {
+    a3 *hb = &ad->f[PML_COMP[a][q]];
+    a3 *eb = &ad->f[PML_COMP[a][q == 3 ? 0 : 1]];
+    size_t t1 = p->tn1[q], t2 = p->tn2[q];
+    for (int l = 0; l < p->d; ++l) {
+      size_t n = p->start + (size_t)l;
+      for (size_t u = 0; u < t1; ++u)
+        for (size_t v = 0; v < t2; ++v) {
+          size_t pi = ((size_t)l * t1 + u) * t2 + v;
+          double chi = pbar[q][pi] + p->cpsi[q][pi] * *fa_at(hb, a, n, u, v);
+          *fa_at(eb, a, n + 1, u, v) += p->am[l] * chi;
+          *fa_at(eb, a, n, u, v) -= p->am[l] * chi;
+          pbar[q][pi] = p->bm[l] * chi;
+        }
+    }
+  }
+  (void)s;
+}
+
+/* transpose of pml_update_e: psi' = be psi + ae (h(n) - h(n-1)); e(n) += cpsi psi' */
+static void adj_pml_e(xo_sim *s, pml_t *p, double **pbar, adj_t *ad) {
+  int a = p->ori / 2;
+  int positive = p->ori % 2;
+  for (int q = 0; q < 2; ++q) {
+    a3 *eb = &ad->f[PML_COMP[a][q]];
+    a3 *hb = &ad->f[PML_COMP[a][q == 0 ? 3 : 2]];
+    size_t t1 = p->tn1[q], t2 = p->tn2[q];
+    for (int l = 0; l < p->d; ++l) {
+      size_t n = p->start + (size_t)l + (positive ? 0 : 1);
+      for (size_t u = 0; u < t1; ++u)
+        for (size_t v = 0; v < t2; ++v) {
+          size_t pi = ((size_t)l * t1 + u) * t2 + v;
+          double chi = pbar[q][pi] + p->cpsi[q][pi] * *fa_at(eb, a, n, u, v);
+          *fa_at(hb, a, n, u, v) += p->ae[l] * chi;
+          *fa_at(hb, a, n - 1, u, v) -= p->ae[l] * chi;
+          pbar[q][pi] = p->be[l] * chi;
+        }
+    }
+  }
+  (void)s;
+}
+
+#define AEX(i, j, k) AT(ad->f[XO_EX], i, j, k)
+#define AEY(i, j, k) AT(ad->f[XO_EY], i, j, k)
+#define AEZ(i, j, k) AT(ad->f[XO_EZ], i, j, k)
+#define AHX(i, j, k) AT(ad->f[XO_HX], i, j, k)
+#define AHY(i, j, k) AT(ad->f[XO_HY], i, j, k)
+#define AHZ(i, j, k) AT(ad->f[XO_HZ], i, j, k)
+
+/* transpose of update_h (all three components over i<nx, j<ny, k<nz) */
+static void adj_update_h(xo_sim *s, adj_t *ad) {
+  size_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
+  a3 *c = s->c;
+  for (size_t i = 0; i < nx; ++i)
+    for (size_t j = 0; j < ny; ++j)
+      for (size_t k = 0; k < nz; ++k) {
+        double ax = AHX(i, j, k), ay = AHY(i, j, k), az = AHZ(i, j, k);
+        double w;
+        w = AT(c[CHXEY], i, j, k) * ax; AEY(i, j, k + 1) += w; AEY(i, j, k) -= w;
+        w = AT(c[CHXEZ], i, j, k) * ax; AEZ(i, j + 1, k) += w; AEZ(i, j, k) -= w;
+        w = AT(c[CHYEZ], i, j, k) * ay; AEZ(i + 1, j, k) += w; AEZ(i, j, k) -= w;
+        w = AT(c[CHYEX], i, j, k) * ay; AEX(i, j, k + 1) += w; AEX(i, j, k) -= w;
+        w = AT(c[CHZEX], i, j, k) * az; AEX(i, j + 1, k) += w; AEX(i, j, k) -= w;
+        w = AT(c[CHZEY], i, j, k) * az; AEY(i + 1, j, k) += w; AEY(i, j, k) -= w;
+        AHX(i, j, k) = AT(c[CHXH], i, j, k) * ax;
+        AHY(i, j, k) = AT(c[CHYH], i, j, k) * ay;
+        AHZ(i, j, k) = AT(c[CHZH], i, j, k) * az;
+      }
+}
+
+/* transpose of update_e (each component over its Yee range) */
+static void adj_update_e(xo_sim *s, adj_t *ad) {
+  size_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
+  a3 *c = s->c;
+  double w;
+  for (size_t i = 0; i < nx; ++i)
+    for (size_t j = 1; j < ny; ++j)
+      for (size_t k = 1; k < nz; ++k) {
+        double a = AEX(i, j, k);
+        w = AT(c[CEXHY], i, j, k) * a; AHY(i, j, k) += w; AHY(i, j, k - 1) -= w;
+        w = AT(c[CEXHZ], i, j, k) * a; AHZ(i, j, k) += w; AHZ(i, j - 1, k) -= w;
+        AEX(i, j, k) = AT(c[CEXE], i, j, k) * a;
+      }
+  for (size_t j = 0; j < ny; ++j)
+    for (size_t k = 1; k < nz; ++k)
+      for (size_t i = 1; i < nx; ++i) {
+        double a = AEY(i, j, k);
+        w = AT(c[CEYHX], i, j, k) * a; AHX(i, j, k) += w; AHX(i, j, k - 1) -= w;
+        w = AT(c[CEYHZ], i, j, k) * a; AHZ(i, j, k) += w; AHZ(i - 1, j, k) -= w;
+        AEY(i, j, k) = AT(c[CEYE], i, j, k) * a;
+      }
+  for (size_t k = 0; k < nz; ++k)
+    for (size_t i = 1; i < nx; ++i)
+      for (size_t j = 1; j < ny; ++j) {
+        double a = AEZ(i, j, k);
+        w = AT(c[CEZHX], i, j, k) * a; AHX(i, j, k) += w; AHX(i, j - 1, k) -= w;
+        w = AT(c[CEZHY], i, j, k) * a; AHY(i, j, k) += w; AHY(i - 1, j, k) -= w;
+        AEZ(i, j, k) = AT(c[CEZE], i, j, k) * a;
+      }
+}
+
+/* the H half of reverse step: CPML-H^T (faces reversed), update_h^T */
+static void adj_half_h(xo_sim *s, adj_t *ad) {
+  for (int b = s->npml - 1; b >= 0; --b) adj_pml_h(s, &s->pml[b], ad->psi + 4 * b, ad);
+  adj_update_h(s, ad);
+}
+/* CPML-E^T (faces reversed); update_e^T follows separately */
+static void adj_pml_e_all(xo_sim *s, adj_t *ad) {
+  for (int b = s->npml - 1; b >= 0; --b) adj_pml_e(s, &s->pml[b], ad->psi + 4 * b, ad);
+}
+
+int xo_sim_set_design(xo_sim *s, const size_t box[6], const float *eps_r) {
+  if (s->inited) FAIL("simulation already initialised");
+  if (box[1] <= box[0] || box[3] <= box[2] || box[5] <= box[4]) FAIL("empty design box");
+  size_t nd = (box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4]);
+  free(s->design_eps);
+  s->design_eps = (float *)malloc(nd * sizeof(float));
+  if (!s->design_eps) FAIL("out of memory");
+  memcpy(s->design_eps, eps_r, nd * sizeof(float));
+  memcpy(s->dbox, box, sizeof s->dbox);
+  s->has_design = 1;
+  return 0;
+}
+
+int xo_sim_set_dft_plane(xo_sim *s, int comp, int axis, size_t index, size_t lo1, size_t hi1,
+                         size_t lo2, size_t hi2, double freq) {
+  if (s->inited) FAIL("simulation already initialised");
+  if (comp < 0 || comp > 5 || axis < 0 || axis > 2) FAIL("bad DFT plane");
+  s->dft_comp = comp; s->dft_axis = axis; s->dft_index = index;
+  s->dft_lo1 = lo1; s->dft_hi1 = hi1; s->dft_lo2 = lo2; s->dft_hi2 = hi2;
+  s->dft_freq = freq;
+  s->has_dft = 1;
+  return 0;
+}
+
+int xo_sim_get_dft(const xo_sim *s, double *out) {
+  if (!s->inited || !s->has_dft) FAIL("no DFT plane");
+  size_t n = (s->dft_hi1 - s->dft_lo1) * (s->dft_hi2 - s->dft_lo2);
+  memcpy(out, s->dft_acc, n * sizeof(double complex));
+  return 0;
+}
+
+static double dft_objective(const xo_sim *s) {
+  size_t n = (s->dft_hi1 - s->dft_lo1) * (s->dft_hi2 - s->dft_lo2);
+  double J = 0;
+  for (size_t q = 0; q < n; ++q) J += creal(s->dft_acc[q]) * creal(s->dft_acc[q]) +
+                                     cimag(s->dft_acc[q]) * cimag(s->dft_acc[q]);
+  return J;
+}
+
+/* dJ/d eps_r of every design cell (C order over the box) and J, for
+   J = sum |F|^2 over the DFT plane after total_steps steps.  Runs the
+   forward loop from step 0, keeping the design-box E fields of every step,
+   then the reverse loop. */
+int xo_sim_grad(xo_sim *s, double *grad, double *J_out) {
+  if (!s->inited || s->cur != 0) FAIL("grad needs a fresh simulation");
+  if (!s->has_design || !s->has_dft) FAIL("grad needs a design box and a DFT plane");
+  const size_t *b = s->dbox;
+  size_t nd = (b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]), T = s->T;
+  double *hist = (double *)calloc((T + 1) * nd * 3, sizeof(double));
+  if (!hist) FAIL("out of memory");
+#define HIST(t, q, c) hist[((size_t)(t) * nd + (q)) * 3 + (c)]
+  for (size_t t = 0; t < T; ++t) {
+    if (xo_sim_step(s, 1)) { free(hist); return -1; }
+    size_t q = 0;
+    for (size_t i = b[0]; i < b[1]; ++i)
+      for (size_t j = b[2]; j < b[3]; ++j)
+        for (size_t k = b[4]; k < b[5]; ++k, ++q)
+          for (int c = 0; c < 3; ++c) HIST(t + 1, q, c) = AT(s->f[c], i, j, k);
+  }
+  *J_out = dft_objective(s);
+  adj_t ad;
+  if (adj_alloc(s, &ad)) { free(hist); return -1; }
+  memset(grad, 0, nd * sizeof(double));
+  size_t n2 = s->dft_hi2 - s->dft_lo2;
+  for (size_t tt = T; tt-- > 0;) {
+    /* DFT tap^T: dJ/dE = 2 Re(conj(F) w_t) on the plane */
+    double complex w = dft_weight(s, tt);
+    for (size_t u = s->dft_lo1; u < s->dft_hi1; ++u)
+      for (size_t v = s->dft_lo2; v < s->dft_hi2; ++v) {
+        double complex F = s->dft_acc[(u - s->dft_lo1) * n2 + (v - s->dft_lo2)];
+        *dft_cell(&ad.f[s->dft_comp], s->dft_axis, s->dft_index, u, v) += 2 * creal(conj(F) * w);
+      }
+    adj_half_h(s, &ad);
+    adj_pml_e_all(s, &ad);
+    /* the design cells' E coefficients depend on eps: with sigma = 0,
+       d E'/d eps = -(E' - E)/eps, and d eps/d eps_r = eps_0 */
+    size_t q = 0;
+    for (size_t i = b[0]; i < b[1]; ++i)
+      for (size_t j = b[2]; j < b[3]; ++j)
+        for (size_t k = b[4]; k < b[5]; ++k, ++q) {
+          double er = (double)s->design_eps[q], acc = 0;
+          for (int c = 0; c < 3; ++c)
+            acc += AT(ad.f[c], i, j, k) * (HIST(tt + 1, q, c) - HIST(tt, q, c));
+          grad[q] -= acc / er;
+        }
+    adj_update_e(s, &ad);
+  }
+#undef HIST
+  adj_free(s, &ad);
+  free(hist);
+  return 0;
+}
+
+/* Reverse-run `nsteps` steps of the field map from a given adjoint state
+   (the six adjoint fields packed in component order, psi adjoints zero),
+   with no objective; the result overwrites `fields`.  For the identity
+   <L x, y> = <x, L^T y> of the step map L. */
+int xo_sim_adjoint_run(xo_sim *s, size_t nsteps, double *fields) {
+  if (!s->inited) FAIL("not initialised");
+  adj_t ad;
+  if (adj_alloc(s, &ad)) return -1;
+  size_t off = 0;
+  for (int q = 0; q < 6; ++q) {
+    size_t n = ad.f[q].n0 * ad.f[q].n1 * ad.f[q].n2;
+    memcpy(ad.f[q].d, fields + off, n * sizeof(double));
+    off += n;
+  }
+  for (size_t t = 0; t < nsteps; ++t) {
+    adj_half_h(s, &ad);
+    adj_pml_e_all(s, &ad);
+    adj_update_e(s, &ad);
+  }
+  off = 0;
+  for (int q = 0; q < 6; ++q) {
+    size_t n = ad.f[q].n0 * ad.f[q].n1 * ad.f[q].n2;
+    memcpy(fields + off, ad.f[q].d, n * sizeof(double));
+    off += n;
+  }
+  adj_free(s, &ad);
   return 0;
 }

@@ -78,13 +78,29 @@ int xo_sim_get_waveform(const xo_sim *s, double *out /* total_steps */);
 /* NF-FF accumulators, 24 currents in the order documented in xfdtd_oracle.c */
 size_t xo_sim_nffft_size(const xo_sim *s);
 int xo_sim_get_nffft(const xo_sim *s, double *out /* interleaved re,im */);
-/* far field: out holds 10 complex arrays (n_theta, n_phi, l_theta, l_phi,
+/* far field: out holds 8 complex arrays (n_theta, n_phi, l_theta, l_phi,
    e_theta, e_phi, h_theta, h_phi) + 2 real (power_theta, power_phi), each of
    nf*ntheta*nphi, packed as documented in xfdtd_oracle.c */
 int xo_sim_farfield(xo_sim *s, double *out);
 int xo_sim_get_probes(const xo_sim *s, double *out /* nprobe * total_steps */);
 size_t xo_sim_current_step(const xo_sim *s);
 int xo_set_threads(int n);
+
+/* Adjoint (no reference counterpart; the checker of the GPU adjoint).
+   Design box [i0,i1)x[j0,j1)x[k0,k1) with per-cell eps_r (float, C order),
+   lossless, mu_r = 1, applied after the objects. */
+int xo_sim_set_design(xo_sim *s, const size_t box[6], const float *eps_r);
+/* DFT monitor of component comp on the plane axis = index, the two other
+   axes in [lo1,hi1) x [lo2,hi2) (increasing axis order), sampled after every
+   step with weight dt exp(-i 2 pi f t dt) */
+int xo_sim_set_dft_plane(xo_sim *s, int comp, int axis, size_t index, size_t lo1, size_t hi1,
+                         size_t lo2, size_t hi2, double freq);
+int xo_sim_get_dft(const xo_sim *s, double *out /* interleaved complex */);
+/* forward run from step 0 to total_steps, then the reverse run: J = sum |F|^2
+   and dJ/d eps_r per design cell */
+int xo_sim_grad(xo_sim *s, double *grad, double *J);
+/* reverse-run nsteps of the field map on packed adjoint fields (in place) */
+int xo_sim_adjoint_run(xo_sim *s, size_t nsteps, double *fields);
 
 #ifdef __cplusplus
 }

@@ -135,6 +135,33 @@ class Probe:
 
 
 @dataclass
+class Design:
+    """Per-cell relative permittivity of a box of cells [i0,i1)x[j0,j1)x[k0,k1)
+    (grid indices), lossless with mu_r = 1, applied after the objects, as one
+    one-cell Object per cell would be (object/object.cpp:53-73).  eps_r is
+    rounded to float like a Material's."""
+    box: Tuple[int, int, int, int, int, int]
+    eps_r: Sequence[float]  # C order over the box
+
+
+@dataclass
+class DftPlane:
+    """Running DFT of one field component on a plane: axis fixed at `index`,
+    the other two axes (in increasing order) over [lo1,hi1) x [lo2,hi2).
+    After step t the sample is weighted by dt exp(-i 2 pi f t dt), the
+    magnetic-current weight of nffft_fd.cpp:102-103.  The objective of the
+    adjoint is J = sum |F|^2 over the plane."""
+    comp: int
+    axis: int
+    index: int
+    lo1: int
+    hi1: int
+    lo2: int
+    hi2: int
+    freq: float
+
+
+@dataclass
 class Scene:
     cell_size: float
     cfl: float = 0.99
@@ -144,6 +171,8 @@ class Scene:
     nffft: Optional[NffftFd] = None
     sources: List[CurrentSource] = field(default_factory=list)
     probes: List[Probe] = field(default_factory=list)
+    design: Optional[Design] = None
+    dft_plane: Optional[DftPlane] = None
 
     def add_object(self, o: Object):
         self.objects.append(o)
