@@ -341,6 +341,26 @@ class Simulation {
   void addNFFFT(std::shared_ptr<NFFFT> nffft);
   void addProbe(Probe probe);
 
+  // ---- differentiation (no counterpart in the reference)
+  // Per-cell relative permittivity of the cells [box[0],box[1]) x
+  // [box[2],box[3]) x [box[4],box[5]) (grid indices), lossless, mu_r = 1,
+  // applied after the objects as one-cell Objects would be; eps_r in C order.
+  void setDesign(const std::array<size_t, 6> &box, std::vector<float> eps_r);
+  // running DFT of one component on a plane (axis fixed at `index`, the
+  // other two axes in increasing order over [lo1,hi1) x [lo2,hi2)), sample
+  // after step t weighted by dt exp(-i 2 pi f t dt); J = sum |F|^2
+  void setDftPlane(EMComponent component, int axis, size_t index, size_t lo1, size_t hi1, size_t lo2,
+                   size_t hi2, double frequency);
+  std::vector<std::complex<double>> getDft() const;
+  // J and dJ/d eps_r per design cell: forward run from step 0 with a state
+  // checkpoint every `checkpoint_every` steps, then the discrete adjoint
+  // back through each recomputed segment
+  double gradEps(std::vector<double> &grad, int checkpoint_every = 100);
+  // transpose of n steps of the field map on packed adjoint fields
+  void adjointRun(size_t n, std::vector<double> &fields);
+  void checkpoint(int slot);
+  void restore(int slot);
+
   // Simulation::run (simulation_update.cpp:5-40): init, the time loop on the
   // GPU, then NF-FF output when an output directory is set.
   void run(size_t total_time_steps);
@@ -390,6 +410,13 @@ class Simulation {
   std::shared_ptr<TFSF> _tfsf;
   std::shared_ptr<NFFFT> _nffft;
   std::vector<Probe> _probes;
+  bool _has_design = false;
+  std::array<size_t, 6> _design_box{};
+  std::vector<float> _design_eps;
+  bool _has_dft = false;
+  int _dft_comp = 0, _dft_axis = 0;
+  size_t _dft_plane[5] = {};  // index, lo1, hi1, lo2, hi2
+  double _dft_freq = 0;
 
   // GridSpace
   double _min[3], _max[3];

@@ -182,6 +182,40 @@ xf_status xf_engine_set_timing(xf_engine *e, int enabled);
 enum { XF_TILE_GLOBAL_TABLES = 2, XF_TILE_X_FASTEST = 4, XF_TILE_XCD_RUNS = 8, XF_TILE_XCD_COLS = 16 };
 xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags);
 
+/* ---- Differentiation (no counterpart in the reference, which runs forward
+   only).  A design box of cells whose permittivity is a parameter, a DFT
+   monitor whose energy is the objective, state checkpoints, and the
+   discrete adjoint. */
+
+/* Design box [i0,i1)x[j0,j1)x[k0,k1) (box[6]) inside the grid and outside
+   every CPML slab (call after xf_engine_set_cpml).  coef: [cell][9] E
+   coefficients of each design cell in the order cexe, ceye, ceze, cexhy,
+   ceyhx, cexhz, cezhx, ceyhz, cezhy (the reference's per-cell arrays,
+   fdtd_basic_coff.cpp:77-91, for a lossless cell); eps_r: the cells'
+   relative permittivity (for the gradient).  Cells in C order. */
+xf_status xf_engine_set_design(xf_engine *e, const int64_t *box, const double *coef, const double *eps_r);
+/* DFT monitor of component comp on the plane axis = index, the other two
+   axes (increasing order) over [lo1,hi1) x [lo2,hi2): after step t,
+   F += f * w[t] in FP64.  w: interleaved complex [total_steps]. */
+xf_status xf_engine_set_dft_plane(xf_engine *e, int comp, int axis, int64_t index, int64_t lo1, int64_t hi1,
+                                  int64_t lo2, int64_t hi2, const double *w, int64_t total_steps);
+xf_status xf_engine_get_dft(xf_engine *e, double *out /* interleaved complex */);
+/* copy of the whole time-loop state (fields, psi, incident grid,
+   accumulators, step counter) into / back from a numbered slot */
+xf_status xf_engine_checkpoint(xf_engine *e, int slot);
+xf_status xf_engine_restore(xf_engine *e, int slot);
+/* J = sum |F|^2 after the full run and dJ/d eps_r of every design cell
+   (C order).  Runs the forward loop from step 0 with a checkpoint every
+   `checkpoint_every` steps, then the reverse loop segment by segment: each
+   segment is recomputed from its checkpoint, keeping the design-box E
+   fields, and the adjoint kernels step back through it.  The engine is left
+   at the end of the forward run.  Design cells must be lossless. */
+xf_status xf_engine_grad_eps(xf_engine *e, int checkpoint_every, double *grad, double *J);
+/* the transpose of nsteps steps of the field map applied to six adjoint
+   fields (packed in component order, reference shapes), in place; the psi
+   adjoints start at zero.  For checking <L x, y> = <x, L^T y>. */
+xf_status xf_engine_adjoint_run(xf_engine *e, int64_t nsteps, double *fields);
+
 /* Batches: several set-up engines of one precision, all at the same step,
    stepped together.  Each per-step kernel (curl + CPML, TFSF faces, 1-D
    incident grids, NF-FF) is one launch over every instance, so a batch of

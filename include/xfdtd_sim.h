@@ -84,6 +84,20 @@ int xf_sim_get_probes(xf_sim *s, double *out);
 /* the engine behind the simulation (valid after xf_sim_init) */
 struct xf_engine *xf_sim_engine(xf_sim *s);
 
+/* Differentiation (no counterpart in the reference; see xfdtd_engine.h).
+   Before xf_sim_init: a design box of per-cell eps_r (lossless, applied after
+   the objects) and a DFT monitor plane whose sum |F|^2 is the objective. */
+int xf_sim_set_design(xf_sim *s, const size_t box[6], const float *eps_r);
+int xf_sim_set_dft_plane(xf_sim *s, int comp, int axis, size_t index, size_t lo1, size_t hi1, size_t lo2,
+                         size_t hi2, double freq);
+int xf_sim_get_dft(xf_sim *s, double *out);
+/* J and dJ/d eps_r per design cell, from step 0 (xf_engine_grad_eps) */
+int xf_sim_grad_eps(xf_sim *s, int checkpoint_every, double *grad, double *J);
+/* transpose of nsteps steps on n packed adjoint field values, in place */
+int xf_sim_adjoint_run(xf_sim *s, size_t nsteps, double *fields, size_t n);
+int xf_sim_checkpoint(xf_sim *s, int slot);
+int xf_sim_restore(xf_sim *s, int slot);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,22 @@ def test_plan_reproduces_reference_coefficient_arrays(builder):
         assert np.array_equal(a, b), (w, np.abs(a - b).max())
 
 
+def test_plan_design_box_coefficients():
+    """Per-cell design permittivity: the plan's E arrays at design cells (and
+    everything else) equal the oracle's per-cell arrays bit for bit."""
+    from xfdtd_amd.sim import Simulation
+    dl = 4e-3
+    sc = S.cube_domain_scene(24, dl, 4, cfl=0.99, objects=[
+        S.Object("m", S.Cube((-0.02, -0.02, -0.02), (0.03, 0.04, 0.04)), S.Material("m", 2.0, 1.5, 0.1, 0.0))])
+    rng = np.random.default_rng(11)
+    sc.design = S.Design((8, 13, 9, 12, 7, 15), rng.uniform(1, 4, 5 * 3 * 8).astype(np.float32))
+    g = Simulation(sc, 10, plan_only=True)
+    o = OracleSim(sc, 10)
+    for w in range(18):
+        a, b = g.plan_coeff(w), o.get_coeff(w)
+        assert np.array_equal(a, b), (w, np.abs(a - b).max())
+
+
 def test_invalid_configurations_raise():
     from xfdtd_amd.sim import Simulation
     dl = 5e-3

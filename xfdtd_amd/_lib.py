@@ -22,6 +22,8 @@ ENGINE_SYMBOLS = [
     "xf_engine_current_step", "xf_engine_get_field", "xf_engine_set_field",
     "xf_engine_kernel_times", "xf_engine_set_timing", "xf_engine_set_tiling",
     "xf_batch_create", "xf_batch_step", "xf_batch_sync", "xf_batch_size", "xf_batch_destroy",
+    "xf_engine_set_design", "xf_engine_set_dft_plane", "xf_engine_get_dft", "xf_engine_checkpoint",
+    "xf_engine_restore", "xf_engine_grad_eps", "xf_engine_adjoint_run",
 ]
 SIM_SYMBOLS = [
     "xf_sim_last_error", "xf_sim_create", "xf_sim_destroy", "xf_sim_add_cube", "xf_sim_add_sphere",
@@ -29,7 +31,8 @@ SIM_SYMBOLS = [
     "xf_sim_add_probe", "xf_sim_plan", "xf_sim_plan_coeff", "xf_sim_init", "xf_sim_step", "xf_sim_sync", "xf_sim_current_step",
     "xf_sim_dims", "xf_sim_dt", "xf_sim_get_field", "xf_sim_set_field", "xf_sim_nffft_size",
     "xf_sim_get_nffft", "xf_sim_farfield", "xf_sim_output_nffft", "xf_sim_get_probes",
-    "xf_sim_engine",
+    "xf_sim_engine", "xf_sim_set_design", "xf_sim_set_dft_plane", "xf_sim_get_dft", "xf_sim_grad_eps",
+    "xf_sim_adjoint_run", "xf_sim_checkpoint", "xf_sim_restore",
 ]
 
 
@@ -89,5 +92,12 @@ def lib():
     L.xf_batch_sync.argtypes = [vp]
     L.xf_batch_size.argtypes = [vp]
     L.xf_batch_destroy.argtypes = [vp]
+    L.xf_sim_set_design.argtypes = [vp, szp, C.POINTER(C.c_float)]
+    L.xf_sim_set_dft_plane.argtypes = [vp, i, i, sz, sz, sz, sz, sz, d]
+    L.xf_sim_get_dft.argtypes = [vp, dp]
+    L.xf_sim_grad_eps.argtypes = [vp, i, dp, dp]
+    L.xf_sim_adjoint_run.argtypes = [vp, sz, dp, sz]
+    L.xf_sim_checkpoint.argtypes = [vp, i]
+    L.xf_sim_restore.argtypes = [vp, i]
     _LIB = L
     return L

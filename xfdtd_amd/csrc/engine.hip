@@ -95,6 +95,22 @@ struct xf_engine {
   // stream of the batch this engine belongs to (work on it precedes any use
   // of the engine's own stream)
   hipStream_t batch_stream = nullptr;
+  // every buffer that is state of the time loop (fields, psi, incident
+  // grid, accumulators): what a checkpoint copies
+  std::vector<std::pair<void *, size_t>> state;
+  std::vector<std::vector<void *>> ckpt;
+  std::vector<long long> ckpt_cur;
+  size_t psi_bytes[3] = {0, 0, 0};  // bytes of each psi array per axis
+  // design box with per-cell E coefficients
+  bool has_design = false;
+  int db[6] = {0, 0, 0, 0, 0, 0};
+  void *dcoef = nullptr;
+  double *d_eps = nullptr;
+  int nd = 0;
+  // DFT plane monitor
+  bool has_dft = false;
+  DftPlane dft{};
+  long long dft_T = 0;
 };
 
 // wait for everything enqueued for this engine, on its own stream and on
@@ -180,6 +196,7 @@ extern "C" xf_status xf_engine_create(const xf_grid_desc *g, xf_engine **out) {
   for (int q = 0; q < 6; ++q) {
     xf_status s = dev_alloc(e, &e->f[q], (size_t)e->ncell * e->esz);
     if (s != XF_OK) { xf_engine_destroy(e); return s; }
+    e->state.push_back({e->f[q], (size_t)e->ncell * e->esz});
   }
   *out = e;
   return XF_OK;
@@ -292,7 +309,10 @@ extern "C" xf_status xf_engine_set_cpml(xf_engine *e, int axis, int d_n, int d_p
     const int sl = e_slots[axis][q];
     if ((s = dev_alloc(e, &e->psi_e[sl], n * e->esz)) != XF_OK) return s;
     if ((s = dev_alloc(e, &e->psi_h[sl], n * e->esz)) != XF_OK) return s;
+    e->state.push_back({e->psi_e[sl], n * e->esz});
+    e->state.push_back({e->psi_h[sl], n * e->esz});
   }
+  e->psi_bytes[axis] = n * e->esz;
   return XF_OK;
 }
 
@@ -324,6 +344,9 @@ extern "C" xf_status xf_engine_set_tfsf(xf_engine *e, const xf_tfsf_desc *d) {
   if ((s = dev_alloc(e, &e->e_inc, (size_t)e->L * e->esz)) != XF_OK) return s;
   if ((s = dev_alloc(e, &e->h_inc, (size_t)(e->L - 1) * e->esz)) != XF_OK) return s;
   if ((s = dev_alloc(e, &e->ab, 2 * e->esz)) != XF_OK) return s;
+  e->state.push_back({e->e_inc, (size_t)e->L * e->esz});
+  e->state.push_back({e->h_inc, (size_t)(e->L - 1) * e->esz});
+  e->state.push_back({e->ab, 2 * e->esz});
   if ((s = upload_real(e, &e->src, d->source, (size_t)d->total_steps)) != XF_OK) return s;
   e->tfsf_T = d->total_steps;
   set_total(e, d->total_steps);
@@ -360,6 +383,7 @@ extern "C" xf_status xf_engine_set_nffft(xf_engine *e, const xf_nffft_desc *d) {
   void *p;
   if ((s = dev_alloc(e, &p, (size_t)e->nf_size * sizeof(double2))) != XF_OK) return s;
   e->acc = (double2 *)p;
+  e->state.push_back({p, (size_t)e->nf_size * sizeof(double2)});
   const size_t tw = (size_t)d->nf * d->total_steps;
   if ((s = dev_alloc(e, &p, tw * sizeof(double2))) != XF_OK) return s;
   e->tj = (double2 *)p;
@@ -508,6 +532,8 @@ static Grid<T> make_grid(xf_engine *e) {
   }
   g.nx = e->nx; g.ny = e->ny; g.nz = e->nz; g.pz = e->pz; g.nmat = e->nmat;
   g.sj = e->sj; g.si = e->si;
+  for (int a = 0; a < 6; ++a) g.db[a] = e->db[a];
+  g.dcoef = (const T *)e->dcoef;
   return g;
 }
 
@@ -618,6 +644,11 @@ static xf_status run_steps(xf_engine *e, long long n) {
                             (long long)e->nf.nxb * e->nf.nyb;
       hipLaunchKernelGGL(k_nffft<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, e->stream, g,
                          e->nf, step);
+    }
+    if (e->has_dft) {
+      const int n = (e->dft.hi1 - e->dft.lo1) * (e->dft.hi2 - e->dft.lo2);
+      hipLaunchKernelGGL(k_dft_plane<T>, dim3((n + 255) / 256), dim3(256), 0, e->stream,
+                         (const T *)e->f[e->dft.comp], e->si, e->sj, e->dft, step);
     }
     if (!e->probe_comp.empty()) {
       const int np = (int)e->probe_comp.size();
@@ -893,6 +924,12 @@ static xf_status batch_run(xf_batch *b, long long n) {
     }
     if (b->n_nf > 0) hipLaunchKernelGGL(kb_nffft<T>, dim3(b->n_nf), dim3(256), 0, b->stream, in, b->map_nf, step);
     for (xf_engine *e : b->e)
+      if (e->has_dft) {
+        const int nn = (e->dft.hi1 - e->dft.lo1) * (e->dft.hi2 - e->dft.lo2);
+        hipLaunchKernelGGL(k_dft_plane<T>, dim3((nn + 255) / 256), dim3(256), 0, b->stream,
+                           (const T *)e->f[e->dft.comp], e->si, e->sj, e->dft, step);
+      }
+    for (xf_engine *e : b->e)
       if (!e->probe_comp.empty()) {
         const int np = (int)e->probe_comp.size();
         hipLaunchKernelGGL(k_probe<T>, dim3((np + 63) / 64), dim3(64), 0, b->stream, make_grid<T>(e),
@@ -928,3 +965,285 @@ extern "C" xf_status xf_batch_sync(xf_batch *b) {
 }
 
 extern "C" int xf_batch_size(const xf_batch *b) { return b ? (int)b->e.size() : 0; }
+
+// ---------------------------------------------------------------------------
+// Design box, DFT plane, checkpoints and the adjoint (no reference
+// counterpart: the reference runs forward only).
+
+extern "C" xf_status xf_engine_set_design(xf_engine *e, const int64_t *box, const double *coef,
+                                          const double *eps_r) {
+  if (!e || !box || !coef || !eps_r) FAILS(XF_EINVAL, "null argument");
+  const int N[3] = {e->nx, e->ny, e->nz};
+  for (int a = 0; a < 3; ++a) {
+    const int64_t lo = box[2 * a], hi = box[2 * a + 1];
+    // inside the grid and outside every CPML slab (E layers 1..dn, N-dp..N-1)
+    if (lo < 1 || hi <= lo || hi > N[a] - 1 || lo < e->dn[a] + 1 || hi > N[a] - e->dp[a])
+      FAILS(XF_EINVAL, "design box [%lld,%lld) on axis %d must lie inside the grid and outside the CPML",
+            (long long)lo, (long long)hi, a);
+  }
+  if (e->has_design) FAILS(XF_ESTATE, "design box already set");
+  const size_t nd = (size_t)(box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4]);
+  xf_status s;
+  if ((s = upload_real(e, &e->dcoef, coef, 9 * nd)) != XF_OK) return s;
+  void *p;
+  if ((s = dev_alloc(e, &p, nd * sizeof(double))) != XF_OK) return s;
+  e->d_eps = (double *)p;
+  HIPCHK(hipMemcpy(e->d_eps, eps_r, nd * sizeof(double), hipMemcpyHostToDevice));
+  for (int a = 0; a < 6; ++a) e->db[a] = (int)box[a];
+  e->nd = (int)nd;
+  e->has_design = true;
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_set_dft_plane(xf_engine *e, int comp, int axis, int64_t index, int64_t lo1,
+                                             int64_t hi1, int64_t lo2, int64_t hi2, const double *w,
+                                             int64_t total_steps) {
+  if (!e || !w || comp < 0 || comp > 5 || axis < 0 || axis > 2) FAILS(XF_EINVAL, "bad DFT plane");
+  if (e->has_dft) FAILS(XF_ESTATE, "DFT plane already set");
+  int sh[3];
+  comp_shape(e, comp, sh);
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  if (index < 0 || index >= sh[axis] || lo1 < 0 || hi1 > sh[a1] || lo1 >= hi1 || lo2 < 0 || hi2 > sh[a2] ||
+      lo2 >= hi2)
+    FAILS(XF_EINVAL, "DFT plane outside the component array");
+  DftPlane &d = e->dft;
+  d.comp = comp; d.axis = axis; d.index = (int)index;
+  d.lo1 = (int)lo1; d.hi1 = (int)hi1; d.lo2 = (int)lo2; d.hi2 = (int)hi2;
+  const size_t n = (size_t)(hi1 - lo1) * (hi2 - lo2);
+  xf_status s;
+  void *p;
+  if ((s = dev_alloc(e, &p, n * sizeof(double2))) != XF_OK) return s;
+  d.acc = (double2 *)p;
+  e->state.push_back({p, n * sizeof(double2)});
+  if ((s = dev_alloc(e, &p, (size_t)total_steps * sizeof(double2))) != XF_OK) return s;
+  HIPCHK(hipMemcpy(p, w, (size_t)total_steps * sizeof(double2), hipMemcpyHostToDevice));
+  d.w = (const double2 *)p;
+  e->dft_T = total_steps;
+  set_total(e, total_steps);
+  e->has_dft = true;
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_get_dft(xf_engine *e, double *out) {
+  if (!e || !out || !e->has_dft) FAILS(XF_ESTATE, "no DFT plane");
+  HIPCHK(engine_wait(e));
+  const size_t n = (size_t)(e->dft.hi1 - e->dft.lo1) * (e->dft.hi2 - e->dft.lo2);
+  HIPCHK(hipMemcpy(out, e->dft.acc, n * sizeof(double2), hipMemcpyDeviceToHost));
+  return XF_OK;
+}
+
+// copies of every state buffer, owned by the caller of these helpers
+static xf_status state_save(xf_engine *e, std::vector<void *> &dst) {
+  if (dst.empty()) {
+    for (auto &b : e->state) {
+      void *p = nullptr;
+      HIPCHK(hipMalloc(&p, b.second));
+      dst.push_back(p);
+    }
+  }
+  for (size_t q = 0; q < e->state.size(); ++q)
+    HIPCHK(hipMemcpyAsync(dst[q], e->state[q].first, e->state[q].second, hipMemcpyDeviceToDevice, e->stream));
+  return XF_OK;
+}
+static xf_status state_load(xf_engine *e, const std::vector<void *> &src) {
+  for (size_t q = 0; q < e->state.size(); ++q)
+    HIPCHK(hipMemcpyAsync(e->state[q].first, src[q], e->state[q].second, hipMemcpyDeviceToDevice, e->stream));
+  return XF_OK;
+}
+static void state_free(std::vector<void *> &v) {
+  for (void *p : v) hipFree(p);
+  v.clear();
+}
+
+extern "C" xf_status xf_engine_checkpoint(xf_engine *e, int slot) {
+  if (!e || slot < 0 || slot > 4095) FAILS(XF_EINVAL, "bad checkpoint slot");
+  HIPCHK(engine_wait(e));
+  if ((int)e->ckpt.size() <= slot) {
+    e->ckpt.resize(slot + 1);
+    e->ckpt_cur.resize(slot + 1, -1);
+  }
+  if (!e->ckpt[slot].empty() && e->ckpt[slot].size() != e->state.size()) state_free(e->ckpt[slot]);
+  xf_status s = state_save(e, e->ckpt[slot]);
+  if (s != XF_OK) return s;
+  e->ckpt_cur[slot] = e->cur;
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_restore(xf_engine *e, int slot) {
+  if (!e || slot < 0 || slot >= (int)e->ckpt.size() || e->ckpt[slot].size() != e->state.size())
+    FAILS(XF_EINVAL, "no checkpoint in slot %d", slot);
+  HIPCHK(engine_wait(e));
+  xf_status s = state_load(e, e->ckpt[slot]);
+  if (s != XF_OK) return s;
+  e->cur = e->ckpt_cur[slot];
+  return XF_OK;
+}
+
+// device buffers of one adjoint run
+struct AdjBufs {
+  std::vector<void *> mem;
+  void *alloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) return nullptr;
+    if (hipMemset(p, 0, bytes ? bytes : 16) != hipSuccess) { hipFree(p); return nullptr; }
+    mem.push_back(p);
+    return p;
+  }
+  ~AdjBufs() {
+    for (void *p : mem) hipFree(p);
+  }
+};
+
+template <typename T>
+static xf_status adj_setup(xf_engine *e, AdjBufs &B, Adj<T> &a) {
+  std::memset(&a, 0, sizeof a);
+  const size_t fb = (size_t)e->ncell * sizeof(T);
+  for (int q = 0; q < 3; ++q) {
+    if (!(a.Ea[q] = (T *)B.alloc(fb)) || !(a.Ha[q] = (T *)B.alloc(fb))) FAILS(XF_ENOMEM, "adjoint fields");
+  }
+  for (int q = 0; q < 6; ++q) {
+    if (!(a.W[q] = (T *)B.alloc(fb)) || !(a.V[q] = (T *)B.alloc(fb))) FAILS(XF_ENOMEM, "adjoint weights");
+  }
+  // psi adjoints: the same layouts as psi (slots of the y, z, x axes)
+  static const int slot_axis[6] = {1, 2, 0, 2, 0, 1};  // P_AY, P_AZ, P_BX, P_BZ, P_CX, P_CY
+  for (int q = 0; q < 6; ++q) {
+    const size_t nb = e->psi_bytes[slot_axis[q]];
+    if (!(a.Pe[q] = (T *)B.alloc(nb)) || !(a.Ph[q] = (T *)B.alloc(nb))) FAILS(XF_ENOMEM, "adjoint psi");
+  }
+  return XF_OK;
+}
+
+template <typename T>
+static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
+  const long long T_ = e->total;
+  const int nseg = (int)((T_ + K - 1) / K);
+  std::vector<std::vector<void *>> ck(nseg + 1);
+  struct Free {
+    std::vector<std::vector<void *>> &v;
+    ~Free() { for (auto &x : v) state_free(x); }
+  } free_ck{ck};
+  // forward run with a checkpoint at the start of every segment and at the end
+  for (int sgi = 0; sgi < nseg; ++sgi) {
+    xf_status s = state_save(e, ck[sgi]);
+    if (s != XF_OK) return s;
+    const long long n = std::min<long long>(K, T_ - (long long)sgi * K);
+    if ((s = run_steps<T>(e, n)) != XF_OK) return s;
+  }
+  xf_status s = state_save(e, ck[nseg]);
+  if (s != XF_OK) return s;
+  // objective from the final DFT accumulators
+  const DftPlane dpl = e->dft;
+  const size_t np = (size_t)(dpl.hi1 - dpl.lo1) * (dpl.hi2 - dpl.lo2);
+  AdjBufs B;
+  double2 *Ffin = (double2 *)B.alloc(np * sizeof(double2));
+  if (!Ffin) FAILS(XF_ENOMEM, "adjoint buffers");
+  HIPCHK(hipMemcpyAsync(Ffin, dpl.acc, np * sizeof(double2), hipMemcpyDeviceToDevice, e->stream));
+  std::vector<double2> F(np);
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipMemcpy(F.data(), Ffin, np * sizeof(double2), hipMemcpyDeviceToHost));
+  double Jv = 0;
+  for (const double2 &f : F) Jv += f.x * f.x + f.y * f.y;
+  *J = Jv;
+  Adj<T> a;
+  if ((s = adj_setup<T>(e, B, a)) != XF_OK) return s;
+  T *hist = (T *)B.alloc((size_t)(K + 1) * e->nd * 3 * sizeof(T));
+  double *g = (double *)B.alloc((size_t)e->nd * sizeof(double));
+  if (!hist || !g) FAILS(XF_ENOMEM, "adjoint buffers");
+  a.dft = dpl;
+  a.dft.acc = Ffin;
+  a.inject = dpl.comp < 3;
+  a.hist = hist;
+  a.eps_r = e->d_eps;
+  a.grad = g;
+  a.grad_on = 1;
+  const Grid<T> gr = make_grid<T>(e);
+  const unsigned nb = (unsigned)((e->ncell + 255) / 256), nsnap = (unsigned)((e->nd + 255) / 256);
+  for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+    // recompute the segment from its checkpoint, keeping the design E fields
+    if ((s = state_load(e, ck[sgi])) != XF_OK) return s;
+    e->cur = (long long)sgi * K;
+    const long long t0 = e->cur, n = std::min<long long>(K, T_ - t0);
+    hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, 0);
+    for (long long l = 0; l < n; ++l) {
+      if ((s = run_steps<T>(e, 1)) != XF_OK) return s;
+      hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, (int)(l + 1));
+    }
+    // reverse steps of the segment
+    for (long long l = n - 1; l >= 0; --l) {
+      hipLaunchKernelGGL(k_adj_e<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, t0 + l, (int)l);
+      hipLaunchKernelGGL(k_adj_h<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, 1);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  // leave the engine at the end of the forward run
+  if ((s = state_load(e, ck[nseg])) != XF_OK) return s;
+  e->cur = T_;
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipMemcpy(grad, g, (size_t)e->nd * sizeof(double), hipMemcpyDeviceToHost));
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_grad_eps(xf_engine *e, int checkpoint_every, double *grad, double *J) {
+  if (!e || !grad || !J || checkpoint_every < 1) FAILS(XF_EINVAL, "bad argument");
+  if (!e->materials_set || !e->has_design || !e->has_dft)
+    FAILS(XF_ESTATE, "the gradient needs materials, a design box and a DFT plane");
+  if (e->cur != 0) FAILS(XF_ESTATE, "the gradient runs from step 0 (engine is at step %lld)", e->cur);
+  if (e->total < 1) FAILS(XF_ESTATE, "no step horizon");
+  if (!e->probe_comp.empty()) {
+    xf_status s = ensure_probes(e, e->total);
+    if (s != XF_OK) return s;
+  }
+  HIPCHK(engine_wait(e));
+  return e->prec == 64 ? grad_run<double>(e, checkpoint_every, grad, J)
+                       : grad_run<float>(e, checkpoint_every, grad, J);
+}
+
+template <typename T>
+static xf_status adjoint_run(xf_engine *e, long long nsteps, double *fields) {
+  AdjBufs B;
+  Adj<T> a;
+  xf_status s = adj_setup<T>(e, B, a);
+  if (s != XF_OK) return s;
+  // upload the adjoint fields into the padded layout
+  std::vector<T> buf((size_t)e->ncell);
+  size_t off = 0;
+  for (int q = 0; q < 6; ++q) {
+    int sh[3];
+    comp_shape(e, q, sh);
+    std::fill(buf.begin(), buf.end(), T(0));
+    for (int i = 0; i < sh[0]; ++i)
+      for (int j = 0; j < sh[1]; ++j)
+        for (int k = 0; k < sh[2]; ++k) buf[(size_t)i * e->si + (size_t)j * e->sj + k] = (T)fields[off++];
+    HIPCHK(hipMemcpy(q < 3 ? (void *)a.Ea[q] : (void *)a.Ha[q - 3], buf.data(), buf.size() * sizeof(T),
+                     hipMemcpyHostToDevice));
+  }
+  const Grid<T> gr = make_grid<T>(e);
+  const unsigned nb = (unsigned)((e->ncell + 255) / 256);
+  // the H transpose of the last step, then E and H transposes per step; the
+  // last H kernel only gathers
+  hipLaunchKernelGGL(k_adj_h<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, 1);
+  for (long long t = 0; t < nsteps; ++t) {
+    hipLaunchKernelGGL(k_adj_e<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, 0LL, 0);
+    hipLaunchKernelGGL(k_adj_h<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, t + 1 < nsteps ? 1 : 0);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
+  off = 0;
+  for (int q = 0; q < 6; ++q) {
+    int sh[3];
+    comp_shape(e, q, sh);
+    HIPCHK(hipMemcpy(buf.data(), q < 3 ? (void *)a.Ea[q] : (void *)a.Ha[q - 3], buf.size() * sizeof(T),
+                     hipMemcpyDeviceToHost));
+    for (int i = 0; i < sh[0]; ++i)
+      for (int j = 0; j < sh[1]; ++j)
+        for (int k = 0; k < sh[2]; ++k) fields[off++] = (double)buf[(size_t)i * e->si + (size_t)j * e->sj + k];
+  }
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_adjoint_run(xf_engine *e, int64_t nsteps, double *fields) {
+  if (!e || !fields || nsteps < 1) FAILS(XF_EINVAL, "bad argument");
+  if (!e->materials_set) FAILS(XF_ESTATE, "materials not set");
+  HIPCHK(engine_wait(e));
+  return e->prec == 64 ? adjoint_run<double>(e, nsteps, fields) : adjoint_run<float>(e, nsteps, fields);
+}

@@ -41,7 +41,21 @@ struct Grid {
   T *psi_e[6], *psi_h[6];
   int nx, ny, nz, pz, nmat;
   long long sj, si;  // strides of j and i in elements
+  // design box [db0,db1) x [db2,db3) x [db4,db5) with per-cell E coefficients
+  // dcoef[cell][9] = cexe, ceye, ceze, cexhy, ceyhx, cexhz, cezhx, ceyhz,
+  // cezhy (an empty box when db0 == db1)
+  int db[6];
+  const T *dcoef;
 };
+
+template <typename T>
+__device__ __forceinline__ bool in_design(const Grid<T> &g, int i, int j, int k) {
+  return i >= g.db[0] && i < g.db[1] && j >= g.db[2] && j < g.db[3] && k >= g.db[4] && k < g.db[5];
+}
+template <typename T>
+__device__ __forceinline__ int design_cell(const Grid<T> &g, int i, int j, int k) {
+  return ((i - g.db[0]) * (g.db[3] - g.db[2]) + (j - g.db[2])) * (g.db[5] - g.db[4]) + (k - g.db[4]);
+}
 
 __device__ __forceinline__ int e_layer(int n, int N, int dn, int dp) {
   if (n >= 1 && n <= dn) return n - 1;
@@ -401,8 +415,16 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
   const T dhx_j = hx - p.n3;     // Hx(j) - Hx(j-1)
   const T dhy_i = hy - hy_prev;  // Hy(i) - Hy(i-1)
   const int mi = (i - tt.i0) * g.nmat + m;
-  const P4<T> cs = tt.self[m];
-  const P2<T> ck = tt.k[tt.kcol + m], cj = tt.j[tt.jcol + m], ci = tt.i[mi];
+  P4<T> cs = tt.self[m];
+  P2<T> ck = tt.k[tt.kcol + m], cj = tt.j[tt.jcol + m], ci = tt.i[mi];
+  if constexpr (SLAB) {
+    // design cells carry their own coefficients (outside every CPML slab)
+    if (in_design(g, i, j, k)) {
+      const T *d = g.dcoef + 9 * design_cell(g, i, j, k);
+      cs.x = d[0]; cs.y = d[1]; cs.z = d[2];
+      ck.x = d[3]; ck.y = d[4]; cj.x = d[5]; cj.y = d[6]; ci.x = d[7]; ci.y = d[8];
+    }
+  }
   // cexe*Ex + cexhy*dHy/dz + cexhz*dHz/dy, and likewise for Ey, Ez
   const T yx = cs.x * ex + ck.x * dhy_k + cj.x * dhz_j;
   const T yy = cs.y * ey + ck.y * dhx_k + ci.x * dhz_i;
@@ -486,11 +508,16 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
   // a wave whose column meets no y or z slab runs the plain path on every
   // plane outside the x slabs
   const bool col_slab = __ballot(q.cp.ly >= 0 || q.cp.lz >= 0) != 0ull;
+  // a wave whose column crosses the design box takes the general path on
+  // the design planes
+  const bool col_design =
+      __ballot(live && j >= g.db[2] && j < g.db[3] && k >= g.db[4] && k < g.db[5]) != 0ull;
   unsigned c = (unsigned)((long long)i0 * g.si + (long long)j * g.sj + k);
   // x-neighbours are carried in registers: Hz(i-1), Hy(i-1)
   T hz_prev = T(0), hy_prev = T(0);
   Plane<T> A;
-  bool slab = col_slab || e_layer(i0, nx, g.dn[0], g.dp[0]) >= 0;
+  bool slab = col_slab || e_layer(i0, nx, g.dn[0], g.dp[0]) >= 0 ||
+              (col_design && i0 >= g.db[0] && i0 < g.db[1]);
   // the first plane's loads go out before the tables are staged
   if (live) {
     if (i0 >= 1) {
@@ -520,7 +547,7 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
     hy_prev = A.g1;
     if (++i >= i1) break;
     c += (unsigned)g.si;
-    slab = col_slab || e_layer(i, nx, g.dn[0], g.dp[0]) >= 0;
+    slab = col_slab || e_layer(i, nx, g.dn[0], g.dp[0]) >= 0 || (col_design && i >= g.db[0] && i < g.db[1]);
     if (slab)
       load_plane_e<T, true>(g, q, c, i, A);
     else
@@ -1099,6 +1126,275 @@ __global__ void __launch_bounds__(256) k_farfield(Nffft n, FarDir d, double2 *ou
     double *o = reinterpret_cast<double *>(out + base);
     o[threadIdx.x] = acc;
   }
+}
+
+// ---------------------------------------------------------------------------
+// DFT monitor of one component on a plane (the adjoint's objective): after
+// step t, F(u, v) += f(cell) * w[t], FP64.  (u, v) run over the two axes
+// other than `axis`, in increasing axis order.
+struct DftPlane {
+  int comp, axis, index, lo1, hi1, lo2, hi2;
+  double2 *acc;       // [hi1-lo1][hi2-lo2]
+  const double2 *w;   // [T]
+};
+
+__device__ __forceinline__ long long plane_cell(const DftPlane &d, long long si, long long sj, int u, int v) {
+  if (d.axis == 0) return (long long)d.index * si + (long long)u * sj + v;
+  if (d.axis == 1) return (long long)u * si + (long long)d.index * sj + v;
+  return (long long)u * si + (long long)v * sj + d.index;
+}
+
+template <typename T>
+__global__ void k_dft_plane(const T *f, long long si, long long sj, DftPlane d, long long step) {
+  const int n2 = d.hi2 - d.lo2, n = (d.hi1 - d.lo1) * n2;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int u = d.lo1 + q / n2, v = d.lo2 + q % n2;
+  const double x = (double)f[plane_cell(d, si, sj, u, v)];
+  const double2 w = d.w[step];
+  double2 a = d.acc[q];
+  a.x = a.x + x * w.x;
+  a.y = a.y + x * w.y;
+  d.acc[q] = a;
+}
+
+// E of every design cell into hist[slot][cell][3] (for the gradient)
+template <typename T>
+__global__ void k_design_snap(Grid<T> g, T *hist, int slot) {
+  const int nj = g.db[3] - g.db[2], nk = g.db[5] - g.db[4];
+  const int nd = (g.db[1] - g.db[0]) * nj * nk;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nd) return;
+  const int i = g.db[0] + q / (nj * nk), j = g.db[2] + (q / nk) % nj, k = g.db[4] + q % nk;
+  const long long c = (long long)i * g.si + (long long)j * g.sj + k;
+  T *h = hist + ((long long)slot * nd + q) * 3;
+  h[0] = g.f[EX][c];
+  h[1] = g.f[EY][c];
+  h[2] = g.f[EZ][c];
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of the step map (transpose of the E and H half-steps with
+// their CPML terms), in the adjoint fields Ea, Ha and psi adjoints Pe, Ph.
+// Each half-step's transpose is split in two: at the cells it updates, the
+// "difference weights" -- the coefficient times the adjoint, plus the CPML
+// a-coefficient times the psi adjoint, for each difference the update takes
+// -- are formed and the cell's own adjoint and psi adjoint are scaled; at
+// the cells those differences read, the weights are gathered.  One kernel
+// per cell type does the gather of one half-step and the scaling of the
+// other, so every value is read and written by the thread that owns it.
+//
+//   H-side weights (W): Wxy chxey (Ey k-diff), Wxz chxez (Ez j), Wyz chyez
+//     (Ez i), Wyx chyex (Ex k), Wzx chzex (Ex j), Wzy chzey (Ey i)
+//   E-side weights (V): Vxy cexhy (Hy k-diff), Vxz cexhz (Hz j), Vyz ceyhz
+//     (Hz i), Vyx ceyhx (Hx k), Vzx cezhx (Hx j), Vzy cezhy (Hy i)
+enum { W_XY = 0, W_XZ, W_YZ, W_YX, W_ZX, W_ZY };
+
+template <typename T>
+struct Adj {
+  T *Ea[3], *Ha[3];
+  T *Pe[6], *Ph[6];   // psi adjoints, psi layouts
+  T *W[6], *V[6];     // padded grid
+  // objective injection (plane of comp < 3) and design gradient
+  DftPlane dft;       // dft.acc = final forward F
+  int inject;
+  const T *hist;      // design E history of the segment [slot][cell][3]
+  const double *eps_r;  // design cells
+  double *grad;
+  int grad_on;
+};
+
+template <typename T>
+__device__ __forceinline__ P2<T> pair_at(const T *tab, int n, int nmat, int m) {
+  return reinterpret_cast<const P2<T> *>(tab)[n * nmat + m];
+}
+
+// per E cell: gather of the H half-step transpose, objective injection,
+// gradient, then the E half-step transpose's weights and scaling
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_e(Grid<T> g, Adj<T> a, long long step, int slot) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(c / g.si), r = (int)(c - (long long)i * g.si), j = r / (int)g.sj, k = r - j * (int)g.sj;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
+  if (i > nx || j > ny || k > nz) return;
+  const long long ck = c - 1, cj = c - g.sj, ci = c - g.si;
+  T ex = a.Ea[0][c], ey = a.Ea[1][c], ez = a.Ea[2][c];
+  const bool sx = i < nx, sy = j < ny, sz = k < nz;  // inside Ex / Ey / Ez shapes
+  // gather: transpose of the H curl differences (forward differences of E)
+  if (sx) ex = ex + (((k >= 1 ? a.W[W_YX][ck] : T(0)) - a.W[W_YX][c]) +
+                     ((j >= 1 ? a.W[W_ZX][cj] : T(0)) - a.W[W_ZX][c]));
+  if (sy) ey = ey + (((k >= 1 ? a.W[W_XY][ck] : T(0)) - a.W[W_XY][c]) +
+                     ((i >= 1 ? a.W[W_ZY][ci] : T(0)) - a.W[W_ZY][c]));
+  if (sz) ez = ez + (((j >= 1 ? a.W[W_XZ][cj] : T(0)) - a.W[W_XZ][c]) +
+                     ((i >= 1 ? a.W[W_YZ][ci] : T(0)) - a.W[W_YZ][c]));
+  // objective: dJ/dE = 2 Re(conj(F) w_t) on the monitor plane
+  if (a.inject) {
+    const DftPlane &d = a.dft;
+    int u, v, w;
+    if (d.axis == 0) { w = i; u = j; v = k; } else if (d.axis == 1) { u = i; w = j; v = k; } else { u = i; v = j; w = k; }
+    if (w == d.index && u >= d.lo1 && u < d.hi1 && v >= d.lo2 && v < d.hi2) {
+      const double2 F = d.acc[(u - d.lo1) * (d.hi2 - d.lo2) + (v - d.lo2)];
+      const double2 wt = d.w[step];
+      const T inj = (T)(2 * (F.x * wt.x + F.y * wt.y));
+      if (d.comp == EX) ex = ex + inj;
+      if (d.comp == EY) ey = ey + inj;
+      if (d.comp == EZ) ez = ez + inj;
+    }
+  }
+  const bool des = in_design(g, i, j, k);
+  const int dq = des ? design_cell(g, i, j, k) : 0;
+  // gradient: with sigma = 0 in the design box, dE'/d eps = -(E' - E)/eps
+  if (a.grad_on && des) {
+    const int nd = (g.db[1] - g.db[0]) * (g.db[3] - g.db[2]) * (g.db[5] - g.db[4]);
+    const T *h1 = a.hist + ((long long)(slot + 1) * nd + dq) * 3;
+    const T *h0 = a.hist + ((long long)slot * nd + dq) * 3;
+    double acc = 0;
+    acc = acc + (double)ex * ((double)h1[0] - (double)h0[0]);
+    acc = acc + (double)ey * ((double)h1[1] - (double)h0[1]);
+    acc = acc + (double)ez * ((double)h1[2] - (double)h0[2]);
+    a.grad[dq] = a.grad[dq] - acc / a.eps_r[dq];
+  }
+  // E half-step transpose at this cell
+  const int m = g.mat[c];
+  const T *self = g.self[0] + 4 * m;
+  T ca_x = self[0], ca_y = self[1], ca_z = self[2];
+  P2<T> pk = pair_at(g.kt[0], k, nmat, m), pj = pair_at(g.jt[0], j, nmat, m), pi = pair_at(g.it[0], i, nmat, m);
+  if (des) {
+    const T *d = g.dcoef + 9 * dq;
+    ca_x = d[0]; ca_y = d[1]; ca_z = d[2];
+    pk.x = d[3]; pk.y = d[4]; pj.x = d[5]; pj.y = d[6]; pi.x = d[7]; pi.y = d[8];
+  }
+  const bool ix = i >= 1 && i < nx, jin = j >= 1 && j < ny, kin = k >= 1 && k < nz;
+  const bool yx = sx && jin && kin, yy = ix && sy && kin, yz = ix && jin && sz;
+  T vxy = yx ? pk.x * ex : T(0), vxz = yx ? pj.x * ex : T(0);
+  T vyz = yy ? pi.x * ey : T(0), vyx = yy ? pk.y * ey : T(0);
+  T vzx = yz ? pj.y * ez : T(0), vzy = yz ? pi.y * ez : T(0);
+  const int lx = e_layer(i, nx, g.dn[0], g.dp[0]), ly = e_layer(j, ny, g.dn[1], g.dp[1]),
+            lz = e_layer(k, nz, g.dn[2], g.dp[2]);
+  if (lx >= 0 || ly >= 0 || lz >= 0) {
+    const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = ny + 1;
+    const P2<T> qk = pair_at(g.kp[0], k, nmat, m), qj = pair_at(g.jp[0], j, nmat, m),
+                qi = pair_at(g.ip[0], i, nmat, m);
+    if (sx && ly >= 0) {
+      T &p = a.Pe[P_AY][psi_y(i, ly, k, Ly, g.pz)];
+      const T chi = p + qj.x * ex;
+      vxz = vxz + g.ae[1][j] * chi;
+      p = g.be[1][j] * chi;
+    }
+    if (sx && lz >= 0) {
+      T &p = a.Pe[P_AZ][psi_z(i, j, lz, nm1y, Lz)];
+      const T chi = p + qk.x * ex;
+      vxy = vxy + g.ae[2][k] * chi;
+      p = g.be[2][k] * chi;
+    }
+    if (sy && lx >= 0) {
+      T &p = a.Pe[P_BX][psi_x(lx, j, k, nm1y, g.pz)];
+      const T chi = p + qi.x * ey;
+      vyz = vyz + g.ae[0][i] * chi;
+      p = g.be[0][i] * chi;
+    }
+    if (sy && lz >= 0) {
+      T &p = a.Pe[P_BZ][psi_z(i, j, lz, nm1y, Lz)];
+      const T chi = p + qk.y * ey;
+      vyx = vyx + g.ae[2][k] * chi;
+      p = g.be[2][k] * chi;
+    }
+    if (sz && lx >= 0) {
+      T &p = a.Pe[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
+      const T chi = p + qi.y * ez;
+      vzy = vzy + g.ae[0][i] * chi;
+      p = g.be[0][i] * chi;
+    }
+    if (sz && ly >= 0) {
+      T &p = a.Pe[P_CY][psi_y(i, ly, k, Ly, g.pz)];
+      const T chi = p + qj.y * ez;
+      vzx = vzx + g.ae[1][j] * chi;
+      p = g.be[1][j] * chi;
+    }
+  }
+  a.V[W_XY][c] = vxy; a.V[W_XZ][c] = vxz; a.V[W_YZ][c] = vyz;
+  a.V[W_YX][c] = vyx; a.V[W_ZX][c] = vzx; a.V[W_ZY][c] = vzy;
+  a.Ea[0][c] = yx ? ca_x * ex : ex;
+  a.Ea[1][c] = yy ? ca_y * ey : ey;
+  a.Ea[2][c] = yz ? ca_z * ez : ez;
+}
+
+// per H cell: gather of the E half-step transpose, then (with h1) the next
+// H half-step transpose's weights and scaling
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_h(Grid<T> g, Adj<T> a, int h1) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(c / g.si), r = (int)(c - (long long)i * g.si), j = r / (int)g.sj, k = r - j * (int)g.sj;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
+  if (i > nx || j > ny || k > nz) return;
+  // inside the Hx (nx+1, ny, nz), Hy (nx, ny+1, nz), Hz (nx, ny, nz+1) shapes
+  const bool sx = j < ny && k < nz, sy = i < nx && k < nz, sz = i < nx && j < ny;
+  const long long ck = c + 1, cj = c + g.sj, ci = c + g.si;
+  T hx = a.Ha[0][c], hy = a.Ha[1][c], hz = a.Ha[2][c];
+  // gather: transpose of the E curl differences (backward differences of H)
+  if (sx) hx = hx + ((a.V[W_YX][c] - a.V[W_YX][ck]) + (a.V[W_ZX][c] - a.V[W_ZX][cj]));
+  if (sy) hy = hy + ((a.V[W_XY][c] - a.V[W_XY][ck]) + (a.V[W_ZY][c] - a.V[W_ZY][ci]));
+  if (sz) hz = hz + ((a.V[W_XZ][c] - a.V[W_XZ][cj]) + (a.V[W_YZ][c] - a.V[W_YZ][ci]));
+  if (!h1) {
+    a.Ha[0][c] = hx; a.Ha[1][c] = hy; a.Ha[2][c] = hz;
+    return;
+  }
+  const int m = g.mat[c];
+  const T *self = g.self[1] + 4 * m;
+  const P2<T> pk = pair_at(g.kt[1], k, nmat, m), pj = pair_at(g.jt[1], j, nmat, m),
+              pi = pair_at(g.it[1], i, nmat, m);
+  const bool yee = i < nx && j < ny && k < nz;
+  T wxy = yee ? pk.x * hx : T(0), wxz = yee ? pj.x * hx : T(0);
+  T wyz = yee ? pi.x * hy : T(0), wyx = yee ? pk.y * hy : T(0);
+  T wzx = yee ? pj.y * hz : T(0), wzy = yee ? pi.y * hz : T(0);
+  const int lx = h_layer(i, nx, g.dn[0], g.dp[0]), ly = h_layer(j, ny, g.dn[1], g.dp[1]),
+            lz = h_layer(k, nz, g.dn[2], g.dp[2]);
+  if (lx >= 0 || ly >= 0 || lz >= 0) {
+    const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = ny + 1;
+    const P2<T> qk = pair_at(g.kp[1], k, nmat, m), qj = pair_at(g.jp[1], j, nmat, m),
+                qi = pair_at(g.ip[1], i, nmat, m);
+    if (sx && ly >= 0) {
+      T &p = a.Ph[P_AY][psi_y(i, ly, k, Ly, g.pz)];
+      const T chi = p + qj.x * hx;
+      wxz = wxz + g.am[1][j] * chi;
+      p = g.bm[1][j] * chi;
+    }
+    if (sx && lz >= 0) {
+      T &p = a.Ph[P_AZ][psi_z(i, j, lz, nm1y, Lz)];
+      const T chi = p + qk.x * hx;
+      wxy = wxy + g.am[2][k] * chi;
+      p = g.bm[2][k] * chi;
+    }
+    if (sy && lx >= 0) {
+      T &p = a.Ph[P_BX][psi_x(lx, j, k, nm1y, g.pz)];
+      const T chi = p + qi.x * hy;
+      wyz = wyz + g.am[0][i] * chi;
+      p = g.bm[0][i] * chi;
+    }
+    if (sy && lz >= 0) {
+      T &p = a.Ph[P_BZ][psi_z(i, j, lz, nm1y, Lz)];
+      const T chi = p + qk.y * hy;
+      wyx = wyx + g.am[2][k] * chi;
+      p = g.bm[2][k] * chi;
+    }
+    if (sz && lx >= 0) {
+      T &p = a.Ph[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
+      const T chi = p + qi.y * hz;
+      wzy = wzy + g.am[0][i] * chi;
+      p = g.bm[0][i] * chi;
+    }
+    if (sz && ly >= 0) {
+      T &p = a.Ph[P_CY][psi_y(i, ly, k, Ly, g.pz)];
+      const T chi = p + qj.y * hz;
+      wzx = wzx + g.am[1][j] * chi;
+      p = g.bm[1][j] * chi;
+    }
+  }
+  a.W[W_XY][c] = wxy; a.W[W_XZ][c] = wxz; a.W[W_YZ][c] = wyz;
+  a.W[W_YX][c] = wyx; a.W[W_ZX][c] = wzx; a.W[W_ZY][c] = wzy;
+  a.Ha[0][c] = yee ? self[0] * hx : hx;
+  a.Ha[1][c] = yee ? self[1] * hy : hy;
+  a.Ha[2][c] = yee ? self[2] * hz : hz;
 }
 
 // ---------------------------------------------------------------------------

@@ -58,6 +58,17 @@ struct PlanData {
     xf_nffft_desc desc{};
     std::vector<std::complex<double>> tj, tm;
   } nf;
+  struct Design {
+    bool present = false;
+    int64_t box[6] = {};
+    std::vector<double> coef, eps_r;
+  } design;
+  struct Dft {
+    bool present = false;
+    int comp = 0, axis = 0;
+    int64_t index = 0, lo1 = 0, hi1 = 0, lo2 = 0, hi2 = 0;
+    std::vector<std::complex<double>> w;
+  } dft;
 };
 
 // shape/cube.cpp:27-37
@@ -354,6 +365,17 @@ void Simulation::plan(size_t total_time_steps) {
             midx[(size_t)i * si + (size_t)j * sj + k] = (uint8_t)(q + 1);
   }
 
+  // ---- design box: the cells keep slot 0 (vacuum mu, no loss) for H and
+  // carry per-cell E coefficients (set below)
+  if (_has_design) {
+    for (int a = 0; a < 3; ++a)
+      if (_design_box[2 * a + 1] > (size_t)N[a] || _design_box[2 * a] >= _design_box[2 * a + 1])
+        throw std::runtime_error("design box outside the grid");
+    for (size_t i = _design_box[0]; i < _design_box[1]; ++i)
+      for (size_t j = _design_box[2]; j < _design_box[3]; ++j)
+        for (size_t k = _design_box[4]; k < _design_box[5]; ++k) midx[i * si + j * sj + k] = 0;
+  }
+
   // ---- lumped current sources (lumped_element/current_source.cpp:34-187).
   // Each source cell gets its own material slot holding the corrected
   // coefficients of its component; the other components keep the base values.
@@ -547,6 +569,55 @@ void Simulation::plan(size_t total_time_steps) {
         if (in_slab_e(a, n)) throw std::runtime_error("current source inside a CPML slab is not supported");
   }
 
+  // ---- design box coefficients: fdtd_basic_coff.cpp:77-91 for each cell's
+  // own permittivity (eps_r rounded to float as a Material's), lossless
+  if (_has_design) {
+    PlanData::Design &D = _plan->design;
+    for (int a = 0; a < 3; ++a) {
+      D.box[2 * a] = (int64_t)_design_box[2 * a];
+      D.box[2 * a + 1] = (int64_t)_design_box[2 * a + 1];
+      for (size_t n = _design_box[2 * a]; n < _design_box[2 * a + 1]; ++n)
+        if (in_slab_e(a, (int)n) || n == 0 || n >= (size_t)N[a])
+          throw std::runtime_error("design box must lie inside the grid and outside the CPML slabs");
+    }
+    const size_t nd = (_design_box[1] - _design_box[0]) * (_design_box[3] - _design_box[2]) *
+                      (_design_box[5] - _design_box[4]);
+    if (_design_eps.size() != nd) throw std::invalid_argument("design eps_r has the wrong size");
+    D.coef.resize(9 * nd);
+    D.eps_r.resize(nd);
+    size_t q = 0;
+    for (size_t i = _design_box[0]; i < _design_box[1]; ++i)
+      for (size_t j = _design_box[2]; j < _design_box[3]; ++j)
+        for (size_t k = _design_box[4]; k < _design_box[5]; ++k, ++q) {
+          const Mat m{static_cast<double>(_design_eps[q]) * constant::EPSILON_0, constant::MU_0, 0.0, 0.0};
+          double *c = &D.coef[9 * q];
+          c[0] = coffSelfE(m, dt);
+          c[1] = coffSelfE(m, dt);
+          c[2] = coffSelfE(m, dt);
+          c[3] = coffEa(m, dt, _hs[2][k]);  // cexhy
+          c[4] = coffEb(m, dt, _hs[2][k]);  // ceyhx
+          c[5] = coffEb(m, dt, _hs[1][j]);  // cexhz
+          c[6] = coffEa(m, dt, _hs[1][j]);  // cezhx
+          c[7] = coffEa(m, dt, _hs[0][i]);  // ceyhz
+          c[8] = coffEb(m, dt, _hs[0][i]);  // cezhy
+          D.eps_r[q] = static_cast<double>(_design_eps[q]);
+        }
+    D.present = true;
+  }
+  // ---- DFT plane: weights dt exp(-i 2 pi f t dt) (nffft_fd.cpp:102-103)
+  if (_has_dft) {
+    PlanData::Dft &F = _plan->dft;
+    F.comp = _dft_comp; F.axis = _dft_axis;
+    F.index = (int64_t)_dft_plane[0]; F.lo1 = (int64_t)_dft_plane[1]; F.hi1 = (int64_t)_dft_plane[2];
+    F.lo2 = (int64_t)_dft_plane[3]; F.hi2 = (int64_t)_dft_plane[4];
+    F.w.resize(total_time_steps);
+    for (size_t t = 0; t < total_time_steps; ++t) {
+      const double ph = -2 * constant::PI * _dft_freq * ((double)t * dt);
+      F.w[t] = std::complex<double>(dt * std::cos(ph), dt * std::sin(ph));
+    }
+    F.present = true;
+  }
+
   // ---- the plan: every table the engine needs
   PlanData &P = *_plan;
   P.nmat = nmat;
@@ -707,6 +778,67 @@ void Simulation::upload() {
   }
   for (const Probe &p : _probes)
     check(xf_engine_add_probe(_engine, compIndex(p.component), (int64_t)p.i, (int64_t)p.j, (int64_t)p.k));
+  if (P.design.present)
+    check(xf_engine_set_design(_engine, P.design.box, P.design.coef.data(), P.design.eps_r.data()));
+  if (P.dft.present)
+    check(xf_engine_set_dft_plane(_engine, P.dft.comp, P.dft.axis, P.dft.index, P.dft.lo1, P.dft.hi1, P.dft.lo2,
+                                  P.dft.hi2, reinterpret_cast<const double *>(P.dft.w.data()),
+                                  (int64_t)P.dft.w.size()));
+}
+
+void Simulation::setDesign(const std::array<size_t, 6> &box, std::vector<float> eps_r) {
+  if (_plan) throw std::runtime_error("Simulation already planned");
+  _design_box = box;
+  _design_eps = std::move(eps_r);
+  _has_design = true;
+}
+
+void Simulation::setDftPlane(EMComponent component, int axis, size_t index, size_t lo1, size_t hi1, size_t lo2,
+                             size_t hi2, double frequency) {
+  if (_plan) throw std::runtime_error("Simulation already planned");
+  if (axis < 0 || axis > 2) throw std::invalid_argument("axis 0..2");
+  _dft_comp = compIndex(component);
+  _dft_axis = axis;
+  _dft_plane[0] = index; _dft_plane[1] = lo1; _dft_plane[2] = hi1; _dft_plane[3] = lo2; _dft_plane[4] = hi2;
+  _dft_freq = frequency;
+  _has_dft = true;
+}
+
+std::vector<std::complex<double>> Simulation::getDft() const {
+  if (!_engine || !_has_dft) throw std::runtime_error("no DFT plane");
+  std::vector<std::complex<double>> out((_dft_plane[2] - _dft_plane[1]) * (_dft_plane[4] - _dft_plane[3]));
+  check(xf_engine_get_dft(_engine, reinterpret_cast<double *>(out.data())));
+  return out;
+}
+
+double Simulation::gradEps(std::vector<double> &grad, int checkpoint_every) {
+  if (!_engine) throw std::runtime_error("Simulation is not initialised");
+  if (!_has_design) throw std::runtime_error("no design box");
+  grad.assign(_design_eps.size(), 0.0);
+  double J = 0;
+  check(xf_engine_grad_eps(_engine, checkpoint_every, grad.data(), &J));
+  return J;
+}
+
+void Simulation::adjointRun(size_t n, std::vector<double> &fields) {
+  if (!_engine) throw std::runtime_error("Simulation is not initialised");
+  size_t total = 0;
+  for (int c = 0; c < 6; ++c) {
+    auto sh = fieldShape(static_cast<EMComponent>(c));
+    total += sh[0] * sh[1] * sh[2];
+  }
+  if (fields.size() != total) throw std::invalid_argument("adjoint fields have the wrong size");
+  check(xf_engine_adjoint_run(_engine, (int64_t)n, fields.data()));
+}
+
+void Simulation::checkpoint(int slot) {
+  if (!_engine) throw std::runtime_error("Simulation is not initialised");
+  check(xf_engine_checkpoint(_engine, slot));
+}
+
+void Simulation::restore(int slot) {
+  if (!_engine) throw std::runtime_error("Simulation is not initialised");
+  check(xf_engine_restore(_engine, slot));
 }
 
 void Simulation::init(size_t total_time_steps) {
@@ -745,6 +877,18 @@ std::vector<double> Simulation::planCoefficient(int which) const {
         }
         out[(i * sh[1] + j) * sh[2] + k] = v;
       }
+  // design cells: their own E coefficients
+  if (P.design.present && comp < 3) {
+    // position of `which` in the per-cell record (cexe, ceye, ceze, cexhy,
+    // ceyhx, cexhz, cezhx, ceyhz, cezhy)
+    static const int slot_of[9] = {0, 3, 5, 1, 7, 4, 2, 6, 8};
+    const int64_t *b = P.design.box;
+    size_t q = 0;
+    for (int64_t i = b[0]; i < b[1]; ++i)
+      for (int64_t j = b[2]; j < b[3]; ++j)
+        for (int64_t k = b[4]; k < b[5]; ++k, ++q)
+          out[((size_t)i * sh[1] + (size_t)j) * sh[2] + (size_t)k] = P.design.coef[9 * q + slot_of[which]];
+  }
   return out;
 }
 

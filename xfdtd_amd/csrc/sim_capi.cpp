@@ -203,3 +203,52 @@ extern "C" int xf_sim_get_probes(xf_sim *s, double *out) {
 }
 
 extern "C" struct xf_engine *xf_sim_engine(xf_sim *s) { return s->sim.engine(); }
+
+extern "C" int xf_sim_set_design(xf_sim *s, const size_t box[6], const float *eps_r) {
+  return guarded([&] {
+    std::array<size_t, 6> b;
+    for (int q = 0; q < 6; ++q) b[q] = box[q];
+    if (b[1] <= b[0] || b[3] <= b[2] || b[5] <= b[4]) throw std::invalid_argument("empty design box");
+    const size_t nd = (b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]);
+    s->sim.setDesign(b, std::vector<float>(eps_r, eps_r + nd));
+  });
+}
+
+extern "C" int xf_sim_set_dft_plane(xf_sim *s, int comp, int axis, size_t index, size_t lo1, size_t hi1,
+                                    size_t lo2, size_t hi2, double freq) {
+  return guarded([&] {
+    if (comp < 0 || comp > 5) throw std::invalid_argument("bad component");
+    s->sim.setDftPlane(static_cast<EMComponent>(comp), axis, index, lo1, hi1, lo2, hi2, freq);
+  });
+}
+
+extern "C" int xf_sim_get_dft(xf_sim *s, double *out) {
+  return guarded([&] {
+    auto v = s->sim.getDft();
+    std::memcpy(out, v.data(), v.size() * sizeof(v[0]));
+  });
+}
+
+extern "C" int xf_sim_grad_eps(xf_sim *s, int checkpoint_every, double *grad, double *J) {
+  return guarded([&] {
+    std::vector<double> g;
+    *J = s->sim.gradEps(g, checkpoint_every);
+    std::memcpy(grad, g.data(), g.size() * sizeof(double));
+  });
+}
+
+extern "C" int xf_sim_adjoint_run(xf_sim *s, size_t nsteps, double *fields, size_t n) {
+  return guarded([&] {
+    std::vector<double> v(fields, fields + n);
+    s->sim.adjointRun(nsteps, v);
+    std::memcpy(fields, v.data(), n * sizeof(double));
+  });
+}
+
+extern "C" int xf_sim_checkpoint(xf_sim *s, int slot) {
+  return guarded([&] { s->sim.checkpoint(slot); });
+}
+
+extern "C" int xf_sim_restore(xf_sim *s, int slot) {
+  return guarded([&] { s->sim.restore(slot); });
+}

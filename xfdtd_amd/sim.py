@@ -68,6 +68,16 @@ class Simulation:
                                                       float(s.resistance), w.amplitude, w.tau, w.t_0))
             for p in scene.probes:
                 self._chk(L.xf_sim_add_probe(self.h, int(p.comp), p.i, p.j, p.k))
+            dz = getattr(scene, "design", None)
+            if dz is not None:
+                box = (C.c_size_t * 6)(*[int(v) for v in dz.box])
+                er = np.ascontiguousarray(np.asarray(dz.eps_r, dtype=np.float32).ravel())
+                self._chk(L.xf_sim_set_design(self.h, box, er.ctypes.data_as(C.POINTER(C.c_float))))
+            dp_ = getattr(scene, "dft_plane", None)
+            if dp_ is not None:
+                self._chk(L.xf_sim_set_dft_plane(self.h, int(dp_.comp), int(dp_.axis), int(dp_.index),
+                                                 int(dp_.lo1), int(dp_.hi1), int(dp_.lo2), int(dp_.hi2),
+                                                 float(dp_.freq)))
             if plan_only:
                 self._chk(L.xf_sim_plan(self.h, self.T))
             else:
@@ -157,6 +167,39 @@ class Simulation:
         out = np.empty((len(self.scene.probes), self.T), dtype=np.float64)
         self._chk(lib().xf_sim_get_probes(self.h, _dp(out)))
         return out
+
+    # ---- differentiation
+    def get_dft(self):
+        dp_ = self.scene.dft_plane
+        out = np.empty(2 * (dp_.hi1 - dp_.lo1) * (dp_.hi2 - dp_.lo2), dtype=np.float64)
+        self._chk(lib().xf_sim_get_dft(self.h, _dp(out)))
+        return out.view(np.complex128).reshape(dp_.hi1 - dp_.lo1, dp_.hi2 - dp_.lo2)
+
+    def grad(self, checkpoint_every=100):
+        """(J, dJ/d eps_r over the design box): forward run from step 0 with
+        checkpoints, then the adjoint (xf_engine_grad_eps)."""
+        b = self.scene.design.box
+        g = np.empty((b[1] - b[0], b[3] - b[2], b[5] - b[4]), dtype=np.float64)
+        J = C.c_double()
+        self._chk(lib().xf_sim_grad_eps(self.h, int(checkpoint_every), _dp(g), C.byref(J)))
+        return J.value, g
+
+    def adjoint_run(self, nsteps, fields):
+        """Transpose of nsteps steps of the field map on six adjoint fields."""
+        packed = np.concatenate([np.ascontiguousarray(f, dtype=np.float64).ravel() for f in fields])
+        self._chk(lib().xf_sim_adjoint_run(self.h, int(nsteps), _dp(packed), packed.size))
+        out, off = [], 0
+        for c in range(6):
+            n = int(np.prod(self.field_shape(c)))
+            out.append(packed[off:off + n].reshape(self.field_shape(c)).copy())
+            off += n
+        return out
+
+    def checkpoint(self, slot):
+        self._chk(lib().xf_sim_checkpoint(self.h, int(slot)))
+
+    def restore(self, slot):
+        self._chk(lib().xf_sim_restore(self.h, int(slot)))
 
     # ---- engine controls
     def set_timing(self, on=True):
