@@ -25,6 +25,14 @@ the autotuner per size class.  The plain loop steps each instance alone and
 waits for it before the next, as one Simulation at a time runs; the batched
 run steps all of them together (xf_batch_step).  Both report
 sum(N^3) * steps / wall time.
+
+adjoint: the design-gradient workload (128^3 with 10-cell CPML, soft Ez
+source, 40^3 design cube of random eps_r, J = sum |DFT Ez|^2 on the plane
+x = 108; xfdtd_amd/workloads.py), 1500 steps, checkpoint every 100.
+cost_ratio = wall time of the gradient (forward with checkpoints + segment
+recompute + adjoint) / wall time of the plain forward run, per precision;
+the FP64 gradient is checked against central differences (delta eps_r =
+1e-3) at 16 seeded design cells, and the FP32 gradient against the FP64 one.
 """
 from __future__ import annotations
 
@@ -174,6 +182,76 @@ def run_batched(count, steps, warmup=10, log=None):
     }
 
 
+def run_adjoint(fd_cells=16, steps=1500, every=100):
+    import numpy as np
+
+    from xfdtd_amd import scene as S
+    from xfdtd_amd.sim import Simulation
+    from xfdtd_amd.workloads import SplitMix64, adjoint_design_scene
+    sc = adjoint_design_scene()
+    out = {"workload": "128^3 incl. 10-cell CPML, dx 4 mm, cfl 0.99, soft Jz source at (20,64,64), design "
+                       "cube [44,84)^3 eps_r ~ U[1,4] (splitmix64 seed 0xC4), J = sum |DFT Ez(108,j,k)|^2, "
+                       "j,k in [10,118), f0 = c0/(20 dx)",
+           "steps": steps, "checkpoint_every": every}
+    grads = {}
+    for prec in (32, 64):
+        fwd = Simulation(sc, steps, precision=prec)
+        fwd.sync()
+        t0 = time.perf_counter()
+        fwd.step(steps)
+        fwd.sync()
+        t_fwd = time.perf_counter() - t0
+        fwd.close()
+        gs = Simulation(sc, steps, precision=prec)
+        gs.sync()
+        t0 = time.perf_counter()
+        J, g = gs.grad(every)
+        t_grad = time.perf_counter() - t0
+        gs.close()
+        # one segment: the design history of the whole run is kept during the
+        # forward run, nothing is recomputed
+        gs = Simulation(sc, steps, precision=prec)
+        gs.sync()
+        t0 = time.perf_counter()
+        _, g1 = gs.grad(steps)
+        t_one = time.perf_counter() - t0
+        gs.close()
+        grads[prec] = g
+        out[f"fp{prec}"] = {"forward_s": round(t_fwd, 4), "grad_s": round(t_grad, 4),
+                            "cost_ratio": round(t_grad / t_fwd, 3),
+                            "grad_no_recompute_s": round(t_one, 4),
+                            "cost_ratio_no_recompute": round(t_one / t_fwd, 3),
+                            "no_recompute_bit_identical": bool(np.array_equal(g, g1)), "J": J}
+    g64, g32 = grads[64].ravel(), grads[32].ravel()
+    out["fp32_vs_fp64_grad_rel"] = float(np.abs(g32 - g64).max() / np.abs(g64).max())
+    if fd_cells > 0:
+        rng = SplitMix64(0xC4F)
+        cells = []
+        while len(cells) < fd_cells:
+            c = rng.integer(0, g64.size - 1)
+            if c not in cells:
+                cells.append(c)
+        eps = np.asarray(sc.design.eps_r, dtype=np.float32)
+        errs = []
+        for c in cells:
+            vals = []
+            hi, lo = np.float32(float(eps[c]) + 1e-3), np.float32(float(eps[c]) - 1e-3)
+            for v in (hi, lo):
+                e = eps.copy()
+                e[c] = v
+                sc.design = S.Design(sc.design.box, e)
+                sim = Simulation(sc, steps, precision=64)
+                sim.step(steps)
+                vals.append(float(np.sum(np.abs(sim.get_dft()) ** 2)))
+                sim.close()
+            fd = (vals[0] - vals[1]) / (float(hi) - float(lo))
+            errs.append(abs(g64[c] - fd) / max(abs(fd), 1e-3 * np.abs(g64).max()))
+        sc.design = S.Design(sc.design.box, eps)
+        out["fd_cells"] = fd_cells
+        out["fp64_grad_vs_central_differences_max_rel"] = float(max(errs))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,6 +266,8 @@ def main():
     ap.add_argument("--tiling", type=str, default="")
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--adjoint", type=int, default=1, help="run the design-gradient leg (0: skip it)")
+    ap.add_argument("--adjoint-fd-cells", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 1 GiB device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -271,6 +351,8 @@ def main():
         "cpu_baseline": None,
     }
     sim.close()
+    if args.adjoint:
+        line["adjoint"] = run_adjoint(args.adjoint_fd_cells)
     if args.batch > 0:
         line["batched"] = run_batched(args.batch, args.batch_steps,
                                       log=lambda m: print(m, file=sys.stderr, flush=True))

@@ -353,9 +353,9 @@ class Simulation {
                    size_t hi2, double frequency);
   std::vector<std::complex<double>> getDft() const;
   // J and dJ/d eps_r per design cell: forward run from step 0 with a state
-  // checkpoint every `checkpoint_every` steps, then the discrete adjoint
-  // back through each recomputed segment
-  double gradEps(std::vector<double> &grad, int checkpoint_every = 100);
+  // checkpoint every `checkpoint_every` steps (0: automatic), then the
+  // discrete adjoint back through each recomputed segment
+  double gradEps(std::vector<double> &grad, int checkpoint_every = 0);
   // transpose of n steps of the field map on packed adjoint fields
   void adjointRun(size_t n, std::vector<double> &fields);
   void checkpoint(int slot);

@@ -208,8 +208,11 @@ xf_status xf_engine_restore(xf_engine *e, int slot);
    (C order).  Runs the forward loop from step 0 with a checkpoint every
    `checkpoint_every` steps, then the reverse loop segment by segment: each
    segment is recomputed from its checkpoint, keeping the design-box E
-   fields, and the adjoint kernels step back through it.  The engine is left
-   at the end of the forward run.  Design cells must be lossless. */
+   fields, and the adjoint kernels step back through it; the last segment's
+   design fields are kept during the forward run, so an interval >= the run
+   length recomputes nothing.  checkpoint_every = 0 chooses: the whole run
+   when its design fields fit in 4 GiB, else 100.  The engine is left at the
+   end of the forward run.  Design cells must be lossless. */
 xf_status xf_engine_grad_eps(xf_engine *e, int checkpoint_every, double *grad, double *J);
 /* the transpose of nsteps steps of the field map applied to six adjoint
    fields (packed in component order, reference shapes), in place; the psi

@@ -125,3 +125,20 @@ def test_fp32_gradient_close_to_fp64():
     J32, g32 = _sim(sc, steps, 32).grad(checkpoint_every=40)
     assert J32 == pytest.approx(J64, rel=1e-4)
     assert _rel(g32, g64) < 1e-3
+
+
+def test_torch_autograd_function():
+    import torch
+    from xfdtd_amd.torch_adjoint import design_objective
+    sc, steps = design_scene()
+    b = sc.design.box
+    eps = torch.tensor(np.asarray(sc.design.eps_r, dtype=np.float32).reshape(b[1] - b[0], b[3] - b[2], b[5] - b[4]),
+                       dtype=torch.float64, requires_grad=True)
+    J = design_objective(eps, sc, steps, checkpoint_every=50)
+    (3.0 * J).backward()
+    Jr, gr = _sim(sc, steps).grad(checkpoint_every=50)
+    assert float(J) == Jr
+    assert np.array_equal(eps.grad.numpy(), 3.0 * gr)
+    with torch.no_grad():
+        J2 = design_objective(eps, sc, steps)
+    assert float(J2) == pytest.approx(Jr, rel=1e-12)

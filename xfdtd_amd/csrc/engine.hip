@@ -1122,19 +1122,34 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
     std::vector<std::vector<void *>> &v;
     ~Free() { for (auto &x : v) state_free(x); }
   } free_ck{ck};
-  // forward run with a checkpoint at the start of every segment and at the end
+  AdjBufs B;
+  T *hist = (T *)B.alloc((size_t)(K + 1) * e->nd * 3 * sizeof(T));
+  double *g = (double *)B.alloc((size_t)e->nd * sizeof(double));
+  if (!hist || !g) FAILS(XF_ENOMEM, "adjoint buffers");
+  const Grid<T> gr = make_grid<T>(e);
+  const unsigned nb = (unsigned)((e->ncell + 255) / 256), nsnap = (unsigned)((e->nd + 255) / 256);
+  // forward run with a checkpoint at the start of every segment but the
+  // last, whose design-box E fields are kept as it runs (it is never
+  // recomputed), and a checkpoint at the end
   for (int sgi = 0; sgi < nseg; ++sgi) {
-    xf_status s = state_save(e, ck[sgi]);
-    if (s != XF_OK) return s;
     const long long n = std::min<long long>(K, T_ - (long long)sgi * K);
-    if ((s = run_steps<T>(e, n)) != XF_OK) return s;
+    xf_status s;
+    if (sgi + 1 < nseg) {
+      if ((s = state_save(e, ck[sgi])) != XF_OK) return s;
+      if ((s = run_steps<T>(e, n)) != XF_OK) return s;
+      continue;
+    }
+    hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, 0);
+    for (long long l = 0; l < n; ++l) {
+      if ((s = run_steps<T>(e, 1)) != XF_OK) return s;
+      hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, (int)(l + 1));
+    }
   }
   xf_status s = state_save(e, ck[nseg]);
   if (s != XF_OK) return s;
   // objective from the final DFT accumulators
   const DftPlane dpl = e->dft;
   const size_t np = (size_t)(dpl.hi1 - dpl.lo1) * (dpl.hi2 - dpl.lo2);
-  AdjBufs B;
   double2 *Ffin = (double2 *)B.alloc(np * sizeof(double2));
   if (!Ffin) FAILS(XF_ENOMEM, "adjoint buffers");
   HIPCHK(hipMemcpyAsync(Ffin, dpl.acc, np * sizeof(double2), hipMemcpyDeviceToDevice, e->stream));
@@ -1146,9 +1161,6 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
   *J = Jv;
   Adj<T> a;
   if ((s = adj_setup<T>(e, B, a)) != XF_OK) return s;
-  T *hist = (T *)B.alloc((size_t)(K + 1) * e->nd * 3 * sizeof(T));
-  double *g = (double *)B.alloc((size_t)e->nd * sizeof(double));
-  if (!hist || !g) FAILS(XF_ENOMEM, "adjoint buffers");
   a.dft = dpl;
   a.dft.acc = Ffin;
   a.inject = dpl.comp < 3;
@@ -1156,17 +1168,17 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
   a.eps_r = e->d_eps;
   a.grad = g;
   a.grad_on = 1;
-  const Grid<T> gr = make_grid<T>(e);
-  const unsigned nb = (unsigned)((e->ncell + 255) / 256), nsnap = (unsigned)((e->nd + 255) / 256);
   for (int sgi = nseg - 1; sgi >= 0; --sgi) {
-    // recompute the segment from its checkpoint, keeping the design E fields
-    if ((s = state_load(e, ck[sgi])) != XF_OK) return s;
-    e->cur = (long long)sgi * K;
-    const long long t0 = e->cur, n = std::min<long long>(K, T_ - t0);
-    hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, 0);
-    for (long long l = 0; l < n; ++l) {
-      if ((s = run_steps<T>(e, 1)) != XF_OK) return s;
-      hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, (int)(l + 1));
+    const long long t0 = (long long)sgi * K, n = std::min<long long>(K, T_ - t0);
+    if (sgi + 1 < nseg) {
+      // recompute the segment from its checkpoint, keeping the design E fields
+      if ((s = state_load(e, ck[sgi])) != XF_OK) return s;
+      e->cur = t0;
+      hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, 0);
+      for (long long l = 0; l < n; ++l) {
+        if ((s = run_steps<T>(e, 1)) != XF_OK) return s;
+        hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, (int)(l + 1));
+      }
     }
     // reverse steps of the segment
     for (long long l = n - 1; l >= 0; --l) {
@@ -1184,7 +1196,7 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
 }
 
 extern "C" xf_status xf_engine_grad_eps(xf_engine *e, int checkpoint_every, double *grad, double *J) {
-  if (!e || !grad || !J || checkpoint_every < 1) FAILS(XF_EINVAL, "bad argument");
+  if (!e || !grad || !J || checkpoint_every < 0) FAILS(XF_EINVAL, "bad argument");
   if (!e->materials_set || !e->has_design || !e->has_dft)
     FAILS(XF_ESTATE, "the gradient needs materials, a design box and a DFT plane");
   if (e->cur != 0) FAILS(XF_ESTATE, "the gradient runs from step 0 (engine is at step %lld)", e->cur);
@@ -1194,8 +1206,14 @@ extern "C" xf_status xf_engine_grad_eps(xf_engine *e, int checkpoint_every, doub
     if (s != XF_OK) return s;
   }
   HIPCHK(engine_wait(e));
-  return e->prec == 64 ? grad_run<double>(e, checkpoint_every, grad, J)
-                       : grad_run<float>(e, checkpoint_every, grad, J);
+  int K = checkpoint_every;
+  if (K == 0) {
+    // automatic: keep the design fields of the whole run when they fit in
+    // 4 GiB (no recompute), else a checkpoint every 100 steps
+    const double bytes = (double)(e->total + 1) * e->nd * 3 * e->esz;
+    K = bytes <= 4.0 * (1 << 30) ? (int)std::min<long long>(e->total, 1 << 30) : 100;
+  }
+  return e->prec == 64 ? grad_run<double>(e, K, grad, J) : grad_run<float>(e, K, grad, J);
 }
 
 template <typename T>

@@ -175,9 +175,10 @@ class Simulation:
         self._chk(lib().xf_sim_get_dft(self.h, _dp(out)))
         return out.view(np.complex128).reshape(dp_.hi1 - dp_.lo1, dp_.hi2 - dp_.lo2)
 
-    def grad(self, checkpoint_every=100):
+    def grad(self, checkpoint_every=0):
         """(J, dJ/d eps_r over the design box): forward run from step 0 with
-        checkpoints, then the adjoint (xf_engine_grad_eps)."""
+        checkpoints, then the adjoint (xf_engine_grad_eps; 0 = automatic
+        checkpoint interval)."""
         b = self.scene.design.box
         g = np.empty((b[1] - b[0], b[3] - b[2], b[5] - b[4]), dtype=np.float64)
         J = C.c_double()

@@ -88,3 +88,27 @@ def batched_scatter_set(count=64, seed=0xC3, n_min=48, n_max=160, **kw):
         n = rng.integer(n_min, n_max)
         out.append(random_scatter_scene(rng, n, **kw))
     return out
+
+
+def adjoint_design_scene(n=128, dl=4e-3, pml=10, src=(20, 64, 64), box=(44, 84), plane_x=108,
+                         plane=(10, 118), seed=0xC4):
+    """The design-gradient workload: an n^3 total grid (CPML included, order
+    3, sigma 0.8 sigma_opt, kappa 1, alpha 0.05), cfl 0.99, a soft z current
+    source (R -> infinity) at Ez(src) with a Gaussian of tau = 20 dt,
+    t0 = 4.5 tau, a design cube [box)^3 of eps_r i.i.d. uniform in [1, 4]
+    (splitmix64 `seed`, rounded to float) and the objective
+    J = sum |DFT Ez(plane_x, j, k)|^2 over j, k in [plane) at f0 = c0/(20 dl)."""
+    sc = S.cube_domain_scene(n, dl, pml, cfl=0.99)
+    dt = S.courant_dt(dl, 0.99)
+    tau = 20 * dt
+    half = n * dl / 2
+    org = tuple(-half + c * dl for c in src)
+    sc.sources.append(S.CurrentSource(S.Cube(org, (dl, dl, dl)), S.ZP, math.inf,
+                                      S.GaussianWaveform(tau, 4.5 * tau, 1.0)))
+    rng = SplitMix64(seed)
+    m = box[1] - box[0]
+    eps = [rng.uniform(1.0, 4.0) for _ in range(m ** 3)]
+    sc.design = S.Design((box[0], box[1]) * 3, eps)
+    sc.dft_plane = S.DftPlane(S.EZ, 0, plane_x, plane[0], plane[1], plane[0], plane[1], S.C_0 / (20 * dl))
+    return sc
+
