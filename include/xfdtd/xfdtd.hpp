@@ -111,18 +111,42 @@ class Material {
   static std::unique_ptr<Material> createPEC() {
     return std::make_unique<Material>("XFDTD_PEC", 1, 1, 1e10, 0);
   }
+  virtual ~Material() = default;
   double getPermittivityE() const { return _eps; }
   double getPermeabilityM() const { return _mu; }
   double getElectricalConductivity() const { return _sigma_e; }
   double getMagneticConductivity() const { return _sigma_m; }
   bool isDispersion() const { return _is_dispersion; }
   const std::string &name() const { return _name; }
+  // Drude parameters of a DrudeMedium (eps_inf, omega_p, nv); a plain
+  // Material holds 1, 0, 0 and is not dispersive
+  double drudeEpsInf() const { return _drude_eps_inf; }
+  double drudeOmegaP() const { return _drude_omega_p; }
+  double drudeNv() const { return _drude_nv; }
+
+ protected:
+  double _drude_eps_inf = 1, _drude_omega_p = 0, _drude_nv = 0;
 
  private:
   std::string _name;
   float _eps_r, _mu_r;
   double _eps, _mu, _sigma_e, _sigma_m;
   bool _is_dispersion;
+};
+
+// material/dispersive_material.h:102-131: a LinearDispersiveMaterial
+// (eps_r = mu_r = 1 in the standard arrays) whose cells take the Drude E
+// update.  The parameters live in the Material base, so an Object built from
+// a copy keeps them.
+class DrudeMedium : public Material {
+ public:
+  DrudeMedium(std::string_view name, double eps_inf, double pole_omega, double nv, double sigma_e = 0,
+              double sigma_m = 0)
+      : Material(name, 1, 1, sigma_e, sigma_m, true) {
+    _drude_eps_inf = eps_inf;
+    _drude_omega_p = pole_omega;
+    _drude_nv = nv;
+  }
 };
 
 class Object {

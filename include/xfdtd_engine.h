@@ -182,6 +182,21 @@ xf_status xf_engine_set_timing(xf_engine *e, int enabled);
 enum { XF_TILE_GLOBAL_TABLES = 2, XF_TILE_X_FASTEST = 4, XF_TILE_XCD_RUNS = 8, XF_TILE_XCD_COLS = 16 };
 xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags);
 
+/* Drude cells (DrudeMedium, material/dispersive_material.cpp:118-193, and
+   the dispersive branch of Simulation::updateE, simulation_update.cpp:
+   221-280).  box[6] = [i0,i1)x[j0,j1)x[k0,k1) holds every Drude cell and lies
+   outside every CPML slab and the design box; coef[m][6] = ca, cb, 1 + k, k,
+   beta, dl/2 of material m as DrudeMedium::init computes them
+   (dispersive_material.cpp:127-144), cb = 0 for a material that is not a
+   Drude medium.  A Drude cell's E components inside the Yee ranges take
+   E' = ca E + cb (curl - dl/2 (1+k) J), J' = k J + beta (E' + E); its H
+   components use the standard tables.  The J arrays start at zero and are
+   part of the checkpointed state.  Call after xf_engine_set_materials. */
+xf_status xf_engine_set_drude(xf_engine *e, const int64_t *box, const double *coef);
+/* the Drude box, and J of component comp (0..2) over it, C order */
+xf_status xf_engine_drude_box(xf_engine *e, int64_t *box);
+xf_status xf_engine_get_drude_j(xf_engine *e, int comp, double *out);
+
 /* ---- Differentiation (no counterpart in the reference, which runs forward
    only).  A design box of cells whose permittivity is a parameter, a DFT
    monitor whose energy is the objective, state checkpoints, and the

@@ -8,6 +8,7 @@
  *
  *   xf_sim_create                Simulation(double cell_size, float cfl)   simulation.cpp:20-23
  *   xf_sim_add_cube / _sphere    addObject(Object(name, Cube|Sphere, Material))  simulation.cpp:115-117
+ *   xf_sim_add_drude_cube / _sphere  addObject(Object(name, shape, DrudeMedium(...)))  dispersive_material.h:112-113
  *   xf_sim_add_pml               addBoundary(PML(...))                      simulation.cpp:119-121
  *   xf_sim_set_tfsf3d            addTFSFSource(TFSF3D(..., GaussianWaveform))  simulation.cpp:128-130
  *   xf_sim_set_nffft_fd          addNFFFT(NffftFd(...))                     simulation.cpp:132-134
@@ -41,6 +42,12 @@ int xf_sim_add_cube(xf_sim *s, const double origin[3], const double size[3], flo
                     float mu_r, double sigma_e, double sigma_m);
 int xf_sim_add_sphere(xf_sim *s, const double center[3], double radius, float eps_r, float mu_r,
                       double sigma_e, double sigma_m);
+/* addObject(Object(name, Cube|Sphere, DrudeMedium(name, eps_inf, pole_omega,
+   nv, sigma_e, sigma_m)))  (material/dispersive_material.h:112-113) */
+int xf_sim_add_drude_cube(xf_sim *s, const double origin[3], const double size[3], double eps_inf,
+                          double omega_p, double nv, double sigma_e, double sigma_m);
+int xf_sim_add_drude_sphere(xf_sim *s, const double center[3], double radius, double eps_inf,
+                            double omega_p, double nv, double sigma_e, double sigma_m);
 int xf_sim_add_pml(xf_sim *s, int orientation, int thickness, int order, double sigma_ratio,
                    double alpha_min, double alpha_max, double kappa_max);
 int xf_sim_set_tfsf3d(xf_sim *s, const size_t distance[3], double e0, double theta, double phi,
@@ -96,6 +103,10 @@ int xf_sim_grad_eps(xf_sim *s, int checkpoint_every, double *grad, double *J);
 /* transpose of nsteps steps on n packed adjoint field values, in place */
 int xf_sim_adjoint_run(xf_sim *s, size_t nsteps, double *fields, size_t n);
 int xf_sim_checkpoint(xf_sim *s, int slot);
+/* the Drude box and the Drude J of component comp (0..2) over it
+   (xf_engine_drude_box, xf_engine_get_drude_j) */
+int xf_sim_drude_box(xf_sim *s, int64_t box[6]);
+int xf_sim_get_drude_j(xf_sim *s, int comp, double *out);
 int xf_sim_restore(xf_sim *s, int slot);
 
 #ifdef __cplusplus

@@ -64,6 +64,8 @@ def lib():
         L.xo_sim_get_dft.argtypes = [vp, dp]
         L.xo_sim_grad.argtypes = [vp, dp, dp]
         L.xo_sim_adjoint_run.argtypes = [vp, sz, dp]
+        L.xo_sim_set_drude.argtypes = [vp, d, d, d]
+        L.xo_sim_get_drude_j.argtypes = [vp, i, dp]
         _LIB = L
     return _LIB
 
@@ -103,6 +105,8 @@ class OracleSim:
                     p, s = _arr3(sh.origin), _arr3(sh.size)
                     self._chk(L.xo_sim_add_cube(self.h, _dp(p), _dp(s), m.eps_r, m.mu_r,
                                                 m.sigma_e, m.sigma_m))
+                if hasattr(m, "omega_p"):
+                    self._chk(L.xo_sim_set_drude(self.h, float(m.eps_inf), float(m.omega_p), float(m.nu)))
             for b in scene.boundaries:
                 self._chk(L.xo_sim_add_pml(self.h, int(b.orientation), int(b.thickness),
                                            int(b.order), b.sigma_ratio, b.alpha_min,
@@ -253,6 +257,11 @@ class OracleSim:
             n = int(np.prod(self.field_shape(c)))
             out.append(packed[off:off + n].reshape(self.field_shape(c)).copy())
             off += n
+        return out
+
+    def get_drude_j(self, comp):
+        out = np.empty(self.field_shape(comp), dtype=np.float64)
+        self._chk(lib().xo_sim_get_drude_j(self.h, int(comp), _dp(out)))
         return out
 
     def get_probes(self):

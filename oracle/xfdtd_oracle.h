@@ -36,6 +36,11 @@ int xo_sim_add_cube(xo_sim *s, const double origin[3], const double size[3],
                     float eps_r, float mu_r, double sigma_e, double sigma_m);
 int xo_sim_add_sphere(xo_sim *s, const double center[3], double radius,
                       float eps_r, float mu_r, double sigma_e, double sigma_m);
+/* make the last added object a DrudeMedium(eps_inf, omega_p, nv) with its
+   sigma_e / sigma_m (its eps_r and mu_r become 1) */
+int xo_sim_set_drude(xo_sim *s, double eps_inf, double omega_p, double nu);
+/* polarisation current J of component comp (0..2, E shapes) */
+int xo_sim_get_drude_j(xo_sim *s, int comp, double *out);
 /* PML(orientation, thickness, order, sigma_ratio, alpha_min, alpha_max, kappa_max) */
 int xo_sim_add_pml(xo_sim *s, int orientation, int thickness, int order,
                    double sigma_ratio, double alpha_min, double alpha_max,

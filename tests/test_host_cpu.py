@@ -57,11 +57,26 @@ def _scene_with_everything():
     return sc
 
 
-@pytest.mark.parametrize("builder", ["everything", "pec_example"])
+def drude_scene(n=30, dl=4e-3, pml=5):
+    """Two Drude objects (a sphere and a lossy cube) and a later plain cube
+    that covers part of the sphere: those cells keep the sphere's Drude E
+    update with the cube's standard coefficients."""
+    wp = 2 * math.pi * 20e9
+    objs = [S.Object("plasma", S.Sphere((0.002, -0.004, 0.0), 7 * dl), S.DrudeMedium("p", 1.0, wp, 2 * math.pi * 1e9)),
+            S.Object("metal", S.Cube((-0.036, 0.01, -0.02), (0.02, 0.012, 0.016)),
+                     S.DrudeMedium("q", 2.0, 0.6 * wp, 2 * math.pi * 3e9, 0.05, 0.0)),
+            S.Object("cap", S.Cube((0.004, -0.03, -0.03), (0.03, 0.06, 0.06)), S.Material("c", 3.0, 1.2, 0.01, 0.0))]
+    return S.cube_domain_scene(n, dl, pml, cfl=0.99, objects=objs, tfsf_gap=pml + 2, nffft_gap=pml + 1,
+                               freqs=[S.C_0 / (20 * dl)], theta=[0.0, 1.0], phi=[0.0])
+
+
+@pytest.mark.parametrize("builder", ["everything", "pec_example", "drude"])
 def test_plan_reproduces_reference_coefficient_arrays(builder):
     from xfdtd_amd.sim import Simulation
     if builder == "everything":
         sc = _scene_with_everything()
+    elif builder == "drude":
+        sc = drude_scene()
     else:
         dl = 5e-3
         sc = S.cube_domain_scene(40, dl, 8, cfl=0.9,
@@ -120,3 +135,21 @@ def test_init_without_device_fails_loudly():
     sc = S.cube_domain_scene(20, 5e-3, 4, cfl=0.9)
     with pytest.raises(RuntimeError, match="HIP device|gfx950|hip"):
         Simulation(sc, 10)
+
+
+def test_plan_refuses_unsupported_drude_placements():
+    from xfdtd_amd.sim import Simulation
+    dl = 4e-3
+    wp = 2 * math.pi * 20e9
+    # a Drude object reaching into the CPML
+    sc = S.cube_domain_scene(24, dl, 4, cfl=0.99, objects=[
+        S.Object("p", S.Cube((-0.05, -0.01, -0.01), (0.06, 0.02, 0.02)), S.DrudeMedium("p", 1.0, wp, 1e9))])
+    with pytest.raises(RuntimeError, match="CPML"):
+        Simulation(sc, 10, plan_only=True)
+    # a current source on a Drude cell
+    sc = S.cube_domain_scene(24, dl, 4, cfl=0.99, objects=[
+        S.Object("p", S.Cube((-0.01, -0.01, -0.01), (0.02, 0.02, 0.02)), S.DrudeMedium("p", 1.0, wp, 1e9))])
+    sc.sources.append(S.CurrentSource(S.Cube((0.0, 0.0, 0.0), (dl, dl, dl)), S.ZP, math.inf,
+                                      S.GaussianWaveform(1e-10, 4e-10)))
+    with pytest.raises(RuntimeError, match="Drude"):
+        Simulation(sc, 10, plan_only=True)

@@ -111,6 +111,11 @@ struct xf_engine {
   bool has_dft = false;
   DftPlane dft{};
   long long dft_T = 0;
+  // Drude box: per-material ADE coefficients and the box's J arrays
+  bool has_drude = false;
+  int xb[6] = {0, 0, 0, 0, 0, 0};
+  void *drc = nullptr;
+  void *dj[3] = {nullptr, nullptr, nullptr};
 };
 
 // wait for everything enqueued for this engine, on its own stream and on
@@ -534,6 +539,9 @@ static Grid<T> make_grid(xf_engine *e) {
   g.sj = e->sj; g.si = e->si;
   for (int a = 0; a < 6; ++a) g.db[a] = e->db[a];
   g.dcoef = (const T *)e->dcoef;
+  for (int a = 0; a < 6; ++a) g.xb[a] = e->xb[a];
+  g.drc = (const T *)e->drc;
+  for (int q = 0; q < 3; ++q) g.dj[q] = (T *)e->dj[q];
   return g;
 }
 
@@ -995,6 +1003,61 @@ extern "C" xf_status xf_engine_set_design(xf_engine *e, const int64_t *box, cons
   return XF_OK;
 }
 
+extern "C" xf_status xf_engine_set_drude(xf_engine *e, const int64_t *box, const double *coef) {
+  if (!e || !box || !coef) FAILS(XF_EINVAL, "null argument");
+  if (!e->materials_set) FAILS(XF_ESTATE, "set the materials before the Drude box");
+  if (e->has_drude) FAILS(XF_ESTATE, "Drude box already set");
+  const int N[3] = {e->nx, e->ny, e->nz};
+  for (int a = 0; a < 3; ++a) {
+    const int64_t lo = box[2 * a], hi = box[2 * a + 1];
+    // inside the grid and outside every CPML slab, like the design box
+    if (lo < (e->dn[a] > 0 ? e->dn[a] + 1 : 0) || hi <= lo || hi > N[a] - e->dp[a])
+      FAILS(XF_EINVAL, "Drude box [%lld,%lld) on axis %d must lie inside the grid and outside the CPML",
+            (long long)lo, (long long)hi, a);
+  }
+  if (e->has_design) {
+    bool overlap = true;
+    for (int a = 0; a < 3; ++a)
+      overlap = overlap && box[2 * a] < e->db[2 * a + 1] && e->db[2 * a] < box[2 * a + 1];
+    if (overlap) FAILS(XF_EINVAL, "the Drude box overlaps the design box");
+  }
+  for (int m = 0; m < e->nmat; ++m)
+    if (coef[6 * m + 1] != 0 && !(coef[6 * m] == coef[6 * m] && coef[6 * m + 1] > 0))
+      FAILS(XF_EINVAL, "bad Drude coefficients of material %d", m);
+  const size_t n = (size_t)(box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4]);
+  xf_status s;
+  if ((s = upload_real(e, &e->drc, coef, 6 * (size_t)e->nmat)) != XF_OK) return s;
+  for (int q = 0; q < 3; ++q) {
+    if ((s = dev_alloc(e, &e->dj[q], n * e->esz)) != XF_OK) return s;
+    e->state.push_back({e->dj[q], n * e->esz});
+  }
+  for (int a = 0; a < 6; ++a) e->xb[a] = (int)box[a];
+  e->has_drude = true;
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_drude_box(xf_engine *e, int64_t *box) {
+  if (!e || !box) FAILS(XF_EINVAL, "null argument");
+  if (!e->has_drude) FAILS(XF_ESTATE, "no Drude box");
+  for (int a = 0; a < 6; ++a) box[a] = e->xb[a];
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_get_drude_j(xf_engine *e, int comp, double *out) {
+  if (!e || !out || comp < 0 || comp > 2) FAILS(XF_EINVAL, "bad argument");
+  if (!e->has_drude) FAILS(XF_ESTATE, "no Drude box");
+  HIPCHK(engine_wait(e));
+  const size_t n = (size_t)(e->xb[1] - e->xb[0]) * (e->xb[3] - e->xb[2]) * (e->xb[5] - e->xb[4]);
+  if (e->prec == 64) {
+    HIPCHK(hipMemcpy(out, e->dj[comp], n * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(n);
+    HIPCHK(hipMemcpy(tmp.data(), e->dj[comp], n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < n; ++q) out[q] = tmp[q];
+  }
+  return XF_OK;
+}
+
 extern "C" xf_status xf_engine_set_dft_plane(xf_engine *e, int comp, int axis, int64_t index, int64_t lo1,
                                              int64_t hi1, int64_t lo2, int64_t hi2, const double *w,
                                              int64_t total_steps) {
@@ -1199,6 +1262,7 @@ extern "C" xf_status xf_engine_grad_eps(xf_engine *e, int checkpoint_every, doub
   if (!e || !grad || !J || checkpoint_every < 0) FAILS(XF_EINVAL, "bad argument");
   if (!e->materials_set || !e->has_design || !e->has_dft)
     FAILS(XF_ESTATE, "the gradient needs materials, a design box and a DFT plane");
+  if (e->has_drude) FAILS(XF_ESTATE, "the adjoint does not cover Drude cells");
   if (e->cur != 0) FAILS(XF_ESTATE, "the gradient runs from step 0 (engine is at step %lld)", e->cur);
   if (e->total < 1) FAILS(XF_ESTATE, "no step horizon");
   if (!e->probe_comp.empty()) {
@@ -1262,6 +1326,7 @@ static xf_status adjoint_run(xf_engine *e, long long nsteps, double *fields) {
 extern "C" xf_status xf_engine_adjoint_run(xf_engine *e, int64_t nsteps, double *fields) {
   if (!e || !fields || nsteps < 1) FAILS(XF_EINVAL, "bad argument");
   if (!e->materials_set) FAILS(XF_ESTATE, "materials not set");
+  if (e->has_drude) FAILS(XF_ESTATE, "the adjoint does not cover Drude cells");
   HIPCHK(engine_wait(e));
   return e->prec == 64 ? adjoint_run<double>(e, nsteps, fields) : adjoint_run<float>(e, nsteps, fields);
 }

@@ -46,6 +46,13 @@ struct Grid {
   // cezhy (an empty box when db0 == db1)
   int db[6];
   const T *dcoef;
+  // Drude box [xb0,xb1) x [xb2,xb3) x [xb4,xb5) (empty when xb0 == xb1):
+  // per-material ADE coefficients drc[m][6] = ca, cb, 1 + k, k, beta, dl/2
+  // (cb = 0 for a material that is not a Drude medium) and the polarisation
+  // currents of the box's cells, dj[component][cell]
+  int xb[6];
+  const T *drc;
+  T *dj[3];
 };
 
 template <typename T>
@@ -55,6 +62,27 @@ __device__ __forceinline__ bool in_design(const Grid<T> &g, int i, int j, int k)
 template <typename T>
 __device__ __forceinline__ int design_cell(const Grid<T> &g, int i, int j, int k) {
   return ((i - g.db[0]) * (g.db[3] - g.db[2]) + (j - g.db[2])) * (g.db[5] - g.db[4]) + (k - g.db[4]);
+}
+template <typename T>
+__device__ __forceinline__ bool in_drude(const Grid<T> &g, int i, int j, int k) {
+  return i >= g.xb[0] && i < g.xb[1] && j >= g.xb[2] && j < g.xb[3] && k >= g.xb[4] && k < g.xb[5];
+}
+template <typename T>
+__device__ __forceinline__ int drude_cell(const Grid<T> &g, int i, int j, int k) {
+  return ((i - g.xb[0]) * (g.xb[3] - g.xb[2]) + (j - g.xb[2])) * (g.xb[5] - g.xb[4]) + (k - g.xb[4]);
+}
+
+// DrudeMedium::updateE{x,y,z} (dispersive_material.cpp:146-193) for one
+// component of one cell: `curl` is the reference's bracket without j_sum,
+// e.g. -(Hy(k) - Hy(k-1)) + (Hz(j) - Hz(j-1)) for Ex.  Returns the new E and
+// advances J in place.
+template <typename T>
+__device__ __forceinline__ T drude_e(const T *c, T *jp, T e_old, T curl) {
+  const T jv = *jp;
+  const T j_sum = T(0) + c[5] * (c[2] * jv);
+  const T e_new = c[0] * e_old + c[1] * (curl - j_sum);
+  *jp = c[3] * jv + c[4] * (e_new + e_old);
+  return e_new;
 }
 
 __device__ __forceinline__ int e_layer(int n, int N, int dn, int dp) {
@@ -437,6 +465,19 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
   T vy = (ix && j < ny && q.kin) ? yy : ey;
   T vz = (ix && q.jin && k < nz) ? yz : ez;
   if constexpr (SLAB) {
+    // Drude cells (outside every CPML slab) replace the Yee update
+    // (simulation_update.cpp:221-280)
+    if (in_drude(g, i, j, k)) {
+      const T *dc = g.drc + 6 * m;
+      if (dc[1] != T(0)) {
+        const int dq = drude_cell(g, i, j, k);
+        if (i < nx && q.jin && q.kin) vx = drude_e(dc, g.dj[0] + dq, ex, -dhy_k + dhz_j);
+        if (ix && j < ny && q.kin) vy = drude_e(dc, g.dj[1] + dq, ey, -dhz_i + dhx_k);
+        if (ix && q.jin && k < nz) vz = drude_e(dc, g.dj[2] + dq, ez, -dhx_j + dhy_i);
+      }
+    }
+  }
+  if constexpr (SLAB) {
     const int ly = q.cp.ly, lz = q.cp.lz, lx = p.lx;
     const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = ny + 1;
     if (i < nx) {
@@ -508,16 +549,18 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
   // a wave whose column meets no y or z slab runs the plain path on every
   // plane outside the x slabs
   const bool col_slab = __ballot(q.cp.ly >= 0 || q.cp.lz >= 0) != 0ull;
-  // a wave whose column crosses the design box takes the general path on
-  // the design planes
+  // a wave whose column crosses the design box or the Drude box takes the
+  // general path on those boxes' planes
   const bool col_design =
       __ballot(live && j >= g.db[2] && j < g.db[3] && k >= g.db[4] && k < g.db[5]) != 0ull;
+  const bool col_drude =
+      __ballot(live && j >= g.xb[2] && j < g.xb[3] && k >= g.xb[4] && k < g.xb[5]) != 0ull;
   unsigned c = (unsigned)((long long)i0 * g.si + (long long)j * g.sj + k);
   // x-neighbours are carried in registers: Hz(i-1), Hy(i-1)
   T hz_prev = T(0), hy_prev = T(0);
   Plane<T> A;
   bool slab = col_slab || e_layer(i0, nx, g.dn[0], g.dp[0]) >= 0 ||
-              (col_design && i0 >= g.db[0] && i0 < g.db[1]);
+              (col_design && i0 >= g.db[0] && i0 < g.db[1]) || (col_drude && i0 >= g.xb[0] && i0 < g.xb[1]);
   // the first plane's loads go out before the tables are staged
   if (live) {
     if (i0 >= 1) {
@@ -547,7 +590,8 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
     hy_prev = A.g1;
     if (++i >= i1) break;
     c += (unsigned)g.si;
-    slab = col_slab || e_layer(i, nx, g.dn[0], g.dp[0]) >= 0 || (col_design && i >= g.db[0] && i < g.db[1]);
+    slab = col_slab || e_layer(i, nx, g.dn[0], g.dp[0]) >= 0 || (col_design && i >= g.db[0] && i < g.db[1]) ||
+           (col_drude && i >= g.xb[0] && i < g.xb[1]);
     if (slab)
       load_plane_e<T, true>(g, q, c, i, A);
     else

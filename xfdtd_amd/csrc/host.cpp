@@ -69,6 +69,11 @@ struct PlanData {
     int64_t index = 0, lo1 = 0, hi1 = 0, lo2 = 0, hi2 = 0;
     std::vector<std::complex<double>> w;
   } dft;
+  struct Drude {
+    bool present = false;
+    int64_t box[6] = {};
+    std::vector<double> coef;  // [nmat][6]: ca, cb, 1 + k, k, beta, dl/2
+  } drude;
 };
 
 // shape/cube.cpp:27-37
@@ -332,9 +337,10 @@ void Simulation::plan(size_t total_time_steps) {
   // (fdtd_basic_coff.cpp:313-339), 1..n = objects in insertion order
   std::vector<Mat> mats;
   mats.push_back({constant::EPSILON_0, constant::MU_0, 0.0, 0.0});
+  bool any_drude = false;
   for (const auto &o : _objects) {
     const Material &m = o->getMaterial();
-    if (m.isDispersion()) throw std::runtime_error("Dispersion is not supported yet.");
+    any_drude = any_drude || m.isDispersion();
     mats.push_back({m.getPermittivityE(), m.getPermeabilityM(), m.getElectricalConductivity(),
                     m.getMagneticConductivity()});
   }
@@ -345,6 +351,12 @@ void Simulation::plan(size_t total_time_steps) {
   // shape here reports "outside".
   const size_t sj = (size_t)nz + 1, si = (size_t)(ny + 1) * sj;
   std::vector<uint8_t> midx((size_t)(nx + 1) * si, 0);
+  // Drude cells: object/object.cpp:109-115 (the intended tagging that the
+  // reference leaves commented out) marks the cells of a dispersive object's
+  // mask with its index; later dispersive objects overwrite earlier ones and
+  // other objects leave the mark alone.  dmask holds 1 + object index.
+  std::vector<uint8_t> dmask(any_drude ? midx.size() : 0, 0);
+  if (_objects.size() > 254) throw std::runtime_error("more than 254 objects");
   for (size_t q = 0; q < _objects.size(); ++q) {
     const Shape *sh = _objects[q]->getShape();
     auto box = sh->getWrappedBox();
@@ -358,11 +370,14 @@ void Simulation::plan(size_t total_time_steps) {
       while (r0[a] < N[a] && _hc[a][r0[a]] < lo[a] - _cell_size) ++r0[a];
       while (r1[a] > 0 && _hc[a][r1[a] - 1] > hi[a] + _cell_size) --r1[a];
     }
+    const bool drude = _objects[q]->getMaterial().isDispersion();
     for (int i = r0[0]; i < r1[0]; ++i)
       for (int j = r0[1]; j < r1[1]; ++j)
         for (int k = r0[2]; k < r1[2]; ++k)
-          if (sh->isPointInside({_hc[0][i], _hc[1][j], _hc[2][k]}))
+          if (sh->isPointInside({_hc[0][i], _hc[1][j], _hc[2][k]})) {
             midx[(size_t)i * si + (size_t)j * sj + k] = (uint8_t)(q + 1);
+            if (drude) dmask[(size_t)i * si + (size_t)j * sj + k] = (uint8_t)(q + 1);
+          }
   }
 
   // ---- design box: the cells keep slot 0 (vacuum mu, no loss) for H and
@@ -373,7 +388,54 @@ void Simulation::plan(size_t total_time_steps) {
         throw std::runtime_error("design box outside the grid");
     for (size_t i = _design_box[0]; i < _design_box[1]; ++i)
       for (size_t j = _design_box[2]; j < _design_box[3]; ++j)
-        for (size_t k = _design_box[4]; k < _design_box[5]; ++k) midx[i * si + j * sj + k] = 0;
+        for (size_t k = _design_box[4]; k < _design_box[5]; ++k) {
+          if (any_drude && dmask[i * si + j * sj + k])
+            throw std::runtime_error("the design box overlaps Drude cells");
+          midx[i * si + j * sj + k] = 0;
+        }
+  }
+
+  // ---- Drude slots: a Drude cell whose material slot is its own Drude
+  // object's uses that slot; one covered by a later plain object gets a slot
+  // with that object's standard coefficients and the Drude object's ADE
+  struct DrudeSlot {
+    int m_base, d;
+  };
+  std::vector<DrudeSlot> dslots;
+  std::vector<int> drude_of(mats.size(), 0);  // slot -> 1 + Drude object (0: none)
+  int64_t dbox[6] = {N[0], 0, N[1], 0, N[2], 0};
+  if (any_drude) {
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) {
+          const size_t c = (size_t)i * si + (size_t)j * sj + k;
+          const int d = dmask[c];
+          if (!d) continue;
+          const int mb = midx[c];
+          int slot = -1;
+          if (mb == d) {
+            slot = mb;
+          } else {
+            for (size_t q = 0; q < dslots.size(); ++q)
+              if (dslots[q].m_base == mb && dslots[q].d == d) slot = (int)(mats.size() + q);
+            if (slot < 0) {
+              slot = (int)(mats.size() + dslots.size());
+              dslots.push_back({mb, d});
+            }
+          }
+          midx[c] = (uint8_t)slot;
+          const int at[3] = {i, j, k};
+          for (int a = 0; a < 3; ++a) {
+            dbox[2 * a] = std::min<int64_t>(dbox[2 * a], at[a]);
+            dbox[2 * a + 1] = std::max<int64_t>(dbox[2 * a + 1], at[a] + 1);
+          }
+        }
+    for (size_t q = 1; q < mats.size(); ++q)
+      if (_objects[q - 1]->getMaterial().isDispersion()) drude_of[q] = (int)q;
+  }
+  for (size_t q = 0; q < dslots.size(); ++q) {
+    mats.push_back(mats[dslots[q].m_base]);
+    drude_of.push_back(dslots[q].d);
   }
 
   // ---- lumped current sources (lumped_element/current_source.cpp:34-187).
@@ -432,6 +494,7 @@ void Simulation::plan(size_t total_time_steps) {
       for (size_t j = st[1]; j < en[1]; ++j)
         for (size_t k = st[2]; k < en[2]; ++k) {
           const int mb = midx[i * si + j * sj + k];
+          if (drude_of[mb]) throw std::runtime_error("a current source on a Drude cell is not supported");
           const Mat &m = mats[mb];
           double da, db, dc;
           if (a == 0) { da = _hs[1][j]; db = _hs[2][k]; dc = _es[0][i]; }
@@ -602,6 +665,35 @@ void Simulation::plan(size_t total_time_steps) {
           c[8] = coffEb(m, dt, _hs[0][i]);  // cezhy
           D.eps_r[q] = static_cast<double>(_design_eps[q]);
         }
+    D.present = true;
+  }
+  // ---- Drude coefficients: DrudeMedium::init (dispersive_material.cpp:
+  // 127-144) with dl the grid's base size (dispersive_material.h:47-48)
+  if (any_drude && dbox[1] > dbox[0]) {
+    PlanData::Drude &D = _plan->drude;
+    for (int a = 0; a < 3; ++a)
+      for (int64_t n = dbox[2 * a]; n < dbox[2 * a + 1]; ++n)
+        if (in_slab_e(a, (int)n) || in_slab_h(a, (int)n))
+          throw std::runtime_error("Drude cells inside a CPML slab are not supported");
+    for (int q = 0; q < 6; ++q) D.box[q] = dbox[q];
+    D.coef.assign(6 * (size_t)nmat, 0.0);
+    const double dl = _cell_size, eps0 = constant::EPSILON_0;
+    for (int slot = 0; slot < nbase; ++slot) {
+      if (!drude_of[slot]) continue;
+      const Material &m = _objects[drude_of[slot] - 1]->getMaterial();
+      const double nv = m.drudeNv(), wp = m.drudeOmegaP(), einf = m.drudeEpsInf();
+      const double sigma_e = m.getElectricalConductivity();
+      const double kk = (2 - nv * dt) / (2 + nv * dt);
+      const double beta = (wp * wp * eps0 * dt) / (2 + nv * dt);
+      const double temp = 2 * eps0 * einf + beta * dt + sigma_e * dt;
+      double *c = &D.coef[6 * (size_t)slot];
+      c[0] = (2 * eps0 * einf - beta * dt - sigma_e * dt) / temp;
+      c[1] = 2 * dt / (temp * dl);
+      c[2] = 1 + kk;
+      c[3] = kk;
+      c[4] = beta;
+      c[5] = 0.5 * dl;
+    }
     D.present = true;
   }
   // ---- DFT plane: weights dt exp(-i 2 pi f t dt) (nffft_fd.cpp:102-103)
@@ -780,6 +872,7 @@ void Simulation::upload() {
     check(xf_engine_add_probe(_engine, compIndex(p.component), (int64_t)p.i, (int64_t)p.j, (int64_t)p.k));
   if (P.design.present)
     check(xf_engine_set_design(_engine, P.design.box, P.design.coef.data(), P.design.eps_r.data()));
+  if (P.drude.present) check(xf_engine_set_drude(_engine, P.drude.box, P.drude.coef.data()));
   if (P.dft.present)
     check(xf_engine_set_dft_plane(_engine, P.dft.comp, P.dft.axis, P.dft.index, P.dft.lo1, P.dft.hi1, P.dft.lo2,
                                   P.dft.hi2, reinterpret_cast<const double *>(P.dft.w.data()),

@@ -64,6 +64,41 @@ extern "C" int xf_sim_add_sphere(xf_sim *s, const double center[3], double radiu
   });
 }
 
+extern "C" int xf_sim_add_drude_cube(xf_sim *s, const double origin[3], const double size[3], double eps_inf,
+                                     double omega_p, double nv, double sigma_e, double sigma_m) {
+  return guarded([&] {
+    s->sim.addObject(std::make_shared<Object>(
+        "drude_cube", std::make_unique<Cube>(PointVector{origin[0], origin[1], origin[2]},
+                                             PointVector{size[0], size[1], size[2]}),
+        DrudeMedium("drude", eps_inf, omega_p, nv, sigma_e, sigma_m)));
+  });
+}
+
+extern "C" int xf_sim_add_drude_sphere(xf_sim *s, const double center[3], double radius, double eps_inf,
+                                       double omega_p, double nv, double sigma_e, double sigma_m) {
+  return guarded([&] {
+    s->sim.addObject(std::make_shared<Object>(
+        "drude_sphere", std::make_unique<Sphere>(PointVector{center[0], center[1], center[2]}, radius),
+        DrudeMedium("drude", eps_inf, omega_p, nv, sigma_e, sigma_m)));
+  });
+}
+
+extern "C" int xf_sim_drude_box(xf_sim *s, int64_t box[6]) {
+  return guarded([&] {
+    xf_engine *e = s->sim.engine();
+    if (!e) throw std::runtime_error("simulation not initialised");
+    if (xf_engine_drude_box(e, box) != XF_OK) throw std::runtime_error(xf_last_error());
+  });
+}
+
+extern "C" int xf_sim_get_drude_j(xf_sim *s, int comp, double *out) {
+  return guarded([&] {
+    xf_engine *e = s->sim.engine();
+    if (!e) throw std::runtime_error("simulation not initialised");
+    if (xf_engine_get_drude_j(e, comp, out) != XF_OK) throw std::runtime_error(xf_last_error());
+  });
+}
+
 extern "C" int xf_sim_add_pml(xf_sim *s, int orientation, int thickness, int order, double sigma_ratio,
                               double alpha_min, double alpha_max, double kappa_max) {
   return guarded([&] {

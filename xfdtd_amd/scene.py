@@ -72,10 +72,34 @@ class Material:
 
 
 @dataclass
+class DrudeMedium:
+    """DrudeMedium(name, eps_inf, pole_omega, nv, sigma_e, sigma_m)
+    (material/dispersive_material.h:112-113).  Its base Material is
+    LinearDispersiveMaterial(name, 1, 1, sigma_e, sigma_m): eps_r = mu_r = 1
+    in the standard coefficient arrays; the E field of its cells follows the
+    Drude update with eps_inf, omega_p (rad/s) and the collision rate nu
+    (1/s)."""
+    name: str
+    eps_inf: float
+    omega_p: float
+    nu: float
+    sigma_e: float = 0.0
+    sigma_m: float = 0.0
+
+    @property
+    def eps_r(self):
+        return 1.0
+
+    @property
+    def mu_r(self):
+        return 1.0
+
+
+@dataclass
 class Object:
     name: str
     shape: object
-    material: Material
+    material: object  # Material or DrudeMedium
 
 
 @dataclass

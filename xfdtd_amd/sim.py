@@ -37,7 +37,13 @@ class Simulation:
             for o in scene.objects:
                 m = o.material
                 sh = o.shape
-                if hasattr(sh, "radius"):
+                if hasattr(m, "omega_p"):
+                    drude = (float(m.eps_inf), float(m.omega_p), float(m.nu), float(m.sigma_e), float(m.sigma_m))
+                    if hasattr(sh, "radius"):
+                        self._chk(L.xf_sim_add_drude_sphere(self.h, _dp(_f64(sh.center)), float(sh.radius), *drude))
+                    else:
+                        self._chk(L.xf_sim_add_drude_cube(self.h, _dp(_f64(sh.origin)), _dp(_f64(sh.size)), *drude))
+                elif hasattr(sh, "radius"):
                     c = _f64(sh.center)
                     self._chk(L.xf_sim_add_sphere(self.h, _dp(c), float(sh.radius), m.eps_r, m.mu_r,
                                                   m.sigma_e, m.sigma_m))
@@ -162,6 +168,18 @@ class Simulation:
 
     def output_nffft(self):
         self._chk(lib().xf_sim_output_nffft(self.h))
+
+    def drude_box(self):
+        b = (C.c_int64 * 6)()
+        self._chk(lib().xf_sim_drude_box(self.h, b))
+        return tuple(int(v) for v in b)
+
+    def get_drude_j(self, comp):
+        """J of component comp over the Drude box [i0,i1)x[j0,j1)x[k0,k1)."""
+        b = self.drude_box()
+        out = np.empty((b[1] - b[0], b[3] - b[2], b[5] - b[4]), dtype=np.float64)
+        self._chk(lib().xf_sim_get_drude_j(self.h, int(comp), _dp(out)))
+        return out
 
     def get_probes(self):
         out = np.empty((len(self.scene.probes), self.T), dtype=np.float64)
