@@ -150,6 +150,11 @@ xf_status xf_engine_add_current_source(xf_engine *e, int comp, int64_t i0,
                                        const double *coff, const double *wave,
                                        int64_t total_steps);
 
+/* the run's total_time_steps (Simulation::run(T), simulation_update.cpp:5-8):
+   stepping beyond it is refused, and probe buffers are sized by it.  Tables
+   with their own length (sources, TFSF, NF-FF, DFT) lower it further. */
+xf_status xf_engine_set_horizon(xf_engine *e, int64_t total_steps);
+
 /* point sample of comp at (i,j,k) after every step */
 xf_status xf_engine_add_probe(xf_engine *e, int comp, int64_t i, int64_t j,
                               int64_t k);
@@ -185,7 +190,7 @@ xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags);
 /* Drude cells (DrudeMedium, material/dispersive_material.cpp:118-193, and
    the dispersive branch of Simulation::updateE, simulation_update.cpp:
    221-280).  box[6] = [i0,i1)x[j0,j1)x[k0,k1) holds every Drude cell and lies
-   outside every CPML slab and the design box; coef[m][6] = ca, cb, 1 + k, k,
+   outside every CPML slab; coef[m][6] = ca, cb, 1 + k, k,
    beta, dl/2 of material m as DrudeMedium::init computes them
    (dispersive_material.cpp:127-144), cb = 0 for a material that is not a
    Drude medium.  A Drude cell's E components inside the Yee ranges take

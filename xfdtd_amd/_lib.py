@@ -24,7 +24,7 @@ ENGINE_SYMBOLS = [
     "xf_batch_create", "xf_batch_step", "xf_batch_sync", "xf_batch_size", "xf_batch_destroy",
     "xf_engine_set_design", "xf_engine_set_dft_plane", "xf_engine_get_dft", "xf_engine_checkpoint",
     "xf_engine_restore", "xf_engine_grad_eps", "xf_engine_adjoint_run", "xf_engine_set_drude",
-    "xf_engine_drude_box", "xf_engine_get_drude_j",
+    "xf_engine_drude_box", "xf_engine_get_drude_j", "xf_engine_set_horizon",
 ]
 SIM_SYMBOLS = [
     "xf_sim_last_error", "xf_sim_create", "xf_sim_destroy", "xf_sim_add_cube", "xf_sim_add_sphere",

@@ -325,6 +325,12 @@ static void set_total(xf_engine *e, long long T) {
   if (e->total < 0 || T < e->total) e->total = T;
 }
 
+extern "C" xf_status xf_engine_set_horizon(xf_engine *e, int64_t total_steps) {
+  if (!e || total_steps < 0) FAILS(XF_EINVAL, "bad argument");
+  set_total(e, (long long)total_steps);
+  return XF_OK;
+}
+
 extern "C" xf_status xf_engine_set_tfsf(xf_engine *e, const xf_tfsf_desc *d) {
   if (!e || !d) FAILS(XF_EINVAL, "null argument");
   if (d->li < 1 || d->lj < 1 || d->lk < 1 || d->ri >= e->nx || d->rj >= e->ny || d->rk >= e->nz ||
@@ -1014,12 +1020,6 @@ extern "C" xf_status xf_engine_set_drude(xf_engine *e, const int64_t *box, const
     if (lo < (e->dn[a] > 0 ? e->dn[a] + 1 : 0) || hi <= lo || hi > N[a] - e->dp[a])
       FAILS(XF_EINVAL, "Drude box [%lld,%lld) on axis %d must lie inside the grid and outside the CPML",
             (long long)lo, (long long)hi, a);
-  }
-  if (e->has_design) {
-    bool overlap = true;
-    for (int a = 0; a < 3; ++a)
-      overlap = overlap && box[2 * a] < e->db[2 * a + 1] && e->db[2 * a] < box[2 * a + 1];
-    if (overlap) FAILS(XF_EINVAL, "the Drude box overlaps the design box");
   }
   for (int m = 0; m < e->nmat; ++m)
     if (coef[6 * m + 1] != 0 && !(coef[6 * m] == coef[6 * m] && coef[6 * m + 1] > 0))

@@ -843,6 +843,7 @@ void Simulation::upload() {
   gd.precision = _precision;
   gd.n_materials = P.nmat;
   check(xf_engine_create(&gd, &_engine));
+  check(xf_engine_set_horizon(_engine, (int64_t)_total_time_steps));
   const double *cbp[12], *cpp[12];
   for (int t = 0; t < 12; ++t) { cbp[t] = P.cb[t].data(); cpp[t] = P.cpsi[t].data(); }
   for (int a = 0; a < 3; ++a)
