@@ -201,6 +201,7 @@ xf_status xf_engine_set_drude(xf_engine *e, const int64_t *box, const double *co
 /* the Drude box, and J of component comp (0..2) over it, C order */
 xf_status xf_engine_drude_box(xf_engine *e, int64_t *box);
 xf_status xf_engine_get_drude_j(xf_engine *e, int comp, double *out);
+xf_status xf_engine_set_drude_j(xf_engine *e, int comp, const double *in);
 
 /* ---- Differentiation (no counterpart in the reference, which runs forward
    only).  A design box of cells whose permittivity is a parameter, a DFT
@@ -236,7 +237,9 @@ xf_status xf_engine_restore(xf_engine *e, int slot);
 xf_status xf_engine_grad_eps(xf_engine *e, int checkpoint_every, double *grad, double *J);
 /* the transpose of nsteps steps of the field map applied to six adjoint
    fields (packed in component order, reference shapes), in place; the psi
-   adjoints start at zero.  For checking <L x, y> = <x, L^T y>. */
+   adjoints start at zero.  With a Drude box the three J adjoints follow in
+   the E shapes (cells outside the box pass through).  For checking
+   <L x, y> = <x, L^T y>. */
 xf_status xf_engine_adjoint_run(xf_engine *e, int64_t nsteps, double *fields);
 
 /* Batches: several set-up engines of one precision, all at the same step,

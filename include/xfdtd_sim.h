@@ -107,6 +107,7 @@ int xf_sim_checkpoint(xf_sim *s, int slot);
    (xf_engine_drude_box, xf_engine_get_drude_j) */
 int xf_sim_drude_box(xf_sim *s, int64_t box[6]);
 int xf_sim_get_drude_j(xf_sim *s, int comp, double *out);
+int xf_sim_set_drude_j(xf_sim *s, int comp, const double *in);
 int xf_sim_restore(xf_sim *s, int slot);
 
 #ifdef __cplusplus

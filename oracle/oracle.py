@@ -66,6 +66,7 @@ def lib():
         L.xo_sim_adjoint_run.argtypes = [vp, sz, dp]
         L.xo_sim_set_drude.argtypes = [vp, d, d, d]
         L.xo_sim_get_drude_j.argtypes = [vp, i, dp]
+        L.xo_sim_set_drude_j.argtypes = [vp, i, dp]
         _LIB = L
     return _LIB
 
@@ -249,15 +250,20 @@ class OracleSim:
         return J.value, g
 
     def adjoint_run(self, nsteps, fields):
-        """Reverse-run nsteps of the field map on six adjoint fields."""
+        """Reverse-run nsteps of the field map on the adjoint state: six
+        fields, then (with Drude cells) the three J adjoints in E shapes."""
         packed = np.concatenate([np.ascontiguousarray(f, dtype=np.float64).ravel() for f in fields])
         self._chk(lib().xo_sim_adjoint_run(self.h, int(nsteps), _dp(packed)))
         out, off = [], 0
-        for c in range(6):
+        for c in list(range(6)) + list(range(len(fields) - 6)):
             n = int(np.prod(self.field_shape(c)))
             out.append(packed[off:off + n].reshape(self.field_shape(c)).copy())
             off += n
         return out
+
+    def set_drude_j(self, comp, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        self._chk(lib().xo_sim_set_drude_j(self.h, int(comp), _dp(a)))
 
     def get_drude_j(self, comp):
         out = np.empty(self.field_shape(comp), dtype=np.float64)

@@ -1523,6 +1523,14 @@ int xo_set_threads(int n) {
 #endif
 }
 
+int xo_sim_set_drude_j(xo_sim *s, int comp, const double *in) {
+  if (!s->inited || !s->has_drude) FAIL("no Drude cells");
+  if (comp < 0 || comp > 2) FAIL("bad component");
+  a3 *a = &s->J[comp];
+  memcpy(a->d, in, a->n0 * a->n1 * a->n2 * sizeof(double));
+  return 0;
+}
+
 int xo_sim_get_drude_j(xo_sim *s, int comp, double *out) {
   if (!s->inited || !s->has_drude) FAIL("no Drude cells");
   if (comp < 0 || comp > 2) FAIL("bad component");
@@ -1615,6 +1623,7 @@ int xo_sim_get_probes(const xo_sim *s, double *out) {
 typedef struct {
   a3 f[6];       /* adjoint of Ex..Hz */
   double **psi;  /* [npml][4] adjoint of each face's psi arrays */
+  a3 J[3];       /* adjoint of the Drude currents (when there are Drude cells) */
 } adj_t;
 
 static int adj_alloc(xo_sim *s, adj_t *a) {
@@ -1624,6 +1633,12 @@ static int adj_alloc(xo_sim *s, adj_t *a) {
     field_shape(s, q, sh);
     if (a3_alloc(&a->f[q], sh[0], sh[1], sh[2], 0.0)) FAIL("out of memory");
   }
+  if (s->has_drude)
+    for (int q = 0; q < 3; ++q) {
+      size_t sh[3];
+      field_shape(s, q, sh);
+      if (a3_alloc(&a->J[q], sh[0], sh[1], sh[2], 0.0)) FAIL("out of memory");
+    }
   a->psi = (double **)calloc((size_t)(s->npml ? s->npml : 1) * 4, sizeof(double *));
   if (!a->psi) FAIL("out of memory");
   for (int b = 0; b < s->npml; ++b)
@@ -1637,6 +1652,7 @@ static int adj_alloc(xo_sim *s, adj_t *a) {
 
 static void adj_free(xo_sim *s, adj_t *a) {
   for (int q = 0; q < 6; ++q) a3_free(&a->f[q]);
+  for (int q = 0; q < 3; ++q) a3_free(&a->J[q]);
   if (a->psi)
     for (int b = 0; b < 4 * s->npml; ++b) free(a->psi[b]);
   free(a->psi);
@@ -1715,14 +1731,39 @@ static void adj_update_h(xo_sim *s, adj_t *ad) {
       }
 }
 
+/* transpose of one Drude cell's update (update_e_drude):
+     E' = ca E + cb (curl - h kp1 J),  J' = k J + beta (E' + E)
+   undone last op first.  ebar, jbar: adjoints of E' and J' in, of E and J
+   out; returns t, the adjoint of E' after the J' transpose, whose curl
+   transpose the caller scatters with weight cb (curl = -dHa + dHb). */
+static double adj_drude_cell(const xo_sim *s, int q, double *ebar, double *jbar) {
+  double kk = s->dk[q], beta = s->dbeta[q], ca = s->dca[q], cb = s->dcb[q];
+  double hk = 0.5 * s->dl * (1 + kk);
+  double jb = *jbar;
+  double t = *ebar + beta * jb;   /* J' = ... + beta E' */
+  *ebar = beta * jb + ca * t;     /* J' = ... + beta E; E' = ca E + ... */
+  *jbar = kk * jb - cb * hk * t;  /* J' = k J; E' = ... - cb h kp1 J */
+  return t;
+}
+
 /* transpose of update_e (each component over its Yee range) */
 static void adj_update_e(xo_sim *s, adj_t *ad) {
   size_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
   a3 *c = s->c;
   double w;
+  const int *dm = s->dmask;
+#define DRUDE_AT(i, j, k) (dm ? dm[((i) * ny + (j)) * nz + (k)] : 0)
   for (size_t i = 0; i < nx; ++i)
     for (size_t j = 1; j < ny; ++j)
       for (size_t k = 1; k < nz; ++k) {
+        int d = DRUDE_AT(i, j, k);
+        if (d) {
+          double cb = s->dcb[d - 1];
+          double t = adj_drude_cell(s, d - 1, &AEX(i, j, k), &AT(ad->J[0], i, j, k));
+          w = -cb * t; AHY(i, j, k) += w; AHY(i, j, k - 1) -= w;
+          w = cb * t; AHZ(i, j, k) += w; AHZ(i, j - 1, k) -= w;
+          continue;
+        }
         double a = AEX(i, j, k);
         w = AT(c[CEXHY], i, j, k) * a; AHY(i, j, k) += w; AHY(i, j, k - 1) -= w;
         w = AT(c[CEXHZ], i, j, k) * a; AHZ(i, j, k) += w; AHZ(i, j - 1, k) -= w;
@@ -1731,6 +1772,14 @@ static void adj_update_e(xo_sim *s, adj_t *ad) {
   for (size_t j = 0; j < ny; ++j)
     for (size_t k = 1; k < nz; ++k)
       for (size_t i = 1; i < nx; ++i) {
+        int d = DRUDE_AT(i, j, k);
+        if (d) {
+          double cb = s->dcb[d - 1];
+          double t = adj_drude_cell(s, d - 1, &AEY(i, j, k), &AT(ad->J[1], i, j, k));
+          w = -cb * t; AHZ(i, j, k) += w; AHZ(i - 1, j, k) -= w;
+          w = cb * t; AHX(i, j, k) += w; AHX(i, j, k - 1) -= w;
+          continue;
+        }
         double a = AEY(i, j, k);
         w = AT(c[CEYHX], i, j, k) * a; AHX(i, j, k) += w; AHX(i, j, k - 1) -= w;
         w = AT(c[CEYHZ], i, j, k) * a; AHZ(i, j, k) += w; AHZ(i - 1, j, k) -= w;
@@ -1739,11 +1788,20 @@ static void adj_update_e(xo_sim *s, adj_t *ad) {
   for (size_t k = 0; k < nz; ++k)
     for (size_t i = 1; i < nx; ++i)
       for (size_t j = 1; j < ny; ++j) {
+        int d = DRUDE_AT(i, j, k);
+        if (d) {
+          double cb = s->dcb[d - 1];
+          double t = adj_drude_cell(s, d - 1, &AEZ(i, j, k), &AT(ad->J[2], i, j, k));
+          w = -cb * t; AHX(i, j, k) += w; AHX(i, j - 1, k) -= w;
+          w = cb * t; AHY(i, j, k) += w; AHY(i - 1, j, k) -= w;
+          continue;
+        }
         double a = AEZ(i, j, k);
         w = AT(c[CEZHX], i, j, k) * a; AHX(i, j, k) += w; AHX(i, j - 1, k) -= w;
         w = AT(c[CEZHY], i, j, k) * a; AHY(i, j, k) += w; AHY(i - 1, j, k) -= w;
         AEZ(i, j, k) = AT(c[CEZE], i, j, k) * a;
       }
+#undef DRUDE_AT
 }
 
 /* the H half of reverse step: CPML-H^T (faces reversed), update_h^T */
@@ -1802,7 +1860,6 @@ static double dft_objective(const xo_sim *s) {
 int xo_sim_grad(xo_sim *s, double *grad, double *J_out) {
   if (!s->inited || s->cur != 0) FAIL("grad needs a fresh simulation");
   if (!s->has_design || !s->has_dft) FAIL("grad needs a design box and a DFT plane");
-  if (s->has_drude) FAIL("the adjoint does not cover Drude cells");
   const size_t *b = s->dbox;
   size_t nd = (b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]), T = s->T;
   double *hist = (double *)calloc((T + 1) * nd * 3, sizeof(double));
@@ -1851,18 +1908,22 @@ int xo_sim_grad(xo_sim *s, double *grad, double *J_out) {
 }
 
 /* Reverse-run `nsteps` steps of the field map from a given adjoint state
-   (the six adjoint fields packed in component order, psi adjoints zero),
-   with no objective; the result overwrites `fields`.  For the identity
+   (the six adjoint fields packed in component order, then, when there are
+   Drude cells, the three J adjoints in E shapes; psi adjoints zero), with no
+   objective; the result overwrites `fields`.  For the identity
    <L x, y> = <x, L^T y> of the step map L. */
 int xo_sim_adjoint_run(xo_sim *s, size_t nsteps, double *fields) {
   if (!s->inited) FAIL("not initialised");
-  if (s->has_drude) FAIL("the adjoint does not cover Drude cells");
   adj_t ad;
   if (adj_alloc(s, &ad)) return -1;
+  a3 *st[9];
+  int nst = s->has_drude ? 9 : 6;
+  for (int q = 0; q < 6; ++q) st[q] = &ad.f[q];
+  for (int q = 0; q < 3; ++q) st[6 + q] = &ad.J[q];
   size_t off = 0;
-  for (int q = 0; q < 6; ++q) {
-    size_t n = ad.f[q].n0 * ad.f[q].n1 * ad.f[q].n2;
-    memcpy(ad.f[q].d, fields + off, n * sizeof(double));
+  for (int q = 0; q < nst; ++q) {
+    size_t n = st[q]->n0 * st[q]->n1 * st[q]->n2;
+    memcpy(st[q]->d, fields + off, n * sizeof(double));
     off += n;
   }
   for (size_t t = 0; t < nsteps; ++t) {
@@ -1871,9 +1932,9 @@ int xo_sim_adjoint_run(xo_sim *s, size_t nsteps, double *fields) {
     adj_update_e(s, &ad);
   }
   off = 0;
-  for (int q = 0; q < 6; ++q) {
-    size_t n = ad.f[q].n0 * ad.f[q].n1 * ad.f[q].n2;
-    memcpy(fields + off, ad.f[q].d, n * sizeof(double));
+  for (int q = 0; q < nst; ++q) {
+    size_t n = st[q]->n0 * st[q]->n1 * st[q]->n2;
+    memcpy(fields + off, st[q]->d, n * sizeof(double));
     off += n;
   }
   adj_free(s, &ad);

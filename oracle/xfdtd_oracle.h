@@ -41,6 +41,7 @@ int xo_sim_add_sphere(xo_sim *s, const double center[3], double radius,
 int xo_sim_set_drude(xo_sim *s, double eps_inf, double omega_p, double nu);
 /* polarisation current J of component comp (0..2, E shapes) */
 int xo_sim_get_drude_j(xo_sim *s, int comp, double *out);
+int xo_sim_set_drude_j(xo_sim *s, int comp, const double *in);
 /* PML(orientation, thickness, order, sigma_ratio, alpha_min, alpha_max, kappa_max) */
 int xo_sim_add_pml(xo_sim *s, int orientation, int thickness, int order,
                    double sigma_ratio, double alpha_min, double alpha_max,

@@ -7,12 +7,14 @@
 - Central finite differences of J = sum |DFT Ez|^2 over a plane with respect
   to the permittivity of single design cells agree with the adjoint
   gradient.
+- Both hold with Drude cells, whose currents J are part of the state (the
+  identity then includes the J components of x and y).
 """
 
 import numpy as np
 import pytest
 
-from adjoint_scenes import box_scene, design_scene
+from adjoint_scenes import box_scene, design_scene, drude_box_scene, drude_design_scene
 from oracle.oracle import OracleSim
 from xfdtd_amd import scene as S
 
@@ -81,3 +83,78 @@ def test_adjoint_gradient_matches_central_differences(oracle_threads):
         fd = (vals[0] - vals[1]) / (float(hi) - float(lo))
         ga = g.reshape(-1)[flat]
         assert ga == pytest.approx(fd, rel=2e-5, abs=1e-6 * np.abs(g).max()), (np.unravel_index(flat, (m,) * 3), ga, fd)
+
+
+def _drude_mask(sc):
+    """E-shaped masks of the Drude cells: where J moves after one step from a
+    random field."""
+    o = OracleSim(sc, 1)
+    rng = np.random.default_rng(1)
+    for c in range(3):
+        o.set_field(c, rng.standard_normal(o.field_shape(c)))
+    o.step(1)
+    return [o.get_drude_j(c) != 0 for c in range(3)]
+
+
+def _identity(sc, steps, x, y):
+    o = OracleSim(sc, steps)
+    for c in range(6):
+        o.set_field(c, x[c])
+    for c in range(3):
+        o.set_drude_j(c, x[6 + c])
+    o.step(steps)
+    out = [o.get_field(c) for c in range(6)] + [o.get_drude_j(c) for c in range(3)]
+    lhs = sum(float(np.sum(a * b)) for a, b in zip(out, y))
+    ly = OracleSim(sc, steps).adjoint_run(steps, y)
+    rhs = sum(float(np.sum(a * b)) for a, b in zip(x, ly))
+    scale = sum(float(np.sum(np.abs(a * b))) for a, b in zip(x, ly))
+    return lhs, rhs, scale
+
+
+def test_adjoint_transpose_identity_with_drude(oracle_threads):
+    """x and y carry the Drude currents (on the Drude cells); a second y with
+    only a J part makes every term of the identity pass through the Drude
+    update and its transpose."""
+    steps = 25
+    sc = drude_box_scene()
+    mask = _drude_mask(sc)
+    assert all(m.sum() > 10 for m in mask)
+    o = OracleSim(sc, 1)
+    shapes = [o.field_shape(c) for c in list(range(6)) + [0, 1, 2]]
+    rng = np.random.default_rng(9)
+    x = [rng.standard_normal(sh) for sh in shapes]
+    for c in range(3):
+        x[6 + c] *= mask[c]
+    y = [rng.standard_normal(sh) for sh in shapes]
+    for c in range(3):
+        y[6 + c] *= mask[c]
+    lhs, rhs, scale = _identity(sc, steps, x, y)
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs)
+    y_j = [np.zeros(sh) for sh in shapes[:6]] + y[6:]
+    lhs, rhs, scale = _identity(sc, steps, x, y_j)
+    assert abs(lhs) > 1e-3 * scale
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs)
+
+
+def test_adjoint_gradient_with_drude_matches_central_differences(oracle_threads):
+    sc, steps = drude_design_scene()
+    J, g = OracleSim(sc, steps).grad()
+    assert J == pytest.approx(_J(sc, steps), rel=1e-14)
+    eps = np.asarray(sc.design.eps_r, dtype=np.float32).copy()
+    rng = np.random.default_rng(4)
+    for flat in rng.choice(eps.size, 4, replace=False):
+        e0 = float(eps[flat])
+        hi, lo = np.float32(e0 + 1e-3), np.float32(e0 - 1e-3)
+        vals = []
+        for v in (hi, lo):
+            e = eps.copy()
+            e[flat] = v
+            sc.design = S.Design(sc.design.box, e)
+            vals.append(_J(sc, steps))
+        sc.design = S.Design(sc.design.box, eps)
+        fd = (vals[0] - vals[1]) / (float(hi) - float(lo))
+        assert g.reshape(-1)[flat] == pytest.approx(fd, rel=2e-5, abs=1e-6 * np.abs(g).max())
+    # the Drude block changes the gradient
+    sc0, _ = design_scene(seed=0xC5)
+    _, g0 = OracleSim(sc0, steps).grad()
+    assert np.abs(g - g0).max() > 1e-3 * np.abs(g0).max()

@@ -8,12 +8,13 @@
   the reference's per-face arrays) to 1e-11 of the result's scale;
 - dJ/d eps_r matches the oracle's to 1e-10 (FP64) for any checkpoint
   interval, and central differences on the GPU forward run;
-- the FP32 gradient stays within 1e-3 of the FP64 one.
+- the FP32 gradient stays within 1e-3 of the FP64 one;
+- all of the above with Drude cells, whose currents are adjoint state too.
 """
 import numpy as np
 import pytest
 
-from adjoint_scenes import box_scene, design_scene
+from adjoint_scenes import box_scene, design_scene, drude_box_scene, drude_design_scene
 from oracle.oracle import OracleSim
 from xfdtd_amd import scene as S
 
@@ -142,3 +143,51 @@ def test_torch_autograd_function():
     with torch.no_grad():
         J2 = design_objective(eps, sc, steps)
     assert float(J2) == pytest.approx(Jr, rel=1e-12)
+
+
+def test_gpu_transpose_identity_with_drude(oracle_threads):
+    steps = 25
+    sc = drude_box_scene()
+    g = _sim(sc, steps)
+    box = g.drude_box()
+    sl = np.s_[box[0]:box[1], box[2]:box[3], box[4]:box[5]]
+    rng = np.random.default_rng(9)
+    shapes = [g.field_shape(c) for c in list(range(6)) + [0, 1, 2]]
+    x = [rng.standard_normal(sh) for sh in shapes]
+    y = [rng.standard_normal(sh) for sh in shapes]
+    for c in range(3):       # J is state on the Drude box only
+        for v in (x, y):
+            keep = np.zeros(shapes[c], bool)
+            keep[sl] = True
+            v[6 + c] *= keep
+    for c in range(6):
+        g.set_field(c, x[c])
+    for c in range(3):
+        g.set_drude_j(c, x[6 + c][sl])
+    g.step(steps)
+    out_j = []
+    for c in range(3):
+        full = np.zeros(shapes[c])
+        full[sl] = g.get_drude_j(c)
+        out_j.append(full)
+    out = [g.get_field(c) for c in range(6)] + out_j
+    lhs = sum(float(np.sum(a * b)) for a, b in zip(out, y))
+    ly = _sim(sc, steps).adjoint_run(steps, y)
+    rhs = sum(float(np.sum(a * b)) for a, b in zip(x, ly))
+    scale = sum(float(np.sum(np.abs(a * b))) for a, b in zip(x, ly))
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs)
+    lo = OracleSim(sc, steps).adjoint_run(steps, y)
+    for c in range(9):
+        assert _rel(ly[c], lo[c]) <= 1e-11, c
+
+
+@pytest.mark.parametrize("every", [7, 1000])
+def test_gradient_with_drude_matches_oracle(every, oracle_threads):
+    sc, steps = drude_design_scene()
+    Jo, go = OracleSim(sc, steps).grad()
+    g = _sim(sc, steps)
+    Jg, gg = g.grad(checkpoint_every=every)
+    assert Jg == pytest.approx(Jo, rel=1e-14)
+    assert _rel(gg, go) <= 1e-10
+    _, g32 = _sim(sc, steps, 32).grad(checkpoint_every=every)
+    assert _rel(g32, go) < 1e-3

@@ -2,7 +2,8 @@
 NF-FF accumulators and the polarisation currents J, in a scene with TFSF,
 CPML, two Drude objects and a plain object covering part of one; the
 filled-cavity pin's scene; batched stepping with a Drude member; and
-checkpoints, which must carry J."""
+checkpoints, which must carry J.  (Gradients with Drude cells are covered
+with the other adjoint tests.)"""
 import math
 
 import numpy as np
@@ -114,12 +115,3 @@ def test_drude_checkpoint_carries_j():
     again = [g.get_field(c) for c in range(6)] + [g.get_drude_j(c) for c in range(3)]
     for a, b in zip(first, again):
         assert np.array_equal(a, b)
-
-
-def test_gradient_refuses_drude_cells():
-    sc = drude_scene()
-    sc.design = S.Design((6, 9, 6, 9, 20, 23), np.full(27, 2.0, np.float32))
-    sc.dft_plane = S.DftPlane(S.EZ, 0, 20, 8, 20, 8, 20, S.C_0 / (20 * sc.cell_size))
-    g = _sim(sc, 10)
-    with pytest.raises(RuntimeError, match="Drude"):
-        g.grad()

@@ -24,7 +24,7 @@ ENGINE_SYMBOLS = [
     "xf_batch_create", "xf_batch_step", "xf_batch_sync", "xf_batch_size", "xf_batch_destroy",
     "xf_engine_set_design", "xf_engine_set_dft_plane", "xf_engine_get_dft", "xf_engine_checkpoint",
     "xf_engine_restore", "xf_engine_grad_eps", "xf_engine_adjoint_run", "xf_engine_set_drude",
-    "xf_engine_drude_box", "xf_engine_get_drude_j", "xf_engine_set_horizon",
+    "xf_engine_drude_box", "xf_engine_get_drude_j", "xf_engine_set_horizon", "xf_engine_set_drude_j",
 ]
 SIM_SYMBOLS = [
     "xf_sim_last_error", "xf_sim_create", "xf_sim_destroy", "xf_sim_add_cube", "xf_sim_add_sphere",
@@ -34,7 +34,7 @@ SIM_SYMBOLS = [
     "xf_sim_get_nffft", "xf_sim_farfield", "xf_sim_output_nffft", "xf_sim_get_probes",
     "xf_sim_engine", "xf_sim_set_design", "xf_sim_set_dft_plane", "xf_sim_get_dft", "xf_sim_grad_eps",
     "xf_sim_adjoint_run", "xf_sim_checkpoint", "xf_sim_restore", "xf_sim_add_drude_cube",
-    "xf_sim_add_drude_sphere", "xf_sim_drude_box", "xf_sim_get_drude_j",
+    "xf_sim_add_drude_sphere", "xf_sim_drude_box", "xf_sim_get_drude_j", "xf_sim_set_drude_j",
 ]
 
 
@@ -105,5 +105,6 @@ def lib():
     L.xf_sim_add_drude_sphere.argtypes = [vp, dp, d, d, d, d, d, d]
     L.xf_sim_drude_box.argtypes = [vp, C.POINTER(C.c_int64)]
     L.xf_sim_get_drude_j.argtypes = [vp, i, dp]
+    L.xf_sim_set_drude_j.argtypes = [vp, i, dp]
     _LIB = L
     return L

@@ -321,6 +321,10 @@ extern "C" xf_status xf_engine_set_cpml(xf_engine *e, int axis, int d_n, int d_p
   return XF_OK;
 }
 
+static size_t drude_cells(const xf_engine *e) {
+  return (size_t)(e->xb[1] - e->xb[0]) * (e->xb[3] - e->xb[2]) * (e->xb[5] - e->xb[4]);
+}
+
 static void set_total(xf_engine *e, long long T) {
   if (e->total < 0 || T < e->total) e->total = T;
 }
@@ -1036,6 +1040,45 @@ extern "C" xf_status xf_engine_set_drude(xf_engine *e, const int64_t *box, const
   return XF_OK;
 }
 
+// Drude J (box layout) <-> E-shaped host arrays: `to_dev` copies the box part
+// of `full` into dev, else the box part of dev into `full` (other cells of
+// `full` are left alone)
+template <typename T>
+static xf_status drude_box_copy(xf_engine *e, int comp, void *dev, double *full, bool to_dev) {
+  int sh[3];
+  comp_shape(e, comp, sh);
+  const int *b = e->xb;
+  const size_t n = drude_cells(e);
+  std::vector<T> box(n);
+  if (!to_dev) HIPCHK(hipMemcpy(box.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost));
+  size_t q = 0;
+  for (int i = b[0]; i < b[1]; ++i)
+    for (int j = b[2]; j < b[3]; ++j)
+      for (int k = b[4]; k < b[5]; ++k, ++q) {
+        double &f = full[((size_t)i * sh[1] + j) * sh[2] + k];
+        if (to_dev)
+          box[q] = (T)f;
+        else
+          f = (double)box[q];
+      }
+  if (to_dev) HIPCHK(hipMemcpy(dev, box.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_set_drude_j(xf_engine *e, int comp, const double *in) {
+  if (!e || !in || comp < 0 || comp > 2) FAILS(XF_EINVAL, "bad argument");
+  if (!e->has_drude) FAILS(XF_ESTATE, "no Drude box");
+  HIPCHK(engine_wait(e));
+  const size_t n = drude_cells(e);
+  if (e->prec == 64) {
+    HIPCHK(hipMemcpy(e->dj[comp], in, n * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> tmp(in, in + n);
+    HIPCHK(hipMemcpy(e->dj[comp], tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return XF_OK;
+}
+
 extern "C" xf_status xf_engine_drude_box(xf_engine *e, int64_t *box) {
   if (!e || !box) FAILS(XF_EINVAL, "null argument");
   if (!e->has_drude) FAILS(XF_ESTATE, "no Drude box");
@@ -1173,6 +1216,9 @@ static xf_status adj_setup(xf_engine *e, AdjBufs &B, Adj<T> &a) {
     const size_t nb = e->psi_bytes[slot_axis[q]];
     if (!(a.Pe[q] = (T *)B.alloc(nb)) || !(a.Ph[q] = (T *)B.alloc(nb))) FAILS(XF_ENOMEM, "adjoint psi");
   }
+  if (e->has_drude)
+    for (int q = 0; q < 3; ++q)
+      if (!(a.Ja[q] = (T *)B.alloc(drude_cells(e) * sizeof(T)))) FAILS(XF_ENOMEM, "adjoint Drude currents");
   return XF_OK;
 }
 
@@ -1262,7 +1308,6 @@ extern "C" xf_status xf_engine_grad_eps(xf_engine *e, int checkpoint_every, doub
   if (!e || !grad || !J || checkpoint_every < 0) FAILS(XF_EINVAL, "bad argument");
   if (!e->materials_set || !e->has_design || !e->has_dft)
     FAILS(XF_ESTATE, "the gradient needs materials, a design box and a DFT plane");
-  if (e->has_drude) FAILS(XF_ESTATE, "the adjoint does not cover Drude cells");
   if (e->cur != 0) FAILS(XF_ESTATE, "the gradient runs from step 0 (engine is at step %lld)", e->cur);
   if (e->total < 1) FAILS(XF_ESTATE, "no step horizon");
   if (!e->probe_comp.empty()) {
@@ -1299,6 +1344,16 @@ static xf_status adjoint_run(xf_engine *e, long long nsteps, double *fields) {
     HIPCHK(hipMemcpy(q < 3 ? (void *)a.Ea[q] : (void *)a.Ha[q - 3], buf.data(), buf.size() * sizeof(T),
                      hipMemcpyHostToDevice));
   }
+  // the Drude J adjoints follow in E shapes; only their box part is state
+  size_t joff[3] = {0, 0, 0};
+  if (e->has_drude)
+    for (int q = 0; q < 3; ++q) {
+      int sh[3];
+      comp_shape(e, q, sh);
+      joff[q] = off;
+      off += (size_t)sh[0] * sh[1] * sh[2];
+      if ((s = drude_box_copy<T>(e, q, a.Ja[q], fields + joff[q], true)) != XF_OK) return s;
+    }
   const Grid<T> gr = make_grid<T>(e);
   const unsigned nb = (unsigned)((e->ncell + 255) / 256);
   // the H transpose of the last step, then E and H transposes per step; the
@@ -1320,13 +1375,15 @@ static xf_status adjoint_run(xf_engine *e, long long nsteps, double *fields) {
       for (int j = 0; j < sh[1]; ++j)
         for (int k = 0; k < sh[2]; ++k) fields[off++] = (double)buf[(size_t)i * e->si + (size_t)j * e->sj + k];
   }
+  if (e->has_drude)
+    for (int q = 0; q < 3; ++q)
+      if ((s = drude_box_copy<T>(e, q, a.Ja[q], fields + joff[q], false)) != XF_OK) return s;
   return XF_OK;
 }
 
 extern "C" xf_status xf_engine_adjoint_run(xf_engine *e, int64_t nsteps, double *fields) {
   if (!e || !fields || nsteps < 1) FAILS(XF_EINVAL, "bad argument");
   if (!e->materials_set) FAILS(XF_ESTATE, "materials not set");
-  if (e->has_drude) FAILS(XF_ESTATE, "the adjoint does not cover Drude cells");
   HIPCHK(engine_wait(e));
   return e->prec == 64 ? adjoint_run<double>(e, nsteps, fields) : adjoint_run<float>(e, nsteps, fields);
 }

@@ -1238,6 +1238,7 @@ template <typename T>
 struct Adj {
   T *Ea[3], *Ha[3];
   T *Pe[6], *Ph[6];   // psi adjoints, psi layouts
+  T *Ja[3];           // Drude J adjoints, Drude-box layout
   T *W[6], *V[6];     // padded grid
   // objective injection (plane of comp < 3) and design gradient
   DftPlane dft;       // dft.acc = final forward F
@@ -1356,11 +1357,46 @@ __global__ void __launch_bounds__(256) k_adj_e(Grid<T> g, Adj<T> a, long long st
       p = g.be[1][j] * chi;
     }
   }
+  T ox = yx ? ca_x * ex : ex, oy = yy ? ca_y * ey : ey, oz = yz ? ca_z * ez : ez;
+  // Drude cells (outside every CPML slab): the transpose of
+  //   E' = ca E + cb (curl - (dl/2)(1+k) J),  J' = k J + beta (E' + E)
+  // taken J' first: t = adjoint of E' + beta * adjoint of J'; the curl
+  // weights are -cb t on the first difference of the reference's bracket and
+  // +cb t on the second
+  if (in_drude(g, i, j, k)) {
+    const T *dc = g.drc + 6 * m;
+    const T cb = dc[1];
+    if (cb != T(0)) {
+      const int dq = drude_cell(g, i, j, k);
+      const T ca = dc[0], hk = dc[5] * dc[2], kk = dc[3], beta = dc[4];
+      if (yx) {
+        const T jb = a.Ja[0][dq], t = ex + beta * jb;
+        vxy = -cb * t;  // -(Hy(k) - Hy(k-1))
+        vxz = cb * t;   // +(Hz(j) - Hz(j-1))
+        ox = beta * jb + ca * t;
+        a.Ja[0][dq] = kk * jb - cb * hk * t;
+      }
+      if (yy) {
+        const T jb = a.Ja[1][dq], t = ey + beta * jb;
+        vyz = -cb * t;  // -(Hz(i) - Hz(i-1))
+        vyx = cb * t;   // +(Hx(k) - Hx(k-1))
+        oy = beta * jb + ca * t;
+        a.Ja[1][dq] = kk * jb - cb * hk * t;
+      }
+      if (yz) {
+        const T jb = a.Ja[2][dq], t = ez + beta * jb;
+        vzx = -cb * t;  // -(Hx(j) - Hx(j-1))
+        vzy = cb * t;   // +(Hy(i) - Hy(i-1))
+        oz = beta * jb + ca * t;
+        a.Ja[2][dq] = kk * jb - cb * hk * t;
+      }
+    }
+  }
   a.V[W_XY][c] = vxy; a.V[W_XZ][c] = vxz; a.V[W_YZ][c] = vyz;
   a.V[W_YX][c] = vyx; a.V[W_ZX][c] = vzx; a.V[W_ZY][c] = vzy;
-  a.Ea[0][c] = yx ? ca_x * ex : ex;
-  a.Ea[1][c] = yy ? ca_y * ey : ey;
-  a.Ea[2][c] = yz ? ca_z * ez : ez;
+  a.Ea[0][c] = ox;
+  a.Ea[1][c] = oy;
+  a.Ea[2][c] = oz;
 }
 
 // per H cell: gather of the E half-step transpose, then (with h1) the next

@@ -921,6 +921,12 @@ void Simulation::adjointRun(size_t n, std::vector<double> &fields) {
     auto sh = fieldShape(static_cast<EMComponent>(c));
     total += sh[0] * sh[1] * sh[2];
   }
+  // with Drude cells the three J adjoints follow, in the E shapes
+  if (_plan && _plan->drude.present)
+    for (int c = 0; c < 3; ++c) {
+      auto sh = fieldShape(static_cast<EMComponent>(c));
+      total += sh[0] * sh[1] * sh[2];
+    }
   if (fields.size() != total) throw std::invalid_argument("adjoint fields have the wrong size");
   check(xf_engine_adjoint_run(_engine, (int64_t)n, fields.data()));
 }

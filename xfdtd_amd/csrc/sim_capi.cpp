@@ -91,6 +91,14 @@ extern "C" int xf_sim_drude_box(xf_sim *s, int64_t box[6]) {
   });
 }
 
+extern "C" int xf_sim_set_drude_j(xf_sim *s, int comp, const double *in) {
+  return guarded([&] {
+    xf_engine *e = s->sim.engine();
+    if (!e) throw std::runtime_error("simulation not initialised");
+    if (xf_engine_set_drude_j(e, comp, in) != XF_OK) throw std::runtime_error(xf_last_error());
+  });
+}
+
 extern "C" int xf_sim_get_drude_j(xf_sim *s, int comp, double *out) {
   return guarded([&] {
     xf_engine *e = s->sim.engine();

@@ -174,6 +174,11 @@ class Simulation:
         self._chk(lib().xf_sim_drude_box(self.h, b))
         return tuple(int(v) for v in b)
 
+    def set_drude_j(self, comp, arr):
+        """Set J of component comp over the Drude box."""
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        self._chk(lib().xf_sim_set_drude_j(self.h, int(comp), _dp(a)))
+
     def get_drude_j(self, comp):
         """J of component comp over the Drude box [i0,i1)x[j0,j1)x[k0,k1)."""
         b = self.drude_box()
@@ -204,11 +209,12 @@ class Simulation:
         return J.value, g
 
     def adjoint_run(self, nsteps, fields):
-        """Transpose of nsteps steps of the field map on six adjoint fields."""
+        """Transpose of nsteps steps of the field map on the adjoint state: six
+        fields, then (with Drude cells) the three J adjoints in E shapes."""
         packed = np.concatenate([np.ascontiguousarray(f, dtype=np.float64).ravel() for f in fields])
         self._chk(lib().xf_sim_adjoint_run(self.h, int(nsteps), _dp(packed), packed.size))
         out, off = [], 0
-        for c in range(6):
+        for c in list(range(6)) + list(range(len(fields) - 6)):
             n = int(np.prod(self.field_shape(c)))
             out.append(packed[off:off + n].reshape(self.field_shape(c)).copy())
             off += n
