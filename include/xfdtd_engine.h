@@ -257,6 +257,14 @@ xf_status xf_batch_step(xf_batch *b, int64_t n);
 xf_status xf_batch_sync(xf_batch *b);
 int xf_batch_size(const xf_batch *b);
 void xf_batch_destroy(xf_batch *b);
+/* xf_engine_grad_eps for every member at once (all members at step 0, with
+   a design box, a DFT plane and the same step horizon): the forward run and
+   the recomputed segments step as a batch, the reverse steps are one launch
+   per kernel over all members.  grads[q] receives member q's dJ/d eps_r,
+   J[q] its objective; the results equal each member's own
+   xf_engine_grad_eps.  checkpoint_every = 0: the whole run when all members'
+   design fields fit in 4 GiB, else 100. */
+xf_status xf_batch_grad_eps(xf_batch *b, int checkpoint_every, double *const *grads, double *J);
 
 #ifdef __cplusplus
 }

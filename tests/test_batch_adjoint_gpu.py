@@ -1,0 +1,57 @@
+"""Batched gradients (xf_batch_grad_eps): every member's J and dJ/d eps_r
+equal its own xf_engine_grad_eps bit for bit (FP64 and FP32, uneven
+checkpoint segments and none), one member matches the oracle's gradient,
+and the members are left at the end of the forward run."""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+from xfdtd_amd.workloads import batched_adjoint_set
+
+pytestmark = pytest.mark.gpu
+
+
+def small_set():
+    """Three members of the batched-gradient workload at n = 34: Drude block,
+    design cube, TFSF at per-member incidence, objective plane past both."""
+    return batched_adjoint_set(count=3, seed=0xC5, n=34, pml=5, box=(12, 18), drude=(8, 12), tfsf_gap=7)
+
+
+def _sim(sc, steps, precision):
+    from xfdtd_amd.sim import Simulation
+    return Simulation(sc, steps, precision=precision)
+
+
+@pytest.mark.parametrize("precision,every", [(64, 7), (64, 0), (32, 9)])
+def test_batch_grad_equals_single(precision, every):
+    from xfdtd_amd.sim import Batch
+    steps = 60
+    scs = small_set()
+    single = [_sim(sc, steps, precision).grad(checkpoint_every=every) for sc in scs]
+    members = [_sim(sc, steps, precision) for sc in scs]
+    b = Batch(members)
+    got = b.grad(checkpoint_every=every)
+    for q, ((J1, g1), (J2, g2)) in enumerate(zip(single, got)):
+        assert J1 == J2, q
+        assert np.array_equal(g1, g2), q
+        assert np.abs(g1).max() > 0
+    # members end at the forward run's last step, with its fields
+    ref = _sim(scs[1], steps, precision)
+    ref.step(steps)
+    assert members[1].current_step == steps
+    for c in range(6):
+        assert np.array_equal(members[1].get_field(c), ref.get_field(c)), c
+    b.close()
+
+
+def test_batch_grad_member_matches_oracle(oracle_threads):
+    from xfdtd_amd.sim import Batch
+    steps = 60
+    scs = small_set()
+    members = [_sim(sc, steps, 64) for sc in scs]
+    b = Batch(members)
+    got = b.grad(checkpoint_every=11)
+    Jo, go = OracleSim(scs[2], steps).grad()
+    assert got[2][0] == pytest.approx(Jo, rel=1e-14)
+    assert np.abs(got[2][1] - go).max() <= 1e-10 * np.abs(go).max()
+    b.close()

@@ -21,7 +21,7 @@ ENGINE_SYMBOLS = [
     "xf_engine_add_probe", "xf_engine_get_probes", "xf_engine_step", "xf_engine_sync",
     "xf_engine_current_step", "xf_engine_get_field", "xf_engine_set_field",
     "xf_engine_kernel_times", "xf_engine_set_timing", "xf_engine_set_tiling",
-    "xf_batch_create", "xf_batch_step", "xf_batch_sync", "xf_batch_size", "xf_batch_destroy",
+    "xf_batch_create", "xf_batch_step", "xf_batch_sync", "xf_batch_size", "xf_batch_destroy", "xf_batch_grad_eps",
     "xf_engine_set_design", "xf_engine_set_dft_plane", "xf_engine_get_dft", "xf_engine_checkpoint",
     "xf_engine_restore", "xf_engine_grad_eps", "xf_engine_adjoint_run", "xf_engine_set_drude",
     "xf_engine_drude_box", "xf_engine_get_drude_j", "xf_engine_set_horizon", "xf_engine_set_drude_j",
@@ -94,6 +94,7 @@ def lib():
     L.xf_batch_sync.argtypes = [vp]
     L.xf_batch_size.argtypes = [vp]
     L.xf_batch_destroy.argtypes = [vp]
+    L.xf_batch_grad_eps.argtypes = [vp, i, C.POINTER(dp), dp]
     L.xf_sim_set_design.argtypes = [vp, szp, C.POINTER(C.c_float)]
     L.xf_sim_set_dft_plane.argtypes = [vp, i, i, sz, sz, sz, sz, sz, d]
     L.xf_sim_get_dft.argtypes = [vp, dp]

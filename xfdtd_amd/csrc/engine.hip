@@ -795,7 +795,8 @@ struct xf_batch {
   hipStream_t stream = nullptr;
   void *inst = nullptr;  // Inst<T>[n] on the device
   int2 *map_curl = nullptr, *map_te = nullptr, *map_th = nullptr, *map_ifa = nullptr, *map_nf = nullptr;
-  int n_curl = 0, n_te = 0, n_th = 0, n_ifa = 0, n_nf = 0;
+  int2 *map_dft = nullptr;
+  int n_curl = 0, n_te = 0, n_th = 0, n_ifa = 0, n_nf = 0, n_dft = 0;
   int threads = 256;
   size_t lds = 0;
   bool use_lds = true;
@@ -814,7 +815,7 @@ template <typename T>
 static xf_status batch_build(xf_batch *b) {
   const int n = (int)b->e.size();
   std::vector<Inst<T>> in(n);
-  std::vector<int2> mc, mte, mth, mifa, mnf;
+  std::vector<int2> mc, mte, mth, mifa, mnf, mdft;
   b->lds = 0;
   b->use_lds = true;
   for (int q = 0; q < n; ++q) {
@@ -842,6 +843,11 @@ static xf_status batch_build(xf_batch *b) {
                             (long long)e->nf.nxb * e->nf.nyb;
       for (long long w = 0; w < (tot + 255) / 256; ++w) mnf.push_back(make_int2(q, (int)w));
     }
+    if (e->has_dft) {
+      a.dft = e->dft;
+      const int nn = (e->dft.hi1 - e->dft.lo1) * (e->dft.hi2 - e->dft.lo2);
+      for (int w = 0; w < (nn + 255) / 256; ++w) mdft.push_back(make_int2(q, w));
+    }
   }
   if (!b->use_lds) b->lds = 0;
   if (mc.size() > 0x7fffffffu) FAILS(XF_EINVAL, "batch too large");
@@ -858,6 +864,8 @@ static xf_status batch_build(xf_batch *b) {
   b->map_ifa = (int2 *)p; b->n_ifa = (int)mifa.size();
   if ((s = batch_upload(b, &p, mnf.data(), mnf.size() * sizeof(int2))) != XF_OK) return s;
   b->map_nf = (int2 *)p; b->n_nf = (int)mnf.size();
+  if ((s = batch_upload(b, &p, mdft.data(), mdft.size() * sizeof(int2))) != XF_OK) return s;
+  b->map_dft = (int2 *)p; b->n_dft = (int)mdft.size();
   return XF_OK;
 }
 
@@ -941,12 +949,7 @@ static xf_status batch_run(xf_batch *b, long long n) {
       hipLaunchKernelGGL(kb_tfsf_h<T>, dim3(b->n_th), dim3(256), 0, b->stream, in, b->map_th);
     }
     if (b->n_nf > 0) hipLaunchKernelGGL(kb_nffft<T>, dim3(b->n_nf), dim3(256), 0, b->stream, in, b->map_nf, step);
-    for (xf_engine *e : b->e)
-      if (e->has_dft) {
-        const int nn = (e->dft.hi1 - e->dft.lo1) * (e->dft.hi2 - e->dft.lo2);
-        hipLaunchKernelGGL(k_dft_plane<T>, dim3((nn + 255) / 256), dim3(256), 0, b->stream,
-                           (const T *)e->f[e->dft.comp], e->si, e->sj, e->dft, step);
-      }
+    if (b->n_dft > 0) hipLaunchKernelGGL(kb_dft_plane<T>, dim3(b->n_dft), dim3(256), 0, b->stream, in, b->map_dft, step);
     for (xf_engine *e : b->e)
       if (!e->probe_comp.empty()) {
         const int np = (int)e->probe_comp.size();
@@ -1139,7 +1142,8 @@ extern "C" xf_status xf_engine_get_dft(xf_engine *e, double *out) {
 }
 
 // copies of every state buffer, owned by the caller of these helpers
-static xf_status state_save(xf_engine *e, std::vector<void *> &dst) {
+static xf_status state_save(xf_engine *e, std::vector<void *> &dst, hipStream_t st = nullptr) {
+  if (!st) st = e->stream;
   if (dst.empty()) {
     for (auto &b : e->state) {
       void *p = nullptr;
@@ -1148,12 +1152,13 @@ static xf_status state_save(xf_engine *e, std::vector<void *> &dst) {
     }
   }
   for (size_t q = 0; q < e->state.size(); ++q)
-    HIPCHK(hipMemcpyAsync(dst[q], e->state[q].first, e->state[q].second, hipMemcpyDeviceToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(dst[q], e->state[q].first, e->state[q].second, hipMemcpyDeviceToDevice, st));
   return XF_OK;
 }
-static xf_status state_load(xf_engine *e, const std::vector<void *> &src) {
+static xf_status state_load(xf_engine *e, const std::vector<void *> &src, hipStream_t st = nullptr) {
+  if (!st) st = e->stream;
   for (size_t q = 0; q < e->state.size(); ++q)
-    HIPCHK(hipMemcpyAsync(e->state[q].first, src[q], e->state[q].second, hipMemcpyDeviceToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->state[q].first, src[q], e->state[q].second, hipMemcpyDeviceToDevice, st));
   return XF_OK;
 }
 static void state_free(std::vector<void *> &v) {
@@ -1386,4 +1391,148 @@ extern "C" xf_status xf_engine_adjoint_run(xf_engine *e, int64_t nsteps, double 
   if (!e->materials_set) FAILS(XF_ESTATE, "materials not set");
   HIPCHK(engine_wait(e));
   return e->prec == 64 ? adjoint_run<double>(e, nsteps, fields) : adjoint_run<float>(e, nsteps, fields);
+}
+
+// ---------------------------------------------------------------------------
+// Batched gradient: xf_engine_grad_eps for every member of a batch at once.
+// The forward and recomputed segments step through batch_run; the design
+// snapshots and the reverse steps are one launch per kernel over all members.
+template <typename T>
+static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double *Js) {
+  const int n = (int)b->e.size();
+  const long long T_ = b->e[0]->total;
+  const int nseg = (int)((T_ + K - 1) / K);
+  hipStream_t st = b->stream;
+  std::vector<std::vector<std::vector<void *>>> ck(n, std::vector<std::vector<void *>>(nseg + 1));
+  struct Free {
+    std::vector<std::vector<std::vector<void *>>> &v;
+    ~Free() {
+      for (auto &m : v)
+        for (auto &x : m) state_free(x);
+    }
+  } free_ck{ck};
+  std::vector<AdjBufs> B(n);
+  std::vector<InstAdj<T>> ia(n);
+  std::vector<double *> gdev(n);
+  std::vector<double2 *> Ffin(n);
+  std::vector<int2> msnap, mcell;
+  for (int q = 0; q < n; ++q) {
+    xf_engine *e = b->e[q];
+    InstAdj<T> &x = ia[q];
+    std::memset(&x, 0, sizeof x);
+    x.g = make_grid<T>(e);
+    x.hist = (T *)B[q].alloc((size_t)(K + 1) * e->nd * 3 * sizeof(T));
+    gdev[q] = (double *)B[q].alloc((size_t)e->nd * sizeof(double));
+    const size_t np = (size_t)(e->dft.hi1 - e->dft.lo1) * (e->dft.hi2 - e->dft.lo2);
+    Ffin[q] = (double2 *)B[q].alloc(np * sizeof(double2));
+    if (!x.hist || !gdev[q] || !Ffin[q]) FAILS(XF_ENOMEM, "adjoint buffers of member %d", q);
+    xf_status s = adj_setup<T>(e, B[q], x.a);
+    if (s != XF_OK) return s;
+    x.a.dft = e->dft;
+    x.a.dft.acc = Ffin[q];
+    x.a.inject = e->dft.comp < 3;
+    x.a.hist = x.hist;
+    x.a.eps_r = e->d_eps;
+    x.a.grad = gdev[q];
+    x.a.grad_on = 1;
+    for (int w = 0; w < (e->nd + 255) / 256; ++w) msnap.push_back(make_int2(q, w));
+    for (long long w = 0; w < (e->ncell + 255) / 256; ++w) mcell.push_back(make_int2(q, (int)w));
+  }
+  if (mcell.size() > 0x7fffffffu) FAILS(XF_EINVAL, "batch too large");
+  AdjBufs D;
+  InstAdj<T> *d_ia = (InstAdj<T> *)D.alloc(n * sizeof(InstAdj<T>));
+  int2 *d_snap = (int2 *)D.alloc(msnap.size() * sizeof(int2));
+  int2 *d_cell = (int2 *)D.alloc(mcell.size() * sizeof(int2));
+  if (!d_ia || !d_snap || !d_cell) FAILS(XF_ENOMEM, "batch adjoint tables");
+  HIPCHK(hipMemcpy(d_ia, ia.data(), n * sizeof(InstAdj<T>), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_snap, msnap.data(), msnap.size() * sizeof(int2), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_cell, mcell.data(), mcell.size() * sizeof(int2), hipMemcpyHostToDevice));
+  const unsigned nsnap = (unsigned)msnap.size(), ncell = (unsigned)mcell.size();
+  auto snap = [&](int slot) {
+    hipLaunchKernelGGL(kb_design_snap<T>, dim3(nsnap), dim3(256), 0, st, d_ia, d_snap, slot);
+  };
+  // forward, with member checkpoints at segment starts and the last
+  // segment's design fields kept
+  for (int sgi = 0; sgi < nseg; ++sgi) {
+    const long long m = std::min<long long>(K, T_ - (long long)sgi * K);
+    xf_status s;
+    if (sgi + 1 < nseg) {
+      for (int q = 0; q < n; ++q)
+        if ((s = state_save(b->e[q], ck[q][sgi], st)) != XF_OK) return s;
+      if ((s = batch_run<T>(b, m)) != XF_OK) return s;
+      continue;
+    }
+    snap(0);
+    for (long long l = 0; l < m; ++l) {
+      if ((s = batch_run<T>(b, 1)) != XF_OK) return s;
+      snap((int)(l + 1));
+    }
+  }
+  for (int q = 0; q < n; ++q) {
+    xf_engine *e = b->e[q];
+    xf_status s = state_save(e, ck[q][nseg], st);
+    if (s != XF_OK) return s;
+    const size_t np = (size_t)(e->dft.hi1 - e->dft.lo1) * (e->dft.hi2 - e->dft.lo2);
+    HIPCHK(hipMemcpyAsync(Ffin[q], e->dft.acc, np * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  for (int q = 0; q < n; ++q) {
+    xf_engine *e = b->e[q];
+    const size_t np = (size_t)(e->dft.hi1 - e->dft.lo1) * (e->dft.hi2 - e->dft.lo2);
+    std::vector<double2> F(np);
+    HIPCHK(hipMemcpy(F.data(), Ffin[q], np * sizeof(double2), hipMemcpyDeviceToHost));
+    double Jv = 0;
+    for (const double2 &f : F) Jv += f.x * f.x + f.y * f.y;
+    Js[q] = Jv;
+  }
+  // reverse, segment by segment
+  for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+    const long long t0 = (long long)sgi * K, m = std::min<long long>(K, T_ - t0);
+    xf_status s;
+    if (sgi + 1 < nseg) {
+      for (int q = 0; q < n; ++q) {
+        if ((s = state_load(b->e[q], ck[q][sgi], st)) != XF_OK) return s;
+        b->e[q]->cur = t0;
+      }
+      snap(0);
+      for (long long l = 0; l < m; ++l) {
+        if ((s = batch_run<T>(b, 1)) != XF_OK) return s;
+        snap((int)(l + 1));
+      }
+    }
+    for (long long l = m - 1; l >= 0; --l) {
+      hipLaunchKernelGGL(kb_adj_e<T>, dim3(ncell), dim3(256), 0, st, d_ia, d_cell, t0 + l, (int)l);
+      hipLaunchKernelGGL(kb_adj_h<T>, dim3(ncell), dim3(256), 0, st, d_ia, d_cell, 1);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  // leave every member at the end of the forward run
+  for (int q = 0; q < n; ++q) {
+    xf_status s = state_load(b->e[q], ck[q][nseg], st);
+    if (s != XF_OK) return s;
+    b->e[q]->cur = T_;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  for (int q = 0; q < n; ++q)
+    HIPCHK(hipMemcpy(grads[q], gdev[q], (size_t)b->e[q]->nd * sizeof(double), hipMemcpyDeviceToHost));
+  return XF_OK;
+}
+
+extern "C" xf_status xf_batch_grad_eps(xf_batch *b, int checkpoint_every, double *const *grads, double *J) {
+  if (!b || !grads || !J || checkpoint_every < 0) FAILS(XF_EINVAL, "bad argument");
+  double bytes = 0;
+  for (size_t q = 0; q < b->e.size(); ++q) {
+    xf_engine *e = b->e[q];
+    if (!grads[q]) FAILS(XF_EINVAL, "null gradient buffer %zu", q);
+    if (!e->has_design || !e->has_dft)
+      FAILS(XF_ESTATE, "member %zu: the gradient needs a design box and a DFT plane", q);
+    if (e->cur != 0) FAILS(XF_ESTATE, "member %zu is at step %lld; the gradient runs from step 0", q, e->cur);
+    if (e->total < 1 || e->total != b->e[0]->total)
+      FAILS(XF_ESTATE, "member %zu: every member needs the same step horizon", q);
+    HIPCHK(hipStreamSynchronize(e->stream));
+    bytes += (double)(e->total + 1) * e->nd * 3 * e->esz;
+  }
+  int K = checkpoint_every;
+  if (K == 0) K = bytes <= 4.0 * (1 << 30) ? (int)std::min<long long>(b->e[0]->total, 1 << 30) : 100;
+  return b->prec == 64 ? batch_grad_run<double>(b, K, grads, J) : batch_grad_run<float>(b, K, grads, J);
 }

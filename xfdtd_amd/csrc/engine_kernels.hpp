@@ -1189,9 +1189,9 @@ __device__ __forceinline__ long long plane_cell(const DftPlane &d, long long si,
 }
 
 template <typename T>
-__global__ void k_dft_plane(const T *f, long long si, long long sj, DftPlane d, long long step) {
+__device__ __forceinline__ void dft_plane_body(const T *f, long long si, long long sj, const DftPlane &d,
+                                               long long step, int q) {
   const int n2 = d.hi2 - d.lo2, n = (d.hi1 - d.lo1) * n2;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const int u = d.lo1 + q / n2, v = d.lo2 + q % n2;
   const double x = (double)f[plane_cell(d, si, sj, u, v)];
@@ -1201,13 +1201,16 @@ __global__ void k_dft_plane(const T *f, long long si, long long sj, DftPlane d, 
   a.y = a.y + x * w.y;
   d.acc[q] = a;
 }
+template <typename T>
+__global__ void k_dft_plane(const T *f, long long si, long long sj, DftPlane d, long long step) {
+  dft_plane_body<T>(f, si, sj, d, step, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // E of every design cell into hist[slot][cell][3] (for the gradient)
 template <typename T>
-__global__ void k_design_snap(Grid<T> g, T *hist, int slot) {
+__device__ __forceinline__ void design_snap_body(const Grid<T> &g, T *hist, int slot, int q) {
   const int nj = g.db[3] - g.db[2], nk = g.db[5] - g.db[4];
   const int nd = (g.db[1] - g.db[0]) * nj * nk;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nd) return;
   const int i = g.db[0] + q / (nj * nk), j = g.db[2] + (q / nk) % nj, k = g.db[4] + q % nk;
   const long long c = (long long)i * g.si + (long long)j * g.sj + k;
@@ -1215,6 +1218,10 @@ __global__ void k_design_snap(Grid<T> g, T *hist, int slot) {
   h[0] = g.f[EX][c];
   h[1] = g.f[EY][c];
   h[2] = g.f[EZ][c];
+}
+template <typename T>
+__global__ void k_design_snap(Grid<T> g, T *hist, int slot) {
+  design_snap_body<T>(g, hist, slot, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1257,8 +1264,8 @@ __device__ __forceinline__ P2<T> pair_at(const T *tab, int n, int nmat, int m) {
 // per E cell: gather of the H half-step transpose, objective injection,
 // gradient, then the E half-step transpose's weights and scaling
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_e(Grid<T> g, Adj<T> a, long long step, int slot) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void adj_e_body(const Grid<T> &g, const Adj<T> &a, long long step, int slot,
+                                           long long c) {
   const int i = (int)(c / g.si), r = (int)(c - (long long)i * g.si), j = r / (int)g.sj, k = r - j * (int)g.sj;
   const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
   if (i > nx || j > ny || k > nz) return;
@@ -1398,12 +1405,15 @@ __global__ void __launch_bounds__(256) k_adj_e(Grid<T> g, Adj<T> a, long long st
   a.Ea[1][c] = oy;
   a.Ea[2][c] = oz;
 }
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_e(Grid<T> g, Adj<T> a, long long step, int slot) {
+  adj_e_body<T>(g, a, step, slot, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // per H cell: gather of the E half-step transpose, then (with h1) the next
 // H half-step transpose's weights and scaling
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_h(Grid<T> g, Adj<T> a, int h1) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void adj_h_body(const Grid<T> &g, const Adj<T> &a, int h1, long long c) {
   const int i = (int)(c / g.si), r = (int)(c - (long long)i * g.si), j = r / (int)g.sj, k = r - j * (int)g.sj;
   const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
   if (i > nx || j > ny || k > nz) return;
@@ -1476,6 +1486,10 @@ __global__ void __launch_bounds__(256) k_adj_h(Grid<T> g, Adj<T> a, int h1) {
   a.Ha[1][c] = yee ? self[1] * hy : hy;
   a.Ha[2][c] = yee ? self[2] * hz : hz;
 }
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_h(Grid<T> g, Adj<T> a, int h1) {
+  adj_h_body<T>(g, a, h1, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // ---------------------------------------------------------------------------
 // Batched launches: many independent simulations of one precision stepped
@@ -1490,6 +1504,16 @@ struct Inst {
   int nEy, nEz, nEx;
   TfsfHSeg hs;
   Nffft nf;
+  DftPlane dft;
+};
+
+// an instance of a batched gradient: its grid, adjoint buffers and the
+// segment's design-box E history
+template <typename T>
+struct InstAdj {
+  Grid<T> g;
+  Adj<T> a;
+  T *hist;
 };
 
 template <typename T, bool LDS>
@@ -1524,6 +1548,31 @@ template <typename T>
 __global__ void kb_nffft(const Inst<T> *__restrict__ in, const int2 *__restrict__ map, long long step) {
   const int2 w = map[blockIdx.x];
   nffft_body<T>(in[w.x].g, in[w.x].nf, step, w.y);
+}
+// 256-thread blocks over each instance's DFT plane
+template <typename T>
+__global__ void kb_dft_plane(const Inst<T> *__restrict__ in, const int2 *__restrict__ map, long long step) {
+  const int2 w = map[blockIdx.x];
+  const Inst<T> &a = in[w.x];
+  dft_plane_body<T>(a.g.f[a.dft.comp], a.g.si, a.g.sj, a.dft, step, w.y * 256 + threadIdx.x);
+}
+// 256-thread blocks over each instance's design box / padded grid
+template <typename T>
+__global__ void kb_design_snap(const InstAdj<T> *__restrict__ in, const int2 *__restrict__ map, int slot) {
+  const int2 w = map[blockIdx.x];
+  design_snap_body<T>(in[w.x].g, in[w.x].hist, slot, w.y * 256 + threadIdx.x);
+}
+template <typename T>
+__global__ void __launch_bounds__(256) kb_adj_e(const InstAdj<T> *__restrict__ in, const int2 *__restrict__ map,
+                                                long long step, int slot) {
+  const int2 w = map[blockIdx.x];
+  adj_e_body<T>(in[w.x].g, in[w.x].a, step, slot, (long long)w.y * 256 + threadIdx.x);
+}
+template <typename T>
+__global__ void __launch_bounds__(256) kb_adj_h(const InstAdj<T> *__restrict__ in, const int2 *__restrict__ map,
+                                                int h1) {
+  const int2 w = map[blockIdx.x];
+  adj_h_body<T>(in[w.x].g, in[w.x].a, h1, (long long)w.y * 256 + threadIdx.x);
 }
 
 }  // namespace xfk

@@ -272,6 +272,20 @@ class Batch:
         if L.xf_batch_sync(self.h) != 0:
             raise RuntimeError("xfdtd_amd: " + L.xf_last_error().decode())
 
+    def grad(self, checkpoint_every=0):
+        """[(J, dJ/d eps_r over the design box)] of every member, from step 0
+        (xf_batch_grad_eps: the members' gradients in one batched run)."""
+        L = lib()
+        gs = []
+        for s in self.sims:
+            b = s.scene.design.box
+            gs.append(np.empty((b[1] - b[0], b[3] - b[2], b[5] - b[4]), dtype=np.float64))
+        ptrs = (C.POINTER(C.c_double) * len(gs))(*[_dp(g) for g in gs])
+        J = np.empty(len(gs), dtype=np.float64)
+        if L.xf_batch_grad_eps(self.h, int(checkpoint_every), ptrs, _dp(J)) != 0:
+            raise RuntimeError("xfdtd_amd: " + L.xf_last_error().decode())
+        return [(float(J[q]), gs[q]) for q in range(len(gs))]
+
     def close(self):
         if getattr(self, "h", None):
             lib().xf_batch_destroy(self.h)

@@ -8,6 +8,8 @@ from __future__ import annotations
 
 import math
 
+import numpy as np
+
 from . import scene as S
 
 _MASK = (1 << 64) - 1
@@ -30,6 +32,19 @@ class SplitMix64:
     def uniform(self, lo=0.0, hi=1.0):
         """53-bit uniform double in [lo, hi)."""
         return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
+
+    def uniform_array(self, n, lo=0.0, hi=1.0):
+        """The next n values of uniform(lo, hi), computed together (the same
+        bits as n calls)."""
+        g = np.uint64(0x9E3779B97F4A7C15)
+        with np.errstate(over="ignore"):
+            z = np.uint64(self.state) + g * np.arange(1, n + 1, dtype=np.uint64)
+            self.state = int(z[-1]) if n else self.state
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+        return lo + (hi - lo) * u
 
     def integer(self, lo, hi):
         """Uniform integer in [lo, hi] (inclusive), by rejection."""
@@ -107,8 +122,44 @@ def adjoint_design_scene(n=128, dl=4e-3, pml=10, src=(20, 64, 64), box=(44, 84),
                                       S.GaussianWaveform(tau, 4.5 * tau, 1.0)))
     rng = SplitMix64(seed)
     m = box[1] - box[0]
-    eps = [rng.uniform(1.0, 4.0) for _ in range(m ** 3)]
+    eps = rng.uniform_array(m ** 3, 1.0, 4.0)
     sc.design = S.Design((box[0], box[1]) * 3, eps)
     sc.dft_plane = S.DftPlane(S.EZ, 0, plane_x, plane[0], plane[1], plane[0], plane[1], S.C_0 / (20 * dl))
     return sc
 
+
+
+def batched_adjoint_set(count=32, seed=0xC5, n=96, dl=4e-3, pml=8, box=(32, 64), drude=(20, 30),
+                        tfsf_gap=12):
+    """The batched design-gradient workload: `count` n^3 total grids (CPML
+    included, order 3, sigma 0.8 sigma_opt, kappa 1, alpha 0.05), cfl 0.9,
+    each with
+
+    - a Drude block of cells [drude)^3: eps_inf 1, omega_p 2 pi 20 GHz,
+      nu 2 pi 1 GHz;
+    - a design cube [box)^3 of eps_r i.i.d. uniform in [1, 4] (rounded to
+      float);
+    - a TFSF Gaussian plane wave (distance tfsf_gap) at theta = arccos(u),
+      u uniform in [0, 1], phi uniform in [0, 2 pi), psi = 0;
+    - the objective J = sum |DFT Ez(n - pml - 10, j, k)|^2 over j, k in
+      [pml, n - pml) at f0 = c0/(20 dl).
+
+    Per instance the splitmix64 stream gives theta, phi, then the design
+    permittivities in C order.
+    """
+    rng = SplitMix64(seed)
+    half = (n - 2 * pml) * dl / 2 + pml * dl
+    out = []
+    for _ in range(count):
+        theta = math.acos(rng.uniform())
+        phi = rng.uniform(0.0, 2 * math.pi)
+        lo = -half + drude[0] * dl
+        plasma = S.Object("plasma", S.Cube((lo, lo, lo), ((drude[1] - drude[0]) * dl,) * 3),
+                          S.DrudeMedium("drude", 1.0, 2 * math.pi * 20e9, 2 * math.pi * 1e9))
+        sc = S.cube_domain_scene(n, dl, pml, cfl=0.9, objects=[plasma], tfsf_gap=tfsf_gap,
+                                 incidence=(theta, phi, 0.0))
+        m = box[1] - box[0]
+        sc.design = S.Design((box[0], box[1]) * 3, rng.uniform_array(m ** 3, 1.0, 4.0))
+        sc.dft_plane = S.DftPlane(S.EZ, 0, n - pml - 10, pml, n - pml, pml, n - pml, S.C_0 / (20 * dl))
+        out.append(sc)
+    return out
