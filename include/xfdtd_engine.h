@@ -232,7 +232,8 @@ xf_status xf_engine_restore(xf_engine *e, int slot);
    fields, and the adjoint kernels step back through it; the last segment's
    design fields are kept during the forward run, so an interval >= the run
    length recomputes nothing.  checkpoint_every = 0 chooses: the whole run
-   when its design fields fit in 4 GiB, else 100.  The engine is left at the
+   when its design fields fit in a quarter of the free device memory (at
+   least 4 GiB), else 100.  The engine is left at the
    end of the forward run.  Design cells must be lossless. */
 xf_status xf_engine_grad_eps(xf_engine *e, int checkpoint_every, double *grad, double *J);
 /* the transpose of nsteps steps of the field map applied to six adjoint
@@ -263,7 +264,8 @@ void xf_batch_destroy(xf_batch *b);
    per kernel over all members.  grads[q] receives member q's dJ/d eps_r,
    J[q] its objective; the results equal each member's own
    xf_engine_grad_eps.  checkpoint_every = 0: the whole run when all members'
-   design fields fit in 4 GiB, else 100. */
+   design fields fit in a quarter of the free device memory (at least
+   4 GiB), else 100. */
 xf_status xf_batch_grad_eps(xf_batch *b, int checkpoint_every, double *const *grads, double *J);
 
 #ifdef __cplusplus

@@ -325,6 +325,14 @@ static size_t drude_cells(const xf_engine *e) {
   return (size_t)(e->xb[1] - e->xb[0]) * (e->xb[3] - e->xb[2]) * (e->xb[5] - e->xb[4]);
 }
 
+// bytes the automatic checkpoint interval lets the design history of a
+// gradient run take: a quarter of the free device memory, at least 4 GiB
+static double history_budget() {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  return std::max(4.0 * (1 << 30), 0.25 * (double)free_b);
+}
+
 static void set_total(xf_engine *e, long long T) {
   if (e->total < 0 || T < e->total) e->total = T;
 }
@@ -1323,9 +1331,9 @@ extern "C" xf_status xf_engine_grad_eps(xf_engine *e, int checkpoint_every, doub
   int K = checkpoint_every;
   if (K == 0) {
     // automatic: keep the design fields of the whole run when they fit in
-    // 4 GiB (no recompute), else a checkpoint every 100 steps
+    // the history budget (no recompute), else a checkpoint every 100 steps
     const double bytes = (double)(e->total + 1) * e->nd * 3 * e->esz;
-    K = bytes <= 4.0 * (1 << 30) ? (int)std::min<long long>(e->total, 1 << 30) : 100;
+    K = bytes <= history_budget() ? (int)std::min<long long>(e->total, 1 << 30) : 100;
   }
   return e->prec == 64 ? grad_run<double>(e, K, grad, J) : grad_run<float>(e, K, grad, J);
 }
@@ -1533,6 +1541,6 @@ extern "C" xf_status xf_batch_grad_eps(xf_batch *b, int checkpoint_every, double
     bytes += (double)(e->total + 1) * e->nd * 3 * e->esz;
   }
   int K = checkpoint_every;
-  if (K == 0) K = bytes <= 4.0 * (1 << 30) ? (int)std::min<long long>(b->e[0]->total, 1 << 30) : 100;
+  if (K == 0) K = bytes <= history_budget() ? (int)std::min<long long>(b->e[0]->total, 1 << 30) : 100;
   return b->prec == 64 ? batch_grad_run<double>(b, K, grads, J) : batch_grad_run<float>(b, K, grads, J);
 }
