@@ -73,6 +73,11 @@ struct xf_engine {
   Nffft nf{};
   double2 *acc = nullptr, *tj = nullptr, *tm = nullptr;
   long long nf_size = 0;
+  // face values of up to NF_RING steps not yet added to the accumulators
+  // (empty between API calls) and the step of the first
+  void *nf_ring = nullptr;
+  int nf_pending = 0;
+  long long nf_step0 = 0;
   // sources, probes
   std::vector<SourceDev> sources;
   long long src_T = 0;
@@ -622,6 +627,22 @@ static xf_status run_steps(xf_engine *e, long long n) {
   // coefficient tables of a tile go to LDS when they fit in 32 KB
   const size_t lds = lds_table_bytes<T>(e->nmat, tl);
   const bool use_lds = lds <= 32768 && !(e->tile_flags & XF_TILE_GLOBAL_TABLES);
+  // design-box and Drude planes need the E kernel's general variant
+  const bool special = e->has_design || e->has_drude;
+  long long nfQ = 0;
+  if (e->has_nf) {
+    nfQ = (long long)e->nf.nyb * e->nf.nzb + (long long)e->nf.nzb * e->nf.nxb + (long long)e->nf.nxb * e->nf.nyb;
+    if (!e->nf_ring) {
+      xf_status s = dev_alloc(e, &e->nf_ring, (size_t)NF_RING * 8 * nfQ * e->esz);
+      if (s != XF_OK) return s;
+    }
+  }
+  auto nf_flush = [&]() {
+    if (e->nf_pending == 0) return;
+    hipLaunchKernelGGL(k_nffft_flush<T>, dim3((unsigned)((8 * nfQ + 255) / 256)), dim3(256), 0, e->stream, e->nf,
+                       (const T *)e->nf_ring, e->nf_pending, e->nf_step0);
+    e->nf_pending = 0;
+  };
   for (long long q = 0; q < n; ++q) {
     const long long step = e->cur;
     hipEvent_t *ev = nullptr;
@@ -637,10 +658,17 @@ static xf_status run_steps(xf_engine *e, long long n) {
       e->ev_used += 4;
     }
     if (ev) hipEventRecord(ev[0], e->stream);
-    if (use_lds)
-      hipLaunchKernelGGL((k_update_e<T, true>), grd, blk, lds, e->stream, g, tl);
-    else
-      hipLaunchKernelGGL((k_update_e<T, false>), grd, blk, 0, e->stream, g, tl);
+    if (special) {
+      if (use_lds)
+        hipLaunchKernelGGL((k_update_e<T, true, true>), grd, blk, lds, e->stream, g, tl);
+      else
+        hipLaunchKernelGGL((k_update_e<T, false, true>), grd, blk, 0, e->stream, g, tl);
+    } else {
+      if (use_lds)
+        hipLaunchKernelGGL((k_update_e<T, true, false>), grd, blk, lds, e->stream, g, tl);
+      else
+        hipLaunchKernelGGL((k_update_e<T, false, false>), grd, blk, 0, e->stream, g, tl);
+    }
     if (ev) hipEventRecord(ev[1], e->stream);
     for (const SourceDev &sd : e->sources) {
       const int nj = sd.j1 - sd.j0, nk = sd.k1 - sd.k0;
@@ -666,10 +694,10 @@ static xf_status run_steps(xf_engine *e, long long n) {
       hipLaunchKernelGGL(k_tfsf_h<T>, dim3((tot + 255) / 256), dim3(256), 0, e->stream, g, t, e->hseg);
     }
     if (e->has_nf) {
-      const long long tot = (long long)e->nf.nyb * e->nf.nzb + (long long)e->nf.nzb * e->nf.nxb +
-                            (long long)e->nf.nxb * e->nf.nyb;
-      hipLaunchKernelGGL(k_nffft<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, e->stream, g,
-                         e->nf, step);
+      if (e->nf_pending == 0) e->nf_step0 = step;
+      hipLaunchKernelGGL(k_nffft_sample<T>, dim3((unsigned)((nfQ + 255) / 256)), dim3(256), 0, e->stream, g, e->nf,
+                         (T *)e->nf_ring, e->nf_pending);
+      if (++e->nf_pending == NF_RING) nf_flush();
     }
     if (e->has_dft) {
       const int n = (e->dft.hi1 - e->dft.lo1) * (e->dft.hi2 - e->dft.lo2);
@@ -684,6 +712,8 @@ static xf_status run_steps(xf_engine *e, long long n) {
     HIPCHK(hipGetLastError());
     e->cur++;
   }
+  nf_flush();
+  HIPCHK(hipGetLastError());
   return XF_OK;
 }
 
@@ -808,6 +838,7 @@ struct xf_batch {
   int threads = 256;
   size_t lds = 0;
   bool use_lds = true;
+  bool special = false;  // some member has a design box or Drude cells
   std::vector<void *> owned;
 };
 
@@ -826,8 +857,10 @@ static xf_status batch_build(xf_batch *b) {
   std::vector<int2> mc, mte, mth, mifa, mnf, mdft;
   b->lds = 0;
   b->use_lds = true;
+  b->special = false;
   for (int q = 0; q < n; ++q) {
     xf_engine *e = b->e[q];
+    b->special = b->special || e->has_design || e->has_drude;
     Inst<T> &a = in[q];
     std::memset(&a, 0, sizeof a);
     a.g = make_grid<T>(e);
@@ -935,10 +968,17 @@ static xf_status batch_run(xf_batch *b, long long n) {
   const dim3 blk(b->threads);
   for (long long q = 0; q < n; ++q) {
     const long long step = b->e[0]->cur;
-    if (b->use_lds)
-      hipLaunchKernelGGL((kb_update_e<T, true>), dim3(b->n_curl), blk, b->lds, b->stream, in, b->map_curl);
-    else
-      hipLaunchKernelGGL((kb_update_e<T, false>), dim3(b->n_curl), blk, 0, b->stream, in, b->map_curl);
+    if (b->special) {
+      if (b->use_lds)
+        hipLaunchKernelGGL((kb_update_e<T, true, true>), dim3(b->n_curl), blk, b->lds, b->stream, in, b->map_curl);
+      else
+        hipLaunchKernelGGL((kb_update_e<T, false, true>), dim3(b->n_curl), blk, 0, b->stream, in, b->map_curl);
+    } else {
+      if (b->use_lds)
+        hipLaunchKernelGGL((kb_update_e<T, true, false>), dim3(b->n_curl), blk, b->lds, b->stream, in, b->map_curl);
+      else
+        hipLaunchKernelGGL((kb_update_e<T, false, false>), dim3(b->n_curl), blk, 0, b->stream, in, b->map_curl);
+    }
     for (xf_engine *e : b->e)
       for (const SourceDev &sd : e->sources) {
         const int nj = sd.j1 - sd.j0, nk = sd.k1 - sd.k0;

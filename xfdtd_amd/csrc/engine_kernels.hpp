@@ -429,7 +429,7 @@ __device__ __forceinline__ void load_plane_e(const Grid<T> &g, const Col<T> &q, 
   }
 }
 
-template <typename T, bool SLAB>
+template <typename T, bool SLAB, bool SPECIAL>
 __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTables<T> &tt, const Col<T> &q,
                                                 const Plane<T> &p, int i, unsigned cell, T hz_prev,
                                                 T hy_prev) {
@@ -445,7 +445,7 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
   const int mi = (i - tt.i0) * g.nmat + m;
   P4<T> cs = tt.self[m];
   P2<T> ck = tt.k[tt.kcol + m], cj = tt.j[tt.jcol + m], ci = tt.i[mi];
-  if constexpr (SLAB) {
+  if constexpr (SLAB && SPECIAL) {
     // design cells carry their own coefficients (outside every CPML slab)
     if (in_design(g, i, j, k)) {
       const T *d = g.dcoef + 9 * design_cell(g, i, j, k);
@@ -464,7 +464,7 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
   T vx = (i < nx && q.jin && q.kin) ? yx : ex;
   T vy = (ix && j < ny && q.kin) ? yy : ey;
   T vz = (ix && q.jin && k < nz) ? yz : ez;
-  if constexpr (SLAB) {
+  if constexpr (SLAB && SPECIAL) {
     // Drude cells (outside every CPML slab) replace the Yee update
     // (simulation_update.cpp:221-280)
     if (in_drude(g, i, j, k)) {
@@ -523,7 +523,9 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
   sto(q.u2, o, vz);
 }
 
-template <typename T, bool LDS>
+// SPECIAL: the engine has a design box or Drude cells, whose planes take
+// the general path; without them that code is not compiled in.
+template <typename T, bool LDS, bool SPECIAL>
 __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl, int block) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   int bz, by, bx;
@@ -552,9 +554,9 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
   // a wave whose column crosses the design box or the Drude box takes the
   // general path on those boxes' planes
   const bool col_design =
-      __ballot(live && j >= g.db[2] && j < g.db[3] && k >= g.db[4] && k < g.db[5]) != 0ull;
+      SPECIAL && __ballot(live && j >= g.db[2] && j < g.db[3] && k >= g.db[4] && k < g.db[5]) != 0ull;
   const bool col_drude =
-      __ballot(live && j >= g.xb[2] && j < g.xb[3] && k >= g.xb[4] && k < g.xb[5]) != 0ull;
+      SPECIAL && __ballot(live && j >= g.xb[2] && j < g.xb[3] && k >= g.xb[4] && k < g.xb[5]) != 0ull;
   unsigned c = (unsigned)((long long)i0 * g.si + (long long)j * g.sj + k);
   // x-neighbours are carried in registers: Hz(i-1), Hy(i-1)
   T hz_prev = T(0), hy_prev = T(0);
@@ -583,9 +585,9 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
   if (!live) return;
   for (int i = i0;;) {
     if (slab)
-      compute_plane_e<T, true>(g, tt, q, A, i, c, hz_prev, hy_prev);
+      compute_plane_e<T, true, SPECIAL>(g, tt, q, A, i, c, hz_prev, hy_prev);
     else
-      compute_plane_e<T, false>(g, tt, q, A, i, c, hz_prev, hy_prev);
+      compute_plane_e<T, false, SPECIAL>(g, tt, q, A, i, c, hz_prev, hy_prev);
     hz_prev = A.g2;
     hy_prev = A.g1;
     if (++i >= i1) break;
@@ -787,9 +789,9 @@ __device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl
   }
 }
 
-template <typename T, bool LDS>
+template <typename T, bool LDS, bool SPECIAL>
 __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl) {
-  update_e_body<T, LDS>(g, tl, blockIdx.x);
+  update_e_body<T, LDS, SPECIAL>(g, tl, blockIdx.x);
 }
 template <typename T, bool LDS>
 __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl) {
@@ -1025,16 +1027,16 @@ __device__ __forceinline__ void acc_add(double2 *a, double v, double2 tw) {
   *a = x;
 }
 
+// the eight face-centre values of surface cell q (faces x, then y, then z;
+// both faces of a pair): false past the last cell
 template <typename T>
-__device__ __forceinline__ void nffft_body(const Grid<T> &g, const Nffft &n, long long step, int block) {
-  int q = block * (int)blockDim.x + (int)threadIdx.x;
+__device__ __forceinline__ bool nffft_values(const Grid<T> &g, const Nffft &n, int q, T *v, int &pair, int &cells,
+                                             int &cell) {
   const int cx = n.nyb * n.nzb, cy = n.nzb * n.nxb, cz = n.nxb * n.nyb;
   const T *Ex = g.f[EX], *Ey = g.f[EY], *Ez = g.f[EZ];
   const T *Hx = g.f[HX], *Hy = g.f[HY], *Hz = g.f[HZ];
   const long long si = g.si, sj = g.sj;
 #define C3(i, j, k) ((long long)(i) * si + (long long)(j) * sj + (k))
-  T v[8];
-  int pair, cells, cell;
   if (q < cx) {
     pair = 0; cells = cx; cell = q;
     const int j = n.lj + q / n.nzb, k = n.lk + q % n.nzb;
@@ -1072,9 +1074,18 @@ __device__ __forceinline__ void nffft_body(const Grid<T> &g, const Nffft &n, lon
     v[6] = T(-1) * (T(0.25) * (Hy[C3(i, j, rk)] + Hy[C3(i, j + 1, rk)] + Hy[C3(i, j, rk - 1)] + Hy[C3(i, j + 1, rk - 1)]));
     v[7] = T(0.25) * (Hx[C3(i + 1, j, rk)] + Hx[C3(i, j, rk)] + Hx[C3(i + 1, j, rk - 1)] + Hx[C3(i, j, rk - 1)]);
   } else {
-    return;
+    return false;
   }
 #undef C3
+  return true;
+}
+
+template <typename T>
+__device__ __forceinline__ void nffft_body(const Grid<T> &g, const Nffft &n, long long step, int block) {
+  const int q = block * (int)blockDim.x + (int)threadIdx.x;
+  T v[8];
+  int pair, cells, cell;
+  if (!nffft_values<T>(g, n, q, v, pair, cells, cell)) return;
   for (int f = 0; f < n.nf; ++f) {
     const double2 twm = n.tm[(long long)f * n.T_total + step];
     const double2 twj = n.tj[(long long)f * n.T_total + step];
@@ -1087,6 +1098,58 @@ __device__ __forceinline__ void nffft_body(const Grid<T> &g, const Nffft &n, lon
 template <typename T>
 __global__ void k_nffft(Grid<T> g, Nffft n, long long step) {
   nffft_body<T>(g, n, step, blockIdx.x);
+}
+
+// The same running DFT over a window of steps: each step stores its face
+// values in ring slot s ([slot][8][cells of all faces]), and k_nffft_flush
+// adds the window's terms to every accumulator in step order -- the same
+// operations as per-step accumulation, with the accumulator read and
+// written once per window.
+constexpr int NF_RING = 16;
+
+template <typename T>
+__global__ void k_nffft_sample(Grid<T> g, Nffft n, T *ring, int slot) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long Q = (long long)n.nyb * n.nzb + (long long)n.nzb * n.nxb + (long long)n.nxb * n.nyb;
+  T v[8];
+  int pair, cells, cell;
+  if (!nffft_values<T>(g, n, q, v, pair, cells, cell)) return;
+  for (int c = 0; c < 8; ++c) ring[((long long)slot * 8 + c) * Q + q] = v[c];
+}
+
+// one thread per (value c, surface cell): its 8 x nf accumulators' share
+template <typename T>
+__global__ void __launch_bounds__(256) k_nffft_flush(Nffft n, const T *__restrict__ ring, int nslots,
+                                                    long long step0) {
+  const int cx = n.nyb * n.nzb, cy = n.nzb * n.nxb, cz = n.nxb * n.nyb;
+  const long long Q = (long long)cx + cy + cz;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 8 * Q) return;
+  const int c = (int)(t / Q);
+  int q = (int)(t - (long long)c * Q);
+  int pair, cells, cell;
+  if (q < cx) { pair = 0; cells = cx; cell = q; }
+  else if ((q -= cx) < cy) { pair = 1; cells = cy; cell = q; }
+  else { q -= cy; pair = 2; cells = cz; cell = q; }
+  T v[NF_RING];
+#pragma unroll
+  for (int s = 0; s < NF_RING; ++s)
+    if (s < nslots) v[s] = ring[((long long)s * 8 + c) * Q + (t - (long long)c * Q)];
+  const double2 *tw = (c % 4) < 2 ? n.tm : n.tj;
+  for (int f = 0; f < n.nf; ++f) {
+    double2 *a = &n.acc[n.base[pair] + ((long long)c * n.nf + f) * cells + cell];
+    double2 x = *a;
+    const double2 *w = tw + (long long)f * n.T_total + step0;
+#pragma unroll
+    for (int s = 0; s < NF_RING; ++s)
+      if (s < nslots) {
+        const double vs = (double)v[s];
+        const double2 ws = w[s];
+        x.x = x.x + vs * ws.x;
+        x.y = x.y + vs * ws.y;
+      }
+    *a = x;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1516,10 +1579,10 @@ struct InstAdj {
   T *hist;
 };
 
-template <typename T, bool LDS>
+template <typename T, bool LDS, bool SPECIAL>
 __global__ void __launch_bounds__(256) kb_update_e(const Inst<T> *__restrict__ in, const int2 *__restrict__ map) {
   const int2 w = map[blockIdx.x];
-  update_e_body<T, LDS>(in[w.x].g, in[w.x].tl, w.y);
+  update_e_body<T, LDS, SPECIAL>(in[w.x].g, in[w.x].tl, w.y);
 }
 template <typename T, bool LDS>
 __global__ void __launch_bounds__(256) kb_update_h(const Inst<T> *__restrict__ in, const int2 *__restrict__ map) {
