@@ -54,6 +54,7 @@ using SpatialIndex = size_t;
 enum class Orientation { XN, XP, YN, YP, ZN, ZP };
 enum class Axis { X, Y, Z };
 enum class EMComponent { EX, EY, EZ, HX, HY, HZ };
+enum class PlaneType { XY, YZ, ZX };  // util/type_define.h:30
 
 // ------------------------------------------------------------------ shapes
 class Shape {
@@ -342,6 +343,61 @@ struct Probe {
   SpatialIndex i, j, k;
 };
 
+// monitor/monitor.h:11-67: an output location for field data
+class Monitor {
+ public:
+  Monitor(std::unique_ptr<Cube> shape, std::string output_dir_path, std::string output_file_name)
+      : _shape{std::move(shape)}, _dir{std::move(output_dir_path)}, _file{std::move(output_file_name)} {}
+  virtual ~Monitor() = default;
+  virtual const std::string &getOutputPath() const { return _dir; }
+  virtual const std::string &getOutputFileName() const { return _file; }
+  virtual void setOutputDirPath(const std::string &p) { _dir = p; }
+  virtual void setOutputFileName(const std::string &f) { _file = f; }
+  const Cube *getShape() const { return _shape.get(); }
+
+ private:
+  std::unique_ptr<Cube> _shape;
+  std::string _dir, _file;
+};
+
+// monitor/field_monitor.h:14-38, field_monitor.cpp:15-121: one component
+// over the grid box of `shape` (one cell thick across the plane), dumped as
+// .npy (float64, C order) by outputData
+class TimeDomainFieldMonitor : public Monitor {
+ public:
+  TimeDomainFieldMonitor(std::unique_ptr<Cube> shape, PlaneType plane_type, EMComponent component,
+                         std::string output_dir_path, std::string output_file_name)
+      : Monitor(std::move(shape), std::move(output_dir_path), std::move(output_file_name)),
+        _plane_type{plane_type}, _component{component} {}
+  PlaneType getPlaneType() const { return _plane_type; }
+  EMComponent getComponent() const { return _component; }
+
+ private:
+  PlaneType _plane_type;
+  EMComponent _component;
+};
+
+// monitor/movie_monitor.h, movie_monitor.cpp:46-59: the wrapped monitor's
+// data after every step t with t % plot_step == 0 or t == total_time_steps - 1,
+// as "%04d.npy" (t) in the wrapped monitor's directory
+class MovieMonitor : public Monitor {
+ public:
+  MovieMonitor(std::unique_ptr<Monitor> monitor, size_t total_time_steps, size_t plot_step)
+      : Monitor(nullptr, "", ""), _monitor{std::move(monitor)}, _total_time_steps{total_time_steps},
+        _plot_step{plot_step} {}
+  const std::string &getOutputPath() const override { return _monitor->getOutputPath(); }
+  const std::string &getOutputFileName() const override { return _monitor->getOutputFileName(); }
+  void setOutputDirPath(const std::string &p) override { _monitor->setOutputDirPath(p); }
+  void setOutputFileName(const std::string &f) override { _monitor->setOutputFileName(f); }
+  Monitor &getMonitor() const { return *_monitor; }
+  size_t getTotalTimeSteps() const { return _total_time_steps; }
+  size_t getPlotStep() const { return _plot_step; }
+
+ private:
+  std::unique_ptr<Monitor> _monitor;
+  size_t _total_time_steps, _plot_step;
+};
+
 // ------------------------------------------------------------- simulation
 class Simulation {
  public:
@@ -364,6 +420,7 @@ class Simulation {
   void addTFSFSource(std::shared_ptr<TFSF> tfsf);
   void addNFFFT(std::shared_ptr<NFFFT> nffft);
   void addProbe(Probe probe);
+  void addMonitor(std::shared_ptr<Monitor> monitor);
 
   // ---- differentiation (no counterpart in the reference)
   // Per-cell relative permittivity of the cells [box[0],box[1]) x
@@ -413,7 +470,10 @@ class Simulation {
   // NF-FF accumulators and far field (NffftFd::calculateFarfield)
   std::vector<std::complex<double>> getNffftAccumulators() const;
   FarField calculateFarField() const;
-  void outputData() const;  // NffftFd::outputData .npy files
+  // Simulation::outputData (simulation.cpp:144-156): NffftFd::outputData when
+  // the NF-FF has an output directory, then every field monitor's final
+  // slice (movie monitors write their frames while stepping)
+  void outputData() const;
   std::vector<double> getProbeData() const;  // [n_probes][total_steps]
 
   xf_engine *engine() const { return _engine; }
@@ -434,6 +494,16 @@ class Simulation {
   std::shared_ptr<TFSF> _tfsf;
   std::shared_ptr<NFFFT> _nffft;
   std::vector<Probe> _probes;
+  std::vector<std::shared_ptr<Monitor>> _monitors;
+  struct MonitorPlan {
+    const Monitor *monitor;   // the TimeDomainFieldMonitor
+    int comp;
+    int64_t box[6];           // i0, ni, j0, nj, k0, nk
+    size_t plot_step, total;  // movie schedule (plot_step 0: final slice only)
+  };
+  std::vector<MonitorPlan> _monitor_plans;
+  void planMonitors();
+  void writeMonitor(const MonitorPlan &m, const std::string &file) const;
   bool _has_design = false;
   std::array<size_t, 6> _design_box{};
   std::vector<float> _design_eps;

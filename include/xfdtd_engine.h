@@ -168,6 +168,10 @@ xf_status xf_engine_sync(xf_engine *e);
 int64_t xf_engine_current_step(const xf_engine *e);
 
 xf_status xf_engine_get_field(xf_engine *e, int comp, double *out);
+/* comp over the box [i0, i0+ni) x [j0, j0+nj) x [k0, k0+nk), box = {i0, ni,
+   j0, nj, k0, nk}, in C order: the slice a TimeDomainFieldMonitor dumps
+   (monitor/field_monitor.cpp:99-121); only the box crosses to the host. */
+xf_status xf_engine_get_box(xf_engine *e, int comp, const int64_t *box, double *out);
 xf_status xf_engine_set_field(xf_engine *e, int comp, const double *in);
 
 /* average device time (ms) of the E and H update kernels over the steps run

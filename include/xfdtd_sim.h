@@ -14,10 +14,14 @@
  *   xf_sim_set_nffft_fd          addNFFFT(NffftFd(...))                     simulation.cpp:132-134
  *   xf_sim_add_current_source    addLumpedElement(CurrentSource(...))       simulation.cpp:123-126
  *   xf_sim_add_probe             addMonitor (point sample)                  simulation.cpp:136-138
+ *   xf_sim_add_field_monitor     addMonitor(TimeDomainFieldMonitor(...))    field_monitor.cpp:15-121
+ *   xf_sim_add_movie_monitor     addMonitor(MovieMonitor(TimeDomainFieldMonitor(...), T, plot_step))
+ *                                                                           movie_monitor.cpp:6-59
  *   xf_sim_init / xf_sim_step    run(T) split into init + time loop         simulation_update.cpp:5-40
  *   xf_sim_get_field             getEx()..getHz()                           simulation.h:92-97
  *   xf_sim_farfield              NffftFd::calculateFarfield                 nffft_fd.cpp:379-421
- *   xf_sim_output_nffft          NffftFd::outputData                        nffft_fd.cpp:579-634
+ *   xf_sim_output_nffft          Simulation::outputData: NffftFd::outputData and the field
+ *                                monitors' final slices                     simulation.cpp:144-156
  *
  * Every call returns 0 on success and -1 on failure; xf_sim_last_error()
  * then holds the message of the C++ exception the reference would throw.
@@ -59,6 +63,15 @@ int xf_sim_add_current_source(xf_sim *s, const double origin[3], const double si
                               int orientation, double resistance, double amplitude, double tau,
                               double t0);
 int xf_sim_add_probe(xf_sim *s, int comp, size_t i, size_t j, size_t k);
+/* TimeDomainFieldMonitor(Cube(origin, size), plane_type (0 XY, 1 YZ, 2 ZX),
+   component, dir, file): the component over the cube's grid box, written by
+   xf_sim_output_nffft (the end of run) as dir/file (.npy, float64) */
+int xf_sim_add_field_monitor(xf_sim *s, const double origin[3], const double size[3], int plane_type, int comp,
+                             const char *dir, const char *file);
+/* MovieMonitor of such a monitor: dir/%04d.npy after every step t with
+   t % plot_step == 0 or t == total_steps - 1, written while stepping */
+int xf_sim_add_movie_monitor(xf_sim *s, const double origin[3], const double size[3], int plane_type, int comp,
+                             const char *dir, size_t total_steps, size_t plot_step);
 
 /* CPU set-up only (Simulation::init without the device): grid, materials,
    coefficient tables, CPML, TFSF and NF-FF tables.  Needs no GPU. */

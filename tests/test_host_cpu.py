@@ -153,3 +153,32 @@ def test_plan_refuses_unsupported_drude_placements():
                                       S.GaussianWaveform(1e-10, 4e-10)))
     with pytest.raises(RuntimeError, match="Drude"):
         Simulation(sc, 10, plan_only=True)
+
+
+def test_plan_refuses_bad_monitors(tmp_path):
+    from xfdtd_amd.sim import Simulation
+    dl = 5e-3
+    base = lambda: S.cube_domain_scene(24, dl, 4, cfl=0.9)
+    half = 12 * dl
+    # an XY monitor two cells thick (field_monitor.cpp:34-37)
+    sc = base()
+    sc.monitors.append(S.TimeDomainFieldMonitor(S.Cube((-half, -half, 0.0), (2 * half, 2 * half, 2 * dl)),
+                                                S.XY, S.EZ, str(tmp_path), "a.npy"))
+    with pytest.raises(RuntimeError, match="invalid grid box"):
+        Simulation(sc, 5, plan_only=True)
+    # a YZ monitor of Ex at i = nx, outside Ex's (nx, ny+1, nz+1) array
+    sc = base()
+    sc.monitors.append(S.TimeDomainFieldMonitor(S.Cube((half, -half, -half), (0.0, dl, dl)),
+                                                S.YZ, S.EX, str(tmp_path), "b.npy"))
+    with pytest.raises(RuntimeError, match="outside the component"):
+        Simulation(sc, 5, plan_only=True)
+    # a movie monitor with plot_step 0
+    sc = base()
+    mon = S.TimeDomainFieldMonitor(S.Cube((-half, -half, 0.0), (dl, dl, 0.0)), S.XY, S.EZ, str(tmp_path))
+    sc.monitors.append(S.MovieMonitor(mon, 5, 0))
+    with pytest.raises(RuntimeError, match="plot_step"):
+        Simulation(sc, 5, plan_only=True)
+    # a good one plans
+    sc = base()
+    sc.monitors.append(S.MovieMonitor(mon, 5, 2))
+    Simulation(sc, 5, plan_only=True)

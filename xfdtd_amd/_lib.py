@@ -25,6 +25,7 @@ ENGINE_SYMBOLS = [
     "xf_engine_set_design", "xf_engine_set_dft_plane", "xf_engine_get_dft", "xf_engine_checkpoint",
     "xf_engine_restore", "xf_engine_grad_eps", "xf_engine_adjoint_run", "xf_engine_set_drude",
     "xf_engine_drude_box", "xf_engine_get_drude_j", "xf_engine_set_horizon", "xf_engine_set_drude_j",
+    "xf_engine_get_box",
 ]
 SIM_SYMBOLS = [
     "xf_sim_last_error", "xf_sim_create", "xf_sim_destroy", "xf_sim_add_cube", "xf_sim_add_sphere",
@@ -35,6 +36,7 @@ SIM_SYMBOLS = [
     "xf_sim_engine", "xf_sim_set_design", "xf_sim_set_dft_plane", "xf_sim_get_dft", "xf_sim_grad_eps",
     "xf_sim_adjoint_run", "xf_sim_checkpoint", "xf_sim_restore", "xf_sim_add_drude_cube",
     "xf_sim_add_drude_sphere", "xf_sim_drude_box", "xf_sim_get_drude_j", "xf_sim_set_drude_j",
+    "xf_sim_add_field_monitor", "xf_sim_add_movie_monitor",
 ]
 
 
@@ -107,5 +109,7 @@ def lib():
     L.xf_sim_drude_box.argtypes = [vp, C.POINTER(C.c_int64)]
     L.xf_sim_get_drude_j.argtypes = [vp, i, dp]
     L.xf_sim_set_drude_j.argtypes = [vp, i, dp]
+    L.xf_sim_add_field_monitor.argtypes = [vp, dp, dp, i, i, C.c_char_p, C.c_char_p]
+    L.xf_sim_add_movie_monitor.argtypes = [vp, dp, dp, i, i, C.c_char_p, sz, sz]
     _LIB = L
     return L

@@ -763,6 +763,34 @@ extern "C" xf_status xf_engine_get_field(xf_engine *e, int comp, double *out) {
   return XF_OK;
 }
 
+template <typename T>
+static xf_status get_box(xf_engine *e, int comp, const int64_t *box, double *out) {
+  const long long n = box[1] * box[3] * box[5];
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, (size_t)n * sizeof(double)));
+  hipLaunchKernelGGL(k_box_gather<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream,
+                     (const T *)e->f[comp], e->si, e->sj, (int)box[0], (int)box[2], (int)box[4], (int)box[3],
+                     (int)box[5], n, d);
+  hipError_t r = hipGetLastError();
+  if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+  if (r == hipSuccess) r = hipMemcpy(out, d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
+  hipFree(d);
+  if (r != hipSuccess) FAILS(XF_EDEVICE, "box read: %s", hipGetErrorString(r));
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_get_box(xf_engine *e, int comp, const int64_t *box, double *out) {
+  if (!e || !box || !out || comp < 0 || comp > 5) FAILS(XF_EINVAL, "bad box request");
+  int sh[3];
+  comp_shape(e, comp, sh);
+  for (int a = 0; a < 3; ++a)
+    if (box[2 * a] < 0 || box[2 * a + 1] < 1 || box[2 * a] + box[2 * a + 1] > sh[a])
+      FAILS(XF_EINVAL, "box [%lld, +%lld) on axis %d outside component %d of extent %d", (long long)box[2 * a],
+            (long long)box[2 * a + 1], a, comp, sh[a]);
+  HIPCHK(engine_wait(e));
+  return e->prec == 64 ? get_box<double>(e, comp, box, out) : get_box<float>(e, comp, box, out);
+}
+
 extern "C" xf_status xf_engine_set_field(xf_engine *e, int comp, const double *in) {
   if (!e || !in || comp < 0 || comp > 5) FAILS(XF_EINVAL, "bad field request");
   int sh[3];

@@ -1007,6 +1007,18 @@ __global__ void k_probe(Grid<T> g, const int *comp, const long long *cell, int n
   buf[(long long)q * T_total + step] = (double)g.f[comp[q]][cell[q]];
 }
 
+// a box of one component gathered in C order as FP64 (field monitors,
+// field_monitor.cpp:99-121)
+template <typename T>
+__global__ void k_box_gather(const T *f, long long si, long long sj, int i0, int j0, int k0, int nj, int nk,
+                             long long n, double *out) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const long long i = i0 + q / ((long long)nj * nk), r = q % ((long long)nj * nk);
+  const long long j = j0 + r / nk, k = k0 + r % nk;
+  out[q] = (double)f[i * si + j * sj + k];
+}
+
 // ---------------------------------------------------------------------------
 // NffftFd::update (nffft_fd.cpp:108-203) with the face-centre stencils of
 // linear_interpolation_emf.cpp:286-388.  One thread per face cell of a face

@@ -227,6 +227,10 @@ void Simulation::addLumpedElement(std::shared_ptr<LumpedElement> l) {
 void Simulation::addTFSFSource(std::shared_ptr<TFSF> t) { _tfsf = std::move(t); }
 void Simulation::addNFFFT(std::shared_ptr<NFFFT> n) { _nffft = std::move(n); }
 void Simulation::addProbe(Probe p) { _probes.push_back(p); }
+void Simulation::addMonitor(std::shared_ptr<Monitor> m) {
+  if (_plan) throw std::runtime_error("Simulation already planned");
+  _monitors.push_back(std::move(m));
+}
 
 // grid/grid_space.cpp:576-618
 void Simulation::resizeSpaceSize(const double nmin[3], const double nmax[3]) {
@@ -325,6 +329,7 @@ void Simulation::plan(size_t total_time_steps) {
   _total_time_steps = total_time_steps;
   // simulation_init.cpp:10-22
   initGridSpace();
+  planMonitors();
   const int nx = (int)_n[0], ny = (int)_n[1], nz = (int)_n[2];
   const int N[3] = {nx, ny, nz};
   // fdtd_basic_coff.cpp:11-29, 342-346: cfl arrives as float
@@ -992,9 +997,31 @@ std::vector<double> Simulation::planCoefficient(int which) const {
   return out;
 }
 
+// Steps in chunks that end at the movie monitors' sample steps; after step t
+// (monitor update slot, simulation_update.cpp:35) each due monitor writes
+// its frame (movie_monitor.cpp:46-59).
 void Simulation::step(size_t n) {
   if (!_engine) throw std::runtime_error("Simulation is not initialised");
-  check(xf_engine_step(_engine, (int64_t)n));
+  size_t cur = currentTimeStep();
+  const size_t end = cur + n;
+  while (cur < end) {
+    size_t next = end;  // one past the next sample step
+    for (const MonitorPlan &m : _monitor_plans) {
+      if (m.plot_step == 0) continue;
+      size_t t = ((cur + m.plot_step - 1) / m.plot_step) * m.plot_step;  // first multiple >= cur
+      if (m.total > 0 && m.total - 1 >= cur && m.total - 1 < t) t = m.total - 1;
+      if (t < end) next = std::min(next, t + 1);
+    }
+    check(xf_engine_step(_engine, (int64_t)(next - cur)));
+    cur = next;
+    const size_t t = cur - 1;
+    for (const MonitorPlan &m : _monitor_plans) {
+      if (m.plot_step == 0 || !(t % m.plot_step == 0 || t == m.total - 1)) continue;
+      char name[32];
+      std::snprintf(name, sizeof name, "%04zu.npy", t);
+      writeMonitor(m, name);
+    }
+  }
 }
 
 void Simulation::sync() {
@@ -1009,7 +1036,7 @@ void Simulation::run(size_t total_time_steps) {
   init(total_time_steps);
   step(total_time_steps);
   sync();
-  if (_nffft && !_nffft->getOutputDirPath().empty()) outputData();
+  outputData();
 }
 
 std::array<size_t, 3> Simulation::fieldShape(EMComponent c) const {
@@ -1140,10 +1167,71 @@ void makeDirs(const std::string &path) {
 
 }  // namespace
 
-// nffft_fd.cpp:579-634
+// grid boxes of the monitors (GridSpace::getGridBox, grid_space.cpp:306-329)
+// and the plane check of TimeDomainFieldMonitor::init (field_monitor.cpp:30-87)
+void Simulation::planMonitors() {
+  _monitor_plans.clear();
+  for (const auto &mp : _monitors) {
+    const Monitor *mon = mp.get();
+    size_t plot = 0, total = 0;
+    if (auto *mv = dynamic_cast<const MovieMonitor *>(mon)) {
+      if (mv->getPlotStep() == 0) throw std::invalid_argument("MovieMonitor plot_step must be positive");
+      plot = mv->getPlotStep();
+      total = mv->getTotalTimeSteps();
+      mon = &mv->getMonitor();
+    }
+    auto *fm = dynamic_cast<const TimeDomainFieldMonitor *>(mon);
+    if (!fm) throw std::runtime_error("only TimeDomainFieldMonitor and MovieMonitor are supported");
+    const Cube *c = fm->getShape();
+    if (!c) throw std::runtime_error("Shape Type Error");
+    const double lo[3] = {c->getXmin(), c->getYmin(), c->getZmin()};
+    const double hi[3] = {c->getXmax(), c->getYmax(), c->getZmax()};
+    MonitorPlan m;
+    m.monitor = fm;
+    m.comp = compIndex(fm->getComponent());
+    m.plot_step = plot;
+    m.total = total;
+    size_t num[3];
+    for (int a = 0; a < 3; ++a) {
+      const size_t s = handleTransform(a, lo[a]), e = handleTransform(a, hi[a]);
+      if (e < s) throw std::range_error("end < start");
+      num[a] = e - s;
+      m.box[2 * a] = (int64_t)s;
+      m.box[2 * a + 1] = (int64_t)(num[a] == 0 ? 1 : num[a]);
+    }
+    const PlaneType pt = fm->getPlaneType();
+    if ((pt == PlaneType::XY && num[2] > 1) || (pt == PlaneType::YZ && num[0] > 1) ||
+        (pt == PlaneType::ZX && num[1] > 1))
+      throw std::runtime_error("Error: invalid grid box");
+    const auto sh = fieldShape(fm->getComponent());
+    for (int a = 0; a < 3; ++a)
+      if ((size_t)(m.box[2 * a] + m.box[2 * a + 1]) > sh[a])
+        throw std::out_of_range("monitor box outside the component array");
+    _monitor_plans.push_back(m);
+  }
+}
+
+void Simulation::writeMonitor(const MonitorPlan &m, const std::string &file) const {
+  const std::string dir = m.monitor->getOutputPath();
+  try {
+    makeDirs(dir);
+  } catch (const std::exception &) {
+    std::fprintf(stderr, "Error: cannot create directory %s\n", dir.c_str());
+    return;
+  }
+  std::vector<double> v((size_t)(m.box[1] * m.box[3] * m.box[5]));
+  check(xf_engine_get_box(_engine, m.comp, m.box, v.data()));
+  writeNpy(dir + "/" + file, v.data(), "<f8", 8, {(size_t)m.box[1], (size_t)m.box[3], (size_t)m.box[5]});
+}
+
+// nffft_fd.cpp:579-634, then the field monitors' final slices
+// (field_monitor.cpp:91-121)
 void Simulation::outputData() const {
+  if (_engine)
+    for (const MonitorPlan &m : _monitor_plans)
+      if (m.plot_step == 0) writeMonitor(m, m.monitor->getOutputFileName());
   auto *nf = dynamic_cast<NffftFd *>(_nffft.get());
-  if (!nf) return;
+  if (!nf || nf->getOutputDirPath().empty()) return;
   const FarField r = calculateFarField();
   const std::string dir = nf->getOutputDirPath();
   try {

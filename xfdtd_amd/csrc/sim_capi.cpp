@@ -156,6 +156,29 @@ extern "C" int xf_sim_add_probe(xf_sim *s, int comp, size_t i, size_t j, size_t 
   });
 }
 
+static std::unique_ptr<TimeDomainFieldMonitor> field_monitor(const double origin[3], const double size[3],
+                                                              int plane_type, int comp, const char *dir,
+                                                              const char *file) {
+  if (comp < 0 || comp > 5) throw std::invalid_argument("bad component");
+  if (plane_type < 0 || plane_type > 2) throw std::invalid_argument("bad plane type");
+  return std::make_unique<TimeDomainFieldMonitor>(
+      std::make_unique<Cube>(PointVector{origin[0], origin[1], origin[2]}, PointVector{size[0], size[1], size[2]}),
+      static_cast<PlaneType>(plane_type), static_cast<EMComponent>(comp), dir ? dir : "", file ? file : "");
+}
+
+extern "C" int xf_sim_add_field_monitor(xf_sim *s, const double origin[3], const double size[3], int plane_type,
+                                        int comp, const char *dir, const char *file) {
+  return guarded([&] { s->sim.addMonitor(field_monitor(origin, size, plane_type, comp, dir, file)); });
+}
+
+extern "C" int xf_sim_add_movie_monitor(xf_sim *s, const double origin[3], const double size[3], int plane_type,
+                                        int comp, const char *dir, size_t total_steps, size_t plot_step) {
+  return guarded([&] {
+    s->sim.addMonitor(std::make_shared<MovieMonitor>(field_monitor(origin, size, plane_type, comp, dir, ""),
+                                                     total_steps, plot_step));
+  });
+}
+
 extern "C" int xf_sim_plan(xf_sim *s, size_t total_steps) {
   return guarded([&] { s->sim.plan(total_steps); });
 }

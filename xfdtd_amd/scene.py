@@ -158,6 +158,33 @@ class Probe:
     k: int
 
 
+XY, YZ, ZX = range(3)  # PlaneType (util/type_define.h:30)
+
+
+@dataclass
+class TimeDomainFieldMonitor:
+    """TimeDomainFieldMonitor(Cube, PlaneType, EMComponent, output_dir,
+    output_file_name) (monitor/field_monitor.h:16-19): `comp` over the grid
+    box of `cube`, one cell thick across `plane_type`, written as .npy at the
+    end of the run."""
+    cube: Cube
+    plane_type: int
+    comp: int
+    output_dir: str
+    output_file_name: str = ""
+
+
+@dataclass
+class MovieMonitor:
+    """MovieMonitor(monitor, total_time_steps, plot_step)
+    (monitor/movie_monitor.h): the wrapped monitor's slice as
+    output_dir/%04d.npy after every step t with t % plot_step == 0 or
+    t == total_time_steps - 1."""
+    monitor: TimeDomainFieldMonitor
+    total_time_steps: int
+    plot_step: int
+
+
 @dataclass
 class Design:
     """Per-cell relative permittivity of a box of cells [i0,i1)x[j0,j1)x[k0,k1)
@@ -195,6 +222,7 @@ class Scene:
     nffft: Optional[NffftFd] = None
     sources: List[CurrentSource] = field(default_factory=list)
     probes: List[Probe] = field(default_factory=list)
+    monitors: List[object] = field(default_factory=list)  # TimeDomainFieldMonitor / MovieMonitor
     design: Optional[Design] = None
     dft_plane: Optional[DftPlane] = None
 

@@ -74,6 +74,17 @@ class Simulation:
                                                       float(s.resistance), w.amplitude, w.tau, w.t_0))
             for p in scene.probes:
                 self._chk(L.xf_sim_add_probe(self.h, int(p.comp), p.i, p.j, p.k))
+            for m in getattr(scene, "monitors", []):
+                inner = m.monitor if hasattr(m, "plot_step") else m
+                org, siz = _f64(inner.cube.origin), _f64(inner.cube.size)
+                d = inner.output_dir.encode()
+                if hasattr(m, "plot_step"):
+                    self._chk(L.xf_sim_add_movie_monitor(self.h, _dp(org), _dp(siz), int(inner.plane_type),
+                                                         int(inner.comp), d, int(m.total_time_steps),
+                                                         int(m.plot_step)))
+                else:
+                    self._chk(L.xf_sim_add_field_monitor(self.h, _dp(org), _dp(siz), int(inner.plane_type),
+                                                         int(inner.comp), d, inner.output_file_name.encode()))
             dz = getattr(scene, "design", None)
             if dz is not None:
                 box = (C.c_size_t * 6)(*[int(v) for v in dz.box])
@@ -167,6 +178,11 @@ class Simulation:
         return res
 
     def output_nffft(self):
+        self._chk(lib().xf_sim_output_nffft(self.h))
+
+    def output_data(self):
+        """Simulation::outputData: NF-FF files (when the NF-FF has an output
+        directory) and the field monitors' final slices."""
         self._chk(lib().xf_sim_output_nffft(self.h))
 
     def drude_box(self):
