@@ -12,6 +12,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -1267,13 +1268,22 @@ extern "C" xf_status xf_engine_restore(xf_engine *e, int slot) {
 }
 
 // device buffers of one adjoint run
+// device buffers of one gradient run, zeroed on the stream that uses them;
+// alloc waits for the fill, so neither that stream's earlier work nor a
+// synchronous copy on the null stream can see the fill land late (a fill on
+// the null stream is not ordered with the engines' non-blocking streams)
 struct AdjBufs {
+  hipStream_t st = nullptr;
   std::vector<void *> mem;
+  explicit AdjBufs(hipStream_t s = nullptr) : st(s) {}
+  AdjBufs(const AdjBufs &) = delete;
+  AdjBufs &operator=(const AdjBufs &) = delete;
   void *alloc(size_t bytes) {
     void *p = nullptr;
     if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) return nullptr;
-    if (hipMemset(p, 0, bytes ? bytes : 16) != hipSuccess) { hipFree(p); return nullptr; }
     mem.push_back(p);
+    if (hipMemsetAsync(p, 0, bytes ? bytes : 16, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+      return nullptr;
     return p;
   }
   ~AdjBufs() {
@@ -1312,7 +1322,7 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
     std::vector<std::vector<void *>> &v;
     ~Free() { for (auto &x : v) state_free(x); }
   } free_ck{ck};
-  AdjBufs B;
+  AdjBufs B(e->stream);
   T *hist = (T *)B.alloc((size_t)(K + 1) * e->nd * 3 * sizeof(T));
   double *g = (double *)B.alloc((size_t)e->nd * sizeof(double));
   if (!hist || !g) FAILS(XF_ENOMEM, "adjoint buffers");
@@ -1408,7 +1418,7 @@ extern "C" xf_status xf_engine_grad_eps(xf_engine *e, int checkpoint_every, doub
 
 template <typename T>
 static xf_status adjoint_run(xf_engine *e, long long nsteps, double *fields) {
-  AdjBufs B;
+  AdjBufs B(e->stream);
   Adj<T> a;
   xf_status s = adj_setup<T>(e, B, a);
   if (s != XF_OK) return s;
@@ -1487,7 +1497,8 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
         for (auto &x : m) state_free(x);
     }
   } free_ck{ck};
-  std::vector<AdjBufs> B(n);
+  std::deque<AdjBufs> B;
+  for (int q = 0; q < n; ++q) B.emplace_back(st);
   std::vector<InstAdj<T>> ia(n);
   std::vector<double *> gdev(n);
   std::vector<double2 *> Ffin(n);
@@ -1515,7 +1526,7 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
     for (long long w = 0; w < (e->ncell + 255) / 256; ++w) mcell.push_back(make_int2(q, (int)w));
   }
   if (mcell.size() > 0x7fffffffu) FAILS(XF_EINVAL, "batch too large");
-  AdjBufs D;
+  AdjBufs D(st);
   InstAdj<T> *d_ia = (InstAdj<T> *)D.alloc(n * sizeof(InstAdj<T>));
   int2 *d_snap = (int2 *)D.alloc(msnap.size() * sizeof(int2));
   int2 *d_cell = (int2 *)D.alloc(mcell.size() * sizeof(int2));
