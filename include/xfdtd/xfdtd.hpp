@@ -307,6 +307,29 @@ class NffftFd : public NFFFT {
   std::vector<double> _frequencies, _wave_number, _theta, _phi;
 };
 
+// NffftBroadBand (nffft/nffft_broadband.cpp:21-28): the NF-FF surface of
+// NffftFd observed in one direction (theta, phi) over every frequency bin of
+// the run, the monostatic radar cross-section path.
+class NffftBroadBand : public NFFFT {
+ public:
+  NffftBroadBand(SpatialIndex distance_x, SpatialIndex distance_y, SpatialIndex distance_z, double theta,
+                 double phi, std::string output_dir_path);
+  double theta() const { return _theta; }
+  double phi() const { return _phi; }
+
+ private:
+  double _theta, _phi;
+};
+
+// NffftBroadBand::calculateFarField (nffft_broadband.cpp:316-345): every
+// array has one entry per bin, 2T-1 bins from -fs/2 to fs/2 (fs = 1/dt)
+struct BroadBandFarField {
+  std::vector<double> frequencies, wave_number;
+  std::vector<std::complex<double>> n_x, n_y, n_z, l_x, l_y, l_z;
+  std::vector<std::complex<double>> n_theta, n_phi, l_theta, l_phi, e_theta, e_phi, h_theta, h_phi;
+  std::vector<double> power_theta, power_phi;
+};
+
 // ---------------------------------------------------------- lumped element
 class LumpedElement {
  public:
@@ -454,6 +477,7 @@ class Simulation {
   void step(size_t n);
   void sync();
   size_t currentTimeStep() const;
+  size_t totalTimeSteps() const { return _total_time_steps; }
 
   // FP64 (default; parity with the reference) or FP32 fields.
   void setPrecision(int bits) { _precision = bits; }
@@ -470,8 +494,15 @@ class Simulation {
   // NF-FF accumulators and far field (NffftFd::calculateFarfield)
   std::vector<std::complex<double>> getNffftAccumulators() const;
   FarField calculateFarField() const;
-  // Simulation::outputData (simulation.cpp:144-156): NffftFd::outputData when
-  // the NF-FF has an output directory, then every field monitor's final
+  // NffftBroadBand: the stored face-current series ([T][8][Q], see
+  // xf_engine_get_broadband_samples) and the far field of every bin; the
+  // surface integrals use `wave_number` (2T-1 entries) when it is given,
+  // else the bins' own
+  std::vector<double> getBroadBandSamples() const;
+  BroadBandFarField calculateBroadBandFarField(const std::vector<double> *wave_number = nullptr) const;
+  // Simulation::outputData (simulation.cpp:144-156): NffftFd::outputData or
+  // NffftBroadBand::outputData (nffft_broadband.cpp:156-200) when the NF-FF
+  // has an output directory, then every field monitor's final
   // slice (movie monitors write their frames while stepping)
   void outputData() const;
   std::vector<double> getProbeData() const;  // [n_probes][total_steps]
@@ -504,6 +535,7 @@ class Simulation {
   std::vector<MonitorPlan> _monitor_plans;
   void planMonitors();
   void writeMonitor(const MonitorPlan &m, const std::string &file) const;
+  void outputBroadBand(const NffftBroadBand &bb) const;
   bool _has_design = false;
   std::array<size_t, 6> _design_box{};
   std::vector<float> _design_eps;

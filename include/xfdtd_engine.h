@@ -20,6 +20,8 @@
  *   xf_engine_get_field     EMF::getEx..getHz                include/simulation/simulation.h:92-97
  *   xf_engine_get_nffft     NffftFd accumulators             nffft/nffft_fd.cpp:126-202
  *   xf_engine_farfield_sums NffftFd::calculateFarfieldX/Y/Z  nffft/nffft_fd.cpp:423-556
+ *   xf_engine_set_nffft_broadband  NffftBroadBand::init / update  nffft/nffft_broadband.cpp:30-155
+ *   xf_engine_broadband_sums       NffftBroadBand::calculateLNXYZ nffft/nffft_broadband.cpp:348-466
  *
  * Conventions: status codes, never exceptions; the engine owns device memory,
  * the caller owns every host buffer; one HIP stream per engine; an engine is
@@ -141,6 +143,33 @@ xf_status xf_engine_farfield_sums(xf_engine *e, double dl,
                                   const double *wave_number,
                                   const double *theta, int ntheta,
                                   const double *phi, int nphi, double *out);
+
+/* Broadband NF-FF surface (NffftBroadBand, nffft/nffft_broadband.cpp:30-91,
+   the monostatic RCS path): after every step t < total_steps the face
+   currents of every surface cell of box {li,ri,lj,rj,lk,rk} (nffft/nffft.cpp:
+   29-33) are stored, with the signs and stencils of NffftFd
+   (nffft_broadband.cpp:94-155).  A batch refuses engines that have one. */
+xf_status xf_engine_set_nffft_broadband(xf_engine *e, const int64_t box[6], int64_t total_steps);
+/* the stored series, FP64, [total_steps][8][Q]: current c in the order of the
+   get_nffft accumulators, Q surface cells (x faces (j,k), then y faces
+   (k,i), then z faces (i,j), the last index fastest) */
+int64_t xf_engine_broadband_samples_size(const xf_engine *e);
+xf_status xf_engine_get_broadband_samples(xf_engine *e, double *out);
+/* Surface integrals of the broadband far field in direction (theta, phi)
+   (nffft_broadband.cpp:348-466): for Ns = 2 total_steps - 1 bins,
+   bin q = m + total_steps - 1 of n_a += ds e^{i delay k_q} fftshift(fft(j_a))
+   and l_a likewise from the magnetic currents, where fft is the forward DFT
+   over Ns points of the series (zero past the last step) and delay is the
+   cell's projection from the box centre (grid/grid_box.cpp:101-105).
+   wave_number[Ns] = k_q (2 pi f_q / c0 for the reference's bins); dl = cell
+   size; out = n_x, n_y, n_z, l_x, l_y, l_z, each [Ns] interleaved complex.
+   The DFTs run as batched FP64 real-to-complex hipFFT transforms. */
+xf_status xf_engine_broadband_sums(xf_engine *e, double dl, const double *wave_number, double theta,
+                                   double phi, double *out);
+/* device scratch the far field may take per batch of transforms (rows and
+   spectra; default 1 GiB): larger batches mean fewer launches.  No
+   reference counterpart (the reference transforms one series at a time). */
+xf_status xf_engine_set_broadband_scratch(xf_engine *e, int64_t bytes);
 
 /* E += coff[cell] * wave[t] over the box [i0,i1)x[j0,j1)x[k0,k1) of component
    comp (XF_EX/EY/EZ); coff has (i1-i0)(j1-j0)(k1-k0) entries, wave total_steps. */

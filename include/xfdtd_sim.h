@@ -20,6 +20,8 @@
  *   xf_sim_init / xf_sim_step    run(T) split into init + time loop         simulation_update.cpp:5-40
  *   xf_sim_get_field             getEx()..getHz()                           simulation.h:92-97
  *   xf_sim_farfield              NffftFd::calculateFarfield                 nffft_fd.cpp:379-421
+ *   xf_sim_set_nffft_broadband   addNFFFT(NffftBroadBand(...))              nffft_broadband.cpp:21-28
+ *   xf_sim_broadband_farfield    NffftBroadBand::calculateFarField          nffft_broadband.cpp:316-466
  *   xf_sim_output_nffft          Simulation::outputData: NffftFd::outputData and the field
  *                                monitors' final slices                     simulation.cpp:144-156
  *
@@ -59,6 +61,10 @@ int xf_sim_set_tfsf3d(xf_sim *s, const size_t distance[3], double e0, double the
 int xf_sim_set_nffft_fd(xf_sim *s, const size_t distance[3], const double *freqs, int nf,
                         const double *theta, int ntheta, const double *phi, int nphi,
                         const char *output_dir);
+/* NffftBroadBand(dx, dy, dz, theta, phi, output_dir): monostatic RCS path;
+   replaces any NffftFd (the reference's Simulation holds one NFFFT) */
+int xf_sim_set_nffft_broadband(xf_sim *s, const size_t distance[3], double theta, double phi,
+                               const char *output_dir);
 int xf_sim_add_current_source(xf_sim *s, const double origin[3], const double size[3],
                               int orientation, double resistance, double amplitude, double tau,
                               double t0);
@@ -99,6 +105,17 @@ int xf_sim_get_nffft(xf_sim *s, double *out);
    h_theta, h_phi) then 2 real arrays (power_theta, power_phi), each
    [nf][ntheta][nphi] */
 int xf_sim_farfield(xf_sim *s, double *out);
+/* broadband NF-FF: face-current series as xf_engine_get_broadband_samples,
+   and the far field of the Ns = 2T-1 bins.  wave_number[Ns] overrides the
+   bins' wave numbers in the surface integrals (NULL: the bins' own).  out:
+   frequencies[Ns], wave_number[Ns], then 14 complex arrays [Ns] (n_x, n_y,
+   n_z, l_x, l_y, l_z, n_theta, n_phi, l_theta, l_phi, e_theta, e_phi,
+   h_theta, h_phi), then power_theta[Ns], power_phi[Ns]: 32 Ns doubles */
+int64_t xf_sim_broadband_samples_size(const xf_sim *s);
+int xf_sim_get_broadband_samples(xf_sim *s, double *out);
+int xf_sim_broadband_farfield(xf_sim *s, const double *wave_number, double *out);
+/* Simulation::outputData: NF-FF files (NffftFd or NffftBroadBand) and the
+   field monitors' final slices */
 int xf_sim_output_nffft(xf_sim *s);
 int xf_sim_get_probes(xf_sim *s, double *out);
 /* the engine behind the simulation (valid after xf_sim_init) */

@@ -33,6 +33,12 @@ def lib():
         L.xo_sim_add_pml.argtypes = [vp, i, i, i, d, d, d, d]
         L.xo_sim_set_tfsf.argtypes = [vp, szp, d, d, d, d, d, d, d]
         L.xo_sim_set_nffft.argtypes = [vp, szp, dp, i, dp, i, dp, i]
+        L.xo_sim_set_nffft_broadband.argtypes = [vp, szp, d, d]
+        L.xo_sim_bb_frequencies.argtypes = [vp, dp, dp]
+        L.xo_sim_bb_sums.argtypes = [vp, dp, dp]
+        L.xo_sim_bb_samples_size.restype = sz
+        L.xo_sim_bb_samples_size.argtypes = [vp]
+        L.xo_sim_get_bb_samples.argtypes = [vp, dp]
         L.xo_sim_add_current_source.argtypes = [vp, dp, dp, i, d, d, d, d]
         L.xo_sim_add_probe.argtypes = [vp, i, sz, sz, sz]
         L.xo_sim_init.argtypes = [vp, sz]
@@ -87,9 +93,12 @@ class OracleSim:
     """One FP64 oracle simulation built from a scene description (duck-typed:
     see xfdtd_amd/scene.py for the attribute names)."""
 
-    def __init__(self, scene, total_steps):
+    def __init__(self, scene, total_steps, broadband=None):
+        """broadband: an NffftBroadBand surface kept beside the scene's own
+        NF-FF surface (the two are independent in the oracle)."""
         L = lib()
         self.scene = scene
+        self.broadband = broadband
         self.T = int(total_steps)
         self.h = L.xo_sim_create(float(scene.cell_size), float(scene.cfl))
         if not self.h:
@@ -119,7 +128,10 @@ class OracleSim:
                 self._chk(L.xo_sim_set_tfsf(self.h, dist, t.e_0, t.theta, t.phi, t.psi,
                                             w.amplitude, w.tau, w.t_0))
             n = scene.nffft
-            if n is not None:
+            if n is not None and not hasattr(n, "frequencies"):
+                dist = (C.c_size_t * 3)(n.distance_x, n.distance_y, n.distance_z)
+                self._chk(L.xo_sim_set_nffft_broadband(self.h, dist, float(n.theta), float(n.phi)))
+            elif n is not None:
                 dist = (C.c_size_t * 3)(n.distance_x, n.distance_y, n.distance_z)
                 fr = _arr3(n.frequencies)
                 th = _arr3(n.theta)
@@ -139,6 +151,10 @@ class OracleSim:
                 box = (C.c_size_t * 6)(*[int(v) for v in dz.box])
                 er = np.ascontiguousarray(np.asarray(dz.eps_r, dtype=np.float32).ravel())
                 self._chk(L.xo_sim_set_design(self.h, box, er.ctypes.data_as(C.POINTER(C.c_float))))
+            if broadband is not None:
+                dist = (C.c_size_t * 3)(broadband.distance_x, broadband.distance_y, broadband.distance_z)
+                self._chk(L.xo_sim_set_nffft_broadband(self.h, dist, float(broadband.theta),
+                                                       float(broadband.phi)))
             dp_ = getattr(scene, "dft_plane", None)
             if dp_ is not None:
                 self._chk(L.xo_sim_set_dft_plane(self.h, int(dp_.comp), int(dp_.axis), int(dp_.index),
@@ -235,6 +251,35 @@ class OracleSim:
         res["power_phi"] = pw[1].copy()
         return res
 
+    # ---- broadband NF-FF (nffft/nffft_broadband.cpp)
+    def bb_frequencies(self):
+        """(frequency, wave number) of the 2T-1 bins."""
+        Ns = 2 * self.T - 1
+        f, k = np.empty(Ns), np.empty(Ns)
+        self._chk(lib().xo_sim_bb_frequencies(self.h, _dp(f), _dp(k)))
+        return f, k
+
+    def bb_sums(self, k=None):
+        """The six surface integrals (n_x, n_y, n_z, l_x, l_y, l_z) per bin;
+        k overrides the bins' wave numbers."""
+        Ns = 2 * self.T - 1
+        out = np.empty(12 * Ns)
+        kk = None if k is None else np.ascontiguousarray(k, dtype=np.float64)
+        self._chk(lib().xo_sim_bb_sums(self.h, None if kk is None else _dp(kk), _dp(out)))
+        return out.view(np.complex128).reshape(6, Ns)
+
+    def get_bb_samples(self):
+        """Face-current series [T][8][Q] (currents in NF-FF accumulator order)."""
+        n = lib().xo_sim_bb_samples_size(self.h)
+        out = np.empty(n)
+        self._chk(lib().xo_sim_get_bb_samples(self.h, _dp(out)))
+        return out.reshape(self.T, 8, -1)
+
+    def bb_farfield(self):
+        """calculateFarField (nffft_broadband.cpp:316-345) from bb_sums."""
+        bb = self.broadband if self.broadband is not None else self.scene.nffft
+        return bb_farfield(self.bb_sums(), self.bb_frequencies(), bb.theta, bb.phi)
+
     def get_dft(self):
         dp_ = self.scene.dft_plane
         out = np.empty(2 * (dp_.hi1 - dp_.lo1) * (dp_.hi2 - dp_.lo2), dtype=np.float64)
@@ -274,3 +319,37 @@ class OracleSim:
         out = np.empty((len(self.scene.probes), self.T), dtype=np.float64)
         self._chk(lib().xo_sim_get_probes(self.h, _dp(out)))
         return out
+
+
+
+def _cdiv(z, r):
+    """std::complex<double> / double: each part divided by r."""
+    out = np.empty_like(z)
+    out.real = z.real / r
+    out.imag = z.imag / r
+    return out
+
+
+def bb_farfield(sums, freqs, theta, phi):
+    """nffft_broadband.cpp:316-345: projections onto (theta, phi), the
+    far-zone fields and the radiated power densities of every bin, from the
+    surface integrals `sums` = (n_x, n_y, n_z, l_x, l_y, l_z) and
+    `freqs` = (frequency, wave number)."""
+    from xfdtd_amd.scene import ETA_0 as eta, PI as pi
+    n_x, n_y, n_z, l_x, l_y, l_z = sums
+    f, k = freqs
+    cos_t, sin_t, cos_p, sin_p = np.cos(theta), np.sin(theta), np.cos(phi), np.sin(phi)
+    n_theta = cos_t * cos_p * n_x + cos_t * sin_p * n_y - sin_t * n_z
+    n_phi = -sin_p * n_x + cos_p * n_y
+    l_theta = cos_t * cos_p * l_x + cos_t * sin_p * l_y - sin_t * l_z
+    l_phi = -sin_p * l_x + cos_p * l_y
+    r = {"frequencies": f, "wave_number": k, "n_theta": n_theta, "n_phi": n_phi,
+         "l_theta": l_theta, "l_phi": l_phi}
+    r["e_theta"] = _cdiv((-1j * k) * (eta * n_theta + l_phi), 4 * pi)
+    r["e_phi"] = _cdiv((1j * k) * (-eta * n_phi + l_theta), 4 * pi)
+    r["h_theta"] = _cdiv((1j * k) * (n_phi - _cdiv(l_theta, eta)), 4 * pi)
+    r["h_phi"] = _cdiv((-1j * k) * (n_theta + _cdiv(l_phi, eta)), 4 * pi)
+    coff = k ** 2 / (32 * pi * pi * eta)
+    r["power_theta"] = coff * np.abs(eta * n_theta + l_phi) ** 2
+    r["power_phi"] = coff * np.abs(-eta * n_phi + l_theta) ** 2
+    return r

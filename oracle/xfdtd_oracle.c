@@ -138,6 +138,15 @@ typedef struct {
   double complex *tj, *tm;
 } nffft_t;
 
+/* NffftBroadBand (nffft/nffft_broadband.cpp:21-91): the same surface and
+   face currents as NffftFd, kept as time series */
+typedef struct {
+  size_t dist[3];
+  double theta, phi;
+  size_t li, ri, lj, rj, lk, rk, nxb, nyb, nzb, Q;
+  double *samples; /* [T][8][Q]: step, current c of nf_off(), surface cell */
+} bb_t;
+
 typedef struct {
   double origin[3], size[3];
   int ori;
@@ -169,6 +178,8 @@ struct xo_sim {
   tfsf_t tf;
   int has_nf;
   nffft_t nf;
+  int has_bb;
+  bb_t bb;
   csrc_t *src;
   int nsrc;
   probe_t *probes;
@@ -249,6 +260,7 @@ void xo_sim_destroy(xo_sim *s) {
   nffft_t *nf = &s->nf;
   free(nf->freqs); free(nf->theta); free(nf->phi); free(nf->k);
   free(nf->acc); free(nf->tj); free(nf->tm);
+  free(s->bb.samples);
   for (int q = 0; q < s->nsrc; ++q) { free(s->src[q].wave); free(s->src[q].coff_i); }
   free(s->src);
   free(s->probes);
@@ -864,6 +876,37 @@ size_t xo_sim_nffft_size(const xo_sim *s) {
   return nf_off(&s->nf, 3, 0);
 }
 
+int xo_sim_set_nffft_broadband(xo_sim *s, const size_t distance[3], double theta, double phi) {
+  if (s->inited) FAIL("simulation already initialised");
+  bb_t *b = &s->bb;
+  free(b->samples);
+  memset(b, 0, sizeof *b);
+  for (int a = 0; a < 3; ++a) b->dist[a] = distance[a];
+  b->theta = theta;
+  b->phi = phi;
+  s->has_bb = 1;
+  return 0;
+}
+
+static int init_bb(xo_sim *s) {
+  bb_t *b = &s->bb;
+  /* nffft/nffft.cpp:22-34 */
+  size_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
+  if (2 * b->dist[0] >= nx || 2 * b->dist[1] >= ny || 2 * b->dist[2] >= nz ||
+      b->dist[0] < 1 || b->dist[1] < 1 || b->dist[2] < 1)
+    FAIL("NF-FF box does not fit the grid");
+  b->li = b->dist[0]; b->ri = nx - b->dist[0];
+  b->lj = b->dist[1]; b->rj = ny - b->dist[1];
+  b->lk = b->dist[2]; b->rk = nz - b->dist[2];
+  b->nxb = b->ri - b->li; b->nyb = b->rj - b->lj; b->nzb = b->rk - b->lk;
+  b->Q = b->nyb * b->nzb + b->nzb * b->nxb + b->nxb * b->nyb;
+  /* nffft_broadband.cpp:36-91 sizes every series 2T-1; the steps past the
+     run's last are never written and are taken as zero */
+  b->samples = (double *)calloc((s->T ? s->T : 1) * 8 * b->Q, sizeof(double));
+  if (!b->samples) FAIL("out of memory");
+  return 0;
+}
+
 static int init_nffft(xo_sim *s) {
   nffft_t *f = &s->nf;
   /* nffft/nffft.cpp:22-34 */
@@ -908,8 +951,49 @@ static void acc_add(double complex *a, double v, double complex tw) {
   *a = CMPLX(creal(*a) + v * creal(tw), cimag(*a) + v * cimag(tw));
 }
 
-/* nffft/nffft_fd.cpp:108-203 with the stencils of
-   electromagnetic_field/linear_interpolation_emf.cpp:286-388 */
+/* The eight face-centre currents of one surface cell, with the signs of
+   nffft/nffft_fd.cpp:108-203 (and nffft_broadband.cpp:94-155, which has the
+   same), and the stencils of
+   electromagnetic_field/linear_interpolation_emf.cpp:286-388.
+   pair 0 (x faces li, ri): cell (a, b) = (j, k)
+   pair 1 (y faces lj, rj): cell (a, b) = (k, i)
+   pair 2 (z faces lk, rk): cell (a, b) = (i, j) */
+static void nf_face_values(const xo_sim *s, int pair, size_t lo, size_t hi, size_t a, size_t b,
+                           double v[8]) {
+  if (pair == 0) {
+    size_t li = lo, ri = hi, j = a, k = b;
+    v[0] = -1.0 * (0.5 * (EZ(li, j, k) + EZ(li, j + 1, k)));
+    v[1] = 0.5 * (EY(li, j, k) + EY(li, j, k + 1));
+    v[2] = 0.25 * (HZ(li, j, k) + HZ(li, j, k + 1) + HZ(li - 1, j, k) + HZ(li - 1, j, k + 1));
+    v[3] = -1.0 * (0.25 * (HY(li, j, k) + HY(li, j + 1, k) + HY(li - 1, j, k) + HY(li - 1, j + 1, k)));
+    v[4] = 0.5 * (EZ(ri, j, k) + EZ(ri, j + 1, k));
+    v[5] = -1.0 * (0.5 * (EY(ri, j, k) + EY(ri, j, k + 1)));
+    v[6] = -1.0 * (0.25 * (HZ(ri, j, k) + HZ(ri, j, k + 1) + HZ(ri - 1, j, k) + HZ(ri - 1, j, k + 1)));
+    v[7] = 0.25 * (HY(ri, j, k) + HY(ri, j + 1, k) + HY(ri - 1, j, k) + HY(ri - 1, j + 1, k));
+  } else if (pair == 1) {
+    size_t lj = lo, rj = hi, k = a, i = b;
+    v[0] = -1.0 * (0.5 * (EX(i, lj, k) + EX(i, lj + 1, k)));
+    v[1] = 0.5 * (EZ(i, lj, k) + EZ(i + 1, lj, k));
+    v[2] = 0.25 * (HX(i + 1, lj, k) + HX(i, lj, k) + HX(i + 1, lj - 1, k) + HX(i, lj - 1, k));
+    v[3] = -1.0 * (0.25 * (HZ(i, lj, k) + HZ(i, lj, k + 1) + HZ(i, lj - 1, k) + HZ(i, lj - 1, k + 1)));
+    v[4] = 0.5 * (EX(i, rj, k) + EX(i, rj + 1, k));
+    v[5] = -1.0 * (0.5 * (EZ(i, rj, k) + EZ(i + 1, rj, k)));
+    v[6] = -1.0 * (0.25 * (HX(i + 1, rj, k) + HX(i, rj, k) + HX(i + 1, rj - 1, k) + HX(i, rj - 1, k)));
+    v[7] = 0.25 * (HZ(i, rj, k) + HZ(i, rj, k + 1) + HZ(i, rj - 1, k) + HZ(i, rj - 1, k + 1));
+  } else {
+    size_t lk = lo, rk = hi, i = a, j = b;
+    v[0] = -1.0 * (0.5 * (EY(i, j, lk) + EY(i + 1, j, lk)));
+    v[1] = 0.5 * (EX(i, j, lk) + EX(i, j, lk + 1));
+    v[2] = 0.25 * (HY(i, j, lk) + HY(i, j + 1, lk) + HY(i, j, lk - 1) + HY(i, j + 1, lk - 1));
+    v[3] = -1.0 * (0.25 * (HX(i + 1, j, lk) + HX(i, j, lk) + HX(i + 1, j, lk - 1) + HX(i, j, lk - 1)));
+    v[4] = 0.5 * (EY(i, j, rk) + EY(i + 1, j, rk));
+    v[5] = -1.0 * (0.5 * (EX(i, j, rk) + EX(i, j, rk + 1)));
+    v[6] = -1.0 * (0.25 * (HY(i, j, rk) + HY(i, j + 1, rk) + HY(i, j, rk - 1) + HY(i, j + 1, rk - 1)));
+    v[7] = 0.25 * (HX(i + 1, j, rk) + HX(i, j, rk) + HX(i + 1, j, rk - 1) + HX(i, j, rk - 1));
+  }
+}
+
+/* nffft/nffft_fd.cpp:108-203: the running DFT of the face currents */
 static void nffft_update(xo_sim *s) {
   nffft_t *f = &s->nf;
   size_t li = f->li, ri = f->ri, lj = f->lj, rj = f->rj, lk = f->lk, rk = f->rk;
@@ -919,14 +1003,7 @@ static void nffft_update(xo_sim *s) {
   for (size_t j = lj; j < rj; ++j)
     for (size_t k = lk; k < rk; ++k) {
       double v[8];
-      v[0] = -1.0 * (0.5 * (EZ(li, j, k) + EZ(li, j + 1, k)));
-      v[1] = 0.5 * (EY(li, j, k) + EY(li, j, k + 1));
-      v[2] = 0.25 * (HZ(li, j, k) + HZ(li, j, k + 1) + HZ(li - 1, j, k) + HZ(li - 1, j, k + 1));
-      v[3] = -1.0 * (0.25 * (HY(li, j, k) + HY(li, j + 1, k) + HY(li - 1, j, k) + HY(li - 1, j + 1, k)));
-      v[4] = 0.5 * (EZ(ri, j, k) + EZ(ri, j + 1, k));
-      v[5] = -1.0 * (0.5 * (EY(ri, j, k) + EY(ri, j, k + 1)));
-      v[6] = -1.0 * (0.25 * (HZ(ri, j, k) + HZ(ri, j, k + 1) + HZ(ri - 1, j, k) + HZ(ri - 1, j, k + 1)));
-      v[7] = 0.25 * (HY(ri, j, k) + HY(ri, j + 1, k) + HY(ri - 1, j, k) + HY(ri - 1, j + 1, k));
+      nf_face_values(s, 0, li, ri, j, k, v);
       size_t cell = (j - lj) * f->nzb + (k - lk);
       for (int n = 0; n < NF; ++n)
         for (int c = 0; c < 8; ++c) {
@@ -937,14 +1014,7 @@ static void nffft_update(xo_sim *s) {
   for (size_t k = lk; k < rk; ++k)
     for (size_t i = li; i < ri; ++i) {
       double v[8];
-      v[0] = -1.0 * (0.5 * (EX(i, lj, k) + EX(i, lj + 1, k)));
-      v[1] = 0.5 * (EZ(i, lj, k) + EZ(i + 1, lj, k));
-      v[2] = 0.25 * (HX(i + 1, lj, k) + HX(i, lj, k) + HX(i + 1, lj - 1, k) + HX(i, lj - 1, k));
-      v[3] = -1.0 * (0.25 * (HZ(i, lj, k) + HZ(i, lj, k + 1) + HZ(i, lj - 1, k) + HZ(i, lj - 1, k + 1)));
-      v[4] = 0.5 * (EX(i, rj, k) + EX(i, rj + 1, k));
-      v[5] = -1.0 * (0.5 * (EZ(i, rj, k) + EZ(i + 1, rj, k)));
-      v[6] = -1.0 * (0.25 * (HX(i + 1, rj, k) + HX(i, rj, k) + HX(i + 1, rj - 1, k) + HX(i, rj - 1, k)));
-      v[7] = 0.25 * (HZ(i, rj, k) + HZ(i, rj, k + 1) + HZ(i, rj - 1, k) + HZ(i, rj - 1, k + 1));
+      nf_face_values(s, 1, lj, rj, k, i, v);
       size_t cell = (k - lk) * f->nxb + (i - li);
       for (int n = 0; n < NF; ++n)
         for (int c = 0; c < 8; ++c) {
@@ -955,20 +1025,40 @@ static void nffft_update(xo_sim *s) {
   for (size_t i = li; i < ri; ++i)
     for (size_t j = lj; j < rj; ++j) {
       double v[8];
-      v[0] = -1.0 * (0.5 * (EY(i, j, lk) + EY(i + 1, j, lk)));
-      v[1] = 0.5 * (EX(i, j, lk) + EX(i, j, lk + 1));
-      v[2] = 0.25 * (HY(i, j, lk) + HY(i, j + 1, lk) + HY(i, j, lk - 1) + HY(i, j + 1, lk - 1));
-      v[3] = -1.0 * (0.25 * (HX(i + 1, j, lk) + HX(i, j, lk) + HX(i + 1, j, lk - 1) + HX(i, j, lk - 1)));
-      v[4] = 0.5 * (EY(i, j, rk) + EY(i + 1, j, rk));
-      v[5] = -1.0 * (0.5 * (EX(i, j, rk) + EX(i, j, rk + 1)));
-      v[6] = -1.0 * (0.25 * (HY(i, j, rk) + HY(i, j + 1, rk) + HY(i, j, rk - 1) + HY(i, j + 1, rk - 1)));
-      v[7] = 0.25 * (HX(i + 1, j, rk) + HX(i, j, rk) + HX(i + 1, j, rk - 1) + HX(i, j, rk - 1));
+      nf_face_values(s, 2, lk, rk, i, j, v);
       size_t cell = (i - li) * f->nyb + (j - lj);
       for (int n = 0; n < NF; ++n)
         for (int c = 0; c < 8; ++c) {
           double complex tw = ((c % 4) < 2) ? f->tm[(size_t)n * T + nt] : f->tj[(size_t)n * T + nt];
           acc_add(&f->acc[nf_off(f, 2, c) + (size_t)n * cz + cell], v[c], tw);
         }
+    }
+}
+
+/* nffft_broadband.cpp:94-155: the step's face currents into the series.
+   Surface cell q runs over the x faces (j, k), then the y faces (k, i),
+   then the z faces (i, j), the last index fastest. */
+static void bb_update(xo_sim *s) {
+  bb_t *b = &s->bb;
+  double *row = b->samples + s->cur * 8 * b->Q;
+  size_t q = 0;
+  for (size_t j = b->lj; j < b->rj; ++j)
+    for (size_t k = b->lk; k < b->rk; ++k, ++q) {
+      double v[8];
+      nf_face_values(s, 0, b->li, b->ri, j, k, v);
+      for (int c = 0; c < 8; ++c) row[(size_t)c * b->Q + q] = v[c];
+    }
+  for (size_t k = b->lk; k < b->rk; ++k)
+    for (size_t i = b->li; i < b->ri; ++i, ++q) {
+      double v[8];
+      nf_face_values(s, 1, b->lj, b->rj, k, i, v);
+      for (int c = 0; c < 8; ++c) row[(size_t)c * b->Q + q] = v[c];
+    }
+  for (size_t i = b->li; i < b->ri; ++i)
+    for (size_t j = b->lj; j < b->rj; ++j, ++q) {
+      double v[8];
+      nf_face_values(s, 2, b->lk, b->rk, i, j, v);
+      for (int c = 0; c < 8; ++c) row[(size_t)c * b->Q + q] = v[c];
     }
 }
 
@@ -1363,6 +1453,7 @@ int xo_sim_init(xo_sim *s, size_t total_steps) {
   }
   if (s->has_tfsf && init_tfsf(s)) return -1;
   if (s->has_nf && init_nffft(s)) return -1;
+  if (s->has_bb && init_bb(s)) return -1;
   for (int q = 0; q < s->nsrc; ++q)
     if (init_csrc(s, &s->src[q])) return -1;
   /* fdtd_basic_coff.cpp:31-122 */
@@ -1496,6 +1587,7 @@ int xo_sim_step(xo_sim *s, size_t nsteps) {
     if (s->has_tfsf) tfsf_update_h(s);
     for (int b = 0; b < s->npml; ++b) pml_update_h(s, &s->pml[b]);
     if (s->has_nf) nffft_update(s);
+    if (s->has_bb) bb_update(s);
     if (s->has_dft) dft_update(s, s->cur);
     for (int p = 0; p < s->nprobe; ++p) {
       const probe_t *pr = &s->probes[p];
@@ -1503,6 +1595,144 @@ int xo_sim_step(xo_sim *s, size_t nsteps) {
     }
     s->cur++;
   }
+  return 0;
+}
+
+/* ------------------------------------------------ broadband far field */
+
+/* nffft_broadband.cpp:320-324: f = linspace(-fs/2, fs/2, 2T-1) with
+   fs = 1/dt (xtensor's start + q * step, step = (stop - start) / (n - 1)),
+   k = 2 pi f / c0 */
+int xo_sim_bb_frequencies(const xo_sim *s, double *freq, double *k) {
+  if (!s->inited || !s->has_bb) FAIL("no broadband NF-FF surface");
+  size_t Ns = 2 * s->T - 1;
+  double fs = 1 / s->dt;
+  double start = -fs / 2, stop = fs / 2;
+  double step = (stop - start) / (double)(Ns > 1 ? Ns - 1 : 1);
+  for (size_t q = 0; q < Ns; ++q) {
+    double f = start + step * (double)q;
+    if (freq) freq[q] = f;
+    if (k) k[q] = 2 * XO_PI * f / C0();
+  }
+  return 0;
+}
+
+/* One surface cell of calculateAuxiliary (nffft_broadband.cpp:441-466):
+   for the cell's four series (j_a, j_b, m_a, m_b) add
+   ds * exp(i delay k) * fftshift(fft(series)) to (n_a, n_b, l_a, l_b).
+   fft is the forward DFT over Ns = 2T-1 points of the series, zero past
+   step T-1: bin q holds m = q - (T-1), X(m) = sum_n x[n] e^{-2 pi i m n / Ns}
+   and X(-m) = conj(X(m)) for the real series. */
+static void bb_cell(const xo_sim *s, const double *k, const double complex *tw, double complex *X,
+                    double delay, double ds, const size_t series[4], double complex *const tgt[4]) {
+  const bb_t *b = &s->bb;
+  size_t T = s->T, Ns = 2 * T - 1;
+  for (int u = 0; u < 4; ++u) {
+    const double *x = b->samples + series[u];
+    for (size_t m = 0; m < T; ++m) {
+      double complex acc = 0;
+      for (size_t n = 0; n < T; ++n) {
+        double v = x[n * 8 * b->Q];
+        double complex w = tw[(m * n) % Ns];
+        acc = CMPLX(creal(acc) + v * creal(w), cimag(acc) + v * cimag(w));
+      }
+      X[m] = acc;
+    }
+    for (size_t q = 0; q < Ns; ++q) {
+      long long m = (long long)q - (long long)(T - 1);
+      double complex xq = m >= 0 ? X[m] : conj(X[-m]);
+      double complex ph = cscale(ds, cexpi(delay * k[q]));
+      tgt[u][q] = cadd(tgt[u][q], cmul(ph, xq));
+    }
+  }
+}
+
+/* nffft_broadband.cpp:316-439 (calculateLNXYZ): out holds the six
+   surface integrals n_x, n_y, n_z, l_x, l_y, l_z over the 2T-1 frequency
+   bins, each interleaved re,im.  k: the bins' wave numbers, or NULL for
+   xo_sim_bb_frequencies'. */
+int xo_sim_bb_sums(xo_sim *s, const double *k_in, double *out) {
+  if (!s->inited || !s->has_bb) FAIL("no broadband NF-FF surface");
+  const bb_t *b = &s->bb;
+  size_t T = s->T, Ns = 2 * T - 1;
+  double *k = (double *)malloc(Ns * sizeof(double));
+  double complex *tw = (double complex *)malloc(Ns * sizeof(double complex));
+  double complex *X = (double complex *)malloc(T * sizeof(double complex));
+  double complex *acc = (double complex *)calloc(6 * Ns, sizeof(double complex));
+  if (!k || !tw || !X || !acc) {
+    free(k); free(tw); free(X); free(acc);
+    FAIL("out of memory");
+  }
+  if (k_in)
+    memcpy(k, k_in, Ns * sizeof(double));
+  else
+    xo_sim_bb_frequencies(s, NULL, k);
+  for (size_t r = 0; r < Ns; ++r) {
+    double a = 2 * XO_PI * (double)r / (double)Ns;
+    tw[r] = CMPLX(cos(a), -sin(a));
+  }
+  /* nffft_broadband.cpp:27-28 */
+  const double rx = sin(b->theta) * cos(b->phi), ry = sin(b->theta) * sin(b->phi), rz = cos(b->theta);
+  /* grid/grid_box.cpp:101-105 */
+  const double cx = (double)(b->li + b->nxb / 2), cy = (double)(b->lj + b->nyb / 2),
+               cz = (double)(b->lk + b->nzb / 2);
+  const double dl = s->dl, ds = dl * dl;
+  const size_t Q = b->Q, qy = b->nyb * b->nzb, qz = qy + b->nzb * b->nxb;
+  double complex *nx_ = acc, *ny_ = acc + Ns, *nz_ = acc + 2 * Ns;
+  double complex *lx_ = acc + 3 * Ns, *ly_ = acc + 4 * Ns, *lz_ = acc + 5 * Ns;
+#define BB_DELAY(I, J, K, OX, OY, OZ)                                                   \
+  ((((double)(I) + (OX)) - cx) * rx * dl + (((double)(J) + (OY)) - cy) * ry * dl) + \
+      (((double)(K) + (OZ)) - cz) * rz * dl
+  /* x faces: (n_y, n_z, l_y, l_z) <- (jy, jz, my, mz) = currents 2, 3, 0, 1 */
+  for (int side = 0; side < 2; ++side) {
+    size_t i = side ? b->ri : b->li;
+    double complex *const tgt[4] = {ny_, nz_, ly_, lz_};
+    for (size_t j = b->lj; j < b->rj; ++j)
+      for (size_t kk = b->lk; kk < b->rk; ++kk) {
+        size_t cell = (j - b->lj) * b->nzb + (kk - b->lk), c0 = 4 * (size_t)side;
+        size_t ser[4] = {(c0 + 2) * Q + cell, (c0 + 3) * Q + cell, c0 * Q + cell, (c0 + 1) * Q + cell};
+        bb_cell(s, k, tw, X, BB_DELAY(i, j, kk, 0.0, 0.5, 0.5), ds, ser, tgt);
+      }
+  }
+  /* y faces: (n_x, n_z, l_x, l_z) <- (jx, jz, mx, mz) = currents 3, 2, 1, 0 */
+  for (int side = 0; side < 2; ++side) {
+    size_t j = side ? b->rj : b->lj;
+    double complex *const tgt[4] = {nx_, nz_, lx_, lz_};
+    for (size_t i = b->li; i < b->ri; ++i)
+      for (size_t kk = b->lk; kk < b->rk; ++kk) {
+        size_t cell = qy + (kk - b->lk) * b->nxb + (i - b->li), c0 = 4 * (size_t)side;
+        size_t ser[4] = {(c0 + 3) * Q + cell, (c0 + 2) * Q + cell, (c0 + 1) * Q + cell, c0 * Q + cell};
+        bb_cell(s, k, tw, X, BB_DELAY(i, j, kk, 0.5, 0.0, 0.5), ds, ser, tgt);
+      }
+  }
+  /* z faces (only for nz > 1): (n_x, n_y, l_x, l_y) <- (jx, jy, mx, my) = currents 2, 3, 0, 1 */
+  if (b->nzb > 1)
+    for (int side = 0; side < 2; ++side) {
+      size_t kk = side ? b->rk : b->lk;
+      double complex *const tgt[4] = {nx_, ny_, lx_, ly_};
+      for (size_t i = b->li; i < b->ri; ++i)
+        for (size_t j = b->lj; j < b->rj; ++j) {
+          size_t cell = qz + (i - b->li) * b->nyb + (j - b->lj), c0 = 4 * (size_t)side;
+          size_t ser[4] = {(c0 + 2) * Q + cell, (c0 + 3) * Q + cell, c0 * Q + cell, (c0 + 1) * Q + cell};
+          bb_cell(s, k, tw, X, BB_DELAY(i, j, kk, 0.5, 0.5, 0.0), ds, ser, tgt);
+        }
+    }
+#undef BB_DELAY
+  for (size_t q = 0; q < 6 * Ns; ++q) {
+    out[2 * q] = creal(acc[q]);
+    out[2 * q + 1] = cimag(acc[q]);
+  }
+  free(k); free(tw); free(X); free(acc);
+  return 0;
+}
+
+size_t xo_sim_bb_samples_size(const xo_sim *s) {
+  return s->inited && s->has_bb ? s->T * 8 * s->bb.Q : 0;
+}
+
+int xo_sim_get_bb_samples(const xo_sim *s, double *out) {
+  if (!s->inited || !s->has_bb) FAIL("no broadband NF-FF surface");
+  memcpy(out, s->bb.samples, s->T * 8 * s->bb.Q * sizeof(double));
   return 0;
 }
 

@@ -54,6 +54,8 @@ int xo_sim_set_tfsf(xo_sim *s, const size_t distance[3], double e0,
 int xo_sim_set_nffft(xo_sim *s, const size_t distance[3], const double *freqs,
                      int nf, const double *theta, int ntheta,
                      const double *phi, int nphi);
+/* NffftBroadBand(dx,dy,dz, theta, phi): face-current series of every step */
+int xo_sim_set_nffft_broadband(xo_sim *s, const size_t distance[3], double theta, double phi);
 /* CurrentSource(Cube, orientation, resistance, GaussianWaveform(amp,tau,t0)) */
 int xo_sim_add_current_source(xo_sim *s, const double origin[3],
                               const double size[3], int orientation,
@@ -89,6 +91,13 @@ int xo_sim_get_nffft(const xo_sim *s, double *out /* interleaved re,im */);
    nf*ntheta*nphi, packed as documented in xfdtd_oracle.c */
 int xo_sim_farfield(xo_sim *s, double *out);
 int xo_sim_get_probes(const xo_sim *s, double *out /* nprobe * total_steps */);
+/* broadband NF-FF: the 2T-1 bin frequencies and wave numbers (either may be
+   NULL); the six surface integrals n_x..l_z per bin (interleaved re,im; k
+   NULL = the bins' own wave numbers); the face-current series [T][8][Q] */
+int xo_sim_bb_frequencies(const xo_sim *s, double *freq, double *k);
+int xo_sim_bb_sums(xo_sim *s, const double *k, double *out);
+size_t xo_sim_bb_samples_size(const xo_sim *s);
+int xo_sim_get_bb_samples(const xo_sim *s, double *out);
 size_t xo_sim_current_step(const xo_sim *s);
 int xo_set_threads(int n);
 

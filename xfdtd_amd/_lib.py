@@ -25,7 +25,8 @@ ENGINE_SYMBOLS = [
     "xf_engine_set_design", "xf_engine_set_dft_plane", "xf_engine_get_dft", "xf_engine_checkpoint",
     "xf_engine_restore", "xf_engine_grad_eps", "xf_engine_adjoint_run", "xf_engine_set_drude",
     "xf_engine_drude_box", "xf_engine_get_drude_j", "xf_engine_set_horizon", "xf_engine_set_drude_j",
-    "xf_engine_get_box",
+    "xf_engine_get_box", "xf_engine_set_nffft_broadband", "xf_engine_broadband_samples_size",
+    "xf_engine_get_broadband_samples", "xf_engine_broadband_sums", "xf_engine_set_broadband_scratch",
 ]
 SIM_SYMBOLS = [
     "xf_sim_last_error", "xf_sim_create", "xf_sim_destroy", "xf_sim_add_cube", "xf_sim_add_sphere",
@@ -36,7 +37,8 @@ SIM_SYMBOLS = [
     "xf_sim_engine", "xf_sim_set_design", "xf_sim_set_dft_plane", "xf_sim_get_dft", "xf_sim_grad_eps",
     "xf_sim_adjoint_run", "xf_sim_checkpoint", "xf_sim_restore", "xf_sim_add_drude_cube",
     "xf_sim_add_drude_sphere", "xf_sim_drude_box", "xf_sim_get_drude_j", "xf_sim_set_drude_j",
-    "xf_sim_add_field_monitor", "xf_sim_add_movie_monitor",
+    "xf_sim_add_field_monitor", "xf_sim_add_movie_monitor", "xf_sim_set_nffft_broadband",
+    "xf_sim_broadband_samples_size", "xf_sim_get_broadband_samples", "xf_sim_broadband_farfield",
 ]
 
 
@@ -111,5 +113,11 @@ def lib():
     L.xf_sim_set_drude_j.argtypes = [vp, i, dp]
     L.xf_sim_add_field_monitor.argtypes = [vp, dp, dp, i, i, C.c_char_p, C.c_char_p]
     L.xf_sim_add_movie_monitor.argtypes = [vp, dp, dp, i, i, C.c_char_p, sz, sz]
+    L.xf_sim_set_nffft_broadband.argtypes = [vp, szp, d, d, C.c_char_p]
+    L.xf_engine_set_broadband_scratch.argtypes = [vp, i64]
+    L.xf_sim_broadband_samples_size.restype = i64
+    L.xf_sim_broadband_samples_size.argtypes = [vp]
+    L.xf_sim_get_broadband_samples.argtypes = [vp, dp]
+    L.xf_sim_broadband_farfield.argtypes = [vp, dp, dp]
     _LIB = L
     return L

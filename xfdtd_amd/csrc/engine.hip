@@ -9,6 +9,7 @@
 // validates that the lumped cells and TFSF faces lie outside every CPML slab,
 // where the two orders touch disjoint cells and give identical bits.
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
 #include <cstdio>
 #include <cstring>
@@ -79,6 +80,12 @@ struct xf_engine {
   void *nf_ring = nullptr;
   int nf_pending = 0;
   long long nf_step0 = 0;
+  // broadband NF-FF: the face values of every step, [step][8][surface cell]
+  bool has_bb = false;
+  Nffft bb{};
+  void *bb_hist = nullptr;
+  long long bb_T = 0;
+  long long bb_scratch = 1LL << 30;  // bytes of rows + spectra per far-field transform
   // sources, probes
   std::vector<SourceDev> sources;
   long long src_T = 0;
@@ -145,6 +152,29 @@ static xf_status dev_alloc(xf_engine *e, void **p, size_t bytes) {
   if (r != hipSuccess) FAILS(XF_EDEVICE, "hipMemsetAsync: %s", hipGetErrorString(r));
   return XF_OK;
 }
+
+// scratch device buffers of one call, zeroed on the stream that uses them;
+// alloc waits for the fill, so neither that stream's earlier work nor a
+// synchronous copy on the null stream can see the fill land late (a fill on
+// the null stream is not ordered with the engines' non-blocking streams)
+struct AdjBufs {
+  hipStream_t st = nullptr;
+  std::vector<void *> mem;
+  explicit AdjBufs(hipStream_t s = nullptr) : st(s) {}
+  AdjBufs(const AdjBufs &) = delete;
+  AdjBufs &operator=(const AdjBufs &) = delete;
+  void *alloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) return nullptr;
+    mem.push_back(p);
+    if (hipMemsetAsync(p, 0, bytes ? bytes : 16, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+      return nullptr;
+    return p;
+  }
+  ~AdjBufs() {
+    for (void *p : mem) hipFree(p);
+  }
+};
 
 // upload FP64 host data as the engine's working precision
 static xf_status upload_real(xf_engine *e, void **dst, const double *src, size_t n) {
@@ -473,6 +503,132 @@ extern "C" xf_status xf_engine_farfield_sums(xf_engine *e, double dl, const doub
   return XF_OK;
 }
 
+// ---------------------------------------------------------------- broadband NF-FF
+
+static long long bb_cells(const Nffft &n) {
+  return (long long)n.nyb * n.nzb + (long long)n.nzb * n.nxb + (long long)n.nxb * n.nyb;
+}
+
+extern "C" xf_status xf_engine_set_nffft_broadband(xf_engine *e, const int64_t box[6], int64_t total_steps) {
+  if (!e || !box) FAILS(XF_EINVAL, "null argument");
+  if (e->has_bb) FAILS(XF_ESTATE, "broadband NF-FF surface already set");
+  if (box[0] < 1 || box[2] < 1 || box[4] < 1 || box[1] >= e->nx || box[3] >= e->ny || box[5] >= e->nz ||
+      box[1] <= box[0] || box[3] <= box[2] || box[5] <= box[4] || total_steps < 1)
+    FAILS(XF_EINVAL, "broadband NF-FF box outside the grid or no steps");
+  Nffft &n = e->bb;
+  n = Nffft{};
+  n.li = (int)box[0]; n.ri = (int)box[1]; n.lj = (int)box[2]; n.rj = (int)box[3];
+  n.lk = (int)box[4]; n.rk = (int)box[5];
+  n.nxb = n.ri - n.li; n.nyb = n.rj - n.lj; n.nzb = n.rk - n.lk;
+  n.T_total = total_steps;
+  const double bytes = (double)total_steps * 8 * bb_cells(n) * e->esz;
+  xf_status s = dev_alloc(e, &e->bb_hist, (size_t)bytes);
+  if (s != XF_OK) return s;
+  e->bb_T = total_steps;
+  set_total(e, total_steps);
+  e->has_bb = true;
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_set_broadband_scratch(xf_engine *e, int64_t bytes) {
+  if (!e || bytes < 1) FAILS(XF_EINVAL, "bad scratch size");
+  e->bb_scratch = bytes;
+  return XF_OK;
+}
+
+extern "C" int64_t xf_engine_broadband_samples_size(const xf_engine *e) {
+  return e && e->has_bb ? e->bb_T * 8 * bb_cells(e->bb) : 0;
+}
+
+extern "C" xf_status xf_engine_get_broadband_samples(xf_engine *e, double *out) {
+  if (!e || !out || !e->has_bb) FAILS(XF_ESTATE, "no broadband NF-FF surface");
+  const size_t n = (size_t)xf_engine_broadband_samples_size(e);
+  HIPCHK(engine_wait(e));
+  if (e->prec == 64) {
+    HIPCHK(hipMemcpy(out, e->bb_hist, n * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(n);
+    HIPCHK(hipMemcpy(tmp.data(), e->bb_hist, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < n; ++q) out[q] = (double)tmp[q];
+  }
+  return XF_OK;
+}
+
+// the surface integrals of the broadband far field: series of `chunk`
+// consecutive (current, cell) pairs are transposed into FP64 rows of Ns
+// points (zero past step T-1), transformed by one batched real-to-complex
+// FFT, and folded into per-slice partial sums of every bin; the partials
+// are added in slice order at the end
+extern "C" xf_status xf_engine_broadband_sums(xf_engine *e, double dl, const double *wave_number, double theta,
+                                              double phi, double *out) {
+  if (!e || !wave_number || !out || !e->has_bb) FAILS(XF_EINVAL, "bad broadband far-field request");
+  const long long T = e->bb_T, Ns = 2 * T - 1, Q = bb_cells(e->bb), S = 8 * Q;
+  if (Ns > 0x7fffffff) FAILS(XF_EINVAL, "too many steps for one transform");
+  HIPCHK(engine_wait(e));
+  BbFar d{};
+  d.n = e->bb;
+  d.T = T;
+  d.Ns = Ns;
+  d.Q = Q;
+  // direction and centre on the host with the reference's libm
+  // (nffft_broadband.cpp:27-28, grid/grid_box.cpp:101-105)
+  d.rx = sin(theta) * cos(phi);
+  d.ry = sin(theta) * sin(phi);
+  d.rz = cos(theta);
+  d.cx = (double)(d.n.li + d.n.nxb / 2);
+  d.cy = (double)(d.n.lj + d.n.nyb / 2);
+  d.cz = (double)(d.n.lk + d.n.nzb / 2);
+  d.dl = dl;
+  d.ds = dl * dl;
+  d.zfaces = d.n.nzb > 1;
+  // chunk of series per transform: bb_scratch bytes of rows and spectra
+  const long long per = Ns * 8 + T * 16;
+  long long chunk = std::max<long long>(1, std::min<long long>(S, e->bb_scratch / per));
+  chunk = std::min<long long>(chunk, 65535LL * 64);  // grid y of the row transpose
+  const int slices = (int)std::min<long long>(256, chunk);
+  AdjBufs B(e->stream);
+  double *rows = (double *)B.alloc((size_t)chunk * Ns * sizeof(double));
+  double2 *spec = (double2 *)B.alloc((size_t)chunk * T * sizeof(double2));
+  double2 *part = (double2 *)B.alloc((size_t)slices * 6 * Ns * sizeof(double2));
+  double *dk = (double *)B.alloc((size_t)Ns * sizeof(double));
+  double2 *dout = (double2 *)B.alloc((size_t)6 * Ns * sizeof(double2));
+  if (!rows || !spec || !part || !dk || !dout) FAILS(XF_ENOMEM, "broadband far-field buffers");
+  HIPCHK(hipMemcpy(dk, wave_number, (size_t)Ns * sizeof(double), hipMemcpyHostToDevice));
+  d.k = dk;
+  hipfftHandle plan;
+  int nfft = (int)Ns;
+  if (hipfftPlanMany(&plan, 1, &nfft, nullptr, 1, (int)Ns, nullptr, 1, (int)T, HIPFFT_D2Z, (int)chunk) !=
+      HIPFFT_SUCCESS)
+    FAILS(XF_EDEVICE, "hipfftPlanMany(%d, batch %lld) failed", nfft, chunk);
+  struct PlanFree {
+    hipfftHandle p;
+    ~PlanFree() { hipfftDestroy(p); }
+  } plan_free{plan};
+  if (hipfftSetStream(plan, e->stream) != HIPFFT_SUCCESS) FAILS(XF_EDEVICE, "hipfftSetStream failed");
+  for (long long s0 = 0; s0 < S; s0 += chunk) {
+    const long long n = std::min<long long>(chunk, S - s0);
+    const dim3 tg((unsigned)((Ns + 63) / 64), (unsigned)((chunk + 63) / 64));
+    if (e->prec == 64)
+      hipLaunchKernelGGL(k_bb_rows<double>, tg, dim3(64, 4), 0, e->stream, (const double *)e->bb_hist, S, T, Ns,
+                         s0, n, chunk, rows);
+    else
+      hipLaunchKernelGGL(k_bb_rows<float>, tg, dim3(64, 4), 0, e->stream, (const float *)e->bb_hist, S, T, Ns, s0,
+                         n, chunk, rows);
+    HIPCHK(hipGetLastError());
+    if (hipfftExecD2Z(plan, rows, reinterpret_cast<hipfftDoubleComplex *>(spec)) != HIPFFT_SUCCESS)
+      FAILS(XF_EDEVICE, "hipfftExecD2Z failed");
+    hipLaunchKernelGGL(k_bb_fold, dim3((unsigned)((Ns + 255) / 256), (unsigned)slices), dim3(256), 0, e->stream, d,
+                       (const double2 *)spec, s0, n, slices, part, s0 == 0 ? 1 : 0);
+    HIPCHK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_bb_sum_slices, dim3((unsigned)((6 * Ns + 255) / 256)), dim3(256), 0, e->stream,
+                     (const double2 *)part, slices, 6 * Ns, dout);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipMemcpy(out, dout, (size_t)6 * Ns * sizeof(double2), hipMemcpyDeviceToHost));
+  return XF_OK;
+}
+
 extern "C" xf_status xf_engine_add_current_source(xf_engine *e, int comp, int64_t i0, int64_t i1,
                                                   int64_t j0, int64_t j1, int64_t k0, int64_t k1,
                                                   const double *coff, const double *wave,
@@ -704,6 +860,11 @@ static xf_status run_steps(xf_engine *e, long long n) {
       const int n = (e->dft.hi1 - e->dft.lo1) * (e->dft.hi2 - e->dft.lo2);
       hipLaunchKernelGGL(k_dft_plane<T>, dim3((n + 255) / 256), dim3(256), 0, e->stream,
                          (const T *)e->f[e->dft.comp], e->si, e->sj, e->dft, step);
+    }
+    if (e->has_bb && step < e->bb_T) {
+      const long long Q = bb_cells(e->bb);
+      hipLaunchKernelGGL(k_nffft_sample<T>, dim3((unsigned)((Q + 255) / 256)), dim3(256), 0, e->stream, g, e->bb,
+                         (T *)e->bb_hist, (int)step);
     }
     if (!e->probe_comp.empty()) {
       const int np = (int)e->probe_comp.size();
@@ -960,6 +1121,7 @@ extern "C" xf_status xf_batch_create(xf_engine *const *engines, int n, xf_batch 
     if (!e) FAILS(XF_EINVAL, "null engine %d", q);
     if (!e->materials_set) FAILS(XF_ESTATE, "engine %d is not set up", q);
     if (e->batch_stream) FAILS(XF_ESTATE, "engine %d already belongs to a batch", q);
+    if (e->has_bb) FAILS(XF_EINVAL, "engine %d has a broadband NF-FF surface, which batches do not step", q);
     if (e->prec != e0->prec) FAILS(XF_EINVAL, "engine %d has precision %d, engine 0 has %d", q, e->prec, e0->prec);
     if (e->cur != e0->cur) FAILS(XF_ESTATE, "engine %d is at step %lld, engine 0 at %lld", q, e->cur, e0->cur);
     for (int r = 0; r < q; ++r)
@@ -1268,28 +1430,6 @@ extern "C" xf_status xf_engine_restore(xf_engine *e, int slot) {
 }
 
 // device buffers of one adjoint run
-// device buffers of one gradient run, zeroed on the stream that uses them;
-// alloc waits for the fill, so neither that stream's earlier work nor a
-// synchronous copy on the null stream can see the fill land late (a fill on
-// the null stream is not ordered with the engines' non-blocking streams)
-struct AdjBufs {
-  hipStream_t st = nullptr;
-  std::vector<void *> mem;
-  explicit AdjBufs(hipStream_t s = nullptr) : st(s) {}
-  AdjBufs(const AdjBufs &) = delete;
-  AdjBufs &operator=(const AdjBufs &) = delete;
-  void *alloc(size_t bytes) {
-    void *p = nullptr;
-    if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) return nullptr;
-    mem.push_back(p);
-    if (hipMemsetAsync(p, 0, bytes ? bytes : 16, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
-      return nullptr;
-    return p;
-  }
-  ~AdjBufs() {
-    for (void *p : mem) hipFree(p);
-  }
-};
 
 template <typename T>
 static xf_status adj_setup(xf_engine *e, AdjBufs &B, Adj<T> &a) {

@@ -1248,6 +1248,114 @@ __global__ void __launch_bounds__(256) k_farfield(Nffft n, FarDir d, double2 *ou
 }
 
 // ---------------------------------------------------------------------------
+// NffftBroadBand::calculateLNXYZ / calculateAuxiliary
+// (nffft_broadband.cpp:348-466).  A series is one face current c (0..7, the
+// order of the NF-FF accumulators) of one surface cell, sampled every step
+// into [step][c][cell] by k_nffft_sample.
+struct BbFar {
+  Nffft n;
+  long long T, Ns, Q;
+  double rx, ry, rz;  // observation direction
+  double cx, cy, cz;  // box centre (grid/grid_box.cpp:101-105)
+  double dl, ds;
+  int zfaces;         // z faces only when the box has nz > 1
+  const double *k;    // wave number of every bin
+};
+
+// series s0 .. s0+chunk-1 as FP64 rows of Ns points, zero past step T-1 and
+// for series past s0+n: 64 x 64 tiles through LDS, coalesced on both sides
+template <typename T>
+__global__ void __launch_bounds__(256) k_bb_rows(const T *__restrict__ hist, long long S, long long Tn, long long Ns,
+                                                 long long s0, long long n, long long chunk,
+                                                 double *__restrict__ rows) {
+  __shared__ double tile[64][65];
+  const long long t0 = (long long)blockIdx.x * 64, c0 = (long long)blockIdx.y * 64;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  for (int r = ty; r < 64; r += 4) {
+    const long long t = t0 + r, c = c0 + tx;
+    double v = 0.0;
+    if (t < Tn && c < n) v = (double)hist[t * S + s0 + c];
+    tile[r][tx] = v;
+  }
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4) {
+    const long long c = c0 + r, t = t0 + tx;
+    if (c < chunk && t < Ns) rows[c * Ns + t] = tile[tx][r];
+  }
+}
+
+// one thread per (bin q, slice of the chunk's series): adds
+// ds e^{i delay k_q} X_s(m), m = q - (T-1), to the bin's sum of the series'
+// target (n_x, n_y, n_z, l_x, l_y, l_z).  spec holds the non-negative half of
+// each real series' DFT; X(-m) = conj(X(m)).
+__global__ void __launch_bounds__(256) k_bb_fold(BbFar d, const double2 *__restrict__ spec, long long s0, long long n,
+                                                 int slices, double2 *part, int first) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int slice = blockIdx.y;
+  if (q >= d.Ns) return;
+  const long long m = q - (d.T - 1), am = m < 0 ? -m : m;
+  const double kq = d.k[q];
+  double2 acc[6];
+#pragma unroll
+  for (int u = 0; u < 6; ++u)
+    acc[u] = first ? make_double2(0.0, 0.0) : part[((long long)slice * 6 + u) * d.Ns + q];
+  const long long per = (n + slices - 1) / slices;
+  const long long a = (long long)slice * per, b = a + per < n ? a + per : n;
+  const Nffft &f = d.n;
+  const long long cxq = (long long)f.nyb * f.nzb, cyq = (long long)f.nzb * f.nxb;
+  for (long long sl = a; sl < b; ++sl) {
+    const long long s = s0 + sl;
+    const int c = (int)(s / d.Q);
+    long long cell = s - (long long)c * d.Q;
+    const bool side = c >= 4;
+    const int cur = c & 3;
+    double x, y, z;
+    int tgt;
+    if (cell < cxq) {
+      const int i = side ? f.ri : f.li, j = f.lj + (int)(cell / f.nzb), k = f.lk + (int)(cell % f.nzb);
+      x = ((double)i + 0.0) - d.cx; y = ((double)j + 0.5) - d.cy; z = ((double)k + 0.5) - d.cz;
+      tgt = cur == 0 ? 4 : cur == 1 ? 5 : cur == 2 ? 1 : 2;  // my, mz, jy, jz
+    } else if ((cell -= cxq) < cyq) {
+      const int k = f.lk + (int)(cell / f.nxb), i = f.li + (int)(cell % f.nxb), j = side ? f.rj : f.lj;
+      x = ((double)i + 0.5) - d.cx; y = ((double)j + 0.0) - d.cy; z = ((double)k + 0.5) - d.cz;
+      tgt = cur == 0 ? 5 : cur == 1 ? 3 : cur == 2 ? 2 : 0;  // mz, mx, jz, jx
+    } else {
+      if (!d.zfaces) continue;
+      cell -= cyq;
+      const int i = f.li + (int)(cell / f.nyb), j = f.lj + (int)(cell % f.nyb), k = side ? f.rk : f.lk;
+      x = ((double)i + 0.5) - d.cx; y = ((double)j + 0.5) - d.cy; z = ((double)k + 0.0) - d.cz;
+      tgt = cur == 0 ? 3 : cur == 1 ? 4 : cur == 2 ? 0 : 1;  // mx, my, jx, jy
+    }
+    const double delay = (x * d.rx * d.dl + y * d.ry * d.dl) + z * d.rz * d.dl;
+    double2 X = spec[sl * d.T + am];
+    if (m < 0) X.y = -X.y;
+    double sn, cs;
+    sincos(delay * kq, &sn, &cs);
+    const double2 t = cmul(make_double2(d.ds * cs, d.ds * sn), X);
+    switch (tgt) {
+      case 0: acc[0] = cadd(acc[0], t); break;
+      case 1: acc[1] = cadd(acc[1], t); break;
+      case 2: acc[2] = cadd(acc[2], t); break;
+      case 3: acc[3] = cadd(acc[3], t); break;
+      case 4: acc[4] = cadd(acc[4], t); break;
+      default: acc[5] = cadd(acc[5], t); break;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 6; ++u) part[((long long)slice * 6 + u) * d.Ns + q] = acc[u];
+}
+
+// out[i] = sum over slices of part[slice][i], in slice order
+__global__ void __launch_bounds__(256) k_bb_sum_slices(const double2 *__restrict__ part, int slices, long long total,
+                                                       double2 *__restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  double2 a = make_double2(0.0, 0.0);
+  for (int s = 0; s < slices; ++s) a = cadd(a, part[(long long)s * total + i]);
+  out[i] = a;
+}
+
+// ---------------------------------------------------------------------------
 // DFT monitor of one component on a plane (the adjoint's objective): after
 // step t, F(u, v) += f(cell) * w[t], FP64.  (u, v) run over the two axes
 // other than `axis`, in increasing axis order.

@@ -58,6 +58,10 @@ struct PlanData {
     xf_nffft_desc desc{};
     std::vector<std::complex<double>> tj, tm;
   } nf;
+  struct Bb {
+    bool present = false;
+    int64_t box[6] = {};  // li, ri, lj, rj, lk, rk
+  } bb;
   struct Design {
     bool present = false;
     int64_t box[6] = {};
@@ -180,6 +184,11 @@ TFSF3D::TFSF3D(SpatialIndex distance_x, SpatialIndex distance_y, SpatialIndex di
   const PointVector &e = _transform_e;
   _transform_h = {k[1] * e[2] - k[2] * e[1], k[2] * e[0] - k[0] * e[2], k[0] * e[1] - k[1] * e[0]};
 }
+
+// nffft/nffft_broadband.cpp:21-28
+NffftBroadBand::NffftBroadBand(SpatialIndex distance_x, SpatialIndex distance_y, SpatialIndex distance_z,
+                               double theta, double phi, std::string output_dir_path)
+    : NFFFT(distance_x, distance_y, distance_z, std::move(output_dir_path)), _theta{theta}, _phi{phi} {}
 
 // include/nffft/nffft_fd.h constructor
 NffftFd::NffftFd(SpatialIndex distance_x, SpatialIndex distance_y, SpatialIndex distance_z,
@@ -810,10 +819,23 @@ void Simulation::plan(size_t total_time_steps) {
     tp.present = true;
   }
 
+  // ---- broadband NF-FF (nffft/nffft.cpp:22-34, nffft_broadband.cpp:30-91)
+  if (auto *bb = dynamic_cast<NffftBroadBand *>(_nffft.get())) {
+    const auto dist = bb->getDistance();
+    for (int a = 0; a < 3; ++a)
+      if (dist[a] < 1 || 2 * dist[a] >= (size_t)N[a]) throw std::runtime_error("NF-FF box does not fit the grid");
+    P.bb.present = true;
+    const int64_t ext[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a) {
+      P.bb.box[2 * a] = (int64_t)dist[a];
+      P.bb.box[2 * a + 1] = ext[a] - (int64_t)dist[a];
+    }
+    for (int q = 0; q < 6; ++q) _nffft_box[q] = (size_t)P.bb.box[q];
+  }
   // ---- NF-FF (nffft/nffft.cpp:22-34, nffft_fd.cpp:17-106)
-  if (_nffft) {
+  if (_nffft && !P.bb.present) {
     auto *nf = dynamic_cast<NffftFd *>(_nffft.get());
-    if (!nf) throw std::runtime_error("only NffftFd is supported");
+    if (!nf) throw std::runtime_error("unsupported NFFFT type");
     const auto dist = nf->getDistance();
     for (int a = 0; a < 3; ++a)
       if (dist[a] < 1 || 2 * dist[a] >= (size_t)N[a]) throw std::runtime_error("NF-FF box does not fit the grid");
@@ -868,6 +890,7 @@ void Simulation::upload() {
     td.source = P.tfsf.source.data();
     check(xf_engine_set_tfsf(_engine, &td));
   }
+  if (P.bb.present) check(xf_engine_set_nffft_broadband(_engine, P.bb.box, (int64_t)_total_time_steps));
   if (P.nf.present) {
     xf_nffft_desc nd = P.nf.desc;
     nd.tj = reinterpret_cast<const double *>(P.nf.tj.data());
@@ -1128,6 +1151,68 @@ FarField Simulation::calculateFarField() const {
   return r;
 }
 
+std::vector<double> Simulation::getBroadBandSamples() const {
+  if (!_engine || !dynamic_cast<NffftBroadBand *>(_nffft.get())) throw std::runtime_error("no broadband NF-FF");
+  std::vector<double> out((size_t)xf_engine_broadband_samples_size(_engine));
+  check(xf_engine_get_broadband_samples(_engine, out.data()));
+  return out;
+}
+
+// nffft_broadband.cpp:316-345: bins f = linspace(-fs/2, fs/2, 2T-1) (xtensor's
+// start + q step), k = 2 pi f / c0; the surface integrals come from the
+// engine, the projections and far-zone fields follow the reference.
+BroadBandFarField Simulation::calculateBroadBandFarField(const std::vector<double> *wave_number) const {
+  auto *bb = dynamic_cast<NffftBroadBand *>(_nffft.get());
+  if (!_engine || !bb) throw std::runtime_error("no broadband NF-FF surface");
+  using C = std::complex<double>;
+  using namespace std::complex_literals;
+  const size_t T = _total_time_steps, Ns = 2 * T - 1;
+  BroadBandFarField r;
+  r.frequencies.resize(Ns);
+  r.wave_number.resize(Ns);
+  const double fs = 1 / _dt;
+  const double start = -fs / 2, stop = fs / 2;
+  const double step = (stop - start) / (double)(Ns > 1 ? Ns - 1 : 1);
+  for (size_t q = 0; q < Ns; ++q) {
+    r.frequencies[q] = start + step * (double)q;
+    r.wave_number[q] = 2 * constant::PI * r.frequencies[q] / constant::C_0;
+  }
+  const std::vector<double> &k = wave_number ? *wave_number : r.wave_number;
+  if (k.size() != Ns) throw std::invalid_argument("wave_number needs 2T-1 entries");
+  std::vector<C> sums(6 * Ns);
+  check(xf_engine_broadband_sums(_engine, _cell_size, k.data(), bb->theta(), bb->phi(),
+                                 reinterpret_cast<double *>(sums.data())));
+  auto part = [&](int u) { return std::vector<C>(sums.begin() + u * Ns, sums.begin() + (u + 1) * Ns); };
+  r.n_x = part(0); r.n_y = part(1); r.n_z = part(2);
+  r.l_x = part(3); r.l_y = part(4); r.l_z = part(5);
+  for (auto *v : {&r.n_theta, &r.n_phi, &r.l_theta, &r.l_phi, &r.e_theta, &r.e_phi, &r.h_theta, &r.h_phi})
+    v->resize(Ns);
+  r.power_theta.resize(Ns);
+  r.power_phi.resize(Ns);
+  const double cos_t = std::cos(bb->theta()), sin_t = std::sin(bb->theta());
+  const double cos_p = std::cos(bb->phi()), sin_p = std::sin(bb->phi());
+  const double eta = constant::ETA_0;
+  for (size_t q = 0; q < Ns; ++q) {
+    const C n_theta = cos_t * cos_p * r.n_x[q] + cos_t * sin_p * r.n_y[q] - sin_t * r.n_z[q];
+    const C n_phi = -sin_p * r.n_x[q] + cos_p * r.n_y[q];
+    const C l_theta = cos_t * cos_p * r.l_x[q] + cos_t * sin_p * r.l_y[q] - sin_t * r.l_z[q];
+    const C l_phi = -sin_p * r.l_x[q] + cos_p * r.l_y[q];
+    const double kq = r.wave_number[q];
+    r.n_theta[q] = n_theta;
+    r.n_phi[q] = n_phi;
+    r.l_theta[q] = l_theta;
+    r.l_phi[q] = l_phi;
+    r.e_theta[q] = (-1i * kq) * (eta * n_theta + l_phi) / (4 * constant::PI);
+    r.e_phi[q] = (1i * kq) * (-eta * n_phi + l_theta) / (4 * constant::PI);
+    r.h_theta[q] = (1i * kq) * (n_phi - l_theta / eta) / (4 * constant::PI);
+    r.h_phi[q] = (-1i * kq) * (n_theta + l_phi / eta) / (4 * constant::PI);
+    const double coff = std::pow(kq, 2) / (32 * constant::PI * constant::PI * eta);
+    r.power_theta[q] = coff * std::pow(std::abs(eta * n_theta + l_phi), 2);
+    r.power_phi[q] = coff * std::pow(std::abs(-eta * n_phi + l_theta), 2);
+  }
+  return r;
+}
+
 namespace {
 
 // minimal .npy (format 1.0) writer, little-endian C order
@@ -1230,6 +1315,10 @@ void Simulation::outputData() const {
   if (_engine)
     for (const MonitorPlan &m : _monitor_plans)
       if (m.plot_step == 0) writeMonitor(m, m.monitor->getOutputFileName());
+  if (auto *bb = dynamic_cast<NffftBroadBand *>(_nffft.get())) {
+    if (_engine && !bb->getOutputDirPath().empty()) outputBroadBand(*bb);
+    return;
+  }
   auto *nf = dynamic_cast<NffftFd *>(_nffft.get());
   if (!nf || nf->getOutputDirPath().empty()) return;
   const FarField r = calculateFarField();
@@ -1253,6 +1342,34 @@ void Simulation::outputData() const {
   writeNpy(p("n_phi.npy"), r.n_phi.data(), "<c16", 16, shp);
   writeNpy(p("l_theta.npy"), r.l_theta.data(), "<c16", 16, shp);
   writeNpy(p("l_phi.npy"), r.l_phi.data(), "<c16", 16, shp);
+  writeNpy(p("power_theta.npy"), r.power_theta.data(), "<f8", 8, shp);
+  writeNpy(p("power_phi.npy"), r.power_phi.data(), "<f8", 8, shp);
+}
+
+// NffftBroadBand::outputData and outputFarFieldParameters
+// (nffft_broadband.cpp:156-200, 468-479)
+void Simulation::outputBroadBand(const NffftBroadBand &bb) const {
+  const BroadBandFarField r = calculateBroadBandFarField();
+  const std::string dir = bb.getOutputDirPath();
+  try {
+    makeDirs(dir);
+  } catch (const std::exception &) {
+    std::fprintf(stderr, "Error: cannot create directory %s\n", dir.c_str());
+    return;
+  }
+  {
+    std::ofstream w(dir + "/far_field_parameter.dat");
+    w << "Fs:\t" << 1 / _dt << '\n';
+    w << "number of samples:\t" << r.frequencies.size() << '\n';
+    w << "Theta:\n" << bb.theta() << '\n';
+    w << "Phi:\n" << bb.phi() << '\n';
+  }
+  const std::vector<size_t> shp = {r.frequencies.size()};
+  auto p = [&](const char *name) { return dir + "/" + name; };
+  writeNpy(p("e_theta.npy"), r.e_theta.data(), "<c16", 16, shp);
+  writeNpy(p("e_phi.npy"), r.e_phi.data(), "<c16", 16, shp);
+  writeNpy(p("h_theta.npy"), r.h_theta.data(), "<c16", 16, shp);
+  writeNpy(p("h_phi.npy"), r.h_phi.data(), "<c16", 16, shp);
   writeNpy(p("power_theta.npy"), r.power_theta.data(), "<f8", 8, shp);
   writeNpy(p("power_phi.npy"), r.power_phi.data(), "<f8", 8, shp);
 }

@@ -136,6 +136,14 @@ extern "C" int xf_sim_set_nffft_fd(xf_sim *s, const size_t distance[3], const do
   });
 }
 
+extern "C" int xf_sim_set_nffft_broadband(xf_sim *s, const size_t distance[3], double theta, double phi,
+                                          const char *output_dir) {
+  return guarded([&] {
+    s->sim.addNFFFT(std::make_shared<NffftBroadBand>(distance[0], distance[1], distance[2], theta, phi,
+                                                     output_dir ? std::string(output_dir) : std::string()));
+  });
+}
+
 extern "C" int xf_sim_add_current_source(xf_sim *s, const double origin[3], const double size[3],
                                          int orientation, double resistance, double amplitude,
                                          double tau, double t0) {
@@ -254,6 +262,36 @@ extern "C" int xf_sim_farfield(xf_sim *s, double *out) {
     for (int q = 0; q < 8; ++q) std::memcpy(out + 2 * M * q, c[q]->data(), M * sizeof(std::complex<double>));
     std::memcpy(out + 16 * M, r.power_theta.data(), M * sizeof(double));
     std::memcpy(out + 17 * M, r.power_phi.data(), M * sizeof(double));
+  });
+}
+
+extern "C" int64_t xf_sim_broadband_samples_size(const xf_sim *s) {
+  return s->sim.engine() ? xf_engine_broadband_samples_size(s->sim.engine()) : 0;
+}
+
+extern "C" int xf_sim_get_broadband_samples(xf_sim *s, double *out) {
+  return guarded([&] {
+    auto v = s->sim.getBroadBandSamples();
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+  });
+}
+
+extern "C" int xf_sim_broadband_farfield(xf_sim *s, const double *wave_number, double *out) {
+  return guarded([&] {
+    const size_t Ns = 2 * s->sim.totalTimeSteps() - 1;
+    std::vector<double> k;
+    if (wave_number) k.assign(wave_number, wave_number + Ns);
+    BroadBandFarField r = s->sim.calculateBroadBandFarField(wave_number ? &k : nullptr);
+    std::memcpy(out, r.frequencies.data(), Ns * sizeof(double));
+    std::memcpy(out + Ns, r.wave_number.data(), Ns * sizeof(double));
+    const std::vector<std::complex<double>> *c[14] = {&r.n_x,     &r.n_y,     &r.n_z,     &r.l_x,   &r.l_y,
+                                                      &r.l_z,     &r.n_theta, &r.n_phi,   &r.l_theta,
+                                                      &r.l_phi,   &r.e_theta, &r.e_phi,   &r.h_theta,
+                                                      &r.h_phi};
+    for (int q = 0; q < 14; ++q)
+      std::memcpy(out + 2 * Ns + 2 * Ns * q, c[q]->data(), Ns * sizeof(std::complex<double>));
+    std::memcpy(out + 30 * Ns, r.power_theta.data(), Ns * sizeof(double));
+    std::memcpy(out + 31 * Ns, r.power_phi.data(), Ns * sizeof(double));
   });
 }
 

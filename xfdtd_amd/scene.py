@@ -14,6 +14,8 @@ Each class carries the constructor arguments of its reference counterpart:
                                           tfsf/tfsf_3d.cpp:13-31
   NffftFd(distance_x, distance_y, distance_z, frequencies, theta, phi)
                                           include/nffft/nffft_fd.h (constructor)
+  NffftBroadBand(distance_x, distance_y, distance_z, theta, phi)
+                                          nffft/nffft_broadband.cpp:21-28
   CurrentSource(cube, orientation, resistance, waveform)
                                           lumped_element/current_source.cpp:5-16
   Simulation(cell_size, cfl=0.99f)        include/simulation/simulation.h:25-27
@@ -143,6 +145,18 @@ class NffftFd:
 
 
 @dataclass
+class NffftBroadBand:
+    """The NF-FF surface of NffftFd, observed in one direction (theta, phi)
+    over every frequency bin of the run (2T-1 bins of 1/dt): the monostatic
+    radar cross-section path."""
+    distance_x: int
+    distance_y: int
+    distance_z: int
+    theta: float
+    phi: float
+
+
+@dataclass
 class CurrentSource:
     cube: Cube
     orientation: int
@@ -219,7 +233,7 @@ class Scene:
     objects: List[Object] = field(default_factory=list)
     boundaries: List[PML] = field(default_factory=list)
     tfsf: Optional[TFSF3D] = None
-    nffft: Optional[NffftFd] = None
+    nffft: Optional[object] = None  # NffftFd or NffftBroadBand
     sources: List[CurrentSource] = field(default_factory=list)
     probes: List[Probe] = field(default_factory=list)
     monitors: List[object] = field(default_factory=list)  # TimeDomainFieldMonitor / MovieMonitor

@@ -61,7 +61,11 @@ class Simulation:
                 self._chk(L.xf_sim_set_tfsf3d(self.h, dist, t.e_0, t.theta, t.phi, t.psi,
                                               w.amplitude, w.tau, w.t_0))
             n = scene.nffft
-            if n is not None:
+            if n is not None and not hasattr(n, "frequencies"):
+                dist = (C.c_size_t * 3)(n.distance_x, n.distance_y, n.distance_z)
+                od = output_dir.encode() if output_dir else None
+                self._chk(L.xf_sim_set_nffft_broadband(self.h, dist, float(n.theta), float(n.phi), od))
+            elif n is not None:
                 dist = (C.c_size_t * 3)(n.distance_x, n.distance_y, n.distance_z)
                 fr, th, ph = _f64(n.frequencies), _f64(n.theta), _f64(n.phi)
                 od = output_dir.encode() if output_dir else None
@@ -176,6 +180,32 @@ class Simulation:
         res["power_theta"] = pw[0].copy()
         res["power_phi"] = pw[1].copy()
         return res
+
+    # ---- broadband NF-FF (NffftBroadBand)
+    def get_bb_samples(self):
+        """Face-current series [T][8][Q] (xf_engine_get_broadband_samples)."""
+        n = lib().xf_sim_broadband_samples_size(self.h)
+        out = np.empty(n, dtype=np.float64)
+        self._chk(lib().xf_sim_get_broadband_samples(self.h, _dp(out)))
+        return out.reshape(self.T, 8, -1)
+
+    def bb_farfield(self, k=None):
+        """NffftBroadBand::calculateFarField over the 2T-1 bins: a dict of
+        frequencies, wave_number, the surface integrals n_x .. l_z, the
+        projections, e/h fields and power densities; k overrides the bins'
+        wave numbers in the surface integrals."""
+        Ns = 2 * self.T - 1
+        out = np.empty(32 * Ns, dtype=np.float64)
+        kk = None if k is None else _f64(k)
+        self._chk(lib().xf_sim_broadband_farfield(self.h, None if kk is None else _dp(kk), _dp(out)))
+        r = {"frequencies": out[:Ns].copy(), "wave_number": out[Ns:2 * Ns].copy()}
+        cplx = out[2 * Ns:30 * Ns].view(np.complex128).reshape(14, Ns)
+        for q, name in enumerate(["n_x", "n_y", "n_z", "l_x", "l_y", "l_z", "n_theta", "n_phi", "l_theta",
+                                  "l_phi", "e_theta", "e_phi", "h_theta", "h_phi"]):
+            r[name] = cplx[q].copy()
+        r["power_theta"] = out[30 * Ns:31 * Ns].copy()
+        r["power_phi"] = out[31 * Ns:].copy()
+        return r
 
     def output_nffft(self):
         self._chk(lib().xf_sim_output_nffft(self.h))
