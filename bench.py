@@ -33,6 +33,14 @@ cost_ratio = wall time of the gradient (forward with checkpoints + segment
 recompute + adjoint) / wall time of the plain forward run, per precision;
 the FP64 gradient is checked against central differences (delta eps_r =
 1e-3) at 16 seeded design cells, and the FP32 gradient against the FP64 one.
+
+batched_adjoint: 32 gradients of 96^3 instances with a Drude block, a design
+cube and TFSF at random incidence (xfdtd_amd/workloads.py), 1200 steps, as
+one batch against a loop of single gradients.
+
+broadband: the reference's monostatic example (96^3 PEC sphere, broadband
+NF-FF, FP64, 1200 steps): stepping with and without the broadband surface,
+and the far field of all 2T-1 bins (hipFFT + fold).
 """
 from __future__ import annotations
 
@@ -319,6 +327,50 @@ def run_batched_adjoint(count=32, steps=1200, every=0):
     }
 
 
+def monostatic_scene(broadband=True):
+    """The reference's monostatic example (examples/3d_pec_sphere_monostatic_rcs.cpp):
+    1.6 m air domain at dl = 2 cm with 8-cell CPML (96^3), PEC sphere of
+    radius 0.5 m, TFSF at 15 along +z, broadband NF-FF box at 12 observed
+    along -z, cfl 0.98."""
+    import math
+
+    from xfdtd_amd import scene as S
+    dl = 2e-2
+    sc = S.Scene(cell_size=dl, cfl=0.98)
+    sc.add_object(S.Object("domain", S.Cube((-0.8,) * 3, (1.6,) * 3), S.Material("air", 1, 1, 0, 0)))
+    sc.add_object(S.Object("scatter", S.Sphere((0, 0, 0), 0.5), S.Material("pec", 1, 1, 1e10, 0)))
+    sc.boundaries = [S.PML(o, 8) for o in range(6)]
+    tau = 20 * dl / 6e8
+    sc.tfsf = S.TFSF3D(15, 15, 15, 1.0, 0.0, 0.0, 0.0, S.GaussianWaveform(tau, 4.5 * tau, 1.0))
+    if broadband:
+        sc.nffft = S.NffftBroadBand(12, 12, 12, math.pi, math.pi)
+    return sc
+
+
+def run_broadband(steps=1200):
+    """The monostatic example in FP64: stepping with and without the
+    broadband surface, and the far field of all 2T-1 bins."""
+    from xfdtd_amd.sim import Simulation
+    out = {"workload": "reference monostatic example: 96^3 incl. 8-cell CPML, dl 2 cm, PEC sphere r 0.5 m, "
+                       "TFSF Gaussian along +z, broadband NF-FF box at 12 observed along -z, FP64",
+           "steps": steps, "bins": 2 * steps - 1}
+    for key, bb in (("step_s_without_surface", False), ("step_s", True)):
+        sim = Simulation(monostatic_scene(bb), steps, precision=64)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.sync()
+        out[key] = round(time.perf_counter() - t0, 4)
+        if bb:
+            sim.bb_farfield()  # hipFFT plans and kernels warm
+            t0 = time.perf_counter()
+            sim.bb_farfield()
+            out["far_field_s"] = round(time.perf_counter() - t0, 4)
+            out["series"] = int(sim.get_bb_samples().shape[2]) * 8
+        sim.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -338,6 +390,8 @@ def main():
     ap.add_argument("--batch-adjoint", type=int, default=32,
                     help="instances of the batched-gradient leg (0: skip it)")
     ap.add_argument("--batch-adjoint-steps", type=int, default=1200)
+    ap.add_argument("--broadband", type=int, default=1200,
+                    help="steps of the monostatic broadband NF-FF leg (0: skip it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 1 GiB device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -425,6 +479,8 @@ def main():
         line["adjoint"] = run_adjoint(args.adjoint_fd_cells)
     if args.batch_adjoint > 0:
         line["batched_adjoint"] = run_batched_adjoint(args.batch_adjoint, args.batch_adjoint_steps)
+    if args.broadband > 0:
+        line["broadband"] = run_broadband(args.broadband)
     if args.batch > 0:
         line["batched"] = run_batched(args.batch, args.batch_steps,
                                       log=lambda m: print(m, file=sys.stderr, flush=True))
