@@ -40,6 +40,12 @@ def lib():
         L.xo_sim_bb_samples_size.argtypes = [vp]
         L.xo_sim_get_bb_samples.argtypes = [vp, dp]
         L.xo_sim_add_current_source.argtypes = [vp, dp, dp, i, d, d, d, d]
+        L.xo_sim_add_lumped.argtypes = [vp, i, dp, dp, i, d, i, dp]
+        L.xo_sim_add_plane.argtypes = [vp, dp, dp, C.c_float, C.c_float, d, d]
+        L.xo_sim_add_line_monitor.argtypes = [vp, i, dp, dp, i]
+        L.xo_sim_get_line_monitors.argtypes = [vp, dp]
+        L.xo_sim_get_inductor_current.argtypes = [vp, i, dp]
+        L.xo_sim_get_lumped_wave.argtypes = [vp, i, dp]
         L.xo_sim_add_probe.argtypes = [vp, i, sz, sz, sz]
         L.xo_sim_init.argtypes = [vp, sz]
         L.xo_sim_step.argtypes = [vp, sz]
@@ -111,6 +117,10 @@ class OracleSim:
                     c = _arr3(sh.center)
                     self._chk(L.xo_sim_add_sphere(self.h, _dp(c), float(sh.radius),
                                                   m.eps_r, m.mu_r, m.sigma_e, m.sigma_m))
+                elif type(o).__name__ == "ObjectPlane":
+                    p, s = _arr3(sh.origin), _arr3(sh.size)
+                    self._chk(L.xo_sim_add_plane(self.h, _dp(p), _dp(s), m.eps_r, m.mu_r,
+                                                 m.sigma_e, m.sigma_m))
                 else:
                     p, s = _arr3(sh.origin), _arr3(sh.size)
                     self._chk(L.xo_sim_add_cube(self.h, _dp(p), _dp(s), m.eps_r, m.mu_r,
@@ -138,12 +148,22 @@ class OracleSim:
                 ph = _arr3(n.phi)
                 self._chk(L.xo_sim_set_nffft(self.h, dist, _dp(fr), len(fr), _dp(th), len(th),
                                              _dp(ph), len(ph)))
+            from xfdtd_amd.scene import lumped_desc
             for s in scene.sources:
                 p, sz = _arr3(s.cube.origin), _arr3(s.cube.size)
-                w = s.waveform
-                self._chk(L.xo_sim_add_current_source(self.h, _dp(p), _dp(sz), int(s.orientation),
-                                                      float(s.resistance), w.amplitude, w.tau,
-                                                      w.t_0))
+                kind, dr, val, wk, wp = lumped_desc(s)
+                wpa = _arr3(wp)
+                self._chk(L.xo_sim_add_lumped(self.h, kind, _dp(p), _dp(sz), int(dr), float(val), wk, _dp(wpa)))
+            self.line_monitors = []
+            for m in getattr(scene, "monitors", []):
+                kind = {"VoltageMonitor": 0, "CurrentMonitor": 1}.get(type(m).__name__)
+                if kind is None:
+                    continue
+                p, sz = _arr3(m.cube.origin), _arr3(m.cube.size)
+                r = L.xo_sim_add_line_monitor(self.h, kind, _dp(p), _dp(sz), int(m.orientation))
+                if r < 0:
+                    self._chk(r)
+                self.line_monitors.append(m)
             for p in scene.probes:
                 self._chk(L.xo_sim_add_probe(self.h, int(p.comp), p.i, p.j, p.k))
             dz = getattr(scene, "design", None)
@@ -250,6 +270,25 @@ class OracleSim:
         res["power_theta"] = pw[0].copy()
         res["power_phi"] = pw[1].copy()
         return res
+
+    # ---- voltage / current monitors and inductors
+    def get_line_monitors(self):
+        """[monitor][step] values of the scene's voltage and current monitors,
+        in the order they appear among the scene's monitors."""
+        out = np.empty((len(self.line_monitors), self.T))
+        if len(self.line_monitors):
+            self._chk(lib().xo_sim_get_line_monitors(self.h, _dp(out)))
+        return out
+
+    def get_lumped_wave(self, q):
+        out = np.empty(self.T)
+        self._chk(lib().xo_sim_get_lumped_wave(self.h, int(q), _dp(out)))
+        return out
+
+    def get_inductor_current(self, q, shape):
+        out = np.empty(int(np.prod(shape)))
+        self._chk(lib().xo_sim_get_inductor_current(self.h, int(q), _dp(out)))
+        return out.reshape(shape)
 
     # ---- broadband NF-FF (nffft/nffft_broadband.cpp)
     def bb_frequencies(self):

@@ -99,6 +99,7 @@ static void a3_free(a3 *a) {
 
 typedef struct {
   int kind; /* 0 cube, 1 sphere */
+  int plane; /* ObjectPlane: a zero-thickness cube that sets sigma_e only */
   double p[3], s[3], r;
   float eps_r, mu_r;
   double sigma_e, sigma_m;
@@ -147,13 +148,31 @@ typedef struct {
   double *samples; /* [T][8][Q]: step, current c of nf_off(), surface cell */
 } bb_t;
 
+/* lumped elements in insertion order (lumped_element/{current_source,voltage_source,resistor,capacitor,inductor}.cpp) */
+enum { LE_CURRENT = 0, LE_VOLTAGE = 1, LE_RESISTOR = 2, LE_CAPACITOR = 3, LE_INDUCTOR = 4 };
+/* waveforms: 0 Gaussian (amp, tau, t0), 1 sinusoid (amp, f, phase),
+   2 cosine-modulated Gaussian (amp, tau, t0, f) */
 typedef struct {
   double origin[3], size[3];
-  int ori;
-  double R, amp, tau, t0, rf;
+  int kind;
+  int ori;   /* orientation (sources) or 2 * axis (R, C, L) */
+  double R;  /* resistance, capacitance or inductance */
+  int wkind;
+  double amp, tau, t0, wp3;
+  double rf;
   size_t is, ie, js, je, ks, ke;
   double *wave, *coff_i;
+  double *jc, *cecjc, *cjcec; /* inductor: current and its two coefficients */
 } csrc_t;
+
+/* voltage (kind 0) and current (kind 1) monitors (monitor/voltage_monitor.cpp,
+   monitor/current_monitor.cpp) */
+typedef struct {
+  int kind, ori;
+  double origin[3], size[3];
+  size_t is, ie, js, je, ks, ke;
+  double *value; /* [T] */
+} lmon_t;
 
 typedef struct {
   int comp;
@@ -182,6 +201,8 @@ struct xo_sim {
   bb_t bb;
   csrc_t *src;
   int nsrc;
+  lmon_t *lmon;
+  int nlmon;
   probe_t *probes;
   int nprobe;
   double *probe_buf;
@@ -261,8 +282,13 @@ void xo_sim_destroy(xo_sim *s) {
   free(nf->freqs); free(nf->theta); free(nf->phi); free(nf->k);
   free(nf->acc); free(nf->tj); free(nf->tm);
   free(s->bb.samples);
-  for (int q = 0; q < s->nsrc; ++q) { free(s->src[q].wave); free(s->src[q].coff_i); }
+  for (int q = 0; q < s->nsrc; ++q) {
+    free(s->src[q].wave); free(s->src[q].coff_i);
+    free(s->src[q].jc); free(s->src[q].cecjc); free(s->src[q].cjcec);
+  }
   free(s->src);
+  for (int q = 0; q < s->nlmon; ++q) free(s->lmon[q].value);
+  free(s->lmon);
   free(s->probes);
   free(s->probe_buf);
   free(s->design_eps);
@@ -373,22 +399,55 @@ int xo_sim_set_nffft(xo_sim *s, const size_t distance[3], const double *freqs,
   return 0;
 }
 
-int xo_sim_add_current_source(xo_sim *s, const double origin[3],
-                              const double size[3], int orientation,
-                              double resistance, double amplitude, double tau,
-                              double t0) {
+int xo_sim_add_lumped(xo_sim *s, int kind, const double origin[3], const double size[3], int dir,
+                      double value, int wkind, const double wp[4]) {
   if (s->inited) FAIL("simulation already initialised");
+  if (kind < LE_CURRENT || kind > LE_INDUCTOR) FAIL("bad lumped element kind");
+  if (kind <= LE_VOLTAGE ? (dir < 0 || dir > 5) : (dir < 0 || dir > 2)) FAIL("bad orientation or axis");
+  if (wkind < 0 || wkind > 2) FAIL("bad waveform");
   csrc_t *n = (csrc_t *)realloc(s->src, (size_t)(s->nsrc + 1) * sizeof(csrc_t));
   if (!n) FAIL("out of memory");
   s->src = n;
   csrc_t *c = &s->src[s->nsrc++];
   memset(c, 0, sizeof *c);
   for (int a = 0; a < 3; ++a) { c->origin[a] = origin[a]; c->size[a] = size[a]; }
-  c->ori = orientation;
-  /* lumped_element/current_source.cpp:12-15 */
-  c->R = (resistance == 0) ? 1e-20 : resistance;
-  c->amp = amplitude; c->tau = tau; c->t0 = t0;
+  c->kind = kind;
+  c->ori = kind <= LE_VOLTAGE ? dir : 2 * dir;
+  /* current_source.cpp:12-15, voltage_source.cpp:19-22: R = 0 becomes 1e-20 */
+  c->R = (kind <= LE_VOLTAGE && value == 0) ? 1e-20 : value;
+  c->wkind = wkind;
+  if (wp) { c->amp = wp[0]; c->tau = wp[1]; c->t0 = wp[2]; c->wp3 = wp[3]; }
   return 0;
+}
+
+int xo_sim_add_current_source(xo_sim *s, const double origin[3],
+                              const double size[3], int orientation,
+                              double resistance, double amplitude, double tau,
+                              double t0) {
+  const double wp[4] = {amplitude, tau, t0, 0.0};
+  return xo_sim_add_lumped(s, LE_CURRENT, origin, size, orientation, resistance, 0, wp);
+}
+
+int xo_sim_add_plane(xo_sim *s, const double origin[3], const double size[3], float eps_r, float mu_r,
+                     double sigma_e, double sigma_m) {
+  if (xo_sim_add_cube(s, origin, size, eps_r, mu_r, sigma_e, sigma_m)) return -1;
+  s->objs[s->nobj - 1].plane = 1;
+  return 0;
+}
+
+int xo_sim_add_line_monitor(xo_sim *s, int kind, const double origin[3], const double size[3],
+                            int orientation) {
+  if (s->inited) FAIL("simulation already initialised");
+  if (kind < 0 || kind > 1 || orientation < 0 || orientation > 5) FAIL("bad monitor");
+  lmon_t *n = (lmon_t *)realloc(s->lmon, (size_t)(s->nlmon + 1) * sizeof(lmon_t));
+  if (!n) FAIL("out of memory");
+  s->lmon = n;
+  lmon_t *m = &s->lmon[s->nlmon++];
+  memset(m, 0, sizeof *m);
+  m->kind = kind;
+  m->ori = orientation;
+  for (int a = 0; a < 3; ++a) { m->origin[a] = origin[a]; m->size[a] = size[a]; }
+  return s->nlmon - 1;
 }
 
 int xo_sim_add_probe(xo_sim *s, int comp, size_t i, size_t j, size_t k) {
@@ -1300,45 +1359,105 @@ static void update_e_drude(xo_sim *s) {
 
 /* --------------------------------------------------------- current source */
 
+static double lumped_wave(const csrc_t *c, double t) {
+  if (c->wkind == 1) /* waveform/sinusoidal_waveform.cpp:18-21 */
+    return c->amp * sin(2 * XO_PI * c->tau * t + c->t0);
+  if (c->wkind == 2) /* waveform/cosine_modulated_gaussian_waveform.cpp:29-33 */
+    return c->amp * cos(2 * XO_PI * c->wp3 * (t - c->t0)) * exp(-pow((t - c->t0) / c->tau, 2));
+  return gaussian(c->amp, c->tau, c->t0, t);
+}
+
+/* cell sizes of a lumped element's cell along its axis a: (da, db, dc) of
+   the lumped elements' init (da, db: the H-node sizes across, dc: the
+   E-node size along).  The reference's Y branches of Resistor, Capacitor
+   and Inductor assign _da twice and never _db; the intended (dz, dx) is
+   taken, as for CurrentSource. */
+static void lumped_sizes(const xo_sim *s, int a, size_t i, size_t j, size_t k, double *da, double *db,
+                         double *dc) {
+  if (a == 0) { *da = s->hs[1][j]; *db = s->hs[2][k]; *dc = s->es[0][i]; }
+  else if (a == 1) { *da = s->hs[2][k]; *db = s->hs[0][i]; *dc = s->es[1][j]; }
+  else { *da = s->hs[0][i]; *db = s->hs[1][j]; *dc = s->es[2][k]; }
+}
+
 static int init_csrc(xo_sim *s, csrc_t *c) {
-  /* lumped_element/current_source.cpp:34-116 */
+  /* the grid box of the element's cube */
   size_t st[3], en[3];
   for (int a = 0; a < 3; ++a) {
     double lo = c->origin[a], hi = c->origin[a] + c->size[a];
     if (transform(s, a, lo, &st[a]) || transform(s, a, hi, &en[a])) return -1;
     if (en[a] < st[a]) FAIL("end < start");
+    if (en[a] == st[a]) FAIL("lumped element box has no cells");
   }
   c->is = st[0]; c->ie = en[0]; c->js = st[1]; c->je = en[1]; c->ks = st[2]; c->ke = en[2];
   int a = c->ori / 2;
-  size_t ni = c->ie - c->is, nj = c->je - c->js, nk = c->ke - c->ks;
-  double rf, af;
-  if (a == 0) {
-    rf = c->R * (double)nk * (double)nj / (double)ni;
-    af = c->amp / (double)(nk * nj);
-  } else if (a == 1) {
-    rf = c->R * (double)nk * (double)ni / (double)nj;
-    af = c->amp / (double)(nk * ni);
+  size_t ni = c->ie - c->is, nj = c->je - c->js, nk = c->ke - c->ks, ncell = ni * nj * nk;
+  double rf = 0, af = c->amp;
+  if (c->kind == LE_CURRENT) {
+    /* current_source.cpp:34-116 */
+    if (a == 0) {
+      rf = c->R * (double)nk * (double)nj / (double)ni;
+      af = c->amp / (double)(nk * nj);
+    } else if (a == 1) {
+      rf = c->R * (double)nk * (double)ni / (double)nj;
+      af = c->amp / (double)(nk * ni);
+    } else {
+      rf = c->R * (double)ni * (double)nj / (double)nk;
+      af = c->amp / (double)(ni * nj);
+    }
+    if (c->ori % 2 == 0) af *= -1;
+  } else if (c->kind == LE_VOLTAGE) {
+    /* voltage_source.cpp:40-107 (its YP branch never runs: the condition
+       tests YN twice; taken as intended) */
+    if (a == 0) {
+      rf = c->R * (double)nj * (double)nk / (double)ni;
+      af = c->amp / (double)ni;
+    } else if (a == 1) {
+      rf = c->R * (double)ni * (double)nk / (double)nj;
+      af = c->amp / (double)nj;
+    } else {
+      rf = c->R * (double)ni * (double)nj / (double)nk;
+      af = c->amp / (double)nk;
+    }
+    if (c->ori % 2 == 0) af *= -1;
+  } else if (c->kind == LE_RESISTOR || c->kind == LE_INDUCTOR) {
+    /* resistor.cpp:22-76, inductor.cpp:19-68 */
+    if (a == 0) rf = c->R * (double)nj * (double)nk / (double)ni;
+    else if (a == 1) rf = c->R * (double)ni * (double)nk / (double)nj;
+    else rf = c->R * (double)ni * (double)nj / (double)nk;
   } else {
-    rf = c->R * (double)ni * (double)nj / (double)nk;
-    af = c->amp / (double)(ni * nj);
+    /* capacitor.cpp:25-61 */
+    if (a == 0) rf = c->R * (double)ni / (double)(nj * nk);
+    else if (a == 1) rf = c->R * (double)nj / (double)(ni * nk);
+    else rf = c->R * (double)nk / (double)(ni * nj);
   }
-  if (c->ori % 2 == 0) af *= -1;
-  c->amp = af; /* setAmplitude(_current_amplitude_factor) */
   c->rf = rf;
-  size_t T = s->T;
-  c->wave = (double *)malloc(sizeof(double) * (T ? T : 1));
-  double start = 0.5 * s->dt, stop = ((double)T - 0.5) * s->dt;
-  double step = (stop - start) / fmax(1.0, (double)T - 1.0);
-  for (size_t n = 0; n < T; ++n) {
-    double tt = (n + 1 == T && T > 1) ? stop : start + step * (double)n;
-    c->wave[n] = gaussian(c->amp, c->tau, c->t0, tt);
+  if (c->kind <= LE_VOLTAGE) {
+    c->amp = af; /* setAmplitude(factor) before init */
+    size_t T = s->T;
+    c->wave = (double *)malloc(sizeof(double) * (T ? T : 1));
+    if (!c->wave) FAIL("out of memory");
+    double start = 0.5 * s->dt, stop = ((double)T - 0.5) * s->dt;
+    double step = (stop - start) / fmax(1.0, (double)T - 1.0);
+    for (size_t n = 0; n < T; ++n) {
+      double tt = (n + 1 == T && T > 1) ? stop : start + step * (double)n;
+      c->wave[n] = lumped_wave(c, tt);
+    }
+    c->coff_i = (double *)calloc(ncell ? ncell : 1, sizeof(double));
+    if (!c->coff_i) FAIL("out of memory");
   }
-  c->coff_i = (double *)calloc((ni * nj * nk) > 0 ? (ni * nj * nk) : 1, sizeof(double));
+  if (c->kind == LE_INDUCTOR) {
+    c->jc = (double *)calloc(ncell ? ncell : 1, sizeof(double));
+    c->cecjc = (double *)calloc(ncell ? ncell : 1, sizeof(double));
+    c->cjcec = (double *)calloc(ncell ? ncell : 1, sizeof(double));
+    if (!c->jc || !c->cecjc || !c->cjcec) FAIL("out of memory");
+  }
   return 0;
 }
 
 static void csrc_correct_coff(xo_sim *s, csrc_t *c, a3 *eps_arr, a3 *sig_arr) {
-  /* lumped_element/current_source.cpp:118-138, 167-187 */
+  /* correctFDTDCoff of current_source.cpp:118-138, 167-187,
+     voltage_source.cpp:109-166, resistor.cpp:78-121, capacitor.cpp:63-122,
+     inductor.cpp:70-113 */
   int a = c->ori / 2;
   double rf = c->rf;
   double dt = s->dt;
@@ -1351,29 +1470,147 @@ static void csrc_correct_coff(xo_sim *s, csrc_t *c, a3 *eps_arr, a3 *sig_arr) {
     for (size_t j = c->js; j < c->je; ++j)
       for (size_t k = c->ks; k < c->ke; ++k) {
         double da, db, dc;
-        if (a == 0) { da = s->hs[1][j]; db = s->hs[2][k]; dc = s->es[0][i]; }
-        else if (a == 1) { da = s->hs[2][k]; db = s->hs[0][i]; dc = s->es[1][j]; }
-        else { da = s->hs[0][i]; db = s->hs[1][j]; dc = s->es[2][k]; }
-        double beta = (dt * dc) / (rf * da * db);
+        lumped_sizes(s, a, i, j, k, &da, &db, &dc);
         double e = AT(*eps_arr, i, j, k), sg = AT(*sig_arr, i, j, k);
+        size_t q = ((i - c->is) * nj + (j - c->js)) * nk + (k - c->ks);
+        if (c->kind == LE_INDUCTOR) {
+          c->cjcec[q] = dt * dc / (rf * da * db);
+          c->cecjc[q] = -(2 * dt) / (2 * e + dt * sg);
+          continue;
+        }
+        double beta;
+        if (c->kind == LE_VOLTAGE) {
+          double alpha = da * db * rf;
+          beta = dt * dc / alpha;
+          c->coff_i[q] = -2 * dt / ((2 * e + dt * sg + beta) * alpha);
+        } else if (c->kind == LE_CAPACITOR) {
+          beta = 2 * rf * dc / (da * db);
+          AT(s->c[ce], i, j, k) = (2 * e - dt * sg + beta) / (2 * e + dt * sg + beta);
+          AT(s->c[ca], i, j, k) = -(2 * dt) / ((2 * e + dt * sg + beta) * db);
+          AT(s->c[cb], i, j, k) = (2 * dt) / ((2 * e + dt * sg + beta) * da);
+          continue;
+        } else {
+          beta = (dt * dc) / (rf * da * db);
+          if (c->kind == LE_CURRENT) c->coff_i[q] = -2 * dt / ((2 * e + dt * sg + beta) * (db * da));
+        }
         AT(s->c[ce], i, j, k) = (2 * e - dt * sg - beta) / (2 * e + dt * sg + beta);
         AT(s->c[ca], i, j, k) = -2 * dt / ((2 * e + dt * sg + beta) * db);
         AT(s->c[cb], i, j, k) = 2 * dt / ((2 * e + dt * sg + beta) * da);
-        c->coff_i[((i - c->is) * nj + (j - c->js)) * nk + (k - c->ks)] =
-            -2 * dt / ((2 * e + dt * sg + beta) * (db * da));
       }
 }
 
 static void csrc_correct_e(xo_sim *s, csrc_t *c) {
-  /* lumped_element/current_source.cpp:140-163 */
+  /* correctE of current_source.cpp:140-163, voltage_source.cpp:168-193,
+     inductor.cpp:115-146 (resistor and capacitor: none) */
+  if (c->kind == LE_RESISTOR || c->kind == LE_CAPACITOR) return;
   int a = c->ori / 2;
   a3 *f = &s->f[a == 0 ? XO_EX : (a == 1 ? XO_EY : XO_EZ)];
-  double w = c->wave[s->cur];
   size_t nj = c->je - c->js, nk = c->ke - c->ks;
+  if (c->kind == LE_INDUCTOR) {
+    for (size_t i = c->is; i < c->ie; ++i)
+      for (size_t j = c->js; j < c->je; ++j)
+        for (size_t k = c->ks; k < c->ke; ++k) {
+          size_t q = ((i - c->is) * nj + (j - c->js)) * nk + (k - c->ks);
+          AT(*f, i, j, k) += c->cecjc[q] * c->jc[q];
+          c->jc[q] = c->cjcec[q] * AT(*f, i, j, k) + c->jc[q];
+        }
+    return;
+  }
+  double w = c->wave[s->cur];
   for (size_t i = c->is; i < c->ie; ++i)
     for (size_t j = c->js; j < c->je; ++j)
       for (size_t k = c->ks; k < c->ke; ++k)
         AT(*f, i, j, k) += c->coff_i[((i - c->is) * nj + (j - c->js)) * nk + (k - c->ks)] * w;
+}
+
+/* ------------------------------------------------------- line monitors */
+
+static int init_lmon(xo_sim *s, lmon_t *m) {
+  size_t st[3], en[3];
+  for (int a = 0; a < 3; ++a) {
+    double lo = m->origin[a], hi = m->origin[a] + m->size[a];
+    if (transform(s, a, lo, &st[a]) || transform(s, a, hi, &en[a])) return -1;
+    if (en[a] < st[a]) FAIL("end < start");
+  }
+  m->is = st[0]; m->ie = en[0]; m->js = st[1]; m->je = en[1]; m->ks = st[2]; m->ke = en[2];
+  int a = m->ori / 2;
+  size_t sh[3];
+  if (m->kind == 0) {
+    /* voltage_monitor.cpp:20-61: E_a over the box */
+    field_shape(s, a, sh);
+    if (m->ie > sh[0] || m->je > sh[1] || m->ke > sh[2]) FAIL("voltage monitor outside the field array");
+  } else {
+    /* current_monitor.cpp:20-64: the loop reads H one cell outside the box
+       on both sides across the axis */
+    const size_t *lo = &st[0], *hi = &en[0];
+    if (lo[a] >= s->n[a]) FAIL("current monitor loop outside the grid");
+    for (int b = 0; b < 3; ++b)
+      if (b != a && (lo[b] < 1 || hi[b] + 1 > s->n[b])) FAIL("current monitor loop outside the grid");
+  }
+  m->value = (double *)calloc(s->T ? s->T : 1, sizeof(double));
+  if (!m->value) FAIL("out of memory");
+  return 0;
+}
+
+static void lmon_update(xo_sim *s, lmon_t *m) {
+  size_t t = s->cur;
+  int a = m->ori / 2;
+  if (m->kind == 0) {
+    /* voltage_monitor.cpp:63-93: value(t) += coff(c) E(i,j,k), with
+       coff = -dc / (cells across), negated for the N orientation */
+    size_t ni = m->ie - m->is, nj = m->je - m->js, nk = m->ke - m->ks;
+    size_t across = a == 0 ? nj * nk : (a == 1 ? ni * nk : ni * nj);
+    a3 *f = &s->f[a];
+    double v = 0.0;
+    for (size_t i = m->is; i < m->ie; ++i)
+      for (size_t j = m->js; j < m->je; ++j)
+        for (size_t k = m->ks; k < m->ke; ++k) {
+          size_t idx = a == 0 ? i : (a == 1 ? j : k);
+          double coff = -s->hs[a][idx] / (double)across;
+          if (m->ori % 2 == 0) coff *= -1;
+          v += coff * AT(*f, i, j, k);
+        }
+    m->value[t] = v;
+    return;
+  }
+  /* current_monitor.cpp:66-122 */
+  const double pos = m->ori % 2 == 0 ? -1.0 : 1.0;
+  size_t is = m->is, ie = m->ie, js = m->js, je = m->je, ks = m->ks, ke = m->ke;
+  double i1 = 0, i2 = 0;
+  if (a == 0) {
+    for (size_t j = js; j <= je; ++j) i1 += (HY(is, j, ks - 1) - HY(is, j, ke)) * s->es[1][j];
+    for (size_t k = ks; k <= ke; ++k) i2 += (HZ(is, je, k) - HZ(is, js - 1, k)) * s->es[2][k];
+    m->value[t] = pos * (i2 + i1);
+  } else if (a == 1) {
+    for (size_t k = ks; k <= ke; ++k) i1 += (HZ(is - 1, js, k) - HZ(ie, js, k)) * s->es[2][k];
+    for (size_t i = is; i <= ie; ++i) i2 += (HX(i, js, ke) - HX(i, js, ks - 1)) * s->es[0][i];
+    m->value[t] = pos * (i1 + i2);
+  } else {
+    for (size_t i = is; i <= ie; ++i) i1 += (HX(i, js - 1, ks) - HX(i, je, ks)) * s->es[0][i];
+    for (size_t j = js; j <= je; ++j) i2 += (HY(ie, j, ks) - HY(is - 1, j, ks)) * s->es[1][j];
+    m->value[t] = pos * (i1 + i2);
+  }
+}
+
+int xo_sim_get_line_monitors(const xo_sim *s, double *out) {
+  if (!s->inited) FAIL("not initialised");
+  for (int q = 0; q < s->nlmon; ++q) memcpy(out + (size_t)q * s->T, s->lmon[q].value, s->T * sizeof(double));
+  return 0;
+}
+
+/* source waveform table of lumped element q (T values at (n + 1/2) dt) */
+int xo_sim_get_lumped_wave(const xo_sim *s, int q, double *out) {
+  if (!s->inited || q < 0 || q >= s->nsrc || !s->src[q].wave) FAIL("no such source");
+  memcpy(out, s->src[q].wave, s->T * sizeof(double));
+  return 0;
+}
+
+/* inductor currents of lumped element q (its box, C order) */
+int xo_sim_get_inductor_current(const xo_sim *s, int q, double *out) {
+  if (!s->inited || q < 0 || q >= s->nsrc || s->src[q].kind != LE_INDUCTOR) FAIL("no such inductor");
+  const csrc_t *c = &s->src[q];
+  memcpy(out, c->jc, (c->ie - c->is) * (c->je - c->js) * (c->ke - c->ks) * sizeof(double));
+  return 0;
 }
 
 /* ------------------------------------------------------------------ init */
@@ -1412,6 +1649,31 @@ int xo_sim_init(xo_sim *s, size_t total_steps) {
   for (int q = 0; q < s->nobj; ++q) {
     const obj_t *o = &s->objs[q];
     double e = (double)o->eps_r * EPS0, m = (double)o->mu_r * MU0();
+    if (o->plane) {
+      /* object/object_plane.cpp:8-50: sigma_e of the two tangential E
+         components on the plane, nothing else */
+      size_t st[3], en[3];
+      for (int a = 0; a < 3; ++a)
+        if (transform(s, a, o->p[a], &st[a]) || transform(s, a, o->p[a] + o->s[a], &en[a])) return -1;
+      size_t is = st[0], ie = en[0], js = st[1], je = en[1], ks = st[2], ke = en[2];
+      if (is == ie) {
+        for (size_t j = js; j < je; ++j)
+          for (size_t k = ks; k <= ke; ++k) AT(sg[1], is, j, k) = o->sigma_e;
+        for (size_t j = js; j <= je; ++j)
+          for (size_t k = ks; k < ke; ++k) AT(sg[2], is, j, k) = o->sigma_e;
+      } else if (js == je) {
+        for (size_t i = is; i < ie; ++i)
+          for (size_t k = ks; k <= ke; ++k) AT(sg[0], i, js, k) = o->sigma_e;
+        for (size_t i = is; i <= ie; ++i)
+          for (size_t k = ks; k < ke; ++k) AT(sg[2], i, js, k) = o->sigma_e;
+      } else if (ks == ke) {
+        for (size_t i = is; i < ie; ++i)
+          for (size_t j = js; j <= je; ++j) AT(sg[0], i, j, ks) = o->sigma_e;
+        for (size_t i = is; i <= ie; ++i)
+          for (size_t j = js; j < je; ++j) AT(sg[1], i, j, ks) = o->sigma_e;
+      }
+      continue;
+    }
     for (size_t i = 0; i < nx; ++i)
       for (size_t j = 0; j < ny; ++j)
         for (size_t k = 0; k < nz; ++k) {
@@ -1456,6 +1718,8 @@ int xo_sim_init(xo_sim *s, size_t total_steps) {
   if (s->has_bb && init_bb(s)) return -1;
   for (int q = 0; q < s->nsrc; ++q)
     if (init_csrc(s, &s->src[q])) return -1;
+  for (int q = 0; q < s->nlmon; ++q)
+    if (init_lmon(s, &s->lmon[q])) return -1;
   /* fdtd_basic_coff.cpp:31-122 */
   for (int q = 0; q < 18; ++q) {
     size_t sh[3];
@@ -1593,6 +1857,7 @@ int xo_sim_step(xo_sim *s, size_t nsteps) {
       const probe_t *pr = &s->probes[p];
       s->probe_buf[(size_t)p * s->T + s->cur] = AT(s->f[pr->comp], pr->i, pr->j, pr->k);
     }
+    for (int m = 0; m < s->nlmon; ++m) lmon_update(s, &s->lmon[m]);
     s->cur++;
   }
   return 0;

@@ -61,6 +61,25 @@ int xo_sim_add_current_source(xo_sim *s, const double origin[3],
                               const double size[3], int orientation,
                               double resistance, double amplitude, double tau,
                               double t0);
+/* Lumped element (lumped_element/{current_source,voltage_source,resistor,capacitor,inductor}.cpp) in insertion order: kind 0
+   CurrentSource, 1 VoltageSource (dir = orientation 0..5, value =
+   resistance), 2 Resistor, 3 Capacitor, 4 Inductor (dir = axis 0..2, value =
+   R, C or L).  Sources carry a waveform: wkind 0 Gaussian (amp, tau, t0),
+   1 sinusoid (amp, frequency, phase), 2 cosine-modulated Gaussian (amp, tau,
+   t0, frequency) in wp[0..3]. */
+int xo_sim_add_lumped(xo_sim *s, int kind, const double origin[3], const double size[3], int dir,
+                      double value, int wkind, const double wp[4]);
+/* ObjectPlane(Cube(origin, size) with one zero size, material): sets the
+   tangential sigma_e on the plane only (object/object_plane.cpp:8-50) */
+int xo_sim_add_plane(xo_sim *s, const double origin[3], const double size[3], float eps_r, float mu_r,
+                     double sigma_e, double sigma_m);
+/* VoltageMonitor (kind 0) / CurrentMonitor (kind 1) over Cube(origin, size)
+   with an orientation; returns the monitor's index */
+int xo_sim_add_line_monitor(xo_sim *s, int kind, const double origin[3], const double size[3],
+                            int orientation);
+int xo_sim_get_line_monitors(const xo_sim *s, double *out /* nmon * total_steps */);
+int xo_sim_get_inductor_current(const xo_sim *s, int q, double *out);
+int xo_sim_get_lumped_wave(const xo_sim *s, int q, double *out /* total_steps */);
 /* point probe sampled after every step (monitor update slot) */
 int xo_sim_add_probe(xo_sim *s, int comp, size_t i, size_t j, size_t k);
 

@@ -18,6 +18,21 @@ Each class carries the constructor arguments of its reference counterpart:
                                           nffft/nffft_broadband.cpp:21-28
   CurrentSource(cube, orientation, resistance, waveform)
                                           lumped_element/current_source.cpp:5-16
+  VoltageSource(cube, orientation, resistance, waveform)
+                                          lumped_element/voltage_source.cpp:12-22
+  Resistor / Capacitor / Inductor(cube, axis, value)
+                                          lumped_element/resistor.cpp:7-8, capacitor.cpp:4-6,
+                                          inductor.cpp:4-5
+  ObjectPlane(name, shape, material)      object/object_plane.cpp:4-6
+  VoltageMonitor / CurrentMonitor(cube, orientation, output_dir, output_file_name)
+                                          monitor/voltage_monitor.cpp:11-18,
+                                          monitor/current_monitor.cpp:11-18
+  SinusoidalWaveform(amplitude, frequency, phase)
+                                          waveform/sinusoidal_waveform.cpp:6-8
+  CosineModulatedGaussianWaveform(amplitude, tau, t_0, modulation_frequency)
+                                          waveform/cosine_modulated_gaussian_waveform.cpp:6-11
+  Port(index, impedance, is_source, voltage, current), Network(ports, frequencies, output_path)
+                                          network/port.cpp:6-13, network/network.cpp:8-11
   Simulation(cell_size, cfl=0.99f)        include/simulation/simulation.h:25-27
 
 A `Scene` bundles them the way `Simulation::add*` does.  Both the HIP engine
@@ -157,11 +172,125 @@ class NffftBroadBand:
 
 
 @dataclass
+class SinusoidalWaveform:
+    amplitude: float
+    frequency: float
+    phase: float = 0.0
+
+
+@dataclass
+class CosineModulatedGaussianWaveform:
+    amplitude: float
+    tau: float
+    t_0: float
+    modulation_frequency: float
+
+
+# include/util/type_define.h Axis
+AX_X, AX_Y, AX_Z = 0, 1, 2
+
+
+@dataclass
 class CurrentSource:
     cube: Cube
     orientation: int
     resistance: float
-    waveform: GaussianWaveform
+    waveform: object  # GaussianWaveform, SinusoidalWaveform or CosineModulatedGaussianWaveform
+
+
+@dataclass
+class VoltageSource:
+    cube: Cube
+    orientation: int
+    resistance: float
+    waveform: object
+
+
+@dataclass
+class Resistor:
+    cube: Cube
+    axis: int
+    resistance: float
+
+
+@dataclass
+class Capacitor:
+    cube: Cube
+    axis: int
+    capacitance: float
+
+
+@dataclass
+class Inductor:
+    cube: Cube
+    axis: int
+    inductance: float
+
+
+@dataclass
+class ObjectPlane:
+    """An Object whose cube has zero size along one axis: it sets sigma_e of
+    the two tangential E components on that plane and nothing else."""
+    name: str
+    shape: Cube
+    material: object
+
+
+@dataclass
+class VoltageMonitor:
+    cube: Cube
+    orientation: int
+    output_dir: str = ""
+    output_file_name: str = ""
+
+
+@dataclass
+class CurrentMonitor:
+    cube: Cube
+    orientation: int
+    output_dir: str = ""
+    output_file_name: str = ""
+
+
+@dataclass
+class Port:
+    index: int
+    impedance: complex
+    is_source: bool
+    voltage: VoltageMonitor
+    current: CurrentMonitor
+
+
+@dataclass
+class Network:
+    ports: List[Port]
+    frequencies: Sequence[float]
+    output_path: str = ""
+
+
+def lumped_desc(le):
+    """(kind, dir, value, waveform kind, waveform params) of a lumped element:
+    kind 0 CurrentSource, 1 VoltageSource, 2 Resistor, 3 Capacitor, 4 Inductor;
+    waveform kind 0 Gaussian (amplitude, tau, t_0), 1 sinusoid (amplitude,
+    frequency, phase), 2 cosine-modulated Gaussian (amplitude, tau, t_0,
+    frequency)."""
+    def wave(w):
+        if isinstance(w, SinusoidalWaveform):
+            return 1, (w.amplitude, w.frequency, w.phase, 0.0)
+        if isinstance(w, CosineModulatedGaussianWaveform):
+            return 2, (w.amplitude, w.tau, w.t_0, w.modulation_frequency)
+        return 0, (w.amplitude, w.tau, w.t_0, 0.0)
+    if isinstance(le, CurrentSource):
+        return (0, le.orientation, le.resistance) + wave(le.waveform)
+    if isinstance(le, VoltageSource):
+        return (1, le.orientation, le.resistance) + wave(le.waveform)
+    if isinstance(le, Resistor):
+        return (2, le.axis, le.resistance, 0, (0.0,) * 4)
+    if isinstance(le, Capacitor):
+        return (3, le.axis, le.capacitance, 0, (0.0,) * 4)
+    if isinstance(le, Inductor):
+        return (4, le.axis, le.inductance, 0, (0.0,) * 4)
+    raise TypeError(f"not a lumped element: {le!r}")
 
 
 @dataclass
@@ -234,9 +363,10 @@ class Scene:
     boundaries: List[PML] = field(default_factory=list)
     tfsf: Optional[TFSF3D] = None
     nffft: Optional[object] = None  # NffftFd or NffftBroadBand
-    sources: List[CurrentSource] = field(default_factory=list)
+    sources: List[object] = field(default_factory=list)  # lumped elements, in insertion order
     probes: List[Probe] = field(default_factory=list)
-    monitors: List[object] = field(default_factory=list)  # TimeDomainFieldMonitor / MovieMonitor
+    monitors: List[object] = field(default_factory=list)  # field, movie, voltage and current monitors
+    network: Optional[Network] = None
     design: Optional[Design] = None
     dft_plane: Optional[DftPlane] = None
 
