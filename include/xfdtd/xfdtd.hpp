@@ -24,6 +24,7 @@
 #include <array>
 #include <cmath>
 #include <complex>
+#include <map>
 #include <cstddef>
 #include <cstdint>
 #include <limits>
@@ -156,6 +157,7 @@ class Object {
       : _name{name}, _shape{std::move(shape)}, _material{std::make_unique<Material>(material)} {}
   Object(std::string_view name, std::unique_ptr<Shape> shape, std::unique_ptr<Material> material)
       : _name{name}, _shape{std::move(shape)}, _material{std::move(material)} {}
+  virtual ~Object() = default;
   std::unique_ptr<Shape> getWrappedBox() const { return _shape->getWrappedBox(); }
   const Shape *getShape() const { return _shape.get(); }
   const Material &getMaterial() const { return *_material; }
@@ -165,6 +167,17 @@ class Object {
   std::string _name;
   std::unique_ptr<Shape> _shape;
   std::unique_ptr<Material> _material;
+};
+
+// object/object_plane.cpp:4-50: a cube with zero size along one axis that
+// sets sigma_e of the two tangential E components on that plane (and
+// nothing else) in place of the cell-centre material of an Object
+class ObjectPlane : public Object {
+ public:
+  ObjectPlane(std::string_view name, std::unique_ptr<Shape> shape, std::unique_ptr<Material> material)
+      : Object(name, std::move(shape), std::move(material)) {}
+  ObjectPlane(std::string_view name, std::unique_ptr<Shape> shape, Material material)
+      : Object(name, std::move(shape), std::move(material)) {}
 };
 
 // --------------------------------------------------------------- waveform
@@ -195,6 +208,36 @@ class GaussianWaveform : public Waveform {
 
  private:
   double _tau, _t_0;
+};
+
+// waveform/sinusoidal_waveform.cpp: amplitude sin(2 pi f t + phase)
+class SinusoidalWaveform : public Waveform {
+ public:
+  SinusoidalWaveform(double amplitude, double frequency, double phase)
+      : Waveform{amplitude}, _frequency{frequency}, _phase{phase} {}
+  SinusoidalWaveform(double frequency, double phase) : _frequency{frequency}, _phase{phase} {}
+  std::unique_ptr<Waveform> clone() const override { return std::make_unique<SinusoidalWaveform>(*this); }
+  double getValueByTime(double time) const override;
+
+ private:
+  double _frequency, _phase;
+};
+
+// waveform/cosine_modulated_gaussian_waveform.cpp:
+// amplitude cos(2 pi f (t - t_0)) exp(-((t - t_0) / tau)^2)
+class CosineModulatedGaussianWaveform : public Waveform {
+ public:
+  CosineModulatedGaussianWaveform(double amplitude, double tau, double t_0, double modulation_frequency)
+      : Waveform{amplitude}, _tau{tau}, _t_0{t_0}, _f{modulation_frequency} {}
+  CosineModulatedGaussianWaveform(double tau, double t_0, double modulation_frequency)
+      : _tau{tau}, _t_0{t_0}, _f{modulation_frequency} {}
+  std::unique_ptr<Waveform> clone() const override {
+    return std::make_unique<CosineModulatedGaussianWaveform>(*this);
+  }
+  double getValueByTime(double time) const override;
+
+ private:
+  double _tau, _t_0, _f;
 };
 
 // --------------------------------------------------------------- boundary
@@ -358,6 +401,62 @@ class CurrentSource : public LumpedElement {
   friend class Simulation;
 };
 
+// lumped_element/voltage_source.cpp:12-22: a voltage source of the given
+// internal resistance (0 becomes 1e-20), spread over the cells along its axis
+class VoltageSource : public LumpedElement {
+ public:
+  VoltageSource(std::unique_ptr<Cube> cube, Orientation orientation, double resistance,
+                std::unique_ptr<Waveform> waveform)
+      : LumpedElement{std::move(cube)}, _orientation{orientation},
+        _resistance{resistance == 0 ? 1e-20 : resistance}, _waveform{std::move(waveform)} {}
+  Orientation getOrientation() const { return _orientation; }
+  double getResistance() const { return _resistance; }
+  Waveform &waveform() { return *_waveform; }
+
+ private:
+  Orientation _orientation;
+  double _resistance;
+  std::unique_ptr<Waveform> _waveform;
+};
+
+// lumped_element/resistor.cpp, capacitor.cpp, inductor.cpp: a resistance,
+// capacitance or inductance along one axis of the cube's cells
+class Resistor : public LumpedElement {
+ public:
+  Resistor(std::unique_ptr<Cube> cube, Axis axis, double resistance)
+      : LumpedElement{std::move(cube)}, _axis{axis}, _resistance{resistance} {}
+  Axis getAxis() const { return _axis; }
+  double getResistance() const { return _resistance; }
+
+ private:
+  Axis _axis;
+  double _resistance;
+};
+
+class Capacitor : public LumpedElement {
+ public:
+  Capacitor(std::unique_ptr<Cube> cube, Axis axis, double capacitance)
+      : LumpedElement{std::move(cube)}, _axis{axis}, _capacitance{capacitance} {}
+  Axis getAxis() const { return _axis; }
+  double getCapacitance() const { return _capacitance; }
+
+ private:
+  Axis _axis;
+  double _capacitance;
+};
+
+class Inductor : public LumpedElement {
+ public:
+  Inductor(std::unique_ptr<Cube> cube, Axis axis, double inductance)
+      : LumpedElement{std::move(cube)}, _axis{axis}, _inductance{inductance} {}
+  Axis getAxis() const { return _axis; }
+  double getInductance() const { return _inductance; }
+
+ private:
+  Axis _axis;
+  double _inductance;
+};
+
 // ---------------------------------------------------------------- monitor
 // Point sample of one component after every step (the monitor slot of the
 // run loop, simulation_update.cpp:215-219).
@@ -421,6 +520,92 @@ class MovieMonitor : public Monitor {
   size_t _total_time_steps, _plot_step;
 };
 
+// monitor/voltage_monitor.cpp, monitor/current_monitor.cpp: the voltage
+// along the orientation's axis over the cube's grid box, or the current
+// through it (the loop integral of H around the box at its start plane),
+// one value per step; outputData writes [time, value] (2 x T, float64)
+class VoltageMonitor : public Monitor {
+ public:
+  VoltageMonitor(std::unique_ptr<Cube> shape, Orientation orientation, std::string output_dir_path,
+                 std::string output_file_name)
+      : Monitor(std::move(shape), std::move(output_dir_path), std::move(output_file_name)),
+        _orientation{orientation} {}
+  Orientation getOrientation() const { return _orientation; }
+  double getValue(size_t i) const { return _value.at(i); }
+  const std::vector<double> &getValueArray() const { return _value; }
+
+ private:
+  Orientation _orientation;
+  std::vector<double> _value;
+  friend class Simulation;
+};
+
+class CurrentMonitor : public Monitor {
+ public:
+  CurrentMonitor(std::unique_ptr<Cube> shape, Orientation orientation, std::string output_dir_path,
+                 std::string output_file_name)
+      : Monitor(std::move(shape), std::move(output_dir_path), std::move(output_file_name)),
+        _orientation{orientation} {}
+  Orientation getOrientation() const { return _orientation; }
+  double getValue(size_t i) const { return _value.at(i); }
+  const std::vector<double> &getValueArray() const { return _value; }
+
+ private:
+  Orientation _orientation;
+  std::vector<double> _value;
+  friend class Simulation;
+};
+
+// ---------------------------------------------------------------- network
+// network/port.cpp: a port of reference impedance z measured by a voltage
+// and a current monitor; a = (V + z I) / (2 sqrt(Re z)), b = (V - conj(z) I)
+// / (2 sqrt(Re z)) of the monitors' DFTs (util/dft.h)
+class Port {
+ public:
+  Port(int index, std::complex<double> impedance, bool is_activated_source, std::shared_ptr<VoltageMonitor> voltage,
+       std::shared_ptr<CurrentMonitor> current)
+      : _index{index}, _impedance{impedance}, _is_source{is_activated_source}, _voltage{std::move(voltage)},
+        _current{std::move(current)} {}
+  void calculateSParameter(const std::vector<double> &frequencies, double dt);
+  int getPortIndex() const { return _index; }
+  bool isActivatedSource() const { return _is_source; }
+  const std::vector<std::complex<double>> &getA() const { return _a; }
+  const std::vector<std::complex<double>> &getB() const { return _b; }
+  const std::shared_ptr<VoltageMonitor> &voltageMonitor() const { return _voltage; }
+  const std::shared_ptr<CurrentMonitor> &currentMonitor() const { return _current; }
+
+ private:
+  int _index;
+  std::complex<double> _impedance;
+  bool _is_source;
+  std::shared_ptr<VoltageMonitor> _voltage;
+  std::shared_ptr<CurrentMonitor> _current;
+  std::vector<std::complex<double>> _a, _b;
+};
+
+// network/network.cpp: S_ji = b_j / a_i for every activated source port i,
+// written as s{10 j + i}.npy = [frequencies; S] (2 x nf, complex128)
+class Network {
+ public:
+  Network(std::vector<std::unique_ptr<Port>> ports, std::vector<double> frequencies, std::string output_path)
+      : _ports{std::move(ports)}, _frequencies{std::move(frequencies)}, _output_path{std::move(output_path)} {}
+  const std::vector<std::unique_ptr<Port>> &ports() const { return _ports; }
+  const std::vector<double> &frequencies() const { return _frequencies; }
+  const std::string &getOutputPath() const { return _output_path; }
+  // S parameters keyed 10 j + i, from the ports' monitor data
+  std::map<int, std::vector<std::complex<double>>> calculateSParameters(double dt);
+  void outputData(double dt);
+
+ private:
+  std::vector<std::unique_ptr<Port>> _ports;
+  std::vector<double> _frequencies;
+  std::string _output_path;
+};
+
+// util/dft.h: sum_j x_j exp(-2 pi i f j dt) dt for every f
+std::vector<std::complex<double>> dft(const std::vector<double> &x, double dt,
+                                      const std::vector<double> &frequencies);
+
 // ------------------------------------------------------------- simulation
 class Simulation {
  public:
@@ -433,6 +618,20 @@ class Simulation {
   Simulation(double cell_size, std::vector<std::shared_ptr<Object>> objects,
              std::vector<std::shared_ptr<Boundary>> boundaries, std::unique_ptr<TFSF> tfsf,
              std::unique_ptr<NFFFT> nffft, float cfl = DEFAULT_CFL);
+  // include/simulation/simulation.h:34-49: lumped-element circuits
+  Simulation(double cell_size, std::vector<std::shared_ptr<Object>> objects,
+             std::vector<std::shared_ptr<LumpedElement>> lumped_elements,
+             std::vector<std::shared_ptr<Boundary>> boundaries = {},
+             std::vector<std::shared_ptr<Monitor>> monitors = {}, float cfl = DEFAULT_CFL);
+  Simulation(double cell_size, std::vector<std::shared_ptr<Object>> objects,
+             std::vector<std::shared_ptr<LumpedElement>> lumped_elements,
+             std::vector<std::shared_ptr<Monitor>> monitors, float cfl = DEFAULT_CFL);
+  Simulation(double cell_size, std::vector<std::shared_ptr<Object>> objects,
+             std::vector<std::shared_ptr<Monitor>> monitors, float cfl = DEFAULT_CFL);
+  Simulation(double cell_size, std::vector<std::shared_ptr<Object>> objects,
+             std::vector<std::shared_ptr<LumpedElement>> lumped_elements,
+             std::vector<std::shared_ptr<Boundary>> boundaries, std::vector<std::shared_ptr<Monitor>> monitors,
+             std::unique_ptr<Network> network, float cfl = DEFAULT_CFL);
   ~Simulation();
   Simulation(const Simulation &) = delete;
   Simulation &operator=(const Simulation &) = delete;
@@ -444,6 +643,10 @@ class Simulation {
   void addNFFFT(std::shared_ptr<NFFFT> nffft);
   void addProbe(Probe probe);
   void addMonitor(std::shared_ptr<Monitor> monitor);
+  void addNetwork(std::shared_ptr<Network> network);
+  // values of the voltage and current monitors (in the order added) after
+  // the steps run so far; also stored in the monitor objects
+  std::vector<std::vector<double>> getLineMonitorData() const;
 
   // ---- differentiation (no counterpart in the reference)
   // Per-cell relative permittivity of the cells [box[0],box[1]) x
@@ -533,9 +736,25 @@ class Simulation {
     size_t plot_step, total;  // movie schedule (plot_step 0: final slice only)
   };
   std::vector<MonitorPlan> _monitor_plans;
+  struct LineTermPlan {
+    int comp, group;
+    int64_t a[3], b[3];
+    double w;
+  };
+  struct LineMonitorPlan {
+    Monitor *monitor;  // a VoltageMonitor or CurrentMonitor
+    std::vector<LineTermPlan> terms;
+    double sign;
+  };
+  std::vector<LineMonitorPlan> _line_plans;
+  std::shared_ptr<Network> _network;
+  void planLineMonitors();
   void planMonitors();
   void writeMonitor(const MonitorPlan &m, const std::string &file) const;
   void outputBroadBand(const NffftBroadBand &bb) const;
+  void outputNffft() const;
+  void writeLineMonitor(const LineMonitorPlan &lp) const;
+  void outputNetwork() const;
   bool _has_design = false;
   std::array<size_t, 6> _design_box{};
   std::vector<float> _design_eps;

@@ -16,6 +16,10 @@
  *   xf_engine_set_tfsf      TFSF3D::init                     tfsf/tfsf_3d.cpp:33-151
  *   xf_engine_set_nffft     NffftFd::init / initDFT          nffft/nffft_fd.cpp:17-106
  *   xf_engine_add_current_source  CurrentSource::correctE    lumped_element/current_source.cpp:140-163
+ *                           (also VoltageSource::correctE   lumped_element/voltage_source.cpp:168-193)
+ *   xf_engine_add_inductor  Inductor::correctE               lumped_element/inductor.cpp:115-146
+ *   xf_engine_add_line_monitor  Voltage/CurrentMonitor::update  monitor/voltage_monitor.cpp:63-93,
+ *                                                            monitor/current_monitor.cpp:66-122
  *   xf_engine_add_probe     TimeDomainFieldMonitor sample    monitor/field_monitor.cpp:29-122
  *   xf_engine_get_field     EMF::getEx..getHz                include/simulation/simulation.h:92-97
  *   xf_engine_get_nffft     NffftFd accumulators             nffft/nffft_fd.cpp:126-202
@@ -183,6 +187,27 @@ xf_status xf_engine_add_current_source(xf_engine *e, int comp, int64_t i0,
    stepping beyond it is refused, and probe buffers are sized by it.  Tables
    with their own length (sources, TFSF, NF-FF, DFT) lower it further. */
 xf_status xf_engine_set_horizon(xf_engine *e, int64_t total_steps);
+
+/* Inductor::correctE (lumped_element/inductor.cpp:70-146) on the box
+   [i0,i1) x [j0,j1) x [k0,k1) of component comp (0..2): after the E
+   update, E += cecjc J, then J = cjcec E + J, per cell (C order over the
+   box).  Lumped corrections run in the order they were added, sources and
+   inductors alike.  J is loop state (checkpointed); gradients refuse
+   engines with inductors. */
+xf_status xf_engine_add_inductor(xf_engine *e, int comp, int64_t i0, int64_t i1, int64_t j0, int64_t j1,
+                                 int64_t k0, int64_t k1, const double *cecjc, const double *cjcec);
+/* VoltageMonitor / CurrentMonitor::update (monitor/voltage_monitor.cpp:63-93,
+   monitor/current_monitor.cpp:66-122) as a term list: after every step
+   t < total_steps, value[t] = sign * (S0 + S1), S_g the FP64 sum in term
+   order of w[q] (f[cell_a] - f[cell_b]) over the terms of group g, with f
+   component comp[q] and cell_b[3q] < 0 meaning f[cell_a] alone.  Cells are
+   (i, j, k) triples.  *index receives the monitor's number. */
+xf_status xf_engine_add_line_monitor(xf_engine *e, int nterms, const int *comp, const int64_t *cell_a,
+                                     const int64_t *cell_b, const double *w, const int *group, double sign,
+                                     int64_t total_steps, int *index);
+int xf_engine_line_monitor_count(const xf_engine *e);
+/* values [monitor][total_steps] */
+xf_status xf_engine_get_line_monitors(xf_engine *e, double *out, int64_t total_steps);
 
 /* point sample of comp at (i,j,k) after every step */
 xf_status xf_engine_add_probe(xf_engine *e, int comp, int64_t i, int64_t j,

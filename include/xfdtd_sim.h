@@ -68,6 +68,34 @@ int xf_sim_set_nffft_broadband(xf_sim *s, const size_t distance[3], double theta
 int xf_sim_add_current_source(xf_sim *s, const double origin[3], const double size[3],
                               int orientation, double resistance, double amplitude, double tau,
                               double t0);
+/* lumped element, in insertion order (lumped_element/{current_source,voltage_source,resistor,capacitor,inductor}.cpp): kind 0
+   CurrentSource, 1 VoltageSource (dir = orientation 0..5, value =
+   resistance), 2 Resistor, 3 Capacitor, 4 Inductor (dir = axis 0..2, value
+   = R, C or L).  Sources take a waveform: wkind 0 Gaussian (amplitude, tau,
+   t0), 1 sinusoid (amplitude, frequency, phase), 2 cosine-modulated
+   Gaussian (amplitude, tau, t0, frequency) in wp[0..3]. */
+int xf_sim_add_lumped(xf_sim *s, int kind, const double origin[3], const double size[3], int dir,
+                      double value, int wkind, const double wp[4]);
+/* ObjectPlane(Cube(origin, size), material) (object/object_plane.cpp:4-50) */
+int xf_sim_add_plane(xf_sim *s, const double origin[3], const double size[3], float eps_r, float mu_r,
+                     double sigma_e, double sigma_m);
+/* VoltageMonitor (kind 0) / CurrentMonitor (kind 1) (Cube(origin, size),
+   orientation, dir, file): one value per step; xf_sim_output_nffft writes
+   dir/file = [time, value].  Monitors are numbered in the order added. */
+int xf_sim_add_line_monitor(xf_sim *s, int kind, const double origin[3], const double size[3],
+                            int orientation, const char *dir, const char *file);
+/* values [monitor][total_steps] of the steps run so far */
+int xf_sim_get_line_monitors(xf_sim *s, double *out);
+/* Network(ports, frequencies, output_dir) of Port(index[q], impedance
+   (re, im), is_source, voltage monitor vmon[q], current monitor imon[q])
+   (network/port.cpp, network/network.cpp) */
+int xf_sim_set_network(xf_sim *s, int nports, const int *index, const double *impedance,
+                       const int *is_source, const int *vmon, const int *imon, const double *freqs, int nf,
+                       const char *output_dir);
+/* number of S parameters (source ports x ports); keys 10 j + i and the
+   values [n][nf] interleaved complex, in increasing key order */
+int xf_sim_network_size(const xf_sim *s);
+int xf_sim_network_sparams(xf_sim *s, int *keys, double *out);
 int xf_sim_add_probe(xf_sim *s, int comp, size_t i, size_t j, size_t k);
 /* TimeDomainFieldMonitor(Cube(origin, size), plane_type (0 XY, 1 YZ, 2 ZX),
    component, dir, file): the component over the cube's grid box, written by

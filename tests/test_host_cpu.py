@@ -70,11 +70,42 @@ def drude_scene(n=30, dl=4e-3, pml=5):
                                freqs=[S.C_0 / (20 * dl)], theta=[0.0, 1.0], phi=[0.0])
 
 
-@pytest.mark.parametrize("builder", ["everything", "pec_example", "drude"])
+def lumped_scene(waveform=None, steps_hint=400):
+    """The reference's basic lumped-element circuit
+    (examples/basic_lumped_element.cpp): a 14 x 8 x 10 mm air box at 1 mm,
+    two PEC plates (ObjectPlane) at z = 0 and 4 mm joined by a voltage
+    source and a 150 ohm resistor, with a capacitor, an inductor, a current
+    source, a later object covering part of the upper plate, and voltage and
+    current monitors of the circuit."""
+    size = 1e-3
+    cube = lambda o, d: S.Cube(tuple(x * size for x in o), tuple(x * size for x in d))
+    sc = S.Scene(cell_size=size, cfl=0.99)
+    sc.add_object(S.Object("domain", cube((-3, -3, -3), (14, 8, 10)), S.Material.air()))
+    sc.add_object(S.ObjectPlane("zn", cube((0, 0, 0), (8, 2, 0)), S.Material.pec()))
+    sc.add_object(S.ObjectPlane("zp", cube((0, 0, 4), (8, 2, 0)), S.Material.pec()))
+    sc.add_object(S.Object("patch", cube((6, 0, 4), (2, 2, 1)), S.Material("d", 3.0, 1.0, 0.01, 0.0)))
+    w = waveform or S.SinusoidalWaveform(1.0, 5e10, 0.0)
+    sc.sources = [S.VoltageSource(cube((0, 0, 0), (1, 2, 4)), S.ZP, 50.0, w),
+                  S.Resistor(cube((7, 0, 0), (1, 2, 4)), S.AX_Z, 150.0),
+                  S.Capacitor(cube((2, 0, 0), (1, 1, 4)), S.AX_Z, 2e-13),
+                  # the inductor's explicit update is stable for dt^2 dc / (L' da db eps) <= 4
+                  S.Inductor(cube((5, 1, 0), (1, 1, 4)), S.AX_Z, 3e-9),
+                  S.CurrentSource(cube((3, -2, 1), (1, 1, 2)), S.YN, 1e5,
+                                  S.CosineModulatedGaussianWaveform(1e-3, 1e-11, 4e-11, 6e10))]
+    sc.monitors = [S.VoltageMonitor(cube((4, 0, 0), (1, 2, 4)), S.ZP, "", "circuit_voltage.npy"),
+                   S.CurrentMonitor(cube((4, 0, 3), (1, 2, 1)), S.XP, "", "circuit_current.npy"),
+                   S.CurrentMonitor(cube((2, 0, 1), (0, 1, 2)), S.ZN, "", "cap_current.npy"),
+                   S.VoltageMonitor(cube((5, 1, 0), (1, 1, 4)), S.ZN, "", "ind_voltage.npy")]
+    return sc
+
+
+@pytest.mark.parametrize("builder", ["everything", "pec_example", "drude", "lumped"])
 def test_plan_reproduces_reference_coefficient_arrays(builder):
     from xfdtd_amd.sim import Simulation
     if builder == "everything":
         sc = _scene_with_everything()
+    elif builder == "lumped":
+        sc = lumped_scene()
     elif builder == "drude":
         sc = drude_scene()
     else:
@@ -182,3 +213,32 @@ def test_plan_refuses_bad_monitors(tmp_path):
     sc = base()
     sc.monitors.append(S.MovieMonitor(mon, 5, 2))
     Simulation(sc, 5, plan_only=True)
+
+
+def test_bad_lumped_and_monitor_placements():
+    """The host refuses a current monitor whose loop leaves the grid, a plane
+    object that is not a plane, a lumped element with no cells and an
+    inductor inside a CPML slab."""
+    from xfdtd_amd.sim import Simulation
+    dl = 1e-3
+    def base():
+        sc = S.Scene(cell_size=dl, cfl=0.99)
+        sc.add_object(S.Object("box", S.Cube((0, 0, 0), (10 * dl, 10 * dl, 10 * dl)), S.Material.air()))
+        return sc
+    sc = base()
+    sc.monitors = [S.CurrentMonitor(S.Cube((0, 0, 2 * dl), (dl, dl, dl)), S.ZP)]
+    with pytest.raises(RuntimeError, match="current monitor loop outside the grid"):
+        Simulation(sc, 10, plan_only=True)
+    sc = base()
+    sc.add_object(S.ObjectPlane("p", S.Cube((dl, dl, dl), (dl, dl, dl)), S.Material.pec()))
+    with pytest.raises(RuntimeError, match="is not a plane"):
+        Simulation(sc, 10, plan_only=True)
+    sc = base()
+    sc.sources = [S.Resistor(S.Cube((dl, dl, dl), (dl, 0.0, dl)), S.AX_Z, 50.0)]
+    with pytest.raises(RuntimeError, match="no cells"):
+        Simulation(sc, 10, plan_only=True)
+    sc = S.cube_domain_scene(20, dl, 4, cfl=0.99)
+    half = 10 * dl
+    sc.sources = [S.Inductor(S.Cube((-half + dl, 0, 0), (dl, dl, dl)), S.AX_Z, 1e-9)]
+    with pytest.raises(RuntimeError, match="inductor inside a CPML slab"):
+        Simulation(sc, 10, plan_only=True)

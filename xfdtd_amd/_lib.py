@@ -27,6 +27,8 @@ ENGINE_SYMBOLS = [
     "xf_engine_drude_box", "xf_engine_get_drude_j", "xf_engine_set_horizon", "xf_engine_set_drude_j",
     "xf_engine_get_box", "xf_engine_set_nffft_broadband", "xf_engine_broadband_samples_size",
     "xf_engine_get_broadband_samples", "xf_engine_broadband_sums", "xf_engine_set_broadband_scratch",
+    "xf_engine_add_inductor", "xf_engine_add_line_monitor", "xf_engine_line_monitor_count",
+    "xf_engine_get_line_monitors",
 ]
 SIM_SYMBOLS = [
     "xf_sim_last_error", "xf_sim_create", "xf_sim_destroy", "xf_sim_add_cube", "xf_sim_add_sphere",
@@ -39,6 +41,8 @@ SIM_SYMBOLS = [
     "xf_sim_add_drude_sphere", "xf_sim_drude_box", "xf_sim_get_drude_j", "xf_sim_set_drude_j",
     "xf_sim_add_field_monitor", "xf_sim_add_movie_monitor", "xf_sim_set_nffft_broadband",
     "xf_sim_broadband_samples_size", "xf_sim_get_broadband_samples", "xf_sim_broadband_farfield",
+    "xf_sim_add_lumped", "xf_sim_add_plane", "xf_sim_add_line_monitor", "xf_sim_get_line_monitors",
+    "xf_sim_set_network", "xf_sim_network_size", "xf_sim_network_sparams",
 ]
 
 
@@ -115,6 +119,14 @@ def lib():
     L.xf_sim_add_movie_monitor.argtypes = [vp, dp, dp, i, i, C.c_char_p, sz, sz]
     L.xf_sim_set_nffft_broadband.argtypes = [vp, szp, d, d, C.c_char_p]
     L.xf_engine_set_broadband_scratch.argtypes = [vp, i64]
+    ip = C.POINTER(C.c_int)
+    L.xf_sim_add_lumped.argtypes = [vp, i, dp, dp, i, d, i, dp]
+    L.xf_sim_add_plane.argtypes = [vp, dp, dp, C.c_float, C.c_float, d, d]
+    L.xf_sim_add_line_monitor.argtypes = [vp, i, dp, dp, i, C.c_char_p, C.c_char_p]
+    L.xf_sim_get_line_monitors.argtypes = [vp, dp]
+    L.xf_sim_set_network.argtypes = [vp, i, ip, dp, ip, ip, ip, dp, i, C.c_char_p]
+    L.xf_sim_network_size.argtypes = [vp]
+    L.xf_sim_network_sparams.argtypes = [vp, ip, dp]
     L.xf_sim_broadband_samples_size.restype = i64
     L.xf_sim_broadband_samples_size.argtypes = [vp]
     L.xf_sim_get_broadband_samples.argtypes = [vp, dp]

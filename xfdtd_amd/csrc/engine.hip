@@ -40,9 +40,22 @@ extern "C" const char *xf_last_error(void) { return g_err.c_str(); }
 
 namespace {
 
+// a lumped E correction on a box: kind 0 adds coff * wave[t] (current and
+// voltage sources), kind 1 is an inductor (E += coff J; J = c2 E + J)
 struct SourceDev {
   int comp, i0, i1, j0, j1, k0, k1;
-  void *coff, *wave;
+  int kind = 0;
+  void *coff = nullptr, *wave = nullptr, *c2 = nullptr, *jc = nullptr;
+};
+
+// voltage / current monitor: value = sign (sum of group 0 + sum of group 1)
+// of terms w (f[ca] - f[cb]) (f[ca] alone when cb < 0), in term order
+struct LineMonDev {
+  void *terms = nullptr;
+  int n = 0;
+  double sign = 1;
+  double *buf = nullptr;  // [T]
+  long long T = 0;
 };
 
 }  // namespace
@@ -88,6 +101,7 @@ struct xf_engine {
   long long bb_scratch = 1LL << 30;  // bytes of rows + spectra per far-field transform
   // sources, probes
   std::vector<SourceDev> sources;
+  std::vector<LineMonDev> line_mons;
   long long src_T = 0;
   std::vector<int> probe_comp;
   std::vector<long long> probe_cell;
@@ -651,6 +665,80 @@ extern "C" xf_status xf_engine_add_current_source(xf_engine *e, int comp, int64_
   return XF_OK;
 }
 
+extern "C" xf_status xf_engine_add_inductor(xf_engine *e, int comp, int64_t i0, int64_t i1, int64_t j0,
+                                            int64_t j1, int64_t k0, int64_t k1, const double *cecjc,
+                                            const double *cjcec) {
+  if (!e || !cecjc || !cjcec || comp < XF_EX || comp > XF_EZ) FAILS(XF_EINVAL, "bad inductor");
+  int sh[3];
+  comp_shape(e, comp, sh);
+  if (i0 < 0 || j0 < 0 || k0 < 0 || i1 > sh[0] || j1 > sh[1] || k1 > sh[2] || i1 <= i0 || j1 <= j0 || k1 <= k0)
+    FAILS(XF_EINVAL, "inductor box outside the component array");
+  SourceDev sd;
+  sd.comp = comp; sd.i0 = (int)i0; sd.i1 = (int)i1; sd.j0 = (int)j0; sd.j1 = (int)j1;
+  sd.k0 = (int)k0; sd.k1 = (int)k1;
+  sd.kind = 1;
+  const size_t nc = (size_t)(i1 - i0) * (j1 - j0) * (k1 - k0);
+  xf_status s;
+  if ((s = upload_real(e, &sd.coff, cecjc, nc)) != XF_OK) return s;
+  if ((s = upload_real(e, &sd.c2, cjcec, nc)) != XF_OK) return s;
+  if ((s = dev_alloc(e, &sd.jc, nc * e->esz)) != XF_OK) return s;
+  e->state.push_back({sd.jc, nc * e->esz});
+  e->sources.push_back(sd);
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_add_line_monitor(xf_engine *e, int nterms, const int *comp, const int64_t *cell_a,
+                                                const int64_t *cell_b, const double *w, const int *group,
+                                                double sign, int64_t total_steps, int *index) {
+  if (!e || nterms < 1 || !comp || !cell_a || !cell_b || !w || !group || !index || total_steps < 1)
+    FAILS(XF_EINVAL, "bad line monitor");
+  std::vector<LineTerm> t((size_t)nterms);
+  for (int q = 0; q < nterms; ++q) {
+    if (comp[q] < 0 || comp[q] > 5 || (group[q] != 0 && group[q] != 1))
+      FAILS(XF_EINVAL, "bad line monitor term %d", q);
+    int sh[3];
+    comp_shape(e, comp[q], sh);
+    const int64_t *a = cell_a + 3 * q, *b = cell_b + 3 * q;
+    if (a[0] < 0 || a[1] < 0 || a[2] < 0 || a[0] >= sh[0] || a[1] >= sh[1] || a[2] >= sh[2])
+      FAILS(XF_EINVAL, "line monitor term %d outside the component array", q);
+    const bool hb = b[0] >= 0;
+    if (hb && (b[1] < 0 || b[2] < 0 || b[0] >= sh[0] || b[1] >= sh[1] || b[2] >= sh[2]))
+      FAILS(XF_EINVAL, "line monitor term %d outside the component array", q);
+    t[q].ca = a[0] * e->si + a[1] * e->sj + a[2];
+    t[q].cb = hb ? b[0] * e->si + b[1] * e->sj + b[2] : -1;
+    t[q].w = w[q];
+    t[q].comp = comp[q];
+    t[q].group = group[q];
+  }
+  LineMonDev m;
+  xf_status s;
+  if ((s = dev_alloc(e, &m.terms, t.size() * sizeof(LineTerm))) != XF_OK) return s;
+  HIPCHK(hipMemcpy(m.terms, t.data(), t.size() * sizeof(LineTerm), hipMemcpyHostToDevice));
+  void *p;
+  if ((s = dev_alloc(e, &p, (size_t)total_steps * sizeof(double))) != XF_OK) return s;
+  m.buf = (double *)p;
+  m.n = nterms;
+  m.sign = sign;
+  m.T = total_steps;
+  set_total(e, total_steps);
+  e->line_mons.push_back(m);
+  *index = (int)e->line_mons.size() - 1;
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_get_line_monitors(xf_engine *e, double *out, int64_t total_steps) {
+  if (!e || !out) FAILS(XF_EINVAL, "null argument");
+  HIPCHK(engine_wait(e));
+  for (size_t q = 0; q < e->line_mons.size(); ++q) {
+    if (e->line_mons[q].T != total_steps) FAILS(XF_ESTATE, "line monitor %zu holds %lld steps", q, e->line_mons[q].T);
+    HIPCHK(hipMemcpy(out + q * (size_t)total_steps, e->line_mons[q].buf, (size_t)total_steps * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  }
+  return XF_OK;
+}
+
+extern "C" int xf_engine_line_monitor_count(const xf_engine *e) { return e ? (int)e->line_mons.size() : 0; }
+
 extern "C" xf_status xf_engine_add_probe(xf_engine *e, int comp, int64_t i, int64_t j, int64_t k) {
   if (!e || comp < 0 || comp > 5) FAILS(XF_EINVAL, "bad probe");
   int sh[3];
@@ -773,6 +861,34 @@ static Tiling make_tiling(const xf_engine *e) {
 
 static int tiling_threads(const Tiling &tl) { return tl.ty == 0 ? tl.tz : tl.tz * tl.ty; }
 
+static bool has_inductor(const xf_engine *e) {
+  for (const SourceDev &sd : e->sources)
+    if (sd.kind == 1) return true;
+  return false;
+}
+
+// lumped E corrections in insertion order (simulation_update.cpp:282-290)
+template <typename T>
+static void launch_lumped(const xf_engine *e, const SourceDev &sd, T *f, hipStream_t st, long long step) {
+  const int nj = sd.j1 - sd.j0, nk = sd.k1 - sd.k0;
+  const int nc = (sd.i1 - sd.i0) * nj * nk;
+  if (sd.kind == 1)
+    hipLaunchKernelGGL(k_inductor<T>, dim3((nc + 255) / 256), dim3(256), 0, st, f, e->si, e->sj, sd.i0, nj, nk,
+                       sd.j0, sd.k0, nc, (const T *)sd.coff, (const T *)sd.c2, (T *)sd.jc);
+  else
+    hipLaunchKernelGGL(k_source<T>, dim3((nc + 255) / 256), dim3(256), 0, st, f, e->si, e->sj, sd.i0, nj, nk,
+                       sd.j0, sd.k0, nc, (const T *)sd.coff, (const T *)sd.wave, step);
+}
+
+// voltage and current monitors of the step (simulation_update.cpp:215-219)
+template <typename T>
+static void launch_line_monitors(const xf_engine *e, const Grid<T> &g, hipStream_t st, long long step) {
+  for (const LineMonDev &m : e->line_mons)
+    if (step < m.T)
+      hipLaunchKernelGGL(k_line_monitor<T>, dim3(1), dim3(64), 0, st, g, (const LineTerm *)m.terms, m.n, m.sign,
+                         m.buf, step);
+}
+
 template <typename T>
 static xf_status run_steps(xf_engine *e, long long n) {
   Grid<T> g = make_grid<T>(e);
@@ -827,13 +943,7 @@ static xf_status run_steps(xf_engine *e, long long n) {
         hipLaunchKernelGGL((k_update_e<T, false, false>), grd, blk, 0, e->stream, g, tl);
     }
     if (ev) hipEventRecord(ev[1], e->stream);
-    for (const SourceDev &sd : e->sources) {
-      const int nj = sd.j1 - sd.j0, nk = sd.k1 - sd.k0;
-      const int nc = (sd.i1 - sd.i0) * nj * nk;
-      hipLaunchKernelGGL(k_source<T>, dim3((nc + 255) / 256), dim3(256), 0, e->stream, g.f[sd.comp],
-                         e->si, e->sj, sd.i0, nj, nk, sd.j0, sd.k0, nc, (const T *)sd.coff,
-                         (const T *)sd.wave, step);
-    }
+    for (const SourceDev &sd : e->sources) launch_lumped<T>(e, sd, g.f[sd.comp], e->stream, step);
     if (e->has_tfsf) {
       const int tot = e->nEy + e->nEz + e->nEx;
       hipLaunchKernelGGL(k_tfsf_e<T>, dim3((tot + 255) / 256), dim3(256), 0, e->stream, g, t, e->nEy,
@@ -871,6 +981,7 @@ static xf_status run_steps(xf_engine *e, long long n) {
       hipLaunchKernelGGL(k_probe<T>, dim3((np + 63) / 64), dim3(64), 0, e->stream, g, e->d_probe_comp,
                          e->d_probe_cell, np, e->probe_buf, e->probe_T, step);
     }
+    launch_line_monitors<T>(e, g, e->stream, step);
     HIPCHK(hipGetLastError());
     e->cur++;
   }
@@ -1171,13 +1282,7 @@ static xf_status batch_run(xf_batch *b, long long n) {
         hipLaunchKernelGGL((kb_update_e<T, false, false>), dim3(b->n_curl), blk, 0, b->stream, in, b->map_curl);
     }
     for (xf_engine *e : b->e)
-      for (const SourceDev &sd : e->sources) {
-        const int nj = sd.j1 - sd.j0, nk = sd.k1 - sd.k0;
-        const int nc = (sd.i1 - sd.i0) * nj * nk;
-        hipLaunchKernelGGL(k_source<T>, dim3((nc + 255) / 256), dim3(256), 0, b->stream, (T *)e->f[sd.comp],
-                           e->si, e->sj, sd.i0, nj, nk, sd.j0, sd.k0, nc, (const T *)sd.coff,
-                           (const T *)sd.wave, step);
-      }
+      for (const SourceDev &sd : e->sources) launch_lumped<T>(e, sd, (T *)e->f[sd.comp], b->stream, step);
     if (b->n_te > 0) hipLaunchKernelGGL(kb_tfsf_e<T>, dim3(b->n_te), dim3(256), 0, b->stream, in, b->map_te);
     if (b->use_lds)
       hipLaunchKernelGGL((kb_update_h<T, true>), dim3(b->n_curl), blk, b->lds, b->stream, in, b->map_curl);
@@ -1195,6 +1300,8 @@ static xf_status batch_run(xf_batch *b, long long n) {
         hipLaunchKernelGGL(k_probe<T>, dim3((np + 63) / 64), dim3(64), 0, b->stream, make_grid<T>(e),
                            e->d_probe_comp, e->d_probe_cell, np, e->probe_buf, e->probe_T, step);
       }
+    for (xf_engine *e : b->e)
+      if (!e->line_mons.empty()) launch_line_monitors<T>(e, make_grid<T>(e), b->stream, step);
     HIPCHK(hipGetLastError());
     for (xf_engine *e : b->e) e->cur++;
   }
@@ -1539,6 +1646,7 @@ extern "C" xf_status xf_engine_grad_eps(xf_engine *e, int checkpoint_every, doub
   if (!e || !grad || !J || checkpoint_every < 0) FAILS(XF_EINVAL, "bad argument");
   if (!e->materials_set || !e->has_design || !e->has_dft)
     FAILS(XF_ESTATE, "the gradient needs materials, a design box and a DFT plane");
+  if (has_inductor(e)) FAILS(XF_EINVAL, "the gradient does not transpose inductors");
   if (e->cur != 0) FAILS(XF_ESTATE, "the gradient runs from step 0 (engine is at step %lld)", e->cur);
   if (e->total < 1) FAILS(XF_ESTATE, "no step horizon");
   if (!e->probe_comp.empty()) {
@@ -1615,6 +1723,7 @@ static xf_status adjoint_run(xf_engine *e, long long nsteps, double *fields) {
 extern "C" xf_status xf_engine_adjoint_run(xf_engine *e, int64_t nsteps, double *fields) {
   if (!e || !fields || nsteps < 1) FAILS(XF_EINVAL, "bad argument");
   if (!e->materials_set) FAILS(XF_ESTATE, "materials not set");
+  if (has_inductor(e)) FAILS(XF_EINVAL, "the adjoint does not transpose inductors");
   HIPCHK(engine_wait(e));
   return e->prec == 64 ? adjoint_run<double>(e, nsteps, fields) : adjoint_run<float>(e, nsteps, fields);
 }
@@ -1753,6 +1862,7 @@ extern "C" xf_status xf_batch_grad_eps(xf_batch *b, int checkpoint_every, double
     if (!grads[q]) FAILS(XF_EINVAL, "null gradient buffer %zu", q);
     if (!e->has_design || !e->has_dft)
       FAILS(XF_ESTATE, "member %zu: the gradient needs a design box and a DFT plane", q);
+    if (has_inductor(e)) FAILS(XF_EINVAL, "member %zu: the gradient does not transpose inductors", q);
     if (e->cur != 0) FAILS(XF_ESTATE, "member %zu is at step %lld; the gradient runs from step 0", q, e->cur);
     if (e->total < 1 || e->total != b->e[0]->total)
       FAILS(XF_ESTATE, "member %zu: every member needs the same step horizon", q);

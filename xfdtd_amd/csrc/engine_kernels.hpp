@@ -998,6 +998,46 @@ __global__ void k_source(T *f, long long si, long long sj, int i0, int nj, int n
   *p = *p + coff[q] * wave[step];
 }
 
+// Inductor::correctE (lumped_element/inductor.cpp:115-146): E += cecjc J,
+// then J = cjcec E + J with the corrected E
+template <typename T>
+__global__ void k_inductor(T *f, long long si, long long sj, int i0, int nj, int nk, int j0, int k0, int ncell,
+                           const T *cecjc, const T *cjcec, T *jc) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncell) return;
+  const int i = i0 + q / (nj * nk), j = j0 + (q / nk) % nj, k = k0 + q % nk;
+  T *p = &f[(long long)i * si + (long long)j * sj + k];
+  const T e = *p + cecjc[q] * jc[q];
+  *p = e;
+  jc[q] = cjcec[q] * e + jc[q];
+}
+
+// VoltageMonitor / CurrentMonitor::update (monitor/voltage_monitor.cpp:63-93,
+// monitor/current_monitor.cpp:66-122): one thread sums the terms in the
+// reference's loop order, in FP64
+struct LineTerm {
+  long long ca, cb;  // cells (cb < 0: none)
+  double w;
+  int comp, group;
+};
+template <typename T>
+__global__ void k_line_monitor(Grid<T> g, const LineTerm *terms, int n, double sign, double *buf, long long step) {
+  if (threadIdx.x != 0) return;
+  double s0 = 0.0, s1 = 0.0;
+  for (int q = 0; q < n; ++q) {
+    const LineTerm t = terms[q];
+    const T *f = g.f[t.comp];
+    double x = (double)f[t.ca];
+    if (t.cb >= 0) x = x - (double)f[t.cb];
+    const double v = x * t.w;
+    if (t.group == 0)
+      s0 = s0 + v;
+    else
+      s1 = s1 + v;
+  }
+  buf[step] = sign * (s0 + s1);
+}
+
 // point probes (monitor update slot, simulation_update.cpp:215-219)
 template <typename T>
 __global__ void k_probe(Grid<T> g, const int *comp, const long long *cell, int n,

@@ -44,8 +44,9 @@ struct PlanData {
   int dn[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
   std::vector<double> be[3], ae[3], bm[3], am[3];
   struct Src {
+    int kind;  // 0 source term, 1 inductor
     int comp, is, ie, js, je, ks, ke;
-    std::vector<double> coff, wave;
+    std::vector<double> coff, wave, c2;
   };
   std::vector<Src> srcs;
   struct Tfsf {
@@ -103,6 +104,61 @@ std::unique_ptr<Shape> Sphere::getWrappedBox() const {
 double GaussianWaveform::getValueByTime(double time) const {
   return getAmplitude() * std::exp(-std::pow((time - _t_0) / _tau, 2));
 }
+double SinusoidalWaveform::getValueByTime(double time) const {
+  return getAmplitude() * std::sin(2 * constant::PI * _frequency * time + _phase);
+}
+double CosineModulatedGaussianWaveform::getValueByTime(double time) const {
+  return getAmplitude() * std::cos(2 * constant::PI * _f * (time - _t_0)) *
+         std::exp(-std::pow((time - _t_0) / _tau, 2));
+}
+
+// util/dft.h:11-26
+std::vector<std::complex<double>> dft(const std::vector<double> &x, double dt,
+                                      const std::vector<double> &frequencies) {
+  using namespace std::complex_literals;
+  std::vector<std::complex<double>> res(frequencies.size());
+  for (size_t i = 0; i < frequencies.size(); ++i) {
+    std::complex<double> sum{0.0, 0.0};
+    for (size_t j = 0; j < x.size(); ++j) {
+      const double t = (double)j * dt;
+      sum += x[j] * std::exp(-2.0 * 1i * constant::PI * frequencies[i] * t);
+    }
+    res[i] = sum * dt;
+  }
+  return res;
+}
+
+// network/port.cpp:19-29
+void Port::calculateSParameter(const std::vector<double> &frequencies, double dt) {
+  const std::complex<double> z = _impedance;
+  const double k = std::sqrt(std::real(z));
+  const auto v = dft(_voltage->getValueArray(), dt, frequencies);
+  const auto i = dft(_current->getValueArray(), dt, frequencies);
+  _a.resize(v.size());
+  _b.resize(v.size());
+  for (size_t q = 0; q < v.size(); ++q) {
+    _a[q] = 0.5 * (v[q] + z * i[q]) / k;
+    _b[q] = 0.5 * (v[q] - std::conj(z) * i[q]) / k;
+  }
+}
+
+// network/network.cpp:27-45
+std::map<int, std::vector<std::complex<double>>> Network::calculateSParameters(double dt) {
+  for (auto &p : _ports) p->calculateSParameter(_frequencies, dt);
+  std::map<int, std::vector<std::complex<double>>> s;
+  for (size_t i = 0; i < _ports.size(); ++i) {
+    if (!_ports[i]->isActivatedSource()) continue;
+    for (size_t j = 0; j < _ports.size(); ++j) {
+      const auto &b = _ports[j]->getB();
+      const auto &a = _ports[i]->getA();
+      std::vector<std::complex<double>> v(b.size());
+      for (size_t q = 0; q < b.size(); ++q) v[q] = b[q] / a[q];
+      s[_ports[j]->getPortIndex() * 10 + _ports[i]->getPortIndex()] = std::move(v);
+    }
+  }
+  return s;
+}
+
 void Waveform::init(const std::vector<double> &time_array) {
   _value.resize(time_array.size());
   for (size_t q = 0; q < time_array.size(); ++q) _value[q] = getValueByTime(time_array[q]);
@@ -224,9 +280,40 @@ Simulation::Simulation(double cell_size, std::vector<std::shared_ptr<Object>> ob
     : Simulation(cell_size, std::move(objects), std::move(boundaries), std::move(tfsf), cfl) {
   _nffft = std::move(nffft);
 }
+Simulation::Simulation(double cell_size, std::vector<std::shared_ptr<Object>> objects,
+                       std::vector<std::shared_ptr<LumpedElement>> lumped_elements,
+                       std::vector<std::shared_ptr<Boundary>> boundaries,
+                       std::vector<std::shared_ptr<Monitor>> monitors, float cfl)
+    : Simulation(cell_size, cfl) {
+  _objects = std::move(objects);
+  _lumped_elements = std::move(lumped_elements);
+  _boundaries = std::move(boundaries);
+  _monitors = std::move(monitors);
+}
+Simulation::Simulation(double cell_size, std::vector<std::shared_ptr<Object>> objects,
+                       std::vector<std::shared_ptr<LumpedElement>> lumped_elements,
+                       std::vector<std::shared_ptr<Monitor>> monitors, float cfl)
+    : Simulation(cell_size, std::move(objects), std::move(lumped_elements), {}, std::move(monitors), cfl) {}
+Simulation::Simulation(double cell_size, std::vector<std::shared_ptr<Object>> objects,
+                       std::vector<std::shared_ptr<Monitor>> monitors, float cfl)
+    : Simulation(cell_size, cfl) {
+  _objects = std::move(objects);
+  _monitors = std::move(monitors);
+}
+Simulation::Simulation(double cell_size, std::vector<std::shared_ptr<Object>> objects,
+                       std::vector<std::shared_ptr<LumpedElement>> lumped_elements,
+                       std::vector<std::shared_ptr<Boundary>> boundaries,
+                       std::vector<std::shared_ptr<Monitor>> monitors, std::unique_ptr<Network> network,
+                       float cfl)
+    : Simulation(cell_size, std::move(objects), std::move(lumped_elements), std::move(boundaries),
+                 std::move(monitors), cfl) {
+  _network = std::move(network);
+}
 Simulation::~Simulation() {
   if (_engine) xf_engine_destroy(_engine);
 }
+
+void Simulation::addNetwork(std::shared_ptr<Network> network) { _network = std::move(network); }
 
 void Simulation::addObject(std::shared_ptr<Object> o) { _objects.push_back(std::move(o)); }
 void Simulation::addBoundary(std::shared_ptr<Boundary> b) { _boundaries.push_back(std::move(b)); }
@@ -371,12 +458,52 @@ void Simulation::plan(size_t total_time_steps) {
   // other objects leave the mark alone.  dmask holds 1 + object index.
   std::vector<uint8_t> dmask(any_drude ? midx.size() : 0, 0);
   if (_objects.size() > 254) throw std::runtime_error("more than 254 objects");
+  // ObjectPlane sigma_e per E component and cell (NaN: none); a later
+  // object whose cell mask covers the cell resets it, as its arrays do
+  std::map<size_t, std::array<double, 3>> psig;
+  const double nan = std::numeric_limits<double>::quiet_NaN();
   for (size_t q = 0; q < _objects.size(); ++q) {
     const Shape *sh = _objects[q]->getShape();
     auto box = sh->getWrappedBox();
     auto *c = dynamic_cast<Cube *>(box.get());
     const double lo[3] = {c->getXmin(), c->getYmin(), c->getZmin()};
     const double hi[3] = {c->getXmax(), c->getYmax(), c->getZmax()};
+    if (dynamic_cast<const ObjectPlane *>(_objects[q].get())) {
+      // object/object_plane.cpp:8-50: the two tangential components on the
+      // plane, [s, e) along one and [s, e] along the other in-plane axis
+      size_t st[3], en[3];
+      for (int a = 0; a < 3; ++a) {
+        st[a] = handleTransform(a, lo[a]);
+        en[a] = handleTransform(a, hi[a]);
+      }
+      int n = -1;
+      for (int a = 2; a >= 0; --a)
+        if (st[a] == en[a]) n = a;
+      if (n < 0) throw std::runtime_error("ObjectPlane " + _objects[q]->getName() + " is not a plane");
+      const double sig = _objects[q]->getMaterial().getElectricalConductivity();
+      for (int cc = 0; cc < 3; ++cc) {
+        if (cc == n) continue;
+        size_t r0[3], r1[3];
+        for (int a = 0; a < 3; ++a) {
+          r0[a] = st[a];
+          r1[a] = a == n ? st[a] + 1 : (a == cc ? en[a] : en[a] + 1);
+        }
+        for (size_t i = r0[0]; i < r1[0]; ++i)
+          for (size_t j = r0[1]; j < r1[1]; ++j)
+            for (size_t k = r0[2]; k < r1[2]; ++k) {
+              auto it = psig.emplace(i * si + j * sj + k, std::array<double, 3>{nan, nan, nan}).first;
+              it->second[cc] = sig;
+            }
+      }
+      continue;
+    }
+    for (auto it = psig.begin(); it != psig.end();) {
+      const size_t cidx = it->first, i = cidx / si, j = (cidx % si) / sj, k = cidx % sj;
+      if ((int)i < nx && (int)j < ny && (int)k < nz && sh->isPointInside({_hc[0][i], _hc[1][j], _hc[2][k]}))
+        it = psig.erase(it);
+      else
+        ++it;
+    }
     int r0[3], r1[3];
     for (int a = 0; a < 3; ++a) {
       r0[a] = 0;
@@ -452,82 +579,184 @@ void Simulation::plan(size_t total_time_steps) {
     drude_of.push_back(dslots[q].d);
   }
 
-  // ---- lumped current sources (lumped_element/current_source.cpp:34-187).
-  // Each source cell gets its own material slot holding the corrected
-  // coefficients of its component; the other components keep the base values.
-  struct SrcCell {
-    int comp, i, j, k;
-    double coff;
-  };
+  // ---- lumped elements in insertion order (lumped_element/
+  // {current_source,voltage_source,resistor,capacitor,inductor}.cpp).  A cell
+  // whose E coefficients a lumped element or a plane changes gets a material
+  // slot of its own kind: its base material with those components replaced.
   struct SrcInfo {
+    int kind;  // 0: source term (current or voltage source), 1: inductor
     int comp, is, ie, js, je, ks, ke;
-    std::vector<double> coff, wave;
+    std::vector<double> coff, wave, c2;
   };
   std::vector<SrcInfo> srcs;
-  struct Override {
-    int m_base, comp, i, j, k;
-    double ce, ca, cb;
+  struct CellOv {
+    int base = 0;
+    uint8_t kind[3] = {0, 0, 0};  // per component: 0 none, 1 lumped ce/ca/cb, 2 plane sigma
+    double v[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   };
-  std::vector<Override> overrides;
+  std::map<size_t, CellOv> ovs;
+  for (const auto &e : psig) {
+    CellOv &o = ovs[e.first];
+    o.base = midx[e.first];
+    for (int cc = 0; cc < 3; ++cc)
+      if (!std::isnan(e.second[cc])) {
+        o.kind[cc] = 2;
+        o.v[cc][0] = e.second[cc];
+      }
+  }
+  auto cell_sigma = [&](size_t cidx, int comp) {
+    auto it = psig.find(cidx);
+    if (it != psig.end() && !std::isnan(it->second[comp])) return it->second[comp];
+    return mats[midx[cidx]].sigma;
+  };
   for (const auto &le : _lumped_elements) {
+    enum { CS, VS, RES, CAP, IND } kind;
     auto *cs = dynamic_cast<CurrentSource *>(le.get());
-    if (!cs) throw std::runtime_error("only CurrentSource lumped elements are supported");
-    const Cube &cube = cs->getCube();
+    auto *vs = dynamic_cast<VoltageSource *>(le.get());
+    auto *rs = dynamic_cast<Resistor *>(le.get());
+    auto *cp = dynamic_cast<Capacitor *>(le.get());
+    auto *nd = dynamic_cast<Inductor *>(le.get());
+    int a;
+    double value;
+    if (cs) { kind = CS; a = static_cast<int>(cs->getOrientation()) / 2; value = cs->getResistance(); }
+    else if (vs) { kind = VS; a = static_cast<int>(vs->getOrientation()) / 2; value = vs->getResistance(); }
+    else if (rs) { kind = RES; a = static_cast<int>(rs->getAxis()); value = rs->getResistance(); }
+    else if (cp) { kind = CAP; a = static_cast<int>(cp->getAxis()); value = cp->getCapacitance(); }
+    else if (nd) { kind = IND; a = static_cast<int>(nd->getAxis()); value = nd->getInductance(); }
+    else throw std::runtime_error("unsupported lumped element");
+    const Cube &cube = le->getCube();
     const double lo[3] = {cube.getXmin(), cube.getYmin(), cube.getZmin()};
     const double hi[3] = {cube.getXmax(), cube.getYmax(), cube.getZmax()};
     size_t st[3], en[3];
-    for (int a = 0; a < 3; ++a) {
-      st[a] = handleTransform(a, lo[a]);
-      en[a] = handleTransform(a, hi[a]);
-      if (en[a] < st[a]) throw std::range_error("end < start");
-      if (en[a] == st[a]) throw std::runtime_error("current source box has no cells");
+    for (int b = 0; b < 3; ++b) {
+      st[b] = handleTransform(b, lo[b]);
+      en[b] = handleTransform(b, hi[b]);
+      if (en[b] < st[b]) throw std::range_error("end < start");
+      if (en[b] == st[b]) throw std::runtime_error("lumped element box has no cells");
     }
     SrcInfo si_;
-    const int a = static_cast<int>(cs->getOrientation()) / 2;
+    si_.kind = kind == IND ? 1 : 0;
     si_.comp = a;  // XF_EX / XF_EY / XF_EZ
     si_.is = (int)st[0]; si_.ie = (int)en[0]; si_.js = (int)st[1]; si_.je = (int)en[1];
     si_.ks = (int)st[2]; si_.ke = (int)en[2];
     const size_t ni = en[0] - st[0], nj = en[1] - st[1], nk = en[2] - st[2];
-    double rf, af;
-    const double R = cs->getResistance();
-    if (a == 0) { rf = R * nk * nj / ni; af = cs->waveform().getAmplitude() / (nk * nj); }
-    else if (a == 1) { rf = R * nk * ni / nj; af = cs->waveform().getAmplitude() / (nk * ni); }
-    else { rf = R * ni * nj / nk; af = cs->waveform().getAmplitude() / (ni * nj); }
-    const int ori = static_cast<int>(cs->getOrientation());
-    if (ori % 2 == 0) af *= -1;
-    cs->waveform().setAmplitude(af);
-    // xt::linspace(0.5 dt, (T - 0.5) dt, T)
-    const size_t T = total_time_steps;
-    std::vector<double> times(T);
-    const double start = 0.5 * dt, stop = (T - 0.5) * dt;
-    const double step = (stop - start) / std::fmax(1.0, (double)T - 1.0);
-    for (size_t n = 0; n < T; ++n) times[n] = (n + 1 == T && T > 1) ? stop : start + step * (double)n;
-    cs->waveform().init(times);
-    si_.wave = cs->waveform().getValue();
+    double rf = 0;
+    if (kind == CS) {
+      // current_source.cpp:34-116
+      double af;
+      if (a == 0) { rf = value * nk * nj / ni; af = cs->waveform().getAmplitude() / (nk * nj); }
+      else if (a == 1) { rf = value * nk * ni / nj; af = cs->waveform().getAmplitude() / (nk * ni); }
+      else { rf = value * ni * nj / nk; af = cs->waveform().getAmplitude() / (ni * nj); }
+      if (static_cast<int>(cs->getOrientation()) % 2 == 0) af *= -1;
+      cs->waveform().setAmplitude(af);
+    } else if (kind == VS) {
+      // voltage_source.cpp:40-107 (its YP branch tests YN twice and never
+      // runs; taken as intended)
+      double af;
+      if (a == 0) { rf = value * nj * nk / ni; af = vs->waveform().getAmplitude() / ni; }
+      else if (a == 1) { rf = value * ni * nk / nj; af = vs->waveform().getAmplitude() / nj; }
+      else { rf = value * ni * nj / nk; af = vs->waveform().getAmplitude() / nk; }
+      if (static_cast<int>(vs->getOrientation()) % 2 == 0) af *= -1;
+      vs->waveform().setAmplitude(af);
+    } else if (kind == RES || kind == IND) {
+      // resistor.cpp:22-76, inductor.cpp:19-68
+      if (a == 0) rf = value * nj * nk / ni;
+      else if (a == 1) rf = value * ni * nk / nj;
+      else rf = value * ni * nj / nk;
+    } else {
+      // capacitor.cpp:25-61
+      if (a == 0) rf = value * ni / (nj * nk);
+      else if (a == 1) rf = value * nj / (ni * nk);
+      else rf = value * nk / (ni * nj);
+    }
+    if (kind == CS || kind == VS) {
+      // waveform times xt::linspace(0.5 dt, (T - 0.5) dt, T)
+      const size_t T = total_time_steps;
+      std::vector<double> times(T);
+      const double start = 0.5 * dt, stop = (T - 0.5) * dt;
+      const double step = (stop - start) / std::fmax(1.0, (double)T - 1.0);
+      for (size_t n = 0; n < T; ++n) times[n] = (n + 1 == T && T > 1) ? stop : start + step * (double)n;
+      Waveform &w = kind == CS ? cs->waveform() : vs->waveform();
+      w.init(times);
+      si_.wave = w.getValue();
+    }
     for (size_t i = st[0]; i < en[0]; ++i)
       for (size_t j = st[1]; j < en[1]; ++j)
         for (size_t k = st[2]; k < en[2]; ++k) {
-          const int mb = midx[i * si + j * sj + k];
-          if (drude_of[mb]) throw std::runtime_error("a current source on a Drude cell is not supported");
-          const Mat &m = mats[mb];
+          const size_t cidx = i * si + j * sj + k;
+          const int mb = midx[cidx];
+          if (drude_of[mb]) throw std::runtime_error("a lumped element on a Drude cell is not supported");
+          if (_has_design && i >= _design_box[0] && i < _design_box[1] && j >= _design_box[2] &&
+              j < _design_box[3] && k >= _design_box[4] && k < _design_box[5])
+            throw std::runtime_error("a lumped element in the design box is not supported");
+          const double eps = mats[mb].eps, sg = cell_sigma(cidx, a);
           double da, db, dc;
+          // the reference's Y branches of Resistor, Capacitor and Inductor
+          // assign _da twice and never _db; the intended (dz, dx) is taken
           if (a == 0) { da = _hs[1][j]; db = _hs[2][k]; dc = _es[0][i]; }
           else if (a == 1) { da = _hs[2][k]; db = _hs[0][i]; dc = _es[1][j]; }
           else { da = _hs[0][i]; db = _hs[1][j]; dc = _es[2][k]; }
-          const double beta = (dt * dc) / (rf * da * db);
-          const double eps = m.eps, sg = m.sigma;
-          Override ov;
-          ov.m_base = mb; ov.comp = a; ov.i = (int)i; ov.j = (int)j; ov.k = (int)k;
-          ov.ce = (2 * eps - dt * sg - beta) / (2 * eps + dt * sg + beta);
-          ov.ca = -2 * dt / ((2 * eps + dt * sg + beta) * db);
-          ov.cb = 2 * dt / ((2 * eps + dt * sg + beta) * da);
-          overrides.push_back(ov);
-          si_.coff.push_back(-2 * dt / ((2 * eps + dt * sg + beta) * (db * da)));
+          if (kind == IND) {
+            si_.c2.push_back(dt * dc / (rf * da * db));
+            si_.coff.push_back(-(2 * dt) / (2 * eps + dt * sg));
+            continue;
+          }
+          double ce, ca_, cb_;
+          if (kind == CAP) {
+            const double beta = 2 * rf * dc / (da * db);
+            ce = (2 * eps - dt * sg + beta) / (2 * eps + dt * sg + beta);
+            ca_ = -(2 * dt) / ((2 * eps + dt * sg + beta) * db);
+            cb_ = (2 * dt) / ((2 * eps + dt * sg + beta) * da);
+          } else {
+            double beta;
+            if (kind == VS) {
+              const double alpha = da * db * rf;
+              beta = dt * dc / alpha;
+              si_.coff.push_back(-2 * dt / ((2 * eps + dt * sg + beta) * alpha));
+            } else {
+              beta = (dt * dc) / (rf * da * db);
+              if (kind == CS) si_.coff.push_back(-2 * dt / ((2 * eps + dt * sg + beta) * (db * da)));
+            }
+            ce = (2 * eps - dt * sg - beta) / (2 * eps + dt * sg + beta);
+            ca_ = -2 * dt / ((2 * eps + dt * sg + beta) * db);
+            cb_ = 2 * dt / ((2 * eps + dt * sg + beta) * da);
+          }
+          CellOv &o = ovs[cidx];
+          o.base = mb;
+          o.kind[a] = 1;
+          o.v[a][0] = ce;
+          o.v[a][1] = ca_;
+          o.v[a][2] = cb_;
         }
-    srcs.push_back(std::move(si_));
+    if (kind != RES && kind != CAP) srcs.push_back(std::move(si_));
   }
+  for (const auto &e : ovs)
+    if (drude_of[e.second.base]) throw std::runtime_error("an ObjectPlane on a Drude cell is not supported");
+  // one slot per distinct (base, overrides): slots of equal overrides share
+  // their table entries, so only cells with identical values share a slot
+  std::map<std::vector<double>, int> slot_of;
+  std::vector<std::pair<size_t, int>> cell_slot;
+  std::vector<const CellOv *> slot_ov;
   const int nbase = (int)mats.size();
-  const int nmat = nbase + (int)overrides.size();
+  for (const auto &e : ovs) {
+    const CellOv &o = e.second;
+    std::vector<double> key = {(double)o.base};
+    for (int cc = 0; cc < 3; ++cc) {
+      key.push_back(o.kind[cc]);
+      for (int u = 0; u < 3; ++u) key.push_back(o.kind[cc] ? o.v[cc][u] : 0.0);
+    }
+    auto it = slot_of.find(key);
+    int slot;
+    if (it == slot_of.end()) {
+      slot = nbase + (int)slot_ov.size();
+      slot_of.emplace(std::move(key), slot);
+      slot_ov.push_back(&o);
+    } else {
+      slot = it->second;
+    }
+    cell_slot.emplace_back(e.first, slot);
+  }
+  const int nmat = nbase + (int)slot_ov.size();
   if (nmat > 255) throw std::runtime_error("more than 255 material slots");
 
   // ---- coefficient tables (fdtd_basic_coff.cpp:93-121)
@@ -566,23 +795,53 @@ void Simulation::plan(size_t total_time_steps) {
     }
   };
   for (int q = 0; q < nbase; ++q) fill_material(q, mats[q]);
-  for (size_t q = 0; q < overrides.size(); ++q) {
-    const Override &ov = overrides[q];
-    const int slot = nbase + (int)q;
-    fill_material(slot, mats[ov.m_base]);
-    // current_source.cpp:182-185: cece, cecha (db), cechb (da) of its component
-    ca[ov.comp * nmat + slot] = ov.ce;
-    if (ov.comp == 0) {
-      cb[XF_CB_EXHY][(size_t)slot * (nz + 1) + ov.k] = ov.ca;
-      cb[XF_CB_EXHZ][(size_t)slot * (ny + 1) + ov.j] = ov.cb;
-    } else if (ov.comp == 1) {
-      cb[XF_CB_EYHZ][(size_t)slot * (nx + 1) + ov.i] = ov.ca;
-      cb[XF_CB_EYHX][(size_t)slot * (nz + 1) + ov.k] = ov.cb;
+  // the override slots: the base material, then each plane component's
+  // coefficients from the plane's sigma_e (whole tables), then each lumped
+  // component's ce / ca (db) / cb (da) at the cells that use the slot
+  auto fill_e_comp = [&](int slot, int comp, const Mat &m) {
+    ca[comp * nmat + slot] = coffSelfE(m, dt);
+    if (comp == 0) {
+      for (int k = 0; k <= nz; ++k) cb[XF_CB_EXHY][(size_t)slot * (nz + 1) + k] = coffEa(m, dt, _hs[2][k]);
+      for (int j = 0; j <= ny; ++j) cb[XF_CB_EXHZ][(size_t)slot * (ny + 1) + j] = coffEb(m, dt, _hs[1][j]);
+    } else if (comp == 1) {
+      for (int i = 0; i <= nx; ++i) cb[XF_CB_EYHZ][(size_t)slot * (nx + 1) + i] = coffEa(m, dt, _hs[0][i]);
+      for (int k = 0; k <= nz; ++k) cb[XF_CB_EYHX][(size_t)slot * (nz + 1) + k] = coffEb(m, dt, _hs[2][k]);
     } else {
-      cb[XF_CB_EZHX][(size_t)slot * (ny + 1) + ov.j] = ov.ca;
-      cb[XF_CB_EZHY][(size_t)slot * (nx + 1) + ov.i] = ov.cb;
+      for (int j = 0; j <= ny; ++j) cb[XF_CB_EZHX][(size_t)slot * (ny + 1) + j] = coffEa(m, dt, _hs[1][j]);
+      for (int i = 0; i <= nx; ++i) cb[XF_CB_EZHY][(size_t)slot * (nx + 1) + i] = coffEb(m, dt, _hs[0][i]);
     }
-    midx[(size_t)ov.i * si + (size_t)ov.j * sj + ov.k] = (uint8_t)slot;
+  };
+  for (size_t q = 0; q < slot_ov.size(); ++q) {
+    const CellOv &o = *slot_ov[q];
+    const int slot = nbase + (int)q;
+    fill_material(slot, mats[o.base]);
+    for (int cc = 0; cc < 3; ++cc)
+      if (o.kind[cc] == 2) {
+        Mat m = mats[o.base];
+        m.sigma = o.v[cc][0];
+        fill_e_comp(slot, cc, m);
+      }
+  }
+  for (const auto &cs_ : cell_slot) {
+    const size_t cidx = cs_.first;
+    const int slot = cs_.second;
+    const CellOv &o = *slot_ov[slot - nbase];
+    const int i = (int)(cidx / si), j = (int)((cidx % si) / sj), k = (int)(cidx % sj);
+    for (int cc = 0; cc < 3; ++cc) {
+      if (o.kind[cc] != 1) continue;
+      ca[cc * nmat + slot] = o.v[cc][0];
+      if (cc == 0) {
+        cb[XF_CB_EXHY][(size_t)slot * (nz + 1) + k] = o.v[cc][1];
+        cb[XF_CB_EXHZ][(size_t)slot * (ny + 1) + j] = o.v[cc][2];
+      } else if (cc == 1) {
+        cb[XF_CB_EYHZ][(size_t)slot * (nx + 1) + i] = o.v[cc][1];
+        cb[XF_CB_EYHX][(size_t)slot * (nz + 1) + k] = o.v[cc][2];
+      } else {
+        cb[XF_CB_EZHX][(size_t)slot * (ny + 1) + j] = o.v[cc][1];
+        cb[XF_CB_EZHY][(size_t)slot * (nx + 1) + i] = o.v[cc][2];
+      }
+    }
+    midx[cidx] = (uint8_t)slot;
   }
 
   // ---- CPML (perfect_match_layer.cpp:49-96, 157-431): per axis, capture
@@ -643,7 +902,9 @@ void Simulation::plan(size_t total_time_steps) {
     const int lo[3] = {s.is, s.js, s.ks}, hi[3] = {s.ie, s.je, s.ke};
     for (int a = 0; a < 3; ++a)
       for (int n = lo[a]; n < hi[a]; ++n)
-        if (in_slab_e(a, n)) throw std::runtime_error("current source inside a CPML slab is not supported");
+        if (in_slab_e(a, n))
+          throw std::runtime_error(s.kind == 1 ? "inductor inside a CPML slab is not supported"
+                                               : "current source inside a CPML slab is not supported");
   }
 
   // ---- design box coefficients: fdtd_basic_coff.cpp:77-91 for each cell's
@@ -737,9 +998,11 @@ void Simulation::plan(size_t total_time_steps) {
   }
   for (auto &s_ : srcs) {
     PlanData::Src ps;
+    ps.kind = s_.kind;
     ps.comp = s_.comp; ps.is = s_.is; ps.ie = s_.ie; ps.js = s_.js; ps.je = s_.je; ps.ks = s_.ks; ps.ke = s_.ke;
     ps.coff = std::move(s_.coff);
     ps.wave = std::move(s_.wave);
+    ps.c2 = std::move(s_.c2);
     P.srcs.push_back(std::move(ps));
   }
 
@@ -878,9 +1141,32 @@ void Simulation::upload() {
       check(xf_engine_set_cpml(_engine, a, P.dn[a], P.dp[a], P.be[a].data(), P.ae[a].data(), P.bm[a].data(),
                                P.am[a].data()));
   check(xf_engine_set_materials(_engine, P.midx.data(), P.ca.data(), cbp, cpp));
-  for (const auto &s : P.srcs)
-    check(xf_engine_add_current_source(_engine, s.comp, s.is, s.ie, s.js, s.je, s.ks, s.ke, s.coff.data(),
-                                       s.wave.data(), (int64_t)_total_time_steps));
+  for (const auto &s : P.srcs) {
+    if (s.kind == 1)
+      check(xf_engine_add_inductor(_engine, s.comp, s.is, s.ie, s.js, s.je, s.ks, s.ke, s.coff.data(),
+                                   s.c2.data()));
+    else
+      check(xf_engine_add_current_source(_engine, s.comp, s.is, s.ie, s.js, s.je, s.ks, s.ke, s.coff.data(),
+                                         s.wave.data(), (int64_t)_total_time_steps));
+  }
+  for (const LineMonitorPlan &lp : _line_plans) {
+    const size_t n = lp.terms.size();
+    std::vector<int> comp(n), group(n);
+    std::vector<int64_t> a(3 * n), b(3 * n);
+    std::vector<double> w(n);
+    for (size_t q = 0; q < n; ++q) {
+      comp[q] = lp.terms[q].comp;
+      group[q] = lp.terms[q].group;
+      w[q] = lp.terms[q].w;
+      for (int u = 0; u < 3; ++u) {
+        a[3 * q + u] = lp.terms[q].a[u];
+        b[3 * q + u] = lp.terms[q].b[u];
+      }
+    }
+    int index = 0;
+    check(xf_engine_add_line_monitor(_engine, (int)n, comp.data(), a.data(), b.data(), w.data(), group.data(),
+                                     lp.sign, (int64_t)_total_time_steps, &index));
+  }
   if (P.tfsf.present) {
     xf_tfsf_desc td = P.tfsf.desc;
     for (int a = 0; a < 3; ++a) {
@@ -1254,10 +1540,83 @@ void makeDirs(const std::string &path) {
 
 // grid boxes of the monitors (GridSpace::getGridBox, grid_space.cpp:306-329)
 // and the plane check of TimeDomainFieldMonitor::init (field_monitor.cpp:30-87)
+// VoltageMonitor / CurrentMonitor as term lists for the engine, in the
+// reference's loop order (monitor/voltage_monitor.cpp:20-93,
+// monitor/current_monitor.cpp:20-122)
+void Simulation::planLineMonitors() {
+  _line_plans.clear();
+  const int64_t N[3] = {(int64_t)_n[0], (int64_t)_n[1], (int64_t)_n[2]};
+  for (const auto &mp : _monitors) {
+    auto *vm = dynamic_cast<VoltageMonitor *>(mp.get());
+    auto *cm = dynamic_cast<CurrentMonitor *>(mp.get());
+    if (!vm && !cm) continue;
+    const Cube *c = mp->getShape();
+    if (!c) throw std::runtime_error("Shape Type Error");
+    const double lo[3] = {c->getXmin(), c->getYmin(), c->getZmin()};
+    const double hi[3] = {c->getXmax(), c->getYmax(), c->getZmax()};
+    int64_t st[3], en[3];
+    for (int a = 0; a < 3; ++a) {
+      st[a] = (int64_t)handleTransform(a, lo[a]);
+      en[a] = (int64_t)handleTransform(a, hi[a]);
+      if (en[a] < st[a]) throw std::range_error("end < start");
+    }
+    const int ori = static_cast<int>(vm ? vm->getOrientation() : cm->getOrientation());
+    const int a = ori / 2;
+    LineMonitorPlan lp;
+    lp.monitor = mp.get();
+    auto term = [&](int comp, int group, int64_t i, int64_t j, int64_t k, int64_t bi, int64_t bj, int64_t bk,
+                    double w) {
+      LineTermPlan t;
+      t.comp = comp;
+      t.group = group;
+      t.a[0] = i; t.a[1] = j; t.a[2] = k;
+      t.b[0] = bi; t.b[1] = bj; t.b[2] = bk;
+      t.w = w;
+      lp.terms.push_back(t);
+    };
+    const int64_t is = st[0], ie = en[0], js = st[1], je = en[1], ks = st[2], ke = en[2];
+    if (vm) {
+      // value += coff(c) E_a(i,j,k), coff = -dc / (cells across), negated for N
+      const double across = (double)(a == 0 ? (je - js) * (ke - ks) : a == 1 ? (ie - is) * (ke - ks)
+                                                                             : (ie - is) * (je - js));
+      for (int64_t i = is; i < ie; ++i)
+        for (int64_t j = js; j < je; ++j)
+          for (int64_t k = ks; k < ke; ++k) {
+            const int64_t idx = a == 0 ? i : (a == 1 ? j : k);
+            double w = -_hs[a][idx] / across;
+            if (ori % 2 == 0) w *= -1;
+            term(a, 0, i, j, k, -1, -1, -1, w);
+          }
+      if (lp.terms.empty()) throw std::runtime_error("voltage monitor box has no cells");
+      lp.sign = 1.0;
+    } else {
+      // the loop integral of H around the box at its start plane
+      bool ok = st[a] < N[a];
+      for (int b = 0; b < 3; ++b)
+        if (b != a && (st[b] < 1 || en[b] + 1 > N[b])) ok = false;
+      if (!ok) throw std::out_of_range("current monitor loop outside the grid");
+      if (a == 0) {
+        for (int64_t j = js; j <= je; ++j) term(4, 0, is, j, ks - 1, is, j, ke, _es[1][j]);
+        for (int64_t k = ks; k <= ke; ++k) term(5, 1, is, je, k, is, js - 1, k, _es[2][k]);
+      } else if (a == 1) {
+        for (int64_t k = ks; k <= ke; ++k) term(5, 0, is - 1, js, k, ie, js, k, _es[2][k]);
+        for (int64_t i = is; i <= ie; ++i) term(3, 1, i, js, ke, i, js, ks - 1, _es[0][i]);
+      } else {
+        for (int64_t i = is; i <= ie; ++i) term(3, 0, i, js - 1, ks, i, je, ks, _es[0][i]);
+        for (int64_t j = js; j <= je; ++j) term(4, 1, ie, j, ks, is - 1, j, ks, _es[1][j]);
+      }
+      lp.sign = ori % 2 == 0 ? -1.0 : 1.0;
+    }
+    _line_plans.push_back(std::move(lp));
+  }
+}
+
 void Simulation::planMonitors() {
   _monitor_plans.clear();
+  planLineMonitors();
   for (const auto &mp : _monitors) {
     const Monitor *mon = mp.get();
+    if (dynamic_cast<const VoltageMonitor *>(mon) || dynamic_cast<const CurrentMonitor *>(mon)) continue;
     size_t plot = 0, total = 0;
     if (auto *mv = dynamic_cast<const MovieMonitor *>(mon)) {
       if (mv->getPlotStep() == 0) throw std::invalid_argument("MovieMonitor plot_step must be positive");
@@ -1266,7 +1625,9 @@ void Simulation::planMonitors() {
       mon = &mv->getMonitor();
     }
     auto *fm = dynamic_cast<const TimeDomainFieldMonitor *>(mon);
-    if (!fm) throw std::runtime_error("only TimeDomainFieldMonitor and MovieMonitor are supported");
+    if (!fm)
+      throw std::runtime_error(
+          "only TimeDomainFieldMonitor, MovieMonitor, VoltageMonitor and CurrentMonitor are supported");
     const Cube *c = fm->getShape();
     if (!c) throw std::runtime_error("Shape Type Error");
     const double lo[3] = {c->getXmin(), c->getYmin(), c->getZmin()};
@@ -1312,15 +1673,89 @@ void Simulation::writeMonitor(const MonitorPlan &m, const std::string &file) con
 // nffft_fd.cpp:579-634, then the field monitors' final slices
 // (field_monitor.cpp:91-121)
 void Simulation::outputData() const {
-  if (_engine)
+  // simulation.cpp:144-156: NF-FF, then the monitors, then the network
+  outputNffft();
+  if (_engine) {
     for (const MonitorPlan &m : _monitor_plans)
       if (m.plot_step == 0) writeMonitor(m, m.monitor->getOutputFileName());
+    if (!_line_plans.empty()) {
+      getLineMonitorData();
+      for (const LineMonitorPlan &lp : _line_plans) writeLineMonitor(lp);
+    }
+    if (_network) {
+      if (_line_plans.empty()) getLineMonitorData();
+      outputNetwork();
+    }
+  }
+}
+
+// VoltageMonitor::outputData (voltage_monitor.cpp:95-117): time
+// linspace(0, T dt, T); CurrentMonitor::outputData (current_monitor.cpp:
+// 124-138): linspace(dt/2, (T + 1/2) dt, T); rows [time, value]
+void Simulation::writeLineMonitor(const LineMonitorPlan &lp) const {
+  const Monitor *m = lp.monitor;
+  const bool voltage = dynamic_cast<const VoltageMonitor *>(m) != nullptr;
+  const std::vector<double> &v = voltage ? static_cast<const VoltageMonitor *>(m)->getValueArray()
+                                         : static_cast<const CurrentMonitor *>(m)->getValueArray();
+  const size_t T = _total_time_steps;
+  std::vector<double> out(2 * T);
+  const double start = voltage ? 0.0 : 0.5 * _dt;
+  const double stop = voltage ? (double)T * _dt : ((double)T + 0.5) * _dt;
+  const double step = (stop - start) / std::fmax(1.0, (double)T - 1.0);
+  for (size_t n = 0; n < T; ++n) out[n] = (n + 1 == T && T > 1) ? stop : start + step * (double)n;
+  for (size_t n = 0; n < T; ++n) out[T + n] = v[n];
+  try {
+    makeDirs(m->getOutputPath());
+  } catch (const std::exception &) {
+    std::fprintf(stderr, "Error: cannot create directory %s\n", m->getOutputPath().c_str());
+    return;
+  }
+  writeNpy(m->getOutputPath() + "/" + m->getOutputFileName(), out.data(), "<f8", 8, {2, T});
+}
+
+// Network::outputData (network.cpp:27-60): s{10 j + i}.npy = [frequencies; S]
+void Simulation::outputNetwork() const {
+  auto s = _network->calculateSParameters(_dt);
+  const std::string &dir = _network->getOutputPath();
+  try {
+    makeDirs(dir);
+  } catch (const std::exception &) {
+    std::fprintf(stderr, "Error: cannot create directory %s\n", dir.c_str());
+    return;
+  }
+  const auto &f = _network->frequencies();
+  for (const auto &e : s) {
+    std::vector<std::complex<double>> data(2 * f.size());
+    for (size_t q = 0; q < f.size(); ++q) {
+      data[q] = f[q];
+      data[f.size() + q] = e.second[q];
+    }
+    writeNpy(dir + "/s" + std::to_string(e.first) + ".npy", data.data(), "<c16", 16, {2, f.size()});
+  }
+}
+
+std::vector<std::vector<double>> Simulation::getLineMonitorData() const {
+  if (!_engine) throw std::runtime_error("Simulation is not initialised");
+  const size_t T = _total_time_steps;
+  std::vector<double> buf(_line_plans.size() * T);
+  if (!_line_plans.empty()) check(xf_engine_get_line_monitors(_engine, buf.data(), (int64_t)T));
+  std::vector<std::vector<double>> out(_line_plans.size());
+  for (size_t q = 0; q < _line_plans.size(); ++q) {
+    out[q].assign(buf.begin() + q * T, buf.begin() + (q + 1) * T);
+    Monitor *m = _line_plans[q].monitor;
+    if (auto *vm = dynamic_cast<VoltageMonitor *>(m)) vm->_value = out[q];
+    if (auto *cm = dynamic_cast<CurrentMonitor *>(m)) cm->_value = out[q];
+  }
+  return out;
+}
+
+void Simulation::outputNffft() const {
   if (auto *bb = dynamic_cast<NffftBroadBand *>(_nffft.get())) {
     if (_engine && !bb->getOutputDirPath().empty()) outputBroadBand(*bb);
     return;
   }
   auto *nf = dynamic_cast<NffftFd *>(_nffft.get());
-  if (!nf || nf->getOutputDirPath().empty()) return;
+  if (!_engine || !nf || nf->getOutputDirPath().empty()) return;
   const FarField r = calculateFarField();
   const std::string dir = nf->getOutputDirPath();
   try {

@@ -11,6 +11,8 @@ using namespace xfdtd;
 struct xf_sim {
   Simulation sim;
   std::shared_ptr<NffftFd> nffft;
+  std::vector<std::shared_ptr<Monitor>> line_monitors;  // voltage / current monitors, in order
+  std::shared_ptr<Network> network;
   xf_sim(double dl, float cfl) : sim(dl, cfl) {}
 };
 
@@ -154,6 +156,115 @@ extern "C" int xf_sim_add_current_source(xf_sim *s, const double origin[3], cons
                                PointVector{size[0], size[1], size[2]}),
         static_cast<Orientation>(orientation), resistance,
         std::make_unique<GaussianWaveform>(amplitude, tau, t0)));
+  });
+}
+
+static std::unique_ptr<Cube> cube_of(const double origin[3], const double size[3]) {
+  return std::make_unique<Cube>(PointVector{origin[0], origin[1], origin[2]}, PointVector{size[0], size[1], size[2]});
+}
+
+static std::unique_ptr<Waveform> waveform_of(int wkind, const double *wp) {
+  if (wkind == 1) return std::make_unique<SinusoidalWaveform>(wp[0], wp[1], wp[2]);
+  if (wkind == 2) return std::make_unique<CosineModulatedGaussianWaveform>(wp[0], wp[1], wp[2], wp[3]);
+  if (wkind != 0) throw std::invalid_argument("bad waveform");
+  return std::make_unique<GaussianWaveform>(wp[0], wp[1], wp[2]);
+}
+
+extern "C" int xf_sim_add_lumped(xf_sim *s, int kind, const double origin[3], const double size[3], int dir,
+                                 double value, int wkind, const double wp[4]) {
+  return guarded([&] {
+    if (kind <= 1 ? (dir < 0 || dir > 5) : (dir < 0 || dir > 2)) throw std::invalid_argument("bad orientation or axis");
+    std::shared_ptr<LumpedElement> le;
+    switch (kind) {
+      case 0:
+        le = std::make_shared<CurrentSource>(cube_of(origin, size), static_cast<Orientation>(dir), value,
+                                             waveform_of(wkind, wp));
+        break;
+      case 1:
+        le = std::make_shared<VoltageSource>(cube_of(origin, size), static_cast<Orientation>(dir), value,
+                                             waveform_of(wkind, wp));
+        break;
+      case 2: le = std::make_shared<Resistor>(cube_of(origin, size), static_cast<Axis>(dir), value); break;
+      case 3: le = std::make_shared<Capacitor>(cube_of(origin, size), static_cast<Axis>(dir), value); break;
+      case 4: le = std::make_shared<Inductor>(cube_of(origin, size), static_cast<Axis>(dir), value); break;
+      default: throw std::invalid_argument("bad lumped element kind");
+    }
+    s->sim.addLumpedElement(std::move(le));
+  });
+}
+
+extern "C" int xf_sim_add_plane(xf_sim *s, const double origin[3], const double size[3], float eps_r, float mu_r,
+                                double sigma_e, double sigma_m) {
+  return guarded([&] {
+    s->sim.addObject(std::make_shared<ObjectPlane>("plane", cube_of(origin, size), mat(eps_r, mu_r, sigma_e, sigma_m)));
+  });
+}
+
+extern "C" int xf_sim_add_line_monitor(xf_sim *s, int kind, const double origin[3], const double size[3],
+                                       int orientation, const char *dir, const char *file) {
+  return guarded([&] {
+    if (orientation < 0 || orientation > 5 || kind < 0 || kind > 1) throw std::invalid_argument("bad monitor");
+    std::shared_ptr<Monitor> m;
+    const std::string d = dir ? dir : "", f = file ? file : "";
+    if (kind == 0)
+      m = std::make_shared<VoltageMonitor>(cube_of(origin, size), static_cast<Orientation>(orientation), d, f);
+    else
+      m = std::make_shared<CurrentMonitor>(cube_of(origin, size), static_cast<Orientation>(orientation), d, f);
+    s->line_monitors.push_back(m);
+    s->sim.addMonitor(m);
+  });
+}
+
+extern "C" int xf_sim_get_line_monitors(xf_sim *s, double *out) {
+  return guarded([&] {
+    auto v = s->sim.getLineMonitorData();
+    size_t off = 0;
+    for (const auto &m : v) {
+      std::memcpy(out + off, m.data(), m.size() * sizeof(double));
+      off += m.size();
+    }
+  });
+}
+
+extern "C" int xf_sim_set_network(xf_sim *s, int nports, const int *index, const double *impedance,
+                                  const int *is_source, const int *vmon, const int *imon, const double *freqs,
+                                  int nf, const char *output_dir) {
+  return guarded([&] {
+    std::vector<std::unique_ptr<Port>> ports;
+    const int nm = (int)s->line_monitors.size();
+    for (int q = 0; q < nports; ++q) {
+      if (vmon[q] < 0 || vmon[q] >= nm || imon[q] < 0 || imon[q] >= nm) throw std::invalid_argument("bad port monitor");
+      auto v = std::dynamic_pointer_cast<VoltageMonitor>(s->line_monitors[vmon[q]]);
+      auto c = std::dynamic_pointer_cast<CurrentMonitor>(s->line_monitors[imon[q]]);
+      if (!v || !c) throw std::invalid_argument("a port needs a voltage and a current monitor");
+      ports.push_back(std::make_unique<Port>(index[q], std::complex<double>(impedance[2 * q], impedance[2 * q + 1]),
+                                             is_source[q] != 0, v, c));
+    }
+    s->network = std::make_shared<Network>(std::move(ports), std::vector<double>(freqs, freqs + nf),
+                                           output_dir ? std::string(output_dir) : std::string());
+    s->sim.addNetwork(s->network);
+  });
+}
+
+extern "C" int xf_sim_network_size(const xf_sim *s) {
+  if (!s->network) return 0;
+  int src = 0;
+  for (const auto &p : s->network->ports()) src += p->isActivatedSource() ? 1 : 0;
+  return src * (int)s->network->ports().size();
+}
+
+extern "C" int xf_sim_network_sparams(xf_sim *s, int *keys, double *out) {
+  return guarded([&] {
+    if (!s->network) throw std::runtime_error("no network");
+    s->sim.getLineMonitorData();
+    auto sp = s->network->calculateSParameters(s->sim.getDt());
+    const size_t nf = s->network->frequencies().size();
+    size_t q = 0;
+    for (const auto &e : sp) {
+      keys[q] = e.first;
+      std::memcpy(out + 2 * nf * q, e.second.data(), nf * sizeof(std::complex<double>));
+      ++q;
+    }
   });
 }
 

@@ -20,6 +20,13 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def _index_of(seq, item):
+    for q, x in enumerate(seq):
+        if x is item:
+            return q
+    raise ValueError("a port's monitor must be one of the scene's monitors")
+
+
 def _f64(v):
     return np.ascontiguousarray(np.array(v, dtype=np.float64))
 
@@ -43,6 +50,9 @@ class Simulation:
                         self._chk(L.xf_sim_add_drude_sphere(self.h, _dp(_f64(sh.center)), float(sh.radius), *drude))
                     else:
                         self._chk(L.xf_sim_add_drude_cube(self.h, _dp(_f64(sh.origin)), _dp(_f64(sh.size)), *drude))
+                elif type(o).__name__ == "ObjectPlane":
+                    self._chk(L.xf_sim_add_plane(self.h, _dp(_f64(sh.origin)), _dp(_f64(sh.size)), m.eps_r,
+                                                 m.mu_r, m.sigma_e, m.sigma_m))
                 elif hasattr(sh, "radius"):
                     c = _f64(sh.center)
                     self._chk(L.xf_sim_add_sphere(self.h, _dp(c), float(sh.radius), m.eps_r, m.mu_r,
@@ -71,14 +81,23 @@ class Simulation:
                 od = output_dir.encode() if output_dir else None
                 self._chk(L.xf_sim_set_nffft_fd(self.h, dist, _dp(fr), len(fr), _dp(th), len(th),
                                                 _dp(ph), len(ph), od))
+            from xfdtd_amd.scene import lumped_desc
             for s in scene.sources:
                 p, sz = _f64(s.cube.origin), _f64(s.cube.size)
-                w = s.waveform
-                self._chk(L.xf_sim_add_current_source(self.h, _dp(p), _dp(sz), int(s.orientation),
-                                                      float(s.resistance), w.amplitude, w.tau, w.t_0))
+                kind, dr, val, wk, wp = lumped_desc(s)
+                self._chk(L.xf_sim_add_lumped(self.h, kind, _dp(p), _dp(sz), int(dr), float(val), wk,
+                                              _dp(_f64(wp))))
             for p in scene.probes:
                 self._chk(L.xf_sim_add_probe(self.h, int(p.comp), p.i, p.j, p.k))
+            self.line_monitors = []
             for m in getattr(scene, "monitors", []):
+                lk = {"VoltageMonitor": 0, "CurrentMonitor": 1}.get(type(m).__name__)
+                if lk is not None:
+                    self._chk(L.xf_sim_add_line_monitor(self.h, lk, _dp(_f64(m.cube.origin)), _dp(_f64(m.cube.size)),
+                                                        int(m.orientation), m.output_dir.encode(),
+                                                        m.output_file_name.encode()))
+                    self.line_monitors.append(m)
+                    continue
                 inner = m.monitor if hasattr(m, "plot_step") else m
                 org, siz = _f64(inner.cube.origin), _f64(inner.cube.size)
                 d = inner.output_dir.encode()
@@ -89,6 +108,17 @@ class Simulation:
                 else:
                     self._chk(L.xf_sim_add_field_monitor(self.h, _dp(org), _dp(siz), int(inner.plane_type),
                                                          int(inner.comp), d, inner.output_file_name.encode()))
+            net = getattr(scene, "network", None)
+            if net is not None:
+                n = len(net.ports)
+                ids = (C.c_int * n)(*[int(p.index) for p in net.ports])
+                imp = _f64([x for p in net.ports for x in (complex(p.impedance).real, complex(p.impedance).imag)])
+                src = (C.c_int * n)(*[1 if p.is_source else 0 for p in net.ports])
+                vm = (C.c_int * n)(*[_index_of(self.line_monitors, p.voltage) for p in net.ports])
+                im = (C.c_int * n)(*[_index_of(self.line_monitors, p.current) for p in net.ports])
+                fr = _f64(net.frequencies)
+                self._chk(L.xf_sim_set_network(self.h, n, ids, _dp(imp), src, vm, im, _dp(fr), len(fr),
+                                               net.output_path.encode()))
             dz = getattr(scene, "design", None)
             if dz is not None:
                 box = (C.c_size_t * 6)(*[int(v) for v in dz.box])
@@ -206,6 +236,26 @@ class Simulation:
         r["power_theta"] = out[30 * Ns:31 * Ns].copy()
         r["power_phi"] = out[31 * Ns:].copy()
         return r
+
+    # ---- voltage / current monitors and networks
+    def get_line_monitors(self):
+        """[monitor][step] values of the voltage and current monitors, in the
+        order they appear among the scene's monitors."""
+        out = np.empty((len(self.line_monitors), self.T), dtype=np.float64)
+        if len(self.line_monitors):
+            self._chk(lib().xf_sim_get_line_monitors(self.h, _dp(out)))
+        return out
+
+    def network_sparams(self):
+        """{10 j + i: S_ji over the network's frequencies}."""
+        L = lib()
+        n = L.xf_sim_network_size(self.h)
+        nf = len(self.scene.network.frequencies)
+        keys = (C.c_int * max(n, 1))()
+        out = np.empty(2 * max(n, 1) * nf, dtype=np.float64)
+        self._chk(L.xf_sim_network_sparams(self.h, keys, _dp(out)))
+        v = out.view(np.complex128).reshape(max(n, 1), nf)
+        return {int(keys[q]): v[q].copy() for q in range(n)}
 
     def output_nffft(self):
         self._chk(lib().xf_sim_output_nffft(self.h))
