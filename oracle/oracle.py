@@ -32,6 +32,8 @@ def lib():
         L.xo_sim_add_sphere.argtypes = [vp, dp, d, C.c_float, C.c_float, d, d]
         L.xo_sim_add_pml.argtypes = [vp, i, i, i, d, d, d, d]
         L.xo_sim_set_tfsf.argtypes = [vp, szp, d, d, d, d, d, d, d]
+        L.xo_sim_add_cylinder.argtypes = [vp, C.c_int, dp, d, d, C.c_float, C.c_float, d, d]
+        L.xo_sim_set_tfsf2d.argtypes = [vp, C.c_size_t, C.c_size_t, d, d, C.c_int, dp]
         L.xo_sim_set_nffft.argtypes = [vp, szp, dp, i, dp, i, dp, i]
         L.xo_sim_set_nffft_broadband.argtypes = [vp, szp, d, d]
         L.xo_sim_bb_frequencies.argtypes = [vp, dp, dp]
@@ -113,7 +115,11 @@ class OracleSim:
             for o in scene.objects:
                 m = o.material
                 sh = o.shape
-                if hasattr(sh, "radius"):
+                if hasattr(sh, "height"):
+                    c = _arr3(sh.center)
+                    self._chk(L.xo_sim_add_cylinder(self.h, int(sh.axis), _dp(c), float(sh.radius),
+                                                    float(sh.height), m.eps_r, m.mu_r, m.sigma_e, m.sigma_m))
+                elif hasattr(sh, "radius"):
                     c = _arr3(sh.center)
                     self._chk(L.xo_sim_add_sphere(self.h, _dp(c), float(sh.radius),
                                                   m.eps_r, m.mu_r, m.sigma_e, m.sigma_m))
@@ -132,7 +138,13 @@ class OracleSim:
                                            int(b.order), b.sigma_ratio, b.alpha_min,
                                            b.alpha_max, b.kappa_max))
             t = scene.tfsf
-            if t is not None:
+            if t is not None and hasattr(t, "phi_inc"):
+                from xfdtd_amd.scene import waveform_desc
+                wk, wp = waveform_desc(t.waveform)
+                wpa = _arr3(wp)
+                self._chk(L.xo_sim_set_tfsf2d(self.h, t.distance_x, t.distance_y, float(t.phi_inc),
+                                              float(t.ez_i0), wk, _dp(wpa)))
+            elif t is not None:
                 dist = (C.c_size_t * 3)(t.distance_x, t.distance_y, t.distance_z)
                 w = t.waveform
                 self._chk(L.xo_sim_set_tfsf(self.h, dist, t.e_0, t.theta, t.phi, t.psi,

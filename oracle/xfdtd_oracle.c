@@ -98,9 +98,10 @@ static void a3_free(a3 *a) {
 }
 
 typedef struct {
-  int kind; /* 0 cube, 1 sphere */
+  int kind; /* 0 cube, 1 sphere, 2 cylinder (p: centre, r: radius, h: height) */
   int plane; /* ObjectPlane: a zero-thickness cube that sets sigma_e only */
-  double p[3], s[3], r;
+  int axis;  /* cylinder axis: 0 x, 1 y, 2 z */
+  double p[3], s[3], r, h;
   float eps_r, mu_r;
   double sigma_e, sigma_m;
   /* DrudeMedium(eps_inf, omega_p, nv) when drude != 0 */
@@ -123,6 +124,10 @@ typedef struct {
 typedef struct {
   size_t dist[3];
   double e0, theta, phi, psi, amp, tau, t0;
+  /* the waveform (kinds of lumped_wave: 0 Gaussian, 1 sinusoid, 2
+     cosine-modulated Gaussian, whose frequency is wp3); dim2: TFSF2D */
+  int wkind, dim2;
+  double wp3;
   size_t li, ri, lj, rj, lk, rk;
   double te[3], th[3], ratio, scaled_dl, ceihi, chiei, ceie, chih, a, b;
   size_t L;
@@ -186,6 +191,9 @@ struct xo_sim {
   double minc[3], maxc[3];
   size_t n[3];
   double *ec[3], *hc[3], *es[3], *hs[3];
+  /* TWO_DIMENSION grid (grid/grid_space.cpp:501-536): nz == 1, the TMz
+     fields Ez, Hx, Hy carry the solution and Ex, Ey, Hz stay zero */
+  int dim2;
   double dt;
   size_t T, cur;
   int inited;
@@ -347,6 +355,21 @@ int xo_sim_add_sphere(xo_sim *s, const double center[3], double radius,
   return add_obj(s, o);
 }
 
+/* shape/cylinder.cpp:11-15 */
+int xo_sim_add_cylinder(xo_sim *s, int axis, const double center[3], double radius, double height,
+                        float eps_r, float mu_r, double sigma_e, double sigma_m) {
+  if (axis < 0 || axis > 2) FAIL("bad cylinder axis");
+  obj_t o;
+  memset(&o, 0, sizeof o);
+  o.kind = 2;
+  o.axis = axis;
+  for (int a = 0; a < 3; ++a) o.p[a] = center[a];
+  o.r = radius;
+  o.h = height;
+  o.eps_r = eps_r; o.mu_r = mu_r; o.sigma_e = sigma_e; o.sigma_m = sigma_m;
+  return add_obj(s, o);
+}
+
 int xo_sim_add_pml(xo_sim *s, int orientation, int thickness, int order,
                    double sigma_ratio, double alpha_min, double alpha_max,
                    double kappa_max) {
@@ -372,6 +395,25 @@ int xo_sim_set_tfsf(xo_sim *s, const size_t distance[3], double e0,
   for (int a = 0; a < 3; ++a) t->dist[a] = distance[a];
   t->e0 = e0; t->theta = theta; t->phi = phi; t->psi = psi;
   t->amp = amplitude; t->tau = tau; t->t0 = t0;
+  s->has_tfsf = 1;
+  return 0;
+}
+
+/* TFSF2D(distance_x, distance_y, phi_inc, ez_i0, waveform)
+   (tfsf/tfsf_2d.cpp:10-24): a TFSF with theta = pi/2, psi = 0 and no z
+   faces.  wp = (amplitude, tau or frequency, t0 or phase, modulation
+   frequency) of waveform kind wkind. */
+int xo_sim_set_tfsf2d(xo_sim *s, size_t distance_x, size_t distance_y, double phi, double ez_i0, int wkind,
+                      const double wp[4]) {
+  if (s->inited) FAIL("simulation already initialised");
+  if (wkind < 0 || wkind > 2) FAIL("bad waveform kind");
+  tfsf_t *t = &s->tf;
+  memset(t, 0, sizeof *t);
+  t->dist[0] = distance_x; t->dist[1] = distance_y; t->dist[2] = 0;
+  t->e0 = ez_i0; t->theta = XO_PI / 2; t->phi = phi; t->psi = 0;
+  t->wkind = wkind;
+  t->amp = wp[0]; t->tau = wp[1]; t->t0 = wp[2]; t->wp3 = wp[3];
+  t->dim2 = 1;
   s->has_tfsf = 1;
   return 0;
 }
@@ -482,6 +524,14 @@ static void wrapped_box(const obj_t *o, double mn[3], double mx[3]) {
   if (o->kind == 0) {
     /* shape/cube.cpp:80-82; cube getXmax = point + size */
     for (int a = 0; a < 3; ++a) { mn[a] = o->p[a]; mx[a] = o->p[a] + o->s[a]; }
+  } else if (o->kind == 2) {
+    /* shape/cylinder.cpp:44-60: radius across, height/2 either side along
+       the axis; the cube's max is point + size */
+    for (int a = 0; a < 3; ++a) {
+      double half = (a == o->axis) ? o->h / 2 : o->r, full = (a == o->axis) ? o->h : 2 * o->r;
+      mn[a] = o->p[a] - half;
+      mx[a] = mn[a] + full;
+    }
   } else {
     /* shape/sphere.cpp:32-36 */
     for (int a = 0; a < 3; ++a) {
@@ -499,6 +549,17 @@ static int is_inside(const obj_t *o, const double pt[3]) {
            is_greater_or_equal(o->p[0] + o->s[0], pt[0]) &&
            is_greater_or_equal(o->p[1] + o->s[1], pt[1]) &&
            is_greater_or_equal(o->p[2] + o->s[2], pt[2]);
+  }
+  if (o->kind == 2) {
+    /* shape/cylinder.cpp:21-42: inside the wrapped box, then within the
+       radius of the axis */
+    double mn[3], mx[3];
+    wrapped_box(o, mn, mx);
+    for (int a = 0; a < 3; ++a)
+      if (!(is_less_or_equal(mn[a], pt[a]) && is_greater_or_equal(mx[a], pt[a]))) return 0;
+    int u = o->axis == 0 ? 1 : 0, v = o->axis == 2 ? 1 : 2;
+    double du = pt[u] - o->p[u], dv = pt[v] - o->p[v];
+    return is_less_or_equal(sqrt(du * du + dv * dv), o->r);
   }
   /* shape/sphere.cpp:27-30 */
   double dx = pt[0] - o->p[0], dy = pt[1] - o->p[1], dz = pt[2] - o->p[2];
@@ -524,11 +585,14 @@ static int init_grid(xo_sim *s) {
     if (ori % 2 == 0) mn[a] = s->minc[a] - size; else mx[a] = s->maxc[a] + size;
     resize_space(s, mn, mx);
   }
-  /* grid/grid_space.cpp:501-536: only the 3-D branch is on this path */
-  for (int a = 0; a < 3; ++a) {
+  /* grid/grid_space.cpp:501-536: THREE_DIMENSION when every axis has more
+     than one cell, TWO_DIMENSION (the TMz branch) when only z has one */
+  for (int a = 0; a < 3; ++a)
     if (s->n[a] < 1) s->n[a] = 1;
-    if (s->n[a] <= 1) FAIL("only 3-D grids are supported (axis %d has %zu cells)", a, s->n[a]);
-  }
+  s->dim2 = s->n[2] == 1;
+  for (int a = 0; a < 2; ++a)
+    if (s->n[a] <= 1)
+      FAIL("only 3-D grids and 2-D grids with one z cell are supported (axis %d has %zu cells)", a, s->n[a]);
   /* grid/grid_space.cpp:625-638, 737-757 */
   for (int a = 0; a < 3; ++a) {
     size_t n = s->n[a];
@@ -743,6 +807,17 @@ static double gaussian(double amp, double tau, double t0, double t) {
   return amp * exp(-pow((t - t0) / tau, 2));
 }
 
+/* a waveform's value at time t: kind 0 Gaussian (amp, tau, t0), 1 sinusoid
+   (amp, f = tau, phase = t0), 2 cosine-modulated Gaussian (amp, tau, t0,
+   f = p3) */
+static double wave_at(int kind, double amp, double tau, double t0, double p3, double t) {
+  if (kind == 1) /* waveform/sinusoidal_waveform.cpp:18-21 */
+    return amp * sin(2 * XO_PI * tau * t + t0);
+  if (kind == 2) /* waveform/cosine_modulated_gaussian_waveform.cpp:29-33 */
+    return amp * cos(2 * XO_PI * p3 * (t - t0)) * exp(-pow((t - t0) / tau, 2));
+  return gaussian(amp, tau, t0, t);
+}
+
 static int init_tfsf(xo_sim *s) {
   tfsf_t *t = &s->tf;
   /* tfsf/tfsf.cpp:15-33, 35-53 */
@@ -757,8 +832,9 @@ static int init_tfsf(xo_sim *s) {
   t->th[1] = kv[2] * t->te[0] - kv[0] * t->te[2];
   t->th[2] = kv[0] * t->te[1] - kv[1] * t->te[0];
   size_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
-  if (2 * t->dist[0] >= nx || 2 * t->dist[1] >= ny || 2 * t->dist[2] >= nz ||
-      t->dist[0] < 1 || t->dist[1] < 1 || t->dist[2] < 1)
+  if (t->dim2 ? (2 * t->dist[0] >= nx || 2 * t->dist[1] >= ny || t->dist[0] < 1 || t->dist[1] < 1)
+              : (2 * t->dist[0] >= nx || 2 * t->dist[1] >= ny || 2 * t->dist[2] >= nz ||
+                 t->dist[0] < 1 || t->dist[1] < 1 || t->dist[2] < 1))
     FAIL("TFSF box does not fit the grid");
   t->li = t->dist[0]; t->ri = nx - t->dist[0];
   t->lj = t->dist[1]; t->rj = ny - t->dist[1];
@@ -766,11 +842,13 @@ static int init_tfsf(xo_sim *s) {
   size_t bx = t->ri - t->li, by = t->rj - t->lj, bz = t->rk - t->lk;
   t->wave = (double *)malloc(sizeof(double) * (s->T ? s->T : 1));
   for (size_t n = 0; n < s->T; ++n)
-    t->wave[n] = gaussian(t->amp, t->tau, t->t0, (double)n * s->dt);
-  /* tfsf/tfsf_3d.cpp:33-151 */
+    t->wave[n] = wave_at(t->wkind, t->amp, t->tau, t->t0, t->wp3, (double)n * s->dt);
+  /* tfsf/tfsf_3d.cpp:33-151; tfsf/tfsf_2d.cpp:26-97, whose diagonal takes
+     sqrt(nz^2) in place of nz^2 */
   t->ratio = 1 / sqrt(pow(sin(t->theta), 4) * (pow(cos(t->phi), 4) + pow(sin(t->phi), 4)) +
                       pow(cos(t->theta), 4));
-  double diag = sqrt(pow((double)bx, 2) + pow((double)by, 2) + pow((double)bz, 2)) * t->ratio;
+  double diag = t->dim2 ? sqrt(pow((double)bx, 2) + pow((double)by, 2) + sqrt(pow((double)bz, 2))) * t->ratio
+                        : sqrt(pow((double)bx, 2) + pow((double)by, 2) + pow((double)bz, 2)) * t->ratio;
   t->L = (size_t)ceil(diag) + 4 + 1;
   t->e_inc = (double *)calloc(t->L, sizeof(double));
   t->h_inc = (double *)calloc(t->L - 1, sizeof(double));
@@ -779,6 +857,7 @@ static int init_tfsf(xo_sim *s) {
   org[0] = (kv[0] >= 0) ? (double)t->li : (double)t->ri;
   org[1] = (kv[1] >= 0) ? (double)t->lj : (double)t->rj;
   org[2] = (kv[2] >= 0) ? (double)t->lk : (double)t->rk;
+  if (t->dim2) org[2] = 0; /* tfsf_2d.cpp:60-77: the origin's z is 0 */
   for (int a = 0; a < 3; ++a) org[a] -= extra[a];
   size_t bn[3] = {bx, by, bz}, st3[3] = {t->li, t->lj, t->lk};
   double *full[3], *half[3];
@@ -849,9 +928,60 @@ static double tfsf_inc(const tfsf_t *t, int comp, size_t i, size_t j, size_t k) 
   return (1 - w) * (scale * arr[idx]) + w * (scale * arr[idx + 1]);
 }
 
+/* TFSF2D::getIncident{Ez,Hx,Hy} (tfsf_2d.cpp:230-257): the 3-D
+   interpolation without the z projections */
+static double tfsf_inc2d(const tfsf_t *t, int comp, size_t i, size_t j) {
+  double proj;
+  const double *arr;
+  double scale;
+  if (comp == XO_EZ) {
+    proj = t->px_full[i - t->li] + t->py_full[j - t->lj];
+    arr = t->e_inc; scale = t->te[2];
+  } else if (comp == XO_HX) {
+    proj = t->px_full[i - t->li] + t->py_half[j - t->lj + 1] - 0.5;
+    arr = t->h_inc; scale = t->th[0];
+  } else {
+    proj = t->px_half[i - t->li + 1] + t->py_full[j - t->lj] - 0.5;
+    arr = t->h_inc; scale = t->th[1];
+  }
+  size_t idx = (size_t)proj;
+  double w = proj - (double)idx;
+  return (1 - w) * (scale * arr[idx]) + w * (scale * arr[idx + 1]);
+}
+
+/* TFSF2D::updateH (tfsf_2d.cpp:148-186): Hy on the x faces, Hx on the y
+   faces, the incident Ez sampled on the total-field side */
+static void tfsf_update_h2d(xo_sim *s) {
+  tfsf_t *t = &s->tf;
+  a3 *hx = &s->f[XO_HX], *hy = &s->f[XO_HY];
+  size_t li = t->li, ri = t->ri, lj = t->lj, rj = t->rj;
+  double cbx = s->dt / (MU0() * s->dl), cby = cbx;
+  for (size_t j = lj; j < rj + 1; ++j) AT(*hy, li - 1, j, 0) -= cbx * tfsf_inc2d(t, XO_EZ, li, j);
+  for (size_t j = lj; j < rj + 1; ++j) AT(*hy, ri, j, 0) += cbx * tfsf_inc2d(t, XO_EZ, ri, j);
+  for (size_t i = li; i < ri + 1; ++i) AT(*hx, i, lj - 1, 0) += cby * tfsf_inc2d(t, XO_EZ, i, lj);
+  for (size_t i = li; i < ri + 1; ++i) AT(*hx, i, rj, 0) -= cby * tfsf_inc2d(t, XO_EZ, i, rj);
+}
+
+/* TFSF2D::updateE (tfsf_2d.cpp:188-228): Ez on all four faces, in xn, xp,
+   yn, yp order (the corners take two corrections) */
+static void tfsf_update_e2d(xo_sim *s) {
+  tfsf_t *t = &s->tf;
+  a3 *ez = &s->f[XO_EZ];
+  size_t li = t->li, ri = t->ri, lj = t->lj, rj = t->rj;
+  double cax = s->dt / (EPS0 * s->dl), cay = cax;
+  for (size_t j = lj; j < rj + 1; ++j) AT(*ez, li, j, 0) -= cax * tfsf_inc2d(t, XO_HY, li - 1, j);
+  for (size_t j = lj; j < rj + 1; ++j) AT(*ez, ri, j, 0) += cax * tfsf_inc2d(t, XO_HY, ri, j);
+  for (size_t i = li; i < ri + 1; ++i) AT(*ez, i, lj, 0) += cay * tfsf_inc2d(t, XO_HX, i, lj - 1);
+  for (size_t i = li; i < ri + 1; ++i) AT(*ez, i, rj, 0) -= cay * tfsf_inc2d(t, XO_HX, i, rj);
+}
+
 /* tfsf/tfsf_3d.cpp:185-297 */
 static void tfsf_update_h(xo_sim *s) {
   tfsf_t *t = &s->tf;
+  if (t->dim2) {
+    tfsf_update_h2d(s);
+    return;
+  }
   a3 *hx = &s->f[XO_HX], *hy = &s->f[XO_HY], *hz = &s->f[XO_HZ];
   size_t li = t->li, ri = t->ri, lj = t->lj, rj = t->rj, lk = t->lk, rk = t->rk;
   double cbx = s->dt / (MU0() * s->dl), cby = cbx, cbz = cbx;
@@ -884,6 +1014,10 @@ static void tfsf_update_h(xo_sim *s) {
 /* tfsf/tfsf_3d.cpp:299-411 */
 static void tfsf_update_e(xo_sim *s) {
   tfsf_t *t = &s->tf;
+  if (t->dim2) {
+    tfsf_update_e2d(s);
+    return;
+  }
   a3 *ex = &s->f[XO_EX], *ey = &s->f[XO_EY], *ez = &s->f[XO_EZ];
   size_t li = t->li, ri = t->ri, lj = t->lj, rj = t->rj, lk = t->lk, rk = t->rk;
   double cax = s->dt / (EPS0 * s->dl), cay = cax, caz = cax;
@@ -951,14 +1085,18 @@ static int init_bb(xo_sim *s) {
   bb_t *b = &s->bb;
   /* nffft/nffft.cpp:22-34 */
   size_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
-  if (2 * b->dist[0] >= nx || 2 * b->dist[1] >= ny || 2 * b->dist[2] >= nz ||
-      b->dist[0] < 1 || b->dist[1] < 1 || b->dist[2] < 1)
+  if (s->dim2 ? (2 * b->dist[0] >= nx || 2 * b->dist[1] >= ny || b->dist[0] < 1 || b->dist[1] < 1 ||
+                 b->dist[2] != 0)
+              : (2 * b->dist[0] >= nx || 2 * b->dist[1] >= ny || 2 * b->dist[2] >= nz ||
+                 b->dist[0] < 1 || b->dist[1] < 1 || b->dist[2] < 1))
     FAIL("NF-FF box does not fit the grid");
   b->li = b->dist[0]; b->ri = nx - b->dist[0];
   b->lj = b->dist[1]; b->rj = ny - b->dist[1];
   b->lk = b->dist[2]; b->rk = nz - b->dist[2];
   b->nxb = b->ri - b->li; b->nyb = b->rj - b->lj; b->nzb = b->rk - b->lk;
-  b->Q = b->nyb * b->nzb + b->nzb * b->nxb + b->nxb * b->nyb;
+  /* on a 2-D grid the z faces are not kept: nffft_broadband.cpp:438-440
+     never integrates them (and their H stencils would reach k = -1) */
+  b->Q = b->nyb * b->nzb + b->nzb * b->nxb + (s->dim2 ? 0 : b->nxb * b->nyb);
   /* nffft_broadband.cpp:36-91 sizes every series 2T-1; the steps past the
      run's last are never written and are taken as zero */
   b->samples = (double *)calloc((s->T ? s->T : 1) * 8 * b->Q, sizeof(double));
@@ -1113,6 +1251,7 @@ static void bb_update(xo_sim *s) {
       nf_face_values(s, 1, b->lj, b->rj, k, i, v);
       for (int c = 0; c < 8; ++c) row[(size_t)c * b->Q + q] = v[c];
     }
+  if (s->dim2) return;
   for (size_t i = b->li; i < b->ri; ++i)
     for (size_t j = b->lj; j < b->rj; ++j, ++q) {
       double v[8];
@@ -1360,11 +1499,7 @@ static void update_e_drude(xo_sim *s) {
 /* --------------------------------------------------------- current source */
 
 static double lumped_wave(const csrc_t *c, double t) {
-  if (c->wkind == 1) /* waveform/sinusoidal_waveform.cpp:18-21 */
-    return c->amp * sin(2 * XO_PI * c->tau * t + c->t0);
-  if (c->wkind == 2) /* waveform/cosine_modulated_gaussian_waveform.cpp:29-33 */
-    return c->amp * cos(2 * XO_PI * c->wp3 * (t - c->t0)) * exp(-pow((t - c->t0) / c->tau, 2));
-  return gaussian(c->amp, c->tau, c->t0, t);
+  return wave_at(c->wkind, c->amp, c->tau, c->t0, c->wp3, t);
 }
 
 /* cell sizes of a lumped element's cell along its axis a: (da, db, dc) of
@@ -1626,7 +1761,14 @@ int xo_sim_init(xo_sim *s, size_t total_steps) {
   /* fdtd_basic_coff.cpp:11-29, 342-346: cfl arrives as float */
   double cfl = (double)s->cfl;
   double dx = s->dl, dy = s->dl, dz = s->dl;
-  s->dt = cfl / (C0() * sqrt(1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz)));
+  if (s->dim2) /* fdtd_basic_coff.cpp:13-19, 347-349: no z term when nz <= 1 */
+    s->dt = cfl / (C0() * sqrt(1.0 / (dx * dx) + 1.0 / (dy * dy)));
+  else
+    s->dt = cfl / (C0() * sqrt(1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz)));
+  if (s->dim2 && (s->has_nf || s->nsrc || s->nlmon || s->has_design || s->has_dft || s->has_drude))
+    FAIL("a 2-D grid takes objects, PML on x and y, TFSF2D, NffftBroadBand and probes only");
+  if (s->has_tfsf && s->tf.dim2 != s->dim2)
+    FAIL(s->dim2 ? "a 2-D grid needs TFSF2D" : "TFSF2D needs a 2-D grid");
   for (int q = 0; q < 6; ++q) {
     size_t sh[3];
     field_shape(s, q, sh);
@@ -1677,7 +1819,8 @@ int xo_sim_init(xo_sim *s, size_t total_steps) {
     for (size_t i = 0; i < nx; ++i)
       for (size_t j = 0; j < ny; ++j)
         for (size_t k = 0; k < nz; ++k) {
-          double pt[3] = {s->hc[0][i], s->hc[1][j], s->hc[2][k]};
+          /* grid/grid_space.cpp:241-247, 284-292: z of a 2-D cell centre is 0 */
+          double pt[3] = {s->hc[0][i], s->hc[1][j], s->dim2 ? 0.0 : s->hc[2][k]};
           if (!is_inside(o, pt)) continue;
           for (int c = 0; c < 3; ++c) {
             AT(eps[c], i, j, k) = e;

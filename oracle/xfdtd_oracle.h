@@ -36,6 +36,9 @@ int xo_sim_add_cube(xo_sim *s, const double origin[3], const double size[3],
                     float eps_r, float mu_r, double sigma_e, double sigma_m);
 int xo_sim_add_sphere(xo_sim *s, const double center[3], double radius,
                       float eps_r, float mu_r, double sigma_e, double sigma_m);
+/* Object(name, Cylinder(axis 0 x / 1 y / 2 z, center, radius, height), Material) */
+int xo_sim_add_cylinder(xo_sim *s, int axis, const double center[3], double radius, double height,
+                        float eps_r, float mu_r, double sigma_e, double sigma_m);
 /* make the last added object a DrudeMedium(eps_inf, omega_p, nv) with its
    sigma_e / sigma_m (its eps_r and mu_r become 1) */
 int xo_sim_set_drude(xo_sim *s, double eps_inf, double omega_p, double nu);
@@ -50,6 +53,11 @@ int xo_sim_add_pml(xo_sim *s, int orientation, int thickness, int order,
 int xo_sim_set_tfsf(xo_sim *s, const size_t distance[3], double e0,
                     double theta, double phi, double psi, double amplitude,
                     double tau, double t0);
+/* TFSF2D(dx, dy, phi_inc, ez_i0, waveform) on a 2-D grid (nz = 1); waveform
+   kind 0 Gaussian (amp, tau, t0), 1 sinusoid (amp, f, phase), 2
+   cosine-modulated Gaussian (amp, tau, t0, f) in wp[4] */
+int xo_sim_set_tfsf2d(xo_sim *s, size_t distance_x, size_t distance_y, double phi, double ez_i0, int wkind,
+                      const double wp[4]);
 /* NffftFd(dx,dy,dz, frequencies, theta, phi) */
 int xo_sim_set_nffft(xo_sim *s, const size_t distance[3], const double *freqs,
                      int nf, const double *theta, int ntheta,

@@ -71,6 +71,17 @@ class Sphere:
 
 
 @dataclass
+class Cylinder:
+    """Cylinder(axis, center, radius, height) (shape/cylinder.cpp:11-15):
+    axis AX_X / AX_Y / AX_Z; a 2-D scene's cylinder has height 0 and lies in
+    the z = 0 plane."""
+    axis: int
+    center: Tuple[float, float, float]
+    radius: float
+    height: float
+
+
+@dataclass
 class Material:
     name: str = "air"
     eps_r: float = 1.0
@@ -147,6 +158,19 @@ class TFSF3D:
     phi: float
     psi: float
     waveform: GaussianWaveform
+
+
+@dataclass
+class TFSF2D:
+    """TFSF2D(distance_x, distance_y, phi_inc, ez_i0, waveform)
+    (tfsf/tfsf_2d.cpp:10-24): a TMz plane wave travelling along phi_inc in
+    the xy plane of a 2-D scene (theta = pi/2, psi = 0); its Ez is
+    -ez_i0 * waveform (the polarisation vector of psi = 0 points along -z)."""
+    distance_x: int
+    distance_y: int
+    phi_inc: float
+    ez_i0: float
+    waveform: object
 
 
 @dataclass
@@ -268,18 +292,24 @@ class Network:
     output_path: str = ""
 
 
+def waveform_desc(w):
+    """(kind, params) of a waveform: 0 Gaussian (amplitude, tau, t_0), 1
+    sinusoid (amplitude, frequency, phase), 2 cosine-modulated Gaussian
+    (amplitude, tau, t_0, frequency)."""
+    if isinstance(w, SinusoidalWaveform):
+        return 1, (w.amplitude, w.frequency, w.phase, 0.0)
+    if isinstance(w, CosineModulatedGaussianWaveform):
+        return 2, (w.amplitude, w.tau, w.t_0, w.modulation_frequency)
+    return 0, (w.amplitude, w.tau, w.t_0, 0.0)
+
+
 def lumped_desc(le):
     """(kind, dir, value, waveform kind, waveform params) of a lumped element:
     kind 0 CurrentSource, 1 VoltageSource, 2 Resistor, 3 Capacitor, 4 Inductor;
     waveform kind 0 Gaussian (amplitude, tau, t_0), 1 sinusoid (amplitude,
     frequency, phase), 2 cosine-modulated Gaussian (amplitude, tau, t_0,
     frequency)."""
-    def wave(w):
-        if isinstance(w, SinusoidalWaveform):
-            return 1, (w.amplitude, w.frequency, w.phase, 0.0)
-        if isinstance(w, CosineModulatedGaussianWaveform):
-            return 2, (w.amplitude, w.tau, w.t_0, w.modulation_frequency)
-        return 0, (w.amplitude, w.tau, w.t_0, 0.0)
+    wave = waveform_desc
     if isinstance(le, CurrentSource):
         return (0, le.orientation, le.resistance) + wave(le.waveform)
     if isinstance(le, VoltageSource):
@@ -416,6 +446,36 @@ def cube_domain_scene(n_total, dl, pml, cfl=0.9, objects=(), tfsf_gap=None,
         sc.nffft = NffftFd(nffft_gap, nffft_gap, nffft_gap, list(freqs),
                            list(theta), list(phi))
     return sc
+
+
+def cylinder_2d_scene(nx, ny, pml, dl, radius, cfl=0.8, tfsf_gap=50, nffft_gap=40, phi_inc=PI / 4,
+                      center_frequency=12e9, max_frequency=20e9, obs_phi=1.25 * PI, material=None):
+    """The reference's 2-D scatter example (examples/2d_cylinder_scatter.cpp:
+    26-106) at any size: an nx x ny vacuum plane with `pml` CPML cells on the
+    four x and y faces, a z-axis Cylinder of `radius` (PEC unless `material`
+    is given) at the plane's centre, a TFSF2D source along phi_inc with a
+    cosine-modulated Gaussian (tau = 1.7 / (f_max - f_c), t_0 = 0.8 tau) and
+    an NffftBroadBand surface looking along (pi/2, obs_phi)."""
+    sc = Scene(cell_size=dl, cfl=cfl)
+    sc.add_object(Object("free_space", Cube((0.0, 0.0, 0.0), (nx * dl, ny * dl, 0.0)), Material.air()))
+    if radius > 0:
+        mat = material if material is not None else Material("pec", 1, 1, 1e24, 0)
+        sc.add_object(Object("objectA", Cylinder(AX_Z, (nx * 0.5 * dl, ny * 0.5 * dl, 0.0), radius, 0.0), mat))
+    sc.boundaries = [PML(o, pml) for o in range(4)]
+    tau = 1.7 / (max_frequency - center_frequency)
+    if tfsf_gap is not None:
+        sc.tfsf = TFSF2D(tfsf_gap, tfsf_gap, phi_inc, 1.0,
+                         CosineModulatedGaussianWaveform(1.0, tau, 0.8 * tau, center_frequency))
+    if nffft_gap is not None:
+        sc.nffft = NffftBroadBand(nffft_gap, nffft_gap, 0, PI / 2, obs_phi)
+    return sc
+
+
+def courant_dt_2d(dl, cfl):
+    """fdtd_basic_coff/fdtd_basic_coff.cpp:347-349 (a grid with nz = 1)."""
+    import numpy as np
+    c = float(np.float32(cfl))
+    return c / (C_0 * math.sqrt(1.0 / (dl * dl) + 1.0 / (dl * dl)))
 
 
 def courant_dt(dl, cfl):
