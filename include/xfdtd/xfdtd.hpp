@@ -99,6 +99,27 @@ class Sphere : public Shape {
   double _radius;
 };
 
+// shape/cylinder.cpp: a cylinder along `axis`; its wrapped box spans the
+// radius across and height/2 either side of the centre along the axis.  A
+// 2-D scene's cylinder has height 0 in the z = 0 plane.
+class Cylinder : public Shape {
+ public:
+  Cylinder(Axis axis, PointVector center, double radius, double height)
+      : _axis{axis}, _center{center}, _radius{radius}, _height{height} {}
+  bool isPointInside(const PointVector &point) const override;
+  std::unique_ptr<Shape> getWrappedBox() const override;
+  std::unique_ptr<Shape> clone() const override { return std::make_unique<Cylinder>(*this); }
+  Axis getAxis() const { return _axis; }
+  double getRadius() const { return _radius; }
+  double getHeight() const { return _height; }
+  PointVector getCenter() const { return _center; }
+
+ private:
+  Axis _axis;
+  PointVector _center;
+  double _radius, _height;
+};
+
 // --------------------------------------------------------------- material
 class Material {
  public:
@@ -287,6 +308,10 @@ class TFSF {
   // tfsf/tfsf.cpp:63-66 with util/dft.h
   std::vector<std::complex<double>> getIncidentWaveFourierTransform(
       const std::vector<double> &frequencies) const;
+  // tfsf/tfsf.cpp:68-78: the waveform (without e_0) zero-padded to 2T-1
+  // points, fftshift(fft(w)), and rfftfreq(2T-1, dt) (T entries, as the
+  // reference writes them)
+  std::pair<std::vector<double>, std::vector<std::complex<double>>> getIncidentWaveFastFourierTransform() const;
   const std::vector<double> &getIncidentWaveValue() const { return _waveform->getValue(); }
   Waveform &waveform() { return *_waveform; }
   double dt() const { return _dt; }
@@ -305,6 +330,21 @@ class TFSF3D : public TFSF {
  public:
   TFSF3D(SpatialIndex distance_x, SpatialIndex distance_y, SpatialIndex distance_z, double e_0,
          double theta_inc, double phi_inc, double psi, std::shared_ptr<Waveform> waveform);
+  PointVector getTransformE() const { return _transform_e; }
+  PointVector getTransformH() const { return _transform_h; }
+
+ private:
+  PointVector _transform_e, _transform_h;
+  friend class Simulation;
+};
+
+// tfsf/tfsf_2d.cpp:10-24: the TMz plane wave of a 2-D scene (a grid with
+// one z cell): theta = pi/2, psi = 0, travelling along phi_inc; its Ez is
+// -ez_i0 * waveform.  Only Ez, Hx and Hy take face corrections.
+class TFSF2D : public TFSF {
+ public:
+  TFSF2D(SpatialIndex distance_x, SpatialIndex distance_y, double phi_inc, double ez_i0,
+         std::shared_ptr<Waveform> waveform);
   PointVector getTransformE() const { return _transform_e; }
   PointVector getTransformH() const { return _transform_h; }
 
@@ -371,6 +411,9 @@ struct BroadBandFarField {
   std::vector<std::complex<double>> n_x, n_y, n_z, l_x, l_y, l_z;
   std::vector<std::complex<double>> n_theta, n_phi, l_theta, l_phi, e_theta, e_phi, h_theta, h_phi;
   std::vector<double> power_theta, power_phi;
+  // 2-D grids: the far-zone Ez of nffft_broadband.cpp:173-179,
+  // 0.5 sqrt(i k) (l_phi + eta0 n_theta) (empty on a 3-D grid)
+  std::vector<std::complex<double>> ez;
 };
 
 // ---------------------------------------------------------- lumped element
@@ -708,6 +751,9 @@ class Simulation {
   // has an output directory, then every field monitor's final
   // slice (movie monitors write their frames while stepping)
   void outputData() const;
+  // simulation.cpp:170-188: frequencies.npy and value.npy of
+  // TFSF::getIncidentWaveFastFourierTransform under `path`
+  void outputTFSFIncidentWaveFastFourierTransform(const std::string &path) const;
   std::vector<double> getProbeData() const;  // [n_probes][total_steps]
 
   xf_engine *engine() const { return _engine; }
@@ -763,9 +809,10 @@ class Simulation {
   size_t _dft_plane[5] = {};  // index, lo1, hi1, lo2, hi2
   double _dft_freq = 0;
 
-  // GridSpace
+  // GridSpace; _two_d: TWO_DIMENSION, one z cell (grid/grid_space.cpp:501-536)
   double _min[3], _max[3];
   size_t _n[3] = {0, 0, 0};
+  bool _two_d = false;
   std::vector<double> _ec[3], _hc[3], _es[3], _hs[3];
   double _dt = 0;
   size_t _total_time_steps = 0;

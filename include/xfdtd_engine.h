@@ -78,7 +78,8 @@ enum {
 };
 
 typedef struct {
-  int64_t nx, ny, nz; /* cell counts */
+  int64_t nx, ny, nz; /* cell counts; nz = 1: a 2-D (TMz) grid, stepped by the
+                          same Yee updates (simulation_update.cpp:157-168) */
   int precision;      /* 64: FP64 fields (parity mode); 32: FP32 fields */
   int n_materials;    /* 1..255 */
 } xf_grid_desc;
@@ -120,6 +121,10 @@ typedef struct {
   double ca[3], cb[3];             /* dt/(eps0 d), dt/(mu0 d) per axis */
   const double *source;            /* e_0 * g(n dt), n < total_steps */
   int64_t total_steps;
+  /* 1: TFSF2D (tfsf/tfsf_2d.cpp:26-259) on a grid with nz = 1: the box is
+     [li,ri] x [lj,rj] x [0,1), only Ez, Hx and Hy take corrections and the
+     incident projections have no z term */
+  int64_t two_d;
 } xf_tfsf_desc;
 xf_status xf_engine_set_tfsf(xf_engine *e, const xf_tfsf_desc *d);
 
@@ -152,7 +157,9 @@ xf_status xf_engine_farfield_sums(xf_engine *e, double dl,
    the monostatic RCS path): after every step t < total_steps the face
    currents of every surface cell of box {li,ri,lj,rj,lk,rk} (nffft/nffft.cpp:
    29-33) are stored, with the signs and stencils of NffftFd
-   (nffft_broadband.cpp:94-155).  A batch refuses engines that have one. */
+   (nffft_broadband.cpp:94-155).  A batch refuses engines that have one.
+   On a 2-D grid (nz = 1) the box is {li,ri,lj,rj,0,1} and the z faces are
+   not kept (nffft_broadband.cpp:438-440 never integrates them). */
 xf_status xf_engine_set_nffft_broadband(xf_engine *e, const int64_t box[6], int64_t total_steps);
 /* the stored series, FP64, [total_steps][8][Q]: current c in the order of the
    get_nffft accumulators, Q surface cells (x faces (j,k), then y faces

@@ -11,6 +11,8 @@
  *   xf_sim_add_drude_cube / _sphere  addObject(Object(name, shape, DrudeMedium(...)))  dispersive_material.h:112-113
  *   xf_sim_add_pml               addBoundary(PML(...))                      simulation.cpp:119-121
  *   xf_sim_set_tfsf3d            addTFSFSource(TFSF3D(..., GaussianWaveform))  simulation.cpp:128-130
+ *   xf_sim_add_cylinder          addObject(Object(name, Cylinder, Material))   shape/cylinder.cpp:11-15
+ *   xf_sim_set_tfsf2d            addTFSFSource(TFSF2D(dx, dy, phi, ez_i0, waveform))  tfsf/tfsf_2d.cpp:10-24
  *   xf_sim_set_nffft_fd          addNFFFT(NffftFd(...))                     simulation.cpp:132-134
  *   xf_sim_add_current_source    addLumpedElement(CurrentSource(...))       simulation.cpp:123-126
  *   xf_sim_add_probe             addMonitor (point sample)                  simulation.cpp:136-138
@@ -22,6 +24,7 @@
  *   xf_sim_farfield              NffftFd::calculateFarfield                 nffft_fd.cpp:379-421
  *   xf_sim_set_nffft_broadband   addNFFFT(NffftBroadBand(...))              nffft_broadband.cpp:21-28
  *   xf_sim_broadband_farfield    NffftBroadBand::calculateFarField          nffft_broadband.cpp:316-466
+ *   xf_sim_broadband_ez          the 2-D far-zone Ez of outputData          nffft_broadband.cpp:171-179
  *   xf_sim_output_nffft          Simulation::outputData: NffftFd::outputData and the field
  *                                monitors' final slices                     simulation.cpp:144-156
  *
@@ -58,6 +61,13 @@ int xf_sim_add_pml(xf_sim *s, int orientation, int thickness, int order, double 
                    double alpha_min, double alpha_max, double kappa_max);
 int xf_sim_set_tfsf3d(xf_sim *s, const size_t distance[3], double e0, double theta, double phi,
                       double psi, double amplitude, double tau, double t0);
+/* Object(name, Cylinder(axis 0 x / 1 y / 2 z, center, radius, height), Material) */
+int xf_sim_add_cylinder(xf_sim *s, int axis, const double center[3], double radius, double height,
+                        float eps_r, float mu_r, double sigma_e, double sigma_m);
+/* TFSF2D(distance_x, distance_y, phi_inc, ez_i0, waveform) for a 2-D scene
+   (objects with no z extent: one z cell); waveform kind as xf_sim_add_lumped */
+int xf_sim_set_tfsf2d(xf_sim *s, size_t distance_x, size_t distance_y, double phi_inc, double ez_i0,
+                      int wkind, const double wp[4]);
 int xf_sim_set_nffft_fd(xf_sim *s, const size_t distance[3], const double *freqs, int nf,
                         const double *theta, int ntheta, const double *phi, int nphi,
                         const char *output_dir);
@@ -142,6 +152,9 @@ int xf_sim_farfield(xf_sim *s, double *out);
 int64_t xf_sim_broadband_samples_size(const xf_sim *s);
 int xf_sim_get_broadband_samples(xf_sim *s, double *out);
 int xf_sim_broadband_farfield(xf_sim *s, const double *wave_number, double *out);
+/* 2-D grids: the far-zone Ez of every bin, 0.5 sqrt(i k) (l_phi + eta0
+   n_theta) (nffft_broadband.cpp:171-179), interleaved complex [Ns] */
+int xf_sim_broadband_ez(xf_sim *s, double *out);
 /* Simulation::outputData: NF-FF files (NffftFd or NffftBroadBand) and the
    field monitors' final slices */
 int xf_sim_output_nffft(xf_sim *s);

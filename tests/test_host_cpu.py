@@ -99,10 +99,16 @@ def lumped_scene(waveform=None, steps_hint=400):
     return sc
 
 
-@pytest.mark.parametrize("builder", ["everything", "pec_example", "drude", "lumped"])
+@pytest.mark.parametrize("builder", ["everything", "pec_example", "drude", "lumped", "cylinder_2d"])
 def test_plan_reproduces_reference_coefficient_arrays(builder):
     from xfdtd_amd.sim import Simulation
-    if builder == "everything":
+    if builder == "cylinder_2d":
+        # a dielectric then a PEC cylinder on a 2-D grid with CPML on x and y
+        dl = 0.75e-3
+        sc = S.cylinder_2d_scene(60, 50, 8, dl, 8.3 * dl, tfsf_gap=14, nffft_gap=11)
+        sc.objects.insert(1, S.Object("d", S.Cylinder(S.AX_Z, (20 * dl, 20 * dl, 0.0), 9.6 * dl, 0.0),
+                                      S.Material("d", 3.5, 1.0, 0.2, 0.0)))
+    elif builder == "everything":
         sc = _scene_with_everything()
     elif builder == "lumped":
         sc = lumped_scene()
@@ -155,6 +161,28 @@ def test_invalid_configurations_raise():
     sc = S.Scene(cell_size=dl)
     with pytest.raises(RuntimeError, match="no objects"):
         Simulation(sc, 10, plan_only=True)
+
+
+def test_2d_refusals():
+    from xfdtd_amd.sim import Simulation
+    dl = 0.75e-3
+    sc = S.cylinder_2d_scene(40, 40, 6, dl, 0, tfsf_gap=None, nffft_gap=None)
+    sc.tfsf = S.TFSF3D(10, 10, 1, 1.0, 1.0, 0.5, 0.0, S.GaussianWaveform(1e-11, 4e-11))
+    with pytest.raises(RuntimeError, match="TFSF2D"):
+        Simulation(sc, 10, plan_only=True)
+    sc = S.cylinder_2d_scene(40, 40, 6, dl, 0, tfsf_gap=12, nffft_gap=None)
+    sc.nffft = S.NffftFd(8, 8, 0, [1e9], [1.0], [0.0])
+    with pytest.raises(RuntimeError, match="2-D grid takes"):
+        Simulation(sc, 10, plan_only=True)
+    sc = S.cylinder_2d_scene(40, 40, 6, dl, 0, tfsf_gap=12, nffft_gap=8)
+    sc.nffft.distance_z = 1
+    with pytest.raises(RuntimeError, match="NF-FF box does not fit"):
+        Simulation(sc, 10, plan_only=True)
+    sc = S.cube_domain_scene(20, 5e-3, 4, tfsf_gap=None)
+    sc.tfsf = S.TFSF2D(6, 6, 0.3, 1.0, S.GaussianWaveform(1e-11, 4e-11))
+    with pytest.raises(RuntimeError, match="TFSF2D needs a 2-D grid"):
+        Simulation(sc, 10, plan_only=True)
+    Simulation(S.cylinder_2d_scene(40, 40, 6, dl, 5 * dl, tfsf_gap=12, nffft_gap=8), 10, plan_only=True).close()
 
 
 def test_init_without_device_fails_loudly():

@@ -32,7 +32,7 @@ ENGINE_SYMBOLS = [
 ]
 SIM_SYMBOLS = [
     "xf_sim_last_error", "xf_sim_create", "xf_sim_destroy", "xf_sim_add_cube", "xf_sim_add_sphere",
-    "xf_sim_add_pml", "xf_sim_set_tfsf3d", "xf_sim_set_nffft_fd", "xf_sim_add_current_source",
+    "xf_sim_add_pml", "xf_sim_set_tfsf3d", "xf_sim_add_cylinder", "xf_sim_set_tfsf2d", "xf_sim_set_nffft_fd", "xf_sim_add_current_source",
     "xf_sim_add_probe", "xf_sim_plan", "xf_sim_plan_coeff", "xf_sim_init", "xf_sim_step", "xf_sim_sync", "xf_sim_current_step",
     "xf_sim_dims", "xf_sim_dt", "xf_sim_get_field", "xf_sim_set_field", "xf_sim_nffft_size",
     "xf_sim_get_nffft", "xf_sim_farfield", "xf_sim_output_nffft", "xf_sim_get_probes",
@@ -40,7 +40,7 @@ SIM_SYMBOLS = [
     "xf_sim_adjoint_run", "xf_sim_checkpoint", "xf_sim_restore", "xf_sim_add_drude_cube",
     "xf_sim_add_drude_sphere", "xf_sim_drude_box", "xf_sim_get_drude_j", "xf_sim_set_drude_j",
     "xf_sim_add_field_monitor", "xf_sim_add_movie_monitor", "xf_sim_set_nffft_broadband",
-    "xf_sim_broadband_samples_size", "xf_sim_get_broadband_samples", "xf_sim_broadband_farfield",
+    "xf_sim_broadband_samples_size", "xf_sim_get_broadband_samples", "xf_sim_broadband_ez", "xf_sim_broadband_farfield",
     "xf_sim_add_lumped", "xf_sim_add_plane", "xf_sim_add_line_monitor", "xf_sim_get_line_monitors",
     "xf_sim_set_network", "xf_sim_network_size", "xf_sim_network_sparams",
 ]
@@ -67,6 +67,9 @@ def lib():
     L.xf_sim_add_sphere.argtypes = [vp, dp, d, C.c_float, C.c_float, d, d]
     L.xf_sim_add_pml.argtypes = [vp, i, i, i, d, d, d, d]
     L.xf_sim_set_tfsf3d.argtypes = [vp, szp, d, d, d, d, d, d, d]
+    L.xf_sim_add_cylinder.argtypes = [vp, C.c_int, dp, d, d, C.c_float, C.c_float, d, d]
+    L.xf_sim_set_tfsf2d.argtypes = [vp, C.c_size_t, C.c_size_t, d, d, C.c_int, dp]
+    L.xf_sim_broadband_ez.argtypes = [vp, dp]
     L.xf_sim_set_nffft_fd.argtypes = [vp, szp, dp, i, dp, i, dp, i, C.c_char_p]
     L.xf_sim_add_current_source.argtypes = [vp, dp, dp, i, d, d, d, d]
     L.xf_sim_add_probe.argtypes = [vp, i, sz, sz, sz]

@@ -81,6 +81,7 @@ struct xf_engine {
   double te[3] = {}, th[3] = {}, ceie = 0, ceihi = 0, chih = 0, chiei = 0, c1 = 0, c2 = 0;
   double tca[3] = {}, tcb[3] = {};
   long long tfsf_T = 0;
+  bool tfsf_two_d = false;
   int nEy = 0, nEz = 0, nEx = 0;
   TfsfHSeg hseg{};
   // NF-FF
@@ -220,8 +221,9 @@ static void comp_shape(const xf_engine *e, int comp, int sh[3]) {
 extern "C" xf_status xf_engine_create(const xf_grid_desc *g, xf_engine **out) {
   if (!g || !out) FAILS(XF_EINVAL, "null argument");
   *out = nullptr;
-  if (g->nx < 2 || g->ny < 2 || g->nz < 2) FAILS(XF_EINVAL, "grid must be 3-D (got %lld,%lld,%lld)",
-                                                 (long long)g->nx, (long long)g->ny, (long long)g->nz);
+  if (g->nx < 2 || g->ny < 2 || g->nz < 1)
+    FAILS(XF_EINVAL, "grid must be 3-D or 2-D with nz = 1 (got %lld,%lld,%lld)", (long long)g->nx,
+          (long long)g->ny, (long long)g->nz);
   if (g->nx > 100000 || g->ny > 100000 || g->nz > 100000) FAILS(XF_EINVAL, "grid too large");
   if (g->precision != 32 && g->precision != 64) FAILS(XF_EINVAL, "precision must be 32 or 64");
   if (g->n_materials < 1 || g->n_materials > 255) FAILS(XF_EINVAL, "n_materials must be 1..255");
@@ -237,7 +239,14 @@ extern "C" xf_status xf_engine_create(const xf_grid_desc *g, xf_engine **out) {
   e->prec = g->precision;
   e->esz = e->prec == 64 ? 8 : 4;
   e->nmat = g->n_materials;
-  e->pz = ((e->nz + 1) + 31) / 32 * 32;
+  // z rows padded to 32 cells; a 2-D grid (nz == 1) keeps its two z slots
+  // unpadded and runs the curl kernels on flat runs of the (j, k) plane
+  e->pz = e->nz == 1 ? 2 : ((e->nz + 1) + 31) / 32 * 32;
+  if (e->nz == 1) {
+    e->tz = 64;
+    e->ty = 0;
+    e->tx = 4;
+  }
   e->sj = e->pz;
   e->si = (long long)(e->ny + 1) * e->pz;
   e->ncell = (long long)(e->nx + 1) * e->si;
@@ -395,8 +404,10 @@ extern "C" xf_status xf_engine_set_horizon(xf_engine *e, int64_t total_steps) {
 
 extern "C" xf_status xf_engine_set_tfsf(xf_engine *e, const xf_tfsf_desc *d) {
   if (!e || !d) FAILS(XF_EINVAL, "null argument");
-  if (d->li < 1 || d->lj < 1 || d->lk < 1 || d->ri >= e->nx || d->rj >= e->ny || d->rk >= e->nz ||
-      d->ri <= d->li || d->rj <= d->lj || d->rk <= d->lk)
+  const bool two_d = d->two_d != 0;
+  if (two_d != (e->nz == 1)) FAILS(XF_EINVAL, two_d ? "TFSF2D needs a grid with nz = 1" : "a grid with nz = 1 needs TFSF2D");
+  const bool zbad = two_d ? (d->lk != 0 || d->rk != 1) : (d->lk < 1 || d->rk >= e->nz || d->rk <= d->lk);
+  if (d->li < 1 || d->lj < 1 || d->ri >= e->nx || d->rj >= e->ny || d->ri <= d->li || d->rj <= d->lj || zbad)
     FAILS(XF_EINVAL, "TFSF box outside the grid");
   if (d->aux_size < 3) FAILS(XF_EINVAL, "TFSF aux size too small");
   e->tli = (int)d->li; e->tri = (int)d->ri; e->tlj = (int)d->lj; e->trj = (int)d->rj;
@@ -428,9 +439,16 @@ extern "C" xf_status xf_engine_set_tfsf(xf_engine *e, const xf_tfsf_desc *d) {
   e->nEz = bn[2] * (2 * A0 + 2 * (A1 - 2));
   e->nEx = bn[0] * (2 * A1 + 2 * (A2 - 2));
   const int ni = bn[0], nj = bn[1], nk = bn[2];
-  const int cnt[12] = {nj * (nk + 1), (nj + 1) * nk, nj * (nk + 1), (nj + 1) * nk,
-                       nk * (ni + 1), (nk + 1) * ni, nk * (ni + 1), (nk + 1) * ni,
-                       ni * (nj + 1), (ni + 1) * nj, ni * (nj + 1), (ni + 1) * nj};
+  int cnt[12] = {nj * (nk + 1), (nj + 1) * nk, nj * (nk + 1), (nj + 1) * nk,
+                 nk * (ni + 1), (nk + 1) * ni, nk * (ni + 1), (nk + 1) * ni,
+                 ni * (nj + 1), (ni + 1) * nj, ni * (nj + 1), (ni + 1) * nj};
+  if (two_d) {
+    // TFSF2D (tfsf_2d.cpp:148-228): Ez on the four faces of the one z
+    // layer, Hy on the x faces and Hx on the y faces
+    e->nEy = e->nEx = 0;
+    for (int q : {0, 2, 5, 7, 8, 9, 10, 11}) cnt[q] = 0;
+  }
+  e->tfsf_two_d = two_d;
   e->hseg.off[0] = 0;
   for (int q = 0; q < 12; ++q) e->hseg.off[q + 1] = e->hseg.off[q] + cnt[q];
   e->has_tfsf = true;
@@ -446,6 +464,7 @@ extern "C" xf_status xf_engine_set_nffft(xf_engine *e, const xf_nffft_desc *d) {
   n.li = (int)d->li; n.ri = (int)d->ri; n.lj = (int)d->lj; n.rj = (int)d->rj;
   n.lk = (int)d->lk; n.rk = (int)d->rk; n.nf = d->nf;
   n.nxb = n.ri - n.li; n.nyb = n.rj - n.lj; n.nzb = n.rk - n.lk;
+  n.zf = 1;
   n.T_total = d->total_steps;
   const long long cells[3] = {(long long)n.nyb * n.nzb, (long long)n.nzb * n.nxb, (long long)n.nxb * n.nyb};
   n.base[0] = 0;
@@ -519,21 +538,24 @@ extern "C" xf_status xf_engine_farfield_sums(xf_engine *e, double dl, const doub
 
 // ---------------------------------------------------------------- broadband NF-FF
 
-static long long bb_cells(const Nffft &n) {
-  return (long long)n.nyb * n.nzb + (long long)n.nzb * n.nxb + (long long)n.nxb * n.nyb;
-}
+static long long bb_cells(const Nffft &n) { return nf_cells(n); }
 
 extern "C" xf_status xf_engine_set_nffft_broadband(xf_engine *e, const int64_t box[6], int64_t total_steps) {
   if (!e || !box) FAILS(XF_EINVAL, "null argument");
   if (e->has_bb) FAILS(XF_ESTATE, "broadband NF-FF surface already set");
-  if (box[0] < 1 || box[2] < 1 || box[4] < 1 || box[1] >= e->nx || box[3] >= e->ny || box[5] >= e->nz ||
-      box[1] <= box[0] || box[3] <= box[2] || box[5] <= box[4] || total_steps < 1)
+  // a 2-D grid (nz == 1) takes the box's one z layer [0, 1) and keeps no
+  // z faces (nffft_broadband.cpp:438-440 never integrates them)
+  const bool two_d = e->nz == 1;
+  const bool zbad = two_d ? (box[4] != 0 || box[5] != 1) : (box[4] < 1 || box[5] >= e->nz || box[5] <= box[4]);
+  if (box[0] < 1 || box[2] < 1 || box[1] >= e->nx || box[3] >= e->ny || box[1] <= box[0] || box[3] <= box[2] ||
+      zbad || total_steps < 1)
     FAILS(XF_EINVAL, "broadband NF-FF box outside the grid or no steps");
   Nffft &n = e->bb;
   n = Nffft{};
   n.li = (int)box[0]; n.ri = (int)box[1]; n.lj = (int)box[2]; n.rj = (int)box[3];
   n.lk = (int)box[4]; n.rk = (int)box[5];
   n.nxb = n.ri - n.li; n.nyb = n.rj - n.lj; n.nzb = n.rk - n.lk;
+  n.zf = two_d ? 0 : 1;
   n.T_total = total_steps;
   const double bytes = (double)total_steps * 8 * bb_cells(n) * e->esz;
   xf_status s = dev_alloc(e, &e->bb_hist, (size_t)bytes);
@@ -817,6 +839,7 @@ template <typename T>
 static Tfsf<T> make_tfsf(xf_engine *e) {
   Tfsf<T> t;
   t.li = e->tli; t.ri = e->tri; t.lj = e->tlj; t.rj = e->trj; t.lk = e->tlk; t.rk = e->trk; t.L = e->L;
+  t.two_d = e->tfsf_two_d ? 1 : 0;
   for (int a = 0; a < 3; ++a) {
     t.pf[a] = (const T *)e->pf[a];
     t.ph[a] = (const T *)e->ph[a];
@@ -904,7 +927,7 @@ static xf_status run_steps(xf_engine *e, long long n) {
   const bool special = e->has_design || e->has_drude;
   long long nfQ = 0;
   if (e->has_nf) {
-    nfQ = (long long)e->nf.nyb * e->nf.nzb + (long long)e->nf.nzb * e->nf.nxb + (long long)e->nf.nxb * e->nf.nyb;
+    nfQ = nf_cells(e->nf);
     if (!e->nf_ring) {
       xf_status s = dev_alloc(e, &e->nf_ring, (size_t)NF_RING * 8 * nfQ * e->esz);
       if (s != XF_OK) return s;
@@ -1233,6 +1256,7 @@ extern "C" xf_status xf_batch_create(xf_engine *const *engines, int n, xf_batch 
     if (!e->materials_set) FAILS(XF_ESTATE, "engine %d is not set up", q);
     if (e->batch_stream) FAILS(XF_ESTATE, "engine %d already belongs to a batch", q);
     if (e->has_bb) FAILS(XF_EINVAL, "engine %d has a broadband NF-FF surface, which batches do not step", q);
+    if (e->nz == 1) FAILS(XF_EINVAL, "engine %d is a 2-D grid, which batches do not step", q);
     if (e->prec != e0->prec) FAILS(XF_EINVAL, "engine %d has precision %d, engine 0 has %d", q, e->prec, e0->prec);
     if (e->cur != e0->cur) FAILS(XF_ESTATE, "engine %d is at step %lld, engine 0 at %lld", q, e->cur, e0->cur);
     for (int r = 0; r < q; ++r)

@@ -803,6 +803,7 @@ __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl) {
 template <typename T>
 struct Tfsf {
   int li, ri, lj, rj, lk, rk, L;
+  int two_d;  // TFSF2D: Ez, Hx, Hy only, projections without z
   const T *pf[3], *ph[3];
   T te[3], th[3];
   T ceie, ceihi, chih, chiei, c1, c2;
@@ -821,11 +822,18 @@ __device__ __forceinline__ T tfsf_inc(const Tfsf<T> &t, int comp, int i, int j, 
              arr = t.e_inc; s = t.te[0]; break;
     case EY: proj = t.pf[0][i - t.li] + t.ph[1][j - t.lj + 1] + t.pf[2][k - t.lk];
              arr = t.e_inc; s = t.te[1]; break;
-    case EZ: proj = t.pf[0][i - t.li] + t.pf[1][j - t.lj] + t.ph[2][k - t.lk + 1];
+    // TFSF2D::getIncident{Ez,Hx,Hy} (tfsf_2d.cpp:230-257) leave out the z
+    // projection
+    case EZ: proj = t.pf[0][i - t.li] + t.pf[1][j - t.lj];
+             if (!t.two_d) proj = proj + t.ph[2][k - t.lk + 1];
              arr = t.e_inc; s = t.te[2]; break;
-    case HX: proj = t.pf[0][i - t.li] + t.ph[1][j - t.lj + 1] + t.ph[2][k - t.lk + 1] - T(0.5);
+    case HX: proj = t.pf[0][i - t.li] + t.ph[1][j - t.lj + 1];
+             if (!t.two_d) proj = proj + t.ph[2][k - t.lk + 1];
+             proj = proj - T(0.5);
              arr = t.h_inc; s = t.th[0]; break;
-    case HY: proj = t.ph[0][i - t.li + 1] + t.pf[1][j - t.lj] + t.ph[2][k - t.lk + 1] - T(0.5);
+    case HY: proj = t.ph[0][i - t.li + 1] + t.pf[1][j - t.lj];
+             if (!t.two_d) proj = proj + t.ph[2][k - t.lk + 1];
+             proj = proj - T(0.5);
              arr = t.h_inc; s = t.th[1]; break;
     default: proj = t.ph[0][i - t.li + 1] + t.ph[1][j - t.lj + 1] + t.pf[2][k - t.lk] - T(0.5);
              arr = t.h_inc; s = t.th[2]; break;
@@ -1066,11 +1074,17 @@ __global__ void k_box_gather(const T *f, long long si, long long sj, int i0, int
 struct Nffft {
   int li, ri, lj, rj, lk, rk, nf;
   int nxb, nyb, nzb;
+  int zf;                  // 1: the z faces are kept (0 on a 2-D grid)
   long long T_total;
   const double2 *tj, *tm;  // [nf][T]
   double2 *acc;
   long long base[3];       // accumulator offset of each face pair
 };
+
+// surface cells: x faces, y faces and (when kept) z faces
+__host__ __device__ inline long long nf_cells(const Nffft &n) {
+  return (long long)n.nyb * n.nzb + (long long)n.nzb * n.nxb + (n.zf ? (long long)n.nxb * n.nyb : 0LL);
+}
 
 __device__ __forceinline__ void acc_add(double2 *a, double v, double2 tw) {
   double2 x = *a;
@@ -1084,7 +1098,7 @@ __device__ __forceinline__ void acc_add(double2 *a, double v, double2 tw) {
 template <typename T>
 __device__ __forceinline__ bool nffft_values(const Grid<T> &g, const Nffft &n, int q, T *v, int &pair, int &cells,
                                              int &cell) {
-  const int cx = n.nyb * n.nzb, cy = n.nzb * n.nxb, cz = n.nxb * n.nyb;
+  const int cx = n.nyb * n.nzb, cy = n.nzb * n.nxb, cz = n.zf ? n.nxb * n.nyb : 0;
   const T *Ex = g.f[EX], *Ey = g.f[EY], *Ez = g.f[EZ];
   const T *Hx = g.f[HX], *Hy = g.f[HY], *Hz = g.f[HZ];
   const long long si = g.si, sj = g.sj;
@@ -1162,7 +1176,7 @@ constexpr int NF_RING = 16;
 template <typename T>
 __global__ void k_nffft_sample(Grid<T> g, Nffft n, T *ring, int slot) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  const long long Q = (long long)n.nyb * n.nzb + (long long)n.nzb * n.nxb + (long long)n.nxb * n.nyb;
+  const long long Q = nf_cells(n);
   T v[8];
   int pair, cells, cell;
   if (!nffft_values<T>(g, n, q, v, pair, cells, cell)) return;
@@ -1173,7 +1187,7 @@ __global__ void k_nffft_sample(Grid<T> g, Nffft n, T *ring, int slot) {
 template <typename T>
 __global__ void __launch_bounds__(256) k_nffft_flush(Nffft n, const T *__restrict__ ring, int nslots,
                                                     long long step0) {
-  const int cx = n.nyb * n.nzb, cy = n.nzb * n.nxb, cz = n.nxb * n.nyb;
+  const int cx = n.nyb * n.nzb, cy = n.nzb * n.nxb, cz = n.zf ? n.nxb * n.nyb : 0;
   const long long Q = (long long)cx + cy + cz;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 8 * Q) return;

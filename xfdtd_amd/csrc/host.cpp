@@ -100,6 +100,25 @@ std::unique_ptr<Shape> Sphere::getWrappedBox() const {
       PointVector{2 * _radius, 2 * _radius, 2 * _radius});
 }
 
+// shape/cylinder.cpp:21-42: inside the wrapped box, then within the radius
+// of the axis
+bool Cylinder::isPointInside(const PointVector &p) const {
+  if (!getWrappedBox()->isPointInside(p)) return false;
+  const int u = _axis == Axis::X ? 1 : 0, v = _axis == Axis::Z ? 1 : 2;
+  const double du = p[u] - _center[u], dv = p[v] - _center[v];
+  return isLessOrEqual(std::sqrt(du * du + dv * dv), _radius);
+}
+// shape/cylinder.cpp:44-60
+std::unique_ptr<Shape> Cylinder::getWrappedBox() const {
+  PointVector lo, size;
+  const int ax = static_cast<int>(_axis);
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = _center[a] - (a == ax ? _height / 2 : _radius);
+    size[a] = a == ax ? _height : 2 * _radius;
+  }
+  return std::make_unique<Cube>(lo, size);
+}
+
 // waveform/gaussian_waveform.cpp:18-24
 double GaussianWaveform::getValueByTime(double time) const {
   return getAmplitude() * std::exp(-std::pow((time - _t_0) / _tau, 2));
@@ -227,11 +246,50 @@ std::vector<std::complex<double>> TFSF::getIncidentWaveFourierTransform(
   return res;
 }
 
+std::pair<std::vector<double>, std::vector<std::complex<double>>> TFSF::getIncidentWaveFastFourierTransform()
+    const {
+  const auto &w = _waveform->getValue();
+  const size_t T = w.size(), Ns = 2 * T - 1;
+  std::pair<std::vector<double>, std::vector<std::complex<double>>> r;
+  r.first.resize(Ns / 2 + 1);
+  for (size_t q = 0; q < r.first.size(); ++q) r.first[q] = (double)q / ((double)Ns * _dt);
+  r.second.resize(Ns);
+  for (size_t q = 0; q < Ns; ++q) {
+    // bin m = q - (T - 1) of the shifted transform, X(m) = sum_n w[n] e^{-2 pi i m n / Ns}
+    const long long m = (long long)q - (long long)(T - 1);
+    const size_t mm = (size_t)((m % (long long)Ns + (long long)Ns) % (long long)Ns);
+    double re = 0, im = 0;
+    for (size_t n = 0; n < T; ++n) {
+      const double a = -2 * constant::PI * (double)((mm * n) % Ns) / (double)Ns;
+      re += w[n] * std::cos(a);
+      im += w[n] * std::sin(a);
+    }
+    r.second[q] = {re, im};
+  }
+  return r;
+}
+
 // tfsf/tfsf_3d.cpp:13-31
 TFSF3D::TFSF3D(SpatialIndex distance_x, SpatialIndex distance_y, SpatialIndex distance_z,
                double e_0, double theta_inc, double phi_inc, double psi,
                std::shared_ptr<Waveform> waveform)
     : TFSF(distance_x, distance_y, distance_z, e_0, theta_inc, phi_inc, psi, std::move(waveform)) {
+  const double sp = _sin_phi_inc, cp = _cos_phi_inc, st = _sin_theta_inc, ct = _cos_theta_inc;
+  const double u[3][3] = {{-sp, ct * cp, st * cp}, {cp, ct * sp, st * sp}, {0, -st, ct}};
+  const double ke[3] = {_sin_psi, _cos_psi, 0};
+  for (int r = 0; r < 3; ++r) _transform_e[r] = u[r][0] * ke[0] + u[r][1] * ke[1] + u[r][2] * ke[2];
+  const PointVector &k = _k;
+  const PointVector &e = _transform_e;
+  _transform_h = {k[1] * e[2] - k[2] * e[1], k[2] * e[0] - k[0] * e[2], k[0] * e[1] - k[1] * e[0]};
+}
+
+// tfsf/tfsf_2d.cpp:10-24 (TFSF with theta = pi/2, psi = 0).  The
+// polarisation (sin psi, cos psi) is taken as (sin psi, cos psi, 0): the
+// reference builds it as a 3-vector from two values, whose intended third
+// entry is 0.
+TFSF2D::TFSF2D(SpatialIndex distance_x, SpatialIndex distance_y, double phi_inc, double ez_i0,
+               std::shared_ptr<Waveform> waveform)
+    : TFSF(distance_x, distance_y, 0, ez_i0, constant::PI / 2, phi_inc, 0, std::move(waveform)) {
   const double sp = _sin_phi_inc, cp = _cos_phi_inc, st = _sin_theta_inc, ct = _cos_theta_inc;
   const double u[3][3] = {{-sp, ct * cp, st * cp}, {cp, ct * sp, st * sp}, {0, -st, ct}};
   const double ke[3] = {_sin_psi, _cos_psi, 0};
@@ -361,10 +419,12 @@ void Simulation::initGridSpace() {
     if (ori % 2 == 0) mn[a] = _min[a] - size; else mx[a] = _max[a] + size;
     resizeSpaceSize(mn, mx);
   }
-  for (int a = 0; a < 3; ++a) {
-    _n[a] = std::max<size_t>(_n[a], 1);
-    if (_n[a] <= 1) throw std::runtime_error("only 3-D grids are supported");
-  }
+  // grid/grid_space.cpp:501-536: THREE_DIMENSION, or TWO_DIMENSION (the
+  // TMz branch of simulation_update.cpp:157-168) with one z cell
+  for (int a = 0; a < 3; ++a) _n[a] = std::max<size_t>(_n[a], 1);
+  _two_d = _n[2] == 1;
+  if (_n[0] <= 1 || _n[1] <= 1)
+    throw std::runtime_error("only 3-D grids and 2-D grids with one z cell are supported");
   for (int a = 0; a < 3; ++a) {
     const size_t n = _n[a];
     _ec[a].resize(n + 1); _hc[a].resize(n); _es[a].resize(n); _hs[a].resize(n + 1);
@@ -431,8 +491,20 @@ void Simulation::plan(size_t total_time_steps) {
   // fdtd_basic_coff.cpp:11-29, 342-346: cfl arrives as float
   const double cfl = static_cast<double>(_cfl);
   const double dx = _cell_size, dy = _cell_size, dz = _cell_size;
-  _dt = cfl / (constant::C_0 * std::sqrt(1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz)));
+  if (_two_d)  // fdtd_basic_coff.cpp:13-19, 347-349
+    _dt = cfl / (constant::C_0 * std::sqrt(1.0 / (dx * dx) + 1.0 / (dy * dy)));
+  else
+    _dt = cfl / (constant::C_0 * std::sqrt(1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz)));
   const double dt = _dt;
+  if (_two_d) {
+    if (!_lumped_elements.empty() || _has_design || _has_dft || !_line_plans.empty() ||
+        (_nffft && !dynamic_cast<NffftBroadBand *>(_nffft.get())))
+      throw std::runtime_error(
+          "a 2-D grid takes objects, PML on x and y, TFSF2D, NffftBroadBand, field monitors and probes only");
+    if (_tfsf && !dynamic_cast<TFSF2D *>(_tfsf.get())) throw std::runtime_error("a 2-D grid needs TFSF2D");
+  } else if (_tfsf && dynamic_cast<TFSF2D *>(_tfsf.get())) {
+    throw std::runtime_error("TFSF2D needs a 2-D grid");
+  }
 
   // ---- materials: 0 = default vacuum of the staggered arrays
   // (fdtd_basic_coff.cpp:313-339), 1..n = objects in insertion order
@@ -462,6 +534,8 @@ void Simulation::plan(size_t total_time_steps) {
   // object whose cell mask covers the cell resets it, as its arrays do
   std::map<size_t, std::array<double, 3>> psig;
   const double nan = std::numeric_limits<double>::quiet_NaN();
+  // z of a cell centre: 0 on a 2-D grid (grid/grid_space.cpp:241-247)
+  auto czc = [&](size_t k) { return _two_d ? 0.0 : _hc[2][k]; };
   for (size_t q = 0; q < _objects.size(); ++q) {
     const Shape *sh = _objects[q]->getShape();
     auto box = sh->getWrappedBox();
@@ -499,7 +573,7 @@ void Simulation::plan(size_t total_time_steps) {
     }
     for (auto it = psig.begin(); it != psig.end();) {
       const size_t cidx = it->first, i = cidx / si, j = (cidx % si) / sj, k = cidx % sj;
-      if ((int)i < nx && (int)j < ny && (int)k < nz && sh->isPointInside({_hc[0][i], _hc[1][j], _hc[2][k]}))
+      if ((int)i < nx && (int)j < ny && (int)k < nz && sh->isPointInside({_hc[0][i], _hc[1][j], czc(k)}))
         it = psig.erase(it);
       else
         ++it;
@@ -515,7 +589,7 @@ void Simulation::plan(size_t total_time_steps) {
     for (int i = r0[0]; i < r1[0]; ++i)
       for (int j = r0[1]; j < r1[1]; ++j)
         for (int k = r0[2]; k < r1[2]; ++k)
-          if (sh->isPointInside({_hc[0][i], _hc[1][j], _hc[2][k]})) {
+          if (sh->isPointInside({_hc[0][i], _hc[1][j], czc(k)})) {
             midx[(size_t)i * si + (size_t)j * sj + k] = (uint8_t)(q + 1);
             if (drude) dmask[(size_t)i * si + (size_t)j * sj + k] = (uint8_t)(q + 1);
           }
@@ -1006,16 +1080,18 @@ void Simulation::plan(size_t total_time_steps) {
     P.srcs.push_back(std::move(ps));
   }
 
-  // ---- TFSF (tfsf/tfsf.cpp:35-53, tfsf_3d.cpp:33-151)
+  // ---- TFSF (tfsf/tfsf.cpp:35-53, tfsf_3d.cpp:33-151, tfsf_2d.cpp:26-97)
   if (_tfsf) {
-    auto *t3 = dynamic_cast<TFSF3D *>(_tfsf.get());
-    if (!t3) throw std::runtime_error("only TFSF3D sources are supported");
+    TFSF *t3 = _tfsf.get();
+    auto *t2d = dynamic_cast<TFSF2D *>(t3);
+    auto *t3d = dynamic_cast<TFSF3D *>(t3);
+    if (!t2d && !t3d) throw std::runtime_error("only TFSF3D and TFSF2D sources are supported");
     const auto dist = t3->getDistance();
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < (t2d ? 2 : 3); ++a)
       if (dist[a] < 1 || 2 * dist[a] >= (size_t)N[a]) throw std::runtime_error("TFSF box does not fit the grid");
     const int l[3] = {(int)dist[0], (int)dist[1], (int)dist[2]};
     const int r[3] = {nx - l[0], ny - l[1], nz - l[2]};
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < (t2d ? 2 : 3); ++a) {
       // corrected E cells sit at l..r, H cells at l-1 and r
       if (in_slab_e(a, l[a]) || in_slab_e(a, r[a]) || in_slab_h(a, l[a] - 1) || in_slab_h(a, r[a]) ||
           dn[a] >= l[a] || (dp[a] > 0 && r[a] >= N[a] - dp[a]))
@@ -1033,10 +1109,15 @@ void Simulation::plan(size_t total_time_steps) {
                                        std::pow(std::cos(th), 4));
     const PointVector k = t3->_k;
     const size_t bn[3] = {(size_t)(r[0] - l[0]), (size_t)(r[1] - l[1]), (size_t)(r[2] - l[2])};
-    const double diag = std::sqrt(std::pow(bn[0], 2) + std::pow(bn[1], 2) + std::pow(bn[2], 2)) * ratio;
+    // tfsf_2d.cpp:40-43 takes sqrt(nz^2) where tfsf_3d.cpp has nz^2
+    const double diag =
+        (t2d ? std::sqrt(std::pow(bn[0], 2) + std::pow(bn[1], 2) + std::sqrt(std::pow(bn[2], 2)))
+             : std::sqrt(std::pow(bn[0], 2) + std::pow(bn[1], 2) + std::pow(bn[2], 2))) *
+        ratio;
     const size_t L = static_cast<size_t>(std::ceil(diag)) + 4 + 1;
     double org[3];
     for (int a = 0; a < 3; ++a) org[a] = (k[a] >= 0) ? (double)l[a] : (double)r[a];
+    if (t2d) org[2] = 0;  // tfsf_2d.cpp:60-77
     for (int a = 0; a < 3; ++a) org[a] -= 2 * k[a] / ratio;
     std::vector<double> *full = tp.full, *half = tp.half;
     for (int a = 0; a < 3; ++a) {
@@ -1048,7 +1129,7 @@ void Simulation::plan(size_t total_time_steps) {
     // every interpolation index (tfsf_3d.cpp:418-422) must fall inside the
     // 1-D grid: bound the sum of the three projections from both sides
     double psum = 0, pmin = -0.5;
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < (t2d ? 2 : 3); ++a) {
       double mx = full[a][0], mn = full[a][0];
       for (double v : half[a]) { mx = std::max(mx, v); mn = std::min(mn, v); }
       for (double v : full[a]) { mx = std::max(mx, v); mn = std::min(mn, v); }
@@ -1064,9 +1145,10 @@ void Simulation::plan(size_t total_time_steps) {
     for (int a = 0; a < 3; ++a) {
       td.proj_full[a] = full[a].data();
       td.proj_half[a] = half[a].data();
-      td.transform_e[a] = t3->_transform_e[a];
-      td.transform_h[a] = t3->_transform_h[a];
+      td.transform_e[a] = t2d ? t2d->_transform_e[a] : t3d->_transform_e[a];
+      td.transform_h[a] = t2d ? t2d->_transform_h[a] : t3d->_transform_h[a];
     }
+    td.two_d = t2d ? 1 : 0;
     td.ceie = 1.0;
     td.chih = 1.0;
     td.ceihi = -(dt / (constant::EPSILON_0 * scaled_dl));
@@ -1086,7 +1168,8 @@ void Simulation::plan(size_t total_time_steps) {
   if (auto *bb = dynamic_cast<NffftBroadBand *>(_nffft.get())) {
     const auto dist = bb->getDistance();
     for (int a = 0; a < 3; ++a)
-      if (dist[a] < 1 || 2 * dist[a] >= (size_t)N[a]) throw std::runtime_error("NF-FF box does not fit the grid");
+      if (_two_d && a == 2 ? dist[a] != 0 : (dist[a] < 1 || 2 * dist[a] >= (size_t)N[a]))
+        throw std::runtime_error("NF-FF box does not fit the grid");
     P.bb.present = true;
     const int64_t ext[3] = {nx, ny, nz};
     for (int a = 0; a < 3; ++a) {
@@ -1496,6 +1579,10 @@ BroadBandFarField Simulation::calculateBroadBandFarField(const std::vector<doubl
     r.power_theta[q] = coff * std::pow(std::abs(eta * n_theta + l_phi), 2);
     r.power_phi[q] = coff * std::pow(std::abs(-eta * n_phi + l_theta), 2);
   }
+  if (_two_d) {
+    r.ez.resize(Ns);
+    for (size_t q = 0; q < Ns; ++q) r.ez[q] = 0.5 * std::sqrt(1i * r.wave_number[q]) * (r.l_phi[q] + eta * r.n_theta[q]);
+  }
   return r;
 }
 
@@ -1801,12 +1888,29 @@ void Simulation::outputBroadBand(const NffftBroadBand &bb) const {
   }
   const std::vector<size_t> shp = {r.frequencies.size()};
   auto p = [&](const char *name) { return dir + "/" + name; };
+  if (_two_d) {  // nffft_broadband.cpp:173-179: the 2-D far field only
+    writeNpy(p("EZ.npy"), r.ez.data(), "<c16", 16, shp);
+    return;
+  }
   writeNpy(p("e_theta.npy"), r.e_theta.data(), "<c16", 16, shp);
   writeNpy(p("e_phi.npy"), r.e_phi.data(), "<c16", 16, shp);
   writeNpy(p("h_theta.npy"), r.h_theta.data(), "<c16", 16, shp);
   writeNpy(p("h_phi.npy"), r.h_phi.data(), "<c16", 16, shp);
   writeNpy(p("power_theta.npy"), r.power_theta.data(), "<f8", 8, shp);
   writeNpy(p("power_phi.npy"), r.power_phi.data(), "<f8", 8, shp);
+}
+
+void Simulation::outputTFSFIncidentWaveFastFourierTransform(const std::string &path) const {
+  if (!_tfsf) return;
+  try {
+    makeDirs(path);
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "Error: %s\n", e.what());
+    return;
+  }
+  const auto [f, v] = _tfsf->getIncidentWaveFastFourierTransform();
+  writeNpy(path + "/frequencies.npy", f.data(), "<f8", 8, {f.size()});
+  writeNpy(path + "/value.npy", v.data(), "<c16", 16, {v.size()});
 }
 
 std::vector<double> Simulation::getProbeData() const {

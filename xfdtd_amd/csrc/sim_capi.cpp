@@ -66,6 +66,18 @@ extern "C" int xf_sim_add_sphere(xf_sim *s, const double center[3], double radiu
   });
 }
 
+extern "C" int xf_sim_add_cylinder(xf_sim *s, int axis, const double center[3], double radius, double height,
+                                   float eps_r, float mu_r, double sigma_e, double sigma_m) {
+  return guarded([&] {
+    if (axis < 0 || axis > 2) throw std::invalid_argument("bad cylinder axis");
+    s->sim.addObject(std::make_shared<Object>(
+        "cylinder",
+        std::make_unique<Cylinder>(static_cast<Axis>(axis), PointVector{center[0], center[1], center[2]}, radius,
+                                   height),
+        mat(eps_r, mu_r, sigma_e, sigma_m)));
+  });
+}
+
 extern "C" int xf_sim_add_drude_cube(xf_sim *s, const double origin[3], const double size[3], double eps_inf,
                                      double omega_p, double nv, double sigma_e, double sigma_m) {
   return guarded([&] {
@@ -161,6 +173,16 @@ extern "C" int xf_sim_add_current_source(xf_sim *s, const double origin[3], cons
 
 static std::unique_ptr<Cube> cube_of(const double origin[3], const double size[3]) {
   return std::make_unique<Cube>(PointVector{origin[0], origin[1], origin[2]}, PointVector{size[0], size[1], size[2]});
+}
+
+static std::unique_ptr<Waveform> waveform_of(int wkind, const double *wp);
+
+extern "C" int xf_sim_set_tfsf2d(xf_sim *s, size_t distance_x, size_t distance_y, double phi_inc, double ez_i0,
+                                 int wkind, const double wp[4]) {
+  return guarded([&] {
+    s->sim.addTFSFSource(
+        std::make_shared<TFSF2D>(distance_x, distance_y, phi_inc, ez_i0, std::shared_ptr<Waveform>(waveform_of(wkind, wp))));
+  });
 }
 
 static std::unique_ptr<Waveform> waveform_of(int wkind, const double *wp) {
@@ -403,6 +425,14 @@ extern "C" int xf_sim_broadband_farfield(xf_sim *s, const double *wave_number, d
       std::memcpy(out + 2 * Ns + 2 * Ns * q, c[q]->data(), Ns * sizeof(std::complex<double>));
     std::memcpy(out + 30 * Ns, r.power_theta.data(), Ns * sizeof(double));
     std::memcpy(out + 31 * Ns, r.power_phi.data(), Ns * sizeof(double));
+  });
+}
+
+extern "C" int xf_sim_broadband_ez(xf_sim *s, double *out) {
+  return guarded([&] {
+    BroadBandFarField r = s->sim.calculateBroadBandFarField(nullptr);
+    if (r.ez.empty()) throw std::runtime_error("the far-zone Ez is defined on 2-D grids only");
+    std::memcpy(out, r.ez.data(), r.ez.size() * sizeof(std::complex<double>));
   });
 }
 

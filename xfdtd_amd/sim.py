@@ -53,6 +53,9 @@ class Simulation:
                 elif type(o).__name__ == "ObjectPlane":
                     self._chk(L.xf_sim_add_plane(self.h, _dp(_f64(sh.origin)), _dp(_f64(sh.size)), m.eps_r,
                                                  m.mu_r, m.sigma_e, m.sigma_m))
+                elif hasattr(sh, "height"):
+                    self._chk(L.xf_sim_add_cylinder(self.h, int(sh.axis), _dp(_f64(sh.center)), float(sh.radius),
+                                                    float(sh.height), m.eps_r, m.mu_r, m.sigma_e, m.sigma_m))
                 elif hasattr(sh, "radius"):
                     c = _f64(sh.center)
                     self._chk(L.xf_sim_add_sphere(self.h, _dp(c), float(sh.radius), m.eps_r, m.mu_r,
@@ -65,7 +68,12 @@ class Simulation:
                 self._chk(L.xf_sim_add_pml(self.h, int(b.orientation), int(b.thickness), int(b.order),
                                            b.sigma_ratio, b.alpha_min, b.alpha_max, b.kappa_max))
             t = scene.tfsf
-            if t is not None:
+            if t is not None and hasattr(t, "phi_inc"):
+                from xfdtd_amd.scene import waveform_desc
+                wk, wp = waveform_desc(t.waveform)
+                self._chk(L.xf_sim_set_tfsf2d(self.h, int(t.distance_x), int(t.distance_y), float(t.phi_inc),
+                                              float(t.ez_i0), wk, _dp(_f64(wp))))
+            elif t is not None:
                 dist = (C.c_size_t * 3)(t.distance_x, t.distance_y, t.distance_z)
                 w = t.waveform
                 self._chk(L.xf_sim_set_tfsf3d(self.h, dist, t.e_0, t.theta, t.phi, t.psi,
@@ -235,6 +243,10 @@ class Simulation:
             r[name] = cplx[q].copy()
         r["power_theta"] = out[30 * Ns:31 * Ns].copy()
         r["power_phi"] = out[31 * Ns:].copy()
+        if self.dims[2] == 1:
+            ez = np.empty(2 * Ns, dtype=np.float64)
+            self._chk(lib().xf_sim_broadband_ez(self.h, _dp(ez)))
+            r["ez"] = ez.view(np.complex128).copy()
         return r
 
     # ---- voltage / current monitors and networks
