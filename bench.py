@@ -41,6 +41,10 @@ one batch against a loop of single gradients.
 broadband: the reference's monostatic example (96^3 PEC sphere, broadband
 NF-FF, FP64, 1200 steps): stepping with and without the broadband surface,
 and the far field of all 2T-1 bins (hipFFT + fold).
+
+tmz: the reference's 2-D example (370 x 320 TMz cells, PEC cylinder, TFSF2D,
+broadband NF-FF, 1400 steps) in FP64 and FP32, its far field, and the
+oracle's step time on the same scene.
 """
 from __future__ import annotations
 
@@ -371,6 +375,47 @@ def run_broadband(steps=1200):
     return out
 
 
+def run_tmz(steps=1400, cpu_steps=200, cpu_threads=16):
+    """The reference's 2-D example (examples/2d_cylinder_scatter.cpp): a
+    370 x 320 TMz grid (350 x 300 plus a 10-cell CPML on x and y) with a PEC
+    cylinder, TFSF2D along pi/4 and a broadband NF-FF box, FP64 and FP32
+    steps, the far field of all 2T-1 bins, and the oracle's step time on the
+    same scene (CPU, cpu_threads threads, cpu_steps steps)."""
+    from xfdtd_amd import scene as S
+    from xfdtd_amd.sim import Simulation
+    dx = 3e8 / 20e9 / 20
+    sc = S.cylinder_2d_scene(350, 300, 10, dx, 0.03, tfsf_gap=50, nffft_gap=40)
+    out = {"workload": "reference 2-D example: 370 x 320 TMz cells incl. 10-cell CPML on x and y, PEC cylinder "
+                       "r 3 cm, TFSF2D along pi/4, broadband NF-FF box at 40",
+           "steps": steps, "cells": 370 * 320}
+    for prec in (64, 32):
+        sim = Simulation(sc, steps, precision=prec)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.sync()
+        dt = time.perf_counter() - t0
+        out[f"fp{prec}_step_s"] = round(dt, 4)
+        out[f"fp{prec}_us_per_step"] = round(1e6 * dt / steps, 2)
+        out[f"fp{prec}_mcells_per_s"] = round(370 * 320 * steps / dt / 1e6, 1)
+        if prec == 64:
+            sim.bb_farfield()
+            t0 = time.perf_counter()
+            sim.bb_farfield()
+            out["far_field_s"] = round(time.perf_counter() - t0, 4)
+        sim.close()
+    if cpu_steps > 0:
+        from oracle.oracle import OracleSim, set_threads
+        threads = set_threads(cpu_threads)
+        o = OracleSim(sc, cpu_steps)
+        t0 = time.perf_counter()
+        o.step(cpu_steps)
+        cdt = time.perf_counter() - t0
+        out["cpu_oracle"] = {"us_per_step": round(1e6 * cdt / cpu_steps, 1), "threads": threads,
+                             "steps": cpu_steps, "kind": "port"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -392,6 +437,8 @@ def main():
     ap.add_argument("--batch-adjoint-steps", type=int, default=1200)
     ap.add_argument("--broadband", type=int, default=1200,
                     help="steps of the monostatic broadband NF-FF leg (0: skip it)")
+    ap.add_argument("--tmz", type=int, default=1400,
+                    help="steps of the 2-D (TMz) cylinder leg (0: skip it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 1 GiB device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -481,6 +528,8 @@ def main():
         line["batched_adjoint"] = run_batched_adjoint(args.batch_adjoint, args.batch_adjoint_steps)
     if args.broadband > 0:
         line["broadband"] = run_broadband(args.broadband)
+    if args.tmz > 0:
+        line["tmz"] = run_tmz(args.tmz, 0 if args.no_cpu else 200, max(1, args.cpu_threads))
     if args.batch > 0:
         line["batched"] = run_batched(args.batch, args.batch_steps,
                                       log=lambda m: print(m, file=sys.stderr, flush=True))
