@@ -32,6 +32,8 @@ def lib():
         L.xo_sim_add_sphere.argtypes = [vp, dp, d, C.c_float, C.c_float, d, d]
         L.xo_sim_add_pml.argtypes = [vp, i, i, i, d, d, d, d]
         L.xo_sim_set_tfsf.argtypes = [vp, szp, d, d, d, d, d, d, d]
+        L.xo_sim_set_debye.argtypes = [vp, d, d, d]
+        L.xo_sim_set_lorentz.argtypes = [vp, d, d, d, d]
         L.xo_sim_add_cylinder.argtypes = [vp, C.c_int, dp, d, d, C.c_float, C.c_float, d, d]
         L.xo_sim_set_tfsf2d.argtypes = [vp, C.c_size_t, C.c_size_t, d, d, C.c_int, dp]
         L.xo_sim_set_nffft.argtypes = [vp, szp, dp, i, dp, i, dp, i]
@@ -131,8 +133,14 @@ class OracleSim:
                     p, s = _arr3(sh.origin), _arr3(sh.size)
                     self._chk(L.xo_sim_add_cube(self.h, _dp(p), _dp(s), m.eps_r, m.mu_r,
                                                 m.sigma_e, m.sigma_m))
-                if hasattr(m, "omega_p"):
+                kind = type(m).__name__
+                if kind == "DrudeMedium":
                     self._chk(L.xo_sim_set_drude(self.h, float(m.eps_inf), float(m.omega_p), float(m.nu)))
+                elif kind == "DebyMedium":
+                    self._chk(L.xo_sim_set_debye(self.h, float(m.eps_s), float(m.eps_inf), float(m.tau)))
+                elif kind == "LorentzMedium":
+                    self._chk(L.xo_sim_set_lorentz(self.h, float(m.eps_s), float(m.eps_inf), float(m.omega_p),
+                                                   float(m.nu)))
             for b in scene.boundaries:
                 self._chk(L.xo_sim_add_pml(self.h, int(b.orientation), int(b.thickness),
                                            int(b.order), b.sigma_ratio, b.alpha_min,

@@ -104,9 +104,11 @@ typedef struct {
   double p[3], s[3], r, h;
   float eps_r, mu_r;
   double sigma_e, sigma_m;
-  /* DrudeMedium(eps_inf, omega_p, nv) when drude != 0 */
+  /* dispersive media (material/dispersive_material.h): drude = 1
+     DrudeMedium(eps_inf, omega_p, nv), 2 DebyMedium(eps_s, eps_inf, tau),
+     3 LorentzMedium(eps_s, eps_inf, omega_p, nv) */
   int drude;
-  double eps_inf, omega_p, nu;
+  double eps_inf, omega_p, nu, eps_s, tau;
 } obj_t;
 
 typedef struct {
@@ -231,8 +233,11 @@ struct xo_sim {
      per-object coefficients and the polarisation currents J (E shapes) */
   int has_drude;
   int *dmask;
-  double *dk, *dbeta, *dca, *dcb;  /* [nobj] */
+  double *dk, *dbeta, *dca, *dcb;  /* [nobj]; Lorentz: alpha, gamma, ca, cb */
+  double *dxi, *dcc;               /* [nobj]; Lorentz only: xi, cc */
+  int has_lorentz;
   a3 J[3];
+  a3 Jp[3], Ep[3];  /* Lorentz: J and E of the step before (E shapes) */
 };
 
 /* coefficient indices */
@@ -303,7 +308,13 @@ void xo_sim_destroy(xo_sim *s) {
   free(s->dft_acc);
   free(s->dmask);
   free(s->dk); free(s->dbeta); free(s->dca); free(s->dcb);
-  for (int q = 0; q < 3; ++q) a3_free(&s->J[q]);
+  for (int q = 0; q < 3; ++q) {
+    a3_free(&s->J[q]);
+    a3_free(&s->Jp[q]);
+    a3_free(&s->Ep[q]);
+  }
+  free(s->dxi);
+  free(s->dcc);
   free(s);
 }
 
@@ -341,6 +352,40 @@ int xo_sim_set_drude(xo_sim *s, double eps_inf, double omega_p, double nu) {
   o->omega_p = omega_p;
   o->nu = nu;
   s->has_drude = 1;
+  return 0;
+}
+
+/* DebyMedium(name, eps_s, eps_inf, tau, sigma_e, sigma_m) and
+   LorentzMedium(name, eps_s, eps_inf, omega_p, nv, sigma_e, sigma_m)
+   (material/dispersive_material.h:59-76, 137-141): the last added object
+   becomes a LinearDispersiveMaterial of that kind */
+int xo_sim_set_debye(xo_sim *s, double eps_s, double eps_inf, double tau) {
+  if (s->inited) FAIL("simulation already initialised");
+  if (s->nobj == 0) FAIL("no object to make dispersive");
+  obj_t *o = &s->objs[s->nobj - 1];
+  o->eps_r = 1.0f;
+  o->mu_r = 1.0f;
+  o->drude = 2;
+  o->eps_s = eps_s;
+  o->eps_inf = eps_inf;
+  o->tau = tau;
+  s->has_drude = 1;
+  return 0;
+}
+
+int xo_sim_set_lorentz(xo_sim *s, double eps_s, double eps_inf, double omega_p, double nu) {
+  if (s->inited) FAIL("simulation already initialised");
+  if (s->nobj == 0) FAIL("no object to make dispersive");
+  obj_t *o = &s->objs[s->nobj - 1];
+  o->eps_r = 1.0f;
+  o->mu_r = 1.0f;
+  o->drude = 3;
+  o->eps_s = eps_s;
+  o->eps_inf = eps_inf;
+  o->omega_p = omega_p;
+  o->nu = nu;
+  s->has_drude = 1;
+  s->has_lorentz = 1;
   return 0;
 }
 
@@ -1397,24 +1442,52 @@ static int init_drude(xo_sim *s) {
   s->dbeta = (double *)calloc((size_t)s->nobj, sizeof(double));
   s->dca = (double *)calloc((size_t)s->nobj, sizeof(double));
   s->dcb = (double *)calloc((size_t)s->nobj, sizeof(double));
-  if (!s->dmask || !s->dk || !s->dbeta || !s->dca || !s->dcb) FAIL("out of memory");
+  s->dxi = (double *)calloc((size_t)s->nobj, sizeof(double));
+  s->dcc = (double *)calloc((size_t)s->nobj, sizeof(double));
+  if (!s->dmask || !s->dk || !s->dbeta || !s->dca || !s->dcb || !s->dxi || !s->dcc) FAIL("out of memory");
   for (int q = 0; q < 3; ++q) {
     size_t sh[3];
     field_shape(s, q, sh);
     if (a3_alloc(&s->J[q], sh[0], sh[1], sh[2], 0.0)) FAIL("out of memory");
+    if (s->has_lorentz &&
+        (a3_alloc(&s->Jp[q], sh[0], sh[1], sh[2], 0.0) || a3_alloc(&s->Ep[q], sh[0], sh[1], sh[2], 0.0)))
+      FAIL("out of memory");
   }
   double dt = s->dt, dl = s->dl;
   for (int q = 0; q < s->nobj; ++q) {
     const obj_t *o = &s->objs[q];
     if (!o->drude) continue;
-    /* DrudeMedium::init (dispersive_material.cpp:127-144); dl is the grid's
-       base size (dispersive_material.h:47-48) */
+    /* dl is the grid's base size (dispersive_material.h:47-48) */
     double sigma_e = o->sigma_e;
-    s->dk[q] = (2 - o->nu * dt) / (2 + o->nu * dt);
-    s->dbeta[q] = (o->omega_p * o->omega_p * EPS0 * dt) / (2 + o->nu * dt);
-    double temp = 2 * EPS0 * o->eps_inf + s->dbeta[q] * dt + sigma_e * dt;
-    s->dca[q] = (2 * EPS0 * o->eps_inf - s->dbeta[q] * dt - sigma_e * dt) / temp;
-    s->dcb[q] = 2 * dt / (temp * dl);
+    if (o->drude == 2) {
+      /* DebyMedium::init (dispersive_material.cpp:205-222) */
+      double delta_eps = o->eps_s - o->eps_inf;
+      s->dk[q] = (2 * o->tau - dt) / (2 * o->tau + dt);
+      s->dbeta[q] = (2 * delta_eps * EPS0 * dt) / (2 * o->tau + dt);
+      double temp = 2 * EPS0 * o->eps_inf + s->dbeta[q] + sigma_e * dt;
+      s->dca[q] = (2 * EPS0 * o->eps_inf + s->dbeta[q] - sigma_e * dt) / temp;
+      s->dcb[q] = (2 * dt) / (temp * dl);
+    } else if (o->drude == 3) {
+      /* LorentzMedium::init (dispersive_material.cpp:17-36): dk = alpha,
+         dbeta = gamma, dca / dcb the E and previous-E weights, dcc the curl
+         weight */
+      double delta_eps = o->eps_s - o->eps_inf;
+      double t0 = o->nu * dt + 1;
+      s->dk[q] = (2 - o->omega_p * o->omega_p * dt * dt) / t0;
+      s->dxi[q] = (o->nu * dt - 1) / t0;
+      s->dbeta[q] = (EPS0 * delta_eps * o->omega_p * o->omega_p * dt * dt) / t0;
+      double t1 = 4 * EPS0 * o->eps_inf + s->dbeta[q] + 2 * sigma_e * dt;
+      s->dca[q] = (4 * EPS0 * o->eps_inf - 2 * sigma_e * dt) / t1;
+      s->dcb[q] = s->dbeta[q] / t1;
+      s->dcc[q] = (4 * dt) / (t1 * dl);
+    } else {
+      /* DrudeMedium::init (dispersive_material.cpp:127-144) */
+      s->dk[q] = (2 - o->nu * dt) / (2 + o->nu * dt);
+      s->dbeta[q] = (o->omega_p * o->omega_p * EPS0 * dt) / (2 + o->nu * dt);
+      double temp = 2 * EPS0 * o->eps_inf + s->dbeta[q] * dt + sigma_e * dt;
+      s->dca[q] = (2 * EPS0 * o->eps_inf - s->dbeta[q] * dt - sigma_e * dt) / temp;
+      s->dcb[q] = 2 * dt / (temp * dl);
+    }
     for (size_t i = 0; i < nx; ++i)
       for (size_t j = 0; j < ny; ++j)
         for (size_t k = 0; k < nz; ++k) {
@@ -1425,13 +1498,39 @@ static int init_drude(xo_sim *s) {
   return 0;
 }
 
-/* simulation/simulation_update.cpp:221-280 with DrudeMedium::updateE{x,y,z}
-   (dispersive_material.cpp:146-193) on the Drude cells */
+/* One dispersive cell of component c: DrudeMedium::updateE{x,y,z}
+   (dispersive_material.cpp:146-193), DebyMedium::updateE{x,y,z} (224-275)
+   or LorentzMedium::updateE{x,y,z} (38-117).  curl is the reference's
+   -(dH_a) + (dH_b) of the component, in H differences. */
+static void disp_cell(xo_sim *s, int q, int c, size_t i, size_t j, size_t k, double curl) {
+  const double dl = s->dl, dt = s->dt;
+  double *e = &AT(s->f[c], i, j, k), *jc = &AT(s->J[c], i, j, k);
+  const double e_old = *e;
+  double j_sum = 0;
+  if (s->objs[q].drude == 3) {
+    double *ep = &AT(s->Ep[c], i, j, k), *jp = &AT(s->Jp[c], i, j, k);
+    const double j_old = *jc;
+    j_sum += 0.5 * dl * ((1 + s->dk[q]) * *jc + s->dxi[q] * *jp);
+    *e = s->dca[q] * *e + s->dcb[q] * *ep + s->dcc[q] * (curl - j_sum);
+    *jc = s->dk[q] * *jc + s->dxi[q] * *jp + s->dbeta[q] * (*e - *ep) / (2 * dt);
+    *ep = e_old;
+    *jp = j_old;
+    return;
+  }
+  j_sum += 0.5 * dl * ((1 + s->dk[q]) * *jc);
+  *e = s->dca[q] * *e + s->dcb[q] * (curl - j_sum);
+  if (s->objs[q].drude == 2)
+    *jc = s->dk[q] * *jc + s->dbeta[q] * (*e - e_old) / dt;
+  else
+    *jc = s->dk[q] * *jc + s->dbeta[q] * (*e + e_old);
+}
+
+/* simulation/simulation_update.cpp:221-280 with the dispersive media's
+   updateE{x,y,z} on their cells */
 static void update_e_drude(xo_sim *s) {
   size_t nx = s->n[0], ny = s->n[1], nz = s->n[2];
   a3 *c = s->c;
   const int *dm = s->dmask;
-  const double dl = s->dl;
 #define DRUDE_AT(i, j, k) dm[((i) * ny + (j)) * nz + (k)]
 #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < nx; ++i)
@@ -1439,14 +1538,7 @@ static void update_e_drude(xo_sim *s) {
       for (size_t k = 1; k < nz; ++k) {
         int d = DRUDE_AT(i, j, k);
         if (d) {
-          int q = d - 1;
-          double *ex = &EX(i, j, k), *jx = &AT(s->J[0], i, j, k);
-          double ex_old = *ex;
-          double j_sum = 0;
-          j_sum += 0.5 * dl * ((1 + s->dk[q]) * *jx);
-          *ex = s->dca[q] * *ex + s->dcb[q] * (-(HY(i, j, k) - HY(i, j, k - 1)) +
-                                               (HZ(i, j, k) - HZ(i, j - 1, k)) - j_sum);
-          *jx = s->dk[q] * *jx + s->dbeta[q] * (*ex + ex_old);
+          disp_cell(s, d - 1, XO_EX, i, j, k, -(HY(i, j, k) - HY(i, j, k - 1)) + (HZ(i, j, k) - HZ(i, j - 1, k)));
           continue;
         }
         EX(i, j, k) = AT(c[CEXE], i, j, k) * EX(i, j, k) +
@@ -1459,14 +1551,7 @@ static void update_e_drude(xo_sim *s) {
       for (size_t i = 1; i < nx; ++i) {
         int d = DRUDE_AT(i, j, k);
         if (d) {
-          int q = d - 1;
-          double *ey = &EY(i, j, k), *jy = &AT(s->J[1], i, j, k);
-          double ey_old = *ey;
-          double j_sum = 0;
-          j_sum += 0.5 * dl * ((1 + s->dk[q]) * *jy);
-          *ey = s->dca[q] * *ey + s->dcb[q] * (-(HZ(i, j, k) - HZ(i - 1, j, k)) +
-                                               (HX(i, j, k) - HX(i, j, k - 1)) - j_sum);
-          *jy = s->dk[q] * *jy + s->dbeta[q] * (*ey + ey_old);
+          disp_cell(s, d - 1, XO_EY, i, j, k, -(HZ(i, j, k) - HZ(i - 1, j, k)) + (HX(i, j, k) - HX(i, j, k - 1)));
           continue;
         }
         EY(i, j, k) = AT(c[CEYE], i, j, k) * EY(i, j, k) +
@@ -1479,14 +1564,7 @@ static void update_e_drude(xo_sim *s) {
       for (size_t j = 1; j < ny; ++j) {
         int d = DRUDE_AT(i, j, k);
         if (d) {
-          int q = d - 1;
-          double *ez = &EZ(i, j, k), *jz = &AT(s->J[2], i, j, k);
-          double ez_old = *ez;
-          double j_sum = 0;
-          j_sum += 0.5 * dl * ((1 + s->dk[q]) * *jz);
-          *ez = s->dca[q] * *ez + s->dcb[q] * (-(HX(i, j, k) - HX(i, j - 1, k)) +
-                                               (HY(i, j, k) - HY(i - 1, j, k)) - j_sum);
-          *jz = s->dk[q] * *jz + s->dbeta[q] * (*ez + ez_old);
+          disp_cell(s, d - 1, XO_EZ, i, j, k, -(HX(i, j, k) - HX(i, j - 1, k)) + (HY(i, j, k) - HY(i - 1, j, k)));
           continue;
         }
         EZ(i, j, k) = AT(c[CEZE], i, j, k) * EZ(i, j, k) +
@@ -2495,8 +2573,15 @@ static double dft_objective(const xo_sim *s) {
    J = sum |F|^2 over the DFT plane after total_steps steps.  Runs the
    forward loop from step 0, keeping the design-box E fields of every step,
    then the reverse loop. */
+static int only_drude_dispersion(const xo_sim *s) {
+  for (int q = 0; q < s->nobj; ++q)
+    if (s->objs[q].drude > 1) return 0;
+  return 1;
+}
+
 int xo_sim_grad(xo_sim *s, double *grad, double *J_out) {
   if (!s->inited || s->cur != 0) FAIL("grad needs a fresh simulation");
+  if (!only_drude_dispersion(s)) FAIL("the adjoint covers Drude media only, not Debye or Lorentz");
   if (!s->has_design || !s->has_dft) FAIL("grad needs a design box and a DFT plane");
   const size_t *b = s->dbox;
   size_t nd = (b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]), T = s->T;
@@ -2552,6 +2637,7 @@ int xo_sim_grad(xo_sim *s, double *grad, double *J_out) {
    <L x, y> = <x, L^T y> of the step map L. */
 int xo_sim_adjoint_run(xo_sim *s, size_t nsteps, double *fields) {
   if (!s->inited) FAIL("not initialised");
+  if (!only_drude_dispersion(s)) FAIL("the adjoint covers Drude media only, not Debye or Lorentz");
   adj_t ad;
   if (adj_alloc(s, &ad)) return -1;
   a3 *st[9];

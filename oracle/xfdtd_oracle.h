@@ -42,6 +42,10 @@ int xo_sim_add_cylinder(xo_sim *s, int axis, const double center[3], double radi
 /* make the last added object a DrudeMedium(eps_inf, omega_p, nv) with its
    sigma_e / sigma_m (its eps_r and mu_r become 1) */
 int xo_sim_set_drude(xo_sim *s, double eps_inf, double omega_p, double nu);
+/* make the last added object a DebyMedium(eps_s, eps_inf, tau) or a
+   LorentzMedium(eps_s, eps_inf, omega_p, nv) with its sigma_e / sigma_m */
+int xo_sim_set_debye(xo_sim *s, double eps_s, double eps_inf, double tau);
+int xo_sim_set_lorentz(xo_sim *s, double eps_s, double eps_inf, double omega_p, double nu);
 /* polarisation current J of component comp (0..2, E shapes) */
 int xo_sim_get_drude_j(xo_sim *s, int comp, double *out);
 int xo_sim_set_drude_j(xo_sim *s, int comp, const double *in);

@@ -1,0 +1,196 @@
+"""Pins of the oracle's Debye and Lorentz media
+(material/dispersive_material.cpp:7-117 LorentzMedium, 195-275 DebyMedium;
+dispatch simulation/simulation_update.cpp:221-280, the same intended
+tagging as DrudeMedium).
+
+- Exact discrete dispersion.  A PEC box filled with the medium, started in
+  its TM110 mode, stays in the space of that mode: its Ez, the curl of H,
+  J and (Lorentz) the previous E and J reduce to scalar amplitudes, and one
+  time step is a small linear map of them.  The map is written here from
+  the ADE recursions of the two media in modal form (the curl-curl of the
+  mode is K^2 = (2/dl)^2 [sin^2(pi/2nx) + sin^2(pi/2ny)] times the mode);
+  the probe series of the 3-D oracle run must be a combination of the
+  map's eigenvalues z^n.  A misplaced previous value, a wrong stencil, a
+  wrong J update or a swapped coefficient leaves a residual orders above
+  rounding (asserted for planted mistakes in the map).  The same map for
+  the Drude medium reproduces the characteristic polynomial that pins it.
+- Physics.  The map's oscillating eigenvalues z = exp(i w dt) give the
+  frequencies and decay rates of the continuous dispersion relation
+  w^2 eps(w) / c^2 = K^2 with eps(w) = eps_inf + (eps_s - eps_inf) /
+  (1 + i w tau) (Debye) or eps_inf + (eps_s - eps_inf) wp^2 /
+  (wp^2 + 2 i w nv - w^2) (Lorentz) to within 1 % (the time-step error).
+- Reductions.  eps_s = eps_inf (Debye) and omega_p = 0 (Lorentz) make the
+  medium a plain dielectric eps_r = eps_inf with the same conductivity: the
+  fields agree with that run to 1e-12.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+from test_drude_cpu import DL, N, _cavity, _fit_residual, _roots
+from xfdtd_amd import scene as S
+
+EPS0, MU0 = S.EPSILON_0, S.MU_0
+
+
+def _k2():
+    return (2 / DL) ** 2 * (math.sin(math.pi / (2 * N[0])) ** 2 + math.sin(math.pi / (2 * N[1])) ** 2)
+
+
+def _modal_map(kind, dt, p, planted=None):
+    """One step of the TM110 mode amplitudes.  State: (e, C, J) for Drude and
+    Debye, (e, C, J, e_prev, J_prev) for Lorentz; C is the discrete curl of
+    H at the last half step."""
+    K2 = _k2()
+    sig = p.get("sigma", 0.0)
+    if kind == "lorentz":
+        de = p["eps_s"] - p["eps_inf"]
+        t0 = p["nu"] * dt + 1
+        al = (2 - p["wp"] ** 2 * dt * dt) / t0
+        xi = (p["nu"] * dt - 1) / t0
+        ga = EPS0 * de * p["wp"] ** 2 * dt * dt / t0
+        t1 = 4 * EPS0 * p["eps_inf"] + ga + 2 * sig * dt
+        ca, cb, cc = (4 * EPS0 * p["eps_inf"] - 2 * sig * dt) / t1, ga / t1, 4 * dt / t1
+
+        def step(x):
+            e, C, J, ep, jp = x
+            Cn = C - dt / MU0 * K2 * e
+            js = 0.5 * ((1 + al) * J + xi * jp)
+            if planted == "jsum_no_prev":
+                js = 0.5 * (1 + al) * J
+            en = ca * e + cb * ep + cc * (Cn - js)
+            if planted == "prev_swapped":
+                en = ca * e + cb * e + cc * (Cn - js)
+            Jn = al * J + xi * jp + ga * (en - ep) / (2 * dt)
+            return [en, Cn, Jn, e, J]
+        n = 5
+    else:
+        if kind == "debye":
+            de = p["eps_s"] - p["eps_inf"]
+            k = (2 * p["tau"] - dt) / (2 * p["tau"] + dt)
+            be = 2 * de * EPS0 * dt / (2 * p["tau"] + dt)
+            tmp = 2 * EPS0 * p["eps_inf"] + be + sig * dt
+            ca, cb = (2 * EPS0 * p["eps_inf"] + be - sig * dt) / tmp, 2 * dt / tmp
+        else:
+            k = (2 - p["nu"] * dt) / (2 + p["nu"] * dt)
+            be = p["wp"] ** 2 * EPS0 * dt / (2 + p["nu"] * dt)
+            tmp = 2 * EPS0 * p["eps_inf"] + be * dt + sig * dt
+            ca, cb = (2 * EPS0 * p["eps_inf"] - be * dt - sig * dt) / tmp, 2 * dt / tmp
+
+        def step(x):
+            e, C, J = x
+            Cn = C - dt / MU0 * K2 * e
+            en = ca * e + cb * (Cn - 0.5 * (1 + k) * J)
+            if kind == "debye":
+                Jn = k * J + be * (en - e) / dt
+                if planted == "debye_as_drude":
+                    Jn = k * J + be / dt * (en + e)
+            else:
+                Jn = k * J + be * (en + e)
+            return [en, Cn, Jn]
+        n = 3
+    M = np.zeros((n, n))
+    for c in range(n):
+        x = [0.0] * n
+        x[c] = 1.0
+        M[:, c] = step(x)
+    return np.linalg.eigvals(M)
+
+
+def _run_tm110(material, steps):
+    sc = _cavity(material)
+    sc.probes = [S.Probe(S.EZ, 4, 3, 1), S.Probe(S.EZ, 7, 6, 2)]
+    o = OracleSim(sc, steps)
+    nx, ny, _ = o.dims
+    i = np.arange(nx + 1)[:, None, None]
+    j = np.arange(ny + 1)[None, :, None]
+    ez = np.sin(math.pi * i / nx) * np.sin(math.pi * j / ny) * np.ones(o.field_shape(S.EZ))
+    ez[0], ez[nx], ez[:, 0], ez[:, ny] = 0, 0, 0, 0
+    o.set_field(S.EZ, ez)
+    o.step(steps)
+    return o, o.get_probes()
+
+
+DEBYE = [dict(eps_s=4.0, eps_inf=2.0, tau=1 / (2 * math.pi * 10e9)),
+         dict(eps_s=6.0, eps_inf=1.5, tau=1 / (2 * math.pi * 40e9), sigma=0.05)]
+LORENTZ = [dict(eps_s=3.0, eps_inf=1.5, wp=2 * math.pi * 15e9, nu=2 * math.pi * 1e9),
+           dict(eps_s=2.25, eps_inf=1.0, wp=2 * math.pi * 25e9, nu=2 * math.pi * 3e9, sigma=0.02)]
+
+
+def _medium(kind, p):
+    if kind == "debye":
+        return S.DebyMedium("d", p["eps_s"], p["eps_inf"], p["tau"], p.get("sigma", 0.0), 0.0)
+    return S.LorentzMedium("l", p["eps_s"], p["eps_inf"], p["wp"], p["nu"], p.get("sigma", 0.0), 0.0)
+
+
+def test_modal_map_reproduces_the_drude_polynomial():
+    dt = S.courant_dt(DL, 0.99)
+    p = dict(eps_inf=2.5, wp=2 * math.pi * 34e9, nu=2 * math.pi * 4e9)
+    z = np.sort_complex(_modal_map("drude", dt, p))
+    ref = np.sort_complex(_roots(dt, p["eps_inf"], p["wp"], p["nu"]))
+    assert np.abs(z - ref).max() < 1e-9
+
+
+@pytest.mark.parametrize("kind,p", [("debye", q) for q in DEBYE] + [("lorentz", q) for q in LORENTZ])
+def test_cavity_follows_discrete_dispersion(kind, p, oracle_threads):
+    steps = 400
+    o, pr = _run_tm110(_medium(kind, p), steps)
+    roots = _modal_map(kind, o.dt, p)
+    assert np.all(np.abs(roots) <= 1 + 1e-12)  # passive scheme
+    for series in pr:
+        assert np.abs(series).max() > 0.05
+        assert _fit_residual(series, roots) < 1e-10, kind
+    assert np.abs(o.get_drude_j(S.EZ)).max() > 0
+    planted = ["debye_as_drude"] if kind == "debye" else ["jsum_no_prev", "prev_swapped"]
+    for wrong in planted:
+        assert _fit_residual(pr[0], _modal_map(kind, o.dt, p, planted=wrong)) > 1e-4, wrong
+
+
+@pytest.mark.parametrize("kind,p", [("debye", DEBYE[0]), ("debye", {**DEBYE[1], "sigma": 0.0}),
+                                    ("lorentz", LORENTZ[0]), ("lorentz", {**LORENTZ[1], "sigma": 0.0})])
+def test_discrete_roots_follow_continuous_dispersion(kind, p):
+    dt = S.courant_dt(DL, 0.99)
+    K2c2 = _k2() / (MU0 * EPS0)  # K^2 c^2
+    ei, de = p["eps_inf"], p["eps_s"] - p["eps_inf"]
+    if kind == "debye":
+        t = p["tau"]
+        poly = [1j * t * ei, ei + de, -1j * t * K2c2, -K2c2]
+    else:
+        wp, nu = p["wp"], p["nu"]
+        poly = [-ei, 2j * nu * ei, ei * wp ** 2 + de * wp ** 2 + K2c2, -2j * nu * K2c2, -K2c2 * wp ** 2]
+    w_cont = np.roots(poly)
+    w_disc = -1j * np.log(_modal_map(kind, dt, p).astype(complex)) / dt
+    osc = [w for w in w_cont if w.real > 1e9]
+    assert len(osc) >= 1
+    for w in osc:
+        near = w_disc[np.argmin(np.abs(w_disc - w))]
+        assert abs(near - w) < 0.01 * abs(w), (w, near)
+
+
+@pytest.mark.parametrize("kind", ["debye", "lorentz"])
+def test_reduces_to_a_dielectric(kind):
+    steps = 40
+    eps_inf, sigma = 2.0, 0.02
+    if kind == "debye":
+        med = S.DebyMedium("d", eps_inf, eps_inf, 1e-11, sigma, 0.0)
+    else:
+        med = S.LorentzMedium("l", 3.0, eps_inf, 0.0, 2 * math.pi * 1e9, sigma, 0.0)
+    runs = []
+    for mat in (med, S.Material("m", eps_inf, 1.0, sigma, 0.0)):
+        o = OracleSim(_cavity(mat), steps)
+        nx, ny, nz = o.dims
+        rng = np.random.default_rng(5)
+        ez = rng.standard_normal(o.field_shape(S.EZ))
+        ez[0], ez[nx], ez[:, 0], ez[:, ny] = 0, 0, 0, 0
+        ex = rng.standard_normal(o.field_shape(S.EX))
+        ex[:, 0], ex[:, ny], ex[:, :, 0], ex[:, :, nz] = 0, 0, 0, 0
+        o.set_field(S.EZ, ez)
+        o.set_field(S.EX, ex)
+        o.step(steps)
+        runs.append([o.get_field(c) for c in range(6)])
+    for c in range(6):
+        scale = np.abs(runs[1][c]).max()
+        assert scale > 0
+        assert np.abs(runs[0][c] - runs[1][c]).max() <= 1e-12 * scale, c

@@ -124,6 +124,56 @@ class DrudeMedium:
 
 
 @dataclass
+class DebyMedium:
+    """DebyMedium(name, eps_s, eps_inf, tau, sigma_e, sigma_m)
+    (material/dispersive_material.h:137-141): eps(w) = eps_inf +
+    (eps_s - eps_inf) / (1 + i w tau); a LinearDispersiveMaterial
+    (eps_r = mu_r = 1 in the standard arrays)."""
+    name: str
+    eps_s: float
+    eps_inf: float
+    tau: float
+    sigma_e: float = 0.0
+    sigma_m: float = 0.0
+
+    @property
+    def eps_r(self):
+        return 1.0
+
+    @property
+    def mu_r(self):
+        return 1.0
+
+
+@dataclass
+class LorentzMedium:
+    """LorentzMedium(name, eps_s, eps_inf, omega_p, nv, sigma_e, sigma_m)
+    (material/dispersive_material.h:59-76): eps(w) = eps_inf +
+    (eps_s - eps_inf) omega_p^2 / (omega_p^2 + 2 i w nv - w^2); a
+    LinearDispersiveMaterial (eps_r = mu_r = 1 in the standard arrays)."""
+    name: str
+    eps_s: float
+    eps_inf: float
+    omega_p: float
+    nu: float
+    sigma_e: float = 0.0
+    sigma_m: float = 0.0
+
+    @property
+    def eps_r(self):
+        return 1.0
+
+    @property
+    def mu_r(self):
+        return 1.0
+
+
+def dispersion_kind(m):
+    """0 plain material, 1 Drude, 2 Debye, 3 Lorentz."""
+    return {"DrudeMedium": 1, "DebyMedium": 2, "LorentzMedium": 3}.get(type(m).__name__, 0)
+
+
+@dataclass
 class Object:
     name: str
     shape: object
