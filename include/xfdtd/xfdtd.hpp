@@ -146,9 +146,16 @@ class Material {
   double drudeEpsInf() const { return _drude_eps_inf; }
   double drudeOmegaP() const { return _drude_omega_p; }
   double drudeNv() const { return _drude_nv; }
+  // the dispersive kind: 0 none, 1 Drude, 2 Debye, 3 Lorentz; the Debye and
+  // Lorentz static permittivity and the Debye relaxation time
+  int dispersionKind() const { return _disp_kind; }
+  double dispersionEpsS() const { return _disp_eps_s; }
+  double dispersionTau() const { return _disp_tau; }
 
  protected:
   double _drude_eps_inf = 1, _drude_omega_p = 0, _drude_nv = 0;
+  int _disp_kind = 0;
+  double _disp_eps_s = 1, _disp_tau = 0;
 
  private:
   std::string _name;
@@ -169,6 +176,36 @@ class DrudeMedium : public Material {
     _drude_eps_inf = eps_inf;
     _drude_omega_p = pole_omega;
     _drude_nv = nv;
+    _disp_kind = 1;
+  }
+};
+
+// material/dispersive_material.h:137-141, dispersive_material.cpp:195-275:
+// eps(w) = eps_inf + (eps_s - eps_inf) / (1 + i w tau)
+class DebyMedium : public Material {
+ public:
+  DebyMedium(std::string_view name, double eps_s, double eps_inf, double tau, double sigma_e = 0,
+             double sigma_m = 0)
+      : Material(name, 1, 1, sigma_e, sigma_m, true) {
+    _drude_eps_inf = eps_inf;
+    _disp_eps_s = eps_s;
+    _disp_tau = tau;
+    _disp_kind = 2;
+  }
+};
+
+// material/dispersive_material.h:59-100, dispersive_material.cpp:7-117:
+// eps(w) = eps_inf + (eps_s - eps_inf) omega_p^2 / (omega_p^2 + 2 i w nv - w^2)
+class LorentzMedium : public Material {
+ public:
+  LorentzMedium(std::string_view name, double eps_s, double eps_inf, double omega_p, double nv,
+                double sigma_e = 0, double sigma_m = 0)
+      : Material(name, 1, 1, sigma_e, sigma_m, true) {
+    _drude_eps_inf = eps_inf;
+    _drude_omega_p = omega_p;
+    _drude_nv = nv;
+    _disp_eps_s = eps_s;
+    _disp_kind = 3;
   }
 };
 

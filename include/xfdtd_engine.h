@@ -24,6 +24,7 @@
  *   xf_engine_get_field     EMF::getEx..getHz                include/simulation/simulation.h:92-97
  *   xf_engine_get_nffft     NffftFd accumulators             nffft/nffft_fd.cpp:126-202
  *   xf_engine_farfield_sums NffftFd::calculateFarfieldX/Y/Z  nffft/nffft_fd.cpp:423-556
+ *   xf_engine_set_dispersive  Drude/Debye/LorentzMedium::updateE  material/dispersive_material.cpp:7-275
  *   xf_engine_set_nffft_broadband  NffftBroadBand::init / update  nffft/nffft_broadband.cpp:30-155
  *   xf_engine_broadband_sums       NffftBroadBand::calculateLNXYZ nffft/nffft_broadband.cpp:348-466
  *
@@ -263,6 +264,23 @@ xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags);
    components use the standard tables.  The J arrays start at zero and are
    part of the checkpointed state.  Call after xf_engine_set_materials. */
 xf_status xf_engine_set_drude(xf_engine *e, const int64_t *box, const double *coef);
+/* The dispersive media of the box, one record of XF_ADE_STRIDE doubles per
+   material: [XF_ADE_KIND] = 0 none, 1 DrudeMedium, 2 DebyMedium, 3
+   LorentzMedium (material/dispersive_material.cpp:7-275, by the tagging
+   xf_engine_set_drude describes); [5] = dl/2 and
+     Drude   [0..4] = ca, cb, 1 + k, k, beta:  as xf_engine_set_drude
+     Debye   [0..4] = ca, cb, 1 + k, k, beta, [9] = dt:
+             E' = ca E + cb (curl - dl/2 (1+k) J), J' = k J + beta (E' - E) / dt
+     Lorentz [0..4] = ca, cb, 1 + alpha, alpha, gamma, [7] = xi, [8] = cc,
+             [10] = 2 dt:  with the previous step's E_p and J_p,
+             E' = ca E + cb E_p + cc (curl - dl/2 ((1+alpha) J + xi J_p)),
+             J' = alpha J + xi J_p + gamma (E' - E_p) / (2 dt), then E_p = E, J_p = J.
+   J (and for Lorentz E_p, J_p) start at zero and are checkpointed state.
+   Gradients refuse Debye and Lorentz media.  Call after
+   xf_engine_set_materials, in place of xf_engine_set_drude. */
+#define XF_ADE_STRIDE 12
+#define XF_ADE_KIND 6
+xf_status xf_engine_set_dispersive(xf_engine *e, const int64_t *box, const double *coef);
 /* the Drude box, and J of component comp (0..2) over it, C order */
 xf_status xf_engine_drude_box(xf_engine *e, int64_t *box);
 xf_status xf_engine_get_drude_j(xf_engine *e, int comp, double *out);

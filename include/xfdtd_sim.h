@@ -57,6 +57,14 @@ int xf_sim_add_drude_cube(xf_sim *s, const double origin[3], const double size[3
                           double omega_p, double nv, double sigma_e, double sigma_m);
 int xf_sim_add_drude_sphere(xf_sim *s, const double center[3], double radius, double eps_inf,
                             double omega_p, double nv, double sigma_e, double sigma_m);
+/* addObject(Object(name, shape, medium)) for any dispersive medium:
+   shape 0 Cube (p = origin[3], size[3]), 1 Sphere (p = center[3], radius),
+   2 Cylinder (p = center[3], radius, height, axis); kind 1
+   DrudeMedium (mp = eps_inf, omega_p, nv), 2 DebyMedium (eps_s, eps_inf,
+   tau), 3 LorentzMedium (eps_s, eps_inf, omega_p, nv)
+   (material/dispersive_material.h:59-141) */
+int xf_sim_add_dispersive_object(xf_sim *s, int shape, const double p[6], int kind, const double mp[4],
+                                 double sigma_e, double sigma_m);
 int xf_sim_add_pml(xf_sim *s, int orientation, int thickness, int order, double sigma_ratio,
                    double alpha_min, double alpha_max, double kappa_max);
 int xf_sim_set_tfsf3d(xf_sim *s, const size_t distance[3], double e0, double theta, double phi,

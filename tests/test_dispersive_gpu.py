@@ -1,0 +1,117 @@
+"""Debye and Lorentz media on the GPU against the oracle.
+
+- FP64 bit for bit: fields, NF-FF accumulators and J in a scene with TFSF,
+  CPML, a Lorentz sphere, a lossy Debye cube, a Drude cube and a plain
+  object covering part of the sphere; the filled TM110 cavities of the
+  oracle pins.
+- FP32: the Lorentz cavity's probe within 1e-4 of the oracle's peak.
+- Checkpoints carry the Lorentz state (J, and the previous step's J and
+  E): a restored run continues bit for bit.
+- Gradients refuse Debye and Lorentz media.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+from test_dispersive_cpu import DEBYE, LORENTZ, _medium
+from test_drude_cpu import _cavity
+from xfdtd_amd import scene as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _sim(sc, steps, precision=64):
+    from xfdtd_amd.sim import Simulation
+    return Simulation(sc, steps, precision=precision)
+
+
+def mixed_scene(n=30, dl=4e-3, pml=5):
+    wp = 2 * math.pi * 20e9
+    objs = [S.Object("lorentz", S.Sphere((0.002, -0.004, 0.0), 7 * dl),
+                     S.LorentzMedium("l", 3.0, 1.5, wp, 2 * math.pi * 2e9)),
+            S.Object("debye", S.Cube((-0.036, 0.01, -0.02), (0.02, 0.012, 0.016)),
+                     S.DebyMedium("d", 5.0, 2.0, 1 / (2 * math.pi * 8e9), 0.03, 0.0)),
+            S.Object("drude", S.Cube((0.02, 0.012, -0.03), (0.012, 0.012, 0.012)),
+                     S.DrudeMedium("p", 1.0, 0.7 * wp, 2 * math.pi * 1e9)),
+            S.Object("cap", S.Cube((0.004, -0.03, -0.03), (0.03, 0.06, 0.06)), S.Material("c", 3.0, 1.2, 0.01, 0.0))]
+    return S.cube_domain_scene(n, dl, pml, cfl=0.99, objects=objs, tfsf_gap=pml + 2, nffft_gap=pml + 1,
+                               freqs=[S.C_0 / (20 * dl)], theta=[0.0, 1.0], phi=[0.0])
+
+
+def test_mixed_dispersive_scene_bitwise(oracle_threads):
+    steps = 80
+    sc = mixed_scene()
+    g, o = _sim(sc, steps), OracleSim(sc, steps)
+    g.step(steps)
+    o.step(steps)
+    for c in range(6):
+        assert np.array_equal(g.get_field(c), o.get_field(c)), c
+    assert np.array_equal(g.get_nffft(), o.get_nffft())
+    b = g.drude_box()
+    for c in range(3):
+        jg = g.get_drude_j(c)
+        assert np.array_equal(jg, o.get_drude_j(c)[b[0]:b[1], b[2]:b[3], b[4]:b[5]]), c
+        assert np.abs(jg).max() > 0
+
+
+def _mode(sim):
+    nx, ny, _ = sim.dims
+    i = np.arange(nx + 1)[:, None, None]
+    j = np.arange(ny + 1)[None, :, None]
+    ez = np.sin(math.pi * i / nx) * np.sin(math.pi * j / ny) * np.ones(sim.field_shape(S.EZ))
+    ez[0], ez[nx], ez[:, 0], ez[:, ny] = 0, 0, 0, 0
+    return ez
+
+
+@pytest.mark.parametrize("kind,p", [("debye", DEBYE[1]), ("lorentz", LORENTZ[0]), ("lorentz", LORENTZ[1])])
+def test_cavity_bitwise(kind, p):
+    steps = 300
+    sc = _cavity(_medium(kind, p))
+    sc.probes = [S.Probe(S.EZ, 4, 3, 1)]
+    g, o = _sim(sc, steps), OracleSim(sc, steps)
+    for s in (g, o):
+        s.set_field(S.EZ, _mode(s))
+        s.step(steps)
+    for c in range(6):
+        assert np.array_equal(g.get_field(c), o.get_field(c)), c
+    assert np.array_equal(g.get_probes(), o.get_probes())
+
+
+def test_lorentz_cavity_fp32_close():
+    steps = 300
+    sc = _cavity(_medium("lorentz", LORENTZ[0]))
+    sc.probes = [S.Probe(S.EZ, 4, 3, 1)]
+    g, o = _sim(sc, steps, 32), OracleSim(sc, steps)
+    for s in (g, o):
+        s.set_field(S.EZ, _mode(s))
+        s.step(steps)
+    a, b = g.get_probes()[0], o.get_probes()[0]
+    assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max()
+
+
+def test_checkpoint_carries_lorentz_state():
+    steps = 120
+    sc = _cavity(_medium("lorentz", LORENTZ[1]))
+    g = _sim(sc, steps)
+    g.set_field(S.EZ, _mode(g))
+    g.step(40)
+    g.checkpoint(0)
+    g.step(60)
+    first = [g.get_field(c) for c in range(6)]
+    g.restore(0)
+    g.step(60)
+    for c in range(6):
+        assert np.array_equal(g.get_field(c), first[c]), c
+
+
+def test_gradient_refuses_debye_and_lorentz():
+    from adjoint_scenes import design_scene
+    for med in (S.DebyMedium("d", 4.0, 2.0, 1e-11), S.LorentzMedium("l", 3.0, 1.5, 1e11, 1e10)):
+        sc, _ = design_scene()
+        dl = sc.cell_size
+        corner = -13 * dl + 7 * dl  # cell (7, 7, 7): outside the CPML and the design box
+        sc.objects = list(sc.objects) + [S.Object("m", S.Cube((corner,) * 3, (2 * dl,) * 3), med)]
+        with pytest.raises(RuntimeError, match="Drude media only"):
+            _sim(sc, 40).grad(checkpoint_every=10)

@@ -16,7 +16,7 @@ _LIB = None
 # every extern "C" symbol declared in include/xfdtd_engine.h and include/xfdtd_sim.h
 ENGINE_SYMBOLS = [
     "xf_engine_create", "xf_engine_destroy", "xf_last_error", "xf_engine_set_materials",
-    "xf_engine_set_cpml", "xf_engine_set_tfsf", "xf_engine_set_nffft", "xf_engine_nffft_size",
+    "xf_engine_set_dispersive", "xf_engine_set_cpml", "xf_engine_set_tfsf", "xf_engine_set_nffft", "xf_engine_nffft_size",
     "xf_engine_get_nffft", "xf_engine_farfield_sums", "xf_engine_add_current_source",
     "xf_engine_add_probe", "xf_engine_get_probes", "xf_engine_step", "xf_engine_sync",
     "xf_engine_current_step", "xf_engine_get_field", "xf_engine_set_field",
@@ -32,7 +32,7 @@ ENGINE_SYMBOLS = [
 ]
 SIM_SYMBOLS = [
     "xf_sim_last_error", "xf_sim_create", "xf_sim_destroy", "xf_sim_add_cube", "xf_sim_add_sphere",
-    "xf_sim_add_pml", "xf_sim_set_tfsf3d", "xf_sim_add_cylinder", "xf_sim_set_tfsf2d", "xf_sim_set_nffft_fd", "xf_sim_add_current_source",
+    "xf_sim_add_pml", "xf_sim_set_tfsf3d", "xf_sim_add_dispersive_object", "xf_sim_add_cylinder", "xf_sim_set_tfsf2d", "xf_sim_set_nffft_fd", "xf_sim_add_current_source",
     "xf_sim_add_probe", "xf_sim_plan", "xf_sim_plan_coeff", "xf_sim_init", "xf_sim_step", "xf_sim_sync", "xf_sim_current_step",
     "xf_sim_dims", "xf_sim_dt", "xf_sim_get_field", "xf_sim_set_field", "xf_sim_nffft_size",
     "xf_sim_get_nffft", "xf_sim_farfield", "xf_sim_output_nffft", "xf_sim_get_probes",
@@ -67,6 +67,7 @@ def lib():
     L.xf_sim_add_sphere.argtypes = [vp, dp, d, C.c_float, C.c_float, d, d]
     L.xf_sim_add_pml.argtypes = [vp, i, i, i, d, d, d, d]
     L.xf_sim_set_tfsf3d.argtypes = [vp, szp, d, d, d, d, d, d, d]
+    L.xf_sim_add_dispersive_object.argtypes = [vp, C.c_int, dp, C.c_int, dp, d, d]
     L.xf_sim_add_cylinder.argtypes = [vp, C.c_int, dp, d, d, C.c_float, C.c_float, d, d]
     L.xf_sim_set_tfsf2d.argtypes = [vp, C.c_size_t, C.c_size_t, d, d, C.c_int, dp]
     L.xf_sim_broadband_ez.argtypes = [vp, dp]

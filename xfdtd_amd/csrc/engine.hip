@@ -144,6 +144,8 @@ struct xf_engine {
   int xb[6] = {0, 0, 0, 0, 0, 0};
   void *drc = nullptr;
   void *dj[3] = {nullptr, nullptr, nullptr};
+  void *djp[3] = {nullptr, nullptr, nullptr}, *dep[3] = {nullptr, nullptr, nullptr};  // Lorentz state
+  bool ade_non_drude = false;  // a Debye or Lorentz record (no adjoint)
 };
 
 // wait for everything enqueued for this engine, on its own stream and on
@@ -831,7 +833,11 @@ static Grid<T> make_grid(xf_engine *e) {
   g.dcoef = (const T *)e->dcoef;
   for (int a = 0; a < 6; ++a) g.xb[a] = e->xb[a];
   g.drc = (const T *)e->drc;
-  for (int q = 0; q < 3; ++q) g.dj[q] = (T *)e->dj[q];
+  for (int q = 0; q < 3; ++q) {
+    g.dj[q] = (T *)e->dj[q];
+    g.djp[q] = (T *)e->djp[q];
+    g.dep[q] = (T *)e->dep[q];
+  }
   return g;
 }
 
@@ -883,6 +889,8 @@ static Tiling make_tiling(const xf_engine *e) {
 }
 
 static int tiling_threads(const Tiling &tl) { return tl.ty == 0 ? tl.tz : tl.tz * tl.ty; }
+
+static bool has_non_drude(const xf_engine *e) { return e->ade_non_drude; }
 
 static bool has_inductor(const xf_engine *e) {
   for (const SourceDev &sd : e->sources)
@@ -1388,6 +1396,17 @@ extern "C" xf_status xf_engine_set_design(xf_engine *e, const int64_t *box, cons
 
 extern "C" xf_status xf_engine_set_drude(xf_engine *e, const int64_t *box, const double *coef) {
   if (!e || !box || !coef) FAILS(XF_EINVAL, "null argument");
+  // the Drude records of the dispersive layout
+  std::vector<double> rec(XF_ADE_STRIDE * (size_t)std::max(e->nmat, 0), 0.0);
+  for (int m = 0; m < e->nmat; ++m) {
+    for (int c = 0; c < 6; ++c) rec[XF_ADE_STRIDE * m + c] = coef[6 * m + c];
+    rec[XF_ADE_STRIDE * m + XF_ADE_KIND] = coef[6 * m + 1] != 0 ? 1 : 0;
+  }
+  return xf_engine_set_dispersive(e, box, rec.data());
+}
+
+extern "C" xf_status xf_engine_set_dispersive(xf_engine *e, const int64_t *box, const double *coef) {
+  if (!e || !box || !coef) FAILS(XF_EINVAL, "null argument");
   if (!e->materials_set) FAILS(XF_ESTATE, "set the materials before the Drude box");
   if (e->has_drude) FAILS(XF_ESTATE, "Drude box already set");
   const int N[3] = {e->nx, e->ny, e->nz};
@@ -1398,15 +1417,33 @@ extern "C" xf_status xf_engine_set_drude(xf_engine *e, const int64_t *box, const
       FAILS(XF_EINVAL, "Drude box [%lld,%lld) on axis %d must lie inside the grid and outside the CPML",
             (long long)lo, (long long)hi, a);
   }
-  for (int m = 0; m < e->nmat; ++m)
-    if (coef[6 * m + 1] != 0 && !(coef[6 * m] == coef[6 * m] && coef[6 * m + 1] > 0))
-      FAILS(XF_EINVAL, "bad Drude coefficients of material %d", m);
+  bool lorentz = false;
+  for (int m = 0; m < e->nmat; ++m) {
+    const double *c = coef + XF_ADE_STRIDE * m;
+    const double kind = c[XF_ADE_KIND];
+    if (kind != 0 && kind != 1 && kind != 2 && kind != 3)
+      FAILS(XF_EINVAL, "bad dispersive kind %g of material %d", kind, m);
+    // Drude and Debye: a finite ca and a positive cb; Lorentz: a positive
+    // curl weight and a positive 2 dt; Debye: a positive dt
+    if ((kind == 1 || kind == 2) && !(c[0] == c[0] && c[1] > 0))
+      FAILS(XF_EINVAL, "bad dispersive coefficients of material %d", m);
+    if ((kind == 2 && !(c[9] > 0)) || (kind == 3 && !(c[0] == c[0] && c[8] > 0 && c[10] > 0)))
+      FAILS(XF_EINVAL, "bad dispersive coefficients of material %d", m);
+    lorentz = lorentz || kind == 3;
+    e->ade_non_drude = e->ade_non_drude || kind == 2 || kind == 3;
+  }
   const size_t n = (size_t)(box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4]);
   xf_status s;
-  if ((s = upload_real(e, &e->drc, coef, 6 * (size_t)e->nmat)) != XF_OK) return s;
+  if ((s = upload_real(e, &e->drc, coef, XF_ADE_STRIDE * (size_t)e->nmat)) != XF_OK) return s;
   for (int q = 0; q < 3; ++q) {
     if ((s = dev_alloc(e, &e->dj[q], n * e->esz)) != XF_OK) return s;
     e->state.push_back({e->dj[q], n * e->esz});
+    if (lorentz) {
+      if ((s = dev_alloc(e, &e->djp[q], n * e->esz)) != XF_OK) return s;
+      if ((s = dev_alloc(e, &e->dep[q], n * e->esz)) != XF_OK) return s;
+      e->state.push_back({e->djp[q], n * e->esz});
+      e->state.push_back({e->dep[q], n * e->esz});
+    }
   }
   for (int a = 0; a < 6; ++a) e->xb[a] = (int)box[a];
   e->has_drude = true;
@@ -1671,6 +1708,7 @@ extern "C" xf_status xf_engine_grad_eps(xf_engine *e, int checkpoint_every, doub
   if (!e->materials_set || !e->has_design || !e->has_dft)
     FAILS(XF_ESTATE, "the gradient needs materials, a design box and a DFT plane");
   if (has_inductor(e)) FAILS(XF_EINVAL, "the gradient does not transpose inductors");
+  if (has_non_drude(e)) FAILS(XF_EINVAL, "the gradient covers Drude media only, not Debye or Lorentz");
   if (e->cur != 0) FAILS(XF_ESTATE, "the gradient runs from step 0 (engine is at step %lld)", e->cur);
   if (e->total < 1) FAILS(XF_ESTATE, "no step horizon");
   if (!e->probe_comp.empty()) {
@@ -1748,6 +1786,7 @@ extern "C" xf_status xf_engine_adjoint_run(xf_engine *e, int64_t nsteps, double 
   if (!e || !fields || nsteps < 1) FAILS(XF_EINVAL, "bad argument");
   if (!e->materials_set) FAILS(XF_ESTATE, "materials not set");
   if (has_inductor(e)) FAILS(XF_EINVAL, "the adjoint does not transpose inductors");
+  if (has_non_drude(e)) FAILS(XF_EINVAL, "the adjoint covers Drude media only, not Debye or Lorentz");
   HIPCHK(engine_wait(e));
   return e->prec == 64 ? adjoint_run<double>(e, nsteps, fields) : adjoint_run<float>(e, nsteps, fields);
 }
@@ -1887,6 +1926,8 @@ extern "C" xf_status xf_batch_grad_eps(xf_batch *b, int checkpoint_every, double
     if (!e->has_design || !e->has_dft)
       FAILS(XF_ESTATE, "member %zu: the gradient needs a design box and a DFT plane", q);
     if (has_inductor(e)) FAILS(XF_EINVAL, "member %zu: the gradient does not transpose inductors", q);
+    if (has_non_drude(e))
+      FAILS(XF_EINVAL, "member %zu: the gradient covers Drude media only, not Debye or Lorentz", q);
     if (e->cur != 0) FAILS(XF_ESTATE, "member %zu is at step %lld; the gradient runs from step 0", q, e->cur);
     if (e->total < 1 || e->total != b->e[0]->total)
       FAILS(XF_ESTATE, "member %zu: every member needs the same step horizon", q);

@@ -15,6 +15,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "xfdtd_engine.h"
 
 namespace xfk {
 
@@ -47,12 +48,14 @@ struct Grid {
   int db[6];
   const T *dcoef;
   // Drude box [xb0,xb1) x [xb2,xb3) x [xb4,xb5) (empty when xb0 == xb1):
-  // per-material ADE coefficients drc[m][6] = ca, cb, 1 + k, k, beta, dl/2
+  // per-material ADE records drc[m][XF_ADE_STRIDE] (kind at XF_ADE_KIND:
+  // 0 none, 1 Drude, 2 Debye, 3 Lorentz; the layout of xf_engine_set_dispersive)
   // (cb = 0 for a material that is not a Drude medium) and the polarisation
   // currents of the box's cells, dj[component][cell]
   int xb[6];
   const T *drc;
   T *dj[3];
+  T *djp[3], *dep[3];  // Lorentz: the previous step's J and E (null without Lorentz media)
 };
 
 template <typename T>
@@ -83,6 +86,31 @@ __device__ __forceinline__ T drude_e(const T *c, T *jp, T e_old, T curl) {
   const T e_new = c[0] * e_old + c[1] * (curl - j_sum);
   *jp = c[3] * jv + c[4] * (e_new + e_old);
   return e_new;
+}
+
+// one dispersive E cell: DrudeMedium / DebyMedium / LorentzMedium::updateE
+// (dispersive_material.cpp:146-193, 224-275, 38-117) in the reference's
+// expression order; jq, eq are the Lorentz previous J and E
+template <typename T>
+__device__ __forceinline__ T ade_e(const T *c, T *jp, T *jq, T *eq, T e_old, T curl) {
+  const T kind = c[XF_ADE_KIND];
+  if (kind == T(3)) {
+    const T jv = *jp, jpv = *jq, epv = *eq;
+    const T j_sum = T(0) + c[5] * (c[2] * jv + c[7] * jpv);
+    const T e_new = c[0] * e_old + c[1] * epv + c[8] * (curl - j_sum);
+    *jp = c[3] * jv + c[7] * jpv + c[4] * (e_new - epv) / c[10];
+    *eq = e_old;
+    *jq = jv;
+    return e_new;
+  }
+  if (kind == T(2)) {
+    const T jv = *jp;
+    const T j_sum = T(0) + c[5] * (c[2] * jv);
+    const T e_new = c[0] * e_old + c[1] * (curl - j_sum);
+    *jp = c[3] * jv + c[4] * (e_new - e_old) / c[9];
+    return e_new;
+  }
+  return drude_e(c, jp, e_old, curl);
 }
 
 __device__ __forceinline__ int e_layer(int n, int N, int dn, int dp) {
@@ -468,12 +496,18 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
     // Drude cells (outside every CPML slab) replace the Yee update
     // (simulation_update.cpp:221-280)
     if (in_drude(g, i, j, k)) {
-      const T *dc = g.drc + 6 * m;
-      if (dc[1] != T(0)) {
+      const T *dc = g.drc + XF_ADE_STRIDE * m;
+      if (dc[XF_ADE_KIND] != T(0)) {
         const int dq = drude_cell(g, i, j, k);
-        if (i < nx && q.jin && q.kin) vx = drude_e(dc, g.dj[0] + dq, ex, -dhy_k + dhz_j);
-        if (ix && j < ny && q.kin) vy = drude_e(dc, g.dj[1] + dq, ey, -dhz_i + dhx_k);
-        if (ix && q.jin && k < nz) vz = drude_e(dc, g.dj[2] + dq, ez, -dhx_j + dhy_i);
+        // the Lorentz state pointers are only read for a Lorentz record
+        T *jq[3], *eq[3];
+        for (int c = 0; c < 3; ++c) {
+          jq[c] = g.djp[c] ? g.djp[c] + dq : nullptr;
+          eq[c] = g.dep[c] ? g.dep[c] + dq : nullptr;
+        }
+        if (i < nx && q.jin && q.kin) vx = ade_e(dc, g.dj[0] + dq, jq[0], eq[0], ex, -dhy_k + dhz_j);
+        if (ix && j < ny && q.kin) vy = ade_e(dc, g.dj[1] + dq, jq[1], eq[1], ey, -dhz_i + dhx_k);
+        if (ix && q.jin && k < nz) vz = ade_e(dc, g.dj[2] + dq, jq[2], eq[2], ez, -dhx_j + dhy_i);
       }
     }
   }
@@ -1608,7 +1642,7 @@ __device__ __forceinline__ void adj_e_body(const Grid<T> &g, const Adj<T> &a, lo
   // weights are -cb t on the first difference of the reference's bracket and
   // +cb t on the second
   if (in_drude(g, i, j, k)) {
-    const T *dc = g.drc + 6 * m;
+    const T *dc = g.drc + XF_ADE_STRIDE * m;
     const T cb = dc[1];
     if (cb != T(0)) {
       const int dq = drude_cell(g, i, j, k);

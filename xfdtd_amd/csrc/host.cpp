@@ -1016,8 +1016,10 @@ void Simulation::plan(size_t total_time_steps) {
         }
     D.present = true;
   }
-  // ---- Drude coefficients: DrudeMedium::init (dispersive_material.cpp:
-  // 127-144) with dl the grid's base size (dispersive_material.h:47-48)
+  // ---- dispersive coefficients, XF_ADE_STRIDE per slot: DrudeMedium::init
+  // (dispersive_material.cpp:127-144), DebyMedium::init (205-222),
+  // LorentzMedium::init (17-36), with dl the grid's base size
+  // (dispersive_material.h:47-48)
   if (any_drude && dbox[1] > dbox[0]) {
     PlanData::Drude &D = _plan->drude;
     for (int a = 0; a < 3; ++a)
@@ -1025,23 +1027,53 @@ void Simulation::plan(size_t total_time_steps) {
         if (in_slab_e(a, (int)n) || in_slab_h(a, (int)n))
           throw std::runtime_error("Drude cells inside a CPML slab are not supported");
     for (int q = 0; q < 6; ++q) D.box[q] = dbox[q];
-    D.coef.assign(6 * (size_t)nmat, 0.0);
+    D.coef.assign(XF_ADE_STRIDE * (size_t)nmat, 0.0);
     const double dl = _cell_size, eps0 = constant::EPSILON_0;
     for (int slot = 0; slot < nbase; ++slot) {
       if (!drude_of[slot]) continue;
       const Material &m = _objects[drude_of[slot] - 1]->getMaterial();
       const double nv = m.drudeNv(), wp = m.drudeOmegaP(), einf = m.drudeEpsInf();
       const double sigma_e = m.getElectricalConductivity();
-      const double kk = (2 - nv * dt) / (2 + nv * dt);
-      const double beta = (wp * wp * eps0 * dt) / (2 + nv * dt);
-      const double temp = 2 * eps0 * einf + beta * dt + sigma_e * dt;
-      double *c = &D.coef[6 * (size_t)slot];
-      c[0] = (2 * eps0 * einf - beta * dt - sigma_e * dt) / temp;
-      c[1] = 2 * dt / (temp * dl);
-      c[2] = 1 + kk;
-      c[3] = kk;
-      c[4] = beta;
+      double *c = &D.coef[XF_ADE_STRIDE * (size_t)slot];
+      const int kind = m.dispersionKind();
+      c[XF_ADE_KIND] = kind;
       c[5] = 0.5 * dl;
+      if (kind == 2) {
+        const double tau = m.dispersionTau(), de = m.dispersionEpsS() - einf;
+        const double kk = (2 * tau - dt) / (2 * tau + dt);
+        const double beta = (2 * de * eps0 * dt) / (2 * tau + dt);
+        const double temp = 2 * eps0 * einf + beta + sigma_e * dt;
+        c[0] = (2 * eps0 * einf + beta - sigma_e * dt) / temp;
+        c[1] = (2 * dt) / (temp * dl);
+        c[2] = 1 + kk;
+        c[3] = kk;
+        c[4] = beta;
+        c[9] = dt;
+      } else if (kind == 3) {
+        const double de = m.dispersionEpsS() - einf;
+        const double t0 = nv * dt + 1;
+        const double alpha = (2 - wp * wp * dt * dt) / t0;
+        const double xi = (nv * dt - 1) / t0;
+        const double gamma = (eps0 * de * wp * wp * dt * dt) / t0;
+        const double t1 = 4 * eps0 * einf + gamma + 2 * sigma_e * dt;
+        c[0] = (4 * eps0 * einf - 2 * sigma_e * dt) / t1;
+        c[1] = gamma / t1;
+        c[2] = 1 + alpha;
+        c[3] = alpha;
+        c[4] = gamma;
+        c[7] = xi;
+        c[8] = (4 * dt) / (t1 * dl);
+        c[10] = 2 * dt;
+      } else {
+        const double kk = (2 - nv * dt) / (2 + nv * dt);
+        const double beta = (wp * wp * eps0 * dt) / (2 + nv * dt);
+        const double temp = 2 * eps0 * einf + beta * dt + sigma_e * dt;
+        c[0] = (2 * eps0 * einf - beta * dt - sigma_e * dt) / temp;
+        c[1] = 2 * dt / (temp * dl);
+        c[2] = 1 + kk;
+        c[3] = kk;
+        c[4] = beta;
+      }
     }
     D.present = true;
   }
@@ -1270,7 +1302,7 @@ void Simulation::upload() {
     check(xf_engine_add_probe(_engine, compIndex(p.component), (int64_t)p.i, (int64_t)p.j, (int64_t)p.k));
   if (P.design.present)
     check(xf_engine_set_design(_engine, P.design.box, P.design.coef.data(), P.design.eps_r.data()));
-  if (P.drude.present) check(xf_engine_set_drude(_engine, P.drude.box, P.drude.coef.data()));
+  if (P.drude.present) check(xf_engine_set_dispersive(_engine, P.drude.box, P.drude.coef.data()));
   if (P.dft.present)
     check(xf_engine_set_dft_plane(_engine, P.dft.comp, P.dft.axis, P.dft.index, P.dft.lo1, P.dft.hi1, P.dft.lo2,
                                   P.dft.hi2, reinterpret_cast<const double *>(P.dft.w.data()),

@@ -97,6 +97,31 @@ extern "C" int xf_sim_add_drude_sphere(xf_sim *s, const double center[3], double
   });
 }
 
+extern "C" int xf_sim_add_dispersive_object(xf_sim *s, int shape, const double p[6], int kind, const double mp[4],
+                                            double sigma_e, double sigma_m) {
+  return guarded([&] {
+    std::unique_ptr<Shape> sh;
+    if (shape == 0)
+      sh = std::make_unique<Cube>(PointVector{p[0], p[1], p[2]}, PointVector{p[3], p[4], p[5]});
+    else if (shape == 1)
+      sh = std::make_unique<Sphere>(PointVector{p[0], p[1], p[2]}, p[3]);
+    else if (shape == 2 && p[5] >= 0 && p[5] <= 2)
+      sh = std::make_unique<Cylinder>(static_cast<Axis>((int)p[5]), PointVector{p[0], p[1], p[2]}, p[3], p[4]);
+    else
+      throw std::invalid_argument("bad shape");
+    std::unique_ptr<Material> m;
+    if (kind == 1)
+      m = std::make_unique<DrudeMedium>("drude", mp[0], mp[1], mp[2], sigma_e, sigma_m);
+    else if (kind == 2)
+      m = std::make_unique<DebyMedium>("debye", mp[0], mp[1], mp[2], sigma_e, sigma_m);
+    else if (kind == 3)
+      m = std::make_unique<LorentzMedium>("lorentz", mp[0], mp[1], mp[2], mp[3], sigma_e, sigma_m);
+    else
+      throw std::invalid_argument("bad dispersive kind");
+    s->sim.addObject(std::make_shared<Object>("dispersive", std::move(sh), std::move(m)));
+  });
+}
+
 extern "C" int xf_sim_drude_box(xf_sim *s, int64_t box[6]) {
   return guarded([&] {
     xf_engine *e = s->sim.engine();

@@ -44,7 +44,19 @@ class Simulation:
             for o in scene.objects:
                 m = o.material
                 sh = o.shape
-                if hasattr(m, "omega_p"):
+                dk = type(m).__name__
+                if dk in ("DebyMedium", "LorentzMedium"):
+                    if hasattr(sh, "height"):
+                        shp, pp = 2, [*sh.center, sh.radius, sh.height, sh.axis]
+                    elif hasattr(sh, "radius"):
+                        shp, pp = 1, [*sh.center, sh.radius, 0.0, 0.0]
+                    else:
+                        shp, pp = 0, [*sh.origin, *sh.size]
+                    mp = ([m.eps_s, m.eps_inf, m.tau, 0.0] if dk == "DebyMedium"
+                          else [m.eps_s, m.eps_inf, m.omega_p, m.nu])
+                    self._chk(L.xf_sim_add_dispersive_object(self.h, shp, _dp(_f64(pp)), 2 if dk == "DebyMedium" else 3,
+                                                             _dp(_f64(mp)), float(m.sigma_e), float(m.sigma_m)))
+                elif hasattr(m, "omega_p"):
                     drude = (float(m.eps_inf), float(m.omega_p), float(m.nu), float(m.sigma_e), float(m.sigma_m))
                     if hasattr(sh, "radius"):
                         self._chk(L.xf_sim_add_drude_sphere(self.h, _dp(_f64(sh.center)), float(sh.radius), *drude))
