@@ -27,7 +27,6 @@ sys.path.insert(0, os.path.dirname(HERE))
 def scenes():
     """name -> (scene builder, steps); the builders are deterministic."""
     from xfdtd_amd import scene as S
-    from test_dispersive_gpu import mixed_scene
     from test_host_cpu import lumped_scene
 
     def sphere_tfsf():
@@ -42,7 +41,19 @@ def scenes():
         return S.cylinder_2d_scene(40, 36, 6, dl, 6.3 * dl, tfsf_gap=10, nffft_gap=8, phi_inc=0.3)
 
     def dispersive():
-        return mixed_scene(n=20, dl=4e-3, pml=4)
+        # a Lorentz sphere, a lossy Debye cube, a Drude cube and a later plain
+        # cube over part of the sphere, all clear of the 4-cell CPML
+        dl, wp = 4e-3, 2 * math.pi * 20e9
+        objs = [S.Object("lorentz", S.Sphere((1e-3, -2e-3, 0.0), 3 * dl),
+                         S.LorentzMedium("l", 3.0, 1.5, wp, 2 * math.pi * 2e9)),
+                S.Object("debye", S.Cube((-0.020, 0.004, -0.008), (0.008, 0.010, 0.008)),
+                         S.DebyMedium("d", 5.0, 2.0, 1 / (2 * math.pi * 8e9), 0.03, 0.0)),
+                S.Object("drude", S.Cube((0.008, 0.006, -0.016), (0.008, 0.008, 0.008)),
+                         S.DrudeMedium("p", 1.0, 0.7 * wp, 2 * math.pi * 1e9)),
+                S.Object("cap", S.Cube((0.002, -0.020, -0.020), (0.014, 0.040, 0.040)),
+                         S.Material("c", 3.0, 1.2, 0.01, 0.0))]
+        return S.cube_domain_scene(20, dl, 4, cfl=0.99, objects=objs, tfsf_gap=6, nffft_gap=5,
+                                   freqs=[S.C_0 / (20 * dl)], theta=[0.0, 1.0], phi=[0.0])
 
     return {"sphere_tfsf": (sphere_tfsf, 40), "cylinder_2d": (cylinder_2d, 80),
             "lumped": (lumped_scene, 60), "dispersive": (dispersive, 40)}
