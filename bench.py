@@ -42,6 +42,9 @@ broadband: the reference's monostatic example (96^3 PEC sphere, broadband
 NF-FF, FP64, 1200 steps): stepping with and without the broadband surface,
 and the far field of all 2T-1 bins (hipFFT + fold).
 
+dispersive: the 400^3 run with its sphere made of a plain dielectric, then
+Drude, Debye and Lorentz media (the E kernel's general path), 100 steps.
+
 tmz: the reference's 2-D example (370 x 320 TMz cells, PEC cylinder, TFSF2D,
 broadband NF-FF, 1400 steps) in FP64 and FP32, its far field, and the
 oracle's step time on the same scene.
@@ -375,6 +378,39 @@ def run_broadband(steps=1200):
     return out
 
 
+def run_dispersive(n=400, steps=100, warmup=5):
+    """The headline 400^3 run (FP32) with its radius-100 sphere made of each
+    medium in turn: a plain eps_r 4 dielectric, then Drude, Debye and
+    Lorentz media of static permittivity about 4.  The dispersive cells take
+    the E kernel's general path; the figures are whole-step Gcells/s and the
+    E and H kernels' HIP-event times."""
+    from xfdtd_amd import scene as S
+    from xfdtd_amd.sim import Simulation
+    media = {"plain": S.Material("sphere", 4.0, 1.0, 0.0, 0.0),
+             "drude": S.DrudeMedium("p", 1.0, 2 * math.pi * 20e9, 2 * math.pi * 1e9),
+             "debye": S.DebyMedium("d", 4.0, 2.0, 1 / (2 * math.pi * 5e9)),
+             "lorentz": S.LorentzMedium("l", 4.0, 2.0, 2 * math.pi * 15e9, 2 * math.pi * 1e9)}
+    out = {"workload": f"{n}^3 headline scene (FP32), sphere of radius 100 cells made of each medium",
+           "steps": steps}
+    for name, mat in media.items():
+        sc, _ = scene_400(n=n)
+        sc.objects[-1] = S.Object("sphere", sc.objects[-1].shape, mat)
+        sim = Simulation(sc, warmup + steps, precision=32)
+        sim.step(warmup)
+        sim.sync()
+        sim.set_timing(True)
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.sync()
+        dt = time.perf_counter() - t0
+        e_ms, h_ms, _ = sim.kernel_times()
+        nx, ny, nz = sim.dims
+        out[name] = {"gcells": round(nx * ny * nz * steps / dt / 1e9, 2), "e_ms": round(e_ms, 4),
+                     "h_ms": round(h_ms, 4)}
+        sim.close()
+    return out
+
+
 def run_tmz(steps=1400, cpu_steps=200, cpu_threads=16):
     """The reference's 2-D example (examples/2d_cylinder_scatter.cpp): a
     370 x 320 TMz grid (350 x 300 plus a 10-cell CPML on x and y) with a PEC
@@ -437,6 +473,8 @@ def main():
     ap.add_argument("--batch-adjoint-steps", type=int, default=1200)
     ap.add_argument("--broadband", type=int, default=1200,
                     help="steps of the monostatic broadband NF-FF leg (0: skip it)")
+    ap.add_argument("--dispersive", type=int, default=100,
+                    help="steps of the dispersive-media leg at --size (0: skip it)")
     ap.add_argument("--tmz", type=int, default=1400,
                     help="steps of the 2-D (TMz) cylinder leg (0: skip it)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -528,6 +566,8 @@ def main():
         line["batched_adjoint"] = run_batched_adjoint(args.batch_adjoint, args.batch_adjoint_steps)
     if args.broadband > 0:
         line["broadband"] = run_broadband(args.broadband)
+    if args.dispersive > 0:
+        line["dispersive"] = run_dispersive(args.size, args.dispersive)
     if args.tmz > 0:
         line["tmz"] = run_tmz(args.tmz, 0 if args.no_cpu else 200, max(1, args.cpu_threads))
     if args.batch > 0:
