@@ -140,3 +140,19 @@ def test_cpp_2d_cylinder_example_against_series(tmp_path):
     assert band.sum() > 40
     assert np.abs(err).max() < 1.0, err
     assert np.abs(err).mean() < 0.5, err
+
+
+@pytest.mark.parametrize("tiling", [(64, 4, 8, 0), (256, 0, 1, 2), (128, 2, 3, 8), (64, 4, 2, 16), (64, 0, 0, 4)])
+def test_2d_launch_tilings_bitwise(tiling, oracle_threads):
+    """On the unpadded nz = 1 layout every launch geometry of the curl
+    kernels (2-D tiles, flat runs, global tables, whole-axis marches, tile
+    orders) gives the oracle's bits."""
+    steps = 100
+    sc = _scene(0.3)
+    g, o = _sim(sc, steps), OracleSim(sc, steps)
+    g.set_tiling(*tiling)
+    g.step(steps)
+    o.step(steps)
+    for c in range(6):
+        assert np.array_equal(g.get_field(c), o.get_field(c)), (tiling, c)
+    assert np.array_equal(g.get_bb_samples(), o.get_bb_samples())
