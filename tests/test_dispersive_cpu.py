@@ -194,3 +194,41 @@ def test_reduces_to_a_dielectric(kind):
         scale = np.abs(runs[1][c]).max()
         assert scale > 0
         assert np.abs(runs[0][c] - runs[1][c]).max() <= 1e-12 * scale, c
+
+
+def mie_sphere_backscatter(x, m):
+    """Backscatter cross section over pi a^2 of a sphere of size parameter
+    x = k a and complex index m (exp(-i w t), Im m >= 0): the Mie series of
+    Bohren and Huffman, Q_back = |sum (2n+1) (-1)^n (a_n - b_n)|^2 / x^2."""
+    from scipy.special import spherical_jn, spherical_yn
+    nmax = int(x + 4 * x ** (1 / 3) + 2) + 10
+    n = np.arange(1, nmax + 1)
+    mx = m * x
+    jx, djx = spherical_jn(n, x), spherical_jn(n, x, derivative=True)
+    yx, dyx = spherical_yn(n, x), spherical_yn(n, x, derivative=True)
+    jm, djm = spherical_jn(n, mx), spherical_jn(n, mx, derivative=True)
+    psi_x, dpsi_x = x * jx, jx + x * djx
+    xi_x, dxi_x = x * (jx + 1j * yx), (jx + 1j * yx) + x * (djx + 1j * dyx)
+    psi_m, dpsi_m = mx * jm, jm + mx * djm
+    a = (m * psi_m * dpsi_x - psi_x * dpsi_m) / (m * psi_m * dxi_x - xi_x * dpsi_m)
+    b = (psi_m * dpsi_x - m * psi_x * dpsi_m) / (psi_m * dxi_x - m * xi_x * dpsi_m)
+    return abs(np.sum((2 * n + 1) * (-1.0) ** n * (a - b))) ** 2 / x ** 2
+
+
+def lorentz_eps(w, eps_s, eps_inf, wp, nu):
+    """The Lorentz permittivity in the exp(-i w t) convention (Im eps > 0)."""
+    return eps_inf + (eps_s - eps_inf) * wp ** 2 / (wp ** 2 - w ** 2 - 2j * w * nu)
+
+
+def test_mie_sphere_helper_limits():
+    """The dielectric Mie helper against two closed limits: the Rayleigh
+    backscatter 4 x^4 |(m^2-1)/(m^2+2)|^2 of a small sphere (Q_back), and a
+    good conductor (|m| = 100) against the PEC series within 3 %."""
+    from test_oracle_cpu import mie_pec_backscatter
+    for m in (1.5, 2.0 + 0.3j, np.sqrt(lorentz_eps(3e15, 2.25, 1.0, 4e16, 0.28e16))):
+        x = 0.01
+        ray = 4 * x ** 4 * abs((m * m - 1) / (m * m + 2)) ** 2
+        assert abs(mie_sphere_backscatter(x, m) / ray - 1) < 1e-3, m
+    for x in (0.7, 2.5):
+        m = np.sqrt(1 + 1e4j)  # skin depth under 1.5 % of the radius
+        assert abs(mie_sphere_backscatter(x, m) / mie_pec_backscatter(x) - 1) < 0.03, x
