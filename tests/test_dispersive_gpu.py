@@ -168,3 +168,46 @@ def test_cpp_lorentz_sphere_monostatic_against_mie(tmp_path):
     assert abs(bias) < 1.5, err
     assert strong.sum() >= 10 and np.abs(err[strong]).mean() < 2.5, err
     assert abs(peak_f - peak_m) < 0.03 * peak_m
+
+
+def test_cpp_drude_sphere_monostatic_against_mie(tmp_path):
+    """The reference's Drude-sphere monostatic test
+    (test/test_3d_fdtd.cpp:213-270) through the same C++ program: a 3.75 mm
+    sphere of DrudeMedium(1, 1.8e11, 2e10) at dl = radius / 30, 1000 steps,
+    FP64, against the Mie series of eps(w) = 1 - wp^2 / (w^2 + i w nu) over
+    5 .. 60 GHz: the band's total within 1.5 dB and the bins within 10 dB of
+    the series' peak within 2.5 dB on average (measured -0.4 dB and 0.7 dB)."""
+    import os
+    import subprocess
+    from test_broadband_gpu import _gaussian_spectrum
+    from test_dispersive_cpu import mie_sphere_backscatter
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "xfdtd_amd", "examples", "lorentz_sphere_monostatic_rcs")
+    out = tmp_path / "drude"
+    steps = 1000
+    print(f"running {exe} {out} {steps} 64 drude", flush=True)
+    r = subprocess.run([exe, str(out), str(steps), "64", "drude"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    print(r.stdout.strip(), flush=True)
+    a = 3.75e-3
+    dl = a / 30
+    pt, pp = np.load(out / "power_theta.npy"), np.load(out / "power_phi.npy")
+    Ns = 2 * steps - 1
+    dt = S.courant_dt(dl, 0.99)
+    f = np.linspace(-1 / dt / 2, 1 / dt / 2, Ns)
+    m = np.arange(Ns) - (steps - 1)
+    band = (f > 5e9) & (f < 6e10)
+    tau = 20 * dl / 6e8
+    einc = _gaussian_spectrum(steps, dt, tau, 4.5 * tau, m[band])
+    sigma = 4 * math.pi * (pt[band] + pp[band]) / (np.abs(einc) ** 2 / (2 * S.ETA_0))
+    w = 2 * math.pi * f[band]
+    eps = 1 - 1.8e11 ** 2 / (w ** 2 + 1j * w * 2e10)
+    mie = np.array([mie_sphere_backscatter(ww / S.C_0 * a, np.sqrt(e)) for ww, e in zip(w, eps)]) * math.pi * a * a
+    err = 10 * np.log10(sigma / mie)
+    bias = 10 * math.log10(sigma.sum() / mie.sum())
+    strong = mie >= 0.1 * mie.max()
+    print(f"Drude sphere vs Mie over {band.sum()} bins: band total {bias:+.2f} dB, mean {np.abs(err).mean():.2f} dB, "
+          f"worst {np.abs(err).max():.2f} dB; {strong.sum()} bins within 10 dB of the peak, mean "
+          f"{np.abs(err[strong]).mean():.2f} dB", flush=True)
+    assert abs(bias) < 1.5, err
+    assert strong.sum() >= 5 and np.abs(err[strong]).mean() < 2.5, err

@@ -6,9 +6,12 @@
 // dl = radius / 30 in a 2.5-radius air cube, 12-cell CPML on every face, TFSF
 // at index 22 with a Gaussian of tau = lambda_min / 6e8 (lambda_min = 20 dl)
 // travelling along +z, broadband NF-FF box at index 15 observed back along
-// -z, the default cfl; then the incident wave's FFT.
+// -z, the default cfl; then the incident wave's FFT.  With the medium
+// argument `drude` it runs testDrudeSphereMonostaticRCS (test_3d_fdtd.cpp:
+// 213-270) instead: radius 3.75 mm and DrudeMedium(1, 1.8e11, 2e10), the
+// rest alike.
 //
-// usage: lorentz_sphere_monostatic_rcs [output_dir] [total_time_steps] [precision]
+// usage: lorentz_sphere_monostatic_rcs [output_dir] [total_time_steps] [precision] [lorentz|drude]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -36,19 +39,25 @@ int main(int argc, char **argv) {
   const std::string out_dir = argc > 1 ? argv[1] : "./lorentz_sphere";
   const size_t total_time_steps = argc > 2 ? std::strtoul(argv[2], nullptr, 10) : 1000;
   const int precision = argc > 3 ? std::atoi(argv[3]) : 64;
+  const bool drude = argc > 4 && std::string(argv[4]) == "drude";
 
-  constexpr double radius = 15e-9;
-  constexpr double dl = radius / 30;
-  constexpr double domain_size = radius * 2.5;
-  constexpr double domain_origin = -domain_size / 2;
+  const double radius = drude ? 3.75e-3 : 15e-9;
+  const double dl = radius / 30;
+  const double domain_size = radius * 2.5;
+  const double domain_origin = -domain_size / 2;
   std::vector<std::shared_ptr<Object>> objects;
   objects.emplace_back(std::make_shared<Object>(
       "domain",
       std::make_unique<Cube>(PointVector{domain_origin, domain_origin, domain_origin},
                              PointVector{domain_size, domain_size, domain_size}),
       Material{"air", 1, 1, 0, 0}));
+  std::unique_ptr<Material> medium;
+  if (drude)
+    medium = std::make_unique<xfdtd::DrudeMedium>("random", 1, 1.8e11, 2e10);
+  else
+    medium = std::make_unique<LorentzMedium>("random", 2.25, 1, 4e16, 0.28e16);
   objects.emplace_back(std::make_shared<Object>("scatter", std::make_unique<Sphere>(PointVector{0, 0, 0}, radius),
-                                                std::make_unique<LorentzMedium>("random", 2.25, 1, 4e16, 0.28e16)));
+                                                std::move(medium)));
 
   std::vector<std::shared_ptr<Boundary>> boundaries;
   for (auto o : {xfdtd::Orientation::XN, xfdtd::Orientation::XP, xfdtd::Orientation::YN,
@@ -56,9 +65,9 @@ int main(int argc, char **argv) {
     boundaries.emplace_back(std::make_shared<PML>(o, 12));
 
   constexpr int n = 20;
-  constexpr double lambda_min = dl * n;
-  constexpr double tau = lambda_min / 6e8;
-  constexpr double t_0 = 4.5 * tau;
+  const double lambda_min = dl * n;
+  const double tau = lambda_min / 6e8;
+  const double t_0 = 4.5 * tau;
   constexpr double incident_theta = 0, incident_phi = 0, polarization = 0;
   constexpr size_t tfsf_index = 22;
   auto tfsf = std::make_unique<TFSF3D>(tfsf_index, tfsf_index, tfsf_index, 1.0, incident_theta, incident_phi,
