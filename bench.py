@@ -441,14 +441,15 @@ def run_tmz(steps=1400, cpu_steps=200, cpu_threads=16):
             out["far_field_s"] = round(time.perf_counter() - t0, 4)
         sim.close()
     if cpu_steps > 0:
+        # the leg's CPU baseline: the FP64 oracle on the same scene
         from oracle.oracle import OracleSim, set_threads
         threads = set_threads(cpu_threads)
         o = OracleSim(sc, cpu_steps)
         t0 = time.perf_counter()
         o.step(cpu_steps)
         cdt = time.perf_counter() - t0
-        out["cpu_oracle"] = {"us_per_step": round(1e6 * cdt / cpu_steps, 1), "threads": threads,
-                             "steps": cpu_steps, "kind": "port"}
+        out["cpu_baseline"] = {"value": round(1e6 * cdt / cpu_steps, 1), "unit": "us per step", "cores": threads,
+                               "kind": "port", "sample": f"FP64 oracle, first {cpu_steps} steps of the same scene"}
     return out
 
 
@@ -485,7 +486,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md, not built.", file=sys.stderr)
+              "multi-GPU sharding is not built.", file=sys.stderr)
         sys.exit(2)
 
     from xfdtd_amd.sim import Simulation
