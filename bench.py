@@ -66,17 +66,9 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Yee Gcells/s & % HBM peak; batched vs plain; adjoint grad time at stated tol"
 
 
-def scene_400(n=400, dl=2.5e-3, pml=12, radius_cells=100, steps_total=2000):
-    import numpy as np
-
-    from xfdtd_amd import scene as S
-    objs = [S.Object("sphere", S.Sphere((0.0, 0.0, 0.0), radius_cells * dl),
-                     S.Material("sphere", 4.0, 1.0, 0.0, 0.0))]
-    sc = S.cube_domain_scene(
-        n, dl, pml, cfl=0.9, objects=objs, tfsf_gap=pml + 4, nffft_gap=pml + 2,
-        freqs=[S.C_0 / (40 * dl), S.C_0 / (30 * dl), S.C_0 / (20 * dl)],
-        theta=np.linspace(0.0, math.pi, 181), phi=[0.0, math.pi / 2])
-    return sc, steps_total
+def scene_400(n=400, steps_total=2000):
+    from xfdtd_amd.workloads import dielectric_sphere_scene
+    return dielectric_sphere_scene(n), steps_total
 
 
 def cpu_model():
@@ -380,8 +372,10 @@ def run_broadband(steps=1200):
 
 def run_dispersive(n=400, steps=100, warmup=5):
     """The headline 400^3 run (FP32) with its radius-100 sphere made of each
-    medium in turn: a plain eps_r 4 dielectric, then Drude, Debye and
-    Lorentz media of static permittivity about 4.  The dispersive cells take
+    medium in turn: a plain eps_r 4 dielectric, a Drude plasma (eps_inf 1,
+    omega_p 2 pi 20 GHz, nu 2 pi 1 GHz), a Debye medium (eps_s 4, eps_inf 2,
+    tau 1 / (2 pi 5 GHz)) and a Lorentz medium (eps_s 4, eps_inf 2, omega_p
+    2 pi 15 GHz, nu 2 pi 1 GHz).  The dispersive cells take
     the E kernel's general path; the figures are whole-step Gcells/s and the
     E and H kernels' HIP-event times."""
     from xfdtd_amd import scene as S
@@ -448,8 +442,94 @@ def run_tmz(steps=1400, cpu_steps=200, cpu_threads=16):
         t0 = time.perf_counter()
         o.step(cpu_steps)
         cdt = time.perf_counter() - t0
+        o.close()
         out["cpu_baseline"] = {"value": round(1e6 * cdt / cpu_steps, 1), "unit": "us per step", "cores": threads,
                                "kind": "port", "sample": f"FP64 oracle, first {cpu_steps} steps of the same scene"}
+    return out
+
+
+def timed_run(sc, total, precision, steps, warmup, tiling=None):
+    """Warm-up steps, then `steps` timed steps of one Simulation with the
+    curl kernels' HIP events recorded on the engine's stream."""
+    from xfdtd_amd.sim import Simulation
+    sim = Simulation(sc, total, precision=precision)
+    if tiling:
+        sim.set_tiling(*tiling)
+    nx, ny, nz = sim.dims
+    sim.step(warmup)
+    sim.sync()
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    ms_e, ms_h, nsamp = sim.kernel_times()
+    sim.set_timing(False)
+    sim.close()
+    return {"grid": [nx, ny, nz], "wall_s": wall, "gcells": nx * ny * nz * steps / wall / 1e9,
+            "ms_per_step": wall / steps * 1e3, "ms_e": ms_e, "ms_h": ms_h, "event_samples": nsamp}
+
+
+def fp64_leg(sc, total, steps, warmup, tiling=None):
+    """The same workload in FP64 (the reference's precision)."""
+    r = timed_run(sc, total, 64, steps, warmup, tiling)
+    nx, ny, nz = r["grid"]
+    cells = nx * ny * nz
+    ms_dom = max(r["ms_e"], r["ms_h"])
+    achieved = (9 * 8 + 1) * cells / (ms_dom * 1e-3) / 1e9
+    return {"value": round(r["gcells"], 3), "unit": "Gcells/s", "steps": steps, "warmup": warmup,
+            "ms_per_step": round(r["ms_per_step"], 4),
+            "avg_ms": {"k_update_e": round(r["ms_e"], 4), "k_update_h": round(r["ms_h"], 4)},
+            "roofline": {"kernel": "k_update_h" if r["ms_h"] >= r["ms_e"] else "k_update_e",
+                         "bytes_per_launch": 73 * cells, "achieved": round(achieved, 1),
+                         "frac": round(achieved / HBM_PEAK_GBS, 4)}}
+
+
+def run_fp32_vs_fp64(n=400, steps=2000):
+    """The whole run in FP64 and in FP32: the FP32 error at the probe points
+    (max over time of |FP32 - FP64| over the series' peak, Ez and Ex at 8
+    points), on the NF-FF far-field power per frequency (relative to that
+    frequency's peak over all directions, and at the monostatic direction)
+    and on the final fields (over each component's final peak)."""
+    import numpy as np
+
+    from xfdtd_amd.sim import Simulation
+    from xfdtd_amd.workloads import SPHERE_400_PROBES, dielectric_sphere_scene
+    sc = dielectric_sphere_scene(n)
+    runs = {}
+    for prec in (64, 32):
+        s = Simulation(sc, steps, precision=prec)
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        runs[prec] = (s, time.perf_counter() - t0)
+    s64, s32 = runs[64][0], runs[32][0]
+    out = {"steps": steps, "fp64_run_s": round(runs[64][1], 3), "fp32_run_s": round(runs[32][1], 3)}
+    if sc.probes:
+        # per point: max over time of |FP32 - FP64| on Ez and Ex over the
+        # point's field peak (the larger of its two series' peaks)
+        p64, p32 = s64.get_probes(), s32.get_probes()
+        perr = {}
+        for q in range(0, len(p64), 2):
+            err = max(np.abs(p32[q] - p64[q]).max(), np.abs(p32[q + 1] - p64[q + 1]).max())
+            peak = max(np.abs(p64[q]).max(), np.abs(p64[q + 1]).max())
+            perr[str(tuple(SPHERE_400_PROBES[q // 2]))] = float(err / peak)
+        out["probes"] = perr
+        out["probe_max_rel"] = max(perr.values())
+    f64, f32 = s64.farfield(), s32.farfield()
+    pw64 = f64["power_theta"] + f64["power_phi"]
+    pw32 = f32["power_theta"] + f32["power_phi"]
+    out["far_power_max_rel"] = [float(np.abs(pw32[q] - pw64[q]).max() / np.abs(pw64[q]).max())
+                                for q in range(pw64.shape[0])]
+    out["monostatic_power_rel"] = [float(abs(pw32[q, -1, 0] - pw64[q, -1, 0]) / abs(pw64[q, -1, 0]))
+                                   for q in range(pw64.shape[0])]
+    fld = []
+    for c in range(6):
+        a, b = s32.get_field(c), s64.get_field(c)
+        fld.append(float(np.abs(a - b).max() / np.abs(b).max()))
+    out["final_field_max_rel"] = fld
+    s64.close()
+    s32.close()
     return out
 
 
@@ -466,7 +546,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tiling", type=str, default="")
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched leg (0: skip it)")
-    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--batch-steps", type=int, default=1500)
     ap.add_argument("--adjoint", type=int, default=1, help="run the design-gradient leg (0: skip it)")
     ap.add_argument("--adjoint-fd-cells", type=int, default=16)
     ap.add_argument("--batch-adjoint", type=int, default=32,
@@ -478,6 +558,10 @@ def main():
                     help="steps of the dispersive-media leg at --size (0: skip it)")
     ap.add_argument("--tmz", type=int, default=1400,
                     help="steps of the 2-D (TMz) cylinder leg (0: skip it)")
+    ap.add_argument("--fp64-steps", type=int, default=100,
+                    help="timed steps of the FP64 run of the same workload (0: skip it)")
+    ap.add_argument("--accuracy-steps", type=int, default=2000,
+                    help="steps of the FP32-against-FP64 accuracy runs (0: skip them)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 1 GiB device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -489,31 +573,16 @@ def main():
               "multi-GPU sharding is not built.", file=sys.stderr)
         sys.exit(2)
 
-    from xfdtd_amd.sim import Simulation
     sc, total = scene_400(n=args.size)
     if args.warmup + args.steps > total:
         total = args.warmup + args.steps
-    sim = Simulation(sc, total, precision=args.precision)
-    if args.tiling:
-        sim.set_tiling(*(int(v) for v in args.tiling.split(",")))
-    nx, ny, nz = sim.dims
+    tiling = tuple(int(v) for v in args.tiling.split(",")) if args.tiling else None
+    run = timed_run(sc, total, args.precision, args.steps, args.warmup, tiling)
+    nx, ny, nz = run["grid"]
     cells = nx * ny * nz
-    sim.step(args.warmup)
-    sim.sync()
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
-    sim.set_timing(True)
-    t0 = time.perf_counter()
-    sim.step(args.steps)
-    sim.sync()
-    wall = time.perf_counter() - t0
-    ms_e, ms_h, nsamp = sim.kernel_times()
-    sim.set_timing(False)
-    value = cells * args.steps / wall / 1e9
+    value = run["gcells"]
+    wall = run["wall_s"]
+    ms_e, ms_h, nsamp = run["ms_e"], run["ms_h"], run["event_samples"]
     s = 4 if args.precision == 32 else 8
     bytes_half = 9 * s + 1  # per cell per half step
     ms_dom = max(ms_e, ms_h)
@@ -560,24 +629,38 @@ def main():
         },
         "cpu_baseline": None,
     }
-    sim.close()
+    legs = []
+    if args.fp64_steps > 0 and args.precision == 32:
+        legs.append(("fp64", lambda: fp64_leg(sc, total, args.fp64_steps, args.warmup, tiling)))
+    if args.accuracy_steps > 0:
+        legs.append(("fp32_vs_fp64", lambda: run_fp32_vs_fp64(args.size, args.accuracy_steps)))
     if args.adjoint:
-        line["adjoint"] = run_adjoint(args.adjoint_fd_cells)
+        legs.append(("adjoint", lambda: run_adjoint(args.adjoint_fd_cells)))
     if args.batch_adjoint > 0:
-        line["batched_adjoint"] = run_batched_adjoint(args.batch_adjoint, args.batch_adjoint_steps)
+        legs.append(("batched_adjoint", lambda: run_batched_adjoint(args.batch_adjoint, args.batch_adjoint_steps)))
     if args.broadband > 0:
-        line["broadband"] = run_broadband(args.broadband)
+        legs.append(("broadband", lambda: run_broadband(args.broadband)))
     if args.dispersive > 0:
-        line["dispersive"] = run_dispersive(args.size, args.dispersive)
+        legs.append(("dispersive", lambda: run_dispersive(args.size, args.dispersive)))
     if args.tmz > 0:
-        line["tmz"] = run_tmz(args.tmz, 0 if args.no_cpu else 200, max(1, args.cpu_threads))
+        legs.append(("tmz", lambda: run_tmz(args.tmz, 0 if args.no_cpu else 200, max(1, args.cpu_threads))))
     if args.batch > 0:
-        line["batched"] = run_batched(args.batch, args.batch_steps,
-                                      log=lambda m: print(m, file=sys.stderr, flush=True))
+        legs.append(("batched", lambda: run_batched(args.batch, args.batch_steps,
+                                                    log=lambda m: print(m, file=sys.stderr, flush=True))))
     if not args.no_cpu and args.cpu_steps > 0:
-        line["cpu_baseline"] = run_cpu_baseline(sc, total, args.cpu_steps)
+        legs.append(("cpu_baseline", lambda: run_cpu_baseline(sc, total, args.cpu_steps)))
         if args.cpu_threads > 1:
-            line["cpu_baseline_threads"] = run_cpu_baseline(sc, total, args.cpu_steps, args.cpu_threads)
+            legs.append(("cpu_baseline_threads",
+                         lambda: run_cpu_baseline(sc, total, args.cpu_steps, args.cpu_threads)))
+    for name, fn in legs:
+        # a leg that fails records its error; the line is printed regardless
+        print(f"bench: {name}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        try:
+            line[name] = fn()
+        except Exception as exc:  # noqa: BLE001
+            line[name] = {"error": f"{type(exc).__name__}: {exc}"}
+        print(f"bench: {name} done in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 

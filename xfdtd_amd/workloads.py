@@ -163,3 +163,47 @@ def batched_adjoint_set(count=32, seed=0xC5, n=96, dl=4e-3, pml=8, box=(32, 64),
         sc.dft_plane = S.DftPlane(S.EZ, 0, n - pml - 10, pml, n - pml, pml, n - pml, S.C_0 / (20 * dl))
         out.append(sc)
     return out
+
+
+# the 8 probe points of the 400^3 dielectric-sphere run, each sampled on Ez and Ex
+SPHERE_400_PROBES = [(200, 200, 200), (200, 200, 310), (200, 200, 90), (120, 200, 200), (200, 330, 200),
+                     (30, 200, 200), (6, 200, 200), (200, 200, 395)]
+
+
+def pec_sphere_bistatic_scene(n=64, dl=5e-3, pml=8, radius_cells=16, cfl=0.9):
+    """The bistatic PEC-sphere scatter run in the shape of the reference's
+    examples/3d_pec_sphere_bistatic_rcs.cpp:34-101: an n^3 vacuum grid
+    (CPML included, order 3, sigma 0.8 sigma_opt, kappa_max 1, alpha 0.05),
+    a PEC sphere (sigma 1e10) of radius_cells at the centre, TFSF 4 cells
+    inside the CPML with an x-polarised Gaussian plane wave along +z
+    (tau = 20 dt, t0 = 4.5 tau), and the NF-FF box 2 cells outside the TFSF
+    box at f = c0/(40 dl), c0/(30 dl), c0/(20 dl) over 181 theta in [0, pi]
+    and phi in {0, pi/2}.  theta = pi is the monostatic direction."""
+    objs = [S.Object("pec", S.Sphere((0.0, 0.0, 0.0), radius_cells * dl), S.Material.pec())]
+    return S.cube_domain_scene(
+        n, dl, pml, cfl=cfl, objects=objs, tfsf_gap=pml + 4, nffft_gap=pml + 2,
+        freqs=[S.C_0 / (40 * dl), S.C_0 / (30 * dl), S.C_0 / (20 * dl)],
+        theta=np.linspace(0.0, math.pi, 181), phi=[0.0, math.pi / 2])
+
+
+def dielectric_sphere_scene(n=400, dl=2.5e-3, pml=12, radius_cells=100, probes=True):
+    """The single-domain dielectric-sphere run: an n^3 grid (CPML included,
+    order 3, sigma 0.8 sigma_opt, kappa_max 1, alpha 0.05), cfl 0.9, an
+    eps_r = 4, sigma = 0 sphere of radius_cells at the centre, TFSF 4 cells
+    inside the CPML (the same Gaussian plane wave as the PEC run), the NF-FF
+    box 2 cells outside it at c0/(40 dl), c0/(30 dl), c0/(20 dl) over 181
+    theta and phi in {0, pi/2}; at n = 400 Ez and Ex probes at the 8 points
+    of SPHERE_400_PROBES.  For grids smaller than 400 the radius is scaled
+    with n (at most n / 4) so the sphere stays clear of the CPML and the TFSF
+    box."""
+    if n != 400:
+        radius_cells = max(2, min(radius_cells, n // 4))
+    objs = [S.Object("sphere", S.Sphere((0.0, 0.0, 0.0), radius_cells * dl),
+                     S.Material("sphere", 4.0, 1.0, 0.0, 0.0))]
+    sc = S.cube_domain_scene(
+        n, dl, pml, cfl=0.9, objects=objs, tfsf_gap=pml + 4, nffft_gap=pml + 2,
+        freqs=[S.C_0 / (40 * dl), S.C_0 / (30 * dl), S.C_0 / (20 * dl)],
+        theta=np.linspace(0.0, math.pi, 181), phi=[0.0, math.pi / 2])
+    if probes and n == 400:
+        sc.probes = [S.Probe(c, *p) for p in SPHERE_400_PROBES for c in (S.EZ, S.EX)]
+    return sc
