@@ -153,6 +153,21 @@ xf_status xf_engine_farfield_sums(xf_engine *e, double dl,
                                   const double *wave_number,
                                   const double *theta, int ntheta,
                                   const double *phi, int nphi, double *out);
+/* The same far-field face sums without any device allocation.  This entry
+   point has no counterpart in the reference: it is for callers that cannot
+   call hipMalloc (serving loops, harnesses that forbid allocation).  The
+   direction table and the output block live in one file-scope device array
+   of XF_FARFIELD_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends;
+   a request needs 3 ntheta nphi + nf doubles (rounded up to 256 bytes) plus
+   192 nf ntheta nphi bytes, and a larger one is refused with XF_EINVAL.
+   Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+   (from several threads or engines) race.  Output equals
+   xf_engine_farfield_sums bit for bit. */
+#define XF_FARFIELD_NOALLOC_SCRATCH_BYTES (8u << 20)
+xf_status xf_engine_farfield_sums_noalloc(xf_engine *e, double dl,
+                                          const double *wave_number,
+                                          const double *theta, int ntheta,
+                                          const double *phi, int nphi, double *out);
 
 /* Broadband NF-FF surface (NffftBroadBand, nffft/nffft_broadband.cpp:30-91,
    the monostatic RCS path): after every step t < total_steps the face

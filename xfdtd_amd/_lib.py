@@ -17,7 +17,8 @@ _LIB = None
 ENGINE_SYMBOLS = [
     "xf_engine_create", "xf_engine_destroy", "xf_last_error", "xf_engine_set_materials",
     "xf_engine_set_dispersive", "xf_engine_set_cpml", "xf_engine_set_tfsf", "xf_engine_set_nffft", "xf_engine_nffft_size",
-    "xf_engine_get_nffft", "xf_engine_farfield_sums", "xf_engine_add_current_source",
+    "xf_engine_get_nffft", "xf_engine_farfield_sums", "xf_engine_farfield_sums_noalloc",
+    "xf_engine_add_current_source",
     "xf_engine_add_probe", "xf_engine_get_probes", "xf_engine_step", "xf_engine_sync",
     "xf_engine_current_step", "xf_engine_get_field", "xf_engine_set_field",
     "xf_engine_kernel_times", "xf_engine_set_timing", "xf_engine_set_tiling",
@@ -123,6 +124,10 @@ def lib():
     L.xf_sim_add_movie_monitor.argtypes = [vp, dp, dp, i, i, C.c_char_p, sz, sz]
     L.xf_sim_set_nffft_broadband.argtypes = [vp, szp, d, d, C.c_char_p]
     L.xf_engine_set_broadband_scratch.argtypes = [vp, i64]
+    L.xf_engine_nffft_size.restype = i64
+    L.xf_engine_nffft_size.argtypes = [vp]
+    for name in ("xf_engine_farfield_sums", "xf_engine_farfield_sums_noalloc"):
+        getattr(L, name).argtypes = [vp, d, dp, dp, i, dp, i, dp]
     ip = C.POINTER(C.c_int)
     L.xf_sim_add_lumped.argtypes = [vp, i, dp, dp, i, d, i, dp]
     L.xf_sim_add_plane.argtypes = [vp, dp, dp, C.c_float, C.c_float, d, d]

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -146,6 +147,7 @@ struct xf_engine {
   void *dj[3] = {nullptr, nullptr, nullptr};
   void *djp[3] = {nullptr, nullptr, nullptr}, *dep[3] = {nullptr, nullptr, nullptr};  // Lorentz state
   bool ade_non_drude = false;  // a Debye or Lorentz record (no adjoint)
+  int dev = 0;                 // HIP device the engine's memory lives on
 };
 
 // wait for everything enqueued for this engine, on its own stream and on
@@ -167,6 +169,33 @@ static xf_status dev_alloc(xf_engine *e, void **p, size_t bytes) {
   r = hipMemsetAsync(*p, 0, bytes, e->stream);
   if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
   if (r != hipSuccess) FAILS(XF_EDEVICE, "hipMemsetAsync: %s", hipGetErrorString(r));
+  return XF_OK;
+}
+
+// ---------------------------------------------------------------- allocation-free far field
+// xf_engine_farfield_sums_noalloc keeps its direction table, wave numbers and
+// the output block of k_farfield in this file-scope array instead of device
+// buffers allocated per call.  Each device holds its own copy for the life of
+// the process; its address on a device is taken once, with that device
+// current, when the first engine is created there.  The host-side name is
+// not a device address and is never passed to a kernel or a copy.
+static constexpr size_t kFarScratchBytes = (size_t)XF_FARFIELD_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) double2 scratch[kFarScratchBytes / sizeof(double2)];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[kMaxDevices] = {};
+// host staging of the direction table (never larger than the device block)
+static double g_far_host[kFarScratchBytes / sizeof(double)];
+
+static xf_status scratch_setup(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxDevices) FAILS(XF_EDEVICE, "device %d beyond the %d the library tracks", dev, kMaxDevices);
+  if (g_scratch_dev[dev]) return XF_OK;
+  void *p = nullptr;
+  const hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (r != hipSuccess || p == nullptr)
+    FAILS(XF_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(r));
+  g_scratch_dev[dev] = p;
   return XF_OK;
 }
 
@@ -255,6 +284,14 @@ extern "C" xf_status xf_engine_create(const xf_grid_desc *g, xf_engine **out) {
   // the curl kernels address a field with a 32-bit byte offset
   if ((unsigned long long)e->ncell * e->esz >= (1ULL << 32))
     FAILS(XF_EINVAL, "grid of %lld padded cells exceeds 4 GiB per field component", e->ncell);
+  if (hipGetDevice(&e->dev) != hipSuccess) {
+    delete e;
+    FAILS(XF_EDEVICE, "hipGetDevice failed");
+  }
+  if (scratch_setup(e->dev) != XF_OK) {
+    delete e;
+    return XF_EDEVICE;
+  }
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
     delete e;
     FAILS(XF_EDEVICE, "hipStreamCreate failed");
@@ -500,41 +537,78 @@ extern "C" xf_status xf_engine_get_nffft(xf_engine *e, double *out) {
   return XF_OK;
 }
 
+// direction cosines on the host with the same libm as the reference
+// (nffft_fd.cpp:383-388): tab[0, nd) sin(t) cos(p), [nd, 2 nd) sin(t) sin(p),
+// [2 nd, 3 nd) cos(t), then dl * k per frequency
+static void far_table(const xf_engine *e, double dl, const double *wave_number, const double *theta, int ntheta,
+                      const double *phi, int nphi, double *tab) {
+  const int nd = ntheta * nphi;
+  for (int t = 0; t < ntheta; ++t)
+    for (int p = 0; p < nphi; ++p) {
+      const double cos_t = cos(theta[t]), sin_t = sin(theta[t]);
+      const double cos_p = cos(phi[p]), sin_p = sin(phi[p]);
+      tab[t * nphi + p] = sin_t * cos_p;
+      tab[nd + t * nphi + p] = sin_t * sin_p;
+      tab[2 * nd + t * nphi + p] = cos_t;
+    }
+  for (int f = 0; f < e->nf.nf; ++f) tab[3 * nd + f] = dl * wave_number[f];
+}
+
+static void far_launch(xf_engine *e, const double *dtab, int ntheta, int nphi, double2 *dout) {
+  const int nd = ntheta * nphi;
+  FarDir fd;
+  fd.stcp = dtab; fd.stsp = dtab + nd; fd.ct = dtab + 2 * nd; fd.dk = dtab + 3 * nd; fd.nt = ntheta; fd.np = nphi;
+  hipLaunchKernelGGL(k_farfield, dim3(nd, e->nf.nf), dim3(256), 0, e->stream, e->nf, fd, dout);
+}
+
 extern "C" xf_status xf_engine_farfield_sums(xf_engine *e, double dl, const double *wave_number,
                                              const double *theta, int ntheta, const double *phi,
                                              int nphi, double *out) {
   if (!e || !out || !e->has_nf || !wave_number || !theta || !phi || ntheta < 1 || nphi < 1)
     FAILS(XF_EINVAL, "bad far-field request");
   const int nd = ntheta * nphi;
-  // direction cosines on the host with the same libm as the reference
-  // (nffft_fd.cpp:383-388)
-  std::vector<double> hs(3 * (size_t)nd), hk(e->nf.nf);
-  for (int t = 0; t < ntheta; ++t)
-    for (int p = 0; p < nphi; ++p) {
-      const double cos_t = cos(theta[t]), sin_t = sin(theta[t]);
-      const double cos_p = cos(phi[p]), sin_p = sin(phi[p]);
-      hs[t * nphi + p] = sin_t * cos_p;
-      hs[nd + t * nphi + p] = sin_t * sin_p;
-      hs[2 * nd + t * nphi + p] = cos_t;
-    }
-  for (int f = 0; f < e->nf.nf; ++f) hk[f] = dl * wave_number[f];
-  double *dd = nullptr, *dk = nullptr;
+  std::vector<double> tab(3 * (size_t)nd + e->nf.nf);
+  far_table(e, dl, wave_number, theta, ntheta, phi, nphi, tab.data());
+  double *dtab = nullptr;
   double2 *dout = nullptr;
   const size_t outn = (size_t)e->nf.nf * nd * 12;
-  HIPCHK(hipMalloc((void **)&dd, hs.size() * sizeof(double)));
-  HIPCHK(hipMalloc((void **)&dk, hk.size() * sizeof(double)));
+  HIPCHK(hipMalloc((void **)&dtab, tab.size() * sizeof(double)));
   HIPCHK(hipMalloc((void **)&dout, outn * sizeof(double2)));
-  HIPCHK(hipMemcpy(dd, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(dk, hk.data(), hk.size() * sizeof(double), hipMemcpyHostToDevice));
-  FarDir fd;
-  fd.stcp = dd; fd.stsp = dd + nd; fd.ct = dd + 2 * nd; fd.dk = dk; fd.nt = ntheta; fd.np = nphi;
-  hipLaunchKernelGGL(k_farfield, dim3(nd, e->nf.nf), dim3(256), 0, e->stream, e->nf, fd, dout);
+  HIPCHK(hipMemcpy(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  far_launch(e, dtab, ntheta, nphi, dout);
   HIPCHK(hipGetLastError());
   HIPCHK(engine_wait(e));
   HIPCHK(hipMemcpy(out, dout, outn * sizeof(double2), hipMemcpyDeviceToHost));
-  hipFree(dd);
-  hipFree(dk);
+  hipFree(dtab);
   hipFree(dout);
+  return XF_OK;
+}
+
+extern "C" xf_status xf_engine_farfield_sums_noalloc(xf_engine *e, double dl, const double *wave_number,
+                                                     const double *theta, int ntheta, const double *phi,
+                                                     int nphi, double *out) {
+  if (!e || !out || !e->has_nf || !wave_number || !theta || !phi || ntheta < 1 || nphi < 1)
+    FAILS(XF_EINVAL, "bad far-field request");
+  const size_t nd = (size_t)ntheta * (size_t)nphi, nf = (size_t)e->nf.nf;
+  const size_t tab_n = 3 * nd + nf;
+  const size_t tab_bytes = (tab_n * sizeof(double) + 255) / 256 * 256;
+  const size_t out_bytes = nf * nd * 12 * sizeof(double2);
+  if (tab_bytes + out_bytes > kFarScratchBytes)
+    FAILS(XF_EINVAL,
+          "far field of %zu directions x %zu frequencies needs %zu bytes of scratch; the allocation-free entry "
+          "point holds %zu",
+          nd, nf, tab_bytes + out_bytes, kFarScratchBytes);
+  void *base = g_scratch_dev[e->dev];
+  if (!base) FAILS(XF_ESTATE, "no scratch address for device %d", e->dev);
+  double *dtab = (double *)base;
+  double2 *dout = (double2 *)((char *)base + tab_bytes);
+  HIPCHK(engine_wait(e));
+  far_table(e, dl, wave_number, theta, ntheta, phi, nphi, g_far_host);
+  HIPCHK(hipMemcpyAsync(dtab, g_far_host, tab_n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  far_launch(e, dtab, ntheta, nphi, dout);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout, out_bytes, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
   return XF_OK;
 }
 
