@@ -5,7 +5,7 @@
   (0.3 rad) TFSF2D projections and at a third origin corner (5 pi/4); the
   surface integrals and the far-zone Ez of every bin within 1e-10 of each
   quantity's peak (the GPU takes the DFTs with hipFFT).
-- FP32 fields within 1e-4 of the FP64 oracle's peak Ez.
+- FP32 fields within 1e-5 of the FP64 oracle's peak Ez.
 - The reference's 2-D example (examples/2d_cylinder_scatter.cpp) through
   the C++ API at full size (370 x 320 cells, 1400 steps, FP64): the files
   (EZ.npy, far_field_parameter.dat, the Ez movie frames, the incident
@@ -86,7 +86,8 @@ def test_2d_fp32_close(oracle_threads):
     o.step(steps)
     a, b = g.get_field(S.EZ), o.get_field(S.EZ)
     err = np.abs(a - b).max() / np.abs(b).max()
-    assert err < 1e-4, err
+    print(f"2-D FP32 Ez error {err:.3e}", flush=True)
+    assert err < 1e-5, err
 
 
 def test_2d_batch_refused():

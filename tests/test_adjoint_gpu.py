@@ -8,7 +8,7 @@
   the reference's per-face arrays) to 1e-11 of the result's scale;
 - dJ/d eps_r matches the oracle's to 1e-10 (FP64) for any checkpoint
   interval, and central differences on the GPU forward run;
-- the FP32 gradient stays within 1e-3 of the FP64 one;
+- the FP32 gradient stays within 1e-4 of the FP64 one;
 - all of the above with Drude cells, whose currents are adjoint state too.
 """
 import numpy as np
@@ -124,8 +124,9 @@ def test_fp32_gradient_close_to_fp64():
     sc, steps = design_scene()
     J64, g64 = _sim(sc, steps, 64).grad(checkpoint_every=40)
     J32, g32 = _sim(sc, steps, 32).grad(checkpoint_every=40)
-    assert J32 == pytest.approx(J64, rel=1e-4)
-    assert _rel(g32, g64) < 1e-3
+    print(f"FP32 J {abs(J32 - J64) / J64:.2e}, gradient {_rel(g32, g64):.2e}", flush=True)
+    assert J32 == pytest.approx(J64, rel=1e-5)
+    assert _rel(g32, g64) < 1e-4
 
 
 def test_torch_autograd_function():
@@ -190,4 +191,5 @@ def test_gradient_with_drude_matches_oracle(every, oracle_threads):
     assert Jg == pytest.approx(Jo, rel=1e-14)
     assert _rel(gg, go) <= 1e-10
     _, g32 = _sim(sc, steps, 32).grad(checkpoint_every=every)
-    assert _rel(g32, go) < 1e-3
+    print(f"Drude FP32 gradient {_rel(g32, go):.2e}", flush=True)
+    assert _rel(g32, go) < 1e-4

@@ -7,7 +7,7 @@
   another order).
 - The far field does not depend on how the series are batched into
   transforms (1e-12).
-- FP32 fields: the power densities within 1e-4 of the FP64 oracle's peak.
+- FP32 fields: the power densities within 1e-5 of the FP64 oracle's peak.
 - The reference's monostatic example (examples/3d_pec_sphere_monostatic_rcs.cpp)
   through the C++ API: the files of NffftBroadBand::outputData, and the
   backscatter cross-section of the PEC sphere against the Mie series.
@@ -103,7 +103,8 @@ def test_bb_fp32_close_to_oracle(oracle_threads):
     for name in ("power_theta", "power_phi"):
         scale = np.abs(ref[name]).max()
         err = np.abs(r[name] - ref[name]).max() / scale
-        assert err < 1e-4, (name, err)
+        print(f"broadband FP32 {name} error {err:.3e}", flush=True)
+        assert err < 1e-5, (name, err)
 
 
 def test_bb_batch_refused():

@@ -4,7 +4,7 @@
   CPML, a Lorentz sphere, a lossy Debye cube, a Drude cube and a plain
   object covering part of the sphere; the filled TM110 cavities of the
   oracle pins.
-- FP32: the Lorentz cavity's probe within 1e-4 of the oracle's peak.
+- FP32: the Lorentz cavity's probe within 1e-5 of the oracle's peak.
 - Checkpoints carry the Lorentz state (J, and the previous step's J and
   E): a restored run continues bit for bit.
 - Gradients refuse Debye and Lorentz media.
@@ -90,7 +90,8 @@ def test_lorentz_cavity_fp32_close():
         s.set_field(S.EZ, _mode(s))
         s.step(steps)
     a, b = g.get_probes()[0], o.get_probes()[0]
-    assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max()
+    print(f"Lorentz cavity FP32 probe error {np.abs(a - b).max() / np.abs(b).max():.3e}", flush=True)
+    assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
 
 
 def test_checkpoint_carries_lorentz_state():

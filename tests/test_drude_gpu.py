@@ -81,7 +81,8 @@ def test_drude_fp32_close_to_oracle(oracle_threads):
     for c in range(6):
         ref = o.get_field(c)
         err = np.abs(g.get_field(c) - ref).max() / np.abs(ref).max()
-        assert err < 1e-4, (c, err)
+        print(f"Drude FP32 component {c} error {err:.3e}", flush=True)
+        assert err < 1e-5, (c, err)
 
 
 def test_drude_batch_member_bitwise():

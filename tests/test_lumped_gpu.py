@@ -4,7 +4,7 @@ networks on the GPU against the oracle.
 - The reference's basic lumped-element circuit with a capacitor, an
   inductor, a current source and a later object over part of a plate: FP64
   fields and every monitor series bit-identical to the oracle; FP32 within
-  1e-4.
+  1e-5.
 - The inductor-filled cavity of the oracle pin: bit-identical.
 - A batch with that circuit as a member steps it bit for bit, and a
   checkpoint restore continues bit for bit (the inductor currents are state).
@@ -55,7 +55,8 @@ def test_lumped_circuit_fp32_close(oracle_threads):
     mg, mo = g.get_line_monitors(), o.get_line_monitors()
     for q in range(4):
         err = np.abs(mg[q] - mo[q]).max() / np.abs(mo[q]).max()
-        assert err < 1e-4, (q, err)
+        print(f"lumped FP32 monitor {q} error {err:.3e}", flush=True)
+        assert err < 1e-5, (q, err)
 
 
 def test_inductor_cavity_bitwise():

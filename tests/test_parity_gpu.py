@@ -156,9 +156,10 @@ def test_fp32_error_vs_fp64_oracle(oracle_threads):
     o.step(steps)
     errs = [_rel(g.get_field(c), o.get_field(c)) for c in range(6)]
     # FP32 fields against the FP64 oracle: error grows like sqrt(steps) ulps
-    assert max(errs) < 1e-4, errs
     ag, ao = g.get_nffft(), o.get_nffft()
-    assert _rel(ag, ao) < 1e-4
+    print(f"FP32 field errors {['%.2e' % e for e in errs]}, NF-FF {_rel(ag, ao):.2e}", flush=True)
+    assert max(errs) < 1e-5, errs
+    assert _rel(ag, ao) < 1e-5
 
 
 def test_step_beyond_horizon_is_refused():

@@ -237,6 +237,14 @@ static xf_status upload_real(xf_engine *e, void **dst, const double *src, size_t
   return XF_OK;
 }
 
+// upload FP64 host data as FP64 whatever the working precision
+static xf_status upload_f64(xf_engine *e, void **dst, const double *src, size_t n) {
+  xf_status s = dev_alloc(e, dst, n * sizeof(double));
+  if (s != XF_OK) return s;
+  if (n > 0) HIPCHK(hipMemcpy(*dst, src, n * sizeof(double), hipMemcpyHostToDevice));
+  return XF_OK;
+}
+
 static void comp_shape(const xf_engine *e, int comp, int sh[3]) {
   const int nx = e->nx, ny = e->ny, nz = e->nz;
   switch (comp) {
@@ -453,9 +461,11 @@ extern "C" xf_status xf_engine_set_tfsf(xf_engine *e, const xf_tfsf_desc *d) {
   e->tlk = (int)d->lk; e->trk = (int)d->rk; e->L = (int)d->aux_size;
   const int bn[3] = {e->tri - e->tli, e->trj - e->tlj, e->trk - e->tlk};
   xf_status s;
+  // the incident line, its projections and source table are FP64 at either
+  // field precision
   for (int a = 0; a < 3; ++a) {
-    if ((s = upload_real(e, &e->pf[a], d->proj_full[a], (size_t)bn[a] + 1)) != XF_OK) return s;
-    if ((s = upload_real(e, &e->ph[a], d->proj_half[a], (size_t)bn[a] + 2)) != XF_OK) return s;
+    if ((s = upload_f64(e, &e->pf[a], d->proj_full[a], (size_t)bn[a] + 1)) != XF_OK) return s;
+    if ((s = upload_f64(e, &e->ph[a], d->proj_half[a], (size_t)bn[a] + 2)) != XF_OK) return s;
     e->te[a] = d->transform_e[a];
     e->th[a] = d->transform_h[a];
     e->tca[a] = d->ca[a];
@@ -464,13 +474,13 @@ extern "C" xf_status xf_engine_set_tfsf(xf_engine *e, const xf_tfsf_desc *d) {
   // projections must stay inside the aux arrays for every face cell
   e->ceie = d->ceie; e->ceihi = d->ceihi; e->chih = d->chih; e->chiei = d->chiei;
   e->c1 = d->abc_c1; e->c2 = d->abc_c2;
-  if ((s = dev_alloc(e, &e->e_inc, (size_t)e->L * e->esz)) != XF_OK) return s;
-  if ((s = dev_alloc(e, &e->h_inc, (size_t)(e->L - 1) * e->esz)) != XF_OK) return s;
-  if ((s = dev_alloc(e, &e->ab, 2 * e->esz)) != XF_OK) return s;
-  e->state.push_back({e->e_inc, (size_t)e->L * e->esz});
-  e->state.push_back({e->h_inc, (size_t)(e->L - 1) * e->esz});
-  e->state.push_back({e->ab, 2 * e->esz});
-  if ((s = upload_real(e, &e->src, d->source, (size_t)d->total_steps)) != XF_OK) return s;
+  if ((s = dev_alloc(e, &e->e_inc, (size_t)e->L * sizeof(double))) != XF_OK) return s;
+  if ((s = dev_alloc(e, &e->h_inc, (size_t)(e->L - 1) * sizeof(double))) != XF_OK) return s;
+  if ((s = dev_alloc(e, &e->ab, 2 * sizeof(double))) != XF_OK) return s;
+  e->state.push_back({e->e_inc, (size_t)e->L * sizeof(double)});
+  e->state.push_back({e->h_inc, (size_t)(e->L - 1) * sizeof(double)});
+  e->state.push_back({e->ab, 2 * sizeof(double)});
+  if ((s = upload_f64(e, &e->src, d->source, (size_t)d->total_steps)) != XF_OK) return s;
   e->tfsf_T = d->total_steps;
   set_total(e, d->total_steps);
   const int A0 = bn[0] + 1, A1 = bn[1] + 1, A2 = bn[2] + 1;
@@ -915,22 +925,22 @@ static Grid<T> make_grid(xf_engine *e) {
   return g;
 }
 
-template <typename T>
-static Tfsf<T> make_tfsf(xf_engine *e) {
-  Tfsf<T> t;
+static Tfsf make_tfsf(xf_engine *e) {
+  Tfsf t;
   t.li = e->tli; t.ri = e->tri; t.lj = e->tlj; t.rj = e->trj; t.lk = e->tlk; t.rk = e->trk; t.L = e->L;
   t.two_d = e->tfsf_two_d ? 1 : 0;
   for (int a = 0; a < 3; ++a) {
-    t.pf[a] = (const T *)e->pf[a];
-    t.ph[a] = (const T *)e->ph[a];
-    t.te[a] = (T)e->te[a];
-    t.th[a] = (T)e->th[a];
-    t.ca[a] = (T)e->tca[a];
-    t.cb[a] = (T)e->tcb[a];
+    t.pf[a] = (const double *)e->pf[a];
+    t.ph[a] = (const double *)e->ph[a];
+    t.te[a] = e->te[a];
+    t.th[a] = e->th[a];
+    t.ca[a] = e->tca[a];
+    t.cb[a] = e->tcb[a];
   }
-  t.ceie = (T)e->ceie; t.ceihi = (T)e->ceihi; t.chih = (T)e->chih; t.chiei = (T)e->chiei;
-  t.c1 = (T)e->c1; t.c2 = (T)e->c2;
-  t.e_inc = (T *)e->e_inc; t.h_inc = (T *)e->h_inc; t.ab = (T *)e->ab; t.src = (const T *)e->src;
+  t.ceie = e->ceie; t.ceihi = e->ceihi; t.chih = e->chih; t.chiei = e->chiei;
+  t.c1 = e->c1; t.c2 = e->c2;
+  t.e_inc = (double *)e->e_inc; t.h_inc = (double *)e->h_inc; t.ab = (double *)e->ab;
+  t.src = (const double *)e->src;
   return t;
 }
 
@@ -997,8 +1007,8 @@ static void launch_line_monitors(const xf_engine *e, const Grid<T> &g, hipStream
 template <typename T>
 static xf_status run_steps(xf_engine *e, long long n) {
   Grid<T> g = make_grid<T>(e);
-  Tfsf<T> t{};
-  if (e->has_tfsf) t = make_tfsf<T>(e);
+  Tfsf t{};
+  if (e->has_tfsf) t = make_tfsf(e);
   const Tiling tl = make_tiling(e);
   dim3 blk(tiling_threads(tl));
   dim3 grd(tl.nblocks);
@@ -1061,7 +1071,7 @@ static xf_status run_steps(xf_engine *e, long long n) {
       hipLaunchKernelGGL((k_update_h<T, false>), grd, blk, 0, e->stream, g, tl);
     if (ev) hipEventRecord(ev[3], e->stream);
     if (e->has_tfsf) {
-      hipLaunchKernelGGL(k_tfsf_ifa<T>, dim3(1), dim3(1024), 0, e->stream, t, step);
+      hipLaunchKernelGGL(k_tfsf_ifa, dim3(1), dim3(1024), 0, e->stream, t, step);
       const int tot = e->hseg.off[12];
       hipLaunchKernelGGL(k_tfsf_h<T>, dim3((tot + 255) / 256), dim3(256), 0, e->stream, g, t, e->hseg);
     }
@@ -1276,7 +1286,7 @@ static xf_status batch_build(xf_batch *b) {
     if (lds > 32768 || (e->tile_flags & XF_TILE_GLOBAL_TABLES)) b->use_lds = false;
     for (int w = 0; w < a.tl.nblocks; ++w) mc.push_back(make_int2(q, w));
     if (e->has_tfsf) {
-      a.t = make_tfsf<T>(e);
+      a.t = make_tfsf(e);
       a.nEy = e->nEy; a.nEz = e->nEz; a.nEx = e->nEx;
       a.hs = e->hseg;
       const int te = (e->nEy + e->nEz + e->nEx + 255) / 256, th = (e->hseg.off[12] + 255) / 256;

@@ -833,24 +833,26 @@ __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl) {
 }
 
 // ---------------------------------------------------------------------------
-// TFSF (tfsf/tfsf_3d.cpp)
-template <typename T>
+// TFSF (tfsf/tfsf_3d.cpp).  The 1-D incident line, its projections and the
+// face-correction arithmetic are FP64 for both field precisions: the line is
+// a few hundred cells, and an FP32 field then takes one rounding per
+// correction, of the FP64 sum, instead of carrying an FP32 line's rounding
+// into the scattered-field region.
 struct Tfsf {
   int li, ri, lj, rj, lk, rk, L;
   int two_d;  // TFSF2D: Ez, Hx, Hy only, projections without z
-  const T *pf[3], *ph[3];
-  T te[3], th[3];
-  T ceie, ceihi, chih, chiei, c1, c2;
-  T ca[3], cb[3];
-  T *e_inc, *h_inc, *ab;
-  const T *src;
+  const double *pf[3], *ph[3];
+  double te[3], th[3];
+  double ceie, ceihi, chih, chiei, c1, c2;
+  double ca[3], cb[3];
+  double *e_inc, *h_inc, *ab;
+  const double *src;
 };
 
 // TFSF3D::getIncident{Ex..Hz} (tfsf_3d.cpp:413-483)
-template <typename T>
-__device__ __forceinline__ T tfsf_inc(const Tfsf<T> &t, int comp, int i, int j, int k) {
-  T proj, s;
-  const T *arr;
+__device__ __forceinline__ double tfsf_inc(const Tfsf &t, int comp, int i, int j, int k) {
+  double proj, s;
+  const double *arr;
   switch (comp) {
     case EX: proj = t.ph[0][i - t.li + 1] + t.pf[1][j - t.lj] + t.pf[2][k - t.lk];
              arr = t.e_inc; s = t.te[0]; break;
@@ -863,33 +865,32 @@ __device__ __forceinline__ T tfsf_inc(const Tfsf<T> &t, int comp, int i, int j, 
              arr = t.e_inc; s = t.te[2]; break;
     case HX: proj = t.pf[0][i - t.li] + t.ph[1][j - t.lj + 1];
              if (!t.two_d) proj = proj + t.ph[2][k - t.lk + 1];
-             proj = proj - T(0.5);
+             proj = proj - 0.5;
              arr = t.h_inc; s = t.th[0]; break;
     case HY: proj = t.ph[0][i - t.li + 1] + t.pf[1][j - t.lj];
              if (!t.two_d) proj = proj + t.ph[2][k - t.lk + 1];
-             proj = proj - T(0.5);
+             proj = proj - 0.5;
              arr = t.h_inc; s = t.th[1]; break;
-    default: proj = t.ph[0][i - t.li + 1] + t.ph[1][j - t.lj + 1] + t.pf[2][k - t.lk] - T(0.5);
+    default: proj = t.ph[0][i - t.li + 1] + t.ph[1][j - t.lj + 1] + t.pf[2][k - t.lk] - 0.5;
              arr = t.h_inc; s = t.th[2]; break;
   }
   const long long idx = (long long)proj;
-  const T w = proj - (T)idx;
-  return (T(1) - w) * (s * arr[idx]) + w * (s * arr[idx + 1]);
+  const double w = proj - (double)idx;
+  return (1.0 - w) * (s * arr[idx]) + w * (s * arr[idx + 1]);
 }
 
 // TFSF3D::updateIncidentField (tfsf_3d.cpp:153-183): one workgroup
-template <typename T>
-__device__ __forceinline__ void tfsf_ifa_body(const Tfsf<T> &t, long long step) {
+__device__ __forceinline__ void tfsf_ifa_body(const Tfsf &t, long long step) {
   const int L = t.L;
-  T *e = t.e_inc, *h = t.h_inc;
-  const T x = e[L - 2], y = e[L - 1];
+  double *e = t.e_inc, *h = t.h_inc;
+  const double x = e[L - 2], y = e[L - 1];
   __syncthreads();
   if (threadIdx.x == 0) e[0] = t.src[step];
   for (int i = 1 + threadIdx.x; i < L - 1; i += blockDim.x)
     e[i] = t.ceie * e[i] + t.ceihi * (h[i] - h[i - 1]);
   __syncthreads();
   if (threadIdx.x == 0) {
-    const T a = t.ab[0], b = t.ab[1];
+    const double a = t.ab[0], b = t.ab[1];
     e[L - 1] = -a + t.c1 * (e[L - 2] + b) + t.c2 * (x + y);
     t.ab[0] = x;
     t.ab[1] = y;
@@ -898,9 +899,8 @@ __device__ __forceinline__ void tfsf_ifa_body(const Tfsf<T> &t, long long step) 
   for (int i = threadIdx.x; i < L - 1; i += blockDim.x)
     h[i] = t.chih * h[i] + t.chiei * (e[i + 1] - e[i]);
 }
-template <typename T>
-__global__ void __launch_bounds__(1024) k_tfsf_ifa(Tfsf<T> t, long long step) {
-  tfsf_ifa_body<T>(t, step);
+__global__ void __launch_bounds__(1024) k_tfsf_ifa(Tfsf t, long long step) {
+  tfsf_ifa_body(t, step);
 }
 
 __device__ __forceinline__ void perimeter(int p, int A, int B, int &u, int &v) {
@@ -915,7 +915,7 @@ __device__ __forceinline__ void perimeter(int p, int A, int B, int &u, int &v) {
 // one thread owns each target cell and applies them in the reference's
 // xn, xp, yn, yp, zn, zp order.
 template <typename T>
-__device__ __forceinline__ void tfsf_e_body(const Grid<T> &g, const Tfsf<T> &t, int nEy, int nEz, int nEx,
+__device__ __forceinline__ void tfsf_e_body(const Grid<T> &g, const Tfsf &t, int nEy, int nEz, int nEx,
                                             int block) {
   int q = block * (int)blockDim.x + (int)threadIdx.x;
   const int li = t.li, ri = t.ri, lj = t.lj, rj = t.rj, lk = t.lk, rk = t.rk;
@@ -926,12 +926,12 @@ __device__ __forceinline__ void tfsf_e_body(const Grid<T> &g, const Tfsf<T> &t, 
     perimeter(q % P, A, B, u, v);
     const int i = li + u, k = lk + v;
     T *p = &g.f[EY][(long long)i * g.si + (long long)j * g.sj + k];
-    T val = *p;
+    double val = *p;
     if (i == li) val = val + t.ca[0] * tfsf_inc(t, HZ, li - 1, j, k);
     if (i == ri) val = val - t.ca[0] * tfsf_inc(t, HZ, ri, j, k);
     if (k == lk) val = val - t.ca[2] * tfsf_inc(t, HX, i, j, lk - 1);
     if (k == rk) val = val + t.ca[2] * tfsf_inc(t, HX, i, j, rk);
-    *p = val;
+    *p = (T)val;
     return;
   }
   q -= nEy;
@@ -942,12 +942,12 @@ __device__ __forceinline__ void tfsf_e_body(const Grid<T> &g, const Tfsf<T> &t, 
     perimeter(q % P, A, B, u, v);
     const int i = li + u, j = lj + v;
     T *p = &g.f[EZ][(long long)i * g.si + (long long)j * g.sj + k];
-    T val = *p;
+    double val = *p;
     if (i == li) val = val - t.ca[0] * tfsf_inc(t, HY, li - 1, j, k);
     if (i == ri) val = val + t.ca[0] * tfsf_inc(t, HY, ri, j, k);
     if (j == lj) val = val + t.ca[1] * tfsf_inc(t, HX, i, lj - 1, k);
     if (j == rj) val = val - t.ca[1] * tfsf_inc(t, HX, i, rj, k);
-    *p = val;
+    *p = (T)val;
     return;
   }
   q -= nEz;
@@ -958,16 +958,16 @@ __device__ __forceinline__ void tfsf_e_body(const Grid<T> &g, const Tfsf<T> &t, 
     perimeter(q % P, A, B, u, v);
     const int j = lj + u, k = lk + v;
     T *p = &g.f[EX][(long long)i * g.si + (long long)j * g.sj + k];
-    T val = *p;
+    double val = *p;
     if (j == lj) val = val - t.ca[1] * tfsf_inc(t, HZ, i, lj - 1, k);
     if (j == rj) val = val + t.ca[1] * tfsf_inc(t, HZ, i, rj, k);
     if (k == lk) val = val + t.ca[2] * tfsf_inc(t, HY, i, j, lk - 1);
     if (k == rk) val = val - t.ca[2] * tfsf_inc(t, HY, i, j, rk);
-    *p = val;
+    *p = (T)val;
   }
 }
 template <typename T>
-__global__ void k_tfsf_e(Grid<T> g, Tfsf<T> t, int nEy, int nEz, int nEx) {
+__global__ void k_tfsf_e(Grid<T> g, Tfsf t, int nEy, int nEz, int nEx) {
   tfsf_e_body<T>(g, t, nEy, nEz, nEx, blockIdx.x);
 }
 
@@ -976,7 +976,7 @@ struct TfsfHSeg {
   int off[13];  // prefix offsets of the 12 face loops
 };
 template <typename T>
-__device__ __forceinline__ void tfsf_h_body(const Grid<T> &g, const Tfsf<T> &t, const TfsfHSeg &s,
+__device__ __forceinline__ void tfsf_h_body(const Grid<T> &g, const Tfsf &t, const TfsfHSeg &s,
                                             int block) {
   const int q = block * (int)blockDim.x + (int)threadIdx.x;
   if (q >= s.off[12]) return;
@@ -985,7 +985,7 @@ __device__ __forceinline__ void tfsf_h_body(const Grid<T> &g, const Tfsf<T> &t, 
   const int r = q - s.off[seg];
   const int li = t.li, ri = t.ri, lj = t.lj, rj = t.rj, lk = t.lk, rk = t.rk;
   int i = 0, j = 0, k = 0, comp = HX, src = EX;
-  T coef = 0;
+  double coef = 0;
   bool add = true;
   switch (seg) {
     case 0: { const int nk = rk + 1 - lk; j = lj + r / nk; k = lk + r % nk; i = li - 1;
@@ -1018,12 +1018,13 @@ __device__ __forceinline__ void tfsf_h_body(const Grid<T> &g, const Tfsf<T> &t, 
   const int si_ = (seg == 0 || seg == 1) ? li : i;
   const int sj_ = (seg == 4 || seg == 5) ? lj : j;
   const int sk_ = (seg == 8 || seg == 9) ? lk : k;
-  const T inc = tfsf_inc(t, src, si_, sj_, sk_);
+  const double inc = tfsf_inc(t, src, si_, sj_, sk_);
   T *p = &g.f[comp][(long long)i * g.si + (long long)j * g.sj + k];
-  *p = add ? (*p + coef * inc) : (*p - coef * inc);
+  const double v = *p;
+  *p = (T)(add ? (v + coef * inc) : (v - coef * inc));
 }
 template <typename T>
-__global__ void k_tfsf_h(Grid<T> g, Tfsf<T> t, TfsfHSeg s) {
+__global__ void k_tfsf_h(Grid<T> g, Tfsf t, TfsfHSeg s) {
   tfsf_h_body<T>(g, t, s, blockIdx.x);
 }
 
@@ -1771,7 +1772,7 @@ template <typename T>
 struct Inst {
   Grid<T> g;
   Tiling tl;
-  Tfsf<T> t;
+  Tfsf t;
   int nEy, nEz, nEx;
   TfsfHSeg hs;
   Nffft nf;
@@ -1813,7 +1814,7 @@ __global__ void kb_tfsf_h(const Inst<T> *__restrict__ in, const int2 *__restrict
 template <typename T>
 __global__ void __launch_bounds__(1024) kb_tfsf_ifa(const Inst<T> *__restrict__ in, const int2 *__restrict__ map,
                                                     long long step) {
-  tfsf_ifa_body<T>(in[map[blockIdx.x].x].t, step);
+  tfsf_ifa_body(in[map[blockIdx.x].x].t, step);
 }
 template <typename T>
 __global__ void kb_nffft(const Inst<T> *__restrict__ in, const int2 *__restrict__ map, long long step) {
