@@ -448,14 +448,17 @@ def run_tmz(steps=1400, cpu_steps=200, cpu_threads=16):
     return out
 
 
-def timed_run(sc, total, precision, steps, warmup, tiling=None):
+def timed_run(sc, total, precision, steps, warmup, tiling=None, sweep=None):
     """Warm-up steps, then `steps` timed steps of one Simulation with the
     curl kernels' HIP events recorded on the engine's stream."""
     from xfdtd_amd.sim import Simulation
     sim = Simulation(sc, total, precision=precision)
     if tiling:
         sim.set_tiling(*tiling)
+    if sweep:
+        sim.set_sweep(*sweep)
     nx, ny, nz = sim.dims
+    fused = sim.sweep_active
     sim.step(warmup)
     sim.sync()
     sim.set_timing(True)
@@ -467,7 +470,21 @@ def timed_run(sc, total, precision, steps, warmup, tiling=None):
     sim.set_timing(False)
     sim.close()
     return {"grid": [nx, ny, nz], "wall_s": wall, "gcells": nx * ny * nz * steps / wall / 1e9,
-            "ms_per_step": wall / steps * 1e3, "ms_e": ms_e, "ms_h": ms_h, "event_samples": nsamp}
+            "ms_per_step": wall / steps * 1e3, "ms_e": ms_e, "ms_h": ms_h, "event_samples": nsamp,
+            "fused": fused}
+
+
+def dominant(run, s):
+    """The dominant field kernel of a timed run, its average HIP-event time and
+    its algorithmic bytes per launch per cell: the fused step (k_sweep) does a
+    whole cell-step, 18 s + 2 bytes (read 3 E + 3 H and write 3 E, read 3 H
+    + 3 E and write 3 H, one material byte per half step); a split kernel does
+    a half step, 9 s + 1 bytes."""
+    if run["fused"]:
+        return "k_sweep", run["ms_e"], 18 * s + 2
+    if run["ms_h"] >= run["ms_e"]:
+        return "k_update_h", run["ms_h"], 9 * s + 1
+    return "k_update_e", run["ms_e"], 9 * s + 1
 
 
 def fp64_leg(sc, total, steps, warmup, tiling=None):
@@ -475,13 +492,30 @@ def fp64_leg(sc, total, steps, warmup, tiling=None):
     r = timed_run(sc, total, 64, steps, warmup, tiling)
     nx, ny, nz = r["grid"]
     cells = nx * ny * nz
-    ms_dom = max(r["ms_e"], r["ms_h"])
-    achieved = (9 * 8 + 1) * cells / (ms_dom * 1e-3) / 1e9
+    kname, ms_dom, bpc = dominant(r, 8)
+    achieved = bpc * cells / (ms_dom * 1e-3) / 1e9
     return {"value": round(r["gcells"], 3), "unit": "Gcells/s", "steps": steps, "warmup": warmup,
             "ms_per_step": round(r["ms_per_step"], 4),
-            "avg_ms": {"k_update_e": round(r["ms_e"], 4), "k_update_h": round(r["ms_h"], 4)},
-            "roofline": {"kernel": "k_update_h" if r["ms_h"] >= r["ms_e"] else "k_update_e",
-                         "bytes_per_launch": 73 * cells, "achieved": round(achieved, 1),
+            "avg_ms": {kname: round(ms_dom, 4)} if r["fused"] else
+                      {"k_update_e": round(r["ms_e"], 4), "k_update_h": round(r["ms_h"], 4)},
+            "roofline": {"kernel": kname, "bytes_per_launch": bpc * cells, "achieved": round(achieved, 1),
+                         "frac": round(achieved / HBM_PEAK_GBS, 4)}}
+
+
+def fused_leg(sc, total, steps, warmup, precision):
+    """The same workload through the fused E+H kernel (time-step scheme 1,
+    k_sweep), for comparison with the split kernels of the headline line."""
+    r = timed_run(sc, total, precision, steps, warmup, sweep=(1, 1, 32))
+    if not r["fused"]:
+        return {"error": "the scene does not take the fused kernel"}
+    nx, ny, nz = r["grid"]
+    s = 4 if precision == 32 else 8
+    kname, ms_dom, bpc = dominant(r, s)
+    achieved = bpc * nx * ny * nz / (ms_dom * 1e-3) / 1e9
+    return {"value": round(r["gcells"], 3), "unit": "Gcells/s", "steps": steps, "warmup": warmup,
+            "ms_per_step": round(r["ms_per_step"], 4), "avg_ms": {kname: round(ms_dom, 4)},
+            "geometry": {"rows": 1, "planes": 32},
+            "roofline": {"kernel": kname, "bytes_per_launch": bpc * nx * ny * nz, "achieved": round(achieved, 1),
                          "frac": round(achieved / HBM_PEAK_GBS, 4)}}
 
 
@@ -558,6 +592,8 @@ def main():
                     help="steps of the dispersive-media leg at --size (0: skip it)")
     ap.add_argument("--tmz", type=int, default=1400,
                     help="steps of the 2-D (TMz) cylinder leg (0: skip it)")
+    ap.add_argument("--fused-steps", type=int, default=100,
+                    help="timed steps of the same workload through the fused E+H kernel (0: skip it)")
     ap.add_argument("--fp64-steps", type=int, default=100,
                     help="timed steps of the FP64 run of the same workload (0: skip it)")
     ap.add_argument("--accuracy-steps", type=int, default=2000,
@@ -584,10 +620,8 @@ def main():
     wall = run["wall_s"]
     ms_e, ms_h, nsamp = run["ms_e"], run["ms_h"], run["event_samples"]
     s = 4 if args.precision == 32 else 8
-    bytes_half = 9 * s + 1  # per cell per half step
-    ms_dom = max(ms_e, ms_h)
-    dom = "k_update_h" if ms_h >= ms_e else "k_update_e"
-    achieved = bytes_half * cells / (ms_dom * 1e-3) / 1e9
+    dom, ms_dom, bytes_cell = dominant(run, s)
+    achieved = bytes_cell * cells / (ms_dom * 1e-3) / 1e9
     traffic, tinfo = (None, None)
     if args.precision == 32 and args.size == 400 and not args.tiling:
         traffic, tinfo = measured_traffic(dom)
@@ -623,13 +657,16 @@ def main():
             "traffic": None if traffic is None else round(traffic / 1e9, 4),
             "traffic_unit": "GB per launch",
             "traffic_source": None if tinfo is None else tinfo.get("source"),
-            "bytes_per_launch": bytes_half * cells,
-            "avg_ms": {"k_update_e": round(ms_e, 4), "k_update_h": round(ms_h, 4)},
+            "bytes_per_launch": bytes_cell * cells,
+            "avg_ms": {dom: round(ms_dom, 4)} if run["fused"] else
+                      {"k_update_e": round(ms_e, 4), "k_update_h": round(ms_h, 4)},
             "event_samples": nsamp,
         },
         "cpu_baseline": None,
     }
     legs = []
+    if args.fused_steps > 0:
+        legs.append(("fused", lambda: fused_leg(sc, total, args.fused_steps, args.warmup, args.precision)))
     if args.fp64_steps > 0 and args.precision == 32:
         legs.append(("fp64", lambda: fp64_leg(sc, total, args.fp64_steps, args.warmup, tiling)))
     if args.accuracy_steps > 0:

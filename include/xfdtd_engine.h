@@ -267,6 +267,21 @@ xf_status xf_engine_set_timing(xf_engine *e, int enabled);
    Default: 64, 4, 8, 0. */
 enum { XF_TILE_GLOBAL_TABLES = 2, XF_TILE_X_FASTEST = 4, XF_TILE_XCD_RUNS = 8, XF_TILE_XCD_COLS = 16 };
 xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags);
+/* Time-step scheme (no counterpart in the reference, whose loop runs
+   updateE and updateH as two sweeps; simulation_update.cpp:9-36).  mode 0
+   (the default): the split E and H kernels of xf_engine_set_tiling.  mode
+   1: one fused kernel per step computes E and then H in a single march over
+   x, reading the step's fields once and writing the next step's into a
+   second copy of the state (fields, psi and the incident line; the engine
+   swaps the copies after each step).  It applies to 3-D engines without
+   design boxes, dispersive media or lumped elements that are not in a
+   batch; others keep the split kernels.  rows: j rows per thread of the
+   fused kernel (1, 2 or 4; 0: 1); planes: x planes per workgroup (1 to 32;
+   0: 8).  mode | 16: tile order by workgroup id instead of one contiguous
+   run of tiles per XCD.  Results are bit-identical for every choice. */
+xf_status xf_engine_set_sweep(xf_engine *e, int mode, int rows, int planes);
+/* 1 when the next step runs the fused kernel, else 0 */
+int xf_engine_sweep_active(const xf_engine *e);
 
 /* Drude cells (DrudeMedium, material/dispersive_material.cpp:118-193, and
    the dispersive branch of Simulation::updateE, simulation_update.cpp:

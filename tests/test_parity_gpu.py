@@ -174,12 +174,13 @@ def test_step_beyond_horizon_is_refused():
                                     (64, 1, 0, 0), (256, 0, 1, 6), (64, 4, 2, 4), (64, 4, 8, 8),
                                     (128, 2, 3, 8), (64, 4, 8, 16), (64, 2, 5, 18)])
 def test_launch_tilings_bitwise(tiling, oracle_threads):
-    """Every launch geometry of the curl kernels (2-D tiles, flat plane runs,
-    global or LDS tables, whole-axis marches, every tile order) gives the
-    same bits."""
+    """Every launch geometry of the split curl kernels (2-D tiles, flat plane
+    runs, global or LDS tables, whole-axis marches, every tile order) gives
+    the same bits."""
     sc = pec_scene(n=34, pml=5, gap=4, radius_cells=6, incidence=(0.5, 0.3, 0.1))
     steps = 70
     g = _gpu(sc, steps)
+    g.set_sweep(0)
     g.set_tiling(*tiling)
     o = OracleSim(sc, steps)
     g.step(steps)

@@ -11,14 +11,17 @@
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <deque>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "engine_kernels.hpp"
+#include "sweep_kernels.hpp"
 #include "xfdtd_engine.h"
 
 using namespace xfk;
@@ -125,8 +128,16 @@ struct xf_engine {
   // of the engine's own stream)
   hipStream_t batch_stream = nullptr;
   // every buffer that is state of the time loop (fields, psi, incident
-  // grid, accumulators): what a checkpoint copies
-  std::vector<std::pair<void *, size_t>> state;
+  // grid, accumulators): what a checkpoint copies.  Buffers that the fused
+  // step double-buffers are named by the engine member that points at the
+  // current copy.
+  struct StateBuf {
+    void *p;
+    void **slot;
+    size_t n;
+    void *ptr() const { return slot ? *slot : p; }
+  };
+  std::vector<StateBuf> state;
   std::vector<std::vector<void *>> ckpt;
   std::vector<long long> ckpt_cur;
   size_t psi_bytes[3] = {0, 0, 0};  // bytes of each psi array per axis
@@ -148,6 +159,22 @@ struct xf_engine {
   void *djp[3] = {nullptr, nullptr, nullptr}, *dep[3] = {nullptr, nullptr, nullptr};  // Lorentz state
   bool ade_non_drude = false;  // a Debye or Lorentz record (no adjoint)
   int dev = 0;                 // HIP device the engine's memory lives on
+  // fused E+H step (k_sweep): off by default (the split kernels are faster
+  // on MI355X); rows of j per thread, x planes per workgroup (0: defaults)
+  int sweep = 0, sweep_rows = 0, sweep_tx = 0;
+  int sweep_flags = 0;  // 1: tile numbers follow workgroup ids (no XCD runs)
+  // second copies of the double-buffered state (allocated at the first
+  // fused step): the fused step reads e->f, psi, the incident line and
+  // writes these, then the two are swapped
+  bool alt_ready = false;
+  void *f_alt[6] = {};
+  void *psi_e_alt[6] = {}, *psi_h_alt[6] = {};
+  // for the fused step both psi sets live in one block each, slots at
+  // psi_off (elements): E slots P_AY..P_CY, then H
+  void *psi_blk = nullptr, *psi_blk_alt = nullptr;
+  long long psi_off[12] = {}, psi_n = 0;
+  void *e_inc_alt = nullptr, *h_inc_alt = nullptr;
+  TfTerms tf_terms{};              // the step's TFSF face terms (fused step)
 };
 
 // wait for everything enqueued for this engine, on its own stream and on
@@ -307,7 +334,7 @@ extern "C" xf_status xf_engine_create(const xf_grid_desc *g, xf_engine **out) {
   for (int q = 0; q < 6; ++q) {
     xf_status s = dev_alloc(e, &e->f[q], (size_t)e->ncell * e->esz);
     if (s != XF_OK) { xf_engine_destroy(e); return s; }
-    e->state.push_back({e->f[q], (size_t)e->ncell * e->esz});
+    e->state.push_back({nullptr, &e->f[q], (size_t)e->ncell * e->esz});
   }
   *out = e;
   return XF_OK;
@@ -420,8 +447,8 @@ extern "C" xf_status xf_engine_set_cpml(xf_engine *e, int axis, int d_n, int d_p
     const int sl = e_slots[axis][q];
     if ((s = dev_alloc(e, &e->psi_e[sl], n * e->esz)) != XF_OK) return s;
     if ((s = dev_alloc(e, &e->psi_h[sl], n * e->esz)) != XF_OK) return s;
-    e->state.push_back({e->psi_e[sl], n * e->esz});
-    e->state.push_back({e->psi_h[sl], n * e->esz});
+    e->state.push_back({nullptr, &e->psi_e[sl], n * e->esz});
+    e->state.push_back({nullptr, &e->psi_h[sl], n * e->esz});
   }
   e->psi_bytes[axis] = n * e->esz;
   return XF_OK;
@@ -477,9 +504,9 @@ extern "C" xf_status xf_engine_set_tfsf(xf_engine *e, const xf_tfsf_desc *d) {
   if ((s = dev_alloc(e, &e->e_inc, (size_t)e->L * sizeof(double))) != XF_OK) return s;
   if ((s = dev_alloc(e, &e->h_inc, (size_t)(e->L - 1) * sizeof(double))) != XF_OK) return s;
   if ((s = dev_alloc(e, &e->ab, 2 * sizeof(double))) != XF_OK) return s;
-  e->state.push_back({e->e_inc, (size_t)e->L * sizeof(double)});
-  e->state.push_back({e->h_inc, (size_t)(e->L - 1) * sizeof(double)});
-  e->state.push_back({e->ab, 2 * sizeof(double)});
+  e->state.push_back({nullptr, &e->e_inc, (size_t)e->L * sizeof(double)});
+  e->state.push_back({nullptr, &e->h_inc, (size_t)(e->L - 1) * sizeof(double)});
+  e->state.push_back({e->ab, nullptr, 2 * sizeof(double)});
   if ((s = upload_f64(e, &e->src, d->source, (size_t)d->total_steps)) != XF_OK) return s;
   e->tfsf_T = d->total_steps;
   set_total(e, d->total_steps);
@@ -524,7 +551,7 @@ extern "C" xf_status xf_engine_set_nffft(xf_engine *e, const xf_nffft_desc *d) {
   void *p;
   if ((s = dev_alloc(e, &p, (size_t)e->nf_size * sizeof(double2))) != XF_OK) return s;
   e->acc = (double2 *)p;
-  e->state.push_back({p, (size_t)e->nf_size * sizeof(double2)});
+  e->state.push_back({p, nullptr, (size_t)e->nf_size * sizeof(double2)});
   const size_t tw = (size_t)d->nf * d->total_steps;
   if ((s = dev_alloc(e, &p, tw * sizeof(double2))) != XF_OK) return s;
   e->tj = (double2 *)p;
@@ -790,7 +817,7 @@ extern "C" xf_status xf_engine_add_inductor(xf_engine *e, int comp, int64_t i0, 
   if ((s = upload_real(e, &sd.coff, cecjc, nc)) != XF_OK) return s;
   if ((s = upload_real(e, &sd.c2, cjcec, nc)) != XF_OK) return s;
   if ((s = dev_alloc(e, &sd.jc, nc * e->esz)) != XF_OK) return s;
-  e->state.push_back({sd.jc, nc * e->esz});
+  e->state.push_back({sd.jc, nullptr, nc * e->esz});
   e->sources.push_back(sd);
   return XF_OK;
 }
@@ -1004,11 +1031,183 @@ static void launch_line_monitors(const xf_engine *e, const Grid<T> &g, hipStream
                          m.buf, step);
 }
 
+// can this engine take the fused E+H step?  It covers Yee, CPML and TFSF;
+// design boxes, dispersive media and lumped elements need the split
+// kernels, and so do 2-D grids and engines in a batch (a batch's launches
+// hold the member's field pointers).
+static int sweep_rows(const xf_engine *e) {
+  if (e->sweep_rows > 0) return e->sweep_rows;
+  return 1;
+}
+
+static SweepTiling make_sweep(const xf_engine *e);
+
+// material slots per LDS record of the fused step (4 or 8; more materials
+// take the split kernels)
+static int sweep_ms(const xf_engine *e) { return e->nmat <= 4 ? 4 : 8; }
+
+// LDS bytes of a fused-step workgroup (its coefficient and CPML tables)
+static size_t sweep_lds_bytes(const xf_engine *e) {
+  const int R = sweep_rows(e), ms = sweep_ms(e);
+  int t;
+  if (ms == 4) t = R == 1 ? SweepLds<1, 4>::total : R == 2 ? SweepLds<2, 4>::total : SweepLds<4, 4>::total;
+  else t = R == 1 ? SweepLds<1, 8>::total : R == 2 ? SweepLds<2, 8>::total : SweepLds<4, 8>::total;
+  return (size_t)t * e->esz;
+}
+
+static bool fused_ok(const xf_engine *e) {
+  return e->sweep == 1 && e->nz > 1 && !e->has_design && !e->has_drude && e->sources.empty() &&
+         e->batch_stream == nullptr && e->nmat <= 8 && make_sweep(e).tx <= SW_TXMAX &&
+         sweep_lds_bytes(e) <= 60 * 1024;
+}
+
+// the second copy of the incident line
+static xf_status alloc_lines_alt(xf_engine *e) {
+  if (!e->has_tfsf || e->e_inc_alt) return XF_OK;
+  xf_status s;
+  if ((s = dev_alloc(e, &e->e_inc_alt, (size_t)e->L * sizeof(double))) != XF_OK) return s;
+  if ((s = dev_alloc(e, &e->h_inc_alt, (size_t)(e->L - 1) * sizeof(double))) != XF_OK) return s;
+  return XF_OK;
+}
+
+static SweepTiling make_sweep(const xf_engine *e) {
+  SweepTiling st;
+  st.waves = SW_WAVES;
+  st.tx = e->sweep_tx > 0 ? e->sweep_tx : 8;
+  const int R = sweep_rows(e);
+  st.nkc = (e->nz + 1 + 63) / 64;
+  st.njg = (e->ny + 1 + st.waves * R - 1) / (st.waves * R);
+  st.nic = (e->nx + 1 + st.tx - 1) / st.tx;
+  st.nblocks = st.nkc * st.njg * st.nic;
+  st.xcd_runs = (e->sweep_flags & 1) ? 0 : 1;
+  return st;
+}
+
+// the second copies of the state the fused step double-buffers, zeroed
+static xf_status alloc_alt(xf_engine *e) {
+  if (e->alt_ready) return XF_OK;
+  xf_status s;
+  for (int q = 0; q < 6; ++q)
+    if ((s = dev_alloc(e, &e->f_alt[q], (size_t)e->ncell * e->esz)) != XF_OK) return s;
+  // both psi sets as one block each (the kernel addresses a slot as block +
+  // offset); the current values move into the first block
+  static const int axis_of[6] = {1, 2, 0, 2, 0, 1};  // ex_y, ex_z, ey_x, ey_z, ez_x, ez_y
+  // every slot the size of the largest slab array, so the kernel finds slot q
+  // at q * slot
+  long long slot = 1;
+  for (int ax = 0; ax < 3; ++ax) slot = std::max(slot, (long long)(e->psi_bytes[ax] / e->esz));
+  for (int q = 0; q < 12; ++q) e->psi_off[q] = q * slot;
+  const long long n = 12 * slot;
+  e->psi_n = slot;
+  if ((s = dev_alloc(e, &e->psi_blk, (size_t)n * e->esz)) != XF_OK) return s;
+  if ((s = dev_alloc(e, &e->psi_blk_alt, (size_t)n * e->esz)) != XF_OK) return s;
+  for (int q = 0; q < 12; ++q) {
+    const size_t nb = e->psi_bytes[axis_of[q % 6]];
+    void **cur = q < 6 ? &e->psi_e[q] : &e->psi_h[q - 6];
+    void **alt = q < 6 ? &e->psi_e_alt[q] : &e->psi_h_alt[q - 6];
+    char *dst = (char *)e->psi_blk + (size_t)e->psi_off[q] * e->esz;
+    if (nb) HIPCHK(hipMemcpyAsync(dst, *cur, nb, hipMemcpyDeviceToDevice, e->stream));
+    // axes without CPML keep their 16-byte placeholders (never addressed)
+    if (nb) {
+      *cur = dst;
+      *alt = (char *)e->psi_blk_alt + (size_t)e->psi_off[q] * e->esz;
+    } else {
+      *alt = *cur;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(e->stream));
+  if (e->has_tfsf) {
+    if ((s = alloc_lines_alt(e)) != XF_OK) return s;
+    // the 24 face-term segments (k_tfsf_terms): sizes over each face's two
+    // in-face index ranges
+    const long long ni = e->tri - e->tli, nj = e->trj - e->tlj, nk = e->trk - e->tlk;
+    TfTerms &w = e->tf_terms;
+    w.stride = 1;
+    for (int q = 0; q < 24; ++q) w.stride = std::max(w.stride, tf_seg_size(q, ni, nj, nk));
+    void *p = nullptr;
+    if ((s = dev_alloc(e, &p, (size_t)(24 * w.stride) * sizeof(double))) != XF_OK) return s;
+    w.buf = (double *)p;
+  }
+  e->alt_ready = true;
+  return XF_OK;
+}
+
+static void swap_sets(xf_engine *e) {
+  for (int q = 0; q < 6; ++q) {
+    std::swap(e->f[q], e->f_alt[q]);
+    std::swap(e->psi_e[q], e->psi_e_alt[q]);
+    std::swap(e->psi_h[q], e->psi_h_alt[q]);
+  }
+  std::swap(e->psi_blk, e->psi_blk_alt);
+  if (e->has_tfsf) {
+    std::swap(e->e_inc, e->e_inc_alt);
+    std::swap(e->h_inc, e->h_inc_alt);
+  }
+}
+
+// SweepArgs of the two directions: block d reads the set that is current
+// after an even (d = 0) or odd (d = 1) number of fused steps and writes the
+// other; rebuilt at every stepping call (a few hundred bytes)
+// arguments of the fused step from the current set to the other one
+template <typename T>
+static SweepArgs<T> sweep_args(xf_engine *e, const SweepTiling &st) {
+  SweepArgs<T> h{};
+  for (int q = 0; q < 6; ++q) {
+    h.src[q] = (const T *)e->f[q];
+    h.dst[q] = (T *)e->f_alt[q];
+  }
+  h.mat = e->mat;
+  for (int side = 0; side < 2; ++side) {
+    h.self[side] = (const T *)e->self[side];
+    h.kt[side] = (const T *)e->kt[side];
+    h.jt[side] = (const T *)e->jt[side];
+    h.it[side] = (const T *)e->it[side];
+    h.kp[side] = (const T *)e->kp[side];
+    h.jp[side] = (const T *)e->jp[side];
+    h.ip[side] = (const T *)e->ip[side];
+  }
+  for (int ax = 0; ax < 3; ++ax) {
+    h.dn[ax] = e->dn[ax]; h.dp[ax] = e->dp[ax];
+    h.be[ax] = (const T *)e->be[ax]; h.ae[ax] = (const T *)e->ae[ax];
+    h.bm[ax] = (const T *)e->bm[ax]; h.am[ax] = (const T *)e->am[ax];
+  }
+  h.psiA = (const T *)e->psi_blk;
+  h.psiB = (T *)e->psi_blk_alt;
+  h.psi_slot = (unsigned)e->psi_n;
+  h.terms = e->tf_terms.buf;
+  h.term_stride = (unsigned)e->tf_terms.stride;
+  h.nx = e->nx; h.ny = e->ny; h.nz = e->nz; h.pz = e->pz; h.nmat = e->nmat;
+  h.sj = (unsigned)e->sj; h.si = (unsigned)e->si;
+  h.tf = e->has_tfsf ? 1 : 0;
+  if (e->has_tfsf) {
+    h.tb[0] = e->tli; h.tb[1] = e->tri; h.tb[2] = e->tlj; h.tb[3] = e->trj; h.tb[4] = e->tlk; h.tb[5] = e->trk;
+  }
+  h.st = st;
+  return h;
+}
+
+template <typename T, int R>
+static void launch_sweep(xf_engine *e, const SweepTiling &st) {
+  const SweepArgs<T> a = sweep_args<T>(e, st);
+  const dim3 grd(st.nblocks), blk(64 * st.waves);
+  const size_t lds = sweep_lds_bytes(e);
+  if (sweep_ms(e) == 4)
+    hipLaunchKernelGGL((k_sweep<T, R, 4>), grd, blk, lds, e->stream, a);
+  else
+    hipLaunchKernelGGL((k_sweep<T, R, 8>), grd, blk, lds, e->stream, a);
+}
+
 template <typename T>
 static xf_status run_steps(xf_engine *e, long long n) {
-  Grid<T> g = make_grid<T>(e);
-  Tfsf t{};
-  if (e->has_tfsf) t = make_tfsf(e);
+  const bool fused = fused_ok(e);
+  const SweepTiling sw = fused ? make_sweep(e) : SweepTiling{};
+  if (fused) {
+    xf_status s = alloc_alt(e);
+    if (s != XF_OK) return s;
+  }
+  const int R = sweep_rows(e);
+  if (fused && R != 1 && R != 2 && R != 4)
+    FAILS(XF_EINVAL, "fused step with %d rows per thread (1, 2 or 4)", R);
   const Tiling tl = make_tiling(e);
   dim3 blk(tiling_threads(tl));
   dim3 grd(tl.nblocks);
@@ -1045,36 +1244,70 @@ static xf_status run_steps(xf_engine *e, long long n) {
       ev = &e->ev[e->ev_used];
       e->ev_used += 4;
     }
-    if (ev) hipEventRecord(ev[0], e->stream);
-    if (special) {
-      if (use_lds)
-        hipLaunchKernelGGL((k_update_e<T, true, true>), grd, blk, lds, e->stream, g, tl);
+    if (fused) {
+      // IFA into the second copy of the line, then the whole step in one
+      // kernel; the event pair brackets the fused kernel, the H pair is empty
+      if (e->has_tfsf) {
+        const Tfsf t0 = make_tfsf(e);
+        hipLaunchKernelGGL(k_tfsf_ifa_pp, dim3(1), dim3(1024), 0, e->stream, t0, (double *)e->e_inc_alt,
+                           (double *)e->h_inc_alt, step);
+        // face terms: E from the H line before the IFA, H from the new E line
+        Tfsf t1 = t0;
+        t1.e_inc = (double *)e->e_inc_alt;
+        hipLaunchKernelGGL(k_tfsf_terms, dim3((unsigned)((e->tf_terms.stride + 255) / 256), 24), dim3(256), 0,
+                           e->stream, t1, e->tf_terms);
+      }
+      if (ev) hipEventRecord(ev[0], e->stream);
+      if (R == 1)
+        launch_sweep<T, 1>(e, sw);
+      else if (R == 2)
+        launch_sweep<T, 2>(e, sw);
       else
-        hipLaunchKernelGGL((k_update_e<T, false, true>), grd, blk, 0, e->stream, g, tl);
+        launch_sweep<T, 4>(e, sw);
+      if (ev) {
+        hipEventRecord(ev[1], e->stream);
+        hipEventRecord(ev[2], e->stream);
+        hipEventRecord(ev[3], e->stream);
+      }
+      swap_sets(e);
     } else {
+      const Grid<T> g = make_grid<T>(e);
+      Tfsf t{};
+      if (e->has_tfsf) t = make_tfsf(e);
+      if (ev) hipEventRecord(ev[0], e->stream);
+      if (special) {
+        if (use_lds)
+          hipLaunchKernelGGL((k_update_e<T, true, true>), grd, blk, lds, e->stream, g, tl);
+        else
+          hipLaunchKernelGGL((k_update_e<T, false, true>), grd, blk, 0, e->stream, g, tl);
+      } else {
+        if (use_lds)
+          hipLaunchKernelGGL((k_update_e<T, true, false>), grd, blk, lds, e->stream, g, tl);
+        else
+          hipLaunchKernelGGL((k_update_e<T, false, false>), grd, blk, 0, e->stream, g, tl);
+      }
+      if (ev) hipEventRecord(ev[1], e->stream);
+      for (const SourceDev &sd : e->sources) launch_lumped<T>(e, sd, g.f[sd.comp], e->stream, step);
+      if (e->has_tfsf) {
+        const int tot = e->nEy + e->nEz + e->nEx;
+        hipLaunchKernelGGL(k_tfsf_e<T>, dim3((tot + 255) / 256), dim3(256), 0, e->stream, g, t, e->nEy,
+                           e->nEz, e->nEx);
+      }
+      if (ev) hipEventRecord(ev[2], e->stream);
       if (use_lds)
-        hipLaunchKernelGGL((k_update_e<T, true, false>), grd, blk, lds, e->stream, g, tl);
+        hipLaunchKernelGGL((k_update_h<T, true>), grd, blk, lds, e->stream, g, tl);
       else
-        hipLaunchKernelGGL((k_update_e<T, false, false>), grd, blk, 0, e->stream, g, tl);
+        hipLaunchKernelGGL((k_update_h<T, false>), grd, blk, 0, e->stream, g, tl);
+      if (ev) hipEventRecord(ev[3], e->stream);
+      if (e->has_tfsf) {
+        hipLaunchKernelGGL(k_tfsf_ifa, dim3(1), dim3(1024), 0, e->stream, t, step);
+        const int tot = e->hseg.off[12];
+        hipLaunchKernelGGL(k_tfsf_h<T>, dim3((tot + 255) / 256), dim3(256), 0, e->stream, g, t, e->hseg);
+      }
     }
-    if (ev) hipEventRecord(ev[1], e->stream);
-    for (const SourceDev &sd : e->sources) launch_lumped<T>(e, sd, g.f[sd.comp], e->stream, step);
-    if (e->has_tfsf) {
-      const int tot = e->nEy + e->nEz + e->nEx;
-      hipLaunchKernelGGL(k_tfsf_e<T>, dim3((tot + 255) / 256), dim3(256), 0, e->stream, g, t, e->nEy,
-                         e->nEz, e->nEx);
-    }
-    if (ev) hipEventRecord(ev[2], e->stream);
-    if (use_lds)
-      hipLaunchKernelGGL((k_update_h<T, true>), grd, blk, lds, e->stream, g, tl);
-    else
-      hipLaunchKernelGGL((k_update_h<T, false>), grd, blk, 0, e->stream, g, tl);
-    if (ev) hipEventRecord(ev[3], e->stream);
-    if (e->has_tfsf) {
-      hipLaunchKernelGGL(k_tfsf_ifa, dim3(1), dim3(1024), 0, e->stream, t, step);
-      const int tot = e->hseg.off[12];
-      hipLaunchKernelGGL(k_tfsf_h<T>, dim3((tot + 255) / 256), dim3(256), 0, e->stream, g, t, e->hseg);
-    }
+    // after the H half-step: NF-FF, DFT plane, broadband series, probes and
+    // line monitors read the step's fields
+    const Grid<T> g = make_grid<T>(e);
     if (e->has_nf) {
       if (e->nf_pending == 0) e->nf_step0 = step;
       hipLaunchKernelGGL(k_nffft_sample<T>, dim3((unsigned)((nfQ + 255) / 256)), dim3(256), 0, e->stream, g, e->nf,
@@ -1237,6 +1470,24 @@ extern "C" xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, 
   e->tile_flags = flags;
   return XF_OK;
 }
+
+extern "C" xf_status xf_engine_set_sweep(xf_engine *e, int mode, int rows, int planes) {
+  if (!e) FAILS(XF_EINVAL, "null engine");
+  const int flags = mode >> 4;
+  mode &= 15;
+  if (mode < 0 || mode > 1 || !(rows == 0 || rows == 1 || rows == 2 || rows == 4) || planes < 0 || planes > 4096 ||
+      (flags & ~1) != 0)
+    FAILS(XF_EINVAL, "sweep mode %d, %d rows, %d planes: need mode 0 or 1, rows 0, 1, 2 or 4, planes >= 0", mode,
+          rows, planes);
+  HIPCHK(engine_wait(e));
+  e->sweep_flags = flags;
+  e->sweep = mode;
+  e->sweep_rows = rows;
+  e->sweep_tx = planes;
+  return XF_OK;
+}
+
+extern "C" int xf_engine_sweep_active(const xf_engine *e) { return e && fused_ok(e) ? 1 : 0; }
 
 // ---------------------------------------------------------------------------
 // Batches: many engines of one precision stepped together.  Each per-step
@@ -1521,12 +1772,12 @@ extern "C" xf_status xf_engine_set_dispersive(xf_engine *e, const int64_t *box, 
   if ((s = upload_real(e, &e->drc, coef, XF_ADE_STRIDE * (size_t)e->nmat)) != XF_OK) return s;
   for (int q = 0; q < 3; ++q) {
     if ((s = dev_alloc(e, &e->dj[q], n * e->esz)) != XF_OK) return s;
-    e->state.push_back({e->dj[q], n * e->esz});
+    e->state.push_back({e->dj[q], nullptr, n * e->esz});
     if (lorentz) {
       if ((s = dev_alloc(e, &e->djp[q], n * e->esz)) != XF_OK) return s;
       if ((s = dev_alloc(e, &e->dep[q], n * e->esz)) != XF_OK) return s;
-      e->state.push_back({e->djp[q], n * e->esz});
-      e->state.push_back({e->dep[q], n * e->esz});
+      e->state.push_back({e->djp[q], nullptr, n * e->esz});
+      e->state.push_back({e->dep[q], nullptr, n * e->esz});
     }
   }
   for (int a = 0; a < 6; ++a) e->xb[a] = (int)box[a];
@@ -1614,7 +1865,7 @@ extern "C" xf_status xf_engine_set_dft_plane(xf_engine *e, int comp, int axis, i
   void *p;
   if ((s = dev_alloc(e, &p, n * sizeof(double2))) != XF_OK) return s;
   d.acc = (double2 *)p;
-  e->state.push_back({p, n * sizeof(double2)});
+  e->state.push_back({p, nullptr, n * sizeof(double2)});
   if ((s = dev_alloc(e, &p, (size_t)total_steps * sizeof(double2))) != XF_OK) return s;
   HIPCHK(hipMemcpy(p, w, (size_t)total_steps * sizeof(double2), hipMemcpyHostToDevice));
   d.w = (const double2 *)p;
@@ -1638,18 +1889,18 @@ static xf_status state_save(xf_engine *e, std::vector<void *> &dst, hipStream_t 
   if (dst.empty()) {
     for (auto &b : e->state) {
       void *p = nullptr;
-      HIPCHK(hipMalloc(&p, b.second));
+      HIPCHK(hipMalloc(&p, b.n));
       dst.push_back(p);
     }
   }
   for (size_t q = 0; q < e->state.size(); ++q)
-    HIPCHK(hipMemcpyAsync(dst[q], e->state[q].first, e->state[q].second, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(dst[q], e->state[q].ptr(), e->state[q].n, hipMemcpyDeviceToDevice, st));
   return XF_OK;
 }
 static xf_status state_load(xf_engine *e, const std::vector<void *> &src, hipStream_t st = nullptr) {
   if (!st) st = e->stream;
   for (size_t q = 0; q < e->state.size(); ++q)
-    HIPCHK(hipMemcpyAsync(e->state[q].first, src[q], e->state[q].second, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->state[q].ptr(), src[q], e->state[q].n, hipMemcpyDeviceToDevice, st));
   return XF_OK;
 }
 static void state_free(std::vector<void *> &v) {

@@ -360,6 +360,15 @@ class Simulation:
             raise RuntimeError(L.xf_last_error().decode())
         return a.value, b.value, n.value
 
+    def set_sweep(self, mode=1, rows=0, planes=0):
+        """Time-step scheme (xf_engine_set_sweep): 0 = split E and H kernels
+        (the default), 1 = the fused E+H kernel where the scene allows it."""
+        self._chk(lib().xf_engine_set_sweep(self.engine, int(mode), int(rows), int(planes)))
+
+    @property
+    def sweep_active(self):
+        return bool(lib().xf_engine_sweep_active(self.engine))
+
     def set_tiling(self, tz, ty, tx, flags=0):
         L = lib()
         if L.xf_engine_set_tiling(self.engine, int(tz), int(ty), int(tx), int(flags)) != 0:
