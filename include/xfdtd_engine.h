@@ -283,6 +283,16 @@ xf_status xf_engine_set_sweep(xf_engine *e, int mode, int rows, int planes);
 /* 1 when the next step runs the fused kernel, else 0 */
 int xf_engine_sweep_active(const xf_engine *e);
 
+/* Divergence check (no counterpart in the reference, whose time loop runs
+   on whatever the fields become; simulation_update.cpp:9-36).  The H kernels flag the
+   first step whose H comes out non-finite (NaN or infinite); every `every`
+   steps (default 100, 0: never) xf_engine_step and xf_batch_step read the
+   flag, a sync point, and if it is set return XF_ESTATE naming that step.
+   The engine then refuses to step until xf_engine_restore. */
+xf_status xf_engine_set_divergence_check(xf_engine *e, int every);
+/* the step at which non-finite values were detected, or -1 */
+int64_t xf_engine_diverged_step(const xf_engine *e);
+
 /* Drude cells (DrudeMedium, material/dispersive_material.cpp:118-193, and
    the dispersive branch of Simulation::updateE, simulation_update.cpp:
    221-280).  box[6] = [i0,i1)x[j0,j1)x[k0,k1) holds every Drude cell and lies

@@ -22,7 +22,7 @@ ENGINE_SYMBOLS = [
     "xf_engine_add_probe", "xf_engine_get_probes", "xf_engine_step", "xf_engine_sync",
     "xf_engine_current_step", "xf_engine_get_field", "xf_engine_set_field",
     "xf_engine_kernel_times", "xf_engine_set_timing", "xf_engine_set_tiling", "xf_engine_set_sweep",
-    "xf_engine_sweep_active",
+    "xf_engine_sweep_active", "xf_engine_set_divergence_check", "xf_engine_diverged_step",
     "xf_batch_create", "xf_batch_step", "xf_batch_sync", "xf_batch_size", "xf_batch_destroy", "xf_batch_grad_eps",
     "xf_engine_set_design", "xf_engine_set_dft_plane", "xf_engine_get_dft", "xf_engine_checkpoint",
     "xf_engine_restore", "xf_engine_grad_eps", "xf_engine_adjoint_run", "xf_engine_set_drude",
@@ -101,6 +101,9 @@ def lib():
     L.xf_engine_set_tiling.argtypes = [vp, i, i, i, i]
     L.xf_engine_set_sweep.argtypes = [vp, i, i, i]
     L.xf_engine_sweep_active.argtypes = [vp]
+    L.xf_engine_set_divergence_check.argtypes = [vp, C.c_int]
+    L.xf_engine_diverged_step.argtypes = [vp]
+    L.xf_engine_diverged_step.restype = i64
     L.xf_engine_sync.argtypes = [vp]
     L.xf_engine_step.argtypes = [vp, i64]
     L.xf_engine_current_step.restype = i64

@@ -163,6 +163,12 @@ struct xf_engine {
   // on MI355X); rows of j per thread, x planes per workgroup (0: defaults)
   int sweep = 0, sweep_rows = 0, sweep_tx = 0;
   int sweep_flags = 0;  // 1: tile numbers follow workgroup ids (no XCD runs)
+  // divergence check: the H kernels lower *div to the first step whose H
+  // came out non-finite; the host reads it every div_every steps (0: never)
+  // and then refuses to step on (div_step >= 0) until a restore
+  void *div = nullptr;
+  int div_every = 100;
+  long long div_step = -1;
   // second copies of the double-buffered state (allocated at the first
   // fused step): the fused step reads e->f, psi, the incident line and
   // writes these, then the two are swapped
@@ -284,6 +290,29 @@ static void comp_shape(const xf_engine *e, int comp, int sh[3]) {
   }
 }
 
+// clear the divergence flag (a blocking copy: no memset node for a stream
+// that may be captured)
+static xf_status reset_divergence(xf_engine *e) {
+  const unsigned long long none = ~0ull;
+  HIPCHK(hipMemcpy(e->div, &none, sizeof none, hipMemcpyHostToDevice));
+  e->div_step = -1;
+  return XF_OK;
+}
+
+// read the divergence flag after the work enqueued on `st` (a sync point)
+static xf_status check_divergence(xf_engine *e, hipStream_t st, int member = -1) {
+  unsigned long long h = ~0ull;
+  HIPCHK(hipMemcpyAsync(&h, e->div, sizeof h, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (h == ~0ull) return XF_OK;
+  e->div_step = (long long)h;
+  if (member >= 0)
+    FAILS(XF_ESTATE, "batch member %d: non-finite H values at step %lld, the run diverged (time step above the "
+          "Courant limit or an unstable medium)", member, e->div_step);
+  FAILS(XF_ESTATE, "non-finite H values at step %lld, the run diverged (time step above the Courant limit or an "
+        "unstable medium)", e->div_step);
+}
+
 extern "C" xf_status xf_engine_create(const xf_grid_desc *g, xf_engine **out) {
   if (!g || !out) FAILS(XF_EINVAL, "null argument");
   *out = nullptr;
@@ -317,8 +346,11 @@ extern "C" xf_status xf_engine_create(const xf_grid_desc *g, xf_engine **out) {
   e->si = (long long)(e->ny + 1) * e->pz;
   e->ncell = (long long)(e->nx + 1) * e->si;
   // the curl kernels address a field with a 32-bit byte offset
-  if ((unsigned long long)e->ncell * e->esz >= (1ULL << 32))
-    FAILS(XF_EINVAL, "grid of %lld padded cells exceeds 4 GiB per field component", e->ncell);
+  if ((unsigned long long)e->ncell * e->esz >= (1ULL << 32)) {
+    const long long nc = e->ncell;
+    delete e;
+    FAILS(XF_EINVAL, "grid of %lld padded cells exceeds 4 GiB per field component", nc);
+  }
   if (hipGetDevice(&e->dev) != hipSuccess) {
     delete e;
     FAILS(XF_EDEVICE, "hipGetDevice failed");
@@ -336,6 +368,9 @@ extern "C" xf_status xf_engine_create(const xf_grid_desc *g, xf_engine **out) {
     if (s != XF_OK) { xf_engine_destroy(e); return s; }
     e->state.push_back({nullptr, &e->f[q], (size_t)e->ncell * e->esz});
   }
+  xf_status s = dev_alloc(e, &e->div, sizeof(unsigned long long));
+  if (s == XF_OK) s = reset_divergence(e);
+  if (s != XF_OK) { xf_engine_destroy(e); return s; }
   *out = e;
   return XF_OK;
 }
@@ -948,7 +983,8 @@ static Grid<T> make_grid(xf_engine *e) {
     g.dj[q] = (T *)e->dj[q];
     g.djp[q] = (T *)e->djp[q];
     g.dep[q] = (T *)e->dep[q];
-  }
+  }  g.div = (unsigned long long *)e->div;
+
   return g;
 }
 
@@ -1183,6 +1219,8 @@ static SweepArgs<T> sweep_args(xf_engine *e, const SweepTiling &st) {
     h.tb[0] = e->tli; h.tb[1] = e->tri; h.tb[2] = e->tlj; h.tb[3] = e->trj; h.tb[4] = e->tlk; h.tb[5] = e->trk;
   }
   h.st = st;
+  h.div = (unsigned long long *)e->div;
+  h.step = e->cur;
   return h;
 }
 
@@ -1295,9 +1333,9 @@ static xf_status run_steps(xf_engine *e, long long n) {
       }
       if (ev) hipEventRecord(ev[2], e->stream);
       if (use_lds)
-        hipLaunchKernelGGL((k_update_h<T, true>), grd, blk, lds, e->stream, g, tl);
+        hipLaunchKernelGGL((k_update_h<T, true>), grd, blk, lds, e->stream, g, tl, step);
       else
-        hipLaunchKernelGGL((k_update_h<T, false>), grd, blk, 0, e->stream, g, tl);
+        hipLaunchKernelGGL((k_update_h<T, false>), grd, blk, 0, e->stream, g, tl, step);
       if (ev) hipEventRecord(ev[3], e->stream);
       if (e->has_tfsf) {
         hipLaunchKernelGGL(k_tfsf_ifa, dim3(1), dim3(1024), 0, e->stream, t, step);
@@ -1332,6 +1370,11 @@ static xf_status run_steps(xf_engine *e, long long n) {
     launch_line_monitors<T>(e, g, e->stream, step);
     HIPCHK(hipGetLastError());
     e->cur++;
+    if (e->div_every > 0 && e->cur % e->div_every == 0) {
+      nf_flush();
+      xf_status s = check_divergence(e, e->stream);
+      if (s != XF_OK) return s;
+    }
   }
   nf_flush();
   HIPCHK(hipGetLastError());
@@ -1342,6 +1385,8 @@ extern "C" xf_status xf_engine_step(xf_engine *e, int64_t n) {
   if (!e) FAILS(XF_EINVAL, "null engine");
   if (!e->materials_set) FAILS(XF_ESTATE, "materials not set");
   if (n < 0) FAILS(XF_EINVAL, "negative step count");
+  if (e->div_step >= 0)
+    FAILS(XF_ESTATE, "the run diverged at step %lld; restore a checkpoint to go on", e->div_step);
   if (e->total >= 0 && e->cur + n > e->total)
     FAILS(XF_ESTATE, "step %lld + %lld beyond the %lld steps the source tables hold", e->cur, (long long)n,
           e->total);
@@ -1488,6 +1533,15 @@ extern "C" xf_status xf_engine_set_sweep(xf_engine *e, int mode, int rows, int p
 }
 
 extern "C" int xf_engine_sweep_active(const xf_engine *e) { return e && fused_ok(e) ? 1 : 0; }
+
+extern "C" xf_status xf_engine_set_divergence_check(xf_engine *e, int every) {
+  if (!e) FAILS(XF_EINVAL, "null engine");
+  if (every < 0) FAILS(XF_EINVAL, "divergence check every %d steps: need >= 0", every);
+  e->div_every = every;
+  return XF_OK;
+}
+
+extern "C" int64_t xf_engine_diverged_step(const xf_engine *e) { return e ? e->div_step : -1; }
 
 // ---------------------------------------------------------------------------
 // Batches: many engines of one precision stepped together.  Each per-step
@@ -1652,9 +1706,9 @@ static xf_status batch_run(xf_batch *b, long long n) {
       for (const SourceDev &sd : e->sources) launch_lumped<T>(e, sd, (T *)e->f[sd.comp], b->stream, step);
     if (b->n_te > 0) hipLaunchKernelGGL(kb_tfsf_e<T>, dim3(b->n_te), dim3(256), 0, b->stream, in, b->map_te);
     if (b->use_lds)
-      hipLaunchKernelGGL((kb_update_h<T, true>), dim3(b->n_curl), blk, b->lds, b->stream, in, b->map_curl);
+      hipLaunchKernelGGL((kb_update_h<T, true>), dim3(b->n_curl), blk, b->lds, b->stream, in, b->map_curl, step);
     else
-      hipLaunchKernelGGL((kb_update_h<T, false>), dim3(b->n_curl), blk, 0, b->stream, in, b->map_curl);
+      hipLaunchKernelGGL((kb_update_h<T, false>), dim3(b->n_curl), blk, 0, b->stream, in, b->map_curl, step);
     if (b->n_ifa > 0) {
       hipLaunchKernelGGL(kb_tfsf_ifa<T>, dim3(b->n_ifa), dim3(1024), 0, b->stream, in, b->map_ifa, step);
       hipLaunchKernelGGL(kb_tfsf_h<T>, dim3(b->n_th), dim3(256), 0, b->stream, in, b->map_th);
@@ -1671,6 +1725,12 @@ static xf_status batch_run(xf_batch *b, long long n) {
       if (!e->line_mons.empty()) launch_line_monitors<T>(e, make_grid<T>(e), b->stream, step);
     HIPCHK(hipGetLastError());
     for (xf_engine *e : b->e) e->cur++;
+    const int every = b->e[0]->div_every;
+    if (every > 0 && b->e[0]->cur % every == 0)
+      for (size_t m = 0; m < b->e.size(); ++m) {
+        xf_status s = check_divergence(b->e[m], b->stream, (int)m);
+        if (s != XF_OK) return s;
+      }
   }
   return XF_OK;
 }
@@ -1681,6 +1741,8 @@ extern "C" xf_status xf_batch_step(xf_batch *b, int64_t n) {
   for (size_t q = 0; q < b->e.size(); ++q) {
     xf_engine *e = b->e[q];
     if (e->cur != b->e[0]->cur) FAILS(XF_ESTATE, "engine %zu was stepped outside the batch", q);
+    if (e->div_step >= 0)
+      FAILS(XF_ESTATE, "engine %zu diverged at step %lld; restore a checkpoint to go on", q, e->div_step);
     if (e->total >= 0 && e->cur + n > e->total)
       FAILS(XF_ESTATE, "engine %zu: step %lld + %lld beyond the %lld steps its tables hold", q, e->cur,
             (long long)n, e->total);
@@ -1929,7 +1991,7 @@ extern "C" xf_status xf_engine_restore(xf_engine *e, int slot) {
   xf_status s = state_load(e, e->ckpt[slot]);
   if (s != XF_OK) return s;
   e->cur = e->ckpt_cur[slot];
-  return XF_OK;
+  return reset_divergence(e);
 }
 
 // device buffers of one adjoint run

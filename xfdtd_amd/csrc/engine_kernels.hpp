@@ -56,6 +56,9 @@ struct Grid {
   const T *drc;
   T *dj[3];
   T *djp[3], *dep[3];  // Lorentz: the previous step's J and E (null without Lorentz media)
+  // divergence flag: the H kernel lowers it to the index of a step whose H
+  // came out non-finite (all ones while every value is finite)
+  unsigned long long *div;
 };
 
 template <typename T>
@@ -684,7 +687,7 @@ __device__ __forceinline__ void load_plane_h(const Grid<T> &g, const Col<T> &q, 
 }
 
 template <typename T, bool SLAB>
-__device__ __forceinline__ void compute_plane_h(const Grid<T> &g, const TileTables<T> &tt, const Col<T> &q,
+__device__ __forceinline__ bool compute_plane_h(const Grid<T> &g, const TileTables<T> &tt, const Col<T> &q,
                                                 const Plane<T> &p, int i, unsigned cell, T ez_c, T ey_c) {
   const int nx = g.nx, j = q.j, k = q.k;
   const bool kin = q.kin, jin = q.jin;
@@ -755,10 +758,12 @@ __device__ __forceinline__ void compute_plane_h(const Grid<T> &g, const TileTabl
   sto(q.u0, o, vx);
   sto(q.u1, o, vy);
   sto(q.u2, o, vz);
+  // non-finite output: the run diverged
+  return !(__builtin_isfinite(vx) && __builtin_isfinite(vy) && __builtin_isfinite(vz));
 }
 
 template <typename T, bool LDS>
-__device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl, int block) {
+__device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl, int block, long long step) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   int bz, by, bx;
   if (!tile_of_block(tl, block, bz, by, bx)) return;
@@ -805,11 +810,12 @@ __device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl
     tt = global_tables<T>(g, 1, cell);
   }
   if (!live) return;
+  bool bad = false;
   for (int i = i0;;) {
     if (slab)
-      compute_plane_h<T, true>(g, tt, q, A, i, c, ez_c, ey_c);
+      bad |= compute_plane_h<T, true>(g, tt, q, A, i, c, ez_c, ey_c);
     else
-      compute_plane_h<T, false>(g, tt, q, A, i, c, ez_c, ey_c);
+      bad |= compute_plane_h<T, false>(g, tt, q, A, i, c, ez_c, ey_c);
     ez_c = A.n4;
     ey_c = A.n5;
     if (++i >= i1) break;
@@ -821,6 +827,7 @@ __device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl
     else
       load_plane_h<T, false>(g, q, c, i, di, A);
   }
+  if (bad) atomicMin(g.div, (unsigned long long)step);
 }
 
 template <typename T, bool LDS, bool SPECIAL>
@@ -828,8 +835,8 @@ __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl) {
   update_e_body<T, LDS, SPECIAL>(g, tl, blockIdx.x);
 }
 template <typename T, bool LDS>
-__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl) {
-  update_h_body<T, LDS>(g, tl, blockIdx.x);
+__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl, long long step) {
+  update_h_body<T, LDS>(g, tl, blockIdx.x, step);
 }
 
 // ---------------------------------------------------------------------------
@@ -1794,9 +1801,10 @@ __global__ void __launch_bounds__(256) kb_update_e(const Inst<T> *__restrict__ i
   update_e_body<T, LDS, SPECIAL>(in[w.x].g, in[w.x].tl, w.y);
 }
 template <typename T, bool LDS>
-__global__ void __launch_bounds__(256) kb_update_h(const Inst<T> *__restrict__ in, const int2 *__restrict__ map) {
+__global__ void __launch_bounds__(256) kb_update_h(const Inst<T> *__restrict__ in, const int2 *__restrict__ map,
+                                                   long long step) {
   const int2 w = map[blockIdx.x];
-  update_h_body<T, LDS>(in[w.x].g, in[w.x].tl, w.y);
+  update_h_body<T, LDS>(in[w.x].g, in[w.x].tl, w.y, step);
 }
 template <typename T>
 __global__ void kb_tfsf_e(const Inst<T> *__restrict__ in, const int2 *__restrict__ map) {

@@ -105,6 +105,8 @@ struct SweepArgs {
   int dn[3], dp[3];
   int tf, tb[6];  // TFSF box li, ri, lj, rj, lk, rk
   SweepTiling st;
+  unsigned long long *div;  // lowered to `step` where an H value comes out non-finite
+  long long step;
 };
 
 // DPP lane moves over the whole wave: every lane takes `src` of its
@@ -459,6 +461,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &a, int block) {
       }
     }
   }
+  bool bad = false;  // a non-finite H value was stored
   // i = i0 - 1 only computes E(i0); from i0 on, each pass computes E(i + 1)
   // and H(i) and stores plane i
   for (int i = i0 - 1; i < i1; ++i) {
@@ -648,6 +651,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &a, int block) {
           }
         }
         if (own) {
+          bad |= !(__builtin_isfinite(vx) && __builtin_isfinite(vy) && __builtin_isfinite(vz));
           const unsigned o = (pbase + roff[r + 1] + kl) * (unsigned)sizeof(T);
           sto(a.dst[EX], o, exv);
           sto(a.dst[EY], o, eyv);
@@ -675,6 +679,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &a, int block) {
       }
     }
   }
+  if (bad) atomicMin(a.div, (unsigned long long)a.step);
 }
 
 template <typename T, int R, int MS>

@@ -165,6 +165,11 @@ class Simulation:
         if rc != 0:
             raise RuntimeError("xfdtd_amd: " + lib().xf_sim_last_error().decode())
 
+    def _echk(self, rc):
+        """Status of a direct engine call (xf_engine_*)."""
+        if rc != 0:
+            raise RuntimeError("xfdtd_amd engine: " + lib().xf_last_error().decode())
+
     def close(self):
         if getattr(self, "h", None):
             lib().xf_sim_destroy(self.h)
@@ -363,11 +368,22 @@ class Simulation:
     def set_sweep(self, mode=1, rows=0, planes=0):
         """Time-step scheme (xf_engine_set_sweep): 0 = split E and H kernels
         (the default), 1 = the fused E+H kernel where the scene allows it."""
-        self._chk(lib().xf_engine_set_sweep(self.engine, int(mode), int(rows), int(planes)))
+        self._echk(lib().xf_engine_set_sweep(self.engine, int(mode), int(rows), int(planes)))
 
     @property
     def sweep_active(self):
         return bool(lib().xf_engine_sweep_active(self.engine))
+
+    def set_divergence_check(self, every):
+        """Read the device's non-finite flag every `every` steps (0: never);
+        a diverged run makes step() raise, naming the step."""
+        self._echk(lib().xf_engine_set_divergence_check(self.engine, int(every)))
+
+    @property
+    def diverged_step(self):
+        """The step at which non-finite H values were detected, or None."""
+        s = lib().xf_engine_diverged_step(self.engine)
+        return None if s < 0 else int(s)
 
     def set_tiling(self, tz, ty, tx, flags=0):
         L = lib()
