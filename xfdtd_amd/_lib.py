@@ -101,6 +101,11 @@ def lib():
     L.xf_engine_set_tiling.argtypes = [vp, i, i, i, i]
     L.xf_engine_set_sweep.argtypes = [vp, i, i, i]
     L.xf_engine_sweep_active.argtypes = [vp]
+    L.xf_engine_get_box.argtypes = [vp, i, C.POINTER(i64), dp]
+    L.xf_engine_get_field.argtypes = [vp, i, dp]
+    L.xf_engine_set_field.argtypes = [vp, i, dp]
+    L.xf_engine_checkpoint.argtypes = [vp, i]
+    L.xf_engine_restore.argtypes = [vp, i]
     L.xf_engine_set_divergence_check.argtypes = [vp, C.c_int]
     L.xf_engine_diverged_step.argtypes = [vp]
     L.xf_engine_diverged_step.restype = i64

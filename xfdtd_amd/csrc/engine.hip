@@ -345,11 +345,12 @@ extern "C" xf_status xf_engine_create(const xf_grid_desc *g, xf_engine **out) {
   e->sj = e->pz;
   e->si = (long long)(e->ny + 1) * e->pz;
   e->ncell = (long long)(e->nx + 1) * e->si;
-  // the curl kernels address a field with a 32-bit byte offset
-  if ((unsigned long long)e->ncell * e->esz >= (1ULL << 32)) {
-    const long long nc = e->ncell;
+  // a curl-kernel tile addresses its planes with 32-bit byte offsets from a
+  // 64-bit base at its first plane: it needs two y-z planes under 4 GiB
+  if (2ULL * (unsigned long long)e->si * e->esz >= (1ULL << 32)) {
+    const long long ps = e->si;
     delete e;
-    FAILS(XF_EINVAL, "grid of %lld padded cells exceeds 4 GiB per field component", nc);
+    FAILS(XF_EINVAL, "a y-z plane of %lld padded cells: two planes exceed the 4 GiB a tile addresses", ps);
   }
   if (hipGetDevice(&e->dev) != hipSuccess) {
     delete e;
@@ -1013,6 +1014,10 @@ static Tiling make_tiling(const xf_engine *e) {
   tl.tz = e->tz;
   tl.ty = e->ty;
   tl.tx = e->tx > 0 ? e->tx : e->nx + 1;
+  // a tile's tx + 1 planes stay within the 4 GiB its 32-bit offsets reach
+  // (results do not depend on tx)
+  const long long cap = (long long)(((1ULL << 32) - 1) / ((unsigned long long)e->si * e->esz)) - 1;
+  if (tl.tx > cap) tl.tx = (int)std::max(1LL, cap);
   if (tl.ty == 0) {
     // flat runs of tz cells over the padded (j,k) plane
     const long long plane = (long long)(e->ny + 1) * e->pz;
@@ -1069,8 +1074,9 @@ static void launch_line_monitors(const xf_engine *e, const Grid<T> &g, hipStream
 
 // can this engine take the fused E+H step?  It covers Yee, CPML and TFSF;
 // design boxes, dispersive media and lumped elements need the split
-// kernels, and so do 2-D grids and engines in a batch (a batch's launches
-// hold the member's field pointers).
+// kernels, and so do 2-D grids, engines in a batch (a batch's launches hold
+// the member's field pointers) and fields over 4 GiB (the fused kernel
+// addresses a whole component with 32-bit offsets).
 static int sweep_rows(const xf_engine *e) {
   if (e->sweep_rows > 0) return e->sweep_rows;
   return 1;
@@ -1092,7 +1098,7 @@ static size_t sweep_lds_bytes(const xf_engine *e) {
 }
 
 static bool fused_ok(const xf_engine *e) {
-  return e->sweep == 1 && e->nz > 1 && !e->has_design && !e->has_drude && e->sources.empty() &&
+  return e->sweep == 1 && e->nz > 1 && (unsigned long long)e->ncell * e->esz < (1ULL << 32) && !e->has_design && !e->has_drude && e->sources.empty() &&
          e->batch_stream == nullptr && e->nmat <= 8 && make_sweep(e).tx <= SW_TXMAX &&
          sweep_lds_bytes(e) <= 60 * 1024;
 }

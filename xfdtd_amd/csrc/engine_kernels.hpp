@@ -573,10 +573,14 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
   const int i0 = bx * tl.tx;
   const int i1 = min(i0 + tl.tx, nx + 1);
   const bool live = k <= nz && j <= ny;
+  // the tile's field and material bases start at its first plane (i0 - 1
+  // for the H(i - 1) loads), so 32-bit byte offsets reach tx + 1 planes of
+  // any grid
+  const long long pb = (long long)(i0 >= 1 ? i0 - 1 : 0) * g.si;
   Col<T> q;
-  q.u0 = g.f[EX]; q.u1 = g.f[EY]; q.u2 = g.f[EZ];
-  q.r0 = g.f[HX]; q.r1 = g.f[HY]; q.r2 = g.f[HZ];
-  q.mat = g.mat;
+  q.u0 = g.f[EX] + pb; q.u1 = g.f[EY] + pb; q.u2 = g.f[EZ] + pb;
+  q.r0 = g.f[HX] + pb; q.r1 = g.f[HY] + pb; q.r2 = g.f[HZ] + pb;
+  q.mat = g.mat + pb;
   q.dk = k >= 1 ? (unsigned)sizeof(T) : 0u;
   q.dj = j >= 1 ? (unsigned)(g.sj * (long long)sizeof(T)) : 0u;
   q.kin = k >= 1 && k < nz;
@@ -594,7 +598,7 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
       SPECIAL && __ballot(live && j >= g.db[2] && j < g.db[3] && k >= g.db[4] && k < g.db[5]) != 0ull;
   const bool col_drude =
       SPECIAL && __ballot(live && j >= g.xb[2] && j < g.xb[3] && k >= g.xb[4] && k < g.xb[5]) != 0ull;
-  unsigned c = (unsigned)((long long)i0 * g.si + (long long)j * g.sj + k);
+  unsigned c = (unsigned)((long long)i0 * g.si - pb + (long long)j * g.sj + k);
   // x-neighbours are carried in registers: Hz(i-1), Hy(i-1)
   T hz_prev = T(0), hy_prev = T(0);
   Plane<T> A;
@@ -774,10 +778,13 @@ __device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl
   const int i1 = min(i0 + tl.tx, nx + 1);
   const bool live = k <= nz && j <= ny;
   const unsigned bsi = (unsigned)(g.si * (long long)sizeof(T));
+  // bases at the tile's first plane: offsets reach its tx planes and the
+  // E(i + 1) plane after them
+  const long long pb = (long long)i0 * g.si;
   Col<T> q;
-  q.u0 = g.f[HX]; q.u1 = g.f[HY]; q.u2 = g.f[HZ];
-  q.r0 = g.f[EX]; q.r1 = g.f[EY]; q.r2 = g.f[EZ];
-  q.mat = g.mat;
+  q.u0 = g.f[HX] + pb; q.u1 = g.f[HY] + pb; q.u2 = g.f[HZ] + pb;
+  q.r0 = g.f[EX] + pb; q.r1 = g.f[EY] + pb; q.r2 = g.f[EZ] + pb;
+  q.mat = g.mat + pb;
   q.dk = k < nz ? (unsigned)sizeof(T) : 0u;
   q.dj = j < ny ? (unsigned)(g.sj * (long long)sizeof(T)) : 0u;
   q.kin = k < nz;
@@ -787,7 +794,7 @@ __device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl
   q.cp = col_cpml(g.bm, g.am, live ? h_layer(j, ny, g.dn[1], g.dp[1]) : -1,
                   live ? h_layer(k, nz, g.dn[2], g.dp[2]) : -1, j, k);
   const bool col_slab = __ballot(q.cp.ly >= 0 || q.cp.lz >= 0) != 0ull;
-  unsigned c = (unsigned)((long long)i0 * g.si + (long long)j * g.sj + k);
+  unsigned c = (unsigned)((long long)j * g.sj + k);
   T ez_c = T(0), ey_c = T(0);
   Plane<T> A;
   bool slab = col_slab || h_layer(i0, nx, g.dn[0], g.dp[0]) >= 0;
