@@ -139,14 +139,21 @@ def run_batched(count, steps, warmup=10, log=None):
     table = autotune.tune(precision=32, log=log)
     scenes = batched_scatter_set(count)
     total = warmup + steps
+    tune_steps = 5 + 40
 
-    def build():
+    def build(tiling=None, horizon=total):
         sims = []
         for sc in scenes:
-            sim = Simulation(sc, total, precision=32)
-            sim.set_tiling(*autotune.tiling_for(table, sim.dims[0]))
+            sim = Simulation(sc, horizon, precision=32)
+            sim.set_tiling(*(tiling or autotune.tiling_for(table, sim.dims[0])))
             sims.append(sim)
         return sims
+
+    # the batch's own tiling, timed on the batched launch (throw-away members)
+    probe = build(horizon=tune_steps * len(autotune.BATCH_CANDIDATES))
+    batch_tiling = autotune.tune_batch(probe, log=log)
+    for sim in probe:
+        sim.close()
 
     cells = 0
     plain = build()
@@ -160,7 +167,7 @@ def run_batched(count, steps, warmup=10, log=None):
         sim.step(steps)
         sim.sync()
     t_plain = time.perf_counter() - t0
-    batch_sims = build()
+    batch_sims = build(batch_tiling)
     b = Batch(batch_sims)
     b.step(warmup)
     b.sync()
@@ -186,6 +193,7 @@ def run_batched(count, steps, warmup=10, log=None):
         "speedup": round(t_plain / t_batch, 3),
         "bit_identical": bool(same),
         "tilings": {f"{lo}-{hi}": list(t) for (lo, hi), t in table.items()},
+        "batch_tiling": list(batch_tiling),
     }
 
 

@@ -142,3 +142,31 @@ def test_batch_refuses_bad_members():
     with pytest.raises(RuntimeError, match="beyond"):
         b.step(11)
     b.close()
+
+
+def test_batch_tiling_tuner_picks_a_candidate_and_keeps_results():
+    """autotune.tune_batch times each candidate on the batched launch and
+    returns one of them; a batch tiled with it steps bit-identically to the
+    members stepped alone with their default tiling."""
+    from xfdtd_amd import autotune
+    from xfdtd_amd.sim import Batch
+    scs = scenes()
+    probe = [_sim(sc, 60, 32) for sc in scs]
+    best = autotune.tune_batch(probe, candidates=autotune.BATCH_CANDIDATES[:3], steps=10, warmup=2)
+    assert best in autotune.BATCH_CANDIDATES[:3]
+    for s in probe:
+        s.close()
+    alone = [_sim(sc, 40, 32) for sc in scs]
+    for s in alone:
+        s.step(40)
+    tuned = [_sim(sc, 40, 32) for sc in scs]
+    for s in tuned:
+        s.set_tiling(*best)
+    b = Batch(tuned)
+    b.step(40)
+    b.sync()
+    b.close()
+    for a, t in zip(alone, tuned):
+        sa, st = _state(a), _state(t)
+        for k in sa:
+            assert np.array_equal(sa[k], st[k]), k

@@ -50,6 +50,38 @@ def tune(precision=32, classes=SIZE_CLASSES, candidates=CANDIDATES, steps=12, wa
     return table
 
 
+BATCH_CANDIDATES = ((64, 4, 8, 0), (256, 0, 8, 0), (64, 4, 16, 0), (256, 0, 16, 0), (64, 2, 16, 0))
+
+
+def tune_batch(sims, candidates=BATCH_CANDIDATES, steps=40, warmup=5, log=None):
+    """Time each candidate tiling on the batched launch itself: every member
+    of `sims` (set-up Simulations of one precision, used up by the timing
+    runs: they advance by warm-up + steps per candidate) takes the
+    candidate, one Batch steps them all.  A batch fills the GPU with any
+    tile size, so its best tiling can differ from the best plain-run tiling
+    of each member's size.  Returns the fastest (tz, ty, tx, flags)."""
+    import time
+
+    from .sim import Batch
+    best = None
+    for cand in candidates:
+        for s in sims:
+            s.set_tiling(*cand)
+        b = Batch(sims)
+        b.step(warmup)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        t = (time.perf_counter() - t0) / steps
+        b.close()
+        if log:
+            log(f"autotune batch of {len(sims)} tiling={cand}: {t * 1e3:.3f} ms per step")
+        if best is None or t < best[0]:
+            best = (t, cand)
+    return best[1]
+
+
 def tiling_for(table, n):
     for (lo, hi), til in table.items():
         if lo <= n <= hi:

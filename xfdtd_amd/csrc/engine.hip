@@ -14,7 +14,6 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
-#include <deque>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -169,6 +168,10 @@ struct xf_engine {
   void *div = nullptr;
   int div_every = 100;
   long long div_step = -1;
+  // re-running a segment from a checkpoint for the adjoint: the observers
+  // (NF-FF, DFT plane, broadband series, probes, line monitors) are skipped;
+  // their state is restored from the checkpoint at the end of the run
+  bool replay = false;
   // second copies of the double-buffered state (allocated at the first
   // fused step): the fused step reads e->f, psi, the incident line and
   // writes these, then the two are swapped
@@ -237,20 +240,39 @@ static xf_status scratch_setup(int dev) {
 // synchronous copy on the null stream can see the fill land late (a fill on
 // the null stream is not ordered with the engines' non-blocking streams)
 struct AdjBufs {
+  // device buffers of one adjoint or gradient run, carved from a few large
+  // allocations (slabs of geometrically growing size): a run takes dozens of
+  // buffers, and one hipMalloc / hipFree each would cost more than the
+  // run's kernels on small grids
   hipStream_t st = nullptr;
   std::vector<void *> mem;
+  char *cur = nullptr;
+  size_t used = 0, cap = 0, next = 16u << 20;
   explicit AdjBufs(hipStream_t s = nullptr) : st(s) {}
   AdjBufs(const AdjBufs &) = delete;
   AdjBufs &operator=(const AdjBufs &) = delete;
-  void *alloc(size_t bytes) {
-    void *p = nullptr;
-    if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) return nullptr;
-    mem.push_back(p);
-    if (hipMemsetAsync(p, 0, bytes ? bytes : 16, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+  // `bytes` of device memory (256-byte aligned), zeroed unless zero = false
+  void *alloc(size_t bytes, bool zero = true) {
+    if (bytes == 0) bytes = 16;
+    const size_t need = (bytes + 255) & ~size_t(255);
+    if (!cur || used + need > cap) {
+      const size_t sz = std::max(need, next);
+      void *p = nullptr;
+      if (hipMalloc(&p, sz) != hipSuccess) return nullptr;
+      mem.push_back(p);
+      cur = (char *)p;
+      cap = sz;
+      used = 0;
+      next = std::min<size_t>(2 * sz, (size_t)4 << 30);
+    }
+    void *p = cur + used;
+    used += need;
+    if (zero && (hipMemsetAsync(p, 0, bytes, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess))
       return nullptr;
     return p;
   }
   ~AdjBufs() {
+    if (st) hipStreamSynchronize(st);
     for (void *p : mem) hipFree(p);
   }
 };
@@ -1352,6 +1374,10 @@ static xf_status run_steps(xf_engine *e, long long n) {
     // after the H half-step: NF-FF, DFT plane, broadband series, probes and
     // line monitors read the step's fields
     const Grid<T> g = make_grid<T>(e);
+    if (e->replay) {
+      e->cur++;
+      continue;
+    }
     if (e->has_nf) {
       if (e->nf_pending == 0) e->nf_step0 = step;
       hipLaunchKernelGGL(k_nffft_sample<T>, dim3((unsigned)((nfQ + 255) / 256)), dim3(256), 0, e->stream, g, e->nf,
@@ -1560,8 +1586,8 @@ struct xf_batch {
   hipStream_t stream = nullptr;
   void *inst = nullptr;  // Inst<T>[n] on the device
   int2 *map_curl = nullptr, *map_te = nullptr, *map_th = nullptr, *map_ifa = nullptr, *map_nf = nullptr;
-  int2 *map_dft = nullptr;
-  int n_curl = 0, n_te = 0, n_th = 0, n_ifa = 0, n_nf = 0, n_dft = 0;
+  int2 *map_dft = nullptr, *map_nff = nullptr;
+  int n_curl = 0, n_te = 0, n_th = 0, n_ifa = 0, n_nf = 0, n_dft = 0, n_nff = 0;
   int threads = 256;
   size_t lds = 0;
   bool use_lds = true;
@@ -1581,7 +1607,7 @@ template <typename T>
 static xf_status batch_build(xf_batch *b) {
   const int n = (int)b->e.size();
   std::vector<Inst<T>> in(n);
-  std::vector<int2> mc, mte, mth, mifa, mnf, mdft;
+  std::vector<int2> mc, mte, mth, mifa, mnf, mnff, mdft;
   b->lds = 0;
   b->use_lds = true;
   b->special = false;
@@ -1607,9 +1633,15 @@ static xf_status batch_build(xf_batch *b) {
     }
     if (e->has_nf) {
       a.nf = e->nf;
-      const long long tot = (long long)e->nf.nyb * e->nf.nzb + (long long)e->nf.nzb * e->nf.nxb +
-                            (long long)e->nf.nxb * e->nf.nyb;
-      for (long long w = 0; w < (tot + 255) / 256; ++w) mnf.push_back(make_int2(q, (int)w));
+      // the member's own sample ring (its plain steps leave it flushed)
+      const long long Q = nf_cells(e->nf);
+      if (!e->nf_ring) {
+        xf_status s = dev_alloc(e, &e->nf_ring, (size_t)NF_RING * 8 * Q * e->esz);
+        if (s != XF_OK) return s;
+      }
+      a.ring = (T *)e->nf_ring;
+      for (long long w = 0; w < (Q + 255) / 256; ++w) mnf.push_back(make_int2(q, (int)w));
+      for (long long w = 0; w < (8 * Q + 255) / 256; ++w) mnff.push_back(make_int2(q, (int)w));
     }
     if (e->has_dft) {
       a.dft = e->dft;
@@ -1632,6 +1664,8 @@ static xf_status batch_build(xf_batch *b) {
   b->map_ifa = (int2 *)p; b->n_ifa = (int)mifa.size();
   if ((s = batch_upload(b, &p, mnf.data(), mnf.size() * sizeof(int2))) != XF_OK) return s;
   b->map_nf = (int2 *)p; b->n_nf = (int)mnf.size();
+  if ((s = batch_upload(b, &p, mnff.data(), mnff.size() * sizeof(int2))) != XF_OK) return s;
+  b->map_nff = (int2 *)p; b->n_nff = (int)mnff.size();
   if ((s = batch_upload(b, &p, mdft.data(), mdft.size() * sizeof(int2))) != XF_OK) return s;
   b->map_dft = (int2 *)p; b->n_dft = (int)mdft.size();
   return XF_OK;
@@ -1691,10 +1725,21 @@ extern "C" xf_status xf_batch_create(xf_engine *const *engines, int n, xf_batch 
   return XF_OK;
 }
 
+// replay: re-running a segment for the adjoint, without the observers
 template <typename T>
-static xf_status batch_run(xf_batch *b, long long n) {
+static xf_status batch_run(xf_batch *b, long long n, bool replay = false) {
   const Inst<T> *in = (const Inst<T> *)b->inst;
   const dim3 blk(b->threads);
+  // NF-FF samples go to each member's ring, flushed every NF_RING steps and
+  // at the end of the call
+  int nf_pending = 0;
+  long long nf_step0 = 0;
+  auto nf_flush = [&]() {
+    if (nf_pending == 0) return;
+    hipLaunchKernelGGL(kb_nffft_flush<T>, dim3(b->n_nff), dim3(256), 0, b->stream, in, b->map_nff, nf_pending,
+                       nf_step0);
+    nf_pending = 0;
+  };
   for (long long q = 0; q < n; ++q) {
     const long long step = b->e[0]->cur;
     if (b->special) {
@@ -1719,25 +1764,35 @@ static xf_status batch_run(xf_batch *b, long long n) {
       hipLaunchKernelGGL(kb_tfsf_ifa<T>, dim3(b->n_ifa), dim3(1024), 0, b->stream, in, b->map_ifa, step);
       hipLaunchKernelGGL(kb_tfsf_h<T>, dim3(b->n_th), dim3(256), 0, b->stream, in, b->map_th);
     }
-    if (b->n_nf > 0) hipLaunchKernelGGL(kb_nffft<T>, dim3(b->n_nf), dim3(256), 0, b->stream, in, b->map_nf, step);
-    if (b->n_dft > 0) hipLaunchKernelGGL(kb_dft_plane<T>, dim3(b->n_dft), dim3(256), 0, b->stream, in, b->map_dft, step);
-    for (xf_engine *e : b->e)
-      if (!e->probe_comp.empty()) {
-        const int np = (int)e->probe_comp.size();
-        hipLaunchKernelGGL(k_probe<T>, dim3((np + 63) / 64), dim3(64), 0, b->stream, make_grid<T>(e),
-                           e->d_probe_comp, e->d_probe_cell, np, e->probe_buf, e->probe_T, step);
+    if (!replay) {
+      if (b->n_nf > 0) {
+        if (nf_pending == 0) nf_step0 = step;
+        hipLaunchKernelGGL(kb_nffft_sample<T>, dim3(b->n_nf), dim3(256), 0, b->stream, in, b->map_nf, nf_pending);
+        if (++nf_pending == NF_RING) nf_flush();
       }
-    for (xf_engine *e : b->e)
-      if (!e->line_mons.empty()) launch_line_monitors<T>(e, make_grid<T>(e), b->stream, step);
+      if (b->n_dft > 0) hipLaunchKernelGGL(kb_dft_plane<T>, dim3(b->n_dft), dim3(256), 0, b->stream, in, b->map_dft, step);
+      for (xf_engine *e : b->e)
+        if (!e->probe_comp.empty()) {
+          const int np = (int)e->probe_comp.size();
+          hipLaunchKernelGGL(k_probe<T>, dim3((np + 63) / 64), dim3(64), 0, b->stream, make_grid<T>(e),
+                             e->d_probe_comp, e->d_probe_cell, np, e->probe_buf, e->probe_T, step);
+        }
+      for (xf_engine *e : b->e)
+        if (!e->line_mons.empty()) launch_line_monitors<T>(e, make_grid<T>(e), b->stream, step);
+    }
     HIPCHK(hipGetLastError());
     for (xf_engine *e : b->e) e->cur++;
     const int every = b->e[0]->div_every;
-    if (every > 0 && b->e[0]->cur % every == 0)
+    if (every > 0 && b->e[0]->cur % every == 0) {
+      nf_flush();
       for (size_t m = 0; m < b->e.size(); ++m) {
         xf_status s = check_divergence(b->e[m], b->stream, (int)m);
         if (s != XF_OK) return s;
       }
+    }
   }
+  nf_flush();
+  HIPCHK(hipGetLastError());
   return XF_OK;
 }
 
@@ -1952,12 +2007,19 @@ extern "C" xf_status xf_engine_get_dft(xf_engine *e, double *out) {
 }
 
 // copies of every state buffer, owned by the caller of these helpers
-static xf_status state_save(xf_engine *e, std::vector<void *> &dst, hipStream_t st = nullptr) {
+// copy the engine's state into `dst` (allocated at the first save: from
+// `arena` when one is given, else one hipMalloc per array)
+static xf_status state_save(xf_engine *e, std::vector<void *> &dst, hipStream_t st = nullptr,
+                            AdjBufs *arena = nullptr) {
   if (!st) st = e->stream;
   if (dst.empty()) {
     for (auto &b : e->state) {
       void *p = nullptr;
-      HIPCHK(hipMalloc(&p, b.n));
+      if (arena) {
+        if (!(p = arena->alloc(b.n, false))) FAILS(XF_ENOMEM, "checkpoint buffers");
+      } else {
+        HIPCHK(hipMalloc(&p, b.n));
+      }
       dst.push_back(p);
     }
   }
@@ -2029,10 +2091,7 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
   const long long T_ = e->total;
   const int nseg = (int)((T_ + K - 1) / K);
   std::vector<std::vector<void *>> ck(nseg + 1);
-  struct Free {
-    std::vector<std::vector<void *>> &v;
-    ~Free() { for (auto &x : v) state_free(x); }
-  } free_ck{ck};
+  AdjBufs CK(e->stream);  // checkpoint storage
   AdjBufs B(e->stream);
   T *hist = (T *)B.alloc((size_t)(K + 1) * e->nd * 3 * sizeof(T));
   double *g = (double *)B.alloc((size_t)e->nd * sizeof(double));
@@ -2046,7 +2105,7 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
     const long long n = std::min<long long>(K, T_ - (long long)sgi * K);
     xf_status s;
     if (sgi + 1 < nseg) {
-      if ((s = state_save(e, ck[sgi])) != XF_OK) return s;
+      if ((s = state_save(e, ck[sgi], nullptr, &CK)) != XF_OK) return s;
       if ((s = run_steps<T>(e, n)) != XF_OK) return s;
       continue;
     }
@@ -2056,7 +2115,7 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
       hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, (int)(l + 1));
     }
   }
-  xf_status s = state_save(e, ck[nseg]);
+  xf_status s = state_save(e, ck[nseg], nullptr, &CK);
   if (s != XF_OK) return s;
   // objective from the final DFT accumulators
   const DftPlane dpl = e->dft;
@@ -2086,10 +2145,13 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
       if ((s = state_load(e, ck[sgi])) != XF_OK) return s;
       e->cur = t0;
       hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, 0);
-      for (long long l = 0; l < n; ++l) {
-        if ((s = run_steps<T>(e, 1)) != XF_OK) return s;
-        hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, (int)(l + 1));
+      e->replay = true;
+      for (long long l = 0; l < n && s == XF_OK; ++l) {
+        if ((s = run_steps<T>(e, 1)) == XF_OK)
+          hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, (int)(l + 1));
       }
+      e->replay = false;
+      if (s != XF_OK) return s;
     }
     // reverse steps of the segment
     for (long long l = n - 1; l >= 0; --l) {
@@ -2205,15 +2267,8 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
   const int nseg = (int)((T_ + K - 1) / K);
   hipStream_t st = b->stream;
   std::vector<std::vector<std::vector<void *>>> ck(n, std::vector<std::vector<void *>>(nseg + 1));
-  struct Free {
-    std::vector<std::vector<std::vector<void *>>> &v;
-    ~Free() {
-      for (auto &m : v)
-        for (auto &x : m) state_free(x);
-    }
-  } free_ck{ck};
-  std::deque<AdjBufs> B;
-  for (int q = 0; q < n; ++q) B.emplace_back(st);
+  AdjBufs CK(st);  // checkpoint storage of every member
+  AdjBufs B(st);  // every member's adjoint buffers
   std::vector<InstAdj<T>> ia(n);
   std::vector<double *> gdev(n);
   std::vector<double2 *> Ffin(n);
@@ -2223,12 +2278,12 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
     InstAdj<T> &x = ia[q];
     std::memset(&x, 0, sizeof x);
     x.g = make_grid<T>(e);
-    x.hist = (T *)B[q].alloc((size_t)(K + 1) * e->nd * 3 * sizeof(T));
-    gdev[q] = (double *)B[q].alloc((size_t)e->nd * sizeof(double));
+    x.hist = (T *)B.alloc((size_t)(K + 1) * e->nd * 3 * sizeof(T));
+    gdev[q] = (double *)B.alloc((size_t)e->nd * sizeof(double));
     const size_t np = (size_t)(e->dft.hi1 - e->dft.lo1) * (e->dft.hi2 - e->dft.lo2);
-    Ffin[q] = (double2 *)B[q].alloc(np * sizeof(double2));
+    Ffin[q] = (double2 *)B.alloc(np * sizeof(double2));
     if (!x.hist || !gdev[q] || !Ffin[q]) FAILS(XF_ENOMEM, "adjoint buffers of member %d", q);
-    xf_status s = adj_setup<T>(e, B[q], x.a);
+    xf_status s = adj_setup<T>(e, B, x.a);
     if (s != XF_OK) return s;
     x.a.dft = e->dft;
     x.a.dft.acc = Ffin[q];
@@ -2260,7 +2315,7 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
     xf_status s;
     if (sgi + 1 < nseg) {
       for (int q = 0; q < n; ++q)
-        if ((s = state_save(b->e[q], ck[q][sgi], st)) != XF_OK) return s;
+        if ((s = state_save(b->e[q], ck[q][sgi], st, &CK)) != XF_OK) return s;
       if ((s = batch_run<T>(b, m)) != XF_OK) return s;
       continue;
     }
@@ -2272,7 +2327,7 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
   }
   for (int q = 0; q < n; ++q) {
     xf_engine *e = b->e[q];
-    xf_status s = state_save(e, ck[q][nseg], st);
+    xf_status s = state_save(e, ck[q][nseg], st, &CK);
     if (s != XF_OK) return s;
     const size_t np = (size_t)(e->dft.hi1 - e->dft.lo1) * (e->dft.hi2 - e->dft.lo2);
     HIPCHK(hipMemcpyAsync(Ffin[q], e->dft.acc, np * sizeof(double2), hipMemcpyDeviceToDevice, st));
@@ -2298,7 +2353,7 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
       }
       snap(0);
       for (long long l = 0; l < m; ++l) {
-        if ((s = batch_run<T>(b, 1)) != XF_OK) return s;
+        if ((s = batch_run<T>(b, 1, true)) != XF_OK) return s;
         snap((int)(l + 1));
       }
     }

@@ -1135,13 +1135,6 @@ __host__ __device__ inline long long nf_cells(const Nffft &n) {
   return (long long)n.nyb * n.nzb + (long long)n.nzb * n.nxb + (n.zf ? (long long)n.nxb * n.nyb : 0LL);
 }
 
-__device__ __forceinline__ void acc_add(double2 *a, double v, double2 tw) {
-  double2 x = *a;
-  x.x = x.x + v * tw.x;
-  x.y = x.y + v * tw.y;
-  *a = x;
-}
-
 // the eight face-centre values of surface cell q (faces x, then y, then z;
 // both faces of a pair): false past the last cell
 template <typename T>
@@ -1195,50 +1188,34 @@ __device__ __forceinline__ bool nffft_values(const Grid<T> &g, const Nffft &n, i
   return true;
 }
 
-template <typename T>
-__device__ __forceinline__ void nffft_body(const Grid<T> &g, const Nffft &n, long long step, int block) {
-  const int q = block * (int)blockDim.x + (int)threadIdx.x;
-  T v[8];
-  int pair, cells, cell;
-  if (!nffft_values<T>(g, n, q, v, pair, cells, cell)) return;
-  for (int f = 0; f < n.nf; ++f) {
-    const double2 twm = n.tm[(long long)f * n.T_total + step];
-    const double2 twj = n.tj[(long long)f * n.T_total + step];
-    for (int c = 0; c < 8; ++c) {
-      double2 *a = &n.acc[n.base[pair] + ((long long)c * n.nf + f) * cells + cell];
-      acc_add(a, (double)v[c], (c % 4) < 2 ? twm : twj);
-    }
-  }
-}
-template <typename T>
-__global__ void k_nffft(Grid<T> g, Nffft n, long long step) {
-  nffft_body<T>(g, n, step, blockIdx.x);
-}
-
-// The same running DFT over a window of steps: each step stores its face
-// values in ring slot s ([slot][8][cells of all faces]), and k_nffft_flush
-// adds the window's terms to every accumulator in step order -- the same
-// operations as per-step accumulation, with the accumulator read and
-// written once per window.
+// The running DFT of NffftFd::update (nffft_fd.cpp:108-203) over a window
+// of steps: each step stores its face values in ring slot s ([slot][8][cells
+// of all faces]), and k_nffft_flush adds the window's terms to every
+// accumulator in step order -- the same operations as accumulating every
+// step, with the accumulator read and written once per window.
 constexpr int NF_RING = 16;
 
 template <typename T>
-__global__ void k_nffft_sample(Grid<T> g, Nffft n, T *ring, int slot) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void nffft_sample_body(const Grid<T> &g, const Nffft &n, T *ring, int slot, int block) {
+  const int q = block * (int)blockDim.x + (int)threadIdx.x;
   const long long Q = nf_cells(n);
   T v[8];
   int pair, cells, cell;
   if (!nffft_values<T>(g, n, q, v, pair, cells, cell)) return;
   for (int c = 0; c < 8; ++c) ring[((long long)slot * 8 + c) * Q + q] = v[c];
 }
+template <typename T>
+__global__ void k_nffft_sample(Grid<T> g, Nffft n, T *ring, int slot) {
+  nffft_sample_body<T>(g, n, ring, slot, blockIdx.x);
+}
 
 // one thread per (value c, surface cell): its 8 x nf accumulators' share
 template <typename T>
-__global__ void __launch_bounds__(256) k_nffft_flush(Nffft n, const T *__restrict__ ring, int nslots,
-                                                    long long step0) {
+__device__ __forceinline__ void nffft_flush_body(const Nffft &n, const T *__restrict__ ring, int nslots,
+                                                 long long step0, int block) {
   const int cx = n.nyb * n.nzb, cy = n.nzb * n.nxb, cz = n.zf ? n.nxb * n.nyb : 0;
   const long long Q = (long long)cx + cy + cz;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = (long long)block * blockDim.x + threadIdx.x;
   if (t >= 8 * Q) return;
   const int c = (int)(t / Q);
   int q = (int)(t - (long long)c * Q);
@@ -1265,6 +1242,11 @@ __global__ void __launch_bounds__(256) k_nffft_flush(Nffft n, const T *__restric
       }
     *a = x;
   }
+}
+template <typename T>
+__global__ void __launch_bounds__(256) k_nffft_flush(Nffft n, const T *__restrict__ ring, int nslots,
+                                                    long long step0) {
+  nffft_flush_body<T>(n, ring, nslots, step0, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1790,6 +1772,7 @@ struct Inst {
   int nEy, nEz, nEx;
   TfsfHSeg hs;
   Nffft nf;
+  T *ring;  // NF-FF sample ring ([NF_RING][8][surface cells])
   DftPlane dft;
 };
 
@@ -1832,9 +1815,15 @@ __global__ void __launch_bounds__(1024) kb_tfsf_ifa(const Inst<T> *__restrict__ 
   tfsf_ifa_body(in[map[blockIdx.x].x].t, step);
 }
 template <typename T>
-__global__ void kb_nffft(const Inst<T> *__restrict__ in, const int2 *__restrict__ map, long long step) {
+__global__ void kb_nffft_sample(const Inst<T> *__restrict__ in, const int2 *__restrict__ map, int slot) {
   const int2 w = map[blockIdx.x];
-  nffft_body<T>(in[w.x].g, in[w.x].nf, step, w.y);
+  nffft_sample_body<T>(in[w.x].g, in[w.x].nf, in[w.x].ring, slot, w.y);
+}
+template <typename T>
+__global__ void __launch_bounds__(256) kb_nffft_flush(const Inst<T> *__restrict__ in, const int2 *__restrict__ map,
+                                                     int nslots, long long step0) {
+  const int2 w = map[blockIdx.x];
+  nffft_flush_body<T>(in[w.x].nf, in[w.x].ring, nslots, step0, w.y);
 }
 // 256-thread blocks over each instance's DFT plane
 template <typename T>
