@@ -293,6 +293,13 @@ xf_status xf_engine_set_divergence_check(xf_engine *e, int every);
 /* the step at which non-finite values were detected, or -1 */
 int64_t xf_engine_diverged_step(const xf_engine *e);
 
+/* Gradient schedule (no counterpart in the reference, which has no
+   adjoint): xf_engine_grad_eps replays each checkpointed segment and then
+   sweeps it backwards; with overlap the sweep of segment s runs on a second
+   stream while segment s - 1 is replayed.  mode -1 (the default): overlap in
+   FP32 only; 0: never; 1: always.  The gradient is bit-identical either way. */
+xf_status xf_engine_set_adjoint_overlap(xf_engine *e, int mode);
+
 /* Drude cells (DrudeMedium, material/dispersive_material.cpp:118-193, and
    the dispersive branch of Simulation::updateE, simulation_update.cpp:
    221-280).  box[6] = [i0,i1)x[j0,j1)x[k0,k1) holds every Drude cell and lies

@@ -98,6 +98,22 @@ def test_gradient_checkpoint_interval_bitwise():
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_gradient_schedules_bitwise(precision):
+    """The adjoint sweeps overlapped with the segment replays on a second
+    stream give the serial schedule's gradient bit for bit, also with the
+    segments uneven (13 does not divide the run)."""
+    sc, steps = design_scene()
+    out = []
+    for mode in (0, 1):
+        g = _sim(sc, steps, precision)
+        g.set_adjoint_overlap(mode)
+        out.append(g.grad(checkpoint_every=13))
+        g.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_gradient_central_differences_on_gpu():
     sc, steps = design_scene()
     J, g = _sim(sc, steps).grad(checkpoint_every=40)

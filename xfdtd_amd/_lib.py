@@ -23,6 +23,7 @@ ENGINE_SYMBOLS = [
     "xf_engine_current_step", "xf_engine_get_field", "xf_engine_set_field",
     "xf_engine_kernel_times", "xf_engine_set_timing", "xf_engine_set_tiling", "xf_engine_set_sweep",
     "xf_engine_sweep_active", "xf_engine_set_divergence_check", "xf_engine_diverged_step",
+    "xf_engine_set_adjoint_overlap",
     "xf_batch_create", "xf_batch_step", "xf_batch_sync", "xf_batch_size", "xf_batch_destroy", "xf_batch_grad_eps",
     "xf_engine_set_design", "xf_engine_set_dft_plane", "xf_engine_get_dft", "xf_engine_checkpoint",
     "xf_engine_restore", "xf_engine_grad_eps", "xf_engine_adjoint_run", "xf_engine_set_drude",
@@ -108,6 +109,7 @@ def lib():
     L.xf_engine_restore.argtypes = [vp, i]
     L.xf_engine_set_divergence_check.argtypes = [vp, C.c_int]
     L.xf_engine_diverged_step.argtypes = [vp]
+    L.xf_engine_set_adjoint_overlap.argtypes = [vp, C.c_int]
     L.xf_engine_diverged_step.restype = i64
     L.xf_engine_sync.argtypes = [vp]
     L.xf_engine_step.argtypes = [vp, i64]

@@ -172,6 +172,10 @@ struct xf_engine {
   // (NF-FF, DFT plane, broadband series, probes, line monitors) are skipped;
   // their state is restored from the checkpoint at the end of the run
   bool replay = false;
+  // gradient: run each segment's adjoint sweep on a second stream beside the
+  // replay of the segment before it (-1: in FP32 only, where the kernels are
+  // latency-bound at the sizes a gradient takes; 0: never; 1: always)
+  int adj_overlap = -1;
   // second copies of the double-buffered state (allocated at the first
   // fused step): the fused step reads e->f, psi, the incident line and
   // writes these, then the two are swapped
@@ -1575,6 +1579,13 @@ extern "C" xf_status xf_engine_set_divergence_check(xf_engine *e, int every) {
 
 extern "C" int64_t xf_engine_diverged_step(const xf_engine *e) { return e ? e->div_step : -1; }
 
+extern "C" xf_status xf_engine_set_adjoint_overlap(xf_engine *e, int mode) {
+  if (!e) FAILS(XF_EINVAL, "null engine");
+  if (mode < -1 || mode > 1) FAILS(XF_EINVAL, "adjoint overlap %d: need -1, 0 or 1", mode);
+  e->adj_overlap = mode;
+  return XF_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Batches: many engines of one precision stepped together.  Each per-step
 // kernel is one launch over all instances; a workgroup finds its instance and
@@ -2071,15 +2082,14 @@ static xf_status adj_setup(xf_engine *e, AdjBufs &B, Adj<T> &a) {
   for (int q = 0; q < 3; ++q) {
     if (!(a.Ea[q] = (T *)B.alloc(fb)) || !(a.Ha[q] = (T *)B.alloc(fb))) FAILS(XF_ENOMEM, "adjoint fields");
   }
-  for (int q = 0; q < 6; ++q) {
-    if (!(a.W[q] = (T *)B.alloc(fb)) || !(a.V[q] = (T *)B.alloc(fb))) FAILS(XF_ENOMEM, "adjoint weights");
-  }
-  // psi adjoints: the same layouts as psi (slots of the y, z, x axes)
+  // psi adjoints, two copies each: the same layouts as psi (slots of the y,
+  // z, x axes)
   static const int slot_axis[6] = {1, 2, 0, 2, 0, 1};  // P_AY, P_AZ, P_BX, P_BZ, P_CX, P_CY
-  for (int q = 0; q < 6; ++q) {
-    const size_t nb = e->psi_bytes[slot_axis[q]];
-    if (!(a.Pe[q] = (T *)B.alloc(nb)) || !(a.Ph[q] = (T *)B.alloc(nb))) FAILS(XF_ENOMEM, "adjoint psi");
-  }
+  for (int c = 0; c < 2; ++c)
+    for (int q = 0; q < 6; ++q) {
+      const size_t nb = e->psi_bytes[slot_axis[q]];
+      if (!(a.Pe[c][q] = (T *)B.alloc(nb)) || !(a.Ph[c][q] = (T *)B.alloc(nb))) FAILS(XF_ENOMEM, "adjoint psi");
+    }
   if (e->has_drude)
     for (int q = 0; q < 3; ++q)
       if (!(a.Ja[q] = (T *)B.alloc(drude_cells(e) * sizeof(T)))) FAILS(XF_ENOMEM, "adjoint Drude currents");
@@ -2093,14 +2103,18 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
   std::vector<std::vector<void *>> ck(nseg + 1);
   AdjBufs CK(e->stream);  // checkpoint storage
   AdjBufs B(e->stream);
-  T *hist = (T *)B.alloc((size_t)(K + 1) * e->nd * 3 * sizeof(T));
+  // two design-history buffers: segment s uses hist[s & 1], so the replay of
+  // segment s - 1 can run while the adjoint sweeps segment s
+  const size_t hb = (size_t)(K + 1) * e->nd * 3 * sizeof(T);
+  T *hist[2] = {(T *)B.alloc(hb), nseg > 1 ? (T *)B.alloc(hb) : nullptr};
   double *g = (double *)B.alloc((size_t)e->nd * sizeof(double));
-  if (!hist || !g) FAILS(XF_ENOMEM, "adjoint buffers");
+  if (!hist[0] || (nseg > 1 && !hist[1]) || !g) FAILS(XF_ENOMEM, "adjoint buffers");
   const Grid<T> gr = make_grid<T>(e);
   const unsigned nb = (unsigned)((e->ncell + 255) / 256), nsnap = (unsigned)((e->nd + 255) / 256);
   // forward run with a checkpoint at the start of every segment but the
   // last, whose design-box E fields are kept as it runs (it is never
   // recomputed), and a checkpoint at the end
+  T *hl = hist[(nseg - 1) & 1];
   for (int sgi = 0; sgi < nseg; ++sgi) {
     const long long n = std::min<long long>(K, T_ - (long long)sgi * K);
     xf_status s;
@@ -2109,10 +2123,10 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
       if ((s = run_steps<T>(e, n)) != XF_OK) return s;
       continue;
     }
-    hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, 0);
+    hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hl, 0);
     for (long long l = 0; l < n; ++l) {
       if ((s = run_steps<T>(e, 1)) != XF_OK) return s;
-      hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, (int)(l + 1));
+      hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hl, (int)(l + 1));
     }
   }
   xf_status s = state_save(e, ck[nseg], nullptr, &CK);
@@ -2134,32 +2148,67 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
   a.dft = dpl;
   a.dft.acc = Ffin;
   a.inject = dpl.comp < 3;
-  a.hist = hist;
   a.eps_r = e->d_eps;
   a.grad = g;
   a.grad_on = 1;
+  // the adjoint sweeps run on a stream of their own (default priority) when
+  // they overlap the replays, which stay on the engine's stream.  Events
+  // order each sweep after its segment's replay and each replay after the
+  // sweep that last used its history buffer.
+  struct Side {
+    hipStream_t st = nullptr;
+    std::vector<hipEvent_t> ev;
+    ~Side() {
+      if (st) hipStreamSynchronize(st);
+      for (hipEvent_t x : ev) hipEventDestroy(x);
+      if (st && own) hipStreamDestroy(st);
+    }
+    bool own = false;
+  } side;
+  const bool overlap = e->adj_overlap < 0 ? e->prec == 32 : e->adj_overlap != 0;
+  if (overlap) {
+    HIPCHK(hipStreamCreateWithFlags(&side.st, hipStreamNonBlocking));
+    side.own = true;
+  }
+  hipStream_t as = overlap ? side.st : e->stream;
+  side.ev.resize(2 * (size_t)nseg, nullptr);
+  for (hipEvent_t &x : side.ev) HIPCHK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+  hipEvent_t *ready = side.ev.data(), *done = side.ev.data() + nseg;  // replay of s ready, sweep of s done
+  HIPCHK(hipEventRecord(ready[nseg - 1], e->stream));
+  int par = 0;  // which psi-adjoint copy holds the current values
   for (int sgi = nseg - 1; sgi >= 0; --sgi) {
-    const long long t0 = (long long)sgi * K, n = std::min<long long>(K, T_ - t0);
-    if (sgi + 1 < nseg) {
-      // recompute the segment from its checkpoint, keeping the design E fields
-      if ((s = state_load(e, ck[sgi])) != XF_OK) return s;
-      e->cur = t0;
-      hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, 0);
+    // replay segment sgi - 1 into the other history buffer, once the sweep
+    // of segment sgi + 1 (its last user) is done
+    if (sgi >= 1) {
+      const int r = sgi - 1;
+      const long long r0 = (long long)r * K, rn = std::min<long long>(K, T_ - r0);
+      T *hr = hist[r & 1];
+      if (sgi + 1 < nseg) HIPCHK(hipStreamWaitEvent(e->stream, done[sgi + 1], 0));
+      if ((s = state_load(e, ck[r])) != XF_OK) return s;
+      e->cur = r0;
+      hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hr, 0);
       e->replay = true;
-      for (long long l = 0; l < n && s == XF_OK; ++l) {
+      for (long long l = 0; l < rn && s == XF_OK; ++l) {
         if ((s = run_steps<T>(e, 1)) == XF_OK)
-          hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hist, (int)(l + 1));
+          hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hr, (int)(l + 1));
       }
       e->replay = false;
       if (s != XF_OK) return s;
+      HIPCHK(hipEventRecord(ready[r], e->stream));
     }
-    // reverse steps of the segment
+    // reverse steps of segment sgi
+    const long long t0 = (long long)sgi * K, n = std::min<long long>(K, T_ - t0);
+    HIPCHK(hipStreamWaitEvent(as, ready[sgi], 0));
+    a.hist = hist[sgi & 1];
     for (long long l = n - 1; l >= 0; --l) {
-      hipLaunchKernelGGL(k_adj_e<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, t0 + l, (int)l);
-      hipLaunchKernelGGL(k_adj_h<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, 1);
+      hipLaunchKernelGGL(k_adj_e<T>, dim3(nb), dim3(256), 0, as, gr, a, t0 + l, (int)l, par, (int)ADJ_STEP);
+      hipLaunchKernelGGL(k_adj_h<T>, dim3(nb), dim3(256), 0, as, gr, a, par);
+      par ^= 1;
     }
     HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(done[sgi], as));
   }
+  HIPCHK(hipStreamSynchronize(as));
   // leave the engine at the end of the forward run
   if ((s = state_load(e, ck[nseg])) != XF_OK) return s;
   e->cur = T_;
@@ -2222,13 +2271,17 @@ static xf_status adjoint_run(xf_engine *e, long long nsteps, double *fields) {
     }
   const Grid<T> gr = make_grid<T>(e);
   const unsigned nb = (unsigned)((e->ncell + 255) / 256);
-  // the H transpose of the last step, then E and H transposes per step; the
-  // last H kernel only gathers
-  hipLaunchKernelGGL(k_adj_h<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, 1);
+  // the given adjoints are the gathered, unscaled values the kernels pass
+  // between them (the first E kernel takes its E adjoint as it is); a last
+  // E kernel applies the pending E scaling
+  int par = 0;
   for (long long t = 0; t < nsteps; ++t) {
-    hipLaunchKernelGGL(k_adj_e<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, 0LL, 0);
-    hipLaunchKernelGGL(k_adj_h<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, t + 1 < nsteps ? 1 : 0);
+    hipLaunchKernelGGL(k_adj_e<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, 0LL, 0, par,
+                       (int)(t == 0 ? ADJ_FIRST : ADJ_STEP));
+    hipLaunchKernelGGL(k_adj_h<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, par);
+    par ^= 1;
   }
+  if (nsteps > 0) hipLaunchKernelGGL(k_adj_e<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, 0LL, 0, par, (int)ADJ_FINAL);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(e->stream));
   off = 0;
@@ -2343,6 +2396,7 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
     Js[q] = Jv;
   }
   // reverse, segment by segment
+  int par = 0;  // which psi-adjoint copy holds the current values
   for (int sgi = nseg - 1; sgi >= 0; --sgi) {
     const long long t0 = (long long)sgi * K, m = std::min<long long>(K, T_ - t0);
     xf_status s;
@@ -2358,8 +2412,9 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
       }
     }
     for (long long l = m - 1; l >= 0; --l) {
-      hipLaunchKernelGGL(kb_adj_e<T>, dim3(ncell), dim3(256), 0, st, d_ia, d_cell, t0 + l, (int)l);
-      hipLaunchKernelGGL(kb_adj_h<T>, dim3(ncell), dim3(256), 0, st, d_ia, d_cell, 1);
+      hipLaunchKernelGGL(kb_adj_e<T>, dim3(ncell), dim3(256), 0, st, d_ia, d_cell, t0 + l, (int)l, par);
+      hipLaunchKernelGGL(kb_adj_h<T>, dim3(ncell), dim3(256), 0, st, d_ia, d_cell, par);
+      par ^= 1;
     }
     HIPCHK(hipGetLastError());
   }

@@ -1493,28 +1493,42 @@ __global__ void k_design_snap(Grid<T> g, T *hist, int slot) {
 }
 
 // ---------------------------------------------------------------------------
-// Discrete adjoint of the step map (transpose of the E and H half-steps with
-// their CPML terms), in the adjoint fields Ea, Ha and psi adjoints Pe, Ph.
-// Each half-step's transpose is split in two: at the cells it updates, the
-// "difference weights" -- the coefficient times the adjoint, plus the CPML
-// a-coefficient times the psi adjoint, for each difference the update takes
-// -- are formed and the cell's own adjoint and psi adjoint are scaled; at
-// the cells those differences read, the weights are gathered.  One kernel
-// per cell type does the gather of one half-step and the scaling of the
-// other, so every value is read and written by the thread that owns it.
+// Discrete adjoint of the step map: the transpose of the E and H half-steps
+// with their CPML terms and Drude currents, in the adjoint fields Ea, Ha,
+// the psi adjoints Pe, Ph and the Drude J adjoints Ja.
+//
+// A half-step's transpose scales each updated cell's own adjoint by the
+// cell's self coefficient and adds to every cell whose difference the update
+// took a "difference weight": the curl coefficient times the adjoint, plus
+// the CPML a-coefficient times the psi adjoint chi = p + q * adjoint (p then
+// becomes b * chi).  The weights are not stored: a kernel evaluates the
+// weights of its own cell and of the neighbours it gathers from out of the
+// neighbours' adjoints and coefficients.  So between kernels the fields hold
+// gathered adjoints whose own-cell scaling is still to come:
+//   Ea: after the H transpose's gather and the objective injection, before
+//       the E transpose's scaling;
+//   Ha: after the E transpose's gather, before the H transpose's scaling.
+// k_adj_e applies the pending E scaling and gathers the H-side weights W;
+// k_adj_h applies the pending H scaling and gathers the E-side weights V.
+// A psi adjoint is read (old) by the cells that gather its weight and
+// updated by the cell it belongs to, so the two copies Pe[par], Pe[par ^ 1]
+// (and Ph) alternate from step to step.  A Drude J adjoint is read by the H
+// gathers of a step and updated by the next E scaling.
 //
 //   H-side weights (W): Wxy chxey (Ey k-diff), Wxz chxez (Ez j), Wyz chyez
 //     (Ez i), Wyx chyex (Ex k), Wzx chzex (Ex j), Wzy chzey (Ey i)
 //   E-side weights (V): Vxy cexhy (Hy k-diff), Vxz cexhz (Hz j), Vyz ceyhz
 //     (Hz i), Vyx ceyhx (Hx k), Vzx cezhx (Hx j), Vzy cezhy (Hy i)
 enum { W_XY = 0, W_XZ, W_YZ, W_YX, W_ZX, W_ZY };
+// k_adj_e modes: a full step; the first step of a run whose E adjoint is
+// given unscaled; the final E scaling alone
+enum { ADJ_STEP = 0, ADJ_FIRST = 1, ADJ_FINAL = 2 };
 
 template <typename T>
 struct Adj {
   T *Ea[3], *Ha[3];
-  T *Pe[6], *Ph[6];   // psi adjoints, psi layouts
-  T *Ja[3];           // Drude J adjoints, Drude-box layout
-  T *W[6], *V[6];     // padded grid
+  T *Pe[2][6], *Ph[2][6];  // psi adjoints (two copies each), psi layouts
+  T *Ja[3];                // Drude J adjoints, Drude-box layout
   // objective injection (plane of comp < 3) and design gradient
   DftPlane dft;       // dft.acc = final forward F
   int inject;
@@ -1529,24 +1543,212 @@ __device__ __forceinline__ P2<T> pair_at(const T *tab, int n, int nmat, int m) {
   return reinterpret_cast<const P2<T> *>(tab)[n * nmat + m];
 }
 
-// per E cell: gather of the H half-step transpose, objective injection,
-// gradient, then the E half-step transpose's weights and scaling
+// H-side weights of cell (i, j, k) (flat index c) from its gathered H adjoint
+// (hx, hy, hz).  The psi terms are formed for the components in `mask` (bit
+// W_*); own: this thread also updates the cell's psi adjoints (all six).
 template <typename T>
-__device__ __forceinline__ void adj_e_body(const Grid<T> &g, const Adj<T> &a, long long step, int slot,
-                                           long long c) {
-  const int i = (int)(c / g.si), r = (int)(c - (long long)i * g.si), j = r / (int)g.sj, k = r - j * (int)g.sj;
+__device__ __forceinline__ void adj_w(const Grid<T> &g, const Adj<T> &a, int par, long long c, int i, int j,
+                                      int k, T hx, T hy, T hz, unsigned mask, bool own, T w[6]) {
   const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
+  const bool sx = j < ny && k < nz, sy = i < nx && k < nz, sz = i < nx && j < ny;
+  const int m = g.mat[c];
+  const P2<T> pk = pair_at(g.kt[1], k, nmat, m), pj = pair_at(g.jt[1], j, nmat, m),
+              pi = pair_at(g.it[1], i, nmat, m);
+  const bool yee = i < nx && j < ny && k < nz;
+  w[W_XY] = yee ? pk.x * hx : T(0);
+  w[W_XZ] = yee ? pj.x * hx : T(0);
+  w[W_YZ] = yee ? pi.x * hy : T(0);
+  w[W_YX] = yee ? pk.y * hy : T(0);
+  w[W_ZX] = yee ? pj.y * hz : T(0);
+  w[W_ZY] = yee ? pi.y * hz : T(0);
+  const int lx = h_layer(i, nx, g.dn[0], g.dp[0]), ly = h_layer(j, ny, g.dn[1], g.dp[1]),
+            lz = h_layer(k, nz, g.dn[2], g.dp[2]);
+  if (lx < 0 && ly < 0 && lz < 0) return;
+  const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = ny + 1;
+  const P2<T> qk = pair_at(g.kp[1], k, nmat, m), qj = pair_at(g.jp[1], j, nmat, m),
+              qi = pair_at(g.ip[1], i, nmat, m);
+  T *const *P = a.Ph[par];
+  T *const *Q = a.Ph[par ^ 1];
+  auto term = [&](bool on, int q, int slot, long long o, T qc, T h, T am, T bm) {
+    if (!on || !(own || ((mask >> q) & 1u))) return;
+    const T chi = P[slot][o] + qc * h;
+    w[q] = w[q] + am * chi;
+    if (own) Q[slot][o] = bm * chi;
+  };
+  if (ly >= 0) {
+    const long long o = psi_y(i, ly, k, Ly, g.pz);
+    term(sx, W_XZ, P_AY, o, qj.x, hx, g.am[1][j], g.bm[1][j]);
+    term(sz, W_ZX, P_CY, o, qj.y, hz, g.am[1][j], g.bm[1][j]);
+  }
+  if (lz >= 0) {
+    const long long o = psi_z(i, j, lz, nm1y, Lz);
+    term(sx, W_XY, P_AZ, o, qk.x, hx, g.am[2][k], g.bm[2][k]);
+    term(sy, W_YX, P_BZ, o, qk.y, hy, g.am[2][k], g.bm[2][k]);
+  }
+  if (lx >= 0) {
+    const long long o = psi_x(lx, j, k, nm1y, g.pz);
+    term(sy, W_YZ, P_BX, o, qi.x, hy, g.am[0][i], g.bm[0][i]);
+    term(sz, W_ZY, P_CX, o, qi.y, hz, g.am[0][i], g.bm[0][i]);
+  }
+}
+
+// E-side weights of cell (i, j, k) from its gathered E adjoint (ex, ey, ez):
+// design cells carry their own coefficients; a Drude cell's weights are
+// -cb t and +cb t of t = e + beta jb, with the J adjoint jb not yet updated
+// for this adjoint (the transpose of E' = ca E + cb (curl - (dl/2)(1+k) J),
+// J' = k J + beta (E' + E), taken J' first).
+template <typename T>
+__device__ __forceinline__ void adj_v(const Grid<T> &g, const Adj<T> &a, int par, long long c, int i, int j,
+                                      int k, T ex, T ey, T ez, unsigned mask, bool own, T v[6]) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
+  const bool sx = i < nx, sy = j < ny, sz = k < nz;  // inside Ex / Ey / Ez shapes
+  const int m = g.mat[c];
+  P2<T> pk = pair_at(g.kt[0], k, nmat, m), pj = pair_at(g.jt[0], j, nmat, m), pi = pair_at(g.it[0], i, nmat, m);
+  if (in_design(g, i, j, k)) {
+    const T *d = g.dcoef + 9 * design_cell(g, i, j, k);
+    pk.x = d[3]; pk.y = d[4]; pj.x = d[5]; pj.y = d[6]; pi.x = d[7]; pi.y = d[8];
+  }
+  const bool ix = i >= 1 && i < nx, jin = j >= 1 && j < ny, kin = k >= 1 && k < nz;
+  const bool yx = sx && jin && kin, yy = ix && sy && kin, yz = ix && jin && sz;
+  v[W_XY] = yx ? pk.x * ex : T(0);
+  v[W_XZ] = yx ? pj.x * ex : T(0);
+  v[W_YZ] = yy ? pi.x * ey : T(0);
+  v[W_YX] = yy ? pk.y * ey : T(0);
+  v[W_ZX] = yz ? pj.y * ez : T(0);
+  v[W_ZY] = yz ? pi.y * ez : T(0);
+  const int lx = e_layer(i, nx, g.dn[0], g.dp[0]), ly = e_layer(j, ny, g.dn[1], g.dp[1]),
+            lz = e_layer(k, nz, g.dn[2], g.dp[2]);
+  if (lx >= 0 || ly >= 0 || lz >= 0) {
+    const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = ny + 1;
+    const P2<T> qk = pair_at(g.kp[0], k, nmat, m), qj = pair_at(g.jp[0], j, nmat, m),
+                qi = pair_at(g.ip[0], i, nmat, m);
+    T *const *P = a.Pe[par];
+    T *const *Q = a.Pe[par ^ 1];
+    auto term = [&](bool on, int q, int slot, long long o, T qc, T e, T ae, T be) {
+      if (!on || !(own || ((mask >> q) & 1u))) return;
+      const T chi = P[slot][o] + qc * e;
+      v[q] = v[q] + ae * chi;
+      if (own) Q[slot][o] = be * chi;
+    };
+    if (ly >= 0) {
+      const long long o = psi_y(i, ly, k, Ly, g.pz);
+      term(sx, W_XZ, P_AY, o, qj.x, ex, g.ae[1][j], g.be[1][j]);
+      term(sz, W_ZX, P_CY, o, qj.y, ez, g.ae[1][j], g.be[1][j]);
+    }
+    if (lz >= 0) {
+      const long long o = psi_z(i, j, lz, nm1y, Lz);
+      term(sx, W_XY, P_AZ, o, qk.x, ex, g.ae[2][k], g.be[2][k]);
+      term(sy, W_YX, P_BZ, o, qk.y, ey, g.ae[2][k], g.be[2][k]);
+    }
+    if (lx >= 0) {
+      const long long o = psi_x(lx, j, k, nm1y, g.pz);
+      term(sy, W_YZ, P_BX, o, qi.x, ey, g.ae[0][i], g.be[0][i]);
+      term(sz, W_ZY, P_CX, o, qi.y, ez, g.ae[0][i], g.be[0][i]);
+    }
+  }
+  // Drude cells lie outside every CPML slab
+  if (in_drude(g, i, j, k)) {
+    const T *dc = g.drc + XF_ADE_STRIDE * m;
+    const T cb = dc[1];
+    if (cb != T(0)) {
+      const int dq = drude_cell(g, i, j, k);
+      const T beta = dc[4];
+      if (yx) {
+        const T t = ex + beta * a.Ja[0][dq];
+        v[W_XY] = -cb * t;  // -(Hy(k) - Hy(k-1))
+        v[W_XZ] = cb * t;   // +(Hz(j) - Hz(j-1))
+      }
+      if (yy) {
+        const T t = ey + beta * a.Ja[1][dq];
+        v[W_YZ] = -cb * t;  // -(Hz(i) - Hz(i-1))
+        v[W_YX] = cb * t;   // +(Hx(k) - Hx(k-1))
+      }
+      if (yz) {
+        const T t = ez + beta * a.Ja[2][dq];
+        v[W_ZX] = -cb * t;  // -(Hx(j) - Hx(j-1))
+        v[W_ZY] = cb * t;   // +(Hy(i) - Hy(i-1))
+      }
+    }
+  }
+}
+
+// the E transpose's own-cell part at cell (i, j, k): ca e on Yee cells; on
+// Drude cells beta jb + ca t with t = e + beta jb, and the J adjoint becomes
+// k jb - cb (dl/2)(1+k) t
+template <typename T>
+__device__ __forceinline__ void adj_scale_e(const Grid<T> &g, const Adj<T> &a, long long c, int i, int j, int k,
+                                            T &ex, T &ey, T &ez) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int m = g.mat[c];
+  const T *self = g.self[0] + 4 * m;
+  T ca_x = self[0], ca_y = self[1], ca_z = self[2];
+  if (in_design(g, i, j, k)) {
+    const T *d = g.dcoef + 9 * design_cell(g, i, j, k);
+    ca_x = d[0]; ca_y = d[1]; ca_z = d[2];
+  }
+  const bool sx = i < nx, sy = j < ny, sz = k < nz;
+  const bool ix = i >= 1 && i < nx, jin = j >= 1 && j < ny, kin = k >= 1 && k < nz;
+  const bool yx = sx && jin && kin, yy = ix && sy && kin, yz = ix && jin && sz;
+  T ox = yx ? ca_x * ex : ex, oy = yy ? ca_y * ey : ey, oz = yz ? ca_z * ez : ez;
+  if (in_drude(g, i, j, k)) {
+    const T *dc = g.drc + XF_ADE_STRIDE * m;
+    const T cb = dc[1];
+    if (cb != T(0)) {
+      const int dq = drude_cell(g, i, j, k);
+      const T ca = dc[0], hk = dc[5] * dc[2], kk = dc[3], beta = dc[4];
+      if (yx) {
+        const T jb = a.Ja[0][dq], t = ex + beta * jb;
+        ox = beta * jb + ca * t;
+        a.Ja[0][dq] = kk * jb - cb * hk * t;
+      }
+      if (yy) {
+        const T jb = a.Ja[1][dq], t = ey + beta * jb;
+        oy = beta * jb + ca * t;
+        a.Ja[1][dq] = kk * jb - cb * hk * t;
+      }
+      if (yz) {
+        const T jb = a.Ja[2][dq], t = ez + beta * jb;
+        oz = beta * jb + ca * t;
+        a.Ja[2][dq] = kk * jb - cb * hk * t;
+      }
+    }
+  }
+  ex = ox;
+  ey = oy;
+  ez = oz;
+}
+
+// per E cell: the pending E scaling, the gather of the H-side weights of the
+// cell and of its k-1, j-1, i-1 neighbours, the objective injection and the
+// gradient
+template <typename T>
+__device__ __forceinline__ void adj_e_body(const Grid<T> &g, const Adj<T> &a, long long step, int slot, int par,
+                                           int mode, long long c) {
+  const int i = (int)(c / g.si), r = (int)(c - (long long)i * g.si), j = r / (int)g.sj, k = r - j * (int)g.sj;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx || j > ny || k > nz) return;
-  const long long ck = c - 1, cj = c - g.sj, ci = c - g.si;
   T ex = a.Ea[0][c], ey = a.Ea[1][c], ez = a.Ea[2][c];
+  if (mode != ADJ_FIRST) adj_scale_e(g, a, c, i, j, k, ex, ey, ez);
+  if (mode == ADJ_FINAL) {
+    a.Ea[0][c] = ex;
+    a.Ea[1][c] = ey;
+    a.Ea[2][c] = ez;
+    return;
+  }
+  const long long ck = c - 1, cj = c - g.sj, ci = c - g.si;
   const bool sx = i < nx, sy = j < ny, sz = k < nz;  // inside Ex / Ey / Ez shapes
   // gather: transpose of the H curl differences (forward differences of E)
-  if (sx) ex = ex + (((k >= 1 ? a.W[W_YX][ck] : T(0)) - a.W[W_YX][c]) +
-                     ((j >= 1 ? a.W[W_ZX][cj] : T(0)) - a.W[W_ZX][c]));
-  if (sy) ey = ey + (((k >= 1 ? a.W[W_XY][ck] : T(0)) - a.W[W_XY][c]) +
-                     ((i >= 1 ? a.W[W_ZY][ci] : T(0)) - a.W[W_ZY][c]));
-  if (sz) ez = ez + (((j >= 1 ? a.W[W_XZ][cj] : T(0)) - a.W[W_XZ][c]) +
-                     ((i >= 1 ? a.W[W_YZ][ci] : T(0)) - a.W[W_YZ][c]));
+  T w[6], wk[6], wj[6], wi[6];
+  adj_w(g, a, par, c, i, j, k, a.Ha[0][c], a.Ha[1][c], a.Ha[2][c], 0u, true, w);
+  if (k >= 1 && (sx || sy))
+    adj_w(g, a, par, ck, i, j, k - 1, a.Ha[0][ck], a.Ha[1][ck], T(0), (1u << W_XY) | (1u << W_YX), false, wk);
+  if (j >= 1 && (sx || sz))
+    adj_w(g, a, par, cj, i, j - 1, k, a.Ha[0][cj], T(0), a.Ha[2][cj], (1u << W_XZ) | (1u << W_ZX), false, wj);
+  if (i >= 1 && (sy || sz))
+    adj_w(g, a, par, ci, i - 1, j, k, T(0), a.Ha[1][ci], a.Ha[2][ci], (1u << W_YZ) | (1u << W_ZY), false, wi);
+  if (sx) ex = ex + (((k >= 1 ? wk[W_YX] : T(0)) - w[W_YX]) + ((j >= 1 ? wj[W_ZX] : T(0)) - w[W_ZX]));
+  if (sy) ey = ey + (((k >= 1 ? wk[W_XY] : T(0)) - w[W_XY]) + ((i >= 1 ? wi[W_ZY] : T(0)) - w[W_ZY]));
+  if (sz) ez = ez + (((j >= 1 ? wj[W_XZ] : T(0)) - w[W_XZ]) + ((i >= 1 ? wi[W_YZ] : T(0)) - w[W_YZ]));
   // objective: dJ/dE = 2 Re(conj(F) w_t) on the monitor plane
   if (a.inject) {
     const DftPlane &d = a.dft;
@@ -1574,189 +1776,50 @@ __device__ __forceinline__ void adj_e_body(const Grid<T> &g, const Adj<T> &a, lo
     acc = acc + (double)ez * ((double)h1[2] - (double)h0[2]);
     a.grad[dq] = a.grad[dq] - acc / a.eps_r[dq];
   }
-  // E half-step transpose at this cell
-  const int m = g.mat[c];
-  const T *self = g.self[0] + 4 * m;
-  T ca_x = self[0], ca_y = self[1], ca_z = self[2];
-  P2<T> pk = pair_at(g.kt[0], k, nmat, m), pj = pair_at(g.jt[0], j, nmat, m), pi = pair_at(g.it[0], i, nmat, m);
-  if (des) {
-    const T *d = g.dcoef + 9 * dq;
-    ca_x = d[0]; ca_y = d[1]; ca_z = d[2];
-    pk.x = d[3]; pk.y = d[4]; pj.x = d[5]; pj.y = d[6]; pi.x = d[7]; pi.y = d[8];
-  }
-  const bool ix = i >= 1 && i < nx, jin = j >= 1 && j < ny, kin = k >= 1 && k < nz;
-  const bool yx = sx && jin && kin, yy = ix && sy && kin, yz = ix && jin && sz;
-  T vxy = yx ? pk.x * ex : T(0), vxz = yx ? pj.x * ex : T(0);
-  T vyz = yy ? pi.x * ey : T(0), vyx = yy ? pk.y * ey : T(0);
-  T vzx = yz ? pj.y * ez : T(0), vzy = yz ? pi.y * ez : T(0);
-  const int lx = e_layer(i, nx, g.dn[0], g.dp[0]), ly = e_layer(j, ny, g.dn[1], g.dp[1]),
-            lz = e_layer(k, nz, g.dn[2], g.dp[2]);
-  if (lx >= 0 || ly >= 0 || lz >= 0) {
-    const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = ny + 1;
-    const P2<T> qk = pair_at(g.kp[0], k, nmat, m), qj = pair_at(g.jp[0], j, nmat, m),
-                qi = pair_at(g.ip[0], i, nmat, m);
-    if (sx && ly >= 0) {
-      T &p = a.Pe[P_AY][psi_y(i, ly, k, Ly, g.pz)];
-      const T chi = p + qj.x * ex;
-      vxz = vxz + g.ae[1][j] * chi;
-      p = g.be[1][j] * chi;
-    }
-    if (sx && lz >= 0) {
-      T &p = a.Pe[P_AZ][psi_z(i, j, lz, nm1y, Lz)];
-      const T chi = p + qk.x * ex;
-      vxy = vxy + g.ae[2][k] * chi;
-      p = g.be[2][k] * chi;
-    }
-    if (sy && lx >= 0) {
-      T &p = a.Pe[P_BX][psi_x(lx, j, k, nm1y, g.pz)];
-      const T chi = p + qi.x * ey;
-      vyz = vyz + g.ae[0][i] * chi;
-      p = g.be[0][i] * chi;
-    }
-    if (sy && lz >= 0) {
-      T &p = a.Pe[P_BZ][psi_z(i, j, lz, nm1y, Lz)];
-      const T chi = p + qk.y * ey;
-      vyx = vyx + g.ae[2][k] * chi;
-      p = g.be[2][k] * chi;
-    }
-    if (sz && lx >= 0) {
-      T &p = a.Pe[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
-      const T chi = p + qi.y * ez;
-      vzy = vzy + g.ae[0][i] * chi;
-      p = g.be[0][i] * chi;
-    }
-    if (sz && ly >= 0) {
-      T &p = a.Pe[P_CY][psi_y(i, ly, k, Ly, g.pz)];
-      const T chi = p + qj.y * ez;
-      vzx = vzx + g.ae[1][j] * chi;
-      p = g.be[1][j] * chi;
-    }
-  }
-  T ox = yx ? ca_x * ex : ex, oy = yy ? ca_y * ey : ey, oz = yz ? ca_z * ez : ez;
-  // Drude cells (outside every CPML slab): the transpose of
-  //   E' = ca E + cb (curl - (dl/2)(1+k) J),  J' = k J + beta (E' + E)
-  // taken J' first: t = adjoint of E' + beta * adjoint of J'; the curl
-  // weights are -cb t on the first difference of the reference's bracket and
-  // +cb t on the second
-  if (in_drude(g, i, j, k)) {
-    const T *dc = g.drc + XF_ADE_STRIDE * m;
-    const T cb = dc[1];
-    if (cb != T(0)) {
-      const int dq = drude_cell(g, i, j, k);
-      const T ca = dc[0], hk = dc[5] * dc[2], kk = dc[3], beta = dc[4];
-      if (yx) {
-        const T jb = a.Ja[0][dq], t = ex + beta * jb;
-        vxy = -cb * t;  // -(Hy(k) - Hy(k-1))
-        vxz = cb * t;   // +(Hz(j) - Hz(j-1))
-        ox = beta * jb + ca * t;
-        a.Ja[0][dq] = kk * jb - cb * hk * t;
-      }
-      if (yy) {
-        const T jb = a.Ja[1][dq], t = ey + beta * jb;
-        vyz = -cb * t;  // -(Hz(i) - Hz(i-1))
-        vyx = cb * t;   // +(Hx(k) - Hx(k-1))
-        oy = beta * jb + ca * t;
-        a.Ja[1][dq] = kk * jb - cb * hk * t;
-      }
-      if (yz) {
-        const T jb = a.Ja[2][dq], t = ez + beta * jb;
-        vzx = -cb * t;  // -(Hx(j) - Hx(j-1))
-        vzy = cb * t;   // +(Hy(i) - Hy(i-1))
-        oz = beta * jb + ca * t;
-        a.Ja[2][dq] = kk * jb - cb * hk * t;
-      }
-    }
-  }
-  a.V[W_XY][c] = vxy; a.V[W_XZ][c] = vxz; a.V[W_YZ][c] = vyz;
-  a.V[W_YX][c] = vyx; a.V[W_ZX][c] = vzx; a.V[W_ZY][c] = vzy;
-  a.Ea[0][c] = ox;
-  a.Ea[1][c] = oy;
-  a.Ea[2][c] = oz;
+  a.Ea[0][c] = ex;
+  a.Ea[1][c] = ey;
+  a.Ea[2][c] = ez;
 }
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_e(Grid<T> g, Adj<T> a, long long step, int slot) {
-  adj_e_body<T>(g, a, step, slot, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(256) k_adj_e(Grid<T> g, Adj<T> a, long long step, int slot, int par, int mode) {
+  adj_e_body<T>(g, a, step, slot, par, mode, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// per H cell: gather of the E half-step transpose, then (with h1) the next
-// H half-step transpose's weights and scaling
+// per H cell: the pending H scaling, then the gather of the E-side weights of
+// the cell and of its k+1, j+1, i+1 neighbours
 template <typename T>
-__device__ __forceinline__ void adj_h_body(const Grid<T> &g, const Adj<T> &a, int h1, long long c) {
+__device__ __forceinline__ void adj_h_body(const Grid<T> &g, const Adj<T> &a, int par, long long c) {
   const int i = (int)(c / g.si), r = (int)(c - (long long)i * g.si), j = r / (int)g.sj, k = r - j * (int)g.sj;
-  const int nx = g.nx, ny = g.ny, nz = g.nz, nmat = g.nmat;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx || j > ny || k > nz) return;
   // inside the Hx (nx+1, ny, nz), Hy (nx, ny+1, nz), Hz (nx, ny, nz+1) shapes
   const bool sx = j < ny && k < nz, sy = i < nx && k < nz, sz = i < nx && j < ny;
   const long long ck = c + 1, cj = c + g.sj, ci = c + g.si;
-  T hx = a.Ha[0][c], hy = a.Ha[1][c], hz = a.Ha[2][c];
-  // gather: transpose of the E curl differences (backward differences of H)
-  if (sx) hx = hx + ((a.V[W_YX][c] - a.V[W_YX][ck]) + (a.V[W_ZX][c] - a.V[W_ZX][cj]));
-  if (sy) hy = hy + ((a.V[W_XY][c] - a.V[W_XY][ck]) + (a.V[W_ZY][c] - a.V[W_ZY][ci]));
-  if (sz) hz = hz + ((a.V[W_XZ][c] - a.V[W_XZ][cj]) + (a.V[W_YZ][c] - a.V[W_YZ][ci]));
-  if (!h1) {
-    a.Ha[0][c] = hx; a.Ha[1][c] = hy; a.Ha[2][c] = hz;
-    return;
-  }
-  const int m = g.mat[c];
-  const T *self = g.self[1] + 4 * m;
-  const P2<T> pk = pair_at(g.kt[1], k, nmat, m), pj = pair_at(g.jt[1], j, nmat, m),
-              pi = pair_at(g.it[1], i, nmat, m);
   const bool yee = i < nx && j < ny && k < nz;
-  T wxy = yee ? pk.x * hx : T(0), wxz = yee ? pj.x * hx : T(0);
-  T wyz = yee ? pi.x * hy : T(0), wyx = yee ? pk.y * hy : T(0);
-  T wzx = yee ? pj.y * hz : T(0), wzy = yee ? pi.y * hz : T(0);
-  const int lx = h_layer(i, nx, g.dn[0], g.dp[0]), ly = h_layer(j, ny, g.dn[1], g.dp[1]),
-            lz = h_layer(k, nz, g.dn[2], g.dp[2]);
-  if (lx >= 0 || ly >= 0 || lz >= 0) {
-    const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = ny + 1;
-    const P2<T> qk = pair_at(g.kp[1], k, nmat, m), qj = pair_at(g.jp[1], j, nmat, m),
-                qi = pair_at(g.ip[1], i, nmat, m);
-    if (sx && ly >= 0) {
-      T &p = a.Ph[P_AY][psi_y(i, ly, k, Ly, g.pz)];
-      const T chi = p + qj.x * hx;
-      wxz = wxz + g.am[1][j] * chi;
-      p = g.bm[1][j] * chi;
-    }
-    if (sx && lz >= 0) {
-      T &p = a.Ph[P_AZ][psi_z(i, j, lz, nm1y, Lz)];
-      const T chi = p + qk.x * hx;
-      wxy = wxy + g.am[2][k] * chi;
-      p = g.bm[2][k] * chi;
-    }
-    if (sy && lx >= 0) {
-      T &p = a.Ph[P_BX][psi_x(lx, j, k, nm1y, g.pz)];
-      const T chi = p + qi.x * hy;
-      wyz = wyz + g.am[0][i] * chi;
-      p = g.bm[0][i] * chi;
-    }
-    if (sy && lz >= 0) {
-      T &p = a.Ph[P_BZ][psi_z(i, j, lz, nm1y, Lz)];
-      const T chi = p + qk.y * hy;
-      wyx = wyx + g.am[2][k] * chi;
-      p = g.bm[2][k] * chi;
-    }
-    if (sz && lx >= 0) {
-      T &p = a.Ph[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
-      const T chi = p + qi.y * hz;
-      wzy = wzy + g.am[0][i] * chi;
-      p = g.bm[0][i] * chi;
-    }
-    if (sz && ly >= 0) {
-      T &p = a.Ph[P_CY][psi_y(i, ly, k, Ly, g.pz)];
-      const T chi = p + qj.y * hz;
-      wzx = wzx + g.am[1][j] * chi;
-      p = g.bm[1][j] * chi;
-    }
-  }
-  a.W[W_XY][c] = wxy; a.W[W_XZ][c] = wxz; a.W[W_YZ][c] = wyz;
-  a.W[W_YX][c] = wyx; a.W[W_ZX][c] = wzx; a.W[W_ZY][c] = wzy;
-  a.Ha[0][c] = yee ? self[0] * hx : hx;
-  a.Ha[1][c] = yee ? self[1] * hy : hy;
-  a.Ha[2][c] = yee ? self[2] * hz : hz;
+  const T *self = g.self[1] + 4 * (int)g.mat[c];
+  T hx = a.Ha[0][c], hy = a.Ha[1][c], hz = a.Ha[2][c];
+  hx = yee ? self[0] * hx : hx;
+  hy = yee ? self[1] * hy : hy;
+  hz = yee ? self[2] * hz : hz;
+  // gather: transpose of the E curl differences (backward differences of H)
+  T v[6], vk[6], vj[6], vi[6];
+  adj_v(g, a, par, c, i, j, k, a.Ea[0][c], a.Ea[1][c], a.Ea[2][c], 0u, true, v);
+  if (k < nz && (sx || sy))
+    adj_v(g, a, par, ck, i, j, k + 1, a.Ea[0][ck], a.Ea[1][ck], T(0), (1u << W_XY) | (1u << W_YX), false, vk);
+  if (j < ny && (sx || sz))
+    adj_v(g, a, par, cj, i, j + 1, k, a.Ea[0][cj], T(0), a.Ea[2][cj], (1u << W_XZ) | (1u << W_ZX), false, vj);
+  if (i < nx && (sy || sz))
+    adj_v(g, a, par, ci, i + 1, j, k, T(0), a.Ea[1][ci], a.Ea[2][ci], (1u << W_YZ) | (1u << W_ZY), false, vi);
+  if (sx) hx = hx + ((v[W_YX] - vk[W_YX]) + (v[W_ZX] - vj[W_ZX]));
+  if (sy) hy = hy + ((v[W_XY] - vk[W_XY]) + (v[W_ZY] - vi[W_ZY]));
+  if (sz) hz = hz + ((v[W_XZ] - vj[W_XZ]) + (v[W_YZ] - vi[W_YZ]));
+  a.Ha[0][c] = hx;
+  a.Ha[1][c] = hy;
+  a.Ha[2][c] = hz;
 }
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_h(Grid<T> g, Adj<T> a, int h1) {
-  adj_h_body<T>(g, a, h1, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(256) k_adj_h(Grid<T> g, Adj<T> a, int par) {
+  adj_h_body<T>(g, a, par, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1840,15 +1903,15 @@ __global__ void kb_design_snap(const InstAdj<T> *__restrict__ in, const int2 *__
 }
 template <typename T>
 __global__ void __launch_bounds__(256) kb_adj_e(const InstAdj<T> *__restrict__ in, const int2 *__restrict__ map,
-                                                long long step, int slot) {
+                                                long long step, int slot, int par) {
   const int2 w = map[blockIdx.x];
-  adj_e_body<T>(in[w.x].g, in[w.x].a, step, slot, (long long)w.y * 256 + threadIdx.x);
+  adj_e_body<T>(in[w.x].g, in[w.x].a, step, slot, par, ADJ_STEP, (long long)w.y * 256 + threadIdx.x);
 }
 template <typename T>
 __global__ void __launch_bounds__(256) kb_adj_h(const InstAdj<T> *__restrict__ in, const int2 *__restrict__ map,
-                                                int h1) {
+                                                int par) {
   const int2 w = map[blockIdx.x];
-  adj_h_body<T>(in[w.x].g, in[w.x].a, h1, (long long)w.y * 256 + threadIdx.x);
+  adj_h_body<T>(in[w.x].g, in[w.x].a, par, (long long)w.y * 256 + threadIdx.x);
 }
 
 }  // namespace xfk

@@ -379,6 +379,12 @@ class Simulation:
         a diverged run makes step() raise, naming the step."""
         self._echk(lib().xf_engine_set_divergence_check(self.engine, int(every)))
 
+    def set_adjoint_overlap(self, mode):
+        """Gradient schedule (xf_engine_set_adjoint_overlap): -1 overlap the
+        adjoint sweeps with the segment replays in FP32 only, 0 never, 1
+        always."""
+        self._echk(lib().xf_engine_set_adjoint_overlap(self.engine, int(mode)))
+
     @property
     def diverged_step(self):
         """The step at which non-finite H values were detected, or None."""
