@@ -64,6 +64,9 @@ class Shape {
   virtual bool isPointInside(const PointVector &point) const = 0;
   virtual std::unique_ptr<Shape> getWrappedBox() const = 0;
   virtual std::unique_ptr<Shape> clone() const = 0;
+  // shape/shape.cpp:4, cube.cpp:16-21, sphere.cpp:17-21: the description
+  // Simulation::checkRun writes
+  virtual explicit operator std::string() const { return "Shape: "; }
 };
 
 class Cube : public Shape {
@@ -80,6 +83,11 @@ class Cube : public Shape {
   double getZmax() const { return _point[2] + _size[2]; }
   PointVector getSize() const { return _size; }
   PointVector getPoint() const { return _point; }
+  explicit operator std::string() const override {
+    return "Cube: " + std::to_string(_point[0]) + " " + std::to_string(_point[1]) + " " +
+           std::to_string(_point[2]) + " " + std::to_string(_size[0]) + " " + std::to_string(_size[1]) + " " +
+           std::to_string(_size[2]);
+  }
 
  private:
   PointVector _point, _size;
@@ -93,6 +101,10 @@ class Sphere : public Shape {
   std::unique_ptr<Shape> clone() const override { return std::make_unique<Sphere>(*this); }
   double getRadius() const { return _radius; }
   PointVector getCenter() const { return _center; }
+  explicit operator std::string() const override {
+    return "Sphere: " + std::to_string(_center[0]) + " " + std::to_string(_center[1]) + " " +
+           std::to_string(_center[2]) + " " + std::to_string(_radius);
+  }
 
  private:
   PointVector _center;
@@ -141,6 +153,13 @@ class Material {
   double getMagneticConductivity() const { return _sigma_m; }
   bool isDispersion() const { return _is_dispersion; }
   const std::string &name() const { return _name; }
+  // material/material.cpp:18-27
+  explicit operator std::string() const {
+    return "Material: " + _name + "\neps_r: " + std::to_string(_eps_r) + "\nmu_r: " + std::to_string(_mu_r) +
+           "\neps: " + std::to_string(_eps) + "\nmu: " + std::to_string(_mu) + "\nsigma_e: " +
+           std::to_string(_sigma_e) + "\nsigma_m: " + std::to_string(_sigma_m) +
+           "\nis_dispersion: " + std::to_string(static_cast<int>(_is_dispersion));
+  }
   // Drude parameters of a DrudeMedium (eps_inf, omega_p, nv); a plain
   // Material holds 1, 0, 0 and is not dispersive
   double drudeEpsInf() const { return _drude_eps_inf; }
@@ -220,6 +239,11 @@ class Object {
   const Shape *getShape() const { return _shape.get(); }
   const Material &getMaterial() const { return *_material; }
   const std::string &getName() const { return _name; }
+  // object/object.cpp:43-47
+  explicit operator std::string() const {
+    return "Object: " + _name + "\n" + static_cast<std::string>(*_shape) + "\n" +
+           static_cast<std::string>(*_material);
+  }
 
  private:
   std::string _name;
@@ -682,6 +706,12 @@ class Network {
   std::string _output_path;
 };
 
+// .npy files (xt::dump_npy in the reference): float64 or complex128, C
+// order, shape {v.size()} when `shape` is empty
+void dumpNpy(const std::string &path, const std::vector<double> &v, const std::vector<size_t> &shape = {});
+void dumpNpy(const std::string &path, const std::vector<std::complex<double>> &v,
+             const std::vector<size_t> &shape = {});
+
 // util/dft.h: sum_j x_j exp(-2 pi i f j dt) dt for every f
 std::vector<std::complex<double>> dft(const std::vector<double> &x, double dt,
                                       const std::vector<double> &frequencies);
@@ -757,6 +787,10 @@ class Simulation {
   // needed and uploads the tables to a new engine.
   void plan(size_t total_time_steps);
   void init(size_t total_time_steps);
+  // Simulation::checkRun (simulation.cpp:98-113): the CPU set-up only, then
+  // "simulation_check" (grid size) and "simulation_object_check" (every
+  // object's description) in the working directory
+  void checkRun(size_t total_time_steps);
   void step(size_t n);
   void sync();
   size_t currentTimeStep() const;

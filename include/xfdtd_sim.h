@@ -128,6 +128,10 @@ int xf_sim_add_movie_monitor(xf_sim *s, const double origin[3], const double siz
 /* CPU set-up only (Simulation::init without the device): grid, materials,
    coefficient tables, CPML, TFSF and NF-FF tables.  Needs no GPU. */
 int xf_sim_plan(xf_sim *s, size_t total_steps);
+/* Simulation::checkRun (simulation.cpp:98-113): plan if not planned, then
+   write "simulation_check" and "simulation_object_check" in the working
+   directory */
+int xf_sim_check_run(xf_sim *s, size_t total_steps);
 /* per-cell coefficient array `which` (0..17: cexe, cexhy, cexhz, ceye, ceyhz,
    ceyhx, ceze, cezhx, cezhy, chxh, chxey, chxez, chyh, chyez, chyex, chzh,
    chzex, chzey; fdtd_basic_coff.h) implied by the plan, in the reference's

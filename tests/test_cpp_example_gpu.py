@@ -61,3 +61,56 @@ def test_cpp_bistatic_example_matches_oracle(tmp_path, oracle_threads):
         assert err <= 1e-10, (name, err)
     assert np.array_equal(np.load(out / "theta.npy"), np.array([math.pi * i / 180 for i in range(180)]))
     assert np.array_equal(np.load(out / "frequencies.npy"), np.array([1e9]))
+
+
+EXE_DIEL = os.path.join(ROOT, "xfdtd_amd", "examples", "diel_sphere_scatter")
+STEPS_DIEL = 300
+
+
+def diel_example_scene():
+    """examples/3d_diel_sphere_scatter.cpp:18-96 through the Python builder
+    (Simulation{dl} takes the default cfl 0.99)."""
+    dl = 7.5e-3
+    size = 2 * 0.1 + 18 * dl
+    sc = S.Scene(cell_size=dl, cfl=0.99)
+    sc.add_object(S.Object("domain", S.Cube((-size / 2,) * 3, (size,) * 3), S.Material.air()))
+    sc.add_object(S.Object("sphere", S.Sphere((0.0, 0.0, 0.0), 0.1), S.Material("sphere", 3.0, 2.0, 0.0, 0.0)))
+    sc.boundaries = [S.PML(o, 8) for o in range(6)]
+    lam = 20 * dl
+    tau = lam / 6e8
+    sc.tfsf = S.TFSF3D(18, 18, 18, 1.0, 0.0, 0.0, 0.0, S.GaussianWaveform(tau, 4.5 * tau, 1.0))
+    sc.nffft = S.NffftFd(15, 15, 15, [1e9], [math.pi * i / 180 for i in range(181)], [0.0])
+    return sc, tau
+
+
+def test_cpp_dielectric_sphere_example_matches_oracle(tmp_path, oracle_threads):
+    """The dielectric-sphere program: NF-FF output against the FP64 oracle
+    (1e-10 of each quantity's peak), and incident_fft.npy against the DFT of
+    the incident Gaussian sampled at t = j dt, j < T (tfsf.cpp:51-66 with
+    util/dft.h): sum_j g(j dt) exp(-2 pi i f j dt) dt."""
+    assert os.path.exists(EXE_DIEL), "build the library first (make -C xfdtd_amd/csrc)"
+    out = tmp_path / "diel"
+    print(f"running {EXE_DIEL} {out} {STEPS_DIEL} 64", flush=True)
+    r = subprocess.run([EXE_DIEL, str(out), str(STEPS_DIEL), "64"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "Dielectric Sphere Scatter" in r.stdout, r.stdout
+    sc, tau = diel_example_scene()
+    o = OracleSim(sc, STEPS_DIEL)
+    o.step(STEPS_DIEL)
+    ref = o.farfield()
+    dt = o.dt
+    o.close()
+    for name in ("e_theta", "e_phi", "h_theta", "h_phi", "power_theta", "power_phi"):
+        got = np.load(out / f"{name}.npy")
+        want = ref[name]
+        assert got.shape == want.shape, (name, got.shape, want.shape)
+        scale = np.abs(want).max()
+        assert scale > 0, name
+        err = np.abs(got - want).max() / scale
+        assert err <= 1e-10, (name, err)
+    inc = np.load(out / "incident_fft.npy")
+    t = np.arange(STEPS_DIEL) * dt
+    g = np.exp(-(((t - 4.5 * tau) / tau) ** 2))
+    want = np.array([np.sum(g * np.exp(-2j * math.pi * 1e9 * t)) * dt])
+    assert inc.shape == (1,) and inc.dtype == np.complex128
+    assert abs(inc[0] - want[0]) <= 1e-12 * abs(want[0]), (inc, want)

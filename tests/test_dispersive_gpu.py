@@ -217,3 +217,82 @@ def test_cpp_drude_sphere_monostatic_against_mie(tmp_path):
     assert abs(bias) < 1.5, err
     assert strong.sum() >= 10 and np.abs(err[strong]).mean() < 2.5, err
     assert abs(peak_f - peak_m) <= max(0.03 * peak_m, 1.01 * (f[1] - f[0]))
+
+
+def debye_eps(w, eps_s, eps_inf, tau, sigma):
+    """The Debye permittivity with conductivity in the exp(-i w t) convention
+    (Im eps > 0): eps_inf + (eps_s - eps_inf) / (1 - i w tau) + i sigma / (w eps_0)."""
+    return eps_inf + (eps_s - eps_inf) / (1 - 1j * w * tau) + 1j * sigma / (w * S.EPSILON_0)
+
+
+def _debye_run(tmp_path, steps, radii):
+    """The Debye-sphere program's backscatter cross-section and the Mie
+    series over the bins of 50 .. 350 MHz (ka 0.26 .. 1.8, the first lobe of
+    the backscatter)."""
+    import os
+    import subprocess
+    from test_broadband_gpu import _gaussian_spectrum
+    from test_dispersive_cpu import mie_sphere_backscatter
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "xfdtd_amd", "examples", "lorentz_sphere_monostatic_rcs")
+    out = tmp_path / f"debye{radii}"
+    print(f"running {exe} {out} {steps} 64 debye {radii}", flush=True)
+    r = subprocess.run([exe, str(out), str(steps), "64", "debye", str(radii)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    print(r.stdout.strip(), flush=True)
+    a = 0.25
+    dl = a / 30
+    pt, pp = np.load(out / "power_theta.npy"), np.load(out / "power_phi.npy")
+    assert np.isfinite(pt).all() and np.isfinite(pp).all()
+    Ns = 2 * steps - 1
+    dt = S.courant_dt(dl, 0.99)
+    f = np.linspace(-1 / dt / 2, 1 / dt / 2, Ns)
+    m = np.arange(Ns) - (steps - 1)
+    band = (f > 5e7) & (f < 3.5e8)
+    tau = 20 * dl / 6e8
+    einc = _gaussian_spectrum(steps, dt, tau, 4.5 * tau, m[band])
+    sigma = 4 * math.pi * (pt[band] + pp[band]) / (np.abs(einc) ** 2 / (2 * S.ETA_0))
+    w = 2 * math.pi * f[band]
+    eps = debye_eps(w, 1.16, 1.01, 6.497e-10, 2.95e-4)
+    mie = np.array([mie_sphere_backscatter(ww / S.C_0 * a, np.sqrt(e)) for ww, e in zip(w, eps)]) * math.pi * a * a
+    return f[band], sigma, mie
+
+
+def test_cpp_debye_sphere_monostatic_against_mie(tmp_path):
+    """The reference's Debye-sphere monostatic test (test/test_3d_fdtd.cpp:
+    268-321) through the C++ program: a 0.25 m sphere of
+    DebyMedium(1.16, 1.01, 6.497e-10, 2.95e-4) at dl = radius / 30, FP64,
+    against the Mie series of eps(w) = 1.01 + 0.15 / (1 - i w tau) +
+    i sigma / (w eps_0) over the first lobe of the backscatter (50 .. 350
+    MHz, ka 0.26 .. 1.8; above it this weak scatterer's backscatter falls
+    to the level of the staircased surface's own scattering).
+
+    - In the reference's geometry (a 2.5-radius air cube, TFSF at index 22)
+      the TFSF surface runs through the sphere, so the scattered field holds
+      the incident wave's mismatch there; at the reference's 1000 steps the
+      curve runs below Mie (measured over the 10 bins: band total -2.2 dB,
+      bins -5.7 .. +1.6 dB).  Checked: the run, its peak inside the lobe,
+      the band total within 5 dB.
+    - With the sphere inside the total-field region (a 3.2-radius cube,
+      4000 steps so the fields have decayed; 38 bins): every bin within
+      1.5 dB (measured: worst 1.35, at the lowest bins), the band total
+      within 0.5 dB (measured -0.13) and the peak at the series' peak
+      frequency within 3 % or one bin (236 against 228 MHz)."""
+    f, sigma, mie = _debye_run(tmp_path, 1000, 2.5)
+    err = 10 * np.log10(sigma / mie)
+    bias = 10 * math.log10(sigma.sum() / mie.sum())
+    print(f"Debye sphere, reference geometry, vs Mie over {f.size} bins: band total {bias:+.2f} dB, "
+          f"bins {err.min():+.2f} .. {err.max():+.2f} dB", flush=True)
+    assert abs(bias) < 5, err
+    assert f[0] < f[np.argmax(sigma)] < f[-1]
+    f, sigma, mie = _debye_run(tmp_path, 4000, 3.2)
+    err = 10 * np.log10(sigma / mie)
+    bias = 10 * math.log10(sigma.sum() / mie.sum())
+    peak_f, peak_m = f[np.argmax(sigma)], f[np.argmax(mie)]
+    print(f"Debye sphere inside the TF region vs Mie over {f.size} bins: band total {bias:+.2f} dB, "
+          f"worst {np.abs(err).max():.2f} dB; peak {peak_f / 1e6:.1f} MHz (Mie {peak_m / 1e6:.1f})", flush=True)
+    assert f.size >= 30
+    assert np.abs(err).max() < 1.5, err
+    assert abs(bias) < 0.5, err
+    assert abs(peak_f - peak_m) <= max(0.03 * peak_m, 1.01 * (f[1] - f[0]))

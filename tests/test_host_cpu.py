@@ -270,3 +270,38 @@ def test_bad_lumped_and_monitor_placements():
     sc.sources = [S.Inductor(S.Cube((-half + dl, 0, 0), (dl, dl, dl)), S.AX_Z, 1e-9)]
     with pytest.raises(RuntimeError, match="inductor inside a CPML slab"):
         Simulation(sc, 10, plan_only=True)
+
+
+def test_check_run_writes_grid_and_object_descriptions(tmp_path, monkeypatch):
+    """Simulation::checkRun (simulation.cpp:98-113): the CPU set-up, then the
+    grid size in "simulation_check" and every object's description
+    (object.cpp:43-47, cube.cpp:16-21, sphere.cpp:17-21, material.cpp:18-27,
+    numbers in std::to_string's %f) in "simulation_object_check"."""
+    from xfdtd_amd.sim import Simulation
+    dl = 5e-3
+    sc = S.Scene(cell_size=dl, cfl=0.9)
+    sc.add_object(S.Object("domain", S.Cube((-0.05, -0.04, -0.03), (0.1, 0.08, 0.06)), S.Material.air()))
+    sc.add_object(S.Object("ball", S.Sphere((0.01, 0.0, -0.005), 0.02), S.Material("glass", 2.25, 1.0, 0.01, 0.0)))
+    s = Simulation(sc, 50, plan_only=True)
+    monkeypatch.chdir(tmp_path)
+    s.check_run()
+    nx, ny, nz = s.dims
+    assert (tmp_path / "simulation_check").read_text() == f"nx:{nx} ny:{ny} nz:{nz}\nTotal size:{nx * ny * nz}\n"
+
+    def f(v):
+        return "%f" % v
+
+    def mat(name, eps_r, mu_r, se, sm):
+        e0, m0 = 8.854187817e-12, 4e-7 * math.pi
+        er, mr = float(np.float32(eps_r)), float(np.float32(mu_r))
+        return (f"Material: {name}\neps_r: {f(er)}\nmu_r: {f(mr)}\neps: {f(er * e0)}\nmu: {f(mr * m0)}\n"
+                f"sigma_e: {f(se)}\nsigma_m: {f(sm)}\nis_dispersion: 0")
+    # the C API names the objects it builds by shape and their materials
+    # "material" (xf_sim_add_cube / xf_sim_add_sphere)
+    want = ("Object:\n"
+            "Object: cube\nCube: " + " ".join(f(v) for v in (-0.05, -0.04, -0.03, 0.1, 0.08, 0.06)) + "\n"
+            + mat("material", 1, 1, 0, 0) + "\n"
+            "Object: sphere\nSphere: " + " ".join(f(v) for v in (0.01, 0.0, -0.005, 0.02)) + "\n"
+            + mat("material", 2.25, 1.0, 0.01, 0.0) + "\n")
+    assert (tmp_path / "simulation_object_check").read_text() == want
+    s.close()

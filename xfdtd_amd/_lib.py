@@ -36,7 +36,7 @@ ENGINE_SYMBOLS = [
 SIM_SYMBOLS = [
     "xf_sim_last_error", "xf_sim_create", "xf_sim_destroy", "xf_sim_add_cube", "xf_sim_add_sphere",
     "xf_sim_add_pml", "xf_sim_set_tfsf3d", "xf_sim_add_dispersive_object", "xf_sim_add_cylinder", "xf_sim_set_tfsf2d", "xf_sim_set_nffft_fd", "xf_sim_add_current_source",
-    "xf_sim_add_probe", "xf_sim_plan", "xf_sim_plan_coeff", "xf_sim_init", "xf_sim_step", "xf_sim_sync", "xf_sim_current_step",
+    "xf_sim_add_probe", "xf_sim_plan", "xf_sim_check_run", "xf_sim_plan_coeff", "xf_sim_init", "xf_sim_step", "xf_sim_sync", "xf_sim_current_step",
     "xf_sim_dims", "xf_sim_dt", "xf_sim_get_field", "xf_sim_set_field", "xf_sim_nffft_size",
     "xf_sim_get_nffft", "xf_sim_farfield", "xf_sim_output_nffft", "xf_sim_get_probes",
     "xf_sim_engine", "xf_sim_set_design", "xf_sim_set_dft_plane", "xf_sim_get_dft", "xf_sim_grad_eps",
@@ -79,6 +79,7 @@ def lib():
     L.xf_sim_add_probe.argtypes = [vp, i, sz, sz, sz]
     L.xf_sim_init.argtypes = [vp, sz, i]
     L.xf_sim_plan.argtypes = [vp, sz]
+    L.xf_sim_check_run.argtypes = [vp, sz]
     L.xf_sim_plan_coeff.argtypes = [vp, i, dp]
     L.xf_sim_step.argtypes = [vp, sz]
     L.xf_sim_sync.argtypes = [vp]

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iostream>
 
 #include "xfdtd/xfdtd.hpp"
 #include "xfdtd_engine.h"
@@ -1456,6 +1457,17 @@ size_t Simulation::currentTimeStep() const {
   return _engine ? (size_t)xf_engine_current_step(_engine) : 0;
 }
 
+void Simulation::checkRun(size_t total_time_steps) {
+  std::cout << "Simulation Check:" << '\n';
+  if (!_plan) plan(total_time_steps);
+  std::ofstream ofs{"simulation_check", std::ios::out};
+  ofs << "nx:" << _n[0] << " ny:" << _n[1] << " nz:" << _n[2] << '\n';
+  ofs << "Total size:" << _n[0] * _n[1] * _n[2] << '\n';
+  std::ofstream object_ofs{"simulation_object_check", std::ios::out};
+  object_ofs << "Object:" << '\n';
+  for (const auto &o : _objects) object_ofs << static_cast<std::string>(*o) << '\n';
+}
+
 void Simulation::run(size_t total_time_steps) {
   init(total_time_steps);
   step(total_time_steps);
@@ -1656,6 +1668,24 @@ void makeDirs(const std::string &path) {
 }
 
 }  // namespace
+
+namespace {
+std::vector<size_t> npyShape(size_t n, const std::vector<size_t> &shape) {
+  if (shape.empty()) return {n};
+  size_t m = 1;
+  for (size_t s : shape) m *= s;
+  if (m != n) throw std::invalid_argument("dumpNpy: shape holds " + std::to_string(m) + " values, data " + std::to_string(n));
+  return shape;
+}
+}  // namespace
+
+void dumpNpy(const std::string &path, const std::vector<double> &v, const std::vector<size_t> &shape) {
+  writeNpy(path, v.data(), "<f8", 8, npyShape(v.size(), shape));
+}
+
+void dumpNpy(const std::string &path, const std::vector<std::complex<double>> &v, const std::vector<size_t> &shape) {
+  writeNpy(path, v.data(), "<c16", 16, npyShape(v.size(), shape));
+}
 
 // grid boxes of the monitors (GridSpace::getGridBox, grid_space.cpp:306-329)
 // and the plane check of TimeDomainFieldMonitor::init (field_monitor.cpp:30-87)

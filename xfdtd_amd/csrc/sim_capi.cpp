@@ -349,6 +349,10 @@ extern "C" int xf_sim_plan(xf_sim *s, size_t total_steps) {
   return guarded([&] { s->sim.plan(total_steps); });
 }
 
+extern "C" int xf_sim_check_run(xf_sim *s, size_t total_steps) {
+  return guarded([&] { s->sim.checkRun(total_steps); });
+}
+
 extern "C" int xf_sim_plan_coeff(xf_sim *s, int which, double *out) {
   return guarded([&] {
     auto v = s->sim.planCoefficient(which);

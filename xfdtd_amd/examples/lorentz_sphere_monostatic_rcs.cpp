@@ -9,9 +9,11 @@
 // -z, the default cfl; then the incident wave's FFT.  With the medium
 // argument `drude` it runs testDrudeSphereMonostaticRCS (test_3d_fdtd.cpp:
 // 213-270) instead: radius 3.75 mm and DrudeMedium(1, 1.8e11, 2e10), the
-// rest alike.
+// rest alike; with `debye`, testDebySphereMonostaticRCS (test_3d_fdtd.cpp:
+// 268-321): radius 0.25 m and DebyMedium(1.16, 1.01, 6.497e-10, 2.95e-4).
 //
-// usage: lorentz_sphere_monostatic_rcs [output_dir] [total_time_steps] [precision] [lorentz|drude]
+// usage: lorentz_sphere_monostatic_rcs [output_dir] [total_time_steps] [precision] [lorentz|drude|debye]
+//                                      [domain edge in radii, default 2.5]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -39,11 +41,19 @@ int main(int argc, char **argv) {
   const std::string out_dir = argc > 1 ? argv[1] : "./lorentz_sphere";
   const size_t total_time_steps = argc > 2 ? std::strtoul(argv[2], nullptr, 10) : 1000;
   const int precision = argc > 3 ? std::atoi(argv[3]) : 64;
-  const bool drude = argc > 4 && std::string(argv[4]) == "drude";
+  const std::string kind = argc > 4 ? argv[4] : "lorentz";
+  if (kind != "lorentz" && kind != "drude" && kind != "debye") {
+    std::fprintf(stderr, "unknown medium %s (lorentz, drude or debye)\n", kind.c_str());
+    return 2;
+  }
 
-  const double radius = drude ? 3.75e-3 : 15e-9;
+  const double radius = kind == "drude" ? 3.75e-3 : kind == "debye" ? 0.25 : 15e-9;
   const double dl = radius / 30;
-  const double domain_size = radius * 2.5;
+  // the air cube's edge in radii (the reference's 2.5 puts the TFSF surface
+  // at index 22 inside the sphere; 3 or more keep the sphere in the total
+  // field region)
+  const double radii = argc > 5 ? std::atof(argv[5]) : 2.5;
+  const double domain_size = radius * radii;
   const double domain_origin = -domain_size / 2;
   std::vector<std::shared_ptr<Object>> objects;
   objects.emplace_back(std::make_shared<Object>(
@@ -52,8 +62,10 @@ int main(int argc, char **argv) {
                              PointVector{domain_size, domain_size, domain_size}),
       Material{"air", 1, 1, 0, 0}));
   std::unique_ptr<Material> medium;
-  if (drude)
+  if (kind == "drude")
     medium = std::make_unique<xfdtd::DrudeMedium>("random", 1, 1.8e11, 2e10);
+  else if (kind == "debye")
+    medium = std::make_unique<xfdtd::DebyMedium>("random", 1.16, 1.01, 6.497e-10, 2.95e-4);
   else
     medium = std::make_unique<LorentzMedium>("random", 2.25, 1, 4e16, 0.28e16);
   objects.emplace_back(std::make_shared<Object>("scatter", std::make_unique<Sphere>(PointVector{0, 0, 0}, radius),

@@ -195,6 +195,11 @@ class Simulation:
         return [(nx, ny + 1, nz + 1), (nx + 1, ny, nz + 1), (nx + 1, ny + 1, nz),
                 (nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)][comp]
 
+    def check_run(self):
+        """Simulation::checkRun: write "simulation_check" (grid size) and
+        "simulation_object_check" (the objects) in the working directory."""
+        self._chk(lib().xf_sim_check_run(self.h, self.T))
+
     def plan_coeff(self, which):
         """Per-cell coefficient array `which` implied by the CPU plan."""
         out = np.empty(self.field_shape(which // 3), dtype=np.float64)
