@@ -589,6 +589,26 @@ class Monitor {
 // monitor/field_monitor.h:14-38, field_monitor.cpp:15-121: one component
 // over the grid box of `shape` (one cell thick across the plane), dumped as
 // .npy (float64, C order) by outputData
+class Simulation;
+
+// A monitor of the caller's own: the reference's Monitor virtuals init /
+// update / outputData (monitor/monitor.h:28-37) as host-side hooks.  init(sim)
+// runs once the engine holds the scene, update(sim, t) after every time step
+// t (the step's fields readable through sim.getField / sim.getFieldBox),
+// outputData() from Simulation::outputData.  The engine waits for each step
+// while a host monitor is present, so on large grids read boxes, not whole
+// fields.  (Boundaries have no such hook: the CPML runs inside the curl
+// kernels, and a host-side updateE / updateH would move the fields across
+// PCIe twice a step; addBoundary takes PML only.)
+class HostMonitor : public Monitor {
+ public:
+  explicit HostMonitor(std::string output_dir_path = "", std::string output_file_name = "")
+      : Monitor(nullptr, std::move(output_dir_path), std::move(output_file_name)) {}
+  virtual void init(const Simulation &) {}
+  virtual void update(const Simulation &sim, size_t time_step) = 0;
+  virtual void outputData() {}
+};
+
 class TimeDomainFieldMonitor : public Monitor {
  public:
   TimeDomainFieldMonitor(std::unique_ptr<Cube> shape, PlaneType plane_type, EMComponent component,
@@ -707,7 +727,8 @@ class Network {
 };
 
 // .npy files (xt::dump_npy in the reference): float64 or complex128, C
-// order, shape {v.size()} when `shape` is empty
+// order, shape {v.size()} when `shape` is empty; the file's directory is
+// created when missing
 void dumpNpy(const std::string &path, const std::vector<double> &v, const std::vector<size_t> &shape = {});
 void dumpNpy(const std::string &path, const std::vector<std::complex<double>> &v,
              const std::vector<size_t> &shape = {});
@@ -805,6 +826,9 @@ class Simulation {
   SpatialIndex getNy() const { return _n[1]; }
   SpatialIndex getNz() const { return _n[2]; }
   std::vector<double> getField(EMComponent c) const;
+  // component c over the grid-index box [box[0],box[1]) x [box[2],box[3]) x
+  // [box[4],box[5]), C order (only the box crosses to the host)
+  std::vector<double> getFieldBox(EMComponent c, const std::array<size_t, 6> &box) const;
   void setField(EMComponent c, const std::vector<double> &v);
   std::array<size_t, 3> fieldShape(EMComponent c) const;
 

@@ -114,3 +114,22 @@ def test_cpp_dielectric_sphere_example_matches_oracle(tmp_path, oracle_threads):
     want = np.array([np.sum(g * np.exp(-2j * math.pi * 1e9 * t)) * dt])
     assert inc.shape == (1,) and inc.dtype == np.complex128
     assert abs(inc[0] - want[0]) <= 1e-12 * abs(want[0]), (inc, want)
+
+
+def test_cpp_host_monitor_hook(tmp_path):
+    """xfdtd::HostMonitor (the reference's Monitor::init / update /
+    outputData virtuals, monitor/monitor.h:28-37) on the host: the example's
+    Ez recorder reads one cell through Simulation::getFieldBox after every
+    step; its series equals the engine's point probe at that cell bit for
+    bit, one value per step."""
+    exe = os.path.join(ROOT, "xfdtd_amd", "examples", "host_monitor_probe")
+    assert os.path.exists(exe), "build the library first (make -C xfdtd_amd/csrc)"
+    out = tmp_path / "hm"
+    steps = 150
+    print(f"running {exe} {out} {steps} 64", flush=True)
+    r = subprocess.run([exe, str(out), str(steps), "64"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    host, probe = np.load(out / "host_ez.npy"), np.load(out / "probe_ez.npy")
+    assert host.shape == (steps,) and probe.shape == (steps,)
+    assert np.abs(probe).max() > 0
+    assert np.array_equal(host, probe)

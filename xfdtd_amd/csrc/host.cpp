@@ -1376,6 +1376,8 @@ void Simulation::init(size_t total_time_steps) {
   if (!_plan) plan(total_time_steps);
   else if (_total_time_steps != total_time_steps) throw std::runtime_error("plan made for another step count");
   upload();
+  for (const auto &mp : _monitors)
+    if (auto *hm = dynamic_cast<HostMonitor *>(mp.get())) hm->init(*this);
 }
 
 // per-cell value of coefficient array `which` (cexe, cexhy, ..., chzey order of
@@ -1429,8 +1431,11 @@ void Simulation::step(size_t n) {
   if (!_engine) throw std::runtime_error("Simulation is not initialised");
   size_t cur = currentTimeStep();
   const size_t end = cur + n;
+  std::vector<HostMonitor *> host;
+  for (const auto &mp : _monitors)
+    if (auto *hm = dynamic_cast<HostMonitor *>(mp.get())) host.push_back(hm);
   while (cur < end) {
-    size_t next = end;  // one past the next sample step
+    size_t next = host.empty() ? end : cur + 1;  // one past the next sample step
     for (const MonitorPlan &m : _monitor_plans) {
       if (m.plot_step == 0) continue;
       size_t t = ((cur + m.plot_step - 1) / m.plot_step) * m.plot_step;  // first multiple >= cur
@@ -1445,6 +1450,10 @@ void Simulation::step(size_t n) {
       char name[32];
       std::snprintf(name, sizeof name, "%04zu.npy", t);
       writeMonitor(m, name);
+    }
+    if (!host.empty()) {
+      sync();
+      for (HostMonitor *hm : host) hm->update(*this, t);
     }
   }
 }
@@ -1492,6 +1501,18 @@ std::vector<double> Simulation::getField(EMComponent c) const {
   const auto sh = fieldShape(c);
   std::vector<double> out(sh[0] * sh[1] * sh[2]);
   check(xf_engine_get_field(_engine, compIndex(c), out.data()));
+  return out;
+}
+
+std::vector<double> Simulation::getFieldBox(EMComponent c, const std::array<size_t, 6> &box) const {
+  if (!_engine) throw std::runtime_error("Simulation is not initialised");
+  const auto sh = fieldShape(c);
+  for (int a = 0; a < 3; ++a)
+    if (box[2 * a] > box[2 * a + 1] || box[2 * a + 1] > sh[a]) throw std::out_of_range("field box outside the component");
+  const int64_t b[6] = {(int64_t)box[0], (int64_t)(box[1] - box[0]), (int64_t)box[2], (int64_t)(box[3] - box[2]),
+                        (int64_t)box[4], (int64_t)(box[5] - box[4])};
+  std::vector<double> out((size_t)b[1] * b[3] * b[5]);
+  check(xf_engine_get_box(_engine, compIndex(c), b, out.data()));
   return out;
 }
 
@@ -1680,11 +1701,15 @@ std::vector<size_t> npyShape(size_t n, const std::vector<size_t> &shape) {
 }  // namespace
 
 void dumpNpy(const std::string &path, const std::vector<double> &v, const std::vector<size_t> &shape) {
-  writeNpy(path, v.data(), "<f8", 8, npyShape(v.size(), shape));
+  const auto sh = npyShape(v.size(), shape);
+  if (path.find('/') != std::string::npos) makeDirs(path.substr(0, path.rfind('/')));
+  writeNpy(path, v.data(), "<f8", 8, sh);
 }
 
 void dumpNpy(const std::string &path, const std::vector<std::complex<double>> &v, const std::vector<size_t> &shape) {
-  writeNpy(path, v.data(), "<c16", 16, npyShape(v.size(), shape));
+  const auto sh = npyShape(v.size(), shape);
+  if (path.find('/') != std::string::npos) makeDirs(path.substr(0, path.rfind('/')));
+  writeNpy(path, v.data(), "<c16", 16, sh);
 }
 
 // grid boxes of the monitors (GridSpace::getGridBox, grid_space.cpp:306-329)
@@ -1765,7 +1790,9 @@ void Simulation::planMonitors() {
   planLineMonitors();
   for (const auto &mp : _monitors) {
     const Monitor *mon = mp.get();
-    if (dynamic_cast<const VoltageMonitor *>(mon) || dynamic_cast<const CurrentMonitor *>(mon)) continue;
+    if (dynamic_cast<const VoltageMonitor *>(mon) || dynamic_cast<const CurrentMonitor *>(mon) ||
+        dynamic_cast<const HostMonitor *>(mon))
+      continue;
     size_t plot = 0, total = 0;
     if (auto *mv = dynamic_cast<const MovieMonitor *>(mon)) {
       if (mv->getPlotStep() == 0) throw std::invalid_argument("MovieMonitor plot_step must be positive");
@@ -1776,7 +1803,7 @@ void Simulation::planMonitors() {
     auto *fm = dynamic_cast<const TimeDomainFieldMonitor *>(mon);
     if (!fm)
       throw std::runtime_error(
-          "only TimeDomainFieldMonitor, MovieMonitor, VoltageMonitor and CurrentMonitor are supported");
+          "only TimeDomainFieldMonitor, MovieMonitor, VoltageMonitor, CurrentMonitor and HostMonitor are supported");
     const Cube *c = fm->getShape();
     if (!c) throw std::runtime_error("Shape Type Error");
     const double lo[3] = {c->getXmin(), c->getYmin(), c->getZmin()};
@@ -1825,6 +1852,8 @@ void Simulation::outputData() const {
   // simulation.cpp:144-156: NF-FF, then the monitors, then the network
   outputNffft();
   if (_engine) {
+    for (const auto &mp : _monitors)
+      if (auto *hm = dynamic_cast<HostMonitor *>(mp.get())) hm->outputData();
     for (const MonitorPlan &m : _monitor_plans)
       if (m.plot_step == 0) writeMonitor(m, m.monitor->getOutputFileName());
     if (!_line_plans.empty()) {
