@@ -276,9 +276,9 @@ xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags);
    swaps the copies after each step).  It applies to 3-D engines without
    design boxes, dispersive media or lumped elements that are not in a
    batch; others keep the split kernels.  rows: j rows per thread of the
-   fused kernel (1, 2 or 4; 0: 1); planes: x planes per workgroup (1 to 32;
-   0: 8).  mode | 16: tile order by workgroup id instead of one contiguous
-   run of tiles per XCD.  Results are bit-identical for every choice. */
+   fused kernel (1, 2 or 4; 0: 2); planes: x planes per workgroup (1 to 32;
+   0: 16).  mode | 16: one contiguous run of tiles per XCD instead of tile
+   order by workgroup id.  Results are bit-identical for every choice. */
 xf_status xf_engine_set_sweep(xf_engine *e, int mode, int rows, int planes);
 /* 1 when the next step runs the fused kernel, else 0 */
 int xf_engine_sweep_active(const xf_engine *e);

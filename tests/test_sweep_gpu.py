@@ -6,6 +6,10 @@ oracle and against the split E/H kernels.
   32 x planes per workgroup), on a PEC sphere with TFSF and
   NF-FF at oblique incidence and on seeded random fields with kappa > 1 CPML
   on every face (every wall and slab range, no source).
+- Fused tiles come in three kinds (plain, z slab and faces only, any):
+  a 144^3 scene that has all three is bit-identical to the oracle in FP64,
+  and the 400^3 dielectric-sphere run from seeded random fields is
+  bit-identical to the split kernels in FP64.
 - FP32: fused and split runs bit-identical.
 - The double-buffered state: a checkpoint taken after an odd number of fused
   steps restores bit for bit; switching between the fused and split schemes
@@ -104,6 +108,50 @@ def test_fused_random_fields_cpml_all_faces_bitwise(rows, planes, oracle_threads
     o.step(steps)
     for c in range(6):
         assert np.array_equal(g.get_field(c), o.get_field(c)), c
+
+
+@pytest.mark.parametrize("rows,planes", [(0, 0), (4, 8), (1, 32)])
+def test_fused_all_tile_kinds_against_oracle(rows, planes, oracle_threads):
+    # 144^3 with 8-cell CPML: the TFSF faces sit at 12 and 132, so the k chunk
+    # [64, 128) and the x and y ranges between the faces hold plain tiles
+    from xfdtd_amd.workloads import dielectric_sphere_scene
+    sc = dielectric_sphere_scene(144, pml=8)
+    steps = 40
+    o = OracleSim(sc, steps)
+    g = _sim(sc, steps)
+    g.set_sweep(1, rows, planes)
+    assert g.sweep_active
+    rng = np.random.default_rng(144)
+    for c in range(6):
+        f = rng.standard_normal(o.field_shape(c))
+        o.set_field(c, f)
+        g.set_field(c, f)
+    print(f"fused step, rows {rows}, planes {planes}: 144^3 dielectric sphere from random fields, FP64", flush=True)
+    g.step(steps)
+    o.step(steps)
+    _same(_state(g), _state(o))
+
+
+def test_fused_400_fp64_equals_split():
+    from xfdtd_amd.workloads import dielectric_sphere_scene
+    sc = dielectric_sphere_scene(400)
+    steps = 12
+    rng = np.random.default_rng(400)
+    runs = []
+    fields = None
+    for mode in (1, 0):
+        g = _sim(sc, steps)
+        g.set_sweep(mode)
+        assert g.sweep_active == (mode == 1)
+        if fields is None:
+            fields = [rng.standard_normal(g.field_shape(c)) for c in range(6)]
+        for c in range(6):
+            g.set_field(c, fields[c])
+        print(f"400^3 dielectric sphere, FP64, {'fused' if mode else 'split'} scheme, {steps} steps", flush=True)
+        g.step(steps)
+        runs.append(_state(g))
+        g.close()
+    _same(runs[0], runs[1])
 
 
 def test_fused_fp32_equals_split():

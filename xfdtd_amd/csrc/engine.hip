@@ -161,7 +161,7 @@ struct xf_engine {
   // fused E+H step (k_sweep): off by default (the split kernels are faster
   // on MI355X); rows of j per thread, x planes per workgroup (0: defaults)
   int sweep = 0, sweep_rows = 0, sweep_tx = 0;
-  int sweep_flags = 0;  // 1: tile numbers follow workgroup ids (no XCD runs)
+  int sweep_flags = 0;  // 1: each XCD takes a contiguous run of tiles
   // divergence check: the H kernels lower *div to the first step whose H
   // came out non-finite; the host reads it every div_every steps (0: never)
   // and then refuses to step on (div_step >= 0) until a restore
@@ -188,6 +188,10 @@ struct xf_engine {
   long long psi_off[12] = {}, psi_n = 0;
   void *e_inc_alt = nullptr, *h_inc_alt = nullptr;
   TfTerms tf_terms{};              // the step's TFSF face terms (fused step)
+  // fused-step tile lists, one run per kind (SW_CORE, SW_ZONLY, SW_ANY) in
+  // one device array, built for rows/planes (sp_R, sp_tx)
+  void *sp_tiles = nullptr;
+  int sp_R = 0, sp_tx = 0, sp_off[3] = {}, sp_cnt[3] = {};
 };
 
 // wait for everything enqueued for this engine, on its own stream and on
@@ -1105,10 +1109,10 @@ static void launch_line_monitors(const xf_engine *e, const Grid<T> &g, hipStream
 // addresses a whole component with 32-bit offsets).
 static int sweep_rows(const xf_engine *e) {
   if (e->sweep_rows > 0) return e->sweep_rows;
-  return 1;
+  return 2;
 }
 
-static SweepTiling make_sweep(const xf_engine *e);
+static int sweep_tx(const xf_engine *e) { return e->sweep_tx > 0 ? e->sweep_tx : 16; }
 
 // material slots per LDS record of the fused step (4 or 8; more materials
 // take the split kernels)
@@ -1125,7 +1129,7 @@ static size_t sweep_lds_bytes(const xf_engine *e) {
 
 static bool fused_ok(const xf_engine *e) {
   return e->sweep == 1 && e->nz > 1 && (unsigned long long)e->ncell * e->esz < (1ULL << 32) && !e->has_design && !e->has_drude && e->sources.empty() &&
-         e->batch_stream == nullptr && e->nmat <= 8 && make_sweep(e).tx <= SW_TXMAX &&
+         e->batch_stream == nullptr && e->nmat <= 8 && sweep_tx(e) <= SW_TXMAX &&
          sweep_lds_bytes(e) <= 60 * 1024;
 }
 
@@ -1138,17 +1142,90 @@ static xf_status alloc_lines_alt(xf_engine *e) {
   return XF_OK;
 }
 
-static SweepTiling make_sweep(const xf_engine *e) {
-  SweepTiling st;
-  st.waves = SW_WAVES;
-  st.tx = e->sweep_tx > 0 ? e->sweep_tx : 8;
-  const int R = sweep_rows(e);
-  st.nkc = (e->nz + 1 + 63) / 64;
-  st.njg = (e->ny + 1 + st.waves * R - 1) / (st.waves * R);
-  st.nic = (e->nx + 1 + st.tx - 1) / st.tx;
-  st.nblocks = st.nkc * st.njg * st.nic;
-  st.xcd_runs = (e->sweep_flags & 1) ? 0 : 1;
-  return st;
+// Does index n of an axis of N cells need the CPML or TFSF code of the fused
+// step: as an E index (E layers [1, dn] and [N-dp, N-1], TFSF E faces lo and
+// hi) or as an H index (H layers [0, dn) and [N-dp, N), TFSF H faces lo - 1
+// and hi)?
+static bool e_special(int n, int N, int dn, int dp, bool tf, int lo, int hi) {
+  return (n >= 1 && n <= dn) || (dp > 0 && n >= N - dp && n <= N - 1) || (tf && (n == lo || n == hi));
+}
+static bool h_special(int n, int N, int dn, int dp, bool tf, int lo, int hi) {
+  return (n >= 0 && n < dn) || (dp > 0 && n >= N - dp && n < N) || (tf && (n == lo - 1 || n == hi));
+}
+
+// Chunks of at most `len` indices over [0, n): runs of indices that need the
+// special code (as E or H index, or whose successor needs it as an E index,
+// since a chunk also computes E one index past its end) and runs that do not
+// are chunked separately, so that plain chunks stay plain.
+static std::vector<std::pair<int, int>> sweep_chunks(int n, int len, int N, int dn, int dp, bool tf, int lo, int hi) {
+  std::vector<std::pair<int, int>> out;
+  auto bad = [&](int q) {
+    return e_special(q, N, dn, dp, tf, lo, hi) || h_special(q, N, dn, dp, tf, lo, hi) ||
+           e_special(q + 1, N, dn, dp, tf, lo, hi);
+  };
+  int a = 0;
+  while (a < n) {
+    const bool b0 = bad(a);
+    int b = a + 1;
+    while (b < n && bad(b) == b0) ++b;
+    const int m = (b - a + len - 1) / len;  // chunks of this run, balanced
+    for (int q = 0; q < m; ++q) out.push_back({a + (int)((long long)(b - a) * q / m), a + (int)((long long)(b - a) * (q + 1) / m)});
+    a = b;
+  }
+  return out;
+}
+
+// Build the fused step's tile lists for R rows per wave and tx planes per
+// chunk: x chunks, j groups of SW_WAVES * R rows and 64-cell k chunks, each
+// tile sorted into the kind whose code it needs.
+static xf_status build_sweep_tiles(xf_engine *e) {
+  const int R = sweep_rows(e), tx = sweep_tx(e);
+  if (e->sp_tiles && e->sp_R == R && e->sp_tx == tx) return XF_OK;
+  const bool tf = e->has_tfsf;
+  const int G = SW_WAVES * R;
+  const auto xc = sweep_chunks(e->nx + 1, tx, e->nx, e->dn[0], e->dp[0], tf, e->tli, e->tri);
+  const auto yc = sweep_chunks(e->ny + 1, G, e->ny, e->dn[1], e->dp[1], tf, e->tlj, e->trj);
+  const int nkc = (e->nz + 1 + 63) / 64;
+  std::vector<SweepTile> lists[3];
+  for (const auto &x : xc)
+    for (const auto &y : yc) {
+      // x: E planes [i0, i1] (the last one a halo plane), H planes [i0, i1);
+      // y: E rows up to the top wave's halo row, H rows below it
+      bool sx = false, sy = false;
+      for (int p = x.first; p <= x.second; ++p)
+        sx |= e_special(p, e->nx, e->dn[0], e->dp[0], tf, e->tli, e->tri) ||
+              (p < x.second && h_special(p, e->nx, e->dn[0], e->dp[0], tf, e->tli, e->tri));
+      const int rows = y.second - y.first, top = y.first + (rows + R - 1) / R * R;
+      for (int j = y.first; j <= top; ++j)
+        sy |= e_special(j, e->ny, e->dn[1], e->dp[1], tf, e->tlj, e->trj) ||
+              (j < top && h_special(j, e->ny, e->dn[1], e->dp[1], tf, e->tlj, e->trj));
+      for (int kc = 0; kc < nkc; ++kc) {
+        bool sz = false;
+        for (int k = 64 * kc; k <= 64 * kc + 64; ++k)
+          sz |= e_special(k, e->nz, e->dn[2], e->dp[2], tf, e->tlk, e->trk) ||
+                (k < 64 * kc + 64 && h_special(k, e->nz, e->dn[2], e->dp[2], tf, e->tlk, e->trk));
+        const int kind = (sx || sy) ? SW_ANY : sz ? SW_ZONLY : SW_CORE;
+        lists[kind].push_back(SweepTile{x.first, x.second, y.first, kc | rows << 16});
+      }
+    }
+  std::vector<SweepTile> all;
+  for (int q = 0; q < 3; ++q) {
+    e->sp_off[q] = (int)all.size();
+    e->sp_cnt[q] = (int)lists[q].size();
+    all.insert(all.end(), lists[q].begin(), lists[q].end());
+  }
+  if (e->sp_tiles) {
+    for (auto &p : e->owned)
+      if (p == e->sp_tiles) p = nullptr;
+    HIPCHK(hipFree(e->sp_tiles));
+    e->sp_tiles = nullptr;
+  }
+  xf_status s = dev_alloc(e, &e->sp_tiles, all.size() * sizeof(SweepTile));
+  if (s != XF_OK) return s;
+  HIPCHK(hipMemcpy(e->sp_tiles, all.data(), all.size() * sizeof(SweepTile), hipMemcpyHostToDevice));
+  e->sp_R = R;
+  e->sp_tx = tx;
+  return XF_OK;
 }
 
 // the second copies of the state the fused step double-buffers, zeroed
@@ -1218,7 +1295,7 @@ static void swap_sets(xf_engine *e) {
 // other; rebuilt at every stepping call (a few hundred bytes)
 // arguments of the fused step from the current set to the other one
 template <typename T>
-static SweepArgs<T> sweep_args(xf_engine *e, const SweepTiling &st) {
+static SweepArgs<T> sweep_args(xf_engine *e) {
   SweepArgs<T> h{};
   for (int q = 0; q < 6; ++q) {
     h.src[q] = (const T *)e->f[q];
@@ -1250,29 +1327,44 @@ static SweepArgs<T> sweep_args(xf_engine *e, const SweepTiling &st) {
   if (e->has_tfsf) {
     h.tb[0] = e->tli; h.tb[1] = e->tri; h.tb[2] = e->tlj; h.tb[3] = e->trj; h.tb[4] = e->tlk; h.tb[5] = e->trk;
   }
-  h.st = st;
+  h.st.tiles = nullptr;
+  h.st.nblocks = 0;
+  h.st.waves = SW_WAVES;
+  h.st.xcd_runs = (e->sweep_flags & 1) ? 1 : 0;
   h.div = (unsigned long long *)e->div;
   h.step = e->cur;
   return h;
 }
 
-template <typename T, int R>
-static void launch_sweep(xf_engine *e, const SweepTiling &st) {
-  const SweepArgs<T> a = sweep_args<T>(e, st);
-  const dim3 grd(st.nblocks), blk(64 * st.waves);
+template <typename T, int R, int KIND>
+static void launch_sweep_kind(xf_engine *e, SweepArgs<T> &a) {
+  if (e->sp_cnt[KIND] == 0) return;
+  a.st.tiles = (const SweepTile *)e->sp_tiles + e->sp_off[KIND];
+  a.st.nblocks = e->sp_cnt[KIND];
+  const dim3 grd(a.st.nblocks), blk(64 * SW_WAVES);
   const size_t lds = sweep_lds_bytes(e);
   if (sweep_ms(e) == 4)
-    hipLaunchKernelGGL((k_sweep<T, R, 4>), grd, blk, lds, e->stream, a);
+    hipLaunchKernelGGL((k_sweep<T, R, 4, KIND>), grd, blk, lds, e->stream, a);
   else
-    hipLaunchKernelGGL((k_sweep<T, R, 8>), grd, blk, lds, e->stream, a);
+    hipLaunchKernelGGL((k_sweep<T, R, 8, KIND>), grd, blk, lds, e->stream, a);
+}
+
+// the fused step: one launch per tile kind (all read set A and write set B,
+// so their order does not matter)
+template <typename T, int R>
+static void launch_sweep(xf_engine *e) {
+  SweepArgs<T> a = sweep_args<T>(e);
+  launch_sweep_kind<T, R, SW_ANY>(e, a);
+  launch_sweep_kind<T, R, SW_ZONLY>(e, a);
+  launch_sweep_kind<T, R, SW_CORE>(e, a);
 }
 
 template <typename T>
 static xf_status run_steps(xf_engine *e, long long n) {
   const bool fused = fused_ok(e);
-  const SweepTiling sw = fused ? make_sweep(e) : SweepTiling{};
   if (fused) {
     xf_status s = alloc_alt(e);
+    if (s == XF_OK) s = build_sweep_tiles(e);
     if (s != XF_OK) return s;
   }
   const int R = sweep_rows(e);
@@ -1329,11 +1421,11 @@ static xf_status run_steps(xf_engine *e, long long n) {
       }
       if (ev) hipEventRecord(ev[0], e->stream);
       if (R == 1)
-        launch_sweep<T, 1>(e, sw);
+        launch_sweep<T, 1>(e);
       else if (R == 2)
-        launch_sweep<T, 2>(e, sw);
+        launch_sweep<T, 2>(e);
       else
-        launch_sweep<T, 4>(e, sw);
+        launch_sweep<T, 4>(e);
       if (ev) {
         hipEventRecord(ev[1], e->stream);
         hipEventRecord(ev[2], e->stream);

@@ -12,8 +12,11 @@
 //   the 64 cells k0..k0+63 (k0 a multiple of 64, so its stores are whole
 //   cache lines).  The k-1 neighbour an E update needs and the k+1 neighbour
 //   an H update needs come from the adjacent lane by DPP (wave_shr /
-//   wave_shl); at the ends, lane 0 loads H at k0 - 1 and lane 63 recomputes
-//   Ex and Ey at k0 + 64 (the next wave's first cell) from its own loads.
+//   wave_shl).  The two wave ends are handled packed: lane r (r <= R) loads H
+//   at k0 - 1 for row j0 + r and computes the new Ex and Ey of the cell
+//   k0 + 64 above the wave for that row, so each plane costs one extra cell
+//   update per wave instead of one per row; lane 0 and lane 63 take row r's
+//   value from lane r.
 //   j runs over R rows held in registers by each thread, plus one halo row
 //   above (j0 + R) whose Ex and Ez are recomputed, and the H values of the row
 //   below (j0 - 1).
@@ -21,6 +24,13 @@
 //   from H(i), H(i + 1), then H(i) from E(i), E(i + 1), and stores E(i), H(i).
 //   The first pass of a chunk (i = i0 - 1) only computes E(i0); the last
 //   computes E(i1) as a halo plane.
+//
+// Tiles come in three kinds, each compiled on its own so that the plain one
+// carries none of the CPML and TFSF code (whose registers would otherwise
+// lower every tile's occupancy): SW_CORE tiles meet no CPML slab and no TFSF
+// face (their halo cells included), SW_ZONLY tiles meet them only along z
+// (the per-lane z-slab psi and the z faces), SW_ANY tiles take everything.
+// The host sorts the tiles into one list per kind and launches each list.
 //
 // The coefficient records, the CPML b/a values of the tile's planes and rows
 // and the addresses the CPML and TFSF branches need are staged in LDS once per
@@ -42,17 +52,26 @@
 
 namespace xfk {
 
-// launch geometry: workgroups of `waves` waves over tiles numbered k chunk
-// fastest, then j group, then i chunk.  The hardware deals workgroup ids to
-// the 8 XCDs round-robin and each XCD has its own L2; with xcd_runs each XCD
-// takes one contiguous run of tile numbers instead, so the tiles that share
-// halo rows and the cache lines at k-chunk ends run on the same L2 at about
-// the same time.
+// Tile kinds of the fused step (see above).
+enum { SW_CORE = 0, SW_ZONLY = 1, SW_ANY = 2 };
+
+// One workgroup's tile: x planes [i0, i1), rows [j0, j0 + rows) in waves of R
+// rows from j0, k chunk kc (cells 64 kc .. 64 kc + 63); kr = kc | rows << 16.
+struct alignas(16) SweepTile {
+  int i0, i1, j0, kr;
+};
+
+// launch geometry of one tile list: workgroups of `waves` waves.  The
+// hardware deals workgroup ids to the 8 XCDs round-robin and each XCD has its
+// own L2.  By default tiles follow workgroup ids (k chunk fastest, then j
+// group, then x chunk), so the whole chip works on neighbouring tiles at a
+// time; with xcd_runs each XCD takes one contiguous run of the list instead,
+// which keeps the rows and cache lines that neighbouring tiles share in one
+// L2 but measures 10-15% slower on the 400^3 grid.
 struct SweepTiling {
-  int nkc, njg, nic;
-  int tx;     // x planes per chunk
-  int waves;  // waves per workgroup, each a strip of R rows
+  const SweepTile *tiles;
   int nblocks;
+  int waves;  // waves per workgroup, each a strip of R rows
   int xcd_runs;
 };
 
@@ -139,12 +158,13 @@ __device__ __forceinline__ int cpml_layer(bool e_side, int n, int N, int dn, int
   return e_side ? e_layer(n, N, dn, dp) : h_layer(n, N, dn, dp);
 }
 
-// LDS layout of a workgroup's tables, in units of T, with MS material slots
-// per record: self records of both sides, curl pairs over the k slots (the
-// 64 lanes and the cell above), j slots (the workgroup's rows and the top
-// halo row) and i slots (up to SW_TXMAX planes and the halo plane), the same
-// for the psi pairs, then the x and y CPML b/a values.  All offsets are
-// compile-time constants.
+// LDS layout of a workgroup's tables, in units of T, with MS material slots:
+// the self records of both sides ([side][MS][4]); for each side and table kind
+// (curl, psi) the pairs over the k slots (the 64 lanes and the cell above) as
+// [MS][SW_KS], so that consecutive lanes read consecutive pairs, over the j
+// slots (the workgroup's rows and the top halo row) as [MS][js] and over the
+// i slots (up to SW_TXMAX planes and the halo plane) as [MS][is]; then the x
+// and y CPML b/a values.  All offsets are compile-time constants.
 constexpr int SW_KS = 65;
 constexpr int SW_WAVES = 4;
 constexpr int SW_TXMAX = 32;
@@ -153,9 +173,9 @@ struct SweepLds {
   static constexpr int js = SW_WAVES * R + 1, is = SW_TXMAX + 1;
   static constexpr int per = 2 * MS * (SW_KS + js + is);
   static constexpr int self(int side) { return side * 4 * MS; }
-  static constexpr int pair(int side, int psi, int axis) {
-    return 8 * MS + (2 * side + psi) * per + (axis >= 1 ? 2 * MS * SW_KS : 0) + (axis >= 2 ? 2 * MS * js : 0);
-  }
+  static constexpr int kpair(int side, int psi) { return 8 * MS + (2 * side + psi) * per; }
+  static constexpr int jpair(int side, int psi) { return kpair(side, psi) + 2 * MS * SW_KS; }
+  static constexpr int ipair(int side, int psi) { return jpair(side, psi) + 2 * MS * js; }
   static constexpr int ba_x = 8 * MS + 4 * per;  // [4][is]: be ae bm am
   static constexpr int ba_y = ba_x + 4 * is;     // [4][js]
   static constexpr int total = ba_y + 4 * js;
@@ -168,9 +188,352 @@ struct ZLay {
   T be, ae, bh, ah;
 };
 
+// value of v in lane `lane` (a compile-time constant after unrolling)
+__device__ __forceinline__ float lane_value(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// What the cell updates of one wave share: grid sizes, CPML extents, the
+// staged tables, the psi blocks and the TFSF box with the step's face terms.
 template <typename T, int R, int MS>
+struct SwCtx {
+  using L = SweepLds<R, MS>;
+  int nx, ny, nz, pz;
+  int dn1, dp1, Ly, Lz, nm1y;
+  int i0, jwg;
+  const T *lds;
+  const T *__restrict__ pA;
+  T *__restrict__ pB;
+  unsigned psl;
+  int li, ri, lj, rj, lk, rk;
+  long long nj, nk;
+  const double *terms;
+  unsigned tstr;
+  __device__ const P4<T> &self(int side, int m) const { return reinterpret_cast<const P4<T> *>(lds + L::self(side))[m]; }
+  __device__ const P2<T> &kp(int side, int psi, int m, int s) const {
+    return reinterpret_cast<const P2<T> *>(lds + L::kpair(side, psi))[m * SW_KS + s];
+  }
+  __device__ const P2<T> &jp(int side, int psi, int m, int s) const {
+    return reinterpret_cast<const P2<T> *>(lds + L::jpair(side, psi))[m * L::js + s];
+  }
+  __device__ const P2<T> &ip(int side, int psi, int m, int s) const {
+    return reinterpret_cast<const P2<T> *>(lds + L::ipair(side, psi))[m * L::is + s];
+  }
+  __device__ const double *seg(int s) const { return terms + (size_t)s * tstr; }
+};
+
+// One E cell (p, j, kk) of material m: Simulation::updateE + PML::updateE +
+// TFSF3D::updateE, in the order of compute_plane_e and tfsf_e_body, for the
+// components DX, DY, DZ.  d*: the curl differences; kslot: the cell's k slot
+// in the staged tables; own: this thread stores the cell's psi; slab: the
+// wave may meet a CPML slab, tfp: the plane may hold TFSF E faces (both
+// wave-uniform); lxe: the plane's E layer along x.  A SW_ZONLY tile has no
+// x or y slab and no x or y face, so only the z terms are compiled in.
+template <typename T, int R, int MS, int KIND, bool DX, bool DY, bool DZ>
+__device__ __forceinline__ void sw_e_cell(const SwCtx<T, R, MS> &c, int p, int j, int kk, int kslot, int m,
+                                          const ZLay<T> &z, T exv, T eyv, T ezv, T dhy_k, T dhz_j, T dhx_k,
+                                          T dhz_i, T dhx_j, T dhy_i, bool own, bool slab, bool tfp, int lxe, T &vx,
+                                          T &vy, T &vz) {
+  using L = SweepLds<R, MS>;
+  const int is = p - c.i0, js = j - c.jwg;
+  const bool kin = kk >= 1 && kk < c.nz, klt = kk < c.nz;
+  const bool jin = j >= 1 && j < c.ny, jlt = j < c.ny;
+  const bool ix = p >= 1 && p < c.nx, plt = p < c.nx;
+  const P4<T> cs = c.self(0, m);
+  P2<T> ck{}, cj{}, ci{};
+  if (DX || DY) ck = c.kp(0, 0, m, kslot);
+  if (DX || DZ) cj = c.jp(0, 0, m, js);
+  if (DY || DZ) ci = c.ip(0, 0, m, is);
+  if (DX) {
+    const T yx = cs.x * exv + ck.x * dhy_k + cj.x * dhz_j;
+    vx = (plt && jin && kin) ? yx : exv;
+  }
+  if (DY) {
+    const T yy = cs.y * eyv + ck.y * dhx_k + ci.x * dhz_i;
+    vy = (ix && jlt && kin) ? yy : eyv;
+  }
+  if (DZ) {
+    const T yz = cs.z * ezv + cj.y * dhx_j + ci.y * dhy_i;
+    vz = (ix && jin && klt) ? yz : ezv;
+  }
+  if (KIND != SW_CORE && slab) {
+    const int pl = min(p, c.nx);  // tables and psi of a plane beyond nx are never used
+    const unsigned jc = (unsigned)min(j, c.ny);
+    const unsigned psl = c.psl;
+    if constexpr (KIND == SW_ANY) {
+      const int lye = j <= c.ny ? cpml_layer(true, j, c.ny, c.dn1, c.dp1) : -1;
+      const T bye = c.lds[L::ba_y + js], aye = c.lds[L::ba_y + L::js + js];
+      const T bxe = c.lds[L::ba_x + is], axe = c.lds[L::ba_x + L::is + is];
+      const unsigned kq = (unsigned)min(kk, c.pz - 1);
+      if (DX && plt) {
+        if (lye >= 0) {
+          const unsigned o = 0 * psl + psi_y(pl, lye, kq, c.Ly, c.pz);
+          const T ps = bye * c.pA[o] + aye * dhz_j;
+          if (own) c.pB[o] = ps;
+          vx = vx + c.jp(0, 1, m, js).x * ps;
+        }
+        if (z.le >= 0) {
+          const unsigned o = 1 * psl + psi_z(pl, jc, z.le, c.nm1y, c.Lz);
+          const T ps = z.be * c.pA[o] + z.ae * dhy_k;
+          if (own) c.pB[o] = ps;
+          vx = vx + c.kp(0, 1, m, kslot).x * ps;
+        }
+      }
+      if (DY && jlt) {
+        if (lxe >= 0) {
+          const unsigned o = 2 * psl + psi_x(lxe, jc, kq, c.nm1y, c.pz);
+          const T ps = bxe * c.pA[o] + axe * dhz_i;
+          if (own) c.pB[o] = ps;
+          vy = vy + c.ip(0, 1, m, is).x * ps;
+        }
+        if (z.le >= 0) {
+          const unsigned o = 3 * psl + psi_z(pl, jc, z.le, c.nm1y, c.Lz);
+          const T ps = z.be * c.pA[o] + z.ae * dhx_k;
+          if (own) c.pB[o] = ps;
+          vy = vy + c.kp(0, 1, m, kslot).y * ps;
+        }
+      }
+      if (DZ && klt) {
+        if (lxe >= 0) {
+          const unsigned o = 4 * psl + psi_x(lxe, jc, kq, c.nm1y, c.pz);
+          const T ps = bxe * c.pA[o] + axe * dhy_i;
+          if (own) c.pB[o] = ps;
+          vz = vz + c.ip(0, 1, m, is).y * ps;
+        }
+        if (lye >= 0) {
+          const unsigned o = 5 * psl + psi_y(pl, lye, kq, c.Ly, c.pz);
+          const T ps = bye * c.pA[o] + aye * dhx_j;
+          if (own) c.pB[o] = ps;
+          vz = vz + c.jp(0, 1, m, js).y * ps;
+        }
+      }
+    } else {
+      // z slab only: Ex and Ey take their z psi
+      if (z.le >= 0) {
+        if (DX && plt) {
+          const unsigned o = 1 * psl + psi_z(pl, jc, z.le, c.nm1y, c.Lz);
+          const T ps = z.be * c.pA[o] + z.ae * dhy_k;
+          if (own) c.pB[o] = ps;
+          vx = vx + c.kp(0, 1, m, kslot).x * ps;
+        }
+        if (DY && jlt) {
+          const unsigned o = 3 * psl + psi_z(pl, jc, z.le, c.nm1y, c.Lz);
+          const T ps = z.be * c.pA[o] + z.ae * dhx_k;
+          if (own) c.pB[o] = ps;
+          vy = vy + c.kp(0, 1, m, kslot).y * ps;
+        }
+      }
+    }
+  }
+  // TFSF E corrections (tfsf_e_body), in FP64, on face cells only
+  if (KIND != SW_CORE && tfp) {
+    const int li = c.li, ri = c.ri, lj = c.lj, rj = c.rj, lk = c.lk, rk = c.rk;
+    const long long nj = c.nj, nk = c.nk;
+    if (j >= lj && j <= rj) {
+      const bool kb = kk >= lk && kk <= rk;
+      if constexpr (KIND == SW_ANY) {
+        // Ey: j in [lj, rj), (i, k) on the perimeter: x faces then z faces
+        if (DY && j < rj && kb && (p == li || p == ri || kk == lk || kk == rk)) {
+          double v = vy;
+          const long long ox = (long long)(j - lj) * (nk + 1) + (kk - lk), oz = (long long)(p - li) * nj + (j - lj);
+          if (p == li) v = v + c.seg(0)[ox];
+          if (p == ri) v = v - c.seg(1)[ox];
+          if (kk == lk) v = v - c.seg(2)[oz];
+          if (kk == rk) v = v + c.seg(3)[oz];
+          vy = (T)v;
+        }
+        // Ez: k in [lk, rk), (i, j) on the perimeter: x faces then y faces
+        if (DZ && kk >= lk && kk < rk && (p == li || p == ri || j == lj || j == rj)) {
+          double v = vz;
+          const long long ox = (long long)(j - lj) * nk + (kk - lk), oy = (long long)(p - li) * nk + (kk - lk);
+          if (p == li) v = v - c.seg(4)[ox];
+          if (p == ri) v = v + c.seg(5)[ox];
+          if (j == lj) v = v + c.seg(6)[oy];
+          if (j == rj) v = v - c.seg(7)[oy];
+          vz = (T)v;
+        }
+        // Ex: i in [li, ri), (j, k) on the perimeter: y faces then z faces
+        if (DX && p < ri && kb && (j == lj || j == rj || kk == lk || kk == rk)) {
+          double v = vx;
+          const long long oy = (long long)(p - li) * (nk + 1) + (kk - lk),
+                          oz = (long long)(p - li) * (nj + 1) + (j - lj);
+          if (j == lj) v = v - c.seg(8)[oy];
+          if (j == rj) v = v + c.seg(9)[oy];
+          if (kk == lk) v = v + c.seg(10)[oz];
+          if (kk == rk) v = v - c.seg(11)[oz];
+          vx = (T)v;
+        }
+      } else {
+        // z faces only (p is neither li nor ri, j neither lj nor rj)
+        if (kk == lk || kk == rk) {
+          if (DY && j < rj) {
+            double v = vy;
+            const long long oz = (long long)(p - li) * nj + (j - lj);
+            if (kk == lk) v = v - c.seg(2)[oz];
+            if (kk == rk) v = v + c.seg(3)[oz];
+            vy = (T)v;
+          }
+          if (DX && p < ri) {
+            double v = vx;
+            const long long oz = (long long)(p - li) * (nj + 1) + (j - lj);
+            if (kk == lk) v = v + c.seg(10)[oz];
+            if (kk == rk) v = v - c.seg(11)[oz];
+            vx = (T)v;
+          }
+        }
+      }
+    }
+  }
+}
+
+// One H cell (i, j, k) of material m: Simulation::updateH + PML::updateH +
+// TFSF3D::updateH in the order of compute_plane_h and tfsf_h_body.
+template <typename T, int R, int MS, int KIND>
+__device__ __forceinline__ void sw_h_cell(const SwCtx<T, R, MS> &c, int i, int j, int k, int lane, int m,
+                                          const ZLay<T> &z, T hxv, T hyv, T hzv, T dey_k, T dez_j, T dez_i, T dex_k,
+                                          T dex_j, T dey_i, bool own, bool slab, bool tfh, int lxh, T &vx, T &vy,
+                                          T &vz) {
+  using L = SweepLds<R, MS>;
+  const int is = i - c.i0, js = j - c.jwg;
+  const bool ilt = i < c.nx, jlt = j < c.ny, klt = k < c.nz;
+  const P4<T> cs = c.self(1, m);
+  const P2<T> ck = c.kp(1, 0, m, lane), cj = c.jp(1, 0, m, js), ci = c.ip(1, 0, m, is);
+  const T yx = cs.x * hxv + ck.x * dey_k + cj.x * dez_j;
+  const T yy = cs.y * hyv + ci.x * dez_i + ck.y * dex_k;
+  const T yz = cs.z * hzv + cj.y * dex_j + ci.y * dey_i;
+  const bool yee = ilt && jlt && klt;
+  vx = yee ? yx : hxv;
+  vy = yee ? yy : hyv;
+  vz = yee ? yz : hzv;
+  if (KIND != SW_CORE && slab) {
+    const unsigned jc = (unsigned)min(j, c.ny);
+    const unsigned psl = c.psl, oh = 6 * psl;
+    if constexpr (KIND == SW_ANY) {
+      const int lyh = j <= c.ny ? cpml_layer(false, j, c.ny, c.dn1, c.dp1) : -1;
+      const T byh = c.lds[L::ba_y + 2 * L::js + js], ayh = c.lds[L::ba_y + 3 * L::js + js];
+      const T bxh = c.lds[L::ba_x + 2 * L::is + is], axh = c.lds[L::ba_x + 3 * L::is + is];
+      const unsigned kl = (unsigned)min(k, c.pz - 1);
+      if (jlt && klt) {
+        if (lyh >= 0) {
+          const unsigned o = oh + 0 * psl + psi_y(i, lyh, kl, c.Ly, c.pz);
+          const T ps = byh * c.pA[o] + ayh * dez_j;
+          if (own) c.pB[o] = ps;
+          vx = vx + c.jp(1, 1, m, js).x * ps;
+        }
+        if (z.lh >= 0) {
+          const unsigned o = oh + 1 * psl + psi_z(i, jc, z.lh, c.nm1y, c.Lz);
+          const T ps = z.bh * c.pA[o] + z.ah * dey_k;
+          if (own) c.pB[o] = ps;
+          vx = vx + c.kp(1, 1, m, lane).x * ps;
+        }
+      }
+      if (ilt && klt) {
+        if (lxh >= 0) {
+          const unsigned o = oh + 2 * psl + psi_x(lxh, jc, kl, c.nm1y, c.pz);
+          const T ps = bxh * c.pA[o] + axh * dez_i;
+          if (own) c.pB[o] = ps;
+          vy = vy + c.ip(1, 1, m, is).x * ps;
+        }
+        if (z.lh >= 0) {
+          const unsigned o = oh + 3 * psl + psi_z(i, jc, z.lh, c.nm1y, c.Lz);
+          const T ps = z.bh * c.pA[o] + z.ah * dex_k;
+          if (own) c.pB[o] = ps;
+          vy = vy + c.kp(1, 1, m, lane).y * ps;
+        }
+      }
+      if (ilt && jlt) {
+        if (lxh >= 0) {
+          const unsigned o = oh + 4 * psl + psi_x(lxh, jc, kl, c.nm1y, c.pz);
+          const T ps = bxh * c.pA[o] + axh * dey_i;
+          if (own) c.pB[o] = ps;
+          vz = vz + c.ip(1, 1, m, is).y * ps;
+        }
+        if (lyh >= 0) {
+          const unsigned o = oh + 5 * psl + psi_y(i, lyh, kl, c.Ly, c.pz);
+          const T ps = byh * c.pA[o] + ayh * dex_j;
+          if (own) c.pB[o] = ps;
+          vz = vz + c.jp(1, 1, m, js).y * ps;
+        }
+      }
+    } else {
+      // z slab only: Hx and Hy take their z psi
+      if (z.lh >= 0 && klt) {
+        if (jlt) {
+          const unsigned o = oh + 1 * psl + psi_z(i, jc, z.lh, c.nm1y, c.Lz);
+          const T ps = z.bh * c.pA[o] + z.ah * dey_k;
+          if (own) c.pB[o] = ps;
+          vx = vx + c.kp(1, 1, m, lane).x * ps;
+        }
+        if (ilt) {
+          const unsigned o = oh + 3 * psl + psi_z(i, jc, z.lh, c.nm1y, c.Lz);
+          const T ps = z.bh * c.pA[o] + z.ah * dex_k;
+          if (own) c.pB[o] = ps;
+          vy = vy + c.kp(1, 1, m, lane).y * ps;
+        }
+      }
+    }
+  }
+  // TFSF H corrections (tfsf_h_body), in FP64; no H cell takes two
+  if (KIND != SW_CORE && tfh) {
+    const int li = c.li, ri = c.ri, lj = c.lj, rj = c.rj, lk = c.lk, rk = c.rk;
+    const long long nj = c.nj, nk = c.nk;
+    if (j >= lj - 1 && j <= rj) {
+      if constexpr (KIND == SW_ANY) {
+        // Hz: x faces (j in [lj, rj), k in [lk, rk]), y faces (i in [li, ri), k in [lk, rk])
+        if (k >= lk && k <= rk) {
+          if (j >= lj && j < rj && (i == li - 1 || i == ri)) {
+            const double t = c.seg(i == ri ? 13 : 12)[(long long)(j - lj) * (nk + 1) + (k - lk)];
+            vz = (T)(i == ri ? (double)vz - t : (double)vz + t);
+          } else if (i >= li && i < ri && (j == lj - 1 || j == rj)) {
+            const double t = c.seg(j == rj ? 15 : 14)[(long long)(i - li) * (nk + 1) + (k - lk)];
+            vz = (T)(j == rj ? (double)vz + t : (double)vz - t);
+          }
+        }
+        // Hy: x faces (j in [lj, rj], k in [lk, rk)), z faces (i in [li, ri), j in [lj, rj])
+        if (j >= lj) {
+          if (k >= lk && k < rk && (i == li - 1 || i == ri)) {
+            const double t = c.seg(i == ri ? 17 : 16)[(long long)(j - lj) * nk + (k - lk)];
+            vy = (T)(i == ri ? (double)vy + t : (double)vy - t);
+          } else if (i >= li && i < ri && (k == lk - 1 || k == rk)) {
+            const double t = c.seg(k == rk ? 19 : 18)[(long long)(i - li) * (nj + 1) + (j - lj)];
+            vy = (T)(k == rk ? (double)vy - t : (double)vy + t);
+          }
+        }
+        // Hx: y faces (k in [lk, rk), i in [li, ri]), z faces (i in [li, ri], j in [lj, rj))
+        if (i >= li && i <= ri) {
+          if (k >= lk && k < rk && (j == lj - 1 || j == rj)) {
+            const double t = c.seg(j == rj ? 21 : 20)[(long long)(i - li) * nk + (k - lk)];
+            vx = (T)(j == rj ? (double)vx - t : (double)vx + t);
+          } else if (j >= lj && j < rj && (k == lk - 1 || k == rk)) {
+            const double t = c.seg(k == rk ? 23 : 22)[(long long)(i - li) * nj + (j - lj)];
+            vx = (T)(k == rk ? (double)vx + t : (double)vx - t);
+          }
+        }
+      } else if (k == lk - 1 || k == rk) {
+        // z faces only (i is neither li - 1 nor ri, j neither lj - 1 nor rj)
+        if (j >= lj && i >= li && i < ri) {
+          const double t = c.seg(k == rk ? 19 : 18)[(long long)(i - li) * (nj + 1) + (j - lj)];
+          vy = (T)(k == rk ? (double)vy - t : (double)vy + t);
+        }
+        if (i >= li && i <= ri && j >= lj && j < rj) {
+          const double t = c.seg(k == rk ? 23 : 22)[(long long)(i - li) * nj + (j - lj)];
+          vx = (T)(k == rk ? (double)vx + t : (double)vx - t);
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int R, int MS, int KIND>
 __device__ __forceinline__ void sweep_body(const SweepArgs<T> &a, int block) {
   extern __shared__ __align__(16) unsigned char sw_lds[];
+  using L = SweepLds<R, MS>;
   const int nx = a.nx, ny = a.ny, nz = a.nz, pz = a.pz, nmat = a.nmat;
   const SweepTiling st = a.st;
   int tile = block;
@@ -178,56 +541,58 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &a, int block) {
     const int q = st.nblocks >> 3, rr = st.nblocks & 7, xcd = block & 7;
     tile = xcd * q + min(xcd, rr) + (block >> 3);
   }
-  const int kc = tile % st.nkc;
-  const int rest = tile / st.nkc;
-  const int jg = rest % st.njg;
-  const int ic = rest / st.njg;
+  const SweepTile tl = st.tiles[tile];
   const int lane = (int)(threadIdx.x & 63u);
   const int wv = (int)(threadIdx.x >> 6);
-  const int jwg = jg * st.waves * R;  // first row of the workgroup
+  const int jwg = tl.j0;  // first row of the workgroup
+  const int jend = jwg + (tl.kr >> 16);
   const int j0 = jwg + wv * R;
-  const int i0 = ic * st.tx;
-  const int i1 = min(i0 + st.tx, nx + 1);
-  const int k0 = kc * 64;
+  const int i0 = tl.i0;
+  const int i1 = tl.i1;
+  const int k0 = (tl.kr & 0xffff) * 64;
   const int k = k0 + lane;
   const int dn0 = a.dn[0], dp0 = a.dp[0], dn1 = a.dn[1], dp1 = a.dp[1], dn2 = a.dn[2], dp2 = a.dp[2];
-  const int Lx = dn0 + dp0, Ly = dn1 + dp1, Lz = dn2 + dp2, nm1y = ny + 1;
-  const bool tf = a.tf != 0;
+  const int Lx = dn0 + dp0, Ly = dn1 + dp1, Lz = dn2 + dp2;
 
   // ---- stage the tile's tables in LDS
-  using L = SweepLds<R, MS>;
   T *lds = reinterpret_cast<T *>(sw_lds);
-  // psi pairs and CPML values only when the tile meets a slab
-  const bool tile_slab = (Lx > 0 && meets_slab(i0, i1, nx, dn0, dp0)) ||
-                         (Ly > 0 && meets_slab(jwg, jwg + L::js - 1, ny, dn1, dp1)) ||
-                         (Lz > 0 && meets_slab(k0 - 1, k0 + 64, nz, dn2, dp2));
+  // psi pairs only in tiles that meet a slab; CPML b/a values of x and y only
+  // in SW_ANY tiles
+  bool tile_slab = false;
+  if constexpr (KIND == SW_ANY)
+    tile_slab = (Lx > 0 && meets_slab(i0, i1, nx, dn0, dp0)) ||
+                (Ly > 0 && meets_slab(jwg, jwg + L::js - 1, ny, dn1, dp1)) ||
+                (Lz > 0 && meets_slab(k0 - 1, k0 + 64, nz, dn2, dp2));
+  else if constexpr (KIND == SW_ZONLY)
+    tile_slab = Lz > 0 && meets_slab(k0 - 1, k0 + 64, nz, dn2, dp2);
   {
     const int nth = (int)blockDim.x, t0 = (int)threadIdx.x;
     for (int q = t0; q < 8 * nmat; q += nth) {
       const int side = q / (4 * nmat), r = q % (4 * nmat);
-      lds[L::self(side) + (r / 4) * 4 + r % 4] = a.self[side][r];
+      lds[L::self(side) + r] = a.self[side][r];
     }
-    // records are [slot][material][2] with MS material slots per k, j, i slot
-    const int npsi = tile_slab ? 2 : 1;
+    // global records are [slot][material][2]; LDS records [material][slot][2]
+    const int npsi = tile_slab ? 2 : 1, n2 = 2 * nmat;
     for (int side = 0; side < 2; ++side)
       for (int psi = 0; psi < npsi; ++psi) {
         const T *tk = psi ? a.kp[side] : a.kt[side];
+        for (int q = t0; q < n2 * SW_KS; q += nth) {
+          const int sl = q / n2, r = q % n2;
+          lds[L::kpair(side, psi) + ((r >> 1) * SW_KS + sl) * 2 + (r & 1)] = tk[min(k0 + sl, nz) * n2 + r];
+        }
+        if (psi && KIND != SW_ANY) continue;  // z psi pairs only
         const T *tj = psi ? a.jp[side] : a.jt[side];
         const T *ti = psi ? a.ip[side] : a.it[side];
-        for (int q = t0; q < 2 * nmat * SW_KS; q += nth) {
-          const int sl = q / (2 * nmat), r = q % (2 * nmat);
-          lds[L::pair(side, psi, 0) + sl * 2 * MS + r] = tk[min(k0 + sl, nz) * 2 * nmat + r];
+        for (int q = t0; q < n2 * L::js; q += nth) {
+          const int sl = q / n2, r = q % n2;
+          lds[L::jpair(side, psi) + ((r >> 1) * L::js + sl) * 2 + (r & 1)] = tj[min(jwg + sl, ny) * n2 + r];
         }
-        for (int q = t0; q < 2 * nmat * L::js; q += nth) {
-          const int sl = q / (2 * nmat), r = q % (2 * nmat);
-          lds[L::pair(side, psi, 1) + sl * 2 * MS + r] = tj[min(jwg + sl, ny) * 2 * nmat + r];
-        }
-        for (int q = t0; q < 2 * nmat * L::is; q += nth) {
-          const int sl = q / (2 * nmat), r = q % (2 * nmat);
-          lds[L::pair(side, psi, 2) + sl * 2 * MS + r] = ti[min(i0 + sl, nx) * 2 * nmat + r];
+        for (int q = t0; q < n2 * L::is; q += nth) {
+          const int sl = q / n2, r = q % n2;
+          lds[L::ipair(side, psi) + ((r >> 1) * L::is + sl) * 2 + (r & 1)] = ti[min(i0 + sl, nx) * n2 + r];
         }
       }
-    if (tile_slab) {
+    if (KIND == SW_ANY && tile_slab) {
       for (int q = t0; q < 4 * L::is; q += nth) {
         const int w = q / L::is, n = min(i0 + q % L::is, nx);
         const T *src = w == 0 ? a.be[0] : w == 1 ? a.ae[0] : w == 2 ? a.bm[0] : a.am[0];
@@ -241,23 +606,45 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &a, int block) {
     }
   }
   __syncthreads();
-  if (j0 > ny) return;  // the whole wave lies beyond the grid
+  if (j0 > ny || j0 >= jend) return;  // the whole wave lies beyond the grid or the tile
 
-  // ---- per-lane constants: this lane's cell k, and for lane 63 the cell
-  // above the wave (kh), for lane 0 the cell below (kb)
+  SwCtx<T, R, MS> c;
+  c.nx = nx; c.ny = ny; c.nz = nz; c.pz = pz;
+  c.dn1 = dn1; c.dp1 = dp1; c.Ly = Ly; c.Lz = Lz; c.nm1y = ny + 1;
+  c.i0 = i0; c.jwg = jwg;
+  c.lds = lds;
+  c.pA = a.psiA; c.pB = a.psiB; c.psl = a.psi_slot;
+  c.li = c.ri = c.lj = c.rj = c.lk = c.rk = 0;
+  c.terms = a.terms; c.tstr = a.term_stride;
+  // TFSF: the E corrections touch cells of [li, ri] x [lj, rj] x [lk, rk]
+  // on its faces, the H corrections cells of [li-1, ri] x [lj-1, rj] x
+  // [lk-1, rk] on its outer faces.  Wave-uniform: does the wave's (j, k)
+  // footprint (the halo row and cell included) meet those boxes?
+  bool tf_e = false, tf_h = false;
+  if (KIND != SW_CORE && a.tf) {
+    c.li = a.tb[0]; c.ri = a.tb[1]; c.lj = a.tb[2]; c.rj = a.tb[3]; c.lk = a.tb[4]; c.rk = a.tb[5];
+    tf_e = j0 <= c.rj && j0 + R >= c.lj && k0 <= c.rk && k0 + 64 >= c.lk;
+    tf_h = j0 <= c.rj && j0 + R - 1 >= c.lj - 1 && k0 <= c.rk && k0 + 63 >= c.lk - 1;
+  }
+  c.nj = c.rj - c.lj;
+  c.nk = c.rk - c.lk;
+  const int li = c.li, ri = c.ri;
+
+  // ---- per-lane constants: this lane's cell k.  The k - 1 neighbour of
+  // lane 0 and the k0 + 64 cell above lane 63 are handled packed: lane r
+  // (r <= R) loads and computes them for row j0 + r, and lane 0 / lane 63
+  // take row r's value from lane r.
   const int kl = min(k, pz - 1);
   const int kh = k0 + 64, khl = min(kh, pz - 1);
-  const int kbl = max(k0 - 1, 0);
-  const bool top = lane == 63, bot = lane == 0;
+  const int kbl = max(k0 - 1, 0), k63 = min(k0 + 63, pz - 1);
   const bool kown = k <= nz;
-  const bool kin_e = k >= 1 && k < nz, klt = k < nz;
-  const bool kin_eh = kh < nz;  // kh >= 64
   auto zlayer = [&](int kk) {
     ZLay<T> z;
-    const bool v = kk <= nz;
-    z.le = v ? cpml_layer(true, kk, nz, dn2, dp2) : -1;
-    z.lh = v ? cpml_layer(false, kk, nz, dn2, dp2) : -1;
+    z.le = z.lh = -1;
     z.be = z.ae = z.bh = z.ah = T(0);
+    if (KIND == SW_CORE || kk > nz) return z;
+    z.le = cpml_layer(true, kk, nz, dn2, dp2);
+    z.lh = cpml_layer(false, kk, nz, dn2, dp2);
     if (z.le >= 0) {
       z.be = a.be[2][kk];
       z.ae = a.ae[2][kk];
@@ -269,7 +656,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &a, int block) {
     return z;
   };
   const ZLay<T> zc = zlayer(k);
-  const ZLay<T> zt = zlayer(kh);  // the halo cell (used by lane 63)
+  const ZLay<T> zt = zlayer(kh);  // the halo cell
   const unsigned sj = a.sj, si = a.si;
   const T *__restrict__ fEx = a.src[EX];
   const T *__restrict__ fEy = a.src[EY];
@@ -278,188 +665,44 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &a, int block) {
   const T *__restrict__ fHy = a.src[HY];
   const T *__restrict__ fHz = a.src[HZ];
   const uint8_t *__restrict__ mat = a.mat;
-  const P4<T> *selfE = reinterpret_cast<const P4<T> *>(lds + L::self(0));
-  const P4<T> *selfH = reinterpret_cast<const P4<T> *>(lds + L::self(1));
-  const P2<T> *kE = reinterpret_cast<const P2<T> *>(lds + L::pair(0, 0, 0));
-  const P2<T> *jE = reinterpret_cast<const P2<T> *>(lds + L::pair(0, 0, 1));
-  const P2<T> *iE = reinterpret_cast<const P2<T> *>(lds + L::pair(0, 0, 2));
-  const P2<T> *kH = reinterpret_cast<const P2<T> *>(lds + L::pair(1, 0, 0));
-  const P2<T> *jH = reinterpret_cast<const P2<T> *>(lds + L::pair(1, 0, 1));
-  const P2<T> *iH = reinterpret_cast<const P2<T> *>(lds + L::pair(1, 0, 2));
-  const P2<T> *kpE = reinterpret_cast<const P2<T> *>(lds + L::pair(0, 1, 0));
-  const P2<T> *jpE = reinterpret_cast<const P2<T> *>(lds + L::pair(0, 1, 1));
-  const P2<T> *ipE = reinterpret_cast<const P2<T> *>(lds + L::pair(0, 1, 2));
-  const P2<T> *kpH = reinterpret_cast<const P2<T> *>(lds + L::pair(1, 1, 0));
-  const P2<T> *jpH = reinterpret_cast<const P2<T> *>(lds + L::pair(1, 1, 1));
-  const P2<T> *ipH = reinterpret_cast<const P2<T> *>(lds + L::pair(1, 1, 2));
-  const T *bax = lds + L::ba_x;
-  const T *bay = lds + L::ba_y;
-  // psi slot offsets (elements) in a block
-  const unsigned psl = a.psi_slot;
-  const unsigned o_ay = 0, o_az = psl, o_bx = 2 * psl, o_bz = 3 * psl, o_cx = 4 * psl, o_cy = 5 * psl, o_h = 6 * psl;
-  const T *__restrict__ pA = a.psiA;
-  T *__restrict__ pB = a.psiB;
   // does any row of this wave, or any lane (or the halo cell), meet a y / z
   // slab?  (wave-uniform)
-  const bool any_z = Lz > 0 && __ballot(zc.le >= 0 || zc.lh >= 0 || zt.le >= 0) != 0ull;
-  const bool any_y = Ly > 0 && meets_slab(j0, j0 + R, ny, dn1, dp1);
-  // TFSF: the E corrections touch cells of [li, ri] x [lj, rj] x [lk, rk]
-  // on its faces, the H corrections cells of [li-1, ri] x [lj-1, rj] x
-  // [lk-1, rk] on its outer faces.  Wave-uniform: does the wave's (j, k)
-  // footprint (the halo cell included) meet those boxes?
-  int li = 0, ri = 0, lj = 0, rj = 0, lk = 0, rk = 0;
-  bool tf_e = false, tf_h = false;
-  if (tf) {
-    li = a.tb[0]; ri = a.tb[1]; lj = a.tb[2]; rj = a.tb[3]; lk = a.tb[4]; rk = a.tb[5];
-    tf_e = j0 <= rj && j0 + R >= lj && k0 <= rk && k0 + 64 >= lk;
-    tf_h = j0 <= rj && j0 + R - 1 >= lj - 1 && k0 <= rk && k0 + 63 >= lk - 1;
-  }
-  const long long nj = rj - lj, nk = rk - lk;
-  // TFSF term segment s of this step
-  const unsigned tstr = a.term_stride;
-  auto seg = [&](int sg) { return a.terms + (size_t)sg * tstr; };
+  const bool any_z = KIND != SW_CORE && Lz > 0 && __ballot(zc.le >= 0 || zc.lh >= 0 || zt.le >= 0) != 0ull;
+  const bool any_y = KIND == SW_ANY && Ly > 0 && meets_slab(j0, j0 + R, ny, dn1, dp1);
 
   // element offset of row r - 1 within a plane (clamped), r in [0, R + 1]
   unsigned roff[R + 2];
 #pragma unroll
   for (int r = -1; r <= R; ++r) roff[r + 1] = (unsigned)min(max(j0 + r, 0), ny) * sj;
+  // this lane's packed row j0 + hr and the row below it
+  const int hr = min(lane, R);
+  const int hj = j0 + hr;
+  const unsigned hro = (unsigned)min(hj, ny) * sj;
+  const unsigned hrom = (unsigned)min(max(hj - 1, 0), ny) * sj;
+  constexpr unsigned S = (unsigned)sizeof(T);
 
-  // One E cell (p, j, kk): Simulation::updateE + PML::updateE + TFSF3D::
-  // updateE, in the order of compute_plane_e and tfsf_e_body.  d*: the
-  // curl differences; (ps, own): psi slot offset within a plane of the cell,
-  // whether this thread stores its psi; yrow: Ey exists (not the halo row);
-  // zrow: Ez is wanted (not the halo cell).
-  auto e_cell = [&](int p, int j, int kk, int kslot, int m, const ZLay<T> &z, bool kin, bool klt_, T exv, T eyv,
-                    T ezv, T dhy_k, T dhz_j, T dhx_k, T dhz_i, T dhx_j, T dhy_i, bool own, bool yrow, bool zrow,
-                    bool slab, bool tfp, int lxe, T &vx, T &vy, T &vz) {
-    const int pl = min(p, nx);  // tables and psi of a plane beyond nx are never used
-    const int is = p - i0, js = j - jwg;
-    const bool jvalid = j <= ny;
-    const bool jin = j >= 1 && j < ny, jlt = j < ny;
-    const bool ix = p >= 1 && p < nx, plt = p < nx;
-    const P4<T> cs = selfE[m];
-    const P2<T> ck = kE[kslot * MS + m], cj = jE[js * MS + m], ci = iE[is * MS + m];
-    const T yx = cs.x * exv + ck.x * dhy_k + cj.x * dhz_j;
-    const T yy = cs.y * eyv + ck.y * dhx_k + ci.x * dhz_i;
-    const T yz = cs.z * ezv + cj.y * dhx_j + ci.y * dhy_i;
-    vx = (plt && jin && kin) ? yx : exv;
-    vy = (ix && jlt && kin) ? yy : eyv;
-    vz = (ix && jin && klt_) ? yz : ezv;
-    if (slab) {
-      const int lye = jvalid ? cpml_layer(true, j, ny, dn1, dp1) : -1;
-      const T bye = bay[js], aye = bay[L::js + js];
-      const T bxe = bax[is], axe = bax[L::is + is];
-      const unsigned jc = (unsigned)min(j, ny);
-      const unsigned kq = (unsigned)min(kk, pz - 1);
-      if (plt) {
-        if (lye >= 0) {
-          const unsigned o = o_ay + psi_y(pl, lye, kq, Ly, pz);
-          const T ps = bye * pA[o] + aye * dhz_j;
-          if (own) pB[o] = ps;
-          vx = vx + jpE[js * MS + m].x * ps;
-        }
-        if (z.le >= 0) {
-          const unsigned o = o_az + psi_z(pl, jc, z.le, nm1y, Lz);
-          const T ps = z.be * pA[o] + z.ae * dhy_k;
-          if (own) pB[o] = ps;
-          vx = vx + kpE[kslot * MS + m].x * ps;
-        }
-      }
-      if (jlt && yrow) {
-        if (lxe >= 0) {
-          const unsigned o = o_bx + psi_x(lxe, jc, kq, nm1y, pz);
-          const T ps = bxe * pA[o] + axe * dhz_i;
-          if (own) pB[o] = ps;
-          vy = vy + ipE[is * MS + m].x * ps;
-        }
-        if (z.le >= 0) {
-          const unsigned o = o_bz + psi_z(pl, jc, z.le, nm1y, Lz);
-          const T ps = z.be * pA[o] + z.ae * dhx_k;
-          if (own) pB[o] = ps;
-          vy = vy + kpE[kslot * MS + m].y * ps;
-        }
-      }
-      if (klt_ && zrow) {
-        if (lxe >= 0) {
-          const unsigned o = o_cx + psi_x(lxe, jc, kq, nm1y, pz);
-          const T ps = bxe * pA[o] + axe * dhy_i;
-          if (own) pB[o] = ps;
-          vz = vz + ipE[is * MS + m].y * ps;
-        }
-        if (lye >= 0) {
-          const unsigned o = o_cy + psi_y(pl, lye, kq, Ly, pz);
-          const T ps = bye * pA[o] + aye * dhx_j;
-          if (own) pB[o] = ps;
-          vz = vz + jpE[js * MS + m].y * ps;
-        }
-      }
-    }
-    // TFSF E corrections (tfsf_e_body), in FP64, on face cells only
-    if (tfp && j >= lj && j <= rj) {
-      const bool kb = kk >= lk && kk <= rk;
-      // Ey: j in [lj, rj), (i, k) on the perimeter: x faces then z faces
-      if (yrow && j < rj && kb && (p == li || p == ri || kk == lk || kk == rk)) {
-        double v = vy;
-        const long long ox = (long long)(j - lj) * (nk + 1) + (kk - lk), oz = (long long)(p - li) * nj + (j - lj);
-        if (p == li) v = v + seg(0)[ox];
-        if (p == ri) v = v - seg(1)[ox];
-        if (kk == lk) v = v - seg(2)[oz];
-        if (kk == rk) v = v + seg(3)[oz];
-        vy = (T)v;
-      }
-      // Ez: k in [lk, rk), (i, j) on the perimeter: x faces then y faces
-      if (zrow && kk >= lk && kk < rk && (p == li || p == ri || j == lj || j == rj)) {
-        double v = vz;
-        const long long ox = (long long)(j - lj) * nk + (kk - lk), oy = (long long)(p - li) * nk + (kk - lk);
-        if (p == li) v = v - seg(4)[ox];
-        if (p == ri) v = v + seg(5)[ox];
-        if (j == lj) v = v + seg(6)[oy];
-        if (j == rj) v = v - seg(7)[oy];
-        vz = (T)v;
-      }
-      // Ex: i in [li, ri), (j, k) on the perimeter: y faces then z faces
-      if (p < ri && kb && (j == lj || j == rj || kk == lk || kk == rk)) {
-        double v = vx;
-        const long long oy = (long long)(p - li) * (nk + 1) + (kk - lk), oz = (long long)(p - li) * (nj + 1) + (j - lj);
-        if (j == lj) v = v - seg(8)[oy];
-        if (j == rj) v = v + seg(9)[oy];
-        if (kk == lk) v = v + seg(10)[oz];
-        if (kk == rk) v = v - seg(11)[oz];
-        vx = (T)v;
-      }
-    }
-  };
-
-  // carried between planes (plane i): H of rows 0..R-1 (Hy also row R), E
-  // of rows 0..R (Ey rows 0..R-1), and for lane 63 the halo cell's E (Ex,
-  // Ey) and H Hz of rows 0..R-1
+  // carried between planes (plane i): E(i) of rows 0..R (Ey rows 0..R-1),
+  // H(i) of rows 0..R-1 (Hy also row R), the material of rows 0..R-1; packed:
+  // the halo cell's new Ex, Ey and old Hz
   T hx[R], hy[R + 1], hz[R];
   T ex[R + 1], ey[R], ez[R + 1];
-  T gx[R], gy[R], ghz[R];
   unsigned mcur[R];
-  // plane i + 1
-  T ax[R + 1], ay[R], az[R + 1];
-  T hxq[R + 2], hyq[R + 2], hzq[R + 2];
-  T hyb[R + 1], hxb[R];             // lane 0: Hy, Hx at k0 - 1
-  T egx[R], egy[R], hgx[R], hgy[R], hgz[R + 1];  // lane 63: the halo cell (hgz from row -1)
-  unsigned mq[R + 1], mg[R];
+  T gx = T(0), gy = T(0), ghz = T(0);
 #pragma unroll
   for (int r = 0; r <= R; ++r) {
-    hy[r] = T(0);
-    if (r < R) hz[r] = hx[r] = ghz[r] = gx[r] = gy[r] = T(0);
+    hy[r] = ex[r] = ez[r] = T(0);
+    if (r < R) hz[r] = hx[r] = ey[r] = T(0), mcur[r] = 0;
   }
   // before the first plane: H(i0 - 1), zero before plane 0
   if (i0 >= 1) {
     const unsigned pbase = (unsigned)(i0 - 1) * si;
 #pragma unroll
     for (int r = 0; r <= R; ++r) {
-      const unsigned c = pbase + roff[r + 1];
-      hy[r] = ldo(fHy, (c + kl) * (unsigned)sizeof(T));
-      if (r < R) {
-        hz[r] = ldo(fHz, (c + kl) * (unsigned)sizeof(T));
-        if (top) ghz[r] = ldo(fHz, (c + khl) * (unsigned)sizeof(T));
-      }
+      const unsigned cc = pbase + roff[r + 1];
+      hy[r] = ldo(fHy, (cc + kl) * S);
+      if (r < R) hz[r] = ldo(fHz, (cc + kl) * S);
     }
+    ghz = ldo(fHz, (pbase + hro + khl) * S);
   }
   bool bad = false;  // a non-finite H value was stored
   // i = i0 - 1 only computes E(i0); from i0 on, each pass computes E(i + 1)
@@ -467,192 +710,95 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &a, int block) {
   for (int i = i0 - 1; i < i1; ++i) {
     const int p = i + 1;
     // ---- loads of plane p (clamped to nx) from set A
+    T hxq[R + 2], hyq[R + 1], hzq[R + 2];
+    T ax[R + 1], ay[R], az[R + 1];
+    unsigned mq[R + 1];
+    T qhyb, qhxb, qex, qey, qhx, qhy, qhz, qhzm, qhy63, qhx63;
+    unsigned qm;
     {
       const unsigned pbase = (unsigned)min(p, nx) * si;
 #pragma unroll
       for (int r = -1; r <= R; ++r) {
-        const unsigned c = pbase + roff[r + 1];
-        const unsigned o = (c + kl) * (unsigned)sizeof(T);
+        const unsigned o = (pbase + roff[r + 1] + kl) * S;
         hxq[r + 1] = ldo(fHx, o);
         hzq[r + 1] = ldo(fHz, o);
-        if (top && r < R) hgz[r + 1] = ldo(fHz, (c + khl) * (unsigned)sizeof(T));
         if (r >= 0) {
-          hyq[r + 1] = ldo(fHy, o);
+          hyq[r] = ldo(fHy, o);
           ax[r] = ldo(fEx, o);
           az[r] = ldo(fEz, o);
-          mq[r] = ldm(mat, c + kl);
-          hyb[r] = T(0);
-          if (bot) hyb[r] = ldo(fHy, (c + kbl) * (unsigned)sizeof(T));
-          if (r < R) {
-            ay[r] = ldo(fEy, o);
-            hxb[r] = T(0);
-            egx[r] = egy[r] = hgx[r] = hgy[r] = T(0);
-            mg[r] = 0;
-            if (bot) hxb[r] = ldo(fHx, (c + kbl) * (unsigned)sizeof(T));
-            if (top) {
-              const unsigned og = (c + khl) * (unsigned)sizeof(T);
-              egx[r] = ldo(fEx, og);
-              egy[r] = ldo(fEy, og);
-              hgx[r] = ldo(fHx, og);
-              hgy[r] = ldo(fHy, og);
-              mg[r] = ldm(mat, c + khl);
-            }
-          }
+          mq[r] = ldm(mat, pbase + roff[r + 1] + kl);
+          if (r < R) ay[r] = ldo(fEy, o);
         }
       }
+      const unsigned ob = (pbase + hro + kbl) * S, oh = (pbase + hro + khl) * S;
+      const unsigned o63 = (pbase + hro + k63) * S;
+      qhyb = ldo(fHy, ob);
+      qhxb = ldo(fHx, ob);
+      qex = ldo(fEx, oh);
+      qey = ldo(fEy, oh);
+      qhx = ldo(fHx, oh);
+      qhy = ldo(fHy, oh);
+      qhz = ldo(fHz, oh);
+      qhzm = ldo(fHz, (pbase + hrom + khl) * S);
+      qhy63 = ldo(fHy, o63);
+      qhx63 = ldo(fHx, o63);
+      qm = ldm(mat, pbase + hro + khl);
     }
-    // ---- E of plane p: rows 0..R of the wave's cells, rows 0..R-1 of the
-    // halo cell (lane 63)
+    // ---- E of plane p: rows 0..R of the wave's cells (row R: Ex, Ez), and
+    // packed, the halo cell of rows 0..R-1 (Ex, Ey)
     {
-      const int lxe = p <= nx ? cpml_layer(true, p, nx, dn0, dp0) : -1;
+      const int lxe = (KIND == SW_ANY && p <= nx) ? cpml_layer(true, p, nx, dn0, dp0) : -1;
       const bool slab = lxe >= 0 || any_y || any_z;
       const bool tfp = tf_e && p >= li && p <= ri;
 #pragma unroll
       for (int r = 0; r <= R; ++r) {
         const int j = j0 + r;
-        const T hxv = hxq[r + 1], hyv = hyq[r + 1], hzv = hzq[r + 1];
-        const T hyk = from_below(hyv, hyb[r]);
-        const T hxk = from_below(hxv, r < R ? hxb[r] : T(0));
-        const T eyv = r < R ? ay[r] : T(0);
-        const T dhz_i = r < R ? hzv - hz[r] : T(0);
-        const bool own = kown && r < R && j <= ny && p < i1;
-        T vx, vy, vz;
-        e_cell(p, j, k, lane, (int)mq[r], zc, kin_e, klt, ax[r], eyv, az[r], hyv - hyk, hzv - hzq[r], hxv - hxk,
-               dhz_i, hxv - hxq[r], hyv - hy[r], own, r < R, true, slab, tfp, lxe, vx, vy, vz);
+        const T hxv = hxq[r + 1], hyv = hyq[r], hzv = hzq[r + 1];
+        const T hyk = from_below(hyv, lane_value(qhyb, r));
+        const bool own = kown && r < R && j <= ny && j < jend && p < i1;
+        T vx, vy = T(0), vz;
         if (r < R) {
-          // the halo cell k0 + 64 of this row (meaningful in lane 63)
-          T wx, wy, wz;
-          const T hzjm = hgz[r];  // Hz(j - 1) at the halo cell
-          e_cell(p, j, kh, 64, (int)mg[r], zt, kin_eh, kh < nz, egx[r], egy[r], T(0), hgy[r] - hyv,
-                 hgz[r + 1] - hzjm, hgx[r] - hxv, hgz[r + 1] - ghz[r], T(0), T(0), false, true, false, slab, tfp,
-                 lxe, wx, wy, wz);
-          egx[r] = wx;
-          egy[r] = wy;
+          const T hxk = from_below(hxv, lane_value(qhxb, r));
+          sw_e_cell<T, R, MS, KIND, true, true, true>(c, p, j, k, lane, (int)mq[r], zc, ax[r], ay[r], az[r],
+                                                      hyv - hyk, hzv - hzq[r], hxv - hxk, hzv - hz[r],
+                                                      hxv - hxq[r], hyv - hy[r], own, slab, tfp, lxe, vx, vy, vz);
           ay[r] = vy;
+        } else {
+          sw_e_cell<T, R, MS, KIND, true, false, true>(c, p, j, k, lane, (int)mq[r], zc, ax[r], T(0), az[r],
+                                                       hyv - hyk, hzv - hzq[r], T(0), T(0), hxv - hxq[r],
+                                                       hyv - hy[r], false, slab, tfp, lxe, vx, vy, vz);
         }
         ax[r] = vx;
         az[r] = vz;
       }
+      // the halo cell (j0 + hr, k0 + 64), meaningful in lanes 0..R-1
+      T wx, wy, wz = T(0);
+      sw_e_cell<T, R, MS, KIND, true, true, false>(c, p, hj, kh, SW_KS - 1, (int)qm, zt, qex, qey, T(0),
+                                                   qhy - qhy63, qhz - qhzm, qhx - qhx63, qhz - ghz, T(0), T(0),
+                                                   false, slab, tfp, lxe, wx, wy, wz);
+      qex = wx;
+      qey = wy;
     }
     if (i >= i0) {
       // ---- H of plane i from H(i) (set A), E(i) and E(i + 1) (set B)
-      const int lxh = cpml_layer(false, i, nx, dn0, dp0);
+      const int lxh = KIND == SW_ANY ? cpml_layer(false, i, nx, dn0, dp0) : -1;
       const bool slab = lxh >= 0 || any_y || any_z;
-      const bool ilt = i < nx;
-      const int is = i - i0;
       const bool tfh = tf_h && i >= li - 1 && i <= ri;
       const unsigned pbase = (unsigned)i * si;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int j = j0 + r;
-        const bool jvalid = j <= ny;
-        const bool jlt = j < ny;
-        const int m = (int)mcur[r];
-        const int js = j - jwg;
-        const T hxv = hx[r], hyv = hy[r], hzv = hz[r];
         const T exv = ex[r], eyv = ey[r], ezv = ez[r];
-        const T eyk = from_above(eyv, gy[r]);
-        const T exk = from_above(exv, gx[r]);
-        const T dey_k = eyk - eyv;
-        const T dez_j = ez[r + 1] - ezv;
-        const T dez_i = az[r] - ezv;
-        const T dex_k = exk - exv;
-        const T dex_j = ex[r + 1] - exv;
-        const T dey_i = ay[r] - eyv;
-        const P4<T> cs = selfH[m];
-        const P2<T> ck = kH[lane * MS + m], cj = jH[js * MS + m], ci = iH[is * MS + m];
-        const T yx = cs.x * hxv + ck.x * dey_k + cj.x * dez_j;
-        const T yy = cs.y * hyv + ci.x * dez_i + ck.y * dex_k;
-        const T yz = cs.z * hzv + cj.y * dex_j + ci.y * dey_i;
-        const bool yee = ilt && jlt && klt;
-        T vx = yee ? yx : hxv;
-        T vy = yee ? yy : hyv;
-        T vz = yee ? yz : hzv;
-        const bool own = kown && jvalid;
-        if (slab) {
-          const int lyh = jvalid ? cpml_layer(false, j, ny, dn1, dp1) : -1;
-          const T byh = bay[2 * L::js + js], ayh = bay[3 * L::js + js];
-          const T bxh = bax[2 * L::is + is], axh = bax[3 * L::is + is];
-          const unsigned jc = (unsigned)min(j, ny);
-          if (jlt && klt) {
-            if (lyh >= 0) {
-              const unsigned o = o_h + o_ay + psi_y(i, lyh, kl, Ly, pz);
-              const T ps = byh * pA[o] + ayh * dez_j;
-              if (own) pB[o] = ps;
-              vx = vx + jpH[js * MS + m].x * ps;
-            }
-            if (zc.lh >= 0) {
-              const unsigned o = o_h + o_az + psi_z(i, jc, zc.lh, nm1y, Lz);
-              const T ps = zc.bh * pA[o] + zc.ah * dey_k;
-              if (own) pB[o] = ps;
-              vx = vx + kpH[lane * MS + m].x * ps;
-            }
-          }
-          if (ilt && klt) {
-            if (lxh >= 0) {
-              const unsigned o = o_h + o_bx + psi_x(lxh, jc, kl, nm1y, pz);
-              const T ps = bxh * pA[o] + axh * dez_i;
-              if (own) pB[o] = ps;
-              vy = vy + ipH[is * MS + m].x * ps;
-            }
-            if (zc.lh >= 0) {
-              const unsigned o = o_h + o_bz + psi_z(i, jc, zc.lh, nm1y, Lz);
-              const T ps = zc.bh * pA[o] + zc.ah * dex_k;
-              if (own) pB[o] = ps;
-              vy = vy + kpH[lane * MS + m].y * ps;
-            }
-          }
-          if (ilt && jlt) {
-            if (lxh >= 0) {
-              const unsigned o = o_h + o_cx + psi_x(lxh, jc, kl, nm1y, pz);
-              const T ps = bxh * pA[o] + axh * dey_i;
-              if (own) pB[o] = ps;
-              vz = vz + ipH[is * MS + m].y * ps;
-            }
-            if (lyh >= 0) {
-              const unsigned o = o_h + o_cy + psi_y(i, lyh, kl, Ly, pz);
-              const T ps = byh * pA[o] + ayh * dex_j;
-              if (own) pB[o] = ps;
-              vz = vz + jpH[js * MS + m].y * ps;
-            }
-          }
-        }
-        // TFSF H corrections (tfsf_h_body), in FP64; no H cell takes two
-        if (tfh && j >= lj - 1 && j <= rj) {
-          // Hz: x faces (j in [lj, rj), k in [lk, rk]), y faces (i in [li, ri), k in [lk, rk])
-          if (k >= lk && k <= rk) {
-            if (j >= lj && j < rj && (i == li - 1 || i == ri)) {
-              const double t = seg(i == ri ? 13 : 12)[(long long)(j - lj) * (nk + 1) + (k - lk)];
-              vz = (T)(i == ri ? (double)vz - t : (double)vz + t);
-            } else if (i >= li && i < ri && (j == lj - 1 || j == rj)) {
-              const double t = seg(j == rj ? 15 : 14)[(long long)(i - li) * (nk + 1) + (k - lk)];
-              vz = (T)(j == rj ? (double)vz + t : (double)vz - t);
-            }
-          }
-          // Hy: x faces (j in [lj, rj], k in [lk, rk)), z faces (i in [li, ri), j in [lj, rj])
-          if (j >= lj) {
-            if (k >= lk && k < rk && (i == li - 1 || i == ri)) {
-              const double t = seg(i == ri ? 17 : 16)[(long long)(j - lj) * nk + (k - lk)];
-              vy = (T)(i == ri ? (double)vy + t : (double)vy - t);
-            } else if (i >= li && i < ri && (k == lk - 1 || k == rk)) {
-              const double t = seg(k == rk ? 19 : 18)[(long long)(i - li) * (nj + 1) + (j - lj)];
-              vy = (T)(k == rk ? (double)vy - t : (double)vy + t);
-            }
-          }
-          // Hx: y faces (k in [lk, rk), i in [li, ri]), z faces (i in [li, ri], j in [lj, rj))
-          if (i >= li && i <= ri) {
-            if (k >= lk && k < rk && (j == lj - 1 || j == rj)) {
-              const double t = seg(j == rj ? 21 : 20)[(long long)(i - li) * nk + (k - lk)];
-              vx = (T)(j == rj ? (double)vx - t : (double)vx + t);
-            } else if (j >= lj && j < rj && (k == lk - 1 || k == rk)) {
-              const double t = seg(k == rk ? 23 : 22)[(long long)(i - li) * nj + (j - lj)];
-              vx = (T)(k == rk ? (double)vx + t : (double)vx - t);
-            }
-          }
-        }
+        const T eyk = from_above(eyv, lane_value(gy, r));
+        const T exk = from_above(exv, lane_value(gx, r));
+        const bool own = kown && j <= ny && j < jend;
+        T vx, vy, vz;
+        sw_h_cell<T, R, MS, KIND>(c, i, j, k, lane, (int)mcur[r], zc, hx[r], hy[r], hz[r], eyk - eyv,
+                                  ez[r + 1] - ezv, az[r] - ezv, exk - exv, ex[r + 1] - exv, ay[r] - eyv, own, slab,
+                                  tfh, lxh, vx, vy, vz);
         if (own) {
           bad |= !(__builtin_isfinite(vx) && __builtin_isfinite(vy) && __builtin_isfinite(vz));
-          const unsigned o = (pbase + roff[r + 1] + kl) * (unsigned)sizeof(T);
+          const unsigned o = (pbase + roff[r + 1] + kl) * S;
           sto(a.dst[EX], o, exv);
           sto(a.dst[EY], o, eyv);
           sto(a.dst[EZ], o, ezv);
@@ -667,24 +813,24 @@ __device__ __forceinline__ void sweep_body(const SweepArgs<T> &a, int block) {
     for (int r = 0; r <= R; ++r) {
       ex[r] = ax[r];
       ez[r] = az[r];
-      hy[r] = hyq[r + 1];
+      hy[r] = hyq[r];
       if (r < R) {
         ey[r] = ay[r];
         hx[r] = hxq[r + 1];
         hz[r] = hzq[r + 1];
         mcur[r] = mq[r];
-        gx[r] = egx[r];
-        gy[r] = egy[r];
-        ghz[r] = hgz[r + 1];
       }
     }
+    gx = qex;
+    gy = qey;
+    ghz = qhz;
   }
   if (bad) atomicMin(a.div, (unsigned long long)a.step);
 }
 
-template <typename T, int R, int MS>
+template <typename T, int R, int MS, int KIND>
 __global__ void __launch_bounds__(256) k_sweep(SweepArgs<T> a) {
-  sweep_body<T, R, MS>(a, (int)blockIdx.x);
+  sweep_body<T, R, MS, KIND>(a, (int)blockIdx.x);
 }
 
 // TFSF3D::updateIncidentField (tfsf_3d.cpp:153-183) from the line (e, h) into
