@@ -85,8 +85,6 @@ struct xf_engine {
   double tca[3] = {}, tcb[3] = {};
   long long tfsf_T = 0;
   bool tfsf_two_d = false;
-  int nEy = 0, nEz = 0, nEx = 0;
-  TfsfHSeg hseg{};
   // NF-FF
   bool has_nf = false;
   Nffft nf{};
@@ -188,6 +186,8 @@ struct xf_engine {
   long long psi_off[12] = {}, psi_n = 0;
   void *e_inc_alt = nullptr, *h_inc_alt = nullptr;
   TfTerms tf_terms{};              // the step's TFSF face terms (fused step)
+  // the TFSF z-face terms the curl kernels add (ZFaces), 8 FP64 segments
+  void *zterms = nullptr;
   // fused-step tile lists, one run per kind (SW_CORE, SW_ZONLY, SW_ANY) in
   // one device array, built for rows/planes (sp_R, sp_tx)
   void *sp_tiles = nullptr;
@@ -576,23 +576,7 @@ extern "C" xf_status xf_engine_set_tfsf(xf_engine *e, const xf_tfsf_desc *d) {
   if ((s = upload_f64(e, &e->src, d->source, (size_t)d->total_steps)) != XF_OK) return s;
   e->tfsf_T = d->total_steps;
   set_total(e, d->total_steps);
-  const int A0 = bn[0] + 1, A1 = bn[1] + 1, A2 = bn[2] + 1;
-  e->nEy = bn[1] * (2 * A0 + 2 * (A2 - 2));
-  e->nEz = bn[2] * (2 * A0 + 2 * (A1 - 2));
-  e->nEx = bn[0] * (2 * A1 + 2 * (A2 - 2));
-  const int ni = bn[0], nj = bn[1], nk = bn[2];
-  int cnt[12] = {nj * (nk + 1), (nj + 1) * nk, nj * (nk + 1), (nj + 1) * nk,
-                 nk * (ni + 1), (nk + 1) * ni, nk * (ni + 1), (nk + 1) * ni,
-                 ni * (nj + 1), (ni + 1) * nj, ni * (nj + 1), (ni + 1) * nj};
-  if (two_d) {
-    // TFSF2D (tfsf_2d.cpp:148-228): Ez on the four faces of the one z
-    // layer, Hy on the x faces and Hx on the y faces
-    e->nEy = e->nEx = 0;
-    for (int q : {0, 2, 5, 7, 8, 9, 10, 11}) cnt[q] = 0;
-  }
   e->tfsf_two_d = two_d;
-  e->hseg.off[0] = 0;
-  for (int q = 0; q < 12; ++q) e->hseg.off[q + 1] = e->hseg.off[q] + cnt[q];
   e->has_tfsf = true;
   return XF_OK;
 }
@@ -1019,8 +1003,38 @@ static Grid<T> make_grid(xf_engine *e) {
   return g;
 }
 
+// Do the curl kernels apply the TFSF z-face corrections (ZFaces)?  On 3-D
+// grids without lumped E corrections, which the reference applies before the
+// TFSF ones (the face kernels run after them); batches keep every face in the
+// face kernels.
+static bool tfsf_zc(const xf_engine *e) { return e->has_tfsf && !e->tfsf_two_d && e->sources.empty(); }
+
+// Cell counts of the face kernels' segments (tfsf_e_body, tfsf_h_body).
+// TFSF2D (tfsf_2d.cpp:148-228) corrects Ez on the four faces of its one z
+// layer, Hy on the x faces and Hx on the y faces.
+static void tfsf_segs(const xf_engine *e, bool zc, TfsfESeg &es, TfsfHSeg &hs) {
+  const int ni = e->tri - e->tli, nj = e->trj - e->tlj, nk = e->trk - e->tlk;
+  const bool two_d = e->tfsf_two_d;
+  int ce[6] = {2 * nj * (nk + 1), zc ? 0 : 2 * (ni - 1) * nj, 2 * (nj + 1) * nk, 2 * (ni - 1) * nk,
+               2 * ni * (nk + 1), zc ? 0 : 2 * ni * (nj - 1)};
+  int ch[12] = {nj * (nk + 1), (nj + 1) * nk, nj * (nk + 1), (nj + 1) * nk,
+                nk * (ni + 1), (nk + 1) * ni, nk * (ni + 1), (nk + 1) * ni,
+                ni * (nj + 1), (ni + 1) * nj, ni * (nj + 1), (ni + 1) * nj};
+  if (two_d) {
+    ce[0] = ce[1] = ce[4] = ce[5] = 0;
+    for (int q : {0, 2, 5, 7, 8, 9, 10, 11}) ch[q] = 0;
+  }
+  if (zc)
+    for (int q = 8; q < 12; ++q) ch[q] = 0;
+  es.off[0] = 0;
+  for (int q = 0; q < 6; ++q) es.off[q + 1] = es.off[q] + ce[q];
+  hs.off[0] = 0;
+  for (int q = 0; q < 12; ++q) hs.off[q + 1] = hs.off[q] + ch[q];
+}
+
 static Tfsf make_tfsf(xf_engine *e) {
   Tfsf t;
+  std::memset(&t, 0, sizeof t);  // padding included: records are compared bytewise (upload_tz)
   t.li = e->tli; t.ri = e->tri; t.lj = e->tlj; t.rj = e->trj; t.lk = e->tlk; t.rk = e->trk; t.L = e->L;
   t.two_d = e->tfsf_two_d ? 1 : 0;
   for (int a = 0; a < 3; ++a) {
@@ -1037,6 +1051,8 @@ static Tfsf make_tfsf(xf_engine *e) {
   t.src = (const double *)e->src;
   return t;
 }
+
+
 
 // launch geometry of the curl kernels from the engine's tiling settings
 static Tiling make_tiling(const xf_engine *e) {
@@ -1373,6 +1389,13 @@ static xf_status run_steps(xf_engine *e, long long n) {
   const Tiling tl = make_tiling(e);
   dim3 blk(tiling_threads(tl));
   dim3 grd(tl.nblocks);
+  // TFSF z faces in the curl kernels (split scheme)
+  const bool zc = !fused && tfsf_zc(e);
+  const int zper = (e->tri - e->tli + 1) * (e->trj - e->tlj + 1);
+  if (zc && !e->zterms) {
+    xf_status s = dev_alloc(e, &e->zterms, (size_t)8 * zper * sizeof(double));
+    if (s != XF_OK) return s;
+  }
   // coefficient tables of a tile go to LDS when they fit in 32 KB
   const size_t lds = lds_table_bytes<T>(e->nmat, tl);
   const bool use_lds = lds <= 32768 && !(e->tile_flags & XF_TILE_GLOBAL_TABLES);
@@ -1435,37 +1458,55 @@ static xf_status run_steps(xf_engine *e, long long n) {
     } else {
       const Grid<T> g = make_grid<T>(e);
       Tfsf t{};
-      if (e->has_tfsf) t = make_tfsf(e);
+      TfsfESeg es{};
+      TfsfHSeg hs{};
+      ZFaces zf{};
+      if (e->has_tfsf) {
+        t = make_tfsf(e);
+        tfsf_segs(e, zc, es, hs);
+        if (zc) {
+          zf.terms = (const double *)e->zterms;
+          zf.li = e->tli; zf.ri = e->tri; zf.lj = e->tlj; zf.rj = e->trj; zf.lk = e->tlk; zf.rk = e->trk;
+          // the E terms of the first step of this call (later steps' come
+          // with the H terms of the step before)
+          if (q == 0)
+            hipLaunchKernelGGL(k_tfsf_zterms, dim3((4 * zper + 255) / 256), dim3(256), 0, e->stream, t,
+                               (double *)e->zterms, 0, 4);
+        }
+      }
       if (ev) hipEventRecord(ev[0], e->stream);
       if (special) {
         if (use_lds)
-          hipLaunchKernelGGL((k_update_e<T, true, true>), grd, blk, lds, e->stream, g, tl);
+          hipLaunchKernelGGL((k_update_e<T, true, true>), grd, blk, lds, e->stream, g, tl, zf);
         else
-          hipLaunchKernelGGL((k_update_e<T, false, true>), grd, blk, 0, e->stream, g, tl);
+          hipLaunchKernelGGL((k_update_e<T, false, true>), grd, blk, 0, e->stream, g, tl, zf);
       } else {
         if (use_lds)
-          hipLaunchKernelGGL((k_update_e<T, true, false>), grd, blk, lds, e->stream, g, tl);
+          hipLaunchKernelGGL((k_update_e<T, true, false>), grd, blk, lds, e->stream, g, tl, zf);
         else
-          hipLaunchKernelGGL((k_update_e<T, false, false>), grd, blk, 0, e->stream, g, tl);
+          hipLaunchKernelGGL((k_update_e<T, false, false>), grd, blk, 0, e->stream, g, tl, zf);
       }
       if (ev) hipEventRecord(ev[1], e->stream);
       for (const SourceDev &sd : e->sources) launch_lumped<T>(e, sd, g.f[sd.comp], e->stream, step);
       if (e->has_tfsf) {
-        const int tot = e->nEy + e->nEz + e->nEx;
-        hipLaunchKernelGGL(k_tfsf_e<T>, dim3((tot + 255) / 256), dim3(256), 0, e->stream, g, t, e->nEy,
-                           e->nEz, e->nEx);
+        if (es.off[6] > 0)
+          hipLaunchKernelGGL(k_tfsf_e<T>, dim3((es.off[6] + 255) / 256), dim3(256), 0, e->stream, g, t, es);
+        // the incident line steps before the H kernel, whose z-face terms
+        // come from the new line (the E corrections above used the old one);
+        // the same line gives the next step's E terms
+        hipLaunchKernelGGL(k_tfsf_ifa, dim3(1), dim3(1024), 0, e->stream, t, step);
+        if (zc)
+          hipLaunchKernelGGL(k_tfsf_zterms, dim3((8 * zper + 255) / 256), dim3(256), 0, e->stream, t,
+                             (double *)e->zterms, 0, 8);
       }
       if (ev) hipEventRecord(ev[2], e->stream);
       if (use_lds)
-        hipLaunchKernelGGL((k_update_h<T, true>), grd, blk, lds, e->stream, g, tl, step);
+        hipLaunchKernelGGL((k_update_h<T, true>), grd, blk, lds, e->stream, g, tl, zf, step);
       else
-        hipLaunchKernelGGL((k_update_h<T, false>), grd, blk, 0, e->stream, g, tl, step);
+        hipLaunchKernelGGL((k_update_h<T, false>), grd, blk, 0, e->stream, g, tl, zf, step);
       if (ev) hipEventRecord(ev[3], e->stream);
-      if (e->has_tfsf) {
-        hipLaunchKernelGGL(k_tfsf_ifa, dim3(1), dim3(1024), 0, e->stream, t, step);
-        const int tot = e->hseg.off[12];
-        hipLaunchKernelGGL(k_tfsf_h<T>, dim3((tot + 255) / 256), dim3(256), 0, e->stream, g, t, e->hseg);
-      }
+      if (e->has_tfsf && hs.off[12] > 0)
+        hipLaunchKernelGGL(k_tfsf_h<T>, dim3((hs.off[12] + 255) / 256), dim3(256), 0, e->stream, g, t, hs);
     }
     // after the H half-step: NF-FF, DFT plane, broadband series, probes and
     // line monitors read the step's fields
@@ -1727,9 +1768,8 @@ static xf_status batch_build(xf_batch *b) {
     for (int w = 0; w < a.tl.nblocks; ++w) mc.push_back(make_int2(q, w));
     if (e->has_tfsf) {
       a.t = make_tfsf(e);
-      a.nEy = e->nEy; a.nEz = e->nEz; a.nEx = e->nEx;
-      a.hs = e->hseg;
-      const int te = (e->nEy + e->nEz + e->nEx + 255) / 256, th = (e->hseg.off[12] + 255) / 256;
+      tfsf_segs(e, false, a.es, a.hs);
+      const int te = (a.es.off[6] + 255) / 256, th = (a.hs.off[12] + 255) / 256;
       for (int w = 0; w < te; ++w) mte.push_back(make_int2(q, w));
       for (int w = 0; w < th; ++w) mth.push_back(make_int2(q, w));
       mifa.push_back(make_int2(q, 0));
@@ -1859,14 +1899,14 @@ static xf_status batch_run(xf_batch *b, long long n, bool replay = false) {
     for (xf_engine *e : b->e)
       for (const SourceDev &sd : e->sources) launch_lumped<T>(e, sd, (T *)e->f[sd.comp], b->stream, step);
     if (b->n_te > 0) hipLaunchKernelGGL(kb_tfsf_e<T>, dim3(b->n_te), dim3(256), 0, b->stream, in, b->map_te);
+    // the incident lines step before the H kernel (its z-face corrections)
+    if (b->n_ifa > 0)
+      hipLaunchKernelGGL(kb_tfsf_ifa<T>, dim3(b->n_ifa), dim3(1024), 0, b->stream, in, b->map_ifa, step);
     if (b->use_lds)
       hipLaunchKernelGGL((kb_update_h<T, true>), dim3(b->n_curl), blk, b->lds, b->stream, in, b->map_curl, step);
     else
       hipLaunchKernelGGL((kb_update_h<T, false>), dim3(b->n_curl), blk, 0, b->stream, in, b->map_curl, step);
-    if (b->n_ifa > 0) {
-      hipLaunchKernelGGL(kb_tfsf_ifa<T>, dim3(b->n_ifa), dim3(1024), 0, b->stream, in, b->map_ifa, step);
-      hipLaunchKernelGGL(kb_tfsf_h<T>, dim3(b->n_th), dim3(256), 0, b->stream, in, b->map_th);
-    }
+    if (b->n_th > 0) hipLaunchKernelGGL(kb_tfsf_h<T>, dim3(b->n_th), dim3(256), 0, b->stream, in, b->map_th);
     if (!replay) {
       if (b->n_nf > 0) {
         if (nf_pending == 0) nf_step0 = step;

@@ -389,6 +389,7 @@ struct Plane {
   int lx;        // layer along x (-1 outside the x slabs)
   T bx, ax;      // its CPML b/a (wave-uniform)
   T ps[6];       // psi values of the cell, by P_* slot
+  double z0, z1; // TFSF z-face terms of the cell (E: Ey, Ex; H: Hy, Hx), 0 if none
 };
 
 // Per-column constants of a curl kernel.
@@ -410,6 +411,92 @@ struct Col {
 };
 
 // ---------------------------------------------------------------------------
+// TFSF (tfsf/tfsf_3d.cpp).  The 1-D incident line, its projections and the
+// face-correction arithmetic are FP64 for both field precisions: the line is
+// a few hundred cells, and an FP32 field then takes one rounding per
+// correction, of the FP64 sum, instead of carrying an FP32 line's rounding
+// into the scattered-field region.
+struct Tfsf {
+  int li, ri, lj, rj, lk, rk, L;
+  int two_d;  // TFSF2D: Ez, Hx, Hy only, projections without z
+  const double *pf[3], *ph[3];
+  double te[3], th[3];
+  double ceie, ceihi, chih, chiei, c1, c2;
+  double ca[3], cb[3];
+  double *e_inc, *h_inc, *ab;
+  const double *src;
+};
+
+// TFSF3D::getIncident{Ex..Hz} (tfsf_3d.cpp:413-483)
+__device__ __forceinline__ double tfsf_inc(const Tfsf &t, int comp, int i, int j, int k) {
+  double proj, s;
+  const double *arr;
+  switch (comp) {
+    case EX: proj = t.ph[0][i - t.li + 1] + t.pf[1][j - t.lj] + t.pf[2][k - t.lk];
+             arr = t.e_inc; s = t.te[0]; break;
+    case EY: proj = t.pf[0][i - t.li] + t.ph[1][j - t.lj + 1] + t.pf[2][k - t.lk];
+             arr = t.e_inc; s = t.te[1]; break;
+    // TFSF2D::getIncident{Ez,Hx,Hy} (tfsf_2d.cpp:230-257) leave out the z
+    // projection
+    case EZ: proj = t.pf[0][i - t.li] + t.pf[1][j - t.lj];
+             if (!t.two_d) proj = proj + t.ph[2][k - t.lk + 1];
+             arr = t.e_inc; s = t.te[2]; break;
+    case HX: proj = t.pf[0][i - t.li] + t.ph[1][j - t.lj + 1];
+             if (!t.two_d) proj = proj + t.ph[2][k - t.lk + 1];
+             proj = proj - 0.5;
+             arr = t.h_inc; s = t.th[0]; break;
+    case HY: proj = t.ph[0][i - t.li + 1] + t.pf[1][j - t.lj];
+             if (!t.two_d) proj = proj + t.ph[2][k - t.lk + 1];
+             proj = proj - 0.5;
+             arr = t.h_inc; s = t.th[1]; break;
+    default: proj = t.ph[0][i - t.li + 1] + t.ph[1][j - t.lj + 1] + t.pf[2][k - t.lk] - 0.5;
+             arr = t.h_inc; s = t.th[2]; break;
+  }
+  const long long idx = (long long)proj;
+  const double w = proj - (double)idx;
+  return (1.0 - w) * (s * arr[idx]) + w * (s * arr[idx + 1]);
+}
+
+// The z faces in the curl kernels (ZFaces).  The cells of a z-face column
+// (k = lk or rk for E, lk - 1 or rk for H) that no x or y face corrects take
+// one correction each, so adding it at the end of the cell's update (after
+// the CPML terms) gives the same operations in the same order as the face
+// kernels.  The products coefficient * incident value are computed once per
+// step by k_tfsf_zterms (tfsf_3d.cpp:368-411 for E, :259-297 for H) into
+// FP64 segments [seg][i - li][j - lj] of (ni + 1) x (nj + 1) entries: E
+// segments 0..3 (Ey at lk and rk: ca_z * Hx_inc at lk - 1 and rk; Ex at lk
+// and rk: ca_z * Hy_inc), H segments 4..7 (Hy at lk - 1 and rk: cb_z * Ex_inc
+// at lk and rk, sampled on the total-field side; Hx: cb_z * Ey_inc).  A
+// curl-kernel lane loads its cell's terms with the plane's other loads.
+struct ZFaces {
+  const double *terms;  // null: the face kernels apply every correction
+  int li, ri, lj, rj, lk, rk;
+};
+__device__ __forceinline__ int zseg_size(const ZFaces &z) { return (z.ri - z.li + 1) * (z.rj - z.lj + 1); }
+__device__ __forceinline__ int zterm_index(const ZFaces &z, int seg, int i, int j) {
+  return seg * zseg_size(z) + (i - z.li) * (z.rj - z.lj + 1) + (j - z.lj);
+}
+__global__ void __launch_bounds__(256) k_tfsf_zterms(Tfsf t, double *buf, int s0, int s1) {
+  const int nj1 = t.rj - t.lj + 1, per = (t.ri - t.li + 1) * nj1;
+  const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (q >= (s1 - s0) * per) return;
+  const int s = s0 + q / per, r = q % per;
+  const int i = t.li + r / nj1, j = t.lj + r % nj1;
+  double v = 0.0;
+  switch (s) {
+    case 0: if (i > t.li && i < t.ri && j < t.rj) v = t.ca[2] * tfsf_inc(t, HX, i, j, t.lk - 1); break;
+    case 1: if (i > t.li && i < t.ri && j < t.rj) v = t.ca[2] * tfsf_inc(t, HX, i, j, t.rk); break;
+    case 2: if (i < t.ri && j > t.lj && j < t.rj) v = t.ca[2] * tfsf_inc(t, HY, i, j, t.lk - 1); break;
+    case 3: if (i < t.ri && j > t.lj && j < t.rj) v = t.ca[2] * tfsf_inc(t, HY, i, j, t.rk); break;
+    case 4: if (i < t.ri) v = t.cb[2] * tfsf_inc(t, EX, i, j, t.lk); break;
+    case 5: if (i < t.ri) v = t.cb[2] * tfsf_inc(t, EX, i, j, t.rk); break;
+    case 6: if (j < t.rj) v = t.cb[2] * tfsf_inc(t, EY, i, j, t.lk); break;
+    default: if (j < t.rj) v = t.cb[2] * tfsf_inc(t, EY, i, j, t.rk); break;
+  }
+  buf[q + s0 * per] = v;
+}
+
+// ---------------------------------------------------------------------------
 // E half-step: Simulation::updateE (simulation_update.cpp:170-196) fused with
 // PML::updateE of every face (perfect_match_layer.cpp:574-718), faces in x,
 // y, z order.  One thread owns a (j,k) column and marches over TX x-planes.
@@ -422,7 +509,7 @@ struct Col {
 // n2 = Hz(j-1), n3 = Hx(j-1).
 template <typename T, bool SLAB>
 __device__ __forceinline__ void load_plane_e(const Grid<T> &g, const Col<T> &q, unsigned cell, int i,
-                                             Plane<T> &p) {
+                                             Plane<T> &p, const ZFaces &zf, bool zplane) {
   const unsigned o = cell * (unsigned)sizeof(T);
   p.m = ldm(q.mat, cell);
   p.f0 = ldo(q.u0, o);
@@ -457,13 +544,20 @@ __device__ __forceinline__ void load_plane_e(const Grid<T> &g, const Col<T> &q, 
       if (lx >= 0) p.ps[P_CX] = g.psi_e[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
       if (ly >= 0) p.ps[P_CY] = g.psi_e[P_CY][psi_y(i, ly, k, Ly, g.pz)];
     }
+    // TFSF z faces: Ey off the x faces, Ex off the y faces
+    p.z0 = p.z1 = 0.0;
+    if (zplane && (k == zf.lk || k == zf.rk)) {
+      const int up = k == zf.rk;
+      if (j >= zf.lj && j < zf.rj && i > zf.li && i < zf.ri) p.z0 = zf.terms[zterm_index(zf, up, i, j)];
+      if (i < zf.ri && j > zf.lj && j < zf.rj) p.z1 = zf.terms[zterm_index(zf, 2 + up, i, j)];
+    }
   }
 }
 
 template <typename T, bool SLAB, bool SPECIAL>
 __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTables<T> &tt, const Col<T> &q,
                                                 const Plane<T> &p, int i, unsigned cell, T hz_prev,
-                                                T hy_prev) {
+                                                T hy_prev, const ZFaces &zf, bool zt) {
   const int nx = g.nx, ny = g.ny, nz = g.nz, j = q.j, k = q.k;
   const int m = (int)p.m;
   const T ex = p.f0, ey = p.f1, ez = p.f2, hx = p.g0, hy = p.g1, hz = p.g2;
@@ -553,6 +647,19 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
         vz = vz + tt.pj[tt.jcol + m].y * ps;
       }
     }
+    // TFSF z faces (wave-uniform zt: the wave holds a z-face column and the
+    // plane lies in [li, ri]); Ey -/+ and Ex +/- at lk / rk
+    if (zt && (k == zf.lk || k == zf.rk)) {
+      const bool up = k == zf.rk;
+      if (j >= zf.lj && j < zf.rj && i > zf.li && i < zf.ri) {
+        const double v = vy;
+        vy = (T)(up ? v + p.z0 : v - p.z0);
+      }
+      if (i >= zf.li && i < zf.ri && j > zf.lj && j < zf.rj) {
+        const double v = vx;
+        vx = (T)(up ? v - p.z1 : v + p.z1);
+      }
+    }
   }
   const unsigned o = cell * (unsigned)sizeof(T);
   sto(q.u0, o, vx);
@@ -562,8 +669,9 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
 
 // SPECIAL: the engine has a design box or Drude cells, whose planes take
 // the general path; without them that code is not compiled in.
+// zf: the TFSF z faces this kernel applies (zf.terms null: none).
 template <typename T, bool LDS, bool SPECIAL>
-__device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl, int block) {
+__device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl, const ZFaces &zf, int block) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   int bz, by, bx;
   if (!tile_of_block(tl, block, bz, by, bx)) return;
@@ -589,9 +697,13 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
   q.k = k;
   q.cp = col_cpml(g.be, g.ae, live ? e_layer(j, ny, g.dn[1], g.dp[1]) : -1,
                   live ? e_layer(k, nz, g.dn[2], g.dp[2]) : -1, j, k);
+  // a wave that holds a TFSF z-face column (k = lk or rk, j in [lj, rj])
+  // applies those corrections on the planes [li, ri], on the slab path
+  const bool zwave =
+      zf.terms && __ballot(live && (k == zf.lk || k == zf.rk) && j >= zf.lj && j <= zf.rj) != 0ull;
   // a wave whose column meets no y or z slab runs the plain path on every
   // plane outside the x slabs
-  const bool col_slab = __ballot(q.cp.ly >= 0 || q.cp.lz >= 0) != 0ull;
+  const bool col_slab = zwave || __ballot(q.cp.ly >= 0 || q.cp.lz >= 0) != 0ull;
   // a wave whose column crosses the design box or the Drude box takes the
   // general path on those boxes' planes
   const bool col_design =
@@ -611,9 +723,9 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
       hy_prev = ldo(q.r1, (c - (unsigned)g.si) * (unsigned)sizeof(T));
     }
     if (slab)
-      load_plane_e<T, true>(g, q, c, i0, A);
+      load_plane_e<T, true>(g, q, c, i0, A, zf, zwave && i0 >= zf.li && i0 <= zf.ri);
     else
-      load_plane_e<T, false>(g, q, c, i0, A);
+      load_plane_e<T, false>(g, q, c, i0, A, zf, false);
   }
   TileTables<T> tt;
   if (LDS) {
@@ -626,9 +738,9 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
   if (!live) return;
   for (int i = i0;;) {
     if (slab)
-      compute_plane_e<T, true, SPECIAL>(g, tt, q, A, i, c, hz_prev, hy_prev);
+      compute_plane_e<T, true, SPECIAL>(g, tt, q, A, i, c, hz_prev, hy_prev, zf, zwave && i >= zf.li && i <= zf.ri);
     else
-      compute_plane_e<T, false, SPECIAL>(g, tt, q, A, i, c, hz_prev, hy_prev);
+      compute_plane_e<T, false, SPECIAL>(g, tt, q, A, i, c, hz_prev, hy_prev, zf, false);
     hz_prev = A.g2;
     hy_prev = A.g1;
     if (++i >= i1) break;
@@ -636,9 +748,9 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
     slab = col_slab || e_layer(i, nx, g.dn[0], g.dp[0]) >= 0 || (col_design && i >= g.db[0] && i < g.db[1]) ||
            (col_drude && i >= g.xb[0] && i < g.xb[1]);
     if (slab)
-      load_plane_e<T, true>(g, q, c, i, A);
+      load_plane_e<T, true>(g, q, c, i, A, zf, zwave && i >= zf.li && i <= zf.ri);
     else
-      load_plane_e<T, false>(g, q, c, i, A);
+      load_plane_e<T, false>(g, q, c, i, A, zf, false);
   }
 }
 
@@ -652,7 +764,7 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
 // n2 = Ez(j+1), n3 = Ex(j+1), n4 = Ez(i+1), n5 = Ey(i+1).
 template <typename T, bool SLAB>
 __device__ __forceinline__ void load_plane_h(const Grid<T> &g, const Col<T> &q, unsigned cell, int i,
-                                             unsigned di, Plane<T> &p) {
+                                             unsigned di, Plane<T> &p, const ZFaces &zf, bool zplane) {
   const unsigned o = cell * (unsigned)sizeof(T);
   p.m = ldm(q.mat, cell);
   p.f0 = ldo(q.u0, o);
@@ -687,12 +799,20 @@ __device__ __forceinline__ void load_plane_h(const Grid<T> &g, const Col<T> &q, 
       if (lx >= 0) p.ps[P_CX] = g.psi_h[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
       if (ly >= 0) p.ps[P_CY] = g.psi_h[P_CY][psi_y(i, ly, k, Ly, g.pz)];
     }
+    // TFSF z faces: Hy and Hx at lk - 1 and rk
+    p.z0 = p.z1 = 0.0;
+    if (zplane && (k == zf.lk - 1 || k == zf.rk)) {
+      const int up = k == zf.rk;
+      if (i < zf.ri && j >= zf.lj && j <= zf.rj) p.z0 = zf.terms[zterm_index(zf, 4 + up, i, j)];
+      if (j >= zf.lj && j < zf.rj) p.z1 = zf.terms[zterm_index(zf, 6 + up, i, j)];
+    }
   }
 }
 
 template <typename T, bool SLAB>
 __device__ __forceinline__ bool compute_plane_h(const Grid<T> &g, const TileTables<T> &tt, const Col<T> &q,
-                                                const Plane<T> &p, int i, unsigned cell, T ez_c, T ey_c) {
+                                                const Plane<T> &p, int i, unsigned cell, T ez_c, T ey_c,
+                                                const ZFaces &zf, bool zt) {
   const int nx = g.nx, j = q.j, k = q.k;
   const bool kin = q.kin, jin = q.jin;
   const int m = (int)p.m;
@@ -757,6 +877,18 @@ __device__ __forceinline__ bool compute_plane_h(const Grid<T> &g, const TileTabl
         vz = vz + tt.pj[tt.jcol + m].y * ps;
       }
     }
+    // TFSF z faces: Hy +/- and Hx -/+ at lk - 1 / rk
+    if (zt && (k == zf.lk - 1 || k == zf.rk)) {
+      const bool up = k == zf.rk;
+      if (i >= zf.li && i < zf.ri && j >= zf.lj && j <= zf.rj) {
+        const double v = vy;
+        vy = (T)(up ? (v - p.z0) : (v + p.z0));
+      }
+      if (i >= zf.li && i <= zf.ri && j >= zf.lj && j < zf.rj) {
+        const double v = vx;
+        vx = (T)(up ? (v + p.z1) : (v - p.z1));
+      }
+    }
   }
   const unsigned o = cell * (unsigned)sizeof(T);
   sto(q.u0, o, vx);
@@ -767,7 +899,8 @@ __device__ __forceinline__ bool compute_plane_h(const Grid<T> &g, const TileTabl
 }
 
 template <typename T, bool LDS>
-__device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl, int block, long long step) {
+__device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl, const ZFaces &zf, int block,
+                                              long long step) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   int bz, by, bx;
   if (!tile_of_block(tl, block, bz, by, bx)) return;
@@ -793,7 +926,10 @@ __device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl
   q.k = k;
   q.cp = col_cpml(g.bm, g.am, live ? h_layer(j, ny, g.dn[1], g.dp[1]) : -1,
                   live ? h_layer(k, nz, g.dn[2], g.dp[2]) : -1, j, k);
-  const bool col_slab = __ballot(q.cp.ly >= 0 || q.cp.lz >= 0) != 0ull;
+  // TFSF z-face columns (k = lk - 1 or rk, j in [lj, rj]), on the slab path
+  const bool zwave =
+      zf.terms && __ballot(live && (k == zf.lk - 1 || k == zf.rk) && j >= zf.lj && j <= zf.rj) != 0ull;
+  const bool col_slab = zwave || __ballot(q.cp.ly >= 0 || q.cp.lz >= 0) != 0ull;
   unsigned c = (unsigned)((long long)j * g.sj + k);
   T ez_c = T(0), ey_c = T(0);
   Plane<T> A;
@@ -804,9 +940,9 @@ __device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl
     ey_c = ldo(q.r1, c * (unsigned)sizeof(T));
     const unsigned di = i0 < nx ? bsi : 0u;
     if (slab)
-      load_plane_h<T, true>(g, q, c, i0, di, A);
+      load_plane_h<T, true>(g, q, c, i0, di, A, zf, zwave && i0 >= zf.li && i0 <= zf.ri);
     else
-      load_plane_h<T, false>(g, q, c, i0, di, A);
+      load_plane_h<T, false>(g, q, c, i0, di, A, zf, false);
   }
   TileTables<T> tt;
   if (LDS) {
@@ -820,9 +956,9 @@ __device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl
   bool bad = false;
   for (int i = i0;;) {
     if (slab)
-      bad |= compute_plane_h<T, true>(g, tt, q, A, i, c, ez_c, ey_c);
+      bad |= compute_plane_h<T, true>(g, tt, q, A, i, c, ez_c, ey_c, zf, zwave && i >= zf.li && i <= zf.ri);
     else
-      bad |= compute_plane_h<T, false>(g, tt, q, A, i, c, ez_c, ey_c);
+      bad |= compute_plane_h<T, false>(g, tt, q, A, i, c, ez_c, ey_c, zf, false);
     ez_c = A.n4;
     ey_c = A.n5;
     if (++i >= i1) break;
@@ -830,67 +966,20 @@ __device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl
     slab = col_slab || h_layer(i, nx, g.dn[0], g.dp[0]) >= 0;
     const unsigned di = i < nx ? bsi : 0u;
     if (slab)
-      load_plane_h<T, true>(g, q, c, i, di, A);
+      load_plane_h<T, true>(g, q, c, i, di, A, zf, zwave && i >= zf.li && i <= zf.ri);
     else
-      load_plane_h<T, false>(g, q, c, i, di, A);
+      load_plane_h<T, false>(g, q, c, i, di, A, zf, false);
   }
   if (bad) atomicMin(g.div, (unsigned long long)step);
 }
 
 template <typename T, bool LDS, bool SPECIAL>
-__global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl) {
-  update_e_body<T, LDS, SPECIAL>(g, tl, blockIdx.x);
+__global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl, ZFaces zf) {
+  update_e_body<T, LDS, SPECIAL>(g, tl, zf, blockIdx.x);
 }
 template <typename T, bool LDS>
-__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl, long long step) {
-  update_h_body<T, LDS>(g, tl, blockIdx.x, step);
-}
-
-// ---------------------------------------------------------------------------
-// TFSF (tfsf/tfsf_3d.cpp).  The 1-D incident line, its projections and the
-// face-correction arithmetic are FP64 for both field precisions: the line is
-// a few hundred cells, and an FP32 field then takes one rounding per
-// correction, of the FP64 sum, instead of carrying an FP32 line's rounding
-// into the scattered-field region.
-struct Tfsf {
-  int li, ri, lj, rj, lk, rk, L;
-  int two_d;  // TFSF2D: Ez, Hx, Hy only, projections without z
-  const double *pf[3], *ph[3];
-  double te[3], th[3];
-  double ceie, ceihi, chih, chiei, c1, c2;
-  double ca[3], cb[3];
-  double *e_inc, *h_inc, *ab;
-  const double *src;
-};
-
-// TFSF3D::getIncident{Ex..Hz} (tfsf_3d.cpp:413-483)
-__device__ __forceinline__ double tfsf_inc(const Tfsf &t, int comp, int i, int j, int k) {
-  double proj, s;
-  const double *arr;
-  switch (comp) {
-    case EX: proj = t.ph[0][i - t.li + 1] + t.pf[1][j - t.lj] + t.pf[2][k - t.lk];
-             arr = t.e_inc; s = t.te[0]; break;
-    case EY: proj = t.pf[0][i - t.li] + t.ph[1][j - t.lj + 1] + t.pf[2][k - t.lk];
-             arr = t.e_inc; s = t.te[1]; break;
-    // TFSF2D::getIncident{Ez,Hx,Hy} (tfsf_2d.cpp:230-257) leave out the z
-    // projection
-    case EZ: proj = t.pf[0][i - t.li] + t.pf[1][j - t.lj];
-             if (!t.two_d) proj = proj + t.ph[2][k - t.lk + 1];
-             arr = t.e_inc; s = t.te[2]; break;
-    case HX: proj = t.pf[0][i - t.li] + t.ph[1][j - t.lj + 1];
-             if (!t.two_d) proj = proj + t.ph[2][k - t.lk + 1];
-             proj = proj - 0.5;
-             arr = t.h_inc; s = t.th[0]; break;
-    case HY: proj = t.ph[0][i - t.li + 1] + t.pf[1][j - t.lj];
-             if (!t.two_d) proj = proj + t.ph[2][k - t.lk + 1];
-             proj = proj - 0.5;
-             arr = t.h_inc; s = t.th[1]; break;
-    default: proj = t.ph[0][i - t.li + 1] + t.ph[1][j - t.lj + 1] + t.pf[2][k - t.lk] - 0.5;
-             arr = t.h_inc; s = t.th[2]; break;
-  }
-  const long long idx = (long long)proj;
-  const double w = proj - (double)idx;
-  return (1.0 - w) * (s * arr[idx]) + w * (s * arr[idx + 1]);
+__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl, ZFaces zf, long long step) {
+  update_h_body<T, LDS>(g, tl, zf, blockIdx.x, step);
 }
 
 // TFSF3D::updateIncidentField (tfsf_3d.cpp:153-183): one workgroup
@@ -917,75 +1006,64 @@ __global__ void __launch_bounds__(1024) k_tfsf_ifa(Tfsf t, long long step) {
   tfsf_ifa_body(t, step);
 }
 
-__device__ __forceinline__ void perimeter(int p, int A, int B, int &u, int &v) {
-  if (p < A) { u = p; v = 0; return; }
-  if (p < 2 * A) { u = p - A; v = B - 1; return; }
-  const int q = p - 2 * A;
-  if (q < B - 2) { u = 0; v = 1 + q; return; }
-  u = A - 1; v = 1 + (q - (B - 2));
-}
-
-// TFSF3D::updateE (tfsf_3d.cpp:299-411).  Edge cells take two corrections;
-// one thread owns each target cell and applies them in the reference's
-// xn, xp, yn, yp, zn, zp order.
+// TFSF3D::updateE (tfsf_3d.cpp:299-411).  The face cells come in six
+// segments, each enumerated with k fastest where k varies, so that a wave's
+// cells share cache lines: Ey on the x faces (i = li, ri; j in [lj, rj); k in
+// [lk, rk]), Ey on the z faces away from the x faces, Ez on the x faces (j in
+// [lj, rj], k in [lk, rk)), Ez on the y faces away from the x faces, Ex on the
+// y faces (i in [li, ri), k in [lk, rk]), Ex on the z faces away from the y
+// faces.  The two z-face segments are empty when the curl kernels apply them
+// (ZFaces).  Edge cells take two corrections; one thread owns each target cell
+// and applies them in the reference's xn, xp, yn, yp, zn, zp order.
+struct TfsfESeg {
+  int off[7];  // prefix offsets of the six segments
+};
 template <typename T>
-__device__ __forceinline__ void tfsf_e_body(const Grid<T> &g, const Tfsf &t, int nEy, int nEz, int nEx,
-                                            int block) {
-  int q = block * (int)blockDim.x + (int)threadIdx.x;
+__device__ __forceinline__ void tfsf_e_body(const Grid<T> &g, const Tfsf &t, const TfsfESeg &s, int block) {
+  const int q = block * (int)blockDim.x + (int)threadIdx.x;
+  if (q >= s.off[6]) return;
+  int seg = 0;
+  while (q >= s.off[seg + 1]) ++seg;
+  const int r = q - s.off[seg];
   const int li = t.li, ri = t.ri, lj = t.lj, rj = t.rj, lk = t.lk, rk = t.rk;
-  if (q < nEy) {
-    const int A = ri - li + 1, B = rk - lk + 1, P = 2 * A + 2 * (B - 2);
-    const int j = lj + q / P;
-    int u, v;
-    perimeter(q % P, A, B, u, v);
-    const int i = li + u, k = lk + v;
-    T *p = &g.f[EY][(long long)i * g.si + (long long)j * g.sj + k];
-    double val = *p;
+  const int ni = ri - li, nj = rj - lj, nk = rk - lk;
+  int i, j, k, comp;
+  switch (seg) {
+    case 0: { const int w = r / (nk + 1); k = lk + r % (nk + 1); j = lj + w % nj; i = w / nj ? ri : li; comp = EY; break; }
+    case 1: { const int w = r / nj; j = lj + r % nj; i = li + 1 + w % (ni - 1); k = w / (ni - 1) ? rk : lk; comp = EY; break; }
+    case 2: { const int w = r / nk; k = lk + r % nk; j = lj + w % (nj + 1); i = w / (nj + 1) ? ri : li; comp = EZ; break; }
+    case 3: { const int w = r / nk; k = lk + r % nk; i = li + 1 + w % (ni - 1); j = w / (ni - 1) ? rj : lj; comp = EZ; break; }
+    case 4: { const int w = r / (nk + 1); k = lk + r % (nk + 1); i = li + w % ni; j = w / ni ? rj : lj; comp = EX; break; }
+    default: { const int w = r / (nj - 1); j = lj + 1 + r % (nj - 1); i = li + w % ni; k = w / ni ? rk : lk; comp = EX; break; }
+  }
+  T *p = &g.f[comp][(long long)i * g.si + (long long)j * g.sj + k];
+  double val = *p;
+  if (comp == EY) {
     if (i == li) val = val + t.ca[0] * tfsf_inc(t, HZ, li - 1, j, k);
     if (i == ri) val = val - t.ca[0] * tfsf_inc(t, HZ, ri, j, k);
     if (k == lk) val = val - t.ca[2] * tfsf_inc(t, HX, i, j, lk - 1);
     if (k == rk) val = val + t.ca[2] * tfsf_inc(t, HX, i, j, rk);
-    *p = (T)val;
-    return;
-  }
-  q -= nEy;
-  if (q < nEz) {
-    const int A = ri - li + 1, B = rj - lj + 1, P = 2 * A + 2 * (B - 2);
-    const int k = lk + q / P;
-    int u, v;
-    perimeter(q % P, A, B, u, v);
-    const int i = li + u, j = lj + v;
-    T *p = &g.f[EZ][(long long)i * g.si + (long long)j * g.sj + k];
-    double val = *p;
+  } else if (comp == EZ) {
     if (i == li) val = val - t.ca[0] * tfsf_inc(t, HY, li - 1, j, k);
     if (i == ri) val = val + t.ca[0] * tfsf_inc(t, HY, ri, j, k);
     if (j == lj) val = val + t.ca[1] * tfsf_inc(t, HX, i, lj - 1, k);
     if (j == rj) val = val - t.ca[1] * tfsf_inc(t, HX, i, rj, k);
-    *p = (T)val;
-    return;
-  }
-  q -= nEz;
-  if (q < nEx) {
-    const int A = rj - lj + 1, B = rk - lk + 1, P = 2 * A + 2 * (B - 2);
-    const int i = li + q / P;
-    int u, v;
-    perimeter(q % P, A, B, u, v);
-    const int j = lj + u, k = lk + v;
-    T *p = &g.f[EX][(long long)i * g.si + (long long)j * g.sj + k];
-    double val = *p;
+  } else {
     if (j == lj) val = val - t.ca[1] * tfsf_inc(t, HZ, i, lj - 1, k);
     if (j == rj) val = val + t.ca[1] * tfsf_inc(t, HZ, i, rj, k);
     if (k == lk) val = val + t.ca[2] * tfsf_inc(t, HY, i, j, lk - 1);
     if (k == rk) val = val - t.ca[2] * tfsf_inc(t, HY, i, j, rk);
-    *p = (T)val;
   }
+  *p = (T)val;
 }
 template <typename T>
-__global__ void k_tfsf_e(Grid<T> g, Tfsf t, int nEy, int nEz, int nEx) {
-  tfsf_e_body<T>(g, t, nEy, nEz, nEx, blockIdx.x);
+__global__ void k_tfsf_e(Grid<T> g, Tfsf t, TfsfESeg s) {
+  tfsf_e_body<T>(g, t, s, blockIdx.x);
 }
 
 // TFSF3D::updateH (tfsf_3d.cpp:185-297).  No H cell takes two corrections.
+// Twelve segments (x faces, y faces, z faces; k fastest where k varies); the
+// four z-face segments are empty when the curl kernels apply them (ZFaces).
 struct TfsfHSeg {
   int off[13];  // prefix offsets of the 12 face loops
 };
@@ -1010,13 +1088,13 @@ __device__ __forceinline__ void tfsf_h_body(const Grid<T> &g, const Tfsf &t, con
               comp = HZ; src = EY; coef = t.cb[0]; add = false; break; }
     case 3: { const int nk = rk - lk; j = lj + r / nk; k = lk + r % nk; i = ri;
               comp = HY; src = EZ; coef = t.cb[0]; add = true; break; }
-    case 4: { const int ni = ri + 1 - li; k = lk + r / ni; i = li + r % ni; j = lj - 1;
+    case 4: { const int nk = rk - lk; i = li + r / nk; k = lk + r % nk; j = lj - 1;
               comp = HX; src = EZ; coef = t.cb[1]; add = true; break; }   // yn hx += cby*ezi(lj)
-    case 5: { const int ni = ri - li; k = lk + r / ni; i = li + r % ni; j = lj - 1;
+    case 5: { const int nk = rk + 1 - lk; i = li + r / nk; k = lk + r % nk; j = lj - 1;
               comp = HZ; src = EX; coef = t.cb[1]; add = false; break; }
-    case 6: { const int ni = ri + 1 - li; k = lk + r / ni; i = li + r % ni; j = rj;
+    case 6: { const int nk = rk - lk; i = li + r / nk; k = lk + r % nk; j = rj;
               comp = HX; src = EZ; coef = t.cb[1]; add = false; break; }
-    case 7: { const int ni = ri - li; k = lk + r / ni; i = li + r % ni; j = rj;
+    case 7: { const int nk = rk + 1 - lk; i = li + r / nk; k = lk + r % nk; j = rj;
               comp = HZ; src = EX; coef = t.cb[1]; add = true; break; }
     case 8: { const int nj = rj + 1 - lj; i = li + r / nj; j = lj + r % nj; k = lk - 1;
               comp = HY; src = EX; coef = t.cb[2]; add = true; break; }   // zn hy += cbz*exi(lk)
@@ -1832,7 +1910,7 @@ struct Inst {
   Grid<T> g;
   Tiling tl;
   Tfsf t;
-  int nEy, nEz, nEx;
+  TfsfESeg es;
   TfsfHSeg hs;
   Nffft nf;
   T *ring;  // NF-FF sample ring ([NF_RING][8][surface cells])
@@ -1851,19 +1929,19 @@ struct InstAdj {
 template <typename T, bool LDS, bool SPECIAL>
 __global__ void __launch_bounds__(256) kb_update_e(const Inst<T> *__restrict__ in, const int2 *__restrict__ map) {
   const int2 w = map[blockIdx.x];
-  update_e_body<T, LDS, SPECIAL>(in[w.x].g, in[w.x].tl, w.y);
+  update_e_body<T, LDS, SPECIAL>(in[w.x].g, in[w.x].tl, ZFaces{}, w.y);
 }
 template <typename T, bool LDS>
 __global__ void __launch_bounds__(256) kb_update_h(const Inst<T> *__restrict__ in, const int2 *__restrict__ map,
                                                    long long step) {
   const int2 w = map[blockIdx.x];
-  update_h_body<T, LDS>(in[w.x].g, in[w.x].tl, w.y, step);
+  update_h_body<T, LDS>(in[w.x].g, in[w.x].tl, ZFaces{}, w.y, step);
 }
 template <typename T>
 __global__ void kb_tfsf_e(const Inst<T> *__restrict__ in, const int2 *__restrict__ map) {
   const int2 w = map[blockIdx.x];
   const Inst<T> &a = in[w.x];
-  tfsf_e_body<T>(a.g, a.t, a.nEy, a.nEz, a.nEx, w.y);
+  tfsf_e_body<T>(a.g, a.t, a.es, w.y);
 }
 template <typename T>
 __global__ void kb_tfsf_h(const Inst<T> *__restrict__ in, const int2 *__restrict__ map) {
