@@ -617,10 +617,27 @@ extern "C" xf_status xf_engine_set_nffft(xf_engine *e, const xf_nffft_desc *d) {
 
 extern "C" int64_t xf_engine_nffft_size(const xf_engine *e) { return e && e->has_nf ? e->nf_size : 0; }
 
+// The device keeps the y-face cells (i, k), k fastest; the interface orders
+// them (k, i).  Reorder `blocks` consecutive y-face blocks of nxb * nzb
+// entries of `w` doubles each, in place.
+static void y_faces_to_interface(double *y, long long blocks, int nxb, int nzb, int w) {
+  std::vector<double> tmp((size_t)nxb * nzb * w);
+  for (long long b = 0; b < blocks; ++b) {
+    double *blk = y + b * (long long)nxb * nzb * w;
+    for (int i = 0; i < nxb; ++i)
+      for (int k = 0; k < nzb; ++k)
+        for (int c = 0; c < w; ++c) tmp[((size_t)k * nxb + i) * w + c] = blk[((size_t)i * nzb + k) * w + c];
+    std::memcpy(blk, tmp.data(), tmp.size() * sizeof(double));
+  }
+}
+
 extern "C" xf_status xf_engine_get_nffft(xf_engine *e, double *out) {
   if (!e || !out || !e->has_nf) FAILS(XF_ESTATE, "no NF-FF surface");
   HIPCHK(engine_wait(e));
   HIPCHK(hipMemcpy(out, e->acc, (size_t)e->nf_size * sizeof(double2), hipMemcpyDeviceToHost));
+  // y pair: 8 currents x nf frequencies, each a [nxb][nzb] complex block
+  const Nffft &n = e->nf;
+  y_faces_to_interface(out + 2 * n.base[1], 8LL * n.nf, n.nxb, n.nzb, 2);
   return XF_OK;
 }
 
@@ -750,6 +767,10 @@ extern "C" xf_status xf_engine_get_broadband_samples(xf_engine *e, double *out) 
     HIPCHK(hipMemcpy(tmp.data(), e->bb_hist, n * sizeof(float), hipMemcpyDeviceToHost));
     for (size_t q = 0; q < n; ++q) out[q] = (double)tmp[q];
   }
+  // [step][8][Q]: the y faces of each (step, current) row in interface order
+  const Nffft &b = e->bb;
+  const long long Q = bb_cells(b), cx = (long long)b.nyb * b.nzb;
+  for (long long r = 0; r < e->bb_T * 8; ++r) y_faces_to_interface(out + r * Q + cx, 1, b.nxb, b.nzb, 1);
   return XF_OK;
 }
 

@@ -1197,7 +1197,11 @@ __global__ void k_box_gather(const T *f, long long si, long long sj, int i0, int
 // ---------------------------------------------------------------------------
 // NffftFd::update (nffft_fd.cpp:108-203) with the face-centre stencils of
 // linear_interpolation_emf.cpp:286-388.  One thread per face cell of a face
-// pair; FP64 accumulators in every precision.
+// pair; FP64 accumulators in every precision.  Device order of the surface
+// cells: x faces (j, k), y faces (i, k), z faces (i, j), the last index
+// fastest, so that the threads of a wave read consecutive k wherever k varies
+// (the reference, and get_nffft, order the y faces (k, i); the host permutes
+// them on the way out).
 struct Nffft {
   int li, ri, lj, rj, lk, rk, nf;
   int nxb, nyb, nzb;
@@ -1237,7 +1241,7 @@ __device__ __forceinline__ bool nffft_values(const Grid<T> &g, const Nffft &n, i
     v[7] = T(0.25) * (Hy[C3(ri, j, k)] + Hy[C3(ri, j + 1, k)] + Hy[C3(ri - 1, j, k)] + Hy[C3(ri - 1, j + 1, k)]);
   } else if ((q -= cx) < cy) {
     pair = 1; cells = cy; cell = q;
-    const int k = n.lk + q / n.nxb, i = n.li + q % n.nxb;
+    const int i = n.li + q / n.nzb, k = n.lk + q % n.nzb;
     const int lj = n.lj, rj = n.rj;
     v[0] = T(-1) * (T(0.5) * (Ex[C3(i, lj, k)] + Ex[C3(i, lj + 1, k)]));
     v[1] = T(0.5) * (Ez[C3(i, lj, k)] + Ez[C3(i + 1, lj, k)]);
@@ -1363,7 +1367,7 @@ __global__ void __launch_bounds__(256) k_farfield(Nffft n, FarDir d, double2 *ou
         dn = dk * ((double)n.li * stcp + r + z);
         dp = dk * ((double)n.ri * stcp + r + z);
       } else if (pair == 1) {
-        const int k = n.lk + q / n.nxb, i = n.li + q % n.nxb;
+        const int i = n.li + q / n.nzb, k = n.lk + q % n.nzb;
         const double x = ((double)i + 0.5) * stcp, z = ((double)k + 0.5) * ct;
         dn = dk * (x + (double)n.lj * stsp + z);
         dp = dk * (x + (double)n.rj * stsp + z);
@@ -1479,7 +1483,7 @@ __global__ void __launch_bounds__(256) k_bb_fold(BbFar d, const double2 *__restr
       x = ((double)i + 0.0) - d.cx; y = ((double)j + 0.5) - d.cy; z = ((double)k + 0.5) - d.cz;
       tgt = cur == 0 ? 4 : cur == 1 ? 5 : cur == 2 ? 1 : 2;  // my, mz, jy, jz
     } else if ((cell -= cxq) < cyq) {
-      const int k = f.lk + (int)(cell / f.nxb), i = f.li + (int)(cell % f.nxb), j = side ? f.rj : f.lj;
+      const int i = f.li + (int)(cell / f.nzb), k = f.lk + (int)(cell % f.nzb), j = side ? f.rj : f.lj;
       x = ((double)i + 0.5) - d.cx; y = ((double)j + 0.0) - d.cy; z = ((double)k + 0.5) - d.cz;
       tgt = cur == 0 ? 5 : cur == 1 ? 3 : cur == 2 ? 2 : 0;  // mz, mx, jz, jx
     } else {
