@@ -66,6 +66,11 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Yee Gcells/s & % HBM peak; batched vs plain; adjoint grad time at stated tol"
 
 
+def sphere_radius(n):
+    """The sphere radius (cells) of dielectric_sphere_scene at grid edge n."""
+    return 100 if n == 400 else max(2, min(100, n // 4))
+
+
 def scene_400(n=400, steps_total=2000):
     from xfdtd_amd.workloads import dielectric_sphere_scene
     return dielectric_sphere_scene(n), steps_total
@@ -379,8 +384,8 @@ def run_broadband(steps=1200):
 
 
 def run_dispersive(n=400, steps=100, warmup=5):
-    """The headline 400^3 run (FP32) with its radius-100 sphere made of each
-    medium in turn: a plain eps_r 4 dielectric, a Drude plasma (eps_inf 1,
+    """The headline 400^3 run (FP32) with its sphere (radius 100 cells at
+    400^3, scaled with smaller grids) made of each medium in turn: a plain eps_r 4 dielectric, a Drude plasma (eps_inf 1,
     omega_p 2 pi 20 GHz, nu 2 pi 1 GHz), a Debye medium (eps_s 4, eps_inf 2,
     tau 1 / (2 pi 5 GHz)) and a Lorentz medium (eps_s 4, eps_inf 2, omega_p
     2 pi 15 GHz, nu 2 pi 1 GHz).  The dispersive cells take
@@ -392,7 +397,7 @@ def run_dispersive(n=400, steps=100, warmup=5):
              "drude": S.DrudeMedium("p", 1.0, 2 * math.pi * 20e9, 2 * math.pi * 1e9),
              "debye": S.DebyMedium("d", 4.0, 2.0, 1 / (2 * math.pi * 5e9)),
              "lorentz": S.LorentzMedium("l", 4.0, 2.0, 2 * math.pi * 15e9, 2 * math.pi * 1e9)}
-    out = {"workload": f"{n}^3 headline scene (FP32), sphere of radius 100 cells made of each medium",
+    out = {"workload": f"{n}^3 headline scene (FP32), sphere of radius {sphere_radius(n)} cells made of each medium",
            "steps": steps}
     for name, mat in media.items():
         sc, _ = scene_400(n=n)
@@ -648,8 +653,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"plain single-domain dielectric sphere {nx}x{ny}x{nz}, dx=2.5mm, "
-                        "12-cell CPML, eps_r=4 sphere r=100 cells, TFSF Gaussian plane wave, "
-                        "NF-FF 3 freqs",
+                        f"12-cell CPML, eps_r=4 sphere r={sphere_radius(args.size)} cells, TFSF Gaussian plane "
+                        "wave, NF-FF 3 freqs",
             "grid": [nx, ny, nz],
             "precision": args.precision,
             "total_steps_of_run": total,
