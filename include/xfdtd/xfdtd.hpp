@@ -784,6 +784,12 @@ class Simulation {
   // [box[2],box[3]) x [box[4],box[5]) (grid indices), lossless, mu_r = 1,
   // applied after the objects as one-cell Objects would be; eps_r in C order.
   void setDesign(const std::array<size_t, 6> &box, std::vector<float> eps_r);
+  // after init: new eps_r for the same design box (the engine keeps its
+  // fields; restore a step-0 checkpoint before the next gradient run)
+  void updateDesign(std::vector<float> eps_r);
+  size_t designCells() const {
+    return (_design_box[1] - _design_box[0]) * (_design_box[3] - _design_box[2]) * (_design_box[5] - _design_box[4]);
+  }
   // running DFT of one component on a plane (axis fixed at `index`, the
   // other two axes in increasing order over [lo1,hi1) x [lo2,hi2)), sample
   // after step t weighted by dt exp(-i 2 pi f t dt); J = sum |F|^2
@@ -899,6 +905,7 @@ class Simulation {
   bool _has_design = false;
   std::array<size_t, 6> _design_box{};
   std::vector<float> _design_eps;
+  void designCoefficients(double dt, std::vector<double> &coef, std::vector<double> &eps) const;
   bool _has_dft = false;
   int _dft_comp = 0, _dft_axis = 0;
   size_t _dft_plane[5] = {};  // index, lo1, hi1, lo2, hi2

@@ -345,6 +345,11 @@ xf_status xf_engine_set_drude_j(xf_engine *e, int comp, const double *in);
    fdtd_basic_coff.cpp:77-91, for a lossless cell); eps_r: the cells'
    relative permittivity (for the gradient).  Cells in C order. */
 xf_status xf_engine_set_design(xf_engine *e, const int64_t *box, const double *coef, const double *eps_r);
+/* New coefficients and permittivity for the cells of the design box already
+   set (same layout as xf_engine_set_design), for an optimisation loop that
+   keeps one engine; the fields are left as they are (rewind with
+   xf_engine_restore to a checkpoint taken at step 0). */
+xf_status xf_engine_update_design(xf_engine *e, const double *coef, const double *eps_r);
 /* DFT monitor of component comp on the plane axis = index, the other two
    axes (increasing order) over [lo1,hi1) x [lo2,hi2): after step t,
    F += f * w[t] in FP64.  w: interleaved complex [total_steps]. */

@@ -178,6 +178,9 @@ struct xf_engine *xf_sim_engine(xf_sim *s);
    Before xf_sim_init: a design box of per-cell eps_r (lossless, applied after
    the objects) and a DFT monitor plane whose sum |F|^2 is the objective. */
 int xf_sim_set_design(xf_sim *s, const size_t box[6], const float *eps_r);
+/* after xf_sim_init: new eps_r for the same design box, coefficients
+   recomputed on the host and uploaded (xf_engine_update_design) */
+int xf_sim_update_design(xf_sim *s, const float *eps_r);
 int xf_sim_set_dft_plane(xf_sim *s, int comp, int axis, size_t index, size_t lo1, size_t hi1, size_t lo2,
                          size_t hi2, double freq);
 int xf_sim_get_dft(xf_sim *s, double *out);

@@ -9,6 +9,9 @@
 - dJ/d eps_r matches the oracle's to 1e-10 (FP64) for any checkpoint
   interval, and central differences on the GPU forward run;
 - the FP32 gradient stays within 1e-4 of the FP64 one;
+- the torch wrapper returns the engine's gradient, and a DesignProblem that
+  keeps one engine across calls (new permittivity, rewind to step 0) gives
+  the bits of a fresh engine for every new permittivity;
 - all of the above with Drude cells, whose currents are adjoint state too.
 """
 import numpy as np
@@ -160,6 +163,35 @@ def test_torch_autograd_function():
     with torch.no_grad():
         J2 = design_objective(eps, sc, steps)
     assert float(J2) == pytest.approx(Jr, rel=1e-12)
+
+
+def test_torch_design_problem_reuses_one_engine():
+    import copy
+
+    import torch
+    from xfdtd_amd.torch_adjoint import DesignProblem
+    sc, steps = design_scene()
+    b = sc.design.box
+    shape = (b[1] - b[0], b[3] - b[2], b[5] - b[4])
+    base = np.asarray(sc.design.eps_r, dtype=np.float32).reshape(shape)
+    prob = DesignProblem(sc, steps, checkpoint_every=50)
+    rng = np.random.default_rng(17)
+    try:
+        for trial in range(3):
+            e = base if trial != 1 else (base + rng.uniform(-0.3, 0.3, shape)).astype(np.float32)
+            print(f"DesignProblem call {trial}: eps_r in [{e.min():.3f}, {e.max():.3f}]", flush=True)
+            eps = torch.tensor(e, dtype=torch.float64, requires_grad=True)
+            J = prob(eps)
+            J.backward()
+            sc2 = copy.copy(sc)
+            sc2.design = S.Design(b, e.ravel())
+            fresh = _sim(sc2, steps)
+            Jr, gr = fresh.grad(checkpoint_every=50)
+            fresh.close()
+            assert float(J.detach()) == Jr
+            assert np.array_equal(eps.grad.numpy(), gr)
+    finally:
+        prob.close()
 
 
 def test_gpu_transpose_identity_with_drude(oracle_threads):

@@ -25,7 +25,7 @@ ENGINE_SYMBOLS = [
     "xf_engine_sweep_active", "xf_engine_set_divergence_check", "xf_engine_diverged_step",
     "xf_engine_set_adjoint_overlap",
     "xf_batch_create", "xf_batch_step", "xf_batch_sync", "xf_batch_size", "xf_batch_destroy", "xf_batch_grad_eps",
-    "xf_engine_set_design", "xf_engine_set_dft_plane", "xf_engine_get_dft", "xf_engine_checkpoint",
+    "xf_engine_set_design", "xf_engine_update_design", "xf_engine_set_dft_plane", "xf_engine_get_dft", "xf_engine_checkpoint",
     "xf_engine_restore", "xf_engine_grad_eps", "xf_engine_adjoint_run", "xf_engine_set_drude",
     "xf_engine_drude_box", "xf_engine_get_drude_j", "xf_engine_set_horizon", "xf_engine_set_drude_j",
     "xf_engine_get_box", "xf_engine_set_nffft_broadband", "xf_engine_broadband_samples_size",
@@ -39,7 +39,7 @@ SIM_SYMBOLS = [
     "xf_sim_add_probe", "xf_sim_plan", "xf_sim_check_run", "xf_sim_plan_coeff", "xf_sim_init", "xf_sim_step", "xf_sim_sync", "xf_sim_current_step",
     "xf_sim_dims", "xf_sim_dt", "xf_sim_get_field", "xf_sim_set_field", "xf_sim_nffft_size",
     "xf_sim_get_nffft", "xf_sim_farfield", "xf_sim_output_nffft", "xf_sim_get_probes",
-    "xf_sim_engine", "xf_sim_set_design", "xf_sim_set_dft_plane", "xf_sim_get_dft", "xf_sim_grad_eps",
+    "xf_sim_engine", "xf_sim_set_design", "xf_sim_update_design", "xf_sim_set_dft_plane", "xf_sim_get_dft", "xf_sim_grad_eps",
     "xf_sim_adjoint_run", "xf_sim_checkpoint", "xf_sim_restore", "xf_sim_add_drude_cube",
     "xf_sim_add_drude_sphere", "xf_sim_drude_box", "xf_sim_get_drude_j", "xf_sim_set_drude_j",
     "xf_sim_add_field_monitor", "xf_sim_add_movie_monitor", "xf_sim_set_nffft_broadband",
@@ -123,6 +123,7 @@ def lib():
     L.xf_batch_destroy.argtypes = [vp]
     L.xf_batch_grad_eps.argtypes = [vp, i, C.POINTER(dp), dp]
     L.xf_sim_set_design.argtypes = [vp, szp, C.POINTER(C.c_float)]
+    L.xf_sim_update_design.argtypes = [vp, C.POINTER(C.c_float)]
     L.xf_sim_set_dft_plane.argtypes = [vp, i, i, sz, sz, sz, sz, sz, d]
     L.xf_sim_get_dft.argtypes = [vp, dp]
     L.xf_sim_grad_eps.argtypes = [vp, i, dp, dp]

@@ -2016,6 +2016,22 @@ extern "C" xf_status xf_engine_set_design(xf_engine *e, const int64_t *box, cons
   return XF_OK;
 }
 
+extern "C" xf_status xf_engine_update_design(xf_engine *e, const double *coef, const double *eps_r) {
+  if (!e || !coef || !eps_r) FAILS(XF_EINVAL, "null argument");
+  if (!e->has_design) FAILS(XF_ESTATE, "no design box");
+  const size_t nd = (size_t)e->nd;
+  HIPCHK(engine_wait(e));
+  if (e->prec == 64) {
+    HIPCHK(hipMemcpy(e->dcoef, coef, 9 * nd * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> tmp(9 * nd);
+    for (size_t q = 0; q < 9 * nd; ++q) tmp[q] = (float)coef[q];
+    HIPCHK(hipMemcpy(e->dcoef, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(e->d_eps, eps_r, nd * sizeof(double), hipMemcpyHostToDevice));
+  return XF_OK;
+}
+
 extern "C" xf_status xf_engine_set_drude(xf_engine *e, const int64_t *box, const double *coef) {
   if (!e || !box || !coef) FAILS(XF_EINVAL, "null argument");
   // the Drude records of the dispersive layout

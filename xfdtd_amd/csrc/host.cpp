@@ -993,28 +993,7 @@ void Simulation::plan(size_t total_time_steps) {
         if (in_slab_e(a, (int)n) || n == 0 || n >= (size_t)N[a])
           throw std::runtime_error("design box must lie inside the grid and outside the CPML slabs");
     }
-    const size_t nd = (_design_box[1] - _design_box[0]) * (_design_box[3] - _design_box[2]) *
-                      (_design_box[5] - _design_box[4]);
-    if (_design_eps.size() != nd) throw std::invalid_argument("design eps_r has the wrong size");
-    D.coef.resize(9 * nd);
-    D.eps_r.resize(nd);
-    size_t q = 0;
-    for (size_t i = _design_box[0]; i < _design_box[1]; ++i)
-      for (size_t j = _design_box[2]; j < _design_box[3]; ++j)
-        for (size_t k = _design_box[4]; k < _design_box[5]; ++k, ++q) {
-          const Mat m{static_cast<double>(_design_eps[q]) * constant::EPSILON_0, constant::MU_0, 0.0, 0.0};
-          double *c = &D.coef[9 * q];
-          c[0] = coffSelfE(m, dt);
-          c[1] = coffSelfE(m, dt);
-          c[2] = coffSelfE(m, dt);
-          c[3] = coffEa(m, dt, _hs[2][k]);  // cexhy
-          c[4] = coffEb(m, dt, _hs[2][k]);  // ceyhx
-          c[5] = coffEb(m, dt, _hs[1][j]);  // cexhz
-          c[6] = coffEa(m, dt, _hs[1][j]);  // cezhx
-          c[7] = coffEa(m, dt, _hs[0][i]);  // ceyhz
-          c[8] = coffEb(m, dt, _hs[0][i]);  // cezhy
-          D.eps_r[q] = static_cast<double>(_design_eps[q]);
-        }
+    designCoefficients(dt, D.coef, D.eps_r);
     D.present = true;
   }
   // ---- dispersive coefficients, XF_ADE_STRIDE per slot: DrudeMedium::init
@@ -1308,6 +1287,45 @@ void Simulation::upload() {
     check(xf_engine_set_dft_plane(_engine, P.dft.comp, P.dft.axis, P.dft.index, P.dft.lo1, P.dft.hi1, P.dft.lo2,
                                   P.dft.hi2, reinterpret_cast<const double *>(P.dft.w.data()),
                                   (int64_t)P.dft.w.size()));
+}
+
+// fdtd_basic_coff.cpp:77-91 for each design cell's own permittivity (eps_r
+// rounded to float as a Material's), lossless: [cell][9] E coefficients in
+// the order of xf_engine_set_design, and the cells' eps_r
+void Simulation::designCoefficients(double dt, std::vector<double> &coef, std::vector<double> &eps) const {
+  const size_t nd = (_design_box[1] - _design_box[0]) * (_design_box[3] - _design_box[2]) *
+                    (_design_box[5] - _design_box[4]);
+  if (_design_eps.size() != nd) throw std::invalid_argument("design eps_r has the wrong size");
+  coef.resize(9 * nd);
+  eps.resize(nd);
+  size_t q = 0;
+  for (size_t i = _design_box[0]; i < _design_box[1]; ++i)
+    for (size_t j = _design_box[2]; j < _design_box[3]; ++j)
+      for (size_t k = _design_box[4]; k < _design_box[5]; ++k, ++q) {
+        const Mat m{static_cast<double>(_design_eps[q]) * constant::EPSILON_0, constant::MU_0, 0.0, 0.0};
+        double *c = &coef[9 * q];
+        c[0] = coffSelfE(m, dt);
+        c[1] = coffSelfE(m, dt);
+        c[2] = coffSelfE(m, dt);
+        c[3] = coffEa(m, dt, _hs[2][k]);  // cexhy
+        c[4] = coffEb(m, dt, _hs[2][k]);  // ceyhx
+        c[5] = coffEb(m, dt, _hs[1][j]);  // cexhz
+        c[6] = coffEa(m, dt, _hs[1][j]);  // cezhx
+        c[7] = coffEa(m, dt, _hs[0][i]);  // ceyhz
+        c[8] = coffEb(m, dt, _hs[0][i]);  // cezhy
+        eps[q] = static_cast<double>(_design_eps[q]);
+      }
+}
+
+void Simulation::updateDesign(std::vector<float> eps_r) {
+  if (!_engine || !_has_design) throw std::runtime_error("no initialised simulation with a design box");
+  const size_t nd = (_design_box[1] - _design_box[0]) * (_design_box[3] - _design_box[2]) *
+                    (_design_box[5] - _design_box[4]);
+  if (eps_r.size() != nd) throw std::invalid_argument("design eps_r has the wrong size");
+  _design_eps = std::move(eps_r);
+  std::vector<double> coef, eps;
+  designCoefficients(getDt(), coef, eps);
+  check(xf_engine_update_design(_engine, coef.data(), eps.data()));
 }
 
 void Simulation::setDesign(const std::array<size_t, 6> &box, std::vector<float> eps_r) {

@@ -488,6 +488,13 @@ extern "C" int xf_sim_set_design(xf_sim *s, const size_t box[6], const float *ep
   });
 }
 
+extern "C" int xf_sim_update_design(xf_sim *s, const float *eps_r) {
+  return guarded([&] {
+    const size_t nd = s->sim.designCells();
+    s->sim.updateDesign(std::vector<float>(eps_r, eps_r + nd));
+  });
+}
+
 extern "C" int xf_sim_set_dft_plane(xf_sim *s, int comp, int axis, size_t index, size_t lo1, size_t hi1,
                                     size_t lo2, size_t hi2, double freq) {
   return guarded([&] {

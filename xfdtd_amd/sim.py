@@ -338,6 +338,16 @@ class Simulation:
         self._chk(lib().xf_sim_grad_eps(self.h, int(checkpoint_every), _dp(g), C.byref(J)))
         return J.value, g
 
+    def update_design(self, eps_r):
+        """New eps_r (the design box's shape, any float array) for the design
+        cells of an initialised simulation (xf_sim_update_design); the fields
+        are left as they are."""
+        b = self.scene.design.box
+        e = np.ascontiguousarray(eps_r, dtype=np.float32)
+        if e.size != (b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]):
+            raise ValueError(f"eps_r has {e.size} values for a design box of {(b[1] - b[0], b[3] - b[2], b[5] - b[4])}")
+        self._chk(lib().xf_sim_update_design(self.h, e.ctypes.data_as(C.POINTER(C.c_float))))
+
     def adjoint_run(self, nsteps, fields):
         """Transpose of nsteps steps of the field map on the adjoint state: six
         fields, then (with Drude cells) the three J adjoints in E shapes."""
