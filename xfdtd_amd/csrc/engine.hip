@@ -73,6 +73,7 @@ struct xf_engine {
   uint8_t *mat = nullptr;
   // coefficient records per side (0: E, 1: H), see Grid in engine_kernels.hpp
   void *self[2] = {}, *kt[2] = {}, *jt[2] = {}, *it[2] = {}, *kp[2] = {}, *jp[2] = {}, *ip[2] = {};
+  bool h_uni = false;  // every material has material 0's H coefficients
   bool materials_set = false;
   int dn[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
   void *be[3] = {}, *ae[3] = {}, *bm[3] = {}, *am[3] = {};
@@ -436,6 +437,22 @@ extern "C" xf_status xf_engine_set_materials(xf_engine *e, const uint8_t *index,
   HIPCHK(hipMemcpy(pm, m.data(), (size_t)e->ncell, hipMemcpyHostToDevice));
   e->mat = (uint8_t *)pm;
   const int nax[3] = {nx, ny, nz};
+  // do all materials share material 0's H coefficients (self terms, curl and
+  // psi tables)?  Then the H kernels need no material byte.
+  {
+    static const int htab[6] = {XF_CB_HXEY, XF_CB_HYEX, XF_CB_HXEZ, XF_CB_HZEX, XF_CB_HYEZ, XF_CB_HZEY};
+    static const int hax[6] = {2, 2, 1, 1, 0, 0};
+    bool uni = true;
+    for (int m = 1; m < nm && uni; ++m) {
+      for (int c = 0; c < 3; ++c) uni = uni && ca[(3 + c) * (size_t)nm + m] == ca[(3 + c) * (size_t)nm];
+      for (int t = 0; t < 6 && uni; ++t) {
+        const size_t n1 = (size_t)nax[hax[t]] + 1;
+        for (size_t n = 0; n < n1 && uni; ++n)
+          uni = cb[htab[t]][m * n1 + n] == cb[htab[t]][n] && cpsi[htab[t]][m * n1 + n] == cpsi[htab[t]][n];
+      }
+    }
+    e->h_uni = uni;
+  }
   // self terms of each side as 4-wide records [nmat][4]
   for (int side = 0; side < 2; ++side) {
     std::vector<double> rec(4 * (size_t)nm, 0.0);
@@ -1010,6 +1027,7 @@ static Grid<T> make_grid(xf_engine *e) {
     g.psi_h[q] = (T *)e->psi_h[q];
   }
   g.nx = e->nx; g.ny = e->ny; g.nz = e->nz; g.pz = e->pz; g.nmat = e->nmat;
+  g.h_uni = e->h_uni ? 1 : 0;
   g.sj = e->sj; g.si = e->si;
   for (int a = 0; a < 6; ++a) g.db[a] = e->db[a];
   g.dcoef = (const T *)e->dcoef;
@@ -1019,8 +1037,8 @@ static Grid<T> make_grid(xf_engine *e) {
     g.dj[q] = (T *)e->dj[q];
     g.djp[q] = (T *)e->djp[q];
     g.dep[q] = (T *)e->dep[q];
-  }  g.div = (unsigned long long *)e->div;
-
+  }
+  g.div = (unsigned long long *)e->div;
   return g;
 }
 

@@ -41,6 +41,9 @@ struct Grid {
   const T *be[3], *ae[3], *bm[3], *am[3];
   T *psi_e[6], *psi_h[6];
   int nx, ny, nz, pz, nmat;
+  // 1: every material has material 0's H coefficients (no magnetic media),
+  // so the H kernels use material 0 and read no material byte
+  int h_uni;
   long long sj, si;  // strides of j and i in elements
   // design box [db0,db1) x [db2,db3) x [db4,db5) with per-cell E coefficients
   // dcoef[cell][9] = cexe, ceye, ceze, cexhy, ceyhx, cexhz, cezhx, ceyhz,
@@ -766,7 +769,8 @@ template <typename T, bool SLAB>
 __device__ __forceinline__ void load_plane_h(const Grid<T> &g, const Col<T> &q, unsigned cell, int i,
                                              unsigned di, Plane<T> &p, const ZFaces &zf, bool zplane) {
   const unsigned o = cell * (unsigned)sizeof(T);
-  p.m = ldm(q.mat, cell);
+  p.m = 0;
+  if (!g.h_uni) p.m = ldm(q.mat, cell);
   p.f0 = ldo(q.u0, o);
   p.f1 = ldo(q.u1, o);
   p.f2 = ldo(q.u2, o);
