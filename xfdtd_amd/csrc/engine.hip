@@ -94,6 +94,7 @@ struct xf_engine {
   // face values of up to NF_RING steps not yet added to the accumulators
   // (empty between API calls) and the step of the first
   void *nf_ring = nullptr;
+  void *nf_zrec = nullptr;  // NF-FF z-face records (NfRec)
   int nf_pending = 0;
   long long nf_step0 = 0;
   // broadband NF-FF: the face values of every step, [step][8][surface cell]
@@ -1448,6 +1449,30 @@ static xf_status run_steps(xf_engine *e, long long n) {
       if (s != XF_OK) return s;
     }
   }
+  // The NF-FF z faces are sampled from records the H kernel leaves (split
+  // scheme, 3-D surface, tiles of whole 64-lane k runs with each face plane
+  // and the plane below it in one wave), unless the TFSF face kernel, which
+  // runs after the H kernel, corrects H on one of the record planes (k in
+  // [tlk - 1, trk]).
+  Nffft nfs = e->nf;
+  nfs.zrec = nullptr;
+  NfRec nr{};
+  if (e->has_nf && e->nf.zf && !fused && !e->replay && e->nf.rk - 1 > e->nf.lk && tl.ty > 0 && tl.tz % 64 == 0 &&
+      e->nf.lk % 64 != 0 && e->nf.rk % 64 != 0) {
+    auto clear = [&](int k) { return !e->has_tfsf || k < e->tlk - 1 || k > e->trk; };
+    const Nffft &n = e->nf;
+    if (clear(n.lk - 1) && clear(n.lk) && clear(n.rk - 1) && clear(n.rk)) {
+      if (!e->nf_zrec) {
+        xf_status s = dev_alloc(e, &e->nf_zrec, (size_t)2 * (n.nxb + 1) * (n.nyb + 1) * 8 * e->esz);
+        if (s != XF_OK) return s;
+      }
+      nfs.zrec = e->nf_zrec;
+      nr.rec = e->nf_zrec;
+      nr.li = n.li; nr.lj = n.lj;
+      nr.nxr = n.nxb + 1; nr.nyr = n.nyb + 1;
+      nr.k0 = n.lk; nr.k1 = n.rk;
+    }
+  }
   auto nf_flush = [&]() {
     if (e->nf_pending == 0) return;
     hipLaunchKernelGGL(k_nffft_flush<T>, dim3((unsigned)((8 * nfQ + 255) / 256)), dim3(256), 0, e->stream, e->nf,
@@ -1540,9 +1565,9 @@ static xf_status run_steps(xf_engine *e, long long n) {
       }
       if (ev) hipEventRecord(ev[2], e->stream);
       if (use_lds)
-        hipLaunchKernelGGL((k_update_h<T, true>), grd, blk, lds, e->stream, g, tl, zf, step);
+        hipLaunchKernelGGL((k_update_h<T, true>), grd, blk, lds, e->stream, g, tl, zf, nr, step);
       else
-        hipLaunchKernelGGL((k_update_h<T, false>), grd, blk, 0, e->stream, g, tl, zf, step);
+        hipLaunchKernelGGL((k_update_h<T, false>), grd, blk, 0, e->stream, g, tl, zf, nr, step);
       if (ev) hipEventRecord(ev[3], e->stream);
       if (e->has_tfsf && hs.off[12] > 0)
         hipLaunchKernelGGL(k_tfsf_h<T>, dim3((hs.off[12] + 255) / 256), dim3(256), 0, e->stream, g, t, hs);
@@ -1556,7 +1581,7 @@ static xf_status run_steps(xf_engine *e, long long n) {
     }
     if (e->has_nf) {
       if (e->nf_pending == 0) e->nf_step0 = step;
-      hipLaunchKernelGGL(k_nffft_sample<T>, dim3((unsigned)((nfQ + 255) / 256)), dim3(256), 0, e->stream, g, e->nf,
+      hipLaunchKernelGGL(k_nffft_sample<T>, dim3((unsigned)((nfQ + 255) / 256)), dim3(256), 0, e->stream, g, nfs,
                          (T *)e->nf_ring, e->nf_pending);
       if (++e->nf_pending == NF_RING) nf_flush();
     }

@@ -475,6 +475,59 @@ struct ZFaces {
   const double *terms;  // null: the face kernels apply every correction
   int li, ri, lj, rj, lk, rk;
 };
+
+// DPP lane moves over the whole wave: every lane takes `src` of its
+// neighbour, the end lane that has none takes `fill`.  One v_mov_dpp per
+// 32 bits with `fill` as the old value, so no lane is masked off while the
+// move reads: a move under a partial EXEC mask would read the masked lanes as
+// out of range.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float src, float fill) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(src), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double src, double fill) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(src);
+  const unsigned long long f = (unsigned long long)__double_as_longlong(fill);
+  const int l2 = __builtin_amdgcn_update_dpp((int)(unsigned)(f & 0xffffffffull), (int)(unsigned)(b & 0xffffffffull),
+                                             CTRL, 0xF, 0xF, false);
+  const int h2 = __builtin_amdgcn_update_dpp((int)(unsigned)(f >> 32), (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)h2 << 32) | (unsigned)l2));
+}
+// the value held by the lane below (k - 1; lane 0 takes `fill`) and above
+// (k + 1; lane 63 takes `fill`)
+template <typename T>
+__device__ __forceinline__ T from_below(T v, T fill) { return dpp_move<0x138>(v, fill); }
+template <typename T>
+__device__ __forceinline__ T from_above(T v, T fill) { return dpp_move<0x130>(v, fill); }
+
+// NF-FF z faces (NffftFd surface planes k = k0 and k1): the H kernel leaves
+// the field values their stencils read in 8-value records
+// rec[face][i - li][j - lj] (i < li + nxr, j < lj + nyr), so the sample
+// kernel reads no z-face cache line of the fields (rec null: it reads the
+// fields).  Record: Ey, Ex, Ex(k+1), Hy at kf, then Hx at kf, 0, Hy and Hx
+// at kf - 1.  The threads of (i, j, kf) and (i, j, kf - 1), adjacent lanes of
+// one wave, store the two halves with one four-wide store.
+struct NfRec {
+  void *rec;
+  int li, lj, nxr, nyr, k0, k1;
+};
+template <typename T>
+__device__ __forceinline__ void nf_record(const NfRec &nr, int i, int j, int k, T ey, T ex, T ex1, T hx, T hy) {
+  const T hx_up = from_above(hx, T(0));  // lane kf - 1 takes Hx(kf)
+  const int face = k >= nr.k1 - 1 ? 1 : 0;
+  const int kf = face ? nr.k1 : nr.k0;
+  if ((k != kf && k != kf - 1) || j < nr.lj || j >= nr.lj + nr.nyr) return;
+  const bool top = k == kf;
+  T *rec = (T *)nr.rec + ((face * nr.nxr + (i - nr.li)) * nr.nyr + (j - nr.lj)) * 8 + (top ? 0 : 4);
+  P4<T> v;
+  v.x = top ? ey : hx_up;
+  v.y = top ? ex : T(0);
+  v.z = top ? ex1 : hy;
+  v.w = top ? hy : hx;
+  *reinterpret_cast<P4<T> *>(rec) = v;
+}
+
 __device__ __forceinline__ int zseg_size(const ZFaces &z) { return (z.ri - z.li + 1) * (z.rj - z.lj + 1); }
 __device__ __forceinline__ int zterm_index(const ZFaces &z, int seg, int i, int j) {
   return seg * zseg_size(z) + (i - z.li) * (z.rj - z.lj + 1) + (j - z.lj);
@@ -816,7 +869,7 @@ __device__ __forceinline__ void load_plane_h(const Grid<T> &g, const Col<T> &q, 
 template <typename T, bool SLAB>
 __device__ __forceinline__ bool compute_plane_h(const Grid<T> &g, const TileTables<T> &tt, const Col<T> &q,
                                                 const Plane<T> &p, int i, unsigned cell, T ez_c, T ey_c,
-                                                const ZFaces &zf, bool zt) {
+                                                const ZFaces &zf, bool zt, const NfRec &nr, bool nfz) {
   const int nx = g.nx, j = q.j, k = q.k;
   const bool kin = q.kin, jin = q.jin;
   const int m = (int)p.m;
@@ -898,13 +951,14 @@ __device__ __forceinline__ bool compute_plane_h(const Grid<T> &g, const TileTabl
   sto(q.u0, o, vx);
   sto(q.u1, o, vy);
   sto(q.u2, o, vz);
+  if (nfz) nf_record<T>(nr, i, j, k, ey_c, ex, p.n1, vx, vy);
   // non-finite output: the run diverged
   return !(__builtin_isfinite(vx) && __builtin_isfinite(vy) && __builtin_isfinite(vz));
 }
 
 template <typename T, bool LDS>
-__device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl, const ZFaces &zf, int block,
-                                              long long step) {
+__device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl, const ZFaces &zf, const NfRec &nr,
+                                              int block, long long step) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   int bz, by, bx;
   if (!tile_of_block(tl, block, bz, by, bx)) return;
@@ -934,6 +988,9 @@ __device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl
   const bool zwave =
       zf.terms && __ballot(live && (k == zf.lk - 1 || k == zf.rk) && j >= zf.lj && j <= zf.rj) != 0ull;
   const bool col_slab = zwave || __ballot(q.cp.ly >= 0 || q.cp.lz >= 0) != 0ull;
+  // a wave that holds cells of the NF-FF z-face records
+  const bool nfwave = nr.rec && __ballot(live && (k == nr.k0 - 1 || k == nr.k0 || k == nr.k1 - 1 || k == nr.k1) &&
+                                         j >= nr.lj && j < nr.lj + nr.nyr) != 0ull;
   unsigned c = (unsigned)((long long)j * g.sj + k);
   T ez_c = T(0), ey_c = T(0);
   Plane<T> A;
@@ -959,10 +1016,12 @@ __device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl
   if (!live) return;
   bool bad = false;
   for (int i = i0;;) {
+    const bool nfz = nfwave && i >= nr.li && i < nr.li + nr.nxr;
     if (slab)
-      bad |= compute_plane_h<T, true>(g, tt, q, A, i, c, ez_c, ey_c, zf, zwave && i >= zf.li && i <= zf.ri);
+      bad |= compute_plane_h<T, true>(g, tt, q, A, i, c, ez_c, ey_c, zf, zwave && i >= zf.li && i <= zf.ri, nr,
+                                      nfz);
     else
-      bad |= compute_plane_h<T, false>(g, tt, q, A, i, c, ez_c, ey_c, zf, false);
+      bad |= compute_plane_h<T, false>(g, tt, q, A, i, c, ez_c, ey_c, zf, false, nr, nfz);
     ez_c = A.n4;
     ey_c = A.n5;
     if (++i >= i1) break;
@@ -982,8 +1041,8 @@ __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, Tiling tl, ZFaces z
   update_e_body<T, LDS, SPECIAL>(g, tl, zf, blockIdx.x);
 }
 template <typename T, bool LDS>
-__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl, ZFaces zf, long long step) {
-  update_h_body<T, LDS>(g, tl, zf, blockIdx.x, step);
+__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, Tiling tl, ZFaces zf, NfRec nr, long long step) {
+  update_h_body<T, LDS>(g, tl, zf, nr, blockIdx.x, step);
 }
 
 // TFSF3D::updateIncidentField (tfsf_3d.cpp:153-183): one workgroup
@@ -1214,6 +1273,8 @@ struct Nffft {
   const double2 *tj, *tm;  // [nf][T]
   double2 *acc;
   long long base[3];       // accumulator offset of each face pair
+  // z-face records left by the H kernel (NfRec) or null
+  const void *zrec;
 };
 
 // surface cells: x faces, y faces and (when kept) z faces
@@ -1255,7 +1316,24 @@ __device__ __forceinline__ bool nffft_values(const Grid<T> &g, const Nffft &n, i
     v[5] = T(-1) * (T(0.5) * (Ez[C3(i, rj, k)] + Ez[C3(i + 1, rj, k)]));
     v[6] = T(-1) * (T(0.25) * (Hx[C3(i + 1, rj, k)] + Hx[C3(i, rj, k)] + Hx[C3(i + 1, rj - 1, k)] + Hx[C3(i, rj - 1, k)]));
     v[7] = T(0.25) * (Hz[C3(i, rj, k)] + Hz[C3(i, rj, k + 1)] + Hz[C3(i, rj - 1, k)] + Hz[C3(i, rj - 1, k + 1)]);
-  } else if ((q -= cy) < cz) {
+  } else if ((q -= cy) < cz && n.zrec) {
+    // the same sums over the records' copies of the fields
+    pair = 2; cells = cz; cell = q;
+    // record: [0] Ey, [1] Ex, [2] Ex(k+1), [3] Hy, [4] Hx at lk (rk); [6] Hy,
+    // [7] Hx at lk - 1 (rk - 1)
+    const int a = q / n.nyb, b = q % n.nyb, nyr = n.nyb + 1;
+    const T *lo = (const T *)n.zrec + ((long long)a * nyr + b) * 8;
+    const T *hi = lo + (long long)(n.nxb + 1) * nyr * 8;
+    const long long di = (long long)nyr * 8, dj = 8;
+    v[0] = T(-1) * (T(0.5) * (lo[0] + lo[di]));
+    v[1] = T(0.5) * (lo[1] + lo[2]);
+    v[2] = T(0.25) * (lo[3] + lo[dj + 3] + lo[6] + lo[dj + 6]);
+    v[3] = T(-1) * (T(0.25) * (lo[di + 4] + lo[4] + lo[di + 7] + lo[7]));
+    v[4] = T(0.5) * (hi[0] + hi[di]);
+    v[5] = T(-1) * (T(0.5) * (hi[1] + hi[2]));
+    v[6] = T(-1) * (T(0.25) * (hi[3] + hi[dj + 3] + hi[6] + hi[dj + 6]));
+    v[7] = T(0.25) * (hi[di + 4] + hi[4] + hi[di + 7] + hi[7]);
+  } else if (q < cz) {
     pair = 2; cells = cz; cell = q;
     const int i = n.li + q / n.nyb, j = n.lj + q % n.nyb;
     const int lk = n.lk, rk = n.rk;
@@ -1943,7 +2021,7 @@ template <typename T, bool LDS>
 __global__ void __launch_bounds__(256) kb_update_h(const Inst<T> *__restrict__ in, const int2 *__restrict__ map,
                                                    long long step) {
   const int2 w = map[blockIdx.x];
-  update_h_body<T, LDS>(in[w.x].g, in[w.x].tl, ZFaces{}, w.y, step);
+  update_h_body<T, LDS>(in[w.x].g, in[w.x].tl, ZFaces{}, NfRec{}, w.y, step);
 }
 template <typename T>
 __global__ void kb_tfsf_e(const Inst<T> *__restrict__ in, const int2 *__restrict__ map) {

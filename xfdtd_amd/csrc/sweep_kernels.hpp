@@ -128,31 +128,6 @@ struct SweepArgs {
   long long step;
 };
 
-// DPP lane moves over the whole wave: every lane takes `src` of its
-// neighbour, the end lane that has none takes `fill`.  One v_mov_dpp per
-// 32 bits with `fill` as the old value, so no lane is masked off while the
-// move reads: a move under a partial EXEC mask would read the masked lanes as
-// out of range.
-template <int CTRL>
-__device__ __forceinline__ float dpp_move(float src, float fill) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(src), CTRL, 0xF, 0xF, false));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_move(double src, double fill) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(src);
-  const unsigned long long f = (unsigned long long)__double_as_longlong(fill);
-  const int l2 = __builtin_amdgcn_update_dpp((int)(unsigned)(f & 0xffffffffull), (int)(unsigned)(b & 0xffffffffull),
-                                             CTRL, 0xF, 0xF, false);
-  const int h2 = __builtin_amdgcn_update_dpp((int)(unsigned)(f >> 32), (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)h2 << 32) | (unsigned)l2));
-}
-// the value held by the lane below (k - 1; lane 0 takes `fill`) and above
-// (k + 1; lane 63 takes `fill`)
-template <typename T>
-__device__ __forceinline__ T from_below(T v, T fill) { return dpp_move<0x138>(v, fill); }
-template <typename T>
-__device__ __forceinline__ T from_above(T v, T fill) { return dpp_move<0x130>(v, fill); }
-
 // CPML layer of index n on one axis and side (-1 outside the slabs)
 __device__ __forceinline__ int cpml_layer(bool e_side, int n, int N, int dn, int dp) {
   return e_side ? e_layer(n, N, dn, dp) : h_layer(n, N, dn, dp);
