@@ -183,9 +183,9 @@ struct xf_engine {
   void *f_alt[6] = {};
   void *psi_e_alt[6] = {}, *psi_h_alt[6] = {};
   // for the fused step both psi sets live in one block each, slots at
-  // psi_off (elements): E slots P_AY..P_CY, then H
+  // psi_n: the elements of one slot (a pair takes two)
   void *psi_blk = nullptr, *psi_blk_alt = nullptr;
-  long long psi_off[12] = {}, psi_n = 0;
+  long long psi_n = 0;
   void *e_inc_alt = nullptr, *h_inc_alt = nullptr;
   TfTerms tf_terms{};              // the step's TFSF face terms (fused step)
   // the TFSF z-face terms the curl kernels add (ZFaces), 8 FP64 segments
@@ -526,14 +526,17 @@ extern "C" xf_status xf_engine_set_cpml(xf_engine *e, int axis, int d_n, int d_p
   if (axis == 0) n = (size_t)Lay * ny1 * e->pz;       // [L][ny+1][pz]
   else if (axis == 1) n = nx1 * Lay * e->pz;          // [nx+1][L][pz]
   else n = nx1 * ny1 * Lay;                           // [nx+1][ny+1][L]
+  // the axis' two arrays interleaved in one allocation per side (slot s0 at
+  // the even elements, s1 at the odd), so a kernel reaches both with one
+  // two-wide access
   static const int e_slots[3][2] = {{P_BX, P_CX}, {P_AY, P_CY}, {P_AZ, P_BZ}};
-  for (int q = 0; q < 2; ++q) {
-    const int sl = e_slots[axis][q];
-    if ((s = dev_alloc(e, &e->psi_e[sl], n * e->esz)) != XF_OK) return s;
-    if ((s = dev_alloc(e, &e->psi_h[sl], n * e->esz)) != XF_OK) return s;
-    e->state.push_back({nullptr, &e->psi_e[sl], n * e->esz});
-    e->state.push_back({nullptr, &e->psi_h[sl], n * e->esz});
-  }
+  const int s0 = e_slots[axis][0], s1 = e_slots[axis][1];
+  if ((s = dev_alloc(e, &e->psi_e[s0], 2 * n * e->esz)) != XF_OK) return s;
+  if ((s = dev_alloc(e, &e->psi_h[s0], 2 * n * e->esz)) != XF_OK) return s;
+  e->psi_e[s1] = (char *)e->psi_e[s0] + e->esz;
+  e->psi_h[s1] = (char *)e->psi_h[s0] + e->esz;
+  e->state.push_back({nullptr, &e->psi_e[s0], 2 * n * e->esz});
+  e->state.push_back({nullptr, &e->psi_h[s0], 2 * n * e->esz});
   e->psi_bytes[axis] = n * e->esz;
   return XF_OK;
 }
@@ -1290,32 +1293,37 @@ static xf_status alloc_alt(xf_engine *e) {
   xf_status s;
   for (int q = 0; q < 6; ++q)
     if ((s = dev_alloc(e, &e->f_alt[q], (size_t)e->ncell * e->esz)) != XF_OK) return s;
-  // both psi sets as one block each (the kernel addresses a slot as block +
-  // offset); the current values move into the first block
-  static const int axis_of[6] = {1, 2, 0, 2, 0, 1};  // ex_y, ex_z, ey_x, ey_z, ez_x, ez_y
-  // every slot the size of the largest slab array, so the kernel finds slot q
-  // at q * slot
+  // both psi sets as one block each: per side the pairs y (AY, CY), z (AZ,
+  // BZ) and x (BX, CX), each in 2 * slot elements interleaved as in their own
+  // allocations (slot = the largest slab array), so the kernel finds pair r
+  // of side h at (3 h + r) * 2 * slot; the current values move into the
+  // first block
+  static const int pfirst[3] = {P_AY, P_AZ, P_BX}, psecond[3] = {P_CY, P_BZ, P_CX}, paxis[3] = {1, 2, 0};
   long long slot = 1;
   for (int ax = 0; ax < 3; ++ax) slot = std::max(slot, (long long)(e->psi_bytes[ax] / e->esz));
-  for (int q = 0; q < 12; ++q) e->psi_off[q] = q * slot;
   const long long n = 12 * slot;
   e->psi_n = slot;
   if ((s = dev_alloc(e, &e->psi_blk, (size_t)n * e->esz)) != XF_OK) return s;
   if ((s = dev_alloc(e, &e->psi_blk_alt, (size_t)n * e->esz)) != XF_OK) return s;
-  for (int q = 0; q < 12; ++q) {
-    const size_t nb = e->psi_bytes[axis_of[q % 6]];
-    void **cur = q < 6 ? &e->psi_e[q] : &e->psi_h[q - 6];
-    void **alt = q < 6 ? &e->psi_e_alt[q] : &e->psi_h_alt[q - 6];
-    char *dst = (char *)e->psi_blk + (size_t)e->psi_off[q] * e->esz;
-    if (nb) HIPCHK(hipMemcpyAsync(dst, *cur, nb, hipMemcpyDeviceToDevice, e->stream));
-    // axes without CPML keep their 16-byte placeholders (never addressed)
-    if (nb) {
-      *cur = dst;
-      *alt = (char *)e->psi_blk_alt + (size_t)e->psi_off[q] * e->esz;
-    } else {
-      *alt = *cur;
+  for (int side = 0; side < 2; ++side)
+    for (int r = 0; r < 3; ++r) {
+      const size_t nb = 2 * e->psi_bytes[paxis[r]];
+      void **cur = side ? e->psi_h : e->psi_e, **alt = side ? e->psi_h_alt : e->psi_e_alt;
+      const size_t off = (size_t)(3 * side + r) * 2 * slot * e->esz;
+      const int a = pfirst[r], b = psecond[r];
+      // axes without CPML keep their placeholders (never addressed)
+      if (nb) {
+        char *dst = (char *)e->psi_blk + off;
+        HIPCHK(hipMemcpyAsync(dst, cur[a], nb, hipMemcpyDeviceToDevice, e->stream));
+        cur[a] = dst;
+        cur[b] = dst + e->esz;
+        alt[a] = (char *)e->psi_blk_alt + off;
+        alt[b] = (char *)alt[a] + e->esz;
+      } else {
+        alt[a] = cur[a];
+        alt[b] = cur[b];
+      }
     }
-  }
   HIPCHK(hipStreamSynchronize(e->stream));
   if (e->has_tfsf) {
     if ((s = alloc_lines_alt(e)) != XF_OK) return s;

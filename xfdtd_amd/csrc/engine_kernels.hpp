@@ -350,6 +350,21 @@ __device__ __forceinline__ unsigned psi_y(int i, int ly, int k, int Ly, int pz) 
 __device__ __forceinline__ unsigned psi_z(int i, int j, int lz, int nm1y, int Lz) {
   return ((unsigned)i * (unsigned)nm1y + (unsigned)j) * (unsigned)Lz + (unsigned)lz;
 }
+// The two psi arrays of an axis share one allocation, interleaved: x (BX,
+// CX), y (AY, CY), z (AZ, BZ), the first at even elements; psi_e[first] and
+// psi_e[second] point at its element 0 and 1, so slot q's cell idx is at
+// psi_e[q][2 idx], and one two-wide access reaches both.
+template <typename T>
+__device__ __forceinline__ P2<T> ld_pair(const T *first, unsigned idx) {
+  return reinterpret_cast<const P2<T> *>(first)[idx];
+}
+template <typename T>
+__device__ __forceinline__ void st_pair(T *first, unsigned idx, T a, T b) {
+  P2<T> v;
+  v.x = a;
+  v.y = b;
+  reinterpret_cast<P2<T> *>(first)[idx] = v;
+}
 __device__ __forceinline__ unsigned psi_x(int lx, int j, int k, int nm1y, int pz) {
   return ((unsigned)lx * (unsigned)nm1y + (unsigned)j) * (unsigned)pz + (unsigned)k;
 }
@@ -588,17 +603,22 @@ __device__ __forceinline__ void load_plane_e(const Grid<T> &g, const Col<T> &q, 
       p.bx = g.be[0][i];
       p.ax = g.ae[0][i];
     }
-    if (i < g.nx) {
-      if (ly >= 0) p.ps[P_AY] = g.psi_e[P_AY][psi_y(i, ly, k, Ly, g.pz)];
-      if (lz >= 0) p.ps[P_AZ] = g.psi_e[P_AZ][psi_z(i, j, lz, nm1y, Lz)];
+    // psi pairs, each one two-wide load: x (BX for Ey, CX for Ez), y (AY for
+    // Ex, CY for Ez), z (AZ for Ex, BZ for Ey)
+    if (lx >= 0 && (j < g.ny || k < g.nz)) {
+      const P2<T> v = ld_pair(g.psi_e[P_BX], psi_x(lx, j, k, nm1y, g.pz));
+      p.ps[P_BX] = v.x;
+      p.ps[P_CX] = v.y;
     }
-    if (j < g.ny) {
-      if (lx >= 0) p.ps[P_BX] = g.psi_e[P_BX][psi_x(lx, j, k, nm1y, g.pz)];
-      if (lz >= 0) p.ps[P_BZ] = g.psi_e[P_BZ][psi_z(i, j, lz, nm1y, Lz)];
+    if (ly >= 0 && (i < g.nx || k < g.nz)) {
+      const P2<T> v = ld_pair(g.psi_e[P_AY], psi_y(i, ly, k, Ly, g.pz));
+      p.ps[P_AY] = v.x;
+      p.ps[P_CY] = v.y;
     }
-    if (k < g.nz) {
-      if (lx >= 0) p.ps[P_CX] = g.psi_e[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
-      if (ly >= 0) p.ps[P_CY] = g.psi_e[P_CY][psi_y(i, ly, k, Ly, g.pz)];
+    if (lz >= 0 && (i < g.nx || j < g.ny)) {
+      const P2<T> v = ld_pair(g.psi_e[P_AZ], psi_z(i, j, lz, nm1y, Lz));
+      p.ps[P_AZ] = v.x;
+      p.ps[P_BZ] = v.y;
     }
     // TFSF z faces: Ey off the x faces, Ex off the y faces
     p.z0 = p.z1 = 0.0;
@@ -667,41 +687,38 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
   if constexpr (SLAB) {
     const int ly = q.cp.ly, lz = q.cp.lz, lx = p.lx;
     const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = ny + 1;
-    if (i < nx) {
-      if (ly >= 0) {
-        const T ps = q.cp.by * p.ps[P_AY] + q.cp.ay * dhz_j;
-        g.psi_e[P_AY][psi_y(i, ly, k, Ly, g.pz)] = ps;
-        vx = vx + tt.pj[tt.jcol + m].x * ps;
-      }
-      if (lz >= 0) {
-        const T ps = q.cp.bz * p.ps[P_AZ] + q.cp.az * dhy_k;
-        g.psi_e[P_AZ][psi_z(i, j, lz, nm1y, Lz)] = ps;
-        vx = vx + tt.pk[tt.kcol + m].x * ps;
-      }
+    // the new psi of each pair (the old value where the component lies
+    // outside its range), stored with one two-wide store; then the terms,
+    // added in the order of the reference's updates
+    const bool ux = i < nx, uy = j < ny, uz = k < nz;
+    T n_ay = p.ps[P_AY], n_az = p.ps[P_AZ], n_bx = p.ps[P_BX], n_bz = p.ps[P_BZ], n_cx = p.ps[P_CX],
+      n_cy = p.ps[P_CY];
+    if (lx >= 0 && (uy || uz)) {
+      if (uy) n_bx = p.bx * p.ps[P_BX] + p.ax * dhz_i;
+      if (uz) n_cx = p.bx * p.ps[P_CX] + p.ax * dhy_i;
+      st_pair(g.psi_e[P_BX], psi_x(lx, j, k, nm1y, g.pz), n_bx, n_cx);
     }
-    if (j < ny) {
-      if (lx >= 0) {
-        const T ps = p.bx * p.ps[P_BX] + p.ax * dhz_i;
-        g.psi_e[P_BX][psi_x(lx, j, k, nm1y, g.pz)] = ps;
-        vy = vy + tt.pi[mi].x * ps;
-      }
-      if (lz >= 0) {
-        const T ps = q.cp.bz * p.ps[P_BZ] + q.cp.az * dhx_k;
-        g.psi_e[P_BZ][psi_z(i, j, lz, nm1y, Lz)] = ps;
-        vy = vy + tt.pk[tt.kcol + m].y * ps;
-      }
+    if (ly >= 0 && (ux || uz)) {
+      if (ux) n_ay = q.cp.by * p.ps[P_AY] + q.cp.ay * dhz_j;
+      if (uz) n_cy = q.cp.by * p.ps[P_CY] + q.cp.ay * dhx_j;
+      st_pair(g.psi_e[P_AY], psi_y(i, ly, k, Ly, g.pz), n_ay, n_cy);
     }
-    if (k < nz) {
-      if (lx >= 0) {
-        const T ps = p.bx * p.ps[P_CX] + p.ax * dhy_i;
-        g.psi_e[P_CX][psi_x(lx, j, k, nm1y, g.pz)] = ps;
-        vz = vz + tt.pi[mi].y * ps;
-      }
-      if (ly >= 0) {
-        const T ps = q.cp.by * p.ps[P_CY] + q.cp.ay * dhx_j;
-        g.psi_e[P_CY][psi_y(i, ly, k, Ly, g.pz)] = ps;
-        vz = vz + tt.pj[tt.jcol + m].y * ps;
-      }
+    if (lz >= 0 && (ux || uy)) {
+      if (ux) n_az = q.cp.bz * p.ps[P_AZ] + q.cp.az * dhy_k;
+      if (uy) n_bz = q.cp.bz * p.ps[P_BZ] + q.cp.az * dhx_k;
+      st_pair(g.psi_e[P_AZ], psi_z(i, j, lz, nm1y, Lz), n_az, n_bz);
+    }
+    if (ux) {
+      if (ly >= 0) vx = vx + tt.pj[tt.jcol + m].x * n_ay;
+      if (lz >= 0) vx = vx + tt.pk[tt.kcol + m].x * n_az;
+    }
+    if (uy) {
+      if (lx >= 0) vy = vy + tt.pi[mi].x * n_bx;
+      if (lz >= 0) vy = vy + tt.pk[tt.kcol + m].y * n_bz;
+    }
+    if (uz) {
+      if (lx >= 0) vz = vz + tt.pi[mi].y * n_cx;
+      if (ly >= 0) vz = vz + tt.pj[tt.jcol + m].y * n_cy;
     }
     // TFSF z faces (wave-uniform zt: the wave holds a z-face column and the
     // plane lies in [li, ri]); Ey -/+ and Ex +/- at lk / rk
@@ -844,17 +861,23 @@ __device__ __forceinline__ void load_plane_h(const Grid<T> &g, const Col<T> &q, 
       p.bx = g.bm[0][i];
       p.ax = g.am[0][i];
     }
-    if (q.jin && q.kin) {
-      if (ly >= 0) p.ps[P_AY] = g.psi_h[P_AY][psi_y(i, ly, k, Ly, g.pz)];
-      if (lz >= 0) p.ps[P_AZ] = g.psi_h[P_AZ][psi_z(i, j, lz, nm1y, Lz)];
+    // psi pairs, each one two-wide load: x (BX for Hy, CX for Hz), y (AY for
+    // Hx, CY for Hz), z (AZ for Hx, BZ for Hy)
+    const bool sx = q.jin && q.kin, sy = i < g.nx && q.kin, sz = i < g.nx && q.jin;
+    if (lx >= 0 && (sy || sz)) {
+      const P2<T> v = ld_pair(g.psi_h[P_BX], psi_x(lx, j, k, nm1y, g.pz));
+      p.ps[P_BX] = v.x;
+      p.ps[P_CX] = v.y;
     }
-    if (i < g.nx && q.kin) {
-      if (lx >= 0) p.ps[P_BX] = g.psi_h[P_BX][psi_x(lx, j, k, nm1y, g.pz)];
-      if (lz >= 0) p.ps[P_BZ] = g.psi_h[P_BZ][psi_z(i, j, lz, nm1y, Lz)];
+    if (ly >= 0 && (sx || sz)) {
+      const P2<T> v = ld_pair(g.psi_h[P_AY], psi_y(i, ly, k, Ly, g.pz));
+      p.ps[P_AY] = v.x;
+      p.ps[P_CY] = v.y;
     }
-    if (i < g.nx && q.jin) {
-      if (lx >= 0) p.ps[P_CX] = g.psi_h[P_CX][psi_x(lx, j, k, nm1y, g.pz)];
-      if (ly >= 0) p.ps[P_CY] = g.psi_h[P_CY][psi_y(i, ly, k, Ly, g.pz)];
+    if (lz >= 0 && (sx || sy)) {
+      const P2<T> v = ld_pair(g.psi_h[P_AZ], psi_z(i, j, lz, nm1y, Lz));
+      p.ps[P_AZ] = v.x;
+      p.ps[P_BZ] = v.y;
     }
     // TFSF z faces: Hy and Hx at lk - 1 and rk
     p.z0 = p.z1 = 0.0;
@@ -895,44 +918,39 @@ __device__ __forceinline__ bool compute_plane_h(const Grid<T> &g, const TileTabl
   if constexpr (SLAB) {
     const int ly = q.cp.ly, lz = q.cp.lz, lx = p.lx;
     const int Ly = g.dn[1] + g.dp[1], Lz = g.dn[2] + g.dp[2], nm1y = g.ny + 1;
-    // Hx (nx+1, ny, nz): CPML over j < ny, k < nz
-    if (jin && kin) {
-      if (ly >= 0) {
-        const T ps = q.cp.by * p.ps[P_AY] + q.cp.ay * dez_j;
-        g.psi_h[P_AY][psi_y(i, ly, k, Ly, g.pz)] = ps;
-        vx = vx + tt.pj[tt.jcol + m].x * ps;
-      }
-      if (lz >= 0) {
-        const T ps = q.cp.bz * p.ps[P_AZ] + q.cp.az * dey_k;
-        g.psi_h[P_AZ][psi_z(i, j, lz, nm1y, Lz)] = ps;
-        vx = vx + tt.pk[tt.kcol + m].x * ps;
-      }
+    // CPML over Hx (nx+1, ny, nz): j < ny, k < nz; Hy (nx, ny+1, nz): i < nx,
+    // k < nz; Hz (nx, ny, nz+1): i < nx, j < ny.  The new psi of each pair
+    // (the old value where the component lies outside its range), stored
+    // with one two-wide store; then the terms in the reference's order.
+    const bool ux = jin && kin, uy = i < nx && kin, uz = i < nx && jin;
+    T n_ay = p.ps[P_AY], n_az = p.ps[P_AZ], n_bx = p.ps[P_BX], n_bz = p.ps[P_BZ], n_cx = p.ps[P_CX],
+      n_cy = p.ps[P_CY];
+    if (lx >= 0 && (uy || uz)) {
+      if (uy) n_bx = p.bx * p.ps[P_BX] + p.ax * dez_i;
+      if (uz) n_cx = p.bx * p.ps[P_CX] + p.ax * dey_i;
+      st_pair(g.psi_h[P_BX], psi_x(lx, j, k, nm1y, g.pz), n_bx, n_cx);
     }
-    // Hy (nx, ny+1, nz): CPML over i < nx, k < nz
-    if (i < nx && kin) {
-      if (lx >= 0) {
-        const T ps = p.bx * p.ps[P_BX] + p.ax * dez_i;
-        g.psi_h[P_BX][psi_x(lx, j, k, nm1y, g.pz)] = ps;
-        vy = vy + tt.pi[mi].x * ps;
-      }
-      if (lz >= 0) {
-        const T ps = q.cp.bz * p.ps[P_BZ] + q.cp.az * dex_k;
-        g.psi_h[P_BZ][psi_z(i, j, lz, nm1y, Lz)] = ps;
-        vy = vy + tt.pk[tt.kcol + m].y * ps;
-      }
+    if (ly >= 0 && (ux || uz)) {
+      if (ux) n_ay = q.cp.by * p.ps[P_AY] + q.cp.ay * dez_j;
+      if (uz) n_cy = q.cp.by * p.ps[P_CY] + q.cp.ay * dex_j;
+      st_pair(g.psi_h[P_AY], psi_y(i, ly, k, Ly, g.pz), n_ay, n_cy);
     }
-    // Hz (nx, ny, nz+1): CPML over i < nx, j < ny
-    if (i < nx && jin) {
-      if (lx >= 0) {
-        const T ps = p.bx * p.ps[P_CX] + p.ax * dey_i;
-        g.psi_h[P_CX][psi_x(lx, j, k, nm1y, g.pz)] = ps;
-        vz = vz + tt.pi[mi].y * ps;
-      }
-      if (ly >= 0) {
-        const T ps = q.cp.by * p.ps[P_CY] + q.cp.ay * dex_j;
-        g.psi_h[P_CY][psi_y(i, ly, k, Ly, g.pz)] = ps;
-        vz = vz + tt.pj[tt.jcol + m].y * ps;
-      }
+    if (lz >= 0 && (ux || uy)) {
+      if (ux) n_az = q.cp.bz * p.ps[P_AZ] + q.cp.az * dey_k;
+      if (uy) n_bz = q.cp.bz * p.ps[P_BZ] + q.cp.az * dex_k;
+      st_pair(g.psi_h[P_AZ], psi_z(i, j, lz, nm1y, Lz), n_az, n_bz);
+    }
+    if (ux) {
+      if (ly >= 0) vx = vx + tt.pj[tt.jcol + m].x * n_ay;
+      if (lz >= 0) vx = vx + tt.pk[tt.kcol + m].x * n_az;
+    }
+    if (uy) {
+      if (lx >= 0) vy = vy + tt.pi[mi].x * n_bx;
+      if (lz >= 0) vy = vy + tt.pk[tt.kcol + m].y * n_bz;
+    }
+    if (uz) {
+      if (lx >= 0) vz = vz + tt.pi[mi].y * n_cx;
+      if (ly >= 0) vz = vz + tt.pj[tt.jcol + m].y * n_cy;
     }
     // TFSF z faces: Hy +/- and Hx -/+ at lk - 1 / rk
     if (zt && (k == zf.lk - 1 || k == zf.rk)) {

@@ -106,6 +106,14 @@ __host__ __device__ inline long long tf_seg_size(int s, long long ni, long long 
 }
 
 // Arguments of one fused step (one direction of the ping-pong).
+// element of slot q (P_AY..P_CY) at psi index idx within one side of a psi
+// block: pair region y, z or x of 2 * psl elements, the slot's half of it
+__device__ __forceinline__ unsigned sw_psi(int q, unsigned idx, unsigned psl) {
+  const int region = (q == P_AY || q == P_CY) ? 0 : (q == P_AZ || q == P_BZ) ? 1 : 2;
+  const int half = (q == P_CY || q == P_BZ || q == P_CX) ? 1 : 0;
+  return (unsigned)region * 2u * psl + 2u * idx + (unsigned)half;
+}
+
 template <typename T>
 struct SweepArgs {
   const T *src[6];  // fields of set A (read)
@@ -114,8 +122,10 @@ struct SweepArgs {
   const T *self[2], *kt[2], *jt[2], *it[2];  // curl coefficient records (E, H)
   const T *kp[2], *jp[2], *ip[2];            // psi coefficient pairs
   const T *be[3], *ae[3], *bm[3], *am[3];
-  const T *psiA;  // psi of set A: one block, slot q (P_AY..P_CY of E, then H) at q * psi_slot
-  T *psiB;        // psi of set B, same layout
+  // psi of set A: one block; per side (E, then H) the interleaved pairs y
+  // (AY, CY), z (AZ, BZ), x (BX, CX) of 2 * psi_slot elements each (sw_psi)
+  const T *psiA;
+  T *psiB;  // psi of set B, same layout
   unsigned psi_slot;
   const double *terms;  // TFSF face terms of the step (segment s at s * term_stride)
   unsigned term_stride;
@@ -248,13 +258,13 @@ __device__ __forceinline__ void sw_e_cell(const SwCtx<T, R, MS> &c, int p, int j
       const unsigned kq = (unsigned)min(kk, c.pz - 1);
       if (DX && plt) {
         if (lye >= 0) {
-          const unsigned o = 0 * psl + psi_y(pl, lye, kq, c.Ly, c.pz);
+          const unsigned o = sw_psi(0, psi_y(pl, lye, kq, c.Ly, c.pz), psl);
           const T ps = bye * c.pA[o] + aye * dhz_j;
           if (own) c.pB[o] = ps;
           vx = vx + c.jp(0, 1, m, js).x * ps;
         }
         if (z.le >= 0) {
-          const unsigned o = 1 * psl + psi_z(pl, jc, z.le, c.nm1y, c.Lz);
+          const unsigned o = sw_psi(1, psi_z(pl, jc, z.le, c.nm1y, c.Lz), psl);
           const T ps = z.be * c.pA[o] + z.ae * dhy_k;
           if (own) c.pB[o] = ps;
           vx = vx + c.kp(0, 1, m, kslot).x * ps;
@@ -262,13 +272,13 @@ __device__ __forceinline__ void sw_e_cell(const SwCtx<T, R, MS> &c, int p, int j
       }
       if (DY && jlt) {
         if (lxe >= 0) {
-          const unsigned o = 2 * psl + psi_x(lxe, jc, kq, c.nm1y, c.pz);
+          const unsigned o = sw_psi(2, psi_x(lxe, jc, kq, c.nm1y, c.pz), psl);
           const T ps = bxe * c.pA[o] + axe * dhz_i;
           if (own) c.pB[o] = ps;
           vy = vy + c.ip(0, 1, m, is).x * ps;
         }
         if (z.le >= 0) {
-          const unsigned o = 3 * psl + psi_z(pl, jc, z.le, c.nm1y, c.Lz);
+          const unsigned o = sw_psi(3, psi_z(pl, jc, z.le, c.nm1y, c.Lz), psl);
           const T ps = z.be * c.pA[o] + z.ae * dhx_k;
           if (own) c.pB[o] = ps;
           vy = vy + c.kp(0, 1, m, kslot).y * ps;
@@ -276,13 +286,13 @@ __device__ __forceinline__ void sw_e_cell(const SwCtx<T, R, MS> &c, int p, int j
       }
       if (DZ && klt) {
         if (lxe >= 0) {
-          const unsigned o = 4 * psl + psi_x(lxe, jc, kq, c.nm1y, c.pz);
+          const unsigned o = sw_psi(4, psi_x(lxe, jc, kq, c.nm1y, c.pz), psl);
           const T ps = bxe * c.pA[o] + axe * dhy_i;
           if (own) c.pB[o] = ps;
           vz = vz + c.ip(0, 1, m, is).y * ps;
         }
         if (lye >= 0) {
-          const unsigned o = 5 * psl + psi_y(pl, lye, kq, c.Ly, c.pz);
+          const unsigned o = sw_psi(5, psi_y(pl, lye, kq, c.Ly, c.pz), psl);
           const T ps = bye * c.pA[o] + aye * dhx_j;
           if (own) c.pB[o] = ps;
           vz = vz + c.jp(0, 1, m, js).y * ps;
@@ -292,13 +302,13 @@ __device__ __forceinline__ void sw_e_cell(const SwCtx<T, R, MS> &c, int p, int j
       // z slab only: Ex and Ey take their z psi
       if (z.le >= 0) {
         if (DX && plt) {
-          const unsigned o = 1 * psl + psi_z(pl, jc, z.le, c.nm1y, c.Lz);
+          const unsigned o = sw_psi(1, psi_z(pl, jc, z.le, c.nm1y, c.Lz), psl);
           const T ps = z.be * c.pA[o] + z.ae * dhy_k;
           if (own) c.pB[o] = ps;
           vx = vx + c.kp(0, 1, m, kslot).x * ps;
         }
         if (DY && jlt) {
-          const unsigned o = 3 * psl + psi_z(pl, jc, z.le, c.nm1y, c.Lz);
+          const unsigned o = sw_psi(3, psi_z(pl, jc, z.le, c.nm1y, c.Lz), psl);
           const T ps = z.be * c.pA[o] + z.ae * dhx_k;
           if (own) c.pB[o] = ps;
           vy = vy + c.kp(0, 1, m, kslot).y * ps;
@@ -396,13 +406,13 @@ __device__ __forceinline__ void sw_h_cell(const SwCtx<T, R, MS> &c, int i, int j
       const unsigned kl = (unsigned)min(k, c.pz - 1);
       if (jlt && klt) {
         if (lyh >= 0) {
-          const unsigned o = oh + 0 * psl + psi_y(i, lyh, kl, c.Ly, c.pz);
+          const unsigned o = oh + sw_psi(0, psi_y(i, lyh, kl, c.Ly, c.pz), psl);
           const T ps = byh * c.pA[o] + ayh * dez_j;
           if (own) c.pB[o] = ps;
           vx = vx + c.jp(1, 1, m, js).x * ps;
         }
         if (z.lh >= 0) {
-          const unsigned o = oh + 1 * psl + psi_z(i, jc, z.lh, c.nm1y, c.Lz);
+          const unsigned o = oh + sw_psi(1, psi_z(i, jc, z.lh, c.nm1y, c.Lz), psl);
           const T ps = z.bh * c.pA[o] + z.ah * dey_k;
           if (own) c.pB[o] = ps;
           vx = vx + c.kp(1, 1, m, lane).x * ps;
@@ -410,13 +420,13 @@ __device__ __forceinline__ void sw_h_cell(const SwCtx<T, R, MS> &c, int i, int j
       }
       if (ilt && klt) {
         if (lxh >= 0) {
-          const unsigned o = oh + 2 * psl + psi_x(lxh, jc, kl, c.nm1y, c.pz);
+          const unsigned o = oh + sw_psi(2, psi_x(lxh, jc, kl, c.nm1y, c.pz), psl);
           const T ps = bxh * c.pA[o] + axh * dez_i;
           if (own) c.pB[o] = ps;
           vy = vy + c.ip(1, 1, m, is).x * ps;
         }
         if (z.lh >= 0) {
-          const unsigned o = oh + 3 * psl + psi_z(i, jc, z.lh, c.nm1y, c.Lz);
+          const unsigned o = oh + sw_psi(3, psi_z(i, jc, z.lh, c.nm1y, c.Lz), psl);
           const T ps = z.bh * c.pA[o] + z.ah * dex_k;
           if (own) c.pB[o] = ps;
           vy = vy + c.kp(1, 1, m, lane).y * ps;
@@ -424,13 +434,13 @@ __device__ __forceinline__ void sw_h_cell(const SwCtx<T, R, MS> &c, int i, int j
       }
       if (ilt && jlt) {
         if (lxh >= 0) {
-          const unsigned o = oh + 4 * psl + psi_x(lxh, jc, kl, c.nm1y, c.pz);
+          const unsigned o = oh + sw_psi(4, psi_x(lxh, jc, kl, c.nm1y, c.pz), psl);
           const T ps = bxh * c.pA[o] + axh * dey_i;
           if (own) c.pB[o] = ps;
           vz = vz + c.ip(1, 1, m, is).y * ps;
         }
         if (lyh >= 0) {
-          const unsigned o = oh + 5 * psl + psi_y(i, lyh, kl, c.Ly, c.pz);
+          const unsigned o = oh + sw_psi(5, psi_y(i, lyh, kl, c.Ly, c.pz), psl);
           const T ps = byh * c.pA[o] + ayh * dex_j;
           if (own) c.pB[o] = ps;
           vz = vz + c.jp(1, 1, m, js).y * ps;
@@ -440,13 +450,13 @@ __device__ __forceinline__ void sw_h_cell(const SwCtx<T, R, MS> &c, int i, int j
       // z slab only: Hx and Hy take their z psi
       if (z.lh >= 0 && klt) {
         if (jlt) {
-          const unsigned o = oh + 1 * psl + psi_z(i, jc, z.lh, c.nm1y, c.Lz);
+          const unsigned o = oh + sw_psi(1, psi_z(i, jc, z.lh, c.nm1y, c.Lz), psl);
           const T ps = z.bh * c.pA[o] + z.ah * dey_k;
           if (own) c.pB[o] = ps;
           vx = vx + c.kp(1, 1, m, lane).x * ps;
         }
         if (ilt) {
-          const unsigned o = oh + 3 * psl + psi_z(i, jc, z.lh, c.nm1y, c.Lz);
+          const unsigned o = oh + sw_psi(3, psi_z(i, jc, z.lh, c.nm1y, c.Lz), psl);
           const T ps = z.bh * c.pA[o] + z.ah * dex_k;
           if (own) c.pB[o] = ps;
           vy = vy + c.kp(1, 1, m, lane).y * ps;
