@@ -108,6 +108,27 @@ def run_cpu_baseline(sc, total, steps, threads=1):
     }
 
 
+def pattern_ceiling(n):
+    """The E curl kernel's access pattern without its arithmetic, CPML, TFSF
+    or tables (xfdtd_amd/pattern_model, built from csrc/pattern_model.hip):
+    the same fields, layout, tiles and loads and stores per cell, timed on
+    this GPU in FP32 and FP64.  Its algorithmic GB/s is the practical ceiling
+    of the split curl kernels."""
+    import subprocess
+    exe = os.path.join(ROOT, "xfdtd_amd", "pattern_model")
+    if not os.path.exists(exe):
+        raise FileNotFoundError(f"{exe} is not built (make -C xfdtd_amd/csrc)")
+    out = subprocess.run([exe, str(n)], check=True, capture_output=True, text=True, timeout=300).stdout
+    res = {}
+    for ln in out.splitlines():
+        if ln.startswith("{"):
+            d = json.loads(ln)
+            res[d.pop("precision")] = d
+    res["pattern"] = ("64 k x 4 j threads marching 8 x planes; per cell read Ex Ey Ez Hx Hy Hz, the material "
+                      "byte, Hy Hx at k-1, Hz Hx at j-1, write Ex Ey Ez in place")
+    return res
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of a curl kernel from the committed counter runs."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
@@ -611,6 +632,8 @@ def main():
                     help="timed steps of the FP64 run of the same workload (0: skip it)")
     ap.add_argument("--accuracy-steps", type=int, default=2000,
                     help="steps of the FP32-against-FP64 accuracy runs (0: skip them)")
+    ap.add_argument("--pattern", type=int, default=1,
+                    help="time the curl kernels' access pattern alone (0: skip it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 1 GiB device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -678,6 +701,8 @@ def main():
         "cpu_baseline": None,
     }
     legs = []
+    if args.pattern:
+        legs.append(("pattern_ceiling", lambda: pattern_ceiling(args.size)))
     if args.fused_steps > 0:
         legs.append(("fused", lambda: fused_leg(sc, total, args.fused_steps, args.warmup, args.precision)))
     if args.fp64_steps > 0 and args.precision == 32:
@@ -711,6 +736,17 @@ def main():
         except Exception as exc:  # noqa: BLE001
             line[name] = {"error": f"{type(exc).__name__}: {exc}"}
         print(f"bench: {name} done in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    pc = line.get("pattern_ceiling") or {}
+    key = "f32" if args.precision == 32 else "f64"
+    if key in pc:
+        ceil_gbs = pc[key]["algorithmic_gbs"]
+        line["roofline"]["pattern_ceiling"] = ceil_gbs
+        line["roofline"]["frac_of_pattern"] = round(achieved / ceil_gbs, 4)
+    f64 = line.get("fp64")
+    if "f64" in pc and isinstance(f64, dict) and "roofline" in f64:
+        ceil_gbs = pc["f64"]["algorithmic_gbs"]
+        f64["roofline"]["pattern_ceiling"] = ceil_gbs
+        f64["roofline"]["frac_of_pattern"] = round(f64["roofline"]["achieved"] / ceil_gbs, 4)
     print(json.dumps(line), flush=True)
 
 
