@@ -2330,7 +2330,7 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
   double *g = (double *)B.alloc((size_t)e->nd * sizeof(double));
   if (!hist[0] || (nseg > 1 && !hist[1]) || !g) FAILS(XF_ENOMEM, "adjoint buffers");
   const Grid<T> gr = make_grid<T>(e);
-  const unsigned nb = (unsigned)((e->ncell + 255) / 256), nsnap = (unsigned)((e->nd + 255) / 256);
+  const unsigned nb = (unsigned)adj_blocks(e->ncell), nsnap = (unsigned)((e->nd + 255) / 256);
   // forward run with a checkpoint at the start of every segment but the
   // last, whose design-box E fields are kept as it runs (it is never
   // recomputed), and a checkpoint at the end
@@ -2490,7 +2490,7 @@ static xf_status adjoint_run(xf_engine *e, long long nsteps, double *fields) {
       if ((s = drude_box_copy<T>(e, q, a.Ja[q], fields + joff[q], true)) != XF_OK) return s;
     }
   const Grid<T> gr = make_grid<T>(e);
-  const unsigned nb = (unsigned)((e->ncell + 255) / 256);
+  const unsigned nb = (unsigned)adj_blocks(e->ncell);
   // the given adjoints are the gathered, unscaled values the kernels pass
   // between them (the first E kernel takes its E adjoint as it is); a last
   // E kernel applies the pending E scaling
@@ -2566,7 +2566,7 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
     x.a.grad = gdev[q];
     x.a.grad_on = 1;
     for (int w = 0; w < (e->nd + 255) / 256; ++w) msnap.push_back(make_int2(q, w));
-    for (long long w = 0; w < (e->ncell + 255) / 256; ++w) mcell.push_back(make_int2(q, (int)w));
+    for (long long w = 0; w < adj_blocks(e->ncell); ++w) mcell.push_back(make_int2(q, (int)w));
   }
   if (mcell.size() > 0x7fffffffu) FAILS(XF_EINVAL, "batch too large");
   AdjBufs D(st);

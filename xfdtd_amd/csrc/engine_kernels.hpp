@@ -1900,36 +1900,62 @@ __device__ __forceinline__ void adj_scale_e(const Grid<T> &g, const Adj<T> &a, l
   ez = oz;
 }
 
+// Cells of the adjoint kernels: each wave of 64 lanes covers ADJ_RUN = 63
+// consecutive cells of the padded layout, and its remaining lane holds the
+// cell next to the wave's end cell along k (k - 1 below lane 1 in the E
+// kernel, k + 1 above lane 62 in the H kernel).  That lane forms only the
+// two weights its neighbour gathers from it, so every cell takes its k
+// neighbour's weights from the adjacent lane by DPP instead of forming them
+// again.  A launch needs adj_blocks(ncell) workgroups of 256.
+constexpr long long ADJ_RUN = 63;
+__host__ __device__ inline long long adj_blocks(long long ncell) { return ((ncell + ADJ_RUN - 1) / ADJ_RUN + 3) / 4; }
+
 // per E cell: the pending E scaling, the gather of the H-side weights of the
 // cell and of its k-1, j-1, i-1 neighbours, the objective injection and the
-// gradient
+// gradient; t is the thread's index in the launch
 template <typename T>
 __device__ __forceinline__ void adj_e_body(const Grid<T> &g, const Adj<T> &a, long long step, int slot, int par,
-                                           int mode, long long c) {
+                                           int mode, long long t) {
+  const int lane = (int)(t & 63);
+  const bool halo = lane == 0;  // the k - 1 cell of lane 1's cell
+  const long long c = (t >> 6) * ADJ_RUN + lane - 1;
+  if (c < 0) return;
   const int i = (int)(c / g.si), r = (int)(c - (long long)i * g.si), j = r / (int)g.sj, k = r - j * (int)g.sj;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx || j > ny || k > nz) return;
-  T ex = a.Ea[0][c], ey = a.Ea[1][c], ez = a.Ea[2][c];
-  if (mode != ADJ_FIRST) adj_scale_e(g, a, c, i, j, k, ex, ey, ez);
-  if (mode == ADJ_FINAL) {
-    a.Ea[0][c] = ex;
-    a.Ea[1][c] = ey;
-    a.Ea[2][c] = ez;
+  T ex = T(0), ey = T(0), ez = T(0);
+  if (!halo) {
+    ex = a.Ea[0][c];
+    ey = a.Ea[1][c];
+    ez = a.Ea[2][c];
+    if (mode != ADJ_FIRST) adj_scale_e(g, a, c, i, j, k, ex, ey, ez);
+    if (mode == ADJ_FINAL) {
+      a.Ea[0][c] = ex;
+      a.Ea[1][c] = ey;
+      a.Ea[2][c] = ez;
+      return;
+    }
+  } else if (mode == ADJ_FINAL) {
     return;
   }
-  const long long ck = c - 1, cj = c - g.sj, ci = c - g.si;
+  const long long cj = c - g.sj, ci = c - g.si;
   const bool sx = i < nx, sy = j < ny, sz = k < nz;  // inside Ex / Ey / Ez shapes
-  // gather: transpose of the H curl differences (forward differences of E)
-  T w[6], wk[6], wj[6], wi[6];
-  adj_w(g, a, par, c, i, j, k, a.Ha[0][c], a.Ha[1][c], a.Ha[2][c], 0u, true, w);
-  if (k >= 1 && (sx || sy))
-    adj_w(g, a, par, ck, i, j, k - 1, a.Ha[0][ck], a.Ha[1][ck], T(0), (1u << W_XY) | (1u << W_YX), false, wk);
+  // gather: transpose of the H curl differences (forward differences of E);
+  // the halo lane forms the two weights the cell above takes, and no psi
+  T w[6], wj[6], wi[6];
+  if (halo)
+    adj_w(g, a, par, c, i, j, k, a.Ha[0][c], a.Ha[1][c], T(0), (1u << W_XY) | (1u << W_YX), false, w);
+  else
+    adj_w(g, a, par, c, i, j, k, a.Ha[0][c], a.Ha[1][c], a.Ha[2][c], 0u, true, w);
+  // the k - 1 cell's weights, formed by the lane below
+  const T wk_xy = from_below(w[W_XY], T(0)), wk_yx = from_below(w[W_YX], T(0));
+  if (halo) return;
   if (j >= 1 && (sx || sz))
     adj_w(g, a, par, cj, i, j - 1, k, a.Ha[0][cj], T(0), a.Ha[2][cj], (1u << W_XZ) | (1u << W_ZX), false, wj);
   if (i >= 1 && (sy || sz))
     adj_w(g, a, par, ci, i - 1, j, k, T(0), a.Ha[1][ci], a.Ha[2][ci], (1u << W_YZ) | (1u << W_ZY), false, wi);
-  if (sx) ex = ex + (((k >= 1 ? wk[W_YX] : T(0)) - w[W_YX]) + ((j >= 1 ? wj[W_ZX] : T(0)) - w[W_ZX]));
-  if (sy) ey = ey + (((k >= 1 ? wk[W_XY] : T(0)) - w[W_XY]) + ((i >= 1 ? wi[W_ZY] : T(0)) - w[W_ZY]));
+  if (sx) ex = ex + (((k >= 1 ? wk_yx : T(0)) - w[W_YX]) + ((j >= 1 ? wj[W_ZX] : T(0)) - w[W_ZX]));
+  if (sy) ey = ey + (((k >= 1 ? wk_xy : T(0)) - w[W_XY]) + ((i >= 1 ? wi[W_ZY] : T(0)) - w[W_ZY]));
   if (sz) ez = ez + (((j >= 1 ? wj[W_XZ] : T(0)) - w[W_XZ]) + ((i >= 1 ? wi[W_YZ] : T(0)) - w[W_YZ]));
   // objective: dJ/dE = 2 Re(conj(F) w_t) on the monitor plane
   if (a.inject) {
@@ -1968,32 +1994,43 @@ __global__ void __launch_bounds__(256) k_adj_e(Grid<T> g, Adj<T> a, long long st
 }
 
 // per H cell: the pending H scaling, then the gather of the E-side weights of
-// the cell and of its k+1, j+1, i+1 neighbours
+// the cell and of its k+1, j+1, i+1 neighbours; t is the thread's index in
+// the launch
 template <typename T>
-__device__ __forceinline__ void adj_h_body(const Grid<T> &g, const Adj<T> &a, int par, long long c) {
+__device__ __forceinline__ void adj_h_body(const Grid<T> &g, const Adj<T> &a, int par, long long t) {
+  const int lane = (int)(t & 63);
+  const bool halo = lane == 63;  // the k + 1 cell of lane 62's cell
+  const long long c = (t >> 6) * ADJ_RUN + lane;
   const int i = (int)(c / g.si), r = (int)(c - (long long)i * g.si), j = r / (int)g.sj, k = r - j * (int)g.sj;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx || j > ny || k > nz) return;
   // inside the Hx (nx+1, ny, nz), Hy (nx, ny+1, nz), Hz (nx, ny, nz+1) shapes
   const bool sx = j < ny && k < nz, sy = i < nx && k < nz, sz = i < nx && j < ny;
-  const long long ck = c + 1, cj = c + g.sj, ci = c + g.si;
+  const long long cj = c + g.sj, ci = c + g.si;
+  // gather: transpose of the E curl differences (backward differences of H);
+  // the halo lane forms the two weights the cell below takes, and no psi
+  T v[6], vj[6], vi[6];
+  if (halo)
+    adj_v(g, a, par, c, i, j, k, a.Ea[0][c], a.Ea[1][c], T(0), (1u << W_XY) | (1u << W_YX), false, v);
+  else
+    adj_v(g, a, par, c, i, j, k, a.Ea[0][c], a.Ea[1][c], a.Ea[2][c], 0u, true, v);
+  // the k + 1 cell's weights, formed by the lane above
+  const T vk_xy = from_above(v[W_XY], T(0)), vk_yx = from_above(v[W_YX], T(0));
+  if (halo) return;
   const bool yee = i < nx && j < ny && k < nz;
   const T *self = g.self[1] + 4 * (int)g.mat[c];
   T hx = a.Ha[0][c], hy = a.Ha[1][c], hz = a.Ha[2][c];
   hx = yee ? self[0] * hx : hx;
   hy = yee ? self[1] * hy : hy;
   hz = yee ? self[2] * hz : hz;
-  // gather: transpose of the E curl differences (backward differences of H)
-  T v[6], vk[6], vj[6], vi[6];
-  adj_v(g, a, par, c, i, j, k, a.Ea[0][c], a.Ea[1][c], a.Ea[2][c], 0u, true, v);
-  if (k < nz && (sx || sy))
-    adj_v(g, a, par, ck, i, j, k + 1, a.Ea[0][ck], a.Ea[1][ck], T(0), (1u << W_XY) | (1u << W_YX), false, vk);
   if (j < ny && (sx || sz))
     adj_v(g, a, par, cj, i, j + 1, k, a.Ea[0][cj], T(0), a.Ea[2][cj], (1u << W_XZ) | (1u << W_ZX), false, vj);
   if (i < nx && (sy || sz))
     adj_v(g, a, par, ci, i + 1, j, k, T(0), a.Ea[1][ci], a.Ea[2][ci], (1u << W_YZ) | (1u << W_ZY), false, vi);
-  if (sx) hx = hx + ((v[W_YX] - vk[W_YX]) + (v[W_ZX] - vj[W_ZX]));
-  if (sy) hy = hy + ((v[W_XY] - vk[W_XY]) + (v[W_ZY] - vi[W_ZY]));
+  // vk: zero where k = nz (no cell above; the term is then unused)
+  const T vk_yx_ = k < nz ? vk_yx : T(0), vk_xy_ = k < nz ? vk_xy : T(0);
+  if (sx) hx = hx + ((v[W_YX] - vk_yx_) + (v[W_ZX] - vj[W_ZX]));
+  if (sy) hy = hy + ((v[W_XY] - vk_xy_) + (v[W_ZY] - vi[W_ZY]));
   if (sz) hz = hz + ((v[W_XZ] - vj[W_XZ]) + (v[W_YZ] - vi[W_YZ]));
   a.Ha[0][c] = hx;
   a.Ha[1][c] = hy;
