@@ -65,6 +65,27 @@ def test_pec_sphere_fp64_bitwise(oracle_threads):
     assert np.array_equal(ag, ao)
 
 
+@pytest.mark.parametrize("nf_gap", [-2, -1, 0])
+def test_nffft_surface_against_tfsf_planes(nf_gap, oracle_threads):
+    """The NF-FF z faces are sampled from records the H kernel leaves unless
+    the TFSF face kernel corrects H on a record plane: an NF-FF box two cells
+    outside the TFSF box takes the records, one cell outside (the TFSF H
+    correction plane) and on the TFSF box itself take the fields.  All three
+    give the oracle's accumulators bit for bit."""
+    dl, pml, gap = 5e-3, 5, 5
+    objs = [S.Object("pec", S.Sphere((0, 0, 0), 5 * dl), S.Material.pec())]
+    sc = S.cube_domain_scene(34, dl, pml, cfl=0.9, objects=objs, tfsf_gap=pml + gap,
+                             nffft_gap=pml + gap + nf_gap, freqs=[S.C_0 / (30 * dl)],
+                             theta=np.linspace(0, math.pi, 7), phi=[0.0], incidence=(0.4, 0.2, 0.0))
+    steps = 60
+    g = _gpu(sc, steps)
+    o = OracleSim(sc, steps)
+    g.step(steps)
+    o.step(steps)
+    assert max(_compare_fields(g, o)) == 0.0
+    assert np.array_equal(g.get_nffft(), o.get_nffft())
+
+
 def test_oblique_dielectric_fp64(oracle_threads):
     dl = 4e-3
     objs = [S.Object("diel", S.Sphere((0.004, -0.002, 0.0), 7 * dl), S.Material("d", 4.0, 1.0, 0.0, 0.0)),
