@@ -1,8 +1,9 @@
 // engine_kernels.hpp -- gfx950 kernels of the FDTD field engine.
 //
-// Layout in HBM: every field component, the material index and the x/y CPML
-// psi arrays use one padded (nx+1) x (ny+1) x pz grid, z fastest, pz a
-// multiple of 32 elements so each z row starts on a 128-byte line.  The six
+// Layout in HBM: every field component and the material index use one padded
+// (nx+1) x (ny+1) x pz grid, z fastest, pz a multiple of 32 elements so each
+// z row starts on a 128-byte line; the x and y CPML psi arrays use the same
+// z rows over their slabs (the two arrays of an axis interleaved).  The six
 // components share one index, so a cell's six values and its material byte
 // are read with one address computation.  A component's reference shape
 // (simulation/simulation_init.cpp:28-35) is a sub-box of this grid; entries
