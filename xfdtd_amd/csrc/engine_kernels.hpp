@@ -1901,15 +1901,13 @@ __device__ __forceinline__ void adj_scale_e(const Grid<T> &g, const Adj<T> &a, l
   ez = oz;
 }
 
-// Cells of the adjoint kernels: each wave of 64 lanes covers ADJ_RUN = 63
-// consecutive cells of the padded layout, and its remaining lane holds the
-// cell next to the wave's end cell along k (k - 1 below lane 1 in the E
-// kernel, k + 1 above lane 62 in the H kernel).  That lane forms only the
-// two weights its neighbour gathers from it, so every cell takes its k
-// neighbour's weights from the adjacent lane by DPP instead of forming them
-// again.  A launch needs adj_blocks(ncell) workgroups of 256.
-constexpr long long ADJ_RUN = 63;
-__host__ __device__ inline long long adj_blocks(long long ncell) { return ((ncell + ADJ_RUN - 1) / ADJ_RUN + 3) / 4; }
+// The adjoint kernels run one thread per cell of the padded layout, 64
+// consecutive cells per wave, so a cell's k neighbour is the adjacent lane:
+// every cell takes the two weights it gathers from its k neighbour from that
+// lane by DPP instead of forming them again; only the wave's end lane (lane 0
+// for k - 1, lane 63 for k + 1) forms its neighbour's itself.  A launch needs
+// adj_blocks(ncell) workgroups of 256.
+__host__ __device__ inline long long adj_blocks(long long ncell) { return (ncell + 255) / 256; }
 
 // per E cell: the pending E scaling, the gather of the H-side weights of the
 // cell and of its k-1, j-1, i-1 neighbours, the objective injection and the
@@ -1917,40 +1915,34 @@ __host__ __device__ inline long long adj_blocks(long long ncell) { return ((ncel
 template <typename T>
 __device__ __forceinline__ void adj_e_body(const Grid<T> &g, const Adj<T> &a, long long step, int slot, int par,
                                            int mode, long long t) {
+  const long long c = t;
   const int lane = (int)(t & 63);
-  const bool halo = lane == 0;  // the k - 1 cell of lane 1's cell
-  const long long c = (t >> 6) * ADJ_RUN + lane - 1;
-  if (c < 0) return;
   const int i = (int)(c / g.si), r = (int)(c - (long long)i * g.si), j = r / (int)g.sj, k = r - j * (int)g.sj;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx || j > ny || k > nz) return;
-  T ex = T(0), ey = T(0), ez = T(0);
-  if (!halo) {
-    ex = a.Ea[0][c];
-    ey = a.Ea[1][c];
-    ez = a.Ea[2][c];
-    if (mode != ADJ_FIRST) adj_scale_e(g, a, c, i, j, k, ex, ey, ez);
-    if (mode == ADJ_FINAL) {
-      a.Ea[0][c] = ex;
-      a.Ea[1][c] = ey;
-      a.Ea[2][c] = ez;
-      return;
-    }
-  } else if (mode == ADJ_FINAL) {
+  T ex = a.Ea[0][c], ey = a.Ea[1][c], ez = a.Ea[2][c];
+  if (mode != ADJ_FIRST) adj_scale_e(g, a, c, i, j, k, ex, ey, ez);
+  if (mode == ADJ_FINAL) {
+    a.Ea[0][c] = ex;
+    a.Ea[1][c] = ey;
+    a.Ea[2][c] = ez;
     return;
   }
   const long long cj = c - g.sj, ci = c - g.si;
   const bool sx = i < nx, sy = j < ny, sz = k < nz;  // inside Ex / Ey / Ez shapes
-  // gather: transpose of the H curl differences (forward differences of E);
-  // the halo lane forms the two weights the cell above takes, and no psi
+  // gather: transpose of the H curl differences (forward differences of E)
   T w[6], wj[6], wi[6];
-  if (halo)
-    adj_w(g, a, par, c, i, j, k, a.Ha[0][c], a.Ha[1][c], T(0), (1u << W_XY) | (1u << W_YX), false, w);
-  else
-    adj_w(g, a, par, c, i, j, k, a.Ha[0][c], a.Ha[1][c], a.Ha[2][c], 0u, true, w);
-  // the k - 1 cell's weights, formed by the lane below
-  const T wk_xy = from_below(w[W_XY], T(0)), wk_yx = from_below(w[W_YX], T(0));
-  if (halo) return;
+  adj_w(g, a, par, c, i, j, k, a.Ha[0][c], a.Ha[1][c], a.Ha[2][c], 0u, true, w);
+  // the k - 1 cell's weights: from the lane below (whose cell it is when
+  // k >= 1), formed here by lane 0
+  T wk_xy = from_below(w[W_XY], T(0)), wk_yx = from_below(w[W_YX], T(0));
+  if (lane == 0 && k >= 1 && (sx || sy)) {
+    T wk[6];
+    adj_w(g, a, par, c - 1, i, j, k - 1, a.Ha[0][c - 1], a.Ha[1][c - 1], T(0), (1u << W_XY) | (1u << W_YX), false,
+          wk);
+    wk_xy = wk[W_XY];
+    wk_yx = wk[W_YX];
+  }
   if (j >= 1 && (sx || sz))
     adj_w(g, a, par, cj, i, j - 1, k, a.Ha[0][cj], T(0), a.Ha[2][cj], (1u << W_XZ) | (1u << W_ZX), false, wj);
   if (i >= 1 && (sy || sz))
@@ -1999,25 +1991,27 @@ __global__ void __launch_bounds__(256) k_adj_e(Grid<T> g, Adj<T> a, long long st
 // the launch
 template <typename T>
 __device__ __forceinline__ void adj_h_body(const Grid<T> &g, const Adj<T> &a, int par, long long t) {
+  const long long c = t;
   const int lane = (int)(t & 63);
-  const bool halo = lane == 63;  // the k + 1 cell of lane 62's cell
-  const long long c = (t >> 6) * ADJ_RUN + lane;
   const int i = (int)(c / g.si), r = (int)(c - (long long)i * g.si), j = r / (int)g.sj, k = r - j * (int)g.sj;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx || j > ny || k > nz) return;
   // inside the Hx (nx+1, ny, nz), Hy (nx, ny+1, nz), Hz (nx, ny, nz+1) shapes
   const bool sx = j < ny && k < nz, sy = i < nx && k < nz, sz = i < nx && j < ny;
   const long long cj = c + g.sj, ci = c + g.si;
-  // gather: transpose of the E curl differences (backward differences of H);
-  // the halo lane forms the two weights the cell below takes, and no psi
+  // gather: transpose of the E curl differences (backward differences of H)
   T v[6], vj[6], vi[6];
-  if (halo)
-    adj_v(g, a, par, c, i, j, k, a.Ea[0][c], a.Ea[1][c], T(0), (1u << W_XY) | (1u << W_YX), false, v);
-  else
-    adj_v(g, a, par, c, i, j, k, a.Ea[0][c], a.Ea[1][c], a.Ea[2][c], 0u, true, v);
-  // the k + 1 cell's weights, formed by the lane above
-  const T vk_xy = from_above(v[W_XY], T(0)), vk_yx = from_above(v[W_YX], T(0));
-  if (halo) return;
+  adj_v(g, a, par, c, i, j, k, a.Ea[0][c], a.Ea[1][c], a.Ea[2][c], 0u, true, v);
+  // the k + 1 cell's weights: from the lane above (whose cell it is when
+  // k < nz), formed here by lane 63
+  T vk_xy = from_above(v[W_XY], T(0)), vk_yx = from_above(v[W_YX], T(0));
+  if (lane == 63 && k < nz && (sx || sy)) {
+    T vk[6];
+    adj_v(g, a, par, c + 1, i, j, k + 1, a.Ea[0][c + 1], a.Ea[1][c + 1], T(0), (1u << W_XY) | (1u << W_YX), false,
+          vk);
+    vk_xy = vk[W_XY];
+    vk_yx = vk[W_YX];
+  }
   const bool yee = i < nx && j < ny && k < nz;
   const T *self = g.self[1] + 4 * (int)g.mat[c];
   T hx = a.Ha[0][c], hy = a.Ha[1][c], hz = a.Ha[2][c];
