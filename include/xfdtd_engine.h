@@ -264,7 +264,8 @@ xf_status xf_engine_set_timing(xf_engine *e, int enabled);
    XF_TILE_XCD_RUNS gives each XCD one contiguous run of tiles instead of
    handing them out in workgroup-id order, XF_TILE_XCD_COLS keeps all k tiles
    of a (j, i) column on one XCD.  Every geometry gives identical results.
-   Default: 64, 4, 8, 0. */
+   Default: 64, 4, 8, 0; 256, 0, 4, 0 for grids of at most 81^3 nodes and
+   64, 0, 4, 0 for 2-D grids. */
 enum { XF_TILE_GLOBAL_TABLES = 2, XF_TILE_X_FASTEST = 4, XF_TILE_XCD_RUNS = 8, XF_TILE_XCD_COLS = 16 };
 xf_status xf_engine_set_tiling(xf_engine *e, int tz, int ty, int tx, int flags);
 /* Time-step scheme (no counterpart in the reference, whose loop runs

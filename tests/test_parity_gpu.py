@@ -79,6 +79,7 @@ def test_nffft_surface_against_tfsf_planes(nf_gap, oracle_threads):
                              theta=np.linspace(0, math.pi, 7), phi=[0.0], incidence=(0.4, 0.2, 0.0))
     steps = 60
     g = _gpu(sc, steps)
+    g.set_tiling(64, 4, 8, 0)  # the records need whole 64-lane k runs
     o = OracleSim(sc, steps)
     g.step(steps)
     o.step(steps)
