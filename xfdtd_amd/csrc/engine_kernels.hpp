@@ -1909,6 +1909,17 @@ __device__ __forceinline__ void adj_scale_e(const Grid<T> &g, const Adj<T> &a, l
 // adj_blocks(ncell) workgroups of 256.
 __host__ __device__ inline long long adj_blocks(long long ncell) { return (ncell + 255) / 256; }
 
+// Workgroup ids go round-robin over the 8 XCDs, each with its own L2.  The
+// batched adjoint kernels hand each XCD one contiguous run of blocks instead,
+// so the cells whose j + 1 / i + 1 (j - 1 / i - 1) neighbours a block reads
+// are read again, as their own, by a block of the same XCD shortly after (or
+// before).  (A single engine's adjoint at 128^3 keeps the plain order: there
+// the whole state sits in the Infinity Cache and the runs were slower.)
+__device__ __forceinline__ long long adj_block(long long b, long long n) {
+  const long long q = n >> 3, r = n & 7, xcd = b & 7, local = b >> 3;
+  return xcd * q + (xcd < r ? xcd : r) + local;
+}
+
 // per E cell: the pending E scaling, the gather of the H-side weights of the
 // cell and of its k-1, j-1, i-1 neighbours, the objective injection and the
 // gradient; t is the thread's index in the launch
@@ -2118,13 +2129,13 @@ __global__ void kb_design_snap(const InstAdj<T> *__restrict__ in, const int2 *__
 template <typename T>
 __global__ void __launch_bounds__(256) kb_adj_e(const InstAdj<T> *__restrict__ in, const int2 *__restrict__ map,
                                                 long long step, int slot, int par) {
-  const int2 w = map[blockIdx.x];
+  const int2 w = map[adj_block(blockIdx.x, gridDim.x)];
   adj_e_body<T>(in[w.x].g, in[w.x].a, step, slot, par, ADJ_STEP, (long long)w.y * 256 + threadIdx.x);
 }
 template <typename T>
 __global__ void __launch_bounds__(256) kb_adj_h(const InstAdj<T> *__restrict__ in, const int2 *__restrict__ map,
                                                 int par) {
-  const int2 w = map[blockIdx.x];
+  const int2 w = map[adj_block(blockIdx.x, gridDim.x)];
   adj_h_body<T>(in[w.x].g, in[w.x].a, par, (long long)w.y * 256 + threadIdx.x);
 }
 
