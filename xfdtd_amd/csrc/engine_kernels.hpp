@@ -1992,10 +1992,11 @@ __device__ __forceinline__ void adj_e_body(const Grid<T> &g, const Adj<T> &a, lo
   a.Ea[1][c] = ey;
   a.Ea[2][c] = ez;
 }
-// FP32: the waves-per-EU hint of 8 (the FP32 kernels fit 8 waves either
-// way) schedules them about 5% faster; FP64 would spill under it
+// Waves-per-EU hints at the occupancy the kernels reach anyway (8 in FP32,
+// 7 in FP64) schedule them faster: about 5% in FP32, 1% in FP64 (a hint of 8
+// would make the FP64 kernels spill)
 template <typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 1)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 7)))
 k_adj_e(Grid<T> g, Adj<T> a, long long step, int slot, int par, int mode) {
   adj_e_body<T>(g, a, step, slot, par, mode, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
@@ -2046,7 +2047,7 @@ __device__ __forceinline__ void adj_h_body(const Grid<T> &g, const Adj<T> &a, in
   a.Ha[2][c] = hz;
 }
 template <typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 1)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 7)))
 k_adj_h(Grid<T> g, Adj<T> a, int par) {
   adj_h_body<T>(g, a, par, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
@@ -2131,13 +2132,13 @@ __global__ void kb_design_snap(const InstAdj<T> *__restrict__ in, const int2 *__
   design_snap_body<T>(in[w.x].g, in[w.x].hist, slot, w.y * 256 + threadIdx.x);
 }
 template <typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 1)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 7)))
 kb_adj_e(const InstAdj<T> *__restrict__ in, const int2 *__restrict__ map, long long step, int slot, int par) {
   const int2 w = map[adj_block(blockIdx.x, gridDim.x)];
   adj_e_body<T>(in[w.x].g, in[w.x].a, step, slot, par, ADJ_STEP, (long long)w.y * 256 + threadIdx.x);
 }
 template <typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 1)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 7)))
 kb_adj_h(const InstAdj<T> *__restrict__ in, const int2 *__restrict__ map, int par) {
   const int2 w = map[adj_block(blockIdx.x, gridDim.x)];
   adj_h_body<T>(in[w.x].g, in[w.x].a, par, (long long)w.y * 256 + threadIdx.x);
