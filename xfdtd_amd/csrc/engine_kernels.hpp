@@ -1931,13 +1931,14 @@ template <typename T>
 __device__ __forceinline__ void adj_e_body(const Grid<T> &g, const Adj<T> &a, long long step, int slot, int par,
                                            int mode, long long t) {
   const long long c = t;
-  const int lane = (int)(t & 63);
+  const int lane = (int)(t & 63), wave = (int)((t >> 6) & 3);
   const int i = (int)(c / g.si), r = (int)(c - (long long)i * g.si), j = r / (int)g.sj, k = r - j * (int)g.sj;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  if (i > nx || j > ny || k > nz) return;
-  T ex = a.Ea[0][c], ey = a.Ea[1][c], ez = a.Ea[2][c];
-  if (mode != ADJ_FIRST) adj_scale_e(g, a, c, i, j, k, ex, ey, ez);
+  const bool valid = i <= nx && j <= ny && k <= nz;
   if (mode == ADJ_FINAL) {
+    if (!valid) return;
+    T ex = a.Ea[0][c], ey = a.Ea[1][c], ez = a.Ea[2][c];
+    adj_scale_e(g, a, c, i, j, k, ex, ey, ez);
     a.Ea[0][c] = ex;
     a.Ea[1][c] = ey;
     a.Ea[2][c] = ez;
@@ -1946,17 +1947,37 @@ __device__ __forceinline__ void adj_e_body(const Grid<T> &g, const Adj<T> &a, lo
   const long long cj = c - g.sj, ci = c - g.si;
   const bool sx = i < nx, sy = j < ny, sz = k < nz;  // inside Ex / Ey / Ez shapes
   // gather: transpose of the H curl differences (forward differences of E)
-  T w[6], wj[6], wi[6];
-  adj_w(g, a, par, c, i, j, k, a.Ha[0][c], a.Ha[1][c], a.Ha[2][c], 0u, true, w);
+  T ex = T(0), ey = T(0), ez = T(0);
+  T w[6] = {T(0), T(0), T(0), T(0), T(0), T(0)}, wj[6], wi[6];
+  if (valid) {
+    ex = a.Ea[0][c];
+    ey = a.Ea[1][c];
+    ez = a.Ea[2][c];
+    if (mode != ADJ_FIRST) adj_scale_e(g, a, c, i, j, k, ex, ey, ez);
+    adj_w(g, a, par, c, i, j, k, a.Ha[0][c], a.Ha[1][c], a.Ha[2][c], 0u, true, w);
+  }
   // the k - 1 cell's weights: from the lane below (whose cell it is when
-  // k >= 1), formed here by lane 0
+  // k >= 1); lane 0 takes lane 63's of the wave below through LDS, and lane
+  // 0 of a workgroup's first wave forms them
   T wk_xy = from_below(w[W_XY], T(0)), wk_yx = from_below(w[W_YX], T(0));
+  __shared__ P2<T> edge[4];
+  if (lane == 63) {
+    edge[wave].x = w[W_XY];
+    edge[wave].y = w[W_YX];
+  }
+  __syncthreads();
+  if (!valid) return;
   if (lane == 0 && k >= 1 && (sx || sy)) {
-    T wk[6];
-    adj_w(g, a, par, c - 1, i, j, k - 1, a.Ha[0][c - 1], a.Ha[1][c - 1], T(0), (1u << W_XY) | (1u << W_YX), false,
-          wk);
-    wk_xy = wk[W_XY];
-    wk_yx = wk[W_YX];
+    if (wave > 0) {
+      wk_xy = edge[wave - 1].x;
+      wk_yx = edge[wave - 1].y;
+    } else {
+      T wk[6];
+      adj_w(g, a, par, c - 1, i, j, k - 1, a.Ha[0][c - 1], a.Ha[1][c - 1], T(0), (1u << W_XY) | (1u << W_YX), false,
+            wk);
+      wk_xy = wk[W_XY];
+      wk_yx = wk[W_YX];
+    }
   }
   if (j >= 1 && (sx || sz))
     adj_w(g, a, par, cj, i, j - 1, k, a.Ha[0][cj], T(0), a.Ha[2][cj], (1u << W_XZ) | (1u << W_ZX), false, wj);
@@ -2011,25 +2032,38 @@ k_adj_e(Grid<T> g, Adj<T> a, long long step, int slot, int par, int mode) {
 template <typename T>
 __device__ __forceinline__ void adj_h_body(const Grid<T> &g, const Adj<T> &a, int par, long long t) {
   const long long c = t;
-  const int lane = (int)(t & 63);
+  const int lane = (int)(t & 63), wave = (int)((t >> 6) & 3);
   const int i = (int)(c / g.si), r = (int)(c - (long long)i * g.si), j = r / (int)g.sj, k = r - j * (int)g.sj;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  if (i > nx || j > ny || k > nz) return;
+  const bool valid = i <= nx && j <= ny && k <= nz;
   // inside the Hx (nx+1, ny, nz), Hy (nx, ny+1, nz), Hz (nx, ny, nz+1) shapes
   const bool sx = j < ny && k < nz, sy = i < nx && k < nz, sz = i < nx && j < ny;
   const long long cj = c + g.sj, ci = c + g.si;
   // gather: transpose of the E curl differences (backward differences of H)
-  T v[6], vj[6], vi[6];
-  adj_v(g, a, par, c, i, j, k, a.Ea[0][c], a.Ea[1][c], a.Ea[2][c], 0u, true, v);
+  T v[6] = {T(0), T(0), T(0), T(0), T(0), T(0)}, vj[6], vi[6];
+  if (valid) adj_v(g, a, par, c, i, j, k, a.Ea[0][c], a.Ea[1][c], a.Ea[2][c], 0u, true, v);
   // the k + 1 cell's weights: from the lane above (whose cell it is when
-  // k < nz), formed here by lane 63
+  // k < nz); lane 63 takes lane 0's of the wave above through LDS, and lane
+  // 63 of a workgroup's last wave forms them
   T vk_xy = from_above(v[W_XY], T(0)), vk_yx = from_above(v[W_YX], T(0));
+  __shared__ P2<T> edge[4];
+  if (lane == 0) {
+    edge[wave].x = v[W_XY];
+    edge[wave].y = v[W_YX];
+  }
+  __syncthreads();
+  if (!valid) return;
   if (lane == 63 && k < nz && (sx || sy)) {
-    T vk[6];
-    adj_v(g, a, par, c + 1, i, j, k + 1, a.Ea[0][c + 1], a.Ea[1][c + 1], T(0), (1u << W_XY) | (1u << W_YX), false,
-          vk);
-    vk_xy = vk[W_XY];
-    vk_yx = vk[W_YX];
+    if (wave < 3) {
+      vk_xy = edge[wave + 1].x;
+      vk_yx = edge[wave + 1].y;
+    } else {
+      T vk[6];
+      adj_v(g, a, par, c + 1, i, j, k + 1, a.Ea[0][c + 1], a.Ea[1][c + 1], T(0), (1u << W_XY) | (1u << W_YX), false,
+            vk);
+      vk_xy = vk[W_XY];
+      vk_yx = vk[W_YX];
+    }
   }
   const bool yee = i < nx && j < ny && k < nz;
   const T *self = g.self[1] + 4 * (int)g.mat[c];
