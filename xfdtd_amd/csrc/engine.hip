@@ -172,6 +172,10 @@ struct xf_engine {
   // (NF-FF, DFT plane, broadband series, probes, line monitors) are skipped;
   // their state is restored from the checkpoint at the end of the run
   bool replay = false;
+  // a replay step's curl launches cover only the tiles that meet the cell
+  // window [win[0], win[1]) x [win[2], win[3]) x [win[4], win[5]) (i, j, k)
+  bool win_on = false;
+  int win[6] = {0, 0, 0, 0, 0, 0};
   // gradient: run each segment's adjoint sweep on a second stream beside the
   // replay of the segment before it (-1: in FP32 only, where the kernels are
   // latency-bound at the sizes a gradient takes; 0: never; 1: always)
@@ -1134,7 +1138,29 @@ static Tiling make_tiling(const xf_engine *e) {
   if (tl.xcd_cols) tl.nblocks = 8 * tl.ntz * ((tl.nty * tl.ntx + 7) / 8);
   tl.x_fastest = (e->tile_flags & XF_TILE_X_FASTEST) != 0;
   tl.xcd_runs = (e->tile_flags & XF_TILE_XCD_RUNS) != 0;
+  tl.oz = tl.oy = tl.ox = 0;
   return tl;
+}
+
+// Narrow a tiling to the box of tiles that meets the engine's cell window
+// (flat tiles: the runs that meet its j rows, every k).  Results inside the
+// window do not depend on the tiles, so only the cells outside it change.
+static void window_tiling(const xf_engine *e, Tiling &tl) {
+  const int *w = e->win;
+  auto span = [](long long lo, long long hi, int size, int count, int &first, int &n) {
+    const long long a = std::max(0LL, lo / size), b = std::min((long long)count, (hi + size - 1) / size);
+    first = (int)a;
+    n = (int)std::max(1LL, b - a);
+  };
+  span(w[0], w[1], tl.tx, tl.ntx, tl.ox, tl.ntx);
+  if (tl.ty == 0) {
+    tl.oy = 0;
+    span((long long)w[2] * e->pz, (long long)w[3] * e->pz, tl.tz, tl.ntz, tl.oz, tl.ntz);
+  } else {
+    span(w[2], w[3], tl.ty, tl.nty, tl.oy, tl.nty);
+    span(w[4], w[5], tl.tz, tl.ntz, tl.oz, tl.ntz);
+  }
+  tl.nblocks = tl.xcd_cols ? 8 * tl.ntz * ((tl.nty * tl.ntx + 7) / 8) : tl.ntz * tl.nty * tl.ntx;
 }
 
 static int tiling_threads(const Tiling &tl) { return tl.ty == 0 ? tl.tz : tl.tz * tl.ty; }
@@ -1442,7 +1468,8 @@ static xf_status run_steps(xf_engine *e, long long n) {
   const int R = sweep_rows(e);
   if (fused && R != 1 && R != 2 && R != 4)
     FAILS(XF_EINVAL, "fused step with %d rows per thread (1, 2 or 4)", R);
-  const Tiling tl = make_tiling(e);
+  Tiling tl = make_tiling(e);
+  if (e->replay && e->win_on && !fused) window_tiling(e, tl);
   dim3 blk(tiling_threads(tl));
   dim3 grd(tl.nblocks);
   // TFSF z faces in the curl kernels (split scheme)
@@ -2416,10 +2443,25 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
       e->cur = r0;
       hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hr, 0);
       e->replay = true;
+      e->win_on = true;
       for (long long l = 0; l < rn && s == XF_OK; ++l) {
+        // Only the design box's E after each step is kept, and a step moves
+        // information by at most one cell along each axis (E reads H at c
+        // and c - 1, H reads E at c and c + 1), so step l only has to be
+        // right within rn - 1 - l cells of the box; the window keeps one
+        // cell more.  The cells left outside go stale and are never read
+        // by a cell inside a later window; the next replay starts from a
+        // checkpoint of the whole state.
+        const int m = (int)std::min<long long>(rn - l, 1 << 20);
+        const int hi[3] = {e->nx + 1, e->ny + 1, e->nz + 1};
+        for (int d = 0; d < 3; ++d) {
+          e->win[2 * d] = std::max(0, e->db[2 * d] - m);
+          e->win[2 * d + 1] = std::min(hi[d], e->db[2 * d + 1] + m);
+        }
         if ((s = run_steps<T>(e, 1)) == XF_OK)
           hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hr, (int)(l + 1));
       }
+      e->win_on = false;
       e->replay = false;
       if (s != XF_OK) return s;
       HIPCHK(hipEventRecord(ready[r], e->stream));

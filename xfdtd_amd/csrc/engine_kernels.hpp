@@ -148,6 +148,7 @@ struct Tiling {
   int x_fastest;            // tile order: x chunks first (else z, y, x)
   int xcd_runs;             // one contiguous run of tile ids per XCD
   int xcd_cols;             // all k tiles of a (j, i) column on one XCD
+  int oz, oy, ox;           // first tile along k, j, i (a launch over a box of tiles)
 };
 
 // tile of workgroup b; false for a padding workgroup of the column order
@@ -749,6 +750,9 @@ __device__ __forceinline__ void update_e_body(const Grid<T> &g, const Tiling &tl
   extern __shared__ __align__(16) unsigned char lds_raw[];
   int bz, by, bx;
   if (!tile_of_block(tl, block, bz, by, bx)) return;
+  bz += tl.oz;
+  by += tl.oy;
+  bx += tl.ox;
   const Cell cell = thread_cell(tl, bz, by, g.pz);
   const int k = cell.k, j = cell.j;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -981,6 +985,9 @@ __device__ __forceinline__ void update_h_body(const Grid<T> &g, const Tiling &tl
   extern __shared__ __align__(16) unsigned char lds_raw[];
   int bz, by, bx;
   if (!tile_of_block(tl, block, bz, by, bx)) return;
+  bz += tl.oz;
+  by += tl.oy;
+  bx += tl.ox;
   const Cell cell = thread_cell(tl, bz, by, g.pz);
   const int k = cell.k, j = cell.j;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
