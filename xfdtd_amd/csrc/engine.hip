@@ -2365,7 +2365,7 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
   double *g = (double *)B.alloc((size_t)e->nd * sizeof(double));
   if (!hist[0] || (nseg > 1 && !hist[1]) || !g) FAILS(XF_ENOMEM, "adjoint buffers");
   const Grid<T> gr = make_grid<T>(e);
-  const unsigned nb = (unsigned)adj_blocks(e->ncell), nsnap = (unsigned)((e->nd + 255) / 256);
+  const unsigned nsnap = (unsigned)((e->nd + 255) / 256);
   // forward run with a checkpoint at the start of every segment but the
   // last, whose design-box E fields are kept as it runs (it is never
   // recomputed), and a checkpoint at the end
@@ -2431,6 +2431,21 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
   hipEvent_t *ready = side.ev.data(), *done = side.ev.data() + nseg;  // replay of s ready, sweep of s done
   HIPCHK(hipEventRecord(ready[nseg - 1], e->stream));
   int par = 0;  // which psi-adjoint copy holds the current values
+  // The adjoints start at zero and are driven only on the objective plane.
+  // Along i, k_adj_e moves them up one plane per reverse step (it reads H
+  // adjoints at i and i - 1) and k_adj_h down one (E adjoints at i and
+  // i + 1); psi and Drude adjoints are per cell.  Before reverse step q
+  // (counted from the run's end) they are therefore zero beyond q planes of
+  // the plane's i extent [pi0, pi1): step q launches only the workgroups
+  // over the planes within q + 2 of it, and the cells beyond, zero in every
+  // array, are left as they are (a full launch would compute zeros there).
+  const long long pi0 = dpl.axis == 0 ? dpl.index : dpl.lo1, pi1 = dpl.axis == 0 ? dpl.index + 1 : dpl.hi1;
+  long long q = 0;
+  auto rev_window = [&](long long &b0) {
+    const long long lo = std::max(0LL, pi0 - (q + 2)), hi = std::min((long long)e->nx + 1, pi1 + q + 2);
+    b0 = lo * e->si / 256;
+    return (unsigned)(adj_blocks(hi * e->si) - b0);
+  };
   for (int sgi = nseg - 1; sgi >= 0; --sgi) {
     // replay segment sgi - 1 into the other history buffer, once the sweep
     // of segment sgi + 1 (its last user) is done
@@ -2470,9 +2485,11 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
     const long long t0 = (long long)sgi * K, n = std::min<long long>(K, T_ - t0);
     HIPCHK(hipStreamWaitEvent(as, ready[sgi], 0));
     a.hist = hist[sgi & 1];
-    for (long long l = n - 1; l >= 0; --l) {
-      hipLaunchKernelGGL(k_adj_e<T>, dim3(nb), dim3(256), 0, as, gr, a, t0 + l, (int)l, par, (int)ADJ_STEP);
-      hipLaunchKernelGGL(k_adj_h<T>, dim3(nb), dim3(256), 0, as, gr, a, par);
+    for (long long l = n - 1; l >= 0; --l, ++q) {
+      long long b0 = 0;
+      const unsigned nw = rev_window(b0);
+      hipLaunchKernelGGL(k_adj_e<T>, dim3(nw), dim3(256), 0, as, gr, a, t0 + l, (int)l, par, (int)ADJ_STEP, b0);
+      hipLaunchKernelGGL(k_adj_h<T>, dim3(nw), dim3(256), 0, as, gr, a, par, b0);
       par ^= 1;
     }
     HIPCHK(hipGetLastError());
@@ -2547,11 +2564,12 @@ static xf_status adjoint_run(xf_engine *e, long long nsteps, double *fields) {
   int par = 0;
   for (long long t = 0; t < nsteps; ++t) {
     hipLaunchKernelGGL(k_adj_e<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, 0LL, 0, par,
-                       (int)(t == 0 ? ADJ_FIRST : ADJ_STEP));
-    hipLaunchKernelGGL(k_adj_h<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, par);
+                       (int)(t == 0 ? ADJ_FIRST : ADJ_STEP), 0LL);
+    hipLaunchKernelGGL(k_adj_h<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, par, 0LL);
     par ^= 1;
   }
-  if (nsteps > 0) hipLaunchKernelGGL(k_adj_e<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, 0LL, 0, par, (int)ADJ_FINAL);
+  if (nsteps > 0)
+    hipLaunchKernelGGL(k_adj_e<T>, dim3(nb), dim3(256), 0, e->stream, gr, a, 0LL, 0, par, (int)ADJ_FINAL, 0LL);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(e->stream));
   off = 0;

@@ -2029,8 +2029,8 @@ __device__ __forceinline__ void adj_e_body(const Grid<T> &g, const Adj<T> &a, lo
 // would make the FP64 kernels spill)
 template <typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 7)))
-k_adj_e(Grid<T> g, Adj<T> a, long long step, int slot, int par, int mode) {
-  adj_e_body<T>(g, a, step, slot, par, mode, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+k_adj_e(Grid<T> g, Adj<T> a, long long step, int slot, int par, int mode, long long b0) {
+  adj_e_body<T>(g, a, step, slot, par, mode, (b0 + (long long)blockIdx.x) * blockDim.x + threadIdx.x);
 }
 
 // per H cell: the pending H scaling, then the gather of the E-side weights of
@@ -2093,8 +2093,8 @@ __device__ __forceinline__ void adj_h_body(const Grid<T> &g, const Adj<T> &a, in
 }
 template <typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 7)))
-k_adj_h(Grid<T> g, Adj<T> a, int par) {
-  adj_h_body<T>(g, a, par, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+k_adj_h(Grid<T> g, Adj<T> a, int par, long long b0) {
+  adj_h_body<T>(g, a, par, (b0 + (long long)blockIdx.x) * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
