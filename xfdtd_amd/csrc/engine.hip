@@ -2467,7 +2467,7 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
         // cell more.  The cells left outside go stale and are never read
         // by a cell inside a later window; the next replay starts from a
         // checkpoint of the whole state.
-        const int m = (int)std::min<long long>(rn - l, 1 << 20);
+        const int m = (int)(rn - l);  // rn <= K, an int
         const int hi[3] = {e->nx + 1, e->ny + 1, e->nz + 1};
         for (int d = 0; d < 3; ++d) {
           e->win[2 * d] = std::max(0, e->db[2 * d] - m);
