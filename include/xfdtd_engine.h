@@ -365,7 +365,10 @@ xf_status xf_engine_restore(xf_engine *e, int slot);
    (C order).  Runs the forward loop from step 0 with a checkpoint every
    `checkpoint_every` steps, then the reverse loop segment by segment: each
    segment is recomputed from its checkpoint, keeping the design-box E
-   fields, and the adjoint kernels step back through it; the last segment's
+   fields (each recomputed step covers only the tiles that can still reach
+   the design box by the segment's end), and the adjoint kernels step back
+   through it (over the planes the adjoint can have reached from the
+   objective plane); the last segment's
    design fields are kept during the forward run, so an interval >= the run
    length recomputes nothing.  checkpoint_every = 0 chooses: the whole run
    when its design fields fit in a quarter of the free device memory (at
