@@ -1163,6 +1163,39 @@ static void window_tiling(const xf_engine *e, Tiling &tl) {
   tl.nblocks = tl.xcd_cols ? 8 * tl.ntz * ((tl.nty * tl.ntx + 7) / 8) : tl.ntz * tl.nty * tl.ntx;
 }
 
+// The cells a replayed gradient step has to get right, one to spare: the
+// design box grown by m cells along each axis, clipped to the grid.
+static void replay_window(const xf_engine *e, int m, int w[6]) {
+  const int hi[3] = {e->nx + 1, e->ny + 1, e->nz + 1};
+  for (int d = 0; d < 3; ++d) {
+    w[2 * d] = std::max(0, e->db[2 * d] - m);
+    w[2 * d + 1] = std::min(hi[d], e->db[2 * d + 1] + m);
+  }
+}
+
+// the smallest m whose replay window is the whole grid
+static int replay_window_full(const xf_engine *e) {
+  const int hi[3] = {e->nx + 1, e->ny + 1, e->nz + 1};
+  int m = 0;
+  for (int d = 0; d < 3; ++d) m = std::max(m, std::max(e->db[2 * d], hi[d] - e->db[2 * d + 1]));
+  return m;
+}
+
+// Whether workgroup b of a curl launch over tiling tl (no tile origin)
+// covers a cell of the window w; flat tiles count by their j rows.
+static bool block_meets_window(const xf_engine *e, const Tiling &tl, int b, const int w[6]) {
+  int bz, by, bx;
+  if (!tile_of_block(tl, b, bz, by, bx)) return false;
+  const long long i0 = (long long)bx * tl.tx;
+  if (i0 >= w[1] || i0 + tl.tx <= w[0]) return false;
+  if (tl.ty == 0) {
+    const long long p0 = (long long)bz * tl.tz, j0 = p0 / e->pz, j1 = (p0 + tl.tz - 1) / e->pz;
+    return j0 < w[3] && j1 >= w[2];
+  }
+  const long long j0 = (long long)by * tl.ty, k0 = (long long)bz * tl.tz;
+  return j0 < w[3] && j0 + tl.ty > w[2] && k0 < w[5] && k0 + tl.tz > w[4];
+}
+
 static int tiling_threads(const Tiling &tl) { return tl.ty == 0 ? tl.tz : tl.tz * tl.ty; }
 
 static bool has_non_drude(const xf_engine *e) { return e->ade_non_drude; }
@@ -1843,6 +1876,7 @@ struct xf_batch {
   size_t lds = 0;
   bool use_lds = true;
   bool special = false;  // some member has a design box or Drude cells
+  std::vector<Tiling> tls;  // each member's curl tiling, as uploaded
   std::vector<void *> owned;
 };
 
@@ -1859,6 +1893,7 @@ static xf_status batch_build(xf_batch *b) {
   const int n = (int)b->e.size();
   std::vector<Inst<T>> in(n);
   std::vector<int2> mc, mte, mth, mifa, mnf, mnff, mdft;
+  b->tls.clear();
   b->lds = 0;
   b->use_lds = true;
   b->special = false;
@@ -1869,6 +1904,7 @@ static xf_status batch_build(xf_batch *b) {
     std::memset(&a, 0, sizeof a);
     a.g = make_grid<T>(e);
     a.tl = make_tiling(e);
+    b->tls.push_back(a.tl);
     const size_t lds = lds_table_bytes<T>(e->nmat, a.tl);
     b->lds = std::max(b->lds, lds);
     if (lds > 32768 || (e->tile_flags & XF_TILE_GLOBAL_TABLES)) b->use_lds = false;
@@ -1977,9 +2013,14 @@ extern "C" xf_status xf_batch_create(xf_engine *const *engines, int n, xf_batch 
 
 // replay: re-running a segment for the adjoint, without the observers
 template <typename T>
-static xf_status batch_run(xf_batch *b, long long n, bool replay = false) {
+// map_curl / n_curl: the curl kernels' workgroups (default: every tile of
+// every member; a replay passes the tiles of its window)
+static xf_status batch_run(xf_batch *b, long long n, bool replay = false, const int2 *map_curl = nullptr,
+                           int n_curl = 0) {
   const Inst<T> *in = (const Inst<T> *)b->inst;
   const dim3 blk(b->threads);
+  const int2 *mc = map_curl ? map_curl : b->map_curl;
+  const dim3 nc((unsigned)(map_curl ? n_curl : b->n_curl));
   // NF-FF samples go to each member's ring, flushed every NF_RING steps and
   // at the end of the call
   int nf_pending = 0;
@@ -1994,14 +2035,14 @@ static xf_status batch_run(xf_batch *b, long long n, bool replay = false) {
     const long long step = b->e[0]->cur;
     if (b->special) {
       if (b->use_lds)
-        hipLaunchKernelGGL((kb_update_e<T, true, true>), dim3(b->n_curl), blk, b->lds, b->stream, in, b->map_curl);
+        hipLaunchKernelGGL((kb_update_e<T, true, true>), nc, blk, b->lds, b->stream, in, mc);
       else
-        hipLaunchKernelGGL((kb_update_e<T, false, true>), dim3(b->n_curl), blk, 0, b->stream, in, b->map_curl);
+        hipLaunchKernelGGL((kb_update_e<T, false, true>), nc, blk, 0, b->stream, in, mc);
     } else {
       if (b->use_lds)
-        hipLaunchKernelGGL((kb_update_e<T, true, false>), dim3(b->n_curl), blk, b->lds, b->stream, in, b->map_curl);
+        hipLaunchKernelGGL((kb_update_e<T, true, false>), nc, blk, b->lds, b->stream, in, mc);
       else
-        hipLaunchKernelGGL((kb_update_e<T, false, false>), dim3(b->n_curl), blk, 0, b->stream, in, b->map_curl);
+        hipLaunchKernelGGL((kb_update_e<T, false, false>), nc, blk, 0, b->stream, in, mc);
     }
     for (xf_engine *e : b->e)
       for (const SourceDev &sd : e->sources) launch_lumped<T>(e, sd, (T *)e->f[sd.comp], b->stream, step);
@@ -2010,9 +2051,9 @@ static xf_status batch_run(xf_batch *b, long long n, bool replay = false) {
     if (b->n_ifa > 0)
       hipLaunchKernelGGL(kb_tfsf_ifa<T>, dim3(b->n_ifa), dim3(1024), 0, b->stream, in, b->map_ifa, step);
     if (b->use_lds)
-      hipLaunchKernelGGL((kb_update_h<T, true>), dim3(b->n_curl), blk, b->lds, b->stream, in, b->map_curl, step);
+      hipLaunchKernelGGL((kb_update_h<T, true>), nc, blk, b->lds, b->stream, in, mc, step);
     else
-      hipLaunchKernelGGL((kb_update_h<T, false>), dim3(b->n_curl), blk, 0, b->stream, in, b->map_curl, step);
+      hipLaunchKernelGGL((kb_update_h<T, false>), nc, blk, 0, b->stream, in, mc, step);
     if (b->n_th > 0) hipLaunchKernelGGL(kb_tfsf_h<T>, dim3(b->n_th), dim3(256), 0, b->stream, in, b->map_th);
     if (!replay) {
       if (b->n_nf > 0) {
@@ -2467,12 +2508,7 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
         // cell more.  The cells left outside go stale and are never read
         // by a cell inside a later window; the next replay starts from a
         // checkpoint of the whole state.
-        const int m = (int)(rn - l);  // rn <= K, an int
-        const int hi[3] = {e->nx + 1, e->ny + 1, e->nz + 1};
-        for (int d = 0; d < 3; ++d) {
-          e->win[2 * d] = std::max(0, e->db[2 * d] - m);
-          e->win[2 * d + 1] = std::min(hi[d], e->db[2 * d + 1] + m);
-        }
+        replay_window(e, (int)(rn - l), e->win);  // rn <= K, an int
         if ((s = run_steps<T>(e, 1)) == XF_OK)
           hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hr, (int)(l + 1));
       }
@@ -2683,6 +2719,32 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
     for (const double2 &f : F) Jv += f.x * f.x + f.y * f.y;
     Js[q] = Jv;
   }
+  // The replays' curl launches cover only the tiles within reach of each
+  // member's design box by the segment's end, as a single engine's do:
+  // replay step l of m takes the map of margin m - l, which lists every
+  // member's tiles that meet its box grown by that many cells.  Margins
+  // from mfull on cover every grid and take the batch's own map.
+  int mfull = 0;
+  for (int q = 0; q < n; ++q) mfull = std::max(mfull, replay_window_full(b->e[q]));
+  const int mw = nseg > 1 ? std::min(K, mfull - 1) : 0;  // margins 1 .. mw get a map
+  std::vector<long long> woff((size_t)mw + 2, 0);
+  int2 *d_wmap = nullptr;
+  if (mw > 0) {
+    std::vector<int2> wmap;
+    const std::vector<Tiling> &tls = b->tls;
+    for (int mg = 1; mg <= mw; ++mg) {
+      woff[mg] = (long long)wmap.size();
+      for (int q = 0; q < n; ++q) {
+        int w[6];
+        replay_window(b->e[q], mg, w);
+        for (int x = 0; x < tls[q].nblocks; ++x)
+          if (block_meets_window(b->e[q], tls[q], x, w)) wmap.push_back(make_int2(q, x));
+      }
+    }
+    woff[mw + 1] = (long long)wmap.size();
+    if (!(d_wmap = (int2 *)B.alloc(wmap.size() * sizeof(int2), false))) FAILS(XF_ENOMEM, "replay maps");
+    HIPCHK(hipMemcpy(d_wmap, wmap.data(), wmap.size() * sizeof(int2), hipMemcpyHostToDevice));
+  }
   // reverse, segment by segment
   int par = 0;  // which psi-adjoint copy holds the current values
   for (int sgi = nseg - 1; sgi >= 0; --sgi) {
@@ -2695,7 +2757,12 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
       }
       snap(0);
       for (long long l = 0; l < m; ++l) {
-        if ((s = batch_run<T>(b, 1, true)) != XF_OK) return s;
+        const long long mg = m - l;
+        if (mg <= mw)
+          s = batch_run<T>(b, 1, true, d_wmap + woff[mg], (int)(woff[mg + 1] - woff[mg]));
+        else
+          s = batch_run<T>(b, 1, true);
+        if (s != XF_OK) return s;
         snap((int)(l + 1));
       }
     }

@@ -152,7 +152,9 @@ struct Tiling {
 };
 
 // tile of workgroup b; false for a padding workgroup of the column order
-__device__ __forceinline__ bool tile_of_block(const Tiling &t, int b, int &bz, int &by, int &bx) {
+// (also called on the host, to list the tiles of a batched launch that meet
+// a window)
+__host__ __device__ __forceinline__ bool tile_of_block(const Tiling &t, int b, int &bz, int &by, int &bx) {
   const int q = t.nblocks >> 3, r = t.nblocks & 7;
   const int xcd = b & 7, local = b >> 3;
   if (t.xcd_cols) {
@@ -162,7 +164,7 @@ __device__ __forceinline__ bool tile_of_block(const Tiling &t, int b, int &bz, i
     bx = col / t.nty;
     return bx < t.ntx;
   }
-  const int id = t.xcd_runs ? xcd * q + min(xcd, r) + local : b;
+  const int id = t.xcd_runs ? xcd * q + (xcd < r ? xcd : r) + local : b;
   if (t.x_fastest) {
     bx = id % t.ntx;
     const int rest = id / t.ntx;
