@@ -2745,6 +2745,48 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
     if (!(d_wmap = (int2 *)B.alloc(wmap.size() * sizeof(int2), false))) FAILS(XF_ENOMEM, "replay maps");
     HIPCHK(hipMemcpy(d_wmap, wmap.data(), wmap.size() * sizeof(int2), hipMemcpyHostToDevice));
   }
+  // The reverse steps launch only the workgroups the adjoints can have
+  // reached, as a single engine's do: a member's cell block w can hold a
+  // nonzero adjoint from reverse step enter(w) on, the first step whose
+  // window (the planes within step + 2 of the objective plane's i extent)
+  // meets the block's planes.  A second map lists every member's blocks
+  // sorted by that step, so reverse step r launches its first ready[r]
+  // entries; from step rfull on every block is in and the launches take
+  // the map in its own order.  The kernels remap a launch's workgroups into
+  // contiguous runs per XCD over whatever map and count they get.
+  std::vector<long long> ready;
+  int2 *d_rev = nullptr;
+  {
+    std::vector<std::pair<int, int2>> ent;
+    ent.reserve(mcell.size());
+    for (int q = 0; q < n; ++q) {
+      const xf_engine *e = b->e[q];
+      const DftPlane &d = e->dft;
+      const long long pi0 = d.axis == 0 ? d.index : d.lo1, pi1 = d.axis == 0 ? d.index + 1 : d.hi1;
+      const long long nb = adj_blocks(e->ncell);
+      for (long long w = 0; w < nb; ++w) {
+        const long long ilo = w * 256 / e->si, ihi = std::min((long long)e->nx, (w * 256 + 255) / e->si);
+        const long long r = std::max(0LL, std::max(pi0 - 2 - ihi, ilo - pi1 - 1));
+        ent.push_back({(int)r, make_int2(q, (int)w)});
+      }
+    }
+    std::stable_sort(ent.begin(), ent.end(),
+                     [](const std::pair<int, int2> &x, const std::pair<int, int2> &y) { return x.first < y.first; });
+    const int rfull = ent.empty() ? 0 : ent.back().first;
+    if (rfull > 0) {
+      // ready[r]: the entries with enter <= r
+      ready.assign((size_t)rfull, 0);
+      std::vector<int2> rev(ent.size());
+      for (size_t x = 0; x < ent.size(); ++x) {
+        rev[x] = ent[x].second;
+        if (ent[x].first < rfull) ++ready[ent[x].first];
+      }
+      for (int r = 1; r < rfull; ++r) ready[r] += ready[r - 1];
+      if (!(d_rev = (int2 *)D.alloc(rev.size() * sizeof(int2), false))) FAILS(XF_ENOMEM, "batch adjoint tables");
+      HIPCHK(hipMemcpy(d_rev, rev.data(), rev.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+  }
+  long long rstep = 0;  // reverse steps taken
   // reverse, segment by segment
   int par = 0;  // which psi-adjoint copy holds the current values
   for (int sgi = nseg - 1; sgi >= 0; --sgi) {
@@ -2766,9 +2808,12 @@ static xf_status batch_grad_run(xf_batch *b, int K, double *const *grads, double
         snap((int)(l + 1));
       }
     }
-    for (long long l = m - 1; l >= 0; --l) {
-      hipLaunchKernelGGL(kb_adj_e<T>, dim3(ncell), dim3(256), 0, st, d_ia, d_cell, t0 + l, (int)l, par);
-      hipLaunchKernelGGL(kb_adj_h<T>, dim3(ncell), dim3(256), 0, st, d_ia, d_cell, par);
+    for (long long l = m - 1; l >= 0; --l, ++rstep) {
+      const bool part = rstep < (long long)ready.size();
+      const int2 *mp = part ? d_rev : d_cell;
+      const dim3 nw((unsigned)(part ? ready[rstep] : ncell));
+      hipLaunchKernelGGL(kb_adj_e<T>, nw, dim3(256), 0, st, d_ia, mp, t0 + l, (int)l, par);
+      hipLaunchKernelGGL(kb_adj_h<T>, nw, dim3(256), 0, st, d_ia, mp, par);
       par ^= 1;
     }
     HIPCHK(hipGetLastError());
