@@ -176,6 +176,9 @@ struct xf_engine {
   // window [win[0], win[1]) x [win[2], win[3]) x [win[4], win[5]) (i, j, k)
   bool win_on = false;
   int win[6] = {0, 0, 0, 0, 0, 0};
+  // a gradient's design-box history slot the split E kernel fills as it
+  // steps (null: none)
+  void *snap = nullptr;
   // gradient: run each segment's adjoint sweep on a second stream beside the
   // replay of the segment before it (-1: in FP32 only, where the kernels are
   // latency-bound at the sizes a gradient takes; 0: never; 1: always)
@@ -1047,6 +1050,7 @@ static Grid<T> make_grid(xf_engine *e) {
   g.sj = e->sj; g.si = e->si;
   for (int a = 0; a < 6; ++a) g.db[a] = e->db[a];
   g.dcoef = (const T *)e->dcoef;
+  g.snap = (T *)e->snap;
   for (int a = 0; a < 6; ++a) g.xb[a] = e->xb[a];
   g.drc = (const T *)e->drc;
   for (int q = 0; q < 3; ++q) {
@@ -1161,6 +1165,28 @@ static void window_tiling(const xf_engine *e, Tiling &tl) {
     span(w[4], w[5], tl.tz, tl.ntz, tl.oz, tl.ntz);
   }
   tl.nblocks = tl.xcd_cols ? 8 * tl.ntz * ((tl.nty * tl.ntx + 7) / 8) : tl.ntz * tl.nty * tl.ntx;
+}
+
+// Whether the split E kernel can write a step's design-box history itself:
+// nothing launched after it changes E in the box (no TFSF face plane, with
+// a cell to spare, and no lumped element meets the box) and the step is
+// not the fused one.
+static bool fused_ok(const xf_engine *e);
+static bool e_kernel_snaps(const xf_engine *e) {
+  if (fused_ok(e) || !e->has_design) return false;
+  auto meets = [](int lo, int hi, int a, int b) { return lo <= b && hi > a; };  // [lo, hi) and [a, b]
+  const int *d = e->db;
+  if (e->has_tfsf) {
+    const int f[3][2] = {{e->tli, e->tri}, {e->tlj, e->trj}, {e->tlk, e->trk}};
+    for (int a = 0; a < 3; ++a)
+      for (int s = 0; s < 2; ++s)
+        if (meets(d[2 * a], d[2 * a + 1], f[a][s] - 1, f[a][s] + 1)) return false;
+  }
+  for (const SourceDev &sd : e->sources)
+    if (meets(d[0], d[1], sd.i0 - 1, sd.i1) && meets(d[2], d[3], sd.j0 - 1, sd.j1) &&
+        meets(d[4], d[5], sd.k0 - 1, sd.k1))
+      return false;
+  return true;
 }
 
 // The cells a replayed gradient step has to get right, one to spare: the
@@ -2409,7 +2435,9 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
   const unsigned nsnap = (unsigned)((e->nd + 255) / 256);
   // forward run with a checkpoint at the start of every segment but the
   // last, whose design-box E fields are kept as it runs (it is never
-  // recomputed), and a checkpoint at the end
+  // recomputed), and a checkpoint at the end.  The E kernel writes each
+  // step's design fields itself when nothing after it changes E in the box.
+  const bool esnap = e_kernel_snaps(e);
   T *hl = hist[(nseg - 1) & 1];
   for (int sgi = 0; sgi < nseg; ++sgi) {
     const long long n = std::min<long long>(K, T_ - (long long)sgi * K);
@@ -2421,8 +2449,11 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
     }
     hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hl, 0);
     for (long long l = 0; l < n; ++l) {
-      if ((s = run_steps<T>(e, 1)) != XF_OK) return s;
-      hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hl, (int)(l + 1));
+      if (esnap) e->snap = hl + (size_t)(l + 1) * e->nd * 3;
+      s = run_steps<T>(e, 1);
+      e->snap = nullptr;
+      if (s != XF_OK) return s;
+      if (!esnap) hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hl, (int)(l + 1));
     }
   }
   xf_status s = state_save(e, ck[nseg], nullptr, &CK);
@@ -2509,7 +2540,10 @@ static xf_status grad_run(xf_engine *e, int K, double *grad, double *J) {
         // by a cell inside a later window; the next replay starts from a
         // checkpoint of the whole state.
         replay_window(e, (int)(rn - l), e->win);  // rn <= K, an int
-        if ((s = run_steps<T>(e, 1)) == XF_OK)
+        if (esnap) e->snap = hr + (size_t)(l + 1) * e->nd * 3;
+        s = run_steps<T>(e, 1);
+        e->snap = nullptr;
+        if (s == XF_OK && !esnap)
           hipLaunchKernelGGL(k_design_snap<T>, dim3(nsnap), dim3(256), 0, e->stream, gr, hr, (int)(l + 1));
       }
       e->win_on = false;

@@ -51,6 +51,9 @@ struct Grid {
   // cezhy (an empty box when db0 == db1)
   int db[6];
   const T *dcoef;
+  // when set, the E kernel also writes each design cell's new (Ex, Ey, Ez)
+  // to snap[design cell][3] (a gradient's design-box history of the step)
+  T *snap;
   // Drude box [xb0,xb1) x [xb2,xb3) x [xb4,xb5) (empty when xb0 == xb1):
   // per-material ADE records drc[m][XF_ADE_STRIDE] (kind at XF_ADE_KIND:
   // 0 none, 1 Drude, 2 Debye, 3 Lorentz; the layout of xf_engine_set_dispersive)
@@ -742,6 +745,14 @@ __device__ __forceinline__ void compute_plane_e(const Grid<T> &g, const TileTabl
   sto(q.u0, o, vx);
   sto(q.u1, o, vy);
   sto(q.u2, o, vz);
+  if constexpr (SLAB && SPECIAL) {
+    if (g.snap && in_design(g, i, j, k)) {
+      T *h = g.snap + (long long)design_cell(g, i, j, k) * 3;
+      h[0] = vx;
+      h[1] = vy;
+      h[2] = vz;
+    }
+  }
 }
 
 // SPECIAL: the engine has a design box or Drude cells, whose planes take
